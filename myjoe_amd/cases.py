@@ -1,0 +1,86 @@
+"""Workload definitions: meshes, Joe.in text and synthetic states.
+
+* ``bump_case``: steady 2D inviscid Euler channel with a 10 %-thick
+  circular-arc bump, gamma 1.4, inlet Mach 0.5, p = rho = 1 (R = 1), HLLC,
+  2nd order with least-squares gradients and the smooth Barth-Jespersen
+  limiter, backward Euler with a CFL ramp 1 -> 50.
+* ``box_case``: the split/jittered box of the single-residual throughput test
+  with inlet/outlet CBC, WALL, FARFIELD and SYMMETRY zones.
+* ``random_state``: rho ~ U(0.5, 1.5), p ~ lognormal(0, 0.2),
+  u_i ~ N(0, 0.3) * c, as conservative variables.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import meshgen
+
+GAMMA = 1.4
+MACH_BUMP = 0.5
+U_BUMP = MACH_BUMP * math.sqrt(GAMMA)  # c = sqrt(gamma p / rho) with p = rho = 1
+
+
+def _common(scheme: str, second_order: bool, grad: str, eps: float, extra: str) -> str:
+    lines = [
+        "GAMMA = 1.4", "P_REF = 1.0", "RHO_REF = 1.0", "T_REF = 1.0", "MU_REF = 0.0",
+        f"SPACE_INTEGRATION = {scheme}",
+        f"GRAD_RECONSTRUCTION SCHEME={grad} LIMITER=YES EPS={eps!r}",
+        "TIME_INTEGRATION = BACKWARD_EULER", "LINEAR_SOLVER_NS = BCGSTAB",
+        "SYMM_BC = ORIGINAL_NUMERIC", "INITIAL_CONDITION_TURB = 0.0", "CHECK_INTERVAL = 1",
+    ]
+    if second_order:
+        lines.append("SPATIAL_SECOND_ORDER")
+    return "\n".join(lines) + "\n" + extra
+
+
+def bump_case(nx: int = 96, ny: int = 32, scheme: str = "HLLC", second_order: bool = True, nsteps: int = 2000,
+              cfl_max: float = 50.0, restart: str = "bump.msh", resid_th: float = 0.0,
+              ls: str = "MAX_ITER=30 ABS_RESID=1.0e-8 REL_RESID=1.0e-2"):
+    mesh = meshgen.bump_channel(nx, ny)
+    u = repr(U_BUMP)
+    extra = "\n".join([
+        f"RESTART = {restart}",
+        f"LINEAR_SOLVER_NS_TRESHOLDS {ls}",
+        "CFL = 1.0",
+        f"CFL_RAMP AFTER_ITER=10 INTERVAL_ITER=1 FACTOR_CFL=1.1 MAX_CFL={cfl_max!r}",
+        f"NSTEPS = {nsteps}",
+        f"RESID_ENERG_TH = {resid_th!r}",
+        f"U_INITIAL = {u} 0.0 0.0",
+        f"inlet = CBC {u} 0.0 0.0 1.0 1.0",
+        f"outlet = CBC {u} 0.0 0.0 1.0 1.0",
+        "bump_wall = WALL",
+        "top = SYMMETRY",
+        "zsym = SYMMETRY",
+    ]) + "\n"
+    return mesh, _common(scheme, second_order, "LEASTSQUARES", 0.01, extra)
+
+
+def box_case(n: int, frac: float = 0.5, jitter: float = 0.2, seed: int = 0, scheme: str = "HLLC",
+             second_order: bool = True, grad: str = "LEASTSQUARES", restart: str = "box.msh"):
+    mesh = meshgen.split_box(n, frac, jitter, seed)
+    extra = "\n".join([
+        f"RESTART = {restart}",
+        "CFL = 1.0",
+        "NSTEPS = 10",
+        "U_INITIAL = 0.3 0.1 0.0",
+        "inlet = CBC 0.5 0.1 0.0 1.0 1.0",
+        "outlet = CBC 0.5 0.1 0.0 1.0 1.0",
+        "wall = WALL",
+        "farfield = FARFIELD 0.4 0.2 0.1 1.0 1.0 1.4",
+        "symm = SYMMETRY",
+    ]) + "\n"
+    return mesh, _common(scheme, second_order, grad, 0.01, extra)
+
+
+def random_state(ncv: int, seed: int = 1):
+    """(rho, rhou[ncv,3], rhoE) drawn as in the single-residual throughput test."""
+    rng = np.random.default_rng(seed)
+    rho = rng.uniform(0.5, 1.5, ncv)
+    p = np.exp(rng.normal(0.0, 0.2, ncv))
+    c = np.sqrt(GAMMA * p / rho)
+    u = rng.normal(0.0, 0.3, (ncv, 3)) * c[:, None]
+    rhou = rho[:, None] * u
+    rhoE = p / (GAMMA - 1.0) + 0.5 * rho * np.sum(u * u, axis=1)
+    return rho, rhou, rhoE

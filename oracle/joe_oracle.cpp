@@ -1,0 +1,1371 @@
+// CPU restatement of Joe's steady-solve hot path -- TEST INFRASTRUCTURE ONLY.
+// See joe_oracle.h for who may load it. Each routine restates the reference
+// routine named at its definition, in the reference's loop structure (face
+// loops scattering into cells, sequential sums), so that compiled like the
+// reference (x86-64 SSE2, no contraction) it reproduces the reference's bits.
+#include "joe_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using std::max;
+using std::min;
+
+namespace {
+
+inline double dot3(const double *a, const double *b) {  // tc_vec3d.h:14-22
+  double d = 0.0;
+  for (int i = 0; i < 3; i++) d += a[i] * b[i];
+  return d;
+}
+inline double norm3(double *n, const double *v) {  // tc_vec3d.h:36-46
+  double mag = sqrt(dot3(v, v));
+  for (int i = 0; i < 3; i++) n[i] = v[i] / mag;
+  return mag;
+}
+
+// ---------------------------------------------------------------------------
+// pointwise flux functions
+// ---------------------------------------------------------------------------
+
+// HLLC, UgpWithCvCompFlow.cpp:120-235 (nScal = 0, surfVeloc = 0)
+void flux_hllc(double F[5], double rhoL, const double *uL, double pL, double gammaL, double kL, double rhoR,
+               const double *uR, double pR, double gammaR, double kR, double area, const double *nVec) {
+  double surfv = 0.0;
+  double unL = dot3(uL, nVec) - surfv;
+  double unLr = unL - surfv;
+  double uLuL = dot3(uL, uL);
+  double cL = sqrt(gammaL * pL / rhoL);
+  double hL = gammaL / (gammaL - 1.0) * pL / rhoL + 0.5 * uLuL + kL;
+  double eL = hL * rhoL - pL;
+  double unR = dot3(uR, nVec) - surfv;
+  double unRr = unR - surfv;
+  double uRuR = dot3(uR, uR);
+  double cR = sqrt(gammaR * pR / rhoR);
+  double hR = gammaR / (gammaR - 1.0) * pR / rhoR + 0.5 * uRuR + kR;
+  double eR = hR * rhoR - pR;
+  double Rrho = sqrt(rhoR / rhoL);
+  double tmp = 1.0 / (1.0 + Rrho);
+  double velRoe[3];
+  for (int i = 0; i < 3; i++) velRoe[i] = tmp * (uL[i] + uR[i] * Rrho);
+  double uRoe = dot3(velRoe, nVec);
+  double gamPdivRho = tmp * ((gammaL * pL / rhoL + 0.5 * (gammaL - 1.0) * uLuL) +
+                             (gammaR * pR / rhoR + 0.5 * (gammaR - 1.0) * uRuR) * Rrho);
+  double cRoe = sqrt(gamPdivRho - ((gammaL + gammaR) * 0.5 - 1.0) * 0.5 * dot3(velRoe, velRoe));
+  double sL = min(uRoe - cRoe, unLr - cL);
+  double sR = max(uRoe + cRoe, unRr + cR);
+  double sM = (pL - pR - rhoL * unL * (sL - unL) + rhoR * unR * (sR - unR)) / (rhoR * (sR - unR) - rhoL * (sL - unL));
+  double pStar = rhoR * (unR - sR) * (unR - sM) + pR;
+  double Frho, Frhou[3], FrhoE;
+  if (sM >= 0.0) {
+    if (sL > 0.0) {
+      Frho = rhoL * unLr;
+      for (int i = 0; i < 3; i++) Frhou[i] = rhoL * uL[i] * unLr + pL * nVec[i];
+      FrhoE = eL * unLr + pL * unL;
+    } else {
+      double invSLmSs = 1.0 / (sL - sM);
+      double sLmuL = sL - unL;
+      double rhoSL = rhoL * sLmuL * invSLmSs;
+      double rhouSL[3];
+      for (int i = 0; i < 3; i++) rhouSL[i] = (rhoL * uL[i] * sLmuL + (pStar - pL) * nVec[i]) * invSLmSs;
+      double eSL = (sLmuL * eL - pL * unL + pStar * sM) * invSLmSs;
+      Frho = rhoSL * sM;
+      for (int i = 0; i < 3; i++) Frhou[i] = rhouSL[i] * sM + pStar * nVec[i];
+      FrhoE = (eSL + pStar) * sM;
+    }
+  } else {
+    if (sR >= 0.0) {
+      double invSRmSs = 1.0 / (sR - sM);
+      double sRmuR = sR - unR;
+      double rhoSR = rhoR * sRmuR * invSRmSs;
+      double rhouSR[3];
+      for (int i = 0; i < 3; i++) rhouSR[i] = (rhoR * uR[i] * sRmuR + (pStar - pR) * nVec[i]) * invSRmSs;
+      double eSR = (sRmuR * eR - pR * unR + pStar * sM) * invSRmSs;
+      Frho = rhoSR * sM;
+      for (int i = 0; i < 3; i++) Frhou[i] = rhouSR[i] * sM + pStar * nVec[i];
+      FrhoE = (eSR + pStar) * sM;
+    } else {
+      Frho = rhoR * unRr;
+      for (int i = 0; i < 3; i++) Frhou[i] = rhoR * uR[i] * unRr + pR * nVec[i];
+      FrhoE = eR * unRr + pR * unR;
+    }
+  }
+  F[0] = Frho * area;
+  for (int i = 0; i < 3; i++) F[1 + i] = Frhou[i] * area;
+  F[4] = FrhoE * area;
+}
+
+// Roe, UgpWithCvCompFlow.cpp:3281-3380. When the Roe-averaged a^2 < 1e-12 the
+// reference returns after the central part (:3318-3319); so does this.
+void flux_roe(double F[5], double rhoL, const double *uL, double pL, double gammaL, double rhoR, const double *uR,
+              double pR, double gammaR, double area, const double *nVec) {
+  double rho0 = rhoL, rho1 = rhoR;
+  const double *u0 = uL, *u1 = uR;
+  double p0 = pL, p1 = pR;
+  double gamma = 0.5 * (gammaL + gammaR);
+  double gm1 = gamma - 1.0;
+  double rhoun0 = rho0 * dot3(u0, nVec);
+  double rhoun1 = rho1 * dot3(u1, nVec);
+  double rhoE0 = p0 / gm1 + 0.5 * rho0 * dot3(u0, u0);
+  double rhoE1 = p1 / gm1 + 0.5 * rho1 * dot3(u1, u1);
+  double H0 = (rhoE0 + p0) / rho0;
+  double H1 = (rhoE1 + p1) / rho1;
+  F[0] = 0.5 * area * (rhoun0 + rhoun1);
+  F[1] = 0.5 * area * (rhoun0 * u0[0] + rhoun1 * u1[0] + (p0 + p1) * nVec[0]);
+  F[2] = 0.5 * area * (rhoun0 * u0[1] + rhoun1 * u1[1] + (p0 + p1) * nVec[1]);
+  F[3] = 0.5 * area * (rhoun0 * u0[2] + rhoun1 * u1[2] + (p0 + p1) * nVec[2]);
+  F[4] = 0.5 * area * (rhoun0 * H0 + rhoun1 * H1);
+  double sqrt_rho0 = sqrt(rho0), sqrt_rho1 = sqrt(rho1);
+  double tmp = 1.0 / (sqrt_rho0 + sqrt_rho1);
+  double uAvg = tmp * (u0[0] * sqrt_rho0 + u1[0] * sqrt_rho1);
+  double vAvg = tmp * (u0[1] * sqrt_rho0 + u1[1] * sqrt_rho1);
+  double wAvg = tmp * (u0[2] * sqrt_rho0 + u1[2] * sqrt_rho1);
+  double hAvg = tmp * (H0 * sqrt_rho0 + H1 * sqrt_rho1);
+  double alphaAvg = 0.5 * (uAvg * uAvg + vAvg * vAvg + wAvg * wAvg);
+  double a2Avg = gm1 * (hAvg - alphaAvg);
+  if (a2Avg < 1.0E-12) return;
+  double aAvg = sqrt(a2Avg);
+  double unAvg = uAvg * nVec[0] + vAvg * nVec[1] + wAvg * nVec[2];
+  double dr = rho1 - rho0;
+  double dru = rho1 * u1[0] - rho0 * u0[0];
+  double drv = rho1 * u1[1] - rho0 * u0[1];
+  double drw = rho1 * u1[2] - rho0 * u0[2];
+  double drE = rhoE1 - rhoE0;
+  double lam1 = fabs(unAvg);
+  double lam2 = fabs(unAvg + aAvg);
+  double lam3 = fabs(unAvg - aAvg);
+  double abv1 = 0.5 * (lam2 + lam3);
+  double abv2 = 0.5 * (lam2 - lam3);
+  double abv3 = abv1 - lam1;
+  double abv4 = gm1 * (alphaAvg * dr - uAvg * dru - vAvg * drv - wAvg * drw + drE);
+  double abv5 = unAvg * dr - nVec[0] * dru - nVec[1] * drv - nVec[2] * drw;
+  double abv6 = abv3 * abv4 / a2Avg - abv2 * abv5 / aAvg;
+  double abv7 = abv3 * abv5 - abv2 * abv4 / aAvg;
+  F[0] -= 0.5 * area * (lam1 * dr + abv6);
+  F[1] -= 0.5 * area * (lam1 * dru + uAvg * abv6 - nVec[0] * abv7);
+  F[2] -= 0.5 * area * (lam1 * drv + vAvg * abv6 - nVec[1] * abv7);
+  F[3] -= 0.5 * area * (lam1 * drw + wAvg * abv6 - nVec[2] * abv7);
+  F[4] -= 0.5 * area * (lam1 * drE + hAvg * abv6 - unAvg * abv7);
+}
+
+// AUSM (Liou-Steffen), UgpWithCvCompFlow.cpp:3566-3641
+void flux_ausm(double F[5], double rhoL, const double *uL, double pL, double gammaL, double kL, double rhoR,
+               const double *uR, double pR, double gammaR, double kR, double area, const double *nVec) {
+  double unL = dot3(uL, nVec);
+  double uLuL = dot3(uL, uL);
+  double cL = sqrt(gammaL * pL / rhoL);
+  double hL = (gammaL / (gammaL - 1.0)) * (pL / rhoL) + 0.5 * uLuL + kL;
+  double mL = unL / cL;
+  double unR = dot3(uR, nVec);
+  double uRuR = dot3(uR, uR);
+  double cR = sqrt(gammaR * pR / rhoR);
+  double hR = (gammaR / (gammaR - 1.0)) * (pR / rhoR) + 0.5 * uRuR + kR;
+  double mR = unR / cR;
+  double mLP = fabs(mL) <= 1.0 ? 0.25 * (mL + 1.0) * (mL + 1.0) : 0.5 * (mL + fabs(mL));
+  double mRM = fabs(mR) <= 1.0 ? -0.25 * (mR - 1.0) * (mR - 1.0) : 0.5 * (mR - fabs(mR));
+  double mF = mLP + mRM;
+  double pLP = fabs(mL) <= 1.0 ? 0.25 * pL * (mL + 1.0) * (mL + 1.0) * (2.0 - mL) : 0.5 * pL * (mL + fabs(mL)) / mL;
+  double pRM = fabs(mR) <= 1.0 ? 0.25 * pR * (mR - 1.0) * (mR - 1.0) * (2.0 + mR) : 0.5 * pR * (mR - fabs(mR)) / mR;
+  double pF = pLP + pRM;
+  double Phi = fabs(mF);
+  F[0] = 0.5 * (mF * ((rhoL * cL) + (rhoR * cR)) - Phi * ((rhoR * cR) - (rhoL * cL)));
+  for (int i = 0; i < 3; i++)
+    F[1 + i] = 0.5 * (mF * ((rhoL * cL * uL[i]) + (rhoR * cR * uR[i])) - Phi * ((rhoR * cR * uR[i]) - (rhoL * cL * uL[i]))) +
+               nVec[i] * pF;
+  F[4] = 0.5 * (mF * ((rhoL * cL * hL) + (rhoR * cR * hR)) - Phi * ((rhoR * cR * hR) - (rhoL * cL * hL)));
+  for (int i = 0; i < 5; i++) F[i] *= area;
+}
+
+// calcJacobianA, UgpWithCvCompFlow.cpp:666-709 (nV area-weighted)
+void jac_a(double A[5][5], const double *vel, double pp, double rrho, const double *nV, double gamma) {
+  double surfVeloc = 0.0;
+  double kapm1 = (gamma - 1.0);
+  double nVel[3] = {vel[0] * nV[0], vel[1] * nV[1], vel[2] * nV[2]};
+  double U_k = nVel[0] + nVel[1] + nVel[2];
+  double vSquHlf = 0.5 * dot3(vel, vel);
+  double c = sqrt(gamma * pp / rrho);
+  double inv_kap_m1 = 1.0 / kapm1;
+  A[0][0] = -surfVeloc;
+  A[0][1] = nV[0];
+  A[0][2] = nV[1];
+  A[0][3] = nV[2];
+  A[0][4] = 0.0;
+  A[1][0] = -vel[0] * (nVel[1] + nVel[2]) + nV[0] * (kapm1 * vSquHlf - vel[0] * vel[0]);
+  A[1][1] = (2. - gamma) * nVel[0] + U_k - surfVeloc;
+  A[1][2] = vel[0] * nV[1] - kapm1 * vel[1] * nV[0];
+  A[1][3] = vel[0] * nV[2] - kapm1 * vel[2] * nV[0];
+  A[1][4] = kapm1 * nV[0];
+  A[2][0] = -vel[1] * (nVel[0] + nVel[2]) + nV[1] * (kapm1 * vSquHlf - vel[1] * vel[1]);
+  A[2][1] = -kapm1 * vel[0] * nV[1] + vel[1] * nV[0];
+  A[2][2] = (2. - gamma) * nVel[1] + U_k - surfVeloc;
+  A[2][3] = vel[1] * nV[2] - kapm1 * vel[2] * nV[1];
+  A[2][4] = kapm1 * nV[1];
+  A[3][0] = -vel[2] * (nVel[0] + nVel[1]) + nV[2] * (kapm1 * vSquHlf - vel[2] * vel[2]);
+  A[3][1] = -kapm1 * vel[0] * nV[2] + vel[2] * nV[0];
+  A[3][2] = -kapm1 * vel[1] * nV[2] + vel[2] * nV[1];
+  A[3][3] = (2. - gamma) * nVel[2] + U_k - surfVeloc;
+  A[3][4] = kapm1 * nV[2];
+  A[4][0] = U_k * ((gamma - 2.) * vSquHlf - c * c * inv_kap_m1);
+  A[4][1] = c * c * inv_kap_m1 * nV[0] - kapm1 * vel[0] * (nVel[1] + nVel[2]) - (kapm1 * vel[0] * vel[0] - vSquHlf) * nV[0];
+  A[4][2] = c * c * inv_kap_m1 * nV[1] - kapm1 * vel[1] * (nVel[0] + nVel[2]) - (kapm1 * vel[1] * vel[1] - vSquHlf) * nV[1];
+  A[4][3] = c * c * inv_kap_m1 * nV[2] - kapm1 * vel[2] * (nVel[0] + nVel[1]) - (kapm1 * vel[2] * vel[2] - vSquHlf) * nV[2];
+  A[4][4] = gamma * U_k - surfVeloc;
+}
+
+// calcSubSonicJacobeanHLLC, UgpWithCvCompFlow.cpp:605-659
+void jac_hllc_subsonic(double (*AL)[5], double (*AR)[5], double rhoL, const double *uL, double pL, double eL,
+                       double qL, double psiL, double SL, double rhoSL, const double *rhouSL, double eSL,
+                       const double *dSMdUL, const double *dSMdUR, const double *dpsdUL, const double *dpsdUR,
+                       double SM, double pS, double gamma, const double *nV) {
+  double gammaMinus1 = (gamma - 1.0);
+  double omL = 1.0 / (SL - SM);
+  if (AL != NULL) {
+    AL[0][0] = SL + rhoSL * dSMdUL[0];
+    AL[0][1] = -nV[0] + rhoSL * dSMdUL[1];
+    AL[0][2] = -nV[1] + rhoSL * dSMdUL[2];
+    AL[0][3] = -nV[2] + rhoSL * dSMdUL[3];
+    AL[0][4] = +rhoSL * dSMdUL[4];
+    AL[1][0] = qL * uL[0] - nV[0] * psiL * gammaMinus1 / 2.0 + nV[0] * dpsdUL[0] + rhouSL[0] * dSMdUL[0];
+    AL[1][1] = SL - qL + nV[0] * (gamma - 2.0) * uL[0] + nV[0] * dpsdUL[1] + rhouSL[0] * dSMdUL[1];
+    AL[1][2] = -uL[0] * nV[1] + nV[0] * gammaMinus1 * uL[1] + nV[0] * dpsdUL[2] + rhouSL[0] * dSMdUL[2];
+    AL[1][3] = -uL[0] * nV[2] + nV[0] * gammaMinus1 * uL[2] + nV[0] * dpsdUL[3] + rhouSL[0] * dSMdUL[3];
+    AL[1][4] = -gammaMinus1 * nV[0] + nV[0] * dpsdUL[4] + rhouSL[0] * dSMdUL[4];
+    AL[2][0] = qL * uL[1] - nV[1] * psiL * gammaMinus1 / 2.0 + nV[1] * dpsdUL[0] + rhouSL[1] * dSMdUL[0];
+    AL[2][1] = -uL[1] * nV[0] + nV[1] * gammaMinus1 * uL[0] + nV[1] * dpsdUL[1] + rhouSL[1] * dSMdUL[1];
+    AL[2][2] = SL - qL + nV[1] * (gamma - 2.0) * uL[1] + nV[1] * dpsdUL[2] + rhouSL[1] * dSMdUL[2];
+    AL[2][3] = -uL[1] * nV[2] + nV[1] * gammaMinus1 * uL[2] + nV[1] * dpsdUL[3] + rhouSL[1] * dSMdUL[3];
+    AL[2][4] = -gammaMinus1 * nV[1] + nV[1] * dpsdUL[4] + rhouSL[1] * dSMdUL[4];
+    AL[3][0] = qL * uL[2] - nV[2] * psiL * gammaMinus1 / 2.0 + nV[2] * dpsdUL[0] + rhouSL[2] * dSMdUL[0];
+    AL[3][1] = -uL[2] * nV[0] + nV[2] * gammaMinus1 * uL[0] + nV[2] * dpsdUL[1] + rhouSL[2] * dSMdUL[1];
+    AL[3][2] = -uL[2] * nV[1] + nV[2] * gammaMinus1 * uL[1] + nV[2] * dpsdUL[2] + rhouSL[2] * dSMdUL[2];
+    AL[3][3] = SL - qL + nV[2] * (gamma - 2.0) * uL[2] + nV[2] * dpsdUL[3] + rhouSL[2] * dSMdUL[3];
+    AL[3][4] = -gammaMinus1 * nV[2] + nV[2] * dpsdUL[4] + rhouSL[2] * dSMdUL[4];
+    AL[4][0] = qL * (eL + pL) / rhoL - qL * psiL * (gamma - 1.0) / 2.0 + SM * dpsdUL[0] + (pS + eSL) * dSMdUL[0];
+    AL[4][1] = -nV[0] * (eL + pL) / rhoL + gammaMinus1 * uL[0] * qL + SM * dpsdUL[1] + (pS + eSL) * dSMdUL[1];
+    AL[4][2] = -nV[1] * (eL + pL) / rhoL + gammaMinus1 * uL[1] * qL + SM * dpsdUL[2] + (pS + eSL) * dSMdUL[2];
+    AL[4][3] = -nV[2] * (eL + pL) / rhoL + gammaMinus1 * uL[2] * qL + SM * dpsdUL[3] + (pS + eSL) * dSMdUL[3];
+    AL[4][4] = SL - qL * gamma + SM * dpsdUL[4] + (pS + eSL) * dSMdUL[4];
+    for (int i = 0; i < 5; i++)
+      for (int j = 0; j < 5; j++) AL[i][j] *= omL;
+  }
+  if (AR != NULL) {
+    for (int i = 0; i < 5; i++) AR[0][i] = omL * rhoSL * dSMdUR[i];
+    for (int i = 0; i < 5; i++) AR[1][i] = omL * (nV[0] * dpsdUR[i] + rhouSL[0] * dSMdUR[i]);
+    for (int i = 0; i < 5; i++) AR[2][i] = omL * (nV[1] * dpsdUR[i] + rhouSL[1] * dSMdUR[i]);
+    for (int i = 0; i < 5; i++) AR[3][i] = omL * (nV[2] * dpsdUR[i] + rhouSL[2] * dSMdUR[i]);
+    for (int i = 0; i < 5; i++) AR[4][i] = omL * (dpsdUR[i] * SM + (pS + eSL) * dSMdUR[i]);
+  }
+}
+
+// calcEulerFluxMatrices_HLLC, UgpWithCvCompFlow.cpp:242-595 (nScal = 0)
+void jac_hllc(double (*A_L)[5], double (*A_R)[5], double rhoL, const double *uL, double pL, double gammaL, double kL,
+              double rhoR, const double *uR, double pR, double gammaR, double kR, double area, const double *nVec) {
+  double unL = dot3(uL, nVec);
+  double uLuL = dot3(uL, uL);
+  double cL = sqrt(gammaL * pL / rhoL);
+  double hL = gammaL / (gammaL - 1.0) * pL / rhoL + 0.5 * uLuL + kL;
+  double eL = hL * rhoL - pL;
+  double unR = dot3(uR, nVec);
+  double uRuR = dot3(uR, uR);
+  double cR = sqrt(gammaR * pR / rhoR);
+  double hR = gammaR / (gammaR - 1.0) * pR / rhoR + 0.5 * uRuR + kR;
+  double eR = hR * rhoR - pR;
+  double Rrho = sqrt(rhoR / rhoL);
+  double tmp = 1.0 / (1.0 + Rrho);
+  double velRoe[3];
+  for (int i = 0; i < 3; i++) velRoe[i] = tmp * (uL[i] + uR[i] * Rrho);
+  double uRoe = dot3(velRoe, nVec);
+  double gamPdivRho = tmp * ((gammaL * pL / rhoL + 0.5 * (gammaL - 1.0) * uLuL) +
+                             (gammaR * pR / rhoR + 0.5 * (gammaR - 1.0) * uRuR) * Rrho);
+  double cRoe = sqrt(gamPdivRho - ((gammaL + gammaR) * 0.5 - 1.0) * 0.5 * dot3(velRoe, velRoe));
+  double sL = min(uRoe - cRoe, unL - cL);
+  double sR = max(uRoe + cRoe, unR + cR);
+  double sM = (pL - pR - rhoL * unL * (sL - unL) + rhoR * unR * (sR - unR)) / (rhoR * (sR - unR) - rhoL * (sL - unL));
+  double pStar = rhoR * (unR - sR) * (unR - sM) + pR;
+  double gAvg = 0.5 * (gammaL + gammaR);
+  if (sM >= 0.0) {
+    if (sL > 0.0) {
+      if (A_L != NULL) {
+        double nVecArea[3];
+        for (int i = 0; i < 3; i++) nVecArea[i] = nVec[i] * area;
+        jac_a(A_L, uL, pL, rhoL, nVecArea, gAvg);
+      }
+      if (A_R != NULL)
+        for (int i = 0; i < 5; i++)
+          for (int j = 0; j < 5; j++) A_R[i][j] = 0.0;
+    } else {
+      double invSLmSs = 1.0 / (sL - sM);
+      double sLmuL = sL - unL;
+      double rhoSL = rhoL * sLmuL * invSLmSs;
+      double rhouSL[3];
+      for (int i = 0; i < 3; i++) rhouSL[i] = (rhoL * uL[i] * sLmuL + (pStar - pL) * nVec[i]) * invSLmSs;
+      double eSL = (sLmuL * eL - pL * unL + pStar * sM) * invSLmSs;
+      double gammaLM1 = (gammaL - 1.0), gammaRM1 = (gammaR - 1.0);
+      double invrhotld = 1.0 / (rhoR * (sR - unR) - rhoL * (sL - unL));
+      double dSMdUL[5], dSMdUR[5], dpsdUL[5], dpsdUR[5];
+      if (A_L != NULL) {
+        dSMdUL[0] = -unL * unL + uLuL * gammaLM1 / 2.0 + sM * sL;
+        dSMdUL[1] = nVec[0] * (2.0 * unL - sL - sM) - gammaLM1 * uL[0];
+        dSMdUL[2] = nVec[1] * (2.0 * unL - sL - sM) - gammaLM1 * uL[1];
+        dSMdUL[3] = nVec[2] * (2.0 * unL - sL - sM) - gammaLM1 * uL[2];
+        dSMdUL[4] = gammaLM1;
+        for (int i = 0; i < 5; i++) {
+          dSMdUL[i] *= invrhotld;
+          dpsdUL[i] = rhoR * (sR - unR) * dSMdUL[i];
+        }
+      }
+      if (A_R != NULL) {
+        dSMdUR[0] = unR * unR - uRuR * gammaRM1 / 2.0 - sM * sR;
+        dSMdUR[1] = -nVec[0] * (2.0 * unR - sR - sM) + gammaRM1 * uR[0];
+        dSMdUR[2] = -nVec[1] * (2.0 * unR - sR - sM) + gammaRM1 * uR[1];
+        dSMdUR[3] = -nVec[2] * (2.0 * unR - sR - sM) + gammaRM1 * uR[2];
+        dSMdUR[4] = -gammaRM1;
+        for (int i = 0; i < 5; i++) {
+          dSMdUR[i] *= invrhotld;
+          dpsdUR[i] = rhoL * (sL - unL) * dSMdUR[i];
+        }
+      }
+      jac_hllc_subsonic(A_L, A_R, rhoL, uL, pL, eL, unL, uLuL, sL, rhoSL, rhouSL, eSL, dSMdUL, dSMdUR, dpsdUL, dpsdUR,
+                        sM, pStar, gAvg, nVec);
+      if (A_L != NULL) {
+        for (int i = 0; i < 5; i++) A_L[0][i] = A_L[0][i] * sM + dSMdUL[i] * rhoSL;
+        for (int i = 0; i < 5; i++) A_L[1][i] = A_L[1][i] * sM + dSMdUL[i] * rhouSL[0] + dpsdUL[i] * nVec[0];
+        for (int i = 0; i < 5; i++) A_L[2][i] = A_L[2][i] * sM + dSMdUL[i] * rhouSL[1] + dpsdUL[i] * nVec[1];
+        for (int i = 0; i < 5; i++) A_L[3][i] = A_L[3][i] * sM + dSMdUL[i] * rhouSL[2] + dpsdUL[i] * nVec[2];
+        for (int i = 0; i < 5; i++) A_L[4][i] = (A_L[4][i] + dpsdUL[i]) * sM + (eSL + pStar) * dSMdUL[i];
+        for (int i = 0; i < 5; i++)
+          for (int j = 0; j < 5; j++) A_L[i][j] *= area;
+      }
+      if (A_R != NULL) {
+        for (int i = 0; i < 5; i++) A_R[0][i] = A_R[0][i] * sM + dSMdUR[i] * rhoSL;
+        for (int i = 0; i < 5; i++) A_R[1][i] = A_R[1][i] * sM + dSMdUR[i] * rhouSL[0] + dpsdUR[i] * nVec[0];
+        for (int i = 0; i < 5; i++) A_R[2][i] = A_R[2][i] * sM + dSMdUR[i] * rhouSL[1] + dpsdUR[i] * nVec[1];
+        for (int i = 0; i < 5; i++) A_R[3][i] = A_R[3][i] * sM + dSMdUR[i] * rhouSL[2] + dpsdUR[i] * nVec[2];
+        for (int i = 0; i < 5; i++) A_R[4][i] = (A_R[4][i] + dpsdUR[i]) * sM + (eSL + pStar) * dSMdUR[i];
+        for (int i = 0; i < 5; i++)
+          for (int j = 0; j < 5; j++) A_R[i][j] *= area;
+      }
+    }
+  } else {
+    if (sR >= 0.0) {
+      double invSRmSs = 1.0 / (sR - sM);
+      double sRmuR = sR - unR;
+      double rhoSR = rhoR * sRmuR * invSRmSs;
+      double rhouSR[3];
+      for (int i = 0; i < 3; i++) rhouSR[i] = (rhoR * uR[i] * sRmuR + (pStar - pR) * nVec[i]) * invSRmSs;
+      double eSR = (sRmuR * eR - pR * unR + pStar * sM) * invSRmSs;
+      double gammaLM1 = (gammaL - 1.0), gammaRM1 = (gammaR - 1.0);
+      double invrhotld = 1.0 / (rhoR * (sR - unR) - rhoL * (sL - unL));
+      double dSMdUL[5], dSMdUR[5], dpsdUL[5], dpsdUR[5];
+      if (A_L != NULL) {
+        dSMdUL[0] = -unL * unL + uLuL * gammaLM1 / 2.0 + sM * sL;
+        dSMdUL[1] = nVec[0] * (2.0 * unL - sL - sM) - gammaLM1 * uL[0];
+        dSMdUL[2] = nVec[1] * (2.0 * unL - sL - sM) - gammaLM1 * uL[1];
+        dSMdUL[3] = nVec[2] * (2.0 * unL - sL - sM) - gammaLM1 * uL[2];
+        dSMdUL[4] = gammaLM1;
+        for (int i = 0; i < 5; i++) {
+          dSMdUL[i] *= invrhotld;
+          dpsdUL[i] = rhoR * (sR - unR) * dSMdUL[i];
+        }
+      }
+      if (A_R != NULL) {
+        dSMdUR[0] = unR * unR - uRuR * gammaRM1 / 2.0 - sM * sR;
+        dSMdUR[1] = -nVec[0] * (2.0 * unR - sR - sM) + gammaRM1 * uR[0];
+        dSMdUR[2] = -nVec[1] * (2.0 * unR - sR - sM) + gammaRM1 * uR[1];
+        dSMdUR[3] = -nVec[2] * (2.0 * unR - sR - sM) + gammaRM1 * uR[2];
+        dSMdUR[4] = -gammaRM1;
+        for (int i = 0; i < 5; i++) {
+          dSMdUR[i] *= invrhotld;
+          dpsdUR[i] = rhoL * (sL - unL) * dSMdUR[i];
+        }
+      }
+      jac_hllc_subsonic(A_R, A_L, rhoR, uR, pR, eR, unR, uRuR, sR, rhoSR, rhouSR, eSR, dSMdUR, dSMdUL, dpsdUR, dpsdUL,
+                        sM, pStar, gAvg, nVec);
+      if (A_L != NULL) {
+        for (int i = 0; i < 5; i++) A_L[0][i] = A_L[0][i] * sM + dSMdUL[i] * rhoSR;
+        for (int i = 0; i < 5; i++) A_L[1][i] = A_L[1][i] * sM + dSMdUL[i] * rhouSR[0] + dpsdUL[i] * nVec[0];
+        for (int i = 0; i < 5; i++) A_L[2][i] = A_L[2][i] * sM + dSMdUL[i] * rhouSR[1] + dpsdUL[i] * nVec[1];
+        for (int i = 0; i < 5; i++) A_L[3][i] = A_L[3][i] * sM + dSMdUL[i] * rhouSR[2] + dpsdUL[i] * nVec[2];
+        for (int i = 0; i < 5; i++) A_L[4][i] = (A_L[4][i] + dpsdUL[i]) * sM + (eSR + pStar) * dSMdUL[i];
+        for (int i = 0; i < 5; i++)
+          for (int j = 0; j < 5; j++) A_L[i][j] *= area;
+      }
+      if (A_R != NULL) {
+        for (int i = 0; i < 5; i++) A_R[0][i] = A_R[0][i] * sM + dSMdUR[i] * rhoSR;
+        for (int i = 0; i < 5; i++) A_R[1][i] = A_R[1][i] * sM + dSMdUR[i] * rhouSR[0] + dpsdUR[i] * nVec[0];
+        for (int i = 0; i < 5; i++) A_R[2][i] = A_R[2][i] * sM + dSMdUR[i] * rhouSR[1] + dpsdUR[i] * nVec[1];
+        for (int i = 0; i < 5; i++) A_R[3][i] = A_R[3][i] * sM + dSMdUR[i] * rhouSR[2] + dpsdUR[i] * nVec[2];
+        for (int i = 0; i < 5; i++) A_R[4][i] = (A_R[4][i] + dpsdUR[i]) * sM + (eSR + pStar) * dSMdUR[i];
+        for (int i = 0; i < 5; i++)
+          for (int j = 0; j < 5; j++) A_R[i][j] *= area;
+      }
+    } else {
+      if (A_R != NULL) {
+        double nVecArea[3];
+        for (int i = 0; i < 3; i++) nVecArea[i] = nVec[i] * area;
+        jac_a(A_R, uR, pR, rhoR, nVecArea, gAvg);
+      }
+      if (A_L != NULL)
+        for (int i = 0; i < 5; i++)
+          for (int j = 0; j < 5; j++) A_L[i][j] = 0.0;
+    }
+  }
+}
+
+// calcPMatrix / calcInvPMatrix, UgpWithCvCompFlow.cpp:3480-3559
+void p_matrix(double P[5][5], const double *vel, double c, double rrho, const double *nV, double gamma) {
+  double kapm1 = (gamma - 1.0);
+  double rhooc = rrho / c;
+  double rhoxc = rrho * c;
+  double vSqu = dot3(vel, vel);
+  P[0][0] = nV[0];
+  P[0][1] = nV[1];
+  P[0][2] = nV[2];
+  P[0][3] = 0.5 * rhooc;
+  P[0][4] = 0.5 * rhooc;
+  P[1][0] = vel[0] * nV[0];
+  P[1][1] = vel[0] * nV[1] - rrho * nV[2];
+  P[1][2] = vel[0] * nV[2] + rrho * nV[1];
+  P[1][3] = 0.5 * (vel[0] * rhooc + rrho * nV[0]);
+  P[1][4] = 0.5 * (vel[0] * rhooc - rrho * nV[0]);
+  P[2][0] = vel[1] * nV[0] + rrho * nV[2];
+  P[2][1] = vel[1] * nV[1];
+  P[2][2] = vel[1] * nV[2] - rrho * nV[0];
+  P[2][3] = 0.5 * (vel[1] * rhooc + rrho * nV[1]);
+  P[2][4] = 0.5 * (vel[1] * rhooc - rrho * nV[1]);
+  P[3][0] = vel[2] * nV[0] - rrho * nV[1];
+  P[3][1] = vel[2] * nV[1] + rrho * nV[0];
+  P[3][2] = vel[2] * nV[2];
+  P[3][3] = 0.5 * (vel[2] * rhooc + rrho * nV[2]);
+  P[3][4] = 0.5 * (vel[2] * rhooc - rrho * nV[2]);
+  P[4][0] = 0.5 * vSqu * nV[0] + rrho * vel[1] * nV[2] - rrho * vel[2] * nV[1];
+  P[4][1] = 0.5 * vSqu * nV[1] - rrho * vel[0] * nV[2] + rrho * vel[2] * nV[0];
+  P[4][2] = 0.5 * vSqu * nV[2] + rrho * vel[0] * nV[1] - rrho * vel[1] * nV[0];
+  P[4][3] = 0.5 * (0.5 * vSqu * rhooc + rrho * (vel[0] * nV[0] + vel[1] * nV[1] + vel[2] * nV[2]) + rhoxc / kapm1);
+  P[4][4] = 0.5 * (0.5 * vSqu * rhooc - rrho * (vel[0] * nV[0] + vel[1] * nV[1] + vel[2] * nV[2]) + rhoxc / kapm1);
+}
+
+void inv_p_matrix(double invP[5][5], const double *vel, double c, double rrho, const double *nV, double gamma) {
+  double kapm1 = (gamma - 1.0);
+  double rhoxc = rrho * c;
+  double c2 = c * c;
+  double vSqu = dot3(vel, vel);
+  invP[0][0] = nV[0] - nV[2] * vel[1] / rrho + nV[1] * vel[2] / rrho - nV[0] * 0.5 * kapm1 * vSqu / c2;
+  invP[0][1] = nV[0] * kapm1 * vel[0] / c2;
+  invP[0][2] = nV[2] / rrho + nV[0] * kapm1 * vel[1] / c2;
+  invP[0][3] = -nV[1] / rrho + nV[0] * kapm1 * vel[2] / c2;
+  invP[0][4] = -nV[0] * kapm1 / c2;
+  invP[1][0] = nV[1] + nV[2] * vel[0] / rrho - nV[0] * vel[2] / rrho - nV[1] * 0.5 * kapm1 * vSqu / c2;
+  invP[1][1] = -nV[2] / rrho + nV[1] * kapm1 * vel[0] / c2;
+  invP[1][2] = nV[1] * kapm1 * vel[1] / c2;
+  invP[1][3] = nV[0] / rrho + nV[1] * kapm1 * vel[2] / c2;
+  invP[1][4] = -nV[1] * kapm1 / c2;
+  invP[2][0] = nV[2] - nV[1] * vel[0] / rrho + nV[0] * vel[1] / rrho - nV[2] * 0.5 * kapm1 * vSqu / c2;
+  invP[2][1] = nV[1] / rrho + nV[2] * kapm1 * vel[0] / c2;
+  invP[2][2] = -nV[0] / rrho + nV[2] * kapm1 * vel[1] / c2;
+  invP[2][3] = nV[2] * kapm1 * vel[2] / c2;
+  invP[2][4] = -nV[2] * kapm1 / c2;
+  invP[3][0] = -(nV[0] * vel[0] + nV[1] * vel[1] + nV[2] * vel[2]) / rrho + 0.5 * kapm1 * vSqu / rhoxc;
+  invP[3][1] = nV[0] / rrho - kapm1 * vel[0] / rhoxc;
+  invP[3][2] = nV[1] / rrho - kapm1 * vel[1] / rhoxc;
+  invP[3][3] = nV[2] / rrho - kapm1 * vel[2] / rhoxc;
+  invP[3][4] = kapm1 / rhoxc;
+  invP[4][0] = (nV[0] * vel[0] + nV[1] * vel[1] + nV[2] * vel[2]) / rrho + 0.5 * kapm1 * vSqu / rhoxc;
+  invP[4][1] = -nV[0] / rrho - kapm1 * vel[0] / rhoxc;
+  invP[4][2] = -nV[1] / rrho - kapm1 * vel[1] / rhoxc;
+  invP[4][3] = -nV[2] / rrho - kapm1 * vel[2] / rhoxc;
+  invP[4][4] = kapm1 / rhoxc;
+}
+
+// calcEulerFluxMatrices_Roe, UgpWithCvCompFlow.cpp:3382-3476 (nScal = 0)
+void jac_roe(double (*A_L)[5], double (*A_R)[5], double rhoL, const double *uL, double pL, double gammaL,
+             double rhoR, const double *uR, double pR, double gammaR, double area, const double *nVec) {
+  double uLuL = dot3(uL, uL);
+  double uRuR = dot3(uR, uR);
+  double Rrho = sqrt(rhoR / rhoL);
+  double rhoRoe = Rrho * rhoL;
+  double tmp = 1.0 / (1.0 + Rrho);
+  double velRoe[3];
+  for (int i = 0; i < 3; i++) velRoe[i] = tmp * (uL[i] + uR[i] * Rrho);
+  double uRoe = dot3(velRoe, nVec);
+  double gamPdivRho = tmp * ((gammaL * pL / rhoL + 0.5 * (gammaL - 1.0) * uLuL) +
+                             (gammaR * pR / rhoR + 0.5 * (gammaR - 1.0) * uRuR) * Rrho);
+  double cRoe = sqrt(gamPdivRho - ((gammaL + gammaR) * 0.5 - 1.0) * 0.5 * dot3(velRoe, velRoe));
+  double nVecArea[3];
+  for (int i = 0; i < 3; i++) nVecArea[i] = nVec[i] * area;
+  double Lambda[5];
+  for (int i = 0; i < 3; i++) Lambda[i] = uRoe;
+  Lambda[3] = uRoe + cRoe;
+  Lambda[4] = uRoe - cRoe;
+  double P[5][5], InvP[5][5];
+  p_matrix(P, velRoe, cRoe, rhoRoe, nVec, 0.5 * (gammaL + gammaR));
+  inv_p_matrix(InvP, velRoe, cRoe, rhoRoe, nVec, 0.5 * (gammaL + gammaR));
+  if (A_L != NULL) jac_a(A_L, uL, pL, rhoL, nVecArea, 0.5 * (gammaL + gammaR));
+  if (A_R != NULL) jac_a(A_R, uR, pR, rhoR, nVecArea, 0.5 * (gammaL + gammaR));
+  for (int i = 0; i < 5; i++)
+    for (int j = 0; j < 5; j++) {
+      double Proj_ModJac = 0.0;
+      for (int k = 0; k < 5; k++) Proj_ModJac += P[i][k] * fabs(Lambda[k]) * InvP[k][j];
+      if (A_L != NULL) A_L[i][j] = 0.5 * A_L[i][j] + 0.5 * Proj_ModJac * area;
+      if (A_R != NULL) A_R[i][j] = 0.5 * A_R[i][j] - 0.5 * Proj_ModJac * area;
+    }
+}
+
+// ludeco / lusolv, UgpWithCv.cpp:2489-2540
+void ludeco(double (*A)[5], int order) {
+  for (int jc = 1; jc < order; jc++) A[0][jc] /= A[0][0];
+  int jrjc = 0;
+  for (;;) {
+    jrjc++;
+    int jrjcm1 = jrjc - 1;
+    int jrjcp1 = jrjc + 1;
+    for (int jr = jrjc; jr < order; jr++) {
+      double sum = A[jr][jrjc];
+      for (int jm = 0; jm <= jrjcm1; jm++) sum -= A[jr][jm] * A[jm][jrjc];
+      A[jr][jrjc] = sum;
+    }
+    if (jrjc == (order - 1)) return;
+    for (int jc = jrjcp1; jc < order; jc++) {
+      double sum = A[jrjc][jc];
+      for (int jm = 0; jm <= jrjcm1; jm++) sum -= A[jrjc][jm] * A[jm][jc];
+      A[jrjc][jc] = sum / A[jrjc][jrjc];
+    }
+  }
+}
+
+void lusolv(double (*A)[5], double *c, int order) {
+  c[0] = c[0] / A[0][0];
+  for (int jr = 1; jr < order; jr++) {
+    int jrm1 = jr - 1;
+    double sum = c[jr];
+    for (int jm = 0; jm <= jrm1; jm++) sum -= A[jr][jm] * c[jm];
+    c[jr] = sum / A[jr][jr];
+  }
+  for (int jrjr = 1; jrjr < order; jrjr++) {
+    int jr = (order - 1) - jrjr;
+    int jrp1 = jr + 1;
+    double sum = c[jr];
+    for (int jmjm = jrp1; jmjm < order; jmjm++) {
+      int jm = (order - 1) - jmjm + jrp1;
+      sum -= A[jr][jm] * c[jm];
+    }
+    c[jr] = sum;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// context: the member arrays of UgpWithCvCompFlow/JoeWithModels the path uses
+// ---------------------------------------------------------------------------
+struct or_ctx {
+  or_mesh m;
+  or_params p;
+  int ncv, nfa, nfa_b, nnz;
+  std::vector<double> rho, rhou, rhoE, vel, press, temp, enthalpy, sos;
+  std::vector<double> rho_bfa, T_bfa, vel_bfa, p_bfa, h_bfa;
+  std::vector<double> grad_rho, grad_p, grad_u, alpha_rho, local_dt;
+  std::vector<double> rhs_rho, rhs_rhou, rhs_rhoE, A, dq, resid;
+  double cfl;
+  int step;
+  bool have_press;
+};
+
+namespace {
+
+// calcStateVariables, UgpWithCvCompFlow.h:1662-1694 (kine = NULL; gamma, RoM
+// constant: initialHookScalarRansCombModel sets them to GAMMA, R_gas)
+void calc_state(or_ctx &c) {
+  const double gam = c.p.gamma, R = c.p.R_gas;
+  for (int icv = 0; icv < c.ncv; icv++) {
+    for (int i = 0; i < 3; i++) c.vel[3 * icv + i] = c.rhou[3 * icv + i] / c.rho[icv];
+    double kinecv = 0.0;
+    double pr = (gam - 1.0) * (c.rhoE[icv] - 0.5 * dot3(&c.rhou[3 * icv], &c.rhou[3 * icv]) / c.rho[icv] - c.rho[icv] * kinecv);
+    if (pr > 0.0) c.press[icv] = pr;
+    c.temp[icv] = c.press[icv] / (c.rho[icv] * R);
+    c.enthalpy[icv] = gam * R / (gam - 1.0) * c.temp[icv];
+    c.sos[icv] = sqrt(gam * c.press[icv] / c.rho[icv]);
+  }
+}
+
+// setBC, JoeWithModels.cpp:3857-4300, with ComputeBCProperties_T
+// (UgpWithCvCompFlow.h:1783-1811) and rho_bfa from the EOS (:4292-4293).
+void set_bc(or_ctx &c) {
+  const double GAMMA = c.p.gamma, R_gas = c.p.R_gas;
+  const or_mesh &m = c.m;
+  for (int iz = 0; iz < m.nzone; iz++) {
+    const double *par = &m.zone_par[8 * iz];
+    const int f0 = m.zone_ifa[2 * iz], f1 = m.zone_ifa[2 * iz + 1];
+    switch (m.zone_bc[iz]) {
+      case OR_BC_CBC: {
+        double u_bc[3] = {par[0], par[1], par[2]}, T_bc = par[3], p_bc = par[4];
+        for (int ifa = f0; ifa <= f1; ifa++) {
+          int icv0 = m.cvofa[2 * ifa];
+          double nVec[3];
+          norm3(nVec, &m.fa_normal[3 * ifa]);
+          if (dot3(u_bc, nVec) > 0.0) {
+            double velMagN = dot3(&c.vel[3 * icv0], nVec);
+            double mach = fabs(velMagN) / c.sos[icv0];
+            c.T_bfa[ifa] = c.temp[icv0];
+            for (int i = 0; i < 3; i++) c.vel_bfa[3 * ifa + i] = c.vel[3 * icv0 + i];
+            c.p_bfa[ifa] = mach >= 1.0 ? c.press[icv0] : p_bc;
+          } else {
+            c.T_bfa[ifa] = T_bc;
+            for (int i = 0; i < 3; i++) c.vel_bfa[3 * ifa + i] = u_bc[i];
+            c.p_bfa[ifa] = p_bc;
+          }
+        }
+        for (int ifa = f0; ifa <= f1; ifa++) c.h_bfa[ifa] = GAMMA * R_gas / (GAMMA - 1.0) * c.T_bfa[ifa];
+        break;
+      }
+      case OR_BC_CBC_SUBSONIC_INLET: {
+        double angleU[3] = {par[0], par[1], par[2]}, Ttot = par[3], ptot = par[4];
+        for (int ifa = f0; ifa <= f1; ifa++) {
+          int icv0 = m.cvofa[2 * ifa];
+          double u[3] = {c.rhou[3 * icv0] / c.rho[icv0], c.rhou[3 * icv0 + 1] / c.rho[icv0], c.rhou[3 * icv0 + 2] / c.rho[icv0]};
+          double wPow2 = dot3(u, u);
+          double vel = sqrt(wPow2);
+          for (int i = 0; i < 3; i++) c.vel_bfa[3 * ifa + i] = angleU[i] * vel;
+          c.T_bfa[ifa] = Ttot;
+          c.p_bfa[ifa] = ptot;
+        }
+        for (int ifa = f0; ifa <= f1; ifa++) c.h_bfa[ifa] = GAMMA * R_gas / (GAMMA - 1.0) * c.T_bfa[ifa];
+        for (int ifa = f0; ifa <= f1; ifa++) {
+          double wPow2 = dot3(&c.vel_bfa[3 * ifa], &c.vel_bfa[3 * ifa]);
+          c.h_bfa[ifa] -= 0.5 * wPow2;
+        }
+        for (int ifa = f0; ifa <= f1; ifa++) c.T_bfa[ifa] = c.h_bfa[ifa] * (GAMMA - 1.0) / (GAMMA * R_gas);
+        for (int ifa = f0; ifa <= f1; ifa++) c.p_bfa[ifa] = ptot * pow(c.T_bfa[ifa] / Ttot, GAMMA / (GAMMA - 1.0));
+        break;
+      }
+      case OR_BC_CBC_SUBSONIC_OUTLET: {
+        double p_bc = par[0];
+        for (int ifa = f0; ifa <= f1; ifa++) {
+          int icv0 = m.cvofa[2 * ifa];
+          c.p_bfa[ifa] = p_bc;
+          for (int i = 0; i < 3; i++) c.vel_bfa[3 * ifa + i] = c.rhou[3 * icv0 + i] / c.rho[icv0];
+          c.T_bfa[ifa] = c.temp[icv0];
+        }
+        for (int ifa = f0; ifa <= f1; ifa++) c.h_bfa[ifa] = GAMMA * R_gas / (GAMMA - 1.0) * c.T_bfa[ifa];
+        break;
+      }
+      case OR_BC_SYMMETRY: {
+        for (int ifa = f0; ifa <= f1; ifa++) {
+          int icv0 = m.cvofa[2 * ifa];
+          double nVec[3];
+          norm3(nVec, &m.fa_normal[3 * ifa]);
+          double u0[3] = {c.vel[3 * icv0], c.vel[3 * icv0 + 1], c.vel[3 * icv0 + 2]};
+          double un = dot3(nVec, u0);
+          for (int i = 0; i < 3; i++) c.vel_bfa[3 * ifa + i] = u0[i] - 1.0 * un * nVec[i];
+          c.T_bfa[ifa] = c.temp[icv0];
+          c.p_bfa[ifa] = c.press[icv0];
+        }
+        for (int ifa = f0; ifa <= f1; ifa++) c.h_bfa[ifa] = GAMMA * R_gas / (GAMMA - 1.0) * c.T_bfa[ifa];
+        break;
+      }
+      case OR_BC_NEUMANN: {
+        for (int ifa = f0; ifa <= f1; ifa++) {
+          int icv0 = m.cvofa[2 * ifa];
+          for (int i = 0; i < 3; i++) c.vel_bfa[3 * ifa + i] = c.rhou[3 * icv0 + i] / c.rho[icv0];
+          c.T_bfa[ifa] = c.temp[icv0];
+          c.p_bfa[ifa] = c.press[icv0];
+        }
+        for (int ifa = f0; ifa <= f1; ifa++) c.h_bfa[ifa] = GAMMA * R_gas / (GAMMA - 1.0) * c.T_bfa[ifa];
+        break;
+      }
+      case OR_BC_WALL: {
+        double T_bc = par[0];
+        for (int ifa = f0; ifa <= f1; ifa++) {
+          int icv0 = m.cvofa[2 * ifa];
+          for (int i = 0; i < 3; i++) c.vel_bfa[3 * ifa + i] = 0.0;
+          c.T_bfa[ifa] = T_bc > 0.0 ? T_bc : c.temp[icv0];
+          c.p_bfa[ifa] = c.press[icv0];
+        }
+        for (int ifa = f0; ifa <= f1; ifa++) c.h_bfa[ifa] = GAMMA * R_gas / (GAMMA - 1.0) * c.T_bfa[ifa];
+        break;
+      }
+      case OR_BC_FARFIELD: {
+        double u_bc[3] = {par[0], par[1], par[2]};
+        double p_bc = par[3], rho_bc = par[4], gamma_bc = par[5];
+        double gm1 = gamma_bc - 1.0;
+        double ovgm1 = 1.0 / gm1;
+        double c_bc = sqrt(gamma_bc * p_bc / rho_bc);
+        double s_bc = pow(rho_bc, gamma_bc) / p_bc;
+        for (int ifa = f0; ifa <= f1; ifa++) {
+          int icv0 = m.cvofa[2 * ifa];
+          double nVec[3];
+          norm3(nVec, &m.fa_normal[3 * ifa]);
+          double qn_bc = dot3(nVec, u_bc);
+          double vn_bc = qn_bc;
+          double qne = dot3(nVec, &c.vel[3 * icv0]);
+          double ce = sqrt(GAMMA * c.press[icv0] / c.rho[icv0]);
+          double ac1 = vn_bc > -c_bc ? qne + 2.0 * ovgm1 * ce : qn_bc + 2.0 * ovgm1 * c_bc;
+          double ac2 = vn_bc > c_bc ? qne - 2.0 * ovgm1 * ce : qn_bc - 2.0 * ovgm1 * c_bc;
+          double qnf = 0.5 * (ac1 + ac2);
+          double cf = 0.25 * (ac1 - ac2) * gm1;
+          double velf[3], sf;
+          if (vn_bc > 0) {
+            for (int i = 0; i < 3; i++) velf[i] = c.vel[3 * icv0 + i] + (qnf - qne) * nVec[i];
+            sf = pow(c.rho[icv0], GAMMA) / c.press[icv0];
+          } else {
+            for (int i = 0; i < 3; i++) velf[i] = u_bc[i] + (qnf - qn_bc) * nVec[i];
+            sf = s_bc;
+          }
+          c.rho_bfa[ifa] = pow((sf * cf * cf / GAMMA), ovgm1);
+          for (int i = 0; i < 3; i++) c.vel_bfa[3 * ifa + i] = velf[i];
+          c.p_bfa[ifa] = c.rho_bfa[ifa] * cf * cf / GAMMA;
+          c.T_bfa[ifa] = c.p_bfa[ifa] / (R_gas * c.rho_bfa[ifa]);
+        }
+        for (int ifa = f0; ifa <= f1; ifa++) c.h_bfa[ifa] = GAMMA * R_gas / (GAMMA - 1.0) * c.T_bfa[ifa];
+        break;
+      }
+    }
+  }
+  for (int ifa = 0; ifa < c.nfa_b; ifa++) c.rho_bfa[ifa] = c.p_bfa[ifa] / (R_gas * c.T_bfa[ifa]);
+}
+
+// calcDt (MAX_BASED), UgpWithCvCompFlow.cpp:5-60, 100-105. mul_fa = mut_fa = 0.
+double calc_dt(or_ctx &c) {
+  const or_mesh &m = c.m;
+  double dt_min = 1.0e20;
+  for (int icv = 0; icv < c.ncv; icv++) {
+    double lambdaMax = 0.0;
+    double cs = sqrt(c.p.gamma * c.press[icv] / c.rho[icv]);
+    for (int foc = m.faocv_i[icv]; foc <= m.faocv_i[icv + 1] - 1; foc++) {
+      int ifa = m.faocv_v[foc];
+      double nVec[3];
+      double area = norm3(nVec, &m.fa_normal[3 * ifa]);
+      double Uk = dot3(&c.rhou[3 * icv], nVec) / c.rho[icv];
+      double lambda = (fabs(Uk) + cs) * area;
+      double lamdiff = 4.0 * (0.0 + 0.0) / c.rho[icv] * area * area / m.cv_volume[icv];
+      lambda = max(lambda, lamdiff);
+      lambdaMax = max(lambdaMax, lambda);
+    }
+    double dt_cv = c.cfl * m.cv_volume[icv] / lambdaMax;
+    dt_min = min(dt_min, dt_cv);
+    c.local_dt[icv] = dt_cv;
+  }
+  if (c.p.time_step_min)
+    for (int icv = 0; icv < c.ncv; icv++) c.local_dt[icv] = dt_min;
+  return dt_min;
+}
+
+// LS gradients with boundary values, UgpWithCv.h:1009-1083
+void grad_scalar_ls(or_ctx &c, double *g, const double *phi, const double *phi_fa) {
+  const or_mesh &m = c.m;
+  for (int icv = 0; icv < c.ncv; icv++) {
+    int noc_f = m.nbocv_i[icv], noc_l = m.nbocv_i[icv + 1] - 1;
+    for (int i = 0; i < 3; i++) g[3 * icv + i] = m.nbocv_lsg[3 * noc_f + i] * phi[icv];
+    for (int noc = noc_f + 1; noc <= noc_l; noc++) {
+      int nb = m.nbocv_v[noc];
+      for (int i = 0; i < 3; i++) g[3 * icv + i] += m.nbocv_lsg[3 * noc + i] * phi[nb];
+    }
+  }
+  for (int ifa = 0; ifa < c.nfa_b; ifa++) {
+    int icv = m.cvofa[2 * ifa];
+    for (int i = 0; i < 3; i++) g[3 * icv + i] += m.bfa_lsg[3 * ifa + i] * phi_fa[ifa];
+  }
+}
+
+void grad_vector_ls(or_ctx &c, double *g, const double *u, const double *u_fa) {
+  const or_mesh &m = c.m;
+  for (int icv = 0; icv < c.ncv; icv++) {
+    int noc_f = m.nbocv_i[icv], noc_l = m.nbocv_i[icv + 1] - 1;
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) g[9 * icv + 3 * i + j] = m.nbocv_lsg[3 * noc_f + j] * u[3 * icv + i];
+    for (int noc = noc_f + 1; noc <= noc_l; noc++) {
+      int nb = m.nbocv_v[noc];
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) g[9 * icv + 3 * i + j] += m.nbocv_lsg[3 * noc + j] * u[3 * nb + i];
+    }
+  }
+  for (int ifa = 0; ifa < c.nfa_b; ifa++) {
+    int icv = m.cvofa[2 * ifa];
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) g[9 * icv + 3 * i + j] += m.bfa_lsg[3 * ifa + j] * u_fa[3 * ifa + i];
+  }
+}
+
+// Green-Gauss gradients (w0 = w1 = 0.5), UgpWithCv.h:1091-1188
+void grad_scalar_gg(or_ctx &c, double *g, const double *p, const double *p_fa) {
+  const or_mesh &m = c.m;
+  for (int k = 0; k < 3 * c.ncv; k++) g[k] = 0.0;
+  for (int ifa = c.nfa_b; ifa < c.nfa; ifa++) {
+    int icv0 = m.cvofa[2 * ifa], icv1 = m.cvofa[2 * ifa + 1];
+    double w0 = 0.5, w1 = 0.5;
+    double faceValue = (w1 * p[icv0] + w0 * p[icv1]) / (w0 + w1);
+    for (int i = 0; i < 3; i++) {
+      g[3 * icv0 + i] += faceValue * m.fa_normal[3 * ifa + i];
+      g[3 * icv1 + i] -= faceValue * m.fa_normal[3 * ifa + i];
+    }
+  }
+  for (int ifa = 0; ifa < c.nfa_b; ifa++) {
+    int icv = m.cvofa[2 * ifa];
+    for (int i = 0; i < 3; i++) g[3 * icv + i] += p_fa[ifa] * m.fa_normal[3 * ifa + i];
+  }
+  for (int icv = 0; icv < c.ncv; icv++)
+    for (int i = 0; i < 3; i++) g[3 * icv + i] /= m.cv_volume[icv];
+}
+
+void grad_vector_gg(or_ctx &c, double *g, const double *u, const double *u_fa) {
+  const or_mesh &m = c.m;
+  for (int k = 0; k < 9 * c.ncv; k++) g[k] = 0.0;
+  for (int ifa = c.nfa_b; ifa < c.nfa; ifa++) {
+    int icv0 = m.cvofa[2 * ifa], icv1 = m.cvofa[2 * ifa + 1];
+    double w0 = 0.5, w1 = 0.5;
+    for (int i = 0; i < 3; i++) {
+      double faceValue = (w1 * u[3 * icv0 + i] + w0 * u[3 * icv1 + i]) / (w0 + w1);
+      for (int j = 0; j < 3; j++) {
+        g[9 * icv0 + 3 * i + j] += faceValue * m.fa_normal[3 * ifa + j];
+        g[9 * icv1 + 3 * i + j] -= faceValue * m.fa_normal[3 * ifa + j];
+      }
+    }
+  }
+  for (int ifa = 0; ifa < c.nfa_b; ifa++) {
+    int icv = m.cvofa[2 * ifa];
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) g[9 * icv + 3 * i + j] += u_fa[3 * ifa + i] * m.fa_normal[3 * ifa + j];
+  }
+  for (int icv = 0; icv < c.ncv; icv++)
+    for (int k = 0; k < 9; k++) g[9 * icv + k] /= m.cv_volume[icv];
+}
+
+// smooth Barth-Jespersen limiters, UgpWithCv.h:881-1000 (BJ undefined, :877)
+void limit_scalar(or_ctx &c, double *g, const double *phi, const double *ref, double *alpha) {
+  const or_mesh &m = c.m;
+  const double eps = c.p.eps_limiter;
+  for (int icv = 0; icv < c.ncv; icv++) {
+    double phiMax = phi[icv], phiMin = phiMax;
+    for (int noc = m.nbocv_i[icv] + 1; noc <= m.nbocv_i[icv + 1] - 1; noc++) {
+      int nb = m.nbocv_v[noc];
+      phiMax = max(phiMax, phi[nb]);
+      phiMin = min(phiMin, phi[nb]);
+    }
+    double alfa = 1.0;
+    if ((phiMax - phiMin) > 1.0e-12) {
+      for (int foc = m.faocv_i[icv]; foc <= m.faocv_i[icv + 1] - 1; foc++) {
+        int ifa = m.faocv_v[foc];
+        if (ifa >= c.nfa_b) {
+          double r0[3];
+          for (int i = 0; i < 3; i++) r0[i] = m.x_fa[3 * ifa + i] - m.x_cv[3 * icv + i];
+          double phifa = phi[icv] + dot3(r0, &g[3 * icv]);
+          double eps2 = max(eps * fabs(ref[icv]), 1.0e-8);  // SDWLSClippingScalar
+          eps2 = pow(eps2, 2.0);
+          double dm = phifa - phi[icv];
+          double dp = (phifa - phi[icv]) >= 0.0 ? phiMax - phi[icv] : phiMin - phi[icv];
+          alfa = min(alfa, (dp * dp + 2.0 * dm * dp + eps2) / (dp * dp + 2 * dm * dm + dm * dp + eps2));
+        }
+      }
+    } else
+      alfa = 0.0;
+    for (int i = 0; i < 3; i++) g[3 * icv + i] *= alfa;
+    if (alpha) alpha[icv] = alfa;
+  }
+}
+
+void limit_vector(or_ctx &c, double *g, const double *phi, const double *ref) {
+  const or_mesh &m = c.m;
+  const double eps = c.p.eps_limiter;
+  for (int icv = 0; icv < c.ncv; icv++) {
+    for (int i = 0; i < 3; i++) {
+      double phiMax = phi[3 * icv + i], phiMin = phiMax;
+      for (int noc = m.nbocv_i[icv] + 1; noc <= m.nbocv_i[icv + 1] - 1; noc++) {
+        int nb = m.nbocv_v[noc];
+        phiMax = max(phiMax, phi[3 * nb + i]);
+        phiMin = min(phiMin, phi[3 * nb + i]);
+      }
+      double alfa = 1.0;
+      if ((phiMax - phiMin) > 1.0e-8) {
+        for (int foc = m.faocv_i[icv]; foc <= m.faocv_i[icv + 1] - 1; foc++) {
+          int ifa = m.faocv_v[foc];
+          if (ifa >= c.nfa_b) {
+            double r0[3];
+            for (int k = 0; k < 3; k++) r0[k] = m.x_fa[3 * ifa + k] - m.x_cv[3 * icv + k];
+            double phifa = phi[3 * icv + i] + dot3(r0, &g[9 * icv + 3 * i]);
+            double eps2 = pow(eps * ref[icv], 2.0);
+            double dm = phifa - phi[3 * icv + i];
+            double dp = (phifa - phi[3 * icv + i]) >= 0.0 ? phiMax - phi[3 * icv + i] : phiMin - phi[3 * icv + i];
+            alfa = min(alfa, (dp * dp + 2.0 * dm * dp + eps2) / (dp * dp + 2 * dm * dm + dm * dp + eps2));
+          }
+        }
+      } else
+        alfa = 0.0;
+      for (int j = 0; j < 3; j++) g[9 * icv + 3 * i + j] *= alfa;
+    }
+  }
+}
+
+void face_flux(const or_ctx &c, double F[5], double rho0, const double *u0, double p0, double rho1, const double *u1,
+               double p1, double area, const double *nVec) {
+  const double g = c.p.gamma;
+  if (c.p.scheme == OR_ROE)
+    flux_roe(F, rho0, u0, p0, g, rho1, u1, p1, g, area, nVec);
+  else if (c.p.scheme == OR_AUSM)
+    flux_ausm(F, rho0, u0, p0, g, 0.0, rho1, u1, p1, g, 0.0, area, nVec);
+  else
+    flux_hllc(F, rho0, u0, p0, g, 0.0, rho1, u1, p1, g, 0.0, area, nVec);
+}
+
+void face_jac(const or_ctx &c, double (*Apl)[5], double (*Ami)[5], double rho0, const double *u0, double p0,
+              double rho1, const double *u1, double p1, double area, const double *nVec) {
+  const double g = c.p.gamma;
+  if (c.p.scheme == OR_ROE)
+    jac_roe(Apl, Ami, rho0, u0, p0, g, rho1, u1, p1, g, area, nVec);
+  else  // HLLC, and AUSM which uses the HLLC Jacobian (JoeWithModels.cpp:3057-3061)
+    jac_hllc(Apl, Ami, rho0, u0, p0, g, 0.0, rho1, u1, p1, g, 0.0, area, nVec);
+}
+
+// calcEulerFlux, JoeWithModels.cpp:2621-3589 (nScal = 0, no shock fix)
+void calc_euler_flux(or_ctx &c, double *A) {
+  const or_mesh &m = c.m;
+  const bool snd = c.p.second_order != 0;
+  if (snd) {
+    // calcCvVectorGrad / calcCvScalarGrad (JoeWithModels.cpp:2682-2694)
+    if (c.p.grad_scheme == OR_GRAD_LS) {
+      grad_vector_ls(c, c.grad_u.data(), c.vel.data(), c.vel_bfa.data());
+    } else {
+      grad_vector_gg(c, c.grad_u.data(), c.vel.data(), c.vel_bfa.data());
+    }
+    if (c.p.limiter) limit_vector(c, c.grad_u.data(), c.vel.data(), c.sos.data());
+    for (int icv = 0; icv < c.ncv; icv++) c.alpha_rho[icv] = 1.0;
+    if (c.p.grad_scheme == OR_GRAD_LS)
+      grad_scalar_ls(c, c.grad_rho.data(), c.rho.data(), c.rho_bfa.data());
+    else
+      grad_scalar_gg(c, c.grad_rho.data(), c.rho.data(), c.rho_bfa.data());
+    if (c.p.limiter) limit_scalar(c, c.grad_rho.data(), c.rho.data(), c.rho.data(), c.alpha_rho.data());
+    if (c.p.grad_scheme == OR_GRAD_LS)
+      grad_scalar_ls(c, c.grad_p.data(), c.press.data(), c.p_bfa.data());
+    else
+      grad_scalar_gg(c, c.grad_p.data(), c.press.data(), c.p_bfa.data());
+    if (c.p.limiter) limit_scalar(c, c.grad_p.data(), c.press.data(), c.press.data(), NULL);
+  }
+  double F[5], Apl[5][5], Ami[5][5];
+  for (int ifa = c.nfa_b; ifa < c.nfa; ifa++) {
+    int icv0 = m.cvofa[2 * ifa], icv1 = m.cvofa[2 * ifa + 1];
+    int noc00 = 0, noc01 = 0, noc11 = 0, noc10 = 0;
+    if (A) {  // getImplDependencyIndex, UgpWithCv.h:745-755
+      noc00 = m.nbocv_i[icv0];
+      noc01 = noc00;
+      while (m.nbocv_v[++noc01] != icv1) {}
+      noc11 = m.nbocv_i[icv1];
+      noc10 = noc11;
+      while (m.nbocv_v[++noc10] != icv0) {}
+    }
+    double nVec[3];
+    double area = norm3(nVec, &m.fa_normal[3 * ifa]);
+    double rho0 = c.rho[icv0], p0 = c.press[icv0];
+    double u0[3] = {c.vel[3 * icv0], c.vel[3 * icv0 + 1], c.vel[3 * icv0 + 2]};
+    double rho1 = c.rho[icv1], p1 = c.press[icv1];
+    double u1[3] = {c.vel[3 * icv1], c.vel[3 * icv1 + 1], c.vel[3 * icv1 + 2]};
+    if (snd) {
+      double r0[3], r1[3];
+      for (int i = 0; i < 3; i++) {
+        r0[i] = m.x_fa[3 * ifa + i] - m.x_cv[3 * icv0 + i];
+        r1[i] = m.x_fa[3 * ifa + i] - m.x_cv[3 * icv1 + i];
+      }
+      rho0 += dot3(r0, &c.grad_rho[3 * icv0]);
+      p0 += dot3(r0, &c.grad_p[3 * icv0]);
+      if ((p0 <= 0.0) || (rho0 <= 0.0)) {
+        p0 = c.press[icv0];
+        rho0 = c.rho[icv0];
+      } else
+        for (int i = 0; i < 3; i++) u0[i] += dot3(r0, &c.grad_u[9 * icv0 + 3 * i]);
+      rho1 += dot3(r1, &c.grad_rho[3 * icv1]);
+      p1 += dot3(r1, &c.grad_p[3 * icv1]);
+      if ((p1 <= 0.0) || (rho1 <= 0.0)) {
+        p1 = c.press[icv1];
+        rho1 = c.rho[icv1];
+      } else
+        for (int i = 0; i < 3; i++) u1[i] += dot3(r1, &c.grad_u[9 * icv1 + 3 * i]);
+    }
+    face_flux(c, F, rho0, u0, p0, rho1, u1, p1, area, nVec);
+    c.rhs_rho[icv0] -= F[0];
+    for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= F[1 + i];
+    c.rhs_rhoE[icv0] -= F[4];
+    c.rhs_rho[icv1] += F[0];
+    for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv1 + i] += F[1 + i];
+    c.rhs_rhoE[icv1] += F[4];
+    if (A) {
+      face_jac(c, Apl, Ami, c.rho[icv0], &c.vel[3 * icv0], c.press[icv0], c.rho[icv1], &c.vel[3 * icv1],
+               c.press[icv1], area, nVec);
+      for (int i = 0; i < 5; i++)
+        for (int j = 0; j < 5; j++) {
+          A[25 * noc00 + 5 * i + j] += Apl[i][j];
+          A[25 * noc01 + 5 * i + j] += Ami[i][j];
+        }
+      for (int i = 0; i < 5; i++)
+        for (int j = 0; j < 5; j++) {
+          A[25 * noc11 + 5 * i + j] -= Ami[i][j];
+          A[25 * noc10 + 5 * i + j] -= Apl[i][j];
+        }
+    }
+  }
+  // boundary zones in faZoneList order (JoeWithModels.cpp:3112-3569)
+  for (int iz = 0; iz < m.nzone; iz++) {
+    const int bc = m.zone_bc[iz];
+    for (int ifa = m.zone_ifa[2 * iz]; ifa <= m.zone_ifa[2 * iz + 1]; ifa++) {
+      int icv0 = m.cvofa[2 * ifa];
+      double nVec[3];
+      double area = norm3(nVec, &m.fa_normal[3 * ifa]);
+      const double *ub = &c.vel_bfa[3 * ifa];
+      if (bc == OR_BC_SYMMETRY || bc == OR_BC_WALL) {
+        face_flux(c, F, c.rho_bfa[ifa], ub, c.p_bfa[ifa], c.rho_bfa[ifa], ub, c.p_bfa[ifa], area, nVec);
+        if (bc == OR_BC_SYMMETRY && c.p.symm_bc == OR_SYMM_ANALYTICAL) {
+          for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= c.p_bfa[ifa] * nVec[i] * area;
+          c.rhs_rho[icv0] -= 0.0;
+          c.rhs_rhoE[icv0] -= 0.0;
+        } else {
+          c.rhs_rho[icv0] -= F[0];
+          for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= F[1 + i];
+          c.rhs_rhoE[icv0] -= F[4];
+        }
+        if (A) {
+          int noc00 = m.nbocv_i[icv0];
+          if (bc == OR_BC_WALL || c.p.symm_bc == OR_SYMM_ORIGINAL_NUMERIC) {
+            face_jac(c, Apl, NULL, c.rho_bfa[ifa], ub, c.p_bfa[ifa], c.rho_bfa[ifa], ub, c.p_bfa[ifa], area, nVec);
+            for (int k = 0; k < 25; k++) A[25 * noc00 + k] += Apl[k / 5][k % 5];
+          } else if (c.p.symm_bc == OR_SYMM_NEW_NUMERIC) {
+            face_jac(c, Apl, Ami, c.rho_bfa[ifa], ub, c.p_bfa[ifa], c.rho_bfa[ifa], ub, c.p_bfa[ifa], area, nVec);
+            for (int k = 0; k < 25; k++) A[25 * noc00 + k] += Apl[k / 5][k % 5] + Ami[k / 5][k % 5];
+          } else {  // ANALYTICAL (JoeWithModels.cpp:3243-3269)
+            double S[5][5];
+            double a2 = c.p.gamma - 1.0;
+            double phi = 0.5 * a2 * (ub[0] * ub[0] + ub[1] * ub[1] + ub[2] * ub[2]);
+            for (int i = 0; i < 5; i++) {
+              S[0][i] = 0.0;
+              S[4][i] = 0.0;
+            }
+            for (int i = 0; i < 3; i++) {
+              S[i + 1][0] = phi * nVec[i] * area;
+              for (int j = 0; j < 3; j++) S[i + 1][j + 1] = -a2 * ub[j] * nVec[i] * area;
+              S[i + 1][4] = a2 * nVec[i] * area;
+            }
+            for (int k = 0; k < 25; k++) A[25 * noc00 + k] += S[k / 5][k % 5];
+          }
+        }
+      } else {
+        double rho0 = c.rho[icv0], p0 = c.press[icv0];
+        double u0[3] = {c.vel[3 * icv0], c.vel[3 * icv0 + 1], c.vel[3 * icv0 + 2]};
+        if (snd) {
+          double r0[3];
+          for (int i = 0; i < 3; i++) r0[i] = m.x_fa[3 * ifa + i] - m.x_cv[3 * icv0 + i];
+          rho0 += dot3(r0, &c.grad_rho[3 * icv0]);
+          p0 += dot3(r0, &c.grad_p[3 * icv0]);
+          if ((p0 <= 0.0) || (rho0 <= 0.0)) {
+            p0 = c.press[icv0];
+            rho0 = c.rho[icv0];
+          } else
+            for (int i = 0; i < 3; i++) u0[i] += dot3(r0, &c.grad_u[9 * icv0 + 3 * i]);
+        }
+        face_flux(c, F, rho0, u0, p0, c.rho_bfa[ifa], ub, c.p_bfa[ifa], area, nVec);
+        c.rhs_rho[icv0] -= F[0];
+        for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= F[1 + i];
+        c.rhs_rhoE[icv0] -= F[4];
+        if (A) {
+          face_jac(c, Apl, NULL, c.rho[icv0], &c.vel[3 * icv0], c.press[icv0], c.rho_bfa[ifa], ub, c.p_bfa[ifa], area,
+                   nVec);
+          int noc00 = m.nbocv_i[icv0];
+          for (int k = 0; k < 25; k++) A[25 * noc00 + k] += Apl[k / 5][k % 5];
+        }
+      }
+    }
+  }
+}
+
+// calcRhs, JoeWithModels.cpp:2564-2594, with the default sourceHook
+// (gravity, JoeWithModels.h:111-129).
+void calc_rhs(or_ctx &c, double *A) {
+  for (int icv = 0; icv < c.ncv; icv++) {
+    c.rhs_rho[icv] = 0.0;
+    for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv + i] = 0.0;
+    c.rhs_rhoE[icv] = 0.0;
+  }
+  calc_euler_flux(c, A);
+  const double *g = c.p.gravity;
+  for (int icv = 0; icv < c.ncv; icv++) {
+    double vol = c.m.cv_volume[icv];
+    c.rhs_rhou[3 * icv + 0] += vol * c.rho[icv] * g[0];
+    c.rhs_rhou[3 * icv + 1] += vol * c.rho[icv] * g[1];
+    c.rhs_rhou[3 * icv + 2] += vol * c.rho[icv] * g[2];
+    c.rhs_rhoE[icv] += vol * (g[0] * c.rhou[3 * icv] + g[1] * c.rhou[3 * icv + 1] + g[2] * c.rhou[3 * icv + 2]);
+    if (A) {
+      double *D = &A[25 * c.m.nbocv_i[icv]];
+      D[5 * 1 + 0] -= vol * g[0];
+      D[5 * 2 + 0] -= vol * g[1];
+      D[5 * 3 + 0] -= vol * g[2];
+      D[5 * 4 + 1] -= vol * g[0];
+      D[5 * 4 + 2] -= vol * g[1];
+      D[5 * 4 + 3] -= vol * g[2];
+    }
+  }
+}
+
+void mat_vec(const or_ctx &c, double *res, const double *A, const double *phi) {  // UgpWithCv.cpp:2580-2596
+  const or_mesh &m = c.m;
+  for (int icv = 0; icv < c.ncv; icv++) {
+    for (int i = 0; i < 5; i++) res[5 * icv + i] = 0.0;
+    for (int noc = m.nbocv_i[icv]; noc <= m.nbocv_i[icv + 1] - 1; noc++) {
+      int nb = m.nbocv_v[noc];
+      for (int i = 0; i < 5; i++)
+        for (int j = 0; j < 5; j++) res[5 * icv + i] += A[25 * noc + 5 * i + j] * phi[5 * nb + j];
+    }
+  }
+}
+
+// solveCvVectorR5Bcgstab, UgpWithCv.cpp:2660-2876 (one rank: halo updates are no-ops)
+int bcgstab(const or_ctx &c, double *phi, const double *A, const double *rhs, double zeroAbs, double zeroRel,
+            int maxiter) {
+  const int ncv = c.ncv, n5 = 5 * ncv;
+  std::vector<double> res(n5), res0(n5), p(n5, 0.0), v(n5, 0.0), t(n5), s(n5), phat(n5, 0.0), shat(n5, 0.0);
+  std::vector<double> LU(25 * (size_t)ncv);
+  for (int icv = 0; icv < ncv; icv++) {
+    std::memcpy(&LU[25 * icv], &A[25 * c.m.nbocv_i[icv]], 25 * sizeof(double));
+    ludeco((double(*)[5]) & LU[25 * icv], 5);
+  }
+  double rho = 1.0, alpha = 1.0, omega = 1.0;
+  mat_vec(c, res.data(), A, phi);
+  for (int k = 0; k < n5; k++) {
+    res[k] = (rhs[k] - res[k]);
+    res0[k] = res[k];
+  }
+  double res0_max = 0.0;
+  for (int icv = 0; icv < ncv; icv++) {
+    double tmp[5];
+    for (int i = 0; i < 5; i++) tmp[i] = res[5 * icv + i];
+    lusolv((double(*)[5]) & LU[25 * icv], tmp, 5);
+    for (int i = 0; i < 5; i++) res0_max = max(res0_max, fabs(tmp[i]));
+  }
+  int iter = 0, done = 0;
+  while (done == 0) {
+    iter++;
+    double rho_prime = -omega * rho;
+    rho = 0.0;
+    for (int k = 0; k < n5; k++) rho += res[k] * res0[k];
+    if (fabs(rho_prime) < 1.0E-20) rho_prime = 1.0E-20;
+    double beta = alpha * rho / rho_prime;
+    for (int k = 0; k < n5; k++) p[k] = res[k] - beta * (p[k] - omega * v[k]);
+    for (int icv = 0; icv < ncv; icv++) {
+      for (int i = 0; i < 5; i++) phat[5 * icv + i] = p[5 * icv + i];
+      lusolv((double(*)[5]) & LU[25 * icv], &phat[5 * icv], 5);
+    }
+    mat_vec(c, v.data(), A, phat.data());
+    double gamma = 0.0;
+    for (int k = 0; k < n5; k++) gamma += v[k] * res0[k];
+    if (fabs(gamma) < 1.0E-20) gamma = 1.0E-20;
+    alpha = rho / gamma;
+    for (int k = 0; k < n5; k++) s[k] = res[k] - alpha * v[k];
+    for (int icv = 0; icv < ncv; icv++) {
+      for (int i = 0; i < 5; i++) shat[5 * icv + i] = s[5 * icv + i];
+      lusolv((double(*)[5]) & LU[25 * icv], &shat[5 * icv], 5);
+    }
+    mat_vec(c, t.data(), A, shat.data());
+    double b0 = 0.0, b1 = 0.0;
+    for (int k = 0; k < n5; k++) {
+      b0 += s[k] * t[k];
+      b1 += t[k] * t[k];
+    }
+    omega = b0 / (b1 + 1.0E-20);
+    for (int k = 0; k < n5; k++) phi[k] += alpha * phat[k] + omega * shat[k];
+    mat_vec(c, res.data(), A, phi);
+    for (int k = 0; k < n5; k++) res[k] = rhs[k] - res[k];
+    double res_max = 0.0;
+    for (int icv = 0; icv < ncv; icv++) {
+      double tmp[5];
+      for (int i = 0; i < 5; i++) tmp[i] = res[5 * icv + i];
+      lusolv((double(*)[5]) & LU[25 * icv], tmp, 5);
+      for (int i = 0; i < 5; i++) res_max = max(res_max, fabs(tmp[i]));
+    }
+    if ((res_max <= zeroAbs) || (res_max / (res0_max + 1.0E-12) <= zeroRel))
+      done = 1;
+    else if (iter > maxiter)
+      done = 2;
+  }
+  return iter;
+}
+
+// state + BC refresh done at the start of runBackwardEuler and after each update
+void refresh(or_ctx &c) {
+  calc_state(c);
+  set_bc(c);
+}
+
+}  // namespace
+
+extern "C" {
+
+or_ctx *or_ctx_create(const or_mesh *m, const or_params *p) {
+  or_ctx *c = new or_ctx();
+  c->m = *m;
+  c->p = *p;
+  c->ncv = m->ncv;
+  c->nfa = m->nfa;
+  c->nfa_b = m->nfa_b;
+  c->nnz = m->nnz;
+  const size_t n = m->ncv, nb = m->nfa_b;
+  c->rho.assign(n, 0.0);
+  c->rhou.assign(3 * n, 0.0);
+  c->rhoE.assign(n, 0.0);
+  c->vel.assign(3 * n, 0.0);
+  c->press.assign(n, 0.0);
+  c->temp.assign(n, 0.0);
+  c->enthalpy.assign(n, 0.0);
+  c->sos.assign(n, 0.0);
+  c->rho_bfa.assign(nb, 0.0);
+  c->T_bfa.assign(nb, 0.0);
+  c->vel_bfa.assign(3 * nb, 0.0);
+  c->p_bfa.assign(nb, 0.0);
+  c->h_bfa.assign(nb, 0.0);
+  c->grad_rho.assign(3 * n, 0.0);
+  c->grad_p.assign(3 * n, 0.0);
+  c->grad_u.assign(9 * n, 0.0);
+  c->alpha_rho.assign(n, 1.0);
+  c->local_dt.assign(n, 0.0);
+  c->rhs_rho.assign(n, 0.0);
+  c->rhs_rhou.assign(3 * n, 0.0);
+  c->rhs_rhoE.assign(n, 0.0);
+  c->A.assign(25 * (size_t)m->nnz, 0.0);
+  c->dq.assign(5 * n, 0.0);
+  c->resid.assign(5, 0.0);
+  c->cfl = p->cfl;
+  c->step = 0;
+  return c;
+}
+
+void or_ctx_destroy(or_ctx *c) { delete c; }
+
+void or_set_state(or_ctx *c, const double *rho, const double *rhou, const double *rhoE) {
+  std::memcpy(c->rho.data(), rho, sizeof(double) * c->ncv);
+  std::memcpy(c->rhou.data(), rhou, sizeof(double) * 3 * c->ncv);
+  std::memcpy(c->rhoE.data(), rhoE, sizeof(double) * c->ncv);
+}
+
+void or_get_state(const or_ctx *c, double *rho, double *rhou, double *rhoE) {
+  std::memcpy(rho, c->rho.data(), sizeof(double) * c->ncv);
+  std::memcpy(rhou, c->rhou.data(), sizeof(double) * 3 * c->ncv);
+  std::memcpy(rhoE, c->rhoE.data(), sizeof(double) * c->ncv);
+}
+
+void or_residual(or_ctx *c, int implicit) {
+  refresh(*c);
+  calc_dt(*c);
+  if (implicit) std::fill(c->A.begin(), c->A.end(), 0.0);
+  calc_rhs(*c, implicit ? c->A.data() : NULL);
+}
+
+// one pass of the loop body of runBackwardEuler (JoeWithModels.cpp:975-1158),
+// assuming state/BC are current (as after refresh).
+static int step_once(or_ctx &c) {
+  c.step++;
+  if ((c.step >= c.p.cfl_after) && (c.step % c.p.cfl_interval == 0) && (c.cfl < c.p.cfl_max)) c.cfl *= c.p.cfl_factor;
+  calc_dt(c);
+  std::fill(c.A.begin(), c.A.end(), 0.0);
+  std::fill(c.dq.begin(), c.dq.end(), 0.0);
+  calc_rhs(c, c.A.data());
+  std::vector<double> rhs(5 * (size_t)c.ncv);
+  for (int icv = 0; icv < c.ncv; icv++) {
+    rhs[5 * icv + 0] = c.p.under_relax * c.rhs_rho[icv];
+    rhs[5 * icv + 1] = c.p.under_relax * c.rhs_rhou[3 * icv + 0];
+    rhs[5 * icv + 2] = c.p.under_relax * c.rhs_rhou[3 * icv + 1];
+    rhs[5 * icv + 3] = c.p.under_relax * c.rhs_rhou[3 * icv + 2];
+    rhs[5 * icv + 4] = c.p.under_relax * c.rhs_rhoE[icv];
+    double tmp = c.m.cv_volume[icv] / (c.local_dt[icv]);
+    for (int i = 0; i < 5; i++) c.A[25 * c.m.nbocv_i[icv] + 6 * i] += tmp;
+  }
+  int it = bcgstab(c, c.dq.data(), c.A.data(), rhs.data(), c.p.ls_abs, c.p.ls_rel, c.p.ls_maxiter);
+  for (int icv = 0; icv < c.ncv; icv++) {
+    c.rho[icv] += c.dq[5 * icv + 0];
+    c.rhou[3 * icv + 0] += c.dq[5 * icv + 1];
+    c.rhou[3 * icv + 1] += c.dq[5 * icv + 2];
+    c.rhou[3 * icv + 2] += c.dq[5 * icv + 3];
+    c.rhoE[icv] += c.dq[5 * icv + 4];
+  }
+  refresh(c);
+  for (int i = 0; i < 5; i++) c.resid[i] = 0.0;
+  for (int icv = 0; icv < c.ncv; icv++) {
+    c.resid[0] += fabs(c.rhs_rho[icv]);
+    for (int i = 0; i < 3; i++) c.resid[i + 1] += fabs(c.rhs_rhou[3 * icv + i]);
+    c.resid[4] += fabs(c.rhs_rhoE[icv]);
+  }
+  return it;
+}
+
+int or_step(or_ctx *c) {
+  if (c->step == 0) refresh(*c);
+  return step_once(*c);
+}
+
+int or_run(or_ctx *c, int max_steps, double *hist) {
+  refresh(*c);
+  // doneSolver(1e20) before the loop, then after every step (JoeWithModels.h:42-95)
+  auto done = [&](double r) {
+    if ((c->p.nsteps >= 0) && (c->step >= c->p.nsteps)) return true;
+    if (r < c->p.resid_energ_th) return true;
+    if (r != r) return true;
+    return false;
+  };
+  int n = 0;
+  if (done(1.0e20)) return 0;
+  while (n < max_steps) {
+    step_once(*c);
+    if (hist)
+      for (int i = 0; i < 5; i++) hist[5 * n + i] = c->resid[i];
+    n++;
+    if (done(c->resid[4])) break;
+  }
+  return n;
+}
+
+long long or_get(const or_ctx *c, const char *name, double *out) {
+  const std::vector<double> *v = nullptr;
+  std::string n(name);
+  if (n == "RHS_RHO") v = &c->rhs_rho;
+  else if (n == "RHS_RHOU") v = &c->rhs_rhou;
+  else if (n == "RHS_RHOE") v = &c->rhs_rhoE;
+  else if (n == "A") v = &c->A;
+  else if (n == "LOCAL_DT") v = &c->local_dt;
+  else if (n == "GRAD_RHO") v = &c->grad_rho;
+  else if (n == "GRAD_P") v = &c->grad_p;
+  else if (n == "GRAD_U") v = &c->grad_u;
+  else if (n == "ALPHA_RHO") v = &c->alpha_rho;
+  else if (n == "VEL") v = &c->vel;
+  else if (n == "PRESS") v = &c->press;
+  else if (n == "TEMP") v = &c->temp;
+  else if (n == "ENTHALPY") v = &c->enthalpy;
+  else if (n == "SOS") v = &c->sos;
+  else if (n == "RHO_BFA") v = &c->rho_bfa;
+  else if (n == "VEL_BFA") v = &c->vel_bfa;
+  else if (n == "P_BFA") v = &c->p_bfa;
+  else if (n == "T_BFA") v = &c->T_bfa;
+  else if (n == "H_BFA") v = &c->h_bfa;
+  else if (n == "DQ") v = &c->dq;
+  else if (n == "RESID") v = &c->resid;
+  else if (n == "RHO") v = &c->rho;
+  else if (n == "RHOU") v = &c->rhou;
+  else if (n == "RHOE") v = &c->rhoE;
+  if (!v) return -1;
+  if (out) std::memcpy(out, v->data(), sizeof(double) * v->size());
+  return (long long)v->size();
+}
+
+double or_cfl(const or_ctx *c) { return c->cfl; }
+int or_nsteps_done(const or_ctx *c) { return c->step; }
+
+/* One linear solve on the current A/rhs of the last or_residual call with
+ * the first-step scaling (JoeWithModels.cpp:1011-1026); result in DQ. */
+int or_linear_solve(or_ctx *c) {
+  std::vector<double> rhs(5 * (size_t)c->ncv);
+  for (int icv = 0; icv < c->ncv; icv++) {
+    rhs[5 * icv + 0] = c->p.under_relax * c->rhs_rho[icv];
+    for (int i = 0; i < 3; i++) rhs[5 * icv + 1 + i] = c->p.under_relax * c->rhs_rhou[3 * icv + i];
+    rhs[5 * icv + 4] = c->p.under_relax * c->rhs_rhoE[icv];
+    double tmp = c->m.cv_volume[icv] / (c->local_dt[icv]);
+    for (int i = 0; i < 5; i++) c->A[25 * c->m.nbocv_i[icv] + 6 * i] += tmp;
+  }
+  std::fill(c->dq.begin(), c->dq.end(), 0.0);
+  return bcgstab(*c, c->dq.data(), c->A.data(), rhs.data(), c->p.ls_abs, c->p.ls_rel, c->p.ls_maxiter);
+}
+
+}  // extern "C"

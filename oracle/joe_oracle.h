@@ -1,0 +1,86 @@
+/* CPU restatement of Joe's steady-solve hot path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load this library, and only as the checker or the timed CPU baseline. The
+ * product (libjoemi) never links or calls it.
+ *
+ * Layout and numbering are the reference's own (Ugp/UgpWithCv members, AoS):
+ * cvofa[nfa][2], faocv/nbocv CSR, x_fa/fa_normal[nfa][3], x_cv[ncv][3],
+ * nbocv_lsg[nnz][3], bfa_lsg[nfa_b][3]; rhou[ncv][3]; A[nnz][5][5].
+ *
+ * Every routine follows the reference file:line given at its definition in
+ * joe_oracle.cpp and is pinned against the reference build (oracle/_ref) and
+ * the committed golden vectors under tests/golden/.
+ */
+#ifndef JOE_ORACLE_H
+#define JOE_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* boundary-condition keywords of Joe.in (JoeWithModels.cpp:3857-4300) */
+enum {
+  OR_BC_CBC = 1,
+  OR_BC_CBC_SUBSONIC_INLET = 2,
+  OR_BC_CBC_SUBSONIC_OUTLET = 3,
+  OR_BC_SYMMETRY = 4,
+  OR_BC_NEUMANN = 5,
+  OR_BC_WALL = 6,
+  OR_BC_FARFIELD = 7
+};
+enum { OR_HLLC = 0, OR_ROE = 1, OR_AUSM = 2 };
+enum { OR_GRAD_GG = 0, OR_GRAD_LS = 1 };
+enum { OR_SYMM_ORIGINAL_NUMERIC = 0, OR_SYMM_NEW_NUMERIC = 1, OR_SYMM_ANALYTICAL = 2 };
+
+typedef struct {
+  int ncv, nfa, nfa_b, nnz;
+  const int *cvofa, *faocv_i, *faocv_v, *nbocv_i, *nbocv_v;
+  const double *x_fa, *fa_normal, *x_cv, *cv_volume, *nbocv_lsg, *bfa_lsg;
+  int nzone;                 /* boundary zones, in faZoneList order */
+  const int *zone_bc;        /* OR_BC_* */
+  const int *zone_ifa;       /* [nzone][2] first, last face */
+  const double *zone_par;    /* [nzone][8] keyword values (see or_ctx_create) */
+} or_mesh;
+
+typedef struct {
+  double gamma, R_gas, mu_ref;
+  double gravity[3];
+  int scheme, second_order, grad_scheme, limiter, symm_bc;
+  double eps_limiter;        /* GRAD_RECONSTRUCTION EPS */
+  double cfl;
+  int time_step_min;         /* 1: CFL_MIN (global dt) */
+  double under_relax;
+  int ls_maxiter;
+  double ls_abs, ls_rel;
+  int cfl_after, cfl_interval;
+  double cfl_factor, cfl_max;
+  int nsteps;
+  double resid_energ_th;
+} or_params;
+
+typedef struct or_ctx or_ctx;
+
+or_ctx *or_ctx_create(const or_mesh *m, const or_params *p);
+void or_ctx_destroy(or_ctx *c);
+/* state in reference layout; rhou is [ncv][3] */
+void or_set_state(or_ctx *c, const double *rho, const double *rhou, const double *rhoE);
+void or_get_state(const or_ctx *c, double *rho, double *rhou, double *rhoE);
+/* calcStateVariables + setBC + calcDt + calcRhs (implicit: also A). */
+void or_residual(or_ctx *c, int implicit);
+/* one backward-Euler step (JoeWithModels.cpp:975-1158); returns BiCGStab iterations */
+int or_step(or_ctx *c);
+/* runBackwardEuler from the current state; returns steps taken. */
+int or_run(or_ctx *c, int max_steps, double *resid_hist /* [max_steps][5] or NULL */);
+/* named arrays: RHS_RHO RHS_RHOU RHS_RHOE A LOCAL_DT GRAD_RHO GRAD_P GRAD_U
+ * ALPHA_RHO VEL PRESS TEMP ENTHALPY SOS RHO_BFA VEL_BFA P_BFA T_BFA H_BFA DQ
+ * RESID (5) ; returns element count or -1 */
+long long or_get(const or_ctx *c, const char *name, double *out);
+/* current CFL and step counter */
+double or_cfl(const or_ctx *c);
+int or_nsteps_done(const or_ctx *c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,232 @@
+"""ctypes front end of the CPU restatement (liboracle.so) -- TEST INFRASTRUCTURE.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
+module, as the checker or the timed CPU baseline; the product never does.
+
+Mesh arrays are the reference's own (as dumped by oracle/_ref/joe_ref_dump or
+exported by libjoemi): a dict with COUNTS, CVOFA, FAOCV_I/V, NBOCV_I/V, X_FA,
+FA_NORMAL, X_CV, CV_VOLUME, NBOCV_LSG, BFA_LSG, ZONES, ZONE_NAMES.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import jmdump
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+BC = {"CBC": 1, "CBC_SUBSONIC_INLET": 2, "CBC_SUBSONIC_OUTLET": 3, "SYMMETRY": 4, "NEUMANN": 5, "WALL": 6,
+      "FARFIELD": 7}
+SCHEME = {"HLLC": 0, "ROE": 1, "AUSM": 2}
+
+
+class _Mesh(C.Structure):
+    _fields_ = [("ncv", C.c_int), ("nfa", C.c_int), ("nfa_b", C.c_int), ("nnz", C.c_int),
+                ("cvofa", C.c_void_p), ("faocv_i", C.c_void_p), ("faocv_v", C.c_void_p),
+                ("nbocv_i", C.c_void_p), ("nbocv_v", C.c_void_p),
+                ("x_fa", C.c_void_p), ("fa_normal", C.c_void_p), ("x_cv", C.c_void_p), ("cv_volume", C.c_void_p),
+                ("nbocv_lsg", C.c_void_p), ("bfa_lsg", C.c_void_p),
+                ("nzone", C.c_int), ("zone_bc", C.c_void_p), ("zone_ifa", C.c_void_p), ("zone_par", C.c_void_p)]
+
+
+class _Params(C.Structure):
+    _fields_ = [("gamma", C.c_double), ("R_gas", C.c_double), ("mu_ref", C.c_double), ("gravity", C.c_double * 3),
+                ("scheme", C.c_int), ("second_order", C.c_int), ("grad_scheme", C.c_int), ("limiter", C.c_int),
+                ("symm_bc", C.c_int), ("eps_limiter", C.c_double), ("cfl", C.c_double), ("time_step_min", C.c_int),
+                ("under_relax", C.c_double), ("ls_maxiter", C.c_int), ("ls_abs", C.c_double), ("ls_rel", C.c_double),
+                ("cfl_after", C.c_int), ("cfl_interval", C.c_int), ("cfl_factor", C.c_double),
+                ("cfl_max", C.c_double), ("nsteps", C.c_int), ("resid_energ_th", C.c_double)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (make -C oracle oracle)")
+        L = C.CDLL(path)
+        L.or_ctx_create.restype = C.c_void_p
+        L.or_ctx_create.argtypes = [C.POINTER(_Mesh), C.POINTER(_Params)]
+        L.or_ctx_destroy.argtypes = [C.c_void_p]
+        L.or_set_state.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.or_get_state.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.or_residual.argtypes = [C.c_void_p, C.c_int]
+        L.or_step.argtypes = [C.c_void_p]
+        L.or_step.restype = C.c_int
+        L.or_run.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.or_run.restype = C.c_int
+        L.or_get.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p]
+        L.or_get.restype = C.c_longlong
+        L.or_cfl.argtypes = [C.c_void_p]
+        L.or_cfl.restype = C.c_double
+        L.or_linear_solve.argtypes = [C.c_void_p]
+        L.or_linear_solve.restype = C.c_int
+        _LIB = L
+    return _LIB
+
+
+def parse_joe_in(text: str) -> dict:
+    """Joe.in as ParamMap reads it (Param.h:60-63, Param.cpp:247-298): tokens
+    split on ' =\\t', '#' starts a comment, repeated keys append; lookup uses
+    the first entry."""
+    params: dict = {}
+    for line in text.splitlines():
+        s = line.split("#", 1)[0]
+        toks = [t for t in s.replace("=", " ").replace("\t", " ").split(" ") if t]
+        if toks:
+            params.setdefault(toks[0], []).append(toks)
+    return params
+
+
+def _after(tokens, key, default=None):
+    if key in tokens[1:-1]:
+        return tokens[tokens.index(key, 1) + 1]
+    if default is None:
+        raise KeyError(key)
+    return default
+
+
+def make_params(joe_in: str) -> tuple[_Params, dict]:
+    P = parse_joe_in(joe_in)
+
+    def get(k, default=None):
+        if k in P:
+            return P[k][0][1] if len(P[k][0]) > 1 else ""
+        if default is None:
+            raise KeyError(k)
+        return default
+
+    p = _Params()
+    p.gamma = float(get("GAMMA", "1.4"))
+    p.R_gas = float(get("P_REF")) / float(get("RHO_REF")) / float(get("T_REF"))
+    p.mu_ref = float(get("MU_REF", "0.0"))
+    if p.mu_ref > 0:
+        raise NotImplementedError("viscous flux not restated")
+    for i, k in enumerate(("GX", "GY", "GZ")):
+        p.gravity[i] = float(get(k, "0.0"))
+    p.scheme = SCHEME[get("SPACE_INTEGRATION", "HLLC")]
+    p.second_order = 1 if "SPATIAL_SECOND_ORDER" in P else 0
+    gr = P.get("GRAD_RECONSTRUCTION", [["GRAD_RECONSTRUCTION", "SCHEME", "GREENGAUSS", "LIMITER", "YES", "EPS",
+                                         "0.001"]])[0]
+    sch = _after(gr, "SCHEME")
+    p.grad_scheme = {"GREENGAUSS": 0, "LEASTSQUARES": 1}[sch]
+    p.limiter = 1 if _after(gr, "LIMITER") in ("YES", "BARTH_JESPERSEN_MOD") else 0
+    p.eps_limiter = float(_after(gr, "EPS"))
+    p.symm_bc = {"ORIGINAL_NUMERIC": 0, "NEW_NUMERIC": 1, "ANALYTICAL": 2}[get("SYMM_BC", "ORIGINAL_NUMERIC")]
+    if "CFL" in P:
+        p.cfl, p.time_step_min = float(get("CFL")), 0
+    else:
+        p.cfl, p.time_step_min = float(get("CFL_MIN")), 1
+    p.under_relax = float(get("UNDER_RELAXATION", "0.3"))
+    ls = P.get("LINEAR_SOLVER_NS_TRESHOLDS", [["X", "MAX_ITER", "30", "ABS_RESID", "1.0e-8", "REL_RESID", "1.0e-2"]])[0]
+    p.ls_maxiter = int(_after(ls, "MAX_ITER"))
+    p.ls_abs = float(_after(ls, "ABS_RESID"))
+    p.ls_rel = float(_after(ls, "REL_RESID"))
+    cr = P.get("CFL_RAMP", [["X", "AFTER_ITER", "100", "INTERVAL_ITER", "10", "FACTOR_CFL", "1.0", "MAX_CFL",
+                             "100.0"]])[0]
+    p.cfl_after = int(_after(cr, "AFTER_ITER"))
+    p.cfl_interval = int(_after(cr, "INTERVAL_ITER"))
+    p.cfl_factor = float(_after(cr, "FACTOR_CFL"))
+    p.cfl_max = float(_after(cr, "MAX_CFL"))
+    p.nsteps = int(get("NSTEPS", "100"))
+    p.resid_energ_th = float(get("RESID_ENERG_TH", "0.0"))
+    return p, P
+
+
+class Oracle:
+    """A CPU solve context over reference-layout mesh arrays."""
+
+    def __init__(self, mesh: dict, joe_in: str):
+        L = lib()
+        self.params, P = make_params(joe_in)
+        counts = mesh["COUNTS"]
+        self.nno, self.nfa, self.nfa_b, self.ncv, _ncvg, self.nnz = (int(x) for x in counts[:6])
+        self._keep = {}
+
+        def arr(name, dt):
+            a = np.ascontiguousarray(mesh[name], dtype=dt)
+            self._keep[name] = a
+            return a.ctypes.data
+
+        zl = [z for z in jmdump.zones(mesh) if z[1] == 1]
+        zbc = np.zeros(len(zl), np.int32)
+        zifa = np.zeros((len(zl), 2), np.int32)
+        zpar = np.zeros((len(zl), 8), np.float64)
+        for i, (name, _k, f, l) in enumerate(zl):
+            toks = P[name][0]
+            kw = toks[1]
+            zbc[i] = BC[kw]
+            zifa[i] = (f, l)
+            if kw == "WALL":
+                zpar[i, 0] = float(toks[toks.index("TEMP") + 1]) if "TEMP" in toks else 0.0
+            else:
+                vals = [float(t) for t in toks[2:]]
+                zpar[i, :len(vals)] = vals
+        self._keep.update(zbc=zbc, zifa=zifa, zpar=zpar)
+        m = _Mesh()
+        m.ncv, m.nfa, m.nfa_b, m.nnz = self.ncv, self.nfa, self.nfa_b, self.nnz
+        m.cvofa = arr("CVOFA", np.int32)
+        m.faocv_i = arr("FAOCV_I", np.int32)
+        m.faocv_v = arr("FAOCV_V", np.int32)
+        m.nbocv_i = arr("NBOCV_I", np.int32)
+        m.nbocv_v = arr("NBOCV_V", np.int32)
+        m.x_fa = arr("X_FA", np.float64)
+        m.fa_normal = arr("FA_NORMAL", np.float64)
+        m.x_cv = arr("X_CV", np.float64)
+        m.cv_volume = arr("CV_VOLUME", np.float64)
+        m.nbocv_lsg = arr("NBOCV_LSG", np.float64)
+        m.bfa_lsg = arr("BFA_LSG", np.float64)
+        m.nzone = len(zl)
+        m.zone_bc = zbc.ctypes.data
+        m.zone_ifa = zifa.ctypes.data
+        m.zone_par = zpar.ctypes.data
+        self._mesh = m
+        self.ctx = L.or_ctx_create(C.byref(m), C.byref(self.params))
+
+    def __del__(self):
+        if getattr(self, "ctx", None) and _LIB is not None:
+            _LIB.or_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def set_state(self, rho, rhou, rhoE):
+        rho = np.ascontiguousarray(rho, np.float64)
+        rhou = np.ascontiguousarray(rhou, np.float64)
+        rhoE = np.ascontiguousarray(rhoE, np.float64)
+        lib().or_set_state(self.ctx, rho.ctypes.data, rhou.ctypes.data, rhoE.ctypes.data)
+
+    def state(self):
+        rho = np.empty(self.ncv)
+        rhou = np.empty((self.ncv, 3))
+        rhoE = np.empty(self.ncv)
+        lib().or_get_state(self.ctx, rho.ctypes.data, rhou.ctypes.data, rhoE.ctypes.data)
+        return rho, rhou, rhoE
+
+    def residual(self, implicit: bool = True):
+        lib().or_residual(self.ctx, 1 if implicit else 0)
+
+    def linear_solve(self) -> int:
+        return lib().or_linear_solve(self.ctx)
+
+    def step(self) -> int:
+        return lib().or_step(self.ctx)
+
+    def run(self, max_steps: int):
+        hist = np.zeros((max_steps, 5))
+        n = lib().or_run(self.ctx, max_steps, hist.ctypes.data)
+        return n, hist[:n]
+
+    def get(self, name: str) -> np.ndarray:
+        n = lib().or_get(self.ctx, name.encode(), None)
+        if n < 0:
+            raise KeyError(name)
+        out = np.empty(n)
+        lib().or_get(self.ctx, name.encode(), out.ctypes.data)
+        return out
+
+    @property
+    def cfl(self) -> float:
+        return lib().or_cfl(self.ctx)
