@@ -1,0 +1,149 @@
+/* joemi -- MI355X-native steady-solve hot path of the Joe compressible
+ * finite-volume solver, behind a plain C ABI.
+ *
+ * The reference (btracey/myjoe, "MUM/Joe") exposes this path as C++
+ * inheritance: a case class derived from JoeWithModels overrides hook virtuals
+ * and calls run(), which drives runBackwardEuler / calcRhs /
+ * solveCoupledLinSysNS on the Ugp mesh. Each entry point below names the
+ * reference interface it stands in for. All pointers are host pointers unless
+ * the name says "device"; arrays use the reference's own layouts and
+ * numbering (Ugp/UgpWithCv members): rhou[ncv][3], A[nbocv_s][5][5], etc.
+ *
+ * Errors never cross the ABI as exceptions: every call returns a jm_status
+ * and jm_last_error() describes the last failure of the calling thread.
+ */
+#ifndef JOEMI_H
+#define JOEMI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  JM_OK = 0,
+  JM_ERR_ARG = 1,          /* bad argument / size */
+  JM_ERR_MESH = 2,         /* mesh cannot be built (connectivity, geometry) */
+  JM_ERR_PARAM = 3,        /* Joe.in parameter missing or not understood */
+  JM_ERR_HIP = 4,          /* HIP runtime failure, including no GPU */
+  JM_ERR_UNSUPPORTED = 5,  /* option of the reference not provided here */
+  JM_ERR_STATE = 6         /* call out of order (e.g. no state set) */
+} jm_status;
+
+enum { JM_FP64 = 0, JM_MIXED = 1 };            /* Jacobian/Krylov precision */
+enum { JM_RHS_IMPLICIT = 1 };                   /* jm_calc_rhs flags */
+
+typedef struct jm_mesh jm_mesh;
+typedef struct jm_ctx jm_ctx;
+
+/* text of the last error on this thread ("" if none) */
+const char *jm_last_error(void);
+const char *jm_version(void);
+
+/* ---------------------------------------------------------------------------
+ * Mesh. Replaces UgpWithCv::initializeFromRestartFile for .msh input
+ * (UgpWithCv.h:316-347): MshFilter::initUgpMin (MshFilter.cpp:44-90),
+ * Ugp::init -> redistReconnectReorder (Ugp.cpp:1341-4277, serial numbering),
+ * UgpWithCv::init (buildNbocv, calcGeometry; UgpWithCv.cpp:768-1095) and
+ * calcLsgCoeff (UgpWithCv.cpp:1783-1913). Host-side; the result is uploaded by
+ * jm_ctx_create.
+ * ------------------------------------------------------------------------- */
+
+/* Faces in mesh-file order, oriented as after the reader's flip rule
+ * (MshFilter.cpp:453-472): node order gives a normal from face_cells[2f] to
+ * face_cells[2f+1]; boundary faces have face_cells[2f+1] = -1. */
+jm_status jm_mesh_from_faces(int nno, const double *x_no, int nfa, const int64_t *face_ptr, const int *face_nodes,
+                             const int *face_cells, const int *face_zone, int ncv, int nzone, const int *zone_ids,
+                             const char *const *zone_names, jm_mesh **out);
+/* Fluent ASCII .msh, as MshFilter reads it */
+jm_status jm_mesh_read_msh(const char *path, jm_mesh **out);
+
+/* A host that already holds the Ugp arrays (the reference's own objects)
+ * hands them over without renumbering: UgpWithCv members cvofa, faocv_i/v,
+ * nbocv_i/v, x_fa, fa_normal, x_cv, cv_volume, nbocv_lsg_coeff,
+ * boundary_fa_lsg_coeff (UgpWithCv.h:201-207) and faZoneList (Ugp.h:132-310)
+ * in list order. */
+typedef struct {
+  int ncv, nfa, nfa_b;
+  const int *cvofa, *faocv_i, *faocv_v, *nbocv_i, *nbocv_v;
+  const double *x_fa, *fa_normal, *x_cv, *cv_volume, *nbocv_lsg, *bfa_lsg;
+  int nzone;
+  const char *const *zone_names;
+  const int *zone_kind; /* 1 boundary, 6 internal (Ugp.h:41-55) */
+  const int *zone_ifa_f, *zone_ifa_l;
+} jm_ugp_view;
+jm_status jm_mesh_from_ugp(const jm_ugp_view *v, jm_mesh **out);
+
+void jm_mesh_destroy(jm_mesh *m);
+/* counts[0..7] = nno, nfa, nfa_b, ncv, nnz (nbocv_s), nzone, faocv_s, noofa_s */
+jm_status jm_mesh_counts(const jm_mesh *m, int64_t counts[8]);
+/* copy a mesh array out by its reference name: X_NO NOOFA_I NOOFA_V CVOFA
+ * FAOCV_I FAOCV_V NBOCV_I NBOCV_V (int32) X_FA FA_NORMAL X_CV CV_VOLUME
+ * NBOCV_LSG BFA_LSG (float64). out may be NULL to query *count. */
+jm_status jm_mesh_array(const jm_mesh *m, const char *name, void *out, int64_t *count);
+/* zone i in faZoneList order */
+jm_status jm_mesh_zone(const jm_mesh *m, int i, char *name, int name_len, int *kind, int *ifa_f, int *ifa_l);
+
+/* ---------------------------------------------------------------------------
+ * Solve context: the per-solve state of UgpWithCvCompFlow / JoeWithModels,
+ * resident on the current HIP device. Re-entrant: no process-wide statics
+ * (the reference keeps function statics in calcDt, setBC and the BiCGStab,
+ * UgpWithCv.cpp:2664-2673). joe_in is the text of a Joe.in file
+ * (UgpWithCvCompFlow::init, UgpWithCvCompFlow.h:57-502). stream may be NULL
+ * (the default stream) or a hipStream_t.
+ * ------------------------------------------------------------------------- */
+jm_status jm_ctx_create(const jm_mesh *m, const char *joe_in, int precision, void *stream, jm_ctx **out);
+void jm_ctx_destroy(jm_ctx *c);
+
+/* conservative state in the reference layout (rho[ncv], rhou[ncv][3],
+ * rhoE[ncv]); stands in for writing the registered RHO/RHOU/RHOE arrays */
+jm_status jm_set_state(jm_ctx *c, const double *rho, const double *rhou, const double *rhoE);
+jm_status jm_get_state(jm_ctx *c, double *rho, double *rhou, double *rhoE);
+/* JoeWithModels::initialHook uniform initial condition (JoeWithModels.cpp:156-186) */
+jm_status jm_init_uniform(jm_ctx *c);
+
+/* calcStateVariables + setBC + calcDt + calcRhs (JoeWithModels.cpp:943-1005,
+ * 2564-3589) on the resident state; with JM_RHS_IMPLICIT the block Jacobian
+ * A[nbocv_s][5][5] is assembled too. Asynchronous on the context's stream. */
+jm_status jm_calc_rhs(jm_ctx *c, int flags);
+
+/* copy a result out (synchronises the stream): RHS_RHO RHS_RHOU RHS_RHOE A
+ * LOCAL_DT GRAD_RHO GRAD_P GRAD_U ALPHA_RHO VEL PRESS RHO_BFA VEL_BFA P_BFA
+ * T_BFA H_BFA DQ RESID; out may be NULL to query *count. */
+jm_status jm_get(jm_ctx *c, const char *name, double *out, int64_t *count);
+
+/* One pass of the runBackwardEuler loop (JoeWithModels.cpp:975-1158): CFL
+ * ramp, local time step, implicit residual, under-relaxed block system with
+ * V/dt on the diagonal, block-Jacobi BiCGStab (UgpWithCv.cpp:2660-2876),
+ * update, state and BC refresh, L1 residual. */
+typedef struct {
+  int step;            /* step counter after this call */
+  int lin_iters;       /* BiCGStab iterations */
+  int lin_status;      /* 1 converged, 2 hit MAX_ITER (reference "done" codes) */
+  double cfl;          /* CFL used */
+  double dt_min;       /* minimum local time step */
+  double resid[5];     /* L1 residuals of rho, rhou(3), rhoE */
+} jm_step_info;
+jm_status jm_step(jm_ctx *c, jm_step_info *info);
+
+/* JoeWithModels::run's steady loop until doneSolver (JoeWithModels.h:42-95):
+ * NSTEPS, RESID_ENERG_TH or NaN. resid_hist may be NULL, else [max_steps][5]. */
+jm_status jm_run(jm_ctx *c, int max_steps, double *resid_hist, int *steps_done);
+
+/* boundaryHook bridge (JoeWithModels.h:105): values for a zone whose Joe.in
+ * keyword is HOOK, per face of the zone (T[n], vel[n][3], p[n]). */
+jm_status jm_set_boundary_state(jm_ctx *c, int zone, const double *T, const double *vel, const double *p);
+
+/* device views for callers that keep data on the GPU: the conservative state
+ * as SoA double[5][ncv] (rho, rhou_x, rhou_y, rhou_z, rhoE) and the residual
+ * in the same layout. */
+jm_status jm_device_state(jm_ctx *c, double **q_dev);
+jm_status jm_device_rhs(jm_ctx *c, double **rhs_dev);
+/* wait for all work queued on the context's stream */
+jm_status jm_sync(jm_ctx *c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* JOEMI_H */

@@ -1,0 +1,305 @@
+// C ABI of libjoemi (include/joemi.h): thin wrappers that turn exceptions
+// into status codes and a per-thread error string.
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/joemi.h"
+#include "jm_ctx.hpp"
+#include "jm_mesh.hpp"
+
+struct jm_mesh {
+  jm::Mesh m;
+};
+struct jm_ctx {
+  std::unique_ptr<jm::Ctx> c;
+  const jm::Mesh *mesh = nullptr;
+  std::vector<std::vector<double>> hook;  // per zone: [n][5] T u v w p
+};
+
+namespace {
+thread_local std::string g_err;
+
+template <typename F>
+jm_status guard(F &&f) {
+  try {
+    g_err.clear();
+    return f();
+  } catch (const std::invalid_argument &e) {
+    g_err = e.what();
+    return JM_ERR_PARAM;
+  } catch (const std::logic_error &e) {
+    g_err = e.what();
+    return JM_ERR_STATE;
+  } catch (const std::runtime_error &e) {
+    g_err = e.what();
+    std::string s = e.what();
+    if (s.rfind("HIP", 0) == 0 || s.find("HIP device") != std::string::npos) return JM_ERR_HIP;
+    return JM_ERR_MESH;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return JM_ERR_ARG;
+  } catch (...) {
+    g_err = "unknown error";
+    return JM_ERR_ARG;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+const char *jm_last_error(void) { return g_err.c_str(); }
+const char *jm_version(void) { return "joemi 0.1 (gfx950)"; }
+
+jm_status jm_mesh_from_faces(int nno, const double *x_no, int nfa, const int64_t *face_ptr, const int *face_nodes,
+                             const int *face_cells, const int *face_zone, int ncv, int nzone, const int *zone_ids,
+                             const char *const *zone_names, jm_mesh **out) {
+  return guard([&] {
+    if (!out || !x_no || !face_ptr || !face_nodes || !face_cells || !face_zone || (nzone > 0 && (!zone_ids || !zone_names)))
+      throw std::exception();
+    jm::FaceInput in;
+    in.nno = nno;
+    in.nfa = nfa;
+    in.ncv = ncv;
+    in.x_no = x_no;
+    in.face_ptr = face_ptr;
+    in.face_nodes = face_nodes;
+    in.face_cells = face_cells;
+    in.face_zone = face_zone;
+    for (int i = 0; i < nzone; i++) in.zone_names.emplace_back(zone_ids[i], zone_names[i]);
+    std::unique_ptr<jm_mesh> m(new jm_mesh);
+    jm::build_mesh(in, m->m);
+    *out = m.release();
+    return JM_OK;
+  });
+}
+
+jm_status jm_mesh_read_msh(const char *path, jm_mesh **out) {
+  return guard([&] {
+    if (!path || !out) throw std::exception();
+    std::unique_ptr<jm_mesh> m(new jm_mesh);
+    jm::read_msh(path, m->m);
+    *out = m.release();
+    return JM_OK;
+  });
+}
+
+jm_status jm_mesh_from_ugp(const jm_ugp_view *v, jm_mesh **out) {
+  return guard([&] {
+    if (!v || !out) throw std::exception();
+    std::unique_ptr<jm_mesh> mm(new jm_mesh);
+    jm::Mesh &m = mm->m;
+    m.ncv = v->ncv;
+    m.nfa = v->nfa;
+    m.nfa_b = v->nfa_b;
+    m.cvofa.assign(v->cvofa, v->cvofa + 2 * (size_t)v->nfa);
+    m.faocv_i.assign(v->faocv_i, v->faocv_i + v->ncv + 1);
+    m.faocv_v.assign(v->faocv_v, v->faocv_v + m.faocv_i[v->ncv]);
+    m.nbocv_i.assign(v->nbocv_i, v->nbocv_i + v->ncv + 1);
+    m.nbocv_v.assign(v->nbocv_v, v->nbocv_v + m.nbocv_i[v->ncv]);
+    m.x_fa.assign(v->x_fa, v->x_fa + 3 * (size_t)v->nfa);
+    m.fa_normal.assign(v->fa_normal, v->fa_normal + 3 * (size_t)v->nfa);
+    m.x_cv.assign(v->x_cv, v->x_cv + 3 * (size_t)v->ncv);
+    m.cv_volume.assign(v->cv_volume, v->cv_volume + v->ncv);
+    m.nbocv_lsg.assign(v->nbocv_lsg, v->nbocv_lsg + 3 * (size_t)m.nbocv_i[v->ncv]);
+    m.bfa_lsg.assign(v->bfa_lsg, v->bfa_lsg + 3 * (size_t)v->nfa_b);
+    for (int i = 0; i < v->nzone; i++) {
+      jm::Zone z;
+      z.name = v->zone_names[i];
+      z.kind = v->zone_kind[i];
+      z.ifa_f = v->zone_ifa_f[i];
+      z.ifa_l = v->zone_ifa_l[i];
+      z.index = i;
+      m.zones.push_back(z);
+    }
+    for (int f = 0; f < m.nfa; f++) {
+      int c0 = m.cvofa[2 * (size_t)f], c1 = m.cvofa[2 * (size_t)f + 1];
+      if (c0 < 0 || c0 >= m.ncv || c1 >= m.ncv || (f >= m.nfa_b && c1 < 0) || (f < m.nfa_b && c1 >= 0))
+        throw std::runtime_error("cvofa inconsistent with nfa_b");
+    }
+    *out = mm.release();
+    return JM_OK;
+  });
+}
+
+void jm_mesh_destroy(jm_mesh *m) { delete m; }
+
+jm_status jm_mesh_counts(const jm_mesh *mm, int64_t counts[8]) {
+  return guard([&] {
+    if (!mm || !counts) throw std::exception();
+    const jm::Mesh &m = mm->m;
+    counts[0] = m.nno;
+    counts[1] = m.nfa;
+    counts[2] = m.nfa_b;
+    counts[3] = m.ncv;
+    counts[4] = m.nbocv_s();
+    counts[5] = (int64_t)m.zones.size();
+    counts[6] = (int64_t)m.faocv_v.size();
+    counts[7] = (int64_t)m.noofa_v.size();
+    return JM_OK;
+  });
+}
+
+jm_status jm_mesh_array(const jm_mesh *mm, const char *name, void *out, int64_t *count) {
+  return guard([&] {
+    if (!mm || !name || !count) throw std::exception();
+    const jm::Mesh &m = mm->m;
+    std::string n(name);
+    const std::vector<int> *vi = nullptr;
+    const std::vector<double> *vd = nullptr;
+    if (n == "NOOFA_I") vi = &m.noofa_i;
+    else if (n == "NOOFA_V") vi = &m.noofa_v;
+    else if (n == "CVOFA") vi = &m.cvofa;
+    else if (n == "FAOCV_I") vi = &m.faocv_i;
+    else if (n == "FAOCV_V") vi = &m.faocv_v;
+    else if (n == "NBOCV_I") vi = &m.nbocv_i;
+    else if (n == "NBOCV_V") vi = &m.nbocv_v;
+    else if (n == "X_NO") vd = &m.x_no;
+    else if (n == "X_FA") vd = &m.x_fa;
+    else if (n == "FA_NORMAL") vd = &m.fa_normal;
+    else if (n == "X_CV") vd = &m.x_cv;
+    else if (n == "CV_VOLUME") vd = &m.cv_volume;
+    else if (n == "NBOCV_LSG") vd = &m.nbocv_lsg;
+    else if (n == "BFA_LSG") vd = &m.bfa_lsg;
+    else throw std::invalid_argument(std::string("unknown mesh array ") + name);
+    if (vi) {
+      *count = (int64_t)vi->size();
+      if (out) std::memcpy(out, vi->data(), vi->size() * sizeof(int));
+    } else {
+      *count = (int64_t)vd->size();
+      if (out) std::memcpy(out, vd->data(), vd->size() * sizeof(double));
+    }
+    return JM_OK;
+  });
+}
+
+jm_status jm_mesh_zone(const jm_mesh *mm, int i, char *name, int name_len, int *kind, int *ifa_f, int *ifa_l) {
+  return guard([&] {
+    if (!mm || i < 0 || i >= (int)mm->m.zones.size()) throw std::exception();
+    const jm::Zone &z = mm->m.zones[i];
+    if (name && name_len > 0) {
+      std::strncpy(name, z.name.c_str(), name_len - 1);
+      name[name_len - 1] = 0;
+    }
+    if (kind) *kind = z.kind;
+    if (ifa_f) *ifa_f = z.ifa_f;
+    if (ifa_l) *ifa_l = z.ifa_l;
+    return JM_OK;
+  });
+}
+
+jm_status jm_ctx_create(const jm_mesh *m, const char *joe_in, int precision, void *stream, jm_ctx **out) {
+  return guard([&] {
+    if (!m || !joe_in || !out || (precision != JM_FP64 && precision != JM_MIXED)) throw std::exception();
+    std::unique_ptr<jm_ctx> c(new jm_ctx);
+    c->c.reset(new jm::Ctx(m->m, joe_in, precision, stream));
+    c->mesh = &m->m;
+    *out = c.release();
+    return JM_OK;
+  });
+}
+
+void jm_ctx_destroy(jm_ctx *c) { delete c; }
+
+jm_status jm_set_state(jm_ctx *c, const double *rho, const double *rhou, const double *rhoE) {
+  return guard([&] {
+    if (!c || !rho || !rhou || !rhoE) throw std::exception();
+    c->c->set_state(rho, rhou, rhoE);
+    return JM_OK;
+  });
+}
+
+jm_status jm_get_state(jm_ctx *c, double *rho, double *rhou, double *rhoE) {
+  return guard([&] {
+    if (!c || !rho || !rhou || !rhoE) throw std::exception();
+    c->c->get_state(rho, rhou, rhoE);
+    return JM_OK;
+  });
+}
+
+jm_status jm_init_uniform(jm_ctx *c) {
+  return guard([&] {
+    if (!c) throw std::exception();
+    c->c->init_uniform();
+    return JM_OK;
+  });
+}
+
+jm_status jm_calc_rhs(jm_ctx *c, int flags) {
+  return guard([&] {
+    if (!c) throw std::exception();
+    c->c->calc_rhs((flags & JM_RHS_IMPLICIT) != 0);
+    return JM_OK;
+  });
+}
+
+jm_status jm_get(jm_ctx *c, const char *name, double *out, int64_t *count) {
+  return guard([&] {
+    if (!c || !name || !count) throw std::exception();
+    *count = c->c->get(name, out);
+    return JM_OK;
+  });
+}
+
+jm_status jm_step(jm_ctx *c, jm_step_info *info) {
+  return guard([&] {
+    if (!c) throw std::exception();
+    jm::StepInfo s = c->c->step();
+    if (info) {
+      info->step = s.step;
+      info->lin_iters = s.lin_iters;
+      info->lin_status = s.lin_status;
+      info->cfl = s.cfl;
+      info->dt_min = s.dt_min;
+      for (int k = 0; k < 5; k++) info->resid[k] = s.resid[k];
+    }
+    return JM_OK;
+  });
+}
+
+jm_status jm_run(jm_ctx *c, int max_steps, double *resid_hist, int *steps_done) {
+  return guard([&] {
+    if (!c || max_steps < 0) throw std::exception();
+    int n = c->c->run(max_steps, resid_hist);
+    if (steps_done) *steps_done = n;
+    return JM_OK;
+  });
+}
+
+jm_status jm_set_boundary_state(jm_ctx *c, int zone, const double *T, const double *vel, const double *p) {
+  return guard([&] {
+    if (!c || !T || !vel || !p || zone < 0 || zone >= (int)c->mesh->zones.size()) throw std::exception();
+    const jm::Zone &z = c->mesh->zones[zone];
+    if (z.kind != jm::kZoneBoundary) throw std::invalid_argument("zone is not a boundary zone");
+    c->c->set_boundary_state(z.ifa_f, z.ifa_l - z.ifa_f + 1, T, vel, p);
+    return JM_OK;
+  });
+}
+
+jm_status jm_device_state(jm_ctx *c, double **q) {
+  return guard([&] {
+    if (!c || !q) throw std::exception();
+    *q = c->c->device_state();
+    return JM_OK;
+  });
+}
+
+jm_status jm_device_rhs(jm_ctx *c, double **r) {
+  return guard([&] {
+    if (!c || !r) throw std::exception();
+    *r = c->c->device_rhs();
+    return JM_OK;
+  });
+}
+
+jm_status jm_sync(jm_ctx *c) {
+  return guard([&] {
+    if (!c) throw std::exception();
+    c->c->sync();
+    return JM_OK;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,91 @@
+// Per-solve device context of the steady-solve hot path (host-side view).
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "jm_mesh.hpp"
+
+namespace jm {
+
+enum BcKind : int {
+  kBcCbc = 1,
+  kBcCbcSubsonicInlet = 2,
+  kBcCbcSubsonicOutlet = 3,
+  kBcSymmetry = 4,
+  kBcNeumann = 5,
+  kBcWall = 6,
+  kBcFarfield = 7,
+  kBcHook = 8
+};
+enum Scheme : int { kHllc = 0, kRoe = 1, kAusm = 2 };
+enum SymmBc : int { kSymmOriginal = 0, kSymmNew = 1, kSymmAnalytical = 2 };
+
+// Parameters of UgpWithCvCompFlow::init / runBackwardEuler read from Joe.in.
+struct SolverParams {
+  double gamma = 1.4, R_gas = 1.0, p_ref = 1.0, rho_ref = 1.0, T_ref = 1.0;
+  double gravity[3] = {0, 0, 0};
+  int scheme = kHllc;
+  bool second_order = false;
+  bool grad_ls = false;  // false: Green-Gauss (the reference default)
+  bool limiter = true;
+  double eps_limiter = 0.001;
+  int symm_bc = kSymmOriginal;
+  int dt_mode = 0;  // 0 CFL (local), 1 CFL_MIN, 2 DT
+  double cfl = 0.0, dt_fixed = 0.0;
+  double under_relax = 0.3;
+  int ls_maxiter = 30;
+  double ls_abs = 1e-8, ls_rel = 1e-2;
+  int cfl_after = 100, cfl_interval = 10;
+  double cfl_factor = 1.0, cfl_max = 100.0;
+  int nsteps = 100;
+  double resid_energ_th = 0.0;
+  double rho_init = 1.0, p_init = 1.0, u_init[3] = {0, 0, 0};
+};
+
+struct ZoneBc {
+  int bc = 0;
+  double par[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+void parse_params(const std::string &joe_in, const Mesh &m, SolverParams &p, std::vector<ZoneBc> &zbc);
+
+struct StepInfo {
+  int step = 0, lin_iters = 0, lin_status = 0;
+  double cfl = 0, dt_min = 0;
+  double resid[5] = {0, 0, 0, 0, 0};
+};
+
+class Ctx {
+ public:
+  Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream);
+  ~Ctx();
+  Ctx(const Ctx &) = delete;
+  Ctx &operator=(const Ctx &) = delete;
+
+  void set_state(const double *rho, const double *rhou, const double *rhoE);
+  void get_state(double *rho, double *rhou, double *rhoE);
+  void init_uniform();
+  void calc_rhs(bool implicit);
+  // returns element count; copies when out != nullptr
+  int64_t get(const std::string &name, double *out);
+  StepInfo step();
+  int run(int max_steps, double *hist);
+  void set_boundary_state(int ifa_f, int nface, const double *T, const double *vel, const double *p);
+  double *device_state();
+  double *device_rhs();
+  void sync();
+
+  // timing hooks for the bench: launch the residual kernels only
+  void launch_residual();
+  int ncv() const;
+
+  struct Impl;
+
+ private:
+  std::unique_ptr<Impl> d_;
+};
+
+}  // namespace jm

@@ -97,9 +97,16 @@ void build_mesh(const FaceInput &in, Mesh &m) {
   for (int i = 0; i < nzone; i++) {
     Zone &z = m.zones[i];
     int b = z.kind == kZoneInternal ? i + n_internal : i;
-    if (z.kind == kZoneInternal && m.nfa_b < 0) m.nfa_b = fa_count[i];
-    z.ifa_f = fa_count[b];
-    z.ifa_l = fa_count[b + 1] - 1;
+    if (z.kind == kZoneInternal) {
+      if (m.nfa_b < 0) m.nfa_b = fa_count[i];
+      z.ifa_f = fa_count[i];  // inter-processor bucket, empty on one rank
+      z.ifa_l = fa_count[i + 1] - 1;
+      z.ifa_i_f = fa_count[b];
+      z.ifa_i_l = fa_count[b + 1] - 1;
+    } else {
+      z.ifa_f = fa_count[b];
+      z.ifa_l = fa_count[b + 1] - 1;
+    }
   }
   if (m.nfa_b < 0) m.nfa_b = nfa;
   std::vector<int> fa_new(nfa);
@@ -516,6 +523,200 @@ void calc_lsg_coeff(Mesh &m) {
       }
     }
   }
+}
+
+}  // namespace jm
+
+// ---------------------------------------------------------------------------
+// Fluent ASCII reader with MshFilter's token rules (MshFilter.cpp:3-69):
+// parentheses set the level, quotes group, indices are hexadecimal.
+// Sections 2 (dimension), 10 (nodes), 12 (cells), 13 (faces), 39/45 (zone
+// names); periodic sections are not supported.
+// ---------------------------------------------------------------------------
+namespace jm {
+
+namespace {
+
+struct MshTok {
+  FILE *fp;
+  int level = 0;
+  std::vector<char> buf;
+  size_t pos = 0, max_pos = 0;
+  explicit MshTok(FILE *f) : fp(f), buf(1 << 16) {}
+  int next(std::string &tok) {
+    tok.clear();
+    int tl = level;
+    bool quote = false;
+    while (true) {
+      if (pos >= max_pos) {
+        max_pos = fread(buf.data(), 1, buf.size(), fp);
+        pos = 0;
+        if (max_pos == 0) return tok.empty() ? -1 : tl;
+      }
+      char c = buf[pos++];
+      if (c == '"') {
+        if (quote) break;
+        quote = true;
+        continue;
+      } else if (quote) {
+        tok.push_back(c);
+      } else if (c == ' ' || c == '\t' || c == '\n' || c == '\r') {
+        if (!tok.empty()) break;
+      } else if (c == '(') {
+        level++;
+        if (!tok.empty()) break;
+        tl++;
+      } else if (c == ')') {
+        level--;
+        if (!tok.empty()) break;
+        tl--;
+      } else {
+        tok.push_back(c);
+      }
+    }
+    return tl;
+  }
+};
+
+long hexv(const std::string &s) { return std::strtol(s.c_str(), nullptr, 16); }
+
+}  // namespace
+
+void read_msh(const std::string &path, Mesh &m) {
+  FILE *fp = std::fopen(path.c_str(), "rb");
+  if (!fp) fail("cannot open msh file " + path);
+  MshTok tk(fp);
+  std::string t;
+  int nno = 0, ncv = 0, nfa = 0;
+  std::vector<double> x_no;
+  std::vector<int64_t> fptr(1, 0);
+  std::vector<int> fnodes, fcells, fzone;
+  std::vector<std::pair<int, std::string>> names;
+  int lvl;
+  try {
+    while ((lvl = tk.next(t)) != -1) {
+      if (lvl != 1) continue;
+      if (t == "0") {
+        tk.next(t);
+      } else if (t == "2") {
+        tk.next(t);
+        if (t != "3") fail("dimension != 3");
+      } else if (t == "10") {
+        tk.next(t);
+        if (t == "0") {
+          tk.next(t);
+          tk.next(t);
+          nno = (int)hexv(t);
+          x_no.assign(3 * (size_t)nno, 0.0);
+          tk.next(t);
+          tk.next(t);
+        } else {
+          tk.next(t);
+          long i0 = hexv(t) - 1;
+          tk.next(t);
+          long i1 = hexv(t) - 1;
+          tk.next(t);
+          tk.next(t);
+          if (t != "3") fail("node section must be 3D");
+          if (i1 >= nno) fail("node index out of range");
+          for (long i = i0; i <= i1; i++)
+            for (int k = 0; k < 3; k++) {
+              tk.next(t);
+              x_no[3 * i + k] = std::strtod(t.c_str(), nullptr);
+            }
+        }
+      } else if (t == "12") {
+        tk.next(t);
+        if (t == "0") {
+          tk.next(t);
+          tk.next(t);
+          ncv = (int)hexv(t);
+          tk.next(t);
+        } else {
+          tk.next(t);
+          long i0 = hexv(t);
+          tk.next(t);
+          long i1 = hexv(t);
+          tk.next(t);
+          tk.next(t);
+          if (hexv(t) == 0)
+            for (long i = i0; i <= i1; i++) tk.next(t);
+        }
+      } else if (t == "13") {
+        tk.next(t);
+        if (t == "0") {
+          tk.next(t);
+          tk.next(t);
+          nfa = (int)hexv(t);
+          tk.next(t);
+        } else {
+          int zone = (int)hexv(t);
+          tk.next(t);
+          long i0 = hexv(t) - 1;
+          tk.next(t);
+          long i1 = hexv(t) - 1;
+          tk.next(t);
+          tk.next(t);
+          int nnf = (int)hexv(t);
+          if ((long)fzone.size() != i0) fail("face sections must be contiguous and in order");
+          for (long i = i0; i <= i1; i++) {
+            int nn = nnf;
+            if (nnf == 0 || nnf == 5) {
+              tk.next(t);
+              nn = (int)hexv(t);
+            }
+            if (nn < 3 || nn > 8) fail("face with unsupported node count");
+            int nl[8];
+            for (int k = 0; k < nn; k++) {
+              tk.next(t);
+              nl[k] = (int)hexv(t) - 1;
+            }
+            tk.next(t);
+            int c0 = (int)hexv(t) - 1;
+            tk.next(t);
+            int c1 = (int)hexv(t) - 1;
+            // MshFilter flip rule (MshFilter.cpp:453-472)
+            if (c0 >= 0) {
+              for (int k = 0; k < nn; k++) fnodes.push_back(nl[nn - k - 1]);
+              fcells.push_back(c0);
+              fcells.push_back(c1);
+            } else {
+              if (c1 < 0) fail("face without cells");
+              for (int k = 0; k < nn; k++) fnodes.push_back(nl[k]);
+              fcells.push_back(c1);
+              fcells.push_back(c0);
+            }
+            fptr.push_back((int64_t)fnodes.size());
+            fzone.push_back(zone);
+          }
+        }
+      } else if (t == "18") {
+        fail("periodic face sections are not supported");
+      } else if (t == "39" || t == "45") {
+        tk.next(t);
+        int zone = std::atoi(t.c_str());
+        tk.next(t);
+        tk.next(t);
+        names.emplace_back(zone, t);
+      }
+    }
+  } catch (...) {
+    std::fclose(fp);
+    throw;
+  }
+  std::fclose(fp);
+  if ((int)fzone.size() != nfa) fail("face count mismatch in msh file");
+  FaceInput in;
+  in.nno = nno;
+  in.nfa = nfa;
+  in.ncv = ncv;
+  in.x_no = x_no.data();
+  in.face_ptr = fptr.data();
+  in.face_nodes = fnodes.data();
+  in.face_cells = fcells.data();
+  in.face_zone = fzone.data();
+  in.zone_names = names;
+  build_mesh(in, m);
 }
 
 }  // namespace jm

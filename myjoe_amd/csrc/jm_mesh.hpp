@@ -23,7 +23,8 @@ struct Zone {
   std::string name;
   int index = 0;  // zone id of the mesh file
   int kind = kZoneBoundary;
-  int ifa_f = 0, ifa_l = -1;
+  int ifa_f = 0, ifa_l = -1;      // boundary: its faces; internal: the (empty) inter-processor part
+  int ifa_i_f = 0, ifa_i_l = -1;  // internal zones: the fully internal faces (Ugp.h:150-151)
 };
 
 struct Mesh {

@@ -1,0 +1,534 @@
+// Pointwise device physics of the steady-solve hot path: HLLC / Roe / AUSM
+// face fluxes, their Jacobians, and the 5x5 block LU of the block-Jacobi
+// preconditioner. Every expression keeps the operand order of the reference
+// routine cited above it, so that compiled without FMA contraction
+// (-ffp-contract=off) the FP64 results equal the reference's bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace jm {
+
+struct Dev {
+  // std::min / std::max semantics (return the first argument on ties / NaN)
+  static __device__ __forceinline__ double mn(double a, double b) { return (b < a) ? b : a; }
+  static __device__ __forceinline__ double mx(double a, double b) { return (a < b) ? b : a; }
+};
+
+// vecDotVec3d (tc_vec3d.h:14-22): sum starts at 0.0
+__device__ __forceinline__ double dot3(const double *a, const double *b) {
+  double d = 0.0;
+  d += a[0] * b[0];
+  d += a[1] * b[1];
+  d += a[2] * b[2];
+  return d;
+}
+
+// HLLC flux, UgpWithCvCompFlow.cpp:120-235 (no scalars, k = 0, surfVeloc = 0)
+__device__ __forceinline__ void flux_hllc(double F[5], double rhoL, const double *uL, double pL, double rhoR,
+                                          const double *uR, double pR, double gam, double area,
+                                          const double *nVec) {
+  const double gammaL = gam, gammaR = gam, kL = 0.0, kR = 0.0, surfv = 0.0;
+  double unL = dot3(uL, nVec) - surfv;
+  double unLr = unL - surfv;
+  double uLuL = dot3(uL, uL);
+  double cL = sqrt(gammaL * pL / rhoL);
+  double hL = gammaL / (gammaL - 1.0) * pL / rhoL + 0.5 * uLuL + kL;
+  double eL = hL * rhoL - pL;
+  double unR = dot3(uR, nVec) - surfv;
+  double unRr = unR - surfv;
+  double uRuR = dot3(uR, uR);
+  double cR = sqrt(gammaR * pR / rhoR);
+  double hR = gammaR / (gammaR - 1.0) * pR / rhoR + 0.5 * uRuR + kR;
+  double eR = hR * rhoR - pR;
+  double Rrho = sqrt(rhoR / rhoL);
+  double tmp = 1.0 / (1.0 + Rrho);
+  double velRoe[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) velRoe[i] = tmp * (uL[i] + uR[i] * Rrho);
+  double uRoe = dot3(velRoe, nVec);
+  double gamPdivRho = tmp * ((gammaL * pL / rhoL + 0.5 * (gammaL - 1.0) * uLuL) +
+                             (gammaR * pR / rhoR + 0.5 * (gammaR - 1.0) * uRuR) * Rrho);
+  double cRoe = sqrt(gamPdivRho - ((gammaL + gammaR) * 0.5 - 1.0) * 0.5 * dot3(velRoe, velRoe));
+  double sL = Dev::mn(uRoe - cRoe, unLr - cL);
+  double sR = Dev::mx(uRoe + cRoe, unRr + cR);
+  double sM = (pL - pR - rhoL * unL * (sL - unL) + rhoR * unR * (sR - unR)) / (rhoR * (sR - unR) - rhoL * (sL - unL));
+  double pStar = rhoR * (unR - sR) * (unR - sM) + pR;
+  double f0, f1, f2, f3, f4;
+  if (sM >= 0.0) {
+    if (sL > 0.0) {
+      f0 = rhoL * unLr;
+      f1 = rhoL * uL[0] * unLr + pL * nVec[0];
+      f2 = rhoL * uL[1] * unLr + pL * nVec[1];
+      f3 = rhoL * uL[2] * unLr + pL * nVec[2];
+      f4 = eL * unLr + pL * unL;
+    } else {
+      double invSLmSs = 1.0 / (sL - sM);
+      double sLmuL = sL - unL;
+      double rhoSL = rhoL * sLmuL * invSLmSs;
+      double r0 = (rhoL * uL[0] * sLmuL + (pStar - pL) * nVec[0]) * invSLmSs;
+      double r1 = (rhoL * uL[1] * sLmuL + (pStar - pL) * nVec[1]) * invSLmSs;
+      double r2 = (rhoL * uL[2] * sLmuL + (pStar - pL) * nVec[2]) * invSLmSs;
+      double eSL = (sLmuL * eL - pL * unL + pStar * sM) * invSLmSs;
+      f0 = rhoSL * sM;
+      f1 = r0 * sM + pStar * nVec[0];
+      f2 = r1 * sM + pStar * nVec[1];
+      f3 = r2 * sM + pStar * nVec[2];
+      f4 = (eSL + pStar) * sM;
+    }
+  } else {
+    if (sR >= 0.0) {
+      double invSRmSs = 1.0 / (sR - sM);
+      double sRmuR = sR - unR;
+      double rhoSR = rhoR * sRmuR * invSRmSs;
+      double r0 = (rhoR * uR[0] * sRmuR + (pStar - pR) * nVec[0]) * invSRmSs;
+      double r1 = (rhoR * uR[1] * sRmuR + (pStar - pR) * nVec[1]) * invSRmSs;
+      double r2 = (rhoR * uR[2] * sRmuR + (pStar - pR) * nVec[2]) * invSRmSs;
+      double eSR = (sRmuR * eR - pR * unR + pStar * sM) * invSRmSs;
+      f0 = rhoSR * sM;
+      f1 = r0 * sM + pStar * nVec[0];
+      f2 = r1 * sM + pStar * nVec[1];
+      f3 = r2 * sM + pStar * nVec[2];
+      f4 = (eSR + pStar) * sM;
+    } else {
+      f0 = rhoR * unRr;
+      f1 = rhoR * uR[0] * unRr + pR * nVec[0];
+      f2 = rhoR * uR[1] * unRr + pR * nVec[1];
+      f3 = rhoR * uR[2] * unRr + pR * nVec[2];
+      f4 = eR * unRr + pR * unR;
+    }
+  }
+  F[0] = f0 * area;
+  F[1] = f1 * area;
+  F[2] = f2 * area;
+  F[3] = f3 * area;
+  F[4] = f4 * area;
+}
+
+// Roe flux without entropy fix, UgpWithCvCompFlow.cpp:3281-3380; returns the
+// central part alone when the Roe-averaged a^2 < 1e-12 (:3318-3319).
+__device__ __forceinline__ void flux_roe(double F[5], double rho0, const double *u0, double p0, double rho1,
+                                         const double *u1, double p1, double gam, double area, const double *nVec) {
+  double gamma = 0.5 * (gam + gam);
+  double gm1 = gamma - 1.0;
+  double rhoun0 = rho0 * dot3(u0, nVec);
+  double rhoun1 = rho1 * dot3(u1, nVec);
+  double rhoE0 = p0 / gm1 + 0.5 * rho0 * dot3(u0, u0);
+  double rhoE1 = p1 / gm1 + 0.5 * rho1 * dot3(u1, u1);
+  double H0 = (rhoE0 + p0) / rho0;
+  double H1 = (rhoE1 + p1) / rho1;
+  F[0] = 0.5 * area * (rhoun0 + rhoun1);
+  F[1] = 0.5 * area * (rhoun0 * u0[0] + rhoun1 * u1[0] + (p0 + p1) * nVec[0]);
+  F[2] = 0.5 * area * (rhoun0 * u0[1] + rhoun1 * u1[1] + (p0 + p1) * nVec[1]);
+  F[3] = 0.5 * area * (rhoun0 * u0[2] + rhoun1 * u1[2] + (p0 + p1) * nVec[2]);
+  F[4] = 0.5 * area * (rhoun0 * H0 + rhoun1 * H1);
+  double sqrt_rho0 = sqrt(rho0), sqrt_rho1 = sqrt(rho1);
+  double tmp = 1.0 / (sqrt_rho0 + sqrt_rho1);
+  double uAvg = tmp * (u0[0] * sqrt_rho0 + u1[0] * sqrt_rho1);
+  double vAvg = tmp * (u0[1] * sqrt_rho0 + u1[1] * sqrt_rho1);
+  double wAvg = tmp * (u0[2] * sqrt_rho0 + u1[2] * sqrt_rho1);
+  double hAvg = tmp * (H0 * sqrt_rho0 + H1 * sqrt_rho1);
+  double alphaAvg = 0.5 * (uAvg * uAvg + vAvg * vAvg + wAvg * wAvg);
+  double a2Avg = gm1 * (hAvg - alphaAvg);
+  if (a2Avg < 1.0E-12) return;
+  double aAvg = sqrt(a2Avg);
+  double unAvg = uAvg * nVec[0] + vAvg * nVec[1] + wAvg * nVec[2];
+  double dr = rho1 - rho0;
+  double dru = rho1 * u1[0] - rho0 * u0[0];
+  double drv = rho1 * u1[1] - rho0 * u0[1];
+  double drw = rho1 * u1[2] - rho0 * u0[2];
+  double drE = rhoE1 - rhoE0;
+  double lam1 = fabs(unAvg);
+  double lam2 = fabs(unAvg + aAvg);
+  double lam3 = fabs(unAvg - aAvg);
+  double abv1 = 0.5 * (lam2 + lam3);
+  double abv2 = 0.5 * (lam2 - lam3);
+  double abv3 = abv1 - lam1;
+  double abv4 = gm1 * (alphaAvg * dr - uAvg * dru - vAvg * drv - wAvg * drw + drE);
+  double abv5 = unAvg * dr - nVec[0] * dru - nVec[1] * drv - nVec[2] * drw;
+  double abv6 = abv3 * abv4 / a2Avg - abv2 * abv5 / aAvg;
+  double abv7 = abv3 * abv5 - abv2 * abv4 / aAvg;
+  F[0] -= 0.5 * area * (lam1 * dr + abv6);
+  F[1] -= 0.5 * area * (lam1 * dru + uAvg * abv6 - nVec[0] * abv7);
+  F[2] -= 0.5 * area * (lam1 * drv + vAvg * abv6 - nVec[1] * abv7);
+  F[3] -= 0.5 * area * (lam1 * drw + wAvg * abv6 - nVec[2] * abv7);
+  F[4] -= 0.5 * area * (lam1 * drE + hAvg * abv6 - unAvg * abv7);
+}
+
+// AUSM (Liou-Steffen), UgpWithCvCompFlow.cpp:3566-3641
+__device__ __forceinline__ void flux_ausm(double F[5], double rhoL, const double *uL, double pL, double rhoR,
+                                          const double *uR, double pR, double gam, double area,
+                                          const double *nVec) {
+  const double gammaL = gam, gammaR = gam, kL = 0.0, kR = 0.0;
+  double unL = dot3(uL, nVec);
+  double uLuL = dot3(uL, uL);
+  double cL = sqrt(gammaL * pL / rhoL);
+  double hL = (gammaL / (gammaL - 1.0)) * (pL / rhoL) + 0.5 * uLuL + kL;
+  double mL = unL / cL;
+  double unR = dot3(uR, nVec);
+  double uRuR = dot3(uR, uR);
+  double cR = sqrt(gammaR * pR / rhoR);
+  double hR = (gammaR / (gammaR - 1.0)) * (pR / rhoR) + 0.5 * uRuR + kR;
+  double mR = unR / cR;
+  double mLP = fabs(mL) <= 1.0 ? 0.25 * (mL + 1.0) * (mL + 1.0) : 0.5 * (mL + fabs(mL));
+  double mRM = fabs(mR) <= 1.0 ? -0.25 * (mR - 1.0) * (mR - 1.0) : 0.5 * (mR - fabs(mR));
+  double mF = mLP + mRM;
+  double pLP = fabs(mL) <= 1.0 ? 0.25 * pL * (mL + 1.0) * (mL + 1.0) * (2.0 - mL) : 0.5 * pL * (mL + fabs(mL)) / mL;
+  double pRM = fabs(mR) <= 1.0 ? 0.25 * pR * (mR - 1.0) * (mR - 1.0) * (2.0 + mR) : 0.5 * pR * (mR - fabs(mR)) / mR;
+  double pF = pLP + pRM;
+  double Phi = fabs(mF);
+  double rcL = rhoL * cL, rcR = rhoR * cR;
+  F[0] = 0.5 * (mF * ((rcL) + (rcR)) - Phi * ((rcR) - (rcL)));
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+    F[1 + i] = 0.5 * (mF * ((rcL * uL[i]) + (rcR * uR[i])) - Phi * ((rcR * uR[i]) - (rcL * uL[i]))) + nVec[i] * pF;
+  F[4] = 0.5 * (mF * ((rcL * hL) + (rcR * hR)) - Phi * ((rcR * hR) - (rcL * hL)));
+#pragma unroll
+  for (int i = 0; i < 5; i++) F[i] *= area;
+}
+
+// calcJacobianA, UgpWithCvCompFlow.cpp:666-709 (nV area-weighted)
+__device__ __forceinline__ void jac_a(double A[5][5], const double *vel, double pp, double rrho, const double *nV,
+                                      double gamma) {
+  const double surfVeloc = 0.0;
+  double kapm1 = (gamma - 1.0);
+  double nVel0 = vel[0] * nV[0], nVel1 = vel[1] * nV[1], nVel2 = vel[2] * nV[2];
+  double U_k = nVel0 + nVel1 + nVel2;
+  double vSquHlf = 0.5 * dot3(vel, vel);
+  double c = sqrt(gamma * pp / rrho);
+  double inv_kap_m1 = 1.0 / kapm1;
+  A[0][0] = -surfVeloc;
+  A[0][1] = nV[0];
+  A[0][2] = nV[1];
+  A[0][3] = nV[2];
+  A[0][4] = 0.0;
+  A[1][0] = -vel[0] * (nVel1 + nVel2) + nV[0] * (kapm1 * vSquHlf - vel[0] * vel[0]);
+  A[1][1] = (2. - gamma) * nVel0 + U_k - surfVeloc;
+  A[1][2] = vel[0] * nV[1] - kapm1 * vel[1] * nV[0];
+  A[1][3] = vel[0] * nV[2] - kapm1 * vel[2] * nV[0];
+  A[1][4] = kapm1 * nV[0];
+  A[2][0] = -vel[1] * (nVel0 + nVel2) + nV[1] * (kapm1 * vSquHlf - vel[1] * vel[1]);
+  A[2][1] = -kapm1 * vel[0] * nV[1] + vel[1] * nV[0];
+  A[2][2] = (2. - gamma) * nVel1 + U_k - surfVeloc;
+  A[2][3] = vel[1] * nV[2] - kapm1 * vel[2] * nV[1];
+  A[2][4] = kapm1 * nV[1];
+  A[3][0] = -vel[2] * (nVel0 + nVel1) + nV[2] * (kapm1 * vSquHlf - vel[2] * vel[2]);
+  A[3][1] = -kapm1 * vel[0] * nV[2] + vel[2] * nV[0];
+  A[3][2] = -kapm1 * vel[1] * nV[2] + vel[2] * nV[1];
+  A[3][3] = (2. - gamma) * nVel2 + U_k - surfVeloc;
+  A[3][4] = kapm1 * nV[2];
+  A[4][0] = U_k * ((gamma - 2.) * vSquHlf - c * c * inv_kap_m1);
+  A[4][1] = c * c * inv_kap_m1 * nV[0] - kapm1 * vel[0] * (nVel1 + nVel2) - (kapm1 * vel[0] * vel[0] - vSquHlf) * nV[0];
+  A[4][2] = c * c * inv_kap_m1 * nV[1] - kapm1 * vel[1] * (nVel0 + nVel2) - (kapm1 * vel[1] * vel[1] - vSquHlf) * nV[1];
+  A[4][3] = c * c * inv_kap_m1 * nV[2] - kapm1 * vel[2] * (nVel0 + nVel1) - (kapm1 * vel[2] * vel[2] - vSquHlf) * nV[2];
+  A[4][4] = gamma * U_k - surfVeloc;
+}
+
+// One side of the HLLC Jacobian pair, UgpWithCvCompFlow.cpp:242-659.
+// which = 0 computes A_L (d F / d U_L), which = 1 computes A_R. The reference
+// computes the two independently (calcSubSonicJacobeanHLLC uses only the
+// matching dSMdU/dpsdU), so evaluating one at a time gives the same bits.
+__device__ __noinline__ void jac_hllc(double J[5][5], int which, double rhoL, const double *uL, double pL, double rhoR,
+                                      const double *uR, double pR, double gam, double area, const double *nVec) {
+  const double gammaL = gam, gammaR = gam, kL = 0.0, kR = 0.0;
+  double unL = dot3(uL, nVec);
+  double uLuL = dot3(uL, uL);
+  double cL = sqrt(gammaL * pL / rhoL);
+  double hL = gammaL / (gammaL - 1.0) * pL / rhoL + 0.5 * uLuL + kL;
+  double eL = hL * rhoL - pL;
+  double unR = dot3(uR, nVec);
+  double uRuR = dot3(uR, uR);
+  double cR = sqrt(gammaR * pR / rhoR);
+  double hR = gammaR / (gammaR - 1.0) * pR / rhoR + 0.5 * uRuR + kR;
+  double eR = hR * rhoR - pR;
+  double Rrho = sqrt(rhoR / rhoL);
+  double tmp = 1.0 / (1.0 + Rrho);
+  double velRoe[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) velRoe[i] = tmp * (uL[i] + uR[i] * Rrho);
+  double uRoe = dot3(velRoe, nVec);
+  double gamPdivRho = tmp * ((gammaL * pL / rhoL + 0.5 * (gammaL - 1.0) * uLuL) +
+                             (gammaR * pR / rhoR + 0.5 * (gammaR - 1.0) * uRuR) * Rrho);
+  double cRoe = sqrt(gamPdivRho - ((gammaL + gammaR) * 0.5 - 1.0) * 0.5 * dot3(velRoe, velRoe));
+  double sL = Dev::mn(uRoe - cRoe, unL - cL);
+  double sR = Dev::mx(uRoe + cRoe, unR + cR);
+  double sM = (pL - pR - rhoL * unL * (sL - unL) + rhoR * unR * (sR - unR)) / (rhoR * (sR - unR) - rhoL * (sL - unL));
+  double pStar = rhoR * (unR - sR) * (unR - sM) + pR;
+  double gAvg = 0.5 * (gammaL + gammaR);
+
+  // fully supersonic to one side: F = F(U_L) or F(U_R)
+  bool left_sup = (sM >= 0.0) && (sL > 0.0);
+  bool right_sup = (sM < 0.0) && !(sR >= 0.0);
+  if (left_sup || right_sup) {
+    bool active = left_sup ? (which == 0) : (which == 1);
+    if (active) {
+      double nVecArea[3] = {nVec[0] * area, nVec[1] * area, nVec[2] * area};
+      if (which == 0)
+        jac_a(J, uL, pL, rhoL, nVecArea, gAvg);
+      else
+        jac_a(J, uR, pR, rhoR, nVecArea, gAvg);
+    } else {
+#pragma unroll
+      for (int i = 0; i < 5; i++)
+#pragma unroll
+        for (int j = 0; j < 5; j++) J[i][j] = 0.0;
+    }
+    return;
+  }
+  // star region on the side of the contact: K = L (sM >= 0) or R (sM < 0)
+  const bool K_is_L = (sM >= 0.0);
+  double rhoK = K_is_L ? rhoL : rhoR;
+  const double *uK = K_is_L ? uL : uR;
+  double pK = K_is_L ? pL : pR, eK = K_is_L ? eL : eR, unK = K_is_L ? unL : unR;
+  double uKuK = K_is_L ? uLuL : uRuR, sK = K_is_L ? sL : sR;
+  double invSKmSs = 1.0 / (sK - sM);
+  double sKmuK = sK - unK;
+  double rhoSK = rhoK * sKmuK * invSKmSs;
+  double rhouSK[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) rhouSK[i] = (rhoK * uK[i] * sKmuK + (pStar - pK) * nVec[i]) * invSKmSs;
+  double eSK = (sKmuK * eK - pK * unK + pStar * sM) * invSKmSs;
+  double invrhotld = 1.0 / (rhoR * (sR - unR) - rhoL * (sL - unL));
+  // derivative of sM and p* with respect to the requested side
+  double dSM[5], dps[5];
+  if (which == 0) {
+    double gammaLM1 = (gammaL - 1.0);
+    dSM[0] = -unL * unL + uLuL * gammaLM1 / 2.0 + sM * sL;
+    dSM[1] = nVec[0] * (2.0 * unL - sL - sM) - gammaLM1 * uL[0];
+    dSM[2] = nVec[1] * (2.0 * unL - sL - sM) - gammaLM1 * uL[1];
+    dSM[3] = nVec[2] * (2.0 * unL - sL - sM) - gammaLM1 * uL[2];
+    dSM[4] = gammaLM1;
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+      dSM[i] *= invrhotld;
+      dps[i] = rhoR * (sR - unR) * dSM[i];
+    }
+  } else {
+    double gammaRM1 = (gammaR - 1.0);
+    dSM[0] = unR * unR - uRuR * gammaRM1 / 2.0 - sM * sR;
+    dSM[1] = -nVec[0] * (2.0 * unR - sR - sM) + gammaRM1 * uR[0];
+    dSM[2] = -nVec[1] * (2.0 * unR - sR - sM) + gammaRM1 * uR[1];
+    dSM[3] = -nVec[2] * (2.0 * unR - sR - sM) + gammaRM1 * uR[2];
+    dSM[4] = -gammaRM1;
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+      dSM[i] *= invrhotld;
+      dps[i] = rhoL * (sL - unL) * dSM[i];
+    }
+  }
+  // calcSubSonicJacobeanHLLC (UgpWithCvCompFlow.cpp:605-659) called with the
+  // star side K first: the requested matrix is its "AL" when it is side K
+  // (own-state terms), its "AR" otherwise.
+  const double gamma = gAvg, gammaMinus1 = (gAvg - 1.0);
+  const double *nV = nVec;
+  double omL = 1.0 / (sK - sM);
+  const bool own = (which == 0) == K_is_L;
+  if (own) {
+    const double qL = unK, psiL = uKuK, SL = sK, SM = sM, pS = pStar;
+    const double *dSMdUL = dSM, *dpsdUL = dps;
+    J[0][0] = SL + rhoSK * dSMdUL[0];
+    J[0][1] = -nV[0] + rhoSK * dSMdUL[1];
+    J[0][2] = -nV[1] + rhoSK * dSMdUL[2];
+    J[0][3] = -nV[2] + rhoSK * dSMdUL[3];
+    J[0][4] = +rhoSK * dSMdUL[4];
+    J[1][0] = qL * uK[0] - nV[0] * psiL * gammaMinus1 / 2.0 + nV[0] * dpsdUL[0] + rhouSK[0] * dSMdUL[0];
+    J[1][1] = SL - qL + nV[0] * (gamma - 2.0) * uK[0] + nV[0] * dpsdUL[1] + rhouSK[0] * dSMdUL[1];
+    J[1][2] = -uK[0] * nV[1] + nV[0] * gammaMinus1 * uK[1] + nV[0] * dpsdUL[2] + rhouSK[0] * dSMdUL[2];
+    J[1][3] = -uK[0] * nV[2] + nV[0] * gammaMinus1 * uK[2] + nV[0] * dpsdUL[3] + rhouSK[0] * dSMdUL[3];
+    J[1][4] = -gammaMinus1 * nV[0] + nV[0] * dpsdUL[4] + rhouSK[0] * dSMdUL[4];
+    J[2][0] = qL * uK[1] - nV[1] * psiL * gammaMinus1 / 2.0 + nV[1] * dpsdUL[0] + rhouSK[1] * dSMdUL[0];
+    J[2][1] = -uK[1] * nV[0] + nV[1] * gammaMinus1 * uK[0] + nV[1] * dpsdUL[1] + rhouSK[1] * dSMdUL[1];
+    J[2][2] = SL - qL + nV[1] * (gamma - 2.0) * uK[1] + nV[1] * dpsdUL[2] + rhouSK[1] * dSMdUL[2];
+    J[2][3] = -uK[1] * nV[2] + nV[1] * gammaMinus1 * uK[2] + nV[1] * dpsdUL[3] + rhouSK[1] * dSMdUL[3];
+    J[2][4] = -gammaMinus1 * nV[1] + nV[1] * dpsdUL[4] + rhouSK[1] * dSMdUL[4];
+    J[3][0] = qL * uK[2] - nV[2] * psiL * gammaMinus1 / 2.0 + nV[2] * dpsdUL[0] + rhouSK[2] * dSMdUL[0];
+    J[3][1] = -uK[2] * nV[0] + nV[2] * gammaMinus1 * uK[0] + nV[2] * dpsdUL[1] + rhouSK[2] * dSMdUL[1];
+    J[3][2] = -uK[2] * nV[1] + nV[2] * gammaMinus1 * uK[1] + nV[2] * dpsdUL[2] + rhouSK[2] * dSMdUL[2];
+    J[3][3] = SL - qL + nV[2] * (gamma - 2.0) * uK[2] + nV[2] * dpsdUL[3] + rhouSK[2] * dSMdUL[3];
+    J[3][4] = -gammaMinus1 * nV[2] + nV[2] * dpsdUL[4] + rhouSK[2] * dSMdUL[4];
+    J[4][0] = qL * (eK + pK) / rhoK - qL * psiL * (gamma - 1.0) / 2.0 + SM * dpsdUL[0] + (pS + eSK) * dSMdUL[0];
+    J[4][1] = -nV[0] * (eK + pK) / rhoK + gammaMinus1 * uK[0] * qL + SM * dpsdUL[1] + (pS + eSK) * dSMdUL[1];
+    J[4][2] = -nV[1] * (eK + pK) / rhoK + gammaMinus1 * uK[1] * qL + SM * dpsdUL[2] + (pS + eSK) * dSMdUL[2];
+    J[4][3] = -nV[2] * (eK + pK) / rhoK + gammaMinus1 * uK[2] * qL + SM * dpsdUL[3] + (pS + eSK) * dSMdUL[3];
+    J[4][4] = SL - qL * gamma + SM * dpsdUL[4] + (pS + eSK) * dSMdUL[4];
+#pragma unroll
+    for (int i = 0; i < 5; i++)
+#pragma unroll
+      for (int j = 0; j < 5; j++) J[i][j] *= omL;
+  } else {
+#pragma unroll
+    for (int i = 0; i < 5; i++) J[0][i] = omL * rhoSK * dSM[i];
+#pragma unroll
+    for (int i = 0; i < 5; i++) J[1][i] = omL * (nV[0] * dps[i] + rhouSK[0] * dSM[i]);
+#pragma unroll
+    for (int i = 0; i < 5; i++) J[2][i] = omL * (nV[1] * dps[i] + rhouSK[1] * dSM[i]);
+#pragma unroll
+    for (int i = 0; i < 5; i++) J[3][i] = omL * (nV[2] * dps[i] + rhouSK[2] * dSM[i]);
+#pragma unroll
+    for (int i = 0; i < 5; i++) J[4][i] = omL * (dps[i] * sM + (pStar + eSK) * dSM[i]);
+  }
+  // back in calcEulerFluxMatrices_HLLC (:361-383 / :492-514 and :415-437 / :523-545)
+#pragma unroll
+  for (int i = 0; i < 5; i++) J[0][i] = J[0][i] * sM + dSM[i] * rhoSK;
+#pragma unroll
+  for (int i = 0; i < 5; i++) J[1][i] = J[1][i] * sM + dSM[i] * rhouSK[0] + dps[i] * nVec[0];
+#pragma unroll
+  for (int i = 0; i < 5; i++) J[2][i] = J[2][i] * sM + dSM[i] * rhouSK[1] + dps[i] * nVec[1];
+#pragma unroll
+  for (int i = 0; i < 5; i++) J[3][i] = J[3][i] * sM + dSM[i] * rhouSK[2] + dps[i] * nVec[2];
+#pragma unroll
+  for (int i = 0; i < 5; i++) J[4][i] = (J[4][i] + dps[i]) * sM + (eSK + pStar) * dSM[i];
+#pragma unroll
+  for (int i = 0; i < 5; i++)
+#pragma unroll
+    for (int j = 0; j < 5; j++) J[i][j] *= area;
+}
+
+// One side of the Roe Jacobian pair, UgpWithCvCompFlow.cpp:3382-3559
+// (A_L = A(U_L)/2 + P|L|P^-1 area/2, A_R = A(U_R)/2 - P|L|P^-1 area/2).
+__device__ __noinline__ void jac_roe(double J[5][5], int which, double rhoL, const double *uL, double pL, double rhoR,
+                                     const double *uR, double pR, double gam, double area, const double *nVec) {
+  const double gammaL = gam, gammaR = gam;
+  double uLuL = dot3(uL, uL);
+  double uRuR = dot3(uR, uR);
+  double Rrho = sqrt(rhoR / rhoL);
+  double rhoRoe = Rrho * rhoL;
+  double tmp = 1.0 / (1.0 + Rrho);
+  double vel[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) vel[i] = tmp * (uL[i] + uR[i] * Rrho);
+  double uRoe = dot3(vel, nVec);
+  double gamPdivRho = tmp * ((gammaL * pL / rhoL + 0.5 * (gammaL - 1.0) * uLuL) +
+                             (gammaR * pR / rhoR + 0.5 * (gammaR - 1.0) * uRuR) * Rrho);
+  double c = sqrt(gamPdivRho - ((gammaL + gammaR) * 0.5 - 1.0) * 0.5 * dot3(vel, vel));
+  double nVecArea[3] = {nVec[0] * area, nVec[1] * area, nVec[2] * area};
+  double Lam[5] = {uRoe, uRoe, uRoe, uRoe + c, uRoe - c};
+  const double gamma = 0.5 * (gammaL + gammaR);
+  const double *nV = nVec;
+  const double rrho = rhoRoe;
+  double kapm1 = (gamma - 1.0);
+  double P[5][5], invP[5][5];
+  {  // calcPMatrix, :3480-3521
+    double rhooc = rrho / c;
+    double rhoxc = rrho * c;
+    double vSqu = dot3(vel, vel);
+    P[0][0] = nV[0];
+    P[0][1] = nV[1];
+    P[0][2] = nV[2];
+    P[0][3] = 0.5 * rhooc;
+    P[0][4] = 0.5 * rhooc;
+    P[1][0] = vel[0] * nV[0];
+    P[1][1] = vel[0] * nV[1] - rrho * nV[2];
+    P[1][2] = vel[0] * nV[2] + rrho * nV[1];
+    P[1][3] = 0.5 * (vel[0] * rhooc + rrho * nV[0]);
+    P[1][4] = 0.5 * (vel[0] * rhooc - rrho * nV[0]);
+    P[2][0] = vel[1] * nV[0] + rrho * nV[2];
+    P[2][1] = vel[1] * nV[1];
+    P[2][2] = vel[1] * nV[2] - rrho * nV[0];
+    P[2][3] = 0.5 * (vel[1] * rhooc + rrho * nV[1]);
+    P[2][4] = 0.5 * (vel[1] * rhooc - rrho * nV[1]);
+    P[3][0] = vel[2] * nV[0] - rrho * nV[1];
+    P[3][1] = vel[2] * nV[1] + rrho * nV[0];
+    P[3][2] = vel[2] * nV[2];
+    P[3][3] = 0.5 * (vel[2] * rhooc + rrho * nV[2]);
+    P[3][4] = 0.5 * (vel[2] * rhooc - rrho * nV[2]);
+    P[4][0] = 0.5 * vSqu * nV[0] + rrho * vel[1] * nV[2] - rrho * vel[2] * nV[1];
+    P[4][1] = 0.5 * vSqu * nV[1] - rrho * vel[0] * nV[2] + rrho * vel[2] * nV[0];
+    P[4][2] = 0.5 * vSqu * nV[2] + rrho * vel[0] * nV[1] - rrho * vel[1] * nV[0];
+    P[4][3] = 0.5 * (0.5 * vSqu * rhooc + rrho * (vel[0] * nV[0] + vel[1] * nV[1] + vel[2] * nV[2]) + rhoxc / kapm1);
+    P[4][4] = 0.5 * (0.5 * vSqu * rhooc - rrho * (vel[0] * nV[0] + vel[1] * nV[1] + vel[2] * nV[2]) + rhoxc / kapm1);
+  }
+  {  // calcInvPMatrix, :3523-3559
+    double rhoxc = rrho * c;
+    double c2 = c * c;
+    double vSqu = dot3(vel, vel);
+    invP[0][0] = nV[0] - nV[2] * vel[1] / rrho + nV[1] * vel[2] / rrho - nV[0] * 0.5 * kapm1 * vSqu / c2;
+    invP[0][1] = nV[0] * kapm1 * vel[0] / c2;
+    invP[0][2] = nV[2] / rrho + nV[0] * kapm1 * vel[1] / c2;
+    invP[0][3] = -nV[1] / rrho + nV[0] * kapm1 * vel[2] / c2;
+    invP[0][4] = -nV[0] * kapm1 / c2;
+    invP[1][0] = nV[1] + nV[2] * vel[0] / rrho - nV[0] * vel[2] / rrho - nV[1] * 0.5 * kapm1 * vSqu / c2;
+    invP[1][1] = -nV[2] / rrho + nV[1] * kapm1 * vel[0] / c2;
+    invP[1][2] = nV[1] * kapm1 * vel[1] / c2;
+    invP[1][3] = nV[0] / rrho + nV[1] * kapm1 * vel[2] / c2;
+    invP[1][4] = -nV[1] * kapm1 / c2;
+    invP[2][0] = nV[2] - nV[1] * vel[0] / rrho + nV[0] * vel[1] / rrho - nV[2] * 0.5 * kapm1 * vSqu / c2;
+    invP[2][1] = nV[1] / rrho + nV[2] * kapm1 * vel[0] / c2;
+    invP[2][2] = -nV[0] / rrho + nV[2] * kapm1 * vel[1] / c2;
+    invP[2][3] = nV[2] * kapm1 * vel[2] / c2;
+    invP[2][4] = -nV[2] * kapm1 / c2;
+    invP[3][0] = -(nV[0] * vel[0] + nV[1] * vel[1] + nV[2] * vel[2]) / rrho + 0.5 * kapm1 * vSqu / rhoxc;
+    invP[3][1] = nV[0] / rrho - kapm1 * vel[0] / rhoxc;
+    invP[3][2] = nV[1] / rrho - kapm1 * vel[1] / rhoxc;
+    invP[3][3] = nV[2] / rrho - kapm1 * vel[2] / rhoxc;
+    invP[3][4] = kapm1 / rhoxc;
+    invP[4][0] = (nV[0] * vel[0] + nV[1] * vel[1] + nV[2] * vel[2]) / rrho + 0.5 * kapm1 * vSqu / rhoxc;
+    invP[4][1] = -nV[0] / rrho - kapm1 * vel[0] / rhoxc;
+    invP[4][2] = -nV[1] / rrho - kapm1 * vel[1] / rhoxc;
+    invP[4][3] = -nV[2] / rrho - kapm1 * vel[2] / rhoxc;
+    invP[4][4] = kapm1 / rhoxc;
+  }
+  if (which == 0)
+    jac_a(J, uL, pL, rhoL, nVecArea, gamma);
+  else
+    jac_a(J, uR, pR, rhoR, nVecArea, gamma);
+#pragma unroll
+  for (int i = 0; i < 5; i++)
+#pragma unroll
+    for (int j = 0; j < 5; j++) {
+      double pm = 0.0;
+#pragma unroll
+      for (int k = 0; k < 5; k++) pm += P[i][k] * fabs(Lam[k]) * invP[k][j];
+      J[i][j] = which == 0 ? 0.5 * J[i][j] + 0.5 * pm * area : 0.5 * J[i][j] - 0.5 * pm * area;
+    }
+}
+
+// ludeco / lusolv on a 5x5 block, UgpWithCv.cpp:2489-2540 (T = double or float)
+template <typename T>
+__device__ __forceinline__ void ludeco5(T A[5][5]) {
+#pragma unroll
+  for (int jc = 1; jc < 5; jc++) A[0][jc] /= A[0][0];
+#pragma unroll
+  for (int jrjc = 1; jrjc < 5; jrjc++) {
+#pragma unroll
+    for (int jr = jrjc; jr < 5; jr++) {
+      T sum = A[jr][jrjc];
+#pragma unroll
+      for (int jm = 0; jm < jrjc; jm++) sum -= A[jr][jm] * A[jm][jrjc];
+      A[jr][jrjc] = sum;
+    }
+    if (jrjc == 4) break;
+#pragma unroll
+    for (int jc = jrjc + 1; jc < 5; jc++) {
+      T sum = A[jrjc][jc];
+#pragma unroll
+      for (int jm = 0; jm < jrjc; jm++) sum -= A[jrjc][jm] * A[jm][jc];
+      A[jrjc][jc] = sum / A[jrjc][jrjc];
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void lusolv5(const T *A /* [25] */, T c[5]) {
+  c[0] = c[0] / A[0];
+#pragma unroll
+  for (int jr = 1; jr < 5; jr++) {
+    T sum = c[jr];
+#pragma unroll
+    for (int jm = 0; jm < jr; jm++) sum -= A[5 * jr + jm] * c[jm];
+    c[jr] = sum / A[5 * jr + jr];
+  }
+#pragma unroll
+  for (int jrjr = 1; jrjr < 5; jrjr++) {
+    int jr = 4 - jrjr;
+    T sum = c[jr];
+#pragma unroll
+    for (int jmjm = jr + 1; jmjm < 5; jmjm++) {
+      int jm = 4 - jmjm + jr + 1;
+      sum -= A[5 * jr + jm] * c[jm];
+    }
+    c[jr] = sum;
+  }
+}
+
+}  // namespace jm

@@ -1,0 +1,1726 @@
+// Device side of the steady-solve hot path: data layout in HBM, kernels and
+// the per-solve context. Compiled for gfx950 with -ffp-contract=off so the
+// FP64 residual path reproduces the reference's arithmetic bit for bit.
+//
+// HBM layout (all per-solve, resident for the life of the context):
+//   q      [5][ncv]   conservative state, SoA (streams)
+//   prim   [ncv][8]   rho u v w p | x_cv (gathered by neighbours: 64 B rows)
+//   grad   [ncv][16]  limited grad rho(3), grad p(3), grad u(9), alpha_rho
+//   bfa    [nfa_b][8] boundary-face state rho u v w p T h
+//   fgeo   [nfa][8]   unit normal, area, face centroid (64 B rows)
+//   cfl    per cell, the cell's faces in the reference's accumulation order
+//          (internal faces ascending, then boundary faces ascending) as
+//          {face | side << 31, neighbour cell or -(zone + 1)}
+//   A      [nnz][5][5] block Jacobian in nbocv order (FP64 or FP32)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "jm_ctx.hpp"
+#include "jm_param.hpp"
+#include "jm_physics.hpp"
+
+#define HIPCHECK(x)                                                                                     \
+  do {                                                                                                  \
+    hipError_t e_ = (x);                                                                                \
+    if (e_ != hipSuccess) throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(e_) + \
+                                                   " at " + #x);                                       \
+  } while (0)
+
+namespace jm {
+
+struct ZoneDev {
+  int bc;
+  int pad[3];
+  double par[8];
+};
+
+struct DMesh {
+  int ncv, nfa, nfa_b, nnz;
+  const double *fgeo;     // [nfa][8]
+  const double *fnorm;    // [nfa][3] area-weighted normal (Green-Gauss only)
+  const int *cfl_i;       // [ncv+1]
+  const int2 *cfl;        // [entries]
+  const int *cfl_noc;     // [entries] off-diagonal block; bit 30 = accumulate
+  const int *nbocv_i;     // [ncv+1]
+  const int *nbocv_v;     // [nnz]
+  const double *lsg;      // [nnz][3]
+  const double *bfa_lsg;  // [nfa_b][3]
+  const int *bfa_cell;    // [nfa_b]
+  const int *bfa_zone;    // [nfa_b]
+  const double *vol;      // [ncv]
+  const ZoneDev *zones;
+};
+
+struct DParams {
+  double gamma, R, eps, g[3];
+  int scheme, second_order, grad_ls, limiter, symm_bc, gravity_on;
+};
+
+// per-solve scalars of the BiCGStab and of reductions (device resident)
+struct SolverScalars {
+  double rho, rho_prime, alpha, omega, beta, gamma;
+  double res0_max, res_max;
+  double dt_min;
+  double resid[5];
+  int iter, done, maxiter, pad;
+  double zero_abs, zero_rel;
+};
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kRedSlots = 8;  // partial-sum slots per block for fused reductions
+
+__device__ __forceinline__ double2 ld2(const double *p) { return *reinterpret_cast<const double2 *>(p); }
+
+// ---------------------------------------------------------------------------
+// calcStateVariables, UgpWithCvCompFlow.h:1662-1694 (gamma, R constant)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_state(int ncv, const double *__restrict__ q, double *__restrict__ prim,
+                                                  double gam, int *__restrict__ negp) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncv) return;
+  double rho = q[c], r0 = q[ncv + c], r1 = q[2 * ncv + c], r2 = q[3 * ncv + c], rE = q[4 * ncv + c];
+  double *P = prim + 8 * (size_t)c;
+  double d = 0.0;
+  d += r0 * r0;
+  d += r1 * r1;
+  d += r2 * r2;
+  double pr = (gam - 1.0) * (rE - 0.5 * d / rho);
+  double p = P[4];
+  if (pr > 0.0)
+    p = pr;
+  else
+    atomicAdd(negp, 1);
+  P[0] = rho;
+  P[1] = r0 / rho;
+  P[2] = r1 / rho;
+  P[3] = r2 / rho;
+  P[4] = p;
+}
+
+// ---------------------------------------------------------------------------
+// setBC, JoeWithModels.cpp:3857-4300 with ComputeBCProperties_T/_H
+// (UgpWithCvCompFlow.h:1783-1839) and the EOS density (:4292-4293)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_bc(DMesh m, DParams P, const double *__restrict__ prim,
+                                               const double *__restrict__ hook, double *__restrict__ bfa) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= m.nfa_b) return;
+  const ZoneDev &z = m.zones[m.bfa_zone[f]];
+  const int c = m.bfa_cell[f];
+  const double *pc = prim + 8 * (size_t)c;
+  const double rho = pc[0], p = pc[4];
+  const double vel[3] = {pc[1], pc[2], pc[3]};
+  const double GAMMA = P.gamma, R_gas = P.R;
+  const double temp = p / (rho * R_gas);
+  const double *geo = m.fgeo + 8 * (size_t)f;
+  const double nVec[3] = {geo[0], geo[1], geo[2]};
+  double T_b = 0.0, p_b = 0.0, h_b, vb[3] = {0.0, 0.0, 0.0};
+  bool h_set = false;
+  switch (z.bc) {
+    case kBcCbc: {
+      const double u_bc[3] = {z.par[0], z.par[1], z.par[2]};
+      if (dot3(u_bc, nVec) > 0.0) {
+        double velMagN = dot3(vel, nVec);
+        double sos = sqrt(GAMMA * p / rho);
+        double mach = fabs(velMagN) / sos;
+        T_b = temp;
+        vb[0] = vel[0]; vb[1] = vel[1]; vb[2] = vel[2];
+        p_b = mach >= 1.0 ? p : z.par[4];
+      } else {
+        T_b = z.par[3];
+        vb[0] = u_bc[0]; vb[1] = u_bc[1]; vb[2] = u_bc[2];
+        p_b = z.par[4];
+      }
+      break;
+    }
+    case kBcCbcSubsonicInlet: {
+      double wPow2 = dot3(vel, vel);
+      double v = sqrt(wPow2);
+      vb[0] = z.par[0] * v; vb[1] = z.par[1] * v; vb[2] = z.par[2] * v;
+      double Ttot = z.par[3], ptot = z.par[4];
+      h_b = GAMMA * R_gas / (GAMMA - 1.0) * Ttot;
+      double w2 = dot3(vb, vb);
+      h_b -= 0.5 * w2;
+      T_b = h_b * (GAMMA - 1.0) / (GAMMA * R_gas);
+      p_b = ptot * pow(T_b / Ttot, GAMMA / (GAMMA - 1.0));
+      h_set = true;
+      break;
+    }
+    case kBcCbcSubsonicOutlet:
+      p_b = z.par[0];
+      vb[0] = vel[0]; vb[1] = vel[1]; vb[2] = vel[2];
+      T_b = temp;
+      break;
+    case kBcSymmetry: {
+      double un = dot3(nVec, vel);
+      for (int i = 0; i < 3; i++) vb[i] = vel[i] - 1.0 * un * nVec[i];
+      T_b = temp;
+      p_b = p;
+      break;
+    }
+    case kBcNeumann:
+      vb[0] = vel[0]; vb[1] = vel[1]; vb[2] = vel[2];
+      T_b = temp;
+      p_b = p;
+      break;
+    case kBcWall:
+      T_b = z.par[0] > 0.0 ? z.par[0] : temp;
+      p_b = p;
+      break;
+    case kBcFarfield: {
+      const double u_bc[3] = {z.par[0], z.par[1], z.par[2]};
+      const double p_bc = z.par[3], rho_bc = z.par[4], gamma_bc = z.par[5];
+      double gm1 = gamma_bc - 1.0;
+      double ovgm1 = 1.0 / gm1;
+      double c_bc = sqrt(gamma_bc * p_bc / rho_bc);
+      double s_bc = pow(rho_bc, gamma_bc) / p_bc;
+      double qn_bc = dot3(nVec, u_bc);
+      double vn_bc = qn_bc;
+      double qne = dot3(nVec, vel);
+      double ce = sqrt(GAMMA * p / rho);
+      double ac1 = vn_bc > -c_bc ? qne + 2.0 * ovgm1 * ce : qn_bc + 2.0 * ovgm1 * c_bc;
+      double ac2 = vn_bc > c_bc ? qne - 2.0 * ovgm1 * ce : qn_bc - 2.0 * ovgm1 * c_bc;
+      double qnf = 0.5 * (ac1 + ac2);
+      double cf = 0.25 * (ac1 - ac2) * gm1;
+      double sf;
+      if (vn_bc > 0) {
+        for (int i = 0; i < 3; i++) vb[i] = vel[i] + (qnf - qne) * nVec[i];
+        sf = pow(rho, GAMMA) / p;
+      } else {
+        for (int i = 0; i < 3; i++) vb[i] = u_bc[i] + (qnf - qn_bc) * nVec[i];
+        sf = s_bc;
+      }
+      double rb = pow((sf * cf * cf / GAMMA), ovgm1);
+      p_b = rb * cf * cf / GAMMA;
+      T_b = p_b / (R_gas * rb);
+      break;
+    }
+    case kBcHook: {
+      const double *h = hook + 5 * (size_t)f;
+      T_b = h[0];
+      vb[0] = h[1]; vb[1] = h[2]; vb[2] = h[3];
+      p_b = h[4];
+      break;
+    }
+    default:
+      break;
+  }
+  if (!h_set) h_b = GAMMA * R_gas / (GAMMA - 1.0) * T_b;
+  double *o = bfa + 8 * (size_t)f;
+  o[0] = p_b / (R_gas * T_b);
+  o[1] = vb[0];
+  o[2] = vb[1];
+  o[3] = vb[2];
+  o[4] = p_b;
+  o[5] = T_b;
+  o[6] = h_b;
+  o[7] = 0.0;
+}
+
+// smooth Barth-Jespersen factor of one face (UgpWithCv.h:913-925, BJ undefined)
+__device__ __forceinline__ double bj_face(double alfa, double phi, double phifa, double phiMax, double phiMin,
+                                          double eps2) {
+  double dm = phifa - phi;
+  double dp = (phifa - phi) >= 0.0 ? phiMax - phi : phiMin - phi;
+  return Dev::mn(alfa, (dp * dp + 2.0 * dm * dp + eps2) / (dp * dp + 2 * dm * dm + dm * dp + eps2));
+}
+
+// ---------------------------------------------------------------------------
+// Gradients of u (vector), rho and p with boundary values, and their smooth
+// Barth-Jespersen limiters: calcCvVectorGrad / calcCvScalarGrad
+// (UgpWithCv.h:803-872), least squares (:1009-1083) or Green-Gauss
+// (:1091-1188), limiters (:881-1000). One thread per cell; the cell's sums
+// follow the reference's order (diagonal, nbocv neighbours, boundary faces).
+// ---------------------------------------------------------------------------
+template <bool LS>
+__global__ void __launch_bounds__(kBlock) k_grad(DMesh m, DParams P, const double *__restrict__ prim,
+                                                 const double *__restrict__ bfa, double *__restrict__ grad) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m.ncv) return;
+  const double *pc = prim + 8 * (size_t)c;
+  const double rho = pc[0], u[3] = {pc[1], pc[2], pc[3]}, p = pc[4];
+  const double xcv[3] = {pc[5], pc[6], pc[7]};
+  double gr[3], gp[3], gu[3][3];
+  double rmax = rho, rmin = rho, pmax = p, pmin = p;
+  double umax[3] = {u[0], u[1], u[2]}, umin[3] = {u[0], u[1], u[2]};
+  const int e0 = m.cfl_i[c], e1 = m.cfl_i[c + 1];
+  if (LS) {
+    const int noc_f = m.nbocv_i[c];
+    const double *l0 = m.lsg + 3 * (size_t)noc_f;
+    for (int j = 0; j < 3; j++) {
+      gr[j] = l0[j] * rho;
+      gp[j] = l0[j] * p;
+    }
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) gu[i][j] = l0[j] * u[i];
+    int k = 1;
+    for (int e = e0; e < e1; e++) {
+      int2 en = m.cfl[e];
+      if (en.y < 0) break;
+      const double *pn = prim + 8 * (size_t)en.y;
+      double2 a = ld2(pn), b = ld2(pn + 2);
+      double rn = a.x, un[3] = {a.y, b.x, b.y}, pnv = pn[4];
+      const double *l = m.lsg + 3 * (size_t)(noc_f + k);
+      double l_0 = l[0], l_1 = l[1], l_2 = l[2];
+      gr[0] += l_0 * rn; gr[1] += l_1 * rn; gr[2] += l_2 * rn;
+      gp[0] += l_0 * pnv; gp[1] += l_1 * pnv; gp[2] += l_2 * pnv;
+      for (int i = 0; i < 3; i++) {
+        gu[i][0] += l_0 * un[i];
+        gu[i][1] += l_1 * un[i];
+        gu[i][2] += l_2 * un[i];
+        umax[i] = Dev::mx(umax[i], un[i]);
+        umin[i] = Dev::mn(umin[i], un[i]);
+      }
+      rmax = Dev::mx(rmax, rn); rmin = Dev::mn(rmin, rn);
+      pmax = Dev::mx(pmax, pnv); pmin = Dev::mn(pmin, pnv);
+      k++;
+    }
+    for (int e = e0; e < e1; e++) {
+      int2 en = m.cfl[e];
+      if (en.y >= 0) continue;
+      int f = en.x & 0x7fffffff;
+      const double *bl = m.bfa_lsg + 3 * (size_t)f;
+      const double *bb = bfa + 8 * (size_t)f;
+      double b0 = bl[0], b1 = bl[1], b2 = bl[2];
+      double rb = bb[0], ub[3] = {bb[1], bb[2], bb[3]}, pb = bb[4];
+      for (int i = 0; i < 3; i++) {
+        gu[i][0] += b0 * ub[i];
+        gu[i][1] += b1 * ub[i];
+        gu[i][2] += b2 * ub[i];
+      }
+      gr[0] += b0 * rb; gr[1] += b1 * rb; gr[2] += b2 * rb;
+      gp[0] += b0 * pb; gp[1] += b1 * pb; gp[2] += b2 * pb;
+    }
+  } else {
+    for (int j = 0; j < 3; j++) {
+      gr[j] = 0.0;
+      gp[j] = 0.0;
+      gu[0][j] = gu[1][j] = gu[2][j] = 0.0;
+    }
+    for (int e = e0; e < e1; e++) {
+      int2 en = m.cfl[e];
+      int f = en.x & 0x7fffffff;
+      const double *fn = m.fnorm + 3 * (size_t)f;
+      double n[3] = {fn[0], fn[1], fn[2]};
+      if (en.y >= 0) {
+        const double *pn = prim + 8 * (size_t)en.y;
+        double rn = pn[0], un[3] = {pn[1], pn[2], pn[3]}, pnv = pn[4];
+        const double w0 = 0.5, w1 = 0.5;
+        const bool s0 = (en.x >= 0);  // side 0: this cell is icv0
+        double fr = s0 ? (w1 * rho + w0 * rn) / (w0 + w1) : (w1 * rn + w0 * rho) / (w0 + w1);
+        double fp = s0 ? (w1 * p + w0 * pnv) / (w0 + w1) : (w1 * pnv + w0 * p) / (w0 + w1);
+        for (int j = 0; j < 3; j++) {
+          if (s0) {
+            gr[j] += fr * n[j];
+            gp[j] += fp * n[j];
+          } else {
+            gr[j] -= fr * n[j];
+            gp[j] -= fp * n[j];
+          }
+        }
+        for (int i = 0; i < 3; i++) {
+          double fu = s0 ? (w1 * u[i] + w0 * un[i]) / (w0 + w1) : (w1 * un[i] + w0 * u[i]) / (w0 + w1);
+          for (int j = 0; j < 3; j++) {
+            if (s0)
+              gu[i][j] += fu * n[j];
+            else
+              gu[i][j] -= fu * n[j];
+          }
+          umax[i] = Dev::mx(umax[i], un[i]);
+          umin[i] = Dev::mn(umin[i], un[i]);
+        }
+        rmax = Dev::mx(rmax, rn); rmin = Dev::mn(rmin, rn);
+        pmax = Dev::mx(pmax, pnv); pmin = Dev::mn(pmin, pnv);
+      } else {
+        const double *bb = bfa + 8 * (size_t)f;
+        double rb = bb[0], ub[3] = {bb[1], bb[2], bb[3]}, pb = bb[4];
+        for (int j = 0; j < 3; j++) {
+          gr[j] += rb * n[j];
+          gp[j] += pb * n[j];
+        }
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) gu[i][j] += ub[i] * n[j];
+      }
+    }
+    const double vol = m.vol[c];
+    for (int j = 0; j < 3; j++) {
+      gr[j] /= vol;
+      gp[j] /= vol;
+    }
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) gu[i][j] /= vol;
+  }
+
+  double au[3] = {1.0, 1.0, 1.0}, ar = 1.0, ap = 1.0;
+  if (P.limiter) {
+    const double sos = sqrt(P.gamma * p / rho);
+    double e2u = P.eps * sos;
+    e2u = e2u * e2u;
+    double e2r = Dev::mx(P.eps * fabs(rho), 1.0e-8);
+    e2r = e2r * e2r;
+    double e2p = Dev::mx(P.eps * fabs(p), 1.0e-8);
+    e2p = e2p * e2p;
+    bool lu[3] = {(umax[0] - umin[0]) > 1.0e-8, (umax[1] - umin[1]) > 1.0e-8, (umax[2] - umin[2]) > 1.0e-8};
+    bool lr = (rmax - rmin) > 1.0e-12, lp = (pmax - pmin) > 1.0e-12;
+    for (int e = e0; e < e1; e++) {
+      int2 en = m.cfl[e];
+      if (en.y < 0) continue;
+      int f = en.x & 0x7fffffff;
+      const double *geo = m.fgeo + 8 * (size_t)f;
+      double r0[3] = {geo[4] - xcv[0], geo[5] - xcv[1], geo[6] - xcv[2]};
+      for (int i = 0; i < 3; i++)
+        if (lu[i]) au[i] = bj_face(au[i], u[i], u[i] + dot3(r0, gu[i]), umax[i], umin[i], e2u);
+      if (lr) ar = bj_face(ar, rho, rho + dot3(r0, gr), rmax, rmin, e2r);
+      if (lp) ap = bj_face(ap, p, p + dot3(r0, gp), pmax, pmin, e2p);
+    }
+    for (int i = 0; i < 3; i++)
+      if (!lu[i]) au[i] = 0.0;
+    if (!lr) ar = 0.0;
+    if (!lp) ap = 0.0;
+  }
+  double *o = grad + 16 * (size_t)c;
+  o[0] = gr[0] * ar; o[1] = gr[1] * ar; o[2] = gr[2] * ar;
+  o[3] = gp[0] * ap; o[4] = gp[1] * ap; o[5] = gp[2] * ap;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) o[6 + 3 * i + j] = gu[i][j] * au[i];
+  o[15] = ar;
+}
+
+// Reconstructed face state of one side (JoeWithModels.cpp:2832-2921): rho and
+// p extrapolated first; both revert to the cell value when either is not
+// positive, and the velocity is then left first order.
+struct Side {
+  double rho, u[3], p;
+};
+
+__device__ __forceinline__ bool reconstruct(Side &s, const double *pc, const double *gc, const double *xfa) {
+  double r[3] = {xfa[0] - pc[5], xfa[1] - pc[6], xfa[2] - pc[7]};
+  s.rho = pc[0] + dot3(r, gc);
+  s.p = pc[4] + dot3(r, gc + 3);
+  s.u[0] = pc[1];
+  s.u[1] = pc[2];
+  s.u[2] = pc[3];
+  if ((s.p <= 0.0) || (s.rho <= 0.0)) {
+    s.p = pc[4];
+    s.rho = pc[0];
+    return false;
+  }
+  s.u[0] += dot3(r, gc + 6);
+  s.u[1] += dot3(r, gc + 9);
+  s.u[2] += dot3(r, gc + 12);
+  return true;
+}
+
+__device__ __forceinline__ void face_flux(int scheme, double F[5], const Side &L, const Side &R, double gam,
+                                          double area, const double *n) {
+  if (scheme == kRoe)
+    flux_roe(F, L.rho, L.u, L.p, R.rho, R.u, R.p, gam, area, n);
+  else if (scheme == kAusm)
+    flux_ausm(F, L.rho, L.u, L.p, R.rho, R.u, R.p, gam, area, n);
+  else
+    flux_hllc(F, L.rho, L.u, L.p, R.rho, R.u, R.p, gam, area, n);
+}
+
+// ---------------------------------------------------------------------------
+// Residual: calcEulerFlux face loops (JoeWithModels.cpp:2786-3569) and the
+// default sourceHook (JoeWithModels.h:111-129), gathered per cell in the
+// reference's accumulation order so every cell sum matches it exactly.
+// ---------------------------------------------------------------------------
+template <int SCHEME, bool SND>
+__global__ void __launch_bounds__(kBlock) k_flux(DMesh m, DParams P, const double *__restrict__ prim,
+                                                 const double *__restrict__ grad, const double *__restrict__ bfa,
+                                                 const double *__restrict__ q, double *__restrict__ rhs,
+                                                 int *__restrict__ nfirst) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m.ncv) return;
+  const double gam = P.gamma;
+  double pc[8], gc[15];
+  {
+    const double *s = prim + 8 * (size_t)c;
+    for (int k = 0; k < 8; k += 2) {
+      double2 v = ld2(s + k);
+      pc[k] = v.x;
+      pc[k + 1] = v.y;
+    }
+    if (SND) {
+      const double *g = grad + 16 * (size_t)c;
+      for (int k = 0; k < 14; k += 2) {
+        double2 v = ld2(g + k);
+        gc[k] = v.x;
+        gc[k + 1] = v.y;
+      }
+      gc[14] = g[14];
+    }
+  }
+  Side own_cc;
+  own_cc.rho = pc[0];
+  own_cc.u[0] = pc[1]; own_cc.u[1] = pc[2]; own_cc.u[2] = pc[3];
+  own_cc.p = pc[4];
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  int nlow = 0;
+  const int e0 = m.cfl_i[c], e1 = m.cfl_i[c + 1];
+  for (int e = e0; e < e1; e++) {
+    int2 en = m.cfl[e];
+    int f = en.x & 0x7fffffff;
+    const double *geo = m.fgeo + 8 * (size_t)f;
+    double n[3], area, xfa[3];
+    {
+      double2 a = ld2(geo), b = ld2(geo + 2), cc = ld2(geo + 4);
+      n[0] = a.x; n[1] = a.y; n[2] = b.x; area = b.y;
+      xfa[0] = cc.x; xfa[1] = cc.y; xfa[2] = geo[6];
+    }
+    double F[5];
+    if (en.y >= 0) {
+      const bool s0 = (en.x >= 0);
+      double pn[8], gn[15];
+      const double *s = prim + 8 * (size_t)en.y;
+      for (int k = 0; k < 8; k += 2) {
+        double2 v = ld2(s + k);
+        pn[k] = v.x;
+        pn[k + 1] = v.y;
+      }
+      Side so, sn;
+      if (SND) {
+        const double *g = grad + 16 * (size_t)en.y;
+        for (int k = 0; k < 14; k += 2) {
+          double2 v = ld2(g + k);
+          gn[k] = v.x;
+          gn[k + 1] = v.y;
+        }
+        gn[14] = g[14];
+        if (!reconstruct(so, pc, gc, xfa)) nlow++;
+        reconstruct(sn, pn, gn, xfa);
+      } else {
+        so = own_cc;
+        sn.rho = pn[0]; sn.u[0] = pn[1]; sn.u[1] = pn[2]; sn.u[2] = pn[3]; sn.p = pn[4];
+      }
+      if (s0) {
+        face_flux(SCHEME, F, so, sn, gam, area, n);
+        for (int k = 0; k < 5; k++) acc[k] -= F[k];
+      } else {
+        face_flux(SCHEME, F, sn, so, gam, area, n);
+        for (int k = 0; k < 5; k++) acc[k] += F[k];
+      }
+    } else {
+      const ZoneDev &z = m.zones[-en.y - 1];
+      const double *bb = bfa + 8 * (size_t)f;
+      Side sb;
+      sb.rho = bb[0]; sb.u[0] = bb[1]; sb.u[1] = bb[2]; sb.u[2] = bb[3]; sb.p = bb[4];
+      if (z.bc == kBcSymmetry || z.bc == kBcWall) {
+        face_flux(SCHEME, F, sb, sb, gam, area, n);
+        if (z.bc == kBcSymmetry && P.symm_bc == kSymmAnalytical) {
+          acc[0] -= 0.0;
+          for (int i = 0; i < 3; i++) acc[1 + i] -= sb.p * n[i] * area;
+          acc[4] -= 0.0;
+        } else {
+          for (int k = 0; k < 5; k++) acc[k] -= F[k];
+        }
+      } else {
+        Side so = own_cc;
+        if (SND && !reconstruct(so, pc, gc, xfa)) nlow++;
+        face_flux(SCHEME, F, so, sb, gam, area, n);
+        for (int k = 0; k < 5; k++) acc[k] -= F[k];
+      }
+    }
+  }
+  // sourceHook: gravity (JoeWithModels.h:111-129)
+  const double vol = m.vol[c];
+  if (P.gravity_on) {
+    acc[1] += vol * pc[0] * P.g[0];
+    acc[2] += vol * pc[0] * P.g[1];
+    acc[3] += vol * pc[0] * P.g[2];
+    acc[4] += vol * (P.g[0] * q[m.ncv + c] + P.g[1] * q[2 * m.ncv + c] + P.g[2] * q[3 * m.ncv + c]);
+  } else {
+    // the same terms with g = 0: +0 for rhou, and -0 for rhoE only when
+    // all three 0*rhou products are -0
+    acc[1] += 0.0;
+    acc[2] += 0.0;
+    acc[3] += 0.0;
+    bool allneg = signbit(pc[1]) && signbit(pc[2]) && signbit(pc[3]);
+    acc[4] += allneg ? -0.0 : 0.0;
+  }
+  for (int k = 0; k < 5; k++) rhs[(size_t)k * m.ncv + c] = acc[k];
+  if (nlow) atomicAdd(nfirst, nlow);
+}
+
+// ---------------------------------------------------------------------------
+// Block Jacobian: the implicit parts of the same face loops
+// (JoeWithModels.cpp:3029-3101, 3213-3277, 3359-3385, 3517-3563) with
+// cell-centre states, gathered per row; optionally the backward-Euler
+// diagonal V/dt and the under-relaxed right-hand side (:1011-1024).
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void store_block(T *dst, const double J[5][5], bool negate, bool accumulate) {
+  for (int i = 0; i < 5; i++)
+    for (int j = 0; j < 5; j++) {
+      double v = negate ? (accumulate ? (double)dst[5 * i + j] - J[i][j] : 0.0 - J[i][j])
+                        : (accumulate ? (double)dst[5 * i + j] + J[i][j] : 0.0 + J[i][j]);
+      dst[5 * i + j] = (T)v;
+    }
+}
+
+__device__ __forceinline__ void face_jac(int scheme, double J[5][5], int which, const Side &L, const Side &R,
+                                         double gam, double area, const double *n) {
+  if (scheme == kRoe)
+    jac_roe(J, which, L.rho, L.u, L.p, R.rho, R.u, R.p, gam, area, n);
+  else
+    jac_hllc(J, which, L.rho, L.u, L.p, R.rho, R.u, R.p, gam, area, n);
+}
+
+template <typename T, int SCHEME>
+__global__ void __launch_bounds__(kBlock) k_jac(DMesh m, DParams P, const double *__restrict__ prim,
+                                                const double *__restrict__ bfa, T *__restrict__ A, int step_mode,
+                                                const double *__restrict__ dt, const double *__restrict__ rhs,
+                                                double relax, T *__restrict__ rhs_ls, T *__restrict__ phi) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m.ncv) return;
+  const double gam = P.gamma;
+  const double *pc = prim + 8 * (size_t)c;
+  Side own;
+  own.rho = pc[0]; own.u[0] = pc[1]; own.u[1] = pc[2]; own.u[2] = pc[3]; own.p = pc[4];
+  double D[5][5];
+  for (int i = 0; i < 5; i++)
+    for (int j = 0; j < 5; j++) D[i][j] = 0.0;
+  double J[5][5];
+  const int e0 = m.cfl_i[c], e1 = m.cfl_i[c + 1];
+  const int row = m.nbocv_i[c];
+  for (int e = e0; e < e1; e++) {
+    int2 en = m.cfl[e];
+    int f = en.x & 0x7fffffff;
+    const double *geo = m.fgeo + 8 * (size_t)f;
+    const double n[3] = {geo[0], geo[1], geo[2]};
+    const double area = geo[3];
+    if (en.y >= 0) {
+      const double *pn = prim + 8 * (size_t)en.y;
+      Side nb;
+      nb.rho = pn[0]; nb.u[0] = pn[1]; nb.u[1] = pn[2]; nb.u[2] = pn[3]; nb.p = pn[4];
+      const int noc = m.cfl_noc[e];
+      const bool accum = (noc >> 30) & 1;
+      T *off = A + 25 * (size_t)(noc & 0x3fffffff);
+      if (en.x >= 0) {  // this cell is icv0: A[noc00] += Apl, A[noc01] += Ami
+        face_jac(SCHEME, J, 0, own, nb, gam, area, n);
+        for (int i = 0; i < 5; i++)
+          for (int j = 0; j < 5; j++) D[i][j] += J[i][j];
+        face_jac(SCHEME, J, 1, own, nb, gam, area, n);
+        store_block(off, J, false, accum);
+      } else {  // this cell is icv1: A[noc11] -= Ami, A[noc10] -= Apl
+        face_jac(SCHEME, J, 1, nb, own, gam, area, n);
+        for (int i = 0; i < 5; i++)
+          for (int j = 0; j < 5; j++) D[i][j] -= J[i][j];
+        face_jac(SCHEME, J, 0, nb, own, gam, area, n);
+        store_block(off, J, true, accum);
+      }
+    } else {
+      const ZoneDev &z = m.zones[-en.y - 1];
+      const double *bb = bfa + 8 * (size_t)f;
+      Side sb;
+      sb.rho = bb[0]; sb.u[0] = bb[1]; sb.u[1] = bb[2]; sb.u[2] = bb[3]; sb.p = bb[4];
+      if (z.bc == kBcSymmetry || z.bc == kBcWall) {
+        if (z.bc == kBcWall || P.symm_bc == kSymmOriginal) {
+          face_jac(SCHEME, J, 0, sb, sb, gam, area, n);
+          for (int i = 0; i < 5; i++)
+            for (int j = 0; j < 5; j++) D[i][j] += J[i][j];
+        } else if (P.symm_bc == kSymmNew) {
+          double K[5][5];
+          face_jac(SCHEME, J, 0, sb, sb, gam, area, n);
+          face_jac(SCHEME, K, 1, sb, sb, gam, area, n);
+          for (int i = 0; i < 5; i++)
+            for (int j = 0; j < 5; j++) D[i][j] += J[i][j] + K[i][j];
+        } else {  // ANALYTICAL, JoeWithModels.cpp:3243-3269
+          double a2 = gam - 1.0;
+          double phi2 = 0.5 * a2 * (sb.u[0] * sb.u[0] + sb.u[1] * sb.u[1] + sb.u[2] * sb.u[2]);
+          for (int i = 0; i < 3; i++) {
+            D[i + 1][0] += phi2 * n[i] * area;
+            for (int j = 0; j < 3; j++) D[i + 1][j + 1] += -a2 * sb.u[j] * n[i] * area;
+            D[i + 1][4] += a2 * n[i] * area;
+          }
+          for (int j = 0; j < 5; j++) {
+            D[0][j] += 0.0;
+            D[4][j] += 0.0;
+          }
+        }
+      } else {
+        face_jac(SCHEME, J, 0, own, sb, gam, area, n);
+        for (int i = 0; i < 5; i++)
+          for (int j = 0; j < 5; j++) D[i][j] += J[i][j];
+      }
+    }
+  }
+  const double vol = m.vol[c];
+  if (P.gravity_on) {
+    D[1][0] -= vol * P.g[0];
+    D[2][0] -= vol * P.g[1];
+    D[3][0] -= vol * P.g[2];
+    D[4][1] -= vol * P.g[0];
+    D[4][2] -= vol * P.g[1];
+    D[4][3] -= vol * P.g[2];
+  }
+  if (step_mode) {
+    double tmp = vol / (dt[c]);
+    for (int i = 0; i < 5; i++) D[i][i] += tmp;
+    for (int k = 0; k < 5; k++) {
+      rhs_ls[5 * (size_t)c + k] = (T)(relax * rhs[(size_t)k * m.ncv + c]);
+      phi[5 * (size_t)c + k] = (T)0.0;
+    }
+  }
+  T *d = A + 25 * (size_t)row;
+  for (int i = 0; i < 5; i++)
+    for (int j = 0; j < 5; j++) d[5 * i + j] = (T)D[i][j];
+}
+
+// ---------------------------------------------------------------------------
+// deterministic block reductions with a "last block finishes" tail
+// ---------------------------------------------------------------------------
+template <int NV>
+__device__ __forceinline__ void block_reduce(double (&v)[NV], double *sh /* [NV][kBlock/64] */, bool use_max) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int k = 0; k < NV; k++) {
+    double x = v[k];
+    for (int off = 32; off > 0; off >>= 1) {
+      double y = __shfl_down(x, off, 64);
+      x = use_max ? Dev::mx(x, y) : x + y;
+    }
+    if (lane == 0) sh[k * 16 + wid] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < NV; k++) {
+      double x = sh[k * 16];
+      for (int w = 1; w < nw; w++) x = use_max ? Dev::mx(x, sh[k * 16 + w]) : x + sh[k * 16 + w];
+      v[k] = x;
+    }
+  }
+}
+
+// returns true in the block that saw the last ticket; partials has
+// gridDim.x * NV slots
+template <int NV>
+__device__ bool last_block(double (&v)[NV], double *partials, unsigned *counter, double *sh, bool use_max) {
+  block_reduce<NV>(v, sh, use_max);
+  __shared__ bool is_last;
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < NV; k++) partials[(size_t)k * gridDim.x + blockIdx.x] = v[k];
+    __threadfence();
+    unsigned t = atomicAdd(counter, 1u);
+    is_last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!is_last) return false;
+  __threadfence();
+  // fixed-order final sum over block partials
+  double w[NV];
+  for (int k = 0; k < NV; k++) {
+    double x = use_max ? 0.0 : 0.0;
+    bool first = true;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
+      double y = __builtin_nontemporal_load(&partials[(size_t)k * gridDim.x + b]);
+      x = first ? y : (use_max ? Dev::mx(x, y) : x + y);
+      first = false;
+    }
+    w[k] = first ? 0.0 : x;
+  }
+  __syncthreads();
+  block_reduce<NV>(w, sh, use_max);
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < NV; k++) v[k] = w[k];
+    *counter = 0u;
+  }
+  return true;
+}
+
+// local time step (MAX_BASED), UgpWithCvCompFlow.cpp:27-60; mul = mut = 0.
+// dt_min reduced into S->dt_min; CFL_MIN mode is applied by k_dt_fill.
+__global__ void __launch_bounds__(kBlock) k_dt(DMesh m, const double *__restrict__ prim, const double *__restrict__ q,
+                                               double gam, double cfl, int mode, double dt_fixed,
+                                               double *__restrict__ dt, double *partials, unsigned *counter,
+                                               SolverScalars *S) {
+  __shared__ double sh[16];
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  double dtc = 1.0e20;
+  if (c < m.ncv) {
+    if (mode == 2) {
+      dtc = dt_fixed;
+    } else {
+      const double *pc = prim + 8 * (size_t)c;
+      const double rho = pc[0], p = pc[4];
+      const double ru[3] = {q[m.ncv + c], q[2 * m.ncv + c], q[3 * m.ncv + c]};
+      double cs = sqrt(gam * p / rho);
+      double lambdaMax = 0.0;
+      for (int e = m.cfl_i[c]; e < m.cfl_i[c + 1]; e++) {
+        int f = m.cfl[e].x & 0x7fffffff;
+        const double *geo = m.fgeo + 8 * (size_t)f;
+        const double n[3] = {geo[0], geo[1], geo[2]};
+        double area = geo[3];
+        double Uk = dot3(ru, n) / rho;
+        double lambda = (fabs(Uk) + cs) * area;
+        lambdaMax = Dev::mx(lambdaMax, lambda);
+      }
+      dtc = cfl * m.vol[c] / lambdaMax;
+    }
+    dt[c] = dtc;
+  }
+  double v[1] = {-dtc};  // min via max of negatives
+  if (last_block<1>(v, partials, counter, sh, true) && threadIdx.x == 0) S->dt_min = -v[0];
+}
+
+__global__ void k_dt_fill(int ncv, double *dt, const SolverScalars *S) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < ncv) dt[c] = S->dt_min;
+}
+
+// ---------------------------------------------------------------------------
+// block-Jacobi BiCGStab, UgpWithCv.cpp:2660-2876. Vectors are [ncv][5]; the
+// matrix-vector products keep the reference's per-row order (blocks in
+// nbocv order, columns ascending) using five lanes per cell.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_lu(int ncv, const int *__restrict__ nbocv_i, const T *__restrict__ A,
+                                               T *__restrict__ LU) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncv) return;
+  T a[5][5];
+  const T *s = A + 25 * (size_t)nbocv_i[c];
+  for (int i = 0; i < 5; i++)
+    for (int j = 0; j < 5; j++) a[i][j] = s[5 * i + j];
+  ludeco5<T>(a);
+  T *d = LU + 25 * (size_t)c;
+  for (int i = 0; i < 5; i++)
+    for (int j = 0; j < 5; j++) d[5 * i + j] = a[i][j];
+}
+
+// y = A x for row component (c, i) = (tid / 5, tid % 5)
+template <typename T>
+__device__ __forceinline__ T spmv_row(const int *__restrict__ nbocv_i, const int *__restrict__ nbocv_v,
+                                      const T *__restrict__ A, const T *__restrict__ x, int c, int i) {
+  T acc = (T)0.0;
+  for (int noc = nbocv_i[c]; noc < nbocv_i[c + 1]; noc++) {
+    const T *a = A + 25 * (size_t)noc + 5 * i;
+    const T *xv = x + 5 * (size_t)nbocv_v[noc];
+    for (int j = 0; j < 5; j++) acc += a[j] * xv[j];
+  }
+  return acc;
+}
+
+constexpr int kCellsPerBlock = kBlock / 5;  // 51 cells, 255 lanes active
+
+// res = rhs - A phi; init: res0 = res, res0_max; always: rho_next = res.res0,
+// res_max = max |LU^-1 res|; the last block runs the reference's stop test and
+// prepares beta for the next iteration.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_resid(int ncv, const int *__restrict__ nbocv_i,
+                                                  const int *__restrict__ nbocv_v, const T *__restrict__ A,
+                                                  const T *__restrict__ LU, const T *__restrict__ phi,
+                                                  const T *__restrict__ rhs, T *__restrict__ res, T *__restrict__ res0,
+                                                  int init, double *partials, unsigned *counter, SolverScalars *S) {
+  __shared__ double sh[32];
+  __shared__ T cellv[kCellsPerBlock][5];
+  if (!init && S->done) return;
+  const int lc = threadIdx.x / 5, i = threadIdx.x % 5;
+  const int c = blockIdx.x * kCellsPerBlock + lc;
+  const bool active = (lc < kCellsPerBlock) && (c < ncv);
+  T r = (T)0.0;
+  if (active) {
+    T ax = spmv_row<T>(nbocv_i, nbocv_v, A, phi, c, i);
+    r = rhs[5 * (size_t)c + i] - ax;
+    res[5 * (size_t)c + i] = r;
+    if (init) res0[5 * (size_t)c + i] = r;
+    cellv[lc][i] = r;
+  }
+  __syncthreads();
+  double v[2] = {0.0, 0.0};
+  if (active) {
+    T r0v = init ? r : res0[5 * (size_t)c + i];
+    v[0] = (double)r * (double)r0v;
+    T tmp[5] = {cellv[lc][0], cellv[lc][1], cellv[lc][2], cellv[lc][3], cellv[lc][4]};
+    lusolv5<T>(LU + 25 * (size_t)c, tmp);
+    v[1] = fabs((double)tmp[i]);
+  }
+  double vs[1] = {v[0]}, vm[1] = {v[1]};
+  // two reductions: sum for the dot, max for the norm (separate partial slots)
+  block_reduce<1>(vs, sh, false);
+  __syncthreads();
+  block_reduce<1>(vm, sh + 16, true);
+  __shared__ bool is_last;
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = vs[0];
+    partials[gridDim.x + blockIdx.x] = vm[0];
+    __threadfence();
+    unsigned t = atomicAdd(counter, 1u);
+    is_last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  double s = 0.0, mxv = 0.0;
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
+    s += __builtin_nontemporal_load(&partials[b]);
+    mxv = Dev::mx(mxv, __builtin_nontemporal_load(&partials[gridDim.x + b]));
+  }
+  vs[0] = s;
+  vm[0] = mxv;
+  __syncthreads();
+  block_reduce<1>(vs, sh, false);
+  __syncthreads();
+  block_reduce<1>(vm, sh + 16, true);
+  if (threadIdx.x == 0) {
+    *counter = 0u;
+    double rho_next = vs[0];
+    if (init) {
+      S->res0_max = vm[0];
+      S->rho = 1.0;
+      S->alpha = 1.0;
+      S->omega = 1.0;
+      S->iter = 0;
+      S->done = 0;
+    } else {
+      S->res_max = vm[0];
+      double res_max = vm[0];
+      if ((res_max <= S->zero_abs) || (res_max / (S->res0_max + 1.0E-12) <= S->zero_rel))
+        S->done = 1;
+      else if (S->iter > S->maxiter)
+        S->done = 2;
+    }
+    if (!S->done) {  // start of the next iteration (UgpWithCv.cpp:2736-2749)
+      S->iter += 1;
+      double rho_prime = -S->omega * S->rho;
+      S->rho = rho_next;
+      if (fabs(rho_prime) < 1.0E-20) rho_prime = 1.0E-20;
+      S->rho_prime = rho_prime;
+      S->beta = S->alpha * S->rho / rho_prime;
+    }
+  }
+}
+
+// p = res - beta (p - omega v); phat = LU^-1 p   (one thread per cell)
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_p(int ncv, const T *__restrict__ LU, const T *__restrict__ res,
+                                              const T *__restrict__ v, T *__restrict__ p, T *__restrict__ phat,
+                                              const SolverScalars *S) {
+  if (S->done) return;
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncv) return;
+  const T beta = (T)S->beta, omega = (T)S->omega;
+  T t[5];
+  for (int k = 0; k < 5; k++) {
+    size_t o = 5 * (size_t)c + k;
+    T pv = res[o] - beta * (p[o] - omega * v[o]);
+    p[o] = pv;
+    t[k] = pv;
+  }
+  lusolv5<T>(LU + 25 * (size_t)c, t);
+  for (int k = 0; k < 5; k++) phat[5 * (size_t)c + k] = t[k];
+}
+
+// y = A x and a fused dot: mode 0: gamma = y.res0 -> alpha;
+// mode 1: (s.t, t.t) -> omega
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_spmv_dot(int ncv, const int *__restrict__ nbocv_i,
+                                                     const int *__restrict__ nbocv_v, const T *__restrict__ A,
+                                                     const T *__restrict__ x, T *__restrict__ y,
+                                                     const T *__restrict__ other, int mode, double *partials,
+                                                     unsigned *counter, SolverScalars *S) {
+  __shared__ double sh[32];
+  if (S->done) return;
+  const int lc = threadIdx.x / 5, i = threadIdx.x % 5;
+  const int c = blockIdx.x * kCellsPerBlock + lc;
+  double v[2] = {0.0, 0.0};
+  if (lc < kCellsPerBlock && c < ncv) {
+    T yv = spmv_row<T>(nbocv_i, nbocv_v, A, x, c, i);
+    y[5 * (size_t)c + i] = yv;
+    double o = (double)other[5 * (size_t)c + i];
+    v[0] = (double)yv * o;        // mode 0: v.res0 ; mode 1: t.s
+    v[1] = (double)yv * (double)yv;  // t.t
+  }
+  if (last_block<2>(v, partials, counter, sh, false) && threadIdx.x == 0) {
+    if (mode == 0) {
+      double gamma = v[0];
+      if (fabs(gamma) < 1.0E-20) gamma = 1.0E-20;
+      S->gamma = gamma;
+      S->alpha = S->rho / gamma;
+    } else {
+      S->omega = v[0] / (v[1] + 1.0E-20);
+    }
+  }
+}
+
+// s = res - alpha v; shat = LU^-1 s
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_s(int ncv, const T *__restrict__ LU, const T *__restrict__ res,
+                                              const T *__restrict__ v, T *__restrict__ s, T *__restrict__ shat,
+                                              const SolverScalars *S) {
+  if (S->done) return;
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncv) return;
+  const T alpha = (T)S->alpha;
+  T t[5];
+  for (int k = 0; k < 5; k++) {
+    size_t o = 5 * (size_t)c + k;
+    T sv = res[o] - alpha * v[o];
+    s[o] = sv;
+    t[k] = sv;
+  }
+  lusolv5<T>(LU + 25 * (size_t)c, t);
+  for (int k = 0; k < 5; k++) shat[5 * (size_t)c + k] = t[k];
+}
+
+// phi += alpha phat + omega shat
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_x(int n5, T *__restrict__ phi, const T *__restrict__ phat,
+                                              const T *__restrict__ shat, const SolverScalars *S) {
+  if (S->done) return;
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n5) return;
+  phi[k] += (T)S->alpha * phat[k] + (T)S->omega * shat[k];
+}
+
+// Q += dQ (JoeWithModels.cpp:1028-1035)
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_update(int ncv, double *__restrict__ q, const T *__restrict__ dq) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncv) return;
+  for (int k = 0; k < 5; k++) q[(size_t)k * ncv + c] += (double)dq[5 * (size_t)c + k];
+}
+
+// L1 residual sums (JoeWithModels.cpp:1119-1130)
+__global__ void __launch_bounds__(kBlock) k_l1(int ncv, const double *__restrict__ rhs, double *partials,
+                                               unsigned *counter, SolverScalars *S) {
+  __shared__ double sh[16 * 5];
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  double v[5] = {0, 0, 0, 0, 0};
+  if (c < ncv)
+    for (int k = 0; k < 5; k++) v[k] = fabs(rhs[(size_t)k * ncv + c]);
+  if (last_block<5>(v, partials, counter, sh, false) && threadIdx.x == 0)
+    for (int k = 0; k < 5; k++) S->resid[k] = v[k];
+}
+
+__global__ void k_to_double(int n, const float *__restrict__ s, double *__restrict__ d) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) d[k] = (double)s[k];
+}
+
+inline int nblk(long long n, int b = kBlock) { return (int)((n + b - 1) / b); }
+
+template <typename T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  void alloc(size_t count) {
+    free();
+    n = count;
+    if (count) HIPCHECK(hipMalloc((void **)&p, sizeof(T) * count));
+  }
+  void upload(const T *h, size_t count) {
+    alloc(count);
+    if (count) HIPCHECK(hipMemcpy(p, h, sizeof(T) * count, hipMemcpyHostToDevice));
+  }
+  void free() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DBuf() { free(); }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Joe.in -> parameters (UgpWithCvCompFlow.h:57-502, JoeWithModels.cpp:881-929,
+// boundary keywords JoeWithModels.cpp:3857-4300)
+// ---------------------------------------------------------------------------
+void parse_params(const std::string &joe_in, const Mesh &m, SolverParams &p, std::vector<ZoneBc> &zbc) {
+  ParamMap pm(joe_in);
+  p.gamma = pm.dbl("GAMMA", 1.4);
+  if (pm.has("ALTITUDE")) throw std::invalid_argument("ALTITUDE is not supported");
+  p.T_ref = pm.dbl("T_REF");
+  p.p_ref = pm.dbl("P_REF");
+  p.rho_ref = pm.dbl("RHO_REF");
+  p.R_gas = p.p_ref / p.rho_ref / p.T_ref;
+  if (pm.dbl("MU_REF", 0.0) > 0.0) throw std::invalid_argument("MU_REF > 0 (viscous flux) is not supported yet");
+  p.gravity[0] = pm.dbl("GX", 0.0);
+  p.gravity[1] = pm.dbl("GY", 0.0);
+  p.gravity[2] = pm.dbl("GZ", 0.0);
+  p.second_order = pm.has("SPATIAL_SECOND_ORDER");
+  if (pm.has("SCALAR_FIRST_ORDER")) {
+    // no transported scalars on this path: nothing to do
+  }
+  std::string si = pm.str("SPACE_INTEGRATION", "HLLC");
+  if (si == "HLLC") p.scheme = kHllc;
+  else if (si == "ROE") p.scheme = kRoe;
+  else if (si == "AUSM") p.scheme = kAusm;
+  else throw std::invalid_argument("SPACE_INTEGRATION = " + si + " is not supported (HLLC, ROE, AUSM)");
+  if (pm.has("GRAD_RECONSTRUCTION")) {
+    std::string s = pm.sub("GRAD_RECONSTRUCTION", "SCHEME");
+    if (s == "LEASTSQUARES") p.grad_ls = true;
+    else if (s == "GREENGAUSS") p.grad_ls = false;
+    else throw std::invalid_argument("GRAD_RECONSTRUCTION SCHEME=" + s + " is not supported");
+    std::string l = pm.sub("GRAD_RECONSTRUCTION", "LIMITER");
+    if (l == "NO" || l == "NOLIMITER") p.limiter = false;
+    else if (l == "YES" || l == "BARTH_JESPERSEN_MOD") p.limiter = true;
+    else throw std::invalid_argument("GRAD_RECONSTRUCTION LIMITER=" + l + " not recognised");
+    p.eps_limiter = pm.sub_d("GRAD_RECONSTRUCTION", "EPS");
+  } else {
+    p.grad_ls = false;
+    p.limiter = true;
+    p.eps_limiter = 0.001;
+  }
+  std::string sb = pm.str("SYMM_BC", "ORIGINAL_NUMERIC");
+  if (sb == "ORIGINAL_NUMERIC") p.symm_bc = kSymmOriginal;
+  else if (sb == "NEW_NUMERIC") p.symm_bc = kSymmNew;
+  else if (sb == "ANALYTICAL") p.symm_bc = kSymmAnalytical;
+  else throw std::invalid_argument("SYMM_BC = " + sb + " not recognised");
+  if (pm.has("CFL")) {
+    p.dt_mode = 0;
+    p.cfl = pm.dbl("CFL");
+  } else if (pm.has("CFL_MIN")) {
+    p.dt_mode = 1;
+    p.cfl = pm.dbl("CFL_MIN");
+  } else if (pm.has("DT")) {
+    p.dt_mode = 2;
+    p.dt_fixed = pm.dbl("DT");
+  } else {
+    throw std::invalid_argument("DT, CFL or CFL_MIN not specified");
+  }
+  if (pm.str("TIME_STEP", "MAX_BASED") != "MAX_BASED") throw std::invalid_argument("TIME_STEP other than MAX_BASED");
+  std::string ti = pm.str("TIME_INTEGRATION", "BACKWARD_EULER");
+  if (ti != "BACKWARD_EULER") throw std::invalid_argument("TIME_INTEGRATION = " + ti + " is not supported");
+  std::string ls = pm.str("LINEAR_SOLVER_NS", "BCGSTAB");
+  if (ls != "BCGSTAB") throw std::invalid_argument("LINEAR_SOLVER_NS = " + ls + " is not supported (BCGSTAB)");
+  p.under_relax = pm.dbl("UNDER_RELAXATION", 0.3);
+  if (pm.has("LINEAR_SOLVER_NS_TRESHOLDS")) {
+    p.ls_maxiter = pm.sub_i("LINEAR_SOLVER_NS_TRESHOLDS", "MAX_ITER");
+    p.ls_abs = pm.sub_d("LINEAR_SOLVER_NS_TRESHOLDS", "ABS_RESID");
+    p.ls_rel = pm.sub_d("LINEAR_SOLVER_NS_TRESHOLDS", "REL_RESID");
+  }
+  if (pm.has("CFL_RAMP")) {
+    p.cfl_after = pm.sub_i("CFL_RAMP", "AFTER_ITER");
+    p.cfl_interval = pm.sub_i("CFL_RAMP", "INTERVAL_ITER");
+    p.cfl_factor = pm.sub_d("CFL_RAMP", "FACTOR_CFL");
+    p.cfl_max = pm.sub_d("CFL_RAMP", "MAX_CFL");
+  }
+  p.nsteps = pm.integer("NSTEPS", 100);
+  p.resid_energ_th = pm.dbl("RESID_ENERG_TH", 0.0);
+  p.rho_init = pm.has("RHO_INITIAL") ? pm.dbl("RHO_INITIAL") : p.rho_ref;
+  p.p_init = pm.has("P_INITIAL") ? pm.dbl("P_INITIAL") : p.p_ref;
+  if (pm.has("U_INITIAL")) {
+    const auto *t = pm.get("U_INITIAL");
+    if (t->size() < 4) throw std::invalid_argument("U_INITIAL needs 3 values");
+    for (int i = 0; i < 3; i++) p.u_init[i] = ParamMap::to_d((*t)[1 + i], "U_INITIAL");
+  }
+  zbc.assign(m.zones.size(), ZoneBc());
+  for (size_t iz = 0; iz < m.zones.size(); iz++) {
+    const Zone &z = m.zones[iz];
+    if (z.kind != kZoneBoundary) continue;
+    const auto *t = pm.get(z.name);
+    if (!t || t->size() < 2) throw std::invalid_argument("no bc set for zone " + z.name);
+    const std::string &kw = (*t)[1];
+    auto val = [&](size_t pos) {
+      if (pos >= t->size()) throw std::invalid_argument("zone " + z.name + ": missing value for " + kw);
+      return ParamMap::to_d((*t)[pos], z.name);
+    };
+    ZoneBc &b = zbc[iz];
+    if (kw == "CBC" || kw == "CBC_SUBSONIC_INLET") {
+      b.bc = kw == "CBC" ? kBcCbc : kBcCbcSubsonicInlet;
+      for (int k = 0; k < 5; k++) b.par[k] = val(2 + k);
+    } else if (kw == "CBC_SUBSONIC_OUTLET") {
+      b.bc = kBcCbcSubsonicOutlet;
+      b.par[0] = val(2);
+    } else if (kw == "SYMMETRY") {
+      b.bc = kBcSymmetry;
+    } else if (kw == "NEUMANN") {
+      b.bc = kBcNeumann;
+    } else if (kw == "WALL") {
+      b.bc = kBcWall;
+      for (size_t k = 0; k < t->size(); k++)
+        if ((*t)[k] == "TEMP" && k > 0) b.par[0] = val(k + 1);
+    } else if (kw == "FARFIELD") {
+      b.bc = kBcFarfield;
+      for (int k = 0; k < 6; k++) b.par[k] = val(2 + k);
+    } else if (kw == "HOOK") {
+      b.bc = kBcHook;
+    } else {
+      throw std::invalid_argument("cannot recognize bc for zone " + z.name + ": " + kw);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct Ctx::Impl {
+  int ncv = 0, nfa = 0, nfa_b = 0, nnz = 0, nzone = 0;
+  int precision = 0;
+  hipStream_t stream = nullptr;
+  SolverParams par;
+  std::vector<ZoneBc> zbc;
+  std::vector<int> zone_slot;  // mesh zone index -> device zone slot
+  DParams dp;
+  DMesh dm;
+  double cfl = 0.0;
+  int step_no = 0;
+  bool have_state = false;
+  // mesh buffers
+  DBuf<double> fgeo, fnorm, lsg, bfa_lsg, vol;
+  DBuf<int> cfl_i, cfl_noc, nbocv_i, nbocv_v, bfa_cell, bfa_zone;
+  DBuf<int2> cfl_e;
+  DBuf<ZoneDev> zones;
+  std::vector<int> h_nbocv_i;
+  bool dup_blocks = false;
+  // state
+  DBuf<double> q, prim, grad, bfa, rhs, dt, hook;
+  DBuf<int> counters;  // [0] negative pressure, [1] first-order fallbacks
+  // solver (T = double or float, both allocated lazily)
+  DBuf<double> A64, LU64, v64[10];
+  DBuf<float> A32, LU32, v32[10];
+  DBuf<double> partials;
+  DBuf<unsigned> red_counter;
+  DBuf<SolverScalars> scal;
+  SolverScalars *h_scal = nullptr;  // pinned
+
+  template <typename T>
+  void solve(int &iters, int &status);
+  template <typename T>
+  T *A_();
+  void launch_state_bc();
+  void launch_grad_flux();
+  template <typename T>
+  void launch_jac(bool step_mode);
+};
+
+template <>
+double *Ctx::Impl::A_<double>() {
+  return A64.p;
+}
+template <>
+float *Ctx::Impl::A_<float>() {
+  return A32.p;
+}
+
+Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) : d_(new Impl) {
+  Impl &d = *d_;
+  d.precision = precision;
+  d.stream = (hipStream_t)stream;
+  parse_params(joe_in, m, d.par, d.zbc);
+  d.ncv = m.ncv;
+  d.nfa = m.nfa;
+  d.nfa_b = m.nfa_b;
+  d.nnz = m.nbocv_s();
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw std::runtime_error("no HIP device available");
+
+  // face records: unit normal and area exactly as normVec3d, then x_fa
+  std::vector<double> fg(8 * (size_t)m.nfa);
+  for (int f = 0; f < m.nfa; f++) {
+    const double *n = &m.fa_normal[3 * (size_t)f];
+    double mag = 0.0;
+    mag += n[0] * n[0];
+    mag += n[1] * n[1];
+    mag += n[2] * n[2];
+    mag = std::sqrt(mag);
+    double *g = &fg[8 * (size_t)f];
+    g[0] = n[0] / mag;
+    g[1] = n[1] / mag;
+    g[2] = n[2] / mag;
+    g[3] = mag;
+    g[4] = m.x_fa[3 * (size_t)f];
+    g[5] = m.x_fa[3 * (size_t)f + 1];
+    g[6] = m.x_fa[3 * (size_t)f + 2];
+    g[7] = 0.0;
+  }
+  d.fgeo.upload(fg.data(), fg.size());
+  if (!d.par.grad_ls) d.fnorm.upload(m.fa_normal.data(), m.fa_normal.size());
+
+  // zones: device slots for boundary zones
+  std::vector<ZoneDev> zd;
+  d.zone_slot.assign(m.zones.size(), -1);
+  for (size_t iz = 0; iz < m.zones.size(); iz++) {
+    if (m.zones[iz].kind != kZoneBoundary) continue;
+    ZoneDev z;
+    std::memset(&z, 0, sizeof(z));
+    z.bc = d.zbc[iz].bc;
+    for (int k = 0; k < 8; k++) z.par[k] = d.zbc[iz].par[k];
+    d.zone_slot[iz] = (int)zd.size();
+    zd.push_back(z);
+  }
+  d.nzone = (int)zd.size();
+  if (zd.empty()) zd.push_back(ZoneDev());
+  d.zones.upload(zd.data(), zd.size());
+  std::vector<int> bz(std::max(1, m.nfa_b), 0), bc(std::max(1, m.nfa_b), 0);
+  for (size_t iz = 0; iz < m.zones.size(); iz++) {
+    const Zone &z = m.zones[iz];
+    if (z.kind != kZoneBoundary) continue;
+    for (int f = z.ifa_f; f <= z.ifa_l; f++) bz[f] = d.zone_slot[iz];
+  }
+  for (int f = 0; f < m.nfa_b; f++) bc[f] = m.cvofa[2 * (size_t)f];
+  d.bfa_zone.upload(bz.data(), bz.size());
+  d.bfa_cell.upload(bc.data(), bc.size());
+
+  // per-cell face lists in accumulation order + off-diagonal block slots
+  std::vector<int> ci(m.ncv + 1, 0), noc;
+  std::vector<int2> cf;
+  cf.reserve(m.faocv_v.size());
+  noc.reserve(m.faocv_v.size());
+  std::vector<int> fi, fb;
+  for (int c = 0; c < m.ncv; c++) {
+    fi.clear();
+    fb.clear();
+    for (int k = m.faocv_i[c]; k < m.faocv_i[c + 1]; k++) {
+      int f = m.faocv_v[k];
+      (f >= m.nfa_b ? fi : fb).push_back(f);
+    }
+    std::sort(fi.begin(), fi.end());
+    std::sort(fb.begin(), fb.end());
+    ci[c] = (int)cf.size();
+    std::vector<int> used;
+    for (int f : fi) {
+      int c0 = m.cvofa[2 * (size_t)f], c1 = m.cvofa[2 * (size_t)f + 1];
+      int side = (c0 == c) ? 0 : 1;
+      int nb = side == 0 ? c1 : c0;
+      // getImplDependencyIndex (UgpWithCv.h:745-755): first match in the row
+      int slot = m.nbocv_i[c];
+      while (m.nbocv_v[++slot] != nb) {
+      }
+      bool acc = std::find(used.begin(), used.end(), slot) != used.end();
+      if (acc) d.dup_blocks = true;
+      used.push_back(slot);
+      cf.push_back(make_int2(f | (side << 31), nb));
+      noc.push_back(slot | (acc ? (1 << 30) : 0));
+    }
+    for (int f : fb) {
+      int iz = -1;
+      for (size_t z = 0; z < m.zones.size(); z++)
+        if (m.zones[z].kind == kZoneBoundary && f >= m.zones[z].ifa_f && f <= m.zones[z].ifa_l) iz = (int)z;
+      if (iz < 0) throw std::runtime_error("boundary face without zone");
+      cf.push_back(make_int2(f, -(d.zone_slot[iz] + 1)));
+      noc.push_back(m.nbocv_i[c]);
+    }
+  }
+  ci[m.ncv] = (int)cf.size();
+  d.cfl_i.upload(ci.data(), ci.size());
+  d.cfl_e.upload(cf.data(), cf.size());
+  d.cfl_noc.upload(noc.data(), noc.size());
+  d.h_nbocv_i = m.nbocv_i;
+  d.nbocv_i.upload(m.nbocv_i.data(), m.nbocv_i.size());
+  d.nbocv_v.upload(m.nbocv_v.data(), m.nbocv_v.size());
+  d.lsg.upload(m.nbocv_lsg.data(), m.nbocv_lsg.size());
+  if (m.nfa_b > 0)
+    d.bfa_lsg.upload(m.bfa_lsg.data(), m.bfa_lsg.size());
+  else
+    d.bfa_lsg.alloc(3);
+  d.vol.upload(m.cv_volume.data(), m.cv_volume.size());
+
+  // state buffers; prim carries x_cv in its last three slots
+  const size_t n = m.ncv;
+  d.q.alloc(5 * n);
+  std::vector<double> pr(8 * n, 0.0);
+  for (size_t c = 0; c < n; c++)
+    for (int i = 0; i < 3; i++) pr[8 * c + 5 + i] = m.x_cv[3 * c + i];
+  d.prim.upload(pr.data(), pr.size());
+  d.grad.alloc(16 * n);
+  HIPCHECK(hipMemset(d.grad.p, 0, sizeof(double) * 16 * n));
+  d.bfa.alloc(8 * (size_t)std::max(1, m.nfa_b));
+  d.rhs.alloc(5 * n);
+  d.dt.alloc(n);
+  d.hook.alloc(5 * (size_t)std::max(1, m.nfa_b));
+  HIPCHECK(hipMemset(d.hook.p, 0, d.hook.n * sizeof(double)));
+  d.counters.alloc(4);
+  HIPCHECK(hipMemset(d.counters.p, 0, 4 * sizeof(int)));
+  d.partials.alloc(8 * (size_t)(nblk(5 * n) + 64));
+  d.red_counter.alloc(1);
+  HIPCHECK(hipMemset(d.red_counter.p, 0, sizeof(unsigned)));
+  d.scal.alloc(1);
+  HIPCHECK(hipMemset(d.scal.p, 0, sizeof(SolverScalars)));
+  HIPCHECK(hipHostMalloc((void **)&d.h_scal, sizeof(SolverScalars)));
+
+  d.dm.ncv = m.ncv;
+  d.dm.nfa = m.nfa;
+  d.dm.nfa_b = m.nfa_b;
+  d.dm.nnz = d.nnz;
+  d.dm.fgeo = d.fgeo.p;
+  d.dm.fnorm = d.fnorm.p;
+  d.dm.cfl_i = d.cfl_i.p;
+  d.dm.cfl = d.cfl_e.p;
+  d.dm.cfl_noc = d.cfl_noc.p;
+  d.dm.nbocv_i = d.nbocv_i.p;
+  d.dm.nbocv_v = d.nbocv_v.p;
+  d.dm.lsg = d.lsg.p;
+  d.dm.bfa_lsg = d.bfa_lsg.p;
+  d.dm.bfa_cell = d.bfa_cell.p;
+  d.dm.bfa_zone = d.bfa_zone.p;
+  d.dm.vol = d.vol.p;
+  d.dm.zones = d.zones.p;
+
+  d.dp.gamma = d.par.gamma;
+  d.dp.R = d.par.R_gas;
+  d.dp.eps = d.par.eps_limiter;
+  for (int i = 0; i < 3; i++) d.dp.g[i] = d.par.gravity[i];
+  d.dp.scheme = d.par.scheme;
+  d.dp.second_order = d.par.second_order;
+  d.dp.grad_ls = d.par.grad_ls;
+  d.dp.limiter = d.par.limiter;
+  d.dp.symm_bc = d.par.symm_bc;
+  d.dp.gravity_on = (d.par.gravity[0] != 0.0 || d.par.gravity[1] != 0.0 || d.par.gravity[2] != 0.0);
+  d.cfl = d.par.cfl;
+  HIPCHECK(hipDeviceSynchronize());
+}
+
+Ctx::~Ctx() {
+  if (d_ && d_->h_scal) (void)hipHostFree(d_->h_scal);
+}
+
+int Ctx::ncv() const { return d_->ncv; }
+
+void Ctx::set_state(const double *rho, const double *rhou, const double *rhoE) {
+  Impl &d = *d_;
+  const size_t n = d.ncv;
+  std::vector<double> h(5 * n);
+  for (size_t c = 0; c < n; c++) {
+    h[c] = rho[c];
+    h[n + c] = rhou[3 * c];
+    h[2 * n + c] = rhou[3 * c + 1];
+    h[3 * n + c] = rhou[3 * c + 2];
+    h[4 * n + c] = rhoE[c];
+  }
+  HIPCHECK(hipMemcpyAsync(d.q.p, h.data(), sizeof(double) * 5 * n, hipMemcpyHostToDevice, d.stream));
+  HIPCHECK(hipStreamSynchronize(d.stream));
+  d.have_state = true;
+  d.step_no = 0;
+  d.cfl = d.par.cfl;
+}
+
+void Ctx::get_state(double *rho, double *rhou, double *rhoE) {
+  Impl &d = *d_;
+  const size_t n = d.ncv;
+  std::vector<double> h(5 * n);
+  HIPCHECK(hipMemcpyAsync(h.data(), d.q.p, sizeof(double) * 5 * n, hipMemcpyDeviceToHost, d.stream));
+  HIPCHECK(hipStreamSynchronize(d.stream));
+  for (size_t c = 0; c < n; c++) {
+    rho[c] = h[c];
+    rhou[3 * c] = h[n + c];
+    rhou[3 * c + 1] = h[2 * n + c];
+    rhou[3 * c + 2] = h[3 * n + c];
+    rhoE[c] = h[4 * n + c];
+  }
+}
+
+// initialHook, uniform branch (JoeWithModels.cpp:156-186)
+void Ctx::init_uniform() {
+  Impl &d = *d_;
+  const size_t n = d.ncv;
+  std::vector<double> rho(n), rhou(3 * n), rhoE(n);
+  const double *u = d.par.u_init;
+  double uu = 0.0;
+  uu += u[0] * u[0];
+  uu += u[1] * u[1];
+  uu += u[2] * u[2];
+  for (size_t c = 0; c < n; c++) {
+    rho[c] = d.par.rho_init;
+    for (int i = 0; i < 3; i++) rhou[3 * c + i] = d.par.rho_init * u[i];
+    rhoE[c] = d.par.p_init / (d.par.gamma - 1.0) + 0.5 * d.par.rho_init * uu;
+  }
+  set_state(rho.data(), rhou.data(), rhoE.data());
+}
+
+void Ctx::Impl::launch_state_bc() {
+  hipLaunchKernelGGL(k_state, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, q.p, prim.p, par.gamma, counters.p);
+  if (nfa_b > 0)
+    hipLaunchKernelGGL(k_bc, dim3(nblk(nfa_b)), dim3(kBlock), 0, stream, dm, dp, prim.p, hook.p, bfa.p);
+}
+
+void Ctx::Impl::launch_grad_flux() {
+  if (par.second_order) {
+    if (par.grad_ls)
+      hipLaunchKernelGGL(k_grad<true>, dim3(nblk(ncv)), dim3(kBlock), 0, stream, dm, dp, prim.p, bfa.p, grad.p);
+    else
+      hipLaunchKernelGGL(k_grad<false>, dim3(nblk(ncv)), dim3(kBlock), 0, stream, dm, dp, prim.p, bfa.p, grad.p);
+  }
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(nblk(ncv)), dim3(kBlock), 0, stream, dm, dp, prim.p, grad.p, bfa.p, q.p, rhs.p,
+                       counters.p + 1);
+  };
+  if (par.second_order) {
+    if (par.scheme == kRoe) go(k_flux<kRoe, true>);
+    else if (par.scheme == kAusm) go(k_flux<kAusm, true>);
+    else go(k_flux<kHllc, true>);
+  } else {
+    if (par.scheme == kRoe) go(k_flux<kRoe, false>);
+    else if (par.scheme == kAusm) go(k_flux<kAusm, false>);
+    else go(k_flux<kHllc, false>);
+  }
+}
+
+template <typename T>
+void Ctx::Impl::launch_jac(bool step_mode) {
+  DBuf<T> *v = nullptr;
+  if constexpr (std::is_same<T, double>::value) v = v64; else v = v32;
+  T *A = A_<T>();
+  if (dup_blocks) HIPCHECK(hipMemsetAsync(A, 0, sizeof(T) * 25 * (size_t)nnz, stream));
+  if (par.scheme == kRoe)
+    hipLaunchKernelGGL((k_jac<T, kRoe>), dim3(nblk(ncv)), dim3(kBlock), 0, stream, dm, dp, prim.p, bfa.p, A,
+                       step_mode ? 1 : 0, dt.p, rhs.p, par.under_relax, v[0].p, v[1].p);
+  else
+    hipLaunchKernelGGL((k_jac<T, kHllc>), dim3(nblk(ncv)), dim3(kBlock), 0, stream, dm, dp, prim.p, bfa.p, A,
+                       step_mode ? 1 : 0, dt.p, rhs.p, par.under_relax, v[0].p, v[1].p);
+}
+
+// Work vectors: v[0] rhs, v[1] phi (= dq), v[2] res, v[3] res0, v[4] p,
+// v[5] v, v[6] phat, v[7] s, v[8] shat, v[9] t.
+template <typename T>
+void Ctx::Impl::solve(int &iters, int &status) {
+  DBuf<T> *v;
+  DBuf<T> *LU;
+  if constexpr (std::is_same<T, double>::value) {
+    v = v64;
+    LU = &LU64;
+  } else {
+    v = v32;
+    LU = &LU32;
+  }
+  T *A = A_<T>();
+  const int n5 = 5 * ncv;
+  T *rhs_ = v[0].p, *phi = v[1].p, *res = v[2].p, *res0 = v[3].p, *p = v[4].p, *vv = v[5].p, *phat = v[6].p,
+    *s = v[7].p, *shat = v[8].p, *t = v[9].p;
+  HIPCHECK(hipMemsetAsync(p, 0, sizeof(T) * n5, stream));
+  HIPCHECK(hipMemsetAsync(vv, 0, sizeof(T) * n5, stream));
+  SolverScalars &hs = *h_scal;
+  std::memset(&hs, 0, sizeof(hs));
+  hs.maxiter = par.ls_maxiter;
+  hs.zero_abs = par.ls_abs;
+  hs.zero_rel = par.ls_rel;
+  HIPCHECK(hipMemcpyAsync(scal.p, &hs, sizeof(SolverScalars), hipMemcpyHostToDevice, stream));
+  hipLaunchKernelGGL(k_lu<T>, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, nbocv_i.p, A, LU->p);
+  const int gb5 = (ncv + kCellsPerBlock - 1) / kCellsPerBlock;
+  hipLaunchKernelGGL(k_resid<T>, dim3(gb5), dim3(kBlock), 0, stream, ncv, nbocv_i.p, nbocv_v.p, A, LU->p, phi, rhs_,
+                     res, res0, 1, partials.p, red_counter.p, scal.p);
+  const int max_it = par.ls_maxiter + 1;
+  int launched = 0;
+  while (launched < max_it) {
+    int chunk = std::min(launched == 0 ? 2 : 4, max_it - launched);
+    for (int k = 0; k < chunk; k++) {
+      hipLaunchKernelGGL(k_p<T>, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, LU->p, res, vv, p, phat, scal.p);
+      hipLaunchKernelGGL(k_spmv_dot<T>, dim3(gb5), dim3(kBlock), 0, stream, ncv, nbocv_i.p, nbocv_v.p, A, phat, vv,
+                         res0, 0, partials.p, red_counter.p, scal.p);
+      hipLaunchKernelGGL(k_s<T>, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, LU->p, res, vv, s, shat, scal.p);
+      hipLaunchKernelGGL(k_spmv_dot<T>, dim3(gb5), dim3(kBlock), 0, stream, ncv, nbocv_i.p, nbocv_v.p, A, shat, t, s,
+                         1, partials.p, red_counter.p, scal.p);
+      hipLaunchKernelGGL(k_x<T>, dim3(nblk(n5)), dim3(kBlock), 0, stream, n5, phi, phat, shat, scal.p);
+      hipLaunchKernelGGL(k_resid<T>, dim3(gb5), dim3(kBlock), 0, stream, ncv, nbocv_i.p, nbocv_v.p, A, LU->p, phi,
+                         rhs_, res, res0, 0, partials.p, red_counter.p, scal.p);
+    }
+    launched += chunk;
+    HIPCHECK(hipMemcpyAsync(h_scal, scal.p, sizeof(SolverScalars), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    if (hs.done) break;
+  }
+  iters = hs.iter;
+  status = hs.done;
+}
+
+template <typename T>
+static void ensure(DBuf<T> &b, size_t n) {
+  if (b.n != n) b.alloc(n);
+}
+
+void Ctx::launch_residual() {
+  Impl &d = *d_;
+  d.launch_state_bc();
+  d.launch_grad_flux();
+}
+
+void Ctx::calc_rhs(bool implicit) {
+  Impl &d = *d_;
+  if (!d.have_state) throw std::logic_error("no state set");
+  d.launch_state_bc();
+  hipLaunchKernelGGL(k_dt, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.dm, d.prim.p, d.q.p, d.par.gamma, d.cfl,
+                     d.par.dt_mode, d.par.dt_fixed, d.dt.p, d.partials.p, d.red_counter.p, d.scal.p);
+  if (d.par.dt_mode == 1) hipLaunchKernelGGL(k_dt_fill, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.dt.p, d.scal.p);
+  d.launch_grad_flux();
+  if (implicit) {
+    if (d.precision == 0) {
+      ensure(d.A64, 25 * (size_t)d.nnz);
+      for (int k = 0; k < 2; k++) ensure(d.v64[k], 5 * (size_t)d.ncv);
+      d.launch_jac<double>(false);
+    } else {
+      ensure(d.A32, 25 * (size_t)d.nnz);
+      for (int k = 0; k < 2; k++) ensure(d.v32[k], 5 * (size_t)d.ncv);
+      d.launch_jac<float>(false);
+    }
+  }
+  HIPCHECK(hipGetLastError());
+}
+
+StepInfo Ctx::step() {
+  Impl &d = *d_;
+  if (!d.have_state) throw std::logic_error("no state set");
+  StepInfo info;
+  if (d.step_no == 0) d.launch_state_bc();
+  d.step_no++;
+  // CFL ramp (JoeWithModels.cpp:978)
+  if ((d.step_no >= d.par.cfl_after) && (d.step_no % d.par.cfl_interval == 0) && (d.cfl < d.par.cfl_max))
+    d.cfl *= d.par.cfl_factor;
+  hipLaunchKernelGGL(k_dt, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.dm, d.prim.p, d.q.p, d.par.gamma, d.cfl,
+                     d.par.dt_mode, d.par.dt_fixed, d.dt.p, d.partials.p, d.red_counter.p, d.scal.p);
+  if (d.par.dt_mode == 1) hipLaunchKernelGGL(k_dt_fill, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.dt.p, d.scal.p);
+  d.launch_grad_flux();
+  int it = 0, st = 0;
+  if (d.precision == 0) {
+    ensure(d.A64, 25 * (size_t)d.nnz);
+    ensure(d.LU64, 25 * (size_t)d.ncv);
+    for (int k = 0; k < 10; k++) ensure(d.v64[k], 5 * (size_t)d.ncv);
+    d.launch_jac<double>(true);
+    d.solve<double>(it, st);
+    hipLaunchKernelGGL(k_update<double>, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.q.p, d.v64[1].p);
+  } else {
+    ensure(d.A32, 25 * (size_t)d.nnz);
+    ensure(d.LU32, 25 * (size_t)d.ncv);
+    for (int k = 0; k < 10; k++) ensure(d.v32[k], 5 * (size_t)d.ncv);
+    d.launch_jac<float>(true);
+    d.solve<float>(it, st);
+    hipLaunchKernelGGL(k_update<float>, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.q.p, d.v32[1].p);
+  }
+  d.launch_state_bc();
+  hipLaunchKernelGGL(k_l1, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.rhs.p, d.partials.p,
+                     d.red_counter.p, d.scal.p);
+  HIPCHECK(hipMemcpyAsync(d.h_scal, d.scal.p, sizeof(SolverScalars), hipMemcpyDeviceToHost, d.stream));
+  HIPCHECK(hipStreamSynchronize(d.stream));
+  info.step = d.step_no;
+  info.lin_iters = it;
+  info.lin_status = st;
+  info.cfl = d.cfl;
+  info.dt_min = d.h_scal->dt_min;
+  for (int k = 0; k < 5; k++) info.resid[k] = d.h_scal->resid[k];
+  return info;
+}
+
+// doneSolver (JoeWithModels.h:42-95): NSTEPS, RESID_ENERG_TH, NaN
+int Ctx::run(int max_steps, double *hist) {
+  Impl &d = *d_;
+  auto done = [&](double r) {
+    if ((d.par.nsteps >= 0) && (d.step_no >= d.par.nsteps)) return true;
+    if (r < d.par.resid_energ_th) return true;
+    if (r != r) return true;
+    return false;
+  };
+  if (done(1.0e20)) return 0;
+  int n = 0;
+  while (n < max_steps) {
+    StepInfo s = step();
+    if (hist)
+      for (int k = 0; k < 5; k++) hist[5 * n + k] = s.resid[k];
+    n++;
+    if (done(s.resid[4])) break;
+  }
+  return n;
+}
+
+void Ctx::set_boundary_state(int ifa_f, int nface, const double *T, const double *vel, const double *p) {
+  Impl &d = *d_;
+  if (ifa_f < 0 || nface < 0 || ifa_f + nface > d.nfa_b) throw std::out_of_range("boundary face range");
+  std::vector<double> h(5 * (size_t)nface);
+  for (int k = 0; k < nface; k++) {
+    h[5 * k] = T[k];
+    h[5 * k + 1] = vel[3 * k];
+    h[5 * k + 2] = vel[3 * k + 1];
+    h[5 * k + 3] = vel[3 * k + 2];
+    h[5 * k + 4] = p[k];
+  }
+  HIPCHECK(hipMemcpyAsync(d.hook.p + 5 * (size_t)ifa_f, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
+                          d.stream));
+  HIPCHECK(hipStreamSynchronize(d.stream));
+}
+
+double *Ctx::device_state() { return d_->q.p; }
+double *Ctx::device_rhs() { return d_->rhs.p; }
+void Ctx::sync() { HIPCHECK(hipStreamSynchronize(d_->stream)); }
+
+int64_t Ctx::get(const std::string &name, double *out) {
+  Impl &d = *d_;
+  HIPCHECK(hipStreamSynchronize(d.stream));
+  const size_t n = d.ncv, nb = d.nfa_b;
+  auto copy = [&](const void *src, size_t bytes, std::vector<double> &h) {
+    h.resize(bytes / sizeof(double));
+    HIPCHECK(hipMemcpy(h.data(), src, bytes, hipMemcpyDeviceToHost));
+  };
+  std::vector<double> h;
+  auto soa = [&](const double *src, int comp, int ncomp) -> int64_t {  // [ncomp][n] -> component
+    if (out) {
+      copy(src + (size_t)comp * n, n * sizeof(double), h);
+      std::memcpy(out, h.data(), n * sizeof(double));
+    }
+    (void)ncomp;
+    return (int64_t)n;
+  };
+  auto rec = [&](const double *src, size_t nrec, int stride, int first, int count) -> int64_t {
+    if (out) {
+      copy(src, nrec * stride * sizeof(double), h);
+      for (size_t r = 0; r < nrec; r++)
+        for (int k = 0; k < count; k++) out[r * count + k] = h[r * stride + first + k];
+    }
+    return (int64_t)(nrec * count);
+  };
+  if (name == "RHS_RHO") return soa(d.rhs.p, 0, 5);
+  if (name == "RHS_RHOE") return soa(d.rhs.p, 4, 5);
+  if (name == "RHS_RHOU") {
+    if (out) {
+      copy(d.rhs.p, 5 * n * sizeof(double), h);
+      for (size_t c = 0; c < n; c++)
+        for (int i = 0; i < 3; i++) out[3 * c + i] = h[(1 + i) * n + c];
+    }
+    return (int64_t)(3 * n);
+  }
+  if (name == "LOCAL_DT") return rec(d.dt.p, n, 1, 0, 1);
+  if (name == "PRESS") return rec(d.prim.p, n, 8, 4, 1);
+  if (name == "VEL") return rec(d.prim.p, n, 8, 1, 3);
+  if (name == "GRAD_RHO") return rec(d.grad.p, n, 16, 0, 3);
+  if (name == "GRAD_P") return rec(d.grad.p, n, 16, 3, 3);
+  if (name == "GRAD_U") return rec(d.grad.p, n, 16, 6, 9);
+  if (name == "ALPHA_RHO") return rec(d.grad.p, n, 16, 15, 1);
+  if (name == "RHO_BFA") return rec(d.bfa.p, nb, 8, 0, 1);
+  if (name == "VEL_BFA") return rec(d.bfa.p, nb, 8, 1, 3);
+  if (name == "P_BFA") return rec(d.bfa.p, nb, 8, 4, 1);
+  if (name == "T_BFA") return rec(d.bfa.p, nb, 8, 5, 1);
+  if (name == "H_BFA") return rec(d.bfa.p, nb, 8, 6, 1);
+  if (name == "RESID") {
+    if (out) {
+      HIPCHECK(hipMemcpy(d.h_scal, d.scal.p, sizeof(SolverScalars), hipMemcpyDeviceToHost));
+      for (int k = 0; k < 5; k++) out[k] = d.h_scal->resid[k];
+    }
+    return 5;
+  }
+  if (name == "COUNTERS") {  // negative pressures, first-order fallbacks
+    if (out) {
+      int c[4];
+      HIPCHECK(hipMemcpy(c, d.counters.p, sizeof(c), hipMemcpyDeviceToHost));
+      for (int k = 0; k < 4; k++) out[k] = c[k];
+    }
+    return 4;
+  }
+  if (name == "A" || name == "DQ") {
+    bool isA = name == "A";
+    size_t cnt = isA ? 25 * (size_t)d.nnz : 5 * n;
+    if (d.precision == 0) {
+      const double *src = isA ? d.A64.p : d.v64[1].p;
+      if (!src) throw std::logic_error(name + " not computed yet");
+      if (out) HIPCHECK(hipMemcpy(out, src, cnt * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+      const float *src = isA ? d.A32.p : d.v32[1].p;
+      if (!src) throw std::logic_error(name + " not computed yet");
+      if (out) {
+        std::vector<float> f(cnt);
+        HIPCHECK(hipMemcpy(f.data(), src, cnt * sizeof(float), hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < cnt; k++) out[k] = f[k];
+      }
+    }
+    return (int64_t)cnt;
+  }
+  throw std::invalid_argument("unknown array " + name);
+}
+
+}  // namespace jm

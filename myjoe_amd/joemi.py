@@ -1,0 +1,212 @@
+"""ctypes binding of libjoemi (include/joemi.h).
+
+The library must have been built in-tree (``python -m myjoe_amd.build``);
+loading fails loudly otherwise -- there is no CPU fallback on this path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libjoemi.so")
+
+JM_FP64, JM_MIXED = 0, 1
+JM_RHS_IMPLICIT = 1
+_STATUS = {0: "JM_OK", 1: "JM_ERR_ARG", 2: "JM_ERR_MESH", 3: "JM_ERR_PARAM", 4: "JM_ERR_HIP",
+           5: "JM_ERR_UNSUPPORTED", 6: "JM_ERR_STATE"}
+
+_lib = None
+
+
+class JoemiError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{_STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class StepInfo(C.Structure):
+    _fields_ = [("step", C.c_int), ("lin_iters", C.c_int), ("lin_status", C.c_int), ("cfl", C.c_double),
+                ("dt_min", C.c_double), ("resid", C.c_double * 5)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run `python -m myjoe_amd.build` (HIP path required)")
+        L = C.CDLL(LIB_PATH)
+        vp, ip, dp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)
+        L.jm_last_error.restype = C.c_char_p
+        L.jm_version.restype = C.c_char_p
+        L.jm_mesh_from_faces.argtypes = [C.c_int, vp, C.c_int, vp, vp, vp, vp, C.c_int, C.c_int, vp, vp, vp]
+        L.jm_mesh_read_msh.argtypes = [C.c_char_p, vp]
+        L.jm_mesh_from_ugp.argtypes = [vp, vp]
+        L.jm_mesh_destroy.argtypes = [vp]
+        L.jm_mesh_destroy.restype = None
+        L.jm_mesh_counts.argtypes = [vp, vp]
+        L.jm_mesh_array.argtypes = [vp, C.c_char_p, vp, vp]
+        L.jm_mesh_zone.argtypes = [vp, C.c_int, C.c_char_p, C.c_int, ip, ip, ip]
+        L.jm_ctx_create.argtypes = [vp, C.c_char_p, C.c_int, vp, vp]
+        L.jm_ctx_destroy.argtypes = [vp]
+        L.jm_ctx_destroy.restype = None
+        L.jm_set_state.argtypes = [vp, vp, vp, vp]
+        L.jm_get_state.argtypes = [vp, vp, vp, vp]
+        L.jm_init_uniform.argtypes = [vp]
+        L.jm_calc_rhs.argtypes = [vp, C.c_int]
+        L.jm_get.argtypes = [vp, C.c_char_p, vp, vp]
+        L.jm_step.argtypes = [vp, vp]
+        L.jm_run.argtypes = [vp, C.c_int, vp, vp]
+        L.jm_set_boundary_state.argtypes = [vp, C.c_int, vp, vp, vp]
+        L.jm_device_state.argtypes = [vp, vp]
+        L.jm_device_rhs.argtypes = [vp, vp]
+        L.jm_sync.argtypes = [vp]
+        _lib = L
+    return _lib
+
+
+def _check(st):
+    if st != 0:
+        raise JoemiError(st, lib().jm_last_error().decode())
+
+
+def _ptr(a):
+    return a.ctypes.data
+
+
+class Mesh:
+    """A mesh in the reference's numbering, built by libjoemi."""
+
+    def __init__(self, handle):
+        self.h = handle
+        counts = np.zeros(8, np.int64)
+        _check(lib().jm_mesh_counts(self.h, _ptr(counts)))
+        self.nno, self.nfa, self.nfa_b, self.ncv, self.nnz, self.nzone, _, _ = (int(x) for x in counts)
+
+    @classmethod
+    def from_faces(cls, mm):
+        """From a myjoe_amd.meshgen.MshMesh (face-file order, Joe orientation)."""
+        h = C.c_void_p()
+        x = np.ascontiguousarray(mm.x_no, np.float64)
+        fp = np.ascontiguousarray(mm.face_ptr, np.int64)
+        fn = np.ascontiguousarray(mm.face_nodes, np.int32)
+        fc = np.ascontiguousarray(mm.face_cells, np.int32)
+        fz = np.ascontiguousarray(mm.face_zone, np.int32)
+        ids = np.array([z[0] for z in mm.zones], np.int32)
+        names = (C.c_char_p * len(mm.zones))(*[z[2].encode() for z in mm.zones])
+        _check(lib().jm_mesh_from_faces(mm.nno, _ptr(x), mm.nfa, _ptr(fp), _ptr(fn), _ptr(fc), _ptr(fz), mm.ncv,
+                                        len(mm.zones), _ptr(ids), C.cast(names, C.c_void_p), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def read_msh(cls, path):
+        h = C.c_void_p()
+        _check(lib().jm_mesh_read_msh(path.encode(), C.byref(h)))
+        return cls(h)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.jm_mesh_destroy(self.h)
+            self.h = None
+
+    def array(self, name):
+        n = C.c_int64()
+        _check(lib().jm_mesh_array(self.h, name.encode(), None, C.byref(n)))
+        dt = np.float64 if name in ("X_NO", "X_FA", "FA_NORMAL", "X_CV", "CV_VOLUME", "NBOCV_LSG", "BFA_LSG") \
+            else np.int32
+        out = np.empty(n.value, dt)
+        _check(lib().jm_mesh_array(self.h, name.encode(), _ptr(out), C.byref(n)))
+        return out
+
+    def zones(self):
+        out = []
+        buf = C.create_string_buffer(256)
+        k, f, l = C.c_int(), C.c_int(), C.c_int()
+        for i in range(self.nzone):
+            _check(lib().jm_mesh_zone(self.h, i, buf, 256, C.byref(k), C.byref(f), C.byref(l)))
+            out.append((buf.value.decode(), k.value, f.value, l.value))
+        return out
+
+    def as_dump(self) -> dict:
+        """Arrays under the reference dump names (for the oracle)."""
+        d = {n: self.array(n) for n in ("X_NO", "NOOFA_I", "NOOFA_V", "CVOFA", "FAOCV_I", "FAOCV_V", "NBOCV_I",
+                                         "NBOCV_V", "X_FA", "FA_NORMAL", "X_CV", "CV_VOLUME", "NBOCV_LSG",
+                                         "BFA_LSG")}
+        d["COUNTS"] = np.array([self.nno, self.nfa, self.nfa_b, self.ncv, self.ncv, self.nnz,
+                                d["FAOCV_V"].size, d["NOOFA_V"].size], np.int32)
+        zl = self.zones()
+        d["ZONES"] = np.array([[z[1], z[2], z[3]] for z in zl], np.int32).reshape(-1)
+        d["ZONE_NAMES"] = np.frombuffer("".join(z[0] + "\n" for z in zl).encode(), np.uint8)
+        return d
+
+
+class Solver:
+    """One steady solve resident on the GPU (a JoeWithModels instance)."""
+
+    def __init__(self, mesh: Mesh, joe_in: str, precision: int = JM_FP64, stream=None):
+        self.mesh = mesh
+        self.h = C.c_void_p()
+        _check(lib().jm_ctx_create(mesh.h, joe_in.encode(), precision, stream, C.byref(self.h)))
+        self.ncv = mesh.ncv
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.jm_ctx_destroy(self.h)
+            self.h = None
+
+    def set_state(self, rho, rhou, rhoE):
+        rho = np.ascontiguousarray(rho, np.float64)
+        rhou = np.ascontiguousarray(rhou, np.float64)
+        rhoE = np.ascontiguousarray(rhoE, np.float64)
+        assert rho.size == self.ncv and rhou.size == 3 * self.ncv and rhoE.size == self.ncv
+        _check(lib().jm_set_state(self.h, _ptr(rho), _ptr(rhou), _ptr(rhoE)))
+
+    def state(self):
+        rho, rhou, rhoE = np.empty(self.ncv), np.empty((self.ncv, 3)), np.empty(self.ncv)
+        _check(lib().jm_get_state(self.h, _ptr(rho), _ptr(rhou), _ptr(rhoE)))
+        return rho, rhou, rhoE
+
+    def init_uniform(self):
+        _check(lib().jm_init_uniform(self.h))
+
+    def calc_rhs(self, implicit=False):
+        _check(lib().jm_calc_rhs(self.h, JM_RHS_IMPLICIT if implicit else 0))
+
+    def get(self, name):
+        n = C.c_int64()
+        _check(lib().jm_get(self.h, name.encode(), None, C.byref(n)))
+        out = np.empty(n.value)
+        _check(lib().jm_get(self.h, name.encode(), _ptr(out), C.byref(n)))
+        return out
+
+    def step(self) -> StepInfo:
+        info = StepInfo()
+        _check(lib().jm_step(self.h, C.byref(info)))
+        return info
+
+    def run(self, max_steps):
+        hist = np.zeros((max_steps, 5))
+        n = C.c_int()
+        _check(lib().jm_run(self.h, max_steps, _ptr(hist), C.byref(n)))
+        return n.value, hist[:n.value]
+
+    def set_boundary_state(self, zone, T, vel, p):
+        T = np.ascontiguousarray(T, np.float64)
+        vel = np.ascontiguousarray(vel, np.float64)
+        p = np.ascontiguousarray(p, np.float64)
+        _check(lib().jm_set_boundary_state(self.h, zone, _ptr(T), _ptr(vel), _ptr(p)))
+
+    def device_state_ptr(self) -> int:
+        q = C.c_void_p()
+        _check(lib().jm_device_state(self.h, C.byref(q)))
+        return q.value
+
+    def device_rhs_ptr(self) -> int:
+        q = C.c_void_p()
+        _check(lib().jm_device_rhs(self.h, C.byref(q)))
+        return q.value
+
+    def sync(self):
+        _check(lib().jm_sync(self.h))

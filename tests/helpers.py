@@ -1,0 +1,33 @@
+"""Shared helpers of the test suite: case construction and comparisons."""
+from __future__ import annotations
+
+import numpy as np
+
+from myjoe_amd import cases
+
+# small meshes that the CPU oracle evaluates in well under a second
+CASES = {
+    "bump24_hllc": lambda: cases.bump_case(24, 8, scheme="HLLC"),
+    "bump24_roe": lambda: cases.bump_case(24, 8, scheme="ROE"),
+    "bump24_ausm": lambda: cases.bump_case(24, 8, scheme="AUSM"),
+    "box5_hllc": lambda: cases.box_case(5, seed=3, scheme="HLLC"),
+    "box5_roe": lambda: cases.box_case(5, seed=3, scheme="ROE"),
+    "box5_ausm": lambda: cases.box_case(5, seed=3, scheme="AUSM"),
+    "box5_gg_hllc": lambda: cases.box_case(5, seed=3, scheme="HLLC", grad="GREENGAUSS"),
+    "box5_first_roe": lambda: cases.box_case(5, seed=3, scheme="ROE", second_order=False),
+}
+
+
+def compare(a, b, rtol=1e-12, name=""):
+    """Bit-exact count plus a relative bound: |a-b| <= rtol * max(|b|, scale),
+    scale = max |b| over the array times 1e-3 (guards values near zero)."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    assert a.shape == b.shape, f"{name}: shape {a.shape} vs {b.shape}"
+    exact = int(np.sum(a == b))
+    scale = max(float(np.max(np.abs(b))) if b.size else 0.0, 1e-300)
+    tol = rtol * np.maximum(np.abs(b), 1e-3 * scale)
+    bad = np.abs(a - b) > tol
+    assert not np.any(bad), (f"{name}: {int(bad.sum())} of {a.size} outside rtol={rtol}; "
+                             f"worst |d|={float(np.max(np.abs(a - b))):.3e}")
+    return exact, a.size
