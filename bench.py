@@ -1,0 +1,175 @@
+#!/usr/bin/env python3
+"""Single-residual throughput on one MI355X.
+
+Workload: [0,1]^3 box of 160^3 hexes, a seeded 50 % split into six Kuhn
+tetrahedra each, interior nodes jittered by U(-0.2h, 0.2h); random FP64 state
+(rho ~ U(0.5,1.5), p ~ lognormal(0,0.2), u_i ~ N(0,0.3) c). One "step" is one
+residual evaluation of the reference's pseudo-time loop -- calcStateVariables,
+setBC, calcDt and calcRhs (2nd order, least-squares gradients, smooth
+Barth-Jespersen limiter) -- over the whole mesh, with the state resident in
+HBM. value = ncv * steps / time in Mcell-it/s for HLLC; Roe and AUSM are
+reported beside it.
+
+    python bench.py [--steps K] [--warmup W] [--n 160] [--gpus 1]
+"""
+from __future__ import annotations
+
+import argparse
+import gc
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "steady solves/min + p50/p99 latency vs plain FP64 GPU loop; residual Mcell-it/s"
+
+
+def algorithmic_bytes(ncv, nfa, nfa_b, nnz):
+    """Compulsory HBM bytes of each residual kernel (every array it needs read
+    or written once): prim rows 64 B (rho u v w p x_cv), limited-gradient
+    rows 120 B (+8 alpha), face rows 56 B (unit normal, area, x_fa), boundary
+    rows 40 B (rho u v w p), face-list entries 8 B, LS coefficients 24 B."""
+    nfa_i = nfa - nfa_b
+    nfoc = 2 * nfa_i + nfa_b
+    return {
+        "k_state": ncv * (40 + 40),
+        "k_bc": nfa_b * (64 + 56 + 8 + 56),
+        "k_grad": ncv * (64 + 4 + 4 + 128) + nnz * 24 + nfoc * 8 + nfa_i * 56 + nfa_b * (40 + 24),
+        "k_flux": ncv * (64 + 120 + 4 + 8 + 40) + nfoc * 8 + nfa * 56 + nfa_b * 40,
+        "k_dt": ncv * (64 + 24 + 4 + 8 + 8) + nfoc * 8 + nfa * 32,
+    }
+
+
+def cpu_baseline(n_box=64, reps=12):
+    """Single-threaded CPU restatement of the reference (bit-identical to it)
+    on a smaller box of the same construction: Mcell-it/s of state + BC + dt +
+    calcRhs."""
+    from myjoe_amd import cases, joemi
+    from oracle.oracle import Oracle
+
+    mm, joe = cases.box_case(n_box, seed=0)
+    mesh = joemi.Mesh.from_faces(mm)
+    del mm
+    o = Oracle(mesh.as_dump(), joe)
+    o.set_state(*cases.random_state(mesh.ncv, seed=1))
+    o.residual(False)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        o.residual(False)
+    dt = time.perf_counter() - t0
+    return {"value": round(mesh.ncv * reps / dt / 1e6, 4), "unit": "Mcell-it/s", "cores": 1, "kind": "port",
+            "sample": f"{n_box}^3 split box ({mesh.ncv} cells), {reps} residual evaluations (state, BC, dt, "
+                      f"calcRhs; HLLC 2nd order LS+limiter) in {dt:.1f} s on 1 host core"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=160)
+    ap.add_argument("--schemes", default="HLLC,ROE,AUSM")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calib-gib", type=float, default=0.0,
+                    help="first run a plain device copy of this many GiB (calibrates profiler byte counters)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 (multi-GPU sharding is a "
+                                   "design in DESIGN.md, not measured)"}))
+        sys.exit(2)
+    if args.copy_calib_gib > 0:
+        import torch
+
+        n = int(args.copy_calib_gib * (1 << 30)) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"# copy calibration: {n * 8} bytes read and written per copy, 3 copies", file=sys.stderr, flush=True)
+        del src, dst
+
+    from myjoe_amd import cases, joemi
+
+    t_setup = time.perf_counter()
+    seed = 0
+    while True:
+        mm, _ = cases.box_case(args.n, seed=seed)
+        mesh = joemi.Mesh.from_faces(mm)
+        del mm
+        sdn, negtet = mesh.quality()
+        if sdn > 0.0:
+            break
+        print(f"# seed {seed}: s.n min {sdn:.3g} <= 0, reseeding", file=sys.stderr, flush=True)
+        seed += 1
+    gc.collect()
+    state = cases.random_state(mesh.ncv, seed=1)
+    print(f"# mesh: ncv={mesh.ncv} nfa={mesh.nfa} nfa_b={mesh.nfa_b} nnz={mesh.nnz} seed={seed} "
+          f"min s.n={sdn:.3f} setup {time.perf_counter() - t_setup:.1f}s", file=sys.stderr, flush=True)
+
+    results = {}
+    kernels = {}
+    for scheme in args.schemes.split(","):
+        joe = cases.box_joe_in(scheme)
+        s = joemi.Solver(mesh, joe, joemi.JM_FP64)
+        s.set_state(*state)
+        for _ in range(args.warmup):
+            s.calc_rhs(False)
+        s.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            s.calc_rhs(False)
+        s.sync()
+        dt = time.perf_counter() - t0
+        results[scheme] = {"ms_per_step": dt / args.steps * 1e3, "mcell_it_s": mesh.ncv * args.steps / dt / 1e6}
+        kernels[scheme] = s.time_kernels(reps=10)
+        print(f"# {scheme}: {results[scheme]['ms_per_step']:.3f} ms/residual, "
+              f"{results[scheme]['mcell_it_s']:.1f} Mcell-it/s; kernels "
+              + " ".join(f"{k}={v:.3f}ms" for k, v in kernels[scheme].items()), file=sys.stderr, flush=True)
+        del s
+        gc.collect()
+
+    head = args.schemes.split(",")[0]
+    kt = kernels[head]
+    ab = algorithmic_bytes(mesh.ncv, mesh.nfa, mesh.nfa_b, mesh.nnz)
+    dom = max(("k_grad", "k_flux", "k_state", "k_dt"), key=lambda k: kt.get(k, 0.0))
+    achieved = ab[dom] / (kt[dom] * 1e-3) / 1e9
+    resid_bytes = ab["k_state"] + ab["k_bc"] + ab["k_grad"] + ab["k_flux"] + ab["k_dt"]
+    out = {
+        "metric": METRIC,
+        "value": round(results[head]["mcell_it_s"], 2),
+        "unit": "Mcell-it/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(results[head]["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"single-residual throughput: {args.n}^3 box, 50% of hexes split into 6 Kuhn tets, "
+                        f"interior nodes jittered 0.2h; {head} 2nd order, LS gradients + BJ-mod limiter",
+            "ncv": mesh.ncv, "nfa": mesh.nfa, "nfa_b": mesh.nfa_b, "nnz": mesh.nnz,
+            "per_scheme_mcell_it_s": {k: round(v["mcell_it_s"], 2) for k, v in results.items()},
+            "kernel_ms": {k: round(v, 4) for k, v in kt.items()},
+            "residual_algorithmic_GBs": round(resid_bytes / (kt["residual"] * 1e-3 + kt["k_dt"] * 1e-3) / 1e9, 1),
+        },
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_launch": int(ab[dom])},
+    }
+    if not args.no_cpu_baseline:
+        del mesh
+        gc.collect()
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -143,6 +143,18 @@ jm_status jm_device_rhs(jm_ctx *c, double **rhs_dev);
 /* wait for all work queued on the context's stream */
 jm_status jm_sync(jm_ctx *c);
 
+/* Measurement hook, no counterpart in the reference: launches each kernel
+ * of one residual evaluation (state, bc, grad, flux, dt) reps times on the
+ * context's stream and returns the mean duration of each in ms, measured
+ * with HIP events on that stream. ms_out has room for 8 values; names gets
+ * a comma-separated list of the kernels timed. */
+jm_status jm_time_residual_kernels(jm_ctx *c, int reps, double *ms_out, char *names, int names_len);
+
+/* Mesh quality of the reference's geometry checks (UgpWithCv.cpp:1000-1091):
+ * minimum over internal faces of s.n/(|s||n|) and the number of sub-tets
+ * with non-positive volume. */
+jm_status jm_mesh_quality(const jm_mesh *m, double *min_sdotn, int *neg_subtets);
+
 #ifdef __cplusplus
 }
 #endif

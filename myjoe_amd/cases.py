@@ -58,9 +58,8 @@ def bump_case(nx: int = 96, ny: int = 32, scheme: str = "HLLC", second_order: bo
     return mesh, _common(scheme, second_order, "LEASTSQUARES", 0.01, extra)
 
 
-def box_case(n: int, frac: float = 0.5, jitter: float = 0.2, seed: int = 0, scheme: str = "HLLC",
-             second_order: bool = True, grad: str = "LEASTSQUARES", restart: str = "box.msh"):
-    mesh = meshgen.split_box(n, frac, jitter, seed)
+def box_joe_in(scheme: str = "HLLC", second_order: bool = True, grad: str = "LEASTSQUARES",
+               restart: str = "box.msh") -> str:
     extra = "\n".join([
         f"RESTART = {restart}",
         "CFL = 1.0",
@@ -72,7 +71,13 @@ def box_case(n: int, frac: float = 0.5, jitter: float = 0.2, seed: int = 0, sche
         "farfield = FARFIELD 0.4 0.2 0.1 1.0 1.0 1.4",
         "symm = SYMMETRY",
     ]) + "\n"
-    return mesh, _common(scheme, second_order, grad, 0.01, extra)
+    return _common(scheme, second_order, grad, 0.01, extra)
+
+
+def box_case(n: int, frac: float = 0.5, jitter: float = 0.2, seed: int = 0, scheme: str = "HLLC",
+             second_order: bool = True, grad: str = "LEASTSQUARES", restart: str = "box.msh"):
+    mesh = meshgen.split_box(n, frac, jitter, seed)
+    return mesh, box_joe_in(scheme, second_order, grad, restart)
 
 
 def random_state(ncv: int, seed: int = 1):

@@ -302,4 +302,25 @@ jm_status jm_sync(jm_ctx *c) {
   });
 }
 
+jm_status jm_time_residual_kernels(jm_ctx *c, int reps, double *ms_out, char *names, int names_len) {
+  return guard([&] {
+    if (!c || !ms_out || reps < 1) throw std::exception();
+    std::string n = c->c->time_kernels(reps, ms_out);
+    if (names && names_len > 0) {
+      std::strncpy(names, n.c_str(), names_len - 1);
+      names[names_len - 1] = 0;
+    }
+    return JM_OK;
+  });
+}
+
+jm_status jm_mesh_quality(const jm_mesh *m, double *min_sdotn, int *neg_subtets) {
+  return guard([&] {
+    if (!m) throw std::exception();
+    if (min_sdotn) *min_sdotn = m->m.min_sdotn;
+    if (neg_subtets) *neg_subtets = m->m.count_neg_subtet;
+    return JM_OK;
+  });
+}
+
 }  // extern "C"

@@ -80,6 +80,8 @@ class Ctx {
 
   // timing hooks for the bench: launch the residual kernels only
   void launch_residual();
+  // mean ms per kernel of one residual evaluation; returns the kernel names
+  std::string time_kernels(int reps, double *ms_out);
   int ncv() const;
 
   struct Impl;

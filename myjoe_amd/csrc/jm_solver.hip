@@ -55,6 +55,7 @@ struct DMesh {
   const int *bfa_zone;    // [nfa_b]
   const double *vol;      // [ncv]
   const ZoneDev *zones;
+  const int *sched;       // [ncv] cell visiting order of the per-cell kernels
 };
 
 struct DParams {
@@ -78,6 +79,15 @@ constexpr int kBlock = 256;
 constexpr int kRedSlots = 8;  // partial-sum slots per block for fused reductions
 
 __device__ __forceinline__ double2 ld2(const double *p) { return *reinterpret_cast<const double2 *>(p); }
+
+// Workgroups are dealt round-robin to the 8 XCDs (each with its own L2);
+// renumber them so every XCD walks one contiguous range of the schedule and
+// neighbouring cells share an L2.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  const int X = 8;
+  const int per = nb / X, rem = nb % X, x = b % X;
+  return x * per + (x < rem ? x : rem) + b / X;
+}
 
 // ---------------------------------------------------------------------------
 // calcStateVariables, UgpWithCvCompFlow.h:1662-1694 (gamma, R constant)
@@ -243,8 +253,9 @@ __device__ __forceinline__ double bj_face(double alfa, double phi, double phifa,
 template <bool LS>
 __global__ void __launch_bounds__(kBlock) k_grad(DMesh m, DParams P, const double *__restrict__ prim,
                                                  const double *__restrict__ bfa, double *__restrict__ grad) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= m.ncv) return;
+  const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (t >= m.ncv) return;
+  const int c = m.sched[t];
   const double *pc = prim + 8 * (size_t)c;
   const double rho = pc[0], u[3] = {pc[1], pc[2], pc[3]}, p = pc[4];
   const double xcv[3] = {pc[5], pc[6], pc[7]};
@@ -439,8 +450,9 @@ __global__ void __launch_bounds__(kBlock) k_flux(DMesh m, DParams P, const doubl
                                                  const double *__restrict__ grad, const double *__restrict__ bfa,
                                                  const double *__restrict__ q, double *__restrict__ rhs,
                                                  int *__restrict__ nfirst) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= m.ncv) return;
+  const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (t >= m.ncv) return;
+  const int c = m.sched[t];
   const double gam = P.gamma;
   double pc[8], gc[15];
   {
@@ -580,8 +592,9 @@ __global__ void __launch_bounds__(kBlock) k_jac(DMesh m, DParams P, const double
                                                 const double *__restrict__ bfa, T *__restrict__ A, int step_mode,
                                                 const double *__restrict__ dt, const double *__restrict__ rhs,
                                                 double relax, T *__restrict__ rhs_ls, T *__restrict__ phi) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= m.ncv) return;
+  const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (t >= m.ncv) return;
+  const int c = m.sched[t];
   const double gam = P.gamma;
   const double *pc = prim + 8 * (size_t)c;
   Side own;
@@ -677,73 +690,123 @@ __global__ void __launch_bounds__(kBlock) k_jac(DMesh m, DParams P, const double
 }
 
 // ---------------------------------------------------------------------------
-// deterministic block reductions with a "last block finishes" tail
+// Deterministic two-stage reductions: every block writes its partial and a
+// one-block finalize kernel combines them in a fixed order. No device-scope
+// fences or tickets: on a multi-XCD part a device-scope release writes back
+// the XCD's L2, which made a one-ticket-per-block tail cost milliseconds.
 // ---------------------------------------------------------------------------
+enum RedOp { kSum = 0, kMax = 1, kMin = 2 };
+
+__device__ __forceinline__ double rcomb(double a, double b, int op) {
+  return op == kSum ? a + b : (op == kMax ? Dev::mx(a, b) : Dev::mn(a, b));
+}
+
+// result valid in thread 0; sh holds NV * 16 doubles
 template <int NV>
-__device__ __forceinline__ void block_reduce(double (&v)[NV], double *sh /* [NV][kBlock/64] */, bool use_max) {
+__device__ __forceinline__ void block_reduce(double (&v)[NV], double *sh, int op) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
   for (int k = 0; k < NV; k++) {
     double x = v[k];
-    for (int off = 32; off > 0; off >>= 1) {
-      double y = __shfl_down(x, off, 64);
-      x = use_max ? Dev::mx(x, y) : x + y;
-    }
+    for (int off = 32; off > 0; off >>= 1) x = rcomb(x, __shfl_down(x, off, 64), op);
     if (lane == 0) sh[k * 16 + wid] = x;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int k = 0; k < NV; k++) {
       double x = sh[k * 16];
-      for (int w = 1; w < nw; w++) x = use_max ? Dev::mx(x, sh[k * 16 + w]) : x + sh[k * 16 + w];
+      for (int w = 1; w < nw; w++) x = rcomb(x, sh[k * 16 + w], op);
       v[k] = x;
     }
   }
 }
 
-// returns true in the block that saw the last ticket; partials has
-// gridDim.x * NV slots
 template <int NV>
-__device__ bool last_block(double (&v)[NV], double *partials, unsigned *counter, double *sh, bool use_max) {
-  block_reduce<NV>(v, sh, use_max);
-  __shared__ bool is_last;
-  if (threadIdx.x == 0) {
+__device__ __forceinline__ void block_partials(double (&v)[NV], double *partials, double *sh, int op) {
+  block_reduce<NV>(v, sh, op);
+  if (threadIdx.x == 0)
     for (int k = 0; k < NV; k++) partials[(size_t)k * gridDim.x + blockIdx.x] = v[k];
-    __threadfence();
-    unsigned t = atomicAdd(counter, 1u);
-    is_last = (t == gridDim.x - 1);
+}
+
+// fixed-order combination of nb partials; valid in thread 0
+__device__ double fin_slot(const double *__restrict__ p, int nb, int op, double *sh) {
+  double x = op == kSum ? 0.0 : (op == kMax ? -__builtin_huge_val() : __builtin_huge_val());
+  bool first = true;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    double y = p[b];
+    x = first ? y : rcomb(x, y, op);
+    first = false;
   }
+  double v[1] = {x};
   __syncthreads();
-  if (!is_last) return false;
-  __threadfence();
-  // fixed-order final sum over block partials
-  double w[NV];
-  for (int k = 0; k < NV; k++) {
-    double x = use_max ? 0.0 : 0.0;
-    bool first = true;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
-      double y = __builtin_nontemporal_load(&partials[(size_t)k * gridDim.x + b]);
-      x = first ? y : (use_max ? Dev::mx(x, y) : x + y);
-      first = false;
+  block_reduce<1>(v, sh, op);
+  return v[0];
+}
+
+enum FinOp { kFinDt = 0, kFinL1 = 1, kFinAlpha = 2, kFinOmega = 3, kFinResid0 = 4, kFinResid = 5 };
+
+// one block: finishes a reduction and applies the scalar update of its owner
+__global__ void __launch_bounds__(kBlock) k_fin(int nb, int op, const double *__restrict__ partials,
+                                                SolverScalars *S) {
+  __shared__ double sh[16];
+  if ((op == kFinAlpha || op == kFinOmega || op == kFinResid) && S->done) return;
+  const bool t0 = threadIdx.x == 0;
+  if (op == kFinDt) {
+    double m = fin_slot(partials, nb, kMin, sh);
+    if (t0) S->dt_min = m;
+  } else if (op == kFinL1) {
+    for (int k = 0; k < 5; k++) {
+      double v = fin_slot(partials + (size_t)k * nb, nb, kSum, sh);
+      if (t0) S->resid[k] = v;
     }
-    w[k] = first ? 0.0 : x;
+  } else if (op == kFinAlpha) {  // UgpWithCv.cpp:2775-2782
+    double gamma = fin_slot(partials, nb, kSum, sh);
+    if (t0) {
+      if (fabs(gamma) < 1.0E-20) gamma = 1.0E-20;
+      S->gamma = gamma;
+      S->alpha = S->rho / gamma;
+    }
+  } else if (op == kFinOmega) {  // :2808-2810
+    double st = fin_slot(partials, nb, kSum, sh);
+    double tt = fin_slot(partials + nb, nb, kSum, sh);
+    if (t0) S->omega = st / (tt + 1.0E-20);
+  } else {  // residual: stop test (:2847-2856) and the next iteration's rho, beta (:2736-2749)
+    double rho_next = fin_slot(partials, nb, kSum, sh);
+    double mxv = fin_slot(partials + nb, nb, kMax, sh);
+    if (t0) {
+      if (op == kFinResid0) {
+        S->res0_max = mxv;
+        S->rho = 1.0;
+        S->alpha = 1.0;
+        S->omega = 1.0;
+        S->iter = 0;
+        S->done = 0;
+      } else {
+        S->res_max = mxv;
+        if ((mxv <= S->zero_abs) || (mxv / (S->res0_max + 1.0E-12) <= S->zero_rel))
+          S->done = 1;
+        else if (S->iter > S->maxiter)
+          S->done = 2;
+      }
+      if (!S->done) {
+        S->iter += 1;
+        double rho_prime = -S->omega * S->rho;
+        S->rho = rho_next;
+        if (fabs(rho_prime) < 1.0E-20) rho_prime = 1.0E-20;
+        S->rho_prime = rho_prime;
+        S->beta = S->alpha * S->rho / rho_prime;
+      }
+    }
   }
-  __syncthreads();
-  block_reduce<NV>(w, sh, use_max);
-  if (threadIdx.x == 0) {
-    for (int k = 0; k < NV; k++) v[k] = w[k];
-    *counter = 0u;
-  }
-  return true;
 }
 
 // local time step (MAX_BASED), UgpWithCvCompFlow.cpp:27-60; mul = mut = 0.
-// dt_min reduced into S->dt_min; CFL_MIN mode is applied by k_dt_fill.
+// Block minima go to partials (k_fin kFinDt); CFL_MIN mode: k_dt_fill.
 __global__ void __launch_bounds__(kBlock) k_dt(DMesh m, const double *__restrict__ prim, const double *__restrict__ q,
                                                double gam, double cfl, int mode, double dt_fixed,
-                                               double *__restrict__ dt, double *partials, unsigned *counter,
-                                               SolverScalars *S) {
+                                               double *__restrict__ dt, double *partials) {
   __shared__ double sh[16];
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int c = t < m.ncv ? m.sched[t] : m.ncv;
   double dtc = 1.0e20;
   if (c < m.ncv) {
     if (mode == 2) {
@@ -767,8 +830,8 @@ __global__ void __launch_bounds__(kBlock) k_dt(DMesh m, const double *__restrict
     }
     dt[c] = dtc;
   }
-  double v[1] = {-dtc};  // min via max of negatives
-  if (last_block<1>(v, partials, counter, sh, true) && threadIdx.x == 0) S->dt_min = -v[0];
+  double v[1] = {dtc};
+  block_partials<1>(v, partials, sh, kMin);
 }
 
 __global__ void k_dt_fill(int ncv, double *dt, const SolverScalars *S) {
@@ -811,15 +874,14 @@ __device__ __forceinline__ T spmv_row(const int *__restrict__ nbocv_i, const int
 
 constexpr int kCellsPerBlock = kBlock / 5;  // 51 cells, 255 lanes active
 
-// res = rhs - A phi; init: res0 = res, res0_max; always: rho_next = res.res0,
-// res_max = max |LU^-1 res|; the last block runs the reference's stop test and
-// prepares beta for the next iteration.
+// res = rhs - A phi; init: res0 = res. Partials: res.res0 (slot 0) and
+// max |LU^-1 res| (slot 1), finished by k_fin kFinResid0 / kFinResid.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_resid(int ncv, const int *__restrict__ nbocv_i,
                                                   const int *__restrict__ nbocv_v, const T *__restrict__ A,
                                                   const T *__restrict__ LU, const T *__restrict__ phi,
                                                   const T *__restrict__ rhs, T *__restrict__ res, T *__restrict__ res0,
-                                                  int init, double *partials, unsigned *counter, SolverScalars *S) {
+                                                  int init, double *partials, const SolverScalars *S) {
   __shared__ double sh[32];
   __shared__ T cellv[kCellsPerBlock][5];
   if (!init && S->done) return;
@@ -835,67 +897,20 @@ __global__ void __launch_bounds__(kBlock) k_resid(int ncv, const int *__restrict
     cellv[lc][i] = r;
   }
   __syncthreads();
-  double v[2] = {0.0, 0.0};
+  double vs[1] = {0.0}, vm[1] = {0.0};
   if (active) {
     T r0v = init ? r : res0[5 * (size_t)c + i];
-    v[0] = (double)r * (double)r0v;
+    vs[0] = (double)r * (double)r0v;
     T tmp[5] = {cellv[lc][0], cellv[lc][1], cellv[lc][2], cellv[lc][3], cellv[lc][4]};
     lusolv5<T>(LU + 25 * (size_t)c, tmp);
-    v[1] = fabs((double)tmp[i]);
+    vm[0] = fabs((double)tmp[i]);
   }
-  double vs[1] = {v[0]}, vm[1] = {v[1]};
-  // two reductions: sum for the dot, max for the norm (separate partial slots)
-  block_reduce<1>(vs, sh, false);
+  block_reduce<1>(vs, sh, kSum);
   __syncthreads();
-  block_reduce<1>(vm, sh + 16, true);
-  __shared__ bool is_last;
+  block_reduce<1>(vm, sh + 16, kMax);
   if (threadIdx.x == 0) {
     partials[blockIdx.x] = vs[0];
     partials[gridDim.x + blockIdx.x] = vm[0];
-    __threadfence();
-    unsigned t = atomicAdd(counter, 1u);
-    is_last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-  double s = 0.0, mxv = 0.0;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
-    s += __builtin_nontemporal_load(&partials[b]);
-    mxv = Dev::mx(mxv, __builtin_nontemporal_load(&partials[gridDim.x + b]));
-  }
-  vs[0] = s;
-  vm[0] = mxv;
-  __syncthreads();
-  block_reduce<1>(vs, sh, false);
-  __syncthreads();
-  block_reduce<1>(vm, sh + 16, true);
-  if (threadIdx.x == 0) {
-    *counter = 0u;
-    double rho_next = vs[0];
-    if (init) {
-      S->res0_max = vm[0];
-      S->rho = 1.0;
-      S->alpha = 1.0;
-      S->omega = 1.0;
-      S->iter = 0;
-      S->done = 0;
-    } else {
-      S->res_max = vm[0];
-      double res_max = vm[0];
-      if ((res_max <= S->zero_abs) || (res_max / (S->res0_max + 1.0E-12) <= S->zero_rel))
-        S->done = 1;
-      else if (S->iter > S->maxiter)
-        S->done = 2;
-    }
-    if (!S->done) {  // start of the next iteration (UgpWithCv.cpp:2736-2749)
-      S->iter += 1;
-      double rho_prime = -S->omega * S->rho;
-      S->rho = rho_next;
-      if (fabs(rho_prime) < 1.0E-20) rho_prime = 1.0E-20;
-      S->rho_prime = rho_prime;
-      S->beta = S->alpha * S->rho / rho_prime;
-    }
   }
 }
 
@@ -919,14 +934,14 @@ __global__ void __launch_bounds__(kBlock) k_p(int ncv, const T *__restrict__ LU,
   for (int k = 0; k < 5; k++) phat[5 * (size_t)c + k] = t[k];
 }
 
-// y = A x and a fused dot: mode 0: gamma = y.res0 -> alpha;
-// mode 1: (s.t, t.t) -> omega
+// y = A x with partial dots (y.other, y.y): mode 0 gives gamma = v.res0
+// (k_fin kFinAlpha), mode 1 gives (s.t, t.t) (k_fin kFinOmega)
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_spmv_dot(int ncv, const int *__restrict__ nbocv_i,
                                                      const int *__restrict__ nbocv_v, const T *__restrict__ A,
                                                      const T *__restrict__ x, T *__restrict__ y,
-                                                     const T *__restrict__ other, int mode, double *partials,
-                                                     unsigned *counter, SolverScalars *S) {
+                                                     const T *__restrict__ other, double *partials,
+                                                     const SolverScalars *S) {
   __shared__ double sh[32];
   if (S->done) return;
   const int lc = threadIdx.x / 5, i = threadIdx.x % 5;
@@ -936,19 +951,10 @@ __global__ void __launch_bounds__(kBlock) k_spmv_dot(int ncv, const int *__restr
     T yv = spmv_row<T>(nbocv_i, nbocv_v, A, x, c, i);
     y[5 * (size_t)c + i] = yv;
     double o = (double)other[5 * (size_t)c + i];
-    v[0] = (double)yv * o;        // mode 0: v.res0 ; mode 1: t.s
-    v[1] = (double)yv * (double)yv;  // t.t
+    v[0] = (double)yv * o;
+    v[1] = (double)yv * (double)yv;
   }
-  if (last_block<2>(v, partials, counter, sh, false) && threadIdx.x == 0) {
-    if (mode == 0) {
-      double gamma = v[0];
-      if (fabs(gamma) < 1.0E-20) gamma = 1.0E-20;
-      S->gamma = gamma;
-      S->alpha = S->rho / gamma;
-    } else {
-      S->omega = v[0] / (v[1] + 1.0E-20);
-    }
-  }
+  block_partials<2>(v, partials, sh, kSum);
 }
 
 // s = res - alpha v; shat = LU^-1 s
@@ -989,16 +995,14 @@ __global__ void __launch_bounds__(kBlock) k_update(int ncv, double *__restrict__
   for (int k = 0; k < 5; k++) q[(size_t)k * ncv + c] += (double)dq[5 * (size_t)c + k];
 }
 
-// L1 residual sums (JoeWithModels.cpp:1119-1130)
-__global__ void __launch_bounds__(kBlock) k_l1(int ncv, const double *__restrict__ rhs, double *partials,
-                                               unsigned *counter, SolverScalars *S) {
+// L1 residual sums (JoeWithModels.cpp:1119-1130), finished by k_fin kFinL1
+__global__ void __launch_bounds__(kBlock) k_l1(int ncv, const double *__restrict__ rhs, double *partials) {
   __shared__ double sh[16 * 5];
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   double v[5] = {0, 0, 0, 0, 0};
   if (c < ncv)
     for (int k = 0; k < 5; k++) v[k] = fabs(rhs[(size_t)k * ncv + c]);
-  if (last_block<5>(v, partials, counter, sh, false) && threadIdx.x == 0)
-    for (int k = 0; k < 5; k++) S->resid[k] = v[k];
+  block_partials<5>(v, partials, sh, kSum);
 }
 
 __global__ void k_to_double(int n, const float *__restrict__ s, double *__restrict__ d) {
@@ -1170,6 +1174,7 @@ struct Ctx::Impl {
   DBuf<double> fgeo, fnorm, lsg, bfa_lsg, vol;
   DBuf<int> cfl_i, cfl_noc, nbocv_i, nbocv_v, bfa_cell, bfa_zone;
   DBuf<int2> cfl_e;
+  DBuf<int> sched;
   DBuf<ZoneDev> zones;
   std::vector<int> h_nbocv_i;
   bool dup_blocks = false;
@@ -1180,7 +1185,6 @@ struct Ctx::Impl {
   DBuf<double> A64, LU64, v64[10];
   DBuf<float> A32, LU32, v32[10];
   DBuf<double> partials;
-  DBuf<unsigned> red_counter;
   DBuf<SolverScalars> scal;
   SolverScalars *h_scal = nullptr;  // pinned
 
@@ -1304,6 +1308,19 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
   }
   ci[m.ncv] = (int)cf.size();
   d.cfl_i.upload(ci.data(), ci.size());
+  // visiting order: within chunks of kSchedChunk consecutive cells, cells
+  // grouped by face count (stable), so a wavefront's lanes loop equally long
+  {
+    const int kSchedChunk = 2048;
+    std::vector<int> sched(m.ncv);
+    for (int c = 0; c < m.ncv; c++) sched[c] = c;
+    for (int c0 = 0; c0 < m.ncv; c0 += kSchedChunk) {
+      int c1 = std::min(m.ncv, c0 + kSchedChunk);
+      std::stable_sort(sched.begin() + c0, sched.begin() + c1,
+                       [&](int a, int b) { return (ci[a + 1] - ci[a]) < (ci[b + 1] - ci[b]); });
+    }
+    d.sched.upload(sched.data(), sched.size());
+  }
   d.cfl_e.upload(cf.data(), cf.size());
   d.cfl_noc.upload(noc.data(), noc.size());
   d.h_nbocv_i = m.nbocv_i;
@@ -1333,8 +1350,6 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
   d.counters.alloc(4);
   HIPCHECK(hipMemset(d.counters.p, 0, 4 * sizeof(int)));
   d.partials.alloc(8 * (size_t)(nblk(5 * n) + 64));
-  d.red_counter.alloc(1);
-  HIPCHECK(hipMemset(d.red_counter.p, 0, sizeof(unsigned)));
   d.scal.alloc(1);
   HIPCHECK(hipMemset(d.scal.p, 0, sizeof(SolverScalars)));
   HIPCHECK(hipHostMalloc((void **)&d.h_scal, sizeof(SolverScalars)));
@@ -1356,6 +1371,7 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
   d.dm.bfa_zone = d.bfa_zone.p;
   d.dm.vol = d.vol.p;
   d.dm.zones = d.zones.p;
+  d.dm.sched = d.sched.p;
 
   d.dp.gamma = d.par.gamma;
   d.dp.R = d.par.R_gas;
@@ -1498,7 +1514,8 @@ void Ctx::Impl::solve(int &iters, int &status) {
   hipLaunchKernelGGL(k_lu<T>, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, nbocv_i.p, A, LU->p);
   const int gb5 = (ncv + kCellsPerBlock - 1) / kCellsPerBlock;
   hipLaunchKernelGGL(k_resid<T>, dim3(gb5), dim3(kBlock), 0, stream, ncv, nbocv_i.p, nbocv_v.p, A, LU->p, phi, rhs_,
-                     res, res0, 1, partials.p, red_counter.p, scal.p);
+                     res, res0, 1, partials.p, scal.p);
+  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, stream, gb5, (int)kFinResid0, partials.p, scal.p);
   const int max_it = par.ls_maxiter + 1;
   int launched = 0;
   while (launched < max_it) {
@@ -1506,13 +1523,16 @@ void Ctx::Impl::solve(int &iters, int &status) {
     for (int k = 0; k < chunk; k++) {
       hipLaunchKernelGGL(k_p<T>, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, LU->p, res, vv, p, phat, scal.p);
       hipLaunchKernelGGL(k_spmv_dot<T>, dim3(gb5), dim3(kBlock), 0, stream, ncv, nbocv_i.p, nbocv_v.p, A, phat, vv,
-                         res0, 0, partials.p, red_counter.p, scal.p);
+                         res0, partials.p, scal.p);
+      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, stream, gb5, (int)kFinAlpha, partials.p, scal.p);
       hipLaunchKernelGGL(k_s<T>, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, LU->p, res, vv, s, shat, scal.p);
       hipLaunchKernelGGL(k_spmv_dot<T>, dim3(gb5), dim3(kBlock), 0, stream, ncv, nbocv_i.p, nbocv_v.p, A, shat, t, s,
-                         1, partials.p, red_counter.p, scal.p);
+                         partials.p, scal.p);
+      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, stream, gb5, (int)kFinOmega, partials.p, scal.p);
       hipLaunchKernelGGL(k_x<T>, dim3(nblk(n5)), dim3(kBlock), 0, stream, n5, phi, phat, shat, scal.p);
       hipLaunchKernelGGL(k_resid<T>, dim3(gb5), dim3(kBlock), 0, stream, ncv, nbocv_i.p, nbocv_v.p, A, LU->p, phi,
-                         rhs_, res, res0, 0, partials.p, red_counter.p, scal.p);
+                         rhs_, res, res0, 0, partials.p, scal.p);
+      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, stream, gb5, (int)kFinResid, partials.p, scal.p);
     }
     launched += chunk;
     HIPCHECK(hipMemcpyAsync(h_scal, scal.p, sizeof(SolverScalars), hipMemcpyDeviceToHost, stream));
@@ -1534,12 +1554,64 @@ void Ctx::launch_residual() {
   d.launch_grad_flux();
 }
 
+std::string Ctx::time_kernels(int reps, double *ms) {
+  Impl &d = *d_;
+  if (!d.have_state) throw std::logic_error("no state set");
+  hipEvent_t e0, e1;
+  HIPCHECK(hipEventCreate(&e0));
+  HIPCHECK(hipEventCreate(&e1));
+  auto timed = [&](auto &&launch) {
+    launch();  // warm
+    HIPCHECK(hipEventRecord(e0, d.stream));
+    for (int r = 0; r < reps; r++) launch();
+    HIPCHECK(hipEventRecord(e1, d.stream));
+    HIPCHECK(hipEventSynchronize(e1));
+    float t = 0.f;
+    HIPCHECK(hipEventElapsedTime(&t, e0, e1));
+    return (double)t / reps;
+  };
+  ms[0] = timed([&] {
+    hipLaunchKernelGGL(k_state, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.q.p, d.prim.p, d.par.gamma,
+                       d.counters.p);
+  });
+  ms[1] = d.nfa_b > 0 ? timed([&] {
+    hipLaunchKernelGGL(k_bc, dim3(nblk(d.nfa_b)), dim3(kBlock), 0, d.stream, d.dm, d.dp, d.prim.p, d.hook.p, d.bfa.p);
+  }) : 0.0;
+  ms[2] = 0.0;
+  if (d.par.second_order) {
+    ms[2] = timed([&] {
+      if (d.par.grad_ls)
+        hipLaunchKernelGGL(k_grad<true>, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.dm, d.dp, d.prim.p, d.bfa.p,
+                           d.grad.p);
+      else
+        hipLaunchKernelGGL(k_grad<false>, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.dm, d.dp, d.prim.p, d.bfa.p,
+                           d.grad.p);
+    });
+  }
+  // flux alone: time the whole grad+flux pair and subtract the gradient
+  double gf = timed([&] { d.launch_grad_flux(); });
+  ms[3] = gf - ms[2];
+  ms[4] = timed([&] {
+    hipLaunchKernelGGL(k_dt, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.dm, d.prim.p, d.q.p, d.par.gamma, d.cfl,
+                       d.par.dt_mode, d.par.dt_fixed, d.dt.p, d.partials.p);
+  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, d.stream, nblk(d.ncv), (int)kFinDt, d.partials.p, d.scal.p);
+  });
+  ms[5] = timed([&] {
+    d.launch_state_bc();
+    d.launch_grad_flux();
+  });
+  HIPCHECK(hipEventDestroy(e0));
+  HIPCHECK(hipEventDestroy(e1));
+  return "k_state,k_bc,k_grad,k_flux,k_dt,residual";
+}
+
 void Ctx::calc_rhs(bool implicit) {
   Impl &d = *d_;
   if (!d.have_state) throw std::logic_error("no state set");
   d.launch_state_bc();
   hipLaunchKernelGGL(k_dt, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.dm, d.prim.p, d.q.p, d.par.gamma, d.cfl,
-                     d.par.dt_mode, d.par.dt_fixed, d.dt.p, d.partials.p, d.red_counter.p, d.scal.p);
+                     d.par.dt_mode, d.par.dt_fixed, d.dt.p, d.partials.p);
+  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, d.stream, nblk(d.ncv), (int)kFinDt, d.partials.p, d.scal.p);
   if (d.par.dt_mode == 1) hipLaunchKernelGGL(k_dt_fill, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.dt.p, d.scal.p);
   d.launch_grad_flux();
   if (implicit) {
@@ -1566,7 +1638,8 @@ StepInfo Ctx::step() {
   if ((d.step_no >= d.par.cfl_after) && (d.step_no % d.par.cfl_interval == 0) && (d.cfl < d.par.cfl_max))
     d.cfl *= d.par.cfl_factor;
   hipLaunchKernelGGL(k_dt, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.dm, d.prim.p, d.q.p, d.par.gamma, d.cfl,
-                     d.par.dt_mode, d.par.dt_fixed, d.dt.p, d.partials.p, d.red_counter.p, d.scal.p);
+                     d.par.dt_mode, d.par.dt_fixed, d.dt.p, d.partials.p);
+  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, d.stream, nblk(d.ncv), (int)kFinDt, d.partials.p, d.scal.p);
   if (d.par.dt_mode == 1) hipLaunchKernelGGL(k_dt_fill, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.dt.p, d.scal.p);
   d.launch_grad_flux();
   int it = 0, st = 0;
@@ -1586,8 +1659,8 @@ StepInfo Ctx::step() {
     hipLaunchKernelGGL(k_update<float>, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.q.p, d.v32[1].p);
   }
   d.launch_state_bc();
-  hipLaunchKernelGGL(k_l1, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.rhs.p, d.partials.p,
-                     d.red_counter.p, d.scal.p);
+  hipLaunchKernelGGL(k_l1, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.rhs.p, d.partials.p);
+  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, d.stream, nblk(d.ncv), (int)kFinL1, d.partials.p, d.scal.p);
   HIPCHECK(hipMemcpyAsync(d.h_scal, d.scal.p, sizeof(SolverScalars), hipMemcpyDeviceToHost, d.stream));
   HIPCHECK(hipStreamSynchronize(d.stream));
   info.step = d.step_no;

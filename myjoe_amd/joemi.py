@@ -63,6 +63,8 @@ def lib():
         L.jm_device_state.argtypes = [vp, vp]
         L.jm_device_rhs.argtypes = [vp, vp]
         L.jm_sync.argtypes = [vp]
+        L.jm_time_residual_kernels.argtypes = [vp, C.c_int, vp, C.c_char_p, C.c_int]
+        L.jm_mesh_quality.argtypes = [vp, vp, vp]
         _lib = L
     return _lib
 
@@ -119,6 +121,11 @@ class Mesh:
         out = np.empty(n.value, dt)
         _check(lib().jm_mesh_array(self.h, name.encode(), _ptr(out), C.byref(n)))
         return out
+
+    def quality(self):
+        q, n = C.c_double(), C.c_int()
+        _check(lib().jm_mesh_quality(self.h, C.byref(q), C.byref(n)))
+        return q.value, n.value
 
     def zones(self):
         out = []
@@ -210,3 +217,10 @@ class Solver:
 
     def sync(self):
         _check(lib().jm_sync(self.h))
+
+    def time_kernels(self, reps=10):
+        ms = np.zeros(8)
+        buf = C.create_string_buffer(256)
+        _check(lib().jm_time_residual_kernels(self.h, reps, _ptr(ms), buf, 256))
+        names = buf.value.decode().split(",")
+        return dict(zip(names, ms[:len(names)]))
