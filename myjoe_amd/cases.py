@@ -23,13 +23,14 @@ MACH_BUMP = 0.5
 U_BUMP = MACH_BUMP * math.sqrt(GAMMA)  # c = sqrt(gamma p / rho) with p = rho = 1
 
 
-def _common(scheme: str, second_order: bool, grad: str, eps: float, extra: str) -> str:
+def _common(scheme: str, second_order: bool, grad: str, eps: float, extra: str,
+            symm_bc: str = "ORIGINAL_NUMERIC") -> str:
     lines = [
         "GAMMA = 1.4", "P_REF = 1.0", "RHO_REF = 1.0", "T_REF = 1.0", "MU_REF = 0.0",
         f"SPACE_INTEGRATION = {scheme}",
         f"GRAD_RECONSTRUCTION SCHEME={grad} LIMITER=YES EPS={eps!r}",
         "TIME_INTEGRATION = BACKWARD_EULER", "LINEAR_SOLVER_NS = BCGSTAB",
-        "SYMM_BC = ORIGINAL_NUMERIC", "INITIAL_CONDITION_TURB = 0.0", "CHECK_INTERVAL = 1",
+        f"SYMM_BC = {symm_bc}", "INITIAL_CONDITION_TURB = 0.0", "CHECK_INTERVAL = 1",
     ]
     if second_order:
         lines.append("SPATIAL_SECOND_ORDER")
@@ -58,26 +59,41 @@ def bump_case(nx: int = 96, ny: int = 32, scheme: str = "HLLC", second_order: bo
     return mesh, _common(scheme, second_order, "LEASTSQUARES", 0.01, extra)
 
 
+BOX_BCS = {
+    "inlet": "CBC 0.5 0.1 0.0 1.0 1.0",
+    "outlet": "CBC 0.5 0.1 0.0 1.0 1.0",
+    "wall": "WALL",
+    "farfield": "FARFIELD 0.4 0.2 0.1 1.0 1.0 1.4",
+    "symm": "SYMMETRY",
+}
+# the other boundary kinds of setBC: total-condition inlet (flow angle, T_tot,
+# p_tot), static-pressure outlet, zero-gradient
+BOX_BCS_ALT = {
+    "inlet": "CBC_SUBSONIC_INLET 0.8 0.6 0.0 1.3 1.4",
+    "outlet": "CBC_SUBSONIC_OUTLET 0.9",
+    "wall": "WALL",
+    "farfield": "NEUMANN",
+    "symm": "SYMMETRY",
+}
+
+
 def box_joe_in(scheme: str = "HLLC", second_order: bool = True, grad: str = "LEASTSQUARES",
-               restart: str = "box.msh") -> str:
+               restart: str = "box.msh", bcs: dict | None = None, symm_bc: str = "ORIGINAL_NUMERIC") -> str:
+    bcs = BOX_BCS if bcs is None else bcs
     extra = "\n".join([
         f"RESTART = {restart}",
         "CFL = 1.0",
         "NSTEPS = 10",
         "U_INITIAL = 0.3 0.1 0.0",
-        "inlet = CBC 0.5 0.1 0.0 1.0 1.0",
-        "outlet = CBC 0.5 0.1 0.0 1.0 1.0",
-        "wall = WALL",
-        "farfield = FARFIELD 0.4 0.2 0.1 1.0 1.0 1.4",
-        "symm = SYMMETRY",
-    ]) + "\n"
-    return _common(scheme, second_order, grad, 0.01, extra)
+    ] + [f"{z} = {bcs[z]}" for z in ("inlet", "outlet", "wall", "farfield", "symm")]) + "\n"
+    return _common(scheme, second_order, grad, 0.01, extra, symm_bc)
 
 
 def box_case(n: int, frac: float = 0.5, jitter: float = 0.2, seed: int = 0, scheme: str = "HLLC",
-             second_order: bool = True, grad: str = "LEASTSQUARES", restart: str = "box.msh"):
+             second_order: bool = True, grad: str = "LEASTSQUARES", restart: str = "box.msh",
+             bcs: dict | None = None, symm_bc: str = "ORIGINAL_NUMERIC"):
     mesh = meshgen.split_box(n, frac, jitter, seed)
-    return mesh, box_joe_in(scheme, second_order, grad, restart)
+    return mesh, box_joe_in(scheme, second_order, grad, restart, bcs, symm_bc)
 
 
 def random_state(ncv: int, seed: int = 1):

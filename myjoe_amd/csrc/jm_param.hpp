@@ -50,7 +50,7 @@ class ParamMap {
   }
   std::string str(const std::string &k) const {
     const Tokens *t = get(k);
-    if (!t || t->size() < 2) throw std::runtime_error("missing parameter " + k);
+    if (!t || t->size() < 2) throw std::invalid_argument("missing parameter " + k);
     return (*t)[1];
   }
   double dbl(const std::string &k, double def) const { return has(k) ? to_d(str(k), k) : def; }
@@ -60,10 +60,10 @@ class ParamMap {
   // value after a named token of line k, e.g. sub("CFL_RAMP", "MAX_CFL")
   std::string sub(const std::string &k, const std::string &name) const {
     const Tokens *t = get(k);
-    if (!t) throw std::runtime_error("missing parameter " + k);
+    if (!t) throw std::invalid_argument("missing parameter " + k);
     for (size_t i = 1; i + 1 < t->size(); i++)
       if ((*t)[i] == name) return (*t)[i + 1];
-    throw std::runtime_error("no " + name + " in parameter " + k);
+    throw std::invalid_argument("no " + name + " in parameter " + k);
   }
   double sub_d(const std::string &k, const std::string &name) const { return to_d(sub(k, name), k); }
   int sub_i(const std::string &k, const std::string &name) const { return to_i(sub(k, name), k); }
@@ -71,13 +71,13 @@ class ParamMap {
   static double to_d(const std::string &s, const std::string &k) {
     char *end = nullptr;
     double v = std::strtod(s.c_str(), &end);
-    if (end == s.c_str()) throw std::runtime_error("parameter " + k + ": '" + s + "' is not a number");
+    if (end == s.c_str()) throw std::invalid_argument("parameter " + k + ": '" + s + "' is not a number");
     return v;
   }
   static int to_i(const std::string &s, const std::string &k) {
     char *end = nullptr;
     long v = std::strtol(s.c_str(), &end, 10);
-    if (end == s.c_str()) throw std::runtime_error("parameter " + k + ": '" + s + "' is not an integer");
+    if (end == s.c_str()) throw std::invalid_argument("parameter " + k + ": '" + s + "' is not an integer");
     return (int)v;
   }
 

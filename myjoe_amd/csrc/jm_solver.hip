@@ -324,8 +324,16 @@ __global__ void __launch_bounds__(kBlock) k_grad(DMesh m, DParams P, const doubl
       if (en.y >= 0) {
         const double *pn = prim + 8 * (size_t)en.y;
         double rn = pn[0], un[3] = {pn[1], pn[2], pn[3]}, pnv = pn[4];
-        const double w0 = 0.5, w1 = 0.5;
+        // distance weights (GG_WITH_DIST, UgpWithCv.h:37, :1104-1110)
+        const double *xf = m.fgeo + 8 * (size_t)f + 4;
+        double ds[3], dn[3];
+        for (int j = 0; j < 3; j++) {
+          ds[j] = xf[j] - xcv[j];
+          dn[j] = xf[j] - pn[5 + j];
+        }
         const bool s0 = (en.x >= 0);  // side 0: this cell is icv0
+        const double wself = sqrt(dot3(ds, ds)), wnb = sqrt(dot3(dn, dn));
+        const double w0 = s0 ? wself : wnb, w1 = s0 ? wnb : wself;
         double fr = s0 ? (w1 * rho + w0 * rn) / (w0 + w1) : (w1 * rn + w0 * rho) / (w0 + w1);
         double fp = s0 ? (w1 * p + w0 * pnv) / (w0 + w1) : (w1 * pnv + w0 * p) / (w0 + w1);
         for (int j = 0; j < 3; j++) {
@@ -1176,7 +1184,8 @@ struct Ctx::Impl {
   DBuf<int2> cfl_e;
   DBuf<int> sched;
   DBuf<ZoneDev> zones;
-  std::vector<int> h_nbocv_i;
+  std::vector<int> perm;              // device cell -> reference cell
+  std::vector<int> h_dev_nbocv_i, h_ref_nbocv_i;
   bool dup_blocks = false;
   // state
   DBuf<double> q, prim, grad, bfa, rhs, dt, hook;
@@ -1207,6 +1216,24 @@ float *Ctx::Impl::A_<float>() {
   return A32.p;
 }
 
+// Morton key of a point in the unit cube [0,1)^3 (21 bits per axis)
+static uint64_t morton3(double x, double y, double z) {
+  auto spread = [](uint64_t v) {
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffULL;
+    v = (v | v << 16) & 0x1f0000ff0000ffULL;
+    v = (v | v << 8) & 0x100f00f00f00f00fULL;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+    v = (v | v << 2) & 0x1249249249249249ULL;
+    return v;
+  };
+  auto q = [](double t) {
+    double u = t < 0.0 ? 0.0 : (t >= 1.0 ? 0.99999999 : t);
+    return (uint64_t)(u * (double)(1 << 21));
+  };
+  return spread(q(x)) | spread(q(y)) << 1 | spread(q(z)) << 2;
+}
+
 Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) : d_(new Impl) {
   Impl &d = *d_;
   d.precision = precision;
@@ -1218,28 +1245,7 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
   d.nnz = m.nbocv_s();
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw std::runtime_error("no HIP device available");
-
-  // face records: unit normal and area exactly as normVec3d, then x_fa
-  std::vector<double> fg(8 * (size_t)m.nfa);
-  for (int f = 0; f < m.nfa; f++) {
-    const double *n = &m.fa_normal[3 * (size_t)f];
-    double mag = 0.0;
-    mag += n[0] * n[0];
-    mag += n[1] * n[1];
-    mag += n[2] * n[2];
-    mag = std::sqrt(mag);
-    double *g = &fg[8 * (size_t)f];
-    g[0] = n[0] / mag;
-    g[1] = n[1] / mag;
-    g[2] = n[2] / mag;
-    g[3] = mag;
-    g[4] = m.x_fa[3 * (size_t)f];
-    g[5] = m.x_fa[3 * (size_t)f + 1];
-    g[6] = m.x_fa[3 * (size_t)f + 2];
-    g[7] = 0.0;
-  }
-  d.fgeo.upload(fg.data(), fg.size());
-  if (!d.par.grad_ls) d.fnorm.upload(m.fa_normal.data(), m.fa_normal.size());
+  const int ncv = m.ncv, nfa = m.nfa, nfa_b = m.nfa_b;
 
   // zones: device slots for boundary zones
   std::vector<ZoneDev> zd;
@@ -1256,33 +1262,30 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
   d.nzone = (int)zd.size();
   if (zd.empty()) zd.push_back(ZoneDev());
   d.zones.upload(zd.data(), zd.size());
-  std::vector<int> bz(std::max(1, m.nfa_b), 0), bc(std::max(1, m.nfa_b), 0);
+  std::vector<int> bz(std::max(1, nfa_b), 0);
   for (size_t iz = 0; iz < m.zones.size(); iz++) {
     const Zone &z = m.zones[iz];
     if (z.kind != kZoneBoundary) continue;
     for (int f = z.ifa_f; f <= z.ifa_l; f++) bz[f] = d.zone_slot[iz];
   }
-  for (int f = 0; f < m.nfa_b; f++) bc[f] = m.cvofa[2 * (size_t)f];
-  d.bfa_zone.upload(bz.data(), bz.size());
-  d.bfa_cell.upload(bc.data(), bc.size());
 
-  // per-cell face lists in accumulation order + off-diagonal block slots
-  std::vector<int> ci(m.ncv + 1, 0), noc;
+  // per-cell face lists in the reference's accumulation order (reference ids)
+  std::vector<int> ci(ncv + 1, 0), slot_of;  // slot_of: position in the reference nbocv row
   std::vector<int2> cf;
   cf.reserve(m.faocv_v.size());
-  noc.reserve(m.faocv_v.size());
-  std::vector<int> fi, fb;
-  for (int c = 0; c < m.ncv; c++) {
+  slot_of.reserve(m.faocv_v.size());
+  std::vector<int> fi, fb, used;
+  for (int c = 0; c < ncv; c++) {
     fi.clear();
     fb.clear();
+    used.clear();
     for (int k = m.faocv_i[c]; k < m.faocv_i[c + 1]; k++) {
       int f = m.faocv_v[k];
-      (f >= m.nfa_b ? fi : fb).push_back(f);
+      (f >= nfa_b ? fi : fb).push_back(f);
     }
     std::sort(fi.begin(), fi.end());
     std::sort(fb.begin(), fb.end());
     ci[c] = (int)cf.size();
-    std::vector<int> used;
     for (int f : fi) {
       int c0 = m.cvofa[2 * (size_t)f], c1 = m.cvofa[2 * (size_t)f + 1];
       int side = (c0 == c) ? 0 : 1;
@@ -1295,57 +1298,162 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
       if (acc) d.dup_blocks = true;
       used.push_back(slot);
       cf.push_back(make_int2(f | (side << 31), nb));
-      noc.push_back(slot | (acc ? (1 << 30) : 0));
+      slot_of.push_back((slot - m.nbocv_i[c]) | (acc ? (1 << 30) : 0));
     }
     for (int f : fb) {
-      int iz = -1;
-      for (size_t z = 0; z < m.zones.size(); z++)
-        if (m.zones[z].kind == kZoneBoundary && f >= m.zones[z].ifa_f && f <= m.zones[z].ifa_l) iz = (int)z;
-      if (iz < 0) throw std::runtime_error("boundary face without zone");
-      cf.push_back(make_int2(f, -(d.zone_slot[iz] + 1)));
-      noc.push_back(m.nbocv_i[c]);
+      cf.push_back(make_int2(f, -(bz[f] + 1)));
+      slot_of.push_back(0);
     }
   }
-  ci[m.ncv] = (int)cf.size();
-  d.cfl_i.upload(ci.data(), ci.size());
-  // visiting order: within chunks of kSchedChunk consecutive cells, cells
-  // grouped by face count (stable), so a wavefront's lanes loop equally long
+  ci[ncv] = (int)cf.size();
+
+  // Device numbering (internal only; the ABI speaks reference numbering):
+  // cells in Morton order of their centroids, cut into tiles of kTile cells
+  // and grouped inside a tile by face count, so a workgroup's cells and their
+  // neighbours are spatially close and a wavefront's lanes loop equally long.
+  // Boundary faces keep their index; internal faces are numbered in order of
+  // first visit by the device-ordered cells. Per-cell sums keep the reference
+  // order, so results do not depend on this numbering.
+  const int kTile = 1024;
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int c = 0; c < ncv; c++)
+    for (int i = 0; i < 3; i++) {
+      lo[i] = std::min(lo[i], m.x_cv[3 * (size_t)c + i]);
+      hi[i] = std::max(hi[i], m.x_cv[3 * (size_t)c + i]);
+    }
+  double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+  if (!(ext > 0.0)) ext = 1.0;
+  std::vector<uint64_t> key(ncv);
+  for (int c = 0; c < ncv; c++)
+    key[c] = morton3((m.x_cv[3 * (size_t)c] - lo[0]) / ext, (m.x_cv[3 * (size_t)c + 1] - lo[1]) / ext,
+                     (m.x_cv[3 * (size_t)c + 2] - lo[2]) / ext);
+  std::vector<int> &perm = d.perm;
+  perm.resize(ncv);
+  for (int c = 0; c < ncv; c++) perm[c] = c;
+  std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return key[a] < key[b]; });
+  for (int t0 = 0; t0 < ncv; t0 += kTile) {
+    int t1 = std::min(ncv, t0 + kTile);
+    std::stable_sort(perm.begin() + t0, perm.begin() + t1,
+                     [&](int a, int b) { return (ci[a + 1] - ci[a]) < (ci[b + 1] - ci[b]); });
+  }
+  std::vector<int> inv(ncv);
+  for (int cd = 0; cd < ncv; cd++) inv[perm[cd]] = cd;
+  std::vector<int> fnew(nfa, -1);
+  for (int f = 0; f < nfa_b; f++) fnew[f] = f;
+  int nf = nfa_b;
+  for (int cd = 0; cd < ncv; cd++) {
+    int c = perm[cd];
+    for (int e = ci[c]; e < ci[c + 1]; e++) {
+      int f = cf[e].x & 0x7fffffff;
+      if (cf[e].y >= 0 && fnew[f] < 0) fnew[f] = nf++;
+    }
+  }
+  if (nf != nfa) throw std::runtime_error("internal face not reached by any cell");
+
+  // face records in device order: unit normal and area as normVec3d, x_fa
+  std::vector<double> fg(8 * (size_t)nfa);
+  std::vector<double> fn;
+  if (!d.par.grad_ls) fn.resize(3 * (size_t)nfa);
+  for (int f = 0; f < nfa; f++) {
+    const double *n = &m.fa_normal[3 * (size_t)f];
+    double mag = 0.0;
+    mag += n[0] * n[0];
+    mag += n[1] * n[1];
+    mag += n[2] * n[2];
+    mag = std::sqrt(mag);
+    double *g = &fg[8 * (size_t)fnew[f]];
+    g[0] = n[0] / mag;
+    g[1] = n[1] / mag;
+    g[2] = n[2] / mag;
+    g[3] = mag;
+    g[4] = m.x_fa[3 * (size_t)f];
+    g[5] = m.x_fa[3 * (size_t)f + 1];
+    g[6] = m.x_fa[3 * (size_t)f + 2];
+    g[7] = 0.0;
+    if (!fn.empty())
+      for (int i = 0; i < 3; i++) fn[3 * (size_t)fnew[f] + i] = n[i];
+  }
+  d.fgeo.upload(fg.data(), fg.size());
+  std::vector<double>().swap(fg);
+  if (!fn.empty()) d.fnorm.upload(fn.data(), fn.size());
+
+  // rows of nbocv / LS coefficients / Jacobian in device cell order
+  std::vector<int> &dni = d.h_dev_nbocv_i;
+  dni.assign(ncv + 1, 0);
+  for (int cd = 0; cd < ncv; cd++) {
+    int c = perm[cd];
+    dni[cd + 1] = dni[cd] + (m.nbocv_i[c + 1] - m.nbocv_i[c]);
+  }
   {
-    const int kSchedChunk = 2048;
-    std::vector<int> sched(m.ncv);
-    for (int c = 0; c < m.ncv; c++) sched[c] = c;
-    for (int c0 = 0; c0 < m.ncv; c0 += kSchedChunk) {
-      int c1 = std::min(m.ncv, c0 + kSchedChunk);
-      std::stable_sort(sched.begin() + c0, sched.begin() + c1,
-                       [&](int a, int b) { return (ci[a + 1] - ci[a]) < (ci[b + 1] - ci[b]); });
+    std::vector<int> nv(d.nnz);
+    std::vector<double> lg(3 * (size_t)d.nnz);
+    for (int cd = 0; cd < ncv; cd++) {
+      int c = perm[cd];
+      for (int j = 0; j < m.nbocv_i[c + 1] - m.nbocv_i[c]; j++) {
+        nv[dni[cd] + j] = inv[m.nbocv_v[m.nbocv_i[c] + j]];
+        for (int i = 0; i < 3; i++) lg[3 * (size_t)(dni[cd] + j) + i] = m.nbocv_lsg[3 * (size_t)(m.nbocv_i[c] + j) + i];
+      }
     }
-    d.sched.upload(sched.data(), sched.size());
+    d.nbocv_i.upload(dni.data(), dni.size());
+    d.nbocv_v.upload(nv.data(), nv.size());
+    d.lsg.upload(lg.data(), lg.size());
   }
-  d.cfl_e.upload(cf.data(), cf.size());
-  d.cfl_noc.upload(noc.data(), noc.size());
-  d.h_nbocv_i = m.nbocv_i;
-  d.nbocv_i.upload(m.nbocv_i.data(), m.nbocv_i.size());
-  d.nbocv_v.upload(m.nbocv_v.data(), m.nbocv_v.size());
-  d.lsg.upload(m.nbocv_lsg.data(), m.nbocv_lsg.size());
-  if (m.nfa_b > 0)
+  {
+    std::vector<int> dci(ncv + 1, 0), noc;
+    std::vector<int2> dcf;
+    dcf.reserve(cf.size());
+    noc.reserve(cf.size());
+    for (int cd = 0; cd < ncv; cd++) {
+      int c = perm[cd];
+      dci[cd] = (int)dcf.size();
+      for (int e = ci[c]; e < ci[c + 1]; e++) {
+        int f = cf[e].x & 0x7fffffff;
+        int side_bit = cf[e].x & (int)0x80000000;
+        if (cf[e].y >= 0) {
+          dcf.push_back(make_int2(fnew[f] | side_bit, inv[cf[e].y]));
+          noc.push_back((dni[cd] + (slot_of[e] & 0x3fffffff)) | (slot_of[e] & (1 << 30)));
+        } else {
+          dcf.push_back(make_int2(f, cf[e].y));
+          noc.push_back(dni[cd]);
+        }
+      }
+    }
+    dci[ncv] = (int)dcf.size();
+    d.cfl_i.upload(dci.data(), dci.size());
+    d.cfl_e.upload(dcf.data(), dcf.size());
+    d.cfl_noc.upload(noc.data(), noc.size());
+  }
+  std::vector<int> bcell(std::max(1, nfa_b), 0);
+  for (int f = 0; f < nfa_b; f++) bcell[f] = inv[m.cvofa[2 * (size_t)f]];
+  d.bfa_zone.upload(bz.data(), bz.size());
+  d.bfa_cell.upload(bcell.data(), bcell.size());
+  if (nfa_b > 0)
     d.bfa_lsg.upload(m.bfa_lsg.data(), m.bfa_lsg.size());
   else
     d.bfa_lsg.alloc(3);
-  d.vol.upload(m.cv_volume.data(), m.cv_volume.size());
+  {
+    std::vector<double> v(ncv);
+    for (int cd = 0; cd < ncv; cd++) v[cd] = m.cv_volume[perm[cd]];
+    d.vol.upload(v.data(), v.size());
+  }
+  std::vector<int> ident(ncv);
+  for (int c = 0; c < ncv; c++) ident[c] = c;
+  d.sched.upload(ident.data(), ident.size());
+  d.h_ref_nbocv_i = m.nbocv_i;
 
   // state buffers; prim carries x_cv in its last three slots
-  const size_t n = m.ncv;
+  const size_t n = ncv;
   d.q.alloc(5 * n);
   std::vector<double> pr(8 * n, 0.0);
-  for (size_t c = 0; c < n; c++)
-    for (int i = 0; i < 3; i++) pr[8 * c + 5 + i] = m.x_cv[3 * c + i];
+  for (size_t cd = 0; cd < n; cd++)
+    for (int i = 0; i < 3; i++) pr[8 * cd + 5 + i] = m.x_cv[3 * (size_t)perm[cd] + i];
   d.prim.upload(pr.data(), pr.size());
   d.grad.alloc(16 * n);
   HIPCHECK(hipMemset(d.grad.p, 0, sizeof(double) * 16 * n));
-  d.bfa.alloc(8 * (size_t)std::max(1, m.nfa_b));
+  d.bfa.alloc(8 * (size_t)std::max(1, nfa_b));
   d.rhs.alloc(5 * n);
   d.dt.alloc(n);
-  d.hook.alloc(5 * (size_t)std::max(1, m.nfa_b));
+  d.hook.alloc(5 * (size_t)std::max(1, nfa_b));
   HIPCHECK(hipMemset(d.hook.p, 0, d.hook.n * sizeof(double)));
   d.counters.alloc(4);
   HIPCHECK(hipMemset(d.counters.p, 0, 4 * sizeof(int)));
@@ -1354,9 +1462,9 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
   HIPCHECK(hipMemset(d.scal.p, 0, sizeof(SolverScalars)));
   HIPCHECK(hipHostMalloc((void **)&d.h_scal, sizeof(SolverScalars)));
 
-  d.dm.ncv = m.ncv;
-  d.dm.nfa = m.nfa;
-  d.dm.nfa_b = m.nfa_b;
+  d.dm.ncv = ncv;
+  d.dm.nfa = nfa;
+  d.dm.nfa_b = nfa_b;
   d.dm.nnz = d.nnz;
   d.dm.fgeo = d.fgeo.p;
   d.dm.fnorm = d.fnorm.p;
@@ -1397,12 +1505,13 @@ void Ctx::set_state(const double *rho, const double *rhou, const double *rhoE) {
   Impl &d = *d_;
   const size_t n = d.ncv;
   std::vector<double> h(5 * n);
-  for (size_t c = 0; c < n; c++) {
-    h[c] = rho[c];
-    h[n + c] = rhou[3 * c];
-    h[2 * n + c] = rhou[3 * c + 1];
-    h[3 * n + c] = rhou[3 * c + 2];
-    h[4 * n + c] = rhoE[c];
+  for (size_t cd = 0; cd < n; cd++) {
+    const size_t c = d.perm[cd];
+    h[cd] = rho[c];
+    h[n + cd] = rhou[3 * c];
+    h[2 * n + cd] = rhou[3 * c + 1];
+    h[3 * n + cd] = rhou[3 * c + 2];
+    h[4 * n + cd] = rhoE[c];
   }
   HIPCHECK(hipMemcpyAsync(d.q.p, h.data(), sizeof(double) * 5 * n, hipMemcpyHostToDevice, d.stream));
   HIPCHECK(hipStreamSynchronize(d.stream));
@@ -1417,12 +1526,13 @@ void Ctx::get_state(double *rho, double *rhou, double *rhoE) {
   std::vector<double> h(5 * n);
   HIPCHECK(hipMemcpyAsync(h.data(), d.q.p, sizeof(double) * 5 * n, hipMemcpyDeviceToHost, d.stream));
   HIPCHECK(hipStreamSynchronize(d.stream));
-  for (size_t c = 0; c < n; c++) {
-    rho[c] = h[c];
-    rhou[3 * c] = h[n + c];
-    rhou[3 * c + 1] = h[2 * n + c];
-    rhou[3 * c + 2] = h[3 * n + c];
-    rhoE[c] = h[4 * n + c];
+  for (size_t cd = 0; cd < n; cd++) {
+    const size_t c = d.perm[cd];
+    rho[c] = h[cd];
+    rhou[3 * c] = h[n + cd];
+    rhou[3 * c + 1] = h[2 * n + cd];
+    rhou[3 * c + 2] = h[3 * n + cd];
+    rhoE[c] = h[4 * n + cd];
   }
 }
 
@@ -1717,49 +1827,54 @@ int64_t Ctx::get(const std::string &name, double *out) {
   Impl &d = *d_;
   HIPCHECK(hipStreamSynchronize(d.stream));
   const size_t n = d.ncv, nb = d.nfa_b;
-  auto copy = [&](const void *src, size_t bytes, std::vector<double> &h) {
+  const std::vector<int> &perm = d.perm;
+  std::vector<double> h;
+  auto copy = [&](const void *src, size_t bytes) {
     h.resize(bytes / sizeof(double));
     HIPCHECK(hipMemcpy(h.data(), src, bytes, hipMemcpyDeviceToHost));
   };
-  std::vector<double> h;
-  auto soa = [&](const double *src, int comp, int ncomp) -> int64_t {  // [ncomp][n] -> component
+  // SoA [5][n] device-ordered component(s) -> reference-ordered AoS
+  auto soa = [&](int comp0, int count) -> int64_t {
     if (out) {
-      copy(src + (size_t)comp * n, n * sizeof(double), h);
-      std::memcpy(out, h.data(), n * sizeof(double));
+      copy(d.rhs.p, 5 * n * sizeof(double));
+      for (size_t cd = 0; cd < n; cd++)
+        for (int i = 0; i < count; i++) out[(size_t)perm[cd] * count + i] = h[(comp0 + i) * n + cd];
     }
-    (void)ncomp;
-    return (int64_t)n;
+    return (int64_t)(n * count);
   };
-  auto rec = [&](const double *src, size_t nrec, int stride, int first, int count) -> int64_t {
+  // per-cell records in device order -> reference order
+  auto cellrec = [&](const double *src, int stride, int first, int count) -> int64_t {
     if (out) {
-      copy(src, nrec * stride * sizeof(double), h);
-      for (size_t r = 0; r < nrec; r++)
-        for (int k = 0; k < count; k++) out[r * count + k] = h[r * stride + first + k];
+      copy(src, n * stride * sizeof(double));
+      for (size_t cd = 0; cd < n; cd++)
+        for (int k = 0; k < count; k++) out[(size_t)perm[cd] * count + k] = h[cd * stride + first + k];
     }
-    return (int64_t)(nrec * count);
+    return (int64_t)(n * count);
   };
-  if (name == "RHS_RHO") return soa(d.rhs.p, 0, 5);
-  if (name == "RHS_RHOE") return soa(d.rhs.p, 4, 5);
-  if (name == "RHS_RHOU") {
+  // boundary-face records keep the reference face index
+  auto bfarec = [&](int first, int count) -> int64_t {
     if (out) {
-      copy(d.rhs.p, 5 * n * sizeof(double), h);
-      for (size_t c = 0; c < n; c++)
-        for (int i = 0; i < 3; i++) out[3 * c + i] = h[(1 + i) * n + c];
+      copy(d.bfa.p, nb * 8 * sizeof(double));
+      for (size_t f = 0; f < nb; f++)
+        for (int k = 0; k < count; k++) out[f * count + k] = h[f * 8 + first + k];
     }
-    return (int64_t)(3 * n);
-  }
-  if (name == "LOCAL_DT") return rec(d.dt.p, n, 1, 0, 1);
-  if (name == "PRESS") return rec(d.prim.p, n, 8, 4, 1);
-  if (name == "VEL") return rec(d.prim.p, n, 8, 1, 3);
-  if (name == "GRAD_RHO") return rec(d.grad.p, n, 16, 0, 3);
-  if (name == "GRAD_P") return rec(d.grad.p, n, 16, 3, 3);
-  if (name == "GRAD_U") return rec(d.grad.p, n, 16, 6, 9);
-  if (name == "ALPHA_RHO") return rec(d.grad.p, n, 16, 15, 1);
-  if (name == "RHO_BFA") return rec(d.bfa.p, nb, 8, 0, 1);
-  if (name == "VEL_BFA") return rec(d.bfa.p, nb, 8, 1, 3);
-  if (name == "P_BFA") return rec(d.bfa.p, nb, 8, 4, 1);
-  if (name == "T_BFA") return rec(d.bfa.p, nb, 8, 5, 1);
-  if (name == "H_BFA") return rec(d.bfa.p, nb, 8, 6, 1);
+    return (int64_t)(nb * count);
+  };
+  if (name == "RHS_RHO") return soa(0, 1);
+  if (name == "RHS_RHOU") return soa(1, 3);
+  if (name == "RHS_RHOE") return soa(4, 1);
+  if (name == "LOCAL_DT") return cellrec(d.dt.p, 1, 0, 1);
+  if (name == "PRESS") return cellrec(d.prim.p, 8, 4, 1);
+  if (name == "VEL") return cellrec(d.prim.p, 8, 1, 3);
+  if (name == "GRAD_RHO") return cellrec(d.grad.p, 16, 0, 3);
+  if (name == "GRAD_P") return cellrec(d.grad.p, 16, 3, 3);
+  if (name == "GRAD_U") return cellrec(d.grad.p, 16, 6, 9);
+  if (name == "ALPHA_RHO") return cellrec(d.grad.p, 16, 15, 1);
+  if (name == "RHO_BFA") return bfarec(0, 1);
+  if (name == "VEL_BFA") return bfarec(1, 3);
+  if (name == "P_BFA") return bfarec(4, 1);
+  if (name == "T_BFA") return bfarec(5, 1);
+  if (name == "H_BFA") return bfarec(6, 1);
   if (name == "RESID") {
     if (out) {
       HIPCHECK(hipMemcpy(d.h_scal, d.scal.p, sizeof(SolverScalars), hipMemcpyDeviceToHost));
@@ -1776,22 +1891,38 @@ int64_t Ctx::get(const std::string &name, double *out) {
     return 4;
   }
   if (name == "A" || name == "DQ") {
-    bool isA = name == "A";
-    size_t cnt = isA ? 25 * (size_t)d.nnz : 5 * n;
+    const bool isA = name == "A";
+    const size_t cnt = isA ? 25 * (size_t)d.nnz : 5 * n;
+    if (!out) return (int64_t)cnt;
+    std::vector<double> dev(cnt);
     if (d.precision == 0) {
       const double *src = isA ? d.A64.p : d.v64[1].p;
       if (!src) throw std::logic_error(name + " not computed yet");
-      if (out) HIPCHECK(hipMemcpy(out, src, cnt * sizeof(double), hipMemcpyDeviceToHost));
+      HIPCHECK(hipMemcpy(dev.data(), src, cnt * sizeof(double), hipMemcpyDeviceToHost));
     } else {
       const float *src = isA ? d.A32.p : d.v32[1].p;
       if (!src) throw std::logic_error(name + " not computed yet");
-      if (out) {
-        std::vector<float> f(cnt);
-        HIPCHECK(hipMemcpy(f.data(), src, cnt * sizeof(float), hipMemcpyDeviceToHost));
-        for (size_t k = 0; k < cnt; k++) out[k] = f[k];
+      std::vector<float> f(cnt);
+      HIPCHECK(hipMemcpy(f.data(), src, cnt * sizeof(float), hipMemcpyDeviceToHost));
+      for (size_t k = 0; k < cnt; k++) dev[k] = f[k];
+    }
+    if (isA) {  // device rows -> reference nbocv rows (same order inside a row)
+      for (size_t cd = 0; cd < n; cd++) {
+        const size_t c = perm[cd];
+        const size_t len = d.h_dev_nbocv_i[cd + 1] - d.h_dev_nbocv_i[cd];
+        std::memcpy(out + 25 * (size_t)d.h_ref_nbocv_i[c], dev.data() + 25 * (size_t)d.h_dev_nbocv_i[cd],
+                    25 * len * sizeof(double));
       }
+    } else {
+      for (size_t cd = 0; cd < n; cd++)
+        for (int k = 0; k < 5; k++) out[5 * (size_t)perm[cd] + k] = dev[5 * cd + k];
     }
     return (int64_t)cnt;
+  }
+  if (name == "CELL_ORDER") {  // device position of each reference cell
+    if (out)
+      for (size_t cd = 0; cd < n; cd++) out[perm[cd]] = (double)cd;
+    return (int64_t)n;
   }
   throw std::invalid_argument("unknown array " + name);
 }

@@ -789,13 +789,29 @@ void grad_vector_ls(or_ctx &c, double *g, const double *u, const double *u_fa) {
   }
 }
 
-// Green-Gauss gradients (w0 = w1 = 0.5), UgpWithCv.h:1091-1188
+// Green-Gauss gradients, UgpWithCv.h:1091-1188. The reference defines
+// GG_WITH_DIST (UgpWithCv.h:37): face values are weighted by the distances
+// from the face centroid to the two cell centroids (:1104-1110).
+static void gg_weights(const or_mesh &m, int ifa, int icv0, int icv1, double &w0, double &w1) {
+  double dx0[3], dx1[3];
+  for (int i = 0; i < 3; i++) {
+    dx0[i] = m.x_fa[3 * ifa + i] - m.x_cv[3 * icv0 + i];
+    dx1[i] = m.x_fa[3 * ifa + i] - m.x_cv[3 * icv1 + i];
+  }
+  double s0 = 0.0, s1 = 0.0;
+  for (int i = 0; i < 3; i++) s0 += dx0[i] * dx0[i];
+  for (int i = 0; i < 3; i++) s1 += dx1[i] * dx1[i];
+  w0 = sqrt(s0);
+  w1 = sqrt(s1);
+}
+
 void grad_scalar_gg(or_ctx &c, double *g, const double *p, const double *p_fa) {
   const or_mesh &m = c.m;
   for (int k = 0; k < 3 * c.ncv; k++) g[k] = 0.0;
   for (int ifa = c.nfa_b; ifa < c.nfa; ifa++) {
     int icv0 = m.cvofa[2 * ifa], icv1 = m.cvofa[2 * ifa + 1];
-    double w0 = 0.5, w1 = 0.5;
+    double w0, w1;
+    gg_weights(m, ifa, icv0, icv1, w0, w1);
     double faceValue = (w1 * p[icv0] + w0 * p[icv1]) / (w0 + w1);
     for (int i = 0; i < 3; i++) {
       g[3 * icv0 + i] += faceValue * m.fa_normal[3 * ifa + i];
@@ -815,7 +831,8 @@ void grad_vector_gg(or_ctx &c, double *g, const double *u, const double *u_fa) {
   for (int k = 0; k < 9 * c.ncv; k++) g[k] = 0.0;
   for (int ifa = c.nfa_b; ifa < c.nfa; ifa++) {
     int icv0 = m.cvofa[2 * ifa], icv1 = m.cvofa[2 * ifa + 1];
-    double w0 = 0.5, w1 = 0.5;
+    double w0, w1;
+    gg_weights(m, ifa, icv0, icv1, w0, w1);
     for (int i = 0; i < 3; i++) {
       double faceValue = (w1 * u[3 * icv0 + i] + w0 * u[3 * icv1 + i]) / (w0 + w1);
       for (int j = 0; j < 3; j++) {

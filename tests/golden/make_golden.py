@@ -1,0 +1,109 @@
+"""Writes the golden vectors under tests/golden/ by running the reference's
+own build (oracle/_ref/joe_ref_dump, made by `make -C oracle ref` from the
+mounted reference sources) on meshes from myjoe_amd.meshgen.
+
+    python tests/golden/make_golden.py
+
+Fixtures (numpy .npz, inputs and the reference's outputs only):
+  mesh_<name>.npz      reference numbering, connectivity, geometry, LS
+                       coefficients and zones after Ugp/UgpWithCv init
+  rhs_<name>_<sch>.npz random state, primitive and boundary states, local dt,
+                       limited gradients, residual, Jacobian and one BiCGStab dq
+  solve_<name>.npz     JoeWithModels::run() state snapshots (steps 1..5) and
+                       the final state after NSTEPS
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from myjoe_amd import cases, meshgen  # noqa: E402
+from oracle import jmdump  # noqa: E402
+
+REF = os.path.join(ROOT, "oracle", "_ref", "joe_ref_dump")
+MESH_KEYS = ["COUNTS", "X_NO", "NOOFA_I", "NOOFA_V", "CVOFA", "FAOCV_I", "FAOCV_V", "NBOCV_I", "NBOCV_V", "X_FA",
+             "FA_NORMAL", "X_CV", "CV_VOLUME", "NBOCV_LSG", "BFA_LSG", "ZONES", "ZONE_NAMES"]
+RHS_KEYS = ["PRESS", "VEL", "RHO_BFA", "VEL_BFA", "P_BFA", "T_BFA", "H_BFA", "LOCAL_DT", "GRAD_RHO", "GRAD_P",
+            "GRAD_U", "ALPHA_RHO", "RHS_RHO", "RHS_RHOU", "RHS_RHOE", "A", "DQ"]
+
+MESHES = {
+    "bump24": lambda scheme, **kw: cases.bump_case(24, 8, scheme=scheme, **kw),
+    "box4": lambda scheme, **kw: cases.box_case(4, seed=3, scheme=scheme, **kw),
+}
+RHS_CASES = [("bump24", "HLLC", {}), ("bump24", "ROE", {}), ("bump24", "AUSM", {}),
+             ("box4", "HLLC", {}), ("box4", "ROE", {}), ("box4", "AUSM", {}),
+             ("box4", "HLLC", {"grad": "GREENGAUSS"}), ("box4", "ROE", {"second_order": False}),
+             ("box4", "HLLC", {"bcs": cases.BOX_BCS_ALT, "symm_bc": "NEW_NUMERIC"}),
+             ("box4", "ROE", {"bcs": cases.BOX_BCS_ALT, "symm_bc": "ANALYTICAL"}),
+             ("box4", "AUSM", {"bcs": cases.BOX_BCS_ALT, "symm_bc": "ANALYTICAL", "grad": "GREENGAUSS"})]
+
+
+def run_ref(args, cwd):
+    r = subprocess.run([REF, *args], cwd=cwd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"reference failed ({' '.join(args)}):\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
+
+
+def tag(kw):
+    t = ""
+    if "bcs" in kw:
+        t += "_alt_" + {"NEW_NUMERIC": "new", "ANALYTICAL": "ana"}[kw["symm_bc"]]
+    if kw.get("grad") == "GREENGAUSS":
+        t += "_gg"
+    if kw.get("second_order") is False:
+        t += "_first"
+    return t
+
+
+def main():
+    if not os.path.exists(REF):
+        raise SystemExit("oracle/_ref/joe_ref_dump is not built (make -C oracle ref, needs the reference tree)")
+    with tempfile.TemporaryDirectory() as wd:
+        done_mesh = set()
+        for name, scheme, kw in RHS_CASES:
+            mm, joe = MESHES[name](scheme, **kw)
+            fn = "box.msh" if name.startswith("box") else "bump.msh"
+            meshgen.write_msh(mm, os.path.join(wd, fn))
+            with open(os.path.join(wd, "Joe.in"), "w") as f:
+                f.write(joe)
+            rho, rhou, rhoE = cases.random_state(mm.ncv, seed=5)
+            jmdump.write(os.path.join(wd, "state.bin"), {"RHO": rho, "RHOU": rhou.reshape(-1), "RHOE": rhoE})
+            run_ref(["rhs", "Joe.in", "state.bin", "out.bin"], wd)
+            d = jmdump.read(os.path.join(wd, "out.bin"))
+            if name not in done_mesh:
+                np.savez_compressed(os.path.join(HERE, f"mesh_{name}.npz"), **{k: d[k] for k in MESH_KEYS})
+                done_mesh.add(name)
+            out = {k: d[k] for k in RHS_KEYS if k in d}
+            out.update(RHO=rho, RHOU=rhou.reshape(-1), RHOE=rhoE, JOE_IN=np.frombuffer(joe.encode(), np.uint8))
+            np.savez_compressed(os.path.join(HERE, f"rhs_{name}_{scheme.lower()}{tag(kw)}.npz"), **out)
+            print("wrote", name, scheme, kw)
+
+        # steady-loop trajectories of JoeWithModels::run (uniform initial state)
+        for name, nx, ny, nsteps in [("bump24", 24, 8, 60), ("bump96", 96, 32, 300)]:
+            mm, joe = cases.bump_case(nx, ny, nsteps=nsteps)
+            meshgen.write_msh(mm, os.path.join(wd, "bump.msh"))
+            with open(os.path.join(wd, "Joe.in"), "w") as f:
+                f.write(joe)
+            run_ref(["solve", "Joe.in", "solve.bin", "5"], wd)
+            d = jmdump.read(os.path.join(wd, "solve.bin"))
+            keep = {k: v for k, v in d.items() if k.split("@")[0] in ("RHO", "RHOU", "RHOE", "STEP")
+                    and (k.endswith(("@1", "@2", "@3", "@4", "@5")) or "@" not in k)}
+            keep["JOE_IN"] = np.frombuffer(joe.encode(), np.uint8)
+            np.savez_compressed(os.path.join(HERE, f"solve_{name}.npz"), **keep)
+            if name == "bump96":
+                run_ref(["init", "Joe.in", "init.bin"], wd)
+                dm = jmdump.read(os.path.join(wd, "init.bin"))
+                np.savez_compressed(os.path.join(HERE, "mesh_bump96.npz"), **{k: dm[k] for k in MESH_KEYS})
+            print("wrote solve", name, "steps", int(d["STEP"][0]))
+
+
+if __name__ == "__main__":
+    main()

@@ -15,6 +15,10 @@ CASES = {
     "box5_ausm": lambda: cases.box_case(5, seed=3, scheme="AUSM"),
     "box5_gg_hllc": lambda: cases.box_case(5, seed=3, scheme="HLLC", grad="GREENGAUSS"),
     "box5_first_roe": lambda: cases.box_case(5, seed=3, scheme="ROE", second_order=False),
+    "box5_alt_new_hllc": lambda: cases.box_case(5, seed=3, scheme="HLLC", bcs=cases.BOX_BCS_ALT,
+                                                symm_bc="NEW_NUMERIC"),
+    "box5_alt_ana_gg_ausm": lambda: cases.box_case(5, seed=3, scheme="AUSM", bcs=cases.BOX_BCS_ALT,
+                                                   symm_bc="ANALYTICAL", grad="GREENGAUSS"),
 }
 
 

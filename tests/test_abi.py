@@ -1,0 +1,91 @@
+"""The C ABI of libjoemi without a GPU: exports, argument checking, Joe.in
+errors and the no-device error. Compute entry points are not called here."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from myjoe_amd import cases, joemi, meshgen
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "joemi.h")
+
+
+def declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(jm_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_every_declared_symbol_is_exported():
+    names = declared()
+    assert len(names) >= 24
+    lib = C.CDLL(joemi.LIB_PATH)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_version_and_last_error():
+    L = joemi.lib()
+    assert L.jm_version().decode().startswith("joemi")
+    assert isinstance(L.jm_last_error(), bytes)
+
+
+def test_null_arguments_rejected():
+    L = joemi.lib()
+    assert L.jm_mesh_read_msh(None, None) == 1
+    assert L.jm_ctx_create(None, b"", 0, None, None) == 1
+    assert L.jm_mesh_counts(None, None) == 1
+
+
+def test_missing_msh_file():
+    with pytest.raises(joemi.JoemiError) as e:
+        joemi.Mesh.read_msh("/nonexistent/none.msh")
+    assert e.value.status == 2
+
+
+def test_bad_connectivity_rejected():
+    mm = meshgen.split_box(2, 0.0, 0.0, 0)
+    mm.face_cells = mm.face_cells.copy()
+    mm.face_cells[0, 0] = mm.ncv + 5  # cell index out of range
+    with pytest.raises(joemi.JoemiError) as e:
+        joemi.Mesh.from_faces(mm)
+    assert e.value.status in (1, 2)
+
+
+@pytest.mark.parametrize("edit,needle", [
+    (lambda s: s.replace("wall = WALL\n", ""), "wall"),
+    (lambda s: s.replace("SPACE_INTEGRATION = HLLC", "SPACE_INTEGRATION = CENTRAL"), "SPACE_INTEGRATION"),
+    (lambda s: s.replace("CBC 0.5 0.1 0.0 1.0 1.0", "CBC 0.5 0.1"), "inlet"),
+    (lambda s: s.replace("P_REF = 1.0", ""), "P_REF"),
+])
+def test_joe_in_errors_before_device(edit, needle):
+    mm, joe = cases.box_case(2, seed=0)
+    mesh = joemi.Mesh.from_faces(mm)
+    with pytest.raises(joemi.JoemiError) as e:
+        joemi.Solver(mesh, edit(joe))
+    assert e.value.status in (3, 5), str(e.value)
+    assert needle in str(e.value)
+
+
+def test_no_device_is_an_error_not_a_fallback():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    mm, joe = cases.box_case(2, seed=0)
+    mesh = joemi.Mesh.from_faces(mm)
+    with pytest.raises(joemi.JoemiError) as e:
+        joemi.Solver(mesh, joe)
+    assert e.value.status == 4
+
+
+def test_mesh_arrays_round_trip_counts():
+    mm = meshgen.split_box(3, 0.5, 0.2, 1)
+    mesh = joemi.Mesh.from_faces(mm)
+    assert mesh.ncv == mm.ncv
+    assert mesh.array("CVOFA").size == 2 * mesh.nfa
+    assert mesh.array("NBOCV_V").size == mesh.nnz
+    assert np.all(np.diff(mesh.array("NBOCV_I")) >= 1)

@@ -1,0 +1,108 @@
+"""The CPU restatement (oracle/) against the reference's own outputs.
+
+Every fixture under tests/golden/ was written by the reference's code
+(oracle/_ref/joe_ref_dump, built from the reference sources by
+tests/golden/make_golden.py). Both are serial, FP64 and compiled without
+floating-point contraction, so the restatement is held to bit-exact equality
+on every field: primitive and boundary states, local time steps, limited
+gradients, residuals, Jacobian blocks, one BiCGStab correction and the
+state trajectory of JoeWithModels::run.
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from myjoe_amd import cases
+from oracle.oracle import Oracle, parse_joe_in
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+RHS_FIXTURES = sorted(os.path.basename(p)[4:-4] for p in glob.glob(os.path.join(GOLD, "rhs_*.npz")))
+FIELDS = ["PRESS", "VEL", "RHO_BFA", "VEL_BFA", "P_BFA", "T_BFA", "H_BFA", "LOCAL_DT", "GRAD_RHO", "GRAD_P",
+          "GRAD_U", "ALPHA_RHO", "RHS_RHO", "RHS_RHOU", "RHS_RHOE", "A"]
+
+
+def load(name):
+    with np.load(os.path.join(GOLD, name + ".npz"), allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def mesh_for(fixture):
+    return load("mesh_" + fixture.split("_")[0])
+
+
+def joe_text(d):
+    return bytes(d["JOE_IN"]).decode()
+
+
+def test_fixture_set_complete():
+    # three schemes, both gradient schemes, first order, every BC kind and SYMM_BC mode
+    assert len(RHS_FIXTURES) >= 11
+    kinds = set()
+    symm = set()
+    for f in RHS_FIXTURES:
+        P = parse_joe_in(joe_text(load("rhs_" + f)))
+        for z in ("inlet", "outlet", "wall", "farfield", "symm", "bump_wall", "top", "zsym"):
+            if z in P:
+                kinds.add(P[z][0][1])
+        symm.add(P["SYMM_BC"][0][1])
+    assert kinds >= {"CBC", "CBC_SUBSONIC_INLET", "CBC_SUBSONIC_OUTLET", "WALL", "FARFIELD", "NEUMANN", "SYMMETRY"}
+    assert symm == {"ORIGINAL_NUMERIC", "NEW_NUMERIC", "ANALYTICAL"}
+
+
+@pytest.mark.parametrize("fixture", RHS_FIXTURES)
+def test_residual_bit_exact(fixture):
+    g = load("rhs_" + fixture)
+    o = Oracle(mesh_for(fixture), joe_text(g))
+    o.set_state(g["RHO"], g["RHOU"], g["RHOE"])
+    o.residual(True)
+    first = "first" in fixture
+    for k in FIELDS:
+        if k not in g:
+            assert first and k.startswith(("GRAD", "ALPHA")), k
+            continue
+        got = o.get(k)
+        assert got.shape == g[k].shape, k
+        nbad = int(np.sum(got != g[k]))
+        assert nbad == 0, f"{fixture}:{k}: {nbad} of {got.size} differ; max |d| {np.max(np.abs(got - g[k])):.3e}"
+    o.linear_solve()
+    assert np.array_equal(o.get("DQ"), g["DQ"]), fixture
+
+
+@pytest.mark.parametrize("name", ["bump24", "bump96"])
+def test_run_trajectory_bit_exact(name):
+    g = load("solve_" + name)
+    joe = joe_text(g)
+    P = parse_joe_in(joe)
+    m = load("mesh_" + name)
+    o = Oracle(m, joe)
+    ncv = int(m["COUNTS"][3])
+    # JoeWithModels::initialHook uniform state (JoeWithModels.cpp:180-205)
+    u = [float(x) for x in P["U_INITIAL"][0][1:4]]
+    rho0, p0, gam = 1.0, 1.0, 1.4
+    uu = 0.0
+    for x in u:
+        uu += x * x
+    o.set_state(np.full(ncv, rho0), np.tile(np.array(u) * rho0, (ncv, 1)),
+                np.full(ncv, p0 / (gam - 1.0) + 0.5 * rho0 * uu))
+    for k in range(1, 6):
+        o.step()
+        rho, rhou, rhoE = o.state()
+        assert np.array_equal(rho, g[f"RHO@{k}"]), (name, k)
+        assert np.array_equal(rhou.reshape(-1), g[f"RHOU@{k}"].reshape(-1)), (name, k)
+        assert np.array_equal(rhoE, g[f"RHOE@{k}"]), (name, k)
+    nsteps = int(g["STEP"][0])
+    n, _ = o.run(nsteps - 5)
+    assert n == nsteps - 5
+    rho, rhou, rhoE = o.state()
+    assert np.array_equal(rho, g["RHO"]) and np.array_equal(rhoE, g["RHOE"]), name
+    assert np.array_equal(rhou.reshape(-1), g["RHOU"].reshape(-1)), name
+
+
+def test_uniform_state_matches_case_definition():
+    # the initial state used above is the one the case text asks for
+    _, joe = cases.bump_case(24, 8)
+    P = parse_joe_in(joe)
+    assert float(P["U_INITIAL"][0][1]) == cases.U_BUMP
+    assert "RHO_INITIAL" not in P and "P_INITIAL" not in P
