@@ -97,7 +97,8 @@ jm_status jm_ctx_create(const jm_mesh *m, const char *joe_in, int precision, voi
 void jm_ctx_destroy(jm_ctx *c);
 
 /* conservative state in the reference layout (rho[ncv], rhou[ncv][3],
- * rhoE[ncv]); stands in for writing the registered RHO/RHOU/RHOE arrays */
+ * rhoE[ncv]); stands in for writing the registered RHO/RHOU/RHOE arrays.
+ * Setting the state starts a new solve: the step counter and CFL restart. */
 jm_status jm_set_state(jm_ctx *c, const double *rho, const double *rhou, const double *rhoE);
 jm_status jm_get_state(jm_ctx *c, double *rho, double *rhou, double *rhoE);
 /* JoeWithModels::initialHook uniform initial condition (JoeWithModels.cpp:156-186) */
@@ -126,6 +127,27 @@ typedef struct {
   double resid[5];     /* L1 residuals of rho, rhou(3), rhoE */
 } jm_step_info;
 jm_status jm_step(jm_ctx *c, jm_step_info *info);
+
+/* The same step split where the reference calls its hooks, for a host whose
+ * JoeWithModels subclass overrides sourceHook or boundaryHook:
+ *   jm_step_begin  CFL ramp, calcDt, calcRhs with the Jacobian
+ *                  (JoeWithModels.cpp:975-1005); with JM_HOST_SOURCE the
+ *                  default sourceHook (gravity, JoeWithModels.h:111-129) is
+ *                  not applied on the device because the host's replaces it;
+ *   jm_add_source  the host sourceHook's terms, reference numbering:
+ *                  rhs_rho[ncv], rhs_rhou[ncv][3], rhs_rhoE[ncv] added to the
+ *                  residual and a_diag[ncv][5][5] (the A[nbocv_i[icv]] blocks)
+ *                  added to the diagonal blocks; any pointer may be NULL;
+ *   jm_step_end    A_diag += V/dt, relaxation, BiCGStab, update, state and
+ *                  setBC refresh, L1 residual (:1011-1136).
+ * After new HOOK boundary values (jm_set_boundary_state), jm_update_bc
+ * re-runs calcStateVariables + setBC (:3857-4300) on the resident state. */
+enum { JM_HOST_SOURCE = 1 };                    /* jm_step_begin flags */
+jm_status jm_step_begin(jm_ctx *c, int flags);
+jm_status jm_add_source(jm_ctx *c, const double *rhs_rho, const double *rhs_rhou, const double *rhs_rhoE,
+                        const double *a_diag);
+jm_status jm_step_end(jm_ctx *c, jm_step_info *info);
+jm_status jm_update_bc(jm_ctx *c);
 
 /* JoeWithModels::run's steady loop until doneSolver (JoeWithModels.h:42-95):
  * NSTEPS, RESID_ENERG_TH or NaN. resid_hist may be NULL, else [max_steps][5]. */

@@ -51,7 +51,25 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([hipcc, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp, *objs], verbose)
     os.replace(tmp, LIB)
+    build_host(verbose)
     return LIB
+
+
+def build_host(verbose: bool = False) -> None:
+    """The C++ JoeWithModels mirror (libjoemi_host.a, host code over the C
+    ABI only) and the example case program bin/joe_cases linked against it."""
+    flags = ["-O2", "-std=c++17", "-Wall", "-Wno-pmf-conversions", "-fPIC"]
+    o = os.path.join(BUILD, "JoeWithModels.o")
+    _run(["g++", *flags, "-c", os.path.join(HERE, "host", "JoeWithModels.cpp"), "-o", o], verbose)
+    ar = os.path.join(HERE, "libjoemi_host.a")
+    if os.path.exists(ar):
+        os.remove(ar)
+    _run(["ar", "rcs", ar, o], verbose)
+    os.makedirs(os.path.join(HERE, "bin"), exist_ok=True)
+    exe = os.path.join(HERE, "bin", "joe_cases")
+    _run(["g++", *flags, os.path.join(HERE, "host", "cases", "joe_cases.cpp"), "-o", exe + ".tmp", ar,
+          f"-L{HERE}", "-ljoemi", "-Wl,-rpath,$ORIGIN/.."], verbose)
+    os.replace(exe + ".tmp", exe)
 
 
 if __name__ == "__main__":

@@ -243,18 +243,53 @@ jm_status jm_get(jm_ctx *c, const char *name, double *out, int64_t *count) {
   });
 }
 
+static void fill_info(const jm::StepInfo &s, jm_step_info *info) {
+  if (!info) return;
+  info->step = s.step;
+  info->lin_iters = s.lin_iters;
+  info->lin_status = s.lin_status;
+  info->cfl = s.cfl;
+  info->dt_min = s.dt_min;
+  for (int k = 0; k < 5; k++) info->resid[k] = s.resid[k];
+}
+
+jm_status jm_step_begin(jm_ctx *c, int flags) {
+  return guard([&] {
+    if (!c || (flags & ~JM_HOST_SOURCE)) throw std::exception();
+    c->c->step_begin((flags & JM_HOST_SOURCE) != 0);
+    return JM_OK;
+  });
+}
+
+jm_status jm_add_source(jm_ctx *c, const double *rhs_rho, const double *rhs_rhou, const double *rhs_rhoE,
+                        const double *a_diag) {
+  return guard([&] {
+    if (!c) throw std::exception();
+    c->c->add_source(rhs_rho, rhs_rhou, rhs_rhoE, a_diag);
+    return JM_OK;
+  });
+}
+
+jm_status jm_step_end(jm_ctx *c, jm_step_info *info) {
+  return guard([&] {
+    if (!c) throw std::exception();
+    fill_info(c->c->step_end(), info);
+    return JM_OK;
+  });
+}
+
+jm_status jm_update_bc(jm_ctx *c) {
+  return guard([&] {
+    if (!c) throw std::exception();
+    c->c->update_bc();
+    return JM_OK;
+  });
+}
+
 jm_status jm_step(jm_ctx *c, jm_step_info *info) {
   return guard([&] {
     if (!c) throw std::exception();
-    jm::StepInfo s = c->c->step();
-    if (info) {
-      info->step = s.step;
-      info->lin_iters = s.lin_iters;
-      info->lin_status = s.lin_status;
-      info->cfl = s.cfl;
-      info->dt_min = s.dt_min;
-      for (int k = 0; k < 5; k++) info->resid[k] = s.resid[k];
-    }
+    fill_info(c->c->step(), info);
     return JM_OK;
   });
 }

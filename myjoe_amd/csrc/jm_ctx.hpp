@@ -72,6 +72,18 @@ class Ctx {
   // returns element count; copies when out != nullptr
   int64_t get(const std::string &name, double *out);
   StepInfo step();
+  // the same step in two halves, for hosts whose sourceHook / boundaryHook
+  // run between them: step_begin does the CFL ramp, local time step,
+  // residual and Jacobian (without the V/dt term; the device's own gravity
+  // source is left out when host_source is set); add_source adds the host's
+  // per-cell source terms (reference numbering) to the residual and to the
+  // diagonal blocks; step_end adds V/dt, relaxes, solves, updates, refreshes
+  // state and boundary values and returns the step's residuals.
+  void step_begin(bool host_source);
+  void add_source(const double *rhs_rho, const double *rhs_rhou, const double *rhs_rhoE, const double *a_diag);
+  StepInfo step_end();
+  // calcStateVariables + setBC on the resident state (after new HOOK values)
+  void update_bc();
   int run(int max_steps, double *hist);
   void set_boundary_state(int ifa_f, int nface, const double *T, const double *vel, const double *p);
   double *device_state();

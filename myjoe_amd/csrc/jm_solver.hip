@@ -1003,6 +1003,40 @@ __global__ void __launch_bounds__(kBlock) k_update(int ncv, double *__restrict__
   for (int k = 0; k < 5; k++) q[(size_t)k * ncv + c] += (double)dq[5 * (size_t)c + k];
 }
 
+// sourceHook terms computed by the host, in device order: src[c] holds five
+// residual terms then the 25 terms of the diagonal block (JoeWithModels.h:111).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_add_source(int ncv, const int *__restrict__ nbocv_i,
+                                                       const double *__restrict__ src, int with_a,
+                                                       double *__restrict__ rhs, T *__restrict__ A) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncv) return;
+  const double *s = src + 30 * (size_t)c;
+  for (int k = 0; k < 5; k++) rhs[(size_t)k * ncv + c] += s[k];
+  if (with_a) {
+    T *d = A + 25 * (size_t)nbocv_i[c];
+    for (int k = 0; k < 25; k++) d[k] = (T)((double)d[k] + s[5 + k]);
+  }
+}
+
+// backward-Euler time term and relaxed right-hand side (JoeWithModels.cpp:1011-1024)
+// for a Jacobian assembled without them
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_time_term(int ncv, const int *__restrict__ nbocv_i,
+                                                      const double *__restrict__ vol, const double *__restrict__ dt,
+                                                      double relax, const double *__restrict__ rhs, T *__restrict__ A,
+                                                      T *__restrict__ rhs_ls, T *__restrict__ phi) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncv) return;
+  const double tmp = vol[c] / (dt[c]);
+  T *d = A + 25 * (size_t)nbocv_i[c];
+  for (int i = 0; i < 5; i++) d[6 * i] = (T)((double)d[6 * i] + tmp);
+  for (int k = 0; k < 5; k++) {
+    rhs_ls[5 * (size_t)c + k] = (T)(relax * rhs[(size_t)k * ncv + c]);
+    phi[5 * (size_t)c + k] = (T)0.0;
+  }
+}
+
 // L1 residual sums (JoeWithModels.cpp:1119-1130), finished by k_fin kFinL1
 __global__ void __launch_bounds__(kBlock) k_l1(int ncv, const double *__restrict__ rhs, double *partials) {
   __shared__ double sh[16 * 5];
@@ -1196,6 +1230,10 @@ struct Ctx::Impl {
   DBuf<double> partials;
   DBuf<SolverScalars> scal;
   SolverScalars *h_scal = nullptr;  // pinned
+  // split step (host hooks): host source terms in device order, phase flag
+  DBuf<double> src;
+  std::vector<double> h_src;
+  bool begun = false;
 
   template <typename T>
   void solve(int &iters, int &status);
@@ -1741,6 +1779,7 @@ void Ctx::calc_rhs(bool implicit) {
 StepInfo Ctx::step() {
   Impl &d = *d_;
   if (!d.have_state) throw std::logic_error("no state set");
+  if (d.begun) throw std::logic_error("step called between step_begin and step_end");
   StepInfo info;
   if (d.step_no == 0) d.launch_state_bc();
   d.step_no++;
@@ -1780,6 +1819,102 @@ StepInfo Ctx::step() {
   info.dt_min = d.h_scal->dt_min;
   for (int k = 0; k < 5; k++) info.resid[k] = d.h_scal->resid[k];
   return info;
+}
+
+void Ctx::step_begin(bool host_source) {
+  Impl &d = *d_;
+  if (!d.have_state) throw std::logic_error("no state set");
+  if (d.begun) throw std::logic_error("step_begin called twice without step_end");
+  if (d.step_no == 0) d.launch_state_bc();
+  d.step_no++;
+  if ((d.step_no >= d.par.cfl_after) && (d.step_no % d.par.cfl_interval == 0) && (d.cfl < d.par.cfl_max))
+    d.cfl *= d.par.cfl_factor;
+  hipLaunchKernelGGL(k_dt, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.dm, d.prim.p, d.q.p, d.par.gamma, d.cfl,
+                     d.par.dt_mode, d.par.dt_fixed, d.dt.p, d.partials.p);
+  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, d.stream, nblk(d.ncv), (int)kFinDt, d.partials.p, d.scal.p);
+  if (d.par.dt_mode == 1) hipLaunchKernelGGL(k_dt_fill, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.dt.p, d.scal.p);
+  // a host sourceHook replaces the default one (gravity)
+  const DParams keep = d.dp;
+  if (host_source) d.dp.gravity_on = 0;
+  d.launch_grad_flux();
+  if (d.precision == 0) {
+    ensure(d.A64, 25 * (size_t)d.nnz);
+    ensure(d.LU64, 25 * (size_t)d.ncv);
+    for (int k = 0; k < 10; k++) ensure(d.v64[k], 5 * (size_t)d.ncv);
+    d.launch_jac<double>(false);
+  } else {
+    ensure(d.A32, 25 * (size_t)d.nnz);
+    ensure(d.LU32, 25 * (size_t)d.ncv);
+    for (int k = 0; k < 10; k++) ensure(d.v32[k], 5 * (size_t)d.ncv);
+    d.launch_jac<float>(false);
+  }
+  d.dp = keep;
+  HIPCHECK(hipGetLastError());
+  d.begun = true;
+}
+
+void Ctx::add_source(const double *rhs_rho, const double *rhs_rhou, const double *rhs_rhoE, const double *a_diag) {
+  Impl &d = *d_;
+  if (!d.begun) throw std::logic_error("add_source outside step_begin/step_end");
+  const size_t n = d.ncv;
+  d.h_src.assign(30 * n, 0.0);
+  for (size_t cd = 0; cd < n; cd++) {
+    const size_t c = d.perm[cd];
+    double *o = &d.h_src[30 * cd];
+    o[0] = rhs_rho ? rhs_rho[c] : 0.0;
+    for (int i = 0; i < 3; i++) o[1 + i] = rhs_rhou ? rhs_rhou[3 * c + i] : 0.0;
+    o[4] = rhs_rhoE ? rhs_rhoE[c] : 0.0;
+    if (a_diag)
+      for (int k = 0; k < 25; k++) o[5 + k] = a_diag[25 * c + k];
+  }
+  ensure(d.src, 30 * n);
+  HIPCHECK(hipMemcpyAsync(d.src.p, d.h_src.data(), 30 * n * sizeof(double), hipMemcpyHostToDevice, d.stream));
+  if (d.precision == 0)
+    hipLaunchKernelGGL(k_add_source<double>, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.nbocv_i.p,
+                       d.src.p, a_diag ? 1 : 0, d.rhs.p, d.A64.p);
+  else
+    hipLaunchKernelGGL(k_add_source<float>, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.nbocv_i.p,
+                       d.src.p, a_diag ? 1 : 0, d.rhs.p, d.A32.p);
+  // the host buffer must outlive the copy
+  HIPCHECK(hipStreamSynchronize(d.stream));
+}
+
+StepInfo Ctx::step_end() {
+  Impl &d = *d_;
+  if (!d.begun) throw std::logic_error("step_end without step_begin");
+  d.begun = false;
+  int it = 0, st = 0;
+  if (d.precision == 0) {
+    hipLaunchKernelGGL(k_time_term<double>, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.nbocv_i.p,
+                       d.vol.p, d.dt.p, d.par.under_relax, d.rhs.p, d.A64.p, d.v64[0].p, d.v64[1].p);
+    d.solve<double>(it, st);
+    hipLaunchKernelGGL(k_update<double>, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.q.p, d.v64[1].p);
+  } else {
+    hipLaunchKernelGGL(k_time_term<float>, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.nbocv_i.p,
+                       d.vol.p, d.dt.p, d.par.under_relax, d.rhs.p, d.A32.p, d.v32[0].p, d.v32[1].p);
+    d.solve<float>(it, st);
+    hipLaunchKernelGGL(k_update<float>, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.q.p, d.v32[1].p);
+  }
+  d.launch_state_bc();
+  hipLaunchKernelGGL(k_l1, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.rhs.p, d.partials.p);
+  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, d.stream, nblk(d.ncv), (int)kFinL1, d.partials.p, d.scal.p);
+  HIPCHECK(hipMemcpyAsync(d.h_scal, d.scal.p, sizeof(SolverScalars), hipMemcpyDeviceToHost, d.stream));
+  HIPCHECK(hipStreamSynchronize(d.stream));
+  StepInfo info;
+  info.step = d.step_no;
+  info.lin_iters = it;
+  info.lin_status = st;
+  info.cfl = d.cfl;
+  info.dt_min = d.h_scal->dt_min;
+  for (int k = 0; k < 5; k++) info.resid[k] = d.h_scal->resid[k];
+  return info;
+}
+
+void Ctx::update_bc() {
+  Impl &d = *d_;
+  if (!d.have_state) throw std::logic_error("no state set");
+  d.launch_state_bc();
+  HIPCHECK(hipGetLastError());
 }
 
 // doneSolver (JoeWithModels.h:42-95): NSTEPS, RESID_ENERG_TH, NaN

@@ -1,0 +1,325 @@
+// Host C++ mirror of JoeWithModels over the joemi C ABI; see JoeWithModels.hpp.
+#include "JoeWithModels.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+#include <stdexcept>
+
+namespace {
+
+// Param.cpp:247-298 tokenizer: separators ' ', '=', tab, CR; '#' starts a comment
+std::vector<std::string> tokens(const std::string &line) {
+  std::vector<std::string> out;
+  std::string cur;
+  for (char ch : line) {
+    if (ch == '#') break;
+    if (ch == ' ' || ch == '=' || ch == '\t' || ch == '\r') {
+      if (!cur.empty()) out.push_back(cur);
+      cur.clear();
+    } else {
+      cur.push_back(ch);
+    }
+  }
+  if (!cur.empty()) out.push_back(cur);
+  return out;
+}
+
+bool ends_with(const std::string &s, const std::string &suf) {
+  return s.size() >= suf.size() && s.compare(s.size() - suf.size(), suf.size(), suf) == 0;
+}
+
+}  // namespace
+
+void JoeWithModels::require(jm_status st, const char *what) const {
+  if (st != JM_OK) {
+    std::ostringstream os;
+    os << "joemi: " << what << " failed (status " << (int)st << "): " << jm_last_error();
+    throw std::runtime_error(os.str());
+  }
+}
+
+JoeWithModels::JoeWithModels(const char *inputFile) {
+  std::ifstream f(inputFile);
+  if (!f) throw std::runtime_error(std::string("cannot open input file ") + inputFile);
+  std::stringstream ss;
+  ss << f.rdbuf();
+  joe_in_ = ss.str();
+  std::istringstream is(joe_in_);
+  std::string line;
+  while (std::getline(is, line)) {
+    std::vector<std::string> t = tokens(line);
+    if (!t.empty()) params_[t[0]].push_back(t);
+  }
+  GAMMA = getDoubleParam("GAMMA", "1.4");
+  p_ref = getDoubleParam("P_REF");
+  rho_ref = getDoubleParam("RHO_REF");
+  T_ref = getDoubleParam("T_REF");
+  R_gas = p_ref / rho_ref / T_ref;
+  gravity[0] = getDoubleParam("GX", "0.0");
+  gravity[1] = getDoubleParam("GY", "0.0");
+  gravity[2] = getDoubleParam("GZ", "0.0");
+  nsteps = getIntParam("NSTEPS", "100");
+  resid_energ_th = getDoubleParam("RESID_ENERG_TH", "0.0");
+
+  // mesh: RESTART names a Fluent .msh (MshFilter path of UgpWithCv::init)
+  const std::string restart = getStringParam("RESTART", "restart.out");
+  if (!ends_with(restart, ".msh"))
+    throw std::runtime_error("RESTART = " + restart + ": only Fluent .msh meshes are read by this build");
+  require(jm_mesh_read_msh(restart.c_str(), &mesh_), "jm_mesh_read_msh");
+  int64_t counts[8];
+  require(jm_mesh_counts(mesh_, counts), "jm_mesh_counts");
+  nno = (int)counts[0];
+  nfa = (int)counts[1];
+  nfa_b = (int)counts[2];
+  ncv = ncv_g = (int)counts[3];
+  nbocv_s = (int)counts[4];
+  const int nzone = (int)counts[5];
+
+  auto load_i = [&](const char *name, std::vector<int> &v) {
+    int64_t n = 0;
+    require(jm_mesh_array(mesh_, name, nullptr, &n), name);
+    v.resize(n);
+    require(jm_mesh_array(mesh_, name, v.data(), &n), name);
+  };
+  auto load_d = [&](const char *name, std::vector<double> &v) {
+    int64_t n = 0;
+    require(jm_mesh_array(mesh_, name, nullptr, &n), name);
+    v.resize(n);
+    require(jm_mesh_array(mesh_, name, v.data(), &n), name);
+  };
+  load_i("CVOFA", cvofa_v_);
+  load_i("NBOCV_I", nbocv_i_v_);
+  load_i("NBOCV_V", nbocv_v_v_);
+  load_d("X_CV", x_cv_v_);
+  load_d("X_FA", x_fa_v_);
+  load_d("FA_NORMAL", fa_normal_v_);
+  load_d("CV_VOLUME", cv_volume_v_);
+  cvofa = reinterpret_cast<int(*)[2]>(cvofa_v_.data());
+  nbocv_i = nbocv_i_v_.data();
+  nbocv_v = nbocv_v_v_.data();
+  x_cv = reinterpret_cast<double(*)[3]>(x_cv_v_.data());
+  x_fa = reinterpret_cast<double(*)[3]>(x_fa_v_.data());
+  fa_normal = reinterpret_cast<double(*)[3]>(fa_normal_v_.data());
+  cv_volume = cv_volume_v_.data();
+
+  char name[256];
+  for (int i = 0; i < nzone; i++) {
+    FaZone z;
+    require(jm_mesh_zone(mesh_, i, name, sizeof(name), &z.kind_, &z.ifa_f, &z.ifa_l), "jm_mesh_zone");
+    z.name_ = name;
+    z.index_ = i;
+    faZoneList.push_back(z);
+    if (z.kind_ == FA_ZONE_BOUNDARY) {
+      auto it = params_.find(z.name_);
+      if (it != params_.end() && it->second[0].size() > 1 && it->second[0][1] == "HOOK") hook_zones_.push_back(i);
+    }
+  }
+
+  // IMPLICIT_PRECISION = FP64 (default) | MIXED: Jacobian, preconditioner
+  // and Krylov in FP32 around the FP64 residual
+  const std::string prec = getStringParam("IMPLICIT_PRECISION", "FP64");
+  if (prec != "FP64" && prec != "MIXED") throw std::runtime_error("IMPLICIT_PRECISION must be FP64 or MIXED");
+  require(jm_ctx_create(mesh_, joe_in_.c_str(), prec == "MIXED" ? JM_MIXED : JM_FP64, nullptr, &ctx_),
+          "jm_ctx_create");
+  allocHost();
+}
+
+JoeWithModels::~JoeWithModels() {
+  if (ctx_) jm_ctx_destroy(ctx_);
+  if (mesh_) jm_mesh_destroy(mesh_);
+}
+
+void JoeWithModels::allocHost() {
+  const size_t n = ncv;
+  rho_v_.assign(n, 0.0);
+  rhou_v_.assign(3 * n, 0.0);
+  rhoE_v_.assign(n, 0.0);
+  vel_v_.assign(3 * n, 0.0);
+  press_v_.assign(n, 0.0);
+  temp_v_.assign(n, 0.0);
+  gamma_v_.assign(n, GAMMA);
+  rom_v_.assign(n, R_gas);
+  rho = rho_v_.data();
+  rhou = reinterpret_cast<double(*)[3]>(rhou_v_.data());
+  rhoE = rhoE_v_.data();
+  vel = reinterpret_cast<double(*)[3]>(vel_v_.data());
+  press = press_v_.data();
+  temp = temp_v_.data();
+  gamma = gamma_v_.data();
+  RoM = rom_v_.data();
+}
+
+bool JoeWithModels::checkParam(const std::string &name) const { return params_.count(name) != 0; }
+
+std::string JoeWithModels::getStringParam(const std::string &name, const std::string &def) const {
+  auto it = params_.find(name);
+  if (it == params_.end() || it->second[0].size() < 2) {
+    if (def.empty()) throw std::runtime_error("missing parameter " + name);
+    return def;
+  }
+  return it->second[0][1];
+}
+
+double JoeWithModels::getDoubleParam(const std::string &name, const std::string &def) const {
+  return std::stod(getStringParam(name, def));
+}
+
+int JoeWithModels::getIntParam(const std::string &name, const std::string &def) const {
+  return std::stoi(getStringParam(name, def));
+}
+
+// uniform branch of JoeWithModels::initialHook (JoeWithModels.cpp:180-211)
+void JoeWithModels::initialHook() {
+  const double rho_init = checkParam("RHO_INITIAL") ? getDoubleParam("RHO_INITIAL") : rho_ref;
+  const double p_init = checkParam("P_INITIAL") ? getDoubleParam("P_INITIAL") : p_ref;
+  double u_init[3] = {0.0, 0.0, 0.0};
+  if (checkParam("U_INITIAL")) {
+    const std::vector<std::string> &t = params_.at("U_INITIAL")[0];
+    if (t.size() < 4) throw std::runtime_error("U_INITIAL needs three values");
+    for (int i = 0; i < 3; i++) u_init[i] = std::stod(t[1 + i]);
+  }
+  double uu = 0.0;
+  for (int i = 0; i < 3; i++) uu += u_init[i] * u_init[i];
+  for (int icv = 0; icv < ncv; icv++) {
+    rho[icv] = rho_init;
+    for (int i = 0; i < 3; i++) rhou[icv][i] = rho_init * u_init[i];
+    rhoE[icv] = p_init / (gamma[icv] - 1.0) + 0.5 * rho_init * uu;
+  }
+}
+
+// default sourceHook: gravity (JoeWithModels.h:111-129). When a case does not
+// override sourceHook the device applies the same terms itself.
+void JoeWithModels::sourceHook(double *rhs_rho, double (*rhs_rhou)[3], double *rhs_rhoE, double (*A)[5][5]) {
+  for (int icv = 0; icv < ncv; icv++) {
+    rhs_rhou[icv][0] += cv_volume[icv] * rho[icv] * gravity[0];
+    rhs_rhou[icv][1] += cv_volume[icv] * rho[icv] * gravity[1];
+    rhs_rhou[icv][2] += cv_volume[icv] * rho[icv] * gravity[2];
+    rhs_rhoE[icv] += cv_volume[icv] * (gravity[0] * rhou[icv][0] + gravity[1] * rhou[icv][1] + gravity[2] * rhou[icv][2]);
+    if (A != nullptr) {
+      A[nbocv_i[icv]][1][0] -= cv_volume[icv] * gravity[0];
+      A[nbocv_i[icv]][2][0] -= cv_volume[icv] * gravity[1];
+      A[nbocv_i[icv]][3][0] -= cv_volume[icv] * gravity[2];
+      A[nbocv_i[icv]][4][1] -= cv_volume[icv] * gravity[0];
+      A[nbocv_i[icv]][4][2] -= cv_volume[icv] * gravity[1];
+      A[nbocv_i[icv]][4][3] -= cv_volume[icv] * gravity[2];
+    }
+  }
+}
+
+// True when the dynamic type replaces sourceHook: compares the function each
+// object would call (GNU bound-member-function extraction).
+bool JoeWithModels::sourceHookOverridden() {
+  typedef void (*Fn)(JoeWithModels *, double *, double (*)[3], double *, double (*)[5][5]);
+  JoeWithModels probe{ProbeTag{}};
+  Fn mine = (Fn)(this->*(&JoeWithModels::sourceHook));
+  Fn base = (Fn)(probe.*(&JoeWithModels::sourceHook));
+  return mine != base;
+}
+
+void JoeWithModels::syncStateFromDevice() {
+  require(jm_get_state(ctx_, rho, &rhou[0][0], rhoE), "jm_get_state");
+  // calcStateVariables (UgpWithCvCompFlow.h:1662-1694), ideal gas
+  for (int icv = 0; icv < ncv; icv++) {
+    for (int i = 0; i < 3; i++) vel[icv][i] = rhou[icv][i] / rho[icv];
+    double d = 0.0;
+    for (int i = 0; i < 3; i++) d += rhou[icv][i] * rhou[icv][i];
+    press[icv] = (gamma[icv] - 1.0) * (rhoE[icv] - 0.5 * d / rho[icv]);
+    temp[icv] = press[icv] / (rho[icv] * RoM[icv]);
+  }
+}
+
+// setBC's HOOK branch (JoeWithModels.cpp:3870-3881) for every HOOK zone, then
+// the device's state and boundary refresh with the new values
+void JoeWithModels::applyBoundaryHooks() {
+  if (hook_zones_.empty()) return;
+  syncStateFromDevice();
+  require(jm_update_bc(ctx_), "jm_update_bc");
+  t_bfa_.assign(nfa_b, 0.0);
+  vel_bfa_.assign(3 * (size_t)nfa_b, 0.0);
+  p_bfa_.assign(nfa_b, 0.0);
+  int64_t n = nfa_b;
+  require(jm_get(ctx_, "T_BFA", t_bfa_.data(), &n), "jm_get T_BFA");
+  n = 3 * (int64_t)nfa_b;
+  require(jm_get(ctx_, "VEL_BFA", vel_bfa_.data(), &n), "jm_get VEL_BFA");
+  n = nfa_b;
+  require(jm_get(ctx_, "P_BFA", p_bfa_.data(), &n), "jm_get P_BFA");
+  for (FaZone &z : faZoneList) {
+    bool hooked = false;
+    for (int k : hook_zones_) hooked |= (k == z.index_);
+    if (!hooked) continue;
+    boundaryHook(t_bfa_.data(), reinterpret_cast<double(*)[3]>(vel_bfa_.data()), p_bfa_.data(), &z);
+    require(jm_set_boundary_state(ctx_, z.index_, t_bfa_.data() + z.ifa_f, vel_bfa_.data() + 3 * (size_t)z.ifa_f,
+                                  p_bfa_.data() + z.ifa_f),
+            "jm_set_boundary_state");
+  }
+  require(jm_update_bc(ctx_), "jm_update_bc");
+}
+
+// doneSolver (JoeWithModels.h:42-95); RUNTIME is not read (the reference's
+// serial timers read zero, so it never stops a serial run either)
+int JoeWithModels::doneSolver(double residEnergy) {
+  int done = 0;
+  if ((nsteps >= 0) && (step >= nsteps)) done = 1;
+  if (residEnergy < resid_energ_th) done = 1;
+  if (residEnergy != residEnergy) done = 1;
+  if (std::FILE *fp = std::fopen("killjoe", "r")) {
+    std::fclose(fp);
+    std::remove("killjoe");
+    done = 1;
+  }
+  return done;
+}
+
+// JoeWithModels::run -> runBackwardEuler (JoeWithModels.cpp:881-1198)
+void JoeWithModels::run() {
+  initialHook();
+  require(jm_set_state(ctx_, rho, &rhou[0][0], rhoE), "jm_set_state");
+  step = 0;
+  applyBoundaryHooks();
+  const bool host_source = sourceHookOverridden();
+  const bool split = host_source || !hook_zones_.empty();
+  if (host_source) {
+    src_rho_.assign(ncv, 0.0);
+    src_rhou_.assign(3 * (size_t)ncv, 0.0);
+    src_rhoE_.assign(ncv, 0.0);
+    src_A_.assign(25 * (size_t)nbocv_s, 0.0);
+    a_diag_.assign(25 * (size_t)ncv, 0.0);
+  }
+  int done = doneSolver(1.0e20);
+  while (!done) {
+    jm_step_info info;
+    if (!split) {
+      require(jm_step(ctx_, &info), "jm_step");
+    } else {
+      require(jm_step_begin(ctx_, host_source ? JM_HOST_SOURCE : 0), "jm_step_begin");
+      if (host_source) {
+        syncStateFromDevice();
+        std::fill(src_rho_.begin(), src_rho_.end(), 0.0);
+        std::fill(src_rhou_.begin(), src_rhou_.end(), 0.0);
+        std::fill(src_rhoE_.begin(), src_rhoE_.end(), 0.0);
+        for (int icv = 0; icv < ncv; icv++)
+          for (int k = 0; k < 25; k++) src_A_[25 * (size_t)nbocv_i[icv] + k] = 0.0;
+        sourceHook(src_rho_.data(), reinterpret_cast<double(*)[3]>(src_rhou_.data()), src_rhoE_.data(),
+                   reinterpret_cast<double(*)[5][5]>(src_A_.data()));
+        for (int icv = 0; icv < ncv; icv++)
+          for (int k = 0; k < 25; k++) a_diag_[25 * (size_t)icv + k] = src_A_[25 * (size_t)nbocv_i[icv] + k];
+        require(jm_add_source(ctx_, src_rho_.data(), src_rhou_.data(), src_rhoE_.data(), a_diag_.data()),
+                "jm_add_source");
+      }
+      require(jm_step_end(ctx_, &info), "jm_step_end");
+    }
+    step = info.step;
+    cfl = info.cfl;
+    lastLinearIterations = info.lin_iters;
+    for (int k = 0; k < 5; k++) Residual[k] = info.resid[k];
+    applyBoundaryHooks();
+    temporalHook();
+    done = doneSolver(Residual[4]);
+  }
+  syncStateFromDevice();
+  finalHook();
+}

@@ -1,0 +1,126 @@
+// Case classes written the way a reference joe.cpp writes them (a class per
+// run number deriving from JoeWithModels, joe.cpp:24-40, 963-987), compiled
+// against the host mirror. Used by the tests to exercise every hook.
+//
+//   joe_cases <case> <Joe.in> <out.bin>
+//     plain     JoeWithModels with no overrides
+//     inlethook boundaryHook sets the zone "inlet" to the constant state a
+//               CBC inflow would impose (T, u, p from INLET_HOOK_STATE)
+//     gravity   sourceHook overridden, calling the default (gravity) hook
+//     perturb   initialHook adds a smooth density perturbation
+//
+// out.bin: step count, the five L1 residuals of every step, then rho[ncv],
+// rhou[ncv][3], rhoE[ncv] (reference numbering), all as doubles.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <iostream>
+#include <vector>
+
+#include "../JoeWithModels.hpp"
+
+class Recorder : public JoeWithModels {
+ public:
+  explicit Recorder(const char *f) : JoeWithModels(f) {}
+  std::vector<double> hist;
+  int temporal_calls = 0, final_calls = 0;
+  void temporalHook() override {
+    temporal_calls++;
+    for (int k = 0; k < 5; k++) hist.push_back(Residual[k]);
+  }
+  void finalHook() override { final_calls++; }
+  void write(const char *path) {
+    std::FILE *fp = std::fopen(path, "wb");
+    if (!fp) throw std::runtime_error("cannot write output");
+    double n = (double)step;
+    std::fwrite(&n, sizeof(double), 1, fp);
+    std::fwrite(hist.data(), sizeof(double), hist.size(), fp);
+    std::fwrite(rho, sizeof(double), ncv, fp);
+    std::fwrite(&rhou[0][0], sizeof(double), 3 * (size_t)ncv, fp);
+    std::fwrite(rhoE, sizeof(double), ncv, fp);
+    std::fclose(fp);
+  }
+};
+
+class InletHook : public Recorder {
+ public:
+  explicit InletHook(const char *f) : Recorder(f) {}
+  int calls = 0;
+  void boundaryHook(double *T_bfa, double (*vel_bfa)[3], double *p_bfa, FaZone *zone) override {
+    if (zone->getNameString() != "inlet") return;
+    calls++;
+    const double T = getDoubleParam("INLET_HOOK_T"), u = getDoubleParam("INLET_HOOK_U"),
+                 p = getDoubleParam("INLET_HOOK_P");
+    for (int ifa = zone->ifa_f; ifa <= zone->ifa_l; ifa++) {
+      T_bfa[ifa] = T;
+      vel_bfa[ifa][0] = u;
+      vel_bfa[ifa][1] = 0.0;
+      vel_bfa[ifa][2] = 0.0;
+      p_bfa[ifa] = p;
+    }
+  }
+};
+
+class GravityHook : public Recorder {
+ public:
+  explicit GravityHook(const char *f) : Recorder(f) {}
+  int calls = 0;
+  void sourceHook(double *rhs_rho, double (*rhs_rhou)[3], double *rhs_rhoE, double (*A)[5][5]) override {
+    calls++;
+    JoeWithModels::sourceHook(rhs_rho, rhs_rhou, rhs_rhoE, A);
+  }
+};
+
+class Perturb : public Recorder {
+ public:
+  explicit Perturb(const char *f) : Recorder(f) {}
+  void initialHook() override {
+    JoeWithModels::initialHook();
+    for (int icv = 0; icv < ncv; icv++) {
+      double s = 1.0 + 0.01 * std::sin(3.0 * x_cv[icv][0]) * std::cos(2.0 * x_cv[icv][1]);
+      rho[icv] *= s;
+      for (int i = 0; i < 3; i++) rhou[icv][i] *= s;
+      rhoE[icv] *= s;
+    }
+  }
+};
+
+int main(int argc, char **argv) {
+  if (argc != 4) {
+    std::cerr << "usage: joe_cases <plain|inlethook|gravity|perturb> <Joe.in> <out.bin>\n";
+    return 2;
+  }
+  try {
+    const std::string c = argv[1];
+    if (c == "plain") {
+      Recorder j(argv[2]);
+      j.run();
+      j.write(argv[3]);
+      std::printf("plain steps=%d temporal=%d final=%d\n", j.step, j.temporal_calls, j.final_calls);
+    } else if (c == "inlethook") {
+      InletHook j(argv[2]);
+      j.run();
+      j.write(argv[3]);
+      std::printf("inlethook steps=%d hook_calls=%d\n", j.step, j.calls);
+      if (j.calls != j.step + 1) return 3;  // once before the loop, once per step
+    } else if (c == "gravity") {
+      GravityHook j(argv[2]);
+      j.run();
+      j.write(argv[3]);
+      std::printf("gravity steps=%d hook_calls=%d\n", j.step, j.calls);
+      if (j.calls != j.step) return 3;
+    } else if (c == "perturb") {
+      Perturb j(argv[2]);
+      j.run();
+      j.write(argv[3]);
+      std::printf("perturb steps=%d\n", j.step);
+    } else {
+      std::cerr << "unknown case " << c << "\n";
+      return 2;
+    }
+  } catch (std::exception &e) {
+    std::cerr << "error: " << e.what() << "\n";
+    return 1;
+  }
+  return 0;
+}
