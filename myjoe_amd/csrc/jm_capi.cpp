@@ -8,6 +8,7 @@
 
 #include "../../include/joemi.h"
 #include "jm_ctx.hpp"
+#include "jm_param.hpp"
 #include "jm_mesh.hpp"
 
 struct jm_mesh {
@@ -27,6 +28,9 @@ jm_status guard(F &&f) {
   try {
     g_err.clear();
     return f();
+  } catch (const jm::Unsupported &e) {
+    g_err = e.what();
+    return JM_ERR_UNSUPPORTED;
   } catch (const std::invalid_argument &e) {
     g_err = e.what();
     return JM_ERR_PARAM;

@@ -13,6 +13,12 @@
 
 namespace jm {
 
+// an option of the reference that this build does not provide (JM_ERR_UNSUPPORTED)
+struct Unsupported : std::invalid_argument {
+  using std::invalid_argument::invalid_argument;
+};
+
+
 class ParamMap {
  public:
   using Tokens = std::vector<std::string>;

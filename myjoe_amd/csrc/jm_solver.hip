@@ -1084,12 +1084,12 @@ struct DBuf {
 void parse_params(const std::string &joe_in, const Mesh &m, SolverParams &p, std::vector<ZoneBc> &zbc) {
   ParamMap pm(joe_in);
   p.gamma = pm.dbl("GAMMA", 1.4);
-  if (pm.has("ALTITUDE")) throw std::invalid_argument("ALTITUDE is not supported");
+  if (pm.has("ALTITUDE")) throw Unsupported("ALTITUDE is not supported");
   p.T_ref = pm.dbl("T_REF");
   p.p_ref = pm.dbl("P_REF");
   p.rho_ref = pm.dbl("RHO_REF");
   p.R_gas = p.p_ref / p.rho_ref / p.T_ref;
-  if (pm.dbl("MU_REF", 0.0) > 0.0) throw std::invalid_argument("MU_REF > 0 (viscous flux) is not supported yet");
+  if (pm.dbl("MU_REF", 0.0) > 0.0) throw Unsupported("MU_REF > 0 (viscous flux) is not supported yet");
   p.gravity[0] = pm.dbl("GX", 0.0);
   p.gravity[1] = pm.dbl("GY", 0.0);
   p.gravity[2] = pm.dbl("GZ", 0.0);
@@ -1101,12 +1101,12 @@ void parse_params(const std::string &joe_in, const Mesh &m, SolverParams &p, std
   if (si == "HLLC") p.scheme = kHllc;
   else if (si == "ROE") p.scheme = kRoe;
   else if (si == "AUSM") p.scheme = kAusm;
-  else throw std::invalid_argument("SPACE_INTEGRATION = " + si + " is not supported (HLLC, ROE, AUSM)");
+  else throw Unsupported("SPACE_INTEGRATION = " + si + " is not supported (HLLC, ROE, AUSM)");
   if (pm.has("GRAD_RECONSTRUCTION")) {
     std::string s = pm.sub("GRAD_RECONSTRUCTION", "SCHEME");
     if (s == "LEASTSQUARES") p.grad_ls = true;
     else if (s == "GREENGAUSS") p.grad_ls = false;
-    else throw std::invalid_argument("GRAD_RECONSTRUCTION SCHEME=" + s + " is not supported");
+    else throw Unsupported("GRAD_RECONSTRUCTION SCHEME=" + s + " is not supported");
     std::string l = pm.sub("GRAD_RECONSTRUCTION", "LIMITER");
     if (l == "NO" || l == "NOLIMITER") p.limiter = false;
     else if (l == "YES" || l == "BARTH_JESPERSEN_MOD") p.limiter = true;
@@ -1136,9 +1136,9 @@ void parse_params(const std::string &joe_in, const Mesh &m, SolverParams &p, std
   }
   if (pm.str("TIME_STEP", "MAX_BASED") != "MAX_BASED") throw std::invalid_argument("TIME_STEP other than MAX_BASED");
   std::string ti = pm.str("TIME_INTEGRATION", "BACKWARD_EULER");
-  if (ti != "BACKWARD_EULER") throw std::invalid_argument("TIME_INTEGRATION = " + ti + " is not supported");
+  if (ti != "BACKWARD_EULER") throw Unsupported("TIME_INTEGRATION = " + ti + " is not supported");
   std::string ls = pm.str("LINEAR_SOLVER_NS", "BCGSTAB");
-  if (ls != "BCGSTAB") throw std::invalid_argument("LINEAR_SOLVER_NS = " + ls + " is not supported (BCGSTAB)");
+  if (ls != "BCGSTAB") throw Unsupported("LINEAR_SOLVER_NS = " + ls + " is not supported (BCGSTAB)");
   p.under_relax = pm.dbl("UNDER_RELAXATION", 0.3);
   if (pm.has("LINEAR_SOLVER_NS_TRESHOLDS")) {
     p.ls_maxiter = pm.sub_i("LINEAR_SOLVER_NS_TRESHOLDS", "MAX_ITER");

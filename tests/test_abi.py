@@ -89,3 +89,13 @@ def test_mesh_arrays_round_trip_counts():
     assert mesh.array("CVOFA").size == 2 * mesh.nfa
     assert mesh.array("NBOCV_V").size == mesh.nnz
     assert np.all(np.diff(mesh.array("NBOCV_I")) >= 1)
+
+
+def test_unsupported_options_have_their_own_status():
+    mm, joe = cases.box_case(2, seed=0)
+    mesh = joemi.Mesh.from_faces(mm)
+    for edit in (lambda s: s.replace("MU_REF = 0.0", "MU_REF = 1.0e-5"),
+                 lambda s: s.replace("LINEAR_SOLVER_NS = BCGSTAB", "LINEAR_SOLVER_NS = PETSC_GMRES")):
+        with pytest.raises(joemi.JoemiError) as e:
+            joemi.Solver(mesh, edit(joe))
+        assert e.value.status == 5, str(e.value)
