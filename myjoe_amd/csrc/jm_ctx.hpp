@@ -22,6 +22,9 @@ enum BcKind : int {
 };
 enum Scheme : int { kHllc = 0, kRoe = 1, kAusm = 2 };
 enum SymmBc : int { kSymmOriginal = 0, kSymmNew = 1, kSymmAnalytical = 2 };
+// device cell numbering: the reference's own, face-count groups inside
+// 1024-cell tiles, Morton order of centroids, or both
+enum CellOrder : int { kOrderNative = 0, kOrderGrouped = 1, kOrderMorton = 2, kOrderMortonGrouped = 3 };
 
 // Parameters of UgpWithCvCompFlow::init / runBackwardEuler read from Joe.in.
 struct SolverParams {
@@ -43,6 +46,7 @@ struct SolverParams {
   int nsteps = 100;
   double resid_energ_th = 0.0;
   double rho_init = 1.0, p_init = 1.0, u_init[3] = {0, 0, 0};
+  int cell_order = kOrderMorton;
 };
 
 struct ZoneBc {

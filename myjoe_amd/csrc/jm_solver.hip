@@ -453,14 +453,22 @@ __device__ __forceinline__ void face_flux(int scheme, double F[5], const Side &L
 // default sourceHook (JoeWithModels.h:111-129), gathered per cell in the
 // reference's accumulation order so every cell sum matches it exactly.
 // ---------------------------------------------------------------------------
+// Local time step arguments (calcDt, UgpWithCvCompFlow.cpp:5-110): mode 0
+// CFL (local), 1 CFL_MIN (global minimum filled afterwards), 2 fixed DT.
+struct DtArgs {
+  double cfl, dt_fixed;
+  int mode;
+};
+
+// One cell's residual; also its MAX_BASED local time step (UgpWithCvCompFlow.cpp:27-60,
+// inviscid: the viscous bound 4(mu+mut)A^2/(rho V) is zero), which needs the
+// same face normals and areas.
 template <int SCHEME, bool SND>
-__global__ void __launch_bounds__(kBlock) k_flux(DMesh m, DParams P, const double *__restrict__ prim,
-                                                 const double *__restrict__ grad, const double *__restrict__ bfa,
-                                                 const double *__restrict__ q, double *__restrict__ rhs,
-                                                 int *__restrict__ nfirst) {
-  const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (t >= m.ncv) return;
-  const int c = m.sched[t];
+__device__ __forceinline__ double flux_cell(const DMesh &m, const DParams &P, const double *__restrict__ prim,
+                                            const double *__restrict__ grad, const double *__restrict__ bfa,
+                                            const double *__restrict__ q, double *__restrict__ rhs,
+                                            int *__restrict__ nfirst, const DtArgs &dta, double *__restrict__ dt,
+                                            const int c) {
   const double gam = P.gamma;
   double pc[8], gc[15];
   {
@@ -486,6 +494,9 @@ __global__ void __launch_bounds__(kBlock) k_flux(DMesh m, DParams P, const doubl
   own_cc.p = pc[4];
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   int nlow = 0;
+  const double ru[3] = {q[m.ncv + c], q[2 * m.ncv + c], q[3 * m.ncv + c]};
+  const double cs = sqrt(gam * pc[4] / pc[0]);
+  double lambdaMax = 0.0;
   const int e0 = m.cfl_i[c], e1 = m.cfl_i[c + 1];
   for (int e = e0; e < e1; e++) {
     int2 en = m.cfl[e];
@@ -496,6 +507,10 @@ __global__ void __launch_bounds__(kBlock) k_flux(DMesh m, DParams P, const doubl
       double2 a = ld2(geo), b = ld2(geo + 2), cc = ld2(geo + 4);
       n[0] = a.x; n[1] = a.y; n[2] = b.x; area = b.y;
       xfa[0] = cc.x; xfa[1] = cc.y; xfa[2] = geo[6];
+    }
+    {
+      const double Uk = dot3(ru, n) / pc[0];
+      lambdaMax = Dev::mx(lambdaMax, (fabs(Uk) + cs) * area);
     }
     double F[5];
     if (en.y >= 0) {
@@ -557,7 +572,7 @@ __global__ void __launch_bounds__(kBlock) k_flux(DMesh m, DParams P, const doubl
     acc[1] += vol * pc[0] * P.g[0];
     acc[2] += vol * pc[0] * P.g[1];
     acc[3] += vol * pc[0] * P.g[2];
-    acc[4] += vol * (P.g[0] * q[m.ncv + c] + P.g[1] * q[2 * m.ncv + c] + P.g[2] * q[3 * m.ncv + c]);
+    acc[4] += vol * (P.g[0] * ru[0] + P.g[1] * ru[1] + P.g[2] * ru[2]);
   } else {
     // the same terms with g = 0: +0 for rhou, and -0 for rhoE only when
     // all three 0*rhou products are -0
@@ -569,6 +584,9 @@ __global__ void __launch_bounds__(kBlock) k_flux(DMesh m, DParams P, const doubl
   }
   for (int k = 0; k < 5; k++) rhs[(size_t)k * m.ncv + c] = acc[k];
   if (nlow) atomicAdd(nfirst, nlow);
+  const double dtc = dta.mode == 2 ? dta.dt_fixed : dta.cfl * vol / lambdaMax;
+  dt[c] = dtc;
+  return dtc;
 }
 
 // ---------------------------------------------------------------------------
@@ -737,14 +755,19 @@ __device__ __forceinline__ void block_partials(double (&v)[NV], double *partials
 
 // fixed-order combination of nb partials; valid in thread 0
 __device__ double fin_slot(const double *__restrict__ p, int nb, int op, double *sh) {
-  double x = op == kSum ? 0.0 : (op == kMax ? -__builtin_huge_val() : __builtin_huge_val());
-  bool first = true;
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-    double y = p[b];
-    x = first ? y : rcomb(x, y, op);
-    first = false;
+  const double id = op == kSum ? 0.0 : (op == kMax ? -__builtin_huge_val() : __builtin_huge_val());
+  // four independent chains per thread keep four loads in flight
+  double x0 = id, x1 = id, x2 = id, x3 = id;
+  const int st = blockDim.x;
+  int b = threadIdx.x;
+  for (; b + 3 * st < nb; b += 4 * st) {
+    x0 = rcomb(x0, p[b], op);
+    x1 = rcomb(x1, p[b + st], op);
+    x2 = rcomb(x2, p[b + 2 * st], op);
+    x3 = rcomb(x3, p[b + 3 * st], op);
   }
-  double v[1] = {x};
+  for (; b < nb; b += st) x0 = rcomb(x0, p[b], op);
+  double v[1] = {rcomb(rcomb(x0, x1, op), rcomb(x2, x3, op), op)};
   __syncthreads();
   block_reduce<1>(v, sh, op);
   return v[0];
@@ -753,7 +776,8 @@ __device__ double fin_slot(const double *__restrict__ p, int nb, int op, double 
 enum FinOp { kFinDt = 0, kFinL1 = 1, kFinAlpha = 2, kFinOmega = 3, kFinResid0 = 4, kFinResid = 5 };
 
 // one block: finishes a reduction and applies the scalar update of its owner
-__global__ void __launch_bounds__(kBlock) k_fin(int nb, int op, const double *__restrict__ partials,
+constexpr int kFinBlock = 1024;
+__global__ void __launch_bounds__(kFinBlock) k_fin(int nb, int op, const double *__restrict__ partials,
                                                 SolverScalars *S) {
   __shared__ double sh[16];
   if ((op == kFinAlpha || op == kFinOmega || op == kFinResid) && S->done) return;
@@ -807,37 +831,18 @@ __global__ void __launch_bounds__(kBlock) k_fin(int nb, int op, const double *__
   }
 }
 
-// local time step (MAX_BASED), UgpWithCvCompFlow.cpp:27-60; mul = mut = 0.
-// Block minima go to partials (k_fin kFinDt); CFL_MIN mode: k_dt_fill.
-__global__ void __launch_bounds__(kBlock) k_dt(DMesh m, const double *__restrict__ prim, const double *__restrict__ q,
-                                               double gam, double cfl, int mode, double dt_fixed,
-                                               double *__restrict__ dt, double *partials) {
+// Residual kernel: one cell per thread (flux_cell), block minima of the local
+// time step to partials (k_fin kFinDt); CFL_MIN mode: k_dt_fill afterwards.
+template <int SCHEME, bool SND>
+__global__ void __launch_bounds__(kBlock) k_flux(DMesh m, DParams P, const double *__restrict__ prim,
+                                                 const double *__restrict__ grad, const double *__restrict__ bfa,
+                                                 const double *__restrict__ q, double *__restrict__ rhs,
+                                                 int *__restrict__ nfirst, DtArgs dta, double *__restrict__ dt,
+                                                 double *__restrict__ partials) {
   __shared__ double sh[16];
   const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  const int c = t < m.ncv ? m.sched[t] : m.ncv;
   double dtc = 1.0e20;
-  if (c < m.ncv) {
-    if (mode == 2) {
-      dtc = dt_fixed;
-    } else {
-      const double *pc = prim + 8 * (size_t)c;
-      const double rho = pc[0], p = pc[4];
-      const double ru[3] = {q[m.ncv + c], q[2 * m.ncv + c], q[3 * m.ncv + c]};
-      double cs = sqrt(gam * p / rho);
-      double lambdaMax = 0.0;
-      for (int e = m.cfl_i[c]; e < m.cfl_i[c + 1]; e++) {
-        int f = m.cfl[e].x & 0x7fffffff;
-        const double *geo = m.fgeo + 8 * (size_t)f;
-        const double n[3] = {geo[0], geo[1], geo[2]};
-        double area = geo[3];
-        double Uk = dot3(ru, n) / rho;
-        double lambda = (fabs(Uk) + cs) * area;
-        lambdaMax = Dev::mx(lambdaMax, lambda);
-      }
-      dtc = cfl * m.vol[c] / lambdaMax;
-    }
-    dt[c] = dtc;
-  }
+  if (t < m.ncv) dtc = flux_cell<SCHEME, SND>(m, P, prim, grad, bfa, q, rhs, nfirst, dta, dt, m.sched[t]);
   double v[1] = {dtc};
   block_partials<1>(v, partials, sh, kMin);
 }
@@ -1090,6 +1095,15 @@ void parse_params(const std::string &joe_in, const Mesh &m, SolverParams &p, std
   p.rho_ref = pm.dbl("RHO_REF");
   p.R_gas = p.p_ref / p.rho_ref / p.T_ref;
   if (pm.dbl("MU_REF", 0.0) > 0.0) throw Unsupported("MU_REF > 0 (viscous flux) is not supported yet");
+  // device cell numbering (this build's own option; results do not depend on it)
+  {
+    const std::string o = pm.str("GPU_CELL_ORDER", "MORTON");
+    if (o == "NATIVE") p.cell_order = kOrderNative;
+    else if (o == "GROUPED") p.cell_order = kOrderGrouped;
+    else if (o == "MORTON") p.cell_order = kOrderMorton;
+    else if (o == "MORTON_GROUPED") p.cell_order = kOrderMortonGrouped;
+    else throw std::invalid_argument("GPU_CELL_ORDER = " + o + " (NATIVE, GROUPED, MORTON, MORTON_GROUPED)");
+  }
   p.gravity[0] = pm.dbl("GX", 0.0);
   p.gravity[1] = pm.dbl("GY", 0.0);
   p.gravity[2] = pm.dbl("GZ", 0.0);
@@ -1345,14 +1359,15 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
   }
   ci[ncv] = (int)cf.size();
 
-  // Device numbering (internal only; the ABI speaks reference numbering):
-  // cells in Morton order of their centroids, cut into tiles of kTile cells
-  // and grouped inside a tile by face count, so a workgroup's cells and their
-  // neighbours are spatially close and a wavefront's lanes loop equally long.
-  // Boundary faces keep their index; internal faces are numbered in order of
-  // first visit by the device-ordered cells. Per-cell sums keep the reference
-  // order, so results do not depend on this numbering.
+  // Device numbering (internal only; the ABI speaks reference numbering),
+  // chosen by GPU_CELL_ORDER: the reference's own order (NATIVE), or Morton
+  // order of the centroids (MORTON), optionally with the cells of every tile
+  // of kTile cells grouped by face count so a wavefront's lanes loop equally
+  // long (GROUPED, MORTON_GROUPED). Boundary faces keep their index; internal
+  // faces are numbered in order of first visit by the device-ordered cells.
+  // Per-cell sums keep the reference order, so results do not depend on it.
   const int kTile = 1024;
+  const int order = d.par.cell_order;
   double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
   for (int c = 0; c < ncv; c++)
     for (int i = 0; i < 3; i++) {
@@ -1368,12 +1383,14 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
   std::vector<int> &perm = d.perm;
   perm.resize(ncv);
   for (int c = 0; c < ncv; c++) perm[c] = c;
-  std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return key[a] < key[b]; });
-  for (int t0 = 0; t0 < ncv; t0 += kTile) {
-    int t1 = std::min(ncv, t0 + kTile);
-    std::stable_sort(perm.begin() + t0, perm.begin() + t1,
-                     [&](int a, int b) { return (ci[a + 1] - ci[a]) < (ci[b + 1] - ci[b]); });
-  }
+  if (order == kOrderMorton || order == kOrderMortonGrouped)
+    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return key[a] < key[b]; });
+  if (order == kOrderMortonGrouped || order == kOrderGrouped)
+    for (int t0 = 0; t0 < ncv; t0 += kTile) {
+      int t1 = std::min(ncv, t0 + kTile);
+      std::stable_sort(perm.begin() + t0, perm.begin() + t1,
+                       [&](int a, int b) { return (ci[a + 1] - ci[a]) < (ci[b + 1] - ci[b]); });
+    }
   std::vector<int> inv(ncv);
   for (int cd = 0; cd < ncv; cd++) inv[perm[cd]] = cd;
   std::vector<int> fnew(nfa, -1);
@@ -1605,9 +1622,13 @@ void Ctx::Impl::launch_grad_flux() {
     else
       hipLaunchKernelGGL(k_grad<false>, dim3(nblk(ncv)), dim3(kBlock), 0, stream, dm, dp, prim.p, bfa.p, grad.p);
   }
+  DtArgs dta;
+  dta.cfl = cfl;
+  dta.dt_fixed = par.dt_fixed;
+  dta.mode = par.dt_mode;
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3(nblk(ncv)), dim3(kBlock), 0, stream, dm, dp, prim.p, grad.p, bfa.p, q.p, rhs.p,
-                       counters.p + 1);
+                       counters.p + 1, dta, dt.p, partials.p);
   };
   if (par.second_order) {
     if (par.scheme == kRoe) go(k_flux<kRoe, true>);
@@ -1618,6 +1639,9 @@ void Ctx::Impl::launch_grad_flux() {
     else if (par.scheme == kAusm) go(k_flux<kAusm, false>);
     else go(k_flux<kHllc, false>);
   }
+  // dt_min of calcDt; CFL_MIN sets every local dt to it
+  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, nblk(ncv), (int)kFinDt, partials.p, scal.p);
+  if (par.dt_mode == 1) hipLaunchKernelGGL(k_dt_fill, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, dt.p, scal.p);
 }
 
 template <typename T>
@@ -1663,7 +1687,7 @@ void Ctx::Impl::solve(int &iters, int &status) {
   const int gb5 = (ncv + kCellsPerBlock - 1) / kCellsPerBlock;
   hipLaunchKernelGGL(k_resid<T>, dim3(gb5), dim3(kBlock), 0, stream, ncv, nbocv_i.p, nbocv_v.p, A, LU->p, phi, rhs_,
                      res, res0, 1, partials.p, scal.p);
-  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, stream, gb5, (int)kFinResid0, partials.p, scal.p);
+  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, gb5, (int)kFinResid0, partials.p, scal.p);
   const int max_it = par.ls_maxiter + 1;
   int launched = 0;
   while (launched < max_it) {
@@ -1672,15 +1696,15 @@ void Ctx::Impl::solve(int &iters, int &status) {
       hipLaunchKernelGGL(k_p<T>, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, LU->p, res, vv, p, phat, scal.p);
       hipLaunchKernelGGL(k_spmv_dot<T>, dim3(gb5), dim3(kBlock), 0, stream, ncv, nbocv_i.p, nbocv_v.p, A, phat, vv,
                          res0, partials.p, scal.p);
-      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, stream, gb5, (int)kFinAlpha, partials.p, scal.p);
+      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, gb5, (int)kFinAlpha, partials.p, scal.p);
       hipLaunchKernelGGL(k_s<T>, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, LU->p, res, vv, s, shat, scal.p);
       hipLaunchKernelGGL(k_spmv_dot<T>, dim3(gb5), dim3(kBlock), 0, stream, ncv, nbocv_i.p, nbocv_v.p, A, shat, t, s,
                          partials.p, scal.p);
-      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, stream, gb5, (int)kFinOmega, partials.p, scal.p);
+      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, gb5, (int)kFinOmega, partials.p, scal.p);
       hipLaunchKernelGGL(k_x<T>, dim3(nblk(n5)), dim3(kBlock), 0, stream, n5, phi, phat, shat, scal.p);
       hipLaunchKernelGGL(k_resid<T>, dim3(gb5), dim3(kBlock), 0, stream, ncv, nbocv_i.p, nbocv_v.p, A, LU->p, phi,
                          rhs_, res, res0, 0, partials.p, scal.p);
-      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, stream, gb5, (int)kFinResid, partials.p, scal.p);
+      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, gb5, (int)kFinResid, partials.p, scal.p);
     }
     launched += chunk;
     HIPCHECK(hipMemcpyAsync(h_scal, scal.p, sizeof(SolverScalars), hipMemcpyDeviceToHost, stream));
@@ -1739,28 +1763,20 @@ std::string Ctx::time_kernels(int reps, double *ms) {
   // flux alone: time the whole grad+flux pair and subtract the gradient
   double gf = timed([&] { d.launch_grad_flux(); });
   ms[3] = gf - ms[2];
+  // (the local time step is computed inside k_flux)
   ms[4] = timed([&] {
-    hipLaunchKernelGGL(k_dt, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.dm, d.prim.p, d.q.p, d.par.gamma, d.cfl,
-                       d.par.dt_mode, d.par.dt_fixed, d.dt.p, d.partials.p);
-  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, d.stream, nblk(d.ncv), (int)kFinDt, d.partials.p, d.scal.p);
-  });
-  ms[5] = timed([&] {
     d.launch_state_bc();
     d.launch_grad_flux();
   });
   HIPCHECK(hipEventDestroy(e0));
   HIPCHECK(hipEventDestroy(e1));
-  return "k_state,k_bc,k_grad,k_flux,k_dt,residual";
+  return "k_state,k_bc,k_grad,k_flux,residual";
 }
 
 void Ctx::calc_rhs(bool implicit) {
   Impl &d = *d_;
   if (!d.have_state) throw std::logic_error("no state set");
   d.launch_state_bc();
-  hipLaunchKernelGGL(k_dt, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.dm, d.prim.p, d.q.p, d.par.gamma, d.cfl,
-                     d.par.dt_mode, d.par.dt_fixed, d.dt.p, d.partials.p);
-  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, d.stream, nblk(d.ncv), (int)kFinDt, d.partials.p, d.scal.p);
-  if (d.par.dt_mode == 1) hipLaunchKernelGGL(k_dt_fill, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.dt.p, d.scal.p);
   d.launch_grad_flux();
   if (implicit) {
     if (d.precision == 0) {
@@ -1786,10 +1802,6 @@ StepInfo Ctx::step() {
   // CFL ramp (JoeWithModels.cpp:978)
   if ((d.step_no >= d.par.cfl_after) && (d.step_no % d.par.cfl_interval == 0) && (d.cfl < d.par.cfl_max))
     d.cfl *= d.par.cfl_factor;
-  hipLaunchKernelGGL(k_dt, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.dm, d.prim.p, d.q.p, d.par.gamma, d.cfl,
-                     d.par.dt_mode, d.par.dt_fixed, d.dt.p, d.partials.p);
-  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, d.stream, nblk(d.ncv), (int)kFinDt, d.partials.p, d.scal.p);
-  if (d.par.dt_mode == 1) hipLaunchKernelGGL(k_dt_fill, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.dt.p, d.scal.p);
   d.launch_grad_flux();
   int it = 0, st = 0;
   if (d.precision == 0) {
@@ -1809,7 +1821,7 @@ StepInfo Ctx::step() {
   }
   d.launch_state_bc();
   hipLaunchKernelGGL(k_l1, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.rhs.p, d.partials.p);
-  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, d.stream, nblk(d.ncv), (int)kFinL1, d.partials.p, d.scal.p);
+  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, d.stream, nblk(d.ncv), (int)kFinL1, d.partials.p, d.scal.p);
   HIPCHECK(hipMemcpyAsync(d.h_scal, d.scal.p, sizeof(SolverScalars), hipMemcpyDeviceToHost, d.stream));
   HIPCHECK(hipStreamSynchronize(d.stream));
   info.step = d.step_no;
@@ -1829,10 +1841,6 @@ void Ctx::step_begin(bool host_source) {
   d.step_no++;
   if ((d.step_no >= d.par.cfl_after) && (d.step_no % d.par.cfl_interval == 0) && (d.cfl < d.par.cfl_max))
     d.cfl *= d.par.cfl_factor;
-  hipLaunchKernelGGL(k_dt, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.dm, d.prim.p, d.q.p, d.par.gamma, d.cfl,
-                     d.par.dt_mode, d.par.dt_fixed, d.dt.p, d.partials.p);
-  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, d.stream, nblk(d.ncv), (int)kFinDt, d.partials.p, d.scal.p);
-  if (d.par.dt_mode == 1) hipLaunchKernelGGL(k_dt_fill, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.dt.p, d.scal.p);
   // a host sourceHook replaces the default one (gravity)
   const DParams keep = d.dp;
   if (host_source) d.dp.gravity_on = 0;
@@ -1897,7 +1905,7 @@ StepInfo Ctx::step_end() {
   }
   d.launch_state_bc();
   hipLaunchKernelGGL(k_l1, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.rhs.p, d.partials.p);
-  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, d.stream, nblk(d.ncv), (int)kFinL1, d.partials.p, d.scal.p);
+  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, d.stream, nblk(d.ncv), (int)kFinL1, d.partials.p, d.scal.p);
   HIPCHECK(hipMemcpyAsync(d.h_scal, d.scal.p, sizeof(SolverScalars), hipMemcpyDeviceToHost, d.stream));
   HIPCHECK(hipStreamSynchronize(d.stream));
   StepInfo info;

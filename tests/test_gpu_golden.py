@@ -78,3 +78,17 @@ def test_device_numbering_is_invisible():
     assert libhip.hipMemcpy(host.ctypes.data, ptr, host.nbytes, 2) == 0  # hipMemcpyDeviceToHost
     assert np.array_equal(host[order], g["RHO"])
     assert np.array_equal(host[4 * mesh.ncv + order], g["RHOE"])
+
+
+@pytest.mark.parametrize("order", ["NATIVE", "GROUPED", "MORTON", "MORTON_GROUPED"])
+def test_cell_order_does_not_change_results(order):
+    print(f"cell order {order}", flush=True)
+    g = load("rhs_box4_roe_alt_ana")
+    mesh = joemi.Mesh.from_faces(MESHES["box4"]())
+    s = joemi.Solver(mesh, joe_text(g) + f"GPU_CELL_ORDER = {order}\n")
+    s.set_state(g["RHO"], g["RHOU"], g["RHOE"])
+    s.calc_rhs(True)
+    for k in FIELDS:
+        compare(s.get(k), g[k], 1e-12, f"{order}:{k}")
+    s.step()
+    compare(s.get("DQ"), g["DQ"], 1e-9, f"{order}:DQ")
