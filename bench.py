@@ -66,6 +66,98 @@ def cpu_baseline(n_box=64, reps=12):
                       f"calcRhs; HLLC 2nd order LS+limiter) in {dt:.1f} s on 1 host core"}
 
 
+def serving_bench(n, seed, min_cells, max_cells, max_active, nsteps, resid_rel, budget_gib):
+    """Steady solves/min and latency of the served mixed-precision solver
+    against a plain FP64 one-at-a-time loop, on airfoil requests (inviscid
+    NACA 4-digit O-meshes; thickness, camber, size, Mach and angle of attack
+    drawn as the serving workload specifies). Capacity: all requests present
+    at t = 0. Latency: Poisson arrivals at 0.8 x the served capacity, replayed
+    in real time for the server; for the one-at-a-time loop the same arrivals
+    are queued in front of its measured per-solve times."""
+    import numpy as np
+
+    from myjoe_amd import cases, joemi
+
+    reqs = cases.serving_requests(n, seed, min_cells, max_cells, rate_per_s=0.0)
+    built = []
+    for r in reqs:
+        mm, joe = cases.airfoil_case(r["ni"], r["nj"], r["thickness"], r["camber"], r["mach"], r["aoa_deg"],
+                                     nsteps=nsteps, resid_rel=resid_rel)
+        built.append((joemi.Mesh.from_faces(mm), joe))
+        del mm
+    cells = sum(m.ncv for m, _ in built)
+    # plain loop: FP64, one solve at a time, host-polled steps
+    service, steps_plain, plain_states = [], [], []
+    for mesh, joe in built:
+        t0 = time.perf_counter()
+        s = joemi.Solver(mesh, joe, joemi.JM_FP64)
+        s.init_uniform()
+        k, _ = s.run(nsteps)
+        st = s.state()
+        service.append(time.perf_counter() - t0)
+        steps_plain.append(k)
+        plain_states.append(st)
+        del s
+    cap_plain = n / sum(service) * 60.0
+    print(f"# serving: plain FP64 loop {cap_plain:.1f} solves/min ({cells} cells in {n} solves)", file=sys.stderr,
+          flush=True)
+    # served: mixed precision, all present at t = 0
+    budget = int(budget_gib * (1 << 30))
+    srv = joemi.Server(budget, max_active)
+    t0 = time.perf_counter()
+    tickets = [srv.submit(m, joe, joemi.JM_MIXED) for m, joe in built]
+    for t in tickets:
+        srv.wait(t)
+    makespan = time.perf_counter() - t0
+    cap_served = n / makespan * 60.0
+    steps_served, err = [], {"rho": 0.0, "rhou": 0.0, "rhoE": 0.0}
+    for t, ref in zip(tickets, plain_states):
+        st, rep = srv.result(t)
+        steps_served.append(rep.steps)
+        for name, a, b in zip(("rho", "rhou", "rhoE"), st, ref):
+            err[name] = max(err[name], float(np.max(np.abs(a - b)) / max(float(np.max(np.abs(b))), 1e-300)))
+        srv.release(t)
+    print(f"# serving: served mixed {cap_served:.1f} solves/min (makespan {makespan:.2f} s)", file=sys.stderr,
+          flush=True)
+    # latency under Poisson arrivals at 0.8 x the served capacity
+    rate = 0.8 * cap_served / 60.0
+    rng = np.random.default_rng(seed + 1)
+    arr = np.cumsum(rng.exponential(1.0 / rate, n))
+    lat = []
+    t0 = time.perf_counter()
+    tk = []
+    for (m, joe), a in zip(built, arr):
+        delay = a - (time.perf_counter() - t0)
+        if delay > 0:
+            time.sleep(delay)
+        tk.append(srv.submit(m, joe, joemi.JM_MIXED))
+    for t in tk:
+        srv.wait(t)
+        _, rep = srv.result(t)
+        lat.append(rep.queued_s + rep.run_s)
+        srv.release(t)
+    srv.close()
+    done_prev, lat_plain = 0.0, []
+    for a, sv in zip(arr, service):
+        start = max(a, done_prev)
+        done_prev = start + sv
+        lat_plain.append(done_prev - a)
+    q = lambda v, p: float(np.percentile(np.array(v), p))
+    return {
+        "workload": f"{n} inviscid NACA 4-digit airfoil solves, {min_cells}-{max_cells} cells (O-mesh, one cell "
+                    f"thick), Mach 0.3-0.8, AoA -2..6 deg; stop at energy L1 drop {resid_rel:g} or {nsteps} steps",
+        "served": {"solves_per_min": round(cap_served, 2), "precision": "FP64 residual, FP32 Jacobian/Krylov",
+                   "max_active": max_active, "p50_s": round(q(lat, 50), 3), "p99_s": round(q(lat, 99), 3),
+                   "mean_steps": round(float(np.mean(steps_served)), 1)},
+        "plain_fp64_loop": {"solves_per_min": round(cap_plain, 2), "p50_s": round(q(lat_plain, 50), 3),
+                            "p99_s": round(q(lat_plain, 99), 3), "mean_steps": round(float(np.mean(steps_plain)), 1)},
+        "speedup_solves_per_min": round(cap_served / cap_plain, 3),
+        "arrival_rate_per_min": round(rate * 60.0, 2),
+        "served_vs_fp64_max_rel_diff": {k: float(f"{v:.3g}") for k, v in err.items()},
+        "cells_total": int(cells),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,8 +166,15 @@ def main():
     ap.add_argument("--n", type=int, default=160)
     ap.add_argument("--schemes", default="HLLC,ROE,AUSM")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--grad-kernel", default=None, help="GPU_GRAD_KERNEL: ENTRY or CELL")
+    ap.add_argument("--flux-kernel", default=None, help="GPU_FLUX_KERNEL: ENTRY or CELL")
     ap.add_argument("--cell-order", default=None,
                     help="device cell numbering (GPU_CELL_ORDER): NATIVE, GROUPED, MORTON, MORTON_GROUPED")
+    ap.add_argument("--serve", type=int, default=16, help="serving requests (0: skip the serving leg)")
+    ap.add_argument("--serve-cells", default="20000,80000", help="min,max cells of a served airfoil")
+    ap.add_argument("--serve-active", type=int, default=8)
+    ap.add_argument("--serve-steps", type=int, default=500)
+    ap.add_argument("--serve-rel", type=float, default=1e-6)
     ap.add_argument("--copy-calib-gib", type=float, default=0.0,
                     help="first run a plain device copy of this many GiB (calibrates profiler byte counters)")
     args = ap.parse_args()
@@ -119,6 +218,10 @@ def main():
         joe = cases.box_joe_in(scheme)
         if args.cell_order:
             joe += f"GPU_CELL_ORDER = {args.cell_order}\n"
+        if args.grad_kernel:
+            joe += f"GPU_GRAD_KERNEL = {args.grad_kernel}\n"
+        if args.flux_kernel:
+            joe += f"GPU_FLUX_KERNEL = {args.flux_kernel}\n"
         s = joemi.Solver(mesh, joe, joemi.JM_FP64)
         s.set_state(*state)
         for _ in range(args.warmup):
@@ -168,9 +271,12 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "algorithmic_bytes_per_launch": int(ab[dom])},
     }
+    del mesh
+    gc.collect()
+    if args.serve > 0:
+        lo, hi = (int(x) for x in args.serve_cells.split(","))
+        out["serving"] = serving_bench(args.serve, 7, lo, hi, args.serve_active, args.serve_steps, args.serve_rel, 64.0)
     if not args.no_cpu_baseline:
-        del mesh
-        gc.collect()
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)
 

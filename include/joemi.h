@@ -127,6 +127,11 @@ typedef struct {
   double resid[5];     /* L1 residuals of rho, rhou(3), rhoE */
 } jm_step_info;
 jm_status jm_step(jm_ctx *c, jm_step_info *info);
+/* The same step driven by the device alone: CFL ramp and step counter on the
+ * device, every BiCGStab iteration launched (iterations after convergence
+ * return at once), captured into a hipGraph on first use and replayed.
+ * Results are bit-identical to jm_step's; one launch per step. */
+jm_status jm_step_graph(jm_ctx *c, jm_step_info *info);
 
 /* The same step split where the reference calls its hooks, for a host whose
  * JoeWithModels subclass overrides sourceHook or boundaryHook:
@@ -176,6 +181,42 @@ jm_status jm_time_residual_kernels(jm_ctx *c, int reps, double *ms_out, char *na
  * minimum over internal faces of s.n/(|s||n|) and the number of sub-tets
  * with non-positive volume. */
 jm_status jm_mesh_quality(const jm_mesh *m, double *min_sdotn, int *neg_subtets);
+
+/* ---------------------------------------------------------------------------
+ * Serving: many steady solves on one GPU. No counterpart in the reference
+ * (one JoeWithModels::run per process); each solve follows the same path as
+ * jm_run. Requests are admitted in submission order while their device
+ * footprint fits mem_budget_bytes and fewer than max_active run; each active
+ * solve keeps its state and Jacobian resident on its own stream and advances
+ * by graph-captured steps (jm_step_graph), overlapping the others. A solve
+ * stops by NSTEPS, RESID_ENERG_TH, NaN or RESID_ENERG_REL (energy L1 below
+ * this fraction of the first step's; this build's own key). The mesh passed
+ * to jm_server_submit must stay alive until the ticket is released. Results
+ * are bit-identical to a solve run alone.
+ * ------------------------------------------------------------------------- */
+/* device bytes a context for this mesh holds once it has stepped (the
+ * server's admission estimate) */
+jm_status jm_ctx_footprint(const jm_mesh *m, int precision, int64_t *bytes);
+
+typedef struct jm_server jm_server;
+typedef struct {
+  int status;         /* 0 queued, 1 running, 2 done, 3 failed */
+  int steps;          /* pseudo-time steps taken */
+  double resid[5];    /* L1 residuals of the last step */
+  double resid0[5];   /* L1 residuals of the first step */
+  double queued_s;    /* submission to admission, seconds */
+  double run_s;       /* admission (context build included) to retirement */
+} jm_solve_report;
+jm_status jm_server_create(int64_t mem_budget_bytes, int max_active, jm_server **out);
+void jm_server_destroy(jm_server *s);
+/* initial state may be NULL (uniform, as jm_init_uniform) */
+jm_status jm_server_submit(jm_server *s, const jm_mesh *m, const char *joe_in, int precision, const double *rho,
+                           const double *rhou, const double *rhoE, int *ticket);
+jm_status jm_server_status(jm_server *s, int ticket, int *status);
+jm_status jm_server_wait(jm_server *s, int ticket);
+/* final state (any pointer may be NULL) and report of a finished solve */
+jm_status jm_server_result(jm_server *s, int ticket, double *rho, double *rhou, double *rhoE, jm_solve_report *rep);
+jm_status jm_server_release(jm_server *s, int ticket);
 
 #ifdef __cplusplus
 }

@@ -17,7 +17,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libjoemi.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
-HOST_SRC = ["jm_mesh.cpp", "jm_capi.cpp"]
+HOST_SRC = ["jm_mesh.cpp", "jm_capi.cpp", "jm_server.cpp"]
 DEV_SRC = ["jm_solver.hip"]
 
 HOST_FLAGS = ["-O2", "-ffp-contract=off", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",

@@ -7,6 +7,8 @@
   step 20; a 10 %-per-step ramp makes the reference itself diverge).
 * ``box_case``: the split/jittered box of the single-residual throughput test
   with inlet/outlet CBC, WALL, FARFIELD and SYMMETRY zones.
+* ``airfoil_case``: inviscid flow past a NACA 4-digit section on an O-mesh
+  (the serving workload's requests), free stream p = rho = 1.
 * ``random_state``: rho ~ U(0.5, 1.5), p ~ lognormal(0, 0.2),
   u_i ~ N(0, 0.3) * c, as conservative variables.
 """
@@ -106,3 +108,46 @@ def random_state(ncv: int, seed: int = 1):
     rhou = rho[:, None] * u
     rhoE = p / (GAMMA - 1.0) + 0.5 * rho * np.sum(u * u, axis=1)
     return rho, rhou, rhoE
+
+
+def airfoil_case(ni: int = 200, nj: int = 100, thickness: float = 0.12, camber: float = 0.0, mach: float = 0.5,
+                 aoa_deg: float = 2.0, scheme: str = "HLLC", nsteps: int = 2000, resid_rel: float = 1e-6,
+                 cfl_max: float = 50.0, restart: str = "airfoil.msh"):
+    """Steady inviscid flow past a NACA 4-digit section: FARFIELD at Mach
+    mach and angle of attack aoa_deg, WALL on the section, SYMMETRY in z.
+    Stops at nsteps or when the energy L1 residual has dropped by resid_rel
+    (RESID_ENERG_REL)."""
+    mesh = meshgen.naca_omesh(ni, nj, thickness, camber)
+    a = math.radians(aoa_deg)
+    c = math.sqrt(GAMMA)
+    u, v = mach * c * math.cos(a), mach * c * math.sin(a)
+    extra = "\n".join([
+        f"RESTART = {restart}",
+        "LINEAR_SOLVER_NS_TRESHOLDS MAX_ITER=30 ABS_RESID=1.0e-8 REL_RESID=1.0e-2",
+        "CFL = 1.0",
+        f"CFL_RAMP AFTER_ITER=20 INTERVAL_ITER=5 FACTOR_CFL=1.2 MAX_CFL={cfl_max!r}",
+        f"NSTEPS = {nsteps}",
+        f"RESID_ENERG_REL = {resid_rel!r}",
+        f"U_INITIAL = {u!r} {v!r} 0.0",
+        "airfoil = WALL",
+        f"farfield = FARFIELD {u!r} {v!r} 0.0 1.0 1.0 1.4",
+        "zsym = SYMMETRY",
+    ]) + "\n"
+    return mesh, _common(scheme, True, "LEASTSQUARES", 0.01, extra)
+
+
+def serving_requests(n: int, seed: int = 0, min_cells: int = 20_000, max_cells: int = 600_000,
+                     rate_per_s: float = 1.0):
+    """n airfoil requests: thickness U(8, 18) %, camber U(0, 4) %, size U(min,
+    max) cells (O-mesh with ni = 2 nj), Mach U(0.3, 0.8), angle of attack
+    U(-2, 6) degrees; Poisson arrival times at rate_per_s."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate_per_s, n)) if rate_per_s > 0 else np.zeros(n)
+    out = []
+    for k in range(n):
+        cells = int(rng.uniform(min_cells, max_cells))
+        nj = max(8, int(round(math.sqrt(cells / 2.0))))
+        out.append(dict(arrival=float(t[k]), ni=2 * nj, nj=nj, thickness=float(rng.uniform(0.08, 0.18)),
+                        camber=float(rng.uniform(0.0, 0.04)), mach=float(rng.uniform(0.3, 0.8)),
+                        aoa_deg=float(rng.uniform(-2.0, 6.0))))
+    return out

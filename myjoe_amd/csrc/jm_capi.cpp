@@ -10,9 +10,13 @@
 #include "jm_ctx.hpp"
 #include "jm_param.hpp"
 #include "jm_mesh.hpp"
+#include "jm_server.hpp"
 
 struct jm_mesh {
   jm::Mesh m;
+};
+struct jm_server {
+  std::unique_ptr<jm::Server> s;
 };
 struct jm_ctx {
   std::unique_ptr<jm::Ctx> c;
@@ -290,6 +294,14 @@ jm_status jm_update_bc(jm_ctx *c) {
   });
 }
 
+jm_status jm_step_graph(jm_ctx *c, jm_step_info *info) {
+  return guard([&] {
+    if (!c) throw std::exception();
+    fill_info(c->c->device_step(), info);
+    return JM_OK;
+  });
+}
+
 jm_status jm_step(jm_ctx *c, jm_step_info *info) {
   return guard([&] {
     if (!c) throw std::exception();
@@ -349,6 +361,83 @@ jm_status jm_time_residual_kernels(jm_ctx *c, int reps, double *ms_out, char *na
       std::strncpy(names, n.c_str(), names_len - 1);
       names[names_len - 1] = 0;
     }
+    return JM_OK;
+  });
+}
+
+jm_status jm_ctx_footprint(const jm_mesh *m, int precision, int64_t *bytes) {
+  return guard([&] {
+    if (!m || !bytes || (precision != JM_FP64 && precision != JM_MIXED)) throw std::exception();
+    *bytes = (int64_t)jm::Ctx::footprint(m->m, precision);
+    return JM_OK;
+  });
+}
+
+jm_status jm_server_create(int64_t mem_budget_bytes, int max_active, jm_server **out) {
+  return guard([&] {
+    if (!out || mem_budget_bytes <= 0 || max_active < 1) throw std::exception();
+    std::unique_ptr<jm_server> s(new jm_server);
+    s->s.reset(new jm::Server((size_t)mem_budget_bytes, max_active));
+    *out = s.release();
+    return JM_OK;
+  });
+}
+
+void jm_server_destroy(jm_server *s) { delete s; }
+
+jm_status jm_server_submit(jm_server *s, const jm_mesh *m, const char *joe_in, int precision, const double *rho,
+                           const double *rhou, const double *rhoE, int *ticket) {
+  return guard([&] {
+    if (!s || !m || !joe_in || !ticket || (precision != JM_FP64 && precision != JM_MIXED)) throw std::exception();
+    if ((rho || rhou || rhoE) && !(rho && rhou && rhoE)) throw std::exception();
+    *ticket = s->s->submit(&m->m, joe_in, precision, rho, rhou, rhoE);
+    return JM_OK;
+  });
+}
+
+jm_status jm_server_status(jm_server *s, int ticket, int *status) {
+  return guard([&] {
+    if (!s || !status) throw std::exception();
+    *status = s->s->status(ticket);
+    return JM_OK;
+  });
+}
+
+jm_status jm_server_wait(jm_server *s, int ticket) {
+  return guard([&] {
+    if (!s) throw std::exception();
+    s->s->wait(ticket);
+    return JM_OK;
+  });
+}
+
+jm_status jm_server_result(jm_server *s, int ticket, double *rho, double *rhou, double *rhoE, jm_solve_report *rep) {
+  return guard([&] {
+    if (!s) throw std::exception();
+    auto r = s->s->finished(ticket);
+    if (rep) {
+      rep->status = r->status;
+      rep->steps = r->steps;
+      for (int k = 0; k < 5; k++) {
+        rep->resid[k] = r->last.resid[k];
+        rep->resid0[k] = r->resid0[k];
+      }
+      rep->queued_s = r->t_start > 0 ? r->t_start - r->t_submit : r->t_done - r->t_submit;
+      rep->run_s = r->t_start > 0 ? r->t_done - r->t_start : 0.0;
+    }
+    if (r->status != jm::Server::kDone) throw std::runtime_error("solve failed: " + r->error);
+    const size_t n = r->rho.size();
+    if (rho) std::memcpy(rho, r->rho.data(), n * sizeof(double));
+    if (rhou) std::memcpy(rhou, r->rhou.data(), 3 * n * sizeof(double));
+    if (rhoE) std::memcpy(rhoE, r->rhoE.data(), n * sizeof(double));
+    return JM_OK;
+  });
+}
+
+jm_status jm_server_release(jm_server *s, int ticket) {
+  return guard([&] {
+    if (!s) throw std::exception();
+    s->s->release(ticket);
     return JM_OK;
   });
 }

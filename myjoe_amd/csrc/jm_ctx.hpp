@@ -25,6 +25,8 @@ enum SymmBc : int { kSymmOriginal = 0, kSymmNew = 1, kSymmAnalytical = 2 };
 // device cell numbering: the reference's own, face-count groups inside
 // 1024-cell tiles, Morton order of centroids, or both
 enum CellOrder : int { kOrderNative = 0, kOrderGrouped = 1, kOrderMorton = 2, kOrderMortonGrouped = 3 };
+// residual kernels: one lane per cell-face entry, or one thread per cell
+enum ResidKernels : int { kResidEntry = 0, kResidCell = 1 };
 
 // Parameters of UgpWithCvCompFlow::init / runBackwardEuler read from Joe.in.
 struct SolverParams {
@@ -44,9 +46,10 @@ struct SolverParams {
   int cfl_after = 100, cfl_interval = 10;
   double cfl_factor = 1.0, cfl_max = 100.0;
   int nsteps = 100;
-  double resid_energ_th = 0.0;
+  double resid_energ_th = 0.0, resid_energ_rel = 0.0;
   double rho_init = 1.0, p_init = 1.0, u_init[3] = {0, 0, 0};
   int cell_order = kOrderMorton;
+  int grad_kernel = kResidCell, flux_kernel = kResidEntry;
 };
 
 struct ZoneBc {
@@ -88,7 +91,19 @@ class Ctx {
   StepInfo step_end();
   // calcStateVariables + setBC on the resident state (after new HOOK values)
   void update_bc();
-  int run(int max_steps, double *hist);
+  // the same step with no host synchronisation inside: captured once into a
+  // hipGraph and replayed on the context's stream (the CFL ramp and step
+  // counter live on the device); _async launches it, _ready polls it,
+  // _result waits and returns its report
+  void device_step_async();
+  bool device_step_ready();
+  StepInfo device_step_result();
+  StepInfo device_step();
+  // device bytes a context for this mesh holds once it has stepped
+  static size_t footprint(const Mesh &m, int precision);
+  int run(int max_steps, double *hist, bool graph = false);
+  // doneSolver (JoeWithModels.h:42-95) after a step, plus RESID_ENERG_REL
+  bool done_solver(const StepInfo &s);
   void set_boundary_state(int ifa_f, int nface, const double *T, const double *vel, const double *p);
   double *device_state();
   double *device_rhs();

@@ -1,0 +1,243 @@
+// Serving loop for many steady solves on one GPU: requests are admitted as
+// long as their device footprint fits the memory budget and an active slot
+// is free; every active solve keeps its state and Jacobian resident, runs on
+// its own HIP stream and advances by graph-captured pseudo-time steps, so the
+// steps of different solves overlap on the device. A solve retires when the
+// reference's stop rule (doneSolver, plus RESID_ENERG_REL) says so.
+//
+// Two host threads: an admission thread builds contexts (host numbering,
+// uploads, initial state), a stepping thread launches and collects steps.
+// Each solve's kernels touch only its own buffers, so its result is the same
+// bits alone or in any mix.
+#include "jm_server.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstring>
+#include <stdexcept>
+
+namespace jm {
+
+namespace {
+double now_s() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+}  // namespace
+
+Server::Server(size_t mem_budget, int max_active) : budget_(mem_budget), max_active_(max_active) {
+  if (max_active < 1) throw std::invalid_argument("max_active must be at least 1");
+  if (hipGetDevice(&device_) != hipSuccess) throw std::runtime_error("no HIP device available");
+  admit_thread_ = std::thread([this] { admit_loop(); });
+  step_thread_ = std::thread([this] { step_loop(); });
+}
+
+Server::~Server() {
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    stop_ = true;
+  }
+  cv_.notify_all();
+  admit_thread_.join();
+  step_thread_.join();
+  for (auto &a : active_) {
+    a.ctx.reset();
+    if (a.stream) (void)hipStreamDestroy(a.stream);
+  }
+  for (auto &a : ready_) {
+    a.ctx.reset();
+    if (a.stream) (void)hipStreamDestroy(a.stream);
+  }
+}
+
+int Server::submit(const Mesh *mesh, const std::string &joe_in, int precision, const double *rho,
+                   const double *rhou, const double *rhoE) {
+  if (!mesh) throw std::invalid_argument("null mesh");
+  SolverParams p;
+  std::vector<ZoneBc> zbc;
+  parse_params(joe_in, *mesh, p, zbc);  // reject bad input at submission
+  auto r = std::make_shared<Request>();
+  r->mesh = mesh;
+  r->joe_in = joe_in;
+  r->precision = precision;
+  r->footprint = Ctx::footprint(*mesh, precision);
+  if (r->footprint > budget_) throw std::invalid_argument("solve does not fit the server's memory budget");
+  if (rho) {
+    const size_t n = mesh->ncv;
+    r->init.resize(5 * n);
+    std::memcpy(r->init.data(), rho, n * sizeof(double));
+    std::memcpy(r->init.data() + n, rhou, 3 * n * sizeof(double));
+    std::memcpy(r->init.data() + 4 * n, rhoE, n * sizeof(double));
+  }
+  r->t_submit = now_s();
+  std::lock_guard<std::mutex> g(mu_);
+  r->id = next_id_++;
+  reqs_[r->id] = r;
+  queue_.push_back(r->id);
+  cv_.notify_all();
+  return r->id;
+}
+
+int Server::status(int id) {
+  std::lock_guard<std::mutex> g(mu_);
+  auto it = reqs_.find(id);
+  if (it == reqs_.end()) throw std::invalid_argument("unknown ticket");
+  return it->second->status;
+}
+
+void Server::wait(int id) {
+  std::unique_lock<std::mutex> g(mu_);
+  auto it = reqs_.find(id);
+  if (it == reqs_.end()) throw std::invalid_argument("unknown ticket");
+  auto r = it->second;
+  cv_.wait(g, [&] { return r->status >= kDone; });
+}
+
+std::shared_ptr<Server::Request> Server::finished(int id) {
+  std::lock_guard<std::mutex> g(mu_);
+  auto it = reqs_.find(id);
+  if (it == reqs_.end()) throw std::invalid_argument("unknown ticket");
+  if (it->second->status < kDone) throw std::logic_error("solve not finished");
+  return it->second;
+}
+
+void Server::release(int id) {
+  std::lock_guard<std::mutex> g(mu_);
+  auto it = reqs_.find(id);
+  if (it == reqs_.end()) throw std::invalid_argument("unknown ticket");
+  if (it->second->status < kDone) throw std::logic_error("solve not finished");
+  reqs_.erase(it);
+}
+
+// admission: FIFO; the head waits until its footprint fits and a slot is free
+void Server::admit_loop() {
+  (void)hipSetDevice(device_);
+  for (;;) {
+    std::shared_ptr<Request> r;
+    {
+      std::unique_lock<std::mutex> g(mu_);
+      cv_.wait(g, [&] {
+        if (stop_) return true;
+        if (queue_.empty()) return false;
+        auto h = reqs_[queue_.front()];
+        return slots_used_ < max_active_ && used_ + h->footprint <= budget_;
+      });
+      if (stop_) return;
+      r = reqs_[queue_.front()];
+      queue_.pop_front();
+      slots_used_++;
+      used_ += r->footprint;
+      r->status = kRunning;
+    }
+    Active a;
+    a.req = r;
+    try {
+      if (hipStreamCreateWithFlags(&a.stream, hipStreamNonBlocking) != hipSuccess)
+        throw std::runtime_error("HIP stream creation failed");
+      a.ctx.reset(new Ctx(*r->mesh, r->joe_in, r->precision, a.stream));
+      if (r->init.empty()) {
+        a.ctx->init_uniform();
+      } else {
+        const size_t n = r->mesh->ncv;
+        a.ctx->set_state(r->init.data(), r->init.data() + n, r->init.data() + 4 * n);
+        r->init.clear();
+        r->init.shrink_to_fit();
+      }
+      r->t_start = now_s();
+      std::lock_guard<std::mutex> g(mu_);
+      ready_.push_back(std::move(a));
+    } catch (std::exception &e) {
+      a.ctx.reset();
+      if (a.stream) (void)hipStreamDestroy(a.stream);
+      std::lock_guard<std::mutex> g(mu_);
+      r->error = e.what();
+      r->status = kFailed;
+      r->t_done = now_s();
+      slots_used_--;
+      used_ -= r->footprint;
+      cv_.notify_all();
+    }
+  }
+}
+
+void Server::retire(Active &a, bool failed, const std::string &err) {
+  auto r = a.req;
+  if (!failed) {
+    try {
+      const size_t n = r->mesh->ncv;
+      r->rho.resize(n);
+      r->rhou.resize(3 * n);
+      r->rhoE.resize(n);
+      a.ctx->get_state(r->rho.data(), r->rhou.data(), r->rhoE.data());
+    } catch (std::exception &e) {
+      failed = true;
+      r->error = e.what();
+    }
+  } else {
+    r->error = err;
+  }
+  a.ctx.reset();
+  if (a.stream) (void)hipStreamDestroy(a.stream);
+  a.stream = nullptr;
+  std::lock_guard<std::mutex> g(mu_);
+  r->t_done = now_s();
+  r->status = failed ? kFailed : kDone;
+  slots_used_--;
+  used_ -= r->footprint;
+  cv_.notify_all();
+}
+
+void Server::step_loop() {
+  (void)hipSetDevice(device_);
+  for (;;) {
+    {
+      std::unique_lock<std::mutex> g(mu_);
+      if (stop_) return;
+      for (auto &a : ready_) active_.push_back(std::move(a));
+      ready_.clear();
+      if (active_.empty()) {
+        cv_.wait_for(g, std::chrono::milliseconds(2), [&] { return stop_ || !ready_.empty(); });
+        continue;
+      }
+    }
+    bool progress = false;
+    for (auto &a : active_) {
+      if (a.inflight) continue;
+      try {
+        a.ctx->device_step_async();
+        a.inflight = true;
+        progress = true;
+      } catch (std::exception &e) {
+        a.dead = true;
+        retire(a, true, e.what());
+      }
+    }
+    for (auto &a : active_) {
+      if (a.dead || !a.inflight) continue;
+      try {
+        if (!a.ctx->device_step_ready()) continue;
+        StepInfo s = a.ctx->device_step_result();
+        a.inflight = false;
+        progress = true;
+        auto &r = *a.req;
+        r.steps = s.step;
+        r.last = s;
+        if (s.step == 1)
+          for (int k = 0; k < 5; k++) r.resid0[k] = s.resid[k];
+        if (a.ctx->done_solver(s)) {
+          a.dead = true;
+          retire(a, false, "");
+        }
+      } catch (std::exception &e) {
+        a.dead = true;
+        retire(a, true, e.what());
+      }
+    }
+    active_.erase(std::remove_if(active_.begin(), active_.end(), [](const Active &a) { return a.dead; }),
+                  active_.end());
+    if (!progress) std::this_thread::sleep_for(std::chrono::microseconds(20));
+  }
+}
+
+}  // namespace jm

@@ -56,6 +56,9 @@ struct DMesh {
   const double *vol;      // [ncv]
   const ZoneDev *zones;
   const int *sched;       // [ncv] cell visiting order of the per-cell kernels
+  const int *ecell;       // [entries] cell of each entry
+  const int *wave_c0;     // [nwave+1] first cell of each 64-lane wavefront
+  int nwave;
 };
 
 struct DParams {
@@ -71,6 +74,15 @@ struct SolverScalars {
   double resid[5];
   int iter, done, maxiter, pad;
   double zero_abs, zero_rel;
+  // pseudo-time step counter and CFL of the device-driven (graph) step
+  int step_no, pad2;
+  double cfl_cur;
+};
+
+// what a device-driven step reports back, written to host-mapped memory
+struct StepMail {
+  int step, lin_iters, lin_status, pad;
+  double cfl, dt_min, resid[5];
 };
 
 namespace {
@@ -413,6 +425,232 @@ __global__ void __launch_bounds__(kBlock) k_grad(DMesh m, DParams P, const doubl
   o[15] = ar;
 }
 
+// Entry-parallel form of k_grad: one lane per cell-face entry, each 64-lane
+// wavefront covering whole cells. Every lane computes its entry's terms;
+// the cell's first lane (its owner) adds them from LDS in entry order, which
+// is the reference's order, so the sums are the same bits as k_grad's.
+// Phase 1: gradient terms. Phase 2: neighbour values and face offsets for
+// the limiter, reduced by the owner, which writes the limited row.
+constexpr int kWaves = kBlock / 64;
+
+struct EntryLane {
+  int e, c, pos;  // entry (-1: idle lane), its cell, position in the cell's list
+};
+
+__device__ __forceinline__ EntryLane entry_lane(const DMesh &m) {
+  EntryLane L{-1, -1, 0};
+  const int w = xcd_block(blockIdx.x, gridDim.x) * kWaves + (threadIdx.x >> 6);
+  if (w < m.nwave) {
+    const int c0 = m.wave_c0[w], c1 = m.wave_c0[w + 1];
+    const int e = m.cfl_i[c0] + (threadIdx.x & 63);
+    if (e < m.cfl_i[c1]) {
+      L.e = e;
+      L.c = m.ecell[e];
+      L.pos = e - m.cfl_i[L.c];
+    }
+  }
+  return L;
+}
+
+template <bool LS>
+__global__ void __launch_bounds__(kBlock) k_grad_e(DMesh m, DParams P, const double *__restrict__ prim,
+                                                   const double *__restrict__ bfa, double *__restrict__ grad) {
+  __shared__ double lds[kBlock * 15];
+  double *slot = lds + 15 * threadIdx.x;
+  const EntryLane L = entry_lane(m);
+  const bool owner = L.e >= 0 && L.pos == 0;
+  double rho = 0.0, u[3] = {0.0, 0.0, 0.0}, p = 0.0, xcv[3] = {0.0, 0.0, 0.0};
+  double nv[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // neighbour rho, u, v, w, p
+  double xfa[3] = {0.0, 0.0, 0.0};
+  bool internal = false;
+  if (L.e >= 0) {
+    const double *pc = prim + 8 * (size_t)L.c;
+    double2 a = ld2(pc), b = ld2(pc + 2), cc = ld2(pc + 4), dd = ld2(pc + 6);
+    rho = a.x; u[0] = a.y; u[1] = b.x; u[2] = b.y; p = cc.x;
+    xcv[0] = cc.y; xcv[1] = dd.x; xcv[2] = dd.y;
+    const int2 en = m.cfl[L.e];
+    const int f = en.x & 0x7fffffff;
+    internal = en.y >= 0;
+    double t[15];
+    if (internal) {
+      const double *pn = prim + 8 * (size_t)en.y;
+      double2 na = ld2(pn), nb = ld2(pn + 2), nc = ld2(pn + 4), nd = ld2(pn + 6);
+      nv[0] = na.x; nv[1] = na.y; nv[2] = nb.x; nv[3] = nb.y; nv[4] = nc.x;
+      const double *geo = m.fgeo + 8 * (size_t)f;
+      xfa[0] = geo[4]; xfa[1] = geo[5]; xfa[2] = geo[6];
+      if (LS) {
+        const double *l = m.lsg + 3 * (size_t)(m.nbocv_i[L.c] + L.pos + 1);
+        const double l0 = l[0], l1 = l[1], l2 = l[2];
+        t[0] = l0 * nv[0]; t[1] = l1 * nv[0]; t[2] = l2 * nv[0];
+        t[3] = l0 * nv[4]; t[4] = l1 * nv[4]; t[5] = l2 * nv[4];
+        for (int i = 0; i < 3; i++) {
+          t[6 + 3 * i] = l0 * nv[1 + i];
+          t[7 + 3 * i] = l1 * nv[1 + i];
+          t[8 + 3 * i] = l2 * nv[1 + i];
+        }
+      } else {
+        // distance weights (GG_WITH_DIST, UgpWithCv.h:37, :1104-1110)
+        const double xn[3] = {nc.y, nd.x, nd.y};
+        double ds[3], dn[3];
+        for (int j = 0; j < 3; j++) {
+          ds[j] = xfa[j] - xcv[j];
+          dn[j] = xfa[j] - xn[j];
+        }
+        const bool s0 = (en.x >= 0);
+        const double wself = sqrt(dot3(ds, ds)), wnb = sqrt(dot3(dn, dn));
+        const double w0 = s0 ? wself : wnb, w1 = s0 ? wnb : wself;
+        const double *fn = m.fnorm + 3 * (size_t)f;
+        const double n[3] = {fn[0], fn[1], fn[2]};
+        const double fr = s0 ? (w1 * rho + w0 * nv[0]) / (w0 + w1) : (w1 * nv[0] + w0 * rho) / (w0 + w1);
+        const double fp = s0 ? (w1 * p + w0 * nv[4]) / (w0 + w1) : (w1 * nv[4] + w0 * p) / (w0 + w1);
+        for (int j = 0; j < 3; j++) {
+          t[j] = s0 ? fr * n[j] : -(fr * n[j]);
+          t[3 + j] = s0 ? fp * n[j] : -(fp * n[j]);
+        }
+        for (int i = 0; i < 3; i++) {
+          const double fu = s0 ? (w1 * u[i] + w0 * nv[1 + i]) / (w0 + w1) : (w1 * nv[1 + i] + w0 * u[i]) / (w0 + w1);
+          for (int j = 0; j < 3; j++) t[6 + 3 * i + j] = s0 ? fu * n[j] : -(fu * n[j]);
+        }
+      }
+    } else {
+      const double *bb = bfa + 8 * (size_t)f;
+      const double rb = bb[0], ub[3] = {bb[1], bb[2], bb[3]}, pb = bb[4];
+      double w[3];
+      if (LS) {
+        const double *bl = m.bfa_lsg + 3 * (size_t)f;
+        w[0] = bl[0]; w[1] = bl[1]; w[2] = bl[2];
+      } else {
+        const double *fn = m.fnorm + 3 * (size_t)f;
+        w[0] = fn[0]; w[1] = fn[1]; w[2] = fn[2];
+      }
+      for (int j = 0; j < 3; j++) {
+        t[j] = w[j] * rb;
+        t[3 + j] = w[j] * pb;
+      }
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) t[6 + 3 * i + j] = w[j] * ub[i];
+    }
+    for (int k = 0; k < 15; k++) slot[k] = t[k];
+  }
+  __syncthreads();
+  // Ordered sums, spread over the cell's lanes: the lane at position pos of a
+  // cell with n entries (n >= 4) sums fields pos, pos + n, ... in entry order,
+  // starting from the diagonal term (LS) or zero (GG).
+  const double *base = lds + 15 * (threadIdx.x - L.pos);
+  const int n = L.e >= 0 ? m.cfl_i[L.c + 1] - m.cfl_i[L.c] : 0;
+  double gs[4] = {0.0, 0.0, 0.0, 0.0};
+  if (L.e >= 0) {
+    double l0[3] = {0.0, 0.0, 0.0};
+    if (LS) {
+      const double *l = m.lsg + 3 * (size_t)m.nbocv_i[L.c];
+      l0[0] = l[0]; l0[1] = l[1]; l0[2] = l[2];
+    }
+    const double vol = LS ? 1.0 : m.vol[L.c];
+    for (int r = 0; r < 4; r++) {
+      const int k = L.pos + r * n;
+      if (k >= 15) break;
+      const double val = k < 3 ? rho : (k < 6 ? p : u[(k - 6) / 3]);
+      double a = LS ? val * l0[k % 3] : 0.0;
+      for (int j = 0; j < n; j++) a += base[15 * j + k];
+      gs[r] = LS ? a : a / vol;
+    }
+  }
+  __syncthreads();
+  // neighbour values (internal faces) and the partial sums; field k is then
+  // at base[15 * (k % n) + 6 + k / n]
+  if (L.e >= 0) {
+    slot[0] = internal ? 1.0 : 0.0;
+    for (int k = 0; k < 5; k++) slot[1 + k] = nv[k];
+    for (int r = 0; r < 4; r++) slot[6 + r] = gs[r];
+  }
+  __syncthreads();
+  // bounds over the cell and its neighbours, spread the same way: quantity
+  // q < 5 is the max of rho, u, v, w, p (q), q >= 5 the min (q - 5); bound q
+  // is then at base[15 * (q % n) + 10 + q / n]
+  if (L.e >= 0 && P.limiter) {
+    for (int r = 0; r < 3; r++) {
+      const int q = L.pos + r * n;
+      if (q >= 10) break;
+      const int fld = q % 5;
+      const double own = fld == 0 ? rho : (fld == 4 ? p : u[fld - 1]);
+      double a = own;
+      for (int j = 0; j < n; j++) {
+        const double *sj = base + 15 * j;
+        if (sj[0] == 0.0) continue;
+        a = q < 5 ? Dev::mx(a, sj[1 + fld]) : Dev::mn(a, sj[1 + fld]);
+      }
+      slot[10 + r] = a;
+    }
+  }
+  __syncthreads();
+  // gather the cell's gradient and bounds (no division: walk lanes, wrap columns)
+  double g[15], bnd[10];
+  if (L.e >= 0) {
+    int li = 0, col = 0;
+    for (int k = 0; k < 15; k++) {
+      g[k] = base[15 * li + 6 + col];
+      if (++li == n) {
+        li = 0;
+        col++;
+      }
+    }
+    if (P.limiter) {
+      li = 0;
+      col = 0;
+      for (int q = 0; q < 10; q++) {
+        bnd[q] = base[15 * li + 10 + col];
+        if (++li == n) {
+          li = 0;
+          col++;
+        }
+      }
+    }
+  }
+  // every internal-face lane: its face's smooth BJ factors (UgpWithCv.h:913-925)
+  double fac[5] = {1.0, 1.0, 1.0, 1.0, 1.0};  // rho, p, u, v, w
+  if (L.e >= 0 && internal && P.limiter) {
+    const double sos = sqrt(P.gamma * p / rho);
+    double e2u = P.eps * sos;
+    e2u = e2u * e2u;
+    double e2r = Dev::mx(P.eps * fabs(rho), 1.0e-8);
+    e2r = e2r * e2r;
+    double e2p = Dev::mx(P.eps * fabs(p), 1.0e-8);
+    e2p = e2p * e2p;
+    const double r0[3] = {xfa[0] - xcv[0], xfa[1] - xcv[1], xfa[2] - xcv[2]};
+    fac[0] = bj_face(1.0, rho, rho + dot3(r0, g), bnd[0], bnd[5], e2r);
+    fac[1] = bj_face(1.0, p, p + dot3(r0, g + 3), bnd[4], bnd[9], e2p);
+    for (int i = 0; i < 3; i++)
+      fac[2 + i] = bj_face(1.0, u[i], u[i] + dot3(r0, g + 6 + 3 * i), bnd[1 + i], bnd[6 + i], e2u);
+  }
+  // slots 0..4 are free now (the reads above use 6..12)
+  if (L.e >= 0)
+    for (int k = 0; k < 5; k++) slot[k] = fac[k];
+  __syncthreads();
+  if (!owner) return;
+  double au[3] = {1.0, 1.0, 1.0}, ar = 1.0, ap = 1.0;
+  if (P.limiter) {
+    // min over faces is exact in any order
+    for (int j = 0; j < n; j++) {
+      const double *sj = base + 15 * j;
+      ar = Dev::mn(ar, sj[0]);
+      ap = Dev::mn(ap, sj[1]);
+      for (int i = 0; i < 3; i++) au[i] = Dev::mn(au[i], sj[2 + i]);
+    }
+    bool lu[3] = {(bnd[1] - bnd[6]) > 1.0e-8, (bnd[2] - bnd[7]) > 1.0e-8, (bnd[3] - bnd[8]) > 1.0e-8};
+    bool lr = (bnd[0] - bnd[5]) > 1.0e-12, lp = (bnd[4] - bnd[9]) > 1.0e-12;
+    for (int i = 0; i < 3; i++)
+      if (!lu[i]) au[i] = 0.0;
+    if (!lr) ar = 0.0;
+    if (!lp) ap = 0.0;
+  }
+  double *o = grad + 16 * (size_t)L.c;
+  o[0] = g[0] * ar; o[1] = g[1] * ar; o[2] = g[2] * ar;
+  o[3] = g[3] * ap; o[4] = g[4] * ap; o[5] = g[5] * ap;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) o[6 + 3 * i + j] = g[6 + 3 * i + j] * au[i];
+  o[15] = ar;
+}
+
 // Reconstructed face state of one side (JoeWithModels.cpp:2832-2921): rho and
 // p extrapolated first; both revert to the cell value when either is not
 // positive, and the velocity is then left first order.
@@ -458,6 +696,7 @@ __device__ __forceinline__ void face_flux(int scheme, double F[5], const Side &L
 struct DtArgs {
   double cfl, dt_fixed;
   int mode;
+  const double *cfl_dev;  // when set, the CFL number is read from device memory (graph-captured steps)
 };
 
 // One cell's residual; also its MAX_BASED local time step (UgpWithCvCompFlow.cpp:27-60,
@@ -584,7 +823,7 @@ __device__ __forceinline__ double flux_cell(const DMesh &m, const DParams &P, co
   }
   for (int k = 0; k < 5; k++) rhs[(size_t)k * m.ncv + c] = acc[k];
   if (nlow) atomicAdd(nfirst, nlow);
-  const double dtc = dta.mode == 2 ? dta.dt_fixed : dta.cfl * vol / lambdaMax;
+  const double dtc = dta.mode == 2 ? dta.dt_fixed : (dta.cfl_dev ? *dta.cfl_dev : dta.cfl) * vol / lambdaMax;
   dt[c] = dtc;
   return dtc;
 }
@@ -847,9 +1086,193 @@ __global__ void __launch_bounds__(kBlock) k_flux(DMesh m, DParams P, const doubl
   block_partials<1>(v, partials, sh, kMin);
 }
 
+// Entry-parallel form of k_flux: one lane per cell-face entry computes that
+// face's flux (signed for this cell) and the face's wave-speed bound; the
+// cell's owner lane adds the terms from LDS in entry order (the reference's
+// order), applies the gravity source, and writes the residual and local dt.
+template <int SCHEME, bool SND>
+__global__ void __launch_bounds__(kBlock) k_flux_e(DMesh m, DParams P, const double *__restrict__ prim,
+                                                   const double *__restrict__ grad, const double *__restrict__ bfa,
+                                                   const double *__restrict__ q, double *__restrict__ rhs,
+                                                   int *__restrict__ nfirst, DtArgs dta, double *__restrict__ dt,
+                                                   double *__restrict__ partials) {
+  __shared__ double lds[kBlock * 6];
+  __shared__ double sh[16];
+  double *slot = lds + 6 * threadIdx.x;
+  const EntryLane L = entry_lane(m);
+  const double gam = P.gamma;
+  double pc[8], gc[15];
+  if (L.e >= 0) {
+    const double *s = prim + 8 * (size_t)L.c;
+    for (int k = 0; k < 8; k += 2) {
+      double2 v = ld2(s + k);
+      pc[k] = v.x;
+      pc[k + 1] = v.y;
+    }
+    if (SND) {
+      const double *g = grad + 16 * (size_t)L.c;
+      for (int k = 0; k < 14; k += 2) {
+        double2 v = ld2(g + k);
+        gc[k] = v.x;
+        gc[k + 1] = v.y;
+      }
+      gc[14] = g[14];
+    }
+    const int2 en = m.cfl[L.e];
+    const int f = en.x & 0x7fffffff;
+    const double *geo = m.fgeo + 8 * (size_t)f;
+    double n[3], area, xfa[3];
+    {
+      double2 a = ld2(geo), b = ld2(geo + 2), cc = ld2(geo + 4);
+      n[0] = a.x; n[1] = a.y; n[2] = b.x; area = b.y;
+      xfa[0] = cc.x; xfa[1] = cc.y; xfa[2] = geo[6];
+    }
+    {
+      const double ru[3] = {q[m.ncv + L.c], q[2 * m.ncv + L.c], q[3 * m.ncv + L.c]};
+      const double cs = sqrt(gam * pc[4] / pc[0]);
+      const double Uk = dot3(ru, n) / pc[0];
+      slot[5] = (fabs(Uk) + cs) * area;
+    }
+    Side own_cc;
+    own_cc.rho = pc[0];
+    own_cc.u[0] = pc[1]; own_cc.u[1] = pc[2]; own_cc.u[2] = pc[3];
+    own_cc.p = pc[4];
+    double F[5];
+    double sgn = -1.0;
+    bool low = false;
+    if (en.y >= 0) {
+      const bool s0 = (en.x >= 0);
+      double pn[8], gn[15];
+      const double *sp = prim + 8 * (size_t)en.y;
+      for (int k = 0; k < 8; k += 2) {
+        double2 v = ld2(sp + k);
+        pn[k] = v.x;
+        pn[k + 1] = v.y;
+      }
+      Side so, sn;
+      if (SND) {
+        const double *g = grad + 16 * (size_t)en.y;
+        for (int k = 0; k < 14; k += 2) {
+          double2 v = ld2(g + k);
+          gn[k] = v.x;
+          gn[k + 1] = v.y;
+        }
+        gn[14] = g[14];
+        low = !reconstruct(so, pc, gc, xfa);
+        reconstruct(sn, pn, gn, xfa);
+      } else {
+        so = own_cc;
+        sn.rho = pn[0]; sn.u[0] = pn[1]; sn.u[1] = pn[2]; sn.u[2] = pn[3]; sn.p = pn[4];
+      }
+      if (s0) {
+        face_flux(SCHEME, F, so, sn, gam, area, n);
+      } else {
+        face_flux(SCHEME, F, sn, so, gam, area, n);
+        sgn = 1.0;
+      }
+    } else {
+      const ZoneDev &z = m.zones[-en.y - 1];
+      const double *bb = bfa + 8 * (size_t)f;
+      Side sb;
+      sb.rho = bb[0]; sb.u[0] = bb[1]; sb.u[1] = bb[2]; sb.u[2] = bb[3]; sb.p = bb[4];
+      if (z.bc == kBcSymmetry || z.bc == kBcWall) {
+        face_flux(SCHEME, F, sb, sb, gam, area, n);
+        if (z.bc == kBcSymmetry && P.symm_bc == kSymmAnalytical) {
+          F[0] = 0.0;
+          for (int i = 0; i < 3; i++) F[1 + i] = sb.p * n[i] * area;
+          F[4] = 0.0;
+        }
+      } else {
+        Side so = own_cc;
+        if (SND) low = !reconstruct(so, pc, gc, xfa);
+        face_flux(SCHEME, F, so, sb, gam, area, n);
+      }
+    }
+    // acc - F == acc + (-F) exactly, so the owner only adds
+    for (int k = 0; k < 5; k++) slot[k] = sgn < 0.0 ? -F[k] : F[k];
+    if (low) atomicAdd(nfirst, 1);
+  }
+  __syncthreads();
+  double dtc = 1.0e20;
+  if (L.e >= 0 && L.pos == 0) {
+    const int c = L.c;
+    const int nent = m.cfl_i[c + 1] - m.cfl_i[c];
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double lambdaMax = 0.0;
+    for (int j = 0; j < nent; j++) {
+      const double *s = slot + 6 * j;
+      for (int k = 0; k < 5; k++) acc[k] += s[k];
+      lambdaMax = Dev::mx(lambdaMax, s[5]);
+    }
+    // sourceHook: gravity (JoeWithModels.h:111-129)
+    const double vol = m.vol[c];
+    if (P.gravity_on) {
+      const double ru[3] = {q[m.ncv + c], q[2 * m.ncv + c], q[3 * m.ncv + c]};
+      acc[1] += vol * pc[0] * P.g[0];
+      acc[2] += vol * pc[0] * P.g[1];
+      acc[3] += vol * pc[0] * P.g[2];
+      acc[4] += vol * (P.g[0] * ru[0] + P.g[1] * ru[1] + P.g[2] * ru[2]);
+    } else {
+      acc[1] += 0.0;
+      acc[2] += 0.0;
+      acc[3] += 0.0;
+      bool allneg = signbit(pc[1]) && signbit(pc[2]) && signbit(pc[3]);
+      acc[4] += allneg ? -0.0 : 0.0;
+    }
+    for (int k = 0; k < 5; k++) rhs[(size_t)k * m.ncv + c] = acc[k];
+    dtc = dta.mode == 2 ? dta.dt_fixed : (dta.cfl_dev ? *dta.cfl_dev : dta.cfl) * vol / lambdaMax;
+    dt[c] = dtc;
+  }
+  double v[1] = {dtc};
+  block_partials<1>(v, partials, sh, kMin);
+}
+
 __global__ void k_dt_fill(int ncv, double *dt, const SolverScalars *S) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c < ncv) dt[c] = S->dt_min;
+}
+
+// CFL ramp of the pseudo-time loop (JoeWithModels.cpp:975-978) on the device
+__global__ void k_ramp(SolverScalars *S, int after, int interval, double factor, double cfl_max) {
+  if (threadIdx.x != 0) return;
+  const int st = S->step_no + 1;
+  S->step_no = st;
+  if ((st >= after) && (interval > 0) && (st % interval == 0) && (S->cfl_cur < cfl_max)) S->cfl_cur *= factor;
+}
+
+__global__ void k_set_step(SolverScalars *S, int step, double cfl) {
+  if (threadIdx.x != 0) return;
+  S->step_no = step;
+  S->cfl_cur = cfl;
+}
+
+// BiCGStab scalars at the start of a solve (leaves dt_min, residuals, step)
+__global__ void k_solver_init(SolverScalars *S, int maxiter, double zero_abs, double zero_rel) {
+  if (threadIdx.x != 0) return;
+  S->rho = S->rho_prime = S->alpha = S->omega = S->beta = S->gamma = 0.0;
+  S->res0_max = S->res_max = 0.0;
+  S->iter = 0;
+  S->done = 0;
+  S->maxiter = maxiter;
+  S->zero_abs = zero_abs;
+  S->zero_rel = zero_rel;
+}
+
+template <typename T>
+__global__ void k_fill(size_t n, T *__restrict__ p, T v) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_publish(const SolverScalars *S, StepMail *out) {
+  if (threadIdx.x != 0) return;
+  out->step = S->step_no;
+  out->lin_iters = S->iter;
+  out->lin_status = S->done;
+  out->cfl = S->cfl_cur;
+  out->dt_min = S->dt_min;
+  for (int k = 0; k < 5; k++) out->resid[k] = S->resid[k];
+  __threadfence_system();
 }
 
 // ---------------------------------------------------------------------------
@@ -1080,6 +1503,11 @@ struct DBuf {
   ~DBuf() { free(); }
 };
 
+template <typename T>
+void ensure(DBuf<T> &b, size_t n) {
+  if (b.n != n) b.alloc(n);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1104,6 +1532,15 @@ void parse_params(const std::string &joe_in, const Mesh &m, SolverParams &p, std
     else if (o == "MORTON_GROUPED") p.cell_order = kOrderMortonGrouped;
     else throw std::invalid_argument("GPU_CELL_ORDER = " + o + " (NATIVE, GROUPED, MORTON, MORTON_GROUPED)");
   }
+  // residual kernel forms (this build's own options; results do not depend on them)
+  auto kform = [&](const char *key, const char *def) {
+    const std::string o = pm.str(key, def);
+    if (o == "ENTRY") return (int)kResidEntry;
+    if (o == "CELL") return (int)kResidCell;
+    throw std::invalid_argument(std::string(key) + " = " + o + " (ENTRY, CELL)");
+  };
+  p.grad_kernel = kform("GPU_GRAD_KERNEL", "CELL");
+  p.flux_kernel = kform("GPU_FLUX_KERNEL", "ENTRY");
   p.gravity[0] = pm.dbl("GX", 0.0);
   p.gravity[1] = pm.dbl("GY", 0.0);
   p.gravity[2] = pm.dbl("GZ", 0.0);
@@ -1167,6 +1604,9 @@ void parse_params(const std::string &joe_in, const Mesh &m, SolverParams &p, std
   }
   p.nsteps = pm.integer("NSTEPS", 100);
   p.resid_energ_th = pm.dbl("RESID_ENERG_TH", 0.0);
+  // this build's own stop rule for served solves: energy L1 residual below
+  // RESID_ENERG_REL times that of the first step (0: off)
+  p.resid_energ_rel = pm.dbl("RESID_ENERG_REL", 0.0);
   p.rho_init = pm.has("RHO_INITIAL") ? pm.dbl("RHO_INITIAL") : p.rho_ref;
   p.p_init = pm.has("P_INITIAL") ? pm.dbl("P_INITIAL") : p.p_ref;
   if (pm.has("U_INITIAL")) {
@@ -1231,6 +1671,8 @@ struct Ctx::Impl {
   DBuf<int> cfl_i, cfl_noc, nbocv_i, nbocv_v, bfa_cell, bfa_zone;
   DBuf<int2> cfl_e;
   DBuf<int> sched;
+  DBuf<int> ecell, wave_c0;
+  int nwave = 0;
   DBuf<ZoneDev> zones;
   std::vector<int> perm;              // device cell -> reference cell
   std::vector<int> h_dev_nbocv_i, h_ref_nbocv_i;
@@ -1250,10 +1692,26 @@ struct Ctx::Impl {
   bool begun = false;
 
   template <typename T>
-  void solve(int &iters, int &status);
+  void solve(int &iters, int &status, bool graph = false);
+  // device-driven step: CFL ramp, residual, Jacobian, every BiCGStab
+  // iteration, update, refresh, L1 and a report to host-mapped memory; no
+  // host synchronisation, so it can be captured in a hipGraph
+  template <typename T>
+  void enqueue_device_step();
+  hipStream_t cap_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  hipEvent_t ev_step = nullptr;
+  StepMail *mail_h = nullptr, *mail_d = nullptr;
+  bool dev_step_dirty = true;
+  bool graph_inflight = false;
+  double resid0_energy = 0.0;
   template <typename T>
   T *A_();
   void launch_state_bc();
+  void launch_grad(hipStream_t s);
+  void launch_flux(hipStream_t s, const double *cfl_dev = nullptr);
+  int flux_blocks() const;
   void launch_grad_flux();
   template <typename T>
   void launch_jac(bool step_mode);
@@ -1476,6 +1934,24 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
     dci[ncv] = (int)dcf.size();
     d.cfl_i.upload(dci.data(), dci.size());
     d.cfl_e.upload(dcf.data(), dcf.size());
+    // entry-parallel kernels: the cell of every entry, and wavefronts of 64
+    // lanes that each cover whole cells (consecutive cells while their
+    // entries fit)
+    std::vector<int> ecell(std::max<size_t>(1, dcf.size()));
+    for (int cd = 0; cd < ncv; cd++)
+      for (int e = dci[cd]; e < dci[cd + 1]; e++) ecell[e] = cd;
+    std::vector<int> wc;
+    int c = 0;
+    while (c < ncv) {
+      wc.push_back(c);
+      const int eb = dci[c];
+      if (dci[c + 1] - eb > 64) throw std::runtime_error("a cell with more than 64 faces");
+      while (c < ncv && dci[c + 1] - eb <= 64) c++;
+    }
+    wc.push_back(ncv);
+    d.nwave = (int)wc.size() - 1;
+    d.ecell.upload(ecell.data(), ecell.size());
+    d.wave_c0.upload(wc.data(), wc.size());
     d.cfl_noc.upload(noc.data(), noc.size());
   }
   std::vector<int> bcell(std::max(1, nfa_b), 0);
@@ -1535,6 +2011,9 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
   d.dm.vol = d.vol.p;
   d.dm.zones = d.zones.p;
   d.dm.sched = d.sched.p;
+  d.dm.ecell = d.ecell.p;
+  d.dm.wave_c0 = d.wave_c0.p;
+  d.dm.nwave = d.nwave;
 
   d.dp.gamma = d.par.gamma;
   d.dp.R = d.par.R_gas;
@@ -1551,7 +2030,120 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
 }
 
 Ctx::~Ctx() {
-  if (d_ && d_->h_scal) (void)hipHostFree(d_->h_scal);
+  if (!d_) return;
+  Impl &d = *d_;
+  if (d.graph_inflight && d.ev_step) (void)hipEventSynchronize(d.ev_step);
+  (void)hipStreamSynchronize(d.stream);
+  if (d.gexec) (void)hipGraphExecDestroy(d.gexec);
+  if (d.graph) (void)hipGraphDestroy(d.graph);
+  if (d.cap_stream) (void)hipStreamDestroy(d.cap_stream);
+  if (d.ev_step) (void)hipEventDestroy(d.ev_step);
+  if (d.mail_h) (void)hipHostFree(d.mail_h);
+  if (d.h_scal) (void)hipHostFree(d.h_scal);
+}
+
+// Device bytes a context holds once it has stepped (mesh, state, Jacobian,
+// preconditioner, Krylov vectors, reduction partials).
+size_t Ctx::footprint(const Mesh &m, int precision) {
+  const size_t ncv = m.ncv, nfa = m.nfa, nfa_b = m.nfa_b, nnz = m.nbocv_s();
+  const size_t ent = 2 * (nfa - nfa_b) + nfa_b;
+  const size_t sJ = precision == 0 ? 8 : 4;
+  size_t b = 0;
+  b += nfa * (8 + 3) * 8;                          // fgeo, fnorm
+  b += nnz * (3 * 8 + 4) + nfa_b * (3 * 8 + 4 + 4);  // lsg, nbocv_v, bfa_lsg, bfa_cell, bfa_zone
+  b += (ncv + 1) * 4 * 2 + ncv * 4 * 2;            // cfl_i, nbocv_i, sched, wave table
+  b += ent * (8 + 4 + 4);                          // entries, their blocks, their cells
+  b += ncv * (5 + 8 + 16 + 5 + 1 + 1) * 8;         // q, prim, grad, rhs, dt, vol
+  b += nfa_b * (8 + 5) * 8;                        // bfa, hook
+  b += (nnz + ncv) * 25 * sJ + 10 * ncv * 5 * sJ;  // A, LU, Krylov vectors
+  b += 8 * ((size_t)nblk(5 * (long long)ncv) + 64) * 8;
+  return b + (1 << 20);
+}
+
+// One pseudo-time step driven entirely by the device, captured once into a
+// hipGraph and replayed; results are identical to step()'s.
+void Ctx::device_step_async() {
+  Impl &d = *d_;
+  if (!d.have_state) throw std::logic_error("no state set");
+  if (d.begun) throw std::logic_error("device step between step_begin and step_end");
+  if (d.graph_inflight) throw std::logic_error("a device step is already in flight");
+  if (d.precision == 0) {
+    ensure(d.A64, 25 * (size_t)d.nnz);
+    ensure(d.LU64, 25 * (size_t)d.ncv);
+    for (int k = 0; k < 10; k++) ensure(d.v64[k], 5 * (size_t)d.ncv);
+  } else {
+    ensure(d.A32, 25 * (size_t)d.nnz);
+    ensure(d.LU32, 25 * (size_t)d.ncv);
+    for (int k = 0; k < 10; k++) ensure(d.v32[k], 5 * (size_t)d.ncv);
+  }
+  if (!d.mail_h) {
+    HIPCHECK(hipHostMalloc((void **)&d.mail_h, sizeof(StepMail), hipHostMallocMapped));
+    std::memset(d.mail_h, 0, sizeof(StepMail));
+    HIPCHECK(hipHostGetDevicePointer((void **)&d.mail_d, d.mail_h, 0));
+    HIPCHECK(hipEventCreateWithFlags(&d.ev_step, hipEventDisableTiming));
+  }
+  if (d.step_no == 0) d.launch_state_bc();
+  if (d.dev_step_dirty) {
+    hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, d.stream, d.scal.p, d.step_no, d.cfl);
+    d.dev_step_dirty = false;
+  }
+  if (!d.gexec) {
+    if (!d.cap_stream) HIPCHECK(hipStreamCreateWithFlags(&d.cap_stream, hipStreamNonBlocking));
+    hipStream_t keep = d.stream;
+    d.stream = d.cap_stream;
+    HIPCHECK(hipStreamBeginCapture(d.cap_stream, hipStreamCaptureModeThreadLocal));
+    try {
+      if (d.precision == 0)
+        d.enqueue_device_step<double>();
+      else
+        d.enqueue_device_step<float>();
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(d.cap_stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      d.stream = keep;
+      throw;
+    }
+    d.stream = keep;
+    HIPCHECK(hipStreamEndCapture(d.cap_stream, &d.graph));
+    HIPCHECK(hipGraphInstantiate(&d.gexec, d.graph, nullptr, nullptr, 0));
+  }
+  HIPCHECK(hipGraphLaunch(d.gexec, d.stream));
+  HIPCHECK(hipEventRecord(d.ev_step, d.stream));
+  d.graph_inflight = true;
+}
+
+bool Ctx::device_step_ready() {
+  Impl &d = *d_;
+  if (!d.graph_inflight) return true;
+  hipError_t e = hipEventQuery(d.ev_step);
+  if (e == hipSuccess) return true;
+  if (e == hipErrorNotReady) return false;
+  HIPCHECK(e);
+  return false;
+}
+
+StepInfo Ctx::device_step_result() {
+  Impl &d = *d_;
+  if (!d.graph_inflight) throw std::logic_error("no device step in flight");
+  HIPCHECK(hipEventSynchronize(d.ev_step));
+  d.graph_inflight = false;
+  const volatile StepMail *m = d.mail_h;
+  StepInfo info;
+  info.step = m->step;
+  info.lin_iters = m->lin_iters;
+  info.lin_status = m->lin_status;
+  info.cfl = m->cfl;
+  info.dt_min = m->dt_min;
+  for (int k = 0; k < 5; k++) info.resid[k] = m->resid[k];
+  d.step_no = info.step;
+  d.cfl = info.cfl;
+  return info;
+}
+
+StepInfo Ctx::device_step() {
+  device_step_async();
+  return device_step_result();
 }
 
 int Ctx::ncv() const { return d_->ncv; }
@@ -1573,6 +2165,7 @@ void Ctx::set_state(const double *rho, const double *rhou, const double *rhoE) {
   d.have_state = true;
   d.step_no = 0;
   d.cfl = d.par.cfl;
+  d.dev_step_dirty = true;
 }
 
 void Ctx::get_state(double *rho, double *rhou, double *rhoE) {
@@ -1615,32 +2208,51 @@ void Ctx::Impl::launch_state_bc() {
     hipLaunchKernelGGL(k_bc, dim3(nblk(nfa_b)), dim3(kBlock), 0, stream, dm, dp, prim.p, hook.p, bfa.p);
 }
 
-void Ctx::Impl::launch_grad_flux() {
-  if (par.second_order) {
-    if (par.grad_ls)
-      hipLaunchKernelGGL(k_grad<true>, dim3(nblk(ncv)), dim3(kBlock), 0, stream, dm, dp, prim.p, bfa.p, grad.p);
-    else
-      hipLaunchKernelGGL(k_grad<false>, dim3(nblk(ncv)), dim3(kBlock), 0, stream, dm, dp, prim.p, bfa.p, grad.p);
+void Ctx::Impl::launch_grad(hipStream_t s) {
+  if (!par.second_order) return;
+  const bool ent = par.grad_kernel == kResidEntry;
+  const dim3 g(ent ? (nwave + kWaves - 1) / kWaves : nblk(ncv));
+  if (ent) {
+    if (par.grad_ls) hipLaunchKernelGGL(k_grad_e<true>, g, dim3(kBlock), 0, s, dm, dp, prim.p, bfa.p, grad.p);
+    else hipLaunchKernelGGL(k_grad_e<false>, g, dim3(kBlock), 0, s, dm, dp, prim.p, bfa.p, grad.p);
+  } else {
+    if (par.grad_ls) hipLaunchKernelGGL(k_grad<true>, g, dim3(kBlock), 0, s, dm, dp, prim.p, bfa.p, grad.p);
+    else hipLaunchKernelGGL(k_grad<false>, g, dim3(kBlock), 0, s, dm, dp, prim.p, bfa.p, grad.p);
   }
+}
+
+int Ctx::Impl::flux_blocks() const {
+  return par.flux_kernel == kResidEntry ? (nwave + kWaves - 1) / kWaves : nblk(ncv);
+}
+
+void Ctx::Impl::launch_flux(hipStream_t s, const double *cfl_dev) {
   DtArgs dta;
   dta.cfl = cfl;
   dta.dt_fixed = par.dt_fixed;
   dta.mode = par.dt_mode;
+  dta.cfl_dev = cfl_dev;
+  const dim3 g(flux_blocks());
   auto go = [&](auto kern) {
-    hipLaunchKernelGGL(kern, dim3(nblk(ncv)), dim3(kBlock), 0, stream, dm, dp, prim.p, grad.p, bfa.p, q.p, rhs.p,
-                       counters.p + 1, dta, dt.p, partials.p);
+    hipLaunchKernelGGL(kern, g, dim3(kBlock), 0, s, dm, dp, prim.p, grad.p, bfa.p, q.p, rhs.p, counters.p + 1, dta,
+                       dt.p, partials.p);
   };
+  const bool ent = par.flux_kernel == kResidEntry;
   if (par.second_order) {
-    if (par.scheme == kRoe) go(k_flux<kRoe, true>);
-    else if (par.scheme == kAusm) go(k_flux<kAusm, true>);
-    else go(k_flux<kHllc, true>);
+    if (par.scheme == kRoe) go(ent ? k_flux_e<kRoe, true> : k_flux<kRoe, true>);
+    else if (par.scheme == kAusm) go(ent ? k_flux_e<kAusm, true> : k_flux<kAusm, true>);
+    else go(ent ? k_flux_e<kHllc, true> : k_flux<kHllc, true>);
   } else {
-    if (par.scheme == kRoe) go(k_flux<kRoe, false>);
-    else if (par.scheme == kAusm) go(k_flux<kAusm, false>);
-    else go(k_flux<kHllc, false>);
+    if (par.scheme == kRoe) go(ent ? k_flux_e<kRoe, false> : k_flux<kRoe, false>);
+    else if (par.scheme == kAusm) go(ent ? k_flux_e<kAusm, false> : k_flux<kAusm, false>);
+    else go(ent ? k_flux_e<kHllc, false> : k_flux<kHllc, false>);
   }
+}
+
+void Ctx::Impl::launch_grad_flux() {
+  launch_grad(stream);
+  launch_flux(stream);
   // dt_min of calcDt; CFL_MIN sets every local dt to it
-  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, nblk(ncv), (int)kFinDt, partials.p, scal.p);
+  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, flux_blocks(), (int)kFinDt, partials.p, scal.p);
   if (par.dt_mode == 1) hipLaunchKernelGGL(k_dt_fill, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, dt.p, scal.p);
 }
 
@@ -1649,7 +2261,10 @@ void Ctx::Impl::launch_jac(bool step_mode) {
   DBuf<T> *v = nullptr;
   if constexpr (std::is_same<T, double>::value) v = v64; else v = v32;
   T *A = A_<T>();
-  if (dup_blocks) HIPCHECK(hipMemsetAsync(A, 0, sizeof(T) * 25 * (size_t)nnz, stream));
+  if (dup_blocks) {
+    const size_t nA = 25 * (size_t)nnz;
+    hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)((nA + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream, nA, A, (T)0);
+  }
   if (par.scheme == kRoe)
     hipLaunchKernelGGL((k_jac<T, kRoe>), dim3(nblk(ncv)), dim3(kBlock), 0, stream, dm, dp, prim.p, bfa.p, A,
                        step_mode ? 1 : 0, dt.p, rhs.p, par.under_relax, v[0].p, v[1].p);
@@ -1661,7 +2276,7 @@ void Ctx::Impl::launch_jac(bool step_mode) {
 // Work vectors: v[0] rhs, v[1] phi (= dq), v[2] res, v[3] res0, v[4] p,
 // v[5] v, v[6] phat, v[7] s, v[8] shat, v[9] t.
 template <typename T>
-void Ctx::Impl::solve(int &iters, int &status) {
+void Ctx::Impl::solve(int &iters, int &status, bool graph) {
   DBuf<T> *v;
   DBuf<T> *LU;
   if constexpr (std::is_same<T, double>::value) {
@@ -1675,14 +2290,10 @@ void Ctx::Impl::solve(int &iters, int &status) {
   const int n5 = 5 * ncv;
   T *rhs_ = v[0].p, *phi = v[1].p, *res = v[2].p, *res0 = v[3].p, *p = v[4].p, *vv = v[5].p, *phat = v[6].p,
     *s = v[7].p, *shat = v[8].p, *t = v[9].p;
-  HIPCHECK(hipMemsetAsync(p, 0, sizeof(T) * n5, stream));
-  HIPCHECK(hipMemsetAsync(vv, 0, sizeof(T) * n5, stream));
+  hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n5)), dim3(kBlock), 0, stream, (size_t)n5, p, (T)0);
+  hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n5)), dim3(kBlock), 0, stream, (size_t)n5, vv, (T)0);
+  hipLaunchKernelGGL(k_solver_init, dim3(1), dim3(64), 0, stream, scal.p, par.ls_maxiter, par.ls_abs, par.ls_rel);
   SolverScalars &hs = *h_scal;
-  std::memset(&hs, 0, sizeof(hs));
-  hs.maxiter = par.ls_maxiter;
-  hs.zero_abs = par.ls_abs;
-  hs.zero_rel = par.ls_rel;
-  HIPCHECK(hipMemcpyAsync(scal.p, &hs, sizeof(SolverScalars), hipMemcpyHostToDevice, stream));
   hipLaunchKernelGGL(k_lu<T>, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, nbocv_i.p, A, LU->p);
   const int gb5 = (ncv + kCellsPerBlock - 1) / kCellsPerBlock;
   hipLaunchKernelGGL(k_resid<T>, dim3(gb5), dim3(kBlock), 0, stream, ncv, nbocv_i.p, nbocv_v.p, A, LU->p, phi, rhs_,
@@ -1691,7 +2302,9 @@ void Ctx::Impl::solve(int &iters, int &status) {
   const int max_it = par.ls_maxiter + 1;
   int launched = 0;
   while (launched < max_it) {
-    int chunk = std::min(launched == 0 ? 2 : 4, max_it - launched);
+    // a captured step launches every iteration (kernels after convergence
+    // return at once); the eager path polls the flag between chunks
+    int chunk = graph ? max_it : std::min(launched == 0 ? 2 : 4, max_it - launched);
     for (int k = 0; k < chunk; k++) {
       hipLaunchKernelGGL(k_p<T>, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, LU->p, res, vv, p, phat, scal.p);
       hipLaunchKernelGGL(k_spmv_dot<T>, dim3(gb5), dim3(kBlock), 0, stream, ncv, nbocv_i.p, nbocv_v.p, A, phat, vv,
@@ -1707,6 +2320,7 @@ void Ctx::Impl::solve(int &iters, int &status) {
       hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, gb5, (int)kFinResid, partials.p, scal.p);
     }
     launched += chunk;
+    if (graph) return;
     HIPCHECK(hipMemcpyAsync(h_scal, scal.p, sizeof(SolverScalars), hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipStreamSynchronize(stream));
     if (hs.done) break;
@@ -1716,8 +2330,23 @@ void Ctx::Impl::solve(int &iters, int &status) {
 }
 
 template <typename T>
-static void ensure(DBuf<T> &b, size_t n) {
-  if (b.n != n) b.alloc(n);
+void Ctx::Impl::enqueue_device_step() {
+  hipLaunchKernelGGL(k_ramp, dim3(1), dim3(64), 0, stream, scal.p, par.cfl_after, par.cfl_interval, par.cfl_factor,
+                     par.cfl_max);
+  launch_grad(stream);
+  launch_flux(stream, &scal.p->cfl_cur);
+  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, flux_blocks(), (int)kFinDt, partials.p, scal.p);
+  if (par.dt_mode == 1) hipLaunchKernelGGL(k_dt_fill, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, dt.p, scal.p);
+  launch_jac<T>(true);
+  int it = 0, st = 0;
+  solve<T>(it, st, true);
+  DBuf<T> *v;
+  if constexpr (std::is_same<T, double>::value) v = v64; else v = v32;
+  hipLaunchKernelGGL(k_update<T>, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, q.p, v[1].p);
+  launch_state_bc();
+  hipLaunchKernelGGL(k_l1, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, rhs.p, partials.p);
+  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, nblk(ncv), (int)kFinL1, partials.p, scal.p);
+  hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, stream, scal.p, mail_d);
 }
 
 void Ctx::launch_residual() {
@@ -1749,20 +2378,8 @@ std::string Ctx::time_kernels(int reps, double *ms) {
   ms[1] = d.nfa_b > 0 ? timed([&] {
     hipLaunchKernelGGL(k_bc, dim3(nblk(d.nfa_b)), dim3(kBlock), 0, d.stream, d.dm, d.dp, d.prim.p, d.hook.p, d.bfa.p);
   }) : 0.0;
-  ms[2] = 0.0;
-  if (d.par.second_order) {
-    ms[2] = timed([&] {
-      if (d.par.grad_ls)
-        hipLaunchKernelGGL(k_grad<true>, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.dm, d.dp, d.prim.p, d.bfa.p,
-                           d.grad.p);
-      else
-        hipLaunchKernelGGL(k_grad<false>, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.dm, d.dp, d.prim.p, d.bfa.p,
-                           d.grad.p);
-    });
-  }
-  // flux alone: time the whole grad+flux pair and subtract the gradient
-  double gf = timed([&] { d.launch_grad_flux(); });
-  ms[3] = gf - ms[2];
+  ms[2] = d.par.second_order ? timed([&] { d.launch_grad(d.stream); }) : 0.0;
+  ms[3] = timed([&] { d.launch_flux(d.stream); });
   // (the local time step is computed inside k_flux)
   ms[4] = timed([&] {
     d.launch_state_bc();
@@ -1794,6 +2411,8 @@ void Ctx::calc_rhs(bool implicit) {
 
 StepInfo Ctx::step() {
   Impl &d = *d_;
+  if (d.graph_inflight) throw std::logic_error("a device step is in flight");
+  d.dev_step_dirty = true;
   if (!d.have_state) throw std::logic_error("no state set");
   if (d.begun) throw std::logic_error("step called between step_begin and step_end");
   StepInfo info;
@@ -1835,6 +2454,8 @@ StepInfo Ctx::step() {
 
 void Ctx::step_begin(bool host_source) {
   Impl &d = *d_;
+  if (d.graph_inflight) throw std::logic_error("a device step is in flight");
+  d.dev_step_dirty = true;
   if (!d.have_state) throw std::logic_error("no state set");
   if (d.begun) throw std::logic_error("step_begin called twice without step_end");
   if (d.step_no == 0) d.launch_state_bc();
@@ -1926,22 +2547,33 @@ void Ctx::update_bc() {
 }
 
 // doneSolver (JoeWithModels.h:42-95): NSTEPS, RESID_ENERG_TH, NaN
-int Ctx::run(int max_steps, double *hist) {
+bool Ctx::done_solver(const StepInfo &s) {
   Impl &d = *d_;
-  auto done = [&](double r) {
-    if ((d.par.nsteps >= 0) && (d.step_no >= d.par.nsteps)) return true;
-    if (r < d.par.resid_energ_th) return true;
-    if (r != r) return true;
-    return false;
-  };
-  if (done(1.0e20)) return 0;
+  if (s.step == 1) d.resid0_energy = s.resid[4];
+  const double r = s.resid[4];
+  if ((d.par.nsteps >= 0) && (d.step_no >= d.par.nsteps)) return true;
+  if (r < d.par.resid_energ_th) return true;
+  if (r != r) return true;
+  if (d.par.resid_energ_rel > 0.0 && d.step_no >= 1 && r < d.par.resid_energ_rel * d.resid0_energy) return true;
+  return false;
+}
+
+int Ctx::run(int max_steps, double *hist, bool graph) {
+  Impl &d = *d_;
+  {
+    StepInfo s0;
+    s0.step = d.step_no;
+    s0.resid[4] = 1.0e20;
+    if (d.step_no == 0 && done_solver(s0)) return 0;
+    if (d.step_no > 0 && ((d.par.nsteps >= 0) && (d.step_no >= d.par.nsteps))) return 0;
+  }
   int n = 0;
   while (n < max_steps) {
-    StepInfo s = step();
+    StepInfo s = graph ? device_step() : step();
     if (hist)
       for (int k = 0; k < 5; k++) hist[5 * n + k] = s.resid[k];
     n++;
-    if (done(s.resid[4])) break;
+    if (done_solver(s)) break;
   }
   return n;
 }

@@ -27,6 +27,11 @@ class JoemiError(RuntimeError):
         self.status = status
 
 
+class SolveReport(C.Structure):
+    _fields_ = [("status", C.c_int), ("steps", C.c_int), ("resid", C.c_double * 5), ("resid0", C.c_double * 5),
+                ("queued_s", C.c_double), ("run_s", C.c_double)]
+
+
 class StepInfo(C.Structure):
     _fields_ = [("step", C.c_int), ("lin_iters", C.c_int), ("lin_status", C.c_int), ("cfl", C.c_double),
                 ("dt_min", C.c_double), ("resid", C.c_double * 5)]
@@ -58,6 +63,7 @@ def lib():
         L.jm_calc_rhs.argtypes = [vp, C.c_int]
         L.jm_get.argtypes = [vp, C.c_char_p, vp, vp]
         L.jm_step.argtypes = [vp, vp]
+        L.jm_step_graph.argtypes = [vp, vp]
         L.jm_run.argtypes = [vp, C.c_int, vp, vp]
         L.jm_set_boundary_state.argtypes = [vp, C.c_int, vp, vp, vp]
         L.jm_device_state.argtypes = [vp, vp]
@@ -65,6 +71,15 @@ def lib():
         L.jm_sync.argtypes = [vp]
         L.jm_time_residual_kernels.argtypes = [vp, C.c_int, vp, C.c_char_p, C.c_int]
         L.jm_mesh_quality.argtypes = [vp, vp, vp]
+        L.jm_ctx_footprint.argtypes = [vp, C.c_int, vp]
+        L.jm_server_create.argtypes = [C.c_int64, C.c_int, vp]
+        L.jm_server_destroy.argtypes = [vp]
+        L.jm_server_destroy.restype = None
+        L.jm_server_submit.argtypes = [vp, vp, C.c_char_p, C.c_int, vp, vp, vp, vp]
+        L.jm_server_status.argtypes = [vp, C.c_int, vp]
+        L.jm_server_wait.argtypes = [vp, C.c_int]
+        L.jm_server_result.argtypes = [vp, C.c_int, vp, vp, vp, vp]
+        L.jm_server_release.argtypes = [vp, C.c_int]
         _lib = L
     return _lib
 
@@ -121,6 +136,12 @@ class Mesh:
         out = np.empty(n.value, dt)
         _check(lib().jm_mesh_array(self.h, name.encode(), _ptr(out), C.byref(n)))
         return out
+
+    def footprint(self, precision=JM_FP64) -> int:
+        """Device bytes a solve on this mesh holds."""
+        b = C.c_int64()
+        _check(lib().jm_ctx_footprint(self.h, precision, C.byref(b)))
+        return b.value
 
     def quality(self):
         q, n = C.c_double(), C.c_int()
@@ -193,6 +214,11 @@ class Solver:
         _check(lib().jm_step(self.h, C.byref(info)))
         return info
 
+    def step_graph(self) -> StepInfo:
+        info = StepInfo()
+        _check(lib().jm_step_graph(self.h, C.byref(info)))
+        return info
+
     def run(self, max_steps):
         hist = np.zeros((max_steps, 5))
         n = C.c_int()
@@ -224,3 +250,53 @@ class Solver:
         _check(lib().jm_time_residual_kernels(self.h, reps, _ptr(ms), buf, 256))
         names = buf.value.decode().split(",")
         return dict(zip(names, ms[:len(names)]))
+
+
+class Server:
+    """Many steady solves on one GPU (jm_server_*): admission by device
+    footprint, one stream and graph-captured steps per active solve."""
+
+    QUEUED, RUNNING, DONE, FAILED = 0, 1, 2, 3
+
+    def __init__(self, mem_budget_bytes: int, max_active: int = 8):
+        self.h = C.c_void_p()
+        _check(lib().jm_server_create(int(mem_budget_bytes), int(max_active), C.byref(self.h)))
+        self._meshes = {}
+
+    def close(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.jm_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, mesh: Mesh, joe_in: str, precision: int = JM_FP64, state=None) -> int:
+        t = C.c_int()
+        if state is None:
+            _check(lib().jm_server_submit(self.h, mesh.h, joe_in.encode(), precision, None, None, None, C.byref(t)))
+        else:
+            rho, rhou, rhoE = (np.ascontiguousarray(a, np.float64) for a in state)
+            _check(lib().jm_server_submit(self.h, mesh.h, joe_in.encode(), precision, _ptr(rho), _ptr(rhou),
+                                          _ptr(rhoE), C.byref(t)))
+        self._meshes[t.value] = mesh  # keep the mesh alive until release
+        return t.value
+
+    def status(self, ticket: int) -> int:
+        st = C.c_int()
+        _check(lib().jm_server_status(self.h, ticket, C.byref(st)))
+        return st.value
+
+    def wait(self, ticket: int):
+        _check(lib().jm_server_wait(self.h, ticket))
+
+    def result(self, ticket: int):
+        ncv = self._meshes[ticket].ncv
+        rho, rhou, rhoE = np.empty(ncv), np.empty((ncv, 3)), np.empty(ncv)
+        rep = SolveReport()
+        _check(lib().jm_server_result(self.h, ticket, _ptr(rho), _ptr(rhou), _ptr(rhoE), C.byref(rep)))
+        return (rho, rhou, rhoE), rep
+
+    def release(self, ticket: int):
+        _check(lib().jm_server_release(self.h, ticket))
+        self._meshes.pop(ticket, None)

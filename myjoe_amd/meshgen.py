@@ -14,6 +14,9 @@ Workloads
 ---------
 * ``bump_channel``: 2D Euler channel with a 10 %-thick circular-arc bump on the
   lower wall, one cell thick in z (96x32x1 in the bump-channel steady solve).
+* ``naca_omesh``: O-mesh of ni x nj hexes around a NACA 4-digit airfoil (closed
+  trailing edge, chord 1) out to a circle of radius 20, one hex layer in z,
+  geometric radial stretching (the serving workload's airfoils).
 * ``split_box``: [0,1]^3 box of n^3 hexes, a seeded fraction split into six
   Kuhn tetrahedra each (globally consistent diagonals), interior nodes
   jittered uniformly by +-jitter*h (160^3, 50 %, 0.2 in the single-residual
@@ -24,7 +27,7 @@ from __future__ import annotations
 import dataclasses
 import numpy as np
 
-__all__ = ["MshMesh", "bump_channel", "split_box", "write_msh"]
+__all__ = ["MshMesh", "bump_channel", "split_box", "naca_omesh", "write_msh"]
 
 
 @dataclasses.dataclass
@@ -127,6 +130,90 @@ def bump_channel(nx: int = 96, ny: int = 32, thickness: float = 0.1, dz: float |
     faces = [(int_nodes, int_cells, 3), (zn_i, zc_i, 4), (zn_o, zc_o, 5), (zn_w, zc_w, 6),
              (zn_t, zc_t, 7), (np.concatenate([zn_z0, zn_z1]), np.concatenate([zc_z0, zc_z1]), 8)]
     return _assemble(x_no, faces, nx * ny, zones)
+
+
+def naca_surface(psi: np.ndarray, thickness: float, camber: float, camber_pos: float = 0.4):
+    """Points of a NACA 4-digit section (closed trailing edge) at the angle
+    parameter psi in [0, 2 pi): x = (1 + cos psi) / 2, the lower surface for
+    psi in [0, pi], the upper one after it (clockwise seen from +z)."""
+    xc = 0.5 * (1.0 + np.cos(psi))
+    t, m, p = thickness, camber, camber_pos
+    yt = 5.0 * t * (0.2969 * np.sqrt(xc) - 0.1260 * xc - 0.3516 * xc ** 2 + 0.2843 * xc ** 3 - 0.1036 * xc ** 4)
+    if m > 0.0:
+        yc = np.where(xc < p, m / p ** 2 * (2 * p * xc - xc ** 2), m / (1 - p) ** 2 * ((1 - 2 * p) + 2 * p * xc - xc ** 2))
+        dy = np.where(xc < p, 2 * m / p ** 2 * (p - xc), 2 * m / (1 - p) ** 2 * (p - xc))
+    else:
+        yc = np.zeros_like(xc)
+        dy = np.zeros_like(xc)
+    th = np.arctan(dy)
+    upper = psi > np.pi
+    sgn = np.where(upper, 1.0, -1.0)
+    return xc - sgn * yt * np.sin(th), yc + sgn * yt * np.cos(th)
+
+
+def naca_omesh(ni: int = 200, nj: int = 100, thickness: float = 0.12, camber: float = 0.0,
+               camber_pos: float = 0.4, radius: float = 20.0, first: float = 2e-3, dz: float = 0.05) -> MshMesh:
+    """O-mesh around a NACA 4-digit airfoil, ni cells around, nj cells out,
+    one hex layer in z. Zones: airfoil (the section), farfield (circle of the
+    given radius about x = 0.5), zsym (both z faces). ncv = ni * nj."""
+    psi = 2.0 * np.pi * np.arange(ni) / ni
+    xs, ys = naca_surface(psi, thickness, camber, camber_pos)
+    xf = 0.5 + radius * np.cos(-psi)
+    yf = radius * np.sin(-psi)
+    # geometric radial spacing: s_1 * radius ~ first
+    target = first / radius
+    lo, hi = 1.0 + 1e-12, 2.0
+    for _ in range(200):
+        r = 0.5 * (lo + hi)
+        if (r - 1.0) / (r ** nj - 1.0) > target:
+            lo = r
+        else:
+            hi = r
+    r = 0.5 * (lo + hi)
+    sj = (r ** np.arange(nj + 1) - 1.0) / (r ** nj - 1.0)
+    X = np.empty((2, nj + 1, ni, 3))
+    for k in range(2):
+        X[k, :, :, 0] = (1.0 - sj[:, None]) * xs[None, :] + sj[:, None] * xf[None, :]
+        X[k, :, :, 1] = (1.0 - sj[:, None]) * ys[None, :] + sj[:, None] * yf[None, :]
+        X[k, :, :, 2] = k * dz
+    x_no = X.reshape(-1, 3)
+
+    def nid(i, j, k):
+        return (k * (nj + 1) + j) * ni + (i % ni)
+
+    I, J = np.meshgrid(np.arange(ni), np.arange(nj), indexing="xy")
+    I = I.reshape(-1)
+    J = J.reshape(-1)
+    cid = J * ni + I
+    corners = np.stack([nid(I, J, 0), nid(I + 1, J, 0), nid(I + 1, J + 1, 0), nid(I, J + 1, 0),
+                        nid(I, J, 1), nid(I + 1, J, 1), nid(I + 1, J + 1, 1), nid(I, J + 1, 1)], 1)
+    blocks_nodes, blocks_cells, order_key = [], [], []
+    # +i face of every cell (periodic around the section)
+    blocks_nodes.append(corners[:, _HEX_FACES[(0, 1)]])
+    blocks_cells.append(np.stack([cid, J * ni + (I + 1) % ni], 1))
+    order_key.append(cid * 2)
+    m = J < nj - 1
+    blocks_nodes.append(corners[m][:, _HEX_FACES[(1, 1)]])
+    blocks_cells.append(np.stack([cid[m], cid[m] + ni], 1))
+    order_key.append(cid[m] * 2 + 1)
+    key = np.concatenate(order_key)
+    o = np.argsort(key, kind="stable")
+    int_nodes = np.concatenate(blocks_nodes)[o]
+    int_cells = np.concatenate(blocks_cells)[o]
+
+    def bnd(mask, axis, side):
+        return corners[mask][:, _HEX_FACES[(axis, side)]], np.stack([cid[mask], -np.ones(mask.sum(), np.int64)], 1)
+
+    zones = [(3, "interior", "int_fluid"), (4, "wall", "airfoil"), (5, "pressure-far-field", "farfield"),
+             (6, "symmetry", "zsym")]
+    zn_w, zc_w = bnd(J == 0, 1, 0)
+    zn_f, zc_f = bnd(J == nj - 1, 1, 1)
+    all_c = np.ones_like(I, bool)
+    zn_z0, zc_z0 = bnd(all_c, 2, 0)
+    zn_z1, zc_z1 = bnd(all_c, 2, 1)
+    faces = [(int_nodes, int_cells, 3), (zn_w, zc_w, 4), (zn_f, zc_f, 5),
+             (np.concatenate([zn_z0, zn_z1]), np.concatenate([zc_z0, zc_z1]), 6)]
+    return _assemble(x_no, faces, ni * nj, zones)
 
 
 # Kuhn tetrahedra of the unit cube: permutation of the axes visited on the

@@ -99,3 +99,16 @@ def test_unsupported_options_have_their_own_status():
         with pytest.raises(joemi.JoemiError) as e:
             joemi.Solver(mesh, edit(joe))
         assert e.value.status == 5, str(e.value)
+
+
+def test_server_without_device_is_an_error():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(joemi.JoemiError) as e:
+        joemi.Server(1 << 30, 2)
+    assert e.value.status == 4
+    mm, _ = cases.box_case(2, seed=0)
+    mesh = joemi.Mesh.from_faces(mm)
+    assert mesh.footprint(joemi.JM_MIXED) < mesh.footprint(joemi.JM_FP64)

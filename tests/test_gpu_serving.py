@@ -1,0 +1,77 @@
+"""The serving loop (jm_server_*): solves admitted by footprint, run
+concurrently on their own streams with graph-captured steps. Each result must
+be the same bits as the same solve run alone through the ordinary step."""
+import numpy as np
+import pytest
+
+from myjoe_amd import cases, joemi
+
+pytestmark = pytest.mark.gpu
+
+REQS = [dict(ni=48, nj=24, thickness=0.12, camber=0.02, mach=0.5, aoa_deg=2.0),
+        dict(ni=64, nj=32, thickness=0.16, camber=0.0, mach=0.7, aoa_deg=-1.0),
+        dict(ni=40, nj=20, thickness=0.09, camber=0.04, mach=0.35, aoa_deg=5.0)]
+
+
+def _case(r, nsteps=40):
+    return cases.airfoil_case(r["ni"], r["nj"], r["thickness"], r["camber"], r["mach"], r["aoa_deg"],
+                              nsteps=nsteps, resid_rel=1e-12)
+
+
+def _alone(mesh, joe, precision):
+    s = joemi.Solver(mesh, joe, precision)
+    s.init_uniform()
+    n, hist = s.run(100000)
+    return s.state(), n, hist
+
+
+def test_results_same_alone_and_in_the_mix():
+    print("serving: mix of three", flush=True)
+    built = []
+    for k, r in enumerate(REQS):
+        mm, joe = _case(r)
+        built.append((joemi.Mesh.from_faces(mm), joe, joemi.JM_MIXED if k == 1 else joemi.JM_FP64))
+    srv = joemi.Server(mem_budget_bytes=8 << 30, max_active=3)
+    tickets = [srv.submit(m, joe, prec) for (m, joe, prec) in built]
+    for t in tickets:
+        srv.wait(t)
+    for t, (m, joe, prec) in zip(tickets, built):
+        (rho, rhou, rhoE), rep = srv.result(t)
+        assert rep.status == joemi.Server.DONE and rep.steps == 40
+        (r2, ru2, rE2), n2, hist2 = _alone(m, joe, prec)
+        assert n2 == 40
+        assert np.array_equal(rho, r2) and np.array_equal(rhou, ru2) and np.array_equal(rhoE, rE2)
+        assert list(rep.resid) == list(hist2[-1])
+        srv.release(t)
+    srv.close()
+
+
+def test_admission_respects_the_memory_budget():
+    print("serving: budget", flush=True)
+    mm, joe = _case(REQS[1], nsteps=10)
+    mesh = joemi.Mesh.from_faces(mm)
+    # a budget below two footprints: the second solve waits for the first
+    fp = mesh.footprint()
+    srv = joemi.Server(mem_budget_bytes=int(1.5 * fp), max_active=4)
+    t1 = srv.submit(mesh, joe)
+    t2 = srv.submit(mesh, joe)
+    srv.wait(t2)
+    (a, _, _), rep1 = srv.result(t1)
+    (b, _, _), rep2 = srv.result(t2)
+    assert rep2.queued_s >= rep1.run_s * 0.5
+    assert np.array_equal(a, b)
+    srv.close()
+    small = joemi.Server(mem_budget_bytes=fp // 2, max_active=1)
+    with pytest.raises(joemi.JoemiError):
+        small.submit(mesh, joe)
+    small.close()
+
+
+def test_bad_request_is_rejected_at_submission():
+    mm, joe = _case(REQS[2], nsteps=5)
+    mesh = joemi.Mesh.from_faces(mm)
+    srv = joemi.Server(mem_budget_bytes=1 << 30, max_active=2)
+    with pytest.raises(joemi.JoemiError) as e:
+        srv.submit(mesh, joe.replace("airfoil = WALL", "airfoil = NOSUCHBC"))
+    assert e.value.status == 3
+    srv.close()
