@@ -568,6 +568,7 @@ struct or_ctx {
   std::vector<double> rho, rhou, rhoE, vel, press, temp, enthalpy, sos;
   std::vector<double> rho_bfa, T_bfa, vel_bfa, p_bfa, h_bfa;
   std::vector<double> grad_rho, grad_p, grad_u, alpha_rho, local_dt;
+  std::vector<double> mul_fa, lamOcp_fa, grad_h;
   std::vector<double> rhs_rho, rhs_rhou, rhs_rhoE, A, dq, resid;
   double cfl;
   int step;
@@ -575,6 +576,28 @@ struct or_ctx {
 };
 
 namespace {
+
+// laminar viscosity of a temperature (UgpWithCvCompFlow.h:1712, 1734, 1795-1798)
+double mu_lam(const or_ctx &c, double T) {
+  if (c.p.visc_mode == 1)
+    return c.p.mu_ref * pow(T / c.p.sl_tref, 1.5) * (c.p.sl_tref + c.p.sl_sref) / (T + c.p.sl_sref);
+  return c.p.mu_ref * pow(T / c.p.T_ref, c.p.mu_power_law);
+}
+
+// distance weights of a face between two cells (GG_WITH_DIST, UgpWithCv.h:37,
+// :1104-1110; the same in calcMaterialProperties)
+static void gg_weights(const or_mesh &m, int ifa, int icv0, int icv1, double &w0, double &w1) {
+  double dx0[3], dx1[3];
+  for (int i = 0; i < 3; i++) {
+    dx0[i] = m.x_fa[3 * ifa + i] - m.x_cv[3 * icv0 + i];
+    dx1[i] = m.x_fa[3 * ifa + i] - m.x_cv[3 * icv1 + i];
+  }
+  double s0 = 0.0, s1 = 0.0;
+  for (int i = 0; i < 3; i++) s0 += dx0[i] * dx0[i];
+  for (int i = 0; i < 3; i++) s1 += dx1[i] * dx1[i];
+  w0 = sqrt(s0);
+  w1 = sqrt(s1);
+}
 
 // calcStateVariables, UgpWithCvCompFlow.h:1662-1694 (kine = NULL; gamma, RoM
 // constant: initialHookScalarRansCombModel sets them to GAMMA, R_gas)
@@ -724,10 +747,33 @@ void set_bc(or_ctx &c) {
       }
     }
   }
+  // every boundary keyword ends with ComputeBCProperties_T (the subsonic
+  // inlet with _H, from the static temperature): laminar viscosity and
+  // conductivity of the boundary faces (UgpWithCvCompFlow.h:1783-1839)
+  if (c.p.mu_ref > 0.0)
+    for (int ifa = 0; ifa < c.nfa_b; ifa++) {
+      c.mul_fa[ifa] = mu_lam(c, c.T_bfa[ifa]);
+      c.lamOcp_fa[ifa] = c.mul_fa[ifa] / c.p.pr;
+    }
   for (int ifa = 0; ifa < c.nfa_b; ifa++) c.rho_bfa[ifa] = c.p_bfa[ifa] / (R_gas * c.T_bfa[ifa]);
 }
 
-// calcDt (MAX_BASED), UgpWithCvCompFlow.cpp:5-60, 100-105. mul_fa = mut_fa = 0.
+// calcMaterialProperties, internal faces (UgpWithCvCompFlow.h:1700-1750):
+// distance-weighted face temperature, then Sutherland or power law
+void calc_material(or_ctx &c) {
+  if (!(c.p.mu_ref > 0.0)) return;
+  const or_mesh &m = c.m;
+  for (int ifa = c.nfa_b; ifa < c.nfa; ifa++) {
+    int icv0 = m.cvofa[2 * ifa], icv1 = m.cvofa[2 * ifa + 1];
+    double w0, w1;
+    gg_weights(m, ifa, icv0, icv1, w0, w1);
+    double temperature = (w1 * c.temp[icv0] + w0 * c.temp[icv1]) / (w0 + w1);
+    c.mul_fa[ifa] = mu_lam(c, temperature);
+    c.lamOcp_fa[ifa] = c.mul_fa[ifa] / c.p.pr;
+  }
+}
+
+// calcDt (MAX_BASED), UgpWithCvCompFlow.cpp:5-60, 100-105. mut_fa = 0.
 double calc_dt(or_ctx &c) {
   const or_mesh &m = c.m;
   double dt_min = 1.0e20;
@@ -740,7 +786,7 @@ double calc_dt(or_ctx &c) {
       double area = norm3(nVec, &m.fa_normal[3 * ifa]);
       double Uk = dot3(&c.rhou[3 * icv], nVec) / c.rho[icv];
       double lambda = (fabs(Uk) + cs) * area;
-      double lamdiff = 4.0 * (0.0 + 0.0) / c.rho[icv] * area * area / m.cv_volume[icv];
+      double lamdiff = 4.0 * (c.mul_fa[ifa] + 0.0) / c.rho[icv] * area * area / m.cv_volume[icv];
       lambda = max(lambda, lamdiff);
       lambdaMax = max(lambdaMax, lambda);
     }
@@ -792,18 +838,6 @@ void grad_vector_ls(or_ctx &c, double *g, const double *u, const double *u_fa) {
 // Green-Gauss gradients, UgpWithCv.h:1091-1188. The reference defines
 // GG_WITH_DIST (UgpWithCv.h:37): face values are weighted by the distances
 // from the face centroid to the two cell centroids (:1104-1110).
-static void gg_weights(const or_mesh &m, int ifa, int icv0, int icv1, double &w0, double &w1) {
-  double dx0[3], dx1[3];
-  for (int i = 0; i < 3; i++) {
-    dx0[i] = m.x_fa[3 * ifa + i] - m.x_cv[3 * icv0 + i];
-    dx1[i] = m.x_fa[3 * ifa + i] - m.x_cv[3 * icv1 + i];
-  }
-  double s0 = 0.0, s1 = 0.0;
-  for (int i = 0; i < 3; i++) s0 += dx0[i] * dx0[i];
-  for (int i = 0; i < 3; i++) s1 += dx1[i] * dx1[i];
-  w0 = sqrt(s0);
-  w1 = sqrt(s1);
-}
 
 void grad_scalar_gg(or_ctx &c, double *g, const double *p, const double *p_fa) {
   const or_mesh &m = c.m;
@@ -1094,6 +1128,207 @@ void calc_euler_flux(or_ctx &c, double *A) {
 
 // calcRhs, JoeWithModels.cpp:2564-2594, with the default sourceHook
 // (gravity, JoeWithModels.h:111-129).
+// addViscFlux, UgpWithCvCompFlow.cpp:1521-1763, laminar (mut = 0, no
+// turbulence model, nonLinear = rij = kine = 0 as allocated at
+// UgpWithCvCompFlow.h:532-545), SIMPLEX_VISCOUS = NO. A0/A1 rows 1-4 are
+// written by dFdU x dUdQ (:712-745); row 0 stays as the caller left it.
+void add_visc_flux(const or_ctx &c, double *Frhou, double &FrhoE, double (*A0)[5], double (*A1)[5], double rho0,
+                   const double *u0, const double *grad_u0, double h0, const double *grad_h0, double T0, double R0,
+                   double gam0, double kine0, double rho1, const double *u1, const double *grad_u1, double h1,
+                   const double *grad_h1, double T1, double R1, double gam1, double kine1, double mul, double mut,
+                   double lambdaOverCp, const double *u_fa, double area, const double *nVec, double smag,
+                   const double *sVec) {
+  const double nonL = 0.0;
+  const double rij_d[3] = {0.0, 0.0, 0.0}, rij_offd[3] = {0.0, 0.0, 0.0};
+  double alpha = dot3(nVec, sVec);
+  double grad_u_f[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) grad_u_f[i][j] = 0.5 * (grad_u0[3 * i + j] + grad_u1[3 * i + j]);
+  double fCorr[3] = {nVec[0] - alpha * sVec[0], nVec[1] - alpha * sVec[1], nVec[2] - alpha * sVec[2]};
+  double tauij_nj[3], tauLamij_nj[3], tauTurbij_nj[3], enth, tmp;
+  for (int i = 0; i < 3; i++)
+    tauLamij_nj[i] = mul * (alpha * (u1[i] - u0[i]) / smag + grad_u_f[i][0] * fCorr[0] + grad_u_f[0][i] * nVec[0] +
+                            grad_u_f[i][1] * fCorr[1] + grad_u_f[1][i] * nVec[1] + grad_u_f[i][2] * fCorr[2] +
+                            grad_u_f[2][i] * nVec[2]);
+  tmp = 2.0 / 3.0 * mul * (grad_u_f[0][0] + grad_u_f[1][1] + grad_u_f[2][2]);
+  tauLamij_nj[0] -= tmp * nVec[0];
+  tauLamij_nj[1] -= tmp * nVec[1];
+  tauLamij_nj[2] -= tmp * nVec[2];
+  for (int i = 0; i < 3; i++)
+    tauTurbij_nj[i] = (1.0 - nonL) * mut *
+                      (alpha * (u1[i] - u0[i]) / smag + grad_u_f[i][0] * fCorr[0] + grad_u_f[0][i] * nVec[0] +
+                       grad_u_f[i][1] * fCorr[1] + grad_u_f[1][i] * nVec[1] + grad_u_f[i][2] * fCorr[2] +
+                       grad_u_f[2][i] * nVec[2]);
+  tmp = (1.0 - nonL) * 2.0 / 3.0 * mut * (grad_u_f[0][0] + grad_u_f[1][1] + grad_u_f[2][2]);
+  tauTurbij_nj[0] -= tmp * nVec[0];
+  tauTurbij_nj[1] -= tmp * nVec[1];
+  tauTurbij_nj[2] -= tmp * nVec[2];
+  tauTurbij_nj[0] += nonL * (rij_d[0] * nVec[0] + rij_offd[0] * nVec[1] + rij_offd[1] * nVec[2]);
+  tauTurbij_nj[1] += nonL * (rij_offd[0] * nVec[0] + rij_d[1] * nVec[1] + rij_offd[2] * nVec[2]);
+  tauTurbij_nj[2] += nonL * (rij_offd[1] * nVec[0] + rij_offd[2] * nVec[1] + rij_d[2] * nVec[2]);
+  for (int i = 0; i < 3; i++) tauij_nj[i] = tauLamij_nj[i] + tauTurbij_nj[i];
+  for (int i = 0; i < 3; i++) Frhou[i] = -area * tauij_nj[i];
+  double keff = lambdaOverCp + mut / c.p.pr_turb;
+  enth = keff * (alpha * (h1 - h0) / smag + 0.5 * ((grad_h0[0] + grad_h1[0]) * fCorr[0] +
+                                                   (grad_h0[1] + grad_h1[1]) * fCorr[1] +
+                                                   (grad_h0[2] + grad_h1[2]) * fCorr[2]));
+  double psi = 0.0;
+  FrhoE = -area * (enth + tauij_nj[0] * u_fa[0] + tauij_nj[1] * u_fa[1] + tauij_nj[2] * u_fa[2] + psi);
+
+  double dfdU[5][5] = {{0.0}}, dUdQ[5][5] = {{0.0}};
+  auto mult = [&](double (*A)[5]) {  // dFdU_times_dUdQ, :717-745
+    A[1][0] = dfdU[1][1] * dUdQ[1][0] + dfdU[1][2] * dUdQ[2][0] + dfdU[1][3] * dUdQ[3][0];
+    A[2][0] = dfdU[2][1] * dUdQ[1][0] + dfdU[2][2] * dUdQ[2][0] + dfdU[2][3] * dUdQ[3][0];
+    A[3][0] = dfdU[3][1] * dUdQ[1][0] + dfdU[3][2] * dUdQ[2][0] + dfdU[3][3] * dUdQ[3][0];
+    A[4][0] = dfdU[4][1] * dUdQ[1][0] + dfdU[4][2] * dUdQ[2][0] + dfdU[4][3] * dUdQ[3][0] + dfdU[4][4] * dUdQ[4][0];
+    A[1][1] = dfdU[1][1] * dUdQ[1][1];
+    A[2][1] = dfdU[2][1] * dUdQ[1][1];
+    A[3][1] = dfdU[3][1] * dUdQ[1][1];
+    A[4][1] = dfdU[4][1] * dUdQ[1][1] + dfdU[4][4] * dUdQ[4][1];
+    A[1][2] = dfdU[1][2] * dUdQ[2][2];
+    A[2][2] = dfdU[2][2] * dUdQ[2][2];
+    A[3][2] = dfdU[3][2] * dUdQ[2][2];
+    A[4][2] = dfdU[4][2] * dUdQ[2][2] + dfdU[4][4] * dUdQ[4][2];
+    A[1][3] = dfdU[1][3] * dUdQ[3][3];
+    A[2][3] = dfdU[2][3] * dUdQ[3][3];
+    A[3][3] = dfdU[3][3] * dUdQ[3][3];
+    A[4][3] = dfdU[4][3] * dUdQ[3][3] + dfdU[4][4] * dUdQ[4][3];
+    A[4][4] = dfdU[4][1] * dUdQ[1][4] + dfdU[4][2] * dUdQ[2][4] + dfdU[4][3] * dUdQ[3][4] + dfdU[4][4] * dUdQ[4][4];
+  };
+  auto side = [&](double sgn, double rho, const double *u, double h, double T, double R, double gam, double kine) {
+    dfdU[1][1] = sgn * (mul + mut) * alpha / smag * area;
+    dfdU[1][2] = 0.0;
+    dfdU[1][3] = 0.0;
+    dfdU[2][1] = 0.0;
+    dfdU[2][2] = sgn * (mul + mut) * alpha / smag * area;
+    dfdU[2][3] = 0.0;
+    dfdU[3][1] = 0.0;
+    dfdU[3][2] = 0.0;
+    dfdU[3][3] = sgn * (mul + mut) * alpha / smag * area;
+    dfdU[4][1] = area * tauij_nj[0] * 0.5 + u_fa[0] * dfdU[1][1];
+    dfdU[4][2] = area * tauij_nj[1] * 0.5 + u_fa[1] * dfdU[2][2];
+    dfdU[4][3] = area * tauij_nj[2] * 0.5 + u_fa[2] * dfdU[3][3];
+    dfdU[4][4] = sgn * area * keff * alpha / smag;
+    double invRho = 1.0 / rho;
+    dUdQ[1][0] = -u[0] * invRho;
+    dUdQ[2][0] = -u[1] * invRho;
+    dUdQ[3][0] = -u[2] * invRho;
+    dUdQ[4][0] = gam * invRho * (-h - kine + R * T + 0.5 * dot3(u, u));
+    dUdQ[1][1] = invRho;
+    dUdQ[4][1] = -gam * u[0] * invRho;
+    dUdQ[2][2] = invRho;
+    dUdQ[4][2] = -gam * u[1] * invRho;
+    dUdQ[3][3] = invRho;
+    dUdQ[4][3] = -gam * u[2] * invRho;
+    dUdQ[4][4] = gam * invRho;
+  };
+  if (A0) {
+    side(-1.0, rho0, u0, h0, T0, R0, gam0, kine0);
+    mult(A0);
+  }
+  if (A1) {
+    side(1.0, rho1, u1, h1, T1, R1, gam1, kine1);
+    mult(A1);
+  }
+}
+
+// calcViscousFluxNS, JoeWithModels.cpp:3591-3855 (laminar, no kine)
+void calc_viscous_flux(or_ctx &c, double *A) {
+  const or_mesh &m = c.m;
+  const double gam = c.p.gamma, R = c.p.R_gas;
+  const bool ls = c.p.grad_scheme == OR_GRAD_LS;
+  const bool lim = c.p.visc_limiter && c.p.limiter;
+  // gradients with boundary values (:3616-3630)
+  if (!c.p.visc_limiter || !c.p.second_order) {
+    if (ls) grad_vector_ls(c, c.grad_u.data(), c.vel.data(), c.vel_bfa.data());
+    else grad_vector_gg(c, c.grad_u.data(), c.vel.data(), c.vel_bfa.data());
+    if (lim) limit_vector(c, c.grad_u.data(), c.vel.data(), c.sos.data());
+  }
+  if (ls) grad_scalar_ls(c, c.grad_h.data(), c.enthalpy.data(), c.h_bfa.data());
+  else grad_scalar_gg(c, c.grad_h.data(), c.enthalpy.data(), c.h_bfa.data());
+  if (lim) limit_scalar(c, c.grad_h.data(), c.enthalpy.data(), c.enthalpy.data(), NULL);
+
+  double A0[5][5], A1[5][5];
+  for (int i = 0; i < 5; i++)
+    for (int j = 0; j < 5; j++) A0[i][j] = A1[i][j] = 0.0;
+  double Frhou[3] = {0.0, 0.0, 0.0}, FrhoE = 0.0;
+  for (int ifa = c.nfa_b; ifa < c.nfa; ifa++) {
+    int icv0 = m.cvofa[2 * ifa], icv1 = m.cvofa[2 * ifa + 1];
+    int noc00 = 0, noc01 = 0, noc11 = 0, noc10 = 0;
+    if (A) {
+      noc00 = m.nbocv_i[icv0];
+      noc01 = noc00;
+      while (m.nbocv_v[++noc01] != icv1) {}
+      noc11 = m.nbocv_i[icv1];
+      noc10 = noc11;
+      while (m.nbocv_v[++noc10] != icv0) {}
+    }
+    double nVec[3];
+    double area = norm3(nVec, &m.fa_normal[3 * ifa]);
+    double sVec[3];
+    for (int i = 0; i < 3; i++) sVec[i] = m.x_cv[3 * icv1 + i] - m.x_cv[3 * icv0 + i];
+    double smag = sqrt(dot3(sVec, sVec));
+    for (int i = 0; i < 3; i++) sVec[i] /= smag;
+    const double *v0 = &c.vel[3 * icv0], *v1 = &c.vel[3 * icv1];
+    double uAux_fa[3] = {0.5 * (v0[0] + v1[0]), 0.5 * (v0[1] + v1[1]), 0.5 * (v0[2] + v1[2])};
+    add_visc_flux(c, Frhou, FrhoE, A ? A0 : NULL, A ? A1 : NULL, c.rho[icv0], v0, &c.grad_u[9 * icv0],
+                  c.enthalpy[icv0], &c.grad_h[3 * icv0], c.temp[icv0], R, gam, 0.0, c.rho[icv1], v1,
+                  &c.grad_u[9 * icv1], c.enthalpy[icv1], &c.grad_h[3 * icv1], c.temp[icv1], R, gam, 0.0,
+                  c.mul_fa[ifa], 0.0, c.lamOcp_fa[ifa], uAux_fa, area, nVec, smag, sVec);
+    if (A) {
+      for (int i = 1; i < 5; i++)
+        for (int j = 0; j < 5; j++) {
+          A[25 * noc00 + 5 * i + j] -= A0[i][j];
+          A[25 * noc01 + 5 * i + j] -= A1[i][j];
+        }
+      for (int i = 1; i < 5; i++)
+        for (int j = 0; j < 5; j++) {
+          A[25 * noc11 + 5 * i + j] += A1[i][j];
+          A[25 * noc10 + 5 * i + j] += A0[i][j];
+        }
+    }
+    for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= Frhou[i];
+    c.rhs_rhoE[icv0] -= FrhoE;
+    for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv1 + i] += Frhou[i];
+    c.rhs_rhoE[icv1] += FrhoE;
+  }
+  // boundary faces, zone by zone (:3714-3851)
+  for (int iz = 0; iz < m.nzone; iz++) {
+    const int bc = m.zone_bc[iz];
+    if (bc == OR_BC_SYMMETRY) continue;  // only the k term, absent without a turbulence model
+    const int f0 = m.zone_ifa[2 * iz], f1 = m.zone_ifa[2 * iz + 1];
+    for (int ifa = f0; ifa <= f1; ifa++) {
+      int icv0 = m.cvofa[2 * ifa];
+      double nVec[3];
+      double area = norm3(nVec, &m.fa_normal[3 * ifa]);
+      double sVec[3];
+      for (int i = 0; i < 3; i++) sVec[i] = m.x_fa[3 * ifa + i] - m.x_cv[3 * icv0 + i];
+      double smag;
+      const double *sArg;
+      if (bc == OR_BC_WALL) {
+        smag = fabs(dot3(sVec, nVec));  // wall distance; the normal replaces s (:3777-3781)
+        sArg = nVec;
+      } else {
+        smag = sqrt(dot3(sVec, sVec));
+        for (int i = 0; i < 3; i++) sVec[i] /= smag;
+        sArg = sVec;
+      }
+      const double *vb = &c.vel_bfa[3 * ifa];
+      add_visc_flux(c, Frhou, FrhoE, A ? A0 : NULL, NULL, c.rho[icv0], &c.vel[3 * icv0], &c.grad_u[9 * icv0],
+                    c.enthalpy[icv0], &c.grad_h[3 * icv0], c.temp[icv0], R, gam, 0.0, c.rho_bfa[ifa], vb,
+                    &c.grad_u[9 * icv0], c.h_bfa[ifa], &c.grad_h[3 * icv0], c.T_bfa[ifa], R, gam, 0.0, c.mul_fa[ifa],
+                    0.0, c.lamOcp_fa[ifa], vb, area, nVec, smag, sArg);
+      if (A) {
+        int noc00 = m.nbocv_i[icv0];
+        for (int i = 1; i < 5; i++)
+          for (int j = 0; j < 5; j++) A[25 * noc00 + 5 * i + j] -= A0[i][j];
+      }
+      for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= Frhou[i];
+      c.rhs_rhoE[icv0] -= FrhoE;
+    }
+  }
+}
+
 void calc_rhs(or_ctx &c, double *A) {
   for (int icv = 0; icv < c.ncv; icv++) {
     c.rhs_rho[icv] = 0.0;
@@ -1101,6 +1336,7 @@ void calc_rhs(or_ctx &c, double *A) {
     c.rhs_rhoE[icv] = 0.0;
   }
   calc_euler_flux(c, A);
+  if (c.p.mu_ref > 0.0) calc_viscous_flux(c, A);
   const double *g = c.p.gravity;
   for (int icv = 0; icv < c.ncv; icv++) {
     double vol = c.m.cv_volume[icv];
@@ -1206,6 +1442,7 @@ int bcgstab(const or_ctx &c, double *phi, const double *A, const double *rhs, do
 // state + BC refresh done at the start of runBackwardEuler and after each update
 void refresh(or_ctx &c) {
   calc_state(c);
+  calc_material(c);
   set_bc(c);
 }
 
@@ -1245,6 +1482,9 @@ or_ctx *or_ctx_create(const or_mesh *m, const or_params *p) {
   c->rhs_rhoE.assign(n, 0.0);
   c->A.assign(25 * (size_t)m->nnz, 0.0);
   c->dq.assign(5 * n, 0.0);
+  c->mul_fa.assign(m->nfa, 0.0);
+  c->lamOcp_fa.assign(m->nfa, 0.0);
+  c->grad_h.assign(3 * n, 0.0);
   c->resid.assign(5, 0.0);
   c->cfl = p->cfl;
   c->step = 0;
@@ -1351,6 +1591,8 @@ long long or_get(const or_ctx *c, const char *name, double *out) {
   else if (n == "PRESS") v = &c->press;
   else if (n == "TEMP") v = &c->temp;
   else if (n == "ENTHALPY") v = &c->enthalpy;
+  else if (n == "MUL_FA") v = &c->mul_fa;
+  else if (n == "GRAD_ENTHALPY") v = &c->grad_h;
   else if (n == "SOS") v = &c->sos;
   else if (n == "RHO_BFA") v = &c->rho_bfa;
   else if (n == "VEL_BFA") v = &c->vel_bfa;

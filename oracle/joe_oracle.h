@@ -57,6 +57,11 @@ typedef struct {
   double cfl_factor, cfl_max;
   int nsteps;
   double resid_energ_th;
+  /* viscous terms (MU_REF > 0): MU_MODE 0 POWERLAW / 1 SUTHERLAND,
+   * MU_POWER_LAW, SL_TREF, SL_SREF, T_REF, PR, PRTURB, VISC_LIMITER */
+  int visc_mode;
+  double mu_power_law, sl_tref, sl_sref, T_ref, pr, pr_turb;
+  int visc_limiter;
 } or_params;
 
 typedef struct or_ctx or_ctx;
@@ -74,7 +79,7 @@ int or_step(or_ctx *c);
 int or_run(or_ctx *c, int max_steps, double *resid_hist /* [max_steps][5] or NULL */);
 /* named arrays: RHS_RHO RHS_RHOU RHS_RHOE A LOCAL_DT GRAD_RHO GRAD_P GRAD_U
  * ALPHA_RHO VEL PRESS TEMP ENTHALPY SOS RHO_BFA VEL_BFA P_BFA T_BFA H_BFA DQ
- * RESID (5) ; returns element count or -1 */
+ * RESID (5) MUL_FA GRAD_ENTHALPY; returns element count or -1 */
 long long or_get(const or_ctx *c, const char *name, double *out);
 /* current CFL and step counter */
 double or_cfl(const or_ctx *c);

@@ -39,7 +39,10 @@ class _Params(C.Structure):
                 ("symm_bc", C.c_int), ("eps_limiter", C.c_double), ("cfl", C.c_double), ("time_step_min", C.c_int),
                 ("under_relax", C.c_double), ("ls_maxiter", C.c_int), ("ls_abs", C.c_double), ("ls_rel", C.c_double),
                 ("cfl_after", C.c_int), ("cfl_interval", C.c_int), ("cfl_factor", C.c_double),
-                ("cfl_max", C.c_double), ("nsteps", C.c_int), ("resid_energ_th", C.c_double)]
+                ("cfl_max", C.c_double), ("nsteps", C.c_int), ("resid_energ_th", C.c_double),
+                ("visc_mode", C.c_int), ("mu_power_law", C.c_double), ("sl_tref", C.c_double),
+                ("sl_sref", C.c_double), ("T_ref", C.c_double), ("pr", C.c_double), ("pr_turb", C.c_double),
+                ("visc_limiter", C.c_int)]
 
 
 def lib():
@@ -104,8 +107,16 @@ def make_params(joe_in: str) -> tuple[_Params, dict]:
     p.gamma = float(get("GAMMA", "1.4"))
     p.R_gas = float(get("P_REF")) / float(get("RHO_REF")) / float(get("T_REF"))
     p.mu_ref = float(get("MU_REF", "0.0"))
-    if p.mu_ref > 0:
-        raise NotImplementedError("viscous flux not restated")
+    # viscous keywords (UgpWithCvCompFlow.h:155-186)
+    mode = get("MU_MODE", "POWERLAW")
+    p.visc_mode = {"POWERLAW": 0, "SUTHERLAND": 1}[mode]
+    p.mu_power_law = float(get("MU_POWER_LAW", "0.76"))
+    p.sl_tref = float(get("SL_TREF", "273.15"))
+    p.sl_sref = float(get("SL_SREF", "110.4"))
+    p.T_ref = float(get("T_REF"))
+    p.pr = float(get("PR", "0.72"))
+    p.pr_turb = float(get("PRTURB", "0.9"))
+    p.visc_limiter = 1 if get("VISC_LIMITER", "YES") == "YES" else 0
     for i, k in enumerate(("GX", "GY", "GZ")):
         p.gravity[i] = float(get(k, "0.0"))
     p.scheme = SCHEME[get("SPACE_INTEGRATION", "HLLC")]

@@ -168,6 +168,7 @@ class RefJoe : public JoeWithModels {
     d.f64("P_BFA", p_bfa, nfa_b);
     d.f64("T_BFA", T_bfa, nfa_b);
     d.f64("H_BFA", h_bfa, nfa_b);
+    if (mu_ref > 0.0) d.f64("MUL_FA", mul_fa, nfa);
   }
 
   // One implicit residual evaluation in the order runBackwardEuler uses
@@ -211,6 +212,8 @@ class RefJoe : public JoeWithModels {
     d.f64("RHS_RHOU", &RHSrhou[0][0], 3LL * ncv);
     d.f64("RHS_RHOE", RHSrhoE, ncv);
     d.f64("A", &A[0][0][0], 25LL * nbocv_s);
+    if (mu_ref > 0.0) d.f64("GRAD_ENTHALPY", &grad_enthalpy[0][0], 3LL * ncv);
+    if (mu_ref > 0.0 && !sndOrder) d.f64("GRAD_U", &grad_u[0][0][0], 9LL * ncv);
     if (sndOrder) {
       d.f64("GRAD_RHO", &grad_rho[0][0], 3LL * ncv);
       d.f64("GRAD_P", &grad_p[0][0], 3LL * ncv);

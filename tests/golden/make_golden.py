@@ -32,7 +32,14 @@ REF = os.path.join(ROOT, "oracle", "_ref", "joe_ref_dump")
 MESH_KEYS = ["COUNTS", "X_NO", "NOOFA_I", "NOOFA_V", "CVOFA", "FAOCV_I", "FAOCV_V", "NBOCV_I", "NBOCV_V", "X_FA",
              "FA_NORMAL", "X_CV", "CV_VOLUME", "NBOCV_LSG", "BFA_LSG", "ZONES", "ZONE_NAMES"]
 RHS_KEYS = ["PRESS", "VEL", "RHO_BFA", "VEL_BFA", "P_BFA", "T_BFA", "H_BFA", "LOCAL_DT", "GRAD_RHO", "GRAD_P",
-            "GRAD_U", "ALPHA_RHO", "RHS_RHO", "RHS_RHOU", "RHS_RHOE", "A", "DQ"]
+            "GRAD_U", "ALPHA_RHO", "RHS_RHO", "RHS_RHOU", "RHS_RHOE", "A", "DQ", "MUL_FA", "GRAD_ENTHALPY"]
+
+# viscous variants: Joe.in lines that replace "MU_REF = 0.0"
+VISC = {
+    "pl": "MU_REF = 0.002\nMU_MODE = POWERLAW\nMU_POWER_LAW = 0.76\nPR = 0.72",
+    "su": "MU_REF = 0.003\nMU_MODE = SUTHERLAND\nSL_TREF = 1.0\nSL_SREF = 0.4\nPR = 0.7",
+    "nl": "MU_REF = 0.002\nMU_MODE = POWERLAW\nVISC_LIMITER = NO",
+}
 
 MESHES = {
     "bump24": lambda scheme, **kw: cases.bump_case(24, 8, scheme=scheme, **kw),
@@ -43,7 +50,13 @@ RHS_CASES = [("bump24", "HLLC", {}), ("bump24", "ROE", {}), ("bump24", "AUSM", {
              ("box4", "HLLC", {"grad": "GREENGAUSS"}), ("box4", "ROE", {"second_order": False}),
              ("box4", "HLLC", {"bcs": cases.BOX_BCS_ALT, "symm_bc": "NEW_NUMERIC"}),
              ("box4", "ROE", {"bcs": cases.BOX_BCS_ALT, "symm_bc": "ANALYTICAL"}),
-             ("box4", "AUSM", {"bcs": cases.BOX_BCS_ALT, "symm_bc": "ANALYTICAL", "grad": "GREENGAUSS"})]
+             ("box4", "AUSM", {"bcs": cases.BOX_BCS_ALT, "symm_bc": "ANALYTICAL", "grad": "GREENGAUSS"}),
+             ("bump24", "HLLC", {"visc": "pl"}),
+             ("box4", "ROE", {"visc": "su"}),
+             ("box4", "AUSM", {"visc": "pl", "bcs": cases.BOX_BCS_ALT, "symm_bc": "NEW_NUMERIC",
+                               "grad": "GREENGAUSS"}),
+             ("box4", "HLLC", {"visc": "su", "second_order": False}),
+             ("box4", "HLLC", {"visc": "nl"})]
 
 
 def run_ref(args, cwd):
@@ -53,7 +66,7 @@ def run_ref(args, cwd):
 
 
 def tag(kw):
-    t = ""
+    t = "_visc_" + kw["visc"] if "visc" in kw else ""
     if "bcs" in kw:
         t += "_alt_" + {"NEW_NUMERIC": "new", "ANALYTICAL": "ana"}[kw["symm_bc"]]
     if kw.get("grad") == "GREENGAUSS":
@@ -69,7 +82,12 @@ def main():
     with tempfile.TemporaryDirectory() as wd:
         done_mesh = set()
         for name, scheme, kw in RHS_CASES:
+            kw = dict(kw)
+            visc = kw.pop("visc", None)
             mm, joe = MESHES[name](scheme, **kw)
+            if visc:
+                kw["visc"] = visc
+                joe = joe.replace("MU_REF = 0.0", VISC[visc])
             fn = "box.msh" if name.startswith("box") else "bump.msh"
             meshgen.write_msh(mm, os.path.join(wd, fn))
             with open(os.path.join(wd, "Joe.in"), "w") as f:
@@ -87,8 +105,10 @@ def main():
             print("wrote", name, scheme, kw)
 
         # steady-loop trajectories of JoeWithModels::run (uniform initial state)
-        for name, nx, ny, nsteps in [("bump24", 24, 8, 60), ("bump96", 96, 32, 300)]:
+        for name, nx, ny, nsteps in [("bump24", 24, 8, 60), ("bump96", 96, 32, 300), ("bump24visc", 24, 8, 60)]:
             mm, joe = cases.bump_case(nx, ny, nsteps=nsteps)
+            if name.endswith("visc"):
+                joe = joe.replace("MU_REF = 0.0", VISC["pl"])
             meshgen.write_msh(mm, os.path.join(wd, "bump.msh"))
             with open(os.path.join(wd, "Joe.in"), "w") as f:
                 f.write(joe)

@@ -21,6 +21,7 @@ GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 RHS_FIXTURES = sorted(os.path.basename(p)[4:-4] for p in glob.glob(os.path.join(GOLD, "rhs_*.npz")))
 FIELDS = ["PRESS", "VEL", "RHO_BFA", "VEL_BFA", "P_BFA", "T_BFA", "H_BFA", "LOCAL_DT", "GRAD_RHO", "GRAD_P",
           "GRAD_U", "ALPHA_RHO", "RHS_RHO", "RHS_RHOU", "RHS_RHOE", "A"]
+VISC_FIELDS = ["MUL_FA", "GRAD_ENTHALPY"]
 
 
 def load(name):
@@ -49,6 +50,10 @@ def test_fixture_set_complete():
         symm.add(P["SYMM_BC"][0][1])
     assert kinds >= {"CBC", "CBC_SUBSONIC_INLET", "CBC_SUBSONIC_OUTLET", "WALL", "FARFIELD", "NEUMANN", "SYMMETRY"}
     assert symm == {"ORIGINAL_NUMERIC", "NEW_NUMERIC", "ANALYTICAL"}
+    # viscous terms: both viscosity laws, with and without the gradient limiter, first order
+    visc = [f for f in RHS_FIXTURES if "visc" in f]
+    assert any("_pl" in f for f in visc) and any("_su" in f for f in visc) and any("_nl" in f for f in visc)
+    assert any("first" in f for f in visc)
 
 
 @pytest.mark.parametrize("fixture", RHS_FIXTURES)
@@ -58,7 +63,7 @@ def test_residual_bit_exact(fixture):
     o.set_state(g["RHO"], g["RHOU"], g["RHOE"])
     o.residual(True)
     first = "first" in fixture
-    for k in FIELDS:
+    for k in FIELDS + (VISC_FIELDS if "visc" in fixture else []):
         if k not in g:
             assert first and k.startswith(("GRAD", "ALPHA")), k
             continue
@@ -70,12 +75,12 @@ def test_residual_bit_exact(fixture):
     assert np.array_equal(o.get("DQ"), g["DQ"]), fixture
 
 
-@pytest.mark.parametrize("name", ["bump24", "bump96"])
+@pytest.mark.parametrize("name", ["bump24", "bump96", "bump24visc"])
 def test_run_trajectory_bit_exact(name):
     g = load("solve_" + name)
     joe = joe_text(g)
     P = parse_joe_in(joe)
-    m = load("mesh_" + name)
+    m = load("mesh_" + name.replace("visc", ""))
     o = Oracle(m, joe)
     ncv = int(m["COUNTS"][3])
     # JoeWithModels::initialHook uniform state (JoeWithModels.cpp:180-205)
