@@ -49,6 +49,9 @@ struct SolverParams {
   double resid_energ_th = 0.0, resid_energ_rel = 0.0;
   double rho_init = 1.0, p_init = 1.0, u_init[3] = {0, 0, 0};
   int cell_order = kOrderMorton;
+  double mu_ref = 0.0, mu_power_law = 0.76, sl_tref = 273.15, sl_sref = 110.4, pr = 0.72, pr_turb = 0.9;
+  int visc_mode = 0;
+  bool visc_limiter = true;
   int grad_kernel = kResidCell, flux_kernel = kResidEntry;
 };
 

@@ -64,6 +64,9 @@ struct DMesh {
 struct DParams {
   double gamma, R, eps, g[3];
   int scheme, second_order, grad_ls, limiter, symm_bc, gravity_on;
+  // viscous terms (MU_REF > 0): viscosity law, conductivity, gradient limiting
+  int visc, visc_mode, visc_limiter, pad;
+  double mu_ref, mu_power_law, sl_tref, sl_sref, T_ref, pr, pr_turb;
 };
 
 // per-solve scalars of the BiCGStab and of reductions (device resident)
@@ -262,18 +265,32 @@ __device__ __forceinline__ double bj_face(double alfa, double phi, double phifa,
 // (:1091-1188), limiters (:881-1000). One thread per cell; the cell's sums
 // follow the reference's order (diagonal, nbocv neighbours, boundary faces).
 // ---------------------------------------------------------------------------
-template <bool LS>
+// enthalpy of a cell state (calcStateVariables: T = p / (rho R),
+// h = gamma R / (gamma - 1) T; UgpWithCvCompFlow.h:1680-1690)
+__device__ __forceinline__ double enthalpy_of(const DParams &P, double rho, double p) {
+  const double T = p / (rho * P.R);
+  return P.gamma * P.R / (P.gamma - 1.0) * T;
+}
+
+// With VISC the kernel also forms the enthalpy gradient and the velocity and
+// enthalpy gradients the viscous flux uses (calcViscousFluxNS,
+// JoeWithModels.cpp:3616-3630): limited when VISC_LIMITER and LIMITER are on,
+// raw otherwise; written to gradv[ncv][12] (grad u 9, grad h 3).
+template <bool LS, bool VISC>
 __global__ void __launch_bounds__(kBlock) k_grad(DMesh m, DParams P, const double *__restrict__ prim,
-                                                 const double *__restrict__ bfa, double *__restrict__ grad) {
+                                                 const double *__restrict__ bfa, double *__restrict__ grad,
+                                                 double *__restrict__ gradv) {
   const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= m.ncv) return;
   const int c = m.sched[t];
   const double *pc = prim + 8 * (size_t)c;
   const double rho = pc[0], u[3] = {pc[1], pc[2], pc[3]}, p = pc[4];
   const double xcv[3] = {pc[5], pc[6], pc[7]};
-  double gr[3], gp[3], gu[3][3];
+  double gr[3], gp[3], gu[3][3], gh[3] = {0.0, 0.0, 0.0};
   double rmax = rho, rmin = rho, pmax = p, pmin = p;
   double umax[3] = {u[0], u[1], u[2]}, umin[3] = {u[0], u[1], u[2]};
+  const double h = VISC ? enthalpy_of(P, rho, p) : 0.0;
+  double hmax = h, hmin = h;
   const int e0 = m.cfl_i[c], e1 = m.cfl_i[c + 1];
   if (LS) {
     const int noc_f = m.nbocv_i[c];
@@ -281,6 +298,7 @@ __global__ void __launch_bounds__(kBlock) k_grad(DMesh m, DParams P, const doubl
     for (int j = 0; j < 3; j++) {
       gr[j] = l0[j] * rho;
       gp[j] = l0[j] * p;
+      if (VISC) gh[j] = l0[j] * h;
     }
     for (int i = 0; i < 3; i++)
       for (int j = 0; j < 3; j++) gu[i][j] = l0[j] * u[i];
@@ -295,6 +313,11 @@ __global__ void __launch_bounds__(kBlock) k_grad(DMesh m, DParams P, const doubl
       double l_0 = l[0], l_1 = l[1], l_2 = l[2];
       gr[0] += l_0 * rn; gr[1] += l_1 * rn; gr[2] += l_2 * rn;
       gp[0] += l_0 * pnv; gp[1] += l_1 * pnv; gp[2] += l_2 * pnv;
+      if (VISC) {
+        const double hn = enthalpy_of(P, rn, pnv);
+        gh[0] += l_0 * hn; gh[1] += l_1 * hn; gh[2] += l_2 * hn;
+        hmax = Dev::mx(hmax, hn); hmin = Dev::mn(hmin, hn);
+      }
       for (int i = 0; i < 3; i++) {
         gu[i][0] += l_0 * un[i];
         gu[i][1] += l_1 * un[i];
@@ -321,6 +344,10 @@ __global__ void __launch_bounds__(kBlock) k_grad(DMesh m, DParams P, const doubl
       }
       gr[0] += b0 * rb; gr[1] += b1 * rb; gr[2] += b2 * rb;
       gp[0] += b0 * pb; gp[1] += b1 * pb; gp[2] += b2 * pb;
+      if (VISC) {
+        const double hb = bb[6];
+        gh[0] += b0 * hb; gh[1] += b1 * hb; gh[2] += b2 * hb;
+      }
     }
   } else {
     for (int j = 0; j < 3; j++) {
@@ -357,6 +384,17 @@ __global__ void __launch_bounds__(kBlock) k_grad(DMesh m, DParams P, const doubl
             gp[j] -= fp * n[j];
           }
         }
+        if (VISC) {
+          const double hn = enthalpy_of(P, rn, pnv);
+          const double fh = s0 ? (w1 * h + w0 * hn) / (w0 + w1) : (w1 * hn + w0 * h) / (w0 + w1);
+          for (int j = 0; j < 3; j++) {
+            if (s0)
+              gh[j] += fh * n[j];
+            else
+              gh[j] -= fh * n[j];
+          }
+          hmax = Dev::mx(hmax, hn); hmin = Dev::mn(hmin, hn);
+        }
         for (int i = 0; i < 3; i++) {
           double fu = s0 ? (w1 * u[i] + w0 * un[i]) / (w0 + w1) : (w1 * un[i] + w0 * u[i]) / (w0 + w1);
           for (int j = 0; j < 3; j++) {
@@ -376,6 +414,7 @@ __global__ void __launch_bounds__(kBlock) k_grad(DMesh m, DParams P, const doubl
         for (int j = 0; j < 3; j++) {
           gr[j] += rb * n[j];
           gp[j] += pb * n[j];
+          if (VISC) gh[j] += bb[6] * n[j];
         }
         for (int i = 0; i < 3; i++)
           for (int j = 0; j < 3; j++) gu[i][j] += ub[i] * n[j];
@@ -385,12 +424,13 @@ __global__ void __launch_bounds__(kBlock) k_grad(DMesh m, DParams P, const doubl
     for (int j = 0; j < 3; j++) {
       gr[j] /= vol;
       gp[j] /= vol;
+      if (VISC) gh[j] /= vol;
     }
     for (int i = 0; i < 3; i++)
       for (int j = 0; j < 3; j++) gu[i][j] /= vol;
   }
 
-  double au[3] = {1.0, 1.0, 1.0}, ar = 1.0, ap = 1.0;
+  double au[3] = {1.0, 1.0, 1.0}, ar = 1.0, ap = 1.0, ah = 1.0;
   if (P.limiter) {
     const double sos = sqrt(P.gamma * p / rho);
     double e2u = P.eps * sos;
@@ -416,6 +456,28 @@ __global__ void __launch_bounds__(kBlock) k_grad(DMesh m, DParams P, const doubl
       if (!lu[i]) au[i] = 0.0;
     if (!lr) ar = 0.0;
     if (!lp) ap = 0.0;
+    if (VISC && P.visc_limiter) {  // enthalpy, reference value h (JoeWithModels.cpp:3619)
+      double e2h = Dev::mx(P.eps * fabs(h), 1.0e-8);
+      e2h = e2h * e2h;
+      if ((hmax - hmin) > 1.0e-12) {
+        for (int e = e0; e < e1; e++) {
+          int2 en = m.cfl[e];
+          if (en.y < 0) continue;
+          const double *geo = m.fgeo + 8 * (size_t)(en.x & 0x7fffffff);
+          double r0[3] = {geo[4] - xcv[0], geo[5] - xcv[1], geo[6] - xcv[2]};
+          ah = bj_face(ah, h, h + dot3(r0, gh), hmax, hmin, e2h);
+        }
+      } else {
+        ah = 0.0;
+      }
+    }
+  }
+  if (VISC) {
+    const bool lim = P.limiter && P.visc_limiter;
+    double *v = gradv + 12 * (size_t)c;
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) v[3 * i + j] = lim ? gu[i][j] * au[i] : gu[i][j];
+    for (int j = 0; j < 3; j++) v[9 + j] = lim ? gh[j] * ah : gh[j];
   }
   double *o = grad + 16 * (size_t)c;
   o[0] = gr[0] * ar; o[1] = gr[1] * ar; o[2] = gr[2] * ar;
@@ -829,6 +891,154 @@ __device__ __forceinline__ double flux_cell(const DMesh &m, const DParams &P, co
 }
 
 // ---------------------------------------------------------------------------
+// Laminar viscous face flux, addViscFlux (UgpWithCvCompFlow.cpp:1521-1763)
+// with mut = 0, no turbulence model (nonLinear = rij = kine = 0) and
+// SIMPLEX_VISCOUS = NO; the zero turbulent terms are kept so that signed
+// zeros come out as in the reference.
+// ---------------------------------------------------------------------------
+struct VSide {
+  double rho, u[3], h, T;
+  const double *gu, *gh;  // grad u [3][3], grad h [3]
+};
+
+__device__ __forceinline__ double mu_lam(const DParams &P, double T) {
+  if (P.visc_mode == 1) return P.mu_ref * pow(T / P.sl_tref, 1.5) * (P.sl_tref + P.sl_sref) / (T + P.sl_sref);
+  return P.mu_ref * pow(T / P.T_ref, P.mu_power_law);
+}
+
+struct VFace {
+  double tau[3];  // tau_ij n_j
+  double keff, alpha;
+};
+
+__device__ __noinline__ void visc_face(const DParams &P, const VSide &s0, const VSide &s1, double mul,
+                                       const double *u_fa, double area, const double *nVec, double smag,
+                                       const double *sVec, double Frhou[3], double &FrhoE, VFace &vf) {
+  const double mut = 0.0, nonL = 0.0;
+  const double alpha = dot3(nVec, sVec);
+  double guf[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) guf[i][j] = 0.5 * (s0.gu[3 * i + j] + s1.gu[3 * i + j]);
+  const double fCorr[3] = {nVec[0] - alpha * sVec[0], nVec[1] - alpha * sVec[1], nVec[2] - alpha * sVec[2]};
+  double tauLam[3], tauTurb[3], tmp;
+  for (int i = 0; i < 3; i++)
+    tauLam[i] = mul * (alpha * (s1.u[i] - s0.u[i]) / smag + guf[i][0] * fCorr[0] + guf[0][i] * nVec[0] +
+                       guf[i][1] * fCorr[1] + guf[1][i] * nVec[1] + guf[i][2] * fCorr[2] + guf[2][i] * nVec[2]);
+  tmp = 2.0 / 3.0 * mul * (guf[0][0] + guf[1][1] + guf[2][2]);
+  for (int i = 0; i < 3; i++) tauLam[i] -= tmp * nVec[i];
+  for (int i = 0; i < 3; i++)
+    tauTurb[i] = (1.0 - nonL) * mut *
+                 (alpha * (s1.u[i] - s0.u[i]) / smag + guf[i][0] * fCorr[0] + guf[0][i] * nVec[0] +
+                  guf[i][1] * fCorr[1] + guf[1][i] * nVec[1] + guf[i][2] * fCorr[2] + guf[2][i] * nVec[2]);
+  tmp = (1.0 - nonL) * 2.0 / 3.0 * mut * (guf[0][0] + guf[1][1] + guf[2][2]);
+  for (int i = 0; i < 3; i++) tauTurb[i] -= tmp * nVec[i];
+  tauTurb[0] += nonL * (0.0 * nVec[0] + 0.0 * nVec[1] + 0.0 * nVec[2]);
+  tauTurb[1] += nonL * (0.0 * nVec[0] + 0.0 * nVec[1] + 0.0 * nVec[2]);
+  tauTurb[2] += nonL * (0.0 * nVec[0] + 0.0 * nVec[1] + 0.0 * nVec[2]);
+  for (int i = 0; i < 3; i++) vf.tau[i] = tauLam[i] + tauTurb[i];
+  for (int i = 0; i < 3; i++) Frhou[i] = -area * vf.tau[i];
+  const double keff = mul / P.pr + mut / P.pr_turb;
+  const double enth = keff * (alpha * (s1.h - s0.h) / smag + 0.5 * ((s0.gh[0] + s1.gh[0]) * fCorr[0] +
+                                                                    (s0.gh[1] + s1.gh[1]) * fCorr[1] +
+                                                                    (s0.gh[2] + s1.gh[2]) * fCorr[2]));
+  const double psi = 0.0;
+  FrhoE = -area * (enth + vf.tau[0] * u_fa[0] + vf.tau[1] * u_fa[1] + vf.tau[2] * u_fa[2] + psi);
+  vf.keff = keff;
+  vf.alpha = alpha;
+}
+
+// one half of the viscous Jacobian (A0: sgn -1 with side 0's state, A1: +1
+// with side 1's), dFdU x dUdQ (:1662-1762, :712-745); rows 1-4 only
+__device__ __noinline__ void visc_jac(const DParams &P, double J[5][5], double sgn, const VSide &s, double mul,
+                                      const VFace &vf, const double *u_fa, double area, double smag) {
+  const double mut = 0.0, kine = 0.0;
+  const double gam = P.gamma, R = P.R;
+  double f11 = sgn * (mul + mut) * vf.alpha / smag * area;
+  double f22 = sgn * (mul + mut) * vf.alpha / smag * area;
+  double f33 = sgn * (mul + mut) * vf.alpha / smag * area;
+  double f41 = area * vf.tau[0] * 0.5 + u_fa[0] * f11;
+  double f42 = area * vf.tau[1] * 0.5 + u_fa[1] * f22;
+  double f43 = area * vf.tau[2] * 0.5 + u_fa[2] * f33;
+  double f44 = sgn * area * vf.keff * vf.alpha / smag;
+  const double f12 = 0.0, f13 = 0.0, f21 = 0.0, f23 = 0.0, f31 = 0.0, f32 = 0.0;
+  const double invRho = 1.0 / s.rho;
+  const double q10 = -s.u[0] * invRho, q20 = -s.u[1] * invRho, q30 = -s.u[2] * invRho;
+  const double q40 = gam * invRho * (-s.h - kine + R * s.T + 0.5 * dot3(s.u, s.u));
+  const double q11 = invRho, q41 = -gam * s.u[0] * invRho;
+  const double q22 = invRho, q42 = -gam * s.u[1] * invRho;
+  const double q33 = invRho, q43 = -gam * s.u[2] * invRho;
+  const double q44 = gam * invRho;
+  const double q14 = 0.0, q24 = 0.0, q34 = 0.0;
+  J[1][0] = f11 * q10 + f12 * q20 + f13 * q30;
+  J[2][0] = f21 * q10 + f22 * q20 + f23 * q30;
+  J[3][0] = f31 * q10 + f32 * q20 + f33 * q30;
+  J[4][0] = f41 * q10 + f42 * q20 + f43 * q30 + f44 * q40;
+  J[1][1] = f11 * q11;
+  J[2][1] = f21 * q11;
+  J[3][1] = f31 * q11;
+  J[4][1] = f41 * q11 + f44 * q41;
+  J[1][2] = f12 * q22;
+  J[2][2] = f22 * q22;
+  J[3][2] = f32 * q22;
+  J[4][2] = f42 * q22 + f44 * q42;
+  J[1][3] = f13 * q33;
+  J[2][3] = f23 * q33;
+  J[3][3] = f33 * q33;
+  J[4][3] = f43 * q33 + f44 * q43;
+  J[4][4] = f41 * q14 + f42 * q24 + f43 * q34 + f44 * q44;
+  J[1][4] = J[2][4] = J[3][4] = 0.0;  // A0/A1 keep these zero (only A[4][4] is set in column 4)
+}
+
+// The viscous part of one cell-face entry: the face's viscous flux signed for
+// this cell (flux(icv0 -> icv1) taken from icv0, added to icv1), and the
+// face's mul for the viscous time-step bound. Symmetry faces contribute
+// nothing (signed -0.0 terms leave the sums unchanged).
+struct VEntry {
+  double c[4];  // rhou(3), rhoE
+  double mul;
+};
+
+__device__ __forceinline__ VSide vside_cell(const DParams &P, const double *pc, const double *gv) {
+  VSide s;
+  s.rho = pc[0];
+  s.u[0] = pc[1]; s.u[1] = pc[2]; s.u[2] = pc[3];
+  s.T = pc[4] / (pc[0] * P.R);
+  s.h = P.gamma * P.R / (P.gamma - 1.0) * s.T;
+  s.gu = gv;
+  s.gh = gv + 9;
+  return s;
+}
+
+__device__ __forceinline__ VSide vside_bfa(const double *bb, const double *gv) {
+  VSide s;
+  s.rho = bb[0];
+  s.u[0] = bb[1]; s.u[1] = bb[2]; s.u[2] = bb[3];
+  s.T = bb[5];
+  s.h = bb[6];
+  s.gu = gv;
+  s.gh = gv + 9;
+  return s;
+}
+
+// geometry of an internal face for the viscous terms: icv0 -> icv1 unit
+// vector and distance, and the distance-weighted face temperature's mul
+// (calcMaterialProperties, UgpWithCvCompFlow.h:1700-1750)
+__device__ __forceinline__ void visc_internal_geom(const DParams &P, const double *x0, const double *x1,
+                                                   const double *xfa, double T0, double T1, double sVec[3],
+                                                   double &smag, double &mul) {
+  for (int i = 0; i < 3; i++) sVec[i] = x1[i] - x0[i];
+  smag = sqrt(dot3(sVec, sVec));
+  for (int i = 0; i < 3; i++) sVec[i] /= smag;
+  double dx0[3], dx1[3];
+  for (int i = 0; i < 3; i++) {
+    dx0[i] = xfa[i] - x0[i];
+    dx1[i] = xfa[i] - x1[i];
+  }
+  const double w0 = sqrt(dot3(dx0, dx0)), w1 = sqrt(dot3(dx1, dx1));
+  mul = mu_lam(P, (w1 * T0 + w0 * T1) / (w0 + w1));
+}
+
+// ---------------------------------------------------------------------------
 // Block Jacobian: the implicit parts of the same face loops
 // (JoeWithModels.cpp:3029-3101, 3213-3277, 3359-3385, 3517-3563) with
 // cell-centre states, gathered per row; optionally the backward-Euler
@@ -852,11 +1062,12 @@ __device__ __forceinline__ void face_jac(int scheme, double J[5][5], int which, 
     jac_hllc(J, which, L.rho, L.u, L.p, R.rho, R.u, R.p, gam, area, n);
 }
 
-template <typename T, int SCHEME>
+template <typename T, int SCHEME, bool VISC>
 __global__ void __launch_bounds__(kBlock) k_jac(DMesh m, DParams P, const double *__restrict__ prim,
                                                 const double *__restrict__ bfa, T *__restrict__ A, int step_mode,
                                                 const double *__restrict__ dt, const double *__restrict__ rhs,
-                                                double relax, T *__restrict__ rhs_ls, T *__restrict__ phi) {
+                                                double relax, T *__restrict__ rhs_ls, T *__restrict__ phi,
+                                                const double *__restrict__ gradv) {
   const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= m.ncv) return;
   const int c = m.sched[t];
@@ -929,6 +1140,66 @@ __global__ void __launch_bounds__(kBlock) k_jac(DMesh m, DParams P, const double
         face_jac(SCHEME, J, 0, own, sb, gam, area, n);
         for (int i = 0; i < 5; i++)
           for (int j = 0; j < 5; j++) D[i][j] += J[i][j];
+      }
+    }
+  }
+  if (VISC) {  // viscous Jacobian blocks, after the Euler ones (JoeWithModels.cpp:3680-3697, 3789-3796, 3835-3842)
+    const VSide vo = vside_cell(P, pc, gradv + 12 * (size_t)c);
+    for (int e = e0; e < e1; e++) {
+      int2 en = m.cfl[e];
+      int f = en.x & 0x7fffffff;
+      const double *geo = m.fgeo + 8 * (size_t)f;
+      const double n[3] = {geo[0], geo[1], geo[2]};
+      const double area = geo[3];
+      const double xfa[3] = {geo[4], geo[5], geo[6]};
+      double Fu[3], FE;
+      VFace vf;
+      if (en.y >= 0) {
+        const double *pn = prim + 8 * (size_t)en.y;
+        const VSide vn = vside_cell(P, pn, gradv + 12 * (size_t)en.y);
+        const bool s0 = en.x >= 0;
+        const VSide &a = s0 ? vo : vn, &b = s0 ? vn : vo;
+        double sV[3], smag, mul;
+        visc_internal_geom(P, s0 ? pc + 5 : pn + 5, s0 ? pn + 5 : pc + 5, xfa, a.T, b.T, sV, smag, mul);
+        const double ufa[3] = {0.5 * (a.u[0] + b.u[0]), 0.5 * (a.u[1] + b.u[1]), 0.5 * (a.u[2] + b.u[2])};
+        visc_face(P, a, b, mul, ufa, area, n, smag, sV, Fu, FE, vf);
+        T *off = A + 25 * (size_t)(m.cfl_noc[e] & 0x3fffffff);
+        if (s0) {  // A[noc00] -= A0, A[noc01] -= A1
+          visc_jac(P, J, -1.0, a, mul, vf, ufa, area, smag);
+          for (int i = 1; i < 5; i++)
+            for (int j = 0; j < 5; j++) D[i][j] -= J[i][j];
+          visc_jac(P, J, 1.0, b, mul, vf, ufa, area, smag);
+          for (int i = 1; i < 5; i++)
+            for (int j = 0; j < 5; j++) off[5 * i + j] = (T)((double)off[5 * i + j] - J[i][j]);
+        } else {  // A[noc11] += A1, A[noc10] += A0
+          visc_jac(P, J, 1.0, b, mul, vf, ufa, area, smag);
+          for (int i = 1; i < 5; i++)
+            for (int j = 0; j < 5; j++) D[i][j] += J[i][j];
+          visc_jac(P, J, -1.0, a, mul, vf, ufa, area, smag);
+          for (int i = 1; i < 5; i++)
+            for (int j = 0; j < 5; j++) off[5 * i + j] = (T)((double)off[5 * i + j] + J[i][j]);
+        }
+      } else {
+        const ZoneDev &z = m.zones[-en.y - 1];
+        if (z.bc == kBcSymmetry) continue;
+        const double *bb = bfa + 8 * (size_t)f;
+        const double mul = mu_lam(P, bb[5]);
+        const VSide vb = vside_bfa(bb, gradv + 12 * (size_t)c);
+        double sV[3] = {xfa[0] - pc[5], xfa[1] - pc[6], xfa[2] - pc[7]};
+        double smag;
+        const double *sArg;
+        if (z.bc == kBcWall) {
+          smag = fabs(dot3(sV, n));
+          sArg = n;
+        } else {
+          smag = sqrt(dot3(sV, sV));
+          for (int i = 0; i < 3; i++) sV[i] /= smag;
+          sArg = sV;
+        }
+        visc_face(P, vo, vb, mul, vb.u, area, n, smag, sArg, Fu, FE, vf);
+        visc_jac(P, J, -1.0, vo, mul, vf, vb.u, area, smag);
+        for (int i = 1; i < 5; i++)
+          for (int j = 0; j < 5; j++) D[i][j] -= J[i][j];
       }
     }
   }
@@ -1090,15 +1361,16 @@ __global__ void __launch_bounds__(kBlock) k_flux(DMesh m, DParams P, const doubl
 // face's flux (signed for this cell) and the face's wave-speed bound; the
 // cell's owner lane adds the terms from LDS in entry order (the reference's
 // order), applies the gravity source, and writes the residual and local dt.
-template <int SCHEME, bool SND>
+template <int SCHEME, bool SND, bool VISC>
 __global__ void __launch_bounds__(kBlock) k_flux_e(DMesh m, DParams P, const double *__restrict__ prim,
                                                    const double *__restrict__ grad, const double *__restrict__ bfa,
                                                    const double *__restrict__ q, double *__restrict__ rhs,
                                                    int *__restrict__ nfirst, DtArgs dta, double *__restrict__ dt,
-                                                   double *__restrict__ partials) {
-  __shared__ double lds[kBlock * 6];
+                                                   double *__restrict__ partials, const double *__restrict__ gradv) {
+  constexpr int NS = VISC ? 10 : 6;  // inviscid 5, wave-speed bound, viscous rhou/rhoE 4
+  __shared__ double lds[kBlock * NS];
   __shared__ double sh[16];
-  double *slot = lds + 6 * threadIdx.x;
+  double *slot = lds + NS * threadIdx.x;
   const EntryLane L = entry_lane(m);
   const double gam = P.gamma;
   double pc[8], gc[15];
@@ -1127,11 +1399,12 @@ __global__ void __launch_bounds__(kBlock) k_flux_e(DMesh m, DParams P, const dou
       n[0] = a.x; n[1] = a.y; n[2] = b.x; area = b.y;
       xfa[0] = cc.x; xfa[1] = cc.y; xfa[2] = geo[6];
     }
+    double lambda;
     {
       const double ru[3] = {q[m.ncv + L.c], q[2 * m.ncv + L.c], q[3 * m.ncv + L.c]};
       const double cs = sqrt(gam * pc[4] / pc[0]);
       const double Uk = dot3(ru, n) / pc[0];
-      slot[5] = (fabs(Uk) + cs) * area;
+      lambda = (fabs(Uk) + cs) * area;
     }
     Side own_cc;
     own_cc.rho = pc[0];
@@ -1140,6 +1413,8 @@ __global__ void __launch_bounds__(kBlock) k_flux_e(DMesh m, DParams P, const dou
     double F[5];
     double sgn = -1.0;
     bool low = false;
+    double vc[4] = {-0.0, -0.0, -0.0, -0.0};
+    double mul = 0.0;
     if (en.y >= 0) {
       const bool s0 = (en.x >= 0);
       double pn[8], gn[15];
@@ -1170,6 +1445,19 @@ __global__ void __launch_bounds__(kBlock) k_flux_e(DMesh m, DParams P, const dou
         face_flux(SCHEME, F, sn, so, gam, area, n);
         sgn = 1.0;
       }
+      if (VISC) {  // calcViscousFluxNS internal faces (JoeWithModels.cpp:3635-3710)
+        const VSide vo = vside_cell(P, pc, gradv + 12 * (size_t)L.c);
+        const VSide vn = vside_cell(P, pn, gradv + 12 * (size_t)en.y);
+        const VSide &a = s0 ? vo : vn, &b = s0 ? vn : vo;
+        double sV[3], smag;
+        visc_internal_geom(P, s0 ? pc + 5 : pn + 5, s0 ? pn + 5 : pc + 5, xfa, a.T, b.T, sV, smag, mul);
+        const double ufa[3] = {0.5 * (a.u[0] + b.u[0]), 0.5 * (a.u[1] + b.u[1]), 0.5 * (a.u[2] + b.u[2])};
+        double Fu[3], FE;
+        VFace vf;
+        visc_face(P, a, b, mul, ufa, area, n, smag, sV, Fu, FE, vf);
+        for (int k = 0; k < 3; k++) vc[k] = s0 ? -Fu[k] : Fu[k];
+        vc[3] = s0 ? -FE : FE;
+      }
     } else {
       const ZoneDev &z = m.zones[-en.y - 1];
       const double *bb = bfa + 8 * (size_t)f;
@@ -1187,7 +1475,37 @@ __global__ void __launch_bounds__(kBlock) k_flux_e(DMesh m, DParams P, const dou
         if (SND) low = !reconstruct(so, pc, gc, xfa);
         face_flux(SCHEME, F, so, sb, gam, area, n);
       }
+      if (VISC) {  // boundary faces (:3714-3851); boundary mul from T_bfa (ComputeBCProperties_T)
+        mul = mu_lam(P, bb[5]);
+        if (z.bc != kBcSymmetry) {
+          const double *gvo = gradv + 12 * (size_t)L.c;
+          const VSide vo = vside_cell(P, pc, gvo);
+          const VSide vb = vside_bfa(bb, gvo);
+          double sV[3] = {xfa[0] - pc[5], xfa[1] - pc[6], xfa[2] - pc[7]};
+          double smag;
+          const double *sArg;
+          if (z.bc == kBcWall) {
+            smag = fabs(dot3(sV, n));
+            sArg = n;
+          } else {
+            smag = sqrt(dot3(sV, sV));
+            for (int i = 0; i < 3; i++) sV[i] /= smag;
+            sArg = sV;
+          }
+          double Fu[3], FE;
+          VFace vf;
+          visc_face(P, vo, vb, mul, vb.u, area, n, smag, sArg, Fu, FE, vf);
+          for (int k = 0; k < 3; k++) vc[k] = -Fu[k];
+          vc[3] = -FE;
+        }
+      }
     }
+    if (VISC) {  // viscous bound of calcDt (UgpWithCvCompFlow.cpp:50-51), mut = 0
+      const double lamdiff = 4.0 * (mul + 0.0) / pc[0] * area * area / m.vol[L.c];
+      lambda = Dev::mx(lambda, lamdiff);
+      for (int k = 0; k < 4; k++) slot[6 + k] = vc[k];
+    }
+    slot[5] = lambda;
     // acc - F == acc + (-F) exactly, so the owner only adds
     for (int k = 0; k < 5; k++) slot[k] = sgn < 0.0 ? -F[k] : F[k];
     if (low) atomicAdd(nfirst, 1);
@@ -1200,10 +1518,15 @@ __global__ void __launch_bounds__(kBlock) k_flux_e(DMesh m, DParams P, const dou
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     double lambdaMax = 0.0;
     for (int j = 0; j < nent; j++) {
-      const double *s = slot + 6 * j;
+      const double *s = slot + NS * j;
       for (int k = 0; k < 5; k++) acc[k] += s[k];
       lambdaMax = Dev::mx(lambdaMax, s[5]);
     }
+    if (VISC)  // the viscous pass follows the Euler pass (JoeWithModels.cpp:2585-2589)
+      for (int j = 0; j < nent; j++) {
+        const double *s = slot + NS * j + 6;
+        for (int k = 0; k < 4; k++) acc[1 + k] += s[k];
+      }
     // sourceHook: gravity (JoeWithModels.h:111-129)
     const double vol = m.vol[c];
     if (P.gravity_on) {
@@ -1522,7 +1845,21 @@ void parse_params(const std::string &joe_in, const Mesh &m, SolverParams &p, std
   p.p_ref = pm.dbl("P_REF");
   p.rho_ref = pm.dbl("RHO_REF");
   p.R_gas = p.p_ref / p.rho_ref / p.T_ref;
-  if (pm.dbl("MU_REF", 0.0) > 0.0) throw Unsupported("MU_REF > 0 (viscous flux) is not supported yet");
+  // viscous terms (UgpWithCvCompFlow.h:155-186)
+  p.mu_ref = pm.dbl("MU_REF", 0.0);
+  if (p.mu_ref > 0.0) {
+    const std::string vm = pm.str("MU_MODE", "POWERLAW");
+    if (vm == "POWERLAW") p.visc_mode = 0;
+    else if (vm == "SUTHERLAND") p.visc_mode = 1;
+    else throw std::invalid_argument("MU_MODE = " + vm + " (POWERLAW, SUTHERLAND)");
+    p.mu_power_law = pm.dbl("MU_POWER_LAW", 0.76);
+    p.sl_tref = pm.dbl("SL_TREF", 273.15);
+    p.sl_sref = pm.dbl("SL_SREF", 110.4);
+    p.pr = pm.dbl("PR", 0.72);
+    p.pr_turb = pm.dbl("PRTURB", 0.9);
+    p.visc_limiter = pm.str("VISC_LIMITER", "YES") == "YES";
+    if (pm.str("SIMPLEX_VISCOUS", "NO") != "NO") throw Unsupported("SIMPLEX_VISCOUS = YES is not supported");
+  }
   // device cell numbering (this build's own option; results do not depend on it)
   {
     const std::string o = pm.str("GPU_CELL_ORDER", "MORTON");
@@ -1679,6 +2016,7 @@ struct Ctx::Impl {
   bool dup_blocks = false;
   // state
   DBuf<double> q, prim, grad, bfa, rhs, dt, hook;
+  DBuf<double> gradv;  // [ncv][12] velocity and enthalpy gradients of the viscous flux
   DBuf<int> counters;  // [0] negative pressure, [1] first-order fallbacks
   // solver (T = double or float, both allocated lazily)
   DBuf<double> A64, LU64, v64[10];
@@ -1712,6 +2050,7 @@ struct Ctx::Impl {
   void launch_grad(hipStream_t s);
   void launch_flux(hipStream_t s, const double *cfl_dev = nullptr);
   int flux_blocks() const;
+  bool flux_entry() const;
   void launch_grad_flux();
   template <typename T>
   void launch_jac(bool step_mode);
@@ -1981,6 +2320,8 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
   d.prim.upload(pr.data(), pr.size());
   d.grad.alloc(16 * n);
   HIPCHECK(hipMemset(d.grad.p, 0, sizeof(double) * 16 * n));
+  d.gradv.alloc(d.par.mu_ref > 0.0 ? 12 * n : 12);
+  HIPCHECK(hipMemset(d.gradv.p, 0, sizeof(double) * d.gradv.n));
   d.bfa.alloc(8 * (size_t)std::max(1, nfa_b));
   d.rhs.alloc(5 * n);
   d.dt.alloc(n);
@@ -2025,6 +2366,16 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
   d.dp.limiter = d.par.limiter;
   d.dp.symm_bc = d.par.symm_bc;
   d.dp.gravity_on = (d.par.gravity[0] != 0.0 || d.par.gravity[1] != 0.0 || d.par.gravity[2] != 0.0);
+  d.dp.visc = d.par.mu_ref > 0.0;
+  d.dp.visc_mode = d.par.visc_mode;
+  d.dp.visc_limiter = d.par.visc_limiter;
+  d.dp.mu_ref = d.par.mu_ref;
+  d.dp.mu_power_law = d.par.mu_power_law;
+  d.dp.sl_tref = d.par.sl_tref;
+  d.dp.sl_sref = d.par.sl_sref;
+  d.dp.T_ref = d.par.T_ref;
+  d.dp.pr = d.par.pr;
+  d.dp.pr_turb = d.par.pr_turb;
   d.cfl = d.par.cfl;
   HIPCHECK(hipDeviceSynchronize());
 }
@@ -2209,21 +2560,28 @@ void Ctx::Impl::launch_state_bc() {
 }
 
 void Ctx::Impl::launch_grad(hipStream_t s) {
-  if (!par.second_order) return;
-  const bool ent = par.grad_kernel == kResidEntry;
+  if (!par.second_order && !dp.visc) return;
+  // the per-entry gradient kernel has no viscous variant
+  const bool ent = par.grad_kernel == kResidEntry && !dp.visc;
   const dim3 g(ent ? (nwave + kWaves - 1) / kWaves : nblk(ncv));
   if (ent) {
     if (par.grad_ls) hipLaunchKernelGGL(k_grad_e<true>, g, dim3(kBlock), 0, s, dm, dp, prim.p, bfa.p, grad.p);
     else hipLaunchKernelGGL(k_grad_e<false>, g, dim3(kBlock), 0, s, dm, dp, prim.p, bfa.p, grad.p);
   } else {
-    if (par.grad_ls) hipLaunchKernelGGL(k_grad<true>, g, dim3(kBlock), 0, s, dm, dp, prim.p, bfa.p, grad.p);
-    else hipLaunchKernelGGL(k_grad<false>, g, dim3(kBlock), 0, s, dm, dp, prim.p, bfa.p, grad.p);
+    if (dp.visc) {
+      if (par.grad_ls) hipLaunchKernelGGL((k_grad<true, true>), g, dim3(kBlock), 0, s, dm, dp, prim.p, bfa.p, grad.p, gradv.p);
+      else hipLaunchKernelGGL((k_grad<false, true>), g, dim3(kBlock), 0, s, dm, dp, prim.p, bfa.p, grad.p, gradv.p);
+    } else {
+      if (par.grad_ls) hipLaunchKernelGGL((k_grad<true, false>), g, dim3(kBlock), 0, s, dm, dp, prim.p, bfa.p, grad.p, gradv.p);
+      else hipLaunchKernelGGL((k_grad<false, false>), g, dim3(kBlock), 0, s, dm, dp, prim.p, bfa.p, grad.p, gradv.p);
+    }
   }
 }
 
-int Ctx::Impl::flux_blocks() const {
-  return par.flux_kernel == kResidEntry ? (nwave + kWaves - 1) / kWaves : nblk(ncv);
-}
+// the per-cell flux kernel has no viscous variant
+bool Ctx::Impl::flux_entry() const { return par.flux_kernel == kResidEntry || dp.visc; }
+
+int Ctx::Impl::flux_blocks() const { return flux_entry() ? (nwave + kWaves - 1) / kWaves : nblk(ncv); }
 
 void Ctx::Impl::launch_flux(hipStream_t s, const double *cfl_dev) {
   DtArgs dta;
@@ -2236,15 +2594,42 @@ void Ctx::Impl::launch_flux(hipStream_t s, const double *cfl_dev) {
     hipLaunchKernelGGL(kern, g, dim3(kBlock), 0, s, dm, dp, prim.p, grad.p, bfa.p, q.p, rhs.p, counters.p + 1, dta,
                        dt.p, partials.p);
   };
-  const bool ent = par.flux_kernel == kResidEntry;
-  if (par.second_order) {
-    if (par.scheme == kRoe) go(ent ? k_flux_e<kRoe, true> : k_flux<kRoe, true>);
-    else if (par.scheme == kAusm) go(ent ? k_flux_e<kAusm, true> : k_flux<kAusm, true>);
-    else go(ent ? k_flux_e<kHllc, true> : k_flux<kHllc, true>);
+  auto goe = [&](auto kern) {
+    hipLaunchKernelGGL(kern, g, dim3(kBlock), 0, s, dm, dp, prim.p, grad.p, bfa.p, q.p, rhs.p, counters.p + 1, dta,
+                       dt.p, partials.p, gradv.p);
+  };
+  if (flux_entry()) {
+    if (dp.visc) {
+      if (par.second_order) {
+        if (par.scheme == kRoe) goe(k_flux_e<kRoe, true, true>);
+        else if (par.scheme == kAusm) goe(k_flux_e<kAusm, true, true>);
+        else goe(k_flux_e<kHllc, true, true>);
+      } else {
+        if (par.scheme == kRoe) goe(k_flux_e<kRoe, false, true>);
+        else if (par.scheme == kAusm) goe(k_flux_e<kAusm, false, true>);
+        else goe(k_flux_e<kHllc, false, true>);
+      }
+    } else {
+      if (par.second_order) {
+        if (par.scheme == kRoe) goe(k_flux_e<kRoe, true, false>);
+        else if (par.scheme == kAusm) goe(k_flux_e<kAusm, true, false>);
+        else goe(k_flux_e<kHllc, true, false>);
+      } else {
+        if (par.scheme == kRoe) goe(k_flux_e<kRoe, false, false>);
+        else if (par.scheme == kAusm) goe(k_flux_e<kAusm, false, false>);
+        else goe(k_flux_e<kHllc, false, false>);
+      }
+    }
   } else {
-    if (par.scheme == kRoe) go(ent ? k_flux_e<kRoe, false> : k_flux<kRoe, false>);
-    else if (par.scheme == kAusm) go(ent ? k_flux_e<kAusm, false> : k_flux<kAusm, false>);
-    else go(ent ? k_flux_e<kHllc, false> : k_flux<kHllc, false>);
+    if (par.second_order) {
+      if (par.scheme == kRoe) go(k_flux<kRoe, true>);
+      else if (par.scheme == kAusm) go(k_flux<kAusm, true>);
+      else go(k_flux<kHllc, true>);
+    } else {
+      if (par.scheme == kRoe) go(k_flux<kRoe, false>);
+      else if (par.scheme == kAusm) go(k_flux<kAusm, false>);
+      else go(k_flux<kHllc, false>);
+    }
   }
 }
 
@@ -2265,12 +2650,17 @@ void Ctx::Impl::launch_jac(bool step_mode) {
     const size_t nA = 25 * (size_t)nnz;
     hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)((nA + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream, nA, A, (T)0);
   }
-  if (par.scheme == kRoe)
-    hipLaunchKernelGGL((k_jac<T, kRoe>), dim3(nblk(ncv)), dim3(kBlock), 0, stream, dm, dp, prim.p, bfa.p, A,
-                       step_mode ? 1 : 0, dt.p, rhs.p, par.under_relax, v[0].p, v[1].p);
-  else
-    hipLaunchKernelGGL((k_jac<T, kHllc>), dim3(nblk(ncv)), dim3(kBlock), 0, stream, dm, dp, prim.p, bfa.p, A,
-                       step_mode ? 1 : 0, dt.p, rhs.p, par.under_relax, v[0].p, v[1].p);
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(nblk(ncv)), dim3(kBlock), 0, stream, dm, dp, prim.p, bfa.p, A, step_mode ? 1 : 0,
+                       dt.p, rhs.p, par.under_relax, v[0].p, v[1].p, gradv.p);
+  };
+  if (dp.visc) {
+    if (par.scheme == kRoe) go(k_jac<T, kRoe, true>);
+    else go(k_jac<T, kHllc, true>);
+  } else {
+    if (par.scheme == kRoe) go(k_jac<T, kRoe, false>);
+    else go(k_jac<T, kHllc, false>);
+  }
 }
 
 // Work vectors: v[0] rhs, v[1] phi (= dq), v[2] res, v[3] res0, v[4] p,
@@ -2643,7 +3033,13 @@ int64_t Ctx::get(const std::string &name, double *out) {
   if (name == "VEL") return cellrec(d.prim.p, 8, 1, 3);
   if (name == "GRAD_RHO") return cellrec(d.grad.p, 16, 0, 3);
   if (name == "GRAD_P") return cellrec(d.grad.p, 16, 3, 3);
-  if (name == "GRAD_U") return cellrec(d.grad.p, 16, 6, 9);
+  // with viscous terms the reference's grad_u is the one calcViscousFluxNS
+  // last formed (JoeWithModels.cpp:3616-3630)
+  if (name == "GRAD_U") return d.dp.visc ? cellrec(d.gradv.p, 12, 0, 9) : cellrec(d.grad.p, 16, 6, 9);
+  if (name == "GRAD_ENTHALPY") {
+    if (!d.dp.visc) throw std::invalid_argument("GRAD_ENTHALPY exists only with MU_REF > 0");
+    return cellrec(d.gradv.p, 12, 9, 3);
+  }
   if (name == "ALPHA_RHO") return cellrec(d.grad.p, 16, 15, 1);
   if (name == "RHO_BFA") return bfarec(0, 1);
   if (name == "VEL_BFA") return bfarec(1, 3);

@@ -94,7 +94,7 @@ def test_mesh_arrays_round_trip_counts():
 def test_unsupported_options_have_their_own_status():
     mm, joe = cases.box_case(2, seed=0)
     mesh = joemi.Mesh.from_faces(mm)
-    for edit in (lambda s: s.replace("MU_REF = 0.0", "MU_REF = 1.0e-5"),
+    for edit in (lambda s: s.replace("MU_REF = 0.0", "MU_REF = 1.0e-5\nSIMPLEX_VISCOUS = YES"),
                  lambda s: s.replace("LINEAR_SOLVER_NS = BCGSTAB", "LINEAR_SOLVER_NS = PETSC_GMRES")):
         with pytest.raises(joemi.JoemiError) as e:
             joemi.Solver(mesh, edit(joe))
