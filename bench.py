@@ -68,12 +68,12 @@ def cpu_baseline(n_box=64, reps=12):
 
 def serving_bench(n, seed, min_cells, max_cells, max_active, nsteps, resid_rel, budget_gib):
     """Steady solves/min and latency of the served mixed-precision solver
-    against a plain FP64 one-at-a-time loop, on airfoil requests (inviscid
-    NACA 4-digit O-meshes; thickness, camber, size, Mach and angle of attack
-    drawn as the serving workload specifies). Capacity: all requests present
-    at t = 0. Latency: Poisson arrivals at 0.8 x the served capacity, replayed
-    in real time for the server; for the one-at-a-time loop the same arrivals
-    are queued in front of its measured per-solve times."""
+    against a plain FP64 one-at-a-time loop, on steady bump-channel requests
+    of random size, bump thickness and Mach (cases.serving_requests), each
+    solved until its energy residual has dropped by resid_rel. Capacity: all
+    requests present at t = 0. Latency: Poisson arrivals at 0.8 x the served
+    capacity, replayed in real time for the server; for the one-at-a-time
+    loop the same arrivals are queued in front of its measured solve times."""
     import numpy as np
 
     from myjoe_amd import cases, joemi
@@ -81,8 +81,8 @@ def serving_bench(n, seed, min_cells, max_cells, max_active, nsteps, resid_rel, 
     reqs = cases.serving_requests(n, seed, min_cells, max_cells, rate_per_s=0.0)
     built = []
     for r in reqs:
-        mm, joe = cases.airfoil_case(r["ni"], r["nj"], r["thickness"], r["camber"], r["mach"], r["aoa_deg"],
-                                     nsteps=nsteps, resid_rel=resid_rel)
+        mm, joe = cases.bump_case(r["nx"], r["ny"], nsteps=nsteps, thickness=r["thickness"], mach=r["mach"],
+                                  resid_rel=resid_rel)
         built.append((joemi.Mesh.from_faces(mm), joe))
         del mm
     cells = sum(m.ncv for m, _ in built)
@@ -110,12 +110,14 @@ def serving_bench(n, seed, min_cells, max_cells, max_active, nsteps, resid_rel, 
         srv.wait(t)
     makespan = time.perf_counter() - t0
     cap_served = n / makespan * 60.0
-    steps_served, err = [], {"rho": 0.0, "rhou": 0.0, "rhoE": 0.0}
+    steps_served, err, drops = [], {"rho": 0.0, "rhou": 0.0, "rhoE": 0.0}, []
     for t, ref in zip(tickets, plain_states):
         st, rep = srv.result(t)
         steps_served.append(rep.steps)
+        drops.append(rep.resid[4] / rep.resid0[4] if rep.resid0[4] > 0 else float("nan"))
         for name, a, b in zip(("rho", "rhou", "rhoE"), st, ref):
-            err[name] = max(err[name], float(np.max(np.abs(a - b)) / max(float(np.max(np.abs(b))), 1e-300)))
+            d = float(np.max(np.abs(a - b))) / max(float(np.max(np.abs(b))), 1e-300)
+            err[name] = d if not (d <= err[name]) else err[name]  # a NaN is kept, not hidden
         srv.release(t)
     print(f"# serving: served mixed {cap_served:.1f} solves/min (makespan {makespan:.2f} s)", file=sys.stderr,
           flush=True)
@@ -144,8 +146,9 @@ def serving_bench(n, seed, min_cells, max_cells, max_active, nsteps, resid_rel, 
         lat_plain.append(done_prev - a)
     q = lambda v, p: float(np.percentile(np.array(v), p))
     return {
-        "workload": f"{n} inviscid NACA 4-digit airfoil solves, {min_cells}-{max_cells} cells (O-mesh, one cell "
-                    f"thick), Mach 0.3-0.8, AoA -2..6 deg; stop at energy L1 drop {resid_rel:g} or {nsteps} steps",
+        "workload": f"{n} steady inviscid bump-channel solves, {min_cells}-{max_cells} cells (nx = 3 ny, one cell "
+                    f"thick), bump 6-12 %, inlet Mach 0.3-0.5, HLLC 2nd order; stop at energy L1 drop {resid_rel:g} "
+                    f"or {nsteps} steps",
         "served": {"solves_per_min": round(cap_served, 2), "precision": "FP64 residual, FP32 Jacobian/Krylov",
                    "max_active": max_active, "p50_s": round(q(lat, 50), 3), "p99_s": round(q(lat, 99), 3),
                    "mean_steps": round(float(np.mean(steps_served)), 1)},
@@ -154,6 +157,7 @@ def serving_bench(n, seed, min_cells, max_cells, max_active, nsteps, resid_rel, 
         "speedup_solves_per_min": round(cap_served / cap_plain, 3),
         "arrival_rate_per_min": round(rate * 60.0, 2),
         "served_vs_fp64_max_rel_diff": {k: float(f"{v:.3g}") for k, v in err.items()},
+        "served_energy_resid_drop_max": float(f"{max(drops):.3g}"),
         "cells_total": int(cells),
     }
 
@@ -171,10 +175,10 @@ def main():
     ap.add_argument("--cell-order", default=None,
                     help="device cell numbering (GPU_CELL_ORDER): NATIVE, GROUPED, MORTON, MORTON_GROUPED")
     ap.add_argument("--serve", type=int, default=16, help="serving requests (0: skip the serving leg)")
-    ap.add_argument("--serve-cells", default="20000,80000", help="min,max cells of a served airfoil")
+    ap.add_argument("--serve-cells", default="3000,12000", help="min,max cells of a served request")
     ap.add_argument("--serve-active", type=int, default=8)
-    ap.add_argument("--serve-steps", type=int, default=500)
-    ap.add_argument("--serve-rel", type=float, default=1e-6)
+    ap.add_argument("--serve-steps", type=int, default=3000)
+    ap.add_argument("--serve-rel", type=float, default=1e-4)
     ap.add_argument("--copy-calib-gib", type=float, default=0.0,
                     help="first run a plain device copy of this many GiB (calibrates profiler byte counters)")
     args = ap.parse_args()

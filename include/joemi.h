@@ -127,10 +127,12 @@ typedef struct {
   double resid[5];     /* L1 residuals of rho, rhou(3), rhoE */
 } jm_step_info;
 jm_status jm_step(jm_ctx *c, jm_step_info *info);
-/* The same step driven by the device alone: CFL ramp and step counter on the
- * device, every BiCGStab iteration launched (iterations after convergence
- * return at once), captured into a hipGraph on first use and replayed.
- * Results are bit-identical to jm_step's; one launch per step. */
+/* The same step driven by the device: CFL ramp and step counter on the
+ * device, the step captured on first use into three hipGraphs (residual,
+ * Jacobian and the first BiCGStab iterations; further iterations in chunks;
+ * update and refresh) that are replayed, the host choosing chunk or tail
+ * from a flag the device writes to mapped memory. Results are bit-identical
+ * to jm_step's. */
 jm_status jm_step_graph(jm_ctx *c, jm_step_info *info);
 
 /* The same step split where the reference calls its hooks, for a host whose

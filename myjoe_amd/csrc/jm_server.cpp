@@ -133,6 +133,7 @@ void Server::admit_loop() {
     Active a;
     a.req = r;
     try {
+      std::lock_guard<std::mutex> hg(hip_mu_);
       if (hipStreamCreateWithFlags(&a.stream, hipStreamNonBlocking) != hipSuccess)
         throw std::runtime_error("HIP stream creation failed");
       a.ctx.reset(new Ctx(*r->mesh, r->joe_in, r->precision, a.stream));
@@ -177,9 +178,12 @@ void Server::retire(Active &a, bool failed, const std::string &err) {
   } else {
     r->error = err;
   }
-  a.ctx.reset();
-  if (a.stream) (void)hipStreamDestroy(a.stream);
-  a.stream = nullptr;
+  {
+    std::lock_guard<std::mutex> hg(hip_mu_);
+    a.ctx.reset();
+    if (a.stream) (void)hipStreamDestroy(a.stream);
+    a.stream = nullptr;
+  }
   std::lock_guard<std::mutex> g(mu_);
   r->t_done = now_s();
   r->status = failed ? kFailed : kDone;
@@ -205,7 +209,13 @@ void Server::step_loop() {
     for (auto &a : active_) {
       if (a.inflight) continue;
       try {
-        a.ctx->device_step_async();
+        if (!a.captured) {
+          std::lock_guard<std::mutex> hg(hip_mu_);
+          a.ctx->device_step_async();
+          a.captured = true;
+        } else {
+          a.ctx->device_step_async();
+        }
         a.inflight = true;
         progress = true;
       } catch (std::exception &e) {

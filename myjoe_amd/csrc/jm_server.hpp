@@ -57,6 +57,7 @@ class Server {
     hipStream_t stream = nullptr;
     bool inflight = false;
     bool dead = false;
+    bool captured = false;
   };
   void admit_loop();
   void step_loop();
@@ -66,6 +67,10 @@ class Server {
   int max_active_;
   int device_ = 0;
   std::mutex mu_;
+  // held while a context is built (synchronous uploads) and while a solve's
+  // first step is captured into a graph: a capture must not overlap another
+  // thread's synchronous HIP calls
+  std::mutex hip_mu_;
   std::condition_variable cv_;
   bool stop_ = false;
   int next_id_ = 1;

@@ -78,13 +78,14 @@ struct SolverScalars {
   int iter, done, maxiter, pad;
   double zero_abs, zero_rel;
   // pseudo-time step counter and CFL of the device-driven (graph) step
-  int step_no, pad2;
+  int step_no, stop;  // stop: the host-polled loop's launch budget is spent
   double cfl_cur;
+  int max_launch, pad3;
 };
 
 // what a device-driven step reports back, written to host-mapped memory
 struct StepMail {
-  int step, lin_iters, lin_status, pad;
+  int step, lin_iters, lin_status, solver_stop;
   double cfl, dt_min, resid[5];
 };
 
@@ -1290,7 +1291,7 @@ constexpr int kFinBlock = 1024;
 __global__ void __launch_bounds__(kFinBlock) k_fin(int nb, int op, const double *__restrict__ partials,
                                                 SolverScalars *S) {
   __shared__ double sh[16];
-  if ((op == kFinAlpha || op == kFinOmega || op == kFinResid) && S->done) return;
+  if ((op == kFinAlpha || op == kFinOmega || op == kFinResid) && (S->done || S->stop)) return;
   const bool t0 = threadIdx.x == 0;
   if (op == kFinDt) {
     double m = fin_slot(partials, nb, kMin, sh);
@@ -1329,6 +1330,9 @@ __global__ void __launch_bounds__(kFinBlock) k_fin(int nb, int op, const double 
         else if (S->iter > S->maxiter)
           S->done = 2;
       }
+      // the host-polled loop launches at most max_launch iterations; a
+      // device-driven step stops at the same point (iter counts them)
+      if (op == kFinResid && !S->done && S->iter + 1 >= S->max_launch) S->stop = 1;
       if (!S->done) {
         S->iter += 1;
         double rho_prime = -S->omega * S->rho;
@@ -1576,6 +1580,8 @@ __global__ void k_solver_init(SolverScalars *S, int maxiter, double zero_abs, do
   S->res0_max = S->res_max = 0.0;
   S->iter = 0;
   S->done = 0;
+  S->stop = 0;
+  S->max_launch = maxiter + 1;
   S->maxiter = maxiter;
   S->zero_abs = zero_abs;
   S->zero_rel = zero_rel;
@@ -1589,6 +1595,7 @@ __global__ void k_fill(size_t n, T *__restrict__ p, T v) {
 
 __global__ void k_publish(const SolverScalars *S, StepMail *out) {
   if (threadIdx.x != 0) return;
+  out->solver_stop = S->done || S->stop;
   out->step = S->step_no;
   out->lin_iters = S->iter;
   out->lin_status = S->done;
@@ -1643,7 +1650,7 @@ __global__ void __launch_bounds__(kBlock) k_resid(int ncv, const int *__restrict
                                                   int init, double *partials, const SolverScalars *S) {
   __shared__ double sh[32];
   __shared__ T cellv[kCellsPerBlock][5];
-  if (!init && S->done) return;
+  if (!init && (S->done || S->stop)) return;
   const int lc = threadIdx.x / 5, i = threadIdx.x % 5;
   const int c = blockIdx.x * kCellsPerBlock + lc;
   const bool active = (lc < kCellsPerBlock) && (c < ncv);
@@ -1678,7 +1685,7 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock) k_p(int ncv, const T *__restrict__ LU, const T *__restrict__ res,
                                               const T *__restrict__ v, T *__restrict__ p, T *__restrict__ phat,
                                               const SolverScalars *S) {
-  if (S->done) return;
+  if (S->done || S->stop) return;
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncv) return;
   const T beta = (T)S->beta, omega = (T)S->omega;
@@ -1702,7 +1709,7 @@ __global__ void __launch_bounds__(kBlock) k_spmv_dot(int ncv, const int *__restr
                                                      const T *__restrict__ other, double *partials,
                                                      const SolverScalars *S) {
   __shared__ double sh[32];
-  if (S->done) return;
+  if (S->done || S->stop) return;
   const int lc = threadIdx.x / 5, i = threadIdx.x % 5;
   const int c = blockIdx.x * kCellsPerBlock + lc;
   double v[2] = {0.0, 0.0};
@@ -1721,7 +1728,7 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock) k_s(int ncv, const T *__restrict__ LU, const T *__restrict__ res,
                                               const T *__restrict__ v, T *__restrict__ s, T *__restrict__ shat,
                                               const SolverScalars *S) {
-  if (S->done) return;
+  if (S->done || S->stop) return;
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncv) return;
   const T alpha = (T)S->alpha;
@@ -1740,7 +1747,7 @@ __global__ void __launch_bounds__(kBlock) k_s(int ncv, const T *__restrict__ LU,
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_x(int n5, T *__restrict__ phi, const T *__restrict__ phat,
                                               const T *__restrict__ shat, const SolverScalars *S) {
-  if (S->done) return;
+  if (S->done || S->stop) return;
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n5) return;
   phi[k] += (T)S->alpha * phat[k] + (T)S->omega * shat[k];
@@ -2030,15 +2037,30 @@ struct Ctx::Impl {
   bool begun = false;
 
   template <typename T>
-  void solve(int &iters, int &status, bool graph = false);
-  // device-driven step: CFL ramp, residual, Jacobian, every BiCGStab
-  // iteration, update, refresh, L1 and a report to host-mapped memory; no
-  // host synchronisation, so it can be captured in a hipGraph
+  void solve(int &iters, int &status);
   template <typename T>
-  void enqueue_device_step();
+  void solve_prologue();
+  template <typename T>
+  void solve_iterations(int count);
+  template <typename T>
+  DBuf<T> *krylov();
+  template <typename T>
+  DBuf<T> *lu_();
+  // device-driven step in three captured graphs: head (CFL ramp, residual,
+  // Jacobian, Krylov start, kDevChunk iterations), chunk (kDevChunk more),
+  // tail (update, refresh, L1); each ends with a report to host-mapped
+  // memory and the host picks chunk or tail from the solver flag
+  static constexpr int kDevChunk = 4;
+  template <typename T>
+  void enqueue_step_head();
+  template <typename T>
+  void enqueue_step_chunk();
+  template <typename T>
+  void enqueue_step_tail();
+  hipGraphExec_t capture(void (Ctx::Impl::*enq)());
   hipStream_t cap_stream = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
+  hipGraphExec_t g_head = nullptr, g_chunk = nullptr, g_tail = nullptr;
+  int dev_phase = 0;  // 1: head or chunk in flight, 2: tail in flight
   hipEvent_t ev_step = nullptr;
   StepMail *mail_h = nullptr, *mail_d = nullptr;
   bool dev_step_dirty = true;
@@ -2059,6 +2081,22 @@ struct Ctx::Impl {
 template <>
 double *Ctx::Impl::A_<double>() {
   return A64.p;
+}
+template <>
+DBuf<double> *Ctx::Impl::krylov<double>() {
+  return v64;
+}
+template <>
+DBuf<float> *Ctx::Impl::krylov<float>() {
+  return v32;
+}
+template <>
+DBuf<double> *Ctx::Impl::lu_<double>() {
+  return &LU64;
+}
+template <>
+DBuf<float> *Ctx::Impl::lu_<float>() {
+  return &LU32;
 }
 template <>
 float *Ctx::Impl::A_<float>() {
@@ -2385,8 +2423,8 @@ Ctx::~Ctx() {
   Impl &d = *d_;
   if (d.graph_inflight && d.ev_step) (void)hipEventSynchronize(d.ev_step);
   (void)hipStreamSynchronize(d.stream);
-  if (d.gexec) (void)hipGraphExecDestroy(d.gexec);
-  if (d.graph) (void)hipGraphDestroy(d.graph);
+  for (hipGraphExec_t g : {d.g_head, d.g_chunk, d.g_tail})
+    if (g) (void)hipGraphExecDestroy(g);
   if (d.cap_stream) (void)hipStreamDestroy(d.cap_stream);
   if (d.ev_step) (void)hipEventDestroy(d.ev_step);
   if (d.mail_h) (void)hipHostFree(d.mail_h);
@@ -2438,47 +2476,50 @@ void Ctx::device_step_async() {
     hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, d.stream, d.scal.p, d.step_no, d.cfl);
     d.dev_step_dirty = false;
   }
-  if (!d.gexec) {
-    if (!d.cap_stream) HIPCHECK(hipStreamCreateWithFlags(&d.cap_stream, hipStreamNonBlocking));
-    hipStream_t keep = d.stream;
-    d.stream = d.cap_stream;
-    HIPCHECK(hipStreamBeginCapture(d.cap_stream, hipStreamCaptureModeThreadLocal));
-    try {
-      if (d.precision == 0)
-        d.enqueue_device_step<double>();
-      else
-        d.enqueue_device_step<float>();
-    } catch (...) {
-      hipGraph_t g = nullptr;
-      (void)hipStreamEndCapture(d.cap_stream, &g);
-      if (g) (void)hipGraphDestroy(g);
-      d.stream = keep;
-      throw;
+  if (!d.g_head) {
+    if (d.precision == 0) {
+      d.g_head = d.capture(&Ctx::Impl::enqueue_step_head<double>);
+      d.g_chunk = d.capture(&Ctx::Impl::enqueue_step_chunk<double>);
+      d.g_tail = d.capture(&Ctx::Impl::enqueue_step_tail<double>);
+    } else {
+      d.g_head = d.capture(&Ctx::Impl::enqueue_step_head<float>);
+      d.g_chunk = d.capture(&Ctx::Impl::enqueue_step_chunk<float>);
+      d.g_tail = d.capture(&Ctx::Impl::enqueue_step_tail<float>);
     }
-    d.stream = keep;
-    HIPCHECK(hipStreamEndCapture(d.cap_stream, &d.graph));
-    HIPCHECK(hipGraphInstantiate(&d.gexec, d.graph, nullptr, nullptr, 0));
   }
-  HIPCHECK(hipGraphLaunch(d.gexec, d.stream));
+  HIPCHECK(hipGraphLaunch(d.g_head, d.stream));
   HIPCHECK(hipEventRecord(d.ev_step, d.stream));
+  d.dev_phase = 1;
   d.graph_inflight = true;
 }
 
+// Advances an in-flight device step: after the head or a chunk, launches
+// another chunk or the tail depending on the solver flag in the report;
+// true once the tail has finished.
 bool Ctx::device_step_ready() {
   Impl &d = *d_;
   if (!d.graph_inflight) return true;
   hipError_t e = hipEventQuery(d.ev_step);
-  if (e == hipSuccess) return true;
   if (e == hipErrorNotReady) return false;
   HIPCHECK(e);
+  if (d.dev_phase == 2) return true;
+  const volatile StepMail *m = d.mail_h;
+  if (m->solver_stop) {
+    HIPCHECK(hipGraphLaunch(d.g_tail, d.stream));
+    d.dev_phase = 2;
+  } else {
+    HIPCHECK(hipGraphLaunch(d.g_chunk, d.stream));
+  }
+  HIPCHECK(hipEventRecord(d.ev_step, d.stream));
   return false;
 }
 
 StepInfo Ctx::device_step_result() {
   Impl &d = *d_;
   if (!d.graph_inflight) throw std::logic_error("no device step in flight");
-  HIPCHECK(hipEventSynchronize(d.ev_step));
+  while (!device_step_ready()) HIPCHECK(hipEventSynchronize(d.ev_step));
   d.graph_inflight = false;
+  d.dev_phase = 0;
   const volatile StepMail *m = d.mail_h;
   StepInfo info;
   info.step = m->step;
@@ -2666,51 +2707,63 @@ void Ctx::Impl::launch_jac(bool step_mode) {
 // Work vectors: v[0] rhs, v[1] phi (= dq), v[2] res, v[3] res0, v[4] p,
 // v[5] v, v[6] phat, v[7] s, v[8] shat, v[9] t.
 template <typename T>
-void Ctx::Impl::solve(int &iters, int &status, bool graph) {
-  DBuf<T> *v;
-  DBuf<T> *LU;
-  if constexpr (std::is_same<T, double>::value) {
-    v = v64;
-    LU = &LU64;
-  } else {
-    v = v32;
-    LU = &LU32;
-  }
+void Ctx::Impl::solve_prologue() {
+  DBuf<T> *v = krylov<T>();
+  DBuf<T> *LU = lu_<T>();
   T *A = A_<T>();
   const int n5 = 5 * ncv;
-  T *rhs_ = v[0].p, *phi = v[1].p, *res = v[2].p, *res0 = v[3].p, *p = v[4].p, *vv = v[5].p, *phat = v[6].p,
-    *s = v[7].p, *shat = v[8].p, *t = v[9].p;
+  T *rhs_ = v[0].p, *phi = v[1].p, *res = v[2].p, *res0 = v[3].p, *p = v[4].p, *vv = v[5].p;
   hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n5)), dim3(kBlock), 0, stream, (size_t)n5, p, (T)0);
   hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n5)), dim3(kBlock), 0, stream, (size_t)n5, vv, (T)0);
   hipLaunchKernelGGL(k_solver_init, dim3(1), dim3(64), 0, stream, scal.p, par.ls_maxiter, par.ls_abs, par.ls_rel);
-  SolverScalars &hs = *h_scal;
   hipLaunchKernelGGL(k_lu<T>, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, nbocv_i.p, A, LU->p);
   const int gb5 = (ncv + kCellsPerBlock - 1) / kCellsPerBlock;
   hipLaunchKernelGGL(k_resid<T>, dim3(gb5), dim3(kBlock), 0, stream, ncv, nbocv_i.p, nbocv_v.p, A, LU->p, phi, rhs_,
                      res, res0, 1, partials.p, scal.p);
   hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, gb5, (int)kFinResid0, partials.p, scal.p);
+}
+
+// n BiCGStab iterations (van der Vorst, UgpWithCv.cpp:2725-2856); each
+// returns at once once the device has set the convergence or stop flag
+template <typename T>
+void Ctx::Impl::solve_iterations(int count) {
+  DBuf<T> *v = krylov<T>();
+  DBuf<T> *LU = lu_<T>();
+  T *A = A_<T>();
+  const int n5 = 5 * ncv;
+  T *rhs_ = v[0].p, *phi = v[1].p, *res = v[2].p, *res0 = v[3].p, *p = v[4].p, *vv = v[5].p, *phat = v[6].p,
+    *s = v[7].p, *shat = v[8].p, *t = v[9].p;
+  const int gb5 = (ncv + kCellsPerBlock - 1) / kCellsPerBlock;
+  for (int k = 0; k < count; k++) {
+    hipLaunchKernelGGL(k_p<T>, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, LU->p, res, vv, p, phat, scal.p);
+    hipLaunchKernelGGL(k_spmv_dot<T>, dim3(gb5), dim3(kBlock), 0, stream, ncv, nbocv_i.p, nbocv_v.p, A, phat, vv,
+                       res0, partials.p, scal.p);
+    hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, gb5, (int)kFinAlpha, partials.p, scal.p);
+    hipLaunchKernelGGL(k_s<T>, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, LU->p, res, vv, s, shat, scal.p);
+    hipLaunchKernelGGL(k_spmv_dot<T>, dim3(gb5), dim3(kBlock), 0, stream, ncv, nbocv_i.p, nbocv_v.p, A, shat, t, s,
+                       partials.p, scal.p);
+    hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, gb5, (int)kFinOmega, partials.p, scal.p);
+    hipLaunchKernelGGL(k_x<T>, dim3(nblk(n5)), dim3(kBlock), 0, stream, n5, phi, phat, shat, scal.p);
+    hipLaunchKernelGGL(k_resid<T>, dim3(gb5), dim3(kBlock), 0, stream, ncv, nbocv_i.p, nbocv_v.p, A, LU->p, phi,
+                       rhs_, res, res0, 0, partials.p, scal.p);
+    hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, gb5, (int)kFinResid, partials.p, scal.p);
+  }
+}
+
+// Work vectors: v[0] rhs, v[1] phi (= dq), v[2] res, v[3] res0, v[4] p,
+// v[5] v, v[6] phat, v[7] s, v[8] shat, v[9] t. The host polls the flag
+// between chunks (2, then 4 iterations); iterations past convergence are
+// no-ops, so the chunking does not change the result.
+template <typename T>
+void Ctx::Impl::solve(int &iters, int &status) {
+  solve_prologue<T>();
   const int max_it = par.ls_maxiter + 1;
   int launched = 0;
+  SolverScalars &hs = *h_scal;
   while (launched < max_it) {
-    // a captured step launches every iteration (kernels after convergence
-    // return at once); the eager path polls the flag between chunks
-    int chunk = graph ? max_it : std::min(launched == 0 ? 2 : 4, max_it - launched);
-    for (int k = 0; k < chunk; k++) {
-      hipLaunchKernelGGL(k_p<T>, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, LU->p, res, vv, p, phat, scal.p);
-      hipLaunchKernelGGL(k_spmv_dot<T>, dim3(gb5), dim3(kBlock), 0, stream, ncv, nbocv_i.p, nbocv_v.p, A, phat, vv,
-                         res0, partials.p, scal.p);
-      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, gb5, (int)kFinAlpha, partials.p, scal.p);
-      hipLaunchKernelGGL(k_s<T>, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, LU->p, res, vv, s, shat, scal.p);
-      hipLaunchKernelGGL(k_spmv_dot<T>, dim3(gb5), dim3(kBlock), 0, stream, ncv, nbocv_i.p, nbocv_v.p, A, shat, t, s,
-                         partials.p, scal.p);
-      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, gb5, (int)kFinOmega, partials.p, scal.p);
-      hipLaunchKernelGGL(k_x<T>, dim3(nblk(n5)), dim3(kBlock), 0, stream, n5, phi, phat, shat, scal.p);
-      hipLaunchKernelGGL(k_resid<T>, dim3(gb5), dim3(kBlock), 0, stream, ncv, nbocv_i.p, nbocv_v.p, A, LU->p, phi,
-                         rhs_, res, res0, 0, partials.p, scal.p);
-      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, gb5, (int)kFinResid, partials.p, scal.p);
-    }
+    int chunk = std::min(launched == 0 ? 2 : 4, max_it - launched);
+    solve_iterations<T>(chunk);
     launched += chunk;
-    if (graph) return;
     HIPCHECK(hipMemcpyAsync(h_scal, scal.p, sizeof(SolverScalars), hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipStreamSynchronize(stream));
     if (hs.done) break;
@@ -2720,7 +2773,7 @@ void Ctx::Impl::solve(int &iters, int &status, bool graph) {
 }
 
 template <typename T>
-void Ctx::Impl::enqueue_device_step() {
+void Ctx::Impl::enqueue_step_head() {
   hipLaunchKernelGGL(k_ramp, dim3(1), dim3(64), 0, stream, scal.p, par.cfl_after, par.cfl_interval, par.cfl_factor,
                      par.cfl_max);
   launch_grad(stream);
@@ -2728,10 +2781,20 @@ void Ctx::Impl::enqueue_device_step() {
   hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, flux_blocks(), (int)kFinDt, partials.p, scal.p);
   if (par.dt_mode == 1) hipLaunchKernelGGL(k_dt_fill, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, dt.p, scal.p);
   launch_jac<T>(true);
-  int it = 0, st = 0;
-  solve<T>(it, st, true);
-  DBuf<T> *v;
-  if constexpr (std::is_same<T, double>::value) v = v64; else v = v32;
+  solve_prologue<T>();
+  solve_iterations<T>(kDevChunk);
+  hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, stream, scal.p, mail_d);
+}
+
+template <typename T>
+void Ctx::Impl::enqueue_step_chunk() {
+  solve_iterations<T>(kDevChunk);
+  hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, stream, scal.p, mail_d);
+}
+
+template <typename T>
+void Ctx::Impl::enqueue_step_tail() {
+  DBuf<T> *v = krylov<T>();
   hipLaunchKernelGGL(k_update<T>, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, q.p, v[1].p);
   launch_state_bc();
   hipLaunchKernelGGL(k_l1, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, rhs.p, partials.p);
@@ -2739,6 +2802,28 @@ void Ctx::Impl::enqueue_device_step() {
   hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, stream, scal.p, mail_d);
 }
 
+hipGraphExec_t Ctx::Impl::capture(void (Ctx::Impl::*enq)()) {
+  if (!cap_stream) HIPCHECK(hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking));
+  hipStream_t keep = stream;
+  stream = cap_stream;
+  HIPCHECK(hipStreamBeginCapture(cap_stream, hipStreamCaptureModeThreadLocal));
+  hipGraph_t g = nullptr;
+  try {
+    (this->*enq)();
+  } catch (...) {
+    (void)hipStreamEndCapture(cap_stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    stream = keep;
+    throw;
+  }
+  stream = keep;
+  HIPCHECK(hipStreamEndCapture(cap_stream, &g));
+  hipGraphExec_t e = nullptr;
+  hipError_t st = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  HIPCHECK(st);
+  return e;
+}
 void Ctx::launch_residual() {
   Impl &d = *d_;
   d.launch_state_bc();

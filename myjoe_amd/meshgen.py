@@ -132,13 +132,18 @@ def bump_channel(nx: int = 96, ny: int = 32, thickness: float = 0.1, dz: float |
     return _assemble(x_no, faces, nx * ny, zones)
 
 
-def naca_surface(psi: np.ndarray, thickness: float, camber: float, camber_pos: float = 0.4):
-    """Points of a NACA 4-digit section (closed trailing edge) at the angle
-    parameter psi in [0, 2 pi): x = (1 + cos psi) / 2, the lower surface for
-    psi in [0, pi], the upper one after it (clockwise seen from +z)."""
+def naca_surface(psi: np.ndarray, thickness: float, camber: float, camber_pos: float = 0.4,
+                 closed_te: bool = False):
+    """Points of a NACA 4-digit section at the angle parameter psi in
+    [0, 2 pi): x = (1 + cos psi) / 2, the lower surface for psi in [0, pi],
+    the upper one after it (clockwise seen from +z). The standard open
+    (blunt) trailing edge by default: its base is the O-mesh edge between the
+    last upper and the first lower point, which avoids the cusp cells of a
+    closed one."""
     xc = 0.5 * (1.0 + np.cos(psi))
     t, m, p = thickness, camber, camber_pos
-    yt = 5.0 * t * (0.2969 * np.sqrt(xc) - 0.1260 * xc - 0.3516 * xc ** 2 + 0.2843 * xc ** 3 - 0.1036 * xc ** 4)
+    c4 = -0.1036 if closed_te else -0.1015
+    yt = 5.0 * t * (0.2969 * np.sqrt(xc) - 0.1260 * xc - 0.3516 * xc ** 2 + 0.2843 * xc ** 3 + c4 * xc ** 4)
     if m > 0.0:
         yc = np.where(xc < p, m / p ** 2 * (2 * p * xc - xc ** 2), m / (1 - p) ** 2 * ((1 - 2 * p) + 2 * p * xc - xc ** 2))
         dy = np.where(xc < p, 2 * m / p ** 2 * (p - xc), 2 * m / (1 - p) ** 2 * (p - xc))

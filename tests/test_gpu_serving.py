@@ -8,14 +8,14 @@ from myjoe_amd import cases, joemi
 
 pytestmark = pytest.mark.gpu
 
-REQS = [dict(ni=48, nj=24, thickness=0.12, camber=0.02, mach=0.5, aoa_deg=2.0),
-        dict(ni=64, nj=32, thickness=0.16, camber=0.0, mach=0.7, aoa_deg=-1.0),
-        dict(ni=40, nj=20, thickness=0.09, camber=0.04, mach=0.35, aoa_deg=5.0)]
+REQS = [dict(nx=48, ny=16, thickness=0.10, mach=0.45),
+        dict(nx=96, ny=32, thickness=0.07, mach=0.35),
+        dict(nx=60, ny=20, thickness=0.12, mach=0.5)]
 
 
 def _case(r, nsteps=40):
-    return cases.airfoil_case(r["ni"], r["nj"], r["thickness"], r["camber"], r["mach"], r["aoa_deg"],
-                              nsteps=nsteps, resid_rel=1e-12)
+    return cases.bump_case(r["nx"], r["ny"], nsteps=nsteps, thickness=r["thickness"], mach=r["mach"],
+                           resid_rel=1e-12)
 
 
 def _alone(mesh, joe, precision):
@@ -72,6 +72,6 @@ def test_bad_request_is_rejected_at_submission():
     mesh = joemi.Mesh.from_faces(mm)
     srv = joemi.Server(mem_budget_bytes=1 << 30, max_active=2)
     with pytest.raises(joemi.JoemiError) as e:
-        srv.submit(mesh, joe.replace("airfoil = WALL", "airfoil = NOSUCHBC"))
+        srv.submit(mesh, joe.replace("bump_wall = WALL", "bump_wall = NOSUCHBC"))
     assert e.value.status == 3
     srv.close()
