@@ -44,6 +44,25 @@ def algorithmic_bytes(ncv, nfa, nfa_b, nnz):
     }
 
 
+def measured_traffic(mesh, kernel):
+    """HBM bytes per launch of a kernel on this mesh from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/), when it was taken
+    on the same mesh; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "residual_160box_hllc_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+    except OSError:
+        return None, None
+    m = d.get("mesh", {})
+    if (m.get("ncv"), m.get("nfa"), m.get("nfa_b"), m.get("nnz")) != (mesh.ncv, mesh.nfa, mesh.nfa_b, mesh.nnz):
+        return None, None
+    k = d.get("kernels", {}).get(kernel)
+    if not k:
+        return None, None
+    return k["fetch_bytes"] + k["write_bytes"], os.path.relpath(path, ROOT)
+
+
 def cpu_baseline(n_box=64, reps=12):
     """Single-threaded CPU restatement of the reference (bit-identical to it)
     on a smaller box of the same construction: Mcell-it/s of state + BC + dt +
@@ -250,6 +269,11 @@ def main():
     dom = max(("k_grad", "k_flux", "k_state"), key=lambda k: kt.get(k, 0.0))
     achieved = ab[dom] / (kt[dom] * 1e-3) / 1e9
     resid_bytes = ab["k_state"] + ab["k_bc"] + ab["k_grad"] + ab["k_flux"]
+    traffic, traffic_src = measured_traffic(mesh, dom) if head == "HLLC" else (None, None)
+    # the residual's compulsory bytes as the scope table counts them (state,
+    # gradient, limiter, flux passes): 780 ncv + 32 nnz + 4 nfoc + 80 nfa_i + 160 nfa_b
+    nfa_i = mesh.nfa - mesh.nfa_b
+    survey_bytes = 780 * mesh.ncv + 32 * mesh.nnz + 4 * (2 * nfa_i + mesh.nfa_b) + 80 * nfa_i + 160 * mesh.nfa_b
     out = {
         "metric": METRIC,
         "value": round(results[head]["mcell_it_s"], 2),
@@ -270,10 +294,12 @@ def main():
             "per_scheme_mcell_it_s": {k: round(v["mcell_it_s"], 2) for k, v in results.items()},
             "kernel_ms": {k: round(v, 4) for k, v in kt.items()},
             "residual_algorithmic_GBs": round(resid_bytes / (kt["residual"] * 1e-3) / 1e9, 1),
+            "residual_frac_of_hbm_peak_scope_table_bytes": round(
+                survey_bytes / (results[head]["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         },
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "algorithmic_bytes_per_launch": int(ab[dom])},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": traffic_src, "algorithmic_bytes_per_launch": int(ab[dom])},
     }
     del mesh
     gc.collect()
