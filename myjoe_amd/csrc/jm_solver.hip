@@ -57,6 +57,7 @@ struct DMesh {
   const ZoneDev *zones;
   const int *sched;       // [ncv] cell visiting order of the per-cell kernels
   const int *ecell;       // [entries] cell of each entry
+  const double *cfl_r0;   // [entries][3] x_fa - x_cv
   const int *wave_c0;     // [nwave+1] first cell of each 64-lane wavefront
   int nwave;
 };
@@ -293,6 +294,7 @@ __global__ void __launch_bounds__(kBlock) k_grad(DMesh m, DParams P, const doubl
   const double h = VISC ? enthalpy_of(P, rho, p) : 0.0;
   double hmax = h, hmin = h;
   const int e0 = m.cfl_i[c], e1 = m.cfl_i[c + 1];
+  int e_int = e0;  // end of the internal entries (they come first)
   if (LS) {
     const int noc_f = m.nbocv_i[c];
     const double *l0 = m.lsg + 3 * (size_t)noc_f;
@@ -329,6 +331,7 @@ __global__ void __launch_bounds__(kBlock) k_grad(DMesh m, DParams P, const doubl
       rmax = Dev::mx(rmax, rn); rmin = Dev::mn(rmin, rn);
       pmax = Dev::mx(pmax, pnv); pmin = Dev::mn(pmin, pnv);
       k++;
+      e_int = e + 1;
     }
     for (int e = e0; e < e1; e++) {
       int2 en = m.cfl[e];
@@ -409,6 +412,7 @@ __global__ void __launch_bounds__(kBlock) k_grad(DMesh m, DParams P, const doubl
         }
         rmax = Dev::mx(rmax, rn); rmin = Dev::mn(rmin, rn);
         pmax = Dev::mx(pmax, pnv); pmin = Dev::mn(pmin, pnv);
+        e_int = e + 1;
       } else {
         const double *bb = bfa + 8 * (size_t)f;
         double rb = bb[0], ub[3] = {bb[1], bb[2], bb[3]}, pb = bb[4];
@@ -442,12 +446,9 @@ __global__ void __launch_bounds__(kBlock) k_grad(DMesh m, DParams P, const doubl
     e2p = e2p * e2p;
     bool lu[3] = {(umax[0] - umin[0]) > 1.0e-8, (umax[1] - umin[1]) > 1.0e-8, (umax[2] - umin[2]) > 1.0e-8};
     bool lr = (rmax - rmin) > 1.0e-12, lp = (pmax - pmin) > 1.0e-12;
-    for (int e = e0; e < e1; e++) {
-      int2 en = m.cfl[e];
-      if (en.y < 0) continue;
-      int f = en.x & 0x7fffffff;
-      const double *geo = m.fgeo + 8 * (size_t)f;
-      double r0[3] = {geo[4] - xcv[0], geo[5] - xcv[1], geo[6] - xcv[2]};
+    for (int e = e0; e < e_int; e++) {
+      const double *rr = m.cfl_r0 + 3 * (size_t)e;
+      double r0[3] = {rr[0], rr[1], rr[2]};
       for (int i = 0; i < 3; i++)
         if (lu[i]) au[i] = bj_face(au[i], u[i], u[i] + dot3(r0, gu[i]), umax[i], umin[i], e2u);
       if (lr) ar = bj_face(ar, rho, rho + dot3(r0, gr), rmax, rmin, e2r);
@@ -461,11 +462,9 @@ __global__ void __launch_bounds__(kBlock) k_grad(DMesh m, DParams P, const doubl
       double e2h = Dev::mx(P.eps * fabs(h), 1.0e-8);
       e2h = e2h * e2h;
       if ((hmax - hmin) > 1.0e-12) {
-        for (int e = e0; e < e1; e++) {
-          int2 en = m.cfl[e];
-          if (en.y < 0) continue;
-          const double *geo = m.fgeo + 8 * (size_t)(en.x & 0x7fffffff);
-          double r0[3] = {geo[4] - xcv[0], geo[5] - xcv[1], geo[6] - xcv[2]};
+        for (int e = e0; e < e_int; e++) {
+          const double *rr = m.cfl_r0 + 3 * (size_t)e;
+          double r0[3] = {rr[0], rr[1], rr[2]};
           ah = bj_face(ah, h, h + dot3(r0, gh), hmax, hmin, e2h);
         }
       } else {
@@ -2016,6 +2015,7 @@ struct Ctx::Impl {
   DBuf<int2> cfl_e;
   DBuf<int> sched;
   DBuf<int> ecell, wave_c0;
+  DBuf<double> cfl_r0;
   int nwave = 0;
   DBuf<ZoneDev> zones;
   std::vector<int> perm;              // device cell -> reference cell
@@ -2311,6 +2311,20 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
     dci[ncv] = (int)dcf.size();
     d.cfl_i.upload(dci.data(), dci.size());
     d.cfl_e.upload(dcf.data(), dcf.size());
+    // face-centroid offset x_fa - x_cv of every entry (the limiter's r0), so
+    // the per-cell limiter loop reads it without gathering face rows
+    {
+      std::vector<double> r0(3 * std::max<size_t>(1, dcf.size()), 0.0);
+      size_t k = 0;
+      for (int cd = 0; cd < ncv; cd++) {
+        const int c = perm[cd];
+        for (int e = ci[c]; e < ci[c + 1]; e++, k++) {
+          const int f = cf[e].x & 0x7fffffff;
+          for (int i = 0; i < 3; i++) r0[3 * k + i] = m.x_fa[3 * (size_t)f + i] - m.x_cv[3 * (size_t)c + i];
+        }
+      }
+      d.cfl_r0.upload(r0.data(), r0.size());
+    }
     // entry-parallel kernels: the cell of every entry, and wavefronts of 64
     // lanes that each cover whole cells (consecutive cells while their
     // entries fit)
@@ -2391,6 +2405,7 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
   d.dm.zones = d.zones.p;
   d.dm.sched = d.sched.p;
   d.dm.ecell = d.ecell.p;
+  d.dm.cfl_r0 = d.cfl_r0.p;
   d.dm.wave_c0 = d.wave_c0.p;
   d.dm.nwave = d.nwave;
 
@@ -2441,7 +2456,7 @@ size_t Ctx::footprint(const Mesh &m, int precision) {
   b += nfa * (8 + 3) * 8;                          // fgeo, fnorm
   b += nnz * (3 * 8 + 4) + nfa_b * (3 * 8 + 4 + 4);  // lsg, nbocv_v, bfa_lsg, bfa_cell, bfa_zone
   b += (ncv + 1) * 4 * 2 + ncv * 4 * 2;            // cfl_i, nbocv_i, sched, wave table
-  b += ent * (8 + 4 + 4);                          // entries, their blocks, their cells
+  b += ent * (8 + 4 + 4 + 24);                     // entries, their blocks, their cells, limiter offsets
   b += ncv * (5 + 8 + 16 + 5 + 1 + 1) * 8;         // q, prim, grad, rhs, dt, vol
   b += nfa_b * (8 + 5) * 8;                        // bfa, hook
   b += (nnz + ncv) * 25 * sJ + 10 * ncv * 5 * sJ;  // A, LU, Krylov vectors
