@@ -11,6 +11,7 @@
   (the serving workload's requests), free stream p = rho = 1.
 * ``random_state``: rho ~ U(0.5, 1.5), p ~ lognormal(0, 0.2),
   u_i ~ N(0, 0.3) * c, as conservative variables.
+* ``with_sa``: adds the Spalart-Allmaras scalar nuSA to a viscous case.
 """
 from __future__ import annotations
 
@@ -157,3 +158,17 @@ def serving_requests(n: int, seed: int = 0, min_cells: int = 3_000, max_cells: i
         out.append(dict(arrival=float(t[k]), nx=3 * ny, ny=ny, thickness=float(rng.uniform(0.06, 0.12)),
                         mach=float(rng.uniform(0.3, 0.5))))
     return out
+
+
+def with_sa(joe_in: str, nu_init: float, zone_values: dict) -> str:
+    """The case with the Spalart-Allmaras model: ``TURB_MODEL = SA`` (the
+    reference picks its RansTurbSA class by run number instead), the initial
+    nuSA (INITIAL_CONDITION_TURB) and ``<zone>.nuSA`` Dirichlet values; zones
+    without a value get zero-gradient nuSA. The case must be viscous."""
+    joe_in = joe_in.replace("INITIAL_CONDITION_TURB = 0.0", f"INITIAL_CONDITION_TURB = {nu_init!r}")
+    return joe_in + "TURB_MODEL = SA\n" + "".join(f"{z}.nuSA = {v!r}\n" for z, v in zone_values.items())
+
+
+# nuSA boundary values of the SA variants of the two test geometries
+SA_BUMP = {"inlet": 0.006, "bump_wall": 0.0}
+SA_BOX = {"inlet": 0.006, "wall": 0.0, "farfield": 0.004}

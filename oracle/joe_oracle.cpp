@@ -31,9 +31,11 @@ inline double norm3(double *n, const double *v) {  // tc_vec3d.h:36-46
 // pointwise flux functions
 // ---------------------------------------------------------------------------
 
-// HLLC, UgpWithCvCompFlow.cpp:120-235 (nScal = 0, surfVeloc = 0)
+// HLLC, UgpWithCvCompFlow.cpp:120-235 (surfVeloc = 0). With Fs, one
+// transported scalar: FrhoScal = Frho * (upwind side value) * area.
 void flux_hllc(double F[5], double rhoL, const double *uL, double pL, double gammaL, double kL, double rhoR,
-               const double *uR, double pR, double gammaR, double kR, double area, const double *nVec) {
+               const double *uR, double pR, double gammaR, double kR, double area, const double *nVec,
+               double phiL = 0.0, double phiR = 0.0, double *Fs = nullptr) {
   double surfv = 0.0;
   double unL = dot3(uL, nVec) - surfv;
   double unLr = unL - surfv;
@@ -59,12 +61,13 @@ void flux_hllc(double F[5], double rhoL, const double *uL, double pL, double gam
   double sR = max(uRoe + cRoe, unRr + cR);
   double sM = (pL - pR - rhoL * unL * (sL - unL) + rhoR * unR * (sR - unR)) / (rhoR * (sR - unR) - rhoL * (sL - unL));
   double pStar = rhoR * (unR - sR) * (unR - sM) + pR;
-  double Frho, Frhou[3], FrhoE;
+  double Frho, Frhou[3], FrhoE, FrhoScal;
   if (sM >= 0.0) {
     if (sL > 0.0) {
       Frho = rhoL * unLr;
       for (int i = 0; i < 3; i++) Frhou[i] = rhoL * uL[i] * unLr + pL * nVec[i];
       FrhoE = eL * unLr + pL * unL;
+      FrhoScal = Frho * phiL;
     } else {
       double invSLmSs = 1.0 / (sL - sM);
       double sLmuL = sL - unL;
@@ -75,6 +78,7 @@ void flux_hllc(double F[5], double rhoL, const double *uL, double pL, double gam
       Frho = rhoSL * sM;
       for (int i = 0; i < 3; i++) Frhou[i] = rhouSL[i] * sM + pStar * nVec[i];
       FrhoE = (eSL + pStar) * sM;
+      FrhoScal = Frho * phiL;
     }
   } else {
     if (sR >= 0.0) {
@@ -87,15 +91,18 @@ void flux_hllc(double F[5], double rhoL, const double *uL, double pL, double gam
       Frho = rhoSR * sM;
       for (int i = 0; i < 3; i++) Frhou[i] = rhouSR[i] * sM + pStar * nVec[i];
       FrhoE = (eSR + pStar) * sM;
+      FrhoScal = Frho * phiR;
     } else {
       Frho = rhoR * unRr;
       for (int i = 0; i < 3; i++) Frhou[i] = rhoR * uR[i] * unRr + pR * nVec[i];
       FrhoE = eR * unRr + pR * unR;
+      FrhoScal = Frho * phiR;
     }
   }
   F[0] = Frho * area;
   for (int i = 0; i < 3; i++) F[1 + i] = Frhou[i] * area;
   F[4] = FrhoE * area;
+  if (Fs) *Fs = FrhoScal * area;
 }
 
 // Roe, UgpWithCvCompFlow.cpp:3281-3380. When the Roe-averaged a^2 < 1e-12 the
@@ -261,8 +268,17 @@ void jac_hllc_subsonic(double (*AL)[5], double (*AR)[5], double rhoL, const doub
 }
 
 // calcEulerFluxMatrices_HLLC, UgpWithCvCompFlow.cpp:242-595 (nScal = 0)
+// Which HLLC wave configuration the Jacobian was taken in, and the mass flux
+// per unit density of the upwind star (or unstarred) state, "FrhostarOverrho"
+// of the scalar rows (UgpWithCvCompFlow.cpp:296-311, 389-405, 534-550, 561-577).
+struct HllcBranch {
+  int k;       // 0: sL > 0; 1: sM >= 0 >= sL; 2: sR >= 0 > sM; 3: sR < 0
+  double fso;  // FrhostarOverrho
+};
+
 void jac_hllc(double (*A_L)[5], double (*A_R)[5], double rhoL, const double *uL, double pL, double gammaL, double kL,
-              double rhoR, const double *uR, double pR, double gammaR, double kR, double area, const double *nVec) {
+              double rhoR, const double *uR, double pR, double gammaR, double kR, double area, const double *nVec,
+              HllcBranch *br = nullptr) {
   double unL = dot3(uL, nVec);
   double uLuL = dot3(uL, uL);
   double cL = sqrt(gammaL * pL / rhoL);
@@ -288,6 +304,7 @@ void jac_hllc(double (*A_L)[5], double (*A_R)[5], double rhoL, const double *uL,
   double gAvg = 0.5 * (gammaL + gammaR);
   if (sM >= 0.0) {
     if (sL > 0.0) {
+      if (br) *br = {0, unL * area};
       if (A_L != NULL) {
         double nVecArea[3];
         for (int i = 0; i < 3; i++) nVecArea[i] = nVec[i] * area;
@@ -303,6 +320,7 @@ void jac_hllc(double (*A_L)[5], double (*A_R)[5], double rhoL, const double *uL,
       double rhouSL[3];
       for (int i = 0; i < 3; i++) rhouSL[i] = (rhoL * uL[i] * sLmuL + (pStar - pL) * nVec[i]) * invSLmSs;
       double eSL = (sLmuL * eL - pL * unL + pStar * sM) * invSLmSs;
+      if (br) *br = {1, invSLmSs * sLmuL * sM * area};
       double gammaLM1 = (gammaL - 1.0), gammaRM1 = (gammaR - 1.0);
       double invrhotld = 1.0 / (rhoR * (sR - unR) - rhoL * (sL - unL));
       double dSMdUL[5], dSMdUR[5], dpsdUL[5], dpsdUR[5];
@@ -357,6 +375,7 @@ void jac_hllc(double (*A_L)[5], double (*A_R)[5], double rhoL, const double *uL,
       double rhouSR[3];
       for (int i = 0; i < 3; i++) rhouSR[i] = (rhoR * uR[i] * sRmuR + (pStar - pR) * nVec[i]) * invSRmSs;
       double eSR = (sRmuR * eR - pR * unR + pStar * sM) * invSRmSs;
+      if (br) *br = {2, invSRmSs * sRmuR * sM * area};
       double gammaLM1 = (gammaL - 1.0), gammaRM1 = (gammaR - 1.0);
       double invrhotld = 1.0 / (rhoR * (sR - unR) - rhoL * (sL - unL));
       double dSMdUL[5], dSMdUR[5], dpsdUL[5], dpsdUR[5];
@@ -403,6 +422,7 @@ void jac_hllc(double (*A_L)[5], double (*A_R)[5], double rhoL, const double *uL,
           for (int j = 0; j < 5; j++) A_R[i][j] *= area;
       }
     } else {
+      if (br) *br = {3, unR * area};
       if (A_R != NULL) {
         double nVecArea[3];
         for (int i = 0; i < 3; i++) nVecArea[i] = nVec[i] * area;
@@ -411,6 +431,40 @@ void jac_hllc(double (*A_L)[5], double (*A_R)[5], double rhoL, const double *uL,
       if (A_L != NULL)
         for (int i = 0; i < 5; i++)
           for (int j = 0; j < 5; j++) A_L[i][j] = 0.0;
+    }
+  }
+}
+
+// Scalar rows of the HLLC Jacobian (A_L_Scal / A_R_Scal, the same lines as
+// HllcBranch): d(rho phi un)/dQ from row 0 of the 5x5 block times the upwind
+// scalar, and d/d(rho phi) on the upwind side. A side whose block was not
+// computed (NULL) is left untouched.
+void jac_hllc_scal(double *SL, double *SR, const double (*A_L)[5], const double (*A_R)[5], const HllcBranch &br,
+                   double phiL, double phiR) {
+  const bool upL = br.k <= 1;
+  const double phi = upL ? phiL : phiR;
+  if (SL && A_L) {
+    if (br.k == 3) {
+      for (int i = 0; i <= 5; i++) SL[i] = 0.0;
+    } else if (upL) {
+      SL[0] = (A_L[0][0] - br.fso) * phi;
+      for (int i = 1; i < 5; i++) SL[i] = A_L[0][i] * phi;
+      SL[5] = br.fso;
+    } else {
+      for (int i = 0; i < 5; i++) SL[i] = A_L[0][i] * phi;
+      SL[5] = 0.0;
+    }
+  }
+  if (SR && A_R) {
+    if (br.k == 0) {
+      for (int i = 0; i <= 5; i++) SR[i] = 0.0;
+    } else if (!upL) {
+      SR[0] = (A_R[0][0] - br.fso) * phi;
+      for (int i = 1; i < 5; i++) SR[i] = A_R[0][i] * phi;
+      SR[5] = br.fso;
+    } else {
+      for (int i = 0; i < 5; i++) SR[i] = A_R[0][i] * phi;
+      SR[5] = 0.0;
     }
   }
 }
@@ -570,6 +624,11 @@ struct or_ctx {
   std::vector<double> grad_rho, grad_p, grad_u, alpha_rho, local_dt;
   std::vector<double> mul_fa, lamOcp_fa, grad_h;
   std::vector<double> rhs_rho, rhs_rhou, rhs_rhoE, A, dq, resid;
+  // Spalart-Allmaras: the transported scalar nuSA (ScalarTranspEq, UgpWithCv.h:50-113),
+  // its matrix rows AScal[6][nnz] (JoeWithModels.cpp:892), mut_fa and the model fields
+  bool sa = false;
+  std::vector<double> phi, phi_bfa, grad_phi, diff, mut_fa, wall_dist, str_mag, vort_mag;
+  std::vector<double> rhs_scal, AScal, dscal;
   double cfl;
   int step;
   bool have_press;
@@ -773,7 +832,8 @@ void calc_material(or_ctx &c) {
   }
 }
 
-// calcDt (MAX_BASED), UgpWithCvCompFlow.cpp:5-60, 100-105. mut_fa = 0.
+// calcDt (MAX_BASED), UgpWithCvCompFlow.cpp:5-60, 100-105. mut_fa is zero
+// without a turbulence model (UgpWithCvCompFlow.h:1568-1580).
 double calc_dt(or_ctx &c) {
   const or_mesh &m = c.m;
   double dt_min = 1.0e20;
@@ -786,7 +846,7 @@ double calc_dt(or_ctx &c) {
       double area = norm3(nVec, &m.fa_normal[3 * ifa]);
       double Uk = dot3(&c.rhou[3 * icv], nVec) / c.rho[icv];
       double lambda = (fabs(Uk) + cs) * area;
-      double lamdiff = 4.0 * (c.mul_fa[ifa] + 0.0) / c.rho[icv] * area * area / m.cv_volume[icv];
+      double lamdiff = 4.0 * (c.mul_fa[ifa] + c.mut_fa[ifa]) / c.rho[icv] * area * area / m.cv_volume[icv];
       lambda = max(lambda, lamdiff);
       lambdaMax = max(lambdaMax, lambda);
     }
@@ -949,24 +1009,40 @@ void limit_vector(or_ctx &c, double *g, const double *phi, const double *ref) {
   }
 }
 
+// Scalars are transported with HLLC only (the build rejects other schemes
+// together with a turbulence model), so Fs / SL / SR are HLLC's.
 void face_flux(const or_ctx &c, double F[5], double rho0, const double *u0, double p0, double rho1, const double *u1,
-               double p1, double area, const double *nVec) {
+               double p1, double area, const double *nVec, double phi0 = 0.0, double phi1 = 0.0,
+               double *Fs = nullptr) {
   const double g = c.p.gamma;
   if (c.p.scheme == OR_ROE)
     flux_roe(F, rho0, u0, p0, g, rho1, u1, p1, g, area, nVec);
   else if (c.p.scheme == OR_AUSM)
     flux_ausm(F, rho0, u0, p0, g, 0.0, rho1, u1, p1, g, 0.0, area, nVec);
   else
-    flux_hllc(F, rho0, u0, p0, g, 0.0, rho1, u1, p1, g, 0.0, area, nVec);
+    flux_hllc(F, rho0, u0, p0, g, 0.0, rho1, u1, p1, g, 0.0, area, nVec, phi0, phi1, Fs);
 }
 
 void face_jac(const or_ctx &c, double (*Apl)[5], double (*Ami)[5], double rho0, const double *u0, double p0,
-              double rho1, const double *u1, double p1, double area, const double *nVec) {
+              double rho1, const double *u1, double p1, double area, const double *nVec, double phi0 = 0.0,
+              double phi1 = 0.0, double *SL = nullptr, double *SR = nullptr) {
   const double g = c.p.gamma;
-  if (c.p.scheme == OR_ROE)
+  if (c.p.scheme == OR_ROE) {
     jac_roe(Apl, Ami, rho0, u0, p0, g, rho1, u1, p1, g, area, nVec);
-  else  // HLLC, and AUSM which uses the HLLC Jacobian (JoeWithModels.cpp:3057-3061)
-    jac_hllc(Apl, Ami, rho0, u0, p0, g, 0.0, rho1, u1, p1, g, 0.0, area, nVec);
+  } else {  // HLLC, and AUSM which uses the HLLC Jacobian (JoeWithModels.cpp:3057-3061)
+    HllcBranch br;
+    jac_hllc(Apl, Ami, rho0, u0, p0, g, 0.0, rho1, u1, p1, g, 0.0, area, nVec, &br);
+    if (SL || SR) jac_hllc_scal(SL, SR, Apl, Ami, br, phi0, phi1);
+  }
+}
+
+// AScal[k][noc] (k = 0..5) += sgn * S[k]
+inline void add_ascal(or_ctx &c, int noc, double sgn, const double *S) {
+  const size_t nnz = c.nnz;
+  for (int k = 0; k <= 5; k++) {
+    if (sgn > 0.0) c.AScal[k * nnz + noc] += S[k];
+    else c.AScal[k * nnz + noc] -= S[k];
+  }
 }
 
 // calcEulerFlux, JoeWithModels.cpp:2621-3589 (nScal = 0, no shock fix)
@@ -992,8 +1068,17 @@ void calc_euler_flux(or_ctx &c, double *A) {
     else
       grad_scalar_gg(c, c.grad_p.data(), c.press.data(), c.p_bfa.data());
     if (c.p.limiter) limit_scalar(c, c.grad_p.data(), c.press.data(), c.press.data(), NULL);
+    // STANDARD reconstruction of the transported scalar (:2736-2741)
+    if (c.sa) {
+      if (c.p.grad_scheme == OR_GRAD_LS)
+        grad_scalar_ls(c, c.grad_phi.data(), c.phi.data(), c.phi_bfa.data());
+      else
+        grad_scalar_gg(c, c.grad_phi.data(), c.phi.data(), c.phi_bfa.data());
+      if (c.p.limiter) limit_scalar(c, c.grad_phi.data(), c.phi.data(), c.phi.data(), NULL);
+    }
   }
-  double F[5], Apl[5][5], Ami[5][5];
+  double F[5], Apl[5][5], Ami[5][5], Fs = 0.0, SL[6], SR[6];
+  const bool sa = c.sa;
   for (int ifa = c.nfa_b; ifa < c.nfa; ifa++) {
     int icv0 = m.cvofa[2 * ifa], icv1 = m.cvofa[2 * ifa + 1];
     int noc00 = 0, noc01 = 0, noc11 = 0, noc10 = 0;
@@ -1011,6 +1096,7 @@ void calc_euler_flux(or_ctx &c, double *A) {
     double u0[3] = {c.vel[3 * icv0], c.vel[3 * icv0 + 1], c.vel[3 * icv0 + 2]};
     double rho1 = c.rho[icv1], p1 = c.press[icv1];
     double u1[3] = {c.vel[3 * icv1], c.vel[3 * icv1 + 1], c.vel[3 * icv1 + 2]};
+    double sc0 = sa ? c.phi[icv0] : 0.0, sc1 = sa ? c.phi[icv1] : 0.0;
     if (snd) {
       double r0[3], r1[3];
       for (int i = 0; i < 3; i++) {
@@ -1022,26 +1108,33 @@ void calc_euler_flux(or_ctx &c, double *A) {
       if ((p0 <= 0.0) || (rho0 <= 0.0)) {
         p0 = c.press[icv0];
         rho0 = c.rho[icv0];
-      } else
+      } else {
         for (int i = 0; i < 3; i++) u0[i] += dot3(r0, &c.grad_u[9 * icv0 + 3 * i]);
+        if (sa) sc0 = sc0 + dot3(r0, &c.grad_phi[3 * icv0]);  // :2862-2869
+      }
       rho1 += dot3(r1, &c.grad_rho[3 * icv1]);
       p1 += dot3(r1, &c.grad_p[3 * icv1]);
       if ((p1 <= 0.0) || (rho1 <= 0.0)) {
         p1 = c.press[icv1];
         rho1 = c.rho[icv1];
-      } else
+      } else {
         for (int i = 0; i < 3; i++) u1[i] += dot3(r1, &c.grad_u[9 * icv1 + 3 * i]);
+        if (sa) sc1 = sc1 + dot3(r1, &c.grad_phi[3 * icv1]);
+      }
     }
-    face_flux(c, F, rho0, u0, p0, rho1, u1, p1, area, nVec);
+    face_flux(c, F, rho0, u0, p0, rho1, u1, p1, area, nVec, sc0, sc1, sa ? &Fs : nullptr);
     c.rhs_rho[icv0] -= F[0];
     for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= F[1 + i];
     c.rhs_rhoE[icv0] -= F[4];
+    if (sa) c.rhs_scal[icv0] -= 1.0 * Fs;  // ScalConvTerm = 1 (:3014-3015)
     c.rhs_rho[icv1] += F[0];
     for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv1 + i] += F[1 + i];
     c.rhs_rhoE[icv1] += F[4];
+    if (sa) c.rhs_scal[icv1] += 1.0 * Fs;
     if (A) {
       face_jac(c, Apl, Ami, c.rho[icv0], &c.vel[3 * icv0], c.press[icv0], c.rho[icv1], &c.vel[3 * icv1],
-               c.press[icv1], area, nVec);
+               c.press[icv1], area, nVec, sa ? c.phi[icv0] : 0.0, sa ? c.phi[icv1] : 0.0, sa ? SL : nullptr,
+               sa ? SR : nullptr);
       for (int i = 0; i < 5; i++)
         for (int j = 0; j < 5; j++) {
           A[25 * noc00 + 5 * i + j] += Apl[i][j];
@@ -1052,6 +1145,12 @@ void calc_euler_flux(or_ctx &c, double *A) {
           A[25 * noc11 + 5 * i + j] -= Ami[i][j];
           A[25 * noc10 + 5 * i + j] -= Apl[i][j];
         }
+      if (sa) {  // :3088-3101
+        add_ascal(c, noc00, 1.0, SL);
+        add_ascal(c, noc01, 1.0, SR);
+        add_ascal(c, noc11, -1.0, SR);
+        add_ascal(c, noc10, -1.0, SL);
+      }
     }
   }
   // boundary zones in faZoneList order (JoeWithModels.cpp:3112-3569)
@@ -1063,7 +1162,9 @@ void calc_euler_flux(or_ctx &c, double *A) {
       double area = norm3(nVec, &m.fa_normal[3 * ifa]);
       const double *ub = &c.vel_bfa[3 * ifa];
       if (bc == OR_BC_SYMMETRY || bc == OR_BC_WALL) {
-        face_flux(c, F, c.rho_bfa[ifa], ub, c.p_bfa[ifa], c.rho_bfa[ifa], ub, c.p_bfa[ifa], area, nVec);
+        const double sb = sa ? c.phi_bfa[ifa] : 0.0;  // both sides phi_bfa (:3134-3137, 3291-3294)
+        face_flux(c, F, c.rho_bfa[ifa], ub, c.p_bfa[ifa], c.rho_bfa[ifa], ub, c.p_bfa[ifa], area, nVec, sb, sb,
+                  sa ? &Fs : nullptr);
         if (bc == OR_BC_SYMMETRY && c.p.symm_bc == OR_SYMM_ANALYTICAL) {
           for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= c.p_bfa[ifa] * nVec[i] * area;
           c.rhs_rho[icv0] -= 0.0;
@@ -1073,8 +1174,15 @@ void calc_euler_flux(or_ctx &c, double *A) {
           for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= F[1 + i];
           c.rhs_rhoE[icv0] -= F[4];
         }
+        if (sa) c.rhs_scal[icv0] -= 1.0 * Fs;  // :3197-3198, 3341-3342
         if (A) {
           int noc00 = m.nbocv_i[icv0];
+          if (sa) {  // scalar rows from the bfa/bfa HLLC Jacobian (:3202-3206, 3346-3350, 3270-3274)
+            double Al[5][5];
+            face_jac(c, Al, NULL, c.rho_bfa[ifa], ub, c.p_bfa[ifa], c.rho_bfa[ifa], ub, c.p_bfa[ifa], area, nVec, sb,
+                     sb, SL, nullptr);
+            add_ascal(c, noc00, 1.0, SL);
+          }
           if (bc == OR_BC_WALL || c.p.symm_bc == OR_SYMM_ORIGINAL_NUMERIC) {
             face_jac(c, Apl, NULL, c.rho_bfa[ifa], ub, c.p_bfa[ifa], c.rho_bfa[ifa], ub, c.p_bfa[ifa], area, nVec);
             for (int k = 0; k < 25; k++) A[25 * noc00 + k] += Apl[k / 5][k % 5];
@@ -1100,6 +1208,7 @@ void calc_euler_flux(or_ctx &c, double *A) {
       } else {
         double rho0 = c.rho[icv0], p0 = c.press[icv0];
         double u0[3] = {c.vel[3 * icv0], c.vel[3 * icv0 + 1], c.vel[3 * icv0 + 2]};
+        double sc0 = sa ? c.phi[icv0] : 0.0, sb = sa ? c.phi_bfa[ifa] : 0.0;  // :3413-3419
         if (snd) {
           double r0[3];
           for (int i = 0; i < 3; i++) r0[i] = m.x_fa[3 * ifa + i] - m.x_cv[3 * icv0 + i];
@@ -1108,18 +1217,22 @@ void calc_euler_flux(or_ctx &c, double *A) {
           if ((p0 <= 0.0) || (rho0 <= 0.0)) {
             p0 = c.press[icv0];
             rho0 = c.rho[icv0];
-          } else
+          } else {
             for (int i = 0; i < 3; i++) u0[i] += dot3(r0, &c.grad_u[9 * icv0 + 3 * i]);
+            if (sa) sc0 = sc0 + dot3(r0, &c.grad_phi[3 * icv0]);  // :3445-3454
+          }
         }
-        face_flux(c, F, rho0, u0, p0, c.rho_bfa[ifa], ub, c.p_bfa[ifa], area, nVec);
+        face_flux(c, F, rho0, u0, p0, c.rho_bfa[ifa], ub, c.p_bfa[ifa], area, nVec, sc0, sb, sa ? &Fs : nullptr);
         c.rhs_rho[icv0] -= F[0];
         for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= F[1 + i];
         c.rhs_rhoE[icv0] -= F[4];
+        if (sa) c.rhs_scal[icv0] -= 1.0 * Fs;
         if (A) {
           face_jac(c, Apl, NULL, c.rho[icv0], &c.vel[3 * icv0], c.press[icv0], c.rho_bfa[ifa], ub, c.p_bfa[ifa], area,
-                   nVec);
+                   nVec, sa ? c.phi[icv0] : 0.0, sb, sa ? SL : nullptr, nullptr);
           int noc00 = m.nbocv_i[icv0];
           for (int k = 0; k < 25; k++) A[25 * noc00 + k] += Apl[k / 5][k % 5];
+          if (sa) add_ascal(c, noc00, 1.0, SL);  // :3559-3561
         }
       }
     }
@@ -1232,7 +1345,7 @@ void add_visc_flux(const or_ctx &c, double *Frhou, double &FrhoE, double (*A0)[5
   }
 }
 
-// calcViscousFluxNS, JoeWithModels.cpp:3591-3855 (laminar, no kine)
+// calcViscousFluxNS, JoeWithModels.cpp:3591-3855 (no kine: laminar or SA)
 void calc_viscous_flux(or_ctx &c, double *A) {
   const or_mesh &m = c.m;
   const double gam = c.p.gamma, R = c.p.R_gas;
@@ -1274,7 +1387,7 @@ void calc_viscous_flux(or_ctx &c, double *A) {
     add_visc_flux(c, Frhou, FrhoE, A ? A0 : NULL, A ? A1 : NULL, c.rho[icv0], v0, &c.grad_u[9 * icv0],
                   c.enthalpy[icv0], &c.grad_h[3 * icv0], c.temp[icv0], R, gam, 0.0, c.rho[icv1], v1,
                   &c.grad_u[9 * icv1], c.enthalpy[icv1], &c.grad_h[3 * icv1], c.temp[icv1], R, gam, 0.0,
-                  c.mul_fa[ifa], 0.0, c.lamOcp_fa[ifa], uAux_fa, area, nVec, smag, sVec);
+                  c.mul_fa[ifa], c.mut_fa[ifa], c.lamOcp_fa[ifa], uAux_fa, area, nVec, smag, sVec);
     if (A) {
       for (int i = 1; i < 5; i++)
         for (int j = 0; j < 5; j++) {
@@ -1317,7 +1430,7 @@ void calc_viscous_flux(or_ctx &c, double *A) {
       add_visc_flux(c, Frhou, FrhoE, A ? A0 : NULL, NULL, c.rho[icv0], &c.vel[3 * icv0], &c.grad_u[9 * icv0],
                     c.enthalpy[icv0], &c.grad_h[3 * icv0], c.temp[icv0], R, gam, 0.0, c.rho_bfa[ifa], vb,
                     &c.grad_u[9 * icv0], c.h_bfa[ifa], &c.grad_h[3 * icv0], c.T_bfa[ifa], R, gam, 0.0, c.mul_fa[ifa],
-                    0.0, c.lamOcp_fa[ifa], vb, area, nVec, smag, sArg);
+                    c.mut_fa[ifa], c.lamOcp_fa[ifa], vb, area, nVec, smag, sArg);
       if (A) {
         int noc00 = m.nbocv_i[icv0];
         for (int i = 1; i < 5; i++)
@@ -1329,12 +1442,319 @@ void calc_viscous_flux(or_ctx &c, double *A) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Spalart-Allmaras as one segregated scalar (RansTurbSA, TurbModel_SA.h)
+// ---------------------------------------------------------------------------
+namespace sa {
+// model constants, getDoubleParam defaults (TurbModel_SA.h:20-27)
+const double cv1_3 = 357.911, cw1 = 3.239067817, cw2 = 0.3, cw3_6 = 64.0, sigma = 0.6666666, cb1 = 0.1355,
+             cb2 = 0.622, KarmanConst_2 = 0.1681;
+// ScalarTranspEq settings of nuSA (TurbModel_SA.h:31-37)
+const double phiZero = 1.0e-9, phiZeroRel = 1.0e-2, lowerBound = 1.0e-10, upperBound = 1.0;
+const int phiMaxiter = 1000;
+}  // namespace sa
+
+// calcWallDistance, UgpWithCvCompFlow.h:1395-1535 (one rank): nearest wall
+// face centroid, then the normal distance when the cell centre lies within
+// the face's "spectral radius"
+void calc_wall_distance(or_ctx &c) {
+  const or_mesh &m = c.m;
+  std::vector<int> wf;
+  for (int iz = 0; iz < m.nzone; iz++)
+    if (m.zone_bc[iz] == OR_BC_WALL)
+      for (int ifa = m.zone_ifa[2 * iz]; ifa <= m.zone_ifa[2 * iz + 1]; ifa++) wf.push_back(ifa);
+  for (int icv = 0; icv < c.ncv; icv++) {
+    double minDist = 1.0e20;
+    int save = 0;
+    for (size_t k = 0; k < wf.size(); k++) {
+      const double *xf = &m.x_fa[3 * wf[k]], *xc = &m.x_cv[3 * icv];
+      double dist = sqrt(pow(xc[0] - xf[0], 2.0) + pow(xc[1] - xf[1], 2.0) + pow(xc[2] - xf[2], 2.0));
+      if (dist < minDist) {
+        save = (int)k;
+        minDist = dist;
+      }
+    }
+    const int f = wf[save];
+    double n[3], s_half[3];
+    double nmag = norm3(n, &m.fa_normal[3 * f]);
+    for (int i = 0; i < 3; i++) s_half[i] = m.x_fa[3 * f + i] - m.x_cv[3 * icv + i];
+    double sMag2 = dot3(s_half, s_half);
+    double dNorm2 = dot3(s_half, n) * dot3(s_half, n);
+    double dTang2 = sMag2 - dNorm2;
+    double specRadius2 = 0.5 * nmag;
+    c.wall_dist[icv] = dTang2 > specRadius2 ? minDist : fabs(dot3(s_half, n));
+  }
+}
+
+// InterpolateAtCellCenterFromFaceValues, UgpWithCvCompFlow.h:1538-1551
+double cell_avg(const or_ctx &c, const std::vector<double> &phi, int icv) {
+  double phiC = 0.0;
+  int foc_f = c.m.faocv_i[icv], foc_l = c.m.faocv_i[icv + 1] - 1;
+  for (int foc = foc_f; foc <= foc_l; foc++) phiC += phi[c.m.faocv_v[foc]];
+  phiC /= (double)(foc_l - foc_f + 1);
+  return phiC;
+}
+
+// calcRansTurbViscMuet of RansTurbSA (TurbModel_SA.h:74-128) with calcGradVel,
+// calcStrainRate, calcVorticity (UgpWithCvCompFlow.h:1305-1392)
+void calc_turb_visc(or_ctx &c) {
+  const or_mesh &m = c.m;
+  if (c.p.grad_scheme == OR_GRAD_LS)
+    grad_vector_ls(c, c.grad_u.data(), c.vel.data(), c.vel_bfa.data());
+  else
+    grad_vector_gg(c, c.grad_u.data(), c.vel.data(), c.vel_bfa.data());
+  if (c.p.limiter) limit_vector(c, c.grad_u.data(), c.vel.data(), c.sos.data());
+  for (int icv = 0; icv < c.ncv; icv++) {
+    const double *g = &c.grad_u[9 * icv];
+    double str = 0.0;
+    double diverg = g[0] + g[4] + g[8];
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++)
+        if (i == j) str += pow(0.5 * (g[3 * i + j] + g[3 * j + i]) - 1. / 3. * diverg, 2.0);
+        else str += pow(0.5 * (g[3 * i + j] + g[3 * j + i]), 2.0);
+    c.str_mag[icv] = sqrt(2.0 * str);
+    double vort = 0.0;
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) vort += pow(0.5 * (g[3 * i + j] - g[3 * j + i]), 2.0);
+    c.vort_mag[icv] = sqrt(2.0 * vort);
+  }
+  for (int ifa = c.nfa_b; ifa < c.nfa; ifa++) {
+    int icv0 = m.cvofa[2 * ifa], icv1 = m.cvofa[2 * ifa + 1];
+    double w0, w1;
+    gg_weights(m, ifa, icv0, icv1, w0, w1);
+    double rho_fa = (w1 * c.rho[icv0] + w0 * c.rho[icv1]) / (w0 + w1);
+    double nuSA_fa = (w1 * c.phi[icv0] + w0 * c.phi[icv1]) / (w0 + w1);
+    double muSA_fa = rho_fa * nuSA_fa;
+    double chi_3 = pow(muSA_fa / c.mul_fa[ifa], 3.0);
+    double fv1 = chi_3 / (chi_3 + sa::cv1_3);
+    c.mut_fa[ifa] = max(muSA_fa * fv1, 0.0);
+  }
+  for (int iz = 0; iz < m.nzone; iz++) {
+    const int f0 = m.zone_ifa[2 * iz], f1 = m.zone_ifa[2 * iz + 1];
+    if (m.zone_bc[iz] == OR_BC_WALL) {
+      for (int ifa = f0; ifa <= f1; ifa++) c.mut_fa[ifa] = 0.0;
+    } else {
+      for (int ifa = f0; ifa <= f1; ifa++) {
+        int icv0 = m.cvofa[2 * ifa];
+        double muSA = c.rho[icv0] * c.phi[icv0];
+        double mul_icv0 = cell_avg(c, c.mul_fa, icv0);
+        double chi_3 = pow(muSA / mul_icv0, 3.0);
+        double fv1 = chi_3 / (chi_3 + sa::cv1_3);
+        c.mut_fa[ifa] = max(muSA * fv1, 0.0);
+      }
+    }
+  }
+}
+
+// setScalarBC, Scalars.cpp:289-360: a zone with "<zone>.nuSA = v" takes v on
+// inflow faces and the cell value on outflow faces; any other zone the cell value
+void set_scalar_bc(or_ctx &c) {
+  const or_mesh &m = c.m;
+  for (int iz = 0; iz < m.nzone; iz++) {
+    const bool dir = m.zone_scal[2 * iz] != 0.0;
+    const double val = m.zone_scal[2 * iz + 1];
+    for (int ifa = m.zone_ifa[2 * iz]; ifa <= m.zone_ifa[2 * iz + 1]; ifa++) {
+      int icv0 = m.cvofa[2 * ifa];
+      if (dir && !(dot3(&c.vel_bfa[3 * ifa], &m.fa_normal[3 * ifa]) > 1.0e-8))
+        c.phi_bfa[ifa] = val;
+      else
+        c.phi_bfa[ifa] = c.phi[icv0];
+    }
+  }
+}
+
+// calcViscousFluxScalar_new, Scalars.cpp:117-247, with the SA diffusivity
+// (diffusivityHookScalarRansTurb, TurbModel_SA.h:130-172)
+void calc_viscous_flux_scalar(or_ctx &c, bool implicit) {
+  const or_mesh &m = c.m;
+  const size_t nnz = c.nnz;
+  double *A5 = &c.AScal[5 * nnz];
+  if (c.p.grad_scheme == OR_GRAD_LS)
+    grad_scalar_ls(c, c.grad_phi.data(), c.phi.data(), c.phi_bfa.data());
+  else
+    grad_scalar_gg(c, c.grad_phi.data(), c.phi.data(), c.phi_bfa.data());
+  if (c.p.limiter) limit_scalar(c, c.grad_phi.data(), c.phi.data(), c.phi.data(), NULL);
+  const double invSigma = 1.0 / sa::sigma;
+  for (int ifa = c.nfa_b; ifa < c.nfa; ifa++) {
+    int icv0 = m.cvofa[2 * ifa], icv1 = m.cvofa[2 * ifa + 1];
+    double w0, w1;
+    gg_weights(m, ifa, icv0, icv1, w0, w1);
+    double rho_fa = (w1 * c.rho[icv0] + w0 * c.rho[icv1]) / (w0 + w1);
+    double nuSA_fa = (w1 * c.phi[icv0] + w0 * c.phi[icv1]) / (w0 + w1);
+    c.diff[ifa] = invSigma * (c.mul_fa[ifa] + rho_fa * nuSA_fa);
+  }
+  for (int iz = 0; iz < m.nzone; iz++)
+    for (int ifa = m.zone_ifa[2 * iz]; ifa <= m.zone_ifa[2 * iz + 1]; ifa++)
+      c.diff[ifa] = m.zone_bc[iz] == OR_BC_WALL ? invSigma * c.mul_fa[ifa]
+                                                : invSigma * (c.mul_fa[ifa] + c.rho_bfa[ifa] * c.phi_bfa[ifa]);
+  for (int ifa = c.nfa_b; ifa < c.nfa; ifa++) {
+    int icv0 = m.cvofa[2 * ifa], icv1 = m.cvofa[2 * ifa + 1];
+    double n[3], sv[3];
+    double nmag = norm3(n, &m.fa_normal[3 * ifa]);
+    for (int i = 0; i < 3; i++) sv[i] = m.x_cv[3 * icv1 + i] - m.x_cv[3 * icv0 + i];
+    double smag = sqrt(dot3(sv, sv));
+    for (int i = 0; i < 3; i++) sv[i] /= smag;
+    double alpha = dot3(n, sv);
+    const double *g0 = &c.grad_phi[3 * icv0], *g1 = &c.grad_phi[3 * icv1];
+    double viscFlux = c.diff[ifa] * nmag *
+                      (alpha * (c.phi[icv1] - c.phi[icv0]) / smag + 0.5 * (g0[0] + g1[0]) * (n[0] - alpha * sv[0]) +
+                       0.5 * (g0[1] + g1[1]) * (n[1] - alpha * sv[1]) + 0.5 * (g0[2] + g1[2]) * (n[2] - alpha * sv[2]));
+    c.rhs_scal[icv0] += viscFlux;
+    c.rhs_scal[icv1] -= viscFlux;
+    if (implicit) {
+      int noc00 = m.nbocv_i[icv0], noc01 = noc00;
+      while (m.nbocv_v[++noc01] != icv1) {}
+      int noc11 = m.nbocv_i[icv1], noc10 = noc11;
+      while (m.nbocv_v[++noc10] != icv0) {}
+      A5[noc00] += c.diff[ifa] * nmag * alpha / smag / c.rho[icv0];
+      A5[noc01] += -c.diff[ifa] * nmag * alpha / smag / c.rho[icv1];
+      A5[noc11] += c.diff[ifa] * nmag * alpha / smag / c.rho[icv1];
+      A5[noc10] += -c.diff[ifa] * nmag * alpha / smag / c.rho[icv0];
+    }
+  }
+  // boundary zones with a nuSA value (Dirichlet, :201-219); others carry no diffusive flux
+  for (int iz = 0; iz < m.nzone; iz++) {
+    if (m.zone_scal[2 * iz] == 0.0) continue;
+    for (int ifa = m.zone_ifa[2 * iz]; ifa <= m.zone_ifa[2 * iz + 1]; ifa++) {
+      int icv0 = m.cvofa[2 * ifa];
+      double n[3], s_half[3];
+      double nmag = norm3(n, &m.fa_normal[3 * ifa]);
+      for (int i = 0; i < 3; i++) s_half[i] = m.x_fa[3 * ifa + i] - m.x_cv[3 * icv0 + i];
+      double smag_half = fabs(dot3(s_half, n));
+      double viscFlux = c.diff[ifa] * nmag * (c.phi_bfa[ifa] - c.phi[icv0]) / smag_half;
+      c.rhs_scal[icv0] += viscFlux;
+      if (implicit) A5[m.nbocv_i[icv0]] += c.diff[ifa] * nmag / smag_half / c.rho[icv0];
+    }
+  }
+}
+
+// sourceHookScalarRansTurb_new, TurbModel_SA.h:282-357
+void sa_source(or_ctx &c, bool implicit) {
+  const or_mesh &m = c.m;
+  double *A5 = &c.AScal[5 * (size_t)c.nnz];
+  for (int icv = 0; icv < c.ncv; icv++) {
+    const double nuSA = c.phi[icv], rho = c.rho[icv], vol = m.cv_volume[icv];
+    double mue = cell_avg(c, c.mul_fa, icv);
+    double Om = max(c.vort_mag[icv], 1.e-8);
+    double Strainrate = max(c.str_mag[icv], 1.e-8);
+    double d = c.wall_dist[icv];
+    double chi = nuSA * rho / mue;
+    double chi_3 = pow(chi, 3.0);
+    double inv_KarmanConst2_d2 = 1.0 / (sa::KarmanConst_2 * d * d);
+    double fv1 = chi_3 / (chi_3 + sa::cv1_3);
+    double fv2 = 1.0 - chi / (1.0 + chi * fv1);
+    double BlendTurbulentProduction = 0.0;
+    double S = max(Om + min(0.0, Strainrate - Om) * BlendTurbulentProduction, 0.0);
+    double Shat = S + nuSA * fv2 * inv_KarmanConst2_d2;
+    double inv_Shat = 1.0 / max(Shat, 1.0e-8);
+    double r = min(nuSA * inv_Shat * inv_KarmanConst2_d2, 10.0);
+    double g = r + sa::cw2 * (pow(r, 6.0) - r);
+    double g_6 = pow(g, 6.0);
+    double fw = g * pow((1.0 + sa::cw3_6) / (g_6 + sa::cw3_6), 1.0 / 6.0);
+    double H1 = sa::cb1 * Shat * nuSA;
+    double H2 = -sa::cw1 * fw * nuSA * nuSA / pow(d, 2.0);
+    double H3 = sa::cb2 / sa::sigma * dot3(&c.grad_phi[3 * icv], &c.grad_phi[3 * icv]);
+    c.rhs_scal[icv] += rho * vol * (H1 + H2 + H3);
+    if (implicit) {
+      double d_H1 = 0.0;
+      double d_H2 = -2.0 * sa::cw1 * fw * nuSA / pow(d, 2.0);
+      A5[m.nbocv_i[icv]] -= (d_H1 + d_H2) * vol;
+    }
+  }
+}
+
+// solveCvScalarBcgstab, UgpWithCv.cpp:2298-2479 (Jacobi preconditioner; one rank)
+int bcgstab_scalar(const or_ctx &c, double *phi, const double *Ap, const double *rhs, double zero, double zeroRel,
+                   int maxiter) {
+  const or_mesh &m = c.m;
+  const int ncv = c.ncv;
+  std::vector<double> res(ncv), res0(ncv), p(ncv, 0.0), v(ncv, 0.0), t(ncv), s(ncv), phat(ncv, 0.0), shat(ncv, 0.0);
+  auto matvec = [&](double *out, const double *x) {
+    for (int icv = 0; icv < ncv; icv++) {
+      int noc_f = m.nbocv_i[icv], noc_l = m.nbocv_i[icv + 1] - 1;
+      out[icv] = Ap[noc_f] * x[icv];
+      for (int noc = noc_f + 1; noc <= noc_l; noc++) out[icv] += Ap[noc] * x[m.nbocv_v[noc]];
+    }
+  };
+  double rho = 1.0, alpha = 1.0, omega = 1.0;
+  matvec(res.data(), phi);
+  for (int icv = 0; icv < ncv; icv++) {
+    res[icv] = rhs[icv] - res[icv];
+    res0[icv] = res[icv];
+  }
+  double res0_max = 0.0;
+  for (int icv = 0; icv < ncv; icv++) res0_max = max(res0_max, fabs(res[icv] / Ap[m.nbocv_i[icv]]));
+  int iter = 0, done = 0;
+  if (res0_max <= zero) done = 1;
+  while (done == 0) {
+    iter++;
+    double rho_prime = -omega * rho;
+    rho = 0.0;
+    for (int icv = 0; icv < ncv; icv++) rho += res[icv] * res0[icv];
+    if (fabs(rho_prime) < 1.0E-20) rho_prime = 1.0E-20;
+    double beta = alpha * rho / rho_prime;
+    for (int icv = 0; icv < ncv; icv++) p[icv] = res[icv] - beta * (p[icv] - omega * v[icv]);
+    for (int icv = 0; icv < ncv; icv++) phat[icv] = p[icv] / Ap[m.nbocv_i[icv]];
+    matvec(v.data(), phat.data());
+    double gamma = 0.0;
+    for (int icv = 0; icv < ncv; icv++) gamma += v[icv] * res0[icv];
+    if (fabs(gamma) < 1.0E-20) gamma = 1.0E-20;
+    alpha = rho / gamma;
+    for (int icv = 0; icv < ncv; icv++) s[icv] = res[icv] - alpha * v[icv];
+    for (int icv = 0; icv < ncv; icv++) shat[icv] = s[icv] / Ap[m.nbocv_i[icv]];
+    matvec(t.data(), shat.data());
+    double b0 = 0.0, b1 = 0.0;
+    for (int icv = 0; icv < ncv; icv++) {
+      b0 += s[icv] * t[icv];
+      b1 += t[icv] * t[icv];
+    }
+    omega = b0 / (b1 + 1.0E-20);
+    for (int icv = 0; icv < ncv; icv++) phi[icv] += alpha * phat[icv] + omega * shat[icv];
+    matvec(res.data(), phi);
+    double res_max = 0.0;
+    for (int icv = 0; icv < ncv; icv++) {
+      res[icv] = rhs[icv] - res[icv];
+      res_max = max(res_max, fabs(res[icv] / Ap[m.nbocv_i[icv]]));
+    }
+    if ((res_max <= zero) || (res_max / (res0_max + 1.0E-12) <= zeroRel) || (iter > maxiter)) done = 1;
+  }
+  return iter;
+}
+
+// The scalar half of a backward-Euler step (JoeWithModels.cpp:1040-1091),
+// after the mean flow took dq: relaxation, vol/dt, the coupling columns
+// AScal[0..4] moved to the right with dq, Jacobi BiCGStab, update and clip.
+void scalar_solve(or_ctx &c, bool update) {
+  const or_mesh &m = c.m;
+  const size_t nnz = c.nnz;
+  for (int icv = 0; icv < c.ncv; icv++) {
+    c.rhs_scal[icv] *= c.p.relax_scal;
+    int noc_f = m.nbocv_i[icv], noc_l = m.nbocv_i[icv + 1] - 1;
+    double tmp = m.cv_volume[icv] / (c.local_dt[icv]);
+    c.AScal[5 * nnz + noc_f] += tmp;
+    for (int noc = noc_f; noc <= noc_l; noc++) {
+      const double *dqn = &c.dq[5 * m.nbocv_v[noc]];
+      c.rhs_scal[icv] = c.rhs_scal[icv] - c.AScal[0 * nnz + noc] * dqn[0] - c.AScal[1 * nnz + noc] * dqn[1] -
+                        c.AScal[2 * nnz + noc] * dqn[2] - c.AScal[3 * nnz + noc] * dqn[3] -
+                        c.AScal[4 * nnz + noc] * dqn[4];
+    }
+  }
+  std::fill(c.dscal.begin(), c.dscal.end(), 0.0);
+  bcgstab_scalar(c, c.dscal.data(), &c.AScal[5 * nnz], c.rhs_scal.data(), sa::phiZero, sa::phiZeroRel,
+                 sa::phiMaxiter);
+  if (update)
+    for (int icv = 0; icv < c.ncv; icv++)
+      c.phi[icv] = min(max((c.phi[icv] * (c.rho[icv] - c.dq[5 * icv]) + c.dscal[icv]) / c.rho[icv], sa::lowerBound),
+                       sa::upperBound);
+}
+
 void calc_rhs(or_ctx &c, double *A) {
   for (int icv = 0; icv < c.ncv; icv++) {
     c.rhs_rho[icv] = 0.0;
     for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv + i] = 0.0;
     c.rhs_rhoE[icv] = 0.0;
   }
+  if (c.sa) std::fill(c.rhs_scal.begin(), c.rhs_scal.end(), 0.0);
   calc_euler_flux(c, A);
   if (c.p.mu_ref > 0.0) calc_viscous_flux(c, A);
   const double *g = c.p.gravity;
@@ -1353,6 +1773,11 @@ void calc_rhs(or_ctx &c, double *A) {
       D[5 * 4 + 2] -= vol * g[1];
       D[5 * 4 + 3] -= vol * g[2];
     }
+  }
+  // scalars: diffusion, then the model source (JoeWithModels.cpp:2595-2618)
+  if (c.sa) {
+    if (c.p.mu_ref > 0.0) calc_viscous_flux_scalar(c, A != NULL);
+    sa_source(c, A != NULL);
   }
 }
 
@@ -1444,6 +1869,10 @@ void refresh(or_ctx &c) {
   calc_state(c);
   calc_material(c);
   set_bc(c);
+  if (c.sa) {
+    set_scalar_bc(c);
+    calc_turb_visc(c);
+  }
 }
 
 }  // namespace
@@ -1485,10 +1914,29 @@ or_ctx *or_ctx_create(const or_mesh *m, const or_params *p) {
   c->mul_fa.assign(m->nfa, 0.0);
   c->lamOcp_fa.assign(m->nfa, 0.0);
   c->grad_h.assign(3 * n, 0.0);
-  c->resid.assign(5, 0.0);
+  c->resid.assign(p->sa ? 6 : 5, 0.0);
+  c->mut_fa.assign(m->nfa, 0.0);
+  c->sa = p->sa != 0;
+  if (c->sa) {
+    c->phi.assign(n, p->sa_init);  // initialHook, JoeWithModels.cpp:219-240
+    c->phi_bfa.assign(nb, 0.0);
+    c->grad_phi.assign(3 * n, 0.0);
+    c->diff.assign(m->nfa, 0.0);
+    c->wall_dist.assign(n, 0.0);
+    c->str_mag.assign(n, 0.0);
+    c->vort_mag.assign(n, 0.0);
+    c->rhs_scal.assign(n, 0.0);
+    c->AScal.assign(6 * (size_t)m->nnz, 0.0);
+    c->dscal.assign(n, 0.0);
+    calc_wall_distance(*c);  // initialHookScalarRansTurbModel, TurbModel_SA.h:62-72
+  }
   c->cfl = p->cfl;
   c->step = 0;
   return c;
+}
+
+void or_set_scalar(or_ctx *c, const double *phi) {
+  if (c->sa) std::memcpy(c->phi.data(), phi, sizeof(double) * c->ncv);
 }
 
 void or_ctx_destroy(or_ctx *c) { delete c; }
@@ -1509,6 +1957,7 @@ void or_residual(or_ctx *c, int implicit) {
   refresh(*c);
   calc_dt(*c);
   if (implicit) std::fill(c->A.begin(), c->A.end(), 0.0);
+  if (c->sa) std::fill(c->AScal.begin(), c->AScal.end(), 0.0);
   calc_rhs(*c, implicit ? c->A.data() : NULL);
 }
 
@@ -1520,6 +1969,7 @@ static int step_once(or_ctx &c) {
   calc_dt(c);
   std::fill(c.A.begin(), c.A.end(), 0.0);
   std::fill(c.dq.begin(), c.dq.end(), 0.0);
+  if (c.sa) std::fill(c.AScal.begin(), c.AScal.end(), 0.0);
   calc_rhs(c, c.A.data());
   std::vector<double> rhs(5 * (size_t)c.ncv);
   for (int icv = 0; icv < c.ncv; icv++) {
@@ -1539,13 +1989,16 @@ static int step_once(or_ctx &c) {
     c.rhou[3 * icv + 2] += c.dq[5 * icv + 3];
     c.rhoE[icv] += c.dq[5 * icv + 4];
   }
+  if (c.sa) scalar_solve(c, true);
   refresh(c);
-  for (int i = 0; i < 5; i++) c.resid[i] = 0.0;
+  for (size_t i = 0; i < c.resid.size(); i++) c.resid[i] = 0.0;
   for (int icv = 0; icv < c.ncv; icv++) {
     c.resid[0] += fabs(c.rhs_rho[icv]);
     for (int i = 0; i < 3; i++) c.resid[i + 1] += fabs(c.rhs_rhou[3 * icv + i]);
     c.resid[4] += fabs(c.rhs_rhoE[icv]);
   }
+  if (c.sa)  // JoeWithModels.cpp:1132-1134
+    for (int icv = 0; icv < c.ncv; icv++) c.resid[5] += fabs(c.rhs_scal[icv] / c.p.under_relax);
   return it;
 }
 
@@ -1567,8 +2020,9 @@ int or_run(or_ctx *c, int max_steps, double *hist) {
   if (done(1.0e20)) return 0;
   while (n < max_steps) {
     step_once(*c);
+    const int nr = (int)c->resid.size();
     if (hist)
-      for (int i = 0; i < 5; i++) hist[5 * n + i] = c->resid[i];
+      for (int i = 0; i < nr; i++) hist[nr * n + i] = c->resid[i];
     n++;
     if (done(c->resid[4])) break;
   }
@@ -1604,6 +2058,19 @@ long long or_get(const or_ctx *c, const char *name, double *out) {
   else if (n == "RHO") v = &c->rho;
   else if (n == "RHOU") v = &c->rhou;
   else if (n == "RHOE") v = &c->rhoE;
+  else if (n == "MUT_FA") v = &c->mut_fa;
+  else if (c->sa) {
+    if (n == "NUSA") v = &c->phi;
+    else if (n == "NUSA_BFA") v = &c->phi_bfa;
+    else if (n == "GRAD_NUSA") v = &c->grad_phi;
+    else if (n == "WALL_DIST") v = &c->wall_dist;
+    else if (n == "STR_MAG") v = &c->str_mag;
+    else if (n == "VORT_MAG") v = &c->vort_mag;
+    else if (n == "DIFF_NUSA") v = &c->diff;
+    else if (n == "RHS_NUSA") v = &c->rhs_scal;
+    else if (n == "ASCAL") v = &c->AScal;
+    else if (n == "DSCAL") v = &c->dscal;
+  }
   if (!v) return -1;
   if (out) std::memcpy(out, v->data(), sizeof(double) * v->size());
   return (long long)v->size();
@@ -1613,7 +2080,8 @@ double or_cfl(const or_ctx *c) { return c->cfl; }
 int or_nsteps_done(const or_ctx *c) { return c->step; }
 
 /* One linear solve on the current A/rhs of the last or_residual call with
- * the first-step scaling (JoeWithModels.cpp:1011-1026); result in DQ. */
+ * the first-step scaling (JoeWithModels.cpp:1011-1026); result in DQ (and,
+ * with SA, the scalar correction in DSCAL). */
 int or_linear_solve(or_ctx *c) {
   std::vector<double> rhs(5 * (size_t)c->ncv);
   for (int icv = 0; icv < c->ncv; icv++) {
@@ -1624,7 +2092,9 @@ int or_linear_solve(or_ctx *c) {
     for (int i = 0; i < 5; i++) c->A[25 * c->m.nbocv_i[icv] + 6 * i] += tmp;
   }
   std::fill(c->dq.begin(), c->dq.end(), 0.0);
-  return bcgstab(*c, c->dq.data(), c->A.data(), rhs.data(), c->p.ls_abs, c->p.ls_rel, c->p.ls_maxiter);
+  int it = bcgstab(*c, c->dq.data(), c->A.data(), rhs.data(), c->p.ls_abs, c->p.ls_rel, c->p.ls_maxiter);
+  if (c->sa) scalar_solve(*c, false);  // DSCAL: the scalar correction of that first step
+  return it;
 }
 
 }  // extern "C"

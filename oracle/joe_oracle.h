@@ -41,6 +41,7 @@ typedef struct {
   const int *zone_bc;        /* OR_BC_* */
   const int *zone_ifa;       /* [nzone][2] first, last face */
   const double *zone_par;    /* [nzone][8] keyword values (see or_ctx_create) */
+  const double *zone_scal;   /* [nzone][2] nuSA boundary: kind (0 none, 1 value), value; NULL without SA */
 } or_mesh;
 
 typedef struct {
@@ -62,6 +63,11 @@ typedef struct {
   int visc_mode;
   double mu_power_law, sl_tref, sl_sref, T_ref, pr, pr_turb;
   int visc_limiter;
+  /* Spalart-Allmaras (RansTurbSA, turbModels/TurbModel_SA.h) as one
+   * segregated scalar nuSA: initial value (INITIAL_CONDITION_TURB) and
+   * UNDER_RELAXATION_SCALARS */
+  int sa;
+  double sa_init, relax_scal;
 } or_params;
 
 typedef struct or_ctx or_ctx;
@@ -73,13 +79,17 @@ void or_set_state(or_ctx *c, const double *rho, const double *rhou, const double
 void or_get_state(const or_ctx *c, double *rho, double *rhou, double *rhoE);
 /* calcStateVariables + setBC + calcDt + calcRhs (implicit: also A). */
 void or_residual(or_ctx *c, int implicit);
+/* the nuSA field (SA only), reference numbering */
+void or_set_scalar(or_ctx *c, const double *phi);
 /* one backward-Euler step (JoeWithModels.cpp:975-1158); returns BiCGStab iterations */
 int or_step(or_ctx *c);
 /* runBackwardEuler from the current state; returns steps taken. */
-int or_run(or_ctx *c, int max_steps, double *resid_hist /* [max_steps][5] or NULL */);
+int or_run(or_ctx *c, int max_steps, double *resid_hist /* [max_steps][5 + nScal] or NULL */);
 /* named arrays: RHS_RHO RHS_RHOU RHS_RHOE A LOCAL_DT GRAD_RHO GRAD_P GRAD_U
  * ALPHA_RHO VEL PRESS TEMP ENTHALPY SOS RHO_BFA VEL_BFA P_BFA T_BFA H_BFA DQ
- * RESID (5) MUL_FA GRAD_ENTHALPY; returns element count or -1 */
+ * RESID (5 + nScal) MUL_FA GRAD_ENTHALPY; with SA also NUSA NUSA_BFA
+ * GRAD_NUSA MUT_FA WALL_DIST STR_MAG VORT_MAG DIFF_NUSA RHS_NUSA ASCAL
+ * ([6][nnz]) DSCAL; returns element count or -1 */
 long long or_get(const or_ctx *c, const char *name, double *out);
 /* current CFL and step counter */
 double or_cfl(const or_ctx *c);

@@ -30,7 +30,8 @@ class _Mesh(C.Structure):
                 ("nbocv_i", C.c_void_p), ("nbocv_v", C.c_void_p),
                 ("x_fa", C.c_void_p), ("fa_normal", C.c_void_p), ("x_cv", C.c_void_p), ("cv_volume", C.c_void_p),
                 ("nbocv_lsg", C.c_void_p), ("bfa_lsg", C.c_void_p),
-                ("nzone", C.c_int), ("zone_bc", C.c_void_p), ("zone_ifa", C.c_void_p), ("zone_par", C.c_void_p)]
+                ("nzone", C.c_int), ("zone_bc", C.c_void_p), ("zone_ifa", C.c_void_p), ("zone_par", C.c_void_p),
+                ("zone_scal", C.c_void_p)]
 
 
 class _Params(C.Structure):
@@ -42,7 +43,7 @@ class _Params(C.Structure):
                 ("cfl_max", C.c_double), ("nsteps", C.c_int), ("resid_energ_th", C.c_double),
                 ("visc_mode", C.c_int), ("mu_power_law", C.c_double), ("sl_tref", C.c_double),
                 ("sl_sref", C.c_double), ("T_ref", C.c_double), ("pr", C.c_double), ("pr_turb", C.c_double),
-                ("visc_limiter", C.c_int)]
+                ("visc_limiter", C.c_int), ("sa", C.c_int), ("sa_init", C.c_double), ("relax_scal", C.c_double)]
 
 
 def lib():
@@ -56,6 +57,7 @@ def lib():
         L.or_ctx_create.argtypes = [C.POINTER(_Mesh), C.POINTER(_Params)]
         L.or_ctx_destroy.argtypes = [C.c_void_p]
         L.or_set_state.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.or_set_scalar.argtypes = [C.c_void_p, C.c_void_p]
         L.or_get_state.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.or_residual.argtypes = [C.c_void_p, C.c_int]
         L.or_step.argtypes = [C.c_void_p]
@@ -145,6 +147,11 @@ def make_params(joe_in: str) -> tuple[_Params, dict]:
     p.cfl_max = float(_after(cr, "MAX_CFL"))
     p.nsteps = int(get("NSTEPS", "100"))
     p.resid_energ_th = float(get("RESID_ENERG_TH", "0.0"))
+    # Spalart-Allmaras (the reference's RansTurbSA mix-in, chosen here by TURB_MODEL)
+    p.sa = 1 if get("TURB_MODEL", "NONE") == "SA" else 0
+    if p.sa:
+        p.sa_init = float(P["INITIAL_CONDITION_TURB"][0][1])
+        p.relax_scal = float(get("UNDER_RELAXATION_SCALARS", "0.2"))
     return p, P
 
 
@@ -177,7 +184,14 @@ class Oracle:
             else:
                 vals = [float(t) for t in toks[2:]]
                 zpar[i, :len(vals)] = vals
-        self._keep.update(zbc=zbc, zifa=zifa, zpar=zpar)
+        # "<zone>.nuSA = v": Dirichlet value of the SA scalar (scalarZoneIsDirichlet)
+        zscal = np.zeros((len(zl), 2), np.float64)
+        for i, (name, _k, f, l) in enumerate(zl):
+            key = name + ".nuSA"
+            if key in P:
+                zscal[i] = (1.0, float(P[key][0][1]))
+        self._keep.update(zbc=zbc, zifa=zifa, zpar=zpar, zscal=zscal)
+        self.nscal = 1 if self.params.sa else 0
         m = _Mesh()
         m.ncv, m.nfa, m.nfa_b, m.nnz = self.ncv, self.nfa, self.nfa_b, self.nnz
         m.cvofa = arr("CVOFA", np.int32)
@@ -195,6 +209,7 @@ class Oracle:
         m.zone_bc = zbc.ctypes.data
         m.zone_ifa = zifa.ctypes.data
         m.zone_par = zpar.ctypes.data
+        m.zone_scal = zscal.ctypes.data
         self._mesh = m
         self.ctx = L.or_ctx_create(C.byref(m), C.byref(self.params))
 
@@ -216,6 +231,10 @@ class Oracle:
         lib().or_get_state(self.ctx, rho.ctypes.data, rhou.ctypes.data, rhoE.ctypes.data)
         return rho, rhou, rhoE
 
+    def set_scalar(self, phi):
+        phi = np.ascontiguousarray(phi, np.float64)
+        lib().or_set_scalar(self.ctx, phi.ctypes.data)
+
     def residual(self, implicit: bool = True):
         lib().or_residual(self.ctx, 1 if implicit else 0)
 
@@ -226,7 +245,7 @@ class Oracle:
         return lib().or_step(self.ctx)
 
     def run(self, max_steps: int):
-        hist = np.zeros((max_steps, 5))
+        hist = np.zeros((max_steps, 5 + self.nscal))
         n = lib().or_run(self.ctx, max_steps, hist.ctypes.data)
         return n, hist[:n]
 

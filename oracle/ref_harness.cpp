@@ -7,11 +7,14 @@
 //   init  <Joe.in> <out.bin>              mesh arrays after Ugp/UgpWithCv init
 //   rhs   <Joe.in> <state.bin> <out.bin>  one implicit calcRhs on a given state
 //   solve <Joe.in> <out.bin> [nsnap]      JoeWithModels::run(); state history
+// rhs-sa / solve-sa run the same with the Spalart-Allmaras mix-in (RansTurbSA,
+// as joe.cpp's MyJoeSA); the state file may then carry NUSA.
 //
 // The dump format is a sequence of records:
 //   char name[32]; int32 kind (0 = f64, 1 = i32, 2 = bytes); int32 pad;
 //   int64 count; payload.
 #include "JoeWithModels.h"
+#include "TurbModel_SA.h"
 
 #include <chrono>
 #include <cstdio>
@@ -98,11 +101,18 @@ class RefJoe : public JoeWithModels {
     }
   }
 
+  // fields of a turbulence model mix-in (none here)
+  virtual void dumpModel(Dump &d) {}
+
   void dumpState(const char *suffix) {
     char n[32];
     snprintf(n, 32, "RHO%s", suffix);  snap->f64(n, rho, ncv);
     snprintf(n, 32, "RHOU%s", suffix); snap->f64(n, &rhou[0][0], 3LL * ncv);
     snprintf(n, 32, "RHOE%s", suffix); snap->f64(n, rhoE, ncv);
+    if (scalarTranspEqVector.size() > 0) {
+      snprintf(n, 32, "NUSA%s", suffix);
+      snap->f64(n, scalarTranspEqVector[0].phi, ncv);
+    }
   }
 
   void temporalHook() {
@@ -186,6 +196,11 @@ class RefJoe : public JoeWithModels {
       for (int i = 0; i < 3; i++) rhou[icv][i] = ru[3 * icv + i];
       rhoE[icv] = re[icv];
     }
+    const int nScal = scalarTranspEqVector.size();
+    if (nScal > 0) {
+      const double *nu = st.f64("NUSA", ncv);
+      for (int icv = 0; icv < ncv; icv++) scalarTranspEqVector[0].phi[icv] = nu[icv];
+    }
     calcStateVariables();
     calcMaterialProperties();
     setBC();
@@ -199,11 +214,40 @@ class RefJoe : public JoeWithModels {
     for (int noc = 0; noc < nbocv_s; noc++)
       for (int i = 0; i < 5; i++)
         for (int j = 0; j < 5; j++) A[noc][i][j] = 0.0;
+    // scalar right-hand sides and matrix rows, laid out as runBackwardEuler's
+    // getMem2D / getMem3D arrays ([nScal][ncv], [nScal][6][nbocv_s])
+    double **rhsScal = NULL, ***AScal = NULL;
+    if (nScal > 0) {
+      rhsScal = new double *[nScal];
+      AScal = new double **[nScal];
+      for (int s = 0; s < nScal; s++) {
+        rhsScal[s] = new double[ncv];
+        AScal[s] = new double *[6];
+        for (int k = 0; k < 6; k++) {
+          AScal[s][k] = new double[nbocv_s];
+          for (int noc = 0; noc < nbocv_s; noc++) AScal[s][k][noc] = 0.0;
+        }
+      }
+    }
 
     auto t0 = std::chrono::steady_clock::now();
-    calcRhs(RHSrho, RHSrhou, RHSrhoE, NULL, A, NULL, true);
+    calcRhs(RHSrho, RHSrhou, RHSrhoE, rhsScal, A, AScal, true);
     auto t1 = std::chrono::steady_clock::now();
     double secs = std::chrono::duration<double>(t1 - t0).count();
+    if (nScal > 0) {
+      ScalarTranspEq &eq = scalarTranspEqVector[0];
+      d.f64("NUSA", eq.phi, ncv);
+      d.f64("NUSA_BFA", eq.phi_bfa, nfa_b);
+      d.f64("GRAD_NUSA", &eq.grad_phi[0][0], 3LL * ncv);
+      d.f64("DIFF_NUSA", eq.diff, nfa);
+      d.f64("MUT_FA", mut_fa, nfa);
+      d.f64("RHS_NUSA", rhsScal[0], ncv);
+      std::vector<double> as(6LL * nbocv_s);
+      for (int k = 0; k < 6; k++)
+        for (int noc = 0; noc < nbocv_s; noc++) as[(size_t)k * nbocv_s + noc] = AScal[0][k][noc];
+      d.f64("ASCAL", as.data(), 6LL * nbocv_s);
+      dumpModel(d);
+    }
 
     dumpPrims(d);
     d.f64("LOCAL_DT", local_dt, ncv);
@@ -243,9 +287,41 @@ class RefJoe : public JoeWithModels {
     double zeroRelLS = getParam("LINEAR_SOLVER_NS_TRESHOLDS")->getDouble("REL_RESID");
     solveCoupledLinSysNS(dq, A, rhs, zeroAbsLS, zeroRelLS, maxIterLS);
     d.f64("DQ", &dq[0][0], 5LL * ncv);
+    if (nScal > 0) {
+      // the scalar half of that step (JoeWithModels.cpp:1054-1082), without the update
+      double underRelaxScalars = getDoubleParam("UNDER_RELAXATION_SCALARS", "0.2");
+      ScalarTranspEq &eq = scalarTranspEqVector[0];
+      double *dScal = new double[ncv_g];
+      for (int icv = 0; icv < ncv_g; icv++) dScal[icv] = 0.0;
+      for (int icv = 0; icv < ncv; ++icv) {
+        rhsScal[0][icv] *= underRelaxScalars;
+        int noc_f = nbocv_i[icv];
+        int noc_l = nbocv_i[icv + 1] - 1;
+        double tmp = cv_volume[icv] / (local_dt[icv]);
+        AScal[0][5][noc_f] += tmp;
+        for (int noc = noc_f; noc <= noc_l; noc++)
+          rhsScal[0][icv] = rhsScal[0][icv] - AScal[0][0][noc] * dq[nbocv_v[noc]][0] -
+                            AScal[0][1][noc] * dq[nbocv_v[noc]][1] - AScal[0][2][noc] * dq[nbocv_v[noc]][2] -
+                            AScal[0][3][noc] * dq[nbocv_v[noc]][3] - AScal[0][4][noc] * dq[nbocv_v[noc]][4];
+      }
+      solveLinSysScalar(dScal, AScal[0][5], rhsScal[0], eq.phiZero, eq.phiZeroRel, eq.phiMaxiter, eq.getName());
+      d.f64("DSCAL", dScal, ncv);
+      delete[] dScal;
+    }
 
     delete[] RHSrho; delete[] RHSrhou; delete[] RHSrhoE; delete[] A;
     delete[] rhs; delete[] dq;
+  }
+};
+
+// the reference's own Spalart-Allmaras case class (joe.cpp:286-296)
+class RefJoeSA : public RefJoe, public RansTurbSA {
+ public:
+  explicit RefJoeSA(char *name) : RefJoe(name), UgpWithCvCompFlow(name) {}
+  void dumpModel(Dump &d) {
+    d.f64("WALL_DIST", wallDist, ncv);
+    d.f64("STR_MAG", strMag, ncv);
+    d.f64("VORT_MAG", vortMag, ncv);
   }
 };
 
@@ -259,8 +335,11 @@ int main(int argc, char **argv) {
     return 2;
   }
   std::string mode = argv[1];
+  const bool with_sa = mode.size() > 3 && mode.compare(mode.size() - 3, 3, "-sa") == 0;
+  if (with_sa) mode.resize(mode.size() - 3);
   try {
-    RefJoe joe(argv[2]);
+    RefJoe *jp = with_sa ? new RefJoeSA(argv[2]) : new RefJoe(argv[2]);
+    RefJoe &joe = *jp;
     if (mode == "init") {
       joe.initializeFromRestartFile(joe.getStringParam("RESTART"));
       Dump d(argv[3]);
@@ -285,6 +364,7 @@ int main(int argc, char **argv) {
       fprintf(stderr, "unknown mode %s\n", mode.c_str());
       return 2;
     }
+    delete jp;
   } catch (int e) {
     fprintf(stderr, "reference threw %d\n", e);
     MPI_Finalize();

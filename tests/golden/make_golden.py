@@ -32,7 +32,9 @@ REF = os.path.join(ROOT, "oracle", "_ref", "joe_ref_dump")
 MESH_KEYS = ["COUNTS", "X_NO", "NOOFA_I", "NOOFA_V", "CVOFA", "FAOCV_I", "FAOCV_V", "NBOCV_I", "NBOCV_V", "X_FA",
              "FA_NORMAL", "X_CV", "CV_VOLUME", "NBOCV_LSG", "BFA_LSG", "ZONES", "ZONE_NAMES"]
 RHS_KEYS = ["PRESS", "VEL", "RHO_BFA", "VEL_BFA", "P_BFA", "T_BFA", "H_BFA", "LOCAL_DT", "GRAD_RHO", "GRAD_P",
-            "GRAD_U", "ALPHA_RHO", "RHS_RHO", "RHS_RHOU", "RHS_RHOE", "A", "DQ", "MUL_FA", "GRAD_ENTHALPY"]
+            "GRAD_U", "ALPHA_RHO", "RHS_RHO", "RHS_RHOU", "RHS_RHOE", "A", "DQ", "MUL_FA", "GRAD_ENTHALPY",
+            "NUSA_BFA", "GRAD_NUSA", "DIFF_NUSA", "MUT_FA", "RHS_NUSA", "ASCAL", "DSCAL", "WALL_DIST", "STR_MAG",
+            "VORT_MAG"]
 
 # viscous variants: Joe.in lines that replace "MU_REF = 0.0"
 VISC = {
@@ -56,7 +58,13 @@ RHS_CASES = [("bump24", "HLLC", {}), ("bump24", "ROE", {}), ("bump24", "AUSM", {
              ("box4", "AUSM", {"visc": "pl", "bcs": cases.BOX_BCS_ALT, "symm_bc": "NEW_NUMERIC",
                                "grad": "GREENGAUSS"}),
              ("box4", "HLLC", {"visc": "su", "second_order": False}),
-             ("box4", "HLLC", {"visc": "nl"})]
+             ("box4", "HLLC", {"visc": "nl"}),
+             # Spalart-Allmaras (segregated nuSA)
+             ("bump24", "HLLC", {"visc": "pl", "sa": True}),
+             ("box4", "HLLC", {"visc": "su", "sa": True}),
+             ("box4", "HLLC", {"visc": "pl", "sa": True, "bcs": cases.BOX_BCS_ALT, "symm_bc": "ANALYTICAL",
+                               "grad": "GREENGAUSS"}),
+             ("box4", "HLLC", {"visc": "nl", "sa": True, "second_order": False})]
 
 
 def run_ref(args, cwd):
@@ -67,6 +75,8 @@ def run_ref(args, cwd):
 
 def tag(kw):
     t = "_visc_" + kw["visc"] if "visc" in kw else ""
+    if kw.get("sa"):
+        t += "_sa"
     if "bcs" in kw:
         t += "_alt_" + {"NEW_NUMERIC": "new", "ANALYTICAL": "ana"}[kw["symm_bc"]]
     if kw.get("grad") == "GREENGAUSS":
@@ -84,37 +94,49 @@ def main():
         for name, scheme, kw in RHS_CASES:
             kw = dict(kw)
             visc = kw.pop("visc", None)
+            with_sa = kw.pop("sa", False)
             mm, joe = MESHES[name](scheme, **kw)
             if visc:
                 kw["visc"] = visc
                 joe = joe.replace("MU_REF = 0.0", VISC[visc])
+            if with_sa:
+                kw["sa"] = True
+                joe = cases.with_sa(joe, 0.005, cases.SA_BUMP if name.startswith("bump") else cases.SA_BOX)
             fn = "box.msh" if name.startswith("box") else "bump.msh"
             meshgen.write_msh(mm, os.path.join(wd, fn))
             with open(os.path.join(wd, "Joe.in"), "w") as f:
                 f.write(joe)
             rho, rhou, rhoE = cases.random_state(mm.ncv, seed=5)
-            jmdump.write(os.path.join(wd, "state.bin"), {"RHO": rho, "RHOU": rhou.reshape(-1), "RHOE": rhoE})
-            run_ref(["rhs", "Joe.in", "state.bin", "out.bin"], wd)
+            state = {"RHO": rho, "RHOU": rhou.reshape(-1), "RHOE": rhoE}
+            if with_sa:
+                state["NUSA"] = np.random.default_rng(11).uniform(0.001, 0.01, mm.ncv)
+            jmdump.write(os.path.join(wd, "state.bin"), state)
+            run_ref(["rhs-sa" if with_sa else "rhs", "Joe.in", "state.bin", "out.bin"], wd)
             d = jmdump.read(os.path.join(wd, "out.bin"))
             if name not in done_mesh:
                 np.savez_compressed(os.path.join(HERE, f"mesh_{name}.npz"), **{k: d[k] for k in MESH_KEYS})
                 done_mesh.add(name)
             out = {k: d[k] for k in RHS_KEYS if k in d}
             out.update(RHO=rho, RHOU=rhou.reshape(-1), RHOE=rhoE, JOE_IN=np.frombuffer(joe.encode(), np.uint8))
+            if with_sa:
+                out["NUSA"] = state["NUSA"]
             np.savez_compressed(os.path.join(HERE, f"rhs_{name}_{scheme.lower()}{tag(kw)}.npz"), **out)
             print("wrote", name, scheme, kw)
 
         # steady-loop trajectories of JoeWithModels::run (uniform initial state)
-        for name, nx, ny, nsteps in [("bump24", 24, 8, 60), ("bump96", 96, 32, 300), ("bump24visc", 24, 8, 60)]:
+        for name, nx, ny, nsteps in [("bump24", 24, 8, 60), ("bump96", 96, 32, 300), ("bump24visc", 24, 8, 60),
+                                     ("bump24sa", 24, 8, 60)]:
             mm, joe = cases.bump_case(nx, ny, nsteps=nsteps)
-            if name.endswith("visc"):
+            if name.endswith(("visc", "sa")):
                 joe = joe.replace("MU_REF = 0.0", VISC["pl"])
+            if name.endswith("sa"):
+                joe = cases.with_sa(joe, 0.006, cases.SA_BUMP)
             meshgen.write_msh(mm, os.path.join(wd, "bump.msh"))
             with open(os.path.join(wd, "Joe.in"), "w") as f:
                 f.write(joe)
-            run_ref(["solve", "Joe.in", "solve.bin", "5"], wd)
+            run_ref(["solve-sa" if name.endswith("sa") else "solve", "Joe.in", "solve.bin", "5"], wd)
             d = jmdump.read(os.path.join(wd, "solve.bin"))
-            keep = {k: v for k, v in d.items() if k.split("@")[0] in ("RHO", "RHOU", "RHOE", "STEP")
+            keep = {k: v for k, v in d.items() if k.split("@")[0] in ("RHO", "RHOU", "RHOE", "NUSA", "STEP")
                     and (k.endswith(("@1", "@2", "@3", "@4", "@5")) or "@" not in k)}
             keep["JOE_IN"] = np.frombuffer(joe.encode(), np.uint8)
             np.savez_compressed(os.path.join(HERE, f"solve_{name}.npz"), **keep)

@@ -22,6 +22,9 @@ RHS_FIXTURES = sorted(os.path.basename(p)[4:-4] for p in glob.glob(os.path.join(
 FIELDS = ["PRESS", "VEL", "RHO_BFA", "VEL_BFA", "P_BFA", "T_BFA", "H_BFA", "LOCAL_DT", "GRAD_RHO", "GRAD_P",
           "GRAD_U", "ALPHA_RHO", "RHS_RHO", "RHS_RHOU", "RHS_RHOE", "A"]
 VISC_FIELDS = ["MUL_FA", "GRAD_ENTHALPY"]
+# Spalart-Allmaras: model fields, the nuSA boundary values, gradient,
+# diffusivity, eddy viscosity, residual and matrix rows AScal[6][nnz]
+SA_FIELDS = ["WALL_DIST", "STR_MAG", "VORT_MAG", "MUT_FA", "NUSA_BFA", "GRAD_NUSA", "DIFF_NUSA", "RHS_NUSA", "ASCAL"]
 
 
 def load(name):
@@ -54,6 +57,10 @@ def test_fixture_set_complete():
     visc = [f for f in RHS_FIXTURES if "visc" in f]
     assert any("_pl" in f for f in visc) and any("_su" in f for f in visc) and any("_nl" in f for f in visc)
     assert any("first" in f for f in visc)
+    # Spalart-Allmaras on both geometries, LS and GG, first order, every BC kind
+    sa = [f for f in RHS_FIXTURES if "_sa" in f]
+    assert len(sa) >= 4 and any("gg" in f for f in sa) and any("first" in f for f in sa)
+    assert any(f.startswith("bump") for f in sa) and any("alt" in f for f in sa)
 
 
 @pytest.mark.parametrize("fixture", RHS_FIXTURES)
@@ -61,9 +68,12 @@ def test_residual_bit_exact(fixture):
     g = load("rhs_" + fixture)
     o = Oracle(mesh_for(fixture), joe_text(g))
     o.set_state(g["RHO"], g["RHOU"], g["RHOE"])
+    sa = "_sa" in fixture
+    if sa:
+        o.set_scalar(g["NUSA"])
     o.residual(True)
     first = "first" in fixture
-    for k in FIELDS + (VISC_FIELDS if "visc" in fixture else []):
+    for k in FIELDS + (VISC_FIELDS if "visc" in fixture else []) + (SA_FIELDS if sa else []):
         if k not in g:
             assert first and k.startswith(("GRAD", "ALPHA")), k
             continue
@@ -73,14 +83,40 @@ def test_residual_bit_exact(fixture):
         assert nbad == 0, f"{fixture}:{k}: {nbad} of {got.size} differ; max |d| {np.max(np.abs(got - g[k])):.3e}"
     o.linear_solve()
     assert np.array_equal(o.get("DQ"), g["DQ"]), fixture
+    if sa:  # the scalar half of the step: relaxation, coupling columns, Jacobi BiCGStab
+        assert np.array_equal(o.get("DSCAL"), g["DSCAL"]), fixture
 
 
-@pytest.mark.parametrize("name", ["bump24", "bump96", "bump24visc"])
+def test_sa_fields_physical():
+    # what the SA closure fixes independently of the reference: wall distance
+    # is positive and bounded by the nearest wall-face centroid distance, the
+    # eddy viscosity is zero on walls and non-negative elsewhere, strain and
+    # vorticity magnitudes are non-negative
+    g = load("rhs_bump24_hllc_visc_pl_sa")
+    m = mesh_for("bump24_hllc_visc_pl_sa")
+    o = Oracle(m, joe_text(g))
+    o.set_state(g["RHO"], g["RHOU"], g["RHOE"])
+    o.set_scalar(g["NUSA"])
+    o.residual(True)
+    from oracle import jmdump
+    walls = [(f, l) for name, k, f, l in jmdump.zones(m) if k == 1 and name == "bump_wall"]
+    wf = np.concatenate([np.arange(f, l + 1) for f, l in walls])
+    x_cv = m["X_CV"].reshape(-1, 3)[: o.ncv]
+    x_fa = m["X_FA"].reshape(-1, 3)
+    dmin = np.min(np.linalg.norm(x_cv[:, None, :] - x_fa[wf][None, :, :], axis=2), axis=1)
+    wd = o.get("WALL_DIST")
+    assert np.all(wd > 0.0) and np.all(wd <= dmin * (1 + 1e-12))
+    mut = o.get("MUT_FA")
+    assert np.all(mut[wf] == 0.0) and np.all(mut >= 0.0)
+    assert np.all(o.get("STR_MAG") >= 0.0) and np.all(o.get("VORT_MAG") >= 0.0)
+
+
+@pytest.mark.parametrize("name", ["bump24", "bump96", "bump24visc", "bump24sa"])
 def test_run_trajectory_bit_exact(name):
     g = load("solve_" + name)
     joe = joe_text(g)
     P = parse_joe_in(joe)
-    m = load("mesh_" + name.replace("visc", ""))
+    m = load("mesh_" + name.replace("visc", "").replace("sa", ""))
     o = Oracle(m, joe)
     ncv = int(m["COUNTS"][3])
     # JoeWithModels::initialHook uniform state (JoeWithModels.cpp:180-205)
@@ -97,12 +133,16 @@ def test_run_trajectory_bit_exact(name):
         assert np.array_equal(rho, g[f"RHO@{k}"]), (name, k)
         assert np.array_equal(rhou.reshape(-1), g[f"RHOU@{k}"].reshape(-1)), (name, k)
         assert np.array_equal(rhoE, g[f"RHOE@{k}"]), (name, k)
+        if name.endswith("sa"):
+            assert np.array_equal(o.get("NUSA"), g[f"NUSA@{k}"]), (name, k)
     nsteps = int(g["STEP"][0])
     n, _ = o.run(nsteps - 5)
     assert n == nsteps - 5
     rho, rhou, rhoE = o.state()
     assert np.array_equal(rho, g["RHO"]) and np.array_equal(rhoE, g["RHOE"]), name
     assert np.array_equal(rhou.reshape(-1), g["RHOU"].reshape(-1)), name
+    if name.endswith("sa"):
+        assert np.array_equal(o.get("NUSA"), g["NUSA"]), name
 
 
 def test_uniform_state_matches_case_definition():
