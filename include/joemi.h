@@ -101,8 +101,14 @@ void jm_ctx_destroy(jm_ctx *c);
  * Setting the state starts a new solve: the step counter and CFL restart. */
 jm_status jm_set_state(jm_ctx *c, const double *rho, const double *rhou, const double *rhoE);
 jm_status jm_get_state(jm_ctx *c, double *rho, double *rhou, double *rhoE);
-/* JoeWithModels::initialHook uniform initial condition (JoeWithModels.cpp:156-186) */
+/* JoeWithModels::initialHook uniform initial condition (JoeWithModels.cpp:156-186);
+ * with TURB_MODEL = SA also nuSA = INITIAL_CONDITION_TURB (:219-240) */
 jm_status jm_init_uniform(jm_ctx *c);
+/* a transported scalar's field in reference numbering, [ncv]: stands in for
+ * writing ScalarTranspEq::phi (UgpWithCv.h:50-113) of "nuSA", the one scalar
+ * of the Spalart-Allmaras mix-in (RansTurbSA, TurbModel_SA.h:31). The
+ * context must have been created with TURB_MODEL = SA. */
+jm_status jm_set_scalar(jm_ctx *c, const char *name, const double *phi);
 
 /* calcStateVariables + setBC + calcDt + calcRhs (JoeWithModels.cpp:943-1005,
  * 2564-3589) on the resident state; with JM_RHS_IMPLICIT the block Jacobian
@@ -111,7 +117,10 @@ jm_status jm_calc_rhs(jm_ctx *c, int flags);
 
 /* copy a result out (synchronises the stream): RHS_RHO RHS_RHOU RHS_RHOE A
  * LOCAL_DT GRAD_RHO GRAD_P GRAD_U ALPHA_RHO VEL PRESS RHO_BFA VEL_BFA P_BFA
- * T_BFA H_BFA DQ RESID; out may be NULL to query *count. */
+ * T_BFA H_BFA DQ RESID; with MU_REF > 0 GRAD_ENTHALPY; with TURB_MODEL = SA
+ * NUSA NUSA_BFA GRAD_NUSA WALL_DIST STR_MAG VORT_MAG RHS_NUSA ASCAL
+ * ([6][nbocv_s], runBackwardEuler's AScal[0]) DSCAL, and RESID has 6 entries.
+ * out may be NULL to query *count. */
 jm_status jm_get(jm_ctx *c, const char *name, double *out, int64_t *count);
 
 /* One pass of the runBackwardEuler loop (JoeWithModels.cpp:975-1158): CFL
@@ -125,6 +134,8 @@ typedef struct {
   double cfl;          /* CFL used */
   double dt_min;       /* minimum local time step */
   double resid[5];     /* L1 residuals of rho, rhou(3), rhoE */
+  double resid_scal;   /* L1 residual of nuSA over UNDER_RELAXATION (TURB_MODEL = SA; else 0) */
+  int scal_iters;      /* its Jacobi BiCGStab iterations (solveCvScalarBcgstab) */
 } jm_step_info;
 jm_status jm_step(jm_ctx *c, jm_step_info *info);
 /* The same step driven by the device: CFL ramp and step counter on the

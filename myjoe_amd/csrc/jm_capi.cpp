@@ -219,6 +219,15 @@ jm_status jm_set_state(jm_ctx *c, const double *rho, const double *rhou, const d
   });
 }
 
+jm_status jm_set_scalar(jm_ctx *c, const char *name, const double *phi) {
+  return guard([&] {
+    if (!c || !name || !phi) throw std::exception();
+    if (std::string(name) != "nuSA") throw std::invalid_argument(std::string("no transported scalar ") + name);
+    c->c->set_scalar(phi);
+    return JM_OK;
+  });
+}
+
 jm_status jm_get_state(jm_ctx *c, double *rho, double *rhou, double *rhoE) {
   return guard([&] {
     if (!c || !rho || !rhou || !rhoE) throw std::exception();
@@ -259,6 +268,8 @@ static void fill_info(const jm::StepInfo &s, jm_step_info *info) {
   info->cfl = s.cfl;
   info->dt_min = s.dt_min;
   for (int k = 0; k < 5; k++) info->resid[k] = s.resid[k];
+  info->resid_scal = s.resid_s;
+  info->scal_iters = s.scal_iters;
 }
 
 jm_status jm_step_begin(jm_ctx *c, int flags) {

@@ -53,11 +53,17 @@ struct SolverParams {
   int visc_mode = 0;
   bool visc_limiter = true;
   int grad_kernel = kResidCell, flux_kernel = kResidEntry;
+  // Spalart-Allmaras as one segregated scalar nuSA (TURB_MODEL = SA): initial
+  // value (INITIAL_CONDITION_TURB) and UNDER_RELAXATION_SCALARS
+  bool sa = false;
+  double sa_init = 0.0, relax_scal = 0.2;
 };
 
 struct ZoneBc {
   int bc = 0;
   double par[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int skind = 0;  // 1: "<zone>.nuSA = sval" (Dirichlet on inflow faces)
+  double sval = 0.0;
 };
 
 void parse_params(const std::string &joe_in, const Mesh &m, SolverParams &p, std::vector<ZoneBc> &zbc);
@@ -66,6 +72,8 @@ struct StepInfo {
   int step = 0, lin_iters = 0, lin_status = 0;
   double cfl = 0, dt_min = 0;
   double resid[5] = {0, 0, 0, 0, 0};
+  double resid_s = 0;  // L1 of the nuSA residual (Spalart-Allmaras)
+  int scal_iters = 0;  // its Jacobi BiCGStab iterations
 };
 
 class Ctx {
@@ -76,6 +84,8 @@ class Ctx {
   Ctx &operator=(const Ctx &) = delete;
 
   void set_state(const double *rho, const double *rhou, const double *rhoE);
+  // the transported scalar nuSA (Spalart-Allmaras), reference numbering
+  void set_scalar(const double *phi);
   void get_state(double *rho, double *rhou, double *rhoE);
   void init_uniform();
   void calc_rhs(bool implicit);
@@ -104,6 +114,7 @@ class Ctx {
   StepInfo device_step();
   // device bytes a context for this mesh holds once it has stepped
   static size_t footprint(const Mesh &m, int precision);
+  static size_t footprint(const Mesh &m, int precision, bool sa);
   int run(int max_steps, double *hist, bool graph = false);
   // doneSolver (JoeWithModels.h:42-95) after a step, plus RESID_ENERG_REL
   bool done_solver(const StepInfo &s);

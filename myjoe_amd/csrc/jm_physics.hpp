@@ -24,10 +24,12 @@ __device__ __forceinline__ double dot3(const double *a, const double *b) {
   return d;
 }
 
-// HLLC flux, UgpWithCvCompFlow.cpp:120-235 (no scalars, k = 0, surfVeloc = 0)
+// HLLC flux, UgpWithCvCompFlow.cpp:120-235 (k = 0, surfVeloc = 0). With Fs,
+// one transported scalar: FrhoScal = Frho * (upwind side's value) * area.
 __device__ __forceinline__ void flux_hllc(double F[5], double rhoL, const double *uL, double pL, double rhoR,
                                           const double *uR, double pR, double gam, double area,
-                                          const double *nVec) {
+                                          const double *nVec, double phiL = 0.0, double phiR = 0.0,
+                                          double *Fs = nullptr) {
   const double gammaL = gam, gammaR = gam, kL = 0.0, kR = 0.0, surfv = 0.0;
   double unL = dot3(uL, nVec) - surfv;
   double unLr = unL - surfv;
@@ -103,6 +105,7 @@ __device__ __forceinline__ void flux_hllc(double F[5], double rhoL, const double
   F[2] = f2 * area;
   F[3] = f3 * area;
   F[4] = f4 * area;
+  if (Fs) *Fs = f0 * (sM >= 0.0 ? phiL : phiR) * area;
 }
 
 // Roe flux without entropy fix, UgpWithCvCompFlow.cpp:3281-3380; returns the
@@ -382,6 +385,71 @@ __device__ __noinline__ void jac_hllc(double J[5][5], int which, double rhoL, co
   for (int i = 0; i < 5; i++)
 #pragma unroll
     for (int j = 0; j < 5; j++) J[i][j] *= area;
+}
+
+// Scalar row of one side of the HLLC Jacobian pair (A_L_Scal / A_R_Scal,
+// UgpWithCvCompFlow.cpp:296-311, 320-324, 389-405, 420-435, 504-519,
+// 534-550, 561-577, 586-590): d(rho phi un)/dQ is row 0 of the 5x5 block
+// times the upwind scalar, less the upwind star mass flux per unit density
+// ("FrhostarOverrho") in column 0, which is also d/d(rho phi); the
+// downwind side has no d/d(rho phi), and a side outside the fan is zero.
+__device__ __noinline__ void jac_hllc_scal(double S[6], int which, double rhoL, const double *uL, double pL,
+                                           double rhoR, const double *uR, double pR, double gam, double area,
+                                           const double *nVec, double phiL, double phiR) {
+  double J[5][5];
+  jac_hllc(J, which, rhoL, uL, pL, rhoR, uR, pR, gam, area, nVec);
+  // the wave speeds again, with the same operations as jac_hllc
+  const double gammaL = gam, gammaR = gam;
+  double unL = dot3(uL, nVec);
+  double uLuL = dot3(uL, uL);
+  double cL = sqrt(gammaL * pL / rhoL);
+  double unR = dot3(uR, nVec);
+  double uRuR = dot3(uR, uR);
+  double cR = sqrt(gammaR * pR / rhoR);
+  double Rrho = sqrt(rhoR / rhoL);
+  double tmp = 1.0 / (1.0 + Rrho);
+  double velRoe[3];
+  for (int i = 0; i < 3; i++) velRoe[i] = tmp * (uL[i] + uR[i] * Rrho);
+  double uRoe = dot3(velRoe, nVec);
+  double gamPdivRho = tmp * ((gammaL * pL / rhoL + 0.5 * (gammaL - 1.0) * uLuL) +
+                             (gammaR * pR / rhoR + 0.5 * (gammaR - 1.0) * uRuR) * Rrho);
+  double cRoe = sqrt(gamPdivRho - ((gammaL + gammaR) * 0.5 - 1.0) * 0.5 * dot3(velRoe, velRoe));
+  double sL = Dev::mn(uRoe - cRoe, unL - cL);
+  double sR = Dev::mx(uRoe + cRoe, unR + cR);
+  double sM = (pL - pR - rhoL * unL * (sL - unL) + rhoR * unR * (sR - unR)) / (rhoR * (sR - unR) - rhoL * (sL - unL));
+  int k;
+  double fso;
+  if (sM >= 0.0) {
+    if (sL > 0.0) {
+      k = 0;
+      fso = unL * area;
+    } else {
+      k = 1;
+      fso = 1.0 / (sL - sM) * (sL - unL) * sM * area;
+    }
+  } else {
+    if (sR >= 0.0) {
+      k = 2;
+      fso = 1.0 / (sR - sM) * (sR - unR) * sM * area;
+    } else {
+      k = 3;
+      fso = unR * area;
+    }
+  }
+  const bool upL = k <= 1;
+  const double phi = upL ? phiL : phiR;
+  const bool zero = which == 0 ? (k == 3) : (k == 0);
+  const bool upwind = which == 0 ? upL : !upL;
+  if (zero) {
+    for (int i = 0; i <= 5; i++) S[i] = 0.0;
+  } else if (upwind) {
+    S[0] = (J[0][0] - fso) * phi;
+    for (int i = 1; i < 5; i++) S[i] = J[0][i] * phi;
+    S[5] = fso;
+  } else {
+    for (int i = 0; i < 5; i++) S[i] = J[0][i] * phi;
+    S[5] = 0.0;
+  }
 }
 
 // One side of the Roe Jacobian pair, UgpWithCvCompFlow.cpp:3382-3559

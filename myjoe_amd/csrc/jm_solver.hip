@@ -36,14 +36,16 @@ namespace jm {
 
 struct ZoneDev {
   int bc;
-  int pad[3];
+  int skind;  // Spalart-Allmaras nuSA boundary: 0 zero gradient, 1 "<zone>.nuSA = sval"
+  int pad[2];
   double par[8];
+  double sval, pad2;
 };
 
 struct DMesh {
   int ncv, nfa, nfa_b, nnz;
   const double *fgeo;     // [nfa][8]
-  const double *fnorm;    // [nfa][3] area-weighted normal (Green-Gauss only)
+  const double *fnorm;    // [nfa][3] area-weighted normal (Green-Gauss or Spalart-Allmaras)
   const int *cfl_i;       // [ncv+1]
   const int2 *cfl;        // [entries]
   const int *cfl_noc;     // [entries] off-diagonal block; bit 30 = accumulate
@@ -60,6 +62,7 @@ struct DMesh {
   const double *cfl_r0;   // [entries][3] x_fa - x_cv
   const int *wave_c0;     // [nwave+1] first cell of each 64-lane wavefront
   int nwave;
+  const int *foc;         // [entries] entry offset of the cell's k-th face in faocv order (Spalart-Allmaras)
 };
 
 struct DParams {
@@ -68,7 +71,14 @@ struct DParams {
   // viscous terms (MU_REF > 0): viscosity law, conductivity, gradient limiting
   int visc, visc_mode, visc_limiter, pad;
   double mu_ref, mu_power_law, sl_tref, sl_sref, T_ref, pr, pr_turb;
+  // Spalart-Allmaras: nuSA [ncv] and per-cell model fields [ncv][8] =
+  // grad nuSA (3), face-averaged mul, strain and vorticity magnitudes, wall distance
+  int sa, pad2;
+  const double *sa_phi, *sa_aux;
 };
+
+// per-cell Spalart-Allmaras fields in DParams::sa_aux
+enum SaAux { kAuxGrad = 0, kAuxMue = 3, kAuxStr = 4, kAuxVort = 5, kAuxWall = 6, kAuxN = 8 };
 
 // per-solve scalars of the BiCGStab and of reductions (device resident)
 struct SolverScalars {
@@ -76,6 +86,7 @@ struct SolverScalars {
   double res0_max, res_max;
   double dt_min;
   double resid[5];
+  double resid_s;  // L1 of the transported scalar's residual (Spalart-Allmaras)
   int iter, done, maxiter, pad;
   double zero_abs, zero_rel;
   // pseudo-time step counter and CFL of the device-driven (graph) step
@@ -241,6 +252,13 @@ __global__ void __launch_bounds__(kBlock) k_bc(DMesh m, DParams P, const double 
       break;
   }
   if (!h_set) h_b = GAMMA * R_gas / (GAMMA - 1.0) * T_b;
+  // setScalarBC (Scalars.cpp:289-360): a "<zone>.nuSA" value on inflow faces
+  // (vel_bfa . fa_normal <= 1e-8), the cell value otherwise
+  double phi_b = 0.0;
+  if (P.sa) {
+    const double *fa_n = m.fnorm + 3 * (size_t)f;
+    phi_b = (z.skind && !(dot3(vb, fa_n) > 1.0e-8)) ? z.sval : P.sa_phi[c];
+  }
   double *o = bfa + 8 * (size_t)f;
   o[0] = p_b / (R_gas * T_b);
   o[1] = vb[0];
@@ -249,7 +267,7 @@ __global__ void __launch_bounds__(kBlock) k_bc(DMesh m, DParams P, const double 
   o[4] = p_b;
   o[5] = T_b;
   o[6] = h_b;
-  o[7] = 0.0;
+  o[7] = phi_b;
 }
 
 // smooth Barth-Jespersen factor of one face (UgpWithCv.h:913-925, BJ undefined)
@@ -892,9 +910,10 @@ __device__ __forceinline__ double flux_cell(const DMesh &m, const DParams &P, co
 
 // ---------------------------------------------------------------------------
 // Laminar viscous face flux, addViscFlux (UgpWithCvCompFlow.cpp:1521-1763)
-// with mut = 0, no turbulence model (nonLinear = rij = kine = 0) and
-// SIMPLEX_VISCOUS = NO; the zero turbulent terms are kept so that signed
-// zeros come out as in the reference.
+// with the eddy viscosity mut of the Spalart-Allmaras model (0 without a
+// turbulence model), nonLinear = rij = kine = 0 and SIMPLEX_VISCOUS = NO;
+// zero turbulent terms are kept so that signed zeros come out as in the
+// reference.
 // ---------------------------------------------------------------------------
 struct VSide {
   double rho, u[3], h, T;
@@ -911,10 +930,10 @@ struct VFace {
   double keff, alpha;
 };
 
-__device__ __noinline__ void visc_face(const DParams &P, const VSide &s0, const VSide &s1, double mul,
+__device__ __noinline__ void visc_face(const DParams &P, const VSide &s0, const VSide &s1, double mul, double mut,
                                        const double *u_fa, double area, const double *nVec, double smag,
                                        const double *sVec, double Frhou[3], double &FrhoE, VFace &vf) {
-  const double mut = 0.0, nonL = 0.0;
+  const double nonL = 0.0;
   const double alpha = dot3(nVec, sVec);
   double guf[3][3];
   for (int i = 0; i < 3; i++)
@@ -950,8 +969,8 @@ __device__ __noinline__ void visc_face(const DParams &P, const VSide &s0, const 
 // one half of the viscous Jacobian (A0: sgn -1 with side 0's state, A1: +1
 // with side 1's), dFdU x dUdQ (:1662-1762, :712-745); rows 1-4 only
 __device__ __noinline__ void visc_jac(const DParams &P, double J[5][5], double sgn, const VSide &s, double mul,
-                                      const VFace &vf, const double *u_fa, double area, double smag) {
-  const double mut = 0.0, kine = 0.0;
+                                      double mut, const VFace &vf, const double *u_fa, double area, double smag) {
+  const double kine = 0.0;
   const double gam = P.gamma, R = P.R;
   double f11 = sgn * (mul + mut) * vf.alpha / smag * area;
   double f22 = sgn * (mul + mut) * vf.alpha / smag * area;
@@ -1025,7 +1044,7 @@ __device__ __forceinline__ VSide vside_bfa(const double *bb, const double *gv) {
 // (calcMaterialProperties, UgpWithCvCompFlow.h:1700-1750)
 __device__ __forceinline__ void visc_internal_geom(const DParams &P, const double *x0, const double *x1,
                                                    const double *xfa, double T0, double T1, double sVec[3],
-                                                   double &smag, double &mul) {
+                                                   double &smag, double &mul, double &w0, double &w1) {
   for (int i = 0; i < 3; i++) sVec[i] = x1[i] - x0[i];
   smag = sqrt(dot3(sVec, sVec));
   for (int i = 0; i < 3; i++) sVec[i] /= smag;
@@ -1034,8 +1053,35 @@ __device__ __forceinline__ void visc_internal_geom(const DParams &P, const doubl
     dx0[i] = xfa[i] - x0[i];
     dx1[i] = xfa[i] - x1[i];
   }
-  const double w0 = sqrt(dot3(dx0, dx0)), w1 = sqrt(dot3(dx1, dx1));
+  w0 = sqrt(dot3(dx0, dx0));
+  w1 = sqrt(dot3(dx1, dx1));
   mul = mu_lam(P, (w1 * T0 + w0 * T1) / (w0 + w1));
+}
+
+// ---------------------------------------------------------------------------
+// Spalart-Allmaras (RansTurbSA, TurbModel_SA.h) as one segregated scalar nuSA
+// ---------------------------------------------------------------------------
+// model constants, getDoubleParam defaults (TurbModel_SA.h:20-27)
+constexpr double kSaCv1_3 = 357.911, kSaCw1 = 3.239067817, kSaCw2 = 0.3, kSaCw3_6 = 64.0, kSaSigma = 0.6666666,
+                 kSaCb1 = 0.1355, kSaCb2 = 0.622, kSaKarman2 = 0.1681;
+// nuSA transport settings (TurbModel_SA.h:31-37)
+constexpr double kSaPhiZero = 1.0e-9, kSaPhiZeroRel = 1.0e-2, kSaLower = 1.0e-10, kSaUpper = 1.0;
+constexpr int kSaMaxIter = 1000;
+
+// eddy viscosity from rho nuSA and the laminar viscosity (calcRansTurbViscMuet,
+// TurbModel_SA.h:82-99 internal faces, :112-120 non-wall boundary faces)
+__device__ __forceinline__ double sa_mut(double muSA, double mul) {
+  double chi_3 = pow(muSA / mul, 3.0);
+  double fv1 = chi_3 / (chi_3 + kSaCv1_3);
+  return Dev::mx(muSA * fv1, 0.0);
+}
+
+// the same at an internal face from distance-weighted rho and nuSA (icv0, icv1 order)
+__device__ __forceinline__ double sa_mut_face(double w0, double w1, double rho0, double rho1, double nu0, double nu1,
+                                              double mul) {
+  double rho_fa = (w1 * rho0 + w0 * rho1) / (w0 + w1);
+  double nuSA_fa = (w1 * nu0 + w0 * nu1) / (w0 + w1);
+  return sa_mut(rho_fa * nuSA_fa, mul);
 }
 
 // ---------------------------------------------------------------------------
@@ -1159,23 +1205,28 @@ __global__ void __launch_bounds__(kBlock) k_jac(DMesh m, DParams P, const double
         const VSide vn = vside_cell(P, pn, gradv + 12 * (size_t)en.y);
         const bool s0 = en.x >= 0;
         const VSide &a = s0 ? vo : vn, &b = s0 ? vn : vo;
-        double sV[3], smag, mul;
-        visc_internal_geom(P, s0 ? pc + 5 : pn + 5, s0 ? pn + 5 : pc + 5, xfa, a.T, b.T, sV, smag, mul);
+        double sV[3], smag, mul, w0, w1;
+        visc_internal_geom(P, s0 ? pc + 5 : pn + 5, s0 ? pn + 5 : pc + 5, xfa, a.T, b.T, sV, smag, mul, w0, w1);
+        double mut = 0.0;
+        if (P.sa) {
+          const double nuo = P.sa_phi[c], nun = P.sa_phi[en.y];
+          mut = sa_mut_face(w0, w1, a.rho, b.rho, s0 ? nuo : nun, s0 ? nun : nuo, mul);
+        }
         const double ufa[3] = {0.5 * (a.u[0] + b.u[0]), 0.5 * (a.u[1] + b.u[1]), 0.5 * (a.u[2] + b.u[2])};
-        visc_face(P, a, b, mul, ufa, area, n, smag, sV, Fu, FE, vf);
+        visc_face(P, a, b, mul, mut, ufa, area, n, smag, sV, Fu, FE, vf);
         T *off = A + 25 * (size_t)(m.cfl_noc[e] & 0x3fffffff);
         if (s0) {  // A[noc00] -= A0, A[noc01] -= A1
-          visc_jac(P, J, -1.0, a, mul, vf, ufa, area, smag);
+          visc_jac(P, J, -1.0, a, mul, mut, vf, ufa, area, smag);
           for (int i = 1; i < 5; i++)
             for (int j = 0; j < 5; j++) D[i][j] -= J[i][j];
-          visc_jac(P, J, 1.0, b, mul, vf, ufa, area, smag);
+          visc_jac(P, J, 1.0, b, mul, mut, vf, ufa, area, smag);
           for (int i = 1; i < 5; i++)
             for (int j = 0; j < 5; j++) off[5 * i + j] = (T)((double)off[5 * i + j] - J[i][j]);
         } else {  // A[noc11] += A1, A[noc10] += A0
-          visc_jac(P, J, 1.0, b, mul, vf, ufa, area, smag);
+          visc_jac(P, J, 1.0, b, mul, mut, vf, ufa, area, smag);
           for (int i = 1; i < 5; i++)
             for (int j = 0; j < 5; j++) D[i][j] += J[i][j];
-          visc_jac(P, J, -1.0, a, mul, vf, ufa, area, smag);
+          visc_jac(P, J, -1.0, a, mul, mut, vf, ufa, area, smag);
           for (int i = 1; i < 5; i++)
             for (int j = 0; j < 5; j++) off[5 * i + j] = (T)((double)off[5 * i + j] + J[i][j]);
         }
@@ -1196,8 +1247,10 @@ __global__ void __launch_bounds__(kBlock) k_jac(DMesh m, DParams P, const double
           for (int i = 0; i < 3; i++) sV[i] /= smag;
           sArg = sV;
         }
-        visc_face(P, vo, vb, mul, vb.u, area, n, smag, sArg, Fu, FE, vf);
-        visc_jac(P, J, -1.0, vo, mul, vf, vb.u, area, smag);
+        const double mut = (P.sa && z.bc != kBcWall) ? sa_mut(pc[0] * P.sa_phi[c], P.sa_aux[kAuxN * (size_t)c + kAuxMue])
+                                                      : 0.0;
+        visc_face(P, vo, vb, mul, mut, vb.u, area, n, smag, sArg, Fu, FE, vf);
+        visc_jac(P, J, -1.0, vo, mul, mut, vf, vb.u, area, smag);
         for (int i = 1; i < 5; i++)
           for (int j = 0; j < 5; j++) D[i][j] -= J[i][j];
       }
@@ -1223,6 +1276,330 @@ __global__ void __launch_bounds__(kBlock) k_jac(DMesh m, DParams P, const double
   T *d = A + 25 * (size_t)row;
   for (int i = 0; i < 5; i++)
     for (int j = 0; j < 5; j++) d[5 * i + j] = (T)D[i][j];
+}
+
+// ---------------------------------------------------------------------------
+// Spalart-Allmaras per-cell fields after k_grad: the limited nuSA gradient
+// (calcCvScalarGrad with limiterNavierS and reference nuSA; JoeWithModels.cpp
+// :2736-2741, Scalars.cpp:134), the face-averaged laminar viscosity in faocv
+// order (InterpolateAtCellCenterFromFaceValues, UgpWithCvCompFlow.h:1538-1551)
+// and the strain and vorticity magnitudes of the limited velocity gradient
+// (calcGradVel, calcStrainRate, calcVorticity, :1305-1392).
+// ---------------------------------------------------------------------------
+template <bool LS>
+__global__ void __launch_bounds__(kBlock) k_sa_cell(DMesh m, DParams P, const double *__restrict__ prim,
+                                                    const double *__restrict__ bfa, const double *__restrict__ grad,
+                                                    double *__restrict__ aux) {
+  const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (t >= m.ncv) return;
+  const int c = m.sched[t];
+  const double *pc = prim + 8 * (size_t)c;
+  const double *phi = P.sa_phi;
+  const double ph = phi[c];
+  const double xcv[3] = {pc[5], pc[6], pc[7]};
+  double g[3];
+  double pmax = ph, pmin = ph;
+  const int e0 = m.cfl_i[c], e1 = m.cfl_i[c + 1];
+  int e_int = e0;
+  if (LS) {
+    const int noc_f = m.nbocv_i[c];
+    const double *l0 = m.lsg + 3 * (size_t)noc_f;
+    for (int j = 0; j < 3; j++) g[j] = l0[j] * ph;
+    int k = 1;
+    for (int e = e0; e < e1; e++) {
+      int2 en = m.cfl[e];
+      if (en.y < 0) break;
+      const double pn = phi[en.y];
+      const double *l = m.lsg + 3 * (size_t)(noc_f + k);
+      for (int j = 0; j < 3; j++) g[j] += l[j] * pn;
+      pmax = Dev::mx(pmax, pn);
+      pmin = Dev::mn(pmin, pn);
+      k++;
+      e_int = e + 1;
+    }
+    for (int e = e0; e < e1; e++) {
+      int2 en = m.cfl[e];
+      if (en.y >= 0) continue;
+      int f = en.x & 0x7fffffff;
+      const double *bl = m.bfa_lsg + 3 * (size_t)f;
+      const double pb = bfa[8 * (size_t)f + 7];
+      for (int j = 0; j < 3; j++) g[j] += bl[j] * pb;
+    }
+  } else {
+    for (int j = 0; j < 3; j++) g[j] = 0.0;
+    for (int e = e0; e < e1; e++) {
+      int2 en = m.cfl[e];
+      int f = en.x & 0x7fffffff;
+      const double *fn = m.fnorm + 3 * (size_t)f;
+      if (en.y >= 0) {
+        const double *pn = prim + 8 * (size_t)en.y;
+        const double phn = phi[en.y];
+        const double *xf = m.fgeo + 8 * (size_t)f + 4;
+        double ds[3], dn[3];
+        for (int j = 0; j < 3; j++) {
+          ds[j] = xf[j] - xcv[j];
+          dn[j] = xf[j] - pn[5 + j];
+        }
+        const bool s0 = (en.x >= 0);
+        const double wself = sqrt(dot3(ds, ds)), wnb = sqrt(dot3(dn, dn));
+        const double w0 = s0 ? wself : wnb, w1 = s0 ? wnb : wself;
+        const double fv = s0 ? (w1 * ph + w0 * phn) / (w0 + w1) : (w1 * phn + w0 * ph) / (w0 + w1);
+        for (int j = 0; j < 3; j++) {
+          if (s0)
+            g[j] += fv * fn[j];
+          else
+            g[j] -= fv * fn[j];
+        }
+        pmax = Dev::mx(pmax, phn);
+        pmin = Dev::mn(pmin, phn);
+        e_int = e + 1;
+      } else {
+        const double pb = bfa[8 * (size_t)f + 7];
+        for (int j = 0; j < 3; j++) g[j] += pb * fn[j];
+      }
+    }
+    const double vol = m.vol[c];
+    for (int j = 0; j < 3; j++) g[j] /= vol;
+  }
+  double al = 1.0;
+  if (P.limiter) {  // limitCvScalarGradBJ (UgpWithCv.h:881-937), reference value nuSA
+    double e2 = Dev::mx(P.eps * fabs(ph), 1.0e-8);
+    e2 = e2 * e2;
+    if ((pmax - pmin) > 1.0e-12) {
+      for (int e = e0; e < e_int; e++) {
+        const double *rr = m.cfl_r0 + 3 * (size_t)e;
+        double r0[3] = {rr[0], rr[1], rr[2]};
+        al = bj_face(al, ph, ph + dot3(r0, g), pmax, pmin, e2);
+      }
+    } else {
+      al = 0.0;
+    }
+  }
+  // face-averaged laminar viscosity over the faocv list
+  const double Tc = pc[4] / (pc[0] * P.R);
+  const int nf = e1 - e0;
+  double mue = 0.0;
+  for (int k = 0; k < nf; k++) {
+    const int e = e0 + m.foc[e0 + k];
+    const int2 en = m.cfl[e];
+    const int f = en.x & 0x7fffffff;
+    double mul;
+    if (en.y >= 0) {
+      const double *pn = prim + 8 * (size_t)en.y;
+      const double Tn = pn[4] / (pn[0] * P.R);
+      const bool s0 = (en.x >= 0);
+      double sV[3], smag, w0, w1;
+      visc_internal_geom(P, s0 ? pc + 5 : pn + 5, s0 ? pn + 5 : pc + 5, m.fgeo + 8 * (size_t)f + 4, s0 ? Tc : Tn,
+                         s0 ? Tn : Tc, sV, smag, mul, w0, w1);
+    } else {
+      mul = mu_lam(P, bfa[8 * (size_t)f + 5]);
+    }
+    mue += mul;
+  }
+  mue /= (double)nf;
+  // strain and vorticity magnitudes (pow(x, 2.0) is x * x)
+  const double *gu = grad + 16 * (size_t)c + 6;
+  double str = 0.0, vort = 0.0;
+  const double diverg = gu[0] + gu[4] + gu[8];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double v = i == j ? 0.5 * (gu[3 * i + j] + gu[3 * j + i]) - 1. / 3. * diverg : 0.5 * (gu[3 * i + j] + gu[3 * j + i]);
+      str += v * v;
+    }
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double v = 0.5 * (gu[3 * i + j] - gu[3 * j + i]);
+      vort += v * v;
+    }
+  double *o = aux + kAuxN * (size_t)c;
+  o[kAuxGrad] = g[0] * al;
+  o[kAuxGrad + 1] = g[1] * al;
+  o[kAuxGrad + 2] = g[2] * al;
+  o[kAuxMue] = mue;
+  o[kAuxStr] = sqrt(2.0 * str);
+  o[kAuxVort] = sqrt(2.0 * vort);
+}
+
+// ---------------------------------------------------------------------------
+// The nuSA equation of one cell, gathered in the reference's order: the HLLC
+// convective flux (calcEulerFlux scalar terms, JoeWithModels.cpp:2824-2869,
+// 3014-3026, 3134-3198, 3291-3342, 3413-3454, 3490-3495) with its Jacobian
+// rows AScal[0..5] at cell-centre states (:3088-3101, 3270-3274, 3385,
+// 3559-3561); diffusion with the SA diffusivity (calcViscousFluxScalar_new,
+// Scalars.cpp:117-247; diffusivityHookScalarRansTurb, TurbModel_SA.h:130-172);
+// the SA source and its diagonal (sourceHookScalarRansTurb_new,
+// TurbModel_SA.h:282-357). AS is [nnz][6] in device row order.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_scal(DMesh m, DParams P, const double *__restrict__ prim,
+                                                 const double *__restrict__ grad, const double *__restrict__ bfa,
+                                                 double *__restrict__ rhs_s, double *__restrict__ AS) {
+  const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (t >= m.ncv) return;
+  const int c = m.sched[t];
+  const double gam = P.gamma;
+  const bool snd = P.second_order != 0;
+  const double *pc = prim + 8 * (size_t)c;
+  const double *gc = grad + 16 * (size_t)c;
+  const double *phi = P.sa_phi;
+  const double *auxc = P.sa_aux + kAuxN * (size_t)c;
+  const double ph = phi[c];
+  Side own;
+  own.rho = pc[0]; own.u[0] = pc[1]; own.u[1] = pc[2]; own.u[2] = pc[3]; own.p = pc[4];
+  double acc = 0.0;
+  double D[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double S[6], F[5], Fs;
+  const int e0 = m.cfl_i[c], e1 = m.cfl_i[c + 1];
+  for (int e = e0; e < e1; e++) {
+    const int2 en = m.cfl[e];
+    const int f = en.x & 0x7fffffff;
+    const double *geo = m.fgeo + 8 * (size_t)f;
+    const double n[3] = {geo[0], geo[1], geo[2]};
+    const double area = geo[3];
+    const double xfa[3] = {geo[4], geo[5], geo[6]};
+    if (en.y >= 0) {
+      const int nb = en.y;
+      const double *pn = prim + 8 * (size_t)nb;
+      const double phn = phi[nb];
+      Side nbs;
+      nbs.rho = pn[0]; nbs.u[0] = pn[1]; nbs.u[1] = pn[2]; nbs.u[2] = pn[3]; nbs.p = pn[4];
+      Side so = own, sn = nbs;
+      double sco = ph, scn = phn;
+      if (snd) {
+        if (reconstruct(so, pc, gc, xfa)) {
+          const double r[3] = {xfa[0] - pc[5], xfa[1] - pc[6], xfa[2] - pc[7]};
+          sco = sco + dot3(r, auxc + kAuxGrad);
+        }
+        if (reconstruct(sn, pn, grad + 16 * (size_t)nb, xfa)) {
+          const double r[3] = {xfa[0] - pn[5], xfa[1] - pn[6], xfa[2] - pn[7]};
+          scn = scn + dot3(r, P.sa_aux + kAuxN * (size_t)nb + kAuxGrad);
+        }
+      }
+      const bool s0 = en.x >= 0;
+      const int noc = m.cfl_noc[e];
+      const bool accum = (noc >> 30) & 1;
+      double *off = AS + 6 * (size_t)(noc & 0x3fffffff);
+      if (s0) {
+        flux_hllc(F, so.rho, so.u, so.p, sn.rho, sn.u, sn.p, gam, area, n, sco, scn, &Fs);
+        acc += -Fs;
+        jac_hllc_scal(S, 0, own.rho, own.u, own.p, nbs.rho, nbs.u, nbs.p, gam, area, n, ph, phn);
+        for (int k = 0; k < 6; k++) D[k] += S[k];
+        jac_hllc_scal(S, 1, own.rho, own.u, own.p, nbs.rho, nbs.u, nbs.p, gam, area, n, ph, phn);
+        for (int k = 0; k < 6; k++) off[k] = accum ? off[k] + S[k] : 0.0 + S[k];
+      } else {
+        flux_hllc(F, sn.rho, sn.u, sn.p, so.rho, so.u, so.p, gam, area, n, scn, sco, &Fs);
+        acc += Fs;
+        jac_hllc_scal(S, 1, nbs.rho, nbs.u, nbs.p, own.rho, own.u, own.p, gam, area, n, phn, ph);
+        for (int k = 0; k < 6; k++) D[k] -= S[k];
+        jac_hllc_scal(S, 0, nbs.rho, nbs.u, nbs.p, own.rho, own.u, own.p, gam, area, n, phn, ph);
+        for (int k = 0; k < 6; k++) off[k] = accum ? off[k] - S[k] : 0.0 - S[k];
+      }
+    } else {
+      const ZoneDev &z = m.zones[-en.y - 1];
+      const double *bb = bfa + 8 * (size_t)f;
+      const double phb = bb[7];
+      const double ub[3] = {bb[1], bb[2], bb[3]};
+      if (z.bc == kBcSymmetry || z.bc == kBcWall) {
+        flux_hllc(F, bb[0], ub, bb[4], bb[0], ub, bb[4], gam, area, n, phb, phb, &Fs);
+        acc += -Fs;
+        jac_hllc_scal(S, 0, bb[0], ub, bb[4], bb[0], ub, bb[4], gam, area, n, phb, phb);
+      } else {
+        Side so = own;
+        double sco = ph;
+        if (snd && reconstruct(so, pc, gc, xfa)) {
+          const double r[3] = {xfa[0] - pc[5], xfa[1] - pc[6], xfa[2] - pc[7]};
+          sco = sco + dot3(r, auxc + kAuxGrad);
+        }
+        flux_hllc(F, so.rho, so.u, so.p, bb[0], ub, bb[4], gam, area, n, sco, phb, &Fs);
+        acc += -Fs;
+        jac_hllc_scal(S, 0, own.rho, own.u, own.p, bb[0], ub, bb[4], gam, area, n, ph, phb);
+      }
+      for (int k = 0; k < 6; k++) D[k] += S[k];
+    }
+  }
+  // diffusion, after the convective pass (JoeWithModels.cpp:2595-2610)
+  const double invSigma = 1.0 / kSaSigma;
+  const double Tc = pc[4] / (pc[0] * P.R);
+  for (int e = e0; e < e1; e++) {
+    const int2 en = m.cfl[e];
+    const int f = en.x & 0x7fffffff;
+    const double *geo = m.fgeo + 8 * (size_t)f;
+    const double n[3] = {geo[0], geo[1], geo[2]};
+    const double nmag = geo[3];
+    const double xfa[3] = {geo[4], geo[5], geo[6]};
+    if (en.y >= 0) {
+      const int nb = en.y;
+      const double *pn = prim + 8 * (size_t)nb;
+      const double Tn = pn[4] / (pn[0] * P.R);
+      const bool s0 = en.x >= 0;
+      double sV[3], smag, mul, w0, w1;
+      visc_internal_geom(P, s0 ? pc + 5 : pn + 5, s0 ? pn + 5 : pc + 5, xfa, s0 ? Tc : Tn, s0 ? Tn : Tc, sV, smag,
+                         mul, w0, w1);
+      const double rho0 = s0 ? pc[0] : pn[0], rho1 = s0 ? pn[0] : pc[0];
+      const double nu0 = s0 ? ph : phi[nb], nu1 = s0 ? phi[nb] : ph;
+      const double *g0 = s0 ? auxc + kAuxGrad : P.sa_aux + kAuxN * (size_t)nb + kAuxGrad;
+      const double *g1 = s0 ? P.sa_aux + kAuxN * (size_t)nb + kAuxGrad : auxc + kAuxGrad;
+      const double rho_fa = (w1 * rho0 + w0 * rho1) / (w0 + w1);
+      const double nuSA_fa = (w1 * nu0 + w0 * nu1) / (w0 + w1);
+      const double diff = invSigma * (mul + rho_fa * nuSA_fa);
+      const double alpha = dot3(n, sV);
+      const double viscFlux = diff * nmag *
+                              (alpha * (nu1 - nu0) / smag + 0.5 * (g0[0] + g1[0]) * (n[0] - alpha * sV[0]) +
+                               0.5 * (g0[1] + g1[1]) * (n[1] - alpha * sV[1]) +
+                               0.5 * (g0[2] + g1[2]) * (n[2] - alpha * sV[2]));
+      const double a = diff * nmag * alpha / smag;
+      double *off = AS + 6 * (size_t)(m.cfl_noc[e] & 0x3fffffff);
+      if (s0) {
+        acc += viscFlux;
+        D[5] += a / rho0;
+        off[5] = off[5] + -(a / rho1);
+      } else {
+        acc += -viscFlux;
+        D[5] += a / rho1;
+        off[5] = off[5] + -(a / rho0);
+      }
+    } else {
+      const ZoneDev &z = m.zones[-en.y - 1];
+      if (!z.skind) continue;  // no diffusive flux without a nuSA value (Scalars.cpp:239-245)
+      const double *bb = bfa + 8 * (size_t)f;
+      const double mul = mu_lam(P, bb[5]);
+      const double diff = z.bc == kBcWall ? invSigma * mul : invSigma * (mul + bb[0] * bb[7]);
+      const double s_half[3] = {xfa[0] - pc[5], xfa[1] - pc[6], xfa[2] - pc[7]};
+      const double smag_half = fabs(dot3(s_half, n));
+      acc += diff * nmag * (bb[7] - ph) / smag_half;
+      D[5] += diff * nmag / smag_half / pc[0];
+    }
+  }
+  // source (TurbModel_SA.h:289-354)
+  {
+    const double rho = pc[0], vol = m.vol[c];
+    const double mue = auxc[kAuxMue];
+    const double Om = Dev::mx(auxc[kAuxVort], 1.e-8);
+    const double Strainrate = Dev::mx(auxc[kAuxStr], 1.e-8);
+    const double d = auxc[kAuxWall];
+    const double chi = ph * rho / mue;
+    const double chi_3 = pow(chi, 3.0);
+    const double inv_KarmanConst2_d2 = 1.0 / (kSaKarman2 * d * d);
+    const double fv1 = chi_3 / (chi_3 + kSaCv1_3);
+    const double fv2 = 1.0 - chi / (1.0 + chi * fv1);
+    const double BlendTurbulentProduction = 0.0;
+    const double Sv = Dev::mx(Om + Dev::mn(0.0, Strainrate - Om) * BlendTurbulentProduction, 0.0);
+    const double Shat = Sv + ph * fv2 * inv_KarmanConst2_d2;
+    const double inv_Shat = 1.0 / Dev::mx(Shat, 1.0e-8);
+    const double r = Dev::mn(ph * inv_Shat * inv_KarmanConst2_d2, 10.0);
+    const double gg = r + kSaCw2 * (pow(r, 6.0) - r);
+    const double g_6 = pow(gg, 6.0);
+    const double fw = gg * pow((1.0 + kSaCw3_6) / (g_6 + kSaCw3_6), 1.0 / 6.0);
+    const double H1 = kSaCb1 * Shat * ph;
+    const double H2 = -kSaCw1 * fw * ph * ph / pow(d, 2.0);
+    const double H3 = kSaCb2 / kSaSigma * dot3(auxc + kAuxGrad, auxc + kAuxGrad);
+    acc += rho * vol * (H1 + H2 + H3);
+    const double d_H1 = 0.0;
+    const double d_H2 = -2.0 * kSaCw1 * fw * ph / pow(d, 2.0);
+    D[5] -= (d_H1 + d_H2) * vol;
+  }
+  rhs_s[c] = acc;
+  double *dg = AS + 6 * (size_t)m.nbocv_i[c];
+  for (int k = 0; k < 6; k++) dg[k] = D[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -1283,7 +1660,11 @@ __device__ double fin_slot(const double *__restrict__ p, int nb, int op, double 
   return v[0];
 }
 
-enum FinOp { kFinDt = 0, kFinL1 = 1, kFinAlpha = 2, kFinOmega = 3, kFinResid0 = 4, kFinResid = 5 };
+enum FinOp {
+  kFinDt = 0, kFinL1 = 1, kFinAlpha = 2, kFinOmega = 3, kFinResid0 = 4, kFinResid = 5,
+  kFinResid0J = 6,  // kFinResid0 of the Jacobi (scalar) solver, which stops at once if already converged
+  kFinL1S = 7       // L1 of the scalar residual
+};
 
 // one block: finishes a reduction and applies the scalar update of its owner
 constexpr int kFinBlock = 1024;
@@ -1300,6 +1681,9 @@ __global__ void __launch_bounds__(kFinBlock) k_fin(int nb, int op, const double 
       double v = fin_slot(partials + (size_t)k * nb, nb, kSum, sh);
       if (t0) S->resid[k] = v;
     }
+  } else if (op == kFinL1S) {
+    double v = fin_slot(partials, nb, kSum, sh);
+    if (t0) S->resid_s = v;
   } else if (op == kFinAlpha) {  // UgpWithCv.cpp:2775-2782
     double gamma = fin_slot(partials, nb, kSum, sh);
     if (t0) {
@@ -1315,13 +1699,13 @@ __global__ void __launch_bounds__(kFinBlock) k_fin(int nb, int op, const double 
     double rho_next = fin_slot(partials, nb, kSum, sh);
     double mxv = fin_slot(partials + nb, nb, kMax, sh);
     if (t0) {
-      if (op == kFinResid0) {
+      if (op == kFinResid0 || op == kFinResid0J) {
         S->res0_max = mxv;
         S->rho = 1.0;
         S->alpha = 1.0;
         S->omega = 1.0;
         S->iter = 0;
-        S->done = 0;
+        S->done = (op == kFinResid0J && mxv <= S->zero_abs) ? 1 : 0;  // UgpWithCv.cpp:2356-2361
       } else {
         S->res_max = mxv;
         if ((mxv <= S->zero_abs) || (mxv / (S->res0_max + 1.0E-12) <= S->zero_rel))
@@ -1417,7 +1801,7 @@ __global__ void __launch_bounds__(kBlock) k_flux_e(DMesh m, DParams P, const dou
     double sgn = -1.0;
     bool low = false;
     double vc[4] = {-0.0, -0.0, -0.0, -0.0};
-    double mul = 0.0;
+    double mul = 0.0, mut = 0.0;
     if (en.y >= 0) {
       const bool s0 = (en.x >= 0);
       double pn[8], gn[15];
@@ -1452,12 +1836,16 @@ __global__ void __launch_bounds__(kBlock) k_flux_e(DMesh m, DParams P, const dou
         const VSide vo = vside_cell(P, pc, gradv + 12 * (size_t)L.c);
         const VSide vn = vside_cell(P, pn, gradv + 12 * (size_t)en.y);
         const VSide &a = s0 ? vo : vn, &b = s0 ? vn : vo;
-        double sV[3], smag;
-        visc_internal_geom(P, s0 ? pc + 5 : pn + 5, s0 ? pn + 5 : pc + 5, xfa, a.T, b.T, sV, smag, mul);
+        double sV[3], smag, w0, w1;
+        visc_internal_geom(P, s0 ? pc + 5 : pn + 5, s0 ? pn + 5 : pc + 5, xfa, a.T, b.T, sV, smag, mul, w0, w1);
+        if (P.sa) {
+          const double nuo = P.sa_phi[L.c], nun = P.sa_phi[en.y];
+          mut = sa_mut_face(w0, w1, a.rho, b.rho, s0 ? nuo : nun, s0 ? nun : nuo, mul);
+        }
         const double ufa[3] = {0.5 * (a.u[0] + b.u[0]), 0.5 * (a.u[1] + b.u[1]), 0.5 * (a.u[2] + b.u[2])};
         double Fu[3], FE;
         VFace vf;
-        visc_face(P, a, b, mul, ufa, area, n, smag, sV, Fu, FE, vf);
+        visc_face(P, a, b, mul, mut, ufa, area, n, smag, sV, Fu, FE, vf);
         for (int k = 0; k < 3; k++) vc[k] = s0 ? -Fu[k] : Fu[k];
         vc[3] = s0 ? -FE : FE;
       }
@@ -1480,6 +1868,8 @@ __global__ void __launch_bounds__(kBlock) k_flux_e(DMesh m, DParams P, const dou
       }
       if (VISC) {  // boundary faces (:3714-3851); boundary mul from T_bfa (ComputeBCProperties_T)
         mul = mu_lam(P, bb[5]);
+        if (P.sa && z.bc != kBcWall)  // mut of every non-wall boundary face, symmetry included (calcDt reads it)
+          mut = sa_mut(pc[0] * P.sa_phi[L.c], P.sa_aux[kAuxN * (size_t)L.c + kAuxMue]);
         if (z.bc != kBcSymmetry) {
           const double *gvo = gradv + 12 * (size_t)L.c;
           const VSide vo = vside_cell(P, pc, gvo);
@@ -1497,14 +1887,14 @@ __global__ void __launch_bounds__(kBlock) k_flux_e(DMesh m, DParams P, const dou
           }
           double Fu[3], FE;
           VFace vf;
-          visc_face(P, vo, vb, mul, vb.u, area, n, smag, sArg, Fu, FE, vf);
+          visc_face(P, vo, vb, mul, mut, vb.u, area, n, smag, sArg, Fu, FE, vf);
           for (int k = 0; k < 3; k++) vc[k] = -Fu[k];
           vc[3] = -FE;
         }
       }
     }
-    if (VISC) {  // viscous bound of calcDt (UgpWithCvCompFlow.cpp:50-51), mut = 0
-      const double lamdiff = 4.0 * (mul + 0.0) / pc[0] * area * area / m.vol[L.c];
+    if (VISC) {  // viscous bound of calcDt (UgpWithCvCompFlow.cpp:50-51)
+      const double lamdiff = 4.0 * (mul + mut) / pc[0] * area * area / m.vol[L.c];
       lambda = Dev::mx(lambda, lamdiff);
       for (int k = 0; k < 4; k++) slot[6 + k] = vc[k];
     }
@@ -1809,6 +2199,175 @@ __global__ void k_to_double(int n, const float *__restrict__ s, double *__restri
   if (k < n) d[k] = (double)s[k];
 }
 
+// ---------------------------------------------------------------------------
+// The segregated nuSA solve (JoeWithModels.cpp:1040-1091) in FP64: right-hand
+// side and V/dt, Jacobi-preconditioned BiCGStab (solveCvScalarBcgstab,
+// UgpWithCv.cpp:2298-2479) on the diagonal-first rows of AS[.][5], update and clip.
+// ---------------------------------------------------------------------------
+// b = relax rhs - sum_noc AScal[0..4][noc] dq[nbocv_v[noc]]; AScal[5][diag] += V/dt
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_sc_prep(int ncv, const int *__restrict__ nbocv_i,
+                                                    const int *__restrict__ nbocv_v, const double *__restrict__ vol,
+                                                    const double *__restrict__ dt, double relax,
+                                                    double *__restrict__ rhs_s, double *__restrict__ AS,
+                                                    const T *__restrict__ dq, double *__restrict__ b,
+                                                    double *__restrict__ x) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncv) return;
+  double r = rhs_s[c] * relax;
+  const int noc_f = nbocv_i[c], noc_l = nbocv_i[c + 1] - 1;
+  const double tmp = vol[c] / (dt[c]);
+  AS[6 * (size_t)noc_f + 5] += tmp;
+  for (int noc = noc_f; noc <= noc_l; noc++) {
+    const double *a = AS + 6 * (size_t)noc;
+    const T *d = dq + 5 * (size_t)nbocv_v[noc];
+    r = r - a[0] * (double)d[0] - a[1] * (double)d[1] - a[2] * (double)d[2] - a[3] * (double)d[3] -
+        a[4] * (double)d[4];
+  }
+  rhs_s[c] = r;
+  b[c] = r;
+  x[c] = 0.0;
+}
+
+__device__ __forceinline__ double sc_row(const int *__restrict__ nbocv_i, const int *__restrict__ nbocv_v,
+                                         const double *__restrict__ AS, const double *__restrict__ x, int c) {
+  const int noc_f = nbocv_i[c], noc_l = nbocv_i[c + 1] - 1;
+  double y = AS[6 * (size_t)noc_f + 5] * x[c];
+  for (int noc = noc_f + 1; noc <= noc_l; noc++) y += AS[6 * (size_t)noc + 5] * x[nbocv_v[noc]];
+  return y;
+}
+
+// res = b - A x (init: res0 = res); partials res.res0 and max |res / a_diag|
+__global__ void __launch_bounds__(kBlock) k_sc_resid(int ncv, const int *__restrict__ nbocv_i,
+                                                     const int *__restrict__ nbocv_v, const double *__restrict__ AS,
+                                                     const double *__restrict__ x, const double *__restrict__ b,
+                                                     double *__restrict__ res, double *__restrict__ res0, int init,
+                                                     double *partials, const SolverScalars *S) {
+  __shared__ double sh[32];
+  if (!init && (S->done || S->stop)) return;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  double vs[1] = {0.0}, vm[1] = {0.0};
+  if (c < ncv) {
+    double r = b[c] - sc_row(nbocv_i, nbocv_v, AS, x, c);
+    res[c] = r;
+    if (init) res0[c] = r;
+    vs[0] = r * (init ? r : res0[c]);
+    vm[0] = fabs(r / AS[6 * (size_t)nbocv_i[c] + 5]);
+  }
+  block_reduce<1>(vs, sh, kSum);
+  __syncthreads();
+  block_reduce<1>(vm, sh + 16, kMax);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = vs[0];
+    partials[gridDim.x + blockIdx.x] = vm[0];
+  }
+}
+
+// p = res - beta (p - omega v); phat = p / a_diag
+__global__ void __launch_bounds__(kBlock) k_sc_p(int ncv, const int *__restrict__ nbocv_i,
+                                                 const double *__restrict__ AS, const double *__restrict__ res,
+                                                 const double *__restrict__ v, double *__restrict__ p,
+                                                 double *__restrict__ phat, const SolverScalars *S) {
+  if (S->done || S->stop) return;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncv) return;
+  const double pv = res[c] - S->beta * (p[c] - S->omega * v[c]);
+  p[c] = pv;
+  phat[c] = pv / AS[6 * (size_t)nbocv_i[c] + 5];
+}
+
+// y = A x with partial dots (y.other, y.y)
+__global__ void __launch_bounds__(kBlock) k_sc_spmv_dot(int ncv, const int *__restrict__ nbocv_i,
+                                                        const int *__restrict__ nbocv_v, const double *__restrict__ AS,
+                                                        const double *__restrict__ x, double *__restrict__ y,
+                                                        const double *__restrict__ other, double *partials,
+                                                        const SolverScalars *S) {
+  __shared__ double sh[32];
+  if (S->done || S->stop) return;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  double v[2] = {0.0, 0.0};
+  if (c < ncv) {
+    const double yv = sc_row(nbocv_i, nbocv_v, AS, x, c);
+    y[c] = yv;
+    v[0] = yv * other[c];
+    v[1] = yv * yv;
+  }
+  block_partials<2>(v, partials, sh, kSum);
+}
+
+// s = res - alpha v; shat = s / a_diag
+__global__ void __launch_bounds__(kBlock) k_sc_s(int ncv, const int *__restrict__ nbocv_i,
+                                                 const double *__restrict__ AS, const double *__restrict__ res,
+                                                 const double *__restrict__ v, double *__restrict__ sv,
+                                                 double *__restrict__ shat, const SolverScalars *S) {
+  if (S->done || S->stop) return;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncv) return;
+  const double s = res[c] - S->alpha * v[c];
+  sv[c] = s;
+  shat[c] = s / AS[6 * (size_t)nbocv_i[c] + 5];
+}
+
+__global__ void __launch_bounds__(kBlock) k_sc_x(int ncv, double *__restrict__ x, const double *__restrict__ phat,
+                                                 const double *__restrict__ shat, const SolverScalars *S) {
+  if (S->done || S->stop) return;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncv) return;
+  x[c] += S->alpha * phat[c] + S->omega * shat[c];
+}
+
+// phi = min(max((phi (rho - dq_rho) + dphi) / rho, lower), upper) with the updated rho
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_sc_update(int ncv, const double *__restrict__ q,
+                                                      const T *__restrict__ dq, const double *__restrict__ x,
+                                                      double *__restrict__ phi) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncv) return;
+  const double rho = q[c];
+  const double v = (phi[c] * (rho - (double)dq[5 * (size_t)c]) + x[c]) / rho;
+  phi[c] = Dev::mn(Dev::mx(v, kSaLower), kSaUpper);
+}
+
+// L1 of the scalar residual divided by the mean-flow relaxation (JoeWithModels.cpp:1132-1134)
+__global__ void __launch_bounds__(kBlock) k_l1s(int ncv, const double *__restrict__ rhs_s, double relax,
+                                                double *partials) {
+  __shared__ double sh[16];
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  double v[1] = {c < ncv ? fabs(rhs_s[c] / relax) : 0.0};
+  block_partials<1>(v, partials, sh, kSum);
+}
+
+// calcWallDistance (UgpWithCvCompFlow.h:1395-1535): nearest wall-face centroid
+// (first in zone order on ties), then the normal distance where the cell
+// centre projects within the face's "spectral radius". wx/wn: [nw][3] face
+// centroids and area-weighted normals of the wall zones, in zone order.
+__global__ void __launch_bounds__(kBlock) k_wall_dist(int ncv, const double *__restrict__ prim, int nw,
+                                                      const double *__restrict__ wx, const double *__restrict__ wn,
+                                                      double *__restrict__ aux) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncv) return;
+  const double x[3] = {prim[8 * (size_t)c + 5], prim[8 * (size_t)c + 6], prim[8 * (size_t)c + 7]};
+  double minDist = 1.0e20;
+  int save = 0;
+  for (int k = 0; k < nw; k++) {
+    const double dx = x[0] - wx[3 * k], dy = x[1] - wx[3 * k + 1], dz = x[2] - wx[3 * k + 2];
+    const double dist = sqrt(dx * dx + dy * dy + dz * dz);
+    if (dist < minDist) {
+      save = k;
+      minDist = dist;
+    }
+  }
+  const double *fn = wn + 3 * save;
+  const double nmag = sqrt(dot3(fn, fn));
+  const double n[3] = {fn[0] / nmag, fn[1] / nmag, fn[2] / nmag};
+  const double s_half[3] = {wx[3 * save] - x[0], wx[3 * save + 1] - x[1], wx[3 * save + 2] - x[2]};
+  const double sMag2 = dot3(s_half, s_half);
+  const double dNorm2 = dot3(s_half, n) * dot3(s_half, n);
+  const double dTang2 = sMag2 - dNorm2;
+  const double specRadius2 = 0.5 * nmag;
+  aux[kAuxN * (size_t)c + kAuxWall] = dTang2 > specRadius2 ? minDist : fabs(dot3(s_half, n));
+}
+
 inline int nblk(long long n, int b = kBlock) { return (int)((n + b - 1) / b); }
 
 template <typename T>
@@ -1888,14 +2447,28 @@ void parse_params(const std::string &joe_in, const Mesh &m, SolverParams &p, std
   p.gravity[1] = pm.dbl("GY", 0.0);
   p.gravity[2] = pm.dbl("GZ", 0.0);
   p.second_order = pm.has("SPATIAL_SECOND_ORDER");
-  if (pm.has("SCALAR_FIRST_ORDER")) {
-    // no transported scalars on this path: nothing to do
+  // turbulence model: the reference chooses its class by run number (joe.cpp
+  // :963-976); here TURB_MODEL = SA selects the RansTurbSA mix-in
+  {
+    const std::string tm = pm.str("TURB_MODEL", "NONE");
+    if (tm == "SA") {
+      p.sa = true;
+      if (!(p.mu_ref > 0.0)) throw std::invalid_argument("TURB_MODEL = SA needs MU_REF > 0");
+      const auto *t = pm.get("INITIAL_CONDITION_TURB");
+      if (!t || t->size() < 2) throw std::invalid_argument("INITIAL_CONDITION_TURB missing (JoeWithModels.cpp:223-232)");
+      p.sa_init = ParamMap::to_d((*t)[1], "INITIAL_CONDITION_TURB");
+      p.relax_scal = pm.dbl("UNDER_RELAXATION_SCALARS", 0.2);
+      if (pm.has("SCALAR_FIRST_ORDER")) throw Unsupported("SCALAR_FIRST_ORDER is not supported");
+    } else if (tm != "NONE") {
+      throw Unsupported("TURB_MODEL = " + tm + " is not supported (NONE, SA)");
+    }
   }
   std::string si = pm.str("SPACE_INTEGRATION", "HLLC");
   if (si == "HLLC") p.scheme = kHllc;
   else if (si == "ROE") p.scheme = kRoe;
   else if (si == "AUSM") p.scheme = kAusm;
   else throw Unsupported("SPACE_INTEGRATION = " + si + " is not supported (HLLC, ROE, AUSM)");
+  if (p.sa && p.scheme != kHllc) throw Unsupported("TURB_MODEL = SA is supported with HLLC only");
   if (pm.has("GRAD_RECONSTRUCTION")) {
     std::string s = pm.sub("GRAD_RECONSTRUCTION", "SCHEME");
     if (s == "LEASTSQUARES") p.grad_ls = true;
@@ -1991,6 +2564,24 @@ void parse_params(const std::string &joe_in, const Mesh &m, SolverParams &p, std
     } else {
       throw std::invalid_argument("cannot recognize bc for zone " + z.name + ": " + kw);
     }
+    // nuSA boundary value (scalarZoneIsHook / Profiles / Dirichlet / Flux,
+    // UgpWithCvCompFlow.h:1881-1958)
+    if (p.sa) {
+      if (pm.has(z.name + ".nuSA.flux")) throw Unsupported("zone " + z.name + ": scalar flux boundaries");
+      if (const auto *st = pm.get(z.name + ".nuSA")) {
+        if (st->size() < 2) throw std::invalid_argument("zone " + z.name + ".nuSA: missing value");
+        const std::string &v = (*st)[1];
+        if (v == "HOOK" || v == "Hook" || v == "hook" || v == "PROFILES")
+          throw Unsupported("zone " + z.name + ".nuSA = " + v + " is not supported");
+        b.skind = 1;
+        b.sval = ParamMap::to_d(v, z.name + ".nuSA");
+      }
+    }
+  }
+  if (p.sa) {
+    bool wall = false;
+    for (const ZoneBc &b : zbc) wall = wall || b.bc == kBcWall;
+    if (!wall) throw std::invalid_argument("TURB_MODEL = SA needs a WALL zone for the wall distance");
   }
 }
 
@@ -2031,6 +2622,17 @@ struct Ctx::Impl {
   DBuf<double> partials;
   DBuf<SolverScalars> scal;
   SolverScalars *h_scal = nullptr;  // pinned
+  // Spalart-Allmaras: nuSA, per-cell fields [ncv][kAuxN], residual, AScal
+  // rows [nnz][6], Jacobi BiCGStab vectors (b, x, res, res0, p, v, phat, s,
+  // shat, t), its scalars, the faocv entry map
+  DBuf<double> sa_phi, sa_aux, rhs_s, AS, sw[10];
+  DBuf<SolverScalars> scal_s;
+  SolverScalars *h_scal_s = nullptr;  // pinned
+  DBuf<int> foc;
+  int scal_iters = 0;
+  template <typename T>
+  void scalar_step(const T *dq);
+  void finish_step(StepInfo &info, int it, int st);
   // split step (host hooks): host source terms in device order, phase flag
   DBuf<double> src;
   std::vector<double> h_src;
@@ -2143,6 +2745,8 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
     std::memset(&z, 0, sizeof(z));
     z.bc = d.zbc[iz].bc;
     for (int k = 0; k < 8; k++) z.par[k] = d.zbc[iz].par[k];
+    z.skind = d.zbc[iz].skind;
+    z.sval = d.zbc[iz].sval;
     d.zone_slot[iz] = (int)zd.size();
     zd.push_back(z);
   }
@@ -2243,7 +2847,7 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
   // face records in device order: unit normal and area as normVec3d, x_fa
   std::vector<double> fg(8 * (size_t)nfa);
   std::vector<double> fn;
-  if (!d.par.grad_ls) fn.resize(3 * (size_t)nfa);
+  if (!d.par.grad_ls || d.par.sa) fn.resize(3 * (size_t)nfa);
   for (int f = 0; f < nfa; f++) {
     const double *n = &m.fa_normal[3 * (size_t)f];
     double mag = 0.0;
@@ -2325,6 +2929,26 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
       }
       d.cfl_r0.upload(r0.data(), r0.size());
     }
+    // Spalart-Allmaras: for each cell, the entry offsets of its faces in the
+    // reference's faocv order (the order of InterpolateAtCellCenterFromFaceValues)
+    if (d.par.sa) {
+      std::vector<int> fo(std::max<size_t>(1, dcf.size()), 0);
+      size_t k = 0;
+      for (int cd = 0; cd < ncv; cd++) {
+        const int c = perm[cd];
+        const int n = ci[c + 1] - ci[c];
+        if (m.faocv_i[c + 1] - m.faocv_i[c] != n) throw std::runtime_error("faocv and the cell's entries disagree");
+        for (int j = 0; j < n; j++, k++) {
+          const int f = m.faocv_v[m.faocv_i[c] + j];
+          int pos = -1;
+          for (int e = ci[c]; e < ci[c + 1]; e++)
+            if ((cf[e].x & 0x7fffffff) == f) pos = e - ci[c];
+          if (pos < 0) throw std::runtime_error("face of faocv not among the cell's entries");
+          fo[k] = pos;
+        }
+      }
+      d.foc.upload(fo.data(), fo.size());
+    }
     // entry-parallel kernels: the cell of every entry, and wavefronts of 64
     // lanes that each cover whole cells (consecutive cells while their
     // entries fit)
@@ -2385,6 +3009,38 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
   d.scal.alloc(1);
   HIPCHECK(hipMemset(d.scal.p, 0, sizeof(SolverScalars)));
   HIPCHECK(hipHostMalloc((void **)&d.h_scal, sizeof(SolverScalars)));
+  if (d.par.sa) {
+    std::vector<double> ph(n, d.par.sa_init);  // initialHook (JoeWithModels.cpp:219-240)
+    d.sa_phi.upload(ph.data(), n);
+    d.sa_aux.alloc(kAuxN * n);
+    HIPCHECK(hipMemset(d.sa_aux.p, 0, sizeof(double) * kAuxN * n));
+    d.rhs_s.alloc(n);
+    d.AS.alloc(6 * (size_t)d.nnz);
+    HIPCHECK(hipMemset(d.AS.p, 0, sizeof(double) * 6 * (size_t)d.nnz));
+    for (int k = 0; k < 10; k++) d.sw[k].alloc(n);
+    d.scal_s.alloc(1);
+    HIPCHECK(hipMemset(d.scal_s.p, 0, sizeof(SolverScalars)));
+    HIPCHECK(hipHostMalloc((void **)&d.h_scal_s, sizeof(SolverScalars)));
+    // wall faces in zone order (calcWallDistance, initialHookScalarRansTurbModel)
+    std::vector<double> wx, wn;
+    for (size_t iz = 0; iz < m.zones.size(); iz++) {
+      const Zone &z = m.zones[iz];
+      if (z.kind != kZoneBoundary || d.zbc[iz].bc != kBcWall) continue;
+      for (int f = z.ifa_f; f <= z.ifa_l; f++)
+        for (int i = 0; i < 3; i++) {
+          wx.push_back(m.x_fa[3 * (size_t)f + i]);
+          wn.push_back(m.fa_normal[3 * (size_t)f + i]);
+        }
+    }
+    if (wx.empty()) throw std::invalid_argument("TURB_MODEL = SA: the WALL zones have no faces");
+    DBuf<double> dwx, dwn;
+    dwx.upload(wx.data(), wx.size());
+    dwn.upload(wn.data(), wn.size());
+    hipLaunchKernelGGL(k_wall_dist, dim3(nblk(ncv)), dim3(kBlock), 0, d.stream, ncv, d.prim.p, (int)(wx.size() / 3),
+                       dwx.p, dwn.p, d.sa_aux.p);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(d.stream));
+  }
 
   d.dm.ncv = ncv;
   d.dm.nfa = nfa;
@@ -2408,6 +3064,7 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
   d.dm.cfl_r0 = d.cfl_r0.p;
   d.dm.wave_c0 = d.wave_c0.p;
   d.dm.nwave = d.nwave;
+  d.dm.foc = d.foc.p;
 
   d.dp.gamma = d.par.gamma;
   d.dp.R = d.par.R_gas;
@@ -2429,6 +3086,9 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
   d.dp.T_ref = d.par.T_ref;
   d.dp.pr = d.par.pr;
   d.dp.pr_turb = d.par.pr_turb;
+  d.dp.sa = d.par.sa ? 1 : 0;
+  d.dp.sa_phi = d.sa_phi.p;
+  d.dp.sa_aux = d.sa_aux.p;
   d.cfl = d.par.cfl;
   HIPCHECK(hipDeviceSynchronize());
 }
@@ -2444,6 +3104,7 @@ Ctx::~Ctx() {
   if (d.ev_step) (void)hipEventDestroy(d.ev_step);
   if (d.mail_h) (void)hipHostFree(d.mail_h);
   if (d.h_scal) (void)hipHostFree(d.h_scal);
+  if (d.h_scal_s) (void)hipHostFree(d.h_scal_s);
 }
 
 // Device bytes a context holds once it has stepped (mesh, state, Jacobian,
@@ -2464,9 +3125,19 @@ size_t Ctx::footprint(const Mesh &m, int precision) {
   return b + (1 << 20);
 }
 
+size_t Ctx::footprint(const Mesh &m, int precision, bool sa) {
+  size_t b = footprint(m, precision);
+  if (sa) {
+    const size_t ncv = m.ncv, nnz = m.nbocv_s(), ent = 2 * (size_t)(m.nfa - m.nfa_b) + m.nfa_b;
+    b += ncv * (1 + kAuxN + 1 + 10) * 8 + nnz * 6 * 8 + ent * 4;  // nuSA, fields, rhs, Krylov, AScal, faocv map
+  }
+  return b;
+}
+
 // One pseudo-time step driven entirely by the device, captured once into a
 // hipGraph and replayed; results are identical to step()'s.
 void Ctx::device_step_async() {
+  if (d_->par.sa) throw Unsupported("the graph-captured step does not carry the Spalart-Allmaras scalar");
   Impl &d = *d_;
   if (!d.have_state) throw std::logic_error("no state set");
   if (d.begun) throw std::logic_error("device step between step_begin and step_end");
@@ -2607,6 +3278,21 @@ void Ctx::init_uniform() {
     rhoE[c] = d.par.p_init / (d.par.gamma - 1.0) + 0.5 * d.par.rho_init * uu;
   }
   set_state(rho.data(), rhou.data(), rhoE.data());
+  if (d.par.sa) {  // uniform nuSA (JoeWithModels.cpp:219-240)
+    std::vector<double> ph(n, d.par.sa_init);
+    set_scalar(ph.data());
+  }
+}
+
+void Ctx::set_scalar(const double *phi) {
+  Impl &d = *d_;
+  if (!d.par.sa) throw std::logic_error("no transported scalar (TURB_MODEL = SA)");
+  const size_t n = d.ncv;
+  std::vector<double> h(n);
+  for (size_t cd = 0; cd < n; cd++) h[cd] = phi[d.perm[cd]];
+  HIPCHECK(hipMemcpyAsync(d.sa_phi.p, h.data(), sizeof(double) * n, hipMemcpyHostToDevice, d.stream));
+  HIPCHECK(hipStreamSynchronize(d.stream));
+  d.dev_step_dirty = true;
 }
 
 void Ctx::Impl::launch_state_bc() {
@@ -2631,6 +3317,12 @@ void Ctx::Impl::launch_grad(hipStream_t s) {
       if (par.grad_ls) hipLaunchKernelGGL((k_grad<true, false>), g, dim3(kBlock), 0, s, dm, dp, prim.p, bfa.p, grad.p, gradv.p);
       else hipLaunchKernelGGL((k_grad<false, false>), g, dim3(kBlock), 0, s, dm, dp, prim.p, bfa.p, grad.p, gradv.p);
     }
+  }
+  if (par.sa) {  // nuSA gradient, face-averaged mul, strain and vorticity
+    if (par.grad_ls)
+      hipLaunchKernelGGL(k_sa_cell<true>, dim3(nblk(ncv)), dim3(kBlock), 0, s, dm, dp, prim.p, bfa.p, grad.p, sa_aux.p);
+    else
+      hipLaunchKernelGGL(k_sa_cell<false>, dim3(nblk(ncv)), dim3(kBlock), 0, s, dm, dp, prim.p, bfa.p, grad.p, sa_aux.p);
   }
 }
 
@@ -2695,6 +3387,54 @@ void Ctx::Impl::launch_grad_flux() {
   // dt_min of calcDt; CFL_MIN sets every local dt to it
   hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, flux_blocks(), (int)kFinDt, partials.p, scal.p);
   if (par.dt_mode == 1) hipLaunchKernelGGL(k_dt_fill, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, dt.p, scal.p);
+  // the nuSA equation: residual and AScal rows (calcRhs scalar terms)
+  if (par.sa)
+    hipLaunchKernelGGL(k_scal, dim3(nblk(ncv)), dim3(kBlock), 0, stream, dm, dp, prim.p, grad.p, bfa.p, rhs_s.p, AS.p);
+}
+
+// The segregated nuSA half of a step (JoeWithModels.cpp:1040-1091) after the
+// mean flow took dq: right-hand side and V/dt, Jacobi BiCGStab polled by the
+// host in chunks (2, then 4 iterations; iterations after convergence are
+// no-ops), update and clip. Work vectors sw: b, x, res, res0, p, v, phat, s, shat, t.
+template <typename T>
+void Ctx::Impl::scalar_step(const T *dq) {
+  double *b = sw[0].p, *x = sw[1].p, *res = sw[2].p, *res0 = sw[3].p, *p = sw[4].p, *vv = sw[5].p, *phat = sw[6].p,
+         *sv = sw[7].p, *shat = sw[8].p, *tv = sw[9].p;
+  const dim3 g(nblk(ncv)), bl(kBlock);
+  hipLaunchKernelGGL(k_sc_prep<T>, g, bl, 0, stream, ncv, nbocv_i.p, nbocv_v.p, vol.p, dt.p, par.relax_scal, rhs_s.p,
+                     AS.p, dq, b, x);
+  hipLaunchKernelGGL(k_fill<double>, g, bl, 0, stream, (size_t)ncv, p, 0.0);
+  hipLaunchKernelGGL(k_fill<double>, g, bl, 0, stream, (size_t)ncv, vv, 0.0);
+  hipLaunchKernelGGL(k_solver_init, dim3(1), dim3(64), 0, stream, scal_s.p, kSaMaxIter, kSaPhiZero, kSaPhiZeroRel);
+  const int nb = nblk(ncv);
+  hipLaunchKernelGGL(k_sc_resid, g, bl, 0, stream, ncv, nbocv_i.p, nbocv_v.p, AS.p, x, b, res, res0, 1, partials.p,
+                     scal_s.p);
+  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, nb, (int)kFinResid0J, partials.p, scal_s.p);
+  const int max_it = kSaMaxIter + 1;
+  int launched = 0;
+  while (launched < max_it) {
+    const int chunk = std::min(launched == 0 ? 2 : 4, max_it - launched);
+    for (int k = 0; k < chunk; k++) {
+      hipLaunchKernelGGL(k_sc_p, g, bl, 0, stream, ncv, nbocv_i.p, AS.p, res, vv, p, phat, scal_s.p);
+      hipLaunchKernelGGL(k_sc_spmv_dot, g, bl, 0, stream, ncv, nbocv_i.p, nbocv_v.p, AS.p, phat, vv, res0, partials.p,
+                         scal_s.p);
+      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, nb, (int)kFinAlpha, partials.p, scal_s.p);
+      hipLaunchKernelGGL(k_sc_s, g, bl, 0, stream, ncv, nbocv_i.p, AS.p, res, vv, sv, shat, scal_s.p);
+      hipLaunchKernelGGL(k_sc_spmv_dot, g, bl, 0, stream, ncv, nbocv_i.p, nbocv_v.p, AS.p, shat, tv, sv, partials.p,
+                         scal_s.p);
+      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, nb, (int)kFinOmega, partials.p, scal_s.p);
+      hipLaunchKernelGGL(k_sc_x, g, bl, 0, stream, ncv, x, phat, shat, scal_s.p);
+      hipLaunchKernelGGL(k_sc_resid, g, bl, 0, stream, ncv, nbocv_i.p, nbocv_v.p, AS.p, x, b, res, res0, 0,
+                         partials.p, scal_s.p);
+      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, nb, (int)kFinResid, partials.p, scal_s.p);
+    }
+    launched += chunk;
+    HIPCHECK(hipMemcpyAsync(h_scal_s, scal_s.p, sizeof(SolverScalars), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    if (h_scal_s->done) break;
+  }
+  scal_iters = h_scal_s->iter;
+  hipLaunchKernelGGL(k_sc_update<T>, g, bl, 0, stream, ncv, q.p, dq, x, sa_phi.p);
 }
 
 template <typename T>
@@ -2920,6 +3660,7 @@ StepInfo Ctx::step() {
     d.launch_jac<double>(true);
     d.solve<double>(it, st);
     hipLaunchKernelGGL(k_update<double>, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.q.p, d.v64[1].p);
+    if (d.par.sa) d.scalar_step<double>(d.v64[1].p);
   } else {
     ensure(d.A32, 25 * (size_t)d.nnz);
     ensure(d.LU32, 25 * (size_t)d.ncv);
@@ -2927,19 +3668,31 @@ StepInfo Ctx::step() {
     d.launch_jac<float>(true);
     d.solve<float>(it, st);
     hipLaunchKernelGGL(k_update<float>, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.q.p, d.v32[1].p);
+    if (d.par.sa) d.scalar_step<float>(d.v32[1].p);
   }
   d.launch_state_bc();
-  hipLaunchKernelGGL(k_l1, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.rhs.p, d.partials.p);
-  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, d.stream, nblk(d.ncv), (int)kFinL1, d.partials.p, d.scal.p);
-  HIPCHECK(hipMemcpyAsync(d.h_scal, d.scal.p, sizeof(SolverScalars), hipMemcpyDeviceToHost, d.stream));
-  HIPCHECK(hipStreamSynchronize(d.stream));
-  info.step = d.step_no;
+  d.finish_step(info, it, st);
+  return info;
+}
+
+// residual L1 sums of the step just taken (JoeWithModels.cpp:1119-1136) and its report
+void Ctx::Impl::finish_step(StepInfo &info, int it, int st) {
+  hipLaunchKernelGGL(k_l1, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, rhs.p, partials.p);
+  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, nblk(ncv), (int)kFinL1, partials.p, scal.p);
+  if (par.sa) {
+    hipLaunchKernelGGL(k_l1s, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, rhs_s.p, par.under_relax, partials.p);
+    hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, nblk(ncv), (int)kFinL1S, partials.p, scal.p);
+  }
+  HIPCHECK(hipMemcpyAsync(h_scal, scal.p, sizeof(SolverScalars), hipMemcpyDeviceToHost, stream));
+  HIPCHECK(hipStreamSynchronize(stream));
+  info.step = step_no;
   info.lin_iters = it;
   info.lin_status = st;
-  info.cfl = d.cfl;
-  info.dt_min = d.h_scal->dt_min;
-  for (int k = 0; k < 5; k++) info.resid[k] = d.h_scal->resid[k];
-  return info;
+  info.cfl = cfl;
+  info.dt_min = h_scal->dt_min;
+  for (int k = 0; k < 5; k++) info.resid[k] = h_scal->resid[k];
+  info.resid_s = par.sa ? h_scal->resid_s : 0.0;
+  info.scal_iters = par.sa ? scal_iters : 0;
 }
 
 void Ctx::step_begin(bool host_source) {
@@ -3008,24 +3761,17 @@ StepInfo Ctx::step_end() {
                        d.vol.p, d.dt.p, d.par.under_relax, d.rhs.p, d.A64.p, d.v64[0].p, d.v64[1].p);
     d.solve<double>(it, st);
     hipLaunchKernelGGL(k_update<double>, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.q.p, d.v64[1].p);
+    if (d.par.sa) d.scalar_step<double>(d.v64[1].p);
   } else {
     hipLaunchKernelGGL(k_time_term<float>, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.nbocv_i.p,
                        d.vol.p, d.dt.p, d.par.under_relax, d.rhs.p, d.A32.p, d.v32[0].p, d.v32[1].p);
     d.solve<float>(it, st);
     hipLaunchKernelGGL(k_update<float>, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.q.p, d.v32[1].p);
+    if (d.par.sa) d.scalar_step<float>(d.v32[1].p);
   }
   d.launch_state_bc();
-  hipLaunchKernelGGL(k_l1, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.rhs.p, d.partials.p);
-  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, d.stream, nblk(d.ncv), (int)kFinL1, d.partials.p, d.scal.p);
-  HIPCHECK(hipMemcpyAsync(d.h_scal, d.scal.p, sizeof(SolverScalars), hipMemcpyDeviceToHost, d.stream));
-  HIPCHECK(hipStreamSynchronize(d.stream));
   StepInfo info;
-  info.step = d.step_no;
-  info.lin_iters = it;
-  info.lin_status = st;
-  info.cfl = d.cfl;
-  info.dt_min = d.h_scal->dt_min;
-  for (int k = 0; k < 5; k++) info.resid[k] = d.h_scal->resid[k];
+  d.finish_step(info, it, st);
   return info;
 }
 
@@ -3146,12 +3892,38 @@ int64_t Ctx::get(const std::string &name, double *out) {
   if (name == "P_BFA") return bfarec(4, 1);
   if (name == "T_BFA") return bfarec(5, 1);
   if (name == "H_BFA") return bfarec(6, 1);
-  if (name == "RESID") {
+  if (name == "RESID") {  // L1 residuals of the last step; with SA the nuSA one sixth
+    const int nr = d.par.sa ? 6 : 5;
     if (out) {
       HIPCHECK(hipMemcpy(d.h_scal, d.scal.p, sizeof(SolverScalars), hipMemcpyDeviceToHost));
       for (int k = 0; k < 5; k++) out[k] = d.h_scal->resid[k];
+      if (d.par.sa) out[5] = d.h_scal->resid_s;
     }
-    return 5;
+    return nr;
+  }
+  if (d.par.sa) {
+    if (name == "NUSA") return cellrec(d.sa_phi.p, 1, 0, 1);
+    if (name == "NUSA_BFA") return bfarec(7, 1);
+    if (name == "GRAD_NUSA") return cellrec(d.sa_aux.p, kAuxN, kAuxGrad, 3);
+    if (name == "WALL_DIST") return cellrec(d.sa_aux.p, kAuxN, kAuxWall, 1);
+    if (name == "STR_MAG") return cellrec(d.sa_aux.p, kAuxN, kAuxStr, 1);
+    if (name == "VORT_MAG") return cellrec(d.sa_aux.p, kAuxN, kAuxVort, 1);
+    if (name == "RHS_NUSA") return cellrec(d.rhs_s.p, 1, 0, 1);
+    if (name == "DSCAL") return cellrec(d.sw[1].p, 1, 0, 1);
+    if (name == "ASCAL") {  // [6][nnz] in the reference's nbocv order, like runBackwardEuler's AScal[iScal]
+      const size_t nnz = d.nnz;
+      if (out) {
+        copy(d.AS.p, 6 * nnz * sizeof(double));
+        for (size_t cd = 0; cd < n; cd++) {
+          const size_t c = perm[cd];
+          const size_t len = d.h_dev_nbocv_i[cd + 1] - d.h_dev_nbocv_i[cd];
+          for (size_t j = 0; j < len; j++)
+            for (int k = 0; k < 6; k++)
+              out[k * nnz + d.h_ref_nbocv_i[c] + j] = h[6 * (d.h_dev_nbocv_i[cd] + j) + k];
+        }
+      }
+      return (int64_t)(6 * nnz);
+    }
   }
   if (name == "COUNTERS") {  // negative pressures, first-order fallbacks
     if (out) {

@@ -34,7 +34,8 @@ class SolveReport(C.Structure):
 
 class StepInfo(C.Structure):
     _fields_ = [("step", C.c_int), ("lin_iters", C.c_int), ("lin_status", C.c_int), ("cfl", C.c_double),
-                ("dt_min", C.c_double), ("resid", C.c_double * 5)]
+                ("dt_min", C.c_double), ("resid", C.c_double * 5), ("resid_scal", C.c_double),
+                ("scal_iters", C.c_int)]
 
 
 def lib():
@@ -58,6 +59,7 @@ def lib():
         L.jm_ctx_destroy.argtypes = [vp]
         L.jm_ctx_destroy.restype = None
         L.jm_set_state.argtypes = [vp, vp, vp, vp]
+        L.jm_set_scalar.argtypes = [vp, C.c_char_p, vp]
         L.jm_get_state.argtypes = [vp, vp, vp, vp]
         L.jm_init_uniform.argtypes = [vp]
         L.jm_calc_rhs.argtypes = [vp, C.c_int]
@@ -190,6 +192,11 @@ class Solver:
         rhoE = np.ascontiguousarray(rhoE, np.float64)
         assert rho.size == self.ncv and rhou.size == 3 * self.ncv and rhoE.size == self.ncv
         _check(lib().jm_set_state(self.h, _ptr(rho), _ptr(rhou), _ptr(rhoE)))
+
+    def set_scalar(self, phi, name: str = "nuSA"):
+        phi = np.ascontiguousarray(phi, np.float64)
+        assert phi.size == self.ncv
+        _check(lib().jm_set_scalar(self.h, name.encode(), _ptr(phi)))
 
     def state(self):
         rho, rhou, rhoE = np.empty(self.ncv), np.empty((self.ncv, 3)), np.empty(self.ncv)
