@@ -119,13 +119,20 @@ JoeWithModels::JoeWithModels(const char *inputFile) {
     }
   }
 
-  // IMPLICIT_PRECISION = FP64 (default) | MIXED: Jacobian, preconditioner
-  // and Krylov in FP32 around the FP64 residual
   const std::string prec = getStringParam("IMPLICIT_PRECISION", "FP64");
   if (prec != "FP64" && prec != "MIXED") throw std::runtime_error("IMPLICIT_PRECISION must be FP64 or MIXED");
-  require(jm_ctx_create(mesh_, joe_in_.c_str(), prec == "MIXED" ? JM_MIXED : JM_FP64, nullptr, &ctx_),
-          "jm_ctx_create");
   allocHost();
+}
+
+// The device context, once the object is complete: IMPLICIT_PRECISION = FP64
+// (default) | MIXED (Jacobian, preconditioner and Krylov in FP32 around the
+// FP64 residual); a RansTurbSA mix-in adds the nuSA equation.
+void JoeWithModels::ensureContext() {
+  if (ctx_) return;
+  std::string text = joe_in_;
+  if (dynamic_cast<RansTurbSA *>(this) && !checkParam("TURB_MODEL")) text += "\nTURB_MODEL = SA\n";
+  const std::string prec = getStringParam("IMPLICIT_PRECISION", "FP64");
+  require(jm_ctx_create(mesh_, text.c_str(), prec == "MIXED" ? JM_MIXED : JM_FP64, nullptr, &ctx_), "jm_ctx_create");
 }
 
 JoeWithModels::~JoeWithModels() {
@@ -276,8 +283,17 @@ int JoeWithModels::doneSolver(double residEnergy) {
 
 // JoeWithModels::run -> runBackwardEuler (JoeWithModels.cpp:881-1198)
 void JoeWithModels::run() {
+  ensureContext();
+  RansTurbSA *sa = dynamic_cast<RansTurbSA *>(this);
+  if (sa) {  // initialHook's uniform nuSA (JoeWithModels.cpp:219-240)
+    sa->nuSA_v_.assign(ncv, std::stod(params_.count("INITIAL_CONDITION_TURB")
+                                          ? getStringParam("INITIAL_CONDITION_TURB")
+                                          : std::string("0.0")));
+    sa->nuSA = sa->nuSA_v_.data();
+  }
   initialHook();
   require(jm_set_state(ctx_, rho, &rhou[0][0], rhoE), "jm_set_state");
+  if (sa) require(jm_set_scalar(ctx_, "nuSA", sa->nuSA), "jm_set_scalar");
   step = 0;
   applyBoundaryHooks();
   const bool host_source = sourceHookOverridden();
@@ -321,5 +337,12 @@ void JoeWithModels::run() {
     done = doneSolver(Residual[4]);
   }
   syncStateFromDevice();
+  if (sa) {
+    int64_t n = ncv;
+    require(jm_get(ctx_, "NUSA", sa->nuSA_v_.data(), &n), "jm_get NUSA");
+    sa->wallDist_v_.assign(ncv, 0.0);
+    require(jm_get(ctx_, "WALL_DIST", sa->wallDist_v_.data(), &n), "jm_get WALL_DIST");
+    sa->wallDist = sa->wallDist_v_.data();
+  }
   finalHook();
 }

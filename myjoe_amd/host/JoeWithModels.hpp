@@ -18,6 +18,9 @@
 //   temporalHook()  after every step; finalHook() after the loop.
 //   doneSolver()    NSTEPS, RESID_ENERG_TH, NaN, a file named killjoe.
 //
+// A case class that also derives from RansTurbSA (as joe.cpp's MyJoeSA,
+// joe.cpp:286-296) solves the Spalart-Allmaras scalar nuSA with the flow.
+//
 // Differences a case author sees: the arrays handed to sourceHook start at
 // zero (the hook's terms are added to the device residual and to the
 // diagonal blocks A[nbocv_i[icv]]; off-diagonal blocks are not read back),
@@ -49,6 +52,24 @@ class FaZone {
   int kind_ = 0, index_ = 0;
 };
 
+// Spalart-Allmaras mix-in (RansTurbSA, TurbModel_SA.h:9-444). Joe.in needs
+// INITIAL_CONDITION_TURB and MU_REF > 0 and may give "<zone>.nuSA = value"
+// boundary values, as for the reference; the model constants are the
+// reference's defaults. nuSA holds the initial field when initialHook runs
+// (it may change it) and the final field after run(); wallDist the wall
+// distance after run().
+class RansTurbSA {
+ public:
+  RansTurbSA() {}
+  virtual ~RansTurbSA() {}
+  double *nuSA = nullptr;
+  double *wallDist = nullptr;
+
+ private:
+  friend class JoeWithModels;
+  std::vector<double> nuSA_v_, wallDist_v_;
+};
+
 class JoeWithModels {
  public:
   // inputFile: a Joe.in (JoeWithModels.h:18-21)
@@ -75,8 +96,12 @@ class JoeWithModels {
 
   // copy the device state into rho/rhou/rhoE and the derived vel/press/temp
   void syncStateFromDevice();
-  // the device solve (valid after construction)
-  jm_ctx *deviceContext() const { return ctx_; }
+  // the device solve, created on first use (run() creates it): the model
+  // mix-ins of the most-derived class choose its transported scalars
+  jm_ctx *deviceContext() {
+    ensureContext();
+    return ctx_;
+  }
   const jm_mesh *deviceMesh() const { return mesh_; }
 
   // mesh in the reference's numbering (Ugp/UgpWithCv members)
@@ -103,6 +128,7 @@ class JoeWithModels {
   struct ProbeTag {};
   explicit JoeWithModels(ProbeTag) {}
   void require(jm_status st, const char *what) const;
+  void ensureContext();
   bool sourceHookOverridden();
   void applyBoundaryHooks();
   void allocHost();

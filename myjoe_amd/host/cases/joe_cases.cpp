@@ -8,6 +8,8 @@
 //               CBC inflow would impose (T, u, p from INLET_HOOK_STATE)
 //     gravity   sourceHook overridden, calling the default (gravity) hook
 //     perturb   initialHook adds a smooth density perturbation
+//     sa        the Spalart-Allmaras mix-in (MyJoeSA, joe.cpp:286-296); nuSA
+//               is appended to out.bin
 //
 // out.bin: step count, the five L1 residuals of every step, then rho[ncv],
 // rhou[ncv][3], rhoE[ncv] (reference numbering), all as doubles.
@@ -85,9 +87,14 @@ class Perturb : public Recorder {
   }
 };
 
+class SaCase : public Recorder, public RansTurbSA {
+ public:
+  explicit SaCase(const char *f) : Recorder(f) {}
+};
+
 int main(int argc, char **argv) {
   if (argc != 4) {
-    std::cerr << "usage: joe_cases <plain|inlethook|gravity|perturb> <Joe.in> <out.bin>\n";
+    std::cerr << "usage: joe_cases <plain|inlethook|gravity|perturb|sa> <Joe.in> <out.bin>\n";
     return 2;
   }
   try {
@@ -114,6 +121,15 @@ int main(int argc, char **argv) {
       j.run();
       j.write(argv[3]);
       std::printf("perturb steps=%d\n", j.step);
+    } else if (c == "sa") {
+      SaCase j(argv[2]);
+      j.run();
+      j.write(argv[3]);
+      std::FILE *fp = std::fopen(argv[3], "ab");
+      if (!fp) throw std::runtime_error("cannot write output");
+      std::fwrite(j.nuSA, sizeof(double), j.ncv, fp);
+      std::fclose(fp);
+      std::printf("sa steps=%d\n", j.step);
     } else {
       std::cerr << "unknown case " << c << "\n";
       return 2;

@@ -101,6 +101,32 @@ def test_unsupported_options_have_their_own_status():
         assert e.value.status == 5, str(e.value)
 
 
+def _sa_box_joe():
+    mm, joe = cases.box_case(2, seed=0)
+    joe = joe.replace("MU_REF = 0.0", "MU_REF = 0.002")
+    return mm, cases.with_sa(joe, 0.005, cases.SA_BOX)
+
+
+@pytest.mark.parametrize("edit,status,needle", [
+    (lambda s: s.replace("MU_REF = 0.002", "MU_REF = 0.0"), 3, "MU_REF"),             # SA needs viscosity
+    (lambda s: s.replace("INITIAL_CONDITION_TURB = 0.005", ""), 3, "INITIAL_CONDITION_TURB"),
+    (lambda s: s.replace("wall = WALL", "wall = SYMMETRY"), 3, "WALL"),              # no wall: no wall distance
+    (lambda s: s.replace("SPACE_INTEGRATION = HLLC", "SPACE_INTEGRATION = ROE"), 5, "HLLC"),
+    (lambda s: s.replace("inlet.nuSA = 0.006", "inlet.nuSA = HOOK"), 5, "inlet.nuSA"),
+    (lambda s: s.replace("TURB_MODEL = SA", "TURB_MODEL = SST"), 5, "TURB_MODEL"),
+])
+def test_sa_input_checked_before_device(edit, status, needle):
+    # TURB_MODEL = SA (the reference's RansTurbSA mix-in) is parsed before any
+    # device work: what it needs and what this build does not provide fail
+    # with the parameter and unsupported statuses
+    mm, joe = _sa_box_joe()
+    mesh = joemi.Mesh.from_faces(mm)
+    with pytest.raises(joemi.JoemiError) as e:
+        joemi.Solver(mesh, edit(joe))
+    assert e.value.status == status, str(e.value)
+    assert needle in str(e.value)
+
+
 def test_server_without_device_is_an_error():
     import torch
 

@@ -131,3 +131,31 @@ def test_initial_hook_override(tmp_path):
     # libm sin/cos of the case program and numpy may differ in the last bit
     np.testing.assert_allclose(hist, hist2, rtol=1e-9, atol=0)
     np.testing.assert_allclose(rho, s.state()[0], rtol=1e-9)
+
+
+@pytest.mark.gpu
+def test_sa_mixin_case_matches_abi_run(tmp_path):
+    # a case class deriving from RansTurbSA (joe.cpp's MyJoeSA) solves nuSA:
+    # the Joe.in carries no model keyword, the class does
+    print("host mirror: sa", flush=True)
+    joe = joe_text(**{"MU_REF = 0.0": "MU_REF = 0.002\nMU_MODE = POWERLAW\nMU_POWER_LAW = 0.76\nPR = 0.72"})
+    joe_sa = cases.with_sa(joe, 0.006, cases.SA_BUMP)
+    joe_case = joe_sa.replace("TURB_MODEL = SA\n", "")
+    mm = write_case(tmp_path, joe_case)
+    out = os.path.join(tmp_path, "sa.bin")
+    r = subprocess.run([EXE, "sa", "Joe.in", out], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = np.fromfile(out, np.float64)
+    n = int(raw[0])
+    ncv = mm.ncv
+    hist = raw[1:1 + 5 * n].reshape(n, 5)
+    rest = raw[1 + 5 * n:]
+    rho, rhou, rhoE, nu = rest[:ncv], rest[ncv:4 * ncv], rest[4 * ncv:5 * ncv], rest[5 * ncv:6 * ncv]
+    s = joemi.Solver(joemi.Mesh.from_faces(mm), joe_sa)
+    s.init_uniform()
+    n2, hist2 = s.run(NSTEPS)
+    assert n == n2 == NSTEPS
+    assert np.array_equal(hist, hist2)
+    r2, ru2, rE2 = s.state()
+    assert np.array_equal(rho, r2) and np.array_equal(rhou, ru2.reshape(-1)) and np.array_equal(rhoE, rE2)
+    assert np.array_equal(nu, s.get("NUSA"))
