@@ -25,8 +25,9 @@ enum SymmBc : int { kSymmOriginal = 0, kSymmNew = 1, kSymmAnalytical = 2 };
 // device cell numbering: the reference's own, face-count groups inside
 // 1024-cell tiles, Morton order of centroids, or both
 enum CellOrder : int { kOrderNative = 0, kOrderGrouped = 1, kOrderMorton = 2, kOrderMortonGrouped = 3 };
-// residual kernels: one lane per cell-face entry, or one thread per cell
-enum ResidKernels : int { kResidEntry = 0, kResidCell = 1 };
+// residual kernels: one lane per cell-face entry, one thread per cell, or
+// (inviscid flux only) one thread per internal face plus an entry gather
+enum ResidKernels : int { kResidEntry = 0, kResidCell = 1, kResidFace = 2 };
 
 // Parameters of UgpWithCvCompFlow::init / runBackwardEuler read from Joe.in.
 struct SolverParams {

@@ -1943,6 +1943,170 @@ __global__ void __launch_bounds__(kBlock) k_flux_e(DMesh m, DParams P, const dou
   block_partials<1>(v, partials, sh, kMin);
 }
 
+// Face form of the inviscid flux pass: k_face_flux evaluates each internal
+// face once (one thread per face, device face order, icv0 -> icv1
+// orientation) into fflux[nfa_i][5]; k_gather_e adds the faces' fluxes to
+// their cells in entry order (the reference's), computes the boundary fluxes
+// and the local time step. The flux of a face does not depend on which cell
+// asks for it, so the sums are the same bits as k_flux_e's, with half the
+// flux arithmetic and neighbour gathers, for 2 x 40 B per internal face.
+template <int SCHEME, bool SND>
+__global__ void __launch_bounds__(kBlock) k_face_flux(DMesh m, DParams P, const double *__restrict__ prim,
+                                                      const double *__restrict__ grad, const int2 *__restrict__ fcell,
+                                                      double *__restrict__ fflux, int *__restrict__ nfirst) {
+  const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int nfi = m.nfa - m.nfa_b;
+  if (i >= nfi) return;
+  const int f = m.nfa_b + i;
+  const int2 cc = fcell[i];
+  const double *geo = m.fgeo + 8 * (size_t)f;
+  double n[3], area, xfa[3];
+  {
+    double2 a = ld2(geo), b = ld2(geo + 2), c = ld2(geo + 4);
+    n[0] = a.x; n[1] = a.y; n[2] = b.x; area = b.y;
+    xfa[0] = c.x; xfa[1] = c.y; xfa[2] = geo[6];
+  }
+  double p0[8], p1[8];
+  {
+    const double *s0 = prim + 8 * (size_t)cc.x, *s1 = prim + 8 * (size_t)cc.y;
+    for (int k = 0; k < 8; k += 2) {
+      double2 v = ld2(s0 + k), w = ld2(s1 + k);
+      p0[k] = v.x; p0[k + 1] = v.y;
+      p1[k] = w.x; p1[k + 1] = w.y;
+    }
+  }
+  Side L, R;
+  int low = 0;
+  if (SND) {
+    double g[15];
+    const double *g0 = grad + 16 * (size_t)cc.x;
+    for (int k = 0; k < 14; k += 2) {
+      double2 v = ld2(g0 + k);
+      g[k] = v.x; g[k + 1] = v.y;
+    }
+    g[14] = g0[14];
+    low += !reconstruct(L, p0, g, xfa);
+    const double *g1 = grad + 16 * (size_t)cc.y;
+    for (int k = 0; k < 14; k += 2) {
+      double2 v = ld2(g1 + k);
+      g[k] = v.x; g[k + 1] = v.y;
+    }
+    g[14] = g1[14];
+    low += !reconstruct(R, p1, g, xfa);
+  } else {
+    L.rho = p0[0]; L.u[0] = p0[1]; L.u[1] = p0[2]; L.u[2] = p0[3]; L.p = p0[4];
+    R.rho = p1[0]; R.u[0] = p1[1]; R.u[1] = p1[2]; R.u[2] = p1[3]; R.p = p1[4];
+  }
+  double F[5];
+  face_flux(SCHEME, F, L, R, P.gamma, area, n);
+  double *o = fflux + 5 * (size_t)i;
+  for (int k = 0; k < 5; k++) o[k] = F[k];
+  if (low) atomicAdd(nfirst, low);
+}
+
+template <int SCHEME, bool SND>
+__global__ void __launch_bounds__(kBlock) k_gather_e(DMesh m, DParams P, const double *__restrict__ prim,
+                                                     const double *__restrict__ grad, const double *__restrict__ bfa,
+                                                     const double *__restrict__ q, const double *__restrict__ fflux,
+                                                     double *__restrict__ rhs, int *__restrict__ nfirst, DtArgs dta,
+                                                     double *__restrict__ dt, double *__restrict__ partials) {
+  constexpr int NS = 6;
+  __shared__ double lds[kBlock * NS];
+  __shared__ double sh[16];
+  double *slot = lds + NS * threadIdx.x;
+  const EntryLane L = entry_lane(m);
+  const double gam = P.gamma;
+  double pc[8];
+  if (L.e >= 0) {
+    const double *s = prim + 8 * (size_t)L.c;
+    for (int k = 0; k < 8; k += 2) {
+      double2 v = ld2(s + k);
+      pc[k] = v.x;
+      pc[k + 1] = v.y;
+    }
+    const int2 en = m.cfl[L.e];
+    const int f = en.x & 0x7fffffff;
+    const double *geo = m.fgeo + 8 * (size_t)f;
+    double n[3], area;
+    {
+      double2 a = ld2(geo), b = ld2(geo + 2);
+      n[0] = a.x; n[1] = a.y; n[2] = b.x; area = b.y;
+    }
+    double lambda;
+    {
+      const double ru[3] = {q[m.ncv + L.c], q[2 * m.ncv + L.c], q[3 * m.ncv + L.c]};
+      const double cs = sqrt(gam * pc[4] / pc[0]);
+      const double Uk = dot3(ru, n) / pc[0];
+      lambda = (fabs(Uk) + cs) * area;
+    }
+    double F[5];
+    double sgn = -1.0;
+    if (en.y >= 0) {
+      const double *ff = fflux + 5 * (size_t)(f - m.nfa_b);
+      for (int k = 0; k < 5; k++) F[k] = ff[k];
+      if (en.x < 0) sgn = 1.0;
+    } else {
+      const double xfa[3] = {geo[4], geo[5], geo[6]};
+      const ZoneDev &z = m.zones[-en.y - 1];
+      const double *bb = bfa + 8 * (size_t)f;
+      Side sb;
+      sb.rho = bb[0]; sb.u[0] = bb[1]; sb.u[1] = bb[2]; sb.u[2] = bb[3]; sb.p = bb[4];
+      if (z.bc == kBcSymmetry || z.bc == kBcWall) {
+        face_flux(SCHEME, F, sb, sb, gam, area, n);
+        if (z.bc == kBcSymmetry && P.symm_bc == kSymmAnalytical) {
+          F[0] = 0.0;
+          for (int i = 0; i < 3; i++) F[1 + i] = sb.p * n[i] * area;
+          F[4] = 0.0;
+        }
+      } else {
+        Side so;
+        so.rho = pc[0]; so.u[0] = pc[1]; so.u[1] = pc[2]; so.u[2] = pc[3]; so.p = pc[4];
+        if (SND) {
+          double gc[15];
+          const double *g = grad + 16 * (size_t)L.c;
+          for (int k = 0; k < 15; k++) gc[k] = g[k];
+          if (!reconstruct(so, pc, gc, xfa)) atomicAdd(nfirst, 1);
+        }
+        face_flux(SCHEME, F, so, sb, gam, area, n);
+      }
+    }
+    slot[5] = lambda;
+    for (int k = 0; k < 5; k++) slot[k] = sgn < 0.0 ? -F[k] : F[k];
+  }
+  __syncthreads();
+  double dtc = 1.0e20;
+  if (L.e >= 0 && L.pos == 0) {
+    const int c = L.c;
+    const int nent = m.cfl_i[c + 1] - m.cfl_i[c];
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double lambdaMax = 0.0;
+    for (int j = 0; j < nent; j++) {
+      const double *s = slot + NS * j;
+      for (int k = 0; k < 5; k++) acc[k] += s[k];
+      lambdaMax = Dev::mx(lambdaMax, s[5]);
+    }
+    const double vol = m.vol[c];
+    if (P.gravity_on) {
+      const double ru[3] = {q[m.ncv + c], q[2 * m.ncv + c], q[3 * m.ncv + c]};
+      acc[1] += vol * pc[0] * P.g[0];
+      acc[2] += vol * pc[0] * P.g[1];
+      acc[3] += vol * pc[0] * P.g[2];
+      acc[4] += vol * (P.g[0] * ru[0] + P.g[1] * ru[1] + P.g[2] * ru[2]);
+    } else {
+      acc[1] += 0.0;
+      acc[2] += 0.0;
+      acc[3] += 0.0;
+      bool allneg = signbit(pc[1]) && signbit(pc[2]) && signbit(pc[3]);
+      acc[4] += allneg ? -0.0 : 0.0;
+    }
+    for (int k = 0; k < 5; k++) rhs[(size_t)k * m.ncv + c] = acc[k];
+    dtc = dta.mode == 2 ? dta.dt_fixed : (dta.cfl_dev ? *dta.cfl_dev : dta.cfl) * vol / lambdaMax;
+    dt[c] = dtc;
+  }
+  double v[1] = {dtc};
+  block_partials<1>(v, partials, sh, kMin);
+}
+
 __global__ void k_dt_fill(int ncv, double *dt, const SolverScalars *S) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c < ncv) dt[c] = S->dt_min;
@@ -2439,7 +2603,8 @@ void parse_params(const std::string &joe_in, const Mesh &m, SolverParams &p, std
     const std::string o = pm.str(key, def);
     if (o == "ENTRY") return (int)kResidEntry;
     if (o == "CELL") return (int)kResidCell;
-    throw std::invalid_argument(std::string(key) + " = " + o + " (ENTRY, CELL)");
+    if (o == "FACE" && std::string(key) == "GPU_FLUX_KERNEL") return (int)kResidFace;
+    throw std::invalid_argument(std::string(key) + " = " + o + " (ENTRY, CELL; FACE for the flux)");
   };
   p.grad_kernel = kform("GPU_GRAD_KERNEL", "CELL");
   p.flux_kernel = kform("GPU_FLUX_KERNEL", "ENTRY");
@@ -2615,6 +2780,9 @@ struct Ctx::Impl {
   // state
   DBuf<double> q, prim, grad, bfa, rhs, dt, hook;
   DBuf<double> gradv;  // [ncv][12] velocity and enthalpy gradients of the viscous flux
+  DBuf<int2> fcell;    // [nfa_i] device cells (icv0, icv1) of each internal face (face form of the flux)
+  DBuf<double> fflux;  // [nfa_i][5] face fluxes of the face form
+  bool face_form() const { return par.flux_kernel == kResidFace && !dp.visc; }
   DBuf<int> counters;  // [0] negative pressure, [1] first-order fallbacks
   // solver (T = double or float, both allocated lazily)
   DBuf<double> A64, LU64, v64[10];
@@ -2843,6 +3011,13 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
     }
   }
   if (nf != nfa) throw std::runtime_error("internal face not reached by any cell");
+  if (d.par.flux_kernel == kResidFace && !(d.par.mu_ref > 0.0) && nfa > nfa_b) {
+    std::vector<int2> fc(nfa - nfa_b);
+    for (int f = nfa_b; f < nfa; f++)
+      fc[fnew[f] - nfa_b] = make_int2(inv[m.cvofa[2 * (size_t)f]], inv[m.cvofa[2 * (size_t)f + 1]]);
+    d.fcell.upload(fc.data(), fc.size());
+    d.fflux.alloc(5 * (size_t)(nfa - nfa_b));
+  }
 
   // face records in device order: unit normal and area as normVec3d, x_fa
   std::vector<double> fg(8 * (size_t)nfa);
@@ -3327,7 +3502,7 @@ void Ctx::Impl::launch_grad(hipStream_t s) {
 }
 
 // the per-cell flux kernel has no viscous variant
-bool Ctx::Impl::flux_entry() const { return par.flux_kernel == kResidEntry || dp.visc; }
+bool Ctx::Impl::flux_entry() const { return par.flux_kernel != kResidCell || dp.visc; }
 
 int Ctx::Impl::flux_blocks() const { return flux_entry() ? (nwave + kWaves - 1) / kWaves : nblk(ncv); }
 
@@ -3346,6 +3521,24 @@ void Ctx::Impl::launch_flux(hipStream_t s, const double *cfl_dev) {
     hipLaunchKernelGGL(kern, g, dim3(kBlock), 0, s, dm, dp, prim.p, grad.p, bfa.p, q.p, rhs.p, counters.p + 1, dta,
                        dt.p, partials.p, gradv.p);
   };
+  if (face_form()) {
+    const dim3 gf(nblk(nfa - nfa_b));
+    auto gof = [&](auto fk, auto gk) {
+      if (nfa > nfa_b) hipLaunchKernelGGL(fk, gf, dim3(kBlock), 0, s, dm, dp, prim.p, grad.p, fcell.p, fflux.p, counters.p + 1);
+      hipLaunchKernelGGL(gk, g, dim3(kBlock), 0, s, dm, dp, prim.p, grad.p, bfa.p, q.p, fflux.p, rhs.p, counters.p + 1,
+                         dta, dt.p, partials.p);
+    };
+    if (par.second_order) {
+      if (par.scheme == kRoe) gof(k_face_flux<kRoe, true>, k_gather_e<kRoe, true>);
+      else if (par.scheme == kAusm) gof(k_face_flux<kAusm, true>, k_gather_e<kAusm, true>);
+      else gof(k_face_flux<kHllc, true>, k_gather_e<kHllc, true>);
+    } else {
+      if (par.scheme == kRoe) gof(k_face_flux<kRoe, false>, k_gather_e<kRoe, false>);
+      else if (par.scheme == kAusm) gof(k_face_flux<kAusm, false>, k_gather_e<kAusm, false>);
+      else gof(k_face_flux<kHllc, false>, k_gather_e<kHllc, false>);
+    }
+    return;
+  }
   if (flux_entry()) {
     if (dp.visc) {
       if (par.second_order) {
