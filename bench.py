@@ -92,7 +92,10 @@ def serving_bench(n, seed, min_cells, max_cells, max_active, nsteps, resid_rel, 
     solved until its energy residual has dropped by resid_rel. Capacity: all
     requests present at t = 0. Latency: Poisson arrivals at 0.8 x the served
     capacity, replayed in real time for the server; for the one-at-a-time
-    loop the same arrivals are queued in front of its measured solve times."""
+    loop the same arrivals are queued in front of its measured solve times.
+    The served solver is the batched server (lockstep steps, each kernel of
+    the step and of the BiCGStab launched once for all active solves); the
+    per-stream graph server's capacity is reported beside it."""
     import numpy as np
 
     from myjoe_amd import cases, joemi
@@ -120,9 +123,20 @@ def serving_bench(n, seed, min_cells, max_cells, max_active, nsteps, resid_rel, 
     cap_plain = n / sum(service) * 60.0
     print(f"# serving: plain FP64 loop {cap_plain:.1f} solves/min ({cells} cells in {n} solves)", file=sys.stderr,
           flush=True)
-    # served: mixed precision, all present at t = 0
     budget = int(budget_gib * (1 << 30))
-    srv = joemi.Server(budget, max_active)
+    # per-stream graph server, capacity only
+    srv = joemi.Server(budget, max_active, batched=False)
+    t0 = time.perf_counter()
+    tickets = [srv.submit(m, joe, joemi.JM_MIXED) for m, joe in built]
+    for t in tickets:
+        srv.wait(t)
+    cap_streams = n / (time.perf_counter() - t0) * 60.0
+    for t in tickets:
+        srv.release(t)
+    srv.close()
+    print(f"# serving: per-stream graph server {cap_streams:.1f} solves/min", file=sys.stderr, flush=True)
+    # served: batched, mixed precision, all present at t = 0
+    srv = joemi.Server(budget, max_active, batched=True)
     t0 = time.perf_counter()
     tickets = [srv.submit(m, joe, joemi.JM_MIXED) for m, joe in built]
     for t in tickets:
@@ -169,8 +183,11 @@ def serving_bench(n, seed, min_cells, max_cells, max_active, nsteps, resid_rel, 
                     f"thick), bump 6-12 %, inlet Mach 0.3-0.5, HLLC 2nd order; stop at energy L1 drop {resid_rel:g} "
                     f"or {nsteps} steps",
         "served": {"solves_per_min": round(cap_served, 2), "precision": "FP64 residual, FP32 Jacobian/Krylov",
+                   "mode": "batched: lockstep steps, each step and BiCGStab kernel launched once for all "
+                                                            "active solves (2 groups)",
                    "max_active": max_active, "p50_s": round(q(lat, 50), 3), "p99_s": round(q(lat, 99), 3),
                    "mean_steps": round(float(np.mean(steps_served)), 1)},
+        "served_per_stream_graphs": {"solves_per_min": round(cap_streams, 2), "max_active": max_active},
         "plain_fp64_loop": {"solves_per_min": round(cap_plain, 2), "p50_s": round(q(lat_plain, 50), 3),
                             "p99_s": round(q(lat_plain, 99), 3), "mean_steps": round(float(np.mean(steps_plain)), 1)},
         "speedup_solves_per_min": round(cap_served / cap_plain, 3),
@@ -195,7 +212,7 @@ def main():
                     help="device cell numbering (GPU_CELL_ORDER): NATIVE, GROUPED, MORTON, MORTON_GROUPED")
     ap.add_argument("--serve", type=int, default=16, help="serving requests (0: skip the serving leg)")
     ap.add_argument("--serve-cells", default="3000,12000", help="min,max cells of a served request")
-    ap.add_argument("--serve-active", type=int, default=8)
+    ap.add_argument("--serve-active", type=int, default=16)
     ap.add_argument("--serve-steps", type=int, default=3000)
     ap.add_argument("--serve-rel", type=float, default=1e-4)
     ap.add_argument("--copy-calib-gib", type=float, default=0.0,

@@ -220,7 +220,16 @@ typedef struct {
   double queued_s;    /* submission to admission, seconds */
   double run_s;       /* admission (context build included) to retirement */
 } jm_solve_report;
-jm_status jm_server_create(int64_t mem_budget_bytes, int max_active, jm_server **out);
+/* flags: JM_SERVER_BATCHED steps the active solves in lockstep (2 groups, each
+ * on its own stream; JOEMI_SERVER_GROUPS = 1..4 in the environment changes it),
+ * each kernel of the step and of the BiCGStab launched once for all of a
+ * group's solves; without it each solve steps on its own stream through
+ * jm_step_graph's captured step. Either way a solve's result is the same bits
+ * as its own jm_step loop's. Batched serving runs the per-entry flux kernel
+ * (GPU_FLUX_KERNEL = ENTRY) and no SA scalar; other requests fail with
+ * JM_ERR_UNSUPPORTED. */
+#define JM_SERVER_BATCHED 1
+jm_status jm_server_create(int64_t mem_budget_bytes, int max_active, int flags, jm_server **out);
 void jm_server_destroy(jm_server *s);
 /* initial state may be NULL (uniform, as jm_init_uniform) */
 jm_status jm_server_submit(jm_server *s, const jm_mesh *m, const char *joe_in, int precision, const double *rho,

@@ -384,11 +384,11 @@ jm_status jm_ctx_footprint(const jm_mesh *m, int precision, int64_t *bytes) {
   });
 }
 
-jm_status jm_server_create(int64_t mem_budget_bytes, int max_active, jm_server **out) {
+jm_status jm_server_create(int64_t mem_budget_bytes, int max_active, int flags, jm_server **out) {
   return guard([&] {
-    if (!out || mem_budget_bytes <= 0 || max_active < 1) throw std::exception();
+    if (!out || mem_budget_bytes <= 0 || max_active < 1 || (flags & ~JM_SERVER_BATCHED)) throw std::exception();
     std::unique_ptr<jm_server> s(new jm_server);
-    s->s.reset(new jm::Server((size_t)mem_budget_bytes, max_active));
+    s->s.reset(new jm::Server((size_t)mem_budget_bytes, max_active, (flags & JM_SERVER_BATCHED) != 0));
     *out = s.release();
     return JM_OK;
   });

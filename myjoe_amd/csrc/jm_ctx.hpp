@@ -69,6 +69,71 @@ struct ZoneBc {
 
 void parse_params(const std::string &joe_in, const Mesh &m, SolverParams &p, std::vector<ZoneBc> &zbc);
 
+// One solve's block system for the batched workgroup BiCGStab (device
+// pointers): A[nnz][5][5], its diagonal-block LU [ncv][25], ten work vectors
+// [ncv][5] (rhs, dq, res, res0, p, v, phat, s, shat, t) in the solve's
+// precision, and its solver scalars.
+struct KrylovJob {
+  int ncv = 0, maxiter = 0;
+  const int *nbocv_i = nullptr, *nbocv_v = nullptr;
+  const void *A = nullptr;
+  void *LU = nullptr;
+  void *v[10] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  void *S = nullptr;
+  // first block and block count of this solve in each launch shape of the
+  // batch (5 lanes per cell, a thread per cell, a thread per unknown);
+  // filled by setup_krylov_batch
+  int b5 = 0, nb5 = 0, b1 = 0, nb1 = 0, bx = 0, nbx = 0;
+};
+
+// device buffers of the batched BiCGStab, grown on demand
+struct KrylovBatchBuf {
+  KrylovJob *jobs_d = nullptr;
+  int *map_d = nullptr;
+  double *partials_d = nullptr;
+  int *flag_h = nullptr, *flag_d = nullptr;
+  size_t cap_jobs = 0, cap_map = 0, cap_partials = 0;
+  int njobs = 0, g5 = 0, g1 = 0, gx = 0, max_it = 0;  // the batch set up last
+  KrylovBatchBuf() = default;
+  KrylovBatchBuf(const KrylovBatchBuf &) = delete;
+  KrylovBatchBuf &operator=(const KrylovBatchBuf &) = delete;
+  ~KrylovBatchBuf();
+};
+
+// the block-Jacobi BiCGStab of njobs solves of one precision, each kernel
+// launched once for all of them: setup_krylov_batch (synchronous, on stream)
+// builds the batch's tables, run_krylov_batch runs the batch set up last on
+// stream, synchronising between chunks of iterations. jobs_h must be pinned
+// host memory.
+void setup_krylov_batch(KrylovJob *jobs_h, int njobs, void *stream, KrylovBatchBuf &buf);
+void run_krylov_batch(int precision, void *stream, KrylovBatchBuf &buf);
+
+class Ctx;
+
+// Solves of one kernel configuration (equal Ctx::batch_key) stepping in
+// lockstep for batched serving: each kernel of the step before the linear
+// solve (CFL ramp, gradients, residual and local time step, Jacobian, solver
+// set-up: pre) and after it (update, state and boundary refresh, L1, report:
+// post) launched once for all of them, each solve keeping its own block
+// partition, partials and scalars (so the same bits as its own step).
+class StepBatch {
+ public:
+  StepBatch();
+  ~StepBatch();
+  StepBatch(const StepBatch &) = delete;
+  StepBatch &operator=(const StepBatch &) = delete;
+  // the batch's tables for these solves (each batch_prepare'd); synchronous
+  void set(Ctx *const *ctxs, int n, void *stream);
+  void pre(void *stream) const;
+  void post(void *stream) const;
+  int size() const;
+
+  struct Impl;
+
+ private:
+  std::unique_ptr<Impl> d_;
+};
+
 struct StepInfo {
   int step = 0, lin_iters = 0, lin_status = 0;
   double cfl = 0, dt_min = 0;
@@ -113,6 +178,18 @@ class Ctx {
   bool device_step_ready();
   StepInfo device_step_result();
   StepInfo device_step();
+  // batched serving: a step split around its linear solve, the halves run
+  // by a StepBatch and the solve by run_krylov_batch, for many solves at
+  // once. batch_prepare makes the buffers, the first refresh and the device
+  // step counter (and rejects what the batched step does not run);
+  // batch_key names the kernel configuration (solves of equal keys share a
+  // StepBatch); batch_job gives the solve's linear system; batch_result reads
+  // the step's report after the caller synchronised its stream.
+  void batch_prepare();
+  uint32_t batch_key() const;
+  void batch_job(KrylovJob &job);
+  StepInfo batch_result();
+  int precision() const;
   // device bytes a context for this mesh holds once it has stepped
   static size_t footprint(const Mesh &m, int precision);
   static size_t footprint(const Mesh &m, int precision, bool sa);
@@ -133,6 +210,7 @@ class Ctx {
   struct Impl;
 
  private:
+  friend class StepBatch;
   std::unique_ptr<Impl> d_;
 };
 

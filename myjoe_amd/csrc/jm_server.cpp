@@ -9,11 +9,23 @@
 // uploads, initial state), a stepping thread launches and collects steps.
 // Each solve's kernels touch only its own buffers, so its result is the same
 // bits alone or in any mix.
+//
+// Batched mode: the stepping thread advances every active solve by one step
+// per round, on the group's stream: each kernel of the step before the
+// linear solve launched once for all solves of a kernel configuration
+// (StepBatch::pre), the block-Jacobi BiCGStab of all of them (each kernel
+// launched once for all solves, nine per iteration whatever their number,
+// with a host poll between chunks of iterations), then the update and
+// refresh, again one launch per kernel (StepBatch::post), and one
+// synchronisation. The tables are rebuilt only when a solve joins or
+// leaves. A solve's result is the same bits as its ordinary step's.
 #include "jm_server.hpp"
+#include "jm_param.hpp"
 
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 
@@ -26,11 +38,30 @@ double now_s() {
 }
 }  // namespace
 
-Server::Server(size_t mem_budget, int max_active) : budget_(mem_budget), max_active_(max_active) {
+Server::Server(size_t mem_budget, int max_active, bool batched)
+    : budget_(mem_budget), max_active_(max_active), batched_(batched) {
   if (max_active < 1) throw std::invalid_argument("max_active must be at least 1");
   if (hipGetDevice(&device_) != hipSuccess) throw std::runtime_error("no HIP device available");
+  if (batched_) {
+    if (const char *e = std::getenv("JOEMI_SERVER_GROUPS")) ngroups_ = std::atoi(e);
+    ngroups_ = std::max(1, std::min(std::min(ngroups_, kMaxGroups), max_active_));
+    bool ok = true;
+    for (int g = 0; g < ngroups_ && ok; g++)
+      ok = hipStreamCreateWithFlags(&bs_[g], hipStreamNonBlocking) == hipSuccess &&
+           hipHostMalloc((void **)&jobs_h_[g], sizeof(KrylovJob) * max_active_) == hipSuccess;
+    if (!ok) {
+      for (int g = 0; g < ngroups_; g++) {
+        if (jobs_h_[g]) (void)hipHostFree(jobs_h_[g]);
+        if (bs_[g]) (void)hipStreamDestroy(bs_[g]);
+      }
+      throw std::runtime_error("HIP resources for batched serving");
+    }
+  }
   admit_thread_ = std::thread([this] { admit_loop(); });
-  step_thread_ = std::thread([this] { step_loop(); });
+  if (batched_)
+    for (int g = 0; g < ngroups_; g++) group_threads_.emplace_back([this, g] { batch_loop(g); });
+  else
+    step_thread_ = std::thread([this] { step_loop(); });
 }
 
 Server::~Server() {
@@ -40,7 +71,8 @@ Server::~Server() {
   }
   cv_.notify_all();
   admit_thread_.join();
-  step_thread_.join();
+  if (step_thread_.joinable()) step_thread_.join();
+  for (auto &t : group_threads_) t.join();
   for (auto &a : active_) {
     a.ctx.reset();
     if (a.stream) (void)hipStreamDestroy(a.stream);
@@ -48,6 +80,15 @@ Server::~Server() {
   for (auto &a : ready_) {
     a.ctx.reset();
     if (a.stream) (void)hipStreamDestroy(a.stream);
+  }
+  for (auto &a : groups_active_)
+    for (auto &b : a) {
+      b.ctx.reset();
+      if (b.stream) (void)hipStreamDestroy(b.stream);
+    }
+  for (int g = 0; g < ngroups_; g++) {
+    if (jobs_h_[g]) (void)hipHostFree(jobs_h_[g]);
+    if (bs_[g]) (void)hipStreamDestroy(bs_[g]);
   }
 }
 
@@ -57,6 +98,9 @@ int Server::submit(const Mesh *mesh, const std::string &joe_in, int precision, c
   SolverParams p;
   std::vector<ZoneBc> zbc;
   parse_params(joe_in, *mesh, p, zbc);  // reject bad input at submission
+  if (batched_ && p.sa) throw Unsupported("batched serving does not carry the Spalart-Allmaras scalar");
+  if (batched_ && p.mu_ref <= 0.0 && p.flux_kernel != kResidEntry)
+    throw Unsupported("batched serving runs the per-entry flux kernel (GPU_FLUX_KERNEL = ENTRY)");
   auto r = std::make_shared<Request>();
   r->mesh = mesh;
   r->joe_in = joe_in;
@@ -248,6 +292,108 @@ void Server::step_loop() {
                   active_.end());
     if (!progress) std::this_thread::sleep_for(std::chrono::microseconds(20));
   }
+}
+
+void Server::batch_loop(int group) {
+  (void)hipSetDevice(device_);
+  hipStream_t bs = bs_[group];
+  KrylovJob *jobs_h = jobs_h_[group];
+  KrylovBatchBuf &kb64 = kbuf64_[group], &kb32 = kbuf32_[group];
+  std::vector<Active> &active_ = groups_active_[group];
+  std::vector<std::unique_ptr<StepBatch>> steps;  // one per kernel configuration present
+  int nsteps = 0;
+  const int cap = (max_active_ + ngroups_ - 1) / ngroups_;
+  std::vector<int> members;  // request ids the tables were built for
+  auto fail = [&](Active &a, const char *what) {
+    a.dead = true;
+    retire(a, true, what);
+  };
+  for (;;) {
+    {
+      std::unique_lock<std::mutex> g(mu_);
+      if (stop_) break;
+      while (!ready_.empty() && (int)active_.size() < cap) {  // each group takes up to its share
+        active_.push_back(std::move(ready_.front()));
+        ready_.erase(ready_.begin());
+      }
+      if (active_.empty()) {
+        cv_.wait_for(g, std::chrono::milliseconds(2), [&] { return stop_ || !ready_.empty(); });
+        continue;
+      }
+    }
+    for (auto &a : active_) {
+      if (a.captured) continue;
+      try {
+        std::lock_guard<std::mutex> hg(hip_mu_);
+        a.ctx->batch_prepare();
+        a.captured = true;
+      } catch (std::exception &e) {
+        fail(a, e.what());
+      }
+    }
+    active_.erase(std::remove_if(active_.begin(), active_.end(), [](const Active &a) { return a.dead; }),
+                  active_.end());
+    if (active_.empty()) continue;
+    std::vector<int> now;
+    for (auto &a : active_) now.push_back(a.req->id);
+    try {
+      if (now != members) {
+        // the step batches (one per kernel configuration) and the BiCGStab
+        // batches (FP64 solves, then FP32 ones) of the new members
+        members.clear();
+        std::vector<uint32_t> keys;
+        for (auto &a : active_)
+          if (std::find(keys.begin(), keys.end(), a.ctx->batch_key()) == keys.end())
+            keys.push_back(a.ctx->batch_key());
+        std::lock_guard<std::mutex> hg(hip_mu_);
+        while (steps.size() < keys.size()) steps.emplace_back(new StepBatch);
+        nsteps = (int)keys.size();
+        for (int k = 0; k < nsteps; k++) {
+          std::vector<Ctx *> cs;
+          for (auto &a : active_)
+            if (a.ctx->batch_key() == keys[k]) cs.push_back(a.ctx.get());
+          steps[k]->set(cs.data(), (int)cs.size(), bs);
+        }
+        int n64 = 0, n32 = 0;
+        for (auto &a : active_)
+          if (a.ctx->precision() == 0) a.ctx->batch_job(jobs_h[n64++]);
+        for (auto &a : active_)
+          if (a.ctx->precision() != 0) a.ctx->batch_job(jobs_h[n64 + n32++]);
+        setup_krylov_batch(jobs_h, n64, bs, kb64);
+        setup_krylov_batch(jobs_h + n64, n32, bs, kb32);
+        members = now;
+      }
+      for (int k = 0; k < nsteps; k++) steps[k]->pre(bs);
+      run_krylov_batch(0, bs, kb64);
+      run_krylov_batch(1, bs, kb32);
+      for (int k = 0; k < nsteps; k++) steps[k]->post(bs);
+      if (hipStreamSynchronize(bs) != hipSuccess) throw std::runtime_error("HIP error in a batched step");
+    } catch (std::exception &e) {
+      members.clear();
+      for (auto &a : active_) fail(a, e.what());
+      active_.clear();
+      continue;
+    }
+    for (auto &a : active_) {
+      try {
+        StepInfo s = a.ctx->batch_result();
+        auto &r = *a.req;
+        r.steps = s.step;
+        r.last = s;
+        if (s.step == 1)
+          for (int k = 0; k < 5; k++) r.resid0[k] = s.resid[k];
+        if (a.ctx->done_solver(s)) {
+          a.dead = true;
+          retire(a, false, "");
+        }
+      } catch (std::exception &e) {
+        fail(a, e.what());
+      }
+    }
+    active_.erase(std::remove_if(active_.begin(), active_.end(), [](const Active &a) { return a.dead; }),
+                  active_.end());
+  }
+  (void)hipStreamSynchronize(bs);
 }
 
 }  // namespace jm

@@ -38,7 +38,10 @@ class Server {
     std::string error;
   };
 
-  Server(size_t mem_budget, int max_active);
+  // batched: every active solve steps in lockstep on one stream, their
+  // linear solves in one launch (one workgroup per solve); otherwise each
+  // solve steps on its own stream with the graph-captured device step
+  Server(size_t mem_budget, int max_active, bool batched = false);
   ~Server();
   Server(const Server &) = delete;
   Server &operator=(const Server &) = delete;
@@ -61,10 +64,23 @@ class Server {
   };
   void admit_loop();
   void step_loop();
+  void batch_loop(int group);
   void retire(Active &a, bool failed, const std::string &err);
 
   size_t budget_;
   int max_active_;
+  bool batched_ = false;
+  // batched mode: ngroups_ lockstep groups (JOEMI_SERVER_GROUPS, 1 to
+  // kMaxGroups), each with its own stepping thread, stream, solver jobs
+  // (pinned) and batched-BiCGStab buffers, so one group's kernels fill the
+  // gaps of the others (its host polls and synchronisations)
+  static constexpr int kMaxGroups = 4;
+  int ngroups_ = 2;
+  hipStream_t bs_[kMaxGroups] = {};
+  KrylovJob *jobs_h_[kMaxGroups] = {};
+  KrylovBatchBuf kbuf64_[kMaxGroups], kbuf32_[kMaxGroups];
+  std::vector<std::thread> group_threads_;
+  std::vector<Active> groups_active_[kMaxGroups];  // each group's stepping thread only
   int device_ = 0;
   std::mutex mu_;
   // held while a context is built (synchronous uploads) and while a solve's

@@ -106,6 +106,13 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kRedSlots = 8;  // partial-sum slots per block for fused reductions
 
+// A block's index and the grid size within one solve's launch: the hardware
+// values for an ordinary launch, the solve's own share of a batched one.
+struct Blk {
+  int id, n;
+};
+__device__ __forceinline__ Blk hw_blk() { return Blk{(int)blockIdx.x, (int)gridDim.x}; }
+
 __device__ __forceinline__ double2 ld2(const double *p) { return *reinterpret_cast<const double2 *>(p); }
 
 // Workgroups are dealt round-robin to the 8 XCDs (each with its own L2);
@@ -120,9 +127,9 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 // ---------------------------------------------------------------------------
 // calcStateVariables, UgpWithCvCompFlow.h:1662-1694 (gamma, R constant)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_state(int ncv, const double *__restrict__ q, double *__restrict__ prim,
+__device__ __forceinline__ void d_state(const Blk &B, int ncv, const double *__restrict__ q, double *__restrict__ prim,
                                                   double gam, int *__restrict__ negp) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  int c = B.id * blockDim.x + threadIdx.x;
   if (c >= ncv) return;
   double rho = q[c], r0 = q[ncv + c], r1 = q[2 * ncv + c], r2 = q[3 * ncv + c], rE = q[4 * ncv + c];
   double *P = prim + 8 * (size_t)c;
@@ -143,13 +150,18 @@ __global__ void __launch_bounds__(kBlock) k_state(int ncv, const double *__restr
   P[4] = p;
 }
 
+__global__ void __launch_bounds__(kBlock) k_state(int ncv, const double *__restrict__ q, double *__restrict__ prim,
+                                                  double gam, int *__restrict__ negp) {
+  d_state(hw_blk(), ncv, q, prim, gam, negp);
+}
+
 // ---------------------------------------------------------------------------
 // setBC, JoeWithModels.cpp:3857-4300 with ComputeBCProperties_T/_H
 // (UgpWithCvCompFlow.h:1783-1839) and the EOS density (:4292-4293)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_bc(DMesh m, DParams P, const double *__restrict__ prim,
+__device__ __forceinline__ void d_bc(const Blk &B, DMesh m, DParams P, const double *__restrict__ prim,
                                                const double *__restrict__ hook, double *__restrict__ bfa) {
-  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  int f = B.id * blockDim.x + threadIdx.x;
   if (f >= m.nfa_b) return;
   const ZoneDev &z = m.zones[m.bfa_zone[f]];
   const int c = m.bfa_cell[f];
@@ -270,6 +282,11 @@ __global__ void __launch_bounds__(kBlock) k_bc(DMesh m, DParams P, const double 
   o[7] = phi_b;
 }
 
+__global__ void __launch_bounds__(kBlock) k_bc(DMesh m, DParams P, const double *__restrict__ prim,
+                                               const double *__restrict__ hook, double *__restrict__ bfa) {
+  d_bc(hw_blk(), m, P, prim, hook, bfa);
+}
+
 // smooth Barth-Jespersen factor of one face (UgpWithCv.h:913-925, BJ undefined)
 __device__ __forceinline__ double bj_face(double alfa, double phi, double phifa, double phiMax, double phiMin,
                                           double eps2) {
@@ -297,10 +314,10 @@ __device__ __forceinline__ double enthalpy_of(const DParams &P, double rho, doub
 // JoeWithModels.cpp:3616-3630): limited when VISC_LIMITER and LIMITER are on,
 // raw otherwise; written to gradv[ncv][12] (grad u 9, grad h 3).
 template <bool LS, bool VISC>
-__global__ void __launch_bounds__(kBlock) k_grad(DMesh m, DParams P, const double *__restrict__ prim,
+__device__ __forceinline__ void d_grad(const Blk &B, DMesh m, DParams P, const double *__restrict__ prim,
                                                  const double *__restrict__ bfa, double *__restrict__ grad,
                                                  double *__restrict__ gradv) {
-  const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int t = xcd_block(B.id, B.n) * blockDim.x + threadIdx.x;
   if (t >= m.ncv) return;
   const int c = m.sched[t];
   const double *pc = prim + 8 * (size_t)c;
@@ -505,6 +522,13 @@ __global__ void __launch_bounds__(kBlock) k_grad(DMesh m, DParams P, const doubl
   o[15] = ar;
 }
 
+template <bool LS, bool VISC>
+__global__ void __launch_bounds__(kBlock) k_grad(DMesh m, DParams P, const double *__restrict__ prim,
+                                                 const double *__restrict__ bfa, double *__restrict__ grad,
+                                                 double *__restrict__ gradv) {
+  d_grad<LS, VISC>(hw_blk(), m, P, prim, bfa, grad, gradv);
+}
+
 // Entry-parallel form of k_grad: one lane per cell-face entry, each 64-lane
 // wavefront covering whole cells. Every lane computes its entry's terms;
 // the cell's first lane (its owner) adds them from LDS in entry order, which
@@ -517,9 +541,9 @@ struct EntryLane {
   int e, c, pos;  // entry (-1: idle lane), its cell, position in the cell's list
 };
 
-__device__ __forceinline__ EntryLane entry_lane(const DMesh &m) {
+__device__ __forceinline__ EntryLane entry_lane(const DMesh &m, const Blk &B) {
   EntryLane L{-1, -1, 0};
-  const int w = xcd_block(blockIdx.x, gridDim.x) * kWaves + (threadIdx.x >> 6);
+  const int w = xcd_block(B.id, B.n) * kWaves + (threadIdx.x >> 6);
   if (w < m.nwave) {
     const int c0 = m.wave_c0[w], c1 = m.wave_c0[w + 1];
     const int e = m.cfl_i[c0] + (threadIdx.x & 63);
@@ -532,12 +556,14 @@ __device__ __forceinline__ EntryLane entry_lane(const DMesh &m) {
   return L;
 }
 
+__device__ __forceinline__ EntryLane entry_lane(const DMesh &m) { return entry_lane(m, hw_blk()); }
+
 template <bool LS>
-__global__ void __launch_bounds__(kBlock) k_grad_e(DMesh m, DParams P, const double *__restrict__ prim,
+__device__ __forceinline__ void d_grad_e(const Blk &B, DMesh m, DParams P, const double *__restrict__ prim,
                                                    const double *__restrict__ bfa, double *__restrict__ grad) {
   __shared__ double lds[kBlock * 15];
   double *slot = lds + 15 * threadIdx.x;
-  const EntryLane L = entry_lane(m);
+  const EntryLane L = entry_lane(m, B);
   const bool owner = L.e >= 0 && L.pos == 0;
   double rho = 0.0, u[3] = {0.0, 0.0, 0.0}, p = 0.0, xcv[3] = {0.0, 0.0, 0.0};
   double nv[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // neighbour rho, u, v, w, p
@@ -729,6 +755,12 @@ __global__ void __launch_bounds__(kBlock) k_grad_e(DMesh m, DParams P, const dou
   for (int i = 0; i < 3; i++)
     for (int j = 0; j < 3; j++) o[6 + 3 * i + j] = g[6 + 3 * i + j] * au[i];
   o[15] = ar;
+}
+
+template <bool LS>
+__global__ void __launch_bounds__(kBlock) k_grad_e(DMesh m, DParams P, const double *__restrict__ prim,
+                                                   const double *__restrict__ bfa, double *__restrict__ grad) {
+  d_grad_e<LS>(hw_blk(), m, P, prim, bfa, grad);
 }
 
 // Reconstructed face state of one side (JoeWithModels.cpp:2832-2921): rho and
@@ -1109,12 +1141,12 @@ __device__ __forceinline__ void face_jac(int scheme, double J[5][5], int which, 
 }
 
 template <typename T, int SCHEME, bool VISC>
-__global__ void __launch_bounds__(kBlock) k_jac(DMesh m, DParams P, const double *__restrict__ prim,
+__device__ __forceinline__ void d_jac(const Blk &B, DMesh m, DParams P, const double *__restrict__ prim,
                                                 const double *__restrict__ bfa, T *__restrict__ A, int step_mode,
                                                 const double *__restrict__ dt, const double *__restrict__ rhs,
                                                 double relax, T *__restrict__ rhs_ls, T *__restrict__ phi,
                                                 const double *__restrict__ gradv) {
-  const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int t = xcd_block(B.id, B.n) * blockDim.x + threadIdx.x;
   if (t >= m.ncv) return;
   const int c = m.sched[t];
   const double gam = P.gamma;
@@ -1276,6 +1308,15 @@ __global__ void __launch_bounds__(kBlock) k_jac(DMesh m, DParams P, const double
   T *d = A + 25 * (size_t)row;
   for (int i = 0; i < 5; i++)
     for (int j = 0; j < 5; j++) d[5 * i + j] = (T)D[i][j];
+}
+
+template <typename T, int SCHEME, bool VISC>
+__global__ void __launch_bounds__(kBlock) k_jac(DMesh m, DParams P, const double *__restrict__ prim,
+                                                const double *__restrict__ bfa, T *__restrict__ A, int step_mode,
+                                                const double *__restrict__ dt, const double *__restrict__ rhs,
+                                                double relax, T *__restrict__ rhs_ls, T *__restrict__ phi,
+                                                const double *__restrict__ gradv) {
+  d_jac<T, SCHEME, VISC>(hw_blk(), m, P, prim, bfa, A, step_mode, dt, rhs, relax, rhs_ls, phi, gradv);
 }
 
 // ---------------------------------------------------------------------------
@@ -1634,10 +1675,15 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV], double *sh, int op
 }
 
 template <int NV>
-__device__ __forceinline__ void block_partials(double (&v)[NV], double *partials, double *sh, int op) {
+__device__ __forceinline__ void block_partials(double (&v)[NV], double *partials, double *sh, int op, const Blk &B) {
   block_reduce<NV>(v, sh, op);
   if (threadIdx.x == 0)
-    for (int k = 0; k < NV; k++) partials[(size_t)k * gridDim.x + blockIdx.x] = v[k];
+    for (int k = 0; k < NV; k++) partials[(size_t)k * B.n + B.id] = v[k];
+}
+
+template <int NV>
+__device__ __forceinline__ void block_partials(double (&v)[NV], double *partials, double *sh, int op) {
+  block_partials<NV>(v, partials, sh, op, hw_blk());
 }
 
 // fixed-order combination of nb partials; valid in thread 0
@@ -1668,9 +1714,9 @@ enum FinOp {
 
 // one block: finishes a reduction and applies the scalar update of its owner
 constexpr int kFinBlock = 1024;
-__global__ void __launch_bounds__(kFinBlock) k_fin(int nb, int op, const double *__restrict__ partials,
-                                                SolverScalars *S) {
-  __shared__ double sh[16];
+// slot k of the partials at partials + k * stride (nb values each)
+__device__ void fin_body(int nb, int op, const double *__restrict__ partials, size_t stride, SolverScalars *S,
+                         double *sh) {
   if ((op == kFinAlpha || op == kFinOmega || op == kFinResid) && (S->done || S->stop)) return;
   const bool t0 = threadIdx.x == 0;
   if (op == kFinDt) {
@@ -1678,7 +1724,7 @@ __global__ void __launch_bounds__(kFinBlock) k_fin(int nb, int op, const double 
     if (t0) S->dt_min = m;
   } else if (op == kFinL1) {
     for (int k = 0; k < 5; k++) {
-      double v = fin_slot(partials + (size_t)k * nb, nb, kSum, sh);
+      double v = fin_slot(partials + (size_t)k * stride, nb, kSum, sh);
       if (t0) S->resid[k] = v;
     }
   } else if (op == kFinL1S) {
@@ -1693,11 +1739,11 @@ __global__ void __launch_bounds__(kFinBlock) k_fin(int nb, int op, const double 
     }
   } else if (op == kFinOmega) {  // :2808-2810
     double st = fin_slot(partials, nb, kSum, sh);
-    double tt = fin_slot(partials + nb, nb, kSum, sh);
+    double tt = fin_slot(partials + stride, nb, kSum, sh);
     if (t0) S->omega = st / (tt + 1.0E-20);
   } else {  // residual: stop test (:2847-2856) and the next iteration's rho, beta (:2736-2749)
     double rho_next = fin_slot(partials, nb, kSum, sh);
-    double mxv = fin_slot(partials + nb, nb, kMax, sh);
+    double mxv = fin_slot(partials + stride, nb, kMax, sh);
     if (t0) {
       if (op == kFinResid0 || op == kFinResid0J) {
         S->res0_max = mxv;
@@ -1728,6 +1774,12 @@ __global__ void __launch_bounds__(kFinBlock) k_fin(int nb, int op, const double 
   }
 }
 
+__global__ void __launch_bounds__(kFinBlock) k_fin(int nb, int op, const double *__restrict__ partials,
+                                                SolverScalars *S) {
+  __shared__ double sh[16];
+  fin_body(nb, op, partials, (size_t)nb, S, sh);
+}
+
 // Residual kernel: one cell per thread (flux_cell), block minima of the local
 // time step to partials (k_fin kFinDt); CFL_MIN mode: k_dt_fill afterwards.
 template <int SCHEME, bool SND>
@@ -1749,7 +1801,7 @@ __global__ void __launch_bounds__(kBlock) k_flux(DMesh m, DParams P, const doubl
 // cell's owner lane adds the terms from LDS in entry order (the reference's
 // order), applies the gravity source, and writes the residual and local dt.
 template <int SCHEME, bool SND, bool VISC>
-__global__ void __launch_bounds__(kBlock) k_flux_e(DMesh m, DParams P, const double *__restrict__ prim,
+__device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const double *__restrict__ prim,
                                                    const double *__restrict__ grad, const double *__restrict__ bfa,
                                                    const double *__restrict__ q, double *__restrict__ rhs,
                                                    int *__restrict__ nfirst, DtArgs dta, double *__restrict__ dt,
@@ -1758,7 +1810,7 @@ __global__ void __launch_bounds__(kBlock) k_flux_e(DMesh m, DParams P, const dou
   __shared__ double lds[kBlock * NS];
   __shared__ double sh[16];
   double *slot = lds + NS * threadIdx.x;
-  const EntryLane L = entry_lane(m);
+  const EntryLane L = entry_lane(m, B);
   const double gam = P.gamma;
   double pc[8], gc[15];
   if (L.e >= 0) {
@@ -1940,7 +1992,16 @@ __global__ void __launch_bounds__(kBlock) k_flux_e(DMesh m, DParams P, const dou
     dt[c] = dtc;
   }
   double v[1] = {dtc};
-  block_partials<1>(v, partials, sh, kMin);
+  block_partials<1>(v, partials, sh, kMin, B);
+}
+
+template <int SCHEME, bool SND, bool VISC>
+__global__ void __launch_bounds__(kBlock) k_flux_e(DMesh m, DParams P, const double *__restrict__ prim,
+                                                   const double *__restrict__ grad, const double *__restrict__ bfa,
+                                                   const double *__restrict__ q, double *__restrict__ rhs,
+                                                   int *__restrict__ nfirst, DtArgs dta, double *__restrict__ dt,
+                                                   double *__restrict__ partials, const double *__restrict__ gradv) {
+  d_flux_e<SCHEME, SND, VISC>(hw_blk(), m, P, prim, grad, bfa, q, rhs, nfirst, dta, dt, partials, gradv);
 }
 
 // Face form of the inviscid flux pass: k_face_flux evaluates each internal
@@ -2107,17 +2168,25 @@ __global__ void __launch_bounds__(kBlock) k_gather_e(DMesh m, DParams P, const d
   block_partials<1>(v, partials, sh, kMin);
 }
 
-__global__ void k_dt_fill(int ncv, double *dt, const SolverScalars *S) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void d_dt_fill(const Blk &B, int ncv, double *dt, const SolverScalars *S) {
+  int c = B.id * blockDim.x + threadIdx.x;
   if (c < ncv) dt[c] = S->dt_min;
 }
 
+__global__ void k_dt_fill(int ncv, double *dt, const SolverScalars *S) {
+  d_dt_fill(hw_blk(), ncv, dt, S);
+}
+
 // CFL ramp of the pseudo-time loop (JoeWithModels.cpp:975-978) on the device
-__global__ void k_ramp(SolverScalars *S, int after, int interval, double factor, double cfl_max) {
+__device__ __forceinline__ void d_ramp(const Blk &B, SolverScalars *S, int after, int interval, double factor, double cfl_max) {
   if (threadIdx.x != 0) return;
   const int st = S->step_no + 1;
   S->step_no = st;
   if ((st >= after) && (interval > 0) && (st % interval == 0) && (S->cfl_cur < cfl_max)) S->cfl_cur *= factor;
+}
+
+__global__ void k_ramp(SolverScalars *S, int after, int interval, double factor, double cfl_max) {
+  d_ramp(hw_blk(), S, after, interval, factor, cfl_max);
 }
 
 __global__ void k_set_step(SolverScalars *S, int step, double cfl) {
@@ -2127,7 +2196,7 @@ __global__ void k_set_step(SolverScalars *S, int step, double cfl) {
 }
 
 // BiCGStab scalars at the start of a solve (leaves dt_min, residuals, step)
-__global__ void k_solver_init(SolverScalars *S, int maxiter, double zero_abs, double zero_rel) {
+__device__ __forceinline__ void d_solver_init(const Blk &B, SolverScalars *S, int maxiter, double zero_abs, double zero_rel) {
   if (threadIdx.x != 0) return;
   S->rho = S->rho_prime = S->alpha = S->omega = S->beta = S->gamma = 0.0;
   S->res0_max = S->res_max = 0.0;
@@ -2140,13 +2209,22 @@ __global__ void k_solver_init(SolverScalars *S, int maxiter, double zero_abs, do
   S->zero_rel = zero_rel;
 }
 
+__global__ void k_solver_init(SolverScalars *S, int maxiter, double zero_abs, double zero_rel) {
+  d_solver_init(hw_blk(), S, maxiter, zero_abs, zero_rel);
+}
+
 template <typename T>
-__global__ void k_fill(size_t n, T *__restrict__ p, T v) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void d_fill(const Blk &B, size_t n, T *__restrict__ p, T v) {
+  const size_t i = (size_t)B.id * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
 
-__global__ void k_publish(const SolverScalars *S, StepMail *out) {
+template <typename T>
+__global__ void k_fill(size_t n, T *__restrict__ p, T v) {
+  d_fill<T>(hw_blk(), n, p, v);
+}
+
+__device__ __forceinline__ void d_publish(const Blk &B, const SolverScalars *S, StepMail *out) {
   if (threadIdx.x != 0) return;
   out->solver_stop = S->done || S->stop;
   out->step = S->step_no;
@@ -2156,6 +2234,10 @@ __global__ void k_publish(const SolverScalars *S, StepMail *out) {
   out->dt_min = S->dt_min;
   for (int k = 0; k < 5; k++) out->resid[k] = S->resid[k];
   __threadfence_system();
+}
+
+__global__ void k_publish(const SolverScalars *S, StepMail *out) {
+  d_publish(hw_blk(), S, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -2306,12 +2388,175 @@ __global__ void __launch_bounds__(kBlock) k_x(int n5, T *__restrict__ phi, const
   phi[k] += (T)S->alpha * phat[k] + (T)S->omega * shat[k];
 }
 
+// ---------------------------------------------------------------------------
+// Batched block-Jacobi BiCGStab for serving: the kernels above, each launched
+// once for every active solve (a block -> solve map per grid shape, each
+// solve keeping its own blocks, partials and solver scalars), so a batch
+// costs nine launches per iteration whatever the number of solves. Every
+// solve is cut into blocks exactly as its own solve would be and its
+// partials are finished by one block in the same order, so its result is the
+// same bits as the per-solve solver's.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(kBlock) kb_lu(const KrylovJob *__restrict__ jobs, const int *__restrict__ map1) {
+  const KrylovJob &J = jobs[map1[blockIdx.x]];
+  const int c = (blockIdx.x - J.b1) * blockDim.x + threadIdx.x;
+  if (c >= J.ncv) return;
+  T a[5][5];
+  const T *src = (const T *)J.A + 25 * (size_t)J.nbocv_i[c];
+  for (int i = 0; i < 5; i++)
+    for (int j = 0; j < 5; j++) a[i][j] = src[5 * i + j];
+  ludeco5<T>(a);
+  T *d = (T *)J.LU + 25 * (size_t)c;
+  for (int i = 0; i < 5; i++)
+    for (int j = 0; j < 5; j++) d[5 * i + j] = a[i][j];
+  for (int k = 0; k < 5; k++) {  // p = v = 0
+    ((T *)J.v[4])[5 * (size_t)c + k] = (T)0;
+    ((T *)J.v[5])[5 * (size_t)c + k] = (T)0;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) kb_resid(const KrylovJob *__restrict__ jobs, const int *__restrict__ map5,
+                                                   int init, double *partials) {
+  __shared__ double sh[32];
+  __shared__ T cellv[kCellsPerBlock][5];
+  const KrylovJob &J = jobs[map5[blockIdx.x]];
+  const SolverScalars *S = (const SolverScalars *)J.S;
+  if (!init && (S->done || S->stop)) return;
+  const T *A = (const T *)J.A, *LU = (const T *)J.LU, *phi = (const T *)J.v[1], *rhs = (const T *)J.v[0];
+  T *res = (T *)J.v[2], *res0 = (T *)J.v[3];
+  const int lc = threadIdx.x / 5, i = threadIdx.x % 5;
+  const int c = (blockIdx.x - J.b5) * kCellsPerBlock + lc;
+  const bool active = (lc < kCellsPerBlock) && (c < J.ncv);
+  T r = (T)0.0;
+  if (active) {
+    T ax = spmv_row<T>(J.nbocv_i, J.nbocv_v, A, phi, c, i);
+    r = rhs[5 * (size_t)c + i] - ax;
+    res[5 * (size_t)c + i] = r;
+    if (init) res0[5 * (size_t)c + i] = r;
+    cellv[lc][i] = r;
+  }
+  __syncthreads();
+  double vs[1] = {0.0}, vm[1] = {0.0};
+  if (active) {
+    T r0v = init ? r : res0[5 * (size_t)c + i];
+    vs[0] = (double)r * (double)r0v;
+    T tmp[5] = {cellv[lc][0], cellv[lc][1], cellv[lc][2], cellv[lc][3], cellv[lc][4]};
+    lusolv5<T>(LU + 25 * (size_t)c, tmp);
+    vm[0] = fabs((double)tmp[i]);
+  }
+  block_reduce<1>(vs, sh, kSum);
+  __syncthreads();
+  block_reduce<1>(vm, sh + 16, kMax);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = vs[0];
+    partials[gridDim.x + blockIdx.x] = vm[0];
+  }
+}
+
+// which = 0: p = res - beta (p - omega v), phat = LU^-1 p; 1: s = res - alpha v, shat = LU^-1 s
+template <typename T>
+__global__ void __launch_bounds__(kBlock) kb_ps(const KrylovJob *__restrict__ jobs, const int *__restrict__ map1,
+                                                int which) {
+  const KrylovJob &J = jobs[map1[blockIdx.x]];
+  const SolverScalars *S = (const SolverScalars *)J.S;
+  if (S->done || S->stop) return;
+  const int c = (blockIdx.x - J.b1) * blockDim.x + threadIdx.x;
+  if (c >= J.ncv) return;
+  const T *res = (const T *)J.v[2], *vv = (const T *)J.v[5], *LU = (const T *)J.LU;
+  T *out = (T *)J.v[which == 0 ? 4 : 7], *hat = (T *)J.v[which == 0 ? 6 : 8];
+  T t[5];
+  if (which == 0) {
+    const T beta = (T)S->beta, omega = (T)S->omega;
+    for (int k = 0; k < 5; k++) {
+      size_t o = 5 * (size_t)c + k;
+      T pv = res[o] - beta * (out[o] - omega * vv[o]);
+      out[o] = pv;
+      t[k] = pv;
+    }
+  } else {
+    const T alpha = (T)S->alpha;
+    for (int k = 0; k < 5; k++) {
+      size_t o = 5 * (size_t)c + k;
+      T sv = res[o] - alpha * vv[o];
+      out[o] = sv;
+      t[k] = sv;
+    }
+  }
+  lusolv5<T>(LU + 25 * (size_t)c, t);
+  for (int k = 0; k < 5; k++) hat[5 * (size_t)c + k] = t[k];
+}
+
+// which = 0: v = A phat with v.res0; 1: t = A shat with s.t, t.t
+template <typename T>
+__global__ void __launch_bounds__(kBlock) kb_spmv_dot(const KrylovJob *__restrict__ jobs,
+                                                      const int *__restrict__ map5, int which, double *partials) {
+  __shared__ double sh[32];
+  const KrylovJob &J = jobs[map5[blockIdx.x]];
+  const SolverScalars *S = (const SolverScalars *)J.S;
+  if (S->done || S->stop) return;
+  const T *x = (const T *)J.v[which == 0 ? 6 : 8], *other = (const T *)J.v[which == 0 ? 3 : 7];
+  T *y = (T *)J.v[which == 0 ? 5 : 9];
+  const int lc = threadIdx.x / 5, i = threadIdx.x % 5;
+  const int c = (blockIdx.x - J.b5) * kCellsPerBlock + lc;
+  double v[2] = {0.0, 0.0};
+  if (lc < kCellsPerBlock && c < J.ncv) {
+    T yv = spmv_row<T>(J.nbocv_i, J.nbocv_v, (const T *)J.A, x, c, i);
+    y[5 * (size_t)c + i] = yv;
+    double o = (double)other[5 * (size_t)c + i];
+    v[0] = (double)yv * o;
+    v[1] = (double)yv * (double)yv;
+  }
+  block_partials<2>(v, partials, sh, kSum);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) kb_x(const KrylovJob *__restrict__ jobs, const int *__restrict__ mapx) {
+  const KrylovJob &J = jobs[mapx[blockIdx.x]];
+  const SolverScalars *S = (const SolverScalars *)J.S;
+  if (S->done || S->stop) return;
+  const int k = (blockIdx.x - J.bx) * blockDim.x + threadIdx.x;
+  if (k >= 5 * J.ncv) return;
+  T *phi = (T *)J.v[1];
+  phi[k] += (T)S->alpha * ((const T *)J.v[6])[k] + (T)S->omega * ((const T *)J.v[8])[k];
+}
+
+// one block per solve: finish its partials (at its first block, stride = the whole grid)
+__global__ void __launch_bounds__(kFinBlock) kb_fin(const KrylovJob *__restrict__ jobs, int op,
+                                                    const double *__restrict__ partials, int grid5) {
+  __shared__ double sh[16];
+  const KrylovJob &J = jobs[blockIdx.x];
+  fin_body(J.nb5, op, partials + J.b5, (size_t)grid5, (SolverScalars *)J.S, sh);
+}
+
+// *flag = 1 when every solve of the batch has converged or spent its launches
+__global__ void kb_all_done(const KrylovJob *__restrict__ jobs, int njobs, int *flag) {
+  __shared__ int open;
+  if (threadIdx.x == 0) open = 0;
+  __syncthreads();
+  for (int j = threadIdx.x; j < njobs; j += blockDim.x) {
+    const SolverScalars *S = (const SolverScalars *)jobs[j].S;
+    if (!(S->done || S->stop)) atomicOr(&open, 1);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    *flag = open ? 0 : 1;
+    __threadfence_system();
+  }
+}
+
 // Q += dQ (JoeWithModels.cpp:1028-1035)
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_update(int ncv, double *__restrict__ q, const T *__restrict__ dq) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void d_update(const Blk &B, int ncv, double *__restrict__ q, const T *__restrict__ dq) {
+  int c = B.id * blockDim.x + threadIdx.x;
   if (c >= ncv) return;
   for (int k = 0; k < 5; k++) q[(size_t)k * ncv + c] += (double)dq[5 * (size_t)c + k];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_update(int ncv, double *__restrict__ q, const T *__restrict__ dq) {
+  d_update<T>(hw_blk(), ncv, q, dq);
 }
 
 // sourceHook terms computed by the host, in device order: src[c] holds five
@@ -2349,13 +2594,125 @@ __global__ void __launch_bounds__(kBlock) k_time_term(int ncv, const int *__rest
 }
 
 // L1 residual sums (JoeWithModels.cpp:1119-1130), finished by k_fin kFinL1
-__global__ void __launch_bounds__(kBlock) k_l1(int ncv, const double *__restrict__ rhs, double *partials) {
+__device__ __forceinline__ void d_l1(const Blk &B, int ncv, const double *__restrict__ rhs, double *partials) {
   __shared__ double sh[16 * 5];
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  int c = B.id * blockDim.x + threadIdx.x;
   double v[5] = {0, 0, 0, 0, 0};
   if (c < ncv)
     for (int k = 0; k < 5; k++) v[k] = fabs(rhs[(size_t)k * ncv + c]);
-  block_partials<5>(v, partials, sh, kSum);
+  block_partials<5>(v, partials, sh, kSum, B);
+}
+
+__global__ void __launch_bounds__(kBlock) k_l1(int ncv, const double *__restrict__ rhs, double *partials) {
+  d_l1(hw_blk(), ncv, rhs, partials);
+}
+
+// ---------------------------------------------------------------------------
+// Batched step for serving: the kernels of a step before and after its
+// linear solve, each launched once for several solves of one kernel
+// configuration (a block -> solve map per launch shape). Each solve runs the
+// same bodies on its own block partition (Blk), partials and scalars as its
+// own launches would, so its results are the same bits.
+// ---------------------------------------------------------------------------
+enum StepShape { kShCell = 0, kShGrad, kShFlux, kShBc, kShFill, kShDtFill, kShapes };
+
+struct StepJob {
+  DMesh dm;
+  DParams dp;
+  double *prim, *bfa, *grad, *gradv, *q, *rhs, *dt, *partials;
+  const double *hook;
+  int *counters;
+  void *A, *rhs_ls, *phi;  // in the solve's precision
+  SolverScalars *scal;
+  StepMail *mail;
+  DtArgs dta;
+  size_t nA;
+  double gamma, relax, cfl_factor, cfl_max, ls_abs, ls_rel;
+  int ncv, cfl_after, cfl_interval, ls_maxiter;
+  int b[kShapes], n[kShapes];  // first block and block count in each launch shape
+};
+
+__device__ __forceinline__ Blk job_blk(const StepJob &J, int shape) {
+  return Blk{(int)blockIdx.x - J.b[shape], J.n[shape]};
+}
+
+// one block per solve
+__global__ void ks_ramp(const StepJob *__restrict__ jobs) {
+  const StepJob &J = jobs[blockIdx.x];
+  d_ramp(Blk{0, 1}, J.scal, J.cfl_after, J.cfl_interval, J.cfl_factor, J.cfl_max);
+}
+
+__global__ void ks_solver_init(const StepJob *__restrict__ jobs) {
+  const StepJob &J = jobs[blockIdx.x];
+  d_solver_init(Blk{0, 1}, J.scal, J.ls_maxiter, J.ls_abs, J.ls_rel);
+}
+
+__global__ void ks_publish(const StepJob *__restrict__ jobs) {
+  const StepJob &J = jobs[blockIdx.x];
+  d_publish(Blk{0, 1}, J.scal, J.mail);
+}
+
+// k_fin of each solve's partials of one launch shape
+__global__ void __launch_bounds__(kFinBlock) ks_fin(const StepJob *__restrict__ jobs, int op, int shape) {
+  __shared__ double sh[16];
+  const StepJob &J = jobs[blockIdx.x];
+  fin_body(J.n[shape], op, J.partials, (size_t)J.n[shape], J.scal, sh);
+}
+
+// block -> solve map of the launch shape at map
+template <bool LS, bool VISC, bool ENTRY>
+__global__ void __launch_bounds__(kBlock) ks_grad(const StepJob *__restrict__ jobs, const int *__restrict__ map) {
+  const StepJob &J = jobs[map[blockIdx.x]];
+  if constexpr (ENTRY)
+    d_grad_e<LS>(job_blk(J, kShGrad), J.dm, J.dp, J.prim, J.bfa, J.grad);
+  else
+    d_grad<LS, VISC>(job_blk(J, kShGrad), J.dm, J.dp, J.prim, J.bfa, J.grad, J.gradv);
+}
+
+template <int SCHEME, bool SND, bool VISC>
+__global__ void __launch_bounds__(kBlock) ks_flux(const StepJob *__restrict__ jobs, const int *__restrict__ map) {
+  const StepJob &J = jobs[map[blockIdx.x]];
+  d_flux_e<SCHEME, SND, VISC>(job_blk(J, kShFlux), J.dm, J.dp, J.prim, J.grad, J.bfa, J.q, J.rhs, J.counters + 1,
+                              J.dta, J.dt, J.partials, J.gradv);
+}
+
+__global__ void __launch_bounds__(kBlock) ks_dt_fill(const StepJob *__restrict__ jobs, const int *__restrict__ map) {
+  const StepJob &J = jobs[map[blockIdx.x]];
+  d_dt_fill(job_blk(J, kShDtFill), J.ncv, J.dt, J.scal);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) ks_fill(const StepJob *__restrict__ jobs, const int *__restrict__ map) {
+  const StepJob &J = jobs[map[blockIdx.x]];
+  d_fill<T>(job_blk(J, kShFill), J.nA, (T *)J.A, (T)0);
+}
+
+template <typename T, int SCHEME, bool VISC>
+__global__ void __launch_bounds__(kBlock) ks_jac(const StepJob *__restrict__ jobs, const int *__restrict__ map) {
+  const StepJob &J = jobs[map[blockIdx.x]];
+  d_jac<T, SCHEME, VISC>(job_blk(J, kShCell), J.dm, J.dp, J.prim, J.bfa, (T *)J.A, 1, J.dt, J.rhs, J.relax,
+                         (T *)J.rhs_ls, (T *)J.phi, J.gradv);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) ks_update(const StepJob *__restrict__ jobs, const int *__restrict__ map) {
+  const StepJob &J = jobs[map[blockIdx.x]];
+  d_update<T>(job_blk(J, kShCell), J.ncv, J.q, (const T *)J.phi);
+}
+
+__global__ void __launch_bounds__(kBlock) ks_state(const StepJob *__restrict__ jobs, const int *__restrict__ map) {
+  const StepJob &J = jobs[map[blockIdx.x]];
+  d_state(job_blk(J, kShCell), J.ncv, J.q, J.prim, J.gamma, J.counters);
+}
+
+__global__ void __launch_bounds__(kBlock) ks_bc(const StepJob *__restrict__ jobs, const int *__restrict__ map) {
+  const StepJob &J = jobs[map[blockIdx.x]];
+  d_bc(job_blk(J, kShBc), J.dm, J.dp, J.prim, J.hook, J.bfa);
+}
+
+__global__ void __launch_bounds__(kBlock) ks_l1(const StepJob *__restrict__ jobs, const int *__restrict__ map) {
+  const StepJob &J = jobs[map[blockIdx.x]];
+  d_l1(job_blk(J, kShCell), J.ncv, J.rhs, J.partials);
 }
 
 __global__ void k_to_double(int n, const float *__restrict__ s, double *__restrict__ d) {
@@ -3397,6 +3754,398 @@ StepInfo Ctx::device_step_result() {
 StepInfo Ctx::device_step() {
   device_step_async();
   return device_step_result();
+}
+
+KrylovBatchBuf::~KrylovBatchBuf() {
+  if (jobs_d) (void)hipFree(jobs_d);
+  if (map_d) (void)hipFree(map_d);
+  if (partials_d) (void)hipFree(partials_d);
+  if (flag_h) (void)hipHostFree(flag_h);
+}
+
+// The batched BiCGStab of njobs solves of one precision. Setup (not
+// capturable): block offsets of each solve in the three launch shapes, the
+// block -> solve maps, and the job table on the device.
+void setup_krylov_batch(KrylovJob *jobs_h, int njobs, void *stream, KrylovBatchBuf &buf) {
+  hipStream_t st = (hipStream_t)stream;
+  int g5 = 0, g1 = 0, gx = 0, max_it = 0;
+  for (int j = 0; j < njobs; j++) {
+    KrylovJob &J = jobs_h[j];
+    J.b5 = g5;
+    J.nb5 = (J.ncv + kCellsPerBlock - 1) / kCellsPerBlock;
+    J.b1 = g1;
+    J.nb1 = nblk(J.ncv);
+    J.bx = gx;
+    J.nbx = nblk(5 * (long long)J.ncv);
+    g5 += J.nb5;
+    g1 += J.nb1;
+    gx += J.nbx;
+    max_it = std::max(max_it, J.maxiter + 1);
+  }
+  buf.njobs = njobs;
+  buf.g5 = g5;
+  buf.g1 = g1;
+  buf.gx = gx;
+  buf.max_it = max_it;
+  if (njobs <= 0) return;
+  std::vector<int> map((size_t)g5 + g1 + gx);
+  for (int j = 0; j < njobs; j++) {
+    const KrylovJob &J = jobs_h[j];
+    for (int b = 0; b < J.nb5; b++) map[J.b5 + b] = j;
+    for (int b = 0; b < J.nb1; b++) map[(size_t)g5 + J.b1 + b] = j;
+    for (int b = 0; b < J.nbx; b++) map[(size_t)g5 + g1 + J.bx + b] = j;
+  }
+  if (buf.cap_jobs < (size_t)njobs) {
+    if (buf.jobs_d) HIPCHECK(hipFree(buf.jobs_d));
+    buf.cap_jobs = std::max<size_t>(njobs, 2 * buf.cap_jobs);
+    HIPCHECK(hipMalloc((void **)&buf.jobs_d, sizeof(KrylovJob) * buf.cap_jobs));
+  }
+  if (buf.cap_map < map.size()) {
+    if (buf.map_d) HIPCHECK(hipFree(buf.map_d));
+    buf.cap_map = std::max(map.size(), 2 * buf.cap_map);
+    HIPCHECK(hipMalloc((void **)&buf.map_d, sizeof(int) * buf.cap_map));
+  }
+  if (buf.cap_partials < 2 * (size_t)g5 + 8) {
+    if (buf.partials_d) HIPCHECK(hipFree(buf.partials_d));
+    buf.cap_partials = std::max<size_t>(2 * (size_t)g5 + 8, 2 * buf.cap_partials);
+    HIPCHECK(hipMalloc((void **)&buf.partials_d, sizeof(double) * buf.cap_partials));
+  }
+  if (!buf.flag_h) {
+    HIPCHECK(hipHostMalloc((void **)&buf.flag_h, sizeof(int), hipHostMallocMapped));
+    HIPCHECK(hipHostGetDevicePointer((void **)&buf.flag_d, buf.flag_h, 0));
+  }
+  HIPCHECK(hipMemcpyAsync(buf.jobs_d, jobs_h, sizeof(KrylovJob) * njobs, hipMemcpyHostToDevice, st));
+  HIPCHECK(hipMemcpyAsync(buf.map_d, map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice, st));
+  HIPCHECK(hipStreamSynchronize(st));
+}
+
+template <typename T>
+static void krylov_batch_prologue(hipStream_t st, const KrylovBatchBuf &buf) {
+  const int *m5 = buf.map_d, *m1 = buf.map_d + buf.g5;
+  hipLaunchKernelGGL(kb_lu<T>, dim3(buf.g1), dim3(kBlock), 0, st, buf.jobs_d, m1);
+  hipLaunchKernelGGL(kb_resid<T>, dim3(buf.g5), dim3(kBlock), 0, st, buf.jobs_d, m5, 1, buf.partials_d);
+  hipLaunchKernelGGL(kb_fin, dim3(buf.njobs), dim3(kFinBlock), 0, st, buf.jobs_d, (int)kFinResid0, buf.partials_d,
+                     buf.g5);
+}
+
+template <typename T>
+static void krylov_batch_iterations(hipStream_t st, const KrylovBatchBuf &buf, int count) {
+  const int *m5 = buf.map_d, *m1 = buf.map_d + buf.g5, *mx = buf.map_d + buf.g5 + buf.g1;
+  const KrylovJob *jd = buf.jobs_d;
+  double *pt = buf.partials_d;
+  const int g5 = buf.g5, g1 = buf.g1, gx = buf.gx, nj = buf.njobs;
+  for (int k = 0; k < count; k++) {
+    hipLaunchKernelGGL(kb_ps<T>, dim3(g1), dim3(kBlock), 0, st, jd, m1, 0);
+    hipLaunchKernelGGL(kb_spmv_dot<T>, dim3(g5), dim3(kBlock), 0, st, jd, m5, 0, pt);
+    hipLaunchKernelGGL(kb_fin, dim3(nj), dim3(kFinBlock), 0, st, jd, (int)kFinAlpha, pt, g5);
+    hipLaunchKernelGGL(kb_ps<T>, dim3(g1), dim3(kBlock), 0, st, jd, m1, 1);
+    hipLaunchKernelGGL(kb_spmv_dot<T>, dim3(g5), dim3(kBlock), 0, st, jd, m5, 1, pt);
+    hipLaunchKernelGGL(kb_fin, dim3(nj), dim3(kFinBlock), 0, st, jd, (int)kFinOmega, pt, g5);
+    hipLaunchKernelGGL(kb_x<T>, dim3(gx), dim3(kBlock), 0, st, jd, mx);
+    hipLaunchKernelGGL(kb_resid<T>, dim3(g5), dim3(kBlock), 0, st, jd, m5, 0, pt);
+    hipLaunchKernelGGL(kb_fin, dim3(nj), dim3(kFinBlock), 0, st, jd, (int)kFinResid, pt, g5);
+  }
+}
+
+// The batch set up last, with host polling between chunks of 2, then 4
+// iterations (a chunk's launches after a solve's stop return at once)
+void run_krylov_batch(int precision, void *stream, KrylovBatchBuf &buf) {
+  if (buf.njobs <= 0) return;
+  hipStream_t st = (hipStream_t)stream;
+  if (precision == 0) krylov_batch_prologue<double>(st, buf);
+  else krylov_batch_prologue<float>(st, buf);
+  int launched = 0;
+  while (launched < buf.max_it) {
+    const int chunk = std::min(launched == 0 ? 2 : 4, buf.max_it - launched);
+    if (precision == 0) krylov_batch_iterations<double>(st, buf, chunk);
+    else krylov_batch_iterations<float>(st, buf, chunk);
+    launched += chunk;
+    hipLaunchKernelGGL(kb_all_done, dim3(1), dim3(256), 0, st, buf.jobs_d, buf.njobs, buf.flag_d);
+    HIPCHECK(hipStreamSynchronize(st));
+    if (*(volatile int *)buf.flag_h) break;
+  }
+  HIPCHECK(hipGetLastError());
+}
+
+int Ctx::precision() const { return d_->precision; }
+
+uint32_t Ctx::batch_key() const {
+  const Impl &d = *d_;
+  const bool grad = d.par.second_order || d.dp.visc;
+  const bool ent = grad && d.par.grad_kernel == kResidEntry && !d.dp.visc;
+  return (uint32_t)d.precision | (uint32_t)d.par.scheme << 1 | (uint32_t)(d.par.second_order ? 1 : 0) << 3 |
+         (uint32_t)(d.par.grad_ls ? 1 : 0) << 4 | (uint32_t)(d.dp.visc ? 1 : 0) << 5 | (uint32_t)(ent ? 1 : 0) << 6;
+}
+
+struct StepBatch::Impl {
+  StepJob *jobs_d = nullptr;
+  int *map_d = nullptr;
+  size_t cap_jobs = 0, cap_map = 0;
+  int njobs = 0;
+  int tot[kShapes] = {};
+  size_t off[kShapes] = {};
+  int precision = 0, scheme = 0;
+  bool snd = false, ls = false, visc = false, entry = false;
+  ~Impl() {
+    if (jobs_d) (void)hipFree(jobs_d);
+    if (map_d) (void)hipFree(map_d);
+  }
+};
+
+StepBatch::StepBatch() : d_(new Impl) {}
+StepBatch::~StepBatch() = default;
+int StepBatch::size() const { return d_->njobs; }
+
+void StepBatch::set(Ctx *const *ctxs, int n, void *stream) {
+  Impl &B = *d_;
+  hipStream_t st = (hipStream_t)stream;
+  B.njobs = 0;
+  if (n <= 0) return;
+  const uint32_t key = ctxs[0]->batch_key();
+  std::vector<StepJob> jobs(n);
+  int tot[kShapes] = {};
+  for (int j = 0; j < n; j++) {
+    if (ctxs[j]->batch_key() != key) throw std::logic_error("a step batch mixes kernel configurations");
+    Ctx::Impl &d = *ctxs[j]->d_;
+    StepJob &J = jobs[j];
+    std::memset(&J, 0, sizeof J);
+    J.dm = d.dm;
+    J.dp = d.dp;
+    J.prim = d.prim.p;
+    J.bfa = d.bfa.p;
+    J.grad = d.grad.p;
+    J.gradv = d.gradv.p;
+    J.q = d.q.p;
+    J.rhs = d.rhs.p;
+    J.dt = d.dt.p;
+    J.partials = d.partials.p;
+    J.hook = d.hook.p;
+    J.counters = d.counters.p;
+    if (d.precision == 0) {
+      J.A = d.A64.p;
+      J.rhs_ls = d.v64[0].p;
+      J.phi = d.v64[1].p;
+    } else {
+      J.A = d.A32.p;
+      J.rhs_ls = d.v32[0].p;
+      J.phi = d.v32[1].p;
+    }
+    J.scal = d.scal.p;
+    J.mail = d.mail_d;
+    J.dta.cfl = d.cfl;
+    J.dta.dt_fixed = d.par.dt_fixed;
+    J.dta.mode = d.par.dt_mode;
+    J.dta.cfl_dev = &d.scal.p->cfl_cur;
+    J.nA = 25 * (size_t)d.nnz;
+    J.gamma = d.par.gamma;
+    J.relax = d.par.under_relax;
+    J.cfl_factor = d.par.cfl_factor;
+    J.cfl_max = d.par.cfl_max;
+    J.ls_abs = d.par.ls_abs;
+    J.ls_rel = d.par.ls_rel;
+    J.ncv = d.ncv;
+    J.cfl_after = d.par.cfl_after;
+    J.cfl_interval = d.par.cfl_interval;
+    J.ls_maxiter = d.par.ls_maxiter;
+    // the grids of the solve's own launches (launch_grad, launch_flux,
+    // launch_jac, launch_state_bc)
+    const bool grad = d.par.second_order || d.dp.visc;
+    const bool ent = d.par.grad_kernel == kResidEntry && !d.dp.visc;
+    J.n[kShCell] = nblk(d.ncv);
+    J.n[kShGrad] = !grad ? 0 : ent ? (d.nwave + kWaves - 1) / kWaves : nblk(d.ncv);
+    J.n[kShFlux] = d.flux_blocks();
+    J.n[kShBc] = d.nfa_b > 0 ? nblk(d.nfa_b) : 0;
+    J.n[kShFill] = d.dup_blocks ? (int)((J.nA + kBlock - 1) / kBlock) : 0;
+    J.n[kShDtFill] = d.par.dt_mode == 1 ? nblk(d.ncv) : 0;
+    for (int k = 0; k < kShapes; k++) {
+      J.b[k] = tot[k];
+      tot[k] += J.n[k];
+    }
+    if (j == 0) {
+      B.precision = d.precision;
+      B.scheme = d.par.scheme;
+      B.snd = d.par.second_order;
+      B.ls = d.par.grad_ls;
+      B.visc = d.dp.visc != 0;
+      B.entry = grad && ent;
+    }
+  }
+  size_t nmap = 0;
+  for (int k = 0; k < kShapes; k++) {
+    B.off[k] = nmap;
+    B.tot[k] = tot[k];
+    nmap += tot[k];
+  }
+  std::vector<int> map(nmap);
+  for (int j = 0; j < n; j++)
+    for (int k = 0; k < kShapes; k++)
+      for (int b = 0; b < jobs[j].n[k]; b++) map[B.off[k] + jobs[j].b[k] + b] = j;
+  if (B.cap_jobs < (size_t)n) {
+    if (B.jobs_d) HIPCHECK(hipFree(B.jobs_d));
+    B.jobs_d = nullptr;
+    B.cap_jobs = std::max<size_t>(n, 2 * B.cap_jobs);
+    HIPCHECK(hipMalloc((void **)&B.jobs_d, sizeof(StepJob) * B.cap_jobs));
+  }
+  if (B.cap_map < nmap) {
+    if (B.map_d) HIPCHECK(hipFree(B.map_d));
+    B.map_d = nullptr;
+    B.cap_map = std::max(nmap, 2 * B.cap_map);
+    HIPCHECK(hipMalloc((void **)&B.map_d, sizeof(int) * B.cap_map));
+  }
+  HIPCHECK(hipMemcpyAsync(B.jobs_d, jobs.data(), sizeof(StepJob) * n, hipMemcpyHostToDevice, st));
+  HIPCHECK(hipMemcpyAsync(B.map_d, map.data(), sizeof(int) * nmap, hipMemcpyHostToDevice, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  B.njobs = n;
+}
+
+template <typename T>
+static void step_batch_jac(const StepBatch::Impl &B, hipStream_t st) {
+  const StepJob *jd = B.jobs_d;
+  if (B.tot[kShFill])
+    hipLaunchKernelGGL(ks_fill<T>, dim3(B.tot[kShFill]), dim3(kBlock), 0, st, jd, B.map_d + B.off[kShFill]);
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(B.tot[kShCell]), dim3(kBlock), 0, st, jd, B.map_d + B.off[kShCell]);
+  };
+  if (B.visc) {
+    if (B.scheme == kRoe) go(ks_jac<T, kRoe, true>);
+    else go(ks_jac<T, kHllc, true>);
+  } else {
+    if (B.scheme == kRoe) go(ks_jac<T, kRoe, false>);
+    else go(ks_jac<T, kHllc, false>);
+  }
+}
+
+void StepBatch::pre(void *stream) const {
+  const Impl &B = *d_;
+  if (B.njobs == 0) return;
+  hipStream_t st = (hipStream_t)stream;
+  const StepJob *jd = B.jobs_d;
+  hipLaunchKernelGGL(ks_ramp, dim3(B.njobs), dim3(64), 0, st, jd);
+  if (B.tot[kShGrad]) {
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(B.tot[kShGrad]), dim3(kBlock), 0, st, jd, B.map_d + B.off[kShGrad]);
+    };
+    if (B.entry) {
+      if (B.ls) go(ks_grad<true, false, true>);
+      else go(ks_grad<false, false, true>);
+    } else if (B.visc) {
+      if (B.ls) go(ks_grad<true, true, false>);
+      else go(ks_grad<false, true, false>);
+    } else {
+      if (B.ls) go(ks_grad<true, false, false>);
+      else go(ks_grad<false, false, false>);
+    }
+  }
+  {
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(B.tot[kShFlux]), dim3(kBlock), 0, st, jd, B.map_d + B.off[kShFlux]);
+    };
+    if (B.visc) {
+      if (B.snd) {
+        if (B.scheme == kRoe) go(ks_flux<kRoe, true, true>);
+        else if (B.scheme == kAusm) go(ks_flux<kAusm, true, true>);
+        else go(ks_flux<kHllc, true, true>);
+      } else {
+        if (B.scheme == kRoe) go(ks_flux<kRoe, false, true>);
+        else if (B.scheme == kAusm) go(ks_flux<kAusm, false, true>);
+        else go(ks_flux<kHllc, false, true>);
+      }
+    } else {
+      if (B.snd) {
+        if (B.scheme == kRoe) go(ks_flux<kRoe, true, false>);
+        else if (B.scheme == kAusm) go(ks_flux<kAusm, true, false>);
+        else go(ks_flux<kHllc, true, false>);
+      } else {
+        if (B.scheme == kRoe) go(ks_flux<kRoe, false, false>);
+        else if (B.scheme == kAusm) go(ks_flux<kAusm, false, false>);
+        else go(ks_flux<kHllc, false, false>);
+      }
+    }
+  }
+  hipLaunchKernelGGL(ks_fin, dim3(B.njobs), dim3(kFinBlock), 0, st, jd, (int)kFinDt, (int)kShFlux);
+  if (B.tot[kShDtFill])
+    hipLaunchKernelGGL(ks_dt_fill, dim3(B.tot[kShDtFill]), dim3(kBlock), 0, st, jd, B.map_d + B.off[kShDtFill]);
+  if (B.precision == 0) step_batch_jac<double>(B, st);
+  else step_batch_jac<float>(B, st);
+  hipLaunchKernelGGL(ks_solver_init, dim3(B.njobs), dim3(64), 0, st, jd);
+  HIPCHECK(hipGetLastError());
+}
+
+void StepBatch::post(void *stream) const {
+  const Impl &B = *d_;
+  if (B.njobs == 0) return;
+  hipStream_t st = (hipStream_t)stream;
+  const StepJob *jd = B.jobs_d;
+  const int *mc = B.map_d + B.off[kShCell];
+  const dim3 gc(B.tot[kShCell]);
+  if (B.precision == 0) hipLaunchKernelGGL(ks_update<double>, gc, dim3(kBlock), 0, st, jd, mc);
+  else hipLaunchKernelGGL(ks_update<float>, gc, dim3(kBlock), 0, st, jd, mc);
+  hipLaunchKernelGGL(ks_state, gc, dim3(kBlock), 0, st, jd, mc);
+  if (B.tot[kShBc])
+    hipLaunchKernelGGL(ks_bc, dim3(B.tot[kShBc]), dim3(kBlock), 0, st, jd, B.map_d + B.off[kShBc]);
+  hipLaunchKernelGGL(ks_l1, gc, dim3(kBlock), 0, st, jd, mc);
+  hipLaunchKernelGGL(ks_fin, dim3(B.njobs), dim3(kFinBlock), 0, st, jd, (int)kFinL1, (int)kShCell);
+  hipLaunchKernelGGL(ks_publish, dim3(B.njobs), dim3(64), 0, st, jd);
+  HIPCHECK(hipGetLastError());
+}
+
+void Ctx::batch_prepare() {
+  Impl &d = *d_;
+  if (d.par.sa) throw Unsupported("batched serving does not carry the Spalart-Allmaras scalar");
+  if (!d.flux_entry() || d.face_form())
+    throw Unsupported("batched serving runs the per-entry flux kernel (GPU_FLUX_KERNEL = ENTRY)");
+  if (!d.have_state) throw std::logic_error("no state set");
+  if (d.precision == 0) {
+    ensure(d.A64, 25 * (size_t)d.nnz);
+    ensure(d.LU64, 25 * (size_t)d.ncv);
+    for (int k = 0; k < 10; k++) ensure(d.v64[k], 5 * (size_t)d.ncv);
+  } else {
+    ensure(d.A32, 25 * (size_t)d.nnz);
+    ensure(d.LU32, 25 * (size_t)d.ncv);
+    for (int k = 0; k < 10; k++) ensure(d.v32[k], 5 * (size_t)d.ncv);
+  }
+  if (!d.mail_h) {
+    HIPCHECK(hipHostMalloc((void **)&d.mail_h, sizeof(StepMail), hipHostMallocMapped));
+    std::memset(d.mail_h, 0, sizeof(StepMail));
+    HIPCHECK(hipHostGetDevicePointer((void **)&d.mail_d, d.mail_h, 0));
+  }
+  if (d.step_no == 0) d.launch_state_bc();
+  hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, d.stream, d.scal.p, d.step_no, d.cfl);
+  d.dev_step_dirty = false;
+  HIPCHECK(hipStreamSynchronize(d.stream));
+}
+
+void Ctx::batch_job(KrylovJob &job) {
+  Impl &d = *d_;
+  job.ncv = d.ncv;
+  job.maxiter = d.par.ls_maxiter;
+  job.nbocv_i = d.nbocv_i.p;
+  job.nbocv_v = d.nbocv_v.p;
+  job.S = d.scal.p;
+  if (d.precision == 0) {
+    job.A = d.A64.p;
+    job.LU = d.LU64.p;
+    for (int k = 0; k < 10; k++) job.v[k] = d.v64[k].p;
+  } else {
+    job.A = d.A32.p;
+    job.LU = d.LU32.p;
+    for (int k = 0; k < 10; k++) job.v[k] = d.v32[k].p;
+  }
+}
+
+StepInfo Ctx::batch_result() {
+  Impl &d = *d_;
+  const volatile StepMail *m = d.mail_h;
+  StepInfo info;
+  info.step = m->step;
+  info.lin_iters = m->lin_iters;
+  info.lin_status = m->lin_status;
+  info.cfl = m->cfl;
+  info.dt_min = m->dt_min;
+  for (int k = 0; k < 5; k++) info.resid[k] = m->resid[k];
+  d.step_no = info.step;
+  d.cfl = info.cfl;
+  return info;
 }
 
 int Ctx::ncv() const { return d_->ncv; }

@@ -74,7 +74,7 @@ def lib():
         L.jm_time_residual_kernels.argtypes = [vp, C.c_int, vp, C.c_char_p, C.c_int]
         L.jm_mesh_quality.argtypes = [vp, vp, vp]
         L.jm_ctx_footprint.argtypes = [vp, C.c_int, vp]
-        L.jm_server_create.argtypes = [C.c_int64, C.c_int, vp]
+        L.jm_server_create.argtypes = [C.c_int64, C.c_int, C.c_int, vp]
         L.jm_server_destroy.argtypes = [vp]
         L.jm_server_destroy.restype = None
         L.jm_server_submit.argtypes = [vp, vp, C.c_char_p, C.c_int, vp, vp, vp, vp]
@@ -261,13 +261,15 @@ class Solver:
 
 class Server:
     """Many steady solves on one GPU (jm_server_*): admission by device
-    footprint, one stream and graph-captured steps per active solve."""
+    footprint; batched=False: one stream and graph-captured steps per active
+    solve; batched=True: the active solves step in lockstep, each kernel of
+    the step and of the BiCGStab launched once for all of them."""
 
     QUEUED, RUNNING, DONE, FAILED = 0, 1, 2, 3
 
-    def __init__(self, mem_budget_bytes: int, max_active: int = 8):
+    def __init__(self, mem_budget_bytes: int, max_active: int = 8, batched: bool = False):
         self.h = C.c_void_p()
-        _check(lib().jm_server_create(int(mem_budget_bytes), int(max_active), C.byref(self.h)))
+        _check(lib().jm_server_create(int(mem_budget_bytes), int(max_active), 1 if batched else 0, C.byref(self.h)))
         self._meshes = {}
 
     def close(self):

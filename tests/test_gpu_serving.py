@@ -1,6 +1,8 @@
 """The serving loop (jm_server_*): solves admitted by footprint, run
-concurrently on their own streams with graph-captured steps. Each result must
-be the same bits as the same solve run alone through the ordinary step."""
+concurrently on their own streams with graph-captured steps, or batched (in
+lockstep, each kernel of the step and of the BiCGStab launched once for all
+active solves). Each result must be the same bits as the same solve run
+alone through the ordinary step."""
 import numpy as np
 import pytest
 
@@ -74,4 +76,79 @@ def test_bad_request_is_rejected_at_submission():
     with pytest.raises(joemi.JoemiError) as e:
         srv.submit(mesh, joe.replace("bump_wall = WALL", "bump_wall = NOSUCHBC"))
     assert e.value.status == 3
+    srv.close()
+
+
+def _served(built, batched, max_active):
+    srv = joemi.Server(mem_budget_bytes=8 << 30, max_active=max_active, batched=batched)
+    tickets = [srv.submit(m, joe, prec) for (m, joe, prec) in built]
+    for t in tickets:
+        srv.wait(t)
+    out = []
+    for t in tickets:
+        state, rep = srv.result(t)
+        assert rep.status == joemi.Server.DONE, rep.status
+        out.append((state, rep))
+        srv.release(t)
+    srv.close()
+    return out
+
+
+def test_batched_results_same_alone_and_in_the_mix():
+    # batched serving: solves in lockstep, each kernel launched once for all
+    # of them; every solve keeps its own blocks and reduction order,
+    # so a result is the same bits alone (a batch of one) as in a batch of three
+    print("serving: batched mix", flush=True)
+    built = []
+    for k, r in enumerate(REQS):
+        mm, joe = _case(r, nsteps=25)
+        built.append((joemi.Mesh.from_faces(mm), joe, joemi.JM_MIXED if k == 1 else joemi.JM_FP64))
+    mix = _served(built, True, 3)
+    for k, b in enumerate(built):
+        (alone,) = _served([b], True, 1)
+        (s1, r1), (s2, r2) = mix[k], alone
+        assert r1.steps == r2.steps == 25
+        for a, c in zip(s1, s2):
+            assert np.array_equal(a, c), k
+        assert list(r1.resid) == list(r2.resid)
+
+
+@pytest.mark.parametrize("precision", [joemi.JM_FP64, joemi.JM_MIXED])
+def test_batched_step_equals_the_ordinary_step(precision):
+    # a solve in a batch is cut into the same blocks and reduced in the same
+    # order as its own host-polled solve: the same bits after 25 steps
+    print(f"serving: batched vs ordinary precision={precision}", flush=True)
+    built = []
+    for r in REQS:
+        mm, joe = _case(r, nsteps=25)
+        built.append((joemi.Mesh.from_faces(mm), joe, precision))
+    served = _served(built, True, 3)
+    for (state, rep), (mesh, joe, prec) in zip(served, built):
+        (r2, ru2, rE2), n2, hist2 = _alone(mesh, joe, prec)
+        assert rep.steps == n2 == 25
+        for a, b in zip(state, (r2, ru2, rE2)):
+            assert np.array_equal(a, b)
+        assert list(rep.resid) == list(hist2[-1])
+
+
+def test_batched_rejects_what_it_does_not_run():
+    # the batched step carries no SA scalar and runs the per-entry flux
+    # kernel only: such requests fail at submission, not mid-run
+    from myjoe_amd import meshgen
+    from tests.test_oracle_golden import joe_text, load
+
+    srv = joemi.Server(mem_budget_bytes=1 << 30, max_active=2, batched=True)
+    g = load("solve_bump24sa")
+    sa_mesh = joemi.Mesh.from_faces(meshgen.bump_channel(24, 8))
+    with pytest.raises(joemi.JoemiError) as e:
+        srv.submit(sa_mesh, joe_text(g))
+    assert e.value.status == 5
+    mm, joe = _case(REQS[0], nsteps=5)
+    mesh = joemi.Mesh.from_faces(mm)
+    with pytest.raises(joemi.JoemiError) as e:
+        srv.submit(mesh, joe + "GPU_FLUX_KERNEL = FACE\n")
+    assert e.value.status == 5
+    t = srv.submit(mesh, joe)  # the same case with the default kernels runs
+    srv.wait(t)
+    assert srv.result(t)[1].status == joemi.Server.DONE
     srv.close()
