@@ -304,6 +304,9 @@ __device__ __forceinline__ double bj_face(double alfa, double phi, double phifa,
 // ---------------------------------------------------------------------------
 // enthalpy of a cell state (calcStateVariables: T = p / (rho R),
 // h = gamma R / (gamma - 1) T; UgpWithCvCompFlow.h:1680-1690)
+// entries of a cell whose loads the per-cell gradient issues together
+constexpr int kGradChunk = 4;
+
 __device__ __forceinline__ double enthalpy_of(const DParams &P, double rho, double p) {
   const double T = p / (rho * P.R);
   return P.gamma * P.R / (P.gamma - 1.0) * T;
@@ -340,33 +343,53 @@ __device__ __forceinline__ void d_grad(const Blk &B, DMesh m, DParams P, const d
     }
     for (int i = 0; i < 3; i++)
       for (int j = 0; j < 3; j++) gu[i][j] = l0[j] * u[i];
-    int k = 1;
-    for (int e = e0; e < e1; e++) {
-      int2 en = m.cfl[e];
-      if (en.y < 0) break;
-      const double *pn = prim + 8 * (size_t)en.y;
-      double2 a = ld2(pn), b = ld2(pn + 2);
-      double rn = a.x, un[3] = {a.y, b.x, b.y}, pnv = pn[4];
-      const double *l = m.lsg + 3 * (size_t)(noc_f + k);
-      double l_0 = l[0], l_1 = l[1], l_2 = l[2];
-      gr[0] += l_0 * rn; gr[1] += l_1 * rn; gr[2] += l_2 * rn;
-      gp[0] += l_0 * pnv; gp[1] += l_1 * pnv; gp[2] += l_2 * pnv;
-      if (VISC) {
-        const double hn = enthalpy_of(P, rn, pnv);
-        gh[0] += l_0 * hn; gh[1] += l_1 * hn; gh[2] += l_2 * hn;
-        hmax = Dev::mx(hmax, hn); hmin = Dev::mn(hmin, hn);
+    // internal entries (they come first) in chunks of kGradChunk: a chunk's
+    // entries, then its neighbour rows and coefficients, are loaded together
+    // (slots past the internal ones read the cell's own row, always valid)
+    // and added in entry order
+    const int noc_l = m.nbocv_i[c + 1] - 1;
+    bool open = true;
+    for (int eb = e0; open && eb < e1; eb += kGradChunk) {
+      int2 en[kGradChunk];
+#pragma unroll
+      for (int q = 0; q < kGradChunk; q++) en[q] = eb + q < e1 ? m.cfl[eb + q] : make_int2(0, -1);
+      double2 a[kGradChunk], b[kGradChunk];
+      double pv[kGradChunk], l[kGradChunk][3];
+#pragma unroll
+      for (int q = 0; q < kGradChunk; q++) {
+        const double *pn = prim + 8 * (size_t)(en[q].y >= 0 ? en[q].y : c);
+        a[q] = ld2(pn);
+        b[q] = ld2(pn + 2);
+        pv[q] = pn[4];
+        const double *lq = m.lsg + 3 * (size_t)min(noc_f + 1 + (eb - e0) + q, noc_l);
+        l[q][0] = lq[0];
+        l[q][1] = lq[1];
+        l[q][2] = lq[2];
       }
-      for (int i = 0; i < 3; i++) {
-        gu[i][0] += l_0 * un[i];
-        gu[i][1] += l_1 * un[i];
-        gu[i][2] += l_2 * un[i];
-        umax[i] = Dev::mx(umax[i], un[i]);
-        umin[i] = Dev::mn(umin[i], un[i]);
+#pragma unroll
+      for (int q = 0; q < kGradChunk; q++) {
+        open = open && en[q].y >= 0;
+        if (!open) break;
+        const double rn = a[q].x, un[3] = {a[q].y, b[q].x, b[q].y}, pnv = pv[q];
+        const double l_0 = l[q][0], l_1 = l[q][1], l_2 = l[q][2];
+        gr[0] += l_0 * rn; gr[1] += l_1 * rn; gr[2] += l_2 * rn;
+        gp[0] += l_0 * pnv; gp[1] += l_1 * pnv; gp[2] += l_2 * pnv;
+        if (VISC) {
+          const double hn = enthalpy_of(P, rn, pnv);
+          gh[0] += l_0 * hn; gh[1] += l_1 * hn; gh[2] += l_2 * hn;
+          hmax = Dev::mx(hmax, hn); hmin = Dev::mn(hmin, hn);
+        }
+        for (int i = 0; i < 3; i++) {
+          gu[i][0] += l_0 * un[i];
+          gu[i][1] += l_1 * un[i];
+          gu[i][2] += l_2 * un[i];
+          umax[i] = Dev::mx(umax[i], un[i]);
+          umin[i] = Dev::mn(umin[i], un[i]);
+        }
+        rmax = Dev::mx(rmax, rn); rmin = Dev::mn(rmin, rn);
+        pmax = Dev::mx(pmax, pnv); pmin = Dev::mn(pmin, pnv);
+        e_int = eb + q + 1;
       }
-      rmax = Dev::mx(rmax, rn); rmin = Dev::mn(rmin, rn);
-      pmax = Dev::mx(pmax, pnv); pmin = Dev::mn(pmin, pnv);
-      k++;
-      e_int = e + 1;
     }
     for (int e = e0; e < e1; e++) {
       int2 en = m.cfl[e];
@@ -481,13 +504,25 @@ __device__ __forceinline__ void d_grad(const Blk &B, DMesh m, DParams P, const d
     e2p = e2p * e2p;
     bool lu[3] = {(umax[0] - umin[0]) > 1.0e-8, (umax[1] - umin[1]) > 1.0e-8, (umax[2] - umin[2]) > 1.0e-8};
     bool lr = (rmax - rmin) > 1.0e-12, lp = (pmax - pmin) > 1.0e-12;
-    for (int e = e0; e < e_int; e++) {
-      const double *rr = m.cfl_r0 + 3 * (size_t)e;
-      double r0[3] = {rr[0], rr[1], rr[2]};
-      for (int i = 0; i < 3; i++)
-        if (lu[i]) au[i] = bj_face(au[i], u[i], u[i] + dot3(r0, gu[i]), umax[i], umin[i], e2u);
-      if (lr) ar = bj_face(ar, rho, rho + dot3(r0, gr), rmax, rmin, e2r);
-      if (lp) ap = bj_face(ap, p, p + dot3(r0, gp), pmax, pmin, e2p);
+    // face offsets in chunks loaded together (slots past e_int repeat the
+    // last entry and are not applied)
+    for (int eb = e0; eb < e_int; eb += kGradChunk) {
+      double r0[kGradChunk][3];
+#pragma unroll
+      for (int q = 0; q < kGradChunk; q++) {
+        const double *rr = m.cfl_r0 + 3 * (size_t)min(eb + q, e_int - 1);
+        r0[q][0] = rr[0];
+        r0[q][1] = rr[1];
+        r0[q][2] = rr[2];
+      }
+#pragma unroll
+      for (int q = 0; q < kGradChunk; q++) {
+        if (eb + q >= e_int) break;
+        for (int i = 0; i < 3; i++)
+          if (lu[i]) au[i] = bj_face(au[i], u[i], u[i] + dot3(r0[q], gu[i]), umax[i], umin[i], e2u);
+        if (lr) ar = bj_face(ar, rho, rho + dot3(r0[q], gr), rmax, rmin, e2r);
+        if (lp) ap = bj_face(ap, p, p + dot3(r0[q], gp), pmax, pmin, e2p);
+      }
     }
     for (int i = 0; i < 3; i++)
       if (!lu[i]) au[i] = 0.0;
