@@ -923,13 +923,9 @@ __device__ __forceinline__ double flux_cell(const DMesh &m, const DParams &P, co
         so = own_cc;
         sn.rho = pn[0]; sn.u[0] = pn[1]; sn.u[1] = pn[2]; sn.u[2] = pn[3]; sn.p = pn[4];
       }
-      if (s0) {
-        face_flux(SCHEME, F, so, sn, gam, area, n);
-        for (int k = 0; k < 5; k++) acc[k] -= F[k];
-      } else {
-        face_flux(SCHEME, F, sn, so, gam, area, n);
-        for (int k = 0; k < 5; k++) acc[k] += F[k];
-      }
+      // one flux on the face's (icv0, icv1) sides; acc - F == acc + (-F)
+      face_flux(SCHEME, F, s0 ? so : sn, s0 ? sn : so, gam, area, n);
+      for (int k = 0; k < 5; k++) acc[k] += s0 ? -F[k] : F[k];
     } else {
       const ZoneDev &z = m.zones[-en.y - 1];
       const double *bb = bfa + 8 * (size_t)f;
@@ -1207,18 +1203,26 @@ __device__ __forceinline__ void d_jac(const Blk &B, DMesh m, DParams P, const do
       const int noc = m.cfl_noc[e];
       const bool accum = (noc >> 30) & 1;
       T *off = A + 25 * (size_t)(noc & 0x3fffffff);
-      if (en.x >= 0) {  // this cell is icv0: A[noc00] += Apl, A[noc01] += Ami
-        face_jac(SCHEME, J, 0, own, nb, gam, area, n);
+      // this cell is icv0: A[noc00] += Apl, A[noc01] += Ami; icv1:
+      // A[noc11] -= Ami, A[noc10] -= Apl. Both blocks are evaluated once on
+      // the lane's own (icv0, icv1) sides, so lanes of either kind share the
+      // arithmetic; the diagonal and the off-diagonal block are different
+      // memory, so the order of their updates does not matter.
+      const bool s0 = en.x >= 0;
+      const Side l = s0 ? own : nb, r = s0 ? nb : own;
+      face_jac(SCHEME, J, 0, l, r, gam, area, n);  // Apl
+      if (s0) {
         for (int i = 0; i < 5; i++)
           for (int j = 0; j < 5; j++) D[i][j] += J[i][j];
-        face_jac(SCHEME, J, 1, own, nb, gam, area, n);
+      } else {
+        store_block(off, J, true, accum);
+      }
+      face_jac(SCHEME, J, 1, l, r, gam, area, n);  // Ami
+      if (s0) {
         store_block(off, J, false, accum);
-      } else {  // this cell is icv1: A[noc11] -= Ami, A[noc10] -= Apl
-        face_jac(SCHEME, J, 1, nb, own, gam, area, n);
+      } else {
         for (int i = 0; i < 5; i++)
           for (int j = 0; j < 5; j++) D[i][j] -= J[i][j];
-        face_jac(SCHEME, J, 0, nb, own, gam, area, n);
-        store_block(off, J, true, accum);
       }
     } else {
       const ZoneDev &z = m.zones[-en.y - 1];
@@ -1282,20 +1286,23 @@ __device__ __forceinline__ void d_jac(const Blk &B, DMesh m, DParams P, const do
         const double ufa[3] = {0.5 * (a.u[0] + b.u[0]), 0.5 * (a.u[1] + b.u[1]), 0.5 * (a.u[2] + b.u[2])};
         visc_face(P, a, b, mul, mut, ufa, area, n, smag, sV, Fu, FE, vf);
         T *off = A + 25 * (size_t)(m.cfl_noc[e] & 0x3fffffff);
-        if (s0) {  // A[noc00] -= A0, A[noc01] -= A1
-          visc_jac(P, J, -1.0, a, mul, mut, vf, ufa, area, smag);
+        // icv0: A[noc00] -= A0, A[noc01] -= A1; icv1: A[noc11] += A1,
+        // A[noc10] += A0 (each block evaluated once, routed by the lane's side)
+        visc_jac(P, J, -1.0, a, mul, mut, vf, ufa, area, smag);  // A0
+        if (s0) {
           for (int i = 1; i < 5; i++)
             for (int j = 0; j < 5; j++) D[i][j] -= J[i][j];
-          visc_jac(P, J, 1.0, b, mul, mut, vf, ufa, area, smag);
-          for (int i = 1; i < 5; i++)
-            for (int j = 0; j < 5; j++) off[5 * i + j] = (T)((double)off[5 * i + j] - J[i][j]);
-        } else {  // A[noc11] += A1, A[noc10] += A0
-          visc_jac(P, J, 1.0, b, mul, mut, vf, ufa, area, smag);
-          for (int i = 1; i < 5; i++)
-            for (int j = 0; j < 5; j++) D[i][j] += J[i][j];
-          visc_jac(P, J, -1.0, a, mul, mut, vf, ufa, area, smag);
+        } else {
           for (int i = 1; i < 5; i++)
             for (int j = 0; j < 5; j++) off[5 * i + j] = (T)((double)off[5 * i + j] + J[i][j]);
+        }
+        visc_jac(P, J, 1.0, b, mul, mut, vf, ufa, area, smag);  // A1
+        if (s0) {
+          for (int i = 1; i < 5; i++)
+            for (int j = 0; j < 5; j++) off[5 * i + j] = (T)((double)off[5 * i + j] - J[i][j]);
+        } else {
+          for (int i = 1; i < 5; i++)
+            for (int j = 0; j < 5; j++) D[i][j] += J[i][j];
         }
       } else {
         const ZoneDev &z = m.zones[-en.y - 1];
@@ -1554,20 +1561,24 @@ __global__ void __launch_bounds__(kBlock) k_scal(DMesh m, DParams P, const doubl
       const int noc = m.cfl_noc[e];
       const bool accum = (noc >> 30) & 1;
       double *off = AS + 6 * (size_t)(noc & 0x3fffffff);
+      // the flux and both AScal rows once, on the face's (icv0, icv1) sides,
+      // routed by this cell's side (acc - x == acc + (-x))
+      const Side fl = s0 ? so : sn, fr = s0 ? sn : so;
+      const Side jl = s0 ? own : nbs, jr = s0 ? nbs : own;
+      const double phl = s0 ? sco : scn, phr = s0 ? scn : sco, pjl = s0 ? ph : phn, pjr = s0 ? phn : ph;
+      flux_hllc(F, fl.rho, fl.u, fl.p, fr.rho, fr.u, fr.p, gam, area, n, phl, phr, &Fs);
+      acc += s0 ? -Fs : Fs;
+      jac_hllc_scal(S, 0, jl.rho, jl.u, jl.p, jr.rho, jr.u, jr.p, gam, area, n, pjl, pjr);
       if (s0) {
-        flux_hllc(F, so.rho, so.u, so.p, sn.rho, sn.u, sn.p, gam, area, n, sco, scn, &Fs);
-        acc += -Fs;
-        jac_hllc_scal(S, 0, own.rho, own.u, own.p, nbs.rho, nbs.u, nbs.p, gam, area, n, ph, phn);
         for (int k = 0; k < 6; k++) D[k] += S[k];
-        jac_hllc_scal(S, 1, own.rho, own.u, own.p, nbs.rho, nbs.u, nbs.p, gam, area, n, ph, phn);
+      } else {
+        for (int k = 0; k < 6; k++) off[k] = accum ? off[k] - S[k] : 0.0 - S[k];
+      }
+      jac_hllc_scal(S, 1, jl.rho, jl.u, jl.p, jr.rho, jr.u, jr.p, gam, area, n, pjl, pjr);
+      if (s0) {
         for (int k = 0; k < 6; k++) off[k] = accum ? off[k] + S[k] : 0.0 + S[k];
       } else {
-        flux_hllc(F, sn.rho, sn.u, sn.p, so.rho, so.u, so.p, gam, area, n, scn, sco, &Fs);
-        acc += Fs;
-        jac_hllc_scal(S, 1, nbs.rho, nbs.u, nbs.p, own.rho, own.u, own.p, gam, area, n, phn, ph);
         for (int k = 0; k < 6; k++) D[k] -= S[k];
-        jac_hllc_scal(S, 0, nbs.rho, nbs.u, nbs.p, own.rho, own.u, own.p, gam, area, n, phn, ph);
-        for (int k = 0; k < 6; k++) off[k] = accum ? off[k] - S[k] : 0.0 - S[k];
       }
     } else {
       const ZoneDev &z = m.zones[-en.y - 1];
@@ -1889,6 +1900,12 @@ __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const
     bool low = false;
     double vc[4] = {-0.0, -0.0, -0.0, -0.0};
     double mul = 0.0, mut = 0.0;
+    // each lane's two sides are chosen first (this cell reconstructed at the
+    // face, the neighbour, or the boundary state) and the flux evaluated once
+    // after the branches, so a wavefront whose lanes face different ways, or
+    // mixes internal and boundary faces, runs the flux arithmetic once
+    Side fl, fr;
+    bool symm_analytic = false;
     if (en.y >= 0) {
       const bool s0 = (en.x >= 0);
       double pn[8], gn[15];
@@ -1913,12 +1930,9 @@ __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const
         so = own_cc;
         sn.rho = pn[0]; sn.u[0] = pn[1]; sn.u[1] = pn[2]; sn.u[2] = pn[3]; sn.p = pn[4];
       }
-      if (s0) {
-        face_flux(SCHEME, F, so, sn, gam, area, n);
-      } else {
-        face_flux(SCHEME, F, sn, so, gam, area, n);
-        sgn = 1.0;
-      }
+      fl = s0 ? so : sn;
+      fr = s0 ? sn : so;
+      if (!s0) sgn = 1.0;
       if (VISC) {  // calcViscousFluxNS internal faces (JoeWithModels.cpp:3635-3710)
         const VSide vo = vside_cell(P, pc, gradv + 12 * (size_t)L.c);
         const VSide vn = vside_cell(P, pn, gradv + 12 * (size_t)en.y);
@@ -1942,16 +1956,14 @@ __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const
       Side sb;
       sb.rho = bb[0]; sb.u[0] = bb[1]; sb.u[1] = bb[2]; sb.u[2] = bb[3]; sb.p = bb[4];
       if (z.bc == kBcSymmetry || z.bc == kBcWall) {
-        face_flux(SCHEME, F, sb, sb, gam, area, n);
-        if (z.bc == kBcSymmetry && P.symm_bc == kSymmAnalytical) {
-          F[0] = 0.0;
-          for (int i = 0; i < 3; i++) F[1 + i] = sb.p * n[i] * area;
-          F[4] = 0.0;
-        }
+        fl = sb;
+        fr = sb;
+        symm_analytic = z.bc == kBcSymmetry && P.symm_bc == kSymmAnalytical;
       } else {
         Side so = own_cc;
         if (SND) low = !reconstruct(so, pc, gc, xfa);
-        face_flux(SCHEME, F, so, sb, gam, area, n);
+        fl = so;
+        fr = sb;
       }
       if (VISC) {  // boundary faces (:3714-3851); boundary mul from T_bfa (ComputeBCProperties_T)
         mul = mu_lam(P, bb[5]);
@@ -1979,6 +1991,12 @@ __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const
           vc[3] = -FE;
         }
       }
+    }
+    face_flux(SCHEME, F, fl, fr, gam, area, n);
+    if (symm_analytic) {
+      F[0] = 0.0;
+      for (int i = 0; i < 3; i++) F[1 + i] = fr.p * n[i] * area;
+      F[4] = 0.0;
     }
     if (VISC) {  // viscous bound of calcDt (UgpWithCvCompFlow.cpp:50-51)
       const double lamdiff = 4.0 * (mul + mut) / pc[0] * area * area / m.vol[L.c];
