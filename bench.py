@@ -85,7 +85,25 @@ def cpu_baseline(n_box=64, reps=12):
                       f"calcRhs; HLLC 2nd order LS+limiter) in {dt:.1f} s on 1 host core"}
 
 
-def serving_bench(n, seed, min_cells, max_cells, max_active, nsteps, resid_rel, budget_gib):
+def streams_capacity(built, budget, max_active):
+    """Solves/min of the per-stream graph server, all requests present at t = 0."""
+    from myjoe_amd import joemi
+
+    srv = joemi.Server(budget, max_active, batched=False)
+    t0 = time.perf_counter()
+    tickets = [srv.submit(m, joe, joemi.JM_MIXED) for m, joe in built]
+    for t in tickets:
+        srv.wait(t)
+    cap = len(built) / (time.perf_counter() - t0) * 60.0
+    for t in tickets:
+        srv.release(t)
+    srv.close()
+    print(f"# serving: per-stream graph server {cap:.1f} solves/min", file=sys.stderr, flush=True)
+    return cap
+
+
+def serving_bench(n, seed, min_cells, max_cells, max_active, nsteps, resid_rel, budget_gib,
+                  legs=("plain", "streams", "batched")):
     """Steady solves/min and latency of the served mixed-precision solver
     against a plain FP64 one-at-a-time loop, on steady bump-channel requests
     of random size, bump thickness and Mach (cases.serving_requests), each
@@ -95,7 +113,8 @@ def serving_bench(n, seed, min_cells, max_cells, max_active, nsteps, resid_rel, 
     loop the same arrivals are queued in front of its measured solve times.
     The served solver is the batched server (lockstep steps, each kernel of
     the step and of the BiCGStab launched once for all active solves); the
-    per-stream graph server's capacity is reported beside it."""
+    per-stream graph server's capacity is reported beside it. legs names the
+    parts to run (plain loop, per-stream graph server, batched server)."""
     import numpy as np
 
     from myjoe_amd import cases, joemi
@@ -110,7 +129,7 @@ def serving_bench(n, seed, min_cells, max_cells, max_active, nsteps, resid_rel, 
     cells = sum(m.ncv for m, _ in built)
     # plain loop: FP64, one solve at a time, host-polled steps
     service, steps_plain, plain_states = [], [], []
-    for mesh, joe in built:
+    for mesh, joe in (built if "plain" in legs else []):
         t0 = time.perf_counter()
         s = joemi.Solver(mesh, joe, joemi.JM_FP64)
         s.init_uniform()
@@ -120,21 +139,20 @@ def serving_bench(n, seed, min_cells, max_cells, max_active, nsteps, resid_rel, 
         steps_plain.append(k)
         plain_states.append(st)
         del s
-    cap_plain = n / sum(service) * 60.0
-    print(f"# serving: plain FP64 loop {cap_plain:.1f} solves/min ({cells} cells in {n} solves)", file=sys.stderr,
-          flush=True)
+    cap_plain = n / sum(service) * 60.0 if service else None
+    if service:
+        print(f"# serving: plain FP64 loop {cap_plain:.1f} solves/min ({cells} cells in {n} solves)",
+              file=sys.stderr, flush=True)
     budget = int(budget_gib * (1 << 30))
-    # per-stream graph server, capacity only
-    srv = joemi.Server(budget, max_active, batched=False)
-    t0 = time.perf_counter()
-    tickets = [srv.submit(m, joe, joemi.JM_MIXED) for m, joe in built]
-    for t in tickets:
-        srv.wait(t)
-    cap_streams = n / (time.perf_counter() - t0) * 60.0
-    for t in tickets:
-        srv.release(t)
-    srv.close()
-    print(f"# serving: per-stream graph server {cap_streams:.1f} solves/min", file=sys.stderr, flush=True)
+    res = {"workload": f"{n} steady inviscid bump-channel solves, {min_cells}-{max_cells} cells (nx = 3 ny, one "
+                       f"cell thick), bump 6-12 %, inlet Mach 0.3-0.5, HLLC 2nd order; stop at energy L1 drop "
+                       f"{resid_rel:g} or {nsteps} steps", "cells_total": int(cells)}
+    if "streams" in legs:
+        res["served_per_stream_graphs"] = {"solves_per_min": round(streams_capacity(built, budget, max_active), 2),
+                                           "max_active": max_active}
+    if "batched" not in legs:
+        return res
+
     # served: batched, mixed precision, all present at t = 0
     srv = joemi.Server(budget, max_active, batched=True)
     t0 = time.perf_counter()
@@ -144,11 +162,11 @@ def serving_bench(n, seed, min_cells, max_cells, max_active, nsteps, resid_rel, 
     makespan = time.perf_counter() - t0
     cap_served = n / makespan * 60.0
     steps_served, err, drops = [], {"rho": 0.0, "rhou": 0.0, "rhoE": 0.0}, []
-    for t, ref in zip(tickets, plain_states):
+    for k, t in enumerate(tickets):
         st, rep = srv.result(t)
         steps_served.append(rep.steps)
         drops.append(rep.resid[4] / rep.resid0[4] if rep.resid0[4] > 0 else float("nan"))
-        for name, a, b in zip(("rho", "rhou", "rhoE"), st, ref):
+        for name, a, b in zip(("rho", "rhou", "rhoE"), st, plain_states[k] if plain_states else ()):
             d = float(np.max(np.abs(a - b))) / max(float(np.max(np.abs(b))), 1e-300)
             err[name] = d if not (d <= err[name]) else err[name]  # a NaN is kept, not hidden
         srv.release(t)
@@ -172,30 +190,27 @@ def serving_bench(n, seed, min_cells, max_cells, max_active, nsteps, resid_rel, 
         lat.append(rep.queued_s + rep.run_s)
         srv.release(t)
     srv.close()
-    done_prev, lat_plain = 0.0, []
-    for a, sv in zip(arr, service):
-        start = max(a, done_prev)
-        done_prev = start + sv
-        lat_plain.append(done_prev - a)
     q = lambda v, p: float(np.percentile(np.array(v), p))
-    return {
-        "workload": f"{n} steady inviscid bump-channel solves, {min_cells}-{max_cells} cells (nx = 3 ny, one cell "
-                    f"thick), bump 6-12 %, inlet Mach 0.3-0.5, HLLC 2nd order; stop at energy L1 drop {resid_rel:g} "
-                    f"or {nsteps} steps",
-        "served": {"solves_per_min": round(cap_served, 2), "precision": "FP64 residual, FP32 Jacobian/Krylov",
-                   "mode": "batched: lockstep steps, each step and BiCGStab kernel launched once for all "
-                                                            "active solves (2 groups)",
-                   "max_active": max_active, "p50_s": round(q(lat, 50), 3), "p99_s": round(q(lat, 99), 3),
-                   "mean_steps": round(float(np.mean(steps_served)), 1)},
-        "served_per_stream_graphs": {"solves_per_min": round(cap_streams, 2), "max_active": max_active},
-        "plain_fp64_loop": {"solves_per_min": round(cap_plain, 2), "p50_s": round(q(lat_plain, 50), 3),
-                            "p99_s": round(q(lat_plain, 99), 3), "mean_steps": round(float(np.mean(steps_plain)), 1)},
-        "speedup_solves_per_min": round(cap_served / cap_plain, 3),
-        "arrival_rate_per_min": round(rate * 60.0, 2),
-        "served_vs_fp64_max_rel_diff": {k: float(f"{v:.3g}") for k, v in err.items()},
-        "served_energy_resid_drop_max": float(f"{max(drops):.3g}"),
-        "cells_total": int(cells),
-    }
+    res["served"] = {"solves_per_min": round(cap_served, 2), "precision": "FP64 residual, FP32 Jacobian/Krylov",
+                     "mode": "batched: lockstep steps, each step and BiCGStab kernel launched once for all "
+                             "active solves (2 groups)",
+                     "max_active": max_active, "p50_s": round(q(lat, 50), 3), "p99_s": round(q(lat, 99), 3),
+                     "mean_steps": round(float(np.mean(steps_served)), 1)}
+    res["arrival_rate_per_min"] = round(rate * 60.0, 2)
+    res["served_energy_resid_drop_max"] = float(f"{max(drops):.3g}")
+    if service:
+        # the one-at-a-time loop meets the same arrivals, queued in front of its measured solve times
+        done_prev, lat_plain = 0.0, []
+        for a, sv in zip(arr, service):
+            start = max(a, done_prev)
+            done_prev = start + sv
+            lat_plain.append(done_prev - a)
+        res["plain_fp64_loop"] = {"solves_per_min": round(cap_plain, 2), "p50_s": round(q(lat_plain, 50), 3),
+                                  "p99_s": round(q(lat_plain, 99), 3),
+                                  "mean_steps": round(float(np.mean(steps_plain)), 1)}
+        res["speedup_solves_per_min"] = round(cap_served / cap_plain, 3)
+        res["served_vs_fp64_max_rel_diff"] = {k: float(f"{v:.3g}") for k, v in err.items()}
+    return res
 
 
 def main():
@@ -215,6 +230,8 @@ def main():
     ap.add_argument("--serve-active", type=int, default=16)
     ap.add_argument("--serve-steps", type=int, default=3000)
     ap.add_argument("--serve-rel", type=float, default=1e-4)
+    ap.add_argument("--serve-legs", default="plain,streams,batched",
+                    help="serving parts to run: plain (FP64 loop), streams (per-stream graph server), batched")
     ap.add_argument("--copy-calib-gib", type=float, default=0.0,
                     help="first run a plain device copy of this many GiB (calibrates profiler byte counters)")
     args = ap.parse_args()
@@ -322,7 +339,8 @@ def main():
     gc.collect()
     if args.serve > 0:
         lo, hi = (int(x) for x in args.serve_cells.split(","))
-        out["serving"] = serving_bench(args.serve, 7, lo, hi, args.serve_active, args.serve_steps, args.serve_rel, 64.0)
+        out["serving"] = serving_bench(args.serve, 7, lo, hi, args.serve_active, args.serve_steps, args.serve_rel, 64.0,
+                                       legs=tuple(args.serve_legs.split(",")))
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)
