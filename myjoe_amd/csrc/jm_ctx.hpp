@@ -94,6 +94,7 @@ struct KrylovBatchBuf {
   int *flag_h = nullptr, *flag_d = nullptr;
   size_t cap_jobs = 0, cap_map = 0, cap_partials = 0;
   int njobs = 0, g5 = 0, g1 = 0, gx = 0, max_it = 0;  // the batch set up last
+  int last_launched = 0;  // iterations the last run launched before every solve stopped
   KrylovBatchBuf() = default;
   KrylovBatchBuf(const KrylovBatchBuf &) = delete;
   KrylovBatchBuf &operator=(const KrylovBatchBuf &) = delete;

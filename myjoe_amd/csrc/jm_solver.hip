@@ -1826,6 +1826,35 @@ __global__ void __launch_bounds__(kFinBlock) k_fin(int nb, int op, const double 
   fin_body(nb, op, partials, (size_t)nb, S, sh);
 }
 
+// fin_slot of a kFinBlock-thread block computed by a smaller block (a divisor
+// of kFinBlock, whole wavefronts): thread t plays the full block's threads
+// t, t + blockDim.x, ..., each wave's lanes in the same shuffle tree, and the
+// sixteen wave results are combined in the same order, so the value is the
+// same bits as k_fin's. Valid in every thread; sh holds 16 doubles.
+__device__ double fin_slot_v(const double *__restrict__ p, int nb, int op, double *sh) {
+  const double id = op == kSum ? 0.0 : (op == kMax ? -__builtin_huge_val() : __builtin_huge_val());
+  const int st = kFinBlock, lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  __syncthreads();  // sh may still be read by a previous call
+  for (int r = 0; r < kFinBlock / (int)blockDim.x; r++) {
+    double x0 = id, x1 = id, x2 = id, x3 = id;
+    int b = threadIdx.x + r * blockDim.x;
+    for (; b + 3 * st < nb; b += 4 * st) {
+      x0 = rcomb(x0, p[b], op);
+      x1 = rcomb(x1, p[b + st], op);
+      x2 = rcomb(x2, p[b + 2 * st], op);
+      x3 = rcomb(x3, p[b + 3 * st], op);
+    }
+    for (; b < nb; b += st) x0 = rcomb(x0, p[b], op);
+    double x = rcomb(rcomb(x0, x1, op), rcomb(x2, x3, op), op);
+    for (int off = 32; off > 0; off >>= 1) x = rcomb(x, __shfl_down(x, off, 64), op);
+    if (lane == 0) sh[wid + r * nw] = x;
+  }
+  __syncthreads();
+  double v = sh[0];
+  for (int w = 1; w < kFinBlock / 64; w++) v = rcomb(v, sh[w], op);
+  return v;
+}
+
 // Residual kernel: one cell per thread (flux_cell), block minima of the local
 // time step to partials (k_fin kFinDt); CFL_MIN mode: k_dt_fill afterwards.
 template <int SCHEME, bool SND>
@@ -2509,12 +2538,26 @@ __global__ void __launch_bounds__(kBlock) kb_resid(const KrylovJob *__restrict__
 }
 
 // which = 0: p = res - beta (p - omega v), phat = LU^-1 p; 1: s = res - alpha v, shat = LU^-1 s
+// (which = 1 also finishes alpha = rho / (v.res0) from kb_spmv_dot's
+// partials, as k_fin kFinAlpha does, in every block; the solve's first block
+// stores it)
 template <typename T>
 __global__ void __launch_bounds__(kBlock) kb_ps(const KrylovJob *__restrict__ jobs, const int *__restrict__ map1,
-                                                int which) {
+                                                int which, const double *__restrict__ partials, int grid5) {
+  __shared__ double sh[16];
   const KrylovJob &J = jobs[map1[blockIdx.x]];
-  const SolverScalars *S = (const SolverScalars *)J.S;
+  SolverScalars *S = (SolverScalars *)J.S;
   if (S->done || S->stop) return;
+  double alpha = 0.0;
+  if (which == 1) {  // UgpWithCv.cpp:2775-2782
+    double gamma = fin_slot_v(partials + J.b5, J.nb5, kSum, sh);
+    if (fabs(gamma) < 1.0E-20) gamma = 1.0E-20;
+    alpha = S->rho / gamma;
+    if ((int)blockIdx.x == J.b1 && threadIdx.x == 0) {
+      S->gamma = gamma;
+      S->alpha = alpha;
+    }
+  }
   const int c = (blockIdx.x - J.b1) * blockDim.x + threadIdx.x;
   if (c >= J.ncv) return;
   const T *res = (const T *)J.v[2], *vv = (const T *)J.v[5], *LU = (const T *)J.LU;
@@ -2529,10 +2572,10 @@ __global__ void __launch_bounds__(kBlock) kb_ps(const KrylovJob *__restrict__ jo
       t[k] = pv;
     }
   } else {
-    const T alpha = (T)S->alpha;
+    const T alpha_t = (T)alpha;
     for (int k = 0; k < 5; k++) {
       size_t o = 5 * (size_t)c + k;
-      T sv = res[o] - alpha * vv[o];
+      T sv = res[o] - alpha_t * vv[o];
       out[o] = sv;
       t[k] = sv;
     }
@@ -2564,15 +2607,24 @@ __global__ void __launch_bounds__(kBlock) kb_spmv_dot(const KrylovJob *__restric
   block_partials<2>(v, partials, sh, kSum);
 }
 
+// phi += alpha phat + omega shat, with omega = (s.t) / (t.t + 1e-20) finished
+// from kb_spmv_dot's partials in every block as k_fin kFinOmega does (the
+// solve's first block stores it)
 template <typename T>
-__global__ void __launch_bounds__(kBlock) kb_x(const KrylovJob *__restrict__ jobs, const int *__restrict__ mapx) {
+__global__ void __launch_bounds__(kBlock) kb_x(const KrylovJob *__restrict__ jobs, const int *__restrict__ mapx,
+                                               const double *__restrict__ partials, int grid5) {
+  __shared__ double sh[16];
   const KrylovJob &J = jobs[mapx[blockIdx.x]];
-  const SolverScalars *S = (const SolverScalars *)J.S;
+  SolverScalars *S = (SolverScalars *)J.S;
   if (S->done || S->stop) return;
+  const double st = fin_slot_v(partials + J.b5, J.nb5, kSum, sh);  // UgpWithCv.cpp:2808-2810
+  const double tt = fin_slot_v(partials + J.b5 + grid5, J.nb5, kSum, sh);
+  const double omega = st / (tt + 1.0E-20);
+  if ((int)blockIdx.x == J.bx && threadIdx.x == 0) S->omega = omega;
   const int k = (blockIdx.x - J.bx) * blockDim.x + threadIdx.x;
   if (k >= 5 * J.ncv) return;
   T *phi = (T *)J.v[1];
-  phi[k] += (T)S->alpha * ((const T *)J.v[6])[k] + (T)S->omega * ((const T *)J.v[8])[k];
+  phi[k] += (T)S->alpha * ((const T *)J.v[6])[k] + (T)omega * ((const T *)J.v[8])[k];
 }
 
 // one block per solve: finish its partials (at its first block, stride = the whole grid)
@@ -3887,36 +3939,38 @@ static void krylov_batch_iterations(hipStream_t st, const KrylovBatchBuf &buf, i
   const KrylovJob *jd = buf.jobs_d;
   double *pt = buf.partials_d;
   const int g5 = buf.g5, g1 = buf.g1, gx = buf.gx, nj = buf.njobs;
-  for (int k = 0; k < count; k++) {
-    hipLaunchKernelGGL(kb_ps<T>, dim3(g1), dim3(kBlock), 0, st, jd, m1, 0);
+  for (int k = 0; k < count; k++) {  // seven launches: alpha and omega are finished in kb_ps and kb_x
+    hipLaunchKernelGGL(kb_ps<T>, dim3(g1), dim3(kBlock), 0, st, jd, m1, 0, pt, g5);
     hipLaunchKernelGGL(kb_spmv_dot<T>, dim3(g5), dim3(kBlock), 0, st, jd, m5, 0, pt);
-    hipLaunchKernelGGL(kb_fin, dim3(nj), dim3(kFinBlock), 0, st, jd, (int)kFinAlpha, pt, g5);
-    hipLaunchKernelGGL(kb_ps<T>, dim3(g1), dim3(kBlock), 0, st, jd, m1, 1);
+    hipLaunchKernelGGL(kb_ps<T>, dim3(g1), dim3(kBlock), 0, st, jd, m1, 1, pt, g5);
     hipLaunchKernelGGL(kb_spmv_dot<T>, dim3(g5), dim3(kBlock), 0, st, jd, m5, 1, pt);
-    hipLaunchKernelGGL(kb_fin, dim3(nj), dim3(kFinBlock), 0, st, jd, (int)kFinOmega, pt, g5);
-    hipLaunchKernelGGL(kb_x<T>, dim3(gx), dim3(kBlock), 0, st, jd, mx);
+    hipLaunchKernelGGL(kb_x<T>, dim3(gx), dim3(kBlock), 0, st, jd, mx, pt, g5);
     hipLaunchKernelGGL(kb_resid<T>, dim3(g5), dim3(kBlock), 0, st, jd, m5, 0, pt);
     hipLaunchKernelGGL(kb_fin, dim3(nj), dim3(kFinBlock), 0, st, jd, (int)kFinResid, pt, g5);
   }
 }
 
-// The batch set up last, with host polling between chunks of 2, then 4
-// iterations (a chunk's launches after a solve's stop return at once)
+// The batch set up last, with host polling between chunks of iterations: the
+// first chunk is the previous call's count less two (consecutive steps of the
+// same solves take similar counts), then chunks of two. Launches after a
+// solve's stop return at once, so the chunking changes no result.
 void run_krylov_batch(int precision, void *stream, KrylovBatchBuf &buf) {
   if (buf.njobs <= 0) return;
   hipStream_t st = (hipStream_t)stream;
   if (precision == 0) krylov_batch_prologue<double>(st, buf);
   else krylov_batch_prologue<float>(st, buf);
-  int launched = 0;
+  int launched = 0, chunk = std::max(2, buf.last_launched - 2);
   while (launched < buf.max_it) {
-    const int chunk = std::min(launched == 0 ? 2 : 4, buf.max_it - launched);
+    chunk = std::min(chunk, buf.max_it - launched);
     if (precision == 0) krylov_batch_iterations<double>(st, buf, chunk);
     else krylov_batch_iterations<float>(st, buf, chunk);
     launched += chunk;
     hipLaunchKernelGGL(kb_all_done, dim3(1), dim3(256), 0, st, buf.jobs_d, buf.njobs, buf.flag_d);
     HIPCHECK(hipStreamSynchronize(st));
     if (*(volatile int *)buf.flag_h) break;
+    chunk = 2;
   }
+  buf.last_launched = launched;
   HIPCHECK(hipGetLastError());
 }
 
