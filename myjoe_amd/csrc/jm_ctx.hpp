@@ -40,6 +40,7 @@ struct SolverParams {
   double eps_limiter = 0.001;
   int symm_bc = kSymmOriginal;
   int dt_mode = 0;  // 0 CFL (local), 1 CFL_MIN, 2 DT
+  bool dt_sum = false;  // TIME_STEP = SUM_BASED (MAX_BASED otherwise)
   double cfl = 0.0, dt_fixed = 0.0;
   double under_relax = 0.3;
   int ls_maxiter = 30;

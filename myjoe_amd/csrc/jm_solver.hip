@@ -843,6 +843,7 @@ __device__ __forceinline__ void face_flux(int scheme, double F[5], const Side &L
 struct DtArgs {
   double cfl, dt_fixed;
   int mode;
+  int sum;                // TIME_STEP = SUM_BASED (faocv-ordered sums; needs DMesh::foc)
   const double *cfl_dev;  // when set, the CFL number is read from device memory (graph-captured steps)
 };
 
@@ -966,7 +967,21 @@ __device__ __forceinline__ double flux_cell(const DMesh &m, const DParams &P, co
   }
   for (int k = 0; k < 5; k++) rhs[(size_t)k * m.ncv + c] = acc[k];
   if (nlow) atomicAdd(nfirst, nlow);
-  const double dtc = dta.mode == 2 ? dta.dt_fixed : (dta.cfl_dev ? *dta.cfl_dev : dta.cfl) * vol / lambdaMax;
+  double dtc;
+  if (dta.mode == 2) {
+    dtc = dta.dt_fixed;
+  } else if (dta.sum) {  // SUM_BASED (UgpWithCvCompFlow.cpp:62-99), inviscid: dt_visc = 1e20
+    double lc = 0.0;
+    for (int k = 0; k < e1 - e0; k++) {
+      const int f = m.cfl[e0 + m.foc[e0 + k]].x & 0x7fffffff;
+      const double *geo = m.fgeo + 8 * (size_t)f;
+      const double n[3] = {geo[0], geo[1], geo[2]};
+      lc += (fabs(dot3(ru, n) / pc[0]) + cs) * geo[3];
+    }
+    dtc = Dev::mn((dta.cfl_dev ? *dta.cfl_dev : dta.cfl) * vol / lc, 1.0e20);
+  } else {
+    dtc = (dta.cfl_dev ? *dta.cfl_dev : dta.cfl) * vol / lambdaMax;
+  }
   dt[c] = dtc;
   return dtc;
 }
@@ -1881,7 +1896,7 @@ __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const
                                                    const double *__restrict__ q, double *__restrict__ rhs,
                                                    int *__restrict__ nfirst, DtArgs dta, double *__restrict__ dt,
                                                    double *__restrict__ partials, const double *__restrict__ gradv) {
-  constexpr int NS = VISC ? 10 : 6;  // inviscid 5, wave-speed bound, viscous rhou/rhoE 4
+  constexpr int NS = VISC ? 11 : 6;  // inviscid 5, wave speed, viscous rhou/rhoE 4, SUM_BASED diffusive term
   __shared__ double lds[kBlock * NS];
   __shared__ double sh[16];
   double *slot = lds + NS * threadIdx.x;
@@ -2027,10 +2042,16 @@ __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const
       for (int i = 0; i < 3; i++) F[1 + i] = fr.p * n[i] * area;
       F[4] = 0.0;
     }
-    if (VISC) {  // viscous bound of calcDt (UgpWithCvCompFlow.cpp:50-51)
-      const double lamdiff = 4.0 * (mul + mut) / pc[0] * area * area / m.vol[L.c];
-      lambda = Dev::mx(lambda, lamdiff);
+    if (VISC) {
       for (int k = 0; k < 4; k++) slot[6 + k] = vc[k];
+      if (dta.sum) {  // SUM_BASED diffusive term (UgpWithCvCompFlow.cpp:75-88)
+        const double volume = en.y >= 0 ? 0.5 * (m.vol[L.c] + m.vol[en.y]) : 0.5 * m.vol[L.c];
+        const double KinVisc = (4.0 / 3.0) * (mul + mut) / pc[0];
+        slot[10] = area * area * KinVisc / volume;
+      } else {  // MAX_BASED viscous bound (:50-51)
+        const double lamdiff = 4.0 * (mul + mut) / pc[0] * area * area / m.vol[L.c];
+        lambda = Dev::mx(lambda, lamdiff);
+      }
     }
     slot[5] = lambda;
     // acc - F == acc + (-F) exactly, so the owner only adds
@@ -2070,7 +2091,20 @@ __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const
       acc[4] += allneg ? -0.0 : 0.0;
     }
     for (int k = 0; k < 5; k++) rhs[(size_t)k * m.ncv + c] = acc[k];
-    dtc = dta.mode == 2 ? dta.dt_fixed : (dta.cfl_dev ? *dta.cfl_dev : dta.cfl) * vol / lambdaMax;
+    const double cflv = dta.cfl_dev ? *dta.cfl_dev : dta.cfl;
+    if (dta.mode == 2) {
+      dtc = dta.dt_fixed;
+    } else if (dta.sum) {  // SUM_BASED (UgpWithCvCompFlow.cpp:62-99): sums in faocv order
+      double lc = 0.0, ld = 0.0;
+      for (int k = 0; k < nent; k++) {
+        const double *s = slot + NS * m.foc[m.cfl_i[c] + k];
+        lc += s[5];
+        if (VISC) ld += s[10];
+      }
+      dtc = Dev::mn(cflv * vol / lc, VISC ? 0.25 * cflv * vol / ld : 1.0e20);
+    } else {
+      dtc = cflv * vol / lambdaMax;
+    }
     dt[c] = dtc;
   }
   double v[1] = {dtc};
@@ -2243,7 +2277,16 @@ __global__ void __launch_bounds__(kBlock) k_gather_e(DMesh m, DParams P, const d
       acc[4] += allneg ? -0.0 : 0.0;
     }
     for (int k = 0; k < 5; k++) rhs[(size_t)k * m.ncv + c] = acc[k];
-    dtc = dta.mode == 2 ? dta.dt_fixed : (dta.cfl_dev ? *dta.cfl_dev : dta.cfl) * vol / lambdaMax;
+    const double cflv = dta.cfl_dev ? *dta.cfl_dev : dta.cfl;
+    if (dta.mode == 2) {
+      dtc = dta.dt_fixed;
+    } else if (dta.sum) {  // SUM_BASED, inviscid (UgpWithCvCompFlow.cpp:62-99): sums in faocv order
+      double lc = 0.0;
+      for (int k = 0; k < nent; k++) lc += slot[NS * m.foc[m.cfl_i[c] + k] + 5];
+      dtc = Dev::mn(cflv * vol / lc, 1.0e20);
+    } else {
+      dtc = cflv * vol / lambdaMax;
+    }
     dt[c] = dtc;
   }
   double v[1] = {dtc};
@@ -3128,7 +3171,10 @@ void parse_params(const std::string &joe_in, const Mesh &m, SolverParams &p, std
   } else {
     throw std::invalid_argument("DT, CFL or CFL_MIN not specified");
   }
-  if (pm.str("TIME_STEP", "MAX_BASED") != "MAX_BASED") throw std::invalid_argument("TIME_STEP other than MAX_BASED");
+  const std::string tstep = pm.str("TIME_STEP", "MAX_BASED");
+  if (tstep != "MAX_BASED" && tstep != "SUM_BASED")
+    throw std::invalid_argument("TIME_STEP = " + tstep + " (MAX_BASED or SUM_BASED)");
+  p.dt_sum = tstep == "SUM_BASED";
   std::string ti = pm.str("TIME_INTEGRATION", "BACKWARD_EULER");
   if (ti != "BACKWARD_EULER") throw Unsupported("TIME_INTEGRATION = " + ti + " is not supported");
   std::string ls = pm.str("LINEAR_SOLVER_NS", "BCGSTAB");
@@ -3566,9 +3612,10 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
       }
       d.cfl_r0.upload(r0.data(), r0.size());
     }
-    // Spalart-Allmaras: for each cell, the entry offsets of its faces in the
-    // reference's faocv order (the order of InterpolateAtCellCenterFromFaceValues)
-    if (d.par.sa) {
+    // Spalart-Allmaras and SUM_BASED time steps: for each cell, the entry
+    // offsets of its faces in the reference's faocv order (the order of
+    // InterpolateAtCellCenterFromFaceValues and of calcDt's face loop)
+    if (d.par.sa || d.par.dt_sum) {
       std::vector<int> fo(std::max<size_t>(1, dcf.size()), 0);
       size_t k = 0;
       for (int cd = 0; cd < ncv; cd++) {
@@ -4042,6 +4089,7 @@ void StepBatch::set(Ctx *const *ctxs, int n, void *stream) {
     J.dta.cfl = d.cfl;
     J.dta.dt_fixed = d.par.dt_fixed;
     J.dta.mode = d.par.dt_mode;
+    J.dta.sum = d.par.dt_sum ? 1 : 0;
     J.dta.cfl_dev = &d.scal.p->cfl_cur;
     J.nA = 25 * (size_t)d.nnz;
     J.gamma = d.par.gamma;
@@ -4367,6 +4415,7 @@ void Ctx::Impl::launch_flux(hipStream_t s, const double *cfl_dev) {
   dta.cfl = cfl;
   dta.dt_fixed = par.dt_fixed;
   dta.mode = par.dt_mode;
+  dta.sum = par.dt_sum ? 1 : 0;
   dta.cfl_dev = cfl_dev;
   const dim3 g(flux_blocks());
   auto go = [&](auto kern) {

@@ -832,11 +832,42 @@ void calc_material(or_ctx &c) {
   }
 }
 
-// calcDt (MAX_BASED), UgpWithCvCompFlow.cpp:5-60, 100-105. mut_fa is zero
-// without a turbulence model (UgpWithCvCompFlow.h:1568-1580).
+// calcDt, UgpWithCvCompFlow.cpp:5-110: MAX_BASED (:30-60) or SUM_BASED
+// (:62-99). mut_fa is zero without a turbulence model
+// (UgpWithCvCompFlow.h:1568-1580).
 double calc_dt(or_ctx &c) {
   const or_mesh &m = c.m;
   double dt_min = 1.0e20;
+  if (c.p.time_step_sum) {
+    for (int icv = 0; icv < c.ncv; icv++) {
+      double cs = sqrt(c.p.gamma * c.press[icv] / c.rho[icv]);
+      double lambda_conv = 0.0, lambda_diff = 0.0;
+      for (int foc = m.faocv_i[icv]; foc <= m.faocv_i[icv + 1] - 1; foc++) {
+        int ifa = m.faocv_v[foc];
+        double nVec[3];
+        double area = norm3(nVec, &m.fa_normal[3 * ifa]);
+        double Uk = dot3(&c.rhou[3 * icv], nVec) / c.rho[icv];
+        int icv0 = m.cvofa[2 * ifa], icv1 = m.cvofa[2 * ifa + 1];
+        double volume = (icv0 >= 0 && icv1 >= 0) ? 0.5 * (m.cv_volume[icv0] + m.cv_volume[icv1])
+                                                 : 0.5 * m.cv_volume[icv];
+        lambda_conv += (fabs(Uk) + cs) * area;
+        if (c.p.mu_ref > 0.0) {
+          double KinVisc = (4.0 / 3.0) * (c.mul_fa[ifa] + c.mut_fa[ifa]) / c.rho[icv];
+          lambda_diff += area * area * KinVisc / volume;
+        } else {
+          lambda_diff = 0.0;
+        }
+      }
+      double dt_conv = c.cfl * m.cv_volume[icv] / lambda_conv;
+      double dt_visc = c.p.mu_ref > 0.0 ? 0.25 * c.cfl * m.cv_volume[icv] / lambda_diff : 1.0e20;
+      double dt_cv = min(dt_conv, dt_visc);
+      dt_min = min(dt_min, dt_cv);
+      c.local_dt[icv] = dt_cv;
+    }
+    if (c.p.time_step_min)
+      for (int icv = 0; icv < c.ncv; icv++) c.local_dt[icv] = dt_min;
+    return dt_min;
+  }
   for (int icv = 0; icv < c.ncv; icv++) {
     double lambdaMax = 0.0;
     double cs = sqrt(c.p.gamma * c.press[icv] / c.rho[icv]);

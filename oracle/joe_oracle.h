@@ -68,6 +68,8 @@ typedef struct {
    * UNDER_RELAXATION_SCALARS */
   int sa;
   double sa_init, relax_scal;
+  /* TIME_STEP: 0 MAX_BASED (default), 1 SUM_BASED */
+  int time_step_sum;
 } or_params;
 
 typedef struct or_ctx or_ctx;

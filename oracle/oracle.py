@@ -43,7 +43,8 @@ class _Params(C.Structure):
                 ("cfl_max", C.c_double), ("nsteps", C.c_int), ("resid_energ_th", C.c_double),
                 ("visc_mode", C.c_int), ("mu_power_law", C.c_double), ("sl_tref", C.c_double),
                 ("sl_sref", C.c_double), ("T_ref", C.c_double), ("pr", C.c_double), ("pr_turb", C.c_double),
-                ("visc_limiter", C.c_int), ("sa", C.c_int), ("sa_init", C.c_double), ("relax_scal", C.c_double)]
+                ("visc_limiter", C.c_int), ("sa", C.c_int), ("sa_init", C.c_double), ("relax_scal", C.c_double),
+                ("time_step_sum", C.c_int)]
 
 
 def lib():
@@ -134,6 +135,7 @@ def make_params(joe_in: str) -> tuple[_Params, dict]:
         p.cfl, p.time_step_min = float(get("CFL")), 0
     else:
         p.cfl, p.time_step_min = float(get("CFL_MIN")), 1
+    p.time_step_sum = {"MAX_BASED": 0, "SUM_BASED": 1}[get("TIME_STEP", "MAX_BASED")]
     p.under_relax = float(get("UNDER_RELAXATION", "0.3"))
     ls = P.get("LINEAR_SOLVER_NS_TRESHOLDS", [["X", "MAX_ITER", "30", "ABS_RESID", "1.0e-8", "REL_RESID", "1.0e-2"]])[0]
     p.ls_maxiter = int(_after(ls, "MAX_ITER"))

@@ -41,6 +41,8 @@ VISC = {
     "pl": "MU_REF = 0.002\nMU_MODE = POWERLAW\nMU_POWER_LAW = 0.76\nPR = 0.72",
     "su": "MU_REF = 0.003\nMU_MODE = SUTHERLAND\nSL_TREF = 1.0\nSL_SREF = 0.4\nPR = 0.7",
     "nl": "MU_REF = 0.002\nMU_MODE = POWERLAW\nVISC_LIMITER = NO",
+    # viscous enough that SUM_BASED's diffusive bound sets the step in some cells
+    "hi": "MU_REF = 0.05\nMU_MODE = POWERLAW\nMU_POWER_LAW = 0.76\nPR = 0.72",
 }
 
 MESHES = {
@@ -64,7 +66,11 @@ RHS_CASES = [("bump24", "HLLC", {}), ("bump24", "ROE", {}), ("bump24", "AUSM", {
              ("box4", "HLLC", {"visc": "su", "sa": True}),
              ("box4", "HLLC", {"visc": "pl", "sa": True, "bcs": cases.BOX_BCS_ALT, "symm_bc": "ANALYTICAL",
                                "grad": "GREENGAUSS"}),
-             ("box4", "HLLC", {"visc": "nl", "sa": True, "second_order": False})]
+             ("box4", "HLLC", {"visc": "nl", "sa": True, "second_order": False}),
+             # TIME_STEP = SUM_BASED local time step
+             ("box4", "HLLC", {"dtsum": True}),
+             ("bump24", "HLLC", {"visc": "hi", "dtsum": True}),
+             ("box4", "HLLC", {"visc": "hi", "sa": True, "dtsum": True})]
 
 
 def run_ref(args, cwd):
@@ -83,10 +89,14 @@ def tag(kw):
         t += "_gg"
     if kw.get("second_order") is False:
         t += "_first"
+    if kw.get("dtsum"):
+        t += "_dtsum"
     return t
 
 
-def main():
+def main(only=None):
+    """All fixtures, or (only = a substring) just the residual fixtures whose
+    file name holds it, leaving the others as they are."""
     if not os.path.exists(REF):
         raise SystemExit("oracle/_ref/joe_ref_dump is not built (make -C oracle ref, needs the reference tree)")
     with tempfile.TemporaryDirectory() as wd:
@@ -95,7 +105,15 @@ def main():
             kw = dict(kw)
             visc = kw.pop("visc", None)
             with_sa = kw.pop("sa", False)
+            dtsum = kw.pop("dtsum", False)
             mm, joe = MESHES[name](scheme, **kw)
+            if dtsum:
+                kw["dtsum"] = True
+                joe += "TIME_STEP = SUM_BASED\n"
+            tk = dict(kw, **({"visc": visc} if visc else {}), **({"sa": True} if with_sa else {}))
+            fixture = f"rhs_{name}_{scheme.lower()}{tag(tk)}"
+            if only and only not in fixture:
+                continue
             if visc:
                 kw["visc"] = visc
                 joe = joe.replace("MU_REF = 0.0", VISC[visc])
@@ -113,16 +131,18 @@ def main():
             jmdump.write(os.path.join(wd, "state.bin"), state)
             run_ref(["rhs-sa" if with_sa else "rhs", "Joe.in", "state.bin", "out.bin"], wd)
             d = jmdump.read(os.path.join(wd, "out.bin"))
-            if name not in done_mesh:
+            if name not in done_mesh and not only:
                 np.savez_compressed(os.path.join(HERE, f"mesh_{name}.npz"), **{k: d[k] for k in MESH_KEYS})
                 done_mesh.add(name)
             out = {k: d[k] for k in RHS_KEYS if k in d}
             out.update(RHO=rho, RHOU=rhou.reshape(-1), RHOE=rhoE, JOE_IN=np.frombuffer(joe.encode(), np.uint8))
             if with_sa:
                 out["NUSA"] = state["NUSA"]
-            np.savez_compressed(os.path.join(HERE, f"rhs_{name}_{scheme.lower()}{tag(kw)}.npz"), **out)
+            np.savez_compressed(os.path.join(HERE, fixture + ".npz"), **out)
             print("wrote", name, scheme, kw)
 
+        if only:
+            return
         # steady-loop trajectories of JoeWithModels::run (uniform initial state)
         for name, nx, ny, nsteps in [("bump24", 24, 8, 60), ("bump96", 96, 32, 300), ("bump24visc", 24, 8, 60),
                                      ("bump24sa", 24, 8, 60)]:
@@ -148,4 +168,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1] if len(sys.argv) > 1 else None)

@@ -60,6 +60,7 @@ def test_bad_connectivity_rejected():
     (lambda s: s.replace("SPACE_INTEGRATION = HLLC", "SPACE_INTEGRATION = CENTRAL"), "SPACE_INTEGRATION"),
     (lambda s: s.replace("CBC 0.5 0.1 0.0 1.0 1.0", "CBC 0.5 0.1"), "inlet"),
     (lambda s: s.replace("P_REF = 1.0", ""), "P_REF"),
+    (lambda s: s + "TIME_STEP = MIN_BASED\n", "TIME_STEP"),
 ])
 def test_joe_in_errors_before_device(edit, needle):
     mm, joe = cases.box_case(2, seed=0)

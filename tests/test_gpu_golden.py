@@ -107,7 +107,8 @@ def test_cell_order_does_not_change_results(order):
     compare(s.get("DQ"), g["DQ"], 1e-9, f"{order}:DQ")
 
 
-@pytest.mark.parametrize("fixture", ["bump24_hllc", "box4_hllc_gg", "box4_roe_first", "box4_ausm_alt_ana_gg"])
+@pytest.mark.parametrize("fixture", ["bump24_hllc", "box4_hllc_gg", "box4_roe_first", "box4_ausm_alt_ana_gg",
+                                     "box4_hllc_dtsum"])
 def test_cell_kernels_match_reference(fixture):
     # the other form of each residual kernel gives the same bits
     print(f"cell kernels {fixture}", flush=True)
@@ -122,7 +123,7 @@ def test_cell_kernels_match_reference(fixture):
 
 
 @pytest.mark.parametrize("fixture", ["bump24_hllc", "box4_hllc_gg", "box4_roe_first", "box4_ausm_alt_ana_gg",
-                                     "box4_roe_alt_ana", "box4_hllc_alt_new"])
+                                     "box4_roe_alt_ana", "box4_hllc_alt_new", "box4_hllc_dtsum"])
 def test_face_flux_form_matches_reference(fixture):
     # the inviscid flux evaluated once per face and gathered per cell gives
     # the same bits as the per-entry form; the first-order fallback count too
