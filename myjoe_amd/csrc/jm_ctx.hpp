@@ -18,7 +18,8 @@ enum BcKind : int {
   kBcNeumann = 5,
   kBcWall = 6,
   kBcFarfield = 7,
-  kBcHook = 8
+  kBcHook = 8,
+  kBcProfiles = 9
 };
 enum Scheme : int { kHllc = 0, kRoe = 1, kAusm = 2 };
 enum SymmBc : int { kSymmOriginal = 0, kSymmNew = 1, kSymmAnalytical = 2 };
@@ -59,6 +60,8 @@ struct SolverParams {
   // value (INITIAL_CONDITION_TURB) and UNDER_RELAXATION_SCALARS
   bool sa = false;
   double sa_init = 0.0, relax_scal = 0.2;
+  // the boundary-profile file of PROFILES zones (the last such zone's)
+  std::string profiles_file;
 };
 
 struct ZoneBc {

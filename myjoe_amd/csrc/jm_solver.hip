@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -159,6 +160,36 @@ __global__ void __launch_bounds__(kBlock) k_state(int ncv, const double *__restr
 // setBC, JoeWithModels.cpp:3857-4300 with ComputeBCProperties_T/_H
 // (UgpWithCvCompFlow.h:1783-1839) and the EOS density (:4292-4293)
 // ---------------------------------------------------------------------------
+// the characteristic far-field state of one boundary face (FARFIELD,
+// JoeWithModels.cpp:4100-4183; PROFILES, :4190-4270)
+__device__ __forceinline__ void farfield_state(const double u_bc[3], double p_bc, double rho_bc, double gamma_bc,
+                                               const double nVec[3], const double vel[3], double rho, double p,
+                                               double GAMMA, double R_gas, double vb[3], double &p_b, double &T_b) {
+  double gm1 = gamma_bc - 1.0;
+  double ovgm1 = 1.0 / gm1;
+  double c_bc = sqrt(gamma_bc * p_bc / rho_bc);
+  double s_bc = pow(rho_bc, gamma_bc) / p_bc;
+  double qn_bc = dot3(nVec, u_bc);
+  double vn_bc = qn_bc;
+  double qne = dot3(nVec, vel);
+  double ce = sqrt(GAMMA * p / rho);
+  double ac1 = vn_bc > -c_bc ? qne + 2.0 * ovgm1 * ce : qn_bc + 2.0 * ovgm1 * c_bc;
+  double ac2 = vn_bc > c_bc ? qne - 2.0 * ovgm1 * ce : qn_bc - 2.0 * ovgm1 * c_bc;
+  double qnf = 0.5 * (ac1 + ac2);
+  double cf = 0.25 * (ac1 - ac2) * gm1;
+  double sf;
+  if (vn_bc > 0) {
+    for (int i = 0; i < 3; i++) vb[i] = vel[i] + (qnf - qne) * nVec[i];
+    sf = pow(rho, GAMMA) / p;
+  } else {
+    for (int i = 0; i < 3; i++) vb[i] = u_bc[i] + (qnf - qn_bc) * nVec[i];
+    sf = s_bc;
+  }
+  double rb = pow((sf * cf * cf / GAMMA), ovgm1);
+  p_b = rb * cf * cf / GAMMA;
+  T_b = p_b / (R_gas * rb);
+}
+
 __device__ __forceinline__ void d_bc(const Blk &B, DMesh m, DParams P, const double *__restrict__ prim,
                                                const double *__restrict__ hook, double *__restrict__ bfa) {
   int f = B.id * blockDim.x + threadIdx.x;
@@ -227,30 +258,13 @@ __device__ __forceinline__ void d_bc(const Blk &B, DMesh m, DParams P, const dou
       break;
     case kBcFarfield: {
       const double u_bc[3] = {z.par[0], z.par[1], z.par[2]};
-      const double p_bc = z.par[3], rho_bc = z.par[4], gamma_bc = z.par[5];
-      double gm1 = gamma_bc - 1.0;
-      double ovgm1 = 1.0 / gm1;
-      double c_bc = sqrt(gamma_bc * p_bc / rho_bc);
-      double s_bc = pow(rho_bc, gamma_bc) / p_bc;
-      double qn_bc = dot3(nVec, u_bc);
-      double vn_bc = qn_bc;
-      double qne = dot3(nVec, vel);
-      double ce = sqrt(GAMMA * p / rho);
-      double ac1 = vn_bc > -c_bc ? qne + 2.0 * ovgm1 * ce : qn_bc + 2.0 * ovgm1 * c_bc;
-      double ac2 = vn_bc > c_bc ? qne - 2.0 * ovgm1 * ce : qn_bc - 2.0 * ovgm1 * c_bc;
-      double qnf = 0.5 * (ac1 + ac2);
-      double cf = 0.25 * (ac1 - ac2) * gm1;
-      double sf;
-      if (vn_bc > 0) {
-        for (int i = 0; i < 3; i++) vb[i] = vel[i] + (qnf - qne) * nVec[i];
-        sf = pow(rho, GAMMA) / p;
-      } else {
-        for (int i = 0; i < 3; i++) vb[i] = u_bc[i] + (qnf - qn_bc) * nVec[i];
-        sf = s_bc;
-      }
-      double rb = pow((sf * cf * cf / GAMMA), ovgm1);
-      p_b = rb * cf * cf / GAMMA;
-      T_b = p_b / (R_gas * rb);
+      farfield_state(u_bc, z.par[3], z.par[4], z.par[5], nVec, vel, rho, p, GAMMA, R_gas, vb, p_b, T_b);
+      break;
+    }
+    case kBcProfiles: {  // JoeWithModels.cpp:4190-4270: the profile's rho, u, p and gamma_bc = 1.4
+      const double *h = hook + 5 * (size_t)f;
+      const double u_bc[3] = {h[1], h[2], h[3]};
+      farfield_state(u_bc, h[4], h[0], 1.4, nVec, vel, rho, p, GAMMA, R_gas, vb, p_b, T_b);
       break;
     }
     case kBcHook: {
@@ -3071,6 +3085,48 @@ void ensure(DBuf<T> &b, size_t n) {
 // Joe.in -> parameters (UgpWithCvCompFlow.h:57-502, JoeWithModels.cpp:881-929,
 // boundary keywords JoeWithModels.cpp:3857-4300)
 // ---------------------------------------------------------------------------
+// The boundary-profile file of PROFILES zones as the reference reads it
+// ("n=<rows>\td=<values>", then rows of y, rho, u, v, p and one value per
+// transported scalar; JoeWithModels.cpp:262-290) and its values at each
+// PROFILES face: linear in the face's y, clamped at both ends, v dropped
+// (:296-326). Returns [nfa_b][5] = rho, u, v, w, p (zero on other faces).
+std::vector<double> profile_faces(const std::string &file, const Mesh &m, const std::vector<ZoneBc> &zbc, int nscal) {
+  FILE *fp = std::fopen(file.c_str(), "rt");
+  if (!fp) throw std::invalid_argument("could not open the boundary-profile file " + file);
+  int nn = 0, nval = 0;
+  const bool head = std::fscanf(fp, "n=%d\td=%d", &nn, &nval) == 2;
+  std::vector<double> pr(head && nn > 0 && nval > 0 ? (size_t)nn * nval : 0);
+  size_t got = 0;
+  for (; got < pr.size(); got++)
+    if (std::fscanf(fp, "%lf", &pr[got]) != 1) break;
+  std::fclose(fp);
+  if (!head || nn < 2 || got != pr.size())
+    throw std::invalid_argument("boundary-profile file " + file + ": needs n=<rows> d=<values> and the rows");
+  if (nval - 5 != nscal)
+    throw std::invalid_argument("boundary-profile file " + file + " does not have the required number of values");
+  std::vector<double> h(5 * (size_t)std::max(1, m.nfa_b), 0.0);
+  for (size_t iz = 0; iz < m.zones.size(); iz++) {
+    if (m.zones[iz].kind != kZoneBoundary || zbc[iz].bc != kBcProfiles) continue;
+    for (int ifa = m.zones[iz].ifa_f; ifa <= m.zones[iz].ifa_l; ifa++) {
+      const double y = m.x_fa[3 * (size_t)ifa + 1];
+      int pos = 1;
+      while ((pr[(size_t)pos * nval] < y) && (pos < nn - 1)) pos++;
+      const double *r1 = &pr[(size_t)(pos - 1) * nval], *r2 = &pr[(size_t)pos * nval];
+      double f;
+      if (y > r2[0]) f = 1.0;
+      else if (y < r1[0]) f = 0.0;
+      else f = (y - r1[0]) / (r2[0] - r1[0]);
+      double *o = &h[5 * (size_t)ifa];
+      o[0] = r1[1] + f * (r2[1] - r1[1]);
+      o[1] = r1[2] + f * (r2[2] - r1[2]);
+      o[2] = 0.0;
+      o[3] = 0.0;
+      o[4] = r1[4] + f * (r2[4] - r1[4]);
+    }
+  }
+  return h;
+}
+
 void parse_params(const std::string &joe_in, const Mesh &m, SolverParams &p, std::vector<ZoneBc> &zbc) {
   ParamMap pm(joe_in);
   p.gamma = pm.dbl("GAMMA", 1.4);
@@ -3234,6 +3290,10 @@ void parse_params(const std::string &joe_in, const Mesh &m, SolverParams &p, std
       for (int k = 0; k < 6; k++) b.par[k] = val(2 + k);
     } else if (kw == "HOOK") {
       b.bc = kBcHook;
+    } else if (kw == "PROFILES") {  // JoeWithModels.cpp:249-256: one file, the last PROFILES zone's
+      b.bc = kBcProfiles;
+      if (t->size() < 3) throw std::invalid_argument("zone " + z.name + ": PROFILES needs a file name");
+      p.profiles_file = (*t)[2];
     } else {
       throw std::invalid_argument("cannot recognize bc for zone " + z.name + ": " + kw);
     }
@@ -3256,6 +3316,7 @@ void parse_params(const std::string &joe_in, const Mesh &m, SolverParams &p, std
     for (const ZoneBc &b : zbc) wall = wall || b.bc == kBcWall;
     if (!wall) throw std::invalid_argument("TURB_MODEL = SA needs a WALL zone for the wall distance");
   }
+  if (!p.profiles_file.empty()) (void)profile_faces(p.profiles_file, m, zbc, p.sa ? 1 : 0);  // checked here, read again at set-up
 }
 
 // ---------------------------------------------------------------------------
@@ -3687,6 +3748,13 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
   d.dt.alloc(n);
   d.hook.alloc(5 * (size_t)std::max(1, nfa_b));
   HIPCHECK(hipMemset(d.hook.p, 0, d.hook.n * sizeof(double)));
+  if (!d.par.profiles_file.empty()) {
+    // PROFILES zones: the face values rho, u, 0, 0, p interpolated in y from
+    // the profile file (JoeWithModels.cpp:262-326), kept in the per-face
+    // boundary buffer that HOOK zones fill from the host
+    std::vector<double> h = profile_faces(d.par.profiles_file, m, d.zbc, d.par.sa ? 1 : 0);
+    HIPCHECK(hipMemcpy(d.hook.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
   d.counters.alloc(4);
   HIPCHECK(hipMemset(d.counters.p, 0, 4 * sizeof(int)));
   d.partials.alloc(8 * (size_t)(nblk(5 * n) + 64));

@@ -621,6 +621,7 @@ struct or_ctx {
   int ncv, nfa, nfa_b, nnz;
   std::vector<double> rho, rhou, rhoE, vel, press, temp, enthalpy, sos;
   std::vector<double> rho_bfa, T_bfa, vel_bfa, p_bfa, h_bfa;
+  std::vector<double> rho_bpr, vel_bpr, p_bpr;  // PROFILES face values (UgpWithCvCompFlow.h:679)
   std::vector<double> grad_rho, grad_p, grad_u, alpha_rho, local_dt;
   std::vector<double> mul_fa, lamOcp_fa, grad_h;
   std::vector<double> rhs_rho, rhs_rhou, rhs_rhoE, A, dq, resid;
@@ -777,6 +778,43 @@ void set_bc(or_ctx &c) {
         double c_bc = sqrt(gamma_bc * p_bc / rho_bc);
         double s_bc = pow(rho_bc, gamma_bc) / p_bc;
         for (int ifa = f0; ifa <= f1; ifa++) {
+          int icv0 = m.cvofa[2 * ifa];
+          double nVec[3];
+          norm3(nVec, &m.fa_normal[3 * ifa]);
+          double qn_bc = dot3(nVec, u_bc);
+          double vn_bc = qn_bc;
+          double qne = dot3(nVec, &c.vel[3 * icv0]);
+          double ce = sqrt(GAMMA * c.press[icv0] / c.rho[icv0]);
+          double ac1 = vn_bc > -c_bc ? qne + 2.0 * ovgm1 * ce : qn_bc + 2.0 * ovgm1 * c_bc;
+          double ac2 = vn_bc > c_bc ? qne - 2.0 * ovgm1 * ce : qn_bc - 2.0 * ovgm1 * c_bc;
+          double qnf = 0.5 * (ac1 + ac2);
+          double cf = 0.25 * (ac1 - ac2) * gm1;
+          double velf[3], sf;
+          if (vn_bc > 0) {
+            for (int i = 0; i < 3; i++) velf[i] = c.vel[3 * icv0 + i] + (qnf - qne) * nVec[i];
+            sf = pow(c.rho[icv0], GAMMA) / c.press[icv0];
+          } else {
+            for (int i = 0; i < 3; i++) velf[i] = u_bc[i] + (qnf - qn_bc) * nVec[i];
+            sf = s_bc;
+          }
+          c.rho_bfa[ifa] = pow((sf * cf * cf / GAMMA), ovgm1);
+          for (int i = 0; i < 3; i++) c.vel_bfa[3 * ifa + i] = velf[i];
+          c.p_bfa[ifa] = c.rho_bfa[ifa] * cf * cf / GAMMA;
+          c.T_bfa[ifa] = c.p_bfa[ifa] / (R_gas * c.rho_bfa[ifa]);
+        }
+        for (int ifa = f0; ifa <= f1; ifa++) c.h_bfa[ifa] = GAMMA * R_gas / (GAMMA - 1.0) * c.T_bfa[ifa];
+        break;
+      }
+      case OR_BC_PROFILES: {  // JoeWithModels.cpp:4190-4270: FARFIELD's characteristic state, per-face profile
+        const double gamma_bc = 1.4;
+        double gm1 = gamma_bc - 1.0;
+        double ovgm1 = 1.0 / gm1;
+        for (int ifa = f0; ifa <= f1; ifa++) {
+          double rho_bc = c.rho_bpr[ifa];
+          double u_bc[3] = {c.vel_bpr[3 * ifa], c.vel_bpr[3 * ifa + 1], c.vel_bpr[3 * ifa + 2]};
+          double p_bc = c.p_bpr[ifa];
+          double c_bc = sqrt(gamma_bc * p_bc / rho_bc);
+          double s_bc = pow(rho_bc, gamma_bc) / p_bc;
           int icv0 = m.cvofa[2 * ifa];
           double nVec[3];
           norm3(nVec, &m.fa_normal[3 * ifa]);
@@ -1928,6 +1966,31 @@ or_ctx *or_ctx_create(const or_mesh *m, const or_params *p) {
   c->enthalpy.assign(n, 0.0);
   c->sos.assign(n, 0.0);
   c->rho_bfa.assign(nb, 0.0);
+  c->rho_bpr.assign(nb, 0.0);
+  c->vel_bpr.assign(3 * nb, 0.0);
+  c->p_bpr.assign(nb, 0.0);
+  // boundary profiles of PROFILES zones, linear in y (JoeWithModels.cpp:296-326)
+  if (m->profiles)
+    for (int z = 0; z < m->nzone; z++) {
+      if (m->zone_bc[z] != OR_BC_PROFILES) continue;
+      const int nn = m->nprof, nv = m->nval_prof;
+      const double *pr = m->profiles;
+      for (int ifa = m->zone_ifa[2 * z]; ifa <= m->zone_ifa[2 * z + 1]; ifa++) {
+        const double y = m->x_fa[3 * ifa + 1];
+        int pos = 1;
+        while ((pr[pos * nv] < y) && (pos < nn - 1)) pos++;
+        double f;
+        if (y > pr[pos * nv]) f = 1.0;
+        else if (y < pr[(pos - 1) * nv]) f = 0.0;
+        else f = (y - pr[(pos - 1) * nv]) / (pr[pos * nv] - pr[(pos - 1) * nv]);
+        const double *r1 = pr + (pos - 1) * nv, *r2 = pr + pos * nv;
+        c->rho_bpr[ifa] = r1[1] + f * (r2[1] - r1[1]);
+        c->vel_bpr[3 * ifa] = r1[2] + f * (r2[2] - r1[2]);
+        c->vel_bpr[3 * ifa + 1] = 0.0;  // the reference drops v (:318)
+        c->vel_bpr[3 * ifa + 2] = 0.0;
+        c->p_bpr[ifa] = r1[4] + f * (r2[4] - r1[4]);
+      }
+    }
   c->T_bfa.assign(nb, 0.0);
   c->vel_bfa.assign(3 * nb, 0.0);
   c->p_bfa.assign(nb, 0.0);

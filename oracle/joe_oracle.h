@@ -27,7 +27,8 @@ enum {
   OR_BC_SYMMETRY = 4,
   OR_BC_NEUMANN = 5,
   OR_BC_WALL = 6,
-  OR_BC_FARFIELD = 7
+  OR_BC_FARFIELD = 7,
+  OR_BC_PROFILES = 8
 };
 enum { OR_HLLC = 0, OR_ROE = 1, OR_AUSM = 2 };
 enum { OR_GRAD_GG = 0, OR_GRAD_LS = 1 };
@@ -42,6 +43,10 @@ typedef struct {
   const int *zone_ifa;       /* [nzone][2] first, last face */
   const double *zone_par;    /* [nzone][8] keyword values (see or_ctx_create) */
   const double *zone_scal;   /* [nzone][2] nuSA boundary: kind (0 none, 1 value), value; NULL without SA */
+  /* the boundary-profile file of PROFILES zones (JoeWithModels.cpp:262-290):
+   * nprof rows of nval_prof values (y, rho, u, v, p, scalars); NULL without */
+  const double *profiles;
+  int nprof, nval_prof;
 } or_mesh;
 
 typedef struct {

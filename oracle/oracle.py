@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import re
 
 import numpy as np
 
@@ -20,7 +21,17 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
 BC = {"CBC": 1, "CBC_SUBSONIC_INLET": 2, "CBC_SUBSONIC_OUTLET": 3, "SYMMETRY": 4, "NEUMANN": 5, "WALL": 6,
-      "FARFIELD": 7}
+      "FARFIELD": 7, "PROFILES": 8}
+
+
+def read_profiles(path):
+    """A boundary-profile file as the reference reads it (JoeWithModels.cpp:272-284):
+    "n=<rows>\td=<values>", then rows of values (y, rho, u, v, p, scalars)."""
+    text = open(path).read()
+    mm = re.match(r"n=(\d+)\s*d=(\d+)", text)
+    n, d = int(mm.group(1)), int(mm.group(2))
+    vals = np.array([float(t) for t in text[mm.end():].split()[:n * d]], np.float64)
+    return vals.reshape(n, d)
 SCHEME = {"HLLC": 0, "ROE": 1, "AUSM": 2}
 
 
@@ -31,7 +42,7 @@ class _Mesh(C.Structure):
                 ("x_fa", C.c_void_p), ("fa_normal", C.c_void_p), ("x_cv", C.c_void_p), ("cv_volume", C.c_void_p),
                 ("nbocv_lsg", C.c_void_p), ("bfa_lsg", C.c_void_p),
                 ("nzone", C.c_int), ("zone_bc", C.c_void_p), ("zone_ifa", C.c_void_p), ("zone_par", C.c_void_p),
-                ("zone_scal", C.c_void_p)]
+                ("zone_scal", C.c_void_p), ("profiles", C.c_void_p), ("nprof", C.c_int), ("nval_prof", C.c_int)]
 
 
 class _Params(C.Structure):
@@ -173,6 +184,7 @@ class Oracle:
             return a.ctypes.data
 
         zl = [z for z in jmdump.zones(mesh) if z[1] == 1]
+        prof_file = None
         zbc = np.zeros(len(zl), np.int32)
         zifa = np.zeros((len(zl), 2), np.int32)
         zpar = np.zeros((len(zl), 8), np.float64)
@@ -183,6 +195,8 @@ class Oracle:
             zifa[i] = (f, l)
             if kw == "WALL":
                 zpar[i, 0] = float(toks[toks.index("TEMP") + 1]) if "TEMP" in toks else 0.0
+            elif kw == "PROFILES":
+                prof_file = toks[2]  # the last PROFILES zone's file serves all of them (:249-256)
             else:
                 vals = [float(t) for t in toks[2:]]
                 zpar[i, :len(vals)] = vals
@@ -212,6 +226,12 @@ class Oracle:
         m.zone_ifa = zifa.ctypes.data
         m.zone_par = zpar.ctypes.data
         m.zone_scal = zscal.ctypes.data
+        if prof_file is not None:
+            prof = np.ascontiguousarray(read_profiles(prof_file))
+            if prof.shape[1] - 5 != self.nscal:
+                raise ValueError(f"{prof_file} does not have the required number of values")
+            self._keep["prof"] = prof
+            m.profiles, m.nprof, m.nval_prof = prof.ctypes.data, prof.shape[0], prof.shape[1]
         self._mesh = m
         self.ctx = L.or_ctx_create(C.byref(m), C.byref(self.params))
 

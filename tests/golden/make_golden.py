@@ -70,7 +70,31 @@ RHS_CASES = [("bump24", "HLLC", {}), ("bump24", "ROE", {}), ("bump24", "AUSM", {
              # TIME_STEP = SUM_BASED local time step
              ("box4", "HLLC", {"dtsum": True}),
              ("bump24", "HLLC", {"visc": "hi", "dtsum": True}),
-             ("box4", "HLLC", {"visc": "hi", "sa": True, "dtsum": True})]
+             ("box4", "HLLC", {"visc": "hi", "sa": True, "dtsum": True}),
+             # PROFILES boundary condition (inlet state interpolated from a profile file)
+             ("bump24", "HLLC", {"prof": True}),
+             ("box4", "ROE", {"prof": True}),
+             ("bump24", "HLLC", {"visc": "pl", "sa": True, "prof": True})]
+
+
+def profile_table(mm, nscal):
+    """Seven rows (y, rho, u, v, p, scalars) covering the middle of the mesh's
+    y range, so faces below the first row and above the last take the end
+    values (the reference's clamps); v is non-zero although the reference
+    drops it."""
+    y0, y1 = float(mm.x_no[:, 1].min()), float(mm.x_no[:, 1].max())
+    ys = np.linspace(y0 + 0.15 * (y1 - y0), y1 - 0.2 * (y1 - y0), 7)
+    t = (ys - ys[0]) / (ys[-1] - ys[0])
+    rows = [ys, 1.0 + 0.05 * np.sin(3.0 * t), 0.45 + 0.15 * t * (1.0 - t), 0.1 + 0.0 * t, 1.0 - 0.03 * t]
+    rows += [0.004 + 0.002 * t] * nscal
+    return np.stack(rows, axis=1)
+
+
+def write_profiles(path, tab):
+    with open(path, "w") as f:
+        f.write(f"n={tab.shape[0]}\td={tab.shape[1]}\n")
+        for r in tab:
+            f.write(" ".join(repr(float(v)) for v in r) + "\n")
 
 
 def run_ref(args, cwd):
@@ -91,6 +115,8 @@ def tag(kw):
         t += "_first"
     if kw.get("dtsum"):
         t += "_dtsum"
+    if kw.get("prof"):
+        t += "_prof"
     return t
 
 
@@ -106,14 +132,23 @@ def main(only=None):
             visc = kw.pop("visc", None)
             with_sa = kw.pop("sa", False)
             dtsum = kw.pop("dtsum", False)
+            prof = kw.pop("prof", False)
             mm, joe = MESHES[name](scheme, **kw)
             if dtsum:
                 kw["dtsum"] = True
                 joe += "TIME_STEP = SUM_BASED\n"
-            tk = dict(kw, **({"visc": visc} if visc else {}), **({"sa": True} if with_sa else {}))
+            tk = dict(kw, **({"visc": visc} if visc else {}), **({"sa": True} if with_sa else {}),
+                      **({"prof": True} if prof else {}))
             fixture = f"rhs_{name}_{scheme.lower()}{tag(tk)}"
             if only and only not in fixture:
                 continue
+            tab = None
+            if prof:
+                kw["prof"] = True
+                line = [ln for ln in joe.splitlines() if ln.startswith("inlet = ")][0]
+                joe = joe.replace(line, "inlet = PROFILES prof.dat")
+                tab = profile_table(mm, 1 if with_sa else 0)
+                write_profiles(os.path.join(wd, "prof.dat"), tab)
             if visc:
                 kw["visc"] = visc
                 joe = joe.replace("MU_REF = 0.0", VISC[visc])
@@ -138,6 +173,8 @@ def main(only=None):
             out.update(RHO=rho, RHOU=rhou.reshape(-1), RHOE=rhoE, JOE_IN=np.frombuffer(joe.encode(), np.uint8))
             if with_sa:
                 out["NUSA"] = state["NUSA"]
+            if tab is not None:
+                out["PROFILE"] = tab
             np.savez_compressed(os.path.join(HERE, fixture + ".npz"), **out)
             print("wrote", name, scheme, kw)
 

@@ -61,6 +61,9 @@ def test_bad_connectivity_rejected():
     (lambda s: s.replace("CBC 0.5 0.1 0.0 1.0 1.0", "CBC 0.5 0.1"), "inlet"),
     (lambda s: s.replace("P_REF = 1.0", ""), "P_REF"),
     (lambda s: s + "TIME_STEP = MIN_BASED\n", "TIME_STEP"),
+    (lambda s: s.replace("inlet = CBC 0.5 0.1 0.0 1.0 1.0", "inlet = PROFILES /nonexistent/prof.dat"),
+     "boundary-profile file"),
+    (lambda s: s.replace("inlet = CBC 0.5 0.1 0.0 1.0 1.0", "inlet = PROFILES"), "PROFILES needs a file"),
 ])
 def test_joe_in_errors_before_device(edit, needle):
     mm, joe = cases.box_case(2, seed=0)
@@ -139,3 +142,18 @@ def test_server_without_device_is_an_error():
     mm, _ = cases.box_case(2, seed=0)
     mesh = joemi.Mesh.from_faces(mm)
     assert mesh.footprint(joemi.JM_MIXED) < mesh.footprint(joemi.JM_FP64)
+
+
+def test_profile_file_checked_before_device(tmp_path):
+    # the PROFILES file needs y, rho, u, v, p and one column per transported
+    # scalar (JoeWithModels.cpp:274-279) and at least two rows
+    mm, joe = cases.box_case(2, seed=0)
+    mesh = joemi.Mesh.from_faces(mm)
+    for body, needle in (("n=3\td=6\n" + "0 1 0.5 0 1 0.1\n" * 3, "required number of values"),
+                         ("n=1\td=5\n0 1 0.5 0 1\n", "needs n=<rows>"),
+                         ("n=3\td=5\n0 1 0.5 0 1\n", "needs n=<rows>")):
+        f = tmp_path / "prof.dat"
+        f.write_text(body)
+        with pytest.raises(joemi.JoemiError) as e:
+            joemi.Solver(mesh, joe.replace("inlet = CBC 0.5 0.1 0.0 1.0 1.0", f"inlet = PROFILES {f}"))
+        assert e.value.status == 3 and needle in str(e.value), str(e.value)

@@ -37,7 +37,24 @@ def mesh_for(fixture):
 
 
 def joe_text(d):
-    return bytes(d["JOE_IN"]).decode()
+    """The fixture's Joe.in; with a PROFILES zone, its profile table written
+    to a file (the format the reference reads) that the text then names."""
+    text = bytes(d["JOE_IN"]).decode()
+    if "PROFILE" in d:
+        import hashlib
+        import tempfile
+
+        tab = np.asarray(d["PROFILE"], np.float64)
+        path = os.path.join(tempfile.gettempdir(), f"joemi_prof_{hashlib.sha1(tab.tobytes()).hexdigest()[:16]}.dat")
+        if not os.path.exists(path):
+            tmp = path + f".{os.getpid()}"
+            with open(tmp, "w") as f:
+                f.write(f"n={tab.shape[0]}\td={tab.shape[1]}\n")
+                for r in tab:
+                    f.write(" ".join(repr(float(v)) for v in r) + "\n")
+            os.replace(tmp, path)
+        text = text.replace("PROFILES prof.dat", "PROFILES " + path)
+    return text
 
 
 def test_fixture_set_complete():
@@ -51,7 +68,8 @@ def test_fixture_set_complete():
             if z in P:
                 kinds.add(P[z][0][1])
         symm.add(P["SYMM_BC"][0][1])
-    assert kinds >= {"CBC", "CBC_SUBSONIC_INLET", "CBC_SUBSONIC_OUTLET", "WALL", "FARFIELD", "NEUMANN", "SYMMETRY"}
+    assert kinds >= {"CBC", "CBC_SUBSONIC_INLET", "CBC_SUBSONIC_OUTLET", "WALL", "FARFIELD", "NEUMANN", "SYMMETRY",
+                     "PROFILES"}
     assert symm == {"ORIGINAL_NUMERIC", "NEW_NUMERIC", "ANALYTICAL"}
     # viscous terms: both viscosity laws, with and without the gradient limiter, first order
     visc = [f for f in RHS_FIXTURES if "visc" in f]
