@@ -63,26 +63,84 @@ def measured_traffic(mesh, kernel):
     return k["fetch_bytes"] + k["write_bytes"], os.path.relpath(path, ROOT)
 
 
-def cpu_baseline(n_box=64, reps=12):
-    """Single-threaded CPU restatement of the reference (bit-identical to it)
-    on a smaller box of the same construction: Mcell-it/s of state + BC + dt +
-    calcRhs."""
+def _time_reference(mm, joe, state, reps):
+    """Seconds per residual of the reference itself (oracle/_ref/joe_ref_dump,
+    built from the reference's sources; its 'time' mode runs state, BC, dt and
+    an explicit calcRhs), or None where that binary is absent."""
+    import subprocess
+    import tempfile
+
+    from myjoe_amd import meshgen
+    from oracle import jmdump
+
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "oracle", "_ref", "joe_ref_dump")
+    if not os.access(exe, os.X_OK):
+        return None
+    with tempfile.TemporaryDirectory() as wd:
+        meshgen.write_msh(mm, os.path.join(wd, "box.msh"))
+        with open(os.path.join(wd, "Joe.in"), "w") as f:
+            f.write(joe)
+        rho, rhou, rhoE = state
+        jmdump.write(os.path.join(wd, "state.bin"), {"RHO": rho, "RHOU": rhou.reshape(-1), "RHOE": rhoE})
+        r = subprocess.run([exe, "time", "Joe.in", "state.bin", str(reps)], cwd=wd, capture_output=True, text=True,
+                           timeout=600)
+        for line in r.stdout.splitlines():
+            if line.startswith("SECONDS_PER_RESIDUAL"):
+                return float(line.split()[1])
+    return None
+
+
+def cpu_baseline(n_box=48, reps=12):
+    """CPU residual throughput on a smaller box of the same construction
+    (state + BC + dt + explicit calcRhs, HLLC 2nd order LS + limiter): the
+    restatement of the reference (bit-identical to it) on every host core
+    (one context per thread, up to 16), the headline; beside it one core of
+    the restatement and, where oracle/_ref is present, one core of the
+    reference itself (it has no threads; the ratio of the two relates the
+    restatement's numbers to the reference's)."""
+    import threading
+
     from myjoe_amd import cases, joemi
     from oracle.oracle import Oracle
 
     mm, joe = cases.box_case(n_box, seed=0)
     mesh = joemi.Mesh.from_faces(mm)
+    state = cases.random_state(mesh.ncv, seed=1)
+    dump = mesh.as_dump()
+    ref_s = _time_reference(mm, joe, state, reps)
     del mm
-    o = Oracle(mesh.as_dump(), joe)
-    o.set_state(*cases.random_state(mesh.ncv, seed=1))
-    o.residual(False)
-    t0 = time.perf_counter()
-    for _ in range(reps):
+
+    def port_run(out, k):
+        o = Oracle(dump, joe)
+        o.set_state(*state)
         o.residual(False)
-    dt = time.perf_counter() - t0
-    return {"value": round(mesh.ncv * reps / dt / 1e6, 4), "unit": "Mcell-it/s", "cores": 1, "kind": "port",
-            "sample": f"{n_box}^3 split box ({mesh.ncv} cells), {reps} residual evaluations (state, BC, dt, "
-                      f"calcRhs; HLLC 2nd order LS+limiter) in {dt:.1f} s on 1 host core"}
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            o.residual(False)
+        out[k] = time.perf_counter() - t0
+
+    one = [0.0]
+    port_run(one, 0)
+    port1 = one[0] / reps
+    cores = max(1, min(16, os.cpu_count() or 1))
+    dts = [0.0] * cores
+    threads = [threading.Thread(target=port_run, args=(dts, k)) for k in range(cores)]
+    t0 = time.perf_counter()
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    wall = time.perf_counter() - t0  # context set-up included, so a lower bound of the rate
+    rate = mesh.ncv * reps * cores / max(dts) / 1e6
+    out = {"value": round(rate, 4), "unit": "Mcell-it/s", "cores": cores, "kind": "port",
+           "sample": f"{n_box}^3 split box ({mesh.ncv} cells), {reps} residual evaluations (state, BC, dt, calcRhs; "
+                     f"HLLC 2nd order LS+limiter) per core on {cores} host cores, one restatement context per "
+                     f"thread ({max(dts):.1f} s, {wall:.1f} s with set-up)",
+           "port_1core": round(mesh.ncv / port1 / 1e6, 4)}
+    if ref_s is not None:
+        out["reference_1core"] = round(mesh.ncv / ref_s / 1e6, 4)
+        out["reference_over_port_time_1core"] = round(ref_s / port1, 3)
+    return out
 
 
 def streams_capacity(built, budget, max_active):

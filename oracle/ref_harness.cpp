@@ -7,6 +7,8 @@
 //   init  <Joe.in> <out.bin>              mesh arrays after Ugp/UgpWithCv init
 //   rhs   <Joe.in> <state.bin> <out.bin>  one implicit calcRhs on a given state
 //   solve <Joe.in> <out.bin> [nsnap]      JoeWithModels::run(); state history
+//   time  <Joe.in> <state.bin> <reps>     seconds per residual evaluation (state,
+//                                         BC, dt, explicit calcRhs), printed
 // rhs-sa / solve-sa run the same with the Spalart-Allmaras mix-in (RansTurbSA,
 // as joe.cpp's MyJoeSA); the state file may then carry NUSA.
 //
@@ -181,6 +183,41 @@ class RefJoe : public JoeWithModels {
     if (mu_ref > 0.0) d.f64("MUL_FA", mul_fa, nfa);
   }
 
+  // reps explicit residual evaluations on a given state, timed: the work of
+  // the restatement's or_residual(c, 0) (calcStateVariables, setBC, calcDt,
+  // calcRhs without A; JoeWithModels.cpp:943-1005)
+  double timeRhs(const Load &st, int reps) {
+    initializeFromRestartFile(getStringParam("RESTART"));
+    initialHook();
+    initialHookScalarRansTurbModel();
+    initialHookScalarRansCombModel();
+    const double *r = st.f64("RHO", ncv);
+    const double *ru = st.f64("RHOU", 3LL * ncv);
+    const double *re = st.f64("RHOE", ncv);
+    double *RHSrho = new double[ncv];
+    double (*RHSrhou)[3] = new double[ncv][3];
+    double *RHSrhoE = new double[ncv];
+    auto t0 = std::chrono::steady_clock::now();
+    for (int k = 0; k < reps; k++) {
+      for (int icv = 0; icv < ncv; icv++) {
+        rho[icv] = r[icv];
+        for (int i = 0; i < 3; i++) rhou[icv][i] = ru[3 * icv + i];
+        rhoE[icv] = re[icv];
+      }
+      calcStateVariables();
+      calcMaterialProperties();
+      setBC();
+      calcRansTurbViscMuet();
+      calcDt(cfl);
+      calcRhs(RHSrho, RHSrhou, RHSrhoE, NULL, NULL, NULL, false);
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    delete[] RHSrho;
+    delete[] RHSrhou;
+    delete[] RHSrhoE;
+    return std::chrono::duration<double>(t1 - t0).count() / reps;
+  }
+
   // One implicit residual evaluation in the order runBackwardEuler uses
   // (JoeWithModels.cpp:943-1005).
   void rhsOnce(const Load &st, Dump &d) {
@@ -350,6 +387,9 @@ int main(int argc, char **argv) {
       Dump d(argv[4]);
       joe.rhsOnce(st, d);
       joe.dumpMesh(d);
+    } else if (mode == "time") {
+      Load st(argv[3]);
+      printf("SECONDS_PER_RESIDUAL %.9e\n", joe.timeRhs(st, atoi(argv[4])));
     } else if (mode == "solve") {
       Dump d(argv[3]);
       joe.snap = &d;
