@@ -58,10 +58,9 @@ struct DMesh {
   const int *bfa_zone;    // [nfa_b]
   const double *vol;      // [ncv]
   const ZoneDev *zones;
-  const int *sched;       // [ncv] cell visiting order of the per-cell kernels
-  const int *ecell;       // [entries] cell of each entry
+  const int2 *ecp;        // [entries] cell of each entry and its position in the cell's list
   const double *cfl_r0;   // [entries][3] x_fa - x_cv
-  const int *wave_c0;     // [nwave+1] first cell of each 64-lane wavefront
+  const int *wave_e0;     // [nwave+1] first entry of each 64-lane wavefront
   int nwave;
   const int *foc;         // [entries] entry offset of the cell's k-th face in faocv order (Spalart-Allmaras)
 };
@@ -336,7 +335,7 @@ __device__ __forceinline__ void d_grad(const Blk &B, DMesh m, DParams P, const d
                                                  double *__restrict__ gradv) {
   const int t = xcd_block(B.id, B.n) * blockDim.x + threadIdx.x;
   if (t >= m.ncv) return;
-  const int c = m.sched[t];
+  const int c = t;
   const double *pc = prim + 8 * (size_t)c;
   const double rho = pc[0], u[3] = {pc[1], pc[2], pc[3]}, p = pc[4];
   const double xcv[3] = {pc[5], pc[6], pc[7]};
@@ -594,12 +593,12 @@ __device__ __forceinline__ EntryLane entry_lane(const DMesh &m, const Blk &B) {
   EntryLane L{-1, -1, 0};
   const int w = xcd_block(B.id, B.n) * kWaves + (threadIdx.x >> 6);
   if (w < m.nwave) {
-    const int c0 = m.wave_c0[w], c1 = m.wave_c0[w + 1];
-    const int e = m.cfl_i[c0] + (threadIdx.x & 63);
-    if (e < m.cfl_i[c1]) {
+    const int e = m.wave_e0[w] + (threadIdx.x & 63);
+    if (e < m.wave_e0[w + 1]) {
+      const int2 cp = m.ecp[e];
       L.e = e;
-      L.c = m.ecell[e];
-      L.pos = e - m.cfl_i[L.c];
+      L.c = cp.x;
+      L.pos = cp.y;
     }
   }
   return L;
@@ -1208,7 +1207,7 @@ __device__ __forceinline__ void d_jac(const Blk &B, DMesh m, DParams P, const do
                                                 const double *__restrict__ gradv) {
   const int t = xcd_block(B.id, B.n) * blockDim.x + threadIdx.x;
   if (t >= m.ncv) return;
-  const int c = m.sched[t];
+  const int c = t;
   const double gam = P.gamma;
   const double *pc = prim + 8 * (size_t)c;
   Side own;
@@ -1404,7 +1403,7 @@ __global__ void __launch_bounds__(kBlock) k_sa_cell(DMesh m, DParams P, const do
                                                     double *__restrict__ aux) {
   const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= m.ncv) return;
-  const int c = m.sched[t];
+  const int c = t;
   const double *pc = prim + 8 * (size_t)c;
   const double *phi = P.sa_phi;
   const double ph = phi[c];
@@ -1547,7 +1546,7 @@ __global__ void __launch_bounds__(kBlock) k_scal(DMesh m, DParams P, const doubl
                                                  double *__restrict__ rhs_s, double *__restrict__ AS) {
   const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= m.ncv) return;
-  const int c = m.sched[t];
+  const int c = t;
   const double gam = P.gamma;
   const bool snd = P.second_order != 0;
   const double *pc = prim + 8 * (size_t)c;
@@ -1895,7 +1894,7 @@ __global__ void __launch_bounds__(kBlock) k_flux(DMesh m, DParams P, const doubl
   __shared__ double sh[16];
   const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   double dtc = 1.0e20;
-  if (t < m.ncv) dtc = flux_cell<SCHEME, SND>(m, P, prim, grad, bfa, q, rhs, nfirst, dta, dt, m.sched[t]);
+  if (t < m.ncv) dtc = flux_cell<SCHEME, SND>(m, P, prim, grad, bfa, q, rhs, nfirst, dta, dt, t);
   double v[1] = {dtc};
   block_partials<1>(v, partials, sh, kMin);
 }
@@ -3338,8 +3337,8 @@ struct Ctx::Impl {
   DBuf<double> fgeo, fnorm, lsg, bfa_lsg, vol;
   DBuf<int> cfl_i, cfl_noc, nbocv_i, nbocv_v, bfa_cell, bfa_zone;
   DBuf<int2> cfl_e;
-  DBuf<int> sched;
-  DBuf<int> ecell, wave_c0;
+  DBuf<int2> ecp;
+  DBuf<int> wave_e0;
   DBuf<double> cfl_r0;
   int nwave = 0;
   DBuf<ZoneDev> zones;
@@ -3694,24 +3693,24 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
       }
       d.foc.upload(fo.data(), fo.size());
     }
-    // entry-parallel kernels: the cell of every entry, and wavefronts of 64
-    // lanes that each cover whole cells (consecutive cells while their
-    // entries fit)
-    std::vector<int> ecell(std::max<size_t>(1, dcf.size()));
+    // entry-parallel kernels: the cell of every entry and its position in
+    // the cell's list, and wavefronts of 64 lanes that each cover whole cells
+    // (consecutive cells while their entries fit), by first entry
+    std::vector<int2> ecp(std::max<size_t>(1, dcf.size()));
     for (int cd = 0; cd < ncv; cd++)
-      for (int e = dci[cd]; e < dci[cd + 1]; e++) ecell[e] = cd;
-    std::vector<int> wc;
+      for (int e = dci[cd]; e < dci[cd + 1]; e++) ecp[e] = make_int2(cd, e - dci[cd]);
+    std::vector<int> we;
     int c = 0;
     while (c < ncv) {
-      wc.push_back(c);
+      we.push_back(dci[c]);
       const int eb = dci[c];
       if (dci[c + 1] - eb > 64) throw std::runtime_error("a cell with more than 64 faces");
       while (c < ncv && dci[c + 1] - eb <= 64) c++;
     }
-    wc.push_back(ncv);
-    d.nwave = (int)wc.size() - 1;
-    d.ecell.upload(ecell.data(), ecell.size());
-    d.wave_c0.upload(wc.data(), wc.size());
+    we.push_back(dci[ncv]);
+    d.nwave = (int)we.size() - 1;
+    d.ecp.upload(ecp.data(), ecp.size());
+    d.wave_e0.upload(we.data(), we.size());
     d.cfl_noc.upload(noc.data(), noc.size());
   }
   std::vector<int> bcell(std::max(1, nfa_b), 0);
@@ -3727,9 +3726,6 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
     for (int cd = 0; cd < ncv; cd++) v[cd] = m.cv_volume[perm[cd]];
     d.vol.upload(v.data(), v.size());
   }
-  std::vector<int> ident(ncv);
-  for (int c = 0; c < ncv; c++) ident[c] = c;
-  d.sched.upload(ident.data(), ident.size());
   d.h_ref_nbocv_i = m.nbocv_i;
 
   // state buffers; prim carries x_cv in its last three slots
@@ -3811,10 +3807,9 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
   d.dm.bfa_zone = d.bfa_zone.p;
   d.dm.vol = d.vol.p;
   d.dm.zones = d.zones.p;
-  d.dm.sched = d.sched.p;
-  d.dm.ecell = d.ecell.p;
+  d.dm.ecp = d.ecp.p;
   d.dm.cfl_r0 = d.cfl_r0.p;
-  d.dm.wave_c0 = d.wave_c0.p;
+  d.dm.wave_e0 = d.wave_e0.p;
   d.dm.nwave = d.nwave;
   d.dm.foc = d.foc.p;
 
@@ -3868,8 +3863,8 @@ size_t Ctx::footprint(const Mesh &m, int precision) {
   size_t b = 0;
   b += nfa * (8 + 3) * 8;                          // fgeo, fnorm
   b += nnz * (3 * 8 + 4) + nfa_b * (3 * 8 + 4 + 4);  // lsg, nbocv_v, bfa_lsg, bfa_cell, bfa_zone
-  b += (ncv + 1) * 4 * 2 + ncv * 4 * 2;            // cfl_i, nbocv_i, sched, wave table
-  b += ent * (8 + 4 + 4 + 24);                     // entries, their blocks, their cells, limiter offsets
+  b += (ncv + 1) * 4 * 2 + ncv * 4;                // cfl_i, nbocv_i, wave table
+  b += ent * (8 + 4 + 8 + 24);                     // entries, their blocks, their cells and positions, limiter offsets
   b += ncv * (5 + 8 + 16 + 5 + 1 + 1) * 8;         // q, prim, grad, rhs, dt, vol
   b += nfa_b * (8 + 5) * 8;                        // bfa, hook
   b += (nnz + ncv) * 25 * sJ + 10 * ncv * 5 * sJ;  // A, LU, Krylov vectors
