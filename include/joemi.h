@@ -59,6 +59,23 @@ jm_status jm_mesh_from_faces(int nno, const double *x_no, int nfa, const int64_t
 /* Fluent ASCII .msh, as MshFilter reads it */
 jm_status jm_mesh_read_msh(const char *path, jm_mesh **out);
 
+/* A Ugp restart file (restart.out; replaces UgpWithCv::initializeFromRestartFile
+ * for restart input: Ugp::readRestart, Ugp.cpp:12473-13240, then the same
+ * init as .msh input). Serial files only (one partition, no periodic zones).
+ * The file's STEP, TIME and cell fields (RHO, RHOU, RHOE, nuSA, ...) are kept
+ * with the mesh, in its numbering: jm_mesh_restart_field copies one (count =
+ * ncv for a scalar, 3 ncv for a vector, [ncv][3]; out may be NULL to ask for
+ * the count) or fails with JM_ERR_ARG if the file had none of that name. */
+jm_status jm_mesh_read_restart(const char *path, jm_mesh **out);
+jm_status jm_mesh_restart_field(const jm_mesh *m, const char *name, double *out, int64_t *count);
+jm_status jm_mesh_restart_step(const jm_mesh *m, int *step, double *time);
+/* Ugp::writeRestart (Ugp.cpp:9902-10660) of a mesh and cell fields in its
+ * numbering (dims[k] = 1: ncv values, 3: [ncv][3]), in the serial layout the
+ * reference writes and reads: the reference's initializeFromRestartFile takes
+ * the file as its own. */
+jm_status jm_write_restart(const jm_mesh *m, const char *path, int nfield, const char *const *names,
+                           const double *const *data, const int *dims, int step, double time);
+
 /* A host that already holds the Ugp arrays (the reference's own objects)
  * hands them over without renumbering: UgpWithCv members cvofa, faocv_i/v,
  * nbocv_i/v, x_fa, fa_normal, x_cv, cv_volume, nbocv_lsg_coeff,
@@ -101,6 +118,11 @@ void jm_ctx_destroy(jm_ctx *c);
  * Setting the state starts a new solve: the step counter and CFL restart. */
 jm_status jm_set_state(jm_ctx *c, const double *rho, const double *rhou, const double *rhoE);
 jm_status jm_get_state(jm_ctx *c, double *rho, double *rhou, double *rhoE);
+/* After jm_set_state: the step count a solve resumed from a restart file
+ * continues from (the STEP value read with it; runBackwardEuler keeps it,
+ * JoeWithModels.cpp:977, unless RESET_STEP, :12-17). The CFL still starts from
+ * Joe.in's, as in the reference. */
+jm_status jm_set_step(jm_ctx *c, int step);
 /* JoeWithModels::initialHook uniform initial condition (JoeWithModels.cpp:156-186);
  * with TURB_MODEL = SA also nuSA = INITIAL_CONDITION_TURB (:219-240) */
 jm_status jm_init_uniform(jm_ctx *c);

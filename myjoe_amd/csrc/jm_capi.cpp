@@ -94,6 +94,52 @@ jm_status jm_mesh_read_msh(const char *path, jm_mesh **out) {
   });
 }
 
+jm_status jm_mesh_read_restart(const char *path, jm_mesh **out) {
+  return guard([&] {
+    if (!path || !out) throw std::exception();
+    std::unique_ptr<jm_mesh> m(new jm_mesh);
+    jm::read_restart(path, m->m);
+    *out = m.release();
+    return JM_OK;
+  });
+}
+
+jm_status jm_mesh_restart_field(const jm_mesh *mm, const char *name, double *out, int64_t *count) {
+  return guard([&] {
+    if (!mm || !name || !count) throw std::exception();
+    for (const auto &f : mm->m.restart_cv)
+      if (f.first == name) {
+        *count = (int64_t)f.second.size();
+        if (out) std::memcpy(out, f.second.data(), f.second.size() * sizeof(double));
+        return JM_OK;
+      }
+    throw std::invalid_argument(std::string("no cell field ") + name + " was read from a restart file");
+  });
+}
+
+jm_status jm_mesh_restart_step(const jm_mesh *mm, int *step, double *time) {
+  return guard([&] {
+    if (!mm) throw std::exception();
+    if (step) *step = mm->m.restart_step;
+    if (time) *time = mm->m.restart_time;
+    return JM_OK;
+  });
+}
+
+jm_status jm_write_restart(const jm_mesh *mm, const char *path, int nfield, const char *const *names,
+                           const double *const *data, const int *dims, int step, double time) {
+  return guard([&] {
+    if (!mm || !path || nfield < 0 || (nfield > 0 && (!names || !data || !dims))) throw std::exception();
+    std::vector<jm::RestartField> fields;
+    for (int k = 0; k < nfield; k++) {
+      if (!names[k] || !data[k]) throw std::exception();
+      fields.push_back({names[k], data[k], dims[k]});
+    }
+    jm::write_restart(path, mm->m, fields, step, time);
+    return JM_OK;
+  });
+}
+
 jm_status jm_mesh_from_ugp(const jm_ugp_view *v, jm_mesh **out) {
   return guard([&] {
     if (!v || !out) throw std::exception();
@@ -215,6 +261,14 @@ jm_status jm_set_state(jm_ctx *c, const double *rho, const double *rhou, const d
   return guard([&] {
     if (!c || !rho || !rhou || !rhoE) throw std::exception();
     c->c->set_state(rho, rhou, rhoE);
+    return JM_OK;
+  });
+}
+
+jm_status jm_set_step(jm_ctx *c, int step) {
+  return guard([&] {
+    if (!c) throw std::exception();
+    c->c->set_step(step);
     return JM_OK;
   });
 }

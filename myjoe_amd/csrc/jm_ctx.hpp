@@ -155,6 +155,8 @@ class Ctx {
   Ctx &operator=(const Ctx &) = delete;
 
   void set_state(const double *rho, const double *rhou, const double *rhoE);
+  // the step count a solve resumed from a restart file continues from
+  void set_step(int step);
   // the transported scalar nuSA (Spalart-Allmaras), reference numbering
   void set_scalar(const double *phi);
   void get_state(double *rho, double *rhou, double *rhoE);

@@ -155,6 +155,8 @@ void build_mesh(const FaceInput &in, Mesh &m) {
     cv_flag[icv] = cv_counts[i]++;
   }
   m.ncv = ncv;
+  m.cv_of_input.resize(in.ncv);
+  for (int c = 0; c < in.ncv; c++) m.cv_of_input[c] = cv_flag[cv_new[c]];
   m.cvofa.resize(2 * (size_t)nfa);
   for (int ifa = 0; ifa < nfa; ifa++) {
     m.cvofa[2 * ifa] = cv_flag[cvofa_tmp[2 * ifa]];
@@ -717,6 +719,285 @@ void read_msh(const std::string &path, Mesh &m) {
   in.face_zone = fzone.data();
   in.zone_names = names;
   build_mesh(in, m);
+}
+
+// ---------------------------------------------------------------------------
+// Ugp restart files (Ugp.cpp:9792-9821 record ids, 9902-10660 writeRestart,
+// 12473-13240 readRestart): int32 magic 123581321 and version 1, then records
+// of a 256-byte header (char name[52], int32 id, int64 skip to the next
+// header, int32 idata[16], double rdata[16]) followed by their data, in the
+// writer's byte order (the reader swaps when the magic comes out swapped).
+// ---------------------------------------------------------------------------
+namespace {
+
+enum : int {
+  kIoMagic = 123581321, kIoVersion = 1, kIoCounts = 10, kIoI0 = 11, kIoD0 = 12, kIoFaCheck = 20,
+  kIoNoofa = 21, kIoCvofa = 22, kIoFaZone = 23, kIoFaD1 = 24, kIoFaD3 = 25, kIoNoCheck = 30, kIoXNo = 31,
+  kIoNoD1 = 32, kIoNoD3 = 33, kIoCvCheck = 40, kIoCvPart = 41, kIoCvD1 = 42, kIoCvD3 = 43, kIoCvD33 = 44,
+  kIoData = 50, kIoEof = 51
+};
+constexpr int kIoHeader = 256, kIoNameLen = 52;
+
+struct IoHeader {
+  char name[kIoNameLen];
+  int32_t id;
+  int64_t skip;
+  int32_t idata[16];
+  double rdata[16];
+};
+static_assert(sizeof(IoHeader) == kIoHeader, "Ugp restart header is 256 bytes");
+
+template <typename T>
+void swap_bytes(T *p, size_t n) {
+  for (size_t i = 0; i < n; i++) {
+    unsigned char *b = reinterpret_cast<unsigned char *>(p + i);
+    std::reverse(b, b + sizeof(T));
+  }
+}
+
+struct RestartFile {
+  std::vector<unsigned char> bytes;
+  bool swap = false;
+  template <typename T>
+  std::vector<T> read(int64_t at, size_t n) const {
+    if (at < 0 || at + (int64_t)(n * sizeof(T)) > (int64_t)bytes.size()) fail("restart file is truncated");
+    std::vector<T> v(n);
+    std::memcpy(v.data(), bytes.data() + at, n * sizeof(T));
+    if (swap) swap_bytes(v.data(), n);
+    return v;
+  }
+  IoHeader header(int64_t at) const {
+    if (at < 0 || at + kIoHeader > (int64_t)bytes.size()) fail("restart file is truncated");
+    IoHeader h;
+    std::memcpy(&h, bytes.data() + at, kIoHeader);
+    if (swap) {
+      swap_bytes(&h.id, 1);
+      swap_bytes(&h.skip, 1);
+      swap_bytes(h.idata, 16);
+      swap_bytes(h.rdata, 16);
+    }
+    h.name[kIoNameLen - 1] = 0;
+    return h;
+  }
+};
+
+}  // namespace
+
+void read_restart(const std::string &path, Mesh &m) {
+  RestartFile f;
+  {
+    FILE *fp = std::fopen(path.c_str(), "rb");
+    if (!fp) fail("cannot open restart file " + path);
+    std::fseek(fp, 0, SEEK_END);
+    long n = std::ftell(fp);
+    std::fseek(fp, 0, SEEK_SET);
+    f.bytes.resize(n > 0 ? (size_t)n : 0);
+    const size_t got = f.bytes.empty() ? 0 : std::fread(f.bytes.data(), 1, f.bytes.size(), fp);
+    std::fclose(fp);
+    if (got != f.bytes.size()) fail("cannot read restart file " + path);
+  }
+  auto head = f.read<int32_t>(0, 2);
+  if (head[0] != kIoMagic) {
+    f.swap = true;
+    head = f.read<int32_t>(0, 2);
+    if (head[0] != kIoMagic) fail(path + " is not a Ugp restart file");
+  }
+  if (head[1] != kIoVersion) fail(path + ": restart version " + std::to_string(head[1]) + " (1 expected)");
+  int nno = -1, nfa = -1, ncv = -1, nnoofa = -1;
+  std::vector<int32_t> counts, noofa_v, cvofa, fzone;
+  std::vector<double> x_no;
+  std::vector<std::pair<int, std::string>> zone_names;
+  std::vector<std::pair<std::string, std::vector<double>>> cvf;
+  std::vector<int> cvf_dim;
+  int step = 0;
+  double time = 0.0;
+  int64_t at = 8;
+  for (bool done = false; !done;) {
+    const IoHeader h = f.header(at);
+    const int64_t data = at + kIoHeader;
+    switch (h.id) {
+      case kIoCounts:
+        nno = h.idata[0];
+        nfa = h.idata[1];
+        ncv = h.idata[2];
+        nnoofa = h.idata[3];
+        if (nno <= 0 || nfa <= 0 || ncv <= 0 || nnoofa <= 0) fail("restart file: bad counts");
+        fzone.assign(nfa, -1);
+        break;
+      case kIoNoofa:
+        if (h.idata[0] != nfa || h.idata[1] != nnoofa) fail("restart file: NOOFA counts disagree");
+        counts = f.read<int32_t>(data, nfa);
+        noofa_v = f.read<int32_t>(data + 4 * (int64_t)nfa, nnoofa);
+        break;
+      case kIoCvofa:
+        if (h.idata[0] != nfa || h.idata[1] != 2) fail("restart file: CVOFA counts disagree");
+        cvofa = f.read<int32_t>(data, 2 * (size_t)nfa);
+        break;
+      case kIoFaZone: {
+        const int kind = h.idata[0], f0 = h.idata[1], f1 = h.idata[2];
+        if (kind != kZoneBoundary && kind != kZoneInternal)
+          fail(std::string("restart file: zone ") + h.name + ": periodic zones are not supported");
+        if (f0 < 0 || f1 >= nfa) fail(std::string("restart file: zone ") + h.name + ": face range");
+        const int id = (int)zone_names.size();
+        zone_names.emplace_back(id, h.name);
+        for (int i = f0; i <= f1; i++) fzone[i] = id;
+        break;
+      }
+      case kIoXNo:
+        if (h.idata[0] != nno || h.idata[1] != 3) fail("restart file: X_NO counts disagree");
+        x_no = f.read<double>(data, 3 * (size_t)nno);
+        break;
+      case kIoI0:
+        if (std::strcmp(h.name, "STEP") == 0) step = h.idata[0];
+        break;
+      case kIoD0:
+        if (std::strcmp(h.name, "TIME") == 0) time = h.rdata[0];
+        break;
+      case kIoCvD1:
+      case kIoCvD3: {
+        const int dim = h.id == kIoCvD1 ? 1 : 3;
+        if (h.idata[0] != ncv) fail(std::string("restart file: field ") + h.name + " has the wrong count");
+        cvf.emplace_back(h.name, f.read<double>(data, (size_t)dim * ncv));
+        cvf_dim.push_back(dim);
+        break;
+      }
+      case kIoEof:
+        done = true;
+        break;
+      default:  // checks, partition, face/node/tensor fields, DATA marker
+        break;
+    }
+    if (h.skip < kIoHeader) fail("restart file: bad record length");
+    at += h.skip;
+  }
+  if (counts.empty() || cvofa.empty() || x_no.empty()) fail("restart file: mesh records missing");
+  std::vector<int64_t> fptr(nfa + 1, 0);
+  for (int i = 0; i < nfa; i++) {
+    if (counts[i] < 3) fail("restart file: a face with fewer than 3 nodes");
+    fptr[i + 1] = fptr[i] + counts[i];
+  }
+  if (fptr[nfa] != nnoofa) fail("restart file: NOOFA sizes disagree");
+  for (int i = 0; i < nfa; i++) {
+    if (fzone[i] < 0) fail("restart file: a face in no zone");
+    if (cvofa[2 * i + 1] < -1) fail("restart file: periodic faces are not supported");
+  }
+  std::vector<int> fnodes(noofa_v.begin(), noofa_v.end()), fcells(cvofa.begin(), cvofa.end());
+  FaceInput in;
+  in.nno = nno;
+  in.nfa = nfa;
+  in.ncv = ncv;
+  in.x_no = x_no.data();
+  in.face_ptr = fptr.data();
+  in.face_nodes = fnodes.data();
+  in.face_cells = fcells.data();
+  in.face_zone = fzone.data();
+  in.zone_names = zone_names;
+  build_mesh(in, m);
+  // the file's cell fields in the final numbering
+  m.restart_cv.clear();
+  for (size_t k = 0; k < cvf.size(); k++) {
+    const int dim = cvf_dim[k];
+    std::vector<double> v(cvf[k].second.size());
+    for (int c = 0; c < ncv; c++)
+      for (int i = 0; i < dim; i++) v[(size_t)dim * m.cv_of_input[c] + i] = cvf[k].second[(size_t)dim * c + i];
+    m.restart_cv.emplace_back(cvf[k].first, std::move(v));
+  }
+  m.restart_step = step;
+  m.restart_time = time;
+}
+
+void write_restart(const std::string &path, const Mesh &m, const std::vector<RestartField> &fields, int step,
+                   double time) {
+  std::vector<unsigned char> out;
+  auto put = [&](const void *p, size_t n) {
+    const unsigned char *b = static_cast<const unsigned char *>(p);
+    out.insert(out.end(), b, b + n);
+  };
+  // writeRestart reuses one header for all zone records, so a short zone
+  // name keeps the tail of a longer one after its terminator; the same here
+  char zone_name[kIoNameLen] = {0};
+  auto header = [&](const char *name, int id, int64_t data_bytes, std::initializer_list<int> idata,
+                    double r0 = 0.0) {
+    IoHeader h;
+    std::memset(&h, 0, sizeof h);
+    if (id == kIoFaZone) {
+      std::snprintf(zone_name, kIoNameLen, "%s", name);
+      std::memcpy(h.name, zone_name, kIoNameLen);
+    } else {
+      std::snprintf(h.name, kIoNameLen, "%s", name);
+    }
+    h.id = id;
+    h.skip = kIoHeader + data_bytes;
+    int k = 0;
+    for (int v : idata) h.idata[k++] = v;
+    h.rdata[0] = r0;
+    put(&h, sizeof h);
+  };
+  const int nno = m.nno, nfa = m.nfa, ncv = m.ncv, nnoofa = m.noofa_i[nfa];
+  // nodes: those of boundary faces first, then the rest, each in mesh order;
+  // faces by zone and cells in mesh order (writeRestart's serial layout)
+  std::vector<int> bnd(nno, 1), no_file(nno);
+  for (int i = m.noofa_i[0]; i < m.noofa_i[m.nfa_b]; i++) bnd[m.noofa_v[i]] = 0;
+  int nb = 0;
+  for (int n = 0; n < nno; n++) nb += bnd[n] == 0;
+  {
+    int k0 = 0, k1 = nb;
+    for (int n = 0; n < nno; n++) no_file[n] = bnd[n] == 0 ? k0++ : k1++;
+  }
+  const int32_t head[2] = {kIoMagic, kIoVersion};
+  put(head, sizeof head);
+  header("NO_FA_CV_NOOFA_COUNTS", kIoCounts, 0, {nno, nfa, ncv, nnoofa});
+  std::vector<int32_t> iv;
+  header("NO_CHECK", kIoNoCheck, 4LL * nno, {nno});
+  iv.resize(nno);
+  for (int n = 0; n < nno; n++) iv[n] = n;
+  put(iv.data(), 4 * iv.size());
+  header("FA_CHECK", kIoFaCheck, 4LL * nfa, {nfa});
+  iv.resize(nfa);
+  for (int i = 0; i < nfa; i++) iv[i] = i;
+  put(iv.data(), 4 * iv.size());
+  header("CV_CHECK", kIoCvCheck, 4LL * ncv, {ncv});
+  iv.resize(ncv);
+  for (int c = 0; c < ncv; c++) iv[c] = c;
+  put(iv.data(), 4 * iv.size());
+  header("NOOFA_I_AND_V", kIoNoofa, 4LL * nfa + 4LL * nnoofa, {nfa, nnoofa});
+  iv.resize(nfa);
+  for (int i = 0; i < nfa; i++) iv[i] = m.noofa_i[i + 1] - m.noofa_i[i];
+  put(iv.data(), 4 * iv.size());
+  iv.resize(nnoofa);
+  for (int i = 0; i < nnoofa; i++) iv[i] = no_file[m.noofa_v[i]];
+  put(iv.data(), 4 * iv.size());
+  header("CVOFA", kIoCvofa, 8LL * nfa, {nfa, 2});
+  iv.assign(m.cvofa.begin(), m.cvofa.end());
+  put(iv.data(), 4 * iv.size());
+  for (const Zone &z : m.zones) {
+    const int f0 = z.kind == kZoneInternal ? z.ifa_i_f : z.ifa_f, f1 = z.kind == kZoneInternal ? z.ifa_i_l : z.ifa_l;
+    header(z.name.c_str(), kIoFaZone, 0, {z.kind, f0, f1});
+  }
+  header("CV_PART", kIoCvPart, 4LL * ncv, {ncv, 1});
+  iv.assign(ncv, 0);
+  put(iv.data(), 4 * iv.size());
+  header("X_NO", kIoXNo, 24LL * nno, {nno, 3});
+  {
+    std::vector<double> x(3 * (size_t)nno);
+    for (int n = 0; n < nno; n++)
+      for (int i = 0; i < 3; i++) x[3 * (size_t)no_file[n] + i] = m.x_no[3 * (size_t)n + i];
+    put(x.data(), 8 * x.size());
+  }
+  header("DATA", kIoData, 0, {});
+  header("STEP", kIoI0, 0, {step});
+  header("TIME", kIoD0, 0, {}, time);
+  for (const RestartField &fd : fields) {
+    if (fd.dim != 1 && fd.dim != 3) fail("restart field " + fd.name + ": dimension 1 or 3");
+    header(fd.name.c_str(), fd.dim == 1 ? kIoCvD1 : kIoCvD3, 8LL * fd.dim * ncv, {ncv, fd.dim == 3 ? 3 : 0});
+    put(fd.data, 8 * (size_t)fd.dim * ncv);
+  }
+  header("EOF", kIoEof, 0, {});
+  FILE *fp = std::fopen(path.c_str(), "wb");
+  if (!fp) fail("cannot write restart file " + path);
+  const size_t wrote = std::fwrite(out.data(), 1, out.size(), fp);
+  const bool closed = std::fclose(fp) == 0;
+  if (wrote != out.size() || !closed) fail("cannot write restart file " + path);
 }
 
 }  // namespace jm

@@ -13,6 +13,7 @@
 
 #include <cstdint>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace jm {
@@ -42,6 +43,11 @@ struct Mesh {
   std::vector<double> bfa_lsg;           // [nfa_b][3]
   int count_neg_subtet = 0;              // sub-tets with non-positive volume
   double min_sdotn = 1.0;                // min over internal faces of s.n / |s||n|
+  std::vector<int> cv_of_input;          // final index of each input (file) cell
+  // cell fields of a restart file, in the final numbering (read_restart)
+  std::vector<std::pair<std::string, std::vector<double>>> restart_cv;
+  int restart_step = 0;
+  double restart_time = 0.0;
 
   int nbocv_s() const { return ncv ? nbocv_i[ncv] : 0; }
 };
@@ -62,6 +68,22 @@ void build_mesh(const FaceInput &in, Mesh &m);
 
 // Reads a Fluent ASCII .msh the way MshFilter does and builds the mesh.
 void read_msh(const std::string &path, Mesh &m);
+
+// Reads a Ugp restart file (serial: one partition, no periodic faces) the way
+// Ugp::readRestart does and builds the mesh; its STEP, TIME and cell fields
+// (scalars and 3-vectors, by name) are kept in m.restart_*.
+void read_restart(const std::string &path, Mesh &m);
+
+// Writes a Ugp restart file of the mesh (Ugp::writeRestart's serial layout)
+// with STEP, TIME and the given cell fields (ncv values for a scalar, 3 ncv
+// for a vector) in the mesh's numbering.
+struct RestartField {
+  std::string name;
+  const double *data;
+  int dim;  // 1 or 3
+};
+void write_restart(const std::string &path, const Mesh &m, const std::vector<RestartField> &fields, int step,
+                   double time);
 
 // Geometry, nbocv and LS coefficients from connectivity + x_no.
 void build_nbocv(Mesh &m);

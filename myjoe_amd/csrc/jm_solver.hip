@@ -3332,6 +3332,7 @@ struct Ctx::Impl {
   DMesh dm;
   double cfl = 0.0;
   int step_no = 0;
+  int step0 = 0;  // the step count the current solve started from (set_state: 0, set_step)
   bool have_state = false;
   // mesh buffers
   DBuf<double> fgeo, fnorm, lsg, bfa_lsg, vol;
@@ -3904,7 +3905,7 @@ void Ctx::device_step_async() {
     HIPCHECK(hipHostGetDevicePointer((void **)&d.mail_d, d.mail_h, 0));
     HIPCHECK(hipEventCreateWithFlags(&d.ev_step, hipEventDisableTiming));
   }
-  if (d.step_no == 0) d.launch_state_bc();
+  if (d.step_no == d.step0) d.launch_state_bc();
   if (d.dev_step_dirty) {
     hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, d.stream, d.scal.p, d.step_no, d.cfl);
     d.dev_step_dirty = false;
@@ -4327,7 +4328,7 @@ void Ctx::batch_prepare() {
     std::memset(d.mail_h, 0, sizeof(StepMail));
     HIPCHECK(hipHostGetDevicePointer((void **)&d.mail_d, d.mail_h, 0));
   }
-  if (d.step_no == 0) d.launch_state_bc();
+  if (d.step_no == d.step0) d.launch_state_bc();
   hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, d.stream, d.scal.p, d.step_no, d.cfl);
   d.dev_step_dirty = false;
   HIPCHECK(hipStreamSynchronize(d.stream));
@@ -4383,8 +4384,19 @@ void Ctx::set_state(const double *rho, const double *rhou, const double *rhoE) {
   HIPCHECK(hipMemcpyAsync(d.q.p, h.data(), sizeof(double) * 5 * n, hipMemcpyHostToDevice, d.stream));
   HIPCHECK(hipStreamSynchronize(d.stream));
   d.have_state = true;
-  d.step_no = 0;
+  d.step_no = d.step0 = 0;
   d.cfl = d.par.cfl;
+  d.dev_step_dirty = true;
+}
+
+// a solve resumed from a restart file continues its step count (the reference
+// keeps the STEP it read: runBackwardEuler does not reset it); the CFL starts
+// again from Joe.in's
+void Ctx::set_step(int step) {
+  Impl &d = *d_;
+  if (!d.have_state) throw std::logic_error("no state set");
+  if (step < 0) throw std::invalid_argument("step must be non-negative");
+  d.step_no = d.step0 = step;
   d.dev_step_dirty = true;
 }
 
@@ -4807,7 +4819,7 @@ StepInfo Ctx::step() {
   if (!d.have_state) throw std::logic_error("no state set");
   if (d.begun) throw std::logic_error("step called between step_begin and step_end");
   StepInfo info;
-  if (d.step_no == 0) d.launch_state_bc();
+  if (d.step_no == d.step0) d.launch_state_bc();
   d.step_no++;
   // CFL ramp (JoeWithModels.cpp:978)
   if ((d.step_no >= d.par.cfl_after) && (d.step_no % d.par.cfl_interval == 0) && (d.cfl < d.par.cfl_max))
@@ -4862,7 +4874,7 @@ void Ctx::step_begin(bool host_source) {
   d.dev_step_dirty = true;
   if (!d.have_state) throw std::logic_error("no state set");
   if (d.begun) throw std::logic_error("step_begin called twice without step_end");
-  if (d.step_no == 0) d.launch_state_bc();
+  if (d.step_no == d.step0) d.launch_state_bc();
   d.step_no++;
   if ((d.step_no >= d.par.cfl_after) && (d.step_no % d.par.cfl_interval == 0) && (d.cfl < d.par.cfl_max))
     d.cfl *= d.par.cfl_factor;
@@ -4946,12 +4958,12 @@ void Ctx::update_bc() {
 // doneSolver (JoeWithModels.h:42-95): NSTEPS, RESID_ENERG_TH, NaN
 bool Ctx::done_solver(const StepInfo &s) {
   Impl &d = *d_;
-  if (s.step == 1) d.resid0_energy = s.resid[4];
+  if (s.step == d.step0 + 1) d.resid0_energy = s.resid[4];
   const double r = s.resid[4];
   if ((d.par.nsteps >= 0) && (d.step_no >= d.par.nsteps)) return true;
   if (r < d.par.resid_energ_th) return true;
   if (r != r) return true;
-  if (d.par.resid_energ_rel > 0.0 && d.step_no >= 1 && r < d.par.resid_energ_rel * d.resid0_energy) return true;
+  if (d.par.resid_energ_rel > 0.0 && d.step_no > d.step0 && r < d.par.resid_energ_rel * d.resid0_energy) return true;
   return false;
 }
 
@@ -4961,8 +4973,8 @@ int Ctx::run(int max_steps, double *hist, bool graph) {
     StepInfo s0;
     s0.step = d.step_no;
     s0.resid[4] = 1.0e20;
-    if (d.step_no == 0 && done_solver(s0)) return 0;
-    if (d.step_no > 0 && ((d.par.nsteps >= 0) && (d.step_no >= d.par.nsteps))) return 0;
+    if (d.step_no == d.step0 && done_solver(s0)) return 0;
+    if (d.step_no > d.step0 && ((d.par.nsteps >= 0) && (d.step_no >= d.par.nsteps))) return 0;
   }
   int n = 0;
   while (n < max_steps) {

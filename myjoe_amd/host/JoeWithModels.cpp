@@ -65,11 +65,14 @@ JoeWithModels::JoeWithModels(const char *inputFile) {
   nsteps = getIntParam("NSTEPS", "100");
   resid_energ_th = getDoubleParam("RESID_ENERG_TH", "0.0");
 
-  // mesh: RESTART names a Fluent .msh (MshFilter path of UgpWithCv::init)
+  // mesh: RESTART names a Fluent .msh (MshFilter path of UgpWithCv::init) or
+  // a restart file (Ugp::readRestart), UgpWithCv.h:316-347
   const std::string restart = getStringParam("RESTART", "restart.out");
-  if (!ends_with(restart, ".msh"))
-    throw std::runtime_error("RESTART = " + restart + ": only Fluent .msh meshes are read by this build");
-  require(jm_mesh_read_msh(restart.c_str(), &mesh_), "jm_mesh_read_msh");
+  const bool from_restart = !ends_with(restart, ".msh");
+  if (from_restart)
+    require(jm_mesh_read_restart(restart.c_str(), &mesh_), "jm_mesh_read_restart");
+  else
+    require(jm_mesh_read_msh(restart.c_str(), &mesh_), "jm_mesh_read_msh");
   int64_t counts[8];
   require(jm_mesh_counts(mesh_, counts), "jm_mesh_counts");
   nno = (int)counts[0];
@@ -122,6 +125,44 @@ JoeWithModels::JoeWithModels(const char *inputFile) {
   const std::string prec = getStringParam("IMPLICIT_PRECISION", "FP64");
   if (prec != "FP64" && prec != "MIXED") throw std::runtime_error("IMPLICIT_PRECISION must be FP64 or MIXED");
   allocHost();
+  if (from_restart) {  // the registered fields the file held (Ugp.cpp:12570-12700)
+    auto field = [&](const char *name, double *out, size_t n) {
+      int64_t cnt = 0;
+      if (jm_mesh_restart_field(mesh_, name, nullptr, &cnt) != JM_OK) return false;
+      if ((size_t)cnt != n) throw std::runtime_error(std::string("restart field ") + name + " has the wrong size");
+      require(jm_mesh_restart_field(mesh_, name, out, &cnt), name);
+      return true;
+    };
+    read_rho_ = field("RHO", rho, ncv);
+    read_rhou_ = field("RHOU", &rhou[0][0], 3 * (size_t)ncv);
+    read_rhoE_ = field("RHOE", rhoE, ncv);
+    require(jm_mesh_restart_step(mesh_, &restart_step_, nullptr), "jm_mesh_restart_step");
+    step = restart_step_;
+  }
+}
+
+bool JoeWithModels::checkDataFlag(const void *field) const {
+  if (field == nullptr) return false;
+  if (field == rho) return read_rho_;
+  if (field == rhou) return read_rhou_;
+  if (field == rhoE) return read_rhoE_;
+  const RansTurbSA *sa = dynamic_cast<const RansTurbSA *>(this);
+  return sa && field == sa->nuSA && read_nuSA_;
+}
+
+void JoeWithModels::writeRestart(const std::string &filename) {
+  std::vector<const char *> names = {"RHO", "RHOU", "RHOE"};
+  std::vector<const double *> data = {rho, &rhou[0][0], rhoE};
+  std::vector<int> dims = {1, 3, 1};
+  const RansTurbSA *sa = dynamic_cast<const RansTurbSA *>(this);
+  if (sa && sa->nuSA) {
+    names.push_back("nuSA");
+    data.push_back(sa->nuSA);
+    dims.push_back(1);
+  }
+  require(jm_write_restart(mesh_, filename.c_str(), (int)names.size(), names.data(), data.data(), dims.data(), step,
+                           0.0),
+          "jm_write_restart");
 }
 
 // The device context, once the object is complete: IMPLICIT_PRECISION = FP64
@@ -191,10 +232,12 @@ void JoeWithModels::initialHook() {
   }
   double uu = 0.0;
   for (int i = 0; i < 3; i++) uu += u_init[i] * u_init[i];
+  // fields read from a restart file keep their values (JoeWithModels.cpp:200-211)
   for (int icv = 0; icv < ncv; icv++) {
-    rho[icv] = rho_init;
-    for (int i = 0; i < 3; i++) rhou[icv][i] = rho_init * u_init[i];
-    rhoE[icv] = p_init / (gamma[icv] - 1.0) + 0.5 * rho_init * uu;
+    if (!checkDataFlag(rho)) rho[icv] = rho_init;
+    if (!checkDataFlag(rhou))
+      for (int i = 0; i < 3; i++) rhou[icv][i] = rho_init * u_init[i];
+    if (!checkDataFlag(rhoE)) rhoE[icv] = p_init / (gamma[icv] - 1.0) + 0.5 * rho_init * uu;
   }
 }
 
@@ -285,16 +328,25 @@ int JoeWithModels::doneSolver(double residEnergy) {
 void JoeWithModels::run() {
   ensureContext();
   RansTurbSA *sa = dynamic_cast<RansTurbSA *>(this);
-  if (sa) {  // initialHook's uniform nuSA (JoeWithModels.cpp:219-240)
+  if (sa) {  // initialHook's uniform nuSA unless the restart file held it (JoeWithModels.cpp:219-240)
     sa->nuSA_v_.assign(ncv, std::stod(params_.count("INITIAL_CONDITION_TURB")
                                           ? getStringParam("INITIAL_CONDITION_TURB")
                                           : std::string("0.0")));
+    int64_t cnt = 0;
+    if (jm_mesh_restart_field(mesh_, "nuSA", nullptr, &cnt) == JM_OK && cnt == ncv) {
+      require(jm_mesh_restart_field(mesh_, "nuSA", sa->nuSA_v_.data(), &cnt), "nuSA");
+      read_nuSA_ = true;
+    }
     sa->nuSA = sa->nuSA_v_.data();
   }
   initialHook();
   require(jm_set_state(ctx_, rho, &rhou[0][0], rhoE), "jm_set_state");
   if (sa) require(jm_set_scalar(ctx_, "nuSA", sa->nuSA), "jm_set_scalar");
-  step = 0;
+  // the STEP of a restart file carries on (runBackwardEuler does not reset
+  // it) unless RESET_STEP (JoeWithModels.cpp:12-17)
+  step = checkParam("RESET_STEP") ? 0 : restart_step_;
+  if (step > 0) require(jm_set_step(ctx_, step), "jm_set_step");
+  const int write_restart = getIntParam("WRITE_RESTART", "-1");
   applyBoundaryHooks();
   const bool host_source = sourceHookOverridden();
   const bool split = host_source || !hook_zones_.empty();
@@ -335,6 +387,16 @@ void JoeWithModels::run() {
     applyBoundaryHooks();
     temporalHook();
     done = doneSolver(Residual[4]);
+    if ((write_restart > 0) && (step % write_restart == 0)) {  // JoeWithModels.cpp:1155-1156
+      syncStateFromDevice();
+      if (sa) {
+        int64_t n = ncv;
+        require(jm_get(ctx_, "NUSA", sa->nuSA_v_.data(), &n), "jm_get NUSA");
+      }
+      char name[32];
+      std::snprintf(name, sizeof name, "restart.%06d.out", step);
+      writeRestart(name);
+    }
   }
   syncStateFromDevice();
   if (sa) {
@@ -345,4 +407,5 @@ void JoeWithModels::run() {
     sa->wallDist = sa->wallDist_v_.data();
   }
   finalHook();
+  writeRestart();  // JoeWithModels.cpp:1175-1176
 }

@@ -18,6 +18,13 @@
 //   temporalHook()  after every step; finalHook() after the loop.
 //   doneSolver()    NSTEPS, RESID_ENERG_TH, NaN, a file named killjoe.
 //
+// RESTART names a Fluent .msh or a restart file of the reference's format
+// (Ugp::readRestart): then the mesh, RHO/RHOU/RHOE, nuSA and STEP come from
+// it (checkDataFlag tells initialHook which fields were read), and the step
+// count carries on unless RESET_STEP. Like runBackwardEuler, run() writes
+// restart.<step>.out every WRITE_RESTART steps and restart.out at the end
+// (writeRestart).
+//
 // A case class that also derives from RansTurbSA (as joe.cpp's MyJoeSA,
 // joe.cpp:286-296) solves the Spalart-Allmaras scalar nuSA with the flow.
 //
@@ -88,6 +95,12 @@ class JoeWithModels {
   void run();
   virtual int doneSolver(double residEnergy);
 
+  // whether a field (rho, rhou, rhoE, or a RansTurbSA's nuSA) was read from
+  // the restart file (Ugp::checkDataFlag)
+  bool checkDataFlag(const void *field) const;
+  // Ugp::writeRestart of the current host state (restart.out by default)
+  void writeRestart(const std::string &filename = "restart.out");
+
   // Joe.in access (Param.h): the first entry of a key, as the reference reads it
   bool checkParam(const std::string &name) const;
   double getDoubleParam(const std::string &name, const std::string &def = "") const;
@@ -143,4 +156,7 @@ class JoeWithModels {
   std::vector<int> cvofa_v_, nbocv_i_v_, nbocv_v_v_;
   std::vector<double> x_cv_v_, x_fa_v_, fa_normal_v_, cv_volume_v_;
   std::vector<double> rho_v_, rhou_v_, rhoE_v_, vel_v_, press_v_, temp_v_, gamma_v_, rom_v_;
+  // read from a restart file: the fields it held, its STEP
+  bool read_rho_ = false, read_rhou_ = false, read_rhoE_ = false, read_nuSA_ = false;
+  int restart_step_ = 0;
 };

@@ -34,7 +34,7 @@ class Recorder : public JoeWithModels {
   void write(const char *path) {
     std::FILE *fp = std::fopen(path, "wb");
     if (!fp) throw std::runtime_error("cannot write output");
-    double n = (double)step;
+    double n = (double)(hist.size() / 5);  // steps this run took (a resumed run starts past 0)
     std::fwrite(&n, sizeof(double), 1, fp);
     std::fwrite(hist.data(), sizeof(double), hist.size(), fp);
     std::fwrite(rho, sizeof(double), ncv, fp);

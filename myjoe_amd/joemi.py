@@ -49,6 +49,10 @@ def lib():
         L.jm_version.restype = C.c_char_p
         L.jm_mesh_from_faces.argtypes = [C.c_int, vp, C.c_int, vp, vp, vp, vp, C.c_int, C.c_int, vp, vp, vp]
         L.jm_mesh_read_msh.argtypes = [C.c_char_p, vp]
+        L.jm_mesh_read_restart.argtypes = [C.c_char_p, vp]
+        L.jm_mesh_restart_field.argtypes = [vp, C.c_char_p, vp, vp]
+        L.jm_mesh_restart_step.argtypes = [vp, ip, dp]
+        L.jm_write_restart.argtypes = [vp, C.c_char_p, C.c_int, vp, vp, vp, C.c_int, C.c_double]
         L.jm_mesh_from_ugp.argtypes = [vp, vp]
         L.jm_mesh_destroy.argtypes = [vp]
         L.jm_mesh_destroy.restype = None
@@ -60,6 +64,7 @@ def lib():
         L.jm_ctx_destroy.restype = None
         L.jm_set_state.argtypes = [vp, vp, vp, vp]
         L.jm_set_scalar.argtypes = [vp, C.c_char_p, vp]
+        L.jm_set_step.argtypes = [vp, C.c_int]
         L.jm_get_state.argtypes = [vp, vp, vp, vp]
         L.jm_init_uniform.argtypes = [vp]
         L.jm_calc_rhs.argtypes = [vp, C.c_int]
@@ -124,6 +129,40 @@ class Mesh:
         h = C.c_void_p()
         _check(lib().jm_mesh_read_msh(path.encode(), C.byref(h)))
         return cls(h)
+
+    @classmethod
+    def read_restart(cls, path):
+        """A Ugp restart file (restart.out): the mesh, and the file's step, time
+        and cell fields in the mesh's numbering (restart_field)."""
+        h = C.c_void_p()
+        _check(lib().jm_mesh_read_restart(path.encode(), C.byref(h)))
+        return cls(h)
+
+    def restart_field(self, name):
+        """A cell field read with the mesh from a restart file ([ncv] or [ncv][3])."""
+        n = C.c_int64()
+        _check(lib().jm_mesh_restart_field(self.h, name.encode(), None, C.byref(n)))
+        out = np.empty(n.value, np.float64)
+        _check(lib().jm_mesh_restart_field(self.h, name.encode(), _ptr(out), C.byref(n)))
+        return out.reshape(-1, 3) if n.value == 3 * self.ncv else out
+
+    def restart_step(self):
+        st, t = C.c_int(), C.c_double()
+        _check(lib().jm_mesh_restart_step(self.h, C.byref(st), C.byref(t)))
+        return st.value, t.value
+
+    def write_restart(self, path, fields: dict, step: int = 0, time: float = 0.0):
+        """A Ugp restart file of this mesh and cell fields in its numbering
+        (name -> [ncv] or [ncv][3] array), readable by the reference."""
+        keep = [np.ascontiguousarray(v, np.float64) for v in fields.values()]
+        dims = np.array([1 if a.size == self.ncv else 3 for a in keep], np.int32)
+        for a, d in zip(keep, dims):
+            if a.size != d * self.ncv:
+                raise ValueError("a restart field needs ncv or 3 ncv values")
+        names = (C.c_char_p * len(keep))(*[k.encode() for k in fields])
+        ptrs = (C.c_void_p * len(keep))(*[a.ctypes.data for a in keep])
+        _check(lib().jm_write_restart(self.h, path.encode(), len(keep), C.cast(names, C.c_void_p),
+                                      C.cast(ptrs, C.c_void_p), _ptr(dims), int(step), float(time)))
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
@@ -192,6 +231,10 @@ class Solver:
         rhoE = np.ascontiguousarray(rhoE, np.float64)
         assert rho.size == self.ncv and rhou.size == 3 * self.ncv and rhoE.size == self.ncv
         _check(lib().jm_set_state(self.h, _ptr(rho), _ptr(rhou), _ptr(rhoE)))
+
+    def set_step(self, step: int):
+        """After set_state: the step count a solve resumed from a restart file continues from."""
+        _check(lib().jm_set_step(self.h, int(step)))
 
     def set_scalar(self, phi, name: str = "nuSA"):
         phi = np.ascontiguousarray(phi, np.float64)

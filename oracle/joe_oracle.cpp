@@ -633,6 +633,7 @@ struct or_ctx {
   double cfl;
   int step;
   bool have_press;
+  bool resume_refresh = false;  // or_set_step: the first step refreshes as at step 0
 };
 
 namespace {
@@ -2097,8 +2098,17 @@ static int step_once(or_ctx &c) {
 }
 
 int or_step(or_ctx *c) {
-  if (c->step == 0) refresh(*c);
+  if (c->step == 0 || c->resume_refresh) refresh(*c);
+  c->resume_refresh = false;
   return step_once(*c);
+}
+
+// a solve resumed from a restart file keeps the step count it read
+// (runBackwardEuler does not reset it, JoeWithModels.cpp:977); CFL from Joe.in
+void or_set_step(or_ctx *c, int step) {
+  c->step = step;
+  c->cfl = c->p.cfl;
+  c->resume_refresh = true;
 }
 
 int or_run(or_ctx *c, int max_steps, double *hist) {

@@ -90,6 +90,8 @@ void or_residual(or_ctx *c, int implicit);
 void or_set_scalar(or_ctx *c, const double *phi);
 /* one backward-Euler step (JoeWithModels.cpp:975-1158); returns BiCGStab iterations */
 int or_step(or_ctx *c);
+/* the step count of a solve resumed from a restart file (its STEP) */
+void or_set_step(or_ctx *c, int step);
 /* runBackwardEuler from the current state; returns steps taken. */
 int or_run(or_ctx *c, int max_steps, double *resid_hist /* [max_steps][5 + nScal] or NULL */);
 /* named arrays: RHS_RHO RHS_RHOU RHS_RHOE A LOCAL_DT GRAD_RHO GRAD_P GRAD_U

@@ -70,6 +70,7 @@ def lib():
         L.or_ctx_destroy.argtypes = [C.c_void_p]
         L.or_set_state.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.or_set_scalar.argtypes = [C.c_void_p, C.c_void_p]
+        L.or_set_step.argtypes = [C.c_void_p, C.c_int]
         L.or_get_state.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.or_residual.argtypes = [C.c_void_p, C.c_int]
         L.or_step.argtypes = [C.c_void_p]
@@ -265,6 +266,10 @@ class Oracle:
 
     def step(self) -> int:
         return lib().or_step(self.ctx)
+
+    def set_step(self, step: int):
+        """The step count of a solve resumed from a restart file."""
+        lib().or_set_step(self.ctx, int(step))
 
     def run(self, max_steps: int):
         hist = np.zeros((max_steps, 5 + self.nscal))

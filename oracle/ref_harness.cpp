@@ -9,6 +9,9 @@
 //   solve <Joe.in> <out.bin> [nsnap]      JoeWithModels::run(); state history
 //   time  <Joe.in> <state.bin> <reps>     seconds per residual evaluation (state,
 //                                         BC, dt, explicit calcRhs), printed
+//   restart-write <Joe.in> <state.bin> <restart>  Ugp::writeRestart of a given state
+//   restart-read  <Joe.in> <out.bin>      mesh arrays and RHO, RHOU, RHOE (NUSA)
+//                                         after initializeFromRestartFile(RESTART)
 // rhs-sa / solve-sa run the same with the Spalart-Allmaras mix-in (RansTurbSA,
 // as joe.cpp's MyJoeSA); the state file may then carry NUSA.
 //
@@ -117,15 +120,20 @@ class RefJoe : public JoeWithModels {
     }
   }
 
+  // the first nsnap steps of this run, numbered from 1 (a run resumed from a
+  // restart file starts at the file's STEP + 1)
+  int snap_first = -1;
   void temporalHook() {
-    if (snap && step <= nsnap) {
+    if (snap_first < 0) snap_first = step;
+    const int k = step - snap_first + 1;
+    if (snap && k <= nsnap) {
       char s[16];
-      snprintf(s, 16, "@%d", step);
+      snprintf(s, 16, "@%d", k);
       dumpState(s);
       char n[32];
-      snprintf(n, 32, "RESIDFIELD@%d", step);
+      snprintf(n, 32, "RESIDFIELD@%d", k);
       snap->f64(n, residField, ncv);
-      snprintf(n, 32, "LOCALDT@%d", step);
+      snprintf(n, 32, "LOCALDT@%d", k);
       snap->f64(n, local_dt, ncv);
     }
   }
@@ -216,6 +224,38 @@ class RefJoe : public JoeWithModels {
     delete[] RHSrhou;
     delete[] RHSrhoE;
     return std::chrono::duration<double>(t1 - t0).count() / reps;
+  }
+
+  // Ugp::writeRestart (Ugp.cpp:9902-10660) of a given state on the mesh of RESTART
+  void writeState(const Load &st, const char *path) {
+    initializeFromRestartFile(getStringParam("RESTART"));
+    initialHook();
+    initialHookScalarRansTurbModel();
+    initialHookScalarRansCombModel();
+    const double *r = st.f64("RHO", ncv);
+    const double *ru = st.f64("RHOU", 3LL * ncv);
+    const double *re = st.f64("RHOE", ncv);
+    for (int icv = 0; icv < ncv; icv++) {
+      rho[icv] = r[icv];
+      for (int i = 0; i < 3; i++) rhou[icv][i] = ru[3 * icv + i];
+      rhoE[icv] = re[icv];
+    }
+    if (scalarTranspEqVector.size() > 0) {
+      const double *nu = st.f64("NUSA", ncv);
+      for (int icv = 0; icv < ncv; icv++) scalarTranspEqVector[0].phi[icv] = nu[icv];
+    }
+    calcStateVariables();
+    writeRestart(path);
+  }
+
+  // the mesh and conserved state initializeFromRestartFile (Ugp.cpp:12473-13240) gives
+  void readState(Dump &d) {
+    initializeFromRestartFile(getStringParam("RESTART"));
+    dumpMesh(d);
+    d.f64("RHO", rho, ncv);
+    d.f64("RHOU", &rhou[0][0], 3LL * ncv);
+    d.f64("RHOE", rhoE, ncv);
+    if (scalarTranspEqVector.size() > 0) d.f64("NUSA", scalarTranspEqVector[0].phi, ncv);
   }
 
   // One implicit residual evaluation in the order runBackwardEuler uses
@@ -387,6 +427,13 @@ int main(int argc, char **argv) {
       Dump d(argv[4]);
       joe.rhsOnce(st, d);
       joe.dumpMesh(d);
+    } else if (mode == "restart-write") {
+      if (argc < 5) return 2;
+      Load st(argv[3]);
+      joe.writeState(st, argv[4]);
+    } else if (mode == "restart-read") {
+      Dump d(argv[3]);
+      joe.readState(d);
     } else if (mode == "time") {
       Load st(argv[3]);
       printf("SECONDS_PER_RESIDUAL %.9e\n", joe.timeRhs(st, atoi(argv[4])));

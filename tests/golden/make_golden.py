@@ -178,6 +178,61 @@ def main(only=None):
             np.savez_compressed(os.path.join(HERE, fixture + ".npz"), **out)
             print("wrote", name, scheme, kw)
 
+        # restart files: the reference writes one for a given state
+        # (Ugp::writeRestart) and reads it back (initializeFromRestartFile),
+        # which renumbers the mesh; the fixture keeps the file, the state and
+        # the read-back arrays
+        for name, with_sa in (("bump24", False), ("box4", False), ("bump24", True)):
+            fixture = f"restart_{name}{'_sa' if with_sa else ''}"
+            if only and only not in fixture:
+                continue
+            mm, joe = MESHES[name]("HLLC")
+            if with_sa:
+                joe = cases.with_sa(joe.replace("MU_REF = 0.0", VISC["pl"]), 0.005, cases.SA_BUMP)
+            fn = "box.msh" if name.startswith("box") else "bump.msh"
+            meshgen.write_msh(mm, os.path.join(wd, fn))
+            with open(os.path.join(wd, "Joe.in"), "w") as f:
+                f.write(joe)
+            rho, rhou, rhoE = cases.random_state(mm.ncv, seed=9)
+            state = {"RHO": rho, "RHOU": rhou.reshape(-1), "RHOE": rhoE}
+            if with_sa:
+                state["NUSA"] = np.random.default_rng(13).uniform(0.001, 0.01, mm.ncv)
+            jmdump.write(os.path.join(wd, "state.bin"), state)
+            sa = "-sa" if with_sa else ""
+            run_ref(["restart-write" + sa, "Joe.in", "state.bin", "restart.out"], wd)
+            with open(os.path.join(wd, "Joe.in"), "w") as f:
+                f.write(joe.replace(f"RESTART = {fn}", "RESTART = restart.out"))
+            run_ref(["restart-read" + sa, "Joe.in", "back.bin"], wd)
+            d = jmdump.read(os.path.join(wd, "back.bin"))
+            raw = open(os.path.join(wd, "restart.out"), "rb").read()
+            out = {k: d[k] for k in MESH_KEYS + ["RHO", "RHOU", "RHOE"] + (["NUSA"] if with_sa else []) if k in d}
+            out.update({"RESTART_BYTES": np.frombuffer(raw, np.uint8), "JOE_IN": np.frombuffer(joe.encode(), np.uint8)})
+            out.update({k + "_IN": v for k, v in state.items()})
+            np.savez_compressed(os.path.join(HERE, fixture + ".npz"), **out)
+            print("wrote", fixture, len(raw), "bytes")
+
+        # a solve resumed from the restart file the reference's run() wrote
+        # after 20 steps: the next 5 steps (its STEP carries on, the CFL ramp
+        # sees steps 21-25)
+        if not only or only in "solve_restart_bump24":
+            _, joe = cases.bump_case(24, 8, nsteps=20)
+            meshgen.write_msh(cases.bump_case(24, 8)[0], os.path.join(wd, "bump.msh"))
+            with open(os.path.join(wd, "Joe.in"), "w") as f:
+                f.write(joe)
+            run_ref(["solve", "Joe.in", "solve0.bin", "0"], wd)
+            raw = open(os.path.join(wd, "restart.out"), "rb").read()
+            joe2 = cases.bump_case(24, 8, nsteps=25)[1].replace("RESTART = bump.msh", "RESTART = restart.out")
+            with open(os.path.join(wd, "Joe.in"), "w") as f:
+                f.write(joe2)
+            run_ref(["solve", "Joe.in", "solve1.bin", "5"], wd)
+            d = jmdump.read(os.path.join(wd, "solve1.bin"))
+            keep = {k: v for k, v in d.items() if k.split("@")[0] in ("RHO", "RHOU", "RHOE", "STEP")
+                    and (k.endswith(("@1", "@2", "@3", "@4", "@5")) or "@" not in k)}
+            keep["JOE_IN"] = np.frombuffer(joe2.encode(), np.uint8)
+            keep["RESTART_BYTES"] = np.frombuffer(raw, np.uint8)
+            np.savez_compressed(os.path.join(HERE, "solve_restart_bump24.npz"), **keep)
+            print("wrote solve_restart_bump24, final step", int(d["STEP"][0]))
+
         if only:
             return
         # steady-loop trajectories of JoeWithModels::run (uniform initial state)
