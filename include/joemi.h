@@ -158,6 +158,8 @@ typedef struct {
   double resid[5];     /* L1 residuals of rho, rhou(3), rhoE */
   double resid_scal;   /* L1 residual of nuSA over UNDER_RELAXATION (TURB_MODEL = SA; else 0) */
   int scal_iters;      /* its Jacobi BiCGStab iterations (solveCvScalarBcgstab) */
+  double resid_l2[5];  /* L2 norms (square root of the sum of squares) of the same residuals;
+                          the reference reports L1 only */
 } jm_step_info;
 jm_status jm_step(jm_ctx *c, jm_step_info *info);
 /* The same step driven by the device: CFL ramp and step counter on the

@@ -322,6 +322,7 @@ static void fill_info(const jm::StepInfo &s, jm_step_info *info) {
   info->cfl = s.cfl;
   info->dt_min = s.dt_min;
   for (int k = 0; k < 5; k++) info->resid[k] = s.resid[k];
+  for (int k = 0; k < 5; k++) info->resid_l2[k] = s.resid_l2[k];
   info->resid_scal = s.resid_s;
   info->scal_iters = s.scal_iters;
 }

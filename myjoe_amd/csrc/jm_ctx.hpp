@@ -143,6 +143,7 @@ struct StepInfo {
   int step = 0, lin_iters = 0, lin_status = 0;
   double cfl = 0, dt_min = 0;
   double resid[5] = {0, 0, 0, 0, 0};
+  double resid_l2[5] = {0, 0, 0, 0, 0};  // L2 norms of the same residual
   double resid_s = 0;  // L1 of the nuSA residual (Spalart-Allmaras)
   int scal_iters = 0;  // its Jacobi BiCGStab iterations
 };

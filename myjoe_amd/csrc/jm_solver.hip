@@ -86,6 +86,7 @@ struct SolverScalars {
   double res0_max, res_max;
   double dt_min;
   double resid[5];
+  double resid_l2[5];  // L2 norms of the same residual
   double resid_s;  // L1 of the transported scalar's residual (Spalart-Allmaras)
   int iter, done, maxiter, pad;
   double zero_abs, zero_rel;
@@ -98,7 +99,7 @@ struct SolverScalars {
 // what a device-driven step reports back, written to host-mapped memory
 struct StepMail {
   int step, lin_iters, lin_status, solver_stop;
-  double cfl, dt_min, resid[5];
+  double cfl, dt_min, resid[5], resid_l2[5];
 };
 
 namespace {
@@ -1801,6 +1802,10 @@ __device__ void fin_body(int nb, int op, const double *__restrict__ partials, si
       double v = fin_slot(partials + (size_t)k * stride, nb, kSum, sh);
       if (t0) S->resid[k] = v;
     }
+    for (int k = 0; k < 5; k++) {
+      double v = fin_slot(partials + (size_t)(5 + k) * stride, nb, kSum, sh);
+      if (t0) S->resid_l2[k] = sqrt(v);
+    }
   } else if (op == kFinL1S) {
     double v = fin_slot(partials, nb, kSum, sh);
     if (t0) S->resid_s = v;
@@ -2371,6 +2376,7 @@ __device__ __forceinline__ void d_publish(const Blk &B, const SolverScalars *S, 
   out->cfl = S->cfl_cur;
   out->dt_min = S->dt_min;
   for (int k = 0; k < 5; k++) out->resid[k] = S->resid[k];
+  for (int k = 0; k < 5; k++) out->resid_l2[k] = S->resid_l2[k];
   __threadfence_system();
 }
 
@@ -2754,14 +2760,19 @@ __global__ void __launch_bounds__(kBlock) k_time_term(int ncv, const int *__rest
   }
 }
 
-// L1 residual sums (JoeWithModels.cpp:1119-1130), finished by k_fin kFinL1
+// L1 residual sums (JoeWithModels.cpp:1119-1130) and the sums of squares of
+// the L2 norms beside them, finished by k_fin kFinL1
 __device__ __forceinline__ void d_l1(const Blk &B, int ncv, const double *__restrict__ rhs, double *partials) {
-  __shared__ double sh[16 * 5];
+  __shared__ double sh[16 * 10];
   int c = B.id * blockDim.x + threadIdx.x;
-  double v[5] = {0, 0, 0, 0, 0};
+  double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (c < ncv)
-    for (int k = 0; k < 5; k++) v[k] = fabs(rhs[(size_t)k * ncv + c]);
-  block_partials<5>(v, partials, sh, kSum, B);
+    for (int k = 0; k < 5; k++) {
+      const double r = rhs[(size_t)k * ncv + c];
+      v[k] = fabs(r);
+      v[5 + k] = r * r;
+    }
+  block_partials<10>(v, partials, sh, kSum, B);
 }
 
 __global__ void __launch_bounds__(kBlock) k_l1(int ncv, const double *__restrict__ rhs, double *partials) {
@@ -3962,6 +3973,7 @@ StepInfo Ctx::device_step_result() {
   info.cfl = m->cfl;
   info.dt_min = m->dt_min;
   for (int k = 0; k < 5; k++) info.resid[k] = m->resid[k];
+  for (int k = 0; k < 5; k++) info.resid_l2[k] = m->resid_l2[k];
   d.step_no = info.step;
   d.cfl = info.cfl;
   return info;
@@ -4362,6 +4374,7 @@ StepInfo Ctx::batch_result() {
   info.cfl = m->cfl;
   info.dt_min = m->dt_min;
   for (int k = 0; k < 5; k++) info.resid[k] = m->resid[k];
+  for (int k = 0; k < 5; k++) info.resid_l2[k] = m->resid_l2[k];
   d.step_no = info.step;
   d.cfl = info.cfl;
   return info;
@@ -4864,6 +4877,7 @@ void Ctx::Impl::finish_step(StepInfo &info, int it, int st) {
   info.cfl = cfl;
   info.dt_min = h_scal->dt_min;
   for (int k = 0; k < 5; k++) info.resid[k] = h_scal->resid[k];
+  for (int k = 0; k < 5; k++) info.resid_l2[k] = h_scal->resid_l2[k];
   info.resid_s = par.sa ? h_scal->resid_s : 0.0;
   info.scal_iters = par.sa ? scal_iters : 0;
 }

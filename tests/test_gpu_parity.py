@@ -85,6 +85,23 @@ def test_steps_track_oracle():
         compare(a, b, 1e-9, k)
 
 
+def test_step_reports_l1_and_l2_of_its_residual():
+    # the step's L1 norms (JoeWithModels.cpp:1119-1130) and the L2 norms
+    # beside them are those of the residual it leaves in RHS_*
+    print("L1/L2 report", flush=True)
+    mm, joe = cases.bump_case(24, 8, nsteps=5)
+    mesh = joemi.Mesh.from_faces(mm)
+    s = joemi.Solver(mesh, joe)
+    s.init_uniform()
+    for _ in range(3):
+        info = s.step()
+        r = np.concatenate([s.get("RHS_RHO").reshape(-1, 1), s.get("RHS_RHOU").reshape(-1, 3),
+                            s.get("RHS_RHOE").reshape(-1, 1)], axis=1)
+        assert np.max(r) > 0
+        np.testing.assert_allclose(np.array(info.resid), np.abs(r).sum(axis=0), rtol=1e-12, atol=0)
+        np.testing.assert_allclose(np.array(info.resid_l2), np.sqrt((r * r).sum(axis=0)), rtol=1e-12, atol=0)
+
+
 def test_mixed_precision_converges_to_fp64_state():
     mm, joe = cases.bump_case(24, 8, nsteps=600)
     mesh = joemi.Mesh.from_faces(mm)
@@ -118,6 +135,7 @@ def test_graph_step_equals_eager_step(precision):
         assert (ia.step, ia.lin_iters, ia.lin_status) == (ib.step, ib.lin_iters, ib.lin_status), k
         assert ia.cfl == ib.cfl and ia.dt_min == ib.dt_min
         assert list(ia.resid) == list(ib.resid), k
+        assert list(ia.resid_l2) == list(ib.resid_l2), k
     for x, y in zip(a.state(), b.state()):
         assert np.array_equal(x, y)
     # mixing the two forms keeps the step counter and CFL consistent
