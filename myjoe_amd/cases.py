@@ -12,6 +12,7 @@
 * ``random_state``: rho ~ U(0.5, 1.5), p ~ lognormal(0, 0.2),
   u_i ~ N(0, 0.3) * c, as conservative variables.
 * ``with_sa``: adds the Spalart-Allmaras scalar nuSA to a viscous case.
+* ``coupled``: the same case under runBackwardEulerCoupled.
 """
 from __future__ import annotations
 
@@ -99,6 +100,20 @@ def box_case(n: int, frac: float = 0.5, jitter: float = 0.2, seed: int = 0, sche
              bcs: dict | None = None, symm_bc: str = "ORIGINAL_NUMERIC"):
     mesh = meshgen.split_box(n, frac, jitter, seed)
     return mesh, box_joe_in(scheme, second_order, grad, restart, bcs, symm_bc)
+
+
+def coupled(joe: str, keep_symm_bc: bool = False) -> str:
+    """The same case run by runBackwardEulerCoupled: TIME_INTEGRATION =
+    BACKWARD_EULER_COUPLED with the reference's coupled linear solver keyword
+    (PETSC_GMRES; solved here by the block-Jacobi BiCGStab). Without
+    keep_symm_bc the SYMM_BC line is dropped, so the default applies (the
+    reference's init registers ORIGINAL_NUMERIC before calcFluxCoupled reads
+    the key with its own default)."""
+    joe = joe.replace("TIME_INTEGRATION = BACKWARD_EULER\n", "TIME_INTEGRATION = BACKWARD_EULER_COUPLED\n")
+    joe = joe.replace("LINEAR_SOLVER_NS = BCGSTAB\n", "LINEAR_SOLVER_NS = PETSC_GMRES\n")
+    if not keep_symm_bc:
+        joe = "\n".join(ln for ln in joe.split("\n") if not ln.startswith("SYMM_BC"))
+    return joe
 
 
 def random_state(ncv: int, seed: int = 1):

@@ -21,7 +21,9 @@ enum BcKind : int {
   kBcHook = 8,
   kBcProfiles = 9
 };
-enum Scheme : int { kHllc = 0, kRoe = 1, kAusm = 2 };
+// kHllcC: the coupled HLLC of runBackwardEulerCoupled (energy from the side
+// enthalpies, its own Jacobian and boundary treatments)
+enum Scheme : int { kHllc = 0, kRoe = 1, kAusm = 2, kHllcC = 3 };
 enum SymmBc : int { kSymmOriginal = 0, kSymmNew = 1, kSymmAnalytical = 2 };
 // device cell numbering: the reference's own, face-count groups inside
 // 1024-cell tiles, Morton order of centroids, or both
@@ -42,6 +44,10 @@ struct SolverParams {
   int symm_bc = kSymmOriginal;
   int dt_mode = 0;  // 0 CFL (local), 1 CFL_MIN, 2 DT
   bool dt_sum = false;  // TIME_STEP = SUM_BASED (MAX_BASED otherwise)
+  // TIME_INTEGRATION = BACKWARD_EULER_COUPLED: calcRhsCoupled's fluxes and
+  // Jacobian (scheme kHllcC; no scalars, inviscid) in the same backward-Euler
+  // loop, the block-Jacobi BiCGStab standing in for the reference's PETSc GMRES
+  bool coupled = false;
   double cfl = 0.0, dt_fixed = 0.0;
   double under_relax = 0.3;
   int ls_maxiter = 30;

@@ -551,6 +551,204 @@ __device__ __noinline__ void jac_roe(double J[5][5], int which, double rhoL, con
     }
 }
 
+// The coupled HLLC of runBackwardEulerCoupled: calcEulerFluxCoupled_HLLC
+// (UgpWithCvCompFlow.cpp:1808-1927) with nScal = 0 and kine = 0. The energy
+// is built from the given side enthalpies h0, h1 (calcThermoProp_p's, or the
+// boundary state's), not from gamma/(gamma-1) p/rho as flux_hllc does.
+__device__ __forceinline__ void flux_hllc_coupled(double F[5], double rhoL, const double *uL, double pL, double h0,
+                                                  double rhoR, const double *uR, double pR, double h1, double gam,
+                                                  double area, const double *nVec) {
+  const double gammaL = gam, gammaR = gam, kL = 0.0, kR = 0.0;
+  double unL = dot3(uL, nVec);
+  double uLuL = dot3(uL, uL);
+  double cL = sqrt(gammaL * pL / rhoL);
+  double rhoeL = (h0 + 0.5 * uLuL + kL) * rhoL - pL;
+  double unR = dot3(uR, nVec);
+  double uRuR = dot3(uR, uR);
+  double cR = sqrt(gammaR * pR / rhoR);
+  double rhoeR = (h1 + 0.5 * uRuR + kR) * rhoR - pR;
+  double Rrho = sqrt(rhoR / rhoL);
+  double tmp = 1.0 / (1.0 + Rrho);
+  double velRoe[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) velRoe[i] = tmp * (uL[i] + uR[i] * Rrho);
+  double uRoe = dot3(velRoe, nVec);
+  double gamPdivRho = tmp * ((gammaL * pL / rhoL + 0.5 * (gammaL - 1.0) * uLuL) +
+                             (gammaR * pR / rhoR + 0.5 * (gammaR - 1.0) * uRuR) * Rrho);
+  double cRoe = sqrt(gamPdivRho - ((gammaL + gammaR) * 0.5 - 1.0) * 0.5 * dot3(velRoe, velRoe));
+  double sL = Dev::mn(uRoe - cRoe, unL - cL);
+  double sR = Dev::mx(uRoe + cRoe, unR + cR);
+  double sM = (pL - pR - rhoL * unL * (sL - unL) + rhoR * unR * (sR - unR)) / (rhoR * (sR - unR) - rhoL * (sL - unL));
+  double pStar = rhoR * (unR - sR) * (unR - sM) + pR;
+  // the upwind side and whether the star state is used, chosen per lane
+  // before one evaluation
+  const bool left = sM >= 0.0;
+  const bool star = left ? !(sL > 0.0) : (sR >= 0.0);
+  const double rho = left ? rhoL : rhoR, p = left ? pL : pR, un = left ? unL : unR, rhoe = left ? rhoeL : rhoeR;
+  const double S = left ? sL : sR;
+  const double *u = left ? uL : uR;
+  if (!star) {
+    F[0] = rho * un;
+#pragma unroll
+    for (int i = 0; i < 3; i++) F[1 + i] = rho * u[i] * un + p * nVec[i];
+    F[4] = (rhoe + p) * un;
+  } else {
+    double inv = 1.0 / (S - sM);
+    double smu = S - un;
+    double rhoS = rho * smu * inv;
+    double rhouS[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) rhouS[i] = (rho * u[i] * smu + (pStar - p) * nVec[i]) * inv;
+    double rhoeS = (smu * rhoe - p * un + pStar * sM) * inv;
+    F[0] = rhoS * sM;
+#pragma unroll
+    for (int i = 0; i < 3; i++) F[1 + i] = rhouS[i] * sM + pStar * nVec[i];
+    F[4] = (rhoeS + pStar) * sM;
+  }
+#pragma unroll
+  for (int i = 0; i < 5; i++) F[i] *= area;
+}
+
+// calcJacobianCoupled_HLLC, UgpWithCvCompFlow.cpp:2180-2241 (nV not normalised)
+__device__ __forceinline__ void jac_coupled_super(double A[5][5], const double *vel, double pp, double rrho, double hh,
+                                                  const double *nV, double gamma) {
+  double kapm1 = (gamma - 1.0);
+  double nVel[3] = {vel[0] * nV[0], vel[1] * nV[1], vel[2] * nV[2]};
+  double U_k = nVel[0] + nVel[1] + nVel[2];
+  double vSquHlf = 0.5 * dot3(vel, vel);
+  A[0][0] = 0.0;
+  A[0][1] = nV[0];
+  A[0][2] = nV[1];
+  A[0][3] = nV[2];
+  A[0][4] = 0.0;
+#pragma unroll
+  for (int r = 0; r < 3; r++) {
+    A[1 + r][0] = -vel[r] * U_k + nV[r] * (kapm1 * vSquHlf + gamma * pp / rrho - kapm1 * hh);
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      A[1 + r][1 + c] = r == c ? (2.0 - gamma) * nVel[r] + U_k : vel[r] * nV[c] - kapm1 * vel[c] * nV[r];
+    A[1 + r][4] = kapm1 * nV[r];
+  }
+  A[4][0] = U_k * ((gamma - 2.0) * vSquHlf - gamma * (hh - pp / rrho));
+#pragma unroll
+  for (int c = 0; c < 3; c++) A[4][1 + c] = (hh + vSquHlf) * nV[c] - kapm1 * vel[c] * U_k;
+  A[4][4] = gamma * U_k;
+}
+
+// dF/dQ of one side of a face for the coupled HLLC (which 0: A_L, 1: A_R),
+// calcEulerFluxMatricesCoupled_HLLC (UgpWithCvCompFlow.cpp:1934-2174) with
+// calcdSMdU_HLLC and calcSubSonicJacobianCoupled_HLLC (:2246-2287), nScal = 0,
+// kine = 0. In every subsonic case the side's own dSM/dU is used with the
+// other side's rho (S - u_n), the star state is the upwind side's, and the
+// supersonic part and diagonal term belong to the upwind side only.
+__device__ __noinline__ void jac_hllc_coupled(double J[5][5], int which, double rhoL, const double *uL, double pL,
+                                              double h0, double rhoR, const double *uR, double pR, double h1,
+                                              double gam, double area, const double *nVec) {
+  const double gammaL = gam, gammaR = gam;
+  double unL = dot3(uL, nVec);
+  double uLuL = dot3(uL, uL);
+  double cL = sqrt(gammaL * pL / rhoL);
+  double hL = h0 + 0.0;
+  double rhoeL = (hL + 0.5 * uLuL) * rhoL - pL;
+  double unR = dot3(uR, nVec);
+  double uRuR = dot3(uR, uR);
+  double cR = sqrt(gammaR * pR / rhoR);
+  double hR = h1 + 0.0;
+  double rhoeR = (hR + 0.5 * uRuR) * rhoR - pR;
+  double Rrho = sqrt(rhoR / rhoL);
+  double tmp = 1.0 / (1.0 + Rrho);
+  double velRoe[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) velRoe[i] = tmp * (uL[i] + uR[i] * Rrho);
+  double uRoe = dot3(velRoe, nVec);
+  double gamPdivRho = tmp * ((gammaL * pL / rhoL + 0.5 * (gammaL - 1.0) * uLuL) +
+                             (gammaR * pR / rhoR + 0.5 * (gammaR - 1.0) * uRuR) * Rrho);
+  double cRoe = sqrt(gamPdivRho - ((gammaL + gammaR) * 0.5 - 1.0) * 0.5 * dot3(velRoe, velRoe));
+  double sL = Dev::mn(uRoe - cRoe, unL - cL);
+  double sR = Dev::mx(uRoe + cRoe, unR + cR);
+  double invrhotld = 1.0 / (rhoR * (sR - unR) - rhoL * (sL - unL));
+  double sM = (pL - pR - rhoL * unL * (sL - unL) + rhoR * unR * (sR - unR)) / (rhoR * (sR - unR) - rhoL * (sL - unL));
+  double pStar = rhoR * (unR - sR) * (unR - sM) + pR;
+  double nVecArea[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) nVecArea[i] = nVec[i] * area;
+  const bool left = sM >= 0.0;  // upwind side
+  const bool sup = left ? (sL > 0.0) : !(sR >= 0.0);
+  const bool own = (which == 0) == left;  // this side is the upwind one
+  // this side's state
+  const double *u = which == 0 ? uL : uR;
+  const double p = which == 0 ? pL : pR, rho = which == 0 ? rhoL : rhoR, h = which == 0 ? hL : hR;
+  if (own) {
+    jac_coupled_super(J, u, p, rho, h, nVecArea, gam);
+  } else {
+#pragma unroll
+    for (int i = 0; i < 5; i++)
+#pragma unroll
+      for (int j = 0; j < 5; j++) J[i][j] = 0.0;
+  }
+  if (sup) return;
+  // the upwind side's star state
+  const double *uu = left ? uL : uR;
+  const double Su = left ? sL : sR, unu = left ? unL : unR, rhou_ = left ? rhoL : rhoR, pu = left ? pL : pR;
+  const double rhoeu = left ? rhoeL : rhoeR;
+  double Omega = 1.0 / (Su - sM);
+  double smu = Su - unu;
+  double rhoS = rhou_ * smu * Omega;
+  double rhouS[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) rhouS[i] = (rhou_ * uu[i] * smu + (pStar - pu) * nVec[i]) * Omega;
+  double rhoeS = (smu * rhoeu - pu * unu + pStar * sM) * Omega;
+  // this side's dSM/dU (calcdSMdU_HLLC with nV the unit normal)
+  const double S = which == 0 ? sL : sR, Factor = which == 0 ? 1.0 : -1.0;
+  double d[5];
+  {
+    double kapm1 = (gam - 1.0);
+    double nVel[3] = {u[0] * nVec[0], u[1] * nVec[1], u[2] * nVec[2]};
+    double U_k = nVel[0] + nVel[1] + nVel[2];
+    double vSquHlf = 0.5 * dot3(u, u);
+    d[0] = Factor * invrhotld * (-U_k * U_k + kapm1 * vSquHlf + gam * p / rho - kapm1 * h + sM * S);
+#pragma unroll
+    for (int i = 0; i < 3; i++) d[1 + i] = Factor * invrhotld * ((2 * U_k - S - sM) * nVec[i] - kapm1 * u[i]);
+    d[4] = Factor * invrhotld * kapm1;
+  }
+  // the other side's rho (S - u_n)
+  const double rhoSmq = which == 0 ? rhoR * (sR - unR) : rhoL * (sL - unL);
+  double OmSM = Omega * sM;
+  double OmSMp1 = OmSM + 1.0;
+#pragma unroll
+  for (int i = 0; i < 5; i++) {
+    J[0][i] = -OmSM * J[0][i] + area * OmSMp1 * d[i] * rhoS;
+    J[1][i] = -OmSM * J[1][i] + area * OmSMp1 * d[i] * (rhouS[0] + rhoSmq * nVec[0]);
+    J[2][i] = -OmSM * J[2][i] + area * OmSMp1 * d[i] * (rhouS[1] + rhoSmq * nVec[1]);
+    J[3][i] = -OmSM * J[3][i] + area * OmSMp1 * d[i] * (rhouS[2] + rhoSmq * nVec[2]);
+    J[4][i] = -OmSM * J[4][i] + area * OmSMp1 * d[i] * (rhoeS + pStar + rhoSmq * sM);
+  }
+  if (own)
+#pragma unroll
+    for (int i = 0; i < 5; i++) J[i][i] += area * Omega * sM * S;
+}
+
+// The ONLY_PRESSURE Jacobian of a symmetry or wall face (A_L), coupled form
+// (UgpWithCvCompFlow.cpp:2091-2137)
+__device__ __forceinline__ void jac_only_pressure(double J[5][5], double rhoL, const double *uL, double pL,
+                                                  double h0, double gam, double area, const double *nVec) {
+  double uLuL = dot3(uL, uL);
+  double hL = h0 + 0.0;
+#pragma unroll
+  for (int j = 0; j < 5; j++) J[0][j] = 0.0;
+  double dpdrho = 0.5 * (gam - 1.0) * uLuL - (gam - 1.0) * hL + gam * pL / rhoL;
+#pragma unroll
+  for (int r = 1; r <= 3; r++) {
+    J[r][0] = dpdrho * nVec[r - 1] * area;
+    J[r][1] = -(gam - 1.0) * uL[0] * nVec[r - 1] * area;
+    J[r][2] = -(gam - 1.0) * uL[1] * nVec[r - 1] * area;
+    J[r][3] = -(gam - 1.0) * uL[2] * nVec[r - 1] * area;
+    J[r][4] = (gam - 1.0) * nVec[r - 1] * area;
+  }
+#pragma unroll
+  for (int j = 0; j < 5; j++) J[4][j] = 0.0;
+}
+
 // ludeco / lusolv on a 5x5 block, UgpWithCv.cpp:2489-2540 (T = double or float)
 template <typename T>
 __device__ __forceinline__ void ludeco5(T A[5][5]) {

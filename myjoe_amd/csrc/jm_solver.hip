@@ -1239,14 +1239,24 @@ __device__ __forceinline__ void d_jac(const Blk &B, DMesh m, DParams P, const do
       // memory, so the order of their updates does not matter.
       const bool s0 = en.x >= 0;
       const Side l = s0 ? own : nb, r = s0 ? nb : own;
-      face_jac(SCHEME, J, 0, l, r, gam, area, n);  // Apl
+      double hl = 0.0, hr = 0.0;
+      if (SCHEME == kHllcC) {  // the cells' enthalpies (JoeWithModels.cpp:5035-5038)
+        hl = enthalpy_of(P, l.rho, l.p);
+        hr = enthalpy_of(P, r.rho, r.p);
+        jac_hllc_coupled(J, 0, l.rho, l.u, l.p, hl, r.rho, r.u, r.p, hr, gam, area, n);
+      } else {
+        face_jac(SCHEME, J, 0, l, r, gam, area, n);  // Apl
+      }
       if (s0) {
         for (int i = 0; i < 5; i++)
           for (int j = 0; j < 5; j++) D[i][j] += J[i][j];
       } else {
         store_block(off, J, true, accum);
       }
-      face_jac(SCHEME, J, 1, l, r, gam, area, n);  // Ami
+      if (SCHEME == kHllcC)
+        jac_hllc_coupled(J, 1, l.rho, l.u, l.p, hl, r.rho, r.u, r.p, hr, gam, area, n);
+      else
+        face_jac(SCHEME, J, 1, l, r, gam, area, n);  // Ami
       if (s0) {
         store_block(off, J, false, accum);
       } else {
@@ -1258,7 +1268,31 @@ __device__ __forceinline__ void d_jac(const Blk &B, DMesh m, DParams P, const do
       const double *bb = bfa + 8 * (size_t)f;
       Side sb;
       sb.rho = bb[0]; sb.u[0] = bb[1]; sb.u[1] = bb[2]; sb.u[2] = bb[3]; sb.p = bb[4];
-      if (z.bc == kBcSymmetry || z.bc == kBcWall) {
+      if (SCHEME == kHllcC) {  // calcFluxCoupled boundary Jacobians (JoeWithModels.cpp:5150-5740)
+        const double hb = bb[6];
+        if (z.bc == kBcSymmetry || z.bc == kBcWall || z.bc == kBcNeumann) {
+          if (z.bc != kBcNeumann && P.symm_bc == kSymmAnalytical) {
+            jac_only_pressure(J, sb.rho, sb.u, sb.p, hb, gam, area, n);
+            for (int i = 0; i < 5; i++)
+              for (int j = 0; j < 5; j++) D[i][j] += J[i][j];
+          } else if (P.symm_bc == kSymmNew) {
+            double K[5][5];
+            jac_hllc_coupled(J, 0, sb.rho, sb.u, sb.p, hb, sb.rho, sb.u, sb.p, hb, gam, area, n);
+            jac_hllc_coupled(K, 1, sb.rho, sb.u, sb.p, hb, sb.rho, sb.u, sb.p, hb, gam, area, n);
+            for (int i = 0; i < 5; i++)
+              for (int j = 0; j < 5; j++) D[i][j] += J[i][j] + K[i][j];
+          } else {
+            jac_hllc_coupled(J, 0, sb.rho, sb.u, sb.p, hb, sb.rho, sb.u, sb.p, hb, gam, area, n);
+            for (int i = 0; i < 5; i++)
+              for (int j = 0; j < 5; j++) D[i][j] += J[i][j];
+          }
+        } else {  // cell-centre state against the boundary state
+          jac_hllc_coupled(J, 0, own.rho, own.u, own.p, enthalpy_of(P, own.rho, own.p), sb.rho, sb.u, sb.p, hb, gam,
+                           area, n);
+          for (int i = 0; i < 5; i++)
+            for (int j = 0; j < 5; j++) D[i][j] += J[i][j];
+        }
+      } else if (z.bc == kBcSymmetry || z.bc == kBcWall) {
         if (z.bc == kBcWall || P.symm_bc == kSymmOriginal) {
           face_jac(SCHEME, J, 0, sb, sb, gam, area, n);
           for (int i = 0; i < 5; i++)
@@ -1968,6 +2002,7 @@ __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const
     // mixes internal and boundary faces, runs the flux arithmetic once
     Side fl, fr;
     bool symm_analytic = false;
+    double hl = 0.0, hr = 0.0;  // side enthalpies of the coupled flux
     if (en.y >= 0) {
       const bool s0 = (en.x >= 0);
       double pn[8], gn[15];
@@ -1995,6 +2030,10 @@ __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const
       fl = s0 ? so : sn;
       fr = s0 ? sn : so;
       if (!s0) sgn = 1.0;
+      if (SCHEME == kHllcC) {  // calcThermoProp_p after reconstruction, or the cell's (JoeWithModels.cpp:4920-4928)
+        hl = enthalpy_of(P, fl.rho, fl.p);
+        hr = enthalpy_of(P, fr.rho, fr.p);
+      }
       if (VISC) {  // calcViscousFluxNS internal faces (JoeWithModels.cpp:3635-3710)
         const VSide vo = vside_cell(P, pc, gradv + 12 * (size_t)L.c);
         const VSide vn = vside_cell(P, pn, gradv + 12 * (size_t)en.y);
@@ -2017,15 +2056,21 @@ __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const
       const double *bb = bfa + 8 * (size_t)f;
       Side sb;
       sb.rho = bb[0]; sb.u[0] = bb[1]; sb.u[1] = bb[2]; sb.u[2] = bb[3]; sb.p = bb[4];
-      if (z.bc == kBcSymmetry || z.bc == kBcWall) {
+      hl = bb[6];
+      hr = bb[6];
+      if (z.bc == kBcSymmetry || z.bc == kBcWall || (SCHEME == kHllcC && z.bc == kBcNeumann)) {
         fl = sb;
         fr = sb;
-        symm_analytic = z.bc == kBcSymmetry && P.symm_bc == kSymmAnalytical;
+        // the coupled loop treats walls as symmetry faces and NEUMANN faces
+        // with the boundary state on both sides (JoeWithModels.cpp:5150-5470)
+        symm_analytic = SCHEME == kHllcC ? (z.bc != kBcNeumann && P.symm_bc == kSymmAnalytical)
+                                         : (z.bc == kBcSymmetry && P.symm_bc == kSymmAnalytical);
       } else {
         Side so = own_cc;
         if (SND) low = !reconstruct(so, pc, gc, xfa);
         fl = so;
         fr = sb;
+        if (SCHEME == kHllcC) hl = enthalpy_of(P, so.rho, so.p);
       }
       if (VISC) {  // boundary faces (:3714-3851); boundary mul from T_bfa (ComputeBCProperties_T)
         mul = mu_lam(P, bb[5]);
@@ -2054,8 +2099,11 @@ __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const
         }
       }
     }
-    face_flux(SCHEME, F, fl, fr, gam, area, n);
-    if (symm_analytic) {
+    if (SCHEME == kHllcC)
+      flux_hllc_coupled(F, fl.rho, fl.u, fl.p, hl, fr.rho, fr.u, fr.p, hr, gam, area, n);
+    else
+      face_flux(SCHEME, F, fl, fr, gam, area, n);
+    if (symm_analytic) {  // the ONLY_PRESSURE flux
       F[0] = 0.0;
       for (int i = 0; i < 3; i++) F[1 + i] = fr.p * n[i] * area;
       F[4] = 0.0;
@@ -3242,9 +3290,24 @@ void parse_params(const std::string &joe_in, const Mesh &m, SolverParams &p, std
     throw std::invalid_argument("TIME_STEP = " + tstep + " (MAX_BASED or SUM_BASED)");
   p.dt_sum = tstep == "SUM_BASED";
   std::string ti = pm.str("TIME_INTEGRATION", "BACKWARD_EULER");
-  if (ti != "BACKWARD_EULER") throw Unsupported("TIME_INTEGRATION = " + ti + " is not supported");
+  if (ti == "BACKWARD_EULER_COUPLED") p.coupled = true;  // JoeWithModels.cpp:39, 4326-4594
+  else if (ti != "BACKWARD_EULER")
+    throw Unsupported("TIME_INTEGRATION = " + ti + " is not supported (BACKWARD_EULER, BACKWARD_EULER_COUPLED)");
   std::string ls = pm.str("LINEAR_SOLVER_NS", "BCGSTAB");
-  if (ls != "BCGSTAB") throw Unsupported("LINEAR_SOLVER_NS = " + ls + " is not supported (BCGSTAB)");
+  if (p.coupled) {
+    // the reference's coupled loop solves with PETSc GMRES only
+    // (UgpWithCvCompFlow.h:1000-1037); this build runs its block-Jacobi
+    // BiCGStab with LINEAR_SOLVER_NS_TRESHOLDS for either keyword
+    if (ls != "PETSC_GMRES" && ls != "BCGSTAB")
+      throw Unsupported("LINEAR_SOLVER_NS = " + ls + " is not supported (PETSC_GMRES, BCGSTAB)");
+    if (p.scheme != kHllc) throw Unsupported("BACKWARD_EULER_COUPLED is supported with HLLC only");
+    if (p.sa) throw Unsupported("BACKWARD_EULER_COUPLED with scalars (TURB_MODEL) is not supported");
+    if (p.mu_ref > 0.0) throw Unsupported("BACKWARD_EULER_COUPLED with MU_REF > 0 is not supported");
+    if (p.flux_kernel != kResidEntry) throw Unsupported("BACKWARD_EULER_COUPLED needs GPU_FLUX_KERNEL = ENTRY");
+    p.scheme = kHllcC;
+  } else if (ls != "BCGSTAB") {
+    throw Unsupported("LINEAR_SOLVER_NS = " + ls + " is not supported (BCGSTAB)");
+  }
   p.under_relax = pm.dbl("UNDER_RELAXATION", 0.3);
   if (pm.has("LINEAR_SOLVER_NS_TRESHOLDS")) {
     p.ls_maxiter = pm.sub_i("LINEAR_SOLVER_NS_TRESHOLDS", "MAX_ITER");
@@ -4323,6 +4386,7 @@ void StepBatch::post(void *stream) const {
 void Ctx::batch_prepare() {
   Impl &d = *d_;
   if (d.par.sa) throw Unsupported("batched serving does not carry the Spalart-Allmaras scalar");
+  if (d.par.coupled) throw Unsupported("batched serving runs the segregated backward-Euler step");
   if (!d.flux_entry() || d.face_form())
     throw Unsupported("batched serving runs the per-entry flux kernel (GPU_FLUX_KERNEL = ENTRY)");
   if (!d.have_state) throw std::logic_error("no state set");
@@ -4547,10 +4611,12 @@ void Ctx::Impl::launch_flux(hipStream_t s, const double *cfl_dev) {
       if (par.second_order) {
         if (par.scheme == kRoe) goe(k_flux_e<kRoe, true, false>);
         else if (par.scheme == kAusm) goe(k_flux_e<kAusm, true, false>);
+        else if (par.scheme == kHllcC) goe(k_flux_e<kHllcC, true, false>);
         else goe(k_flux_e<kHllc, true, false>);
       } else {
         if (par.scheme == kRoe) goe(k_flux_e<kRoe, false, false>);
         else if (par.scheme == kAusm) goe(k_flux_e<kAusm, false, false>);
+        else if (par.scheme == kHllcC) goe(k_flux_e<kHllcC, false, false>);
         else goe(k_flux_e<kHllc, false, false>);
       }
     }
@@ -4641,6 +4707,7 @@ void Ctx::Impl::launch_jac(bool step_mode) {
     else go(k_jac<T, kHllc, true>);
   } else {
     if (par.scheme == kRoe) go(k_jac<T, kRoe, false>);
+    else if (par.scheme == kHllcC) go(k_jac<T, kHllcC, false>);
     else go(k_jac<T, kHllc, false>);
   }
 }

@@ -348,7 +348,11 @@ void JoeWithModels::run() {
   if (step > 0) require(jm_set_step(ctx_, step), "jm_set_step");
   const int write_restart = getIntParam("WRITE_RESTART", "-1");
   applyBoundaryHooks();
-  const bool host_source = sourceHookOverridden();
+  // runBackwardEulerCoupled calls sourceHookCoupled (default: gravity, which
+  // the device applies) instead of sourceHook (JoeWithModels.cpp:4596-4611),
+  // so a sourceHook override takes no part in a coupled run
+  const bool coupled = getStringParam("TIME_INTEGRATION", "BACKWARD_EULER") == "BACKWARD_EULER_COUPLED";
+  const bool host_source = !coupled && sourceHookOverridden();
   const bool split = host_source || !hook_zones_.empty();
   if (host_source) {
     src_rho_.assign(ncv, 0.0);

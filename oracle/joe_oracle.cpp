@@ -569,6 +569,245 @@ void jac_roe(double (*A_L)[5], double (*A_R)[5], double rhoL, const double *uL, 
 }
 
 // ludeco / lusolv, UgpWithCv.cpp:2489-2540
+// calcEulerFluxCoupled_HLLC, UgpWithCvCompFlow.cpp:1808-1927, nScal = 0,
+// kine = 0: the energy is built from the given enthalpy h (calcThermoProp_p's
+// or the cell's), not from gamma/(gamma-1) p/rho as the segregated flux does.
+// only_pressure: the "ONLY_PRESSURE" treatment of symmetry and wall faces.
+void flux_hllc_coupled(double F[5], double rhoL, const double *uL, double pL, double h0, double gammaL, double rhoR,
+                       const double *uR, double pR, double h1, double gammaR, double area, const double *nVec,
+                       bool only_pressure) {
+  if (only_pressure) {
+    F[0] = 0.0;
+    for (int i = 1; i < 4; i++) F[i] = pL * nVec[i - 1];
+    F[4] = 0.0;
+    for (int i = 0; i < 5; i++) F[i] *= area;
+    return;
+  }
+  const double kL = 0.0, kR = 0.0;
+  double unL = dot3(uL, nVec);
+  double uLuL = dot3(uL, uL);
+  double cL = sqrt(gammaL * pL / rhoL);
+  double rhoeL = (h0 + 0.5 * uLuL + kL) * rhoL - pL;
+  double unR = dot3(uR, nVec);
+  double uRuR = dot3(uR, uR);
+  double cR = sqrt(gammaR * pR / rhoR);
+  double rhoeR = (h1 + 0.5 * uRuR + kR) * rhoR - pR;
+  double Rrho = sqrt(rhoR / rhoL);
+  double tmp = 1.0 / (1.0 + Rrho);
+  double velRoe[3];
+  for (int i = 0; i < 3; i++) velRoe[i] = tmp * (uL[i] + uR[i] * Rrho);
+  double uRoe = dot3(velRoe, nVec);
+  double gamPdivRho = tmp * ((gammaL * pL / rhoL + 0.5 * (gammaL - 1.0) * uLuL) +
+                             (gammaR * pR / rhoR + 0.5 * (gammaR - 1.0) * uRuR) * Rrho);
+  double cRoe = sqrt(gamPdivRho - ((gammaL + gammaR) * 0.5 - 1.0) * 0.5 * dot3(velRoe, velRoe));
+  double sL = min(uRoe - cRoe, unL - cL);
+  double sR = max(uRoe + cRoe, unR + cR);
+  double sM = (pL - pR - rhoL * unL * (sL - unL) + rhoR * unR * (sR - unR)) / (rhoR * (sR - unR) - rhoL * (sL - unL));
+  double pStar = rhoR * (unR - sR) * (unR - sM) + pR;
+  if (sM >= 0.0) {
+    if (sL > 0.0) {
+      F[0] = rhoL * unL;
+      for (int i = 0; i < 3; i++) F[1 + i] = rhoL * uL[i] * unL + pL * nVec[i];
+      F[4] = (rhoeL + pL) * unL;
+    } else {
+      double invSLmSs = 1.0 / (sL - sM);
+      double sLmuL = sL - unL;
+      double rhoSL = rhoL * sLmuL * invSLmSs;
+      double rhouSL[3];
+      for (int i = 0; i < 3; i++) rhouSL[i] = (rhoL * uL[i] * sLmuL + (pStar - pL) * nVec[i]) * invSLmSs;
+      double rhoeSL = (sLmuL * rhoeL - pL * unL + pStar * sM) * invSLmSs;
+      F[0] = rhoSL * sM;
+      for (int i = 0; i < 3; i++) F[1 + i] = rhouSL[i] * sM + pStar * nVec[i];
+      F[4] = (rhoeSL + pStar) * sM;
+    }
+  } else {
+    if (sR >= 0.0) {
+      double invSRmSs = 1.0 / (sR - sM);
+      double sRmuR = sR - unR;
+      double rhoSR = rhoR * sRmuR * invSRmSs;
+      double rhouSR[3];
+      for (int i = 0; i < 3; i++) rhouSR[i] = (rhoR * uR[i] * sRmuR + (pStar - pR) * nVec[i]) * invSRmSs;
+      double rhoeSR = (sRmuR * rhoeR - pR * unR + pStar * sM) * invSRmSs;
+      F[0] = rhoSR * sM;
+      for (int i = 0; i < 3; i++) F[1 + i] = rhouSR[i] * sM + pStar * nVec[i];
+      F[4] = (rhoeSR + pStar) * sM;
+    } else {
+      F[0] = rhoR * unR;
+      for (int i = 0; i < 3; i++) F[1 + i] = rhoR * uR[i] * unR + pR * nVec[i];
+      F[4] = (rhoeR + pR) * unR;
+    }
+  }
+  for (int i = 0; i < 5; i++) F[i] *= area;
+}
+
+// calcJacobianCoupled_HLLC, UgpWithCvCompFlow.cpp:2180-2241 (nV not normalised)
+static void jac_coupled_super(double (*A)[5], const double *vel, double pp, double rrho, double hh, const double *nV,
+                              double gamma) {
+  double kapm1 = (gamma - 1.0);
+  double nVel[3] = {vel[0] * nV[0], vel[1] * nV[1], vel[2] * nV[2]};
+  double U_k = nVel[0] + nVel[1] + nVel[2];
+  double vSquHlf = 0.5 * dot3(vel, vel);
+  A[0][0] = 0.0;
+  A[0][1] = nV[0];
+  A[0][2] = nV[1];
+  A[0][3] = nV[2];
+  A[0][4] = 0.0;
+  A[1][0] = -vel[0] * U_k + nV[0] * (kapm1 * vSquHlf + gamma * pp / rrho - kapm1 * hh);
+  A[1][1] = (2.0 - gamma) * nVel[0] + U_k;
+  A[1][2] = vel[0] * nV[1] - kapm1 * vel[1] * nV[0];
+  A[1][3] = vel[0] * nV[2] - kapm1 * vel[2] * nV[0];
+  A[1][4] = kapm1 * nV[0];
+  A[2][0] = -vel[1] * U_k + nV[1] * (kapm1 * vSquHlf + gamma * pp / rrho - kapm1 * hh);
+  A[2][1] = vel[1] * nV[0] - kapm1 * vel[0] * nV[1];
+  A[2][2] = (2.0 - gamma) * nVel[1] + U_k;
+  A[2][3] = vel[1] * nV[2] - kapm1 * vel[2] * nV[1];
+  A[2][4] = kapm1 * nV[1];
+  A[3][0] = -vel[2] * U_k + nV[2] * (kapm1 * vSquHlf + gamma * pp / rrho - kapm1 * hh);
+  A[3][1] = vel[2] * nV[0] - kapm1 * vel[0] * nV[2];
+  A[3][2] = vel[2] * nV[1] - kapm1 * vel[1] * nV[2];
+  A[3][3] = (2.0 - gamma) * nVel[2] + U_k;
+  A[3][4] = kapm1 * nV[2];
+  A[4][0] = U_k * ((gamma - 2.0) * vSquHlf - gamma * (hh - pp / rrho));
+  A[4][1] = (hh + vSquHlf) * nV[0] - kapm1 * vel[0] * U_k;
+  A[4][2] = (hh + vSquHlf) * nV[1] - kapm1 * vel[1] * U_k;
+  A[4][3] = (hh + vSquHlf) * nV[2] - kapm1 * vel[2] * U_k;
+  A[4][4] = gamma * U_k;
+}
+
+// calcdSMdU_HLLC, UgpWithCvCompFlow.cpp:2246-2265 (nV the unit normal)
+static void dsm_du(double d[5], const double *vel, double pp, double rrho, double hh, const double *nV, double gamma,
+                   double SM, double S, double invertrho, double Factor) {
+  double kapm1 = (gamma - 1.0);
+  double nVel[3] = {vel[0] * nV[0], vel[1] * nV[1], vel[2] * nV[2]};
+  double U_k = nVel[0] + nVel[1] + nVel[2];
+  double vSquHlf = 0.5 * dot3(vel, vel);
+  d[0] = Factor * invertrho * (-U_k * U_k + kapm1 * vSquHlf + gamma * pp / rrho - kapm1 * hh + SM * S);
+  for (int i = 0; i < 3; i++) d[1 + i] = Factor * invertrho * ((2 * U_k - S - SM) * nV[i] - kapm1 * vel[i]);
+  d[4] = Factor * invertrho * kapm1;
+}
+
+// calcSubSonicJacobianCoupled_HLLC, UgpWithCvCompFlow.cpp:2270-2287
+static void jac_coupled_sub(double (*A)[5], const double *dSMdU, double rhoS, const double *rhouS, double rhoeS,
+                            double pS, double Omega, double sM, double rhoSmq, const double *nV, double area) {
+  double OmSM = Omega * sM;
+  double OmSMp1 = OmSM + 1.0;
+  for (int i = 0; i < 5; i++) {
+    A[0][i] = -OmSM * A[0][i] + area * OmSMp1 * dSMdU[i] * rhoS;
+    A[1][i] = -OmSM * A[1][i] + area * OmSMp1 * dSMdU[i] * (rhouS[0] + rhoSmq * nV[0]);
+    A[2][i] = -OmSM * A[2][i] + area * OmSMp1 * dSMdU[i] * (rhouS[1] + rhoSmq * nV[1]);
+    A[3][i] = -OmSM * A[3][i] + area * OmSMp1 * dSMdU[i] * (rhouS[2] + rhoSmq * nV[2]);
+    A[4][i] = -OmSM * A[4][i] + area * OmSMp1 * dSMdU[i] * (rhoeS + pS + rhoSmq * sM);
+  }
+}
+
+// calcEulerFluxMatricesCoupled_HLLC, UgpWithCvCompFlow.cpp:1934-2174, nScal = 0,
+// kine = 0. A_L or A_R may be NULL.
+void jac_hllc_coupled(double (*A_L)[5], double (*A_R)[5], double rhoL, const double *uL, double pL, double h0,
+                      double gammaL, double rhoR, const double *uR, double pR, double h1, double gammaR, double area,
+                      const double *nVec, bool only_pressure) {
+  if (only_pressure) {
+    if (A_L) {
+      double uLuL = dot3(uL, uL);
+      double hL = h0 + 0.0;
+      for (int j = 0; j < 5; j++) A_L[0][j] = 0.0;
+      double dpdrho = 0.5 * (gammaL - 1.0) * uLuL - (gammaL - 1.0) * hL + gammaL * pL / rhoL;
+      for (int r = 1; r <= 3; r++) {
+        A_L[r][0] = dpdrho * nVec[r - 1] * area;
+        A_L[r][1] = -(gammaL - 1.0) * uL[0] * nVec[r - 1] * area;
+        A_L[r][2] = -(gammaL - 1.0) * uL[1] * nVec[r - 1] * area;
+        A_L[r][3] = -(gammaL - 1.0) * uL[2] * nVec[r - 1] * area;
+        A_L[r][4] = (gammaL - 1.0) * nVec[r - 1] * area;
+      }
+      for (int j = 0; j < 5; j++) A_L[4][j] = 0.0;
+    }
+    if (A_R)
+      for (int i = 0; i < 5; i++)
+        for (int j = 0; j < 5; j++) A_R[i][j] = 0.0;
+    return;
+  }
+  double dSMdUL[5], dSMdUR[5];
+  double unL = dot3(uL, nVec);
+  double uLuL = dot3(uL, uL);
+  double cL = sqrt(gammaL * pL / rhoL);
+  double hL = h0 + 0.0;
+  double rhoeL = (hL + 0.5 * uLuL) * rhoL - pL;
+  double unR = dot3(uR, nVec);
+  double uRuR = dot3(uR, uR);
+  double cR = sqrt(gammaR * pR / rhoR);
+  double hR = h1 + 0.0;
+  double rhoeR = (hR + 0.5 * uRuR) * rhoR - pR;
+  double Rrho = sqrt(rhoR / rhoL);
+  double tmp = 1.0 / (1.0 + Rrho);
+  double velRoe[3];
+  for (int i = 0; i < 3; i++) velRoe[i] = tmp * (uL[i] + uR[i] * Rrho);
+  double uRoe = dot3(velRoe, nVec);
+  double gamPdivRho = tmp * ((gammaL * pL / rhoL + 0.5 * (gammaL - 1.0) * uLuL) +
+                             (gammaR * pR / rhoR + 0.5 * (gammaR - 1.0) * uRuR) * Rrho);
+  double cRoe = sqrt(gamPdivRho - ((gammaL + gammaR) * 0.5 - 1.0) * 0.5 * dot3(velRoe, velRoe));
+  double sL = min(uRoe - cRoe, unL - cL);
+  double sR = max(uRoe + cRoe, unR + cR);
+  double invrhotld = 1.0 / (rhoR * (sR - unR) - rhoL * (sL - unL));
+  double sM = (pL - pR - rhoL * unL * (sL - unL) + rhoR * unR * (sR - unR)) / (rhoR * (sR - unR) - rhoL * (sL - unL));
+  double pStar = rhoR * (unR - sR) * (unR - sM) + pR;
+  double nVecArea[3];
+  for (int i = 0; i < 3; i++) nVecArea[i] = nVec[i] * area;
+  auto zero = [](double (*A)[5]) {
+    for (int i = 0; i < 5; i++)
+      for (int j = 0; j < 5; j++) A[i][j] = 0.0;
+  };
+  if (sM >= 0.0) {
+    if (sL > 0.0) {
+      if (A_L) jac_coupled_super(A_L, uL, pL, rhoL, hL, nVecArea, gammaL);
+      if (A_R) zero(A_R);
+    } else {
+      double OmegaL = 1.0 / (sL - sM);
+      double sLmuL = sL - unL;
+      double rhoSL = rhoL * sLmuL * OmegaL;
+      double rhouSL[3];
+      for (int i = 0; i < 3; i++) rhouSL[i] = (rhoL * uL[i] * sLmuL + (pStar - pL) * nVec[i]) * OmegaL;
+      double rhoeSL = (sLmuL * rhoeL - pL * unL + pStar * sM) * OmegaL;
+      if (A_L) {
+        dsm_du(dSMdUL, uL, pL, rhoL, hL, nVec, gammaL, sM, sL, invrhotld, 1.0);
+        jac_coupled_super(A_L, uL, pL, rhoL, hL, nVecArea, gammaL);
+        double rhoSmqR = rhoR * (sR - unR);
+        jac_coupled_sub(A_L, dSMdUL, rhoSL, rhouSL, rhoeSL, pStar, OmegaL, sM, rhoSmqR, nVec, area);
+        for (int i = 0; i < 5; i++) A_L[i][i] += area * OmegaL * sM * sL;
+      }
+      if (A_R) {
+        dsm_du(dSMdUR, uR, pR, rhoR, hR, nVec, gammaR, sM, sR, invrhotld, -1.0);
+        zero(A_R);
+        double rhoSmqL = rhoL * (sL - unL);
+        jac_coupled_sub(A_R, dSMdUR, rhoSL, rhouSL, rhoeSL, pStar, OmegaL, sM, rhoSmqL, nVec, area);
+      }
+    }
+  } else {
+    if (sR >= 0.0) {
+      double OmegaR = 1.0 / (sR - sM);
+      double sRmuR = sR - unR;
+      double rhoSR = rhoR * sRmuR * OmegaR;
+      double rhouSR[3];
+      for (int i = 0; i < 3; i++) rhouSR[i] = (rhoR * uR[i] * sRmuR + (pStar - pR) * nVec[i]) * OmegaR;
+      double rhoeSR = (sRmuR * rhoeR - pR * unR + pStar * sM) * OmegaR;
+      if (A_R) {
+        dsm_du(dSMdUR, uR, pR, rhoR, hR, nVec, gammaR, sM, sR, invrhotld, -1.0);
+        jac_coupled_super(A_R, uR, pR, rhoR, hR, nVecArea, gammaR);
+        double rhoSmqL = rhoL * (sL - unL);
+        jac_coupled_sub(A_R, dSMdUR, rhoSR, rhouSR, rhoeSR, pStar, OmegaR, sM, rhoSmqL, nVec, area);
+        for (int i = 0; i < 5; i++) A_R[i][i] += area * OmegaR * sM * sR;
+      }
+      if (A_L) {
+        dsm_du(dSMdUL, uL, pL, rhoL, hL, nVec, gammaL, sM, sL, invrhotld, 1.0);
+        zero(A_L);
+        double rhoSmqR = rhoR * (sR - unR);
+        jac_coupled_sub(A_L, dSMdUL, rhoSR, rhouSR, rhoeSR, pStar, OmegaR, sM, rhoSmqR, nVec, area);
+      }
+    } else {
+      if (A_R) jac_coupled_super(A_R, uR, pR, rhoR, hR, nVecArea, gammaR);
+      if (A_L) zero(A_L);
+    }
+  }
+}
+
 void ludeco(double (*A)[5], int order) {
   for (int jc = 1; jc < order; jc++) A[0][jc] /= A[0][0];
   int jrjc = 0;
@@ -1116,10 +1355,10 @@ inline void add_ascal(or_ctx &c, int noc, double sgn, const double *S) {
 }
 
 // calcEulerFlux, JoeWithModels.cpp:2621-3589 (nScal = 0, no shock fix)
-void calc_euler_flux(or_ctx &c, double *A) {
-  const or_mesh &m = c.m;
-  const bool snd = c.p.second_order != 0;
-  if (snd) {
+// the gradients calcEulerFlux and calcFluxCoupled take before their face
+// loops (JoeWithModels.cpp:2682-2741, 4691-4745)
+static void euler_gradients(or_ctx &c) {
+  {
     // calcCvVectorGrad / calcCvScalarGrad (JoeWithModels.cpp:2682-2694)
     if (c.p.grad_scheme == OR_GRAD_LS) {
       grad_vector_ls(c, c.grad_u.data(), c.vel.data(), c.vel_bfa.data());
@@ -1147,6 +1386,12 @@ void calc_euler_flux(or_ctx &c, double *A) {
       if (c.p.limiter) limit_scalar(c, c.grad_phi.data(), c.phi.data(), c.phi.data(), NULL);
     }
   }
+}
+
+void calc_euler_flux(or_ctx &c, double *A) {
+  const or_mesh &m = c.m;
+  const bool snd = c.p.second_order != 0;
+  if (snd) euler_gradients(c);
   double F[5], Apl[5][5], Ami[5][5], Fs = 0.0, SL[6], SR[6];
   const bool sa = c.sa;
   for (int ifa = c.nfa_b; ifa < c.nfa; ifa++) {
@@ -1303,6 +1548,151 @@ void calc_euler_flux(or_ctx &c, double *A) {
           int noc00 = m.nbocv_i[icv0];
           for (int k = 0; k < 25; k++) A[25 * noc00 + k] += Apl[k / 5][k % 5];
           if (sa) add_ascal(c, noc00, 1.0, SL);  // :3559-3561
+        }
+      }
+    }
+  }
+}
+
+// calcFluxCoupled, JoeWithModels.cpp:4613-5825, for nScal = 0, MU_REF = 0 and
+// HLLC: the coupled HLLC flux and Jacobian, each face side's enthalpy from
+// calcThermoProp_p after reconstruction (the cell's when first order), and
+// the coupled boundary treatments: SYMMETRY and WALL by SYMM_BC (ANALYTICAL:
+// the ONLY_PRESSURE flux and Jacobian), NEUMANN with the boundary state on
+// both sides, the others reconstructed against the boundary state. The
+// effective SYMM_BC default is ORIGINAL_NUMERIC here too: init registers it
+// (UgpWithCvCompFlow.h:479) before calcFluxCoupled asks with ANALYTICAL.
+void calc_flux_coupled(or_ctx &c, double *A) {
+  const or_mesh &m = c.m;
+  const bool snd = c.p.second_order != 0;
+  if (snd) euler_gradients(c);
+  const double g = c.p.gamma, R = c.p.R_gas;
+  auto thermo_h = [&](double rho, double p) {  // calcThermoProp_p, UgpWithCvCompFlow.h:1765-1771
+    double T = p / (rho * R);
+    return g * R / (g - 1.0) * T;
+  };
+  double F[5], Apl[5][5], Ami[5][5];
+  for (int ifa = c.nfa_b; ifa < c.nfa; ifa++) {  // :4797-5105
+    int icv0 = m.cvofa[2 * ifa], icv1 = m.cvofa[2 * ifa + 1];
+    int noc00 = 0, noc01 = 0, noc11 = 0, noc10 = 0;
+    if (A) {
+      noc00 = m.nbocv_i[icv0];
+      noc01 = noc00;
+      while (m.nbocv_v[++noc01] != icv1) {}
+      noc11 = m.nbocv_i[icv1];
+      noc10 = noc11;
+      while (m.nbocv_v[++noc10] != icv0) {}
+    }
+    double nVec[3];
+    double area = norm3(nVec, &m.fa_normal[3 * ifa]);
+    double rho0 = c.rho[icv0], p0 = c.press[icv0], h0 = c.enthalpy[icv0];
+    double u0[3] = {c.vel[3 * icv0], c.vel[3 * icv0 + 1], c.vel[3 * icv0 + 2]};
+    double rho1 = c.rho[icv1], p1 = c.press[icv1], h1 = c.enthalpy[icv1];
+    double u1[3] = {c.vel[3 * icv1], c.vel[3 * icv1 + 1], c.vel[3 * icv1 + 2]};
+    if (snd) {
+      double r0[3], r1[3];
+      for (int i = 0; i < 3; i++) {
+        r0[i] = m.x_fa[3 * ifa + i] - m.x_cv[3 * icv0 + i];
+        r1[i] = m.x_fa[3 * ifa + i] - m.x_cv[3 * icv1 + i];
+      }
+      rho0 += dot3(r0, &c.grad_rho[3 * icv0]);
+      p0 += dot3(r0, &c.grad_p[3 * icv0]);
+      if ((p0 <= 0.0) || (rho0 <= 0.0)) {
+        p0 = c.press[icv0];
+        rho0 = c.rho[icv0];
+      } else {
+        for (int i = 0; i < 3; i++) u0[i] += dot3(r0, &c.grad_u[9 * icv0 + 3 * i]);
+      }
+      rho1 += dot3(r1, &c.grad_rho[3 * icv1]);
+      p1 += dot3(r1, &c.grad_p[3 * icv1]);
+      if ((p1 <= 0.0) || (rho1 <= 0.0)) {
+        p1 = c.press[icv1];
+        rho1 = c.rho[icv1];
+      } else {
+        for (int i = 0; i < 3; i++) u1[i] += dot3(r1, &c.grad_u[9 * icv1 + 3 * i]);
+      }
+      h0 = thermo_h(rho0, p0);
+      h1 = thermo_h(rho1, p1);
+    }
+    flux_hllc_coupled(F, rho0, u0, p0, h0, g, rho1, u1, p1, h1, g, area, nVec, false);
+    c.rhs_rho[icv0] -= F[0];
+    for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= F[1 + i];
+    c.rhs_rhoE[icv0] -= F[4];
+    c.rhs_rho[icv1] += F[0];
+    for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv1 + i] += F[1 + i];
+    c.rhs_rhoE[icv1] += F[4];
+    if (A) {  // cell-centre states (:5035-5038)
+      jac_hllc_coupled(Apl, Ami, c.rho[icv0], &c.vel[3 * icv0], c.press[icv0], c.enthalpy[icv0], g, c.rho[icv1],
+                       &c.vel[3 * icv1], c.press[icv1], c.enthalpy[icv1], g, area, nVec, false);
+      for (int i = 0; i < 5; i++)
+        for (int j = 0; j < 5; j++) {
+          A[25 * noc00 + 5 * i + j] += Apl[i][j];
+          A[25 * noc01 + 5 * i + j] += Ami[i][j];
+        }
+      for (int i = 0; i < 5; i++)
+        for (int j = 0; j < 5; j++) {
+          A[25 * noc11 + 5 * i + j] -= Ami[i][j];
+          A[25 * noc10 + 5 * i + j] -= Apl[i][j];
+        }
+    }
+  }
+  for (int iz = 0; iz < m.nzone; iz++) {  // :5113-5796
+    const int bc = m.zone_bc[iz];
+    for (int ifa = m.zone_ifa[2 * iz]; ifa <= m.zone_ifa[2 * iz + 1]; ifa++) {
+      int icv0 = m.cvofa[2 * ifa];
+      int noc00 = m.nbocv_i[icv0];
+      double nVec[3];
+      double area = norm3(nVec, &m.fa_normal[3 * ifa]);
+      const double *ub = &c.vel_bfa[3 * ifa];
+      const double rb = c.rho_bfa[ifa], pb = c.p_bfa[ifa], hb = c.h_bfa[ifa];
+      if (bc == OR_BC_SYMMETRY || bc == OR_BC_WALL) {
+        const bool op = c.p.symm_bc == OR_SYMM_ANALYTICAL;
+        flux_hllc_coupled(F, rb, ub, pb, hb, g, rb, ub, pb, hb, g, area, nVec, op);
+        c.rhs_rho[icv0] -= F[0];
+        for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= F[1 + i];
+        c.rhs_rhoE[icv0] -= F[4];
+        if (A) {
+          const bool nn = c.p.symm_bc == OR_SYMM_NEW_NUMERIC;
+          jac_hllc_coupled(Apl, nn ? Ami : NULL, rb, ub, pb, hb, g, rb, ub, pb, hb, g, area, nVec, op);
+          for (int k = 0; k < 25; k++)
+            if (nn) A[25 * noc00 + k] += Apl[k / 5][k % 5] + Ami[k / 5][k % 5];
+            else A[25 * noc00 + k] += Apl[k / 5][k % 5];
+        }
+      } else if (bc == OR_BC_NEUMANN) {
+        flux_hllc_coupled(F, rb, ub, pb, hb, g, rb, ub, pb, hb, g, area, nVec, false);
+        c.rhs_rho[icv0] -= F[0];
+        for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= F[1 + i];
+        c.rhs_rhoE[icv0] -= F[4];
+        if (A) {
+          jac_hllc_coupled(Apl, Ami, rb, ub, pb, hb, g, rb, ub, pb, hb, g, area, nVec, false);
+          for (int k = 0; k < 25; k++)
+            if (c.p.symm_bc == OR_SYMM_NEW_NUMERIC) A[25 * noc00 + k] += Apl[k / 5][k % 5] + Ami[k / 5][k % 5];
+            else A[25 * noc00 + k] += Apl[k / 5][k % 5];
+        }
+      } else {
+        double rho0 = c.rho[icv0], p0 = c.press[icv0], h0 = c.enthalpy[icv0];
+        double u0[3] = {c.vel[3 * icv0], c.vel[3 * icv0 + 1], c.vel[3 * icv0 + 2]};
+        if (snd) {
+          double r0[3];
+          for (int i = 0; i < 3; i++) r0[i] = m.x_fa[3 * ifa + i] - m.x_cv[3 * icv0 + i];
+          rho0 += dot3(r0, &c.grad_rho[3 * icv0]);
+          p0 += dot3(r0, &c.grad_p[3 * icv0]);
+          if ((p0 <= 0.0) || (rho0 <= 0.0)) {
+            p0 = c.press[icv0];
+            rho0 = c.rho[icv0];
+          } else {
+            for (int i = 0; i < 3; i++) u0[i] += dot3(r0, &c.grad_u[9 * icv0 + 3 * i]);
+          }
+          h0 = thermo_h(rho0, p0);
+        }
+        flux_hllc_coupled(F, rho0, u0, p0, h0, g, rb, ub, pb, hb, g, area, nVec, false);
+        c.rhs_rho[icv0] -= F[0];
+        for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= F[1 + i];
+        c.rhs_rhoE[icv0] -= F[4];
+        if (A) {  // cell-centre state against the boundary state (:5731-5735)
+          jac_hllc_coupled(Apl, NULL, c.rho[icv0], &c.vel[3 * icv0], c.press[icv0], c.enthalpy[icv0], g, rb, ub, pb,
+                           hb, g, area, nVec, false);
+          for (int k = 0; k < 25; k++) A[25 * noc00 + k] += Apl[k / 5][k % 5];
         }
       }
     }
@@ -1825,8 +2215,13 @@ void calc_rhs(or_ctx &c, double *A) {
     c.rhs_rhoE[icv] = 0.0;
   }
   if (c.sa) std::fill(c.rhs_scal.begin(), c.rhs_scal.end(), 0.0);
-  calc_euler_flux(c, A);
-  if (c.p.mu_ref > 0.0) calc_viscous_flux(c, A);
+  if (c.p.coupled) {  // calcRhsCoupled, JoeWithModels.cpp:4596-4611
+    calc_flux_coupled(c, A);
+  } else {
+    calc_euler_flux(c, A);
+    if (c.p.mu_ref > 0.0) calc_viscous_flux(c, A);
+  }
+  // sourceHook / sourceHookCoupled: gravity (JoeWithModels.h:111-152)
   const double *g = c.p.gravity;
   for (int icv = 0; icv < c.ncv; icv++) {
     double vol = c.m.cv_volume[icv];

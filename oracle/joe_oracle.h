@@ -75,6 +75,10 @@ typedef struct {
   double sa_init, relax_scal;
   /* TIME_STEP: 0 MAX_BASED (default), 1 SUM_BASED */
   int time_step_sum;
+  /* TIME_INTEGRATION = BACKWARD_EULER_COUPLED: calcRhsCoupled's fluxes and
+   * Jacobian (nScal = 0, MU_REF = 0, HLLC); the loop is runBackwardEuler's
+   * with the block-Jacobi BiCGStab in place of PETSc GMRES */
+  int coupled;
 } or_params;
 
 typedef struct or_ctx or_ctx;

@@ -55,7 +55,7 @@ class _Params(C.Structure):
                 ("visc_mode", C.c_int), ("mu_power_law", C.c_double), ("sl_tref", C.c_double),
                 ("sl_sref", C.c_double), ("T_ref", C.c_double), ("pr", C.c_double), ("pr_turb", C.c_double),
                 ("visc_limiter", C.c_int), ("sa", C.c_int), ("sa_init", C.c_double), ("relax_scal", C.c_double),
-                ("time_step_sum", C.c_int)]
+                ("time_step_sum", C.c_int), ("coupled", C.c_int)]
 
 
 def lib():
@@ -142,6 +142,13 @@ def make_params(joe_in: str) -> tuple[_Params, dict]:
     p.grad_scheme = {"GREENGAUSS": 0, "LEASTSQUARES": 1}[sch]
     p.limiter = 1 if _after(gr, "LIMITER") in ("YES", "BARTH_JESPERSEN_MOD") else 0
     p.eps_limiter = float(_after(gr, "EPS"))
+    ti = get("TIME_INTEGRATION", "BACKWARD_EULER")
+    p.coupled = 1 if ti == "BACKWARD_EULER_COUPLED" else 0
+    if p.coupled and (p.mu_ref > 0.0 or get("SPACE_INTEGRATION", "HLLC") != "HLLC"):
+        raise ValueError("the coupled restatement covers inviscid HLLC only")
+    # calcFluxCoupled's own default is ANALYTICAL (JoeWithModels.cpp:4615), but
+    # init has already registered ORIGINAL_NUMERIC by reading the key with that
+    # default while printing it (UgpWithCvCompFlow.h:479; Param.cpp:139-149)
     p.symm_bc = {"ORIGINAL_NUMERIC": 0, "NEW_NUMERIC": 1, "ANALYTICAL": 2}[get("SYMM_BC", "ORIGINAL_NUMERIC")]
     if "CFL" in P:
         p.cfl, p.time_step_min = float(get("CFL")), 0

@@ -12,6 +12,9 @@
 //   restart-write <Joe.in> <state.bin> <restart>  Ugp::writeRestart of a given state
 //   restart-read  <Joe.in> <out.bin>      mesh arrays and RHO, RHOU, RHOE (NUSA)
 //                                         after initializeFromRestartFile(RESTART)
+//   rhs-coupled <Joe.in> <state.bin> <out.bin>  one calcRhsCoupled (the
+//                                         residual and (5+nScal)^2 blocks of
+//                                         runBackwardEulerCoupled), no scalars
 // rhs-sa / solve-sa run the same with the Spalart-Allmaras mix-in (RansTurbSA,
 // as joe.cpp's MyJoeSA); the state file may then carry NUSA.
 //
@@ -20,6 +23,7 @@
 //   int64 count; payload.
 #include "JoeWithModels.h"
 #include "TurbModel_SA.h"
+#include "myMem.h"
 
 #include <chrono>
 #include <cstdio>
@@ -258,6 +262,61 @@ class RefJoe : public JoeWithModels {
     if (scalarTranspEqVector.size() > 0) d.f64("NUSA", scalarTranspEqVector[0].phi, ncv);
   }
 
+  // One coupled residual evaluation in the order runBackwardEulerCoupled uses
+  // (JoeWithModels.cpp:4380-4429): state, BC, local time step, then
+  // calcRhsCoupled into getMem2D / getMem3D arrays as that loop allocates them.
+  void rhsCoupledOnce(const Load &st, Dump &d) {
+    initializeFromRestartFile(getStringParam("RESTART"));
+    initialHook();
+    initialHookScalarRansTurbModel();
+    initialHookScalarRansCombModel();
+    const double *r = st.f64("RHO", ncv);
+    const double *ru = st.f64("RHOU", 3LL * ncv);
+    const double *re = st.f64("RHOE", ncv);
+    for (int icv = 0; icv < ncv; icv++) {
+      rho[icv] = r[icv];
+      for (int i = 0; i < 3; i++) rhou[icv][i] = ru[3 * icv + i];
+      rhoE[icv] = re[icv];
+    }
+    const int nScal = scalarTranspEqVector.size();
+    const int b = 5 + nScal;
+    calcStateVariables();
+    calcMaterialProperties();
+    setBC();
+    calcRansTurbViscMuet();
+    double dtmin = calcDt(cfl);
+    double ***A;
+    getMem3D(&A, 0, nbocv_s - 1, 0, b - 1, 0, b - 1, "harness A", true);
+    double **rhs;
+    getMem2D(&rhs, 0, ncv - 1, 0, b - 1, "harness rhs", true);
+    for (int noc = 0; noc < nbocv_s; noc++)
+      for (int i = 0; i < b; i++)
+        for (int j = 0; j < b; j++) A[noc][i][j] = 0.0;
+    calcRhsCoupled(rhs, A, nScal, true);
+    std::vector<double> v(ncv), vu(3LL * ncv), ab((size_t)b * b * nbocv_s);
+    for (int icv = 0; icv < ncv; icv++) v[icv] = rhs[icv][0];
+    d.f64("RHS_RHO", v.data(), ncv);
+    for (int icv = 0; icv < ncv; icv++)
+      for (int i = 0; i < 3; i++) vu[3 * icv + i] = rhs[icv][1 + i];
+    d.f64("RHS_RHOU", vu.data(), 3LL * ncv);
+    for (int icv = 0; icv < ncv; icv++) v[icv] = rhs[icv][4];
+    d.f64("RHS_RHOE", v.data(), ncv);
+    for (int noc = 0; noc < nbocv_s; noc++)
+      for (int i = 0; i < b; i++)
+        for (int j = 0; j < b; j++) ab[((size_t)noc * b + i) * b + j] = A[noc][i][j];
+    d.f64("A", ab.data(), (long long)ab.size());
+    dumpPrims(d);
+    d.f64("LOCAL_DT", local_dt, ncv);
+    d.f64("DT_MIN", &dtmin, 1);
+    if (sndOrder) {
+      d.f64("GRAD_RHO", &grad_rho[0][0], 3LL * ncv);
+      d.f64("GRAD_P", &grad_p[0][0], 3LL * ncv);
+      d.f64("GRAD_U", &grad_u[0][0][0], 9LL * ncv);
+    }
+    freeMem3D(A, 0, nbocv_s - 1, 0, b - 1, 0, b - 1);
+    freeMem2D(rhs, 0, ncv - 1, 0, b - 1);
+  }
+
   // One implicit residual evaluation in the order runBackwardEuler uses
   // (JoeWithModels.cpp:943-1005).
   void rhsOnce(const Load &st, Dump &d) {
@@ -426,6 +485,12 @@ int main(int argc, char **argv) {
       Load st(argv[3]);
       Dump d(argv[4]);
       joe.rhsOnce(st, d);
+      joe.dumpMesh(d);
+    } else if (mode == "rhs-coupled") {
+      if (argc < 5) return 2;
+      Load st(argv[3]);
+      Dump d(argv[4]);
+      joe.rhsCoupledOnce(st, d);
       joe.dumpMesh(d);
     } else if (mode == "restart-write") {
       if (argc < 5) return 2;

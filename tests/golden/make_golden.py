@@ -11,6 +11,8 @@ Fixtures (numpy .npz, inputs and the reference's outputs only):
                        limited gradients, residual, Jacobian and one BiCGStab dq
   solve_<name>.npz     JoeWithModels::run() state snapshots (steps 1..5) and
                        the final state after NSTEPS
+  coupled_<name>_hllc  random state and calcRhsCoupled's residual, Jacobian
+                       blocks, primitive and boundary states, local dt
 """
 from __future__ import annotations
 
@@ -75,6 +77,29 @@ RHS_CASES = [("bump24", "HLLC", {}), ("bump24", "ROE", {}), ("bump24", "AUSM", {
              ("bump24", "HLLC", {"prof": True}),
              ("box4", "ROE", {"prof": True}),
              ("bump24", "HLLC", {"visc": "pl", "sa": True, "prof": True})]
+
+
+# runBackwardEulerCoupled's residual: SYMM_BC absent (the key the reference's
+# init registers) on both geometries, ANALYTICAL (the ONLY_PRESSURE forms), the
+# other two modes with NEUMANN and the subsonic inlet/outlet, Green-Gauss,
+# first order
+COUPLED_CASES = [("bump24", {}), ("box4", {}), ("box4", {"symm_bc": "ANALYTICAL"}),
+                 ("box4", {"bcs": cases.BOX_BCS_ALT, "symm_bc": "NEW_NUMERIC"}),
+                 ("box4", {"bcs": cases.BOX_BCS_ALT, "symm_bc": "ORIGINAL_NUMERIC", "grad": "GREENGAUSS"}),
+                 ("box4", {"second_order": False})]
+
+
+def coupled_tag(kw):
+    t = ""
+    if "bcs" in kw:
+        t += "_alt"
+    if "symm_bc" in kw:
+        t += "_" + {"NEW_NUMERIC": "new", "ANALYTICAL": "ana", "ORIGINAL_NUMERIC": "orig"}[kw["symm_bc"]]
+    if kw.get("grad") == "GREENGAUSS":
+        t += "_gg"
+    if kw.get("second_order") is False:
+        t += "_first"
+    return t
 
 
 def profile_table(mm, nscal):
@@ -177,6 +202,29 @@ def main(only=None):
                 out["PROFILE"] = tab
             np.savez_compressed(os.path.join(HERE, fixture + ".npz"), **out)
             print("wrote", name, scheme, kw)
+
+        # the coupled residual (calcRhsCoupled of runBackwardEulerCoupled,
+        # nScal = 0, inviscid HLLC): residual and Jacobian blocks
+        for name, kw in COUPLED_CASES:
+            kw = dict(kw)
+            symm = kw.get("symm_bc")
+            fixture = f"coupled_{name}_hllc" + coupled_tag(kw)
+            if only and only not in fixture:
+                continue
+            mm, joe = MESHES[name]("HLLC", **kw)
+            joe = cases.coupled(joe, keep_symm_bc=symm is not None)
+            fn = "box.msh" if name.startswith("box") else "bump.msh"
+            meshgen.write_msh(mm, os.path.join(wd, fn))
+            with open(os.path.join(wd, "Joe.in"), "w") as f:
+                f.write(joe)
+            rho, rhou, rhoE = cases.random_state(mm.ncv, seed=7)
+            jmdump.write(os.path.join(wd, "state.bin"), {"RHO": rho, "RHOU": rhou.reshape(-1), "RHOE": rhoE})
+            run_ref(["rhs-coupled", "Joe.in", "state.bin", "out.bin"], wd)
+            d = jmdump.read(os.path.join(wd, "out.bin"))
+            out = {k: d[k] for k in RHS_KEYS if k in d}
+            out.update(RHO=rho, RHOU=rhou.reshape(-1), RHOE=rhoE, JOE_IN=np.frombuffer(joe.encode(), np.uint8))
+            np.savez_compressed(os.path.join(HERE, fixture + ".npz"), **out)
+            print("wrote", fixture)
 
         # restart files: the reference writes one for a given state
         # (Ugp::writeRestart) and reads it back (initializeFromRestartFile),
