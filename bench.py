@@ -7,8 +7,8 @@ tetrahedra each, interior nodes jittered by U(-0.2h, 0.2h); random FP64 state
 residual evaluation of the reference's pseudo-time loop -- calcStateVariables,
 setBC, calcDt and calcRhs (2nd order, least-squares gradients, smooth
 Barth-Jespersen limiter) -- over the whole mesh, with the state resident in
-HBM. value = ncv * steps / time in Mcell-it/s for HLLC; Roe and AUSM are
-reported beside it.
+HBM. value = ncv * steps / time in Mcell-it/s for HLLC; Roe, AUSM and the
+coupled loop's HLLC residual (calcRhsCoupled) are reported beside it.
 
     python bench.py [--steps K] [--warmup W] [--n 160] [--gpus 1]
 """
@@ -277,7 +277,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=160)
-    ap.add_argument("--schemes", default="HLLC,ROE,AUSM")
+    ap.add_argument("--schemes", default="HLLC,ROE,AUSM,HLLC_COUPLED")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--grad-kernel", default=None, help="GPU_GRAD_KERNEL: ENTRY or CELL")
     ap.add_argument("--flux-kernel", default=None, help="GPU_FLUX_KERNEL: ENTRY or CELL")
@@ -330,7 +330,10 @@ def main():
     results = {}
     kernels = {}
     for scheme in args.schemes.split(","):
-        joe = cases.box_joe_in(scheme)
+        if scheme == "HLLC_COUPLED":  # calcRhsCoupled's residual (runBackwardEulerCoupled)
+            joe = cases.coupled(cases.box_joe_in("HLLC"), keep_symm_bc=True)
+        else:
+            joe = cases.box_joe_in(scheme)
         if args.cell_order:
             joe += f"GPU_CELL_ORDER = {args.cell_order}\n"
         if args.grad_kernel:

@@ -586,7 +586,9 @@ __device__ __forceinline__ void flux_hllc_coupled(double F[5], double rhoL, cons
   const bool star = left ? !(sL > 0.0) : (sR >= 0.0);
   const double rho = left ? rhoL : rhoR, p = left ? pL : pR, un = left ? unL : unR, rhoe = left ? rhoeL : rhoeR;
   const double S = left ? sL : sR;
-  const double *u = left ? uL : uR;
+  // values, not a pointer, are selected: a pointer chosen between the two
+  // sides' arrays would put both in scratch memory
+  const double u[3] = {left ? uL[0] : uR[0], left ? uL[1] : uR[1], left ? uL[2] : uR[2]};
   if (!star) {
     F[0] = rho * un;
 #pragma unroll
@@ -675,8 +677,8 @@ __device__ __noinline__ void jac_hllc_coupled(double J[5][5], int which, double 
   const bool left = sM >= 0.0;  // upwind side
   const bool sup = left ? (sL > 0.0) : !(sR >= 0.0);
   const bool own = (which == 0) == left;  // this side is the upwind one
-  // this side's state
-  const double *u = which == 0 ? uL : uR;
+  // this side's state (values selected, not a pointer: see flux_hllc_coupled)
+  const double u[3] = {which == 0 ? uL[0] : uR[0], which == 0 ? uL[1] : uR[1], which == 0 ? uL[2] : uR[2]};
   const double p = which == 0 ? pL : pR, rho = which == 0 ? rhoL : rhoR, h = which == 0 ? hL : hR;
   if (own) {
     jac_coupled_super(J, u, p, rho, h, nVecArea, gam);
@@ -688,7 +690,7 @@ __device__ __noinline__ void jac_hllc_coupled(double J[5][5], int which, double 
   }
   if (sup) return;
   // the upwind side's star state
-  const double *uu = left ? uL : uR;
+  const double uu[3] = {left ? uL[0] : uR[0], left ? uL[1] : uR[1], left ? uL[2] : uR[2]};
   const double Su = left ? sL : sR, unu = left ? unL : unR, rhou_ = left ? rhoL : rhoR, pu = left ? pL : pR;
   const double rhoeu = left ? rhoeL : rhoeR;
   double Omega = 1.0 / (Su - sM);
