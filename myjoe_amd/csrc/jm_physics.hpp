@@ -231,7 +231,7 @@ __device__ __forceinline__ void jac_a(double A[5][5], const double *vel, double 
 // which = 0 computes A_L (d F / d U_L), which = 1 computes A_R. The reference
 // computes the two independently (calcSubSonicJacobeanHLLC uses only the
 // matching dSMdU/dpsdU), so evaluating one at a time gives the same bits.
-__device__ __noinline__ void jac_hllc(double J[5][5], int which, double rhoL, const double *uL, double pL, double rhoR,
+__device__ __forceinline__ void jac_hllc(double J[5][5], int which, double rhoL, const double *uL, double pL, double rhoR,
                                       const double *uR, double pR, double gam, double area, const double *nVec) {
   const double gammaL = gam, gammaR = gam, kL = 0.0, kR = 0.0;
   double unL = dot3(uL, nVec);
@@ -393,7 +393,7 @@ __device__ __noinline__ void jac_hllc(double J[5][5], int which, double rhoL, co
 // times the upwind scalar, less the upwind star mass flux per unit density
 // ("FrhostarOverrho") in column 0, which is also d/d(rho phi); the
 // downwind side has no d/d(rho phi), and a side outside the fan is zero.
-__device__ __noinline__ void jac_hllc_scal(double S[6], int which, double rhoL, const double *uL, double pL,
+__device__ __forceinline__ void jac_hllc_scal(double S[6], int which, double rhoL, const double *uL, double pL,
                                            double rhoR, const double *uR, double pR, double gam, double area,
                                            const double *nVec, double phiL, double phiR) {
   double J[5][5];
@@ -454,7 +454,7 @@ __device__ __noinline__ void jac_hllc_scal(double S[6], int which, double rhoL, 
 
 // One side of the Roe Jacobian pair, UgpWithCvCompFlow.cpp:3382-3559
 // (A_L = A(U_L)/2 + P|L|P^-1 area/2, A_R = A(U_R)/2 - P|L|P^-1 area/2).
-__device__ __noinline__ void jac_roe(double J[5][5], int which, double rhoL, const double *uL, double pL, double rhoR,
+__device__ __forceinline__ void jac_roe(double J[5][5], int which, double rhoL, const double *uL, double pL, double rhoR,
                                      const double *uR, double pR, double gam, double area, const double *nVec) {
   const double gammaL = gam, gammaR = gam;
   double uLuL = dot3(uL, uL);
@@ -643,7 +643,7 @@ __device__ __forceinline__ void jac_coupled_super(double A[5][5], const double *
 // kine = 0. In every subsonic case the side's own dSM/dU is used with the
 // other side's rho (S - u_n), the star state is the upwind side's, and the
 // supersonic part and diagonal term belong to the upwind side only.
-__device__ __noinline__ void jac_hllc_coupled(double J[5][5], int which, double rhoL, const double *uL, double pL,
+__device__ __forceinline__ void jac_hllc_coupled(double J[5][5], int which, double rhoL, const double *uL, double pL,
                                               double h0, double rhoR, const double *uR, double pR, double h1,
                                               double gam, double area, const double *nVec) {
   const double gammaL = gam, gammaR = gam;

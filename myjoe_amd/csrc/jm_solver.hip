@@ -1022,7 +1022,7 @@ struct VFace {
   double keff, alpha;
 };
 
-__device__ __noinline__ void visc_face(const DParams &P, const VSide &s0, const VSide &s1, double mul, double mut,
+__device__ __forceinline__ void visc_face(const DParams &P, const VSide &s0, const VSide &s1, double mul, double mut,
                                        const double *u_fa, double area, const double *nVec, double smag,
                                        const double *sVec, double Frhou[3], double &FrhoE, VFace &vf) {
   const double nonL = 0.0;
@@ -1060,7 +1060,7 @@ __device__ __noinline__ void visc_face(const DParams &P, const VSide &s0, const 
 
 // one half of the viscous Jacobian (A0: sgn -1 with side 0's state, A1: +1
 // with side 1's), dFdU x dUdQ (:1662-1762, :712-745); rows 1-4 only
-__device__ __noinline__ void visc_jac(const DParams &P, double J[5][5], double sgn, const VSide &s, double mul,
+__device__ __forceinline__ void visc_jac(const DParams &P, double J[5][5], double sgn, const VSide &s, double mul,
                                       double mut, const VFace &vf, const double *u_fa, double area, double smag) {
   const double kine = 0.0;
   const double gam = P.gamma, R = P.R;
