@@ -1375,14 +1375,13 @@ __device__ __forceinline__ void d_jac(const Blk &B, DMesh m, DParams P, const do
         const VSide vb = vside_bfa(bb, gradv + 12 * (size_t)c);
         double sV[3] = {xfa[0] - pc[5], xfa[1] - pc[6], xfa[2] - pc[7]};
         double smag;
-        const double *sArg;
+        double sArg[3];  // a copy, not a pointer to n or sV, keeps both in registers
         if (z.bc == kBcWall) {
           smag = fabs(dot3(sV, n));
-          sArg = n;
+          for (int i = 0; i < 3; i++) sArg[i] = n[i];
         } else {
           smag = sqrt(dot3(sV, sV));
-          for (int i = 0; i < 3; i++) sV[i] /= smag;
-          sArg = sV;
+          for (int i = 0; i < 3; i++) sArg[i] = sV[i] / smag;
         }
         const double mut = (P.sa && z.bc != kBcWall) ? sa_mut(pc[0] * P.sa_phi[c], P.sa_aux[kAuxN * (size_t)c + kAuxMue])
                                                       : 0.0;
