@@ -41,6 +41,10 @@ def algorithmic_bytes(ncv, nfa, nfa_b, nnz):
         "k_grad": ncv * (64 + 4 + 4 + 128) + nnz * 24 + nfoc * 8 + nfa_i * 56 + nfa_b * (40 + 24),
         # fluxes + local time step: + rhou (24 B) and dt (8 B) per cell
         "k_flux": ncv * (64 + 120 + 4 + 8 + 40 + 24 + 8) + nfoc * 8 + nfa * 56 + nfa_b * 40,
+        # Jacobian assembly, FP64 blocks: A written once, per cell its prim
+        # row, entry range, diagonal index; per entry the face entry and the
+        # block index; face rows; boundary rows
+        "k_jac": nnz * 25 * 8 + ncv * (64 + 4 + 8) + nfoc * 12 + nfa * 32 + nfa_b * 40,
     }
 
 
@@ -352,6 +356,16 @@ def main():
         dt = time.perf_counter() - t0
         results[scheme] = {"ms_per_step": dt / args.steps * 1e3, "mcell_it_s": mesh.ncv * args.steps / dt / 1e6}
         kernels[scheme] = s.time_kernels(reps=10)
+        if scheme == args.schemes.split(",")[0]:
+            # the Jacobian assembly of an implicit step (FP64 blocks): the
+            # residual with A, less the residual alone, wall clock over 5 calls
+            s.calc_rhs(True)
+            s.sync()
+            t0 = time.perf_counter()
+            for _ in range(5):
+                s.calc_rhs(True)
+            s.sync()
+            jac_ms = (time.perf_counter() - t0) / 5 * 1e3 - results[scheme]["ms_per_step"]
         print(f"# {scheme}: {results[scheme]['ms_per_step']:.3f} ms/residual, "
               f"{results[scheme]['mcell_it_s']:.1f} Mcell-it/s; kernels "
               + " ".join(f"{k}={v:.3f}ms" for k, v in kernels[scheme].items()), file=sys.stderr, flush=True)
@@ -388,6 +402,8 @@ def main():
             "ncv": mesh.ncv, "nfa": mesh.nfa, "nfa_b": mesh.nfa_b, "nnz": mesh.nnz,
             "per_scheme_mcell_it_s": {k: round(v["mcell_it_s"], 2) for k, v in results.items()},
             "kernel_ms": {k: round(v, 4) for k, v in kt.items()},
+            "jacobian_fp64_ms": round(jac_ms, 3),
+            "jacobian_algorithmic_GBs": round(ab["k_jac"] / (jac_ms * 1e-3) / 1e9, 1),
             "residual_algorithmic_GBs": round(resid_bytes / (kt["residual"] * 1e-3) / 1e9, 1),
             "residual_frac_of_hbm_peak_scope_table_bytes": round(
                 survey_bytes / (results[head]["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
