@@ -1150,6 +1150,90 @@ __device__ __forceinline__ void visc_internal_geom(const DParams &P, const doubl
   mul = mu_lam(P, (w1 * T0 + w0 * T1) / (w0 + w1));
 }
 
+// one of two sides, selected field by field (a conditional between two
+// local structs bound to a reference would put both in scratch memory)
+__device__ __forceinline__ VSide vsel(bool c, const VSide &x, const VSide &y) {
+  VSide r;
+  r.rho = c ? x.rho : y.rho;
+  for (int i = 0; i < 3; i++) r.u[i] = c ? x.u[i] : y.u[i];
+  r.h = c ? x.h : y.h;
+  r.T = c ? x.T : y.T;
+  r.gu = c ? x.gu : y.gu;
+  r.gh = c ? x.gh : y.gh;
+  return r;
+}
+
+// The coupled loop's viscous flux, calcViscousFluxCoupled
+// (UgpWithCvCompFlow.cpp:3040-3124), nScal = 0, no turbulence model: F[5]
+// (F[0] = 0), and tau_ij n_j and keff for the blocks below.
+__device__ __forceinline__ void visc_coupled(const DParams &P, const VSide &s0, const VSide &s1, double mul,
+                                             double mut, const double *u_fa, double area, const double *nVec,
+                                             double smag, const double *sVec, double alpha, double F[5],
+                                             double tau[3], double &keff) {
+  double g[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) g[i][j] = 0.5 * (s0.gu[3 * i + j] + s1.gu[3 * i + j]);
+  double fCorr[3] = {nVec[0] - alpha * sVec[0], nVec[1] - alpha * sVec[1], nVec[2] - alpha * sVec[2]};
+  const double mu = mul + mut;
+  F[0] = 0.0;
+  for (int i = 0; i < 3; i++)
+    tau[i] = mu * (alpha * (s1.u[i] - s0.u[i]) / smag + g[i][0] * fCorr[0] + g[0][i] * nVec[0] + g[i][1] * fCorr[1] +
+                   g[1][i] * nVec[1] + g[i][2] * fCorr[2] + g[2][i] * nVec[2]);
+  double tmp = 2.0 / 3.0 * mu * (g[0][0] + g[1][1] + g[2][2]);
+  for (int i = 0; i < 3; i++) tau[i] -= tmp * nVec[i];
+  for (int i = 0; i < 3; i++) F[1 + i] = -area * tau[i];
+  keff = mul / P.pr + mut / P.pr_turb;
+  double enth = keff * (alpha * (s1.h - s0.h) / smag +
+                        0.5 * ((s0.gh[0] + s1.gh[0]) * fCorr[0] + (s0.gh[1] + s1.gh[1]) * fCorr[1] +
+                               (s0.gh[2] + s1.gh[2]) * fCorr[2]));
+  double psi = 0.0;
+  F[4] = -area * (enth + tau[0] * u_fa[0] + tau[1] * u_fa[1] + tau[2] * u_fa[2] + psi);
+}
+
+// its blocks dF/dU dU/dQ (:3140-3268): which 0 is A0 (side 0's state, + sign),
+// 1 is A1 (side 1's, - sign); entries the reference does not write are zero
+__device__ __forceinline__ void visc_coupled_jac(const DParams &P, double J[5][5], int which, const VSide &s,
+                                                 double mu, double keff, const double tau[3], const double *u_fa,
+                                                 double area, double alpha, double smag) {
+  for (int i = 0; i < 5; i++)
+    for (int j = 0; j < 5; j++) J[i][j] = 0.0;
+  double aas = area * alpha / smag;
+  double d11 = which == 0 ? mu * aas : -mu * aas;
+  double d44 = which == 0 ? keff * aas : -keff * aas;
+  double d4[3];
+  for (int i = 0; i < 3; i++) d4[i] = -area * tau[i] * 0.5 + u_fa[i] * d11;
+  double invRho = 1.0 / s.rho;
+  double q0[3], q4[3];
+  for (int i = 0; i < 3; i++) q0[i] = -invRho * s.u[i];
+  double q40 = invRho * P.gamma * (-s.h - 0.0 + P.R * s.T + 0.5 * dot3(s.u, s.u));
+  for (int i = 0; i < 3; i++) q4[i] = -invRho * s.u[i] * P.gamma;
+  double q44 = invRho * P.gamma;
+  for (int i = 0; i < 3; i++) {
+    J[1 + i][0] = d11 * q0[i];
+    J[1 + i][1 + i] = d11 * invRho;
+  }
+  J[4][0] = d4[0] * q0[0] + d4[1] * q0[1] + d4[2] * q0[2] + d44 * q40;
+  for (int i = 0; i < 3; i++) J[4][1 + i] = d4[i] * invRho + d44 * q4[i];
+  J[4][4] = d44 * q44;
+}
+
+// the coupled loop's boundary-face viscous geometry (JoeWithModels.cpp
+// :5373-5404, 5548-5575, 5744-5775): along the normal with alpha = 1 on
+// NEUMANN and WALL faces, the normalised cell-to-face vector otherwise
+__device__ __forceinline__ void visc_coupled_bgeom(int bc, const double *xfa, const double *xc, const double *n,
+                                                   double sArg[3], double &smag, double &alpha) {
+  double sV[3] = {xfa[0] - xc[0], xfa[1] - xc[1], xfa[2] - xc[2]};
+  if (bc == kBcNeumann || bc == kBcWall) {
+    smag = fabs(dot3(sV, n));
+    for (int i = 0; i < 3; i++) sArg[i] = n[i];
+    alpha = 1.0;
+  } else {
+    smag = sqrt(dot3(sV, sV));
+    for (int i = 0; i < 3; i++) sArg[i] = sV[i] / smag;
+    alpha = dot3(n, sArg);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Spalart-Allmaras (RansTurbSA, TurbModel_SA.h) as one segregated scalar nuSA
 // ---------------------------------------------------------------------------
@@ -1263,6 +1347,30 @@ __device__ __forceinline__ void d_jac(const Blk &B, DMesh m, DParams P, const do
         for (int i = 0; i < 5; i++)
           for (int j = 0; j < 5; j++) D[i][j] -= J[i][j];
       }
+      if (VISC && SCHEME == kHllcC) {  // the same face's viscous blocks next (JoeWithModels.cpp:5096-5104)
+        const VSide vo = vside_cell(P, pc, gradv + 12 * (size_t)c);
+        const VSide vn = vside_cell(P, pn, gradv + 12 * (size_t)en.y);
+        const VSide a = vsel(s0, vo, vn), b = vsel(s0, vn, vo);
+        const double xa[3] = {s0 ? pc[5] : pn[5], s0 ? pc[6] : pn[6], s0 ? pc[7] : pn[7]};
+        const double xb[3] = {s0 ? pn[5] : pc[5], s0 ? pn[6] : pc[6], s0 ? pn[7] : pc[7]};
+        const double xfa[3] = {geo[4], geo[5], geo[6]};
+        double sV[3], smag, mul, w0, w1;
+        visc_internal_geom(P, xa, xb, xfa, a.T, b.T, sV, smag, mul, w0, w1);
+        const double alpha = dot3(n, sV);
+        const double ufa[3] = {0.5 * (a.u[0] + b.u[0]), 0.5 * (a.u[1] + b.u[1]), 0.5 * (a.u[2] + b.u[2])};
+        double VF[5], tau[3], keff;
+        visc_coupled(P, a, b, mul, 0.0, ufa, area, n, smag, sV, alpha, VF, tau, keff);
+        // icv0: A[noc00] += A0, A[noc01] += A1; icv1: A[noc11] -= A1, A[noc10] -= A0
+        visc_coupled_jac(P, J, s0 ? 0 : 1, vsel(s0, a, b), mul, keff, tau, ufa, area, alpha, smag);
+        for (int i = 0; i < 5; i++)
+          for (int j = 0; j < 5; j++)
+            if (s0) D[i][j] += J[i][j];
+            else D[i][j] -= J[i][j];
+        visc_coupled_jac(P, J, s0 ? 1 : 0, vsel(s0, b, a), mul, keff, tau, ufa, area, alpha, smag);
+        for (int i = 0; i < 5; i++)
+          for (int j = 0; j < 5; j++)
+            off[5 * i + j] = (T)(s0 ? (double)off[5 * i + j] + J[i][j] : (double)off[5 * i + j] - J[i][j]);
+      }
     } else {
       const ZoneDev &z = m.zones[-en.y - 1];
       const double *bb = bfa + 8 * (size_t)f;
@@ -1289,6 +1397,21 @@ __device__ __forceinline__ void d_jac(const Blk &B, DMesh m, DParams P, const do
         } else {  // cell-centre state against the boundary state
           jac_hllc_coupled(J, 0, own.rho, own.u, own.p, enthalpy_of(P, own.rho, own.p), sb.rho, sb.u, sb.p, hb, gam,
                            area, n);
+          for (int i = 0; i < 5; i++)
+            for (int j = 0; j < 5; j++) D[i][j] += J[i][j];
+        }
+        if (VISC && z.bc != kBcSymmetry) {  // the face's viscous block A0 (no implicit term on symmetry faces)
+          const double *gvo = gradv + 12 * (size_t)c;
+          const VSide vo = vside_cell(P, pc, gvo);
+          const VSide vb = vside_bfa(bb, gvo);
+          const double xfa[3] = {geo[4], geo[5], geo[6]};
+          double sArg[3], smag, alpha;
+          visc_coupled_bgeom(z.bc, xfa, pc + 5, n, sArg, smag, alpha);
+          const double mul = mu_lam(P, bb[5]);
+          const VSide l = vsel(z.bc == kBcNeumann, vb, vo);
+          double VF[5], tau[3], keff;
+          visc_coupled(P, l, vb, mul, 0.0, vb.u, area, n, smag, sArg, alpha, VF, tau, keff);
+          visc_coupled_jac(P, J, 0, l, mul, keff, tau, vb.u, area, alpha, smag);
           for (int i = 0; i < 5; i++)
             for (int j = 0; j < 5; j++) D[i][j] += J[i][j];
         }
@@ -1323,7 +1446,7 @@ __device__ __forceinline__ void d_jac(const Blk &B, DMesh m, DParams P, const do
       }
     }
   }
-  if (VISC) {  // viscous Jacobian blocks, after the Euler ones (JoeWithModels.cpp:3680-3697, 3789-3796, 3835-3842)
+  if (VISC && SCHEME != kHllcC) {  // viscous Jacobian blocks, after the Euler ones (JoeWithModels.cpp:3680-3697, 3789-3796, 3835-3842)
     const VSide vo = vside_cell(P, pc, gradv + 12 * (size_t)c);
     for (int e = e0; e < e1; e++) {
       int2 en = m.cfl[e];
@@ -1947,7 +2070,10 @@ __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const
                                                    const double *__restrict__ q, double *__restrict__ rhs,
                                                    int *__restrict__ nfirst, DtArgs dta, double *__restrict__ dt,
                                                    double *__restrict__ partials, const double *__restrict__ gradv) {
-  constexpr int NS = VISC ? 11 : 6;  // inviscid 5, wave speed, viscous rhou/rhoE 4, SUM_BASED diffusive term
+  // inviscid 5, wave speed, viscous rhou/rhoE 4 (coupled: all 5), SUM_BASED diffusive term
+  constexpr bool CPL = SCHEME == kHllcC;
+  constexpr int NS = VISC ? (CPL ? 12 : 11) : 6;
+  constexpr int kSumSlot = CPL ? 11 : 10;
   __shared__ double lds[kBlock * NS];
   __shared__ double sh[16];
   double *slot = lds + NS * threadIdx.x;
@@ -1994,6 +2120,7 @@ __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const
     double sgn = -1.0;
     bool low = false;
     double vc[4] = {-0.0, -0.0, -0.0, -0.0};
+    double vcc[5] = {-0.0, -0.0, -0.0, -0.0, -0.0};  // the coupled viscous terms (-0.0: nothing added)
     double mul = 0.0, mut = 0.0;
     // each lane's two sides are chosen first (this cell reconstructed at the
     // face, the neighbour, or the boundary state) and the flux evaluated once
@@ -2044,11 +2171,17 @@ __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const
           mut = sa_mut_face(w0, w1, a.rho, b.rho, s0 ? nuo : nun, s0 ? nun : nuo, mul);
         }
         const double ufa[3] = {0.5 * (a.u[0] + b.u[0]), 0.5 * (a.u[1] + b.u[1]), 0.5 * (a.u[2] + b.u[2])};
-        double Fu[3], FE;
-        VFace vf;
-        visc_face(P, a, b, mul, mut, ufa, area, n, smag, sV, Fu, FE, vf);
-        for (int k = 0; k < 3; k++) vc[k] = s0 ? -Fu[k] : Fu[k];
-        vc[3] = s0 ? -FE : FE;
+        if (CPL) {  // calcFluxCoupled internal faces (JoeWithModels.cpp:5076-5094)
+          double VF[5], tau[3], keff;
+          visc_coupled(P, a, b, mul, mut, ufa, area, n, smag, sV, dot3(n, sV), VF, tau, keff);
+          for (int k = 0; k < 5; k++) vcc[k] = s0 ? -VF[k] : VF[k];
+        } else {
+          double Fu[3], FE;
+          VFace vf;
+          visc_face(P, a, b, mul, mut, ufa, area, n, smag, sV, Fu, FE, vf);
+          for (int k = 0; k < 3; k++) vc[k] = s0 ? -Fu[k] : Fu[k];
+          vc[3] = s0 ? -FE : FE;
+        }
       }
     } else {
       const ZoneDev &z = m.zones[-en.y - 1];
@@ -2071,7 +2204,23 @@ __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const
         fr = sb;
         if (SCHEME == kHllcC) hl = enthalpy_of(P, so.rho, so.p);
       }
-      if (VISC) {  // boundary faces (:3714-3851); boundary mul from T_bfa (ComputeBCProperties_T)
+      if (VISC && CPL) {  // calcFluxCoupled boundary faces; mul from T_bfa
+        mul = mu_lam(P, bb[5]);
+        const double *gvo = gradv + 12 * (size_t)L.c;
+        if (z.bc == kBcSymmetry) {  // 2/3 (mu + mut) tr(grad u) on the momentum only (:5272-5284)
+          double tmp = 2.0 / 3.0 * ((mul + mut) * (gvo[0] + gvo[4] + gvo[8]) + bb[0] * 0.0);
+          for (int i = 0; i < 3; i++) vcc[1 + i] = -(area * tmp * n[i]);
+        } else {
+          const VSide vo = vside_cell(P, pc, gvo);
+          const VSide vb = vside_bfa(bb, gvo);
+          double sArg[3], smag, alpha;
+          visc_coupled_bgeom(z.bc, xfa, pc + 5, n, sArg, smag, alpha);
+          double VF[5], tau[3], keff;
+          visc_coupled(P, vsel(z.bc == kBcNeumann, vb, vo), vb, mul, z.bc == kBcWall ? 0.0 : mut, vb.u, area, n,
+                       smag, sArg, alpha, VF, tau, keff);
+          for (int k = 0; k < 5; k++) vcc[k] = -VF[k];
+        }
+      } else if (VISC) {  // boundary faces (:3714-3851); boundary mul from T_bfa (ComputeBCProperties_T)
         mul = mu_lam(P, bb[5]);
         if (P.sa && z.bc != kBcWall)  // mut of every non-wall boundary face, symmetry included (calcDt reads it)
           mut = sa_mut(pc[0] * P.sa_phi[L.c], P.sa_aux[kAuxN * (size_t)L.c + kAuxMue]);
@@ -2108,11 +2257,14 @@ __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const
       F[4] = 0.0;
     }
     if (VISC) {
-      for (int k = 0; k < 4; k++) slot[6 + k] = vc[k];
+      if (CPL)
+        for (int k = 0; k < 5; k++) slot[6 + k] = vcc[k];
+      else
+        for (int k = 0; k < 4; k++) slot[6 + k] = vc[k];
       if (dta.sum) {  // SUM_BASED diffusive term (UgpWithCvCompFlow.cpp:75-88)
         const double volume = en.y >= 0 ? 0.5 * (m.vol[L.c] + m.vol[en.y]) : 0.5 * m.vol[L.c];
         const double KinVisc = (4.0 / 3.0) * (mul + mut) / pc[0];
-        slot[10] = area * area * KinVisc / volume;
+        slot[kSumSlot] = area * area * KinVisc / volume;
       } else {  // MAX_BASED viscous bound (:50-51)
         const double lamdiff = 4.0 * (mul + mut) / pc[0] * area * area / m.vol[L.c];
         lambda = Dev::mx(lambda, lamdiff);
@@ -2133,9 +2285,11 @@ __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const
     for (int j = 0; j < nent; j++) {
       const double *s = slot + NS * j;
       for (int k = 0; k < 5; k++) acc[k] += s[k];
+      if (VISC && CPL)  // the coupled loop adds each face's viscous flux after its Euler flux (:5024-5094)
+        for (int k = 0; k < 5; k++) acc[k] += s[6 + k];
       lambdaMax = Dev::mx(lambdaMax, s[5]);
     }
-    if (VISC)  // the viscous pass follows the Euler pass (JoeWithModels.cpp:2585-2589)
+    if (VISC && !CPL)  // the viscous pass follows the Euler pass (JoeWithModels.cpp:2585-2589)
       for (int j = 0; j < nent; j++) {
         const double *s = slot + NS * j + 6;
         for (int k = 0; k < 4; k++) acc[1 + k] += s[k];
@@ -2164,7 +2318,7 @@ __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const
       for (int k = 0; k < nent; k++) {
         const double *s = slot + NS * m.foc[m.cfl_i[c] + k];
         lc += s[5];
-        if (VISC) ld += s[10];
+        if (VISC) ld += s[kSumSlot];
       }
       dtc = Dev::mn(cflv * vol / lc, VISC ? 0.25 * cflv * vol / ld : 1.0e20);
     } else {
@@ -3301,7 +3455,9 @@ void parse_params(const std::string &joe_in, const Mesh &m, SolverParams &p, std
       throw Unsupported("LINEAR_SOLVER_NS = " + ls + " is not supported (PETSC_GMRES, BCGSTAB)");
     if (p.scheme != kHllc) throw Unsupported("BACKWARD_EULER_COUPLED is supported with HLLC only");
     if (p.sa) throw Unsupported("BACKWARD_EULER_COUPLED with scalars (TURB_MODEL) is not supported");
-    if (p.mu_ref > 0.0) throw Unsupported("BACKWARD_EULER_COUPLED with MU_REF > 0 is not supported");
+    // the coupled face loop limits grad u and grad h with the limiter of
+    // GRAD_RECONSTRUCTION alone (JoeWithModels.cpp:4691-4716)
+    p.visc_limiter = true;
     if (p.flux_kernel != kResidEntry) throw Unsupported("BACKWARD_EULER_COUPLED needs GPU_FLUX_KERNEL = ENTRY");
     p.scheme = kHllcC;
   } else if (ls != "BCGSTAB") {
@@ -4600,10 +4756,12 @@ void Ctx::Impl::launch_flux(hipStream_t s, const double *cfl_dev) {
       if (par.second_order) {
         if (par.scheme == kRoe) goe(k_flux_e<kRoe, true, true>);
         else if (par.scheme == kAusm) goe(k_flux_e<kAusm, true, true>);
+        else if (par.scheme == kHllcC) goe(k_flux_e<kHllcC, true, true>);
         else goe(k_flux_e<kHllc, true, true>);
       } else {
         if (par.scheme == kRoe) goe(k_flux_e<kRoe, false, true>);
         else if (par.scheme == kAusm) goe(k_flux_e<kAusm, false, true>);
+        else if (par.scheme == kHllcC) goe(k_flux_e<kHllcC, false, true>);
         else goe(k_flux_e<kHllc, false, true>);
       }
     } else {
@@ -4703,6 +4861,7 @@ void Ctx::Impl::launch_jac(bool step_mode) {
   };
   if (dp.visc) {
     if (par.scheme == kRoe) go(k_jac<T, kRoe, true>);
+    else if (par.scheme == kHllcC) go(k_jac<T, kHllcC, true>);
     else go(k_jac<T, kHllc, true>);
   } else {
     if (par.scheme == kRoe) go(k_jac<T, kRoe, false>);

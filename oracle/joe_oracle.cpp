@@ -1554,6 +1554,60 @@ void calc_euler_flux(or_ctx &c, double *A) {
   }
 }
 
+// calcViscousFluxCoupled, UgpWithCvCompFlow.cpp:3040-3274, nScal = 0, no
+// turbulence model (kine = 0, psi = 0). gu0/gu1 are [3][3] velocity
+// gradients, gh0/gh1 enthalpy gradients. A0 / A1 (may be NULL) get only the
+// entries the reference writes; the caller zeroes them once.
+static void visc_coupled(double F[5], double (*A0)[5], double (*A1)[5], double rho0, const double *u0,
+                         const double *gu0, double h0, const double *gh0, double T0, double R0, double gam0,
+                         double rho1, const double *u1, const double *gu1, double h1, const double *gh1, double T1,
+                         double R1, double gam1, double mul, double mut, double lamOcp, double pr_turb,
+                         const double *u_fa, double area, const double *nVec, double smag, const double *sVec,
+                         double alpha) {
+  const double kine0 = 0.0, kine1 = 0.0;
+  double g[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) g[i][j] = 0.5 * (gu0[3 * i + j] + gu1[3 * i + j]);
+  double fCorr[3] = {nVec[0] - alpha * sVec[0], nVec[1] - alpha * sVec[1], nVec[2] - alpha * sVec[2]};
+  double mu = mul + mut;
+  double tau[3];
+  F[0] = 0.0;
+  for (int i = 0; i < 3; i++)
+    tau[i] = mu * (alpha * (u1[i] - u0[i]) / smag + g[i][0] * fCorr[0] + g[0][i] * nVec[0] + g[i][1] * fCorr[1] +
+                   g[1][i] * nVec[1] + g[i][2] * fCorr[2] + g[2][i] * nVec[2]);
+  double tmp = 2.0 / 3.0 * mu * (g[0][0] + g[1][1] + g[2][2]);
+  for (int i = 0; i < 3; i++) tau[i] -= tmp * nVec[i];
+  for (int i = 0; i < 3; i++) F[1 + i] = -area * tau[i];
+  double keff = lamOcp + mut / pr_turb;
+  double enth = keff * (alpha * (h1 - h0) / smag + 0.5 * ((gh0[0] + gh1[0]) * fCorr[0] + (gh0[1] + gh1[1]) * fCorr[1] +
+                                                         (gh0[2] + gh1[2]) * fCorr[2]));
+  double psi = 0.0;
+  F[4] = -area * (enth + tau[0] * u_fa[0] + tau[1] * u_fa[1] + tau[2] * u_fa[2] + psi);
+  auto side = [&](double (*A)[5], double sg, double rho, const double *u, double h, double T, double R, double gam,
+                  double kine) {
+    double aas = area * alpha / smag;
+    double d11 = sg > 0.0 ? mu * aas : -mu * aas;
+    double d44 = sg > 0.0 ? keff * aas : -keff * aas;
+    double d4[3];
+    for (int i = 0; i < 3; i++) d4[i] = -area * tau[i] * 0.5 + u_fa[i] * d11;
+    double invRho = 1.0 / rho;
+    double q0[3], q4[3];
+    for (int i = 0; i < 3; i++) q0[i] = -invRho * u[i];
+    double q40 = invRho * gam * (-h - kine + R * T + 0.5 * dot3(u, u));
+    for (int i = 0; i < 3; i++) q4[i] = -invRho * u[i] * gam;
+    double q44 = invRho * gam;
+    for (int i = 0; i < 3; i++) {
+      A[1 + i][0] = d11 * q0[i];
+      A[1 + i][1 + i] = d11 * invRho;
+    }
+    A[4][0] = d4[0] * q0[0] + d4[1] * q0[1] + d4[2] * q0[2] + d44 * q40;
+    for (int i = 0; i < 3; i++) A[4][1 + i] = d4[i] * invRho + d44 * q4[i];
+    A[4][4] = d44 * q44;
+  };
+  if (A0) side(A0, 1.0, rho0, u0, h0, T0, R0, gam0, kine0);
+  if (A1) side(A1, -1.0, rho1, u1, h1, T1, R1, gam1, kine1);
+}
+
 // calcFluxCoupled, JoeWithModels.cpp:4613-5825, for nScal = 0, MU_REF = 0 and
 // HLLC: the coupled HLLC flux and Jacobian, each face side's enthalpy from
 // calcThermoProp_p after reconstruction (the cell's when first order), and
@@ -1565,7 +1619,22 @@ void calc_euler_flux(or_ctx &c, double *A) {
 void calc_flux_coupled(or_ctx &c, double *A) {
   const or_mesh &m = c.m;
   const bool snd = c.p.second_order != 0;
+  const bool visc = c.p.mu_ref > 0.0;
   if (snd) euler_gradients(c);
+  const bool ls = c.p.grad_scheme == OR_GRAD_LS;
+  if (visc) {  // grad_u also when first order, and grad_enthalpy, both limited by the limiter of GRAD_RECONSTRUCTION (:4691-4716)
+    if (!snd) {
+      if (ls) grad_vector_ls(c, c.grad_u.data(), c.vel.data(), c.vel_bfa.data());
+      else grad_vector_gg(c, c.grad_u.data(), c.vel.data(), c.vel_bfa.data());
+      if (c.p.limiter) limit_vector(c, c.grad_u.data(), c.vel.data(), c.sos.data());
+    }
+    if (ls) grad_scalar_ls(c, c.grad_h.data(), c.enthalpy.data(), c.h_bfa.data());
+    else grad_scalar_gg(c, c.grad_h.data(), c.enthalpy.data(), c.h_bfa.data());
+    if (c.p.limiter) limit_scalar(c, c.grad_h.data(), c.enthalpy.data(), c.enthalpy.data(), NULL);
+  }
+  double V0[5][5], V1[5][5], VF[5];
+  for (int i = 0; i < 5; i++)
+    for (int j = 0; j < 5; j++) V0[i][j] = V1[i][j] = 0.0;
   const double g = c.p.gamma, R = c.p.R_gas;
   auto thermo_h = [&](double rho, double p) {  // calcThermoProp_p, UgpWithCvCompFlow.h:1765-1771
     double T = p / (rho * R);
@@ -1635,6 +1704,35 @@ void calc_flux_coupled(or_ctx &c, double *A) {
           A[25 * noc10 + 5 * i + j] -= Apl[i][j];
         }
     }
+    if (visc) {  // the same face's viscous flux and blocks (:5076-5105)
+      double sVec[3];
+      for (int i = 0; i < 3; i++) sVec[i] = m.x_cv[3 * icv1 + i] - m.x_cv[3 * icv0 + i];
+      double smag = sqrt(dot3(sVec, sVec));
+      for (int i = 0; i < 3; i++) sVec[i] /= smag;
+      double alpha = dot3(nVec, sVec);
+      const double *v0 = &c.vel[3 * icv0], *v1 = &c.vel[3 * icv1];
+      double uAux_fa[3] = {0.5 * (v0[0] + v1[0]), 0.5 * (v0[1] + v1[1]), 0.5 * (v0[2] + v1[2])};
+      visc_coupled(VF, A ? V0 : NULL, A ? V1 : NULL, c.rho[icv0], v0, &c.grad_u[9 * icv0], c.enthalpy[icv0],
+                   &c.grad_h[3 * icv0], c.temp[icv0], R, g, c.rho[icv1], v1, &c.grad_u[9 * icv1], c.enthalpy[icv1],
+                   &c.grad_h[3 * icv1], c.temp[icv1], R, g, c.mul_fa[ifa], c.mut_fa[ifa], c.lamOcp_fa[ifa],
+                   c.p.pr_turb, uAux_fa, area, nVec, smag, sVec, alpha);
+      c.rhs_rho[icv0] -= VF[0];
+      for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= VF[1 + i];
+      c.rhs_rhoE[icv0] -= VF[4];
+      c.rhs_rho[icv1] += VF[0];
+      for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv1 + i] += VF[1 + i];
+      c.rhs_rhoE[icv1] += VF[4];
+      if (A) {
+        for (int k = 0; k < 25; k++) {
+          A[25 * noc00 + k] += V0[k / 5][k % 5];
+          A[25 * noc01 + k] += V1[k / 5][k % 5];
+        }
+        for (int k = 0; k < 25; k++) {
+          A[25 * noc11 + k] -= V1[k / 5][k % 5];
+          A[25 * noc10 + k] -= V0[k / 5][k % 5];
+        }
+      }
+    }
   }
   for (int iz = 0; iz < m.nzone; iz++) {  // :5113-5796
     const int bc = m.zone_bc[iz];
@@ -1658,6 +1756,24 @@ void calc_flux_coupled(or_ctx &c, double *A) {
             if (nn) A[25 * noc00 + k] += Apl[k / 5][k % 5] + Ami[k / 5][k % 5];
             else A[25 * noc00 + k] += Apl[k / 5][k % 5];
         }
+        if (visc && bc == OR_BC_SYMMETRY) {  // 2/3 (mu + mut) tr(grad u) only (:5272-5284)
+          const double *gu = &c.grad_u[9 * icv0];
+          double tmp = 2.0 / 3.0 * ((c.mul_fa[ifa] + c.mut_fa[ifa]) * (gu[0] + gu[4] + gu[8]) + rb * 0.0);
+          for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= area * tmp * nVec[i];
+        } else if (visc) {  // wall: the cell against the boundary state, mut = 0 (:5548-5575)
+          double sVec[3];
+          for (int i = 0; i < 3; i++) sVec[i] = m.x_fa[3 * ifa + i] - m.x_cv[3 * icv0 + i];
+          double smag = fabs(dot3(sVec, nVec));
+          const double *gu = &c.grad_u[9 * icv0], *gh = &c.grad_h[3 * icv0];
+          visc_coupled(VF, A ? V0 : NULL, NULL, c.rho[icv0], &c.vel[3 * icv0], gu, c.enthalpy[icv0], gh, c.temp[icv0],
+                       R, g, rb, ub, gu, hb, gh, c.T_bfa[ifa], R, g, c.mul_fa[ifa], 0.0, c.lamOcp_fa[ifa], c.p.pr_turb,
+                       ub, area, nVec, smag, nVec, 1.0);
+          c.rhs_rho[icv0] -= VF[0];
+          for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= VF[1 + i];
+          c.rhs_rhoE[icv0] -= VF[4];
+          if (A)
+            for (int k = 0; k < 25; k++) A[25 * noc00 + k] += V0[k / 5][k % 5];
+        }
       } else if (bc == OR_BC_NEUMANN) {
         flux_hllc_coupled(F, rb, ub, pb, hb, g, rb, ub, pb, hb, g, area, nVec, false);
         c.rhs_rho[icv0] -= F[0];
@@ -1668,6 +1784,20 @@ void calc_flux_coupled(or_ctx &c, double *A) {
           for (int k = 0; k < 25; k++)
             if (c.p.symm_bc == OR_SYMM_NEW_NUMERIC) A[25 * noc00 + k] += Apl[k / 5][k % 5] + Ami[k / 5][k % 5];
             else A[25 * noc00 + k] += Apl[k / 5][k % 5];
+        }
+        if (visc) {  // the boundary state on both sides, s along the normal (:5373-5404)
+          double sVec[3];
+          for (int i = 0; i < 3; i++) sVec[i] = m.x_fa[3 * ifa + i] - m.x_cv[3 * icv0 + i];
+          double smag = fabs(dot3(sVec, nVec));
+          const double *gu = &c.grad_u[9 * icv0], *gh = &c.grad_h[3 * icv0];
+          visc_coupled(VF, A ? V0 : NULL, A ? V1 : NULL, rb, ub, gu, hb, gh, c.T_bfa[ifa], R, g, rb, ub, gu, hb, gh,
+                       c.T_bfa[ifa], R, g, c.mul_fa[ifa], c.mut_fa[ifa], c.lamOcp_fa[ifa], c.p.pr_turb, ub, area,
+                       nVec, smag, nVec, 1.0);
+          c.rhs_rho[icv0] -= VF[0];
+          for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= VF[1 + i];
+          c.rhs_rhoE[icv0] -= VF[4];
+          if (A)
+            for (int k = 0; k < 25; k++) A[25 * noc00 + k] += V0[k / 5][k % 5];
         }
       } else {
         double rho0 = c.rho[icv0], p0 = c.press[icv0], h0 = c.enthalpy[icv0];
@@ -1693,6 +1823,22 @@ void calc_flux_coupled(or_ctx &c, double *A) {
           jac_hllc_coupled(Apl, NULL, c.rho[icv0], &c.vel[3 * icv0], c.press[icv0], c.enthalpy[icv0], g, rb, ub, pb,
                            hb, g, area, nVec, false);
           for (int k = 0; k < 25; k++) A[25 * noc00 + k] += Apl[k / 5][k % 5];
+        }
+        if (visc) {  // the cell against the boundary state (:5744-5775)
+          double sVec[3];
+          for (int i = 0; i < 3; i++) sVec[i] = m.x_fa[3 * ifa + i] - m.x_cv[3 * icv0 + i];
+          double smag = sqrt(dot3(sVec, sVec));
+          for (int i = 0; i < 3; i++) sVec[i] /= smag;
+          double alpha = dot3(nVec, sVec);
+          const double *gu = &c.grad_u[9 * icv0], *gh = &c.grad_h[3 * icv0];
+          visc_coupled(VF, A ? V0 : NULL, NULL, c.rho[icv0], &c.vel[3 * icv0], gu, c.enthalpy[icv0], gh, c.temp[icv0],
+                       R, g, rb, ub, gu, hb, gh, c.T_bfa[ifa], R, g, c.mul_fa[ifa], c.mut_fa[ifa], c.lamOcp_fa[ifa],
+                       c.p.pr_turb, ub, area, nVec, smag, sVec, alpha);
+          c.rhs_rho[icv0] -= VF[0];
+          for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= VF[1 + i];
+          c.rhs_rhoE[icv0] -= VF[4];
+          if (A)
+            for (int k = 0; k < 25; k++) A[25 * noc00 + k] += V0[k / 5][k % 5];
         }
       }
     }

@@ -144,8 +144,8 @@ def make_params(joe_in: str) -> tuple[_Params, dict]:
     p.eps_limiter = float(_after(gr, "EPS"))
     ti = get("TIME_INTEGRATION", "BACKWARD_EULER")
     p.coupled = 1 if ti == "BACKWARD_EULER_COUPLED" else 0
-    if p.coupled and (p.mu_ref > 0.0 or get("SPACE_INTEGRATION", "HLLC") != "HLLC"):
-        raise ValueError("the coupled restatement covers inviscid HLLC only")
+    if p.coupled and get("SPACE_INTEGRATION", "HLLC") != "HLLC":
+        raise ValueError("the coupled restatement covers HLLC only")
     # calcFluxCoupled's own default is ANALYTICAL (JoeWithModels.cpp:4615), but
     # init has already registered ORIGINAL_NUMERIC by reading the key with that
     # default while printing it (UgpWithCvCompFlow.h:479; Param.cpp:139-149)

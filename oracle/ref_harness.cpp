@@ -311,7 +311,11 @@ class RefJoe : public JoeWithModels {
     if (sndOrder) {
       d.f64("GRAD_RHO", &grad_rho[0][0], 3LL * ncv);
       d.f64("GRAD_P", &grad_p[0][0], 3LL * ncv);
-      d.f64("GRAD_U", &grad_u[0][0][0], 9LL * ncv);
+    }
+    if (sndOrder || mu_ref > 0.0) d.f64("GRAD_U", &grad_u[0][0][0], 9LL * ncv);
+    if (mu_ref > 0.0) {
+      d.f64("GRAD_ENTHALPY", &grad_enthalpy[0][0], 3LL * ncv);
+      d.f64("MUL_FA", mul_fa, nfa);
     }
     freeMem3D(A, 0, nbocv_s - 1, 0, b - 1, 0, b - 1);
     freeMem2D(rhs, 0, ncv - 1, 0, b - 1);

@@ -86,11 +86,16 @@ RHS_CASES = [("bump24", "HLLC", {}), ("bump24", "ROE", {}), ("bump24", "AUSM", {
 COUPLED_CASES = [("bump24", {}), ("box4", {}), ("box4", {"symm_bc": "ANALYTICAL"}),
                  ("box4", {"bcs": cases.BOX_BCS_ALT, "symm_bc": "NEW_NUMERIC"}),
                  ("box4", {"bcs": cases.BOX_BCS_ALT, "symm_bc": "ORIGINAL_NUMERIC", "grad": "GREENGAUSS"}),
-                 ("box4", {"second_order": False})]
+                 ("box4", {"second_order": False}),
+                 # laminar (calcViscousFluxCoupled): both viscosity laws, the
+                 # alternate boundary kinds, first order
+                 ("bump24", {"visc": "pl"}), ("box4", {"visc": "su", "symm_bc": "ANALYTICAL"}),
+                 ("box4", {"visc": "pl", "bcs": cases.BOX_BCS_ALT, "symm_bc": "NEW_NUMERIC", "grad": "GREENGAUSS"}),
+                 ("box4", {"visc": "su", "second_order": False})]
 
 
 def coupled_tag(kw):
-    t = ""
+    t = "_visc_" + kw["visc"] if "visc" in kw else ""
     if "bcs" in kw:
         t += "_alt"
     if "symm_bc" in kw:
@@ -211,7 +216,10 @@ def main(only=None):
             fixture = f"coupled_{name}_hllc" + coupled_tag(kw)
             if only and only not in fixture:
                 continue
+            visc = kw.pop("visc", None)
             mm, joe = MESHES[name]("HLLC", **kw)
+            if visc:
+                joe = joe.replace("MU_REF = 0.0", VISC[visc])
             joe = cases.coupled(joe, keep_symm_bc=symm is not None)
             fn = "box.msh" if name.startswith("box") else "bump.msh"
             meshgen.write_msh(mm, os.path.join(wd, fn))

@@ -1,6 +1,7 @@
 """runBackwardEulerCoupled (TIME_INTEGRATION = BACKWARD_EULER_COUPLED) without
-scalars, inviscid, HLLC: calcRhsCoupled's residual and Jacobian blocks
-(JoeWithModels.cpp:4596-5825, UgpWithCvCompFlow.cpp:1808-2287).
+scalars, HLLC, inviscid or laminar: calcRhsCoupled's residual and Jacobian
+blocks (JoeWithModels.cpp:4596-5825, UgpWithCvCompFlow.cpp:1808-2287,
+3040-3274).
 
 Pinned against the reference's own calcRhsCoupled (tests/golden/coupled_*.npz,
 written by make_golden.py through the harness's rhs-coupled mode). The
@@ -23,6 +24,7 @@ from tests.test_oracle_golden import GOLD, joe_text, load, mesh_for, parse_joe_i
 COUPLED = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLD, "coupled_*.npz")))
 FIELDS = ["PRESS", "VEL", "RHO_BFA", "VEL_BFA", "P_BFA", "T_BFA", "H_BFA", "LOCAL_DT", "GRAD_RHO", "GRAD_P",
           "GRAD_U", "RHS_RHO", "RHS_RHOU", "RHS_RHOE", "A"]
+VISC_FIELDS = ["GRAD_ENTHALPY", "MUL_FA"]
 
 
 def _mesh_key(fixture):
@@ -45,6 +47,10 @@ def test_coupled_fixture_set_complete():
     assert symm == {None, "ORIGINAL_NUMERIC", "NEW_NUMERIC", "ANALYTICAL"}
     assert any("first" in f for f in COUPLED) and any("gg" in f for f in COUPLED)
     assert any("bump" in f for f in COUPLED)
+    # laminar: both viscosity laws, first order, the alternate boundary kinds
+    visc = [f for f in COUPLED if "visc" in f]
+    assert any("_pl" in f for f in visc) and any("_su" in f for f in visc)
+    assert any("first" in f for f in visc) and any("alt" in f for f in visc)
 
 
 @pytest.mark.parametrize("fixture", COUPLED)
@@ -55,7 +61,7 @@ def test_oracle_coupled_bit_exact(fixture):
     o = Oracle(mesh_for(_mesh_key(fixture)), joe_text(g))
     o.set_state(g["RHO"], g["RHOU"], g["RHOE"])
     o.residual(True)
-    for k in FIELDS:
+    for k in FIELDS + (VISC_FIELDS if "visc" in fixture else []):
         if k not in g:
             assert "first" in fixture and k.startswith("GRAD"), k
             continue
@@ -118,7 +124,7 @@ def test_coupled_loop_reaches_the_segregated_steady_state():
 
 @pytest.mark.parametrize("edit,needle", [
     (lambda s: s.replace("SPACE_INTEGRATION = HLLC", "SPACE_INTEGRATION = ROE"), "HLLC only"),
-    (lambda s: s.replace("MU_REF = 0.0", "MU_REF = 0.002"), "MU_REF"),
+    (lambda s: s.replace("MU_REF = 0.0", "MU_REF = 0.002") + "TURB_MODEL = SA\n", "scalars"),
     (lambda s: s + "GPU_FLUX_KERNEL = CELL\n", "GPU_FLUX_KERNEL"),
     (lambda s: s.replace("PETSC_GMRES", "LU"), "LINEAR_SOLVER_NS"),
 ])
@@ -146,7 +152,7 @@ def test_gpu_coupled_matches_reference(fixture):
     s.set_state(g["RHO"], g["RHOU"], g["RHOE"])
     s.calc_rhs(True)
     exact = total = 0
-    for k in FIELDS:
+    for k in FIELDS + (["GRAD_ENTHALPY"] if "visc" in fixture else []):
         if k not in g:
             continue
         e, n = compare(s.get(k), g[k], 1e-12, f"{fixture}:{k}")
@@ -156,14 +162,17 @@ def test_gpu_coupled_matches_reference(fixture):
 
 
 @pytest.mark.gpu
-def test_gpu_coupled_steps_track_oracle():
+@pytest.mark.parametrize("visc", [False, True])
+def test_gpu_coupled_steps_track_oracle(visc):
     # the coupled loop on the device against the oracle's, 40 steps (the
     # trajectory bars of the segregated loop: L1 history 1e-7, state 1e-9)
     from oracle.oracle import Oracle
     from tests.helpers import compare
 
-    print("coupled steps", flush=True)
+    print(f"coupled steps visc={visc}", flush=True)
     mm, _, cpl = _bump_pair(40)
+    if visc:
+        cpl = cpl.replace("MU_REF = 0.0", "MU_REF = 0.002\nMU_MODE = POWERLAW")
     mesh = joemi.Mesh.from_faces(mm)
     o = Oracle(mesh.as_dump(), cpl)
     s = joemi.Solver(mesh, cpl)
