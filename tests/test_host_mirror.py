@@ -29,10 +29,12 @@ def write_case(d, joe):
     return mm
 
 
-def run_case(d, case):
+def run_case(d, case, expect_rc=0, expect_out=None):
     out = os.path.join(d, case + ".bin")
     r = subprocess.run([EXE, case, "Joe.in", out], cwd=d, capture_output=True, text=True, timeout=120)
-    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.returncode == expect_rc, r.stdout + r.stderr
+    if expect_out:
+        assert expect_out in r.stdout, r.stdout
     raw = np.fromfile(out, np.float64)
     n = int(raw[0])
     hist = raw[1:1 + 5 * n].reshape(n, 5)
@@ -111,6 +113,30 @@ def test_source_hook_override_equals_device_gravity(tmp_path):
     assert not np.array_equal(dev[2], none[2])  # gravity acts
     assert host[0] == dev[0]
     for a, b in zip(host[1:], dev[1:]):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.gpu
+def test_coupled_case_matches_abi_run_and_ignores_source_hook(tmp_path):
+    # TIME_INTEGRATION = BACKWARD_EULER_COUPLED through the mirror: the ABI's
+    # coupled run bit for bit; the reference's coupled loop calls
+    # sourceHookCoupled (default: gravity), not an overridden sourceHook, so
+    # the case with a host sourceHook equals the plain one
+    print("host mirror: coupled", flush=True)
+    joe = cases.coupled(joe_text() + "GX = 0.05\nGY = -0.2\nGZ = 0.0\n")
+    mm = write_case(tmp_path, joe)
+    plain = run_case(tmp_path, "plain")
+    # the case program checks that its sourceHook ran every step and exits
+    # with 3 otherwise, after writing its state: here it must not have run
+    host = run_case(tmp_path, "gravity", expect_rc=3, expect_out="hook_calls=0")
+    assert host[0] == plain[0] == NSTEPS
+    for a, b in zip(host[1:], plain[1:]):
+        assert np.array_equal(a, b)
+    s = joemi.Solver(joemi.Mesh.from_faces(mm), joe)
+    s.init_uniform()
+    n2, hist2 = s.run(NSTEPS)
+    assert n2 == NSTEPS and np.array_equal(plain[1], hist2)
+    for a, b in zip(plain[2:], s.state()):
         assert np.array_equal(a, b)
 
 
