@@ -4457,9 +4457,11 @@ static void step_batch_jac(const StepBatch::Impl &B, hipStream_t st) {
   };
   if (B.visc) {
     if (B.scheme == kRoe) go(ks_jac<T, kRoe, true>);
+    else if (B.scheme == kHllcC) go(ks_jac<T, kHllcC, true>);
     else go(ks_jac<T, kHllc, true>);
   } else {
     if (B.scheme == kRoe) go(ks_jac<T, kRoe, false>);
+    else if (B.scheme == kHllcC) go(ks_jac<T, kHllcC, false>);
     else go(ks_jac<T, kHllc, false>);
   }
 }
@@ -4493,20 +4495,24 @@ void StepBatch::pre(void *stream) const {
       if (B.snd) {
         if (B.scheme == kRoe) go(ks_flux<kRoe, true, true>);
         else if (B.scheme == kAusm) go(ks_flux<kAusm, true, true>);
+        else if (B.scheme == kHllcC) go(ks_flux<kHllcC, true, true>);
         else go(ks_flux<kHllc, true, true>);
       } else {
         if (B.scheme == kRoe) go(ks_flux<kRoe, false, true>);
         else if (B.scheme == kAusm) go(ks_flux<kAusm, false, true>);
+        else if (B.scheme == kHllcC) go(ks_flux<kHllcC, false, true>);
         else go(ks_flux<kHllc, false, true>);
       }
     } else {
       if (B.snd) {
         if (B.scheme == kRoe) go(ks_flux<kRoe, true, false>);
         else if (B.scheme == kAusm) go(ks_flux<kAusm, true, false>);
+        else if (B.scheme == kHllcC) go(ks_flux<kHllcC, true, false>);
         else go(ks_flux<kHllc, true, false>);
       } else {
         if (B.scheme == kRoe) go(ks_flux<kRoe, false, false>);
         else if (B.scheme == kAusm) go(ks_flux<kAusm, false, false>);
+        else if (B.scheme == kHllcC) go(ks_flux<kHllcC, false, false>);
         else go(ks_flux<kHllc, false, false>);
       }
     }
@@ -4541,7 +4547,6 @@ void StepBatch::post(void *stream) const {
 void Ctx::batch_prepare() {
   Impl &d = *d_;
   if (d.par.sa) throw Unsupported("batched serving does not carry the Spalart-Allmaras scalar");
-  if (d.par.coupled) throw Unsupported("batched serving runs the segregated backward-Euler step");
   if (!d.flux_entry() || d.face_form())
     throw Unsupported("batched serving runs the per-entry flux kernel (GPU_FLUX_KERNEL = ENTRY)");
   if (!d.have_state) throw std::logic_error("no state set");

@@ -131,6 +131,25 @@ def test_batched_step_equals_the_ordinary_step(precision):
         assert list(rep.resid) == list(hist2[-1])
 
 
+def test_batched_coupled_step_equals_the_ordinary_step():
+    # coupled-loop requests (BACKWARD_EULER_COUPLED) batch as well: the same
+    # bits as their own host-polled solves, beside segregated requests
+    from myjoe_amd import cases
+
+    print("serving: batched coupled", flush=True)
+    built = []
+    for k, r in enumerate(REQS):
+        mm, joe = _case(r, nsteps=25)
+        built.append((joemi.Mesh.from_faces(mm), cases.coupled(joe) if k != 1 else joe, joemi.JM_MIXED))
+    served = _served(built, True, 3)
+    for (state, rep), (mesh, joe, prec) in zip(served, built):
+        (r2, ru2, rE2), n2, hist2 = _alone(mesh, joe, prec)
+        assert rep.steps == n2 == 25
+        for a, b in zip(state, (r2, ru2, rE2)):
+            assert np.array_equal(a, b)
+        assert list(rep.resid) == list(hist2[-1])
+
+
 def test_batched_rejects_what_it_does_not_run():
     # the batched step carries no SA scalar and runs the per-entry flux
     # kernel only: such requests fail at submission, not mid-run
