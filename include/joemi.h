@@ -198,6 +198,11 @@ jm_status jm_run(jm_ctx *c, int max_steps, double *resid_hist, int *steps_done);
 /* boundaryHook bridge (JoeWithModels.h:105): values for a zone whose Joe.in
  * keyword is HOOK, per face of the zone (T[n], vel[n][3], p[n]). */
 jm_status jm_set_boundary_state(jm_ctx *c, int zone, const double *T, const double *vel, const double *p);
+/* the read side of the same bridge: the boundary values setBC last computed
+ * on the zone's faces (T_bfa[n], vel_bfa[n][3], p_bfa[n]; JoeWithModels.cpp
+ * :3857-4300), which the reference hands to boundaryHook (:3872-3881) before
+ * the hook overwrites them. Any pointer may be NULL. Needs a state. */
+jm_status jm_get_boundary_state(jm_ctx *c, int zone, double *T, double *vel, double *p);
 
 /* device views for callers that keep data on the GPU: the conservative state
  * as SoA double[5][ncv] (rho, rhou_x, rhou_y, rhou_z, rhoE) and the residual
@@ -206,6 +211,21 @@ jm_status jm_device_state(jm_ctx *c, double **q_dev);
 jm_status jm_device_rhs(jm_ctx *c, double **rhs_dev);
 /* wait for all work queued on the context's stream */
 jm_status jm_sync(jm_ctx *c);
+
+/* solveCvVectorR5Bcgstab (UgpWithCv.cpp:2660-2876: block-Jacobi BiCGStab,
+ * unpivoted LU of each diagonal block, stop when max |LU^-1 r| <= zero_abs,
+ * or <= zero_rel times its first value, or after max_iter iterations) on a
+ * 5x5 block system the caller holds in DEVICE memory in the reference's
+ * layout: nbocv_i[ncv+1], nbocv_v[nnz] (int32, the diagonal first in each
+ * row), A[nnz][5][5], rhs[ncv][5], phi[ncv][5] (the initial guess in, the
+ * solution out). The reference's callers (JoeWithModels::runBackwardEuler via
+ * solveCoupledLinSysNS, JoeWithModels.cpp:1026) pass dq = 0 as the guess.
+ * Runs on stream (NULL: the default stream) and returns when done; the work
+ * space (block LU, eight vectors, partials) is allocated for the call.
+ * iters / status (1 converged, 2 hit max_iter) may be NULL. */
+jm_status jm_block_bcgstab(int ncv, const int *nbocv_i, const int *nbocv_v, const double *A, const double *rhs,
+                           double *phi, double zero_abs, double zero_rel, int max_iter, void *stream, int *iters,
+                           int *status);
 
 /* Measurement hook, no counterpart in the reference: launches each kernel
  * of one residual evaluation (state, bc, grad, flux, dt) reps times on the

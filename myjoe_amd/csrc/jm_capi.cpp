@@ -395,6 +395,39 @@ jm_status jm_set_boundary_state(jm_ctx *c, int zone, const double *T, const doub
   });
 }
 
+jm_status jm_block_bcgstab(int ncv, const int *nbocv_i, const int *nbocv_v, const double *A, const double *rhs,
+                           double *phi, double zero_abs, double zero_rel, int max_iter, void *stream, int *iters,
+                           int *status) {
+  return guard([&] {
+    jm::BlockSolve bs;
+    bs.ncv = ncv;
+    bs.nbocv_i = nbocv_i;
+    bs.nbocv_v = nbocv_v;
+    bs.A = A;
+    bs.rhs = rhs;
+    bs.phi = phi;
+    bs.zero_abs = zero_abs;
+    bs.zero_rel = zero_rel;
+    bs.max_iter = max_iter;
+    bs.stream = stream;
+    int it = 0, st = 0;
+    jm::block_bcgstab(bs, it, st);
+    if (iters) *iters = it;
+    if (status) *status = st;
+    return JM_OK;
+  });
+}
+
+jm_status jm_get_boundary_state(jm_ctx *c, int zone, double *T, double *vel, double *p) {
+  return guard([&] {
+    if (!c || zone < 0 || zone >= (int)c->mesh->zones.size()) throw std::exception();
+    const jm::Zone &z = c->mesh->zones[zone];
+    if (z.kind != jm::kZoneBoundary) throw std::invalid_argument("zone is not a boundary zone");
+    c->c->get_boundary_state(z.ifa_f, z.ifa_l - z.ifa_f + 1, T, vel, p);
+    return JM_OK;
+  });
+}
+
 jm_status jm_device_state(jm_ctx *c, double **q) {
   return guard([&] {
     if (!c || !q) throw std::exception();

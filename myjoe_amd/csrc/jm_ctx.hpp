@@ -121,6 +121,22 @@ void run_krylov_batch(int precision, void *stream, KrylovBatchBuf &buf);
 
 class Ctx;
 
+// A 5x5 block system on the device in the reference's layout (UgpWithCv
+// members nbocv_i[ncv+1], nbocv_v[nnz] with the diagonal first, A[nnz][5][5],
+// rhs[ncv][5], phi[ncv][5] the initial guess and solution) and the stop rule
+// of solveCvVectorR5Bcgstab.
+struct BlockSolve {
+  int ncv = 0;
+  const int *nbocv_i = nullptr, *nbocv_v = nullptr;
+  const double *A = nullptr, *rhs = nullptr;
+  double *phi = nullptr;
+  double zero_abs = 1e-8, zero_rel = 1e-2;
+  int max_iter = 30;
+  void *stream = nullptr;
+};
+// solveCvVectorR5Bcgstab on it with a work space allocated for the call
+void block_bcgstab(const BlockSolve &bs, int &iters, int &status);
+
 // Solves of one kernel configuration (equal Ctx::batch_key) stepping in
 // lockstep for batched serving: each kernel of the step before the linear
 // solve (CFL ramp, gradients, residual and local time step, Jacobian, solver
@@ -211,6 +227,8 @@ class Ctx {
   // doneSolver (JoeWithModels.h:42-95) after a step, plus RESID_ENERG_REL
   bool done_solver(const StepInfo &s);
   void set_boundary_state(int ifa_f, int nface, const double *T, const double *vel, const double *p);
+  // the boundary values setBC last computed on a face range (T_bfa, vel_bfa, p_bfa)
+  void get_boundary_state(int ifa_f, int nface, double *T, double *vel, double *p);
   double *device_state();
   double *device_rhs();
   void sync();

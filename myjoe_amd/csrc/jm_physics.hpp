@@ -751,23 +751,25 @@ __device__ __forceinline__ void jac_only_pressure(double J[5][5], double rhoL, c
   for (int j = 0; j < 5; j++) J[4][j] = 0.0;
 }
 
-// ludeco / lusolv on a 5x5 block, UgpWithCv.cpp:2489-2540 (T = double or float)
-template <typename T>
-__device__ __forceinline__ void ludeco5(T A[5][5]) {
+// ludeco / lusolv on an NB x NB block (UgpWithCv.cpp:2489-2540; NB = 5 for
+// the flow equations, 5 + nScal in the coupled loop; T = double or float),
+// in the reference's loop order
+template <typename T, int NB>
+__device__ __forceinline__ void ludeco_n(T A[NB][NB]) {
 #pragma unroll
-  for (int jc = 1; jc < 5; jc++) A[0][jc] /= A[0][0];
+  for (int jc = 1; jc < NB; jc++) A[0][jc] /= A[0][0];
 #pragma unroll
-  for (int jrjc = 1; jrjc < 5; jrjc++) {
+  for (int jrjc = 1; jrjc < NB; jrjc++) {
 #pragma unroll
-    for (int jr = jrjc; jr < 5; jr++) {
+    for (int jr = jrjc; jr < NB; jr++) {
       T sum = A[jr][jrjc];
 #pragma unroll
       for (int jm = 0; jm < jrjc; jm++) sum -= A[jr][jm] * A[jm][jrjc];
       A[jr][jrjc] = sum;
     }
-    if (jrjc == 4) break;
+    if (jrjc == NB - 1) break;
 #pragma unroll
-    for (int jc = jrjc + 1; jc < 5; jc++) {
+    for (int jc = jrjc + 1; jc < NB; jc++) {
       T sum = A[jrjc][jc];
 #pragma unroll
       for (int jm = 0; jm < jrjc; jm++) sum -= A[jrjc][jm] * A[jm][jc];
@@ -776,27 +778,37 @@ __device__ __forceinline__ void ludeco5(T A[5][5]) {
   }
 }
 
-template <typename T>
-__device__ __forceinline__ void lusolv5(const T *A /* [25] */, T c[5]) {
+template <typename T, int NB>
+__device__ __forceinline__ void lusolv_n(const T *A /* [NB * NB] */, T c[NB]) {
   c[0] = c[0] / A[0];
 #pragma unroll
-  for (int jr = 1; jr < 5; jr++) {
+  for (int jr = 1; jr < NB; jr++) {
     T sum = c[jr];
 #pragma unroll
-    for (int jm = 0; jm < jr; jm++) sum -= A[5 * jr + jm] * c[jm];
-    c[jr] = sum / A[5 * jr + jr];
+    for (int jm = 0; jm < jr; jm++) sum -= A[NB * jr + jm] * c[jm];
+    c[jr] = sum / A[NB * jr + jr];
   }
 #pragma unroll
-  for (int jrjr = 1; jrjr < 5; jrjr++) {
-    int jr = 4 - jrjr;
+  for (int jrjr = 1; jrjr < NB; jrjr++) {
+    int jr = (NB - 1) - jrjr;
     T sum = c[jr];
 #pragma unroll
-    for (int jmjm = jr + 1; jmjm < 5; jmjm++) {
-      int jm = 4 - jmjm + jr + 1;
-      sum -= A[5 * jr + jm] * c[jm];
+    for (int jmjm = jr + 1; jmjm < NB; jmjm++) {
+      int jm = (NB - 1) - jmjm + jr + 1;
+      sum -= A[NB * jr + jm] * c[jm];
     }
     c[jr] = sum;
   }
+}
+
+template <typename T>
+__device__ __forceinline__ void ludeco5(T A[5][5]) {
+  ludeco_n<T, 5>(A);
+}
+
+template <typename T>
+__device__ __forceinline__ void lusolv5(const T *A /* [25] */, T c[5]) {
+  lusolv_n<T, 5>(A, c);
 }
 
 }  // namespace jm

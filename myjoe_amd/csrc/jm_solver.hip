@@ -106,6 +106,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kRedSlots = 8;  // partial-sum slots per block for fused reductions
+inline int nblk(long long n, int b = kBlock) { return (int)((n + b - 1) / b); }
 
 // A block's index and the grid size within one solve's launch: the hardware
 // values for an ordinary launch, the solve's own share of a batched one.
@@ -2586,69 +2587,77 @@ __global__ void k_publish(const SolverScalars *S, StepMail *out) {
 }
 
 // ---------------------------------------------------------------------------
-// block-Jacobi BiCGStab, UgpWithCv.cpp:2660-2876. Vectors are [ncv][5]; the
+// block-Jacobi BiCGStab, UgpWithCv.cpp:2660-2876, on NB x NB blocks (NB = 5,
+// or 5 + nScal in the coupled loop). Vectors are [ncv][NB]; the
 // matrix-vector products keep the reference's per-row order (blocks in
-// nbocv order, columns ascending) using five lanes per cell.
+// nbocv order, columns ascending) using NB lanes per cell.
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, int NB>
 __global__ void __launch_bounds__(kBlock) k_lu(int ncv, const int *__restrict__ nbocv_i, const T *__restrict__ A,
                                                T *__restrict__ LU) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncv) return;
-  T a[5][5];
-  const T *s = A + 25 * (size_t)nbocv_i[c];
-  for (int i = 0; i < 5; i++)
-    for (int j = 0; j < 5; j++) a[i][j] = s[5 * i + j];
-  ludeco5<T>(a);
-  T *d = LU + 25 * (size_t)c;
-  for (int i = 0; i < 5; i++)
-    for (int j = 0; j < 5; j++) d[5 * i + j] = a[i][j];
+  T a[NB][NB];
+  const T *s = A + NB * NB * (size_t)nbocv_i[c];
+  for (int i = 0; i < NB; i++)
+    for (int j = 0; j < NB; j++) a[i][j] = s[NB * i + j];
+  ludeco_n<T, NB>(a);
+  T *d = LU + NB * NB * (size_t)c;
+  for (int i = 0; i < NB; i++)
+    for (int j = 0; j < NB; j++) d[NB * i + j] = a[i][j];
 }
 
-// y = A x for row component (c, i) = (tid / 5, tid % 5)
-template <typename T>
+// y = A x for row component (c, i) = (tid / NB, tid % NB)
+template <typename T, int NB>
 __device__ __forceinline__ T spmv_row(const int *__restrict__ nbocv_i, const int *__restrict__ nbocv_v,
                                       const T *__restrict__ A, const T *__restrict__ x, int c, int i) {
   T acc = (T)0.0;
   for (int noc = nbocv_i[c]; noc < nbocv_i[c + 1]; noc++) {
-    const T *a = A + 25 * (size_t)noc + 5 * i;
-    const T *xv = x + 5 * (size_t)nbocv_v[noc];
-    for (int j = 0; j < 5; j++) acc += a[j] * xv[j];
+    const T *a = A + NB * NB * (size_t)noc + NB * i;
+    const T *xv = x + NB * (size_t)nbocv_v[noc];
+    for (int j = 0; j < NB; j++) acc += a[j] * xv[j];
   }
   return acc;
 }
 
-constexpr int kCellsPerBlock = kBlock / 5;  // 51 cells, 255 lanes active
+// cells per block of the NB-lanes-per-cell kernels (5: 51 cells, 255 lanes)
+template <int NB>
+constexpr int cells_per_block() {
+  return kBlock / NB;
+}
+constexpr int kCellsPerBlock = cells_per_block<5>();
 
 // res = rhs - A phi; init: res0 = res. Partials: res.res0 (slot 0) and
 // max |LU^-1 res| (slot 1), finished by k_fin kFinResid0 / kFinResid.
-template <typename T>
+template <typename T, int NB>
 __global__ void __launch_bounds__(kBlock) k_resid(int ncv, const int *__restrict__ nbocv_i,
                                                   const int *__restrict__ nbocv_v, const T *__restrict__ A,
                                                   const T *__restrict__ LU, const T *__restrict__ phi,
                                                   const T *__restrict__ rhs, T *__restrict__ res, T *__restrict__ res0,
                                                   int init, double *partials, const SolverScalars *S) {
+  constexpr int CPB = cells_per_block<NB>();
   __shared__ double sh[32];
-  __shared__ T cellv[kCellsPerBlock][5];
+  __shared__ T cellv[CPB][NB];
   if (!init && (S->done || S->stop)) return;
-  const int lc = threadIdx.x / 5, i = threadIdx.x % 5;
-  const int c = blockIdx.x * kCellsPerBlock + lc;
-  const bool active = (lc < kCellsPerBlock) && (c < ncv);
+  const int lc = threadIdx.x / NB, i = threadIdx.x % NB;
+  const int c = blockIdx.x * CPB + lc;
+  const bool active = (lc < CPB) && (c < ncv);
   T r = (T)0.0;
   if (active) {
-    T ax = spmv_row<T>(nbocv_i, nbocv_v, A, phi, c, i);
-    r = rhs[5 * (size_t)c + i] - ax;
-    res[5 * (size_t)c + i] = r;
-    if (init) res0[5 * (size_t)c + i] = r;
+    T ax = spmv_row<T, NB>(nbocv_i, nbocv_v, A, phi, c, i);
+    r = rhs[NB * (size_t)c + i] - ax;
+    res[NB * (size_t)c + i] = r;
+    if (init) res0[NB * (size_t)c + i] = r;
     cellv[lc][i] = r;
   }
   __syncthreads();
   double vs[1] = {0.0}, vm[1] = {0.0};
   if (active) {
-    T r0v = init ? r : res0[5 * (size_t)c + i];
+    T r0v = init ? r : res0[NB * (size_t)c + i];
     vs[0] = (double)r * (double)r0v;
-    T tmp[5] = {cellv[lc][0], cellv[lc][1], cellv[lc][2], cellv[lc][3], cellv[lc][4]};
-    lusolv5<T>(LU + 25 * (size_t)c, tmp);
+    T tmp[NB];
+    for (int k = 0; k < NB; k++) tmp[k] = cellv[lc][k];
+    lusolv_n<T, NB>(LU + NB * NB * (size_t)c, tmp);
     vm[0] = fabs((double)tmp[i]);
   }
   block_reduce<1>(vs, sh, kSum);
@@ -2661,7 +2670,7 @@ __global__ void __launch_bounds__(kBlock) k_resid(int ncv, const int *__restrict
 }
 
 // p = res - beta (p - omega v); phat = LU^-1 p   (one thread per cell)
-template <typename T>
+template <typename T, int NB>
 __global__ void __launch_bounds__(kBlock) k_p(int ncv, const T *__restrict__ LU, const T *__restrict__ res,
                                               const T *__restrict__ v, T *__restrict__ p, T *__restrict__ phat,
                                               const SolverScalars *S) {
@@ -2669,34 +2678,35 @@ __global__ void __launch_bounds__(kBlock) k_p(int ncv, const T *__restrict__ LU,
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncv) return;
   const T beta = (T)S->beta, omega = (T)S->omega;
-  T t[5];
-  for (int k = 0; k < 5; k++) {
-    size_t o = 5 * (size_t)c + k;
+  T t[NB];
+  for (int k = 0; k < NB; k++) {
+    size_t o = NB * (size_t)c + k;
     T pv = res[o] - beta * (p[o] - omega * v[o]);
     p[o] = pv;
     t[k] = pv;
   }
-  lusolv5<T>(LU + 25 * (size_t)c, t);
-  for (int k = 0; k < 5; k++) phat[5 * (size_t)c + k] = t[k];
+  lusolv_n<T, NB>(LU + NB * NB * (size_t)c, t);
+  for (int k = 0; k < NB; k++) phat[NB * (size_t)c + k] = t[k];
 }
 
 // y = A x with partial dots (y.other, y.y): mode 0 gives gamma = v.res0
 // (k_fin kFinAlpha), mode 1 gives (s.t, t.t) (k_fin kFinOmega)
-template <typename T>
+template <typename T, int NB>
 __global__ void __launch_bounds__(kBlock) k_spmv_dot(int ncv, const int *__restrict__ nbocv_i,
                                                      const int *__restrict__ nbocv_v, const T *__restrict__ A,
                                                      const T *__restrict__ x, T *__restrict__ y,
                                                      const T *__restrict__ other, double *partials,
                                                      const SolverScalars *S) {
+  constexpr int CPB = cells_per_block<NB>();
   __shared__ double sh[32];
   if (S->done || S->stop) return;
-  const int lc = threadIdx.x / 5, i = threadIdx.x % 5;
-  const int c = blockIdx.x * kCellsPerBlock + lc;
+  const int lc = threadIdx.x / NB, i = threadIdx.x % NB;
+  const int c = blockIdx.x * CPB + lc;
   double v[2] = {0.0, 0.0};
-  if (lc < kCellsPerBlock && c < ncv) {
-    T yv = spmv_row<T>(nbocv_i, nbocv_v, A, x, c, i);
-    y[5 * (size_t)c + i] = yv;
-    double o = (double)other[5 * (size_t)c + i];
+  if (lc < CPB && c < ncv) {
+    T yv = spmv_row<T, NB>(nbocv_i, nbocv_v, A, x, c, i);
+    y[NB * (size_t)c + i] = yv;
+    double o = (double)other[NB * (size_t)c + i];
     v[0] = (double)yv * o;
     v[1] = (double)yv * (double)yv;
   }
@@ -2704,7 +2714,7 @@ __global__ void __launch_bounds__(kBlock) k_spmv_dot(int ncv, const int *__restr
 }
 
 // s = res - alpha v; shat = LU^-1 s
-template <typename T>
+template <typename T, int NB>
 __global__ void __launch_bounds__(kBlock) k_s(int ncv, const T *__restrict__ LU, const T *__restrict__ res,
                                               const T *__restrict__ v, T *__restrict__ s, T *__restrict__ shat,
                                               const SolverScalars *S) {
@@ -2712,15 +2722,15 @@ __global__ void __launch_bounds__(kBlock) k_s(int ncv, const T *__restrict__ LU,
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncv) return;
   const T alpha = (T)S->alpha;
-  T t[5];
-  for (int k = 0; k < 5; k++) {
-    size_t o = 5 * (size_t)c + k;
+  T t[NB];
+  for (int k = 0; k < NB; k++) {
+    size_t o = NB * (size_t)c + k;
     T sv = res[o] - alpha * v[o];
     s[o] = sv;
     t[k] = sv;
   }
-  lusolv5<T>(LU + 25 * (size_t)c, t);
-  for (int k = 0; k < 5; k++) shat[5 * (size_t)c + k] = t[k];
+  lusolv_n<T, NB>(LU + NB * NB * (size_t)c, t);
+  for (int k = 0; k < NB; k++) shat[NB * (size_t)c + k] = t[k];
 }
 
 // phi += alpha phat + omega shat
@@ -2731,6 +2741,88 @@ __global__ void __launch_bounds__(kBlock) k_x(int n5, T *__restrict__ phi, const
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n5) return;
   phi[k] += (T)S->alpha * phat[k] + (T)S->omega * shat[k];
+}
+
+// One block system and its work space for the kernels above: A[nnz][NB][NB]
+// in nbocv order, LU[ncv][NB][NB], ten vectors [ncv][NB] (v[0] rhs, v[1] phi
+// (the solution), v[2] res, v[3] res0, v[4] p, v[5] v, v[6] phat, v[7] s,
+// v[8] shat, v[9] t), reduction partials and the solver scalars.
+struct BcgSys {
+  int ncv = 0, maxiter = 30;
+  double zero_abs = 1e-8, zero_rel = 1e-2;
+  const int *nbocv_i = nullptr, *nbocv_v = nullptr;
+  const void *A = nullptr;
+  void *LU = nullptr;
+  void *v[10] = {};
+  double *partials = nullptr;
+  SolverScalars *S = nullptr;
+  hipStream_t stream = nullptr;
+};
+
+// start of a solve (UgpWithCv.cpp:2660-2735): p = v = 0, block LU of the
+// diagonal, the first residual and its stop test
+template <typename T, int NB>
+void bcg_prologue(const BcgSys &y) {
+  const int ncv = y.ncv, n = NB * ncv;
+  const T *A = (const T *)y.A;
+  T *LU = (T *)y.LU;
+  T *rhs_ = (T *)y.v[0], *phi = (T *)y.v[1], *res = (T *)y.v[2], *res0 = (T *)y.v[3], *p = (T *)y.v[4],
+    *vv = (T *)y.v[5];
+  hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n)), dim3(kBlock), 0, y.stream, (size_t)n, p, (T)0);
+  hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n)), dim3(kBlock), 0, y.stream, (size_t)n, vv, (T)0);
+  hipLaunchKernelGGL(k_solver_init, dim3(1), dim3(64), 0, y.stream, y.S, y.maxiter, y.zero_abs, y.zero_rel);
+  hipLaunchKernelGGL((k_lu<T, NB>), dim3(nblk(ncv)), dim3(kBlock), 0, y.stream, ncv, y.nbocv_i, A, LU);
+  const int gb = (ncv + cells_per_block<NB>() - 1) / cells_per_block<NB>();
+  hipLaunchKernelGGL((k_resid<T, NB>), dim3(gb), dim3(kBlock), 0, y.stream, ncv, y.nbocv_i, y.nbocv_v, A, LU, phi,
+                     rhs_, res, res0, 1, y.partials, y.S);
+  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, y.stream, gb, (int)kFinResid0, y.partials, y.S);
+}
+
+// count BiCGStab iterations (van der Vorst, UgpWithCv.cpp:2725-2856); each
+// returns at once once the device has set the convergence or stop flag
+template <typename T, int NB>
+void bcg_iterations(const BcgSys &y, int count) {
+  const int ncv = y.ncv, n = NB * ncv;
+  const T *A = (const T *)y.A, *LU = (const T *)y.LU;
+  T *rhs_ = (T *)y.v[0], *phi = (T *)y.v[1], *res = (T *)y.v[2], *res0 = (T *)y.v[3], *p = (T *)y.v[4],
+    *vv = (T *)y.v[5], *phat = (T *)y.v[6], *s = (T *)y.v[7], *shat = (T *)y.v[8], *t = (T *)y.v[9];
+  const int gb = (ncv + cells_per_block<NB>() - 1) / cells_per_block<NB>();
+  hipStream_t st = y.stream;
+  for (int k = 0; k < count; k++) {
+    hipLaunchKernelGGL((k_p<T, NB>), dim3(nblk(ncv)), dim3(kBlock), 0, st, ncv, LU, res, vv, p, phat, y.S);
+    hipLaunchKernelGGL((k_spmv_dot<T, NB>), dim3(gb), dim3(kBlock), 0, st, ncv, y.nbocv_i, y.nbocv_v, A, phat, vv,
+                       res0, y.partials, y.S);
+    hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, st, gb, (int)kFinAlpha, y.partials, y.S);
+    hipLaunchKernelGGL((k_s<T, NB>), dim3(nblk(ncv)), dim3(kBlock), 0, st, ncv, LU, res, vv, s, shat, y.S);
+    hipLaunchKernelGGL((k_spmv_dot<T, NB>), dim3(gb), dim3(kBlock), 0, st, ncv, y.nbocv_i, y.nbocv_v, A, shat, t, s,
+                       y.partials, y.S);
+    hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, st, gb, (int)kFinOmega, y.partials, y.S);
+    hipLaunchKernelGGL(k_x<T>, dim3(nblk(n)), dim3(kBlock), 0, st, n, phi, phat, shat, y.S);
+    hipLaunchKernelGGL((k_resid<T, NB>), dim3(gb), dim3(kBlock), 0, st, ncv, y.nbocv_i, y.nbocv_v, A, LU, phi, rhs_,
+                       res, res0, 0, y.partials, y.S);
+    hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, st, gb, (int)kFinResid, y.partials, y.S);
+  }
+}
+
+// a whole solve, host-polled: the flag is read between chunks (2, then 4
+// iterations) into hs (any host memory: the copy is waited for); iterations
+// past convergence are no-ops, so the chunking does not change the result
+template <typename T, int NB>
+void bcg_solve(const BcgSys &y, SolverScalars *hs, int &iters, int &status) {
+  bcg_prologue<T, NB>(y);
+  const int max_it = y.maxiter + 1;
+  int launched = 0;
+  while (launched < max_it) {
+    int chunk = std::min(launched == 0 ? 2 : 4, max_it - launched);
+    bcg_iterations<T, NB>(y, chunk);
+    launched += chunk;
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(hs, y.S, sizeof(SolverScalars), hipMemcpyDeviceToHost, y.stream));
+    HIPCHECK(hipStreamSynchronize(y.stream));
+    if (hs->done) break;
+  }
+  iters = hs->iter;
+  status = hs->done;
 }
 
 // ---------------------------------------------------------------------------
@@ -2776,7 +2868,7 @@ __global__ void __launch_bounds__(kBlock) kb_resid(const KrylovJob *__restrict__
   const bool active = (lc < kCellsPerBlock) && (c < J.ncv);
   T r = (T)0.0;
   if (active) {
-    T ax = spmv_row<T>(J.nbocv_i, J.nbocv_v, A, phi, c, i);
+    T ax = spmv_row<T, 5>(J.nbocv_i, J.nbocv_v, A, phi, c, i);
     r = rhs[5 * (size_t)c + i] - ax;
     res[5 * (size_t)c + i] = r;
     if (init) res0[5 * (size_t)c + i] = r;
@@ -2861,7 +2953,7 @@ __global__ void __launch_bounds__(kBlock) kb_spmv_dot(const KrylovJob *__restric
   const int c = (blockIdx.x - J.b5) * kCellsPerBlock + lc;
   double v[2] = {0.0, 0.0};
   if (lc < kCellsPerBlock && c < J.ncv) {
-    T yv = spmv_row<T>(J.nbocv_i, J.nbocv_v, (const T *)J.A, x, c, i);
+    T yv = spmv_row<T, 5>(J.nbocv_i, J.nbocv_v, (const T *)J.A, x, c, i);
     y[5 * (size_t)c + i] = yv;
     double o = (double)other[5 * (size_t)c + i];
     v[0] = (double)yv * o;
@@ -3262,7 +3354,6 @@ __global__ void __launch_bounds__(kBlock) k_wall_dist(int ncv, const double *__r
   aux[kAuxN * (size_t)c + kAuxWall] = dTang2 > specRadius2 ? minDist : fabs(dot3(s_half, n));
 }
 
-inline int nblk(long long n, int b = kBlock) { return (int)((n + b - 1) / b); }
 
 template <typename T>
 struct DBuf {
@@ -3612,6 +3703,8 @@ struct Ctx::Impl {
   void solve_iterations(int count);
   template <typename T>
   DBuf<T> *krylov();
+  template <typename T>
+  BcgSys bcg_sys();
   template <typename T>
   DBuf<T> *lu_();
   // device-driven step in three captured graphs: head (CFL ramp, residual,
@@ -4877,70 +4970,39 @@ void Ctx::Impl::launch_jac(bool step_mode) {
 
 // Work vectors: v[0] rhs, v[1] phi (= dq), v[2] res, v[3] res0, v[4] p,
 // v[5] v, v[6] phat, v[7] s, v[8] shat, v[9] t.
+// the context's block system as the BiCGStab kernels take it
+template <typename T>
+BcgSys Ctx::Impl::bcg_sys() {
+  BcgSys y;
+  y.ncv = ncv;
+  y.maxiter = par.ls_maxiter;
+  y.zero_abs = par.ls_abs;
+  y.zero_rel = par.ls_rel;
+  y.nbocv_i = nbocv_i.p;
+  y.nbocv_v = nbocv_v.p;
+  y.A = A_<T>();
+  y.LU = lu_<T>()->p;
+  DBuf<T> *v = krylov<T>();
+  for (int k = 0; k < 10; k++) y.v[k] = v[k].p;
+  y.partials = partials.p;
+  y.S = scal.p;
+  y.stream = stream;
+  return y;
+}
+
 template <typename T>
 void Ctx::Impl::solve_prologue() {
-  DBuf<T> *v = krylov<T>();
-  DBuf<T> *LU = lu_<T>();
-  T *A = A_<T>();
-  const int n5 = 5 * ncv;
-  T *rhs_ = v[0].p, *phi = v[1].p, *res = v[2].p, *res0 = v[3].p, *p = v[4].p, *vv = v[5].p;
-  hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n5)), dim3(kBlock), 0, stream, (size_t)n5, p, (T)0);
-  hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n5)), dim3(kBlock), 0, stream, (size_t)n5, vv, (T)0);
-  hipLaunchKernelGGL(k_solver_init, dim3(1), dim3(64), 0, stream, scal.p, par.ls_maxiter, par.ls_abs, par.ls_rel);
-  hipLaunchKernelGGL(k_lu<T>, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, nbocv_i.p, A, LU->p);
-  const int gb5 = (ncv + kCellsPerBlock - 1) / kCellsPerBlock;
-  hipLaunchKernelGGL(k_resid<T>, dim3(gb5), dim3(kBlock), 0, stream, ncv, nbocv_i.p, nbocv_v.p, A, LU->p, phi, rhs_,
-                     res, res0, 1, partials.p, scal.p);
-  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, gb5, (int)kFinResid0, partials.p, scal.p);
+  bcg_prologue<T, 5>(bcg_sys<T>());
 }
 
-// n BiCGStab iterations (van der Vorst, UgpWithCv.cpp:2725-2856); each
-// returns at once once the device has set the convergence or stop flag
 template <typename T>
 void Ctx::Impl::solve_iterations(int count) {
-  DBuf<T> *v = krylov<T>();
-  DBuf<T> *LU = lu_<T>();
-  T *A = A_<T>();
-  const int n5 = 5 * ncv;
-  T *rhs_ = v[0].p, *phi = v[1].p, *res = v[2].p, *res0 = v[3].p, *p = v[4].p, *vv = v[5].p, *phat = v[6].p,
-    *s = v[7].p, *shat = v[8].p, *t = v[9].p;
-  const int gb5 = (ncv + kCellsPerBlock - 1) / kCellsPerBlock;
-  for (int k = 0; k < count; k++) {
-    hipLaunchKernelGGL(k_p<T>, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, LU->p, res, vv, p, phat, scal.p);
-    hipLaunchKernelGGL(k_spmv_dot<T>, dim3(gb5), dim3(kBlock), 0, stream, ncv, nbocv_i.p, nbocv_v.p, A, phat, vv,
-                       res0, partials.p, scal.p);
-    hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, gb5, (int)kFinAlpha, partials.p, scal.p);
-    hipLaunchKernelGGL(k_s<T>, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, LU->p, res, vv, s, shat, scal.p);
-    hipLaunchKernelGGL(k_spmv_dot<T>, dim3(gb5), dim3(kBlock), 0, stream, ncv, nbocv_i.p, nbocv_v.p, A, shat, t, s,
-                       partials.p, scal.p);
-    hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, gb5, (int)kFinOmega, partials.p, scal.p);
-    hipLaunchKernelGGL(k_x<T>, dim3(nblk(n5)), dim3(kBlock), 0, stream, n5, phi, phat, shat, scal.p);
-    hipLaunchKernelGGL(k_resid<T>, dim3(gb5), dim3(kBlock), 0, stream, ncv, nbocv_i.p, nbocv_v.p, A, LU->p, phi,
-                       rhs_, res, res0, 0, partials.p, scal.p);
-    hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, gb5, (int)kFinResid, partials.p, scal.p);
-  }
+  bcg_iterations<T, 5>(bcg_sys<T>(), count);
 }
 
-// Work vectors: v[0] rhs, v[1] phi (= dq), v[2] res, v[3] res0, v[4] p,
-// v[5] v, v[6] phat, v[7] s, v[8] shat, v[9] t. The host polls the flag
-// between chunks (2, then 4 iterations); iterations past convergence are
-// no-ops, so the chunking does not change the result.
 template <typename T>
 void Ctx::Impl::solve(int &iters, int &status) {
-  solve_prologue<T>();
-  const int max_it = par.ls_maxiter + 1;
-  int launched = 0;
-  SolverScalars &hs = *h_scal;
-  while (launched < max_it) {
-    int chunk = std::min(launched == 0 ? 2 : 4, max_it - launched);
-    solve_iterations<T>(chunk);
-    launched += chunk;
-    HIPCHECK(hipMemcpyAsync(h_scal, scal.p, sizeof(SolverScalars), hipMemcpyDeviceToHost, stream));
-    HIPCHECK(hipStreamSynchronize(stream));
-    if (hs.done) break;
-  }
-  iters = hs.iter;
-  status = hs.done;
+  bcg_solve<T, 5>(bcg_sys<T>(), h_scal, iters, status);
 }
 
 template <typename T>
@@ -5247,6 +5309,24 @@ void Ctx::set_boundary_state(int ifa_f, int nface, const double *T, const double
   HIPCHECK(hipStreamSynchronize(d.stream));
 }
 
+void Ctx::get_boundary_state(int ifa_f, int nface, double *T, double *vel, double *p) {
+  Impl &d = *d_;
+  if (ifa_f < 0 || nface < 0 || ifa_f + nface > d.nfa_b) throw std::out_of_range("boundary face range");
+  if (!d.have_state) throw std::logic_error("no state set");
+  std::vector<double> h(8 * (size_t)nface);
+  if (nface)
+    HIPCHECK(hipMemcpyAsync(h.data(), d.bfa.p + 8 * (size_t)ifa_f, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                            d.stream));
+  HIPCHECK(hipStreamSynchronize(d.stream));
+  for (int k = 0; k < nface; k++) {  // bfa rows: rho u v w p T h phi
+    const double *b = &h[8 * (size_t)k];
+    if (T) T[k] = b[5];
+    if (vel)
+      for (int i = 0; i < 3; i++) vel[3 * k + i] = b[1 + i];
+    if (p) p[k] = b[4];
+  }
+}
+
 double *Ctx::device_state() { return d_->q.p; }
 double *Ctx::device_rhs() { return d_->rhs.p; }
 void Ctx::sync() { HIPCHECK(hipStreamSynchronize(d_->stream)); }
@@ -5385,6 +5465,64 @@ int64_t Ctx::get(const std::string &name, double *out) {
     return (int64_t)n;
   }
   throw std::invalid_argument("unknown array " + name);
+}
+
+// ---------------------------------------------------------------------------
+// solveCvVectorR5Bcgstab (UgpWithCv.cpp:2660-2876) on a 5x5 block system the
+// caller holds on the device, with the same kernels and launch sequence as
+// the context's own linear solve. The work space (block LU, eight vectors,
+// partials, solver scalars) is carved from one device buffer.
+// ---------------------------------------------------------------------------
+namespace {
+// doubles of work space a solve of ncv cells takes
+size_t bcg_workspace_doubles(int ncv) {
+  const size_t n = (size_t)ncv;
+  const size_t gb = (n + kCellsPerBlock - 1) / kCellsPerBlock;
+  const size_t scal = (sizeof(SolverScalars) + 7) / 8;
+  return 25 * n + 8 * 5 * n + 2 * (gb + 64) + scal + 32;
+}
+
+BcgSys bcg_on_workspace(const BlockSolve &bs, double *ws, hipStream_t stream) {
+  const size_t n = (size_t)bs.ncv;
+  const size_t gb = (n + kCellsPerBlock - 1) / kCellsPerBlock;
+  BcgSys y;
+  y.ncv = bs.ncv;
+  y.maxiter = bs.max_iter;
+  y.zero_abs = bs.zero_abs;
+  y.zero_rel = bs.zero_rel;
+  y.nbocv_i = bs.nbocv_i;
+  y.nbocv_v = bs.nbocv_v;
+  y.A = bs.A;
+  y.v[0] = (void *)bs.rhs;
+  y.v[1] = bs.phi;
+  size_t o = 0;
+  y.S = (SolverScalars *)(ws + o);  // first, so it keeps the buffer's alignment
+  o += (sizeof(SolverScalars) + 7) / 8 + 32;
+  y.LU = ws + o;
+  o += 25 * n;
+  for (int k = 2; k < 10; k++) {
+    y.v[k] = ws + o;
+    o += 5 * n;
+  }
+  y.partials = ws + o;
+  o += 2 * (gb + 64);
+  y.stream = stream;
+  return y;
+}
+
+void bcg_check(const BlockSolve &bs) {
+  if (bs.ncv <= 0 || !bs.nbocv_i || !bs.nbocv_v || !bs.A || !bs.rhs || !bs.phi || bs.max_iter < 0)
+    throw std::invalid_argument("block BiCGStab: ncv > 0, device arrays and max_iter >= 0 needed");
+}
+}  // namespace
+
+void block_bcgstab(const BlockSolve &bs, int &iters, int &status) {
+  bcg_check(bs);
+  hipStream_t st = (hipStream_t)bs.stream;
+  DBuf<double> ws;
+  ws.alloc(bcg_workspace_doubles(bs.ncv));
+  SolverScalars hs;
+  bcg_solve<double, 5>(bcg_on_workspace(bs, ws.p, st), &hs, iters, status);
 }
 
 }  // namespace jm

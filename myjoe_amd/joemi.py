@@ -73,6 +73,8 @@ def lib():
         L.jm_step_graph.argtypes = [vp, vp]
         L.jm_run.argtypes = [vp, C.c_int, vp, vp]
         L.jm_set_boundary_state.argtypes = [vp, C.c_int, vp, vp, vp]
+        L.jm_get_boundary_state.argtypes = [vp, C.c_int, vp, vp, vp]
+        L.jm_block_bcgstab.argtypes = [C.c_int, vp, vp, vp, vp, vp, C.c_double, C.c_double, C.c_int, vp, vp, vp]
         L.jm_device_state.argtypes = [vp, vp]
         L.jm_device_rhs.argtypes = [vp, vp]
         L.jm_sync.argtypes = [vp]
@@ -100,6 +102,25 @@ def _ptr(a):
     return a.ctypes.data
 
 
+class UgpView(C.Structure):
+    """jm_ugp_view: the Ugp / UgpWithCv arrays of a host that already holds them."""
+    _fields_ = [("ncv", C.c_int), ("nfa", C.c_int), ("nfa_b", C.c_int)] + \
+        [(n, C.c_void_p) for n in ("cvofa", "faocv_i", "faocv_v", "nbocv_i", "nbocv_v", "x_fa", "fa_normal",
+                                   "x_cv", "cv_volume", "nbocv_lsg", "bfa_lsg")] + \
+        [("nzone", C.c_int)] + [(n, C.c_void_p) for n in ("zone_names", "zone_kind", "zone_ifa_f", "zone_ifa_l")]
+
+
+def block_bcgstab(nbocv_i, nbocv_v, A, rhs, phi, zero_abs=1e-8, zero_rel=1e-2, max_iter=30, stream=None):
+    """jm_block_bcgstab on device arrays (torch tensors on the GPU: int32
+    nbocv_i[ncv+1], nbocv_v[nnz]; float64 A[nnz][5][5], rhs[ncv][5], phi[ncv][5],
+    phi overwritten with the solution). Returns (iterations, status)."""
+    ncv = rhs.numel() // 5
+    it, st = C.c_int(), C.c_int()
+    _check(lib().jm_block_bcgstab(ncv, nbocv_i.data_ptr(), nbocv_v.data_ptr(), A.data_ptr(), rhs.data_ptr(),
+                                  phi.data_ptr(), zero_abs, zero_rel, max_iter, stream, C.byref(it), C.byref(st)))
+    return it.value, st.value
+
+
 class Mesh:
     """A mesh in the reference's numbering, built by libjoemi."""
 
@@ -122,6 +143,31 @@ class Mesh:
         names = (C.c_char_p * len(mm.zones))(*[z[2].encode() for z in mm.zones])
         _check(lib().jm_mesh_from_faces(mm.nno, _ptr(x), mm.nfa, _ptr(fp), _ptr(fn), _ptr(fc), _ptr(fz), mm.ncv,
                                         len(mm.zones), _ptr(ids), C.cast(names, C.c_void_p), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_ugp(cls, d: dict):
+        """From the reference's own arrays (dump names as in as_dump: CVOFA,
+        FAOCV_I/V, NBOCV_I/V, X_FA, FA_NORMAL, X_CV, CV_VOLUME, NBOCV_LSG,
+        BFA_LSG, COUNTS, ZONES, ZONE_NAMES), taken over without renumbering."""
+        cnt = np.asarray(d["COUNTS"]).reshape(-1)
+        nfa, nfa_b, ncv = int(cnt[1]), int(cnt[2]), int(cnt[3])
+        keep = {k: np.ascontiguousarray(np.asarray(d[k]).reshape(-1), np.int32)
+                for k in ("CVOFA", "FAOCV_I", "FAOCV_V", "NBOCV_I", "NBOCV_V")}
+        keep.update({k: np.ascontiguousarray(np.asarray(d[k]).reshape(-1)[:3 * ncv if k == "X_CV" else None],
+                                             np.float64)
+                     for k in ("X_FA", "FA_NORMAL", "X_CV", "NBOCV_LSG", "BFA_LSG")})
+        keep["CV_VOLUME"] = np.ascontiguousarray(np.asarray(d["CV_VOLUME"]).reshape(-1)[:ncv], np.float64)
+        zt = np.asarray(d["ZONES"], np.int32).reshape(-1, 3)
+        names = bytes(np.asarray(d["ZONE_NAMES"], np.uint8)).decode().split("\n")[:len(zt)]
+        zk, zf, zl = (np.ascontiguousarray(zt[:, i]) for i in range(3))
+        cnames = (C.c_char_p * len(names))(*[n.encode() for n in names])
+        v = UgpView(ncv, nfa, nfa_b, *[keep[k].ctypes.data for k in (
+            "CVOFA", "FAOCV_I", "FAOCV_V", "NBOCV_I", "NBOCV_V", "X_FA", "FA_NORMAL", "X_CV", "CV_VOLUME",
+            "NBOCV_LSG", "BFA_LSG")], len(names), C.cast(cnames, C.c_void_p), zk.ctypes.data, zf.ctypes.data,
+            zl.ctypes.data)
+        h = C.c_void_p()
+        _check(lib().jm_mesh_from_ugp(C.byref(v), C.byref(h)))
         return cls(h)
 
     @classmethod
@@ -280,6 +326,12 @@ class Solver:
         vel = np.ascontiguousarray(vel, np.float64)
         p = np.ascontiguousarray(p, np.float64)
         _check(lib().jm_set_boundary_state(self.h, zone, _ptr(T), _ptr(vel), _ptr(p)))
+
+    def boundary_state(self, zone, nface):
+        """(T[n], vel[n][3], p[n]) that setBC last computed on a zone's faces."""
+        T, vel, p = np.zeros(nface), np.zeros((nface, 3)), np.zeros(nface)
+        _check(lib().jm_get_boundary_state(self.h, zone, _ptr(T), _ptr(vel), _ptr(p)))
+        return T, vel, p
 
     def device_state_ptr(self) -> int:
         q = C.c_void_p()

@@ -157,3 +157,11 @@ def test_profile_file_checked_before_device(tmp_path):
         with pytest.raises(joemi.JoemiError) as e:
             joemi.Solver(mesh, joe.replace("inlet = CBC 0.5 0.1 0.0 1.0 1.0", f"inlet = PROFILES {f}"))
         assert e.value.status == 3 and needle in str(e.value), str(e.value)
+
+
+def test_block_bcgstab_arguments_checked_before_device():
+    from myjoe_amd import joemi as J
+
+    L = J.lib()
+    st = L.jm_block_bcgstab(0, None, None, None, None, None, 1e-8, 1e-2, 30, None, None, None)
+    assert st == 3 and "ncv" in L.jm_last_error().decode()

@@ -144,3 +144,68 @@ def test_face_flux_form_matches_reference(fixture):
     assert np.array_equal(s.get("COUNTERS")[:2], e.get("COUNTERS")[:2]), fixture
     s.step()
     compare(s.get("DQ"), g["DQ"], 1e-9, f"face:{fixture}:DQ")
+
+
+@pytest.mark.parametrize("name", ["bump24", "bump96", "bump24visc"])
+def test_graph_step_matches_reference(name):
+    # the device-driven step (CFL ramp and step counter on the device, the
+    # step replayed from captured hipGraphs) against the reference's own first
+    # five steps of JoeWithModels::run, not against the host-polled step
+    print(f"golden graph solve {name}", flush=True)
+    g = load("solve_" + name)
+    nx, ny = (96, 32) if name == "bump96" else (24, 8)
+    mesh = joemi.Mesh.from_faces(meshgen.bump_channel(nx, ny))
+    s = joemi.Solver(mesh, joe_text(g))
+    s.init_uniform()
+    for k in range(1, 6):
+        info = s.step_graph()
+        assert info.step == k and info.lin_iters >= 1
+        rho, rhou, rhoE = s.state()
+        compare(rho, g[f"RHO@{k}"], 1e-9, f"{name} graph rho@{k}")
+        compare(rhou.reshape(-1), g[f"RHOU@{k}"].reshape(-1), 1e-9, f"{name} graph rhou@{k}")
+        compare(rhoE, g[f"RHOE@{k}"], 1e-9, f"{name} graph rhoE@{k}")
+
+
+@pytest.mark.parametrize("fixture", ["box4_hllc", "box4_roe_alt_ana", "bump24_hllc_visc_pl_sa"])
+def test_boundary_state_read_matches_reference(fixture):
+    # jm_get_boundary_state gives each zone's setBC values as the reference
+    # hands them to boundaryHook (T_bfa, vel_bfa, p_bfa)
+    print(f"golden boundary state {fixture}", flush=True)
+    g = load("rhs_" + fixture)
+    mesh = joemi.Mesh.from_faces(MESHES[fixture.split("_")[0]]())
+    s = joemi.Solver(mesh, joe_text(g))
+    s.set_state(g["RHO"], g["RHOU"], g["RHOE"])
+    if "_sa" in fixture:
+        s.set_scalar(g["NUSA"])
+    s.calc_rhs(False)
+    vb = g["VEL_BFA"].reshape(-1, 3)
+    nb = 0
+    for iz, (zname, kind, f0, f1) in enumerate(mesh.zones()):
+        if kind != 1:
+            continue
+        T, vel, p = s.boundary_state(iz, f1 - f0 + 1)
+        compare(T, g["T_BFA"][f0:f1 + 1], 1e-12, f"{fixture}:{zname}:T")
+        compare(vel, vb[f0:f1 + 1], 1e-12, f"{fixture}:{zname}:vel")
+        compare(p, g["P_BFA"][f0:f1 + 1], 1e-12, f"{fixture}:{zname}:p")
+        nb += f1 - f0 + 1
+    assert nb == mesh.nfa_b
+
+
+@pytest.mark.parametrize("fixture", ["box4_hllc", "box4_hllc_visc_su_sa", "bump24_hllc_visc_pl_sa"])
+def test_residual_on_reference_arrays(fixture):
+    # jm_mesh_from_ugp, the entry a host holding the reference's own Ugp
+    # arrays binds: the mesh is built from the reference's dumped arrays
+    # (tests/golden/mesh_*.npz) without renumbering, and the residual matches
+    # the reference's on it
+    print(f"golden ugp mesh {fixture}", flush=True)
+    g = load("rhs_" + fixture)
+    mesh = joemi.Mesh.from_ugp(load("mesh_" + fixture.split("_")[0]))
+    s = joemi.Solver(mesh, joe_text(g))
+    s.set_state(g["RHO"], g["RHOU"], g["RHOE"])
+    sa = "_sa" in fixture
+    if sa:
+        s.set_scalar(g["NUSA"])
+    s.calc_rhs(True)
+    for k in FIELDS + (["GRAD_ENTHALPY"] if "visc" in fixture else []) + (SA_GPU_FIELDS if sa else []):
+        if k in g:
+            compare(s.get(k), g[k], 1e-12, f"ugp {fixture}:{k}")

@@ -107,3 +107,29 @@ def test_ls_coefficients_reproduce_linear_fields():
     for f in range(mesh.nfa_b):
         g[cvofa[f, 0]] += blsg[f] * phib[f]
     assert np.max(np.abs(g - gvec)) < 1e-10
+
+
+@pytest.mark.parametrize("name", sorted(MESHES))
+def test_from_ugp_keeps_reference_arrays(name):
+    # jm_mesh_from_ugp takes the reference's own arrays over unchanged
+    g = load("mesh_" + name)
+    mesh = joemi.Mesh.from_ugp(g)
+    assert (mesh.ncv, mesh.nfa, mesh.nfa_b) == tuple(int(x) for x in g["COUNTS"][[3, 1, 2]])
+    for k in ["CVOFA", "FAOCV_I", "FAOCV_V", "NBOCV_I", "NBOCV_V"]:
+        assert np.array_equal(mesh.array(k).reshape(-1), g[k].reshape(-1)), f"{name}:{k}"
+    ncv = mesh.ncv
+    for k in ["X_FA", "FA_NORMAL", "NBOCV_LSG", "BFA_LSG"]:
+        assert np.array_equal(mesh.array(k).reshape(-1), g[k].reshape(-1)), f"{name}:{k}"
+    assert np.array_equal(mesh.array("X_CV").reshape(-1), g["X_CV"].reshape(-1)[:3 * ncv])
+    assert np.array_equal(mesh.array("CV_VOLUME").reshape(-1), g["CV_VOLUME"].reshape(-1)[:ncv])
+    zt = g["ZONES"].reshape(-1, 3)
+    assert [(z[1], z[2], z[3]) for z in mesh.zones()] == [tuple(int(v) for v in r) for r in zt]
+
+
+def test_from_ugp_rejects_inconsistent_arrays():
+    g = dict(load("mesh_box4"))
+    cv = g["CVOFA"].copy().reshape(-1)
+    cv[1] = 0  # a boundary face given a second cell
+    g["CVOFA"] = cv
+    with pytest.raises(joemi.JoemiError):
+        joemi.Mesh.from_ugp(g)
