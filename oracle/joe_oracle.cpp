@@ -573,14 +573,18 @@ void jac_roe(double (*A_L)[5], double (*A_R)[5], double rhoL, const double *uL, 
 // kine = 0: the energy is built from the given enthalpy h (calcThermoProp_p's
 // or the cell's), not from gamma/(gamma-1) p/rho as the segregated flux does.
 // only_pressure: the "ONLY_PRESSURE" treatment of symmetry and wall faces.
+// With scal (one transported scalar, ConvTerm = 1; scal[0] left, scal[1]
+// right), *Fs gets its flux: the upwind side's value times the unscaled mass
+// flux, then times the area (:1851-1896, 1926).
 void flux_hllc_coupled(double F[5], double rhoL, const double *uL, double pL, double h0, double gammaL, double rhoR,
                        const double *uR, double pR, double h1, double gammaR, double area, const double *nVec,
-                       bool only_pressure) {
+                       bool only_pressure, const double *scal = NULL, double *Fs = NULL) {
   if (only_pressure) {
     F[0] = 0.0;
     for (int i = 1; i < 4; i++) F[i] = pL * nVec[i - 1];
     F[4] = 0.0;
     for (int i = 0; i < 5; i++) F[i] *= area;
+    if (Fs) *Fs = 0.0 * area;
     return;
   }
   const double kL = 0.0, kR = 0.0;
@@ -637,7 +641,138 @@ void flux_hllc_coupled(double F[5], double rhoL, const double *uL, double pL, do
       F[4] = (rhoeR + pR) * unR;
     }
   }
+  if (Fs) *Fs = (1.0 * F[0] * (sM >= 0.0 ? scal[0] : scal[1])) * area;
   for (int i = 0; i < 5; i++) F[i] *= area;
+}
+
+// The scalar row of calcEulerFluxMatricesCoupled_HLLC (UgpWithCvCompFlow.cpp
+// :1934-2174) for one transported scalar with no pressure derivative
+// (dpress_dscal = 0: RansTurbSA leaves pressureDerivativeHookScalarRansTurb
+// empty, UgpWithCvCompFlow.h:1590): the supersonic rows of
+// calcJacobianCoupled_HLLC (:2233-2240), the subsonic transformation
+// (:2281-2284) and star-state terms (:2022-2026, 2074-2078), then dU/dQ
+// (:2150-2174). With dpress_dscal = 0 the flow rows' scalar column stays zero
+// and their column 0 unchanged, so only this row is formed: SL = A_L[5][0..5],
+// SR = A_R[5][0..5] (either may be NULL).
+void jac_hllc_coupled_srow(double *SL, double *SR, double rhoL, const double *uL, double pL, double h0, double scalL,
+                           double gammaL, double rhoR, const double *uR, double pR, double h1, double scalR,
+                           double gammaR, double area, const double *nVec, bool only_pressure) {
+  auto zero = [](double *r) {
+    for (int j = 0; j < 6; j++) r[j] = 0.0;
+  };
+  auto finish = [](double *r, double scal, double rho) {
+    for (int j = 0; j < 6; j++) r[j] *= 1.0;  // ConvTerm
+    r[0] -= scal / rho * r[5];
+    r[5] /= rho;
+  };
+  if (only_pressure) {
+    if (SL) {
+      zero(SL);
+      finish(SL, scalL, rhoL);
+    }
+    if (SR) {
+      zero(SR);
+      finish(SR, scalR, rhoR);
+    }
+    return;
+  }
+  double unL = dot3(uL, nVec);
+  double uLuL = dot3(uL, uL);
+  double cL = sqrt(gammaL * pL / rhoL);
+  double hL = h0 + 0.0;
+  double unR = dot3(uR, nVec);
+  double uRuR = dot3(uR, uR);
+  double cR = sqrt(gammaR * pR / rhoR);
+  double hR = h1 + 0.0;
+  double Rrho = sqrt(rhoR / rhoL);
+  double tmp = 1.0 / (1.0 + Rrho);
+  double velRoe[3];
+  for (int i = 0; i < 3; i++) velRoe[i] = tmp * (uL[i] + uR[i] * Rrho);
+  double uRoe = dot3(velRoe, nVec);
+  double gamPdivRho = tmp * ((gammaL * pL / rhoL + 0.5 * (gammaL - 1.0) * uLuL) +
+                             (gammaR * pR / rhoR + 0.5 * (gammaR - 1.0) * uRuR) * Rrho);
+  double cRoe = sqrt(gamPdivRho - ((gammaL + gammaR) * 0.5 - 1.0) * 0.5 * dot3(velRoe, velRoe));
+  double sL = min(uRoe - cRoe, unL - cL);
+  double sR = max(uRoe + cRoe, unR + cR);
+  double invrhotld = 1.0 / (rhoR * (sR - unR) - rhoL * (sL - unL));
+  double sM = (pL - pR - rhoL * unL * (sL - unL) + rhoR * unR * (sR - unR)) / (rhoR * (sR - unR) - rhoL * (sL - unL));
+  double nVecArea[3];
+  for (int i = 0; i < 3; i++) nVecArea[i] = nVec[i] * area;
+  // calcJacobianCoupled_HLLC's scalar row (nV = nVecArea)
+  auto super = [&](double *r, const double *vel, double rrho, double scal) {
+    double nVel[3] = {vel[0] * nVecArea[0], vel[1] * nVecArea[1], vel[2] * nVecArea[2]};
+    double U_k = nVel[0] + nVel[1] + nVel[2];
+    r[0] = 0.0;
+    r[1] = scal * nVecArea[0];
+    r[2] = scal * nVecArea[1];
+    r[3] = scal * nVecArea[2];
+    r[4] = 0.0;
+    r[5] = 0.0;
+    r[5] = rrho * U_k;
+  };
+  // calcdSMdU_HLLC (unit normal), the scalar entry dpress_dscal = 0
+  auto dsm = [&](double d[6], const double *vel, double pp, double rrho, double hh, double gamma, double S,
+                 double Factor) {
+    double kapm1 = (gamma - 1.0);
+    double nVel[3] = {vel[0] * nVec[0], vel[1] * nVec[1], vel[2] * nVec[2]};
+    double U_k = nVel[0] + nVel[1] + nVel[2];
+    double vSquHlf = 0.5 * dot3(vel, vel);
+    d[0] = Factor * invrhotld * (-U_k * U_k + kapm1 * vSquHlf + gamma * pp / rrho - kapm1 * hh + sM * S);
+    for (int i = 0; i < 3; i++) d[1 + i] = Factor * invrhotld * ((2 * U_k - S - sM) * nVec[i] - kapm1 * vel[i]);
+    d[4] = Factor * invrhotld * kapm1;
+    d[5] = Factor * invrhotld * 0.0;
+  };
+  auto sub = [&](double *r, const double *d, double rhoS, double scal, double Omega) {
+    double OmSM = Omega * sM;
+    double OmSMp1 = OmSM + 1.0;
+    for (int i = 0; i < 6; i++) r[i] = -OmSM * r[i] + area * OmSMp1 * d[i] * rhoS * scal;
+  };
+  double d[6];
+  if (sM >= 0.0) {
+    if (sL > 0.0) {
+      if (SL) super(SL, uL, rhoL, scalL);
+      if (SR) zero(SR);
+    } else {
+      double OmegaL = 1.0 / (sL - sM);
+      double sLmuL = sL - unL;
+      double rhoSL = rhoL * sLmuL * OmegaL;
+      if (SL) {
+        dsm(d, uL, pL, rhoL, hL, gammaL, sL, 1.0);
+        super(SL, uL, rhoL, scalL);
+        sub(SL, d, rhoSL, scalL, OmegaL);
+        SL[0] += area * OmegaL * sM * sL * scalL;
+        SL[5] += area * OmegaL * sM * sL * rhoL;
+      }
+      if (SR) {
+        dsm(d, uR, pR, rhoR, hR, gammaR, sR, -1.0);
+        zero(SR);
+        sub(SR, d, rhoSL, scalL, OmegaL);
+      }
+    }
+  } else {
+    if (sR >= 0.0) {
+      double OmegaR = 1.0 / (sR - sM);
+      double sRmuR = sR - unR;
+      double rhoSR = rhoR * sRmuR * OmegaR;
+      if (SR) {
+        dsm(d, uR, pR, rhoR, hR, gammaR, sR, -1.0);
+        super(SR, uR, rhoR, scalR);
+        sub(SR, d, rhoSR, scalR, OmegaR);
+        SR[0] += area * OmegaR * sM * sR * scalR;
+        SR[5] += area * OmegaR * sM * sR * rhoR;
+      }
+      if (SL) {
+        dsm(d, uL, pL, rhoL, hL, gammaL, sL, 1.0);
+        zero(SL);
+        sub(SL, d, rhoSR, scalR, OmegaR);
+      }
+    } else {
+      if (SR) super(SR, uR, rhoR, scalR);
+      if (SL) zero(SL);
+    }
+  }
+  if (SL) finish(SL, scalL, rhoL);
+  if (SR) finish(SR, scalR, rhoR);
 }
 
 // calcJacobianCoupled_HLLC, UgpWithCvCompFlow.cpp:2180-2241 (nV not normalised)
@@ -869,6 +1004,8 @@ struct or_ctx {
   bool sa = false;
   std::vector<double> phi, phi_bfa, grad_phi, diff, mut_fa, wall_dist, str_mag, vort_mag;
   std::vector<double> rhs_scal, AScal, dscal;
+  // the coupled loop with the scalar: its 6x6 blocks and correction [ncv][6]
+  std::vector<double> A6, dq6;
   double cfl;
   int step;
   bool have_press;
@@ -1563,7 +1700,7 @@ static void visc_coupled(double F[5], double (*A0)[5], double (*A1)[5], double r
                          double rho1, const double *u1, const double *gu1, double h1, const double *gh1, double T1,
                          double R1, double gam1, double mul, double mut, double lamOcp, double pr_turb,
                          const double *u_fa, double area, const double *nVec, double smag, const double *sVec,
-                         double alpha) {
+                         double alpha, bool turb = false) {
   const double kine0 = 0.0, kine1 = 0.0;
   double g[3][3];
   for (int i = 0; i < 3; i++)
@@ -1576,12 +1713,14 @@ static void visc_coupled(double F[5], double (*A0)[5], double (*A1)[5], double r
     tau[i] = mu * (alpha * (u1[i] - u0[i]) / smag + g[i][0] * fCorr[0] + g[0][i] * nVec[0] + g[i][1] * fCorr[1] +
                    g[1][i] * nVec[1] + g[i][2] * fCorr[2] + g[2][i] * nVec[2]);
   double tmp = 2.0 / 3.0 * mu * (g[0][0] + g[1][1] + g[2][2]);
+  if (turb) tmp += 1.0 / 3.0 * (rho0 + rho1) * 0.0;  // kine_fa = 0 (:3084-3085)
   for (int i = 0; i < 3; i++) tau[i] -= tmp * nVec[i];
   for (int i = 0; i < 3; i++) F[1 + i] = -area * tau[i];
   double keff = lamOcp + mut / pr_turb;
   double enth = keff * (alpha * (h1 - h0) / smag + 0.5 * ((gh0[0] + gh1[0]) * fCorr[0] + (gh0[1] + gh1[1]) * fCorr[1] +
                                                          (gh0[2] + gh1[2]) * fCorr[2]));
   double psi = 0.0;
+  if (turb) psi = mu * (alpha * (kine1 - kine0) / smag);  // :3108-3110
   F[4] = -area * (enth + tau[0] * u_fa[0] + tau[1] * u_fa[1] + tau[2] * u_fa[2] + psi);
   auto side = [&](double (*A)[5], double sg, double rho, const double *u, double h, double T, double R, double gam,
                   double kine) {
@@ -1608,6 +1747,32 @@ static void visc_coupled(double F[5], double (*A0)[5], double (*A1)[5], double r
   if (A1) side(A1, -1.0, rho1, u1, h1, T1, R1, gam1, kine1);
 }
 
+// The transported scalar's row of calcViscousFluxCoupled (UgpWithCvCompFlow.cpp
+// :3113-3120 flux, :3158-3167 and :3217-3226 dF/dU, :3185-3190 and :3244-3249
+// dU/dQ): *Fs and the row's entries A0[5][0], A0[5][5] (a0, may be NULL) and
+// A1[5][0], A1[5][5] (a1, may be NULL).
+static void visc_coupled_srow(double *Fs, double *a0, double *a1, double rho0, double scal0, const double *gs0,
+                              double rho1, double scal1, const double *gs1, double diff, double DiffTerm, double area,
+                              const double *nVec, double smag, const double *sVec, double alpha) {
+  double fCorr[3] = {nVec[0] - alpha * sVec[0], nVec[1] - alpha * sVec[1], nVec[2] - alpha * sVec[2]};
+  *Fs = -area * DiffTerm * diff *
+        (alpha * (scal1 - scal0) / smag +
+         0.5 * ((gs0[0] + gs1[0]) * fCorr[0] + (gs0[1] + gs1[1]) * fCorr[1] + (gs0[2] + gs1[2]) * fCorr[2]));
+  double area_alpha_smag = area * alpha / smag;
+  if (a0) {
+    double dFdU = DiffTerm * diff * area_alpha_smag;
+    double invRho = 1.0 / rho0;
+    a0[0] = dFdU * (-invRho * scal0);
+    a0[1] = dFdU * invRho;
+  }
+  if (a1) {
+    double dFdU = -DiffTerm * diff * area_alpha_smag;
+    double invRho = 1.0 / rho1;
+    a1[0] = dFdU * (-invRho * scal1);
+    a1[1] = dFdU * invRho;
+  }
+}
+
 // calcFluxCoupled, JoeWithModels.cpp:4613-5825, for nScal = 0, MU_REF = 0 and
 // HLLC: the coupled HLLC flux and Jacobian, each face side's enthalpy from
 // calcThermoProp_p after reconstruction (the cell's when first order), and
@@ -1616,6 +1781,8 @@ static void visc_coupled(double F[5], double (*A0)[5], double (*A1)[5], double r
 // both sides, the others reconstructed against the boundary state. The
 // effective SYMM_BC default is ORIGINAL_NUMERIC here too: init registers it
 // (UgpWithCvCompFlow.h:479) before calcFluxCoupled asks with ANALYTICAL.
+void sa_diffusivity(or_ctx &c);
+
 void calc_flux_coupled(or_ctx &c, double *A) {
   const or_mesh &m = c.m;
   const bool snd = c.p.second_order != 0;
@@ -1632,6 +1799,37 @@ void calc_flux_coupled(or_ctx &c, double *A) {
     else grad_scalar_gg(c, c.grad_h.data(), c.enthalpy.data(), c.h_bfa.data());
     if (c.p.limiter) limit_scalar(c, c.grad_h.data(), c.enthalpy.data(), c.enthalpy.data(), NULL);
   }
+  // one transported scalar (Spalart-Allmaras) in the coupled system: its
+  // limited gradient (STANDARD reconstruction or MU_REF > 0, :4718-4738; done
+  // by euler_gradients when second order), the diffusivity (:4739-4743), and
+  // its row of each 6x6 block, kept in AScal[k][nnz] = A[noc][5][k]
+  const bool sa = c.sa;
+  const size_t nnz = c.nnz;
+  double *AS = (sa && A) ? c.AScal.data() : NULL;
+  if (sa) {
+    if (!snd) {
+      if (ls) grad_scalar_ls(c, c.grad_phi.data(), c.phi.data(), c.phi_bfa.data());
+      else grad_scalar_gg(c, c.grad_phi.data(), c.phi.data(), c.phi_bfa.data());
+      if (c.p.limiter) limit_scalar(c, c.grad_phi.data(), c.phi.data(), c.phi.data(), NULL);
+    }
+    sa_diffusivity(c);
+  }
+  auto srow = [&](int noc, const double *r, double sg) {
+    for (int k = 0; k < 6; k++) {
+      if (sg > 0.0) AS[k * nnz + noc] += r[k];
+      else AS[k * nnz + noc] -= r[k];
+    }
+  };
+  auto vrow = [&](int noc, const double *a, double sg) {  // entries [5][0] and [5][5]
+    if (sg > 0.0) {
+      AS[0 * nnz + noc] += a[0];
+      AS[5 * nnz + noc] += a[1];
+    } else {
+      AS[0 * nnz + noc] -= a[0];
+      AS[5 * nnz + noc] -= a[1];
+    }
+  };
+  const double zero3[3] = {0.0, 0.0, 0.0};
   double V0[5][5], V1[5][5], VF[5];
   for (int i = 0; i < 5; i++)
     for (int j = 0; j < 5; j++) V0[i][j] = V1[i][j] = 0.0;
@@ -1640,7 +1838,7 @@ void calc_flux_coupled(or_ctx &c, double *A) {
     double T = p / (rho * R);
     return g * R / (g - 1.0) * T;
   };
-  double F[5], Apl[5][5], Ami[5][5];
+  double F[5], Apl[5][5], Ami[5][5], Fs = 0.0, SL[6], SR[6], VFs = 0.0, a0[2], a1[2];
   for (int ifa = c.nfa_b; ifa < c.nfa; ifa++) {  // :4797-5105
     int icv0 = m.cvofa[2 * ifa], icv1 = m.cvofa[2 * ifa + 1];
     int noc00 = 0, noc01 = 0, noc11 = 0, noc10 = 0;
@@ -1658,6 +1856,7 @@ void calc_flux_coupled(or_ctx &c, double *A) {
     double u0[3] = {c.vel[3 * icv0], c.vel[3 * icv0 + 1], c.vel[3 * icv0 + 2]};
     double rho1 = c.rho[icv1], p1 = c.press[icv1], h1 = c.enthalpy[icv1];
     double u1[3] = {c.vel[3 * icv1], c.vel[3 * icv1 + 1], c.vel[3 * icv1 + 2]};
+    double sc[2] = {sa ? c.phi[icv0] : 0.0, sa ? c.phi[icv1] : 0.0};
     if (snd) {
       double r0[3], r1[3];
       for (int i = 0; i < 3; i++) {
@@ -1671,6 +1870,7 @@ void calc_flux_coupled(or_ctx &c, double *A) {
         rho0 = c.rho[icv0];
       } else {
         for (int i = 0; i < 3; i++) u0[i] += dot3(r0, &c.grad_u[9 * icv0 + 3 * i]);
+        if (sa) sc[0] += dot3(r0, &c.grad_phi[3 * icv0]);
       }
       rho1 += dot3(r1, &c.grad_rho[3 * icv1]);
       p1 += dot3(r1, &c.grad_p[3 * icv1]);
@@ -1679,17 +1879,20 @@ void calc_flux_coupled(or_ctx &c, double *A) {
         rho1 = c.rho[icv1];
       } else {
         for (int i = 0; i < 3; i++) u1[i] += dot3(r1, &c.grad_u[9 * icv1 + 3 * i]);
+        if (sa) sc[1] += dot3(r1, &c.grad_phi[3 * icv1]);
       }
       h0 = thermo_h(rho0, p0);
       h1 = thermo_h(rho1, p1);
     }
-    flux_hllc_coupled(F, rho0, u0, p0, h0, g, rho1, u1, p1, h1, g, area, nVec, false);
+    flux_hllc_coupled(F, rho0, u0, p0, h0, g, rho1, u1, p1, h1, g, area, nVec, false, sc, sa ? &Fs : NULL);
     c.rhs_rho[icv0] -= F[0];
     for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= F[1 + i];
     c.rhs_rhoE[icv0] -= F[4];
+    if (sa) c.rhs_scal[icv0] -= Fs;
     c.rhs_rho[icv1] += F[0];
     for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv1 + i] += F[1 + i];
     c.rhs_rhoE[icv1] += F[4];
+    if (sa) c.rhs_scal[icv1] += Fs;
     if (A) {  // cell-centre states (:5035-5038)
       jac_hllc_coupled(Apl, Ami, c.rho[icv0], &c.vel[3 * icv0], c.press[icv0], c.enthalpy[icv0], g, c.rho[icv1],
                        &c.vel[3 * icv1], c.press[icv1], c.enthalpy[icv1], g, area, nVec, false);
@@ -1703,6 +1906,15 @@ void calc_flux_coupled(or_ctx &c, double *A) {
           A[25 * noc11 + 5 * i + j] -= Ami[i][j];
           A[25 * noc10 + 5 * i + j] -= Apl[i][j];
         }
+      if (sa) {
+        jac_hllc_coupled_srow(SL, SR, c.rho[icv0], &c.vel[3 * icv0], c.press[icv0], c.enthalpy[icv0], c.phi[icv0], g,
+                              c.rho[icv1], &c.vel[3 * icv1], c.press[icv1], c.enthalpy[icv1], c.phi[icv1], g, area,
+                              nVec, false);
+        srow(noc00, SL, 1.0);
+        srow(noc01, SR, 1.0);
+        srow(noc11, SR, -1.0);
+        srow(noc10, SL, -1.0);
+      }
     }
     if (visc) {  // the same face's viscous flux and blocks (:5076-5105)
       double sVec[3];
@@ -1715,13 +1927,19 @@ void calc_flux_coupled(or_ctx &c, double *A) {
       visc_coupled(VF, A ? V0 : NULL, A ? V1 : NULL, c.rho[icv0], v0, &c.grad_u[9 * icv0], c.enthalpy[icv0],
                    &c.grad_h[3 * icv0], c.temp[icv0], R, g, c.rho[icv1], v1, &c.grad_u[9 * icv1], c.enthalpy[icv1],
                    &c.grad_h[3 * icv1], c.temp[icv1], R, g, c.mul_fa[ifa], c.mut_fa[ifa], c.lamOcp_fa[ifa],
-                   c.p.pr_turb, uAux_fa, area, nVec, smag, sVec, alpha);
+                   c.p.pr_turb, uAux_fa, area, nVec, smag, sVec, alpha, sa);
+      if (sa)
+        visc_coupled_srow(&VFs, A ? a0 : NULL, A ? a1 : NULL, c.rho[icv0], c.phi[icv0], &c.grad_phi[3 * icv0],
+                          c.rho[icv1], c.phi[icv1], &c.grad_phi[3 * icv1], c.diff[ifa], 1.0, area, nVec, smag, sVec,
+                          alpha);
       c.rhs_rho[icv0] -= VF[0];
       for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= VF[1 + i];
       c.rhs_rhoE[icv0] -= VF[4];
+      if (sa) c.rhs_scal[icv0] -= VFs;
       c.rhs_rho[icv1] += VF[0];
       for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv1 + i] += VF[1 + i];
       c.rhs_rhoE[icv1] += VF[4];
+      if (sa) c.rhs_scal[icv1] += VFs;
       if (A) {
         for (int k = 0; k < 25; k++) {
           A[25 * noc00 + k] += V0[k / 5][k % 5];
@@ -1731,11 +1949,20 @@ void calc_flux_coupled(or_ctx &c, double *A) {
           A[25 * noc11 + k] -= V1[k / 5][k % 5];
           A[25 * noc10 + k] -= V0[k / 5][k % 5];
         }
+        if (sa) {
+          vrow(noc00, a0, 1.0);
+          vrow(noc01, a1, 1.0);
+          vrow(noc11, a1, -1.0);
+          vrow(noc10, a0, -1.0);
+        }
       }
     }
   }
   for (int iz = 0; iz < m.nzone; iz++) {  // :5113-5796
     const int bc = m.zone_bc[iz];
+    // a zone without a nuSA value carries no viscous scalar flux (DiffTerm = 0,
+    // :5453-5461, 5599-5607); NEUMANN zones keep it (:5293-5296)
+    const double DiffTerm = (bc == OR_BC_NEUMANN || m.zone_scal[2 * iz] != 0.0) ? 1.0 : 0.0;
     for (int ifa = m.zone_ifa[2 * iz]; ifa <= m.zone_ifa[2 * iz + 1]; ifa++) {
       int icv0 = m.cvofa[2 * ifa];
       int noc00 = m.nbocv_i[icv0];
@@ -1743,18 +1970,29 @@ void calc_flux_coupled(or_ctx &c, double *A) {
       double area = norm3(nVec, &m.fa_normal[3 * ifa]);
       const double *ub = &c.vel_bfa[3 * ifa];
       const double rb = c.rho_bfa[ifa], pb = c.p_bfa[ifa], hb = c.h_bfa[ifa];
+      const double phib = sa ? c.phi_bfa[ifa] : 0.0;
+      const double scb[2] = {phib, phib};
       if (bc == OR_BC_SYMMETRY || bc == OR_BC_WALL) {
         const bool op = c.p.symm_bc == OR_SYMM_ANALYTICAL;
-        flux_hllc_coupled(F, rb, ub, pb, hb, g, rb, ub, pb, hb, g, area, nVec, op);
+        flux_hllc_coupled(F, rb, ub, pb, hb, g, rb, ub, pb, hb, g, area, nVec, op, scb, sa ? &Fs : NULL);
         c.rhs_rho[icv0] -= F[0];
         for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= F[1 + i];
         c.rhs_rhoE[icv0] -= F[4];
+        if (sa) c.rhs_scal[icv0] -= Fs;
         if (A) {
           const bool nn = c.p.symm_bc == OR_SYMM_NEW_NUMERIC;
           jac_hllc_coupled(Apl, nn ? Ami : NULL, rb, ub, pb, hb, g, rb, ub, pb, hb, g, area, nVec, op);
           for (int k = 0; k < 25; k++)
             if (nn) A[25 * noc00 + k] += Apl[k / 5][k % 5] + Ami[k / 5][k % 5];
             else A[25 * noc00 + k] += Apl[k / 5][k % 5];
+          if (sa) {
+            jac_hllc_coupled_srow(SL, nn ? SR : NULL, rb, ub, pb, hb, phib, g, rb, ub, pb, hb, phib, g, area, nVec,
+                                  op);
+            if (nn)
+              for (int k = 0; k < 6; k++) AS[k * nnz + noc00] += SL[k] + SR[k];
+            else
+              srow(noc00, SL, 1.0);
+          }
         }
         if (visc && bc == OR_BC_SYMMETRY) {  // 2/3 (mu + mut) tr(grad u) only (:5272-5284)
           const double *gu = &c.grad_u[9 * icv0];
@@ -1767,23 +2005,37 @@ void calc_flux_coupled(or_ctx &c, double *A) {
           const double *gu = &c.grad_u[9 * icv0], *gh = &c.grad_h[3 * icv0];
           visc_coupled(VF, A ? V0 : NULL, NULL, c.rho[icv0], &c.vel[3 * icv0], gu, c.enthalpy[icv0], gh, c.temp[icv0],
                        R, g, rb, ub, gu, hb, gh, c.T_bfa[ifa], R, g, c.mul_fa[ifa], 0.0, c.lamOcp_fa[ifa], c.p.pr_turb,
-                       ub, area, nVec, smag, nVec, 1.0);
+                       ub, area, nVec, smag, nVec, 1.0, sa);
+          if (sa)
+            visc_coupled_srow(&VFs, A ? a0 : NULL, NULL, c.rho[icv0], c.phi[icv0], &c.grad_phi[3 * icv0], rb, phib,
+                              &c.grad_phi[3 * icv0], c.diff[ifa], DiffTerm, area, nVec, smag, nVec, 1.0);
           c.rhs_rho[icv0] -= VF[0];
           for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= VF[1 + i];
           c.rhs_rhoE[icv0] -= VF[4];
-          if (A)
+          if (sa) c.rhs_scal[icv0] -= VFs;
+          if (A) {
             for (int k = 0; k < 25; k++) A[25 * noc00 + k] += V0[k / 5][k % 5];
+            if (sa) vrow(noc00, a0, 1.0);
+          }
         }
       } else if (bc == OR_BC_NEUMANN) {
-        flux_hllc_coupled(F, rb, ub, pb, hb, g, rb, ub, pb, hb, g, area, nVec, false);
+        flux_hllc_coupled(F, rb, ub, pb, hb, g, rb, ub, pb, hb, g, area, nVec, false, scb, sa ? &Fs : NULL);
         c.rhs_rho[icv0] -= F[0];
         for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= F[1 + i];
         c.rhs_rhoE[icv0] -= F[4];
+        if (sa) c.rhs_scal[icv0] -= Fs;
         if (A) {
           jac_hllc_coupled(Apl, Ami, rb, ub, pb, hb, g, rb, ub, pb, hb, g, area, nVec, false);
           for (int k = 0; k < 25; k++)
             if (c.p.symm_bc == OR_SYMM_NEW_NUMERIC) A[25 * noc00 + k] += Apl[k / 5][k % 5] + Ami[k / 5][k % 5];
             else A[25 * noc00 + k] += Apl[k / 5][k % 5];
+          if (sa) {
+            jac_hllc_coupled_srow(SL, SR, rb, ub, pb, hb, phib, g, rb, ub, pb, hb, phib, g, area, nVec, false);
+            if (c.p.symm_bc == OR_SYMM_NEW_NUMERIC)
+              for (int k = 0; k < 6; k++) AS[k * nnz + noc00] += SL[k] + SR[k];
+            else
+              srow(noc00, SL, 1.0);
+          }
         }
         if (visc) {  // the boundary state on both sides, s along the normal (:5373-5404)
           double sVec[3];
@@ -1792,16 +2044,23 @@ void calc_flux_coupled(or_ctx &c, double *A) {
           const double *gu = &c.grad_u[9 * icv0], *gh = &c.grad_h[3 * icv0];
           visc_coupled(VF, A ? V0 : NULL, A ? V1 : NULL, rb, ub, gu, hb, gh, c.T_bfa[ifa], R, g, rb, ub, gu, hb, gh,
                        c.T_bfa[ifa], R, g, c.mul_fa[ifa], c.mut_fa[ifa], c.lamOcp_fa[ifa], c.p.pr_turb, ub, area,
-                       nVec, smag, nVec, 1.0);
+                       nVec, smag, nVec, 1.0, sa);
+          if (sa)  // the scalar gradients are zero here (:5381-5386)
+            visc_coupled_srow(&VFs, A ? a0 : NULL, NULL, rb, phib, zero3, rb, phib, zero3, c.diff[ifa], DiffTerm, area,
+                              nVec, smag, nVec, 1.0);
           c.rhs_rho[icv0] -= VF[0];
           for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= VF[1 + i];
           c.rhs_rhoE[icv0] -= VF[4];
-          if (A)
+          if (sa) c.rhs_scal[icv0] -= VFs;
+          if (A) {
             for (int k = 0; k < 25; k++) A[25 * noc00 + k] += V0[k / 5][k % 5];
+            if (sa) vrow(noc00, a0, 1.0);
+          }
         }
       } else {
         double rho0 = c.rho[icv0], p0 = c.press[icv0], h0 = c.enthalpy[icv0];
         double u0[3] = {c.vel[3 * icv0], c.vel[3 * icv0 + 1], c.vel[3 * icv0 + 2]};
+        double sc[2] = {sa ? c.phi[icv0] : 0.0, phib};
         if (snd) {
           double r0[3];
           for (int i = 0; i < 3; i++) r0[i] = m.x_fa[3 * ifa + i] - m.x_cv[3 * icv0 + i];
@@ -1812,17 +2071,24 @@ void calc_flux_coupled(or_ctx &c, double *A) {
             rho0 = c.rho[icv0];
           } else {
             for (int i = 0; i < 3; i++) u0[i] += dot3(r0, &c.grad_u[9 * icv0 + 3 * i]);
+            if (sa) sc[0] += dot3(r0, &c.grad_phi[3 * icv0]);
           }
           h0 = thermo_h(rho0, p0);
         }
-        flux_hllc_coupled(F, rho0, u0, p0, h0, g, rb, ub, pb, hb, g, area, nVec, false);
+        flux_hllc_coupled(F, rho0, u0, p0, h0, g, rb, ub, pb, hb, g, area, nVec, false, sc, sa ? &Fs : NULL);
         c.rhs_rho[icv0] -= F[0];
         for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= F[1 + i];
         c.rhs_rhoE[icv0] -= F[4];
+        if (sa) c.rhs_scal[icv0] -= Fs;
         if (A) {  // cell-centre state against the boundary state (:5731-5735)
           jac_hllc_coupled(Apl, NULL, c.rho[icv0], &c.vel[3 * icv0], c.press[icv0], c.enthalpy[icv0], g, rb, ub, pb,
                            hb, g, area, nVec, false);
           for (int k = 0; k < 25; k++) A[25 * noc00 + k] += Apl[k / 5][k % 5];
+          if (sa) {
+            jac_hllc_coupled_srow(SL, NULL, c.rho[icv0], &c.vel[3 * icv0], c.press[icv0], c.enthalpy[icv0],
+                                  c.phi[icv0], g, rb, ub, pb, hb, phib, g, area, nVec, false);
+            srow(noc00, SL, 1.0);
+          }
         }
         if (visc) {  // the cell against the boundary state (:5744-5775)
           double sVec[3];
@@ -1833,12 +2099,20 @@ void calc_flux_coupled(or_ctx &c, double *A) {
           const double *gu = &c.grad_u[9 * icv0], *gh = &c.grad_h[3 * icv0];
           visc_coupled(VF, A ? V0 : NULL, NULL, c.rho[icv0], &c.vel[3 * icv0], gu, c.enthalpy[icv0], gh, c.temp[icv0],
                        R, g, rb, ub, gu, hb, gh, c.T_bfa[ifa], R, g, c.mul_fa[ifa], c.mut_fa[ifa], c.lamOcp_fa[ifa],
-                       c.p.pr_turb, ub, area, nVec, smag, sVec, alpha);
+                       c.p.pr_turb, ub, area, nVec, smag, sVec, alpha, sa);
+          // the boundary side's scalar is the cell's reconstructed face value
+          // (Scalar0), as the reference passes it (:5764-5769)
+          if (sa)
+            visc_coupled_srow(&VFs, A ? a0 : NULL, NULL, c.rho[icv0], c.phi[icv0], &c.grad_phi[3 * icv0], rb, sc[0],
+                              &c.grad_phi[3 * icv0], c.diff[ifa], DiffTerm, area, nVec, smag, sVec, alpha);
           c.rhs_rho[icv0] -= VF[0];
           for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= VF[1 + i];
           c.rhs_rhoE[icv0] -= VF[4];
-          if (A)
+          if (sa) c.rhs_scal[icv0] -= VFs;
+          if (A) {
             for (int k = 0; k < 25; k++) A[25 * noc00 + k] += V0[k / 5][k % 5];
+            if (sa) vrow(noc00, a0, 1.0);
+          }
         }
       }
     }
@@ -2169,17 +2443,9 @@ void set_scalar_bc(or_ctx &c) {
   }
 }
 
-// calcViscousFluxScalar_new, Scalars.cpp:117-247, with the SA diffusivity
-// (diffusivityHookScalarRansTurb, TurbModel_SA.h:130-172)
-void calc_viscous_flux_scalar(or_ctx &c, bool implicit) {
+// diffusivityHookScalarRansTurb of RansTurbSA (TurbModel_SA.h:130-172)
+void sa_diffusivity(or_ctx &c) {
   const or_mesh &m = c.m;
-  const size_t nnz = c.nnz;
-  double *A5 = &c.AScal[5 * nnz];
-  if (c.p.grad_scheme == OR_GRAD_LS)
-    grad_scalar_ls(c, c.grad_phi.data(), c.phi.data(), c.phi_bfa.data());
-  else
-    grad_scalar_gg(c, c.grad_phi.data(), c.phi.data(), c.phi_bfa.data());
-  if (c.p.limiter) limit_scalar(c, c.grad_phi.data(), c.phi.data(), c.phi.data(), NULL);
   const double invSigma = 1.0 / sa::sigma;
   for (int ifa = c.nfa_b; ifa < c.nfa; ifa++) {
     int icv0 = m.cvofa[2 * ifa], icv1 = m.cvofa[2 * ifa + 1];
@@ -2193,6 +2459,20 @@ void calc_viscous_flux_scalar(or_ctx &c, bool implicit) {
     for (int ifa = m.zone_ifa[2 * iz]; ifa <= m.zone_ifa[2 * iz + 1]; ifa++)
       c.diff[ifa] = m.zone_bc[iz] == OR_BC_WALL ? invSigma * c.mul_fa[ifa]
                                                 : invSigma * (c.mul_fa[ifa] + c.rho_bfa[ifa] * c.phi_bfa[ifa]);
+}
+
+// calcViscousFluxScalar_new, Scalars.cpp:117-247, with the SA diffusivity
+// (diffusivityHookScalarRansTurb, TurbModel_SA.h:130-172)
+void calc_viscous_flux_scalar(or_ctx &c, bool implicit) {
+  const or_mesh &m = c.m;
+  const size_t nnz = c.nnz;
+  double *A5 = &c.AScal[5 * nnz];
+  if (c.p.grad_scheme == OR_GRAD_LS)
+    grad_scalar_ls(c, c.grad_phi.data(), c.phi.data(), c.phi_bfa.data());
+  else
+    grad_scalar_gg(c, c.grad_phi.data(), c.phi.data(), c.phi_bfa.data());
+  if (c.p.limiter) limit_scalar(c, c.grad_phi.data(), c.phi.data(), c.phi.data(), NULL);
+  sa_diffusivity(c);
   for (int ifa = c.nfa_b; ifa < c.nfa; ifa++) {
     int icv0 = m.cvofa[2 * ifa], icv1 = m.cvofa[2 * ifa + 1];
     double n[3], sv[3];
@@ -2386,8 +2666,11 @@ void calc_rhs(or_ctx &c, double *A) {
     }
   }
   // scalars: diffusion, then the model source (JoeWithModels.cpp:2595-2618)
+  // (the coupled loop's scalar diffusion is part of calcFluxCoupled; its
+  // source is sourceHookRansTurbCoupled, TurbModel_SA.h:359-434, the same
+  // terms into the block's scalar row)
   if (c.sa) {
-    if (c.p.mu_ref > 0.0) calc_viscous_flux_scalar(c, A != NULL);
+    if (c.p.mu_ref > 0.0 && !c.p.coupled) calc_viscous_flux_scalar(c, A != NULL);
     sa_source(c, A != NULL);
   }
 }
@@ -2473,6 +2756,166 @@ int bcgstab(const or_ctx &c, double *phi, const double *A, const double *rhs, do
       done = 2;
   }
   return iter;
+}
+
+// ludeco / lusolv (UgpWithCv.cpp:2489-2540) on a flat b x b block
+void ludeco_b(double *A, int b) {
+  for (int jc = 1; jc < b; jc++) A[jc] /= A[0];
+  int jrjc = 0;
+  for (;;) {
+    jrjc++;
+    int jrjcm1 = jrjc - 1;
+    int jrjcp1 = jrjc + 1;
+    for (int jr = jrjc; jr < b; jr++) {
+      double sum = A[b * jr + jrjc];
+      for (int jm = 0; jm <= jrjcm1; jm++) sum -= A[b * jr + jm] * A[b * jm + jrjc];
+      A[b * jr + jrjc] = sum;
+    }
+    if (jrjc == (b - 1)) return;
+    for (int jc = jrjcp1; jc < b; jc++) {
+      double sum = A[b * jrjc + jc];
+      for (int jm = 0; jm <= jrjcm1; jm++) sum -= A[b * jrjc + jm] * A[b * jm + jc];
+      A[b * jrjc + jc] = sum / A[b * jrjc + jrjc];
+    }
+  }
+}
+
+void lusolv_b(const double *A, double *c, int b) {
+  c[0] = c[0] / A[0];
+  for (int jr = 1; jr < b; jr++) {
+    int jrm1 = jr - 1;
+    double sum = c[jr];
+    for (int jm = 0; jm <= jrm1; jm++) sum -= A[b * jr + jm] * c[jm];
+    c[jr] = sum / A[b * jr + jr];
+  }
+  for (int jrjr = 1; jrjr < b; jrjr++) {
+    int jr = (b - 1) - jrjr;
+    int jrp1 = jr + 1;
+    double sum = c[jr];
+    for (int jmjm = jrp1; jmjm < b; jmjm++) {
+      int jm = (b - 1) - jmjm + jrp1;
+      sum -= A[b * jr + jm] * c[jm];
+    }
+    c[jr] = sum;
+  }
+}
+
+// The coupled system's block-Jacobi BiCGStab on b x b blocks (b = 5 + nScal):
+// the recurrences and stop rule of solveCvVectorR5Bcgstab (UgpWithCv.cpp
+// :2660-2876) on the wider blocks. The reference solves this system with
+// PETSc GMRES (PetscSolver.h:278-310), which is not part of this path; the
+// converged state is set by the residual, not by the Krylov method.
+int bcgstab_b(const or_ctx &c, double *phi, const double *A, const double *rhs, double zeroAbs, double zeroRel,
+              int maxiter, int b) {
+  const or_mesh &m = c.m;
+  const int ncv = c.ncv, n = b * ncv, bb = b * b;
+  std::vector<double> res(n), res0(n), p(n, 0.0), v(n, 0.0), t(n), s(n), phat(n, 0.0), shat(n, 0.0);
+  std::vector<double> LU((size_t)bb * ncv), tmp(b);
+  for (int icv = 0; icv < ncv; icv++) {
+    std::memcpy(&LU[(size_t)bb * icv], &A[(size_t)bb * m.nbocv_i[icv]], bb * sizeof(double));
+    ludeco_b(&LU[(size_t)bb * icv], b);
+  }
+  auto mat_vec_b = [&](double *out, const double *x) {
+    for (int icv = 0; icv < ncv; icv++) {
+      for (int i = 0; i < b; i++) out[b * icv + i] = 0.0;
+      for (int noc = m.nbocv_i[icv]; noc <= m.nbocv_i[icv + 1] - 1; noc++) {
+        int nb = m.nbocv_v[noc];
+        for (int i = 0; i < b; i++)
+          for (int j = 0; j < b; j++) out[b * icv + i] += A[(size_t)bb * noc + b * i + j] * x[b * nb + j];
+      }
+    }
+  };
+  auto max_prec = [&](const std::vector<double> &r) {
+    double mx = 0.0;
+    for (int icv = 0; icv < ncv; icv++) {
+      for (int i = 0; i < b; i++) tmp[i] = r[b * icv + i];
+      lusolv_b(&LU[(size_t)bb * icv], tmp.data(), b);
+      for (int i = 0; i < b; i++) mx = max(mx, fabs(tmp[i]));
+    }
+    return mx;
+  };
+  double rho = 1.0, alpha = 1.0, omega = 1.0;
+  mat_vec_b(res.data(), phi);
+  for (int k = 0; k < n; k++) {
+    res[k] = (rhs[k] - res[k]);
+    res0[k] = res[k];
+  }
+  double res0_max = max_prec(res);
+  int iter = 0, done = 0;
+  while (done == 0) {
+    iter++;
+    double rho_prime = -omega * rho;
+    rho = 0.0;
+    for (int k = 0; k < n; k++) rho += res[k] * res0[k];
+    if (fabs(rho_prime) < 1.0E-20) rho_prime = 1.0E-20;
+    double beta = alpha * rho / rho_prime;
+    for (int k = 0; k < n; k++) p[k] = res[k] - beta * (p[k] - omega * v[k]);
+    for (int icv = 0; icv < ncv; icv++) {
+      for (int i = 0; i < b; i++) phat[b * icv + i] = p[b * icv + i];
+      lusolv_b(&LU[(size_t)bb * icv], &phat[b * icv], b);
+    }
+    mat_vec_b(v.data(), phat.data());
+    double gamma = 0.0;
+    for (int k = 0; k < n; k++) gamma += v[k] * res0[k];
+    if (fabs(gamma) < 1.0E-20) gamma = 1.0E-20;
+    alpha = rho / gamma;
+    for (int k = 0; k < n; k++) s[k] = res[k] - alpha * v[k];
+    for (int icv = 0; icv < ncv; icv++) {
+      for (int i = 0; i < b; i++) shat[b * icv + i] = s[b * icv + i];
+      lusolv_b(&LU[(size_t)bb * icv], &shat[b * icv], b);
+    }
+    mat_vec_b(t.data(), shat.data());
+    double b0 = 0.0, b1 = 0.0;
+    for (int k = 0; k < n; k++) {
+      b0 += s[k] * t[k];
+      b1 += t[k] * t[k];
+    }
+    omega = b0 / (b1 + 1.0E-20);
+    for (int k = 0; k < n; k++) phi[k] += alpha * phat[k] + omega * shat[k];
+    mat_vec_b(res.data(), phi);
+    for (int k = 0; k < n; k++) res[k] = rhs[k] - res[k];
+    double res_max = max_prec(res);
+    if ((res_max <= zeroAbs) || (res_max / (res0_max + 1.0E-12) <= zeroRel))
+      done = 1;
+    else if (iter > maxiter)
+      done = 2;
+  }
+  return iter;
+}
+
+// the 6x6 blocks of the coupled system with one scalar: the flow rows of A
+// (their scalar column is zero with no pressure derivative) and the scalar
+// row from AScal
+void assemble_blocks6(const or_ctx &c, std::vector<double> &A6) {
+  const size_t nnz = c.nnz;
+  A6.assign(36 * nnz, 0.0);
+  for (size_t noc = 0; noc < nnz; noc++) {
+    for (int i = 0; i < 5; i++)
+      for (int j = 0; j < 5; j++) A6[36 * noc + 6 * i + j] = c.A[25 * noc + 5 * i + j];
+    for (int k = 0; k < 6; k++) A6[36 * noc + 30 + k] = c.AScal[k * nnz + noc];
+  }
+}
+
+// one coupled step's linear system and solve with the transported scalar
+// (runBackwardEulerCoupled, JoeWithModels.cpp:4455-4473): every row of the
+// residual under-relaxed by UNDER_RELAXATION, V/dt on all six diagonal
+// entries; dq6 [ncv][6]
+int coupled_solve6(or_ctx &c) {
+  const int ncv = c.ncv;
+  std::vector<double> rhs(6 * (size_t)ncv);
+  for (int icv = 0; icv < ncv; icv++) {
+    rhs[6 * icv + 0] = c.rhs_rho[icv] * c.p.under_relax;
+    for (int i = 0; i < 3; i++) rhs[6 * icv + 1 + i] = c.rhs_rhou[3 * icv + i] * c.p.under_relax;
+    rhs[6 * icv + 4] = c.rhs_rhoE[icv] * c.p.under_relax;
+    rhs[6 * icv + 5] = c.rhs_scal[icv] * c.p.under_relax;
+  }
+  assemble_blocks6(c, c.A6);
+  for (int icv = 0; icv < ncv; icv++) {
+    double tmp = c.m.cv_volume[icv] / (c.local_dt[icv]);
+    for (int i = 0; i < 6; i++) c.A6[36 * (size_t)c.m.nbocv_i[icv] + 7 * i] += tmp;
+  }
+  c.dq6.assign(6 * (size_t)ncv, 0.0);
+  return bcgstab_b(c, c.dq6.data(), c.A6.data(), rhs.data(), c.p.ls_abs, c.p.ls_rel, c.p.ls_maxiter, 6);
 }
 
 // state + BC refresh done at the start of runBackwardEuler and after each update
@@ -2595,6 +3038,7 @@ void or_residual(or_ctx *c, int implicit) {
   if (implicit) std::fill(c->A.begin(), c->A.end(), 0.0);
   if (c->sa) std::fill(c->AScal.begin(), c->AScal.end(), 0.0);
   calc_rhs(*c, implicit ? c->A.data() : NULL);
+  if (implicit && c->sa && c->p.coupled) assemble_blocks6(*c, c->A6);
 }
 
 // one pass of the loop body of runBackwardEuler (JoeWithModels.cpp:975-1158),
@@ -2607,6 +3051,31 @@ static int step_once(or_ctx &c) {
   std::fill(c.dq.begin(), c.dq.end(), 0.0);
   if (c.sa) std::fill(c.AScal.begin(), c.AScal.end(), 0.0);
   calc_rhs(c, c.A.data());
+  if (c.sa && c.p.coupled) {  // runBackwardEulerCoupled with the scalar (JoeWithModels.cpp:4437-4516)
+    for (size_t i = 0; i < c.resid.size(); i++) c.resid[i] = 0.0;
+    for (int icv = 0; icv < c.ncv; icv++) {
+      c.resid[0] += fabs(c.rhs_rho[icv]);
+      for (int i = 0; i < 3; i++) c.resid[i + 1] += fabs(c.rhs_rhou[3 * icv + i]);
+      c.resid[4] += fabs(c.rhs_rhoE[icv]);
+      c.resid[5] += fabs(c.rhs_scal[icv]);
+    }
+    int it = coupled_solve6(c);
+    for (int icv = 0; icv < c.ncv; icv++) {
+      const double *d = &c.dq6[6 * (size_t)icv];
+      c.rho[icv] += d[0];
+      c.rhou[3 * icv + 0] += d[1];
+      c.rhou[3 * icv + 1] += d[2];
+      c.rhou[3 * icv + 2] += d[3];
+      c.rhoE[icv] += d[4];
+      for (int k = 0; k < 5; k++) c.dq[5 * icv + k] = d[k];
+    }
+    for (int icv = 0; icv < c.ncv; icv++)  // update + clip (:4510-4516)
+      c.phi[icv] = min(max((c.phi[icv] * (c.rho[icv] - c.dq6[6 * (size_t)icv]) + c.dq6[6 * (size_t)icv + 5]) / c.rho[icv],
+                           sa::lowerBound),
+                       sa::upperBound);
+    refresh(c);
+    return it;
+  }
   std::vector<double> rhs(5 * (size_t)c.ncv);
   for (int icv = 0; icv < c.ncv; icv++) {
     rhs[5 * icv + 0] = c.p.under_relax * c.rhs_rho[icv];
@@ -2680,7 +3149,7 @@ long long or_get(const or_ctx *c, const char *name, double *out) {
   if (n == "RHS_RHO") v = &c->rhs_rho;
   else if (n == "RHS_RHOU") v = &c->rhs_rhou;
   else if (n == "RHS_RHOE") v = &c->rhs_rhoE;
-  else if (n == "A") v = &c->A;
+  else if (n == "A") v = (c->sa && c->p.coupled) ? &c->A6 : &c->A;
   else if (n == "LOCAL_DT") v = &c->local_dt;
   else if (n == "GRAD_RHO") v = &c->grad_rho;
   else if (n == "GRAD_P") v = &c->grad_p;
@@ -2698,7 +3167,7 @@ long long or_get(const or_ctx *c, const char *name, double *out) {
   else if (n == "P_BFA") v = &c->p_bfa;
   else if (n == "T_BFA") v = &c->T_bfa;
   else if (n == "H_BFA") v = &c->h_bfa;
-  else if (n == "DQ") v = &c->dq;
+  else if (n == "DQ") v = (c->sa && c->p.coupled) ? &c->dq6 : &c->dq;
   else if (n == "RESID") v = &c->resid;
   else if (n == "RHO") v = &c->rho;
   else if (n == "RHOU") v = &c->rhou;
@@ -2728,6 +3197,7 @@ int or_nsteps_done(const or_ctx *c) { return c->step; }
  * the first-step scaling (JoeWithModels.cpp:1011-1026); result in DQ (and,
  * with SA, the scalar correction in DSCAL). */
 int or_linear_solve(or_ctx *c) {
+  if (c->sa && c->p.coupled) return coupled_solve6(*c);  // DQ [ncv][6]
   std::vector<double> rhs(5 * (size_t)c->ncv);
   for (int icv = 0; icv < c->ncv; icv++) {
     rhs[5 * icv + 0] = c->p.under_relax * c->rhs_rho[icv];

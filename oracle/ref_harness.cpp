@@ -14,7 +14,8 @@
 //                                         after initializeFromRestartFile(RESTART)
 //   rhs-coupled <Joe.in> <state.bin> <out.bin>  one calcRhsCoupled (the
 //                                         residual and (5+nScal)^2 blocks of
-//                                         runBackwardEulerCoupled), no scalars
+//                                         runBackwardEulerCoupled); with -sa
+//                                         the nuSA row and the model fields
 // rhs-sa / solve-sa run the same with the Spalart-Allmaras mix-in (RansTurbSA,
 // as joe.cpp's MyJoeSA); the state file may then carry NUSA.
 //
@@ -280,6 +281,10 @@ class RefJoe : public JoeWithModels {
     }
     const int nScal = scalarTranspEqVector.size();
     const int b = 5 + nScal;
+    if (nScal > 0) {
+      const double *nu = st.f64("NUSA", ncv);
+      for (int icv = 0; icv < ncv; icv++) scalarTranspEqVector[0].phi[icv] = nu[icv];
+    }
     calcStateVariables();
     calcMaterialProperties();
     setBC();
@@ -301,6 +306,17 @@ class RefJoe : public JoeWithModels {
     d.f64("RHS_RHOU", vu.data(), 3LL * ncv);
     for (int icv = 0; icv < ncv; icv++) v[icv] = rhs[icv][4];
     d.f64("RHS_RHOE", v.data(), ncv);
+    if (nScal > 0) {  // the transported scalar's row of the coupled residual and its fields
+      for (int icv = 0; icv < ncv; icv++) v[icv] = rhs[icv][5];
+      d.f64("RHS_NUSA", v.data(), ncv);
+      ScalarTranspEq &eq = scalarTranspEqVector[0];
+      d.f64("NUSA", eq.phi, ncv);
+      d.f64("NUSA_BFA", eq.phi_bfa, nfa_b);
+      d.f64("GRAD_NUSA", &eq.grad_phi[0][0], 3LL * ncv);
+      d.f64("DIFF_NUSA", eq.diff, nfa);
+      d.f64("MUT_FA", mut_fa, nfa);
+      dumpModel(d);
+    }
     for (int noc = 0; noc < nbocv_s; noc++)
       for (int i = 0; i < b; i++)
         for (int j = 0; j < b; j++) ab[((size_t)noc * b + i) * b + j] = A[noc][i][j];

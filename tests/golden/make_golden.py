@@ -94,6 +94,20 @@ COUPLED_CASES = [("bump24", {}), ("box4", {}), ("box4", {"symm_bc": "ANALYTICAL"
                  ("box4", {"visc": "su", "second_order": False})]
 
 
+# the same residual with the Spalart-Allmaras model in the coupled system
+# (nScal = 1, 6x6 blocks): both geometries, LS and GG, first order, all three
+# SYMM_BC modes, every boundary kind, walls and other zones with and without a
+# nuSA value (the viscous scalar flux of a zone without one is switched off)
+SA_BOX_NOWALL = {"inlet": 0.006, "farfield": 0.004}
+COUPLED_SA_CASES = [("bump24", {"visc": "pl"}, cases.SA_BUMP),
+                    ("box4", {"visc": "su"}, cases.SA_BOX),
+                    ("box4", {"visc": "pl", "bcs": cases.BOX_BCS_ALT, "symm_bc": "ANALYTICAL", "grad": "GREENGAUSS"},
+                     cases.SA_BOX),
+                    ("box4", {"visc": "pl", "bcs": cases.BOX_BCS_ALT, "symm_bc": "NEW_NUMERIC"}, SA_BOX_NOWALL),
+                    ("box4", {"visc": "nl", "second_order": False}, cases.SA_BOX),
+                    ("box4", {"visc": "su", "symm_bc": "ORIGINAL_NUMERIC"}, SA_BOX_NOWALL)]
+
+
 def coupled_tag(kw):
     t = "_visc_" + kw["visc"] if "visc" in kw else ""
     if "bcs" in kw:
@@ -231,6 +245,33 @@ def main(only=None):
             d = jmdump.read(os.path.join(wd, "out.bin"))
             out = {k: d[k] for k in RHS_KEYS if k in d}
             out.update(RHO=rho, RHOU=rhou.reshape(-1), RHOE=rhoE, JOE_IN=np.frombuffer(joe.encode(), np.uint8))
+            np.savez_compressed(os.path.join(HERE, fixture + ".npz"), **out)
+            print("wrote", fixture)
+
+        for name, kw, sa_vals in COUPLED_SA_CASES:
+            kw = dict(kw)
+            symm = kw.get("symm_bc")
+            fixture = f"coupled_{name}_hllc" + coupled_tag(kw) + "_sa" + ("_nowall" if "wall" not in sa_vals and
+                                                                        name == "box4" else "")
+            if only and only not in fixture:
+                continue
+            visc = kw.pop("visc")
+            mm, joe = MESHES[name]("HLLC", **kw)
+            joe = joe.replace("MU_REF = 0.0", VISC[visc])
+            joe = cases.with_sa(cases.coupled(joe, keep_symm_bc=symm is not None), 0.005, sa_vals)
+            fn = "box.msh" if name.startswith("box") else "bump.msh"
+            meshgen.write_msh(mm, os.path.join(wd, fn))
+            with open(os.path.join(wd, "Joe.in"), "w") as f:
+                f.write(joe)
+            rho, rhou, rhoE = cases.random_state(mm.ncv, seed=9)
+            nusa = np.random.default_rng(13).uniform(0.001, 0.01, mm.ncv)
+            jmdump.write(os.path.join(wd, "state.bin"), {"RHO": rho, "RHOU": rhou.reshape(-1), "RHOE": rhoE,
+                                                         "NUSA": nusa})
+            run_ref(["rhs-coupled-sa", "Joe.in", "state.bin", "out.bin"], wd)
+            d = jmdump.read(os.path.join(wd, "out.bin"))
+            out = {k: d[k] for k in RHS_KEYS if k in d}
+            out.update(RHO=rho, RHOU=rhou.reshape(-1), RHOE=rhoE, NUSA=nusa,
+                       JOE_IN=np.frombuffer(joe.encode(), np.uint8))
             np.savez_compressed(os.path.join(HERE, fixture + ".npz"), **out)
             print("wrote", fixture)
 

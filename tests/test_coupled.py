@@ -25,6 +25,10 @@ COUPLED = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLD, 
 FIELDS = ["PRESS", "VEL", "RHO_BFA", "VEL_BFA", "P_BFA", "T_BFA", "H_BFA", "LOCAL_DT", "GRAD_RHO", "GRAD_P",
           "GRAD_U", "RHS_RHO", "RHS_RHOU", "RHS_RHOE", "A"]
 VISC_FIELDS = ["GRAD_ENTHALPY", "MUL_FA"]
+# with the Spalart-Allmaras scalar in the coupled system (6x6 blocks): its
+# residual row, boundary values, limited gradient, diffusivity, eddy
+# viscosity and model fields
+SA_FIELDS = ["RHS_NUSA", "NUSA_BFA", "GRAD_NUSA", "DIFF_NUSA", "MUT_FA", "WALL_DIST", "STR_MAG", "VORT_MAG"]
 
 
 def _mesh_key(fixture):
@@ -60,8 +64,11 @@ def test_oracle_coupled_bit_exact(fixture):
     g = load(fixture)
     o = Oracle(mesh_for(_mesh_key(fixture)), joe_text(g))
     o.set_state(g["RHO"], g["RHOU"], g["RHOE"])
+    sa = "_sa" in fixture
+    if sa:
+        o.set_scalar(g["NUSA"])
     o.residual(True)
-    for k in FIELDS + (VISC_FIELDS if "visc" in fixture else []):
+    for k in FIELDS + (VISC_FIELDS if "visc" in fixture else []) + (SA_FIELDS if sa else []):
         if k not in g:
             assert "first" in fixture and k.startswith("GRAD"), k
             continue
