@@ -133,16 +133,20 @@ jm_status jm_init_uniform(jm_ctx *c);
 jm_status jm_set_scalar(jm_ctx *c, const char *name, const double *phi);
 
 /* calcStateVariables + setBC + calcDt + calcRhs (JoeWithModels.cpp:943-1005,
- * 2564-3589) on the resident state; with JM_RHS_IMPLICIT the block Jacobian
- * A[nbocv_s][5][5] is assembled too. Asynchronous on the context's stream. */
+ * 2564-3589; calcRhsCoupled, :4596-4611, with TIME_INTEGRATION =
+ * BACKWARD_EULER_COUPLED) on the resident state; with JM_RHS_IMPLICIT the
+ * block Jacobian A[nbocv_s][b][b] is assembled too (b = 5; 6 in the coupled
+ * loop with TURB_MODEL = SA, the scalar's row and column in every block).
+ * Asynchronous on the context's stream. */
 jm_status jm_calc_rhs(jm_ctx *c, int flags);
 
 /* copy a result out (synchronises the stream): RHS_RHO RHS_RHOU RHS_RHOE A
  * LOCAL_DT GRAD_RHO GRAD_P GRAD_U ALPHA_RHO VEL PRESS RHO_BFA VEL_BFA P_BFA
  * T_BFA H_BFA DQ RESID; with MU_REF > 0 GRAD_ENTHALPY; with TURB_MODEL = SA
  * NUSA NUSA_BFA GRAD_NUSA WALL_DIST STR_MAG VORT_MAG RHS_NUSA ASCAL
- * ([6][nbocv_s], runBackwardEuler's AScal[0]) DSCAL, and RESID has 6 entries.
- * out may be NULL to query *count. */
+ * ([6][nbocv_s], runBackwardEuler's AScal[0]) DSCAL, and RESID has 6 entries
+ * (in the coupled loop A is [nbocv_s][6][6] and DQ [ncv][6], and ASCAL and
+ * DSCAL do not exist). out may be NULL to query *count. */
 jm_status jm_get(jm_ctx *c, const char *name, double *out, int64_t *count);
 
 /* One pass of the runBackwardEuler loop (JoeWithModels.cpp:975-1158): CFL
@@ -156,8 +160,11 @@ typedef struct {
   double cfl;          /* CFL used */
   double dt_min;       /* minimum local time step */
   double resid[5];     /* L1 residuals of rho, rhou(3), rhoE */
-  double resid_scal;   /* L1 residual of nuSA over UNDER_RELAXATION (TURB_MODEL = SA; else 0) */
-  int scal_iters;      /* its Jacobi BiCGStab iterations (solveCvScalarBcgstab) */
+  double resid_scal;   /* L1 residual of nuSA (TURB_MODEL = SA; else 0): over UNDER_RELAXATION
+                          in the segregated loop (JoeWithModels.cpp:1132-1134), as it is in the
+                          coupled loop (:4443-4453) */
+  int scal_iters;      /* its Jacobi BiCGStab iterations (solveCvScalarBcgstab; 0 in the coupled
+                          loop, where nuSA is part of the block system) */
   double resid_l2[5];  /* L2 norms (square root of the sum of squares) of the same residuals;
                           the reference reports L1 only */
 } jm_step_info;
@@ -190,6 +197,16 @@ jm_status jm_add_source(jm_ctx *c, const double *rhs_rho, const double *rhs_rhou
                         const double *a_diag);
 jm_status jm_step_end(jm_ctx *c, jm_step_info *info);
 jm_status jm_update_bc(jm_ctx *c);
+/* sourceHookCoupled (JoeWithModels.h:134-152) between jm_step_begin and
+ * jm_step_end of a TIME_INTEGRATION = BACKWARD_EULER_COUPLED context: rhs
+ * [ncv][b] added to the residual rows (rho, rhou(3), rhoE, then the
+ * scalars) and a_diag [ncv][b][b] to the diagonal blocks A[nbocv_i[icv]],
+ * reference numbering; b = 5 + nScal must be the system's block size (5, or
+ * 6 with TURB_MODEL = SA); either pointer may be NULL. The host's terms are
+ * added after the model's source (sourceHookRansTurbCoupled), where the
+ * reference adds them before it (JoeWithModels.cpp:4607-4609): a scalar row
+ * term may differ in its last bit. */
+jm_status jm_add_source_coupled(jm_ctx *c, int b, const double *rhs, const double *a_diag);
 
 /* JoeWithModels::run's steady loop until doneSolver (JoeWithModels.h:42-95):
  * NSTEPS, RESID_ENERG_TH or NaN. resid_hist may be NULL, else [max_steps][5]. */

@@ -35,6 +35,7 @@ jm_status guard(F &&f) {
   } catch (const jm::Unsupported &e) {
     g_err = e.what();
     return JM_ERR_UNSUPPORTED;
+
   } catch (const std::invalid_argument &e) {
     g_err = e.what();
     return JM_ERR_PARAM;
@@ -340,6 +341,14 @@ jm_status jm_add_source(jm_ctx *c, const double *rhs_rho, const double *rhs_rhou
   return guard([&] {
     if (!c) throw std::exception();
     c->c->add_source(rhs_rho, rhs_rhou, rhs_rhoE, a_diag);
+    return JM_OK;
+  });
+}
+
+jm_status jm_add_source_coupled(jm_ctx *c, int b, const double *rhs, const double *a_diag) {
+  return guard([&] {
+    if (!c) throw std::exception();
+    c->c->add_source_coupled(rhs, a_diag, b);
     return JM_OK;
   });
 }

@@ -45,8 +45,9 @@ struct SolverParams {
   int dt_mode = 0;  // 0 CFL (local), 1 CFL_MIN, 2 DT
   bool dt_sum = false;  // TIME_STEP = SUM_BASED (MAX_BASED otherwise)
   // TIME_INTEGRATION = BACKWARD_EULER_COUPLED: calcRhsCoupled's fluxes and
-  // Jacobian (scheme kHllcC; no scalars, inviscid) in the same backward-Euler
-  // loop, the block-Jacobi BiCGStab standing in for the reference's PETSc GMRES
+  // Jacobian (scheme kHllcC; with TURB_MODEL = SA the scalar in 6x6 blocks) in
+  // the same backward-Euler loop, the block-Jacobi BiCGStab standing in for
+  // the reference's PETSc GMRES
   bool coupled = false;
   double cfl = 0.0, dt_fixed = 0.0;
   double under_relax = 0.3;
@@ -197,6 +198,9 @@ class Ctx {
   // state and boundary values and returns the step's residuals.
   void step_begin(bool host_source);
   void add_source(const double *rhs_rho, const double *rhs_rhou, const double *rhs_rhoE, const double *a_diag);
+  // sourceHookCoupled's terms in the system's own blocks: rhs [ncv][b],
+  // a_diag [ncv][b][b] (reference numbering; b the system's block size)
+  void add_source_coupled(const double *rhs, const double *a_diag, int b);
   StepInfo step_end();
   // calcStateVariables + setBC on the resident state (after new HOOK values)
   void update_bc();

@@ -552,12 +552,16 @@ __device__ __forceinline__ void jac_roe(double J[5][5], int which, double rhoL, 
 }
 
 // The coupled HLLC of runBackwardEulerCoupled: calcEulerFluxCoupled_HLLC
-// (UgpWithCvCompFlow.cpp:1808-1927) with nScal = 0 and kine = 0. The energy
-// is built from the given side enthalpies h0, h1 (calcThermoProp_p's, or the
-// boundary state's), not from gamma/(gamma-1) p/rho as flux_hllc does.
+// (UgpWithCvCompFlow.cpp:1808-1927) with kine = 0. The energy is built from
+// the given side enthalpies h0, h1 (calcThermoProp_p's, or the boundary
+// state's), not from gamma/(gamma-1) p/rho as flux_hllc does. With Fs, one
+// transported scalar (ConvTerm = 1) gets the upwind side's value (scL or scR
+// by the sign of sM) times the unscaled mass flux, then times the area
+// (:1851-1896, 1926).
 __device__ __forceinline__ void flux_hllc_coupled(double F[5], double rhoL, const double *uL, double pL, double h0,
                                                   double rhoR, const double *uR, double pR, double h1, double gam,
-                                                  double area, const double *nVec) {
+                                                  double area, const double *nVec, double scL = 0.0,
+                                                  double scR = 0.0, double *Fs = nullptr) {
   const double gammaL = gam, gammaR = gam, kL = 0.0, kR = 0.0;
   double unL = dot3(uL, nVec);
   double uLuL = dot3(uL, uL);
@@ -607,6 +611,7 @@ __device__ __forceinline__ void flux_hllc_coupled(double F[5], double rhoL, cons
     for (int i = 0; i < 3; i++) F[1 + i] = rhouS[i] * sM + pStar * nVec[i];
     F[4] = (rhoeS + pStar) * sM;
   }
+  if (Fs) *Fs = (1.0 * F[0] * (left ? scL : scR)) * area;
 #pragma unroll
   for (int i = 0; i < 5; i++) F[i] *= area;
 }
@@ -728,6 +733,124 @@ __device__ __forceinline__ void jac_hllc_coupled(double J[5][5], int which, doub
   if (own)
 #pragma unroll
     for (int i = 0; i < 5; i++) J[i][i] += area * Omega * sM * S;
+}
+
+// The transported scalar's row of the coupled HLLC Jacobian of one face side
+// (which 0: A_L[5][0..5], 1: A_R[5][0..5]): calcEulerFluxMatricesCoupled_HLLC
+// (UgpWithCvCompFlow.cpp:1934-2174) with one scalar and dpress_dscal = 0
+// (RansTurbSA leaves pressureDerivativeHookScalarRansTurb empty,
+// UgpWithCvCompFlow.h:1590): the supersonic row of calcJacobianCoupled_HLLC
+// (:2233-2240), the subsonic transformation (:2281-2284) with the star-state
+// terms (:2022-2026, 2074-2078), then dU/dQ (:2150-2174). With dpress_dscal = 0
+// the flow rows keep a zero scalar column and their column 0 unchanged, so
+// only this row is new. The same side selection as jac_hllc_coupled.
+__device__ __forceinline__ void jac_hllc_coupled_srow(double S[6], int which, double rhoL, const double *uL,
+                                                      double pL, double h0, double scL, double rhoR,
+                                                      const double *uR, double pR, double h1, double scR,
+                                                      double gam, double area, const double *nVec) {
+  const double gammaL = gam, gammaR = gam;
+  double unL = dot3(uL, nVec);
+  double uLuL = dot3(uL, uL);
+  double cL = sqrt(gammaL * pL / rhoL);
+  double hL = h0 + 0.0;
+  double unR = dot3(uR, nVec);
+  double uRuR = dot3(uR, uR);
+  double cR = sqrt(gammaR * pR / rhoR);
+  double hR = h1 + 0.0;
+  double Rrho = sqrt(rhoR / rhoL);
+  double tmp = 1.0 / (1.0 + Rrho);
+  double velRoe[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) velRoe[i] = tmp * (uL[i] + uR[i] * Rrho);
+  double uRoe = dot3(velRoe, nVec);
+  double gamPdivRho = tmp * ((gammaL * pL / rhoL + 0.5 * (gammaL - 1.0) * uLuL) +
+                             (gammaR * pR / rhoR + 0.5 * (gammaR - 1.0) * uRuR) * Rrho);
+  double cRoe = sqrt(gamPdivRho - ((gammaL + gammaR) * 0.5 - 1.0) * 0.5 * dot3(velRoe, velRoe));
+  double sL = Dev::mn(uRoe - cRoe, unL - cL);
+  double sR = Dev::mx(uRoe + cRoe, unR + cR);
+  double invrhotld = 1.0 / (rhoR * (sR - unR) - rhoL * (sL - unL));
+  double sM = (pL - pR - rhoL * unL * (sL - unL) + rhoR * unR * (sR - unR)) / (rhoR * (sR - unR) - rhoL * (sL - unL));
+  double nVecArea[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) nVecArea[i] = nVec[i] * area;
+  const bool left = sM >= 0.0;  // upwind side
+  const bool sup = left ? (sL > 0.0) : !(sR >= 0.0);
+  const bool own = (which == 0) == left;
+  const double u[3] = {which == 0 ? uL[0] : uR[0], which == 0 ? uL[1] : uR[1], which == 0 ? uL[2] : uR[2]};
+  const double p = which == 0 ? pL : pR, rho = which == 0 ? rhoL : rhoR, h = which == 0 ? hL : hR;
+  const double sc = which == 0 ? scL : scR;
+  if (own) {  // calcJacobianCoupled_HLLC's scalar row, nV = nVecArea
+    double nVel[3] = {u[0] * nVecArea[0], u[1] * nVecArea[1], u[2] * nVecArea[2]};
+    double U_k = nVel[0] + nVel[1] + nVel[2];
+    S[0] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) S[1 + i] = sc * nVecArea[i];
+    S[4] = 0.0;
+    S[5] = rho * U_k;
+  } else {
+#pragma unroll
+    for (int j = 0; j < 6; j++) S[j] = 0.0;
+  }
+  if (!sup) {
+    const double Su = left ? sL : sR, unu = left ? unL : unR, rhou_ = left ? rhoL : rhoR, scu = left ? scL : scR;
+    double Omega = 1.0 / (Su - sM);
+    double rhoS = rhou_ * (Su - unu) * Omega;
+    // this side's dSM/dU (calcdSMdU_HLLC with nV the unit normal; the
+    // scalar entry is Factor * invrhotld * dpress_dscal = 0)
+    const double Sd = which == 0 ? sL : sR, Factor = which == 0 ? 1.0 : -1.0;
+    double d[6];
+    {
+      double kapm1 = (gam - 1.0);
+      double nVel[3] = {u[0] * nVec[0], u[1] * nVec[1], u[2] * nVec[2]};
+      double U_k = nVel[0] + nVel[1] + nVel[2];
+      double vSquHlf = 0.5 * dot3(u, u);
+      d[0] = Factor * invrhotld * (-U_k * U_k + kapm1 * vSquHlf + gam * p / rho - kapm1 * h + sM * Sd);
+#pragma unroll
+      for (int i = 0; i < 3; i++) d[1 + i] = Factor * invrhotld * ((2 * U_k - Sd - sM) * nVec[i] - kapm1 * u[i]);
+      d[4] = Factor * invrhotld * kapm1;
+      d[5] = Factor * invrhotld * 0.0;
+    }
+    double OmSM = Omega * sM;
+    double OmSMp1 = OmSM + 1.0;
+#pragma unroll
+    for (int i = 0; i < 6; i++) S[i] = -OmSM * S[i] + area * OmSMp1 * d[i] * rhoS * scu;
+    if (own) {
+      S[0] += area * Omega * sM * Su * sc;
+      S[5] += area * Omega * sM * Su * rho;
+    }
+  }
+  // dU/dQ of this side (ConvTerm = 1)
+#pragma unroll
+  for (int j = 0; j < 6; j++) S[j] *= 1.0;
+  S[0] -= sc / rho * S[5];
+  S[5] /= rho;
+}
+
+// The transported scalar's row of calcViscousFluxCoupled (UgpWithCvCompFlow.cpp
+// :3113-3120 flux, :3158-3167 and :3217-3226 dF/dU, :3185-3190 and :3244-3249
+// dU/dQ): Fs and the row's entries [5][0], [5][5] of side 0 (a0) and side 1 (a1)
+__device__ __forceinline__ double visc_coupled_srow(double a0[2], double a1[2], double rho0, double scal0,
+                                                    const double *gs0, double rho1, double scal1, const double *gs1,
+                                                    double diff, double DiffTerm, double area, const double *nVec,
+                                                    double smag, const double *sVec, double alpha) {
+  double fCorr[3] = {nVec[0] - alpha * sVec[0], nVec[1] - alpha * sVec[1], nVec[2] - alpha * sVec[2]};
+  double Fs = -area * DiffTerm * diff *
+              (alpha * (scal1 - scal0) / smag +
+               0.5 * ((gs0[0] + gs1[0]) * fCorr[0] + (gs0[1] + gs1[1]) * fCorr[1] + (gs0[2] + gs1[2]) * fCorr[2]));
+  double area_alpha_smag = area * alpha / smag;
+  {
+    double dFdU = DiffTerm * diff * area_alpha_smag;
+    double invRho = 1.0 / rho0;
+    a0[0] = dFdU * (-invRho * scal0);
+    a0[1] = dFdU * invRho;
+  }
+  {
+    double dFdU = -DiffTerm * diff * area_alpha_smag;
+    double invRho = 1.0 / rho1;
+    a1[0] = dFdU * (-invRho * scal1);
+    a1[1] = dFdU * invRho;
+  }
+  return Fs;
 }
 
 // The ONLY_PRESSURE Jacobian of a symmetry or wall face (A_L), coupled form

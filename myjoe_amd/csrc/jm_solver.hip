@@ -1165,12 +1165,13 @@ __device__ __forceinline__ VSide vsel(bool c, const VSide &x, const VSide &y) {
 }
 
 // The coupled loop's viscous flux, calcViscousFluxCoupled
-// (UgpWithCvCompFlow.cpp:3040-3124), nScal = 0, no turbulence model: F[5]
-// (F[0] = 0), and tau_ij n_j and keff for the blocks below.
+// (UgpWithCvCompFlow.cpp:3040-3124): F[5] (F[0] = 0), and tau_ij n_j and keff
+// for the blocks below. With a turbulence model (turb) the kinetic-energy
+// terms are formed with kine = 0 (:3084-3085, 3108-3110).
 __device__ __forceinline__ void visc_coupled(const DParams &P, const VSide &s0, const VSide &s1, double mul,
                                              double mut, const double *u_fa, double area, const double *nVec,
                                              double smag, const double *sVec, double alpha, double F[5],
-                                             double tau[3], double &keff) {
+                                             double tau[3], double &keff, bool turb = false) {
   double g[3][3];
   for (int i = 0; i < 3; i++)
     for (int j = 0; j < 3; j++) g[i][j] = 0.5 * (s0.gu[3 * i + j] + s1.gu[3 * i + j]);
@@ -1181,6 +1182,7 @@ __device__ __forceinline__ void visc_coupled(const DParams &P, const VSide &s0, 
     tau[i] = mu * (alpha * (s1.u[i] - s0.u[i]) / smag + g[i][0] * fCorr[0] + g[0][i] * nVec[0] + g[i][1] * fCorr[1] +
                    g[1][i] * nVec[1] + g[i][2] * fCorr[2] + g[2][i] * nVec[2]);
   double tmp = 2.0 / 3.0 * mu * (g[0][0] + g[1][1] + g[2][2]);
+  if (turb) tmp += 1.0 / 3.0 * (s0.rho + s1.rho) * 0.0;
   for (int i = 0; i < 3; i++) tau[i] -= tmp * nVec[i];
   for (int i = 0; i < 3; i++) F[1 + i] = -area * tau[i];
   keff = mul / P.pr + mut / P.pr_turb;
@@ -1188,6 +1190,7 @@ __device__ __forceinline__ void visc_coupled(const DParams &P, const VSide &s0, 
                         0.5 * ((s0.gh[0] + s1.gh[0]) * fCorr[0] + (s0.gh[1] + s1.gh[1]) * fCorr[1] +
                                (s0.gh[2] + s1.gh[2]) * fCorr[2]));
   double psi = 0.0;
+  if (turb) psi = mu * (alpha * (0.0 - 0.0) / smag);
   F[4] = -area * (enth + tau[0] * u_fa[0] + tau[1] * u_fa[1] + tau[2] * u_fa[2] + psi);
 }
 
@@ -1267,14 +1270,18 @@ __device__ __forceinline__ double sa_mut_face(double w0, double w1, double rho0,
 // cell-centre states, gathered per row; optionally the backward-Euler
 // diagonal V/dt and the under-relaxed right-hand side (:1011-1024).
 // ---------------------------------------------------------------------------
-template <typename T>
+// the flow rows of a block of row stride NB (5, or 6 in the coupled loop with
+// one scalar, whose flow rows have a zero scalar column)
+template <typename T, int NB = 5>
 __device__ __forceinline__ void store_block(T *dst, const double J[5][5], bool negate, bool accumulate) {
-  for (int i = 0; i < 5; i++)
+  for (int i = 0; i < 5; i++) {
     for (int j = 0; j < 5; j++) {
-      double v = negate ? (accumulate ? (double)dst[5 * i + j] - J[i][j] : 0.0 - J[i][j])
-                        : (accumulate ? (double)dst[5 * i + j] + J[i][j] : 0.0 + J[i][j]);
-      dst[5 * i + j] = (T)v;
+      double v = negate ? (accumulate ? (double)dst[NB * i + j] - J[i][j] : 0.0 - J[i][j])
+                        : (accumulate ? (double)dst[NB * i + j] + J[i][j] : 0.0 + J[i][j]);
+      dst[NB * i + j] = (T)v;
     }
+    if (NB == 6 && !accumulate) dst[NB * i + 5] = (T)0.0;
+  }
 }
 
 __device__ __forceinline__ void face_jac(int scheme, double J[5][5], int which, const Side &L, const Side &R,
@@ -1285,7 +1292,7 @@ __device__ __forceinline__ void face_jac(int scheme, double J[5][5], int which, 
     jac_hllc(J, which, L.rho, L.u, L.p, R.rho, R.u, R.p, gam, area, n);
 }
 
-template <typename T, int SCHEME, bool VISC>
+template <typename T, int SCHEME, bool VISC, int NB = 5>
 __device__ __forceinline__ void d_jac(const Blk &B, DMesh m, DParams P, const double *__restrict__ prim,
                                                 const double *__restrict__ bfa, T *__restrict__ A, int step_mode,
                                                 const double *__restrict__ dt, const double *__restrict__ rhs,
@@ -1316,7 +1323,7 @@ __device__ __forceinline__ void d_jac(const Blk &B, DMesh m, DParams P, const do
       nb.rho = pn[0]; nb.u[0] = pn[1]; nb.u[1] = pn[2]; nb.u[2] = pn[3]; nb.p = pn[4];
       const int noc = m.cfl_noc[e];
       const bool accum = (noc >> 30) & 1;
-      T *off = A + 25 * (size_t)(noc & 0x3fffffff);
+      T *off = A + NB * NB * (size_t)(noc & 0x3fffffff);
       // this cell is icv0: A[noc00] += Apl, A[noc01] += Ami; icv1:
       // A[noc11] -= Ami, A[noc10] -= Apl. Both blocks are evaluated once on
       // the lane's own (icv0, icv1) sides, so lanes of either kind share the
@@ -1336,14 +1343,14 @@ __device__ __forceinline__ void d_jac(const Blk &B, DMesh m, DParams P, const do
         for (int i = 0; i < 5; i++)
           for (int j = 0; j < 5; j++) D[i][j] += J[i][j];
       } else {
-        store_block(off, J, true, accum);
+        store_block<T, NB>(off, J, true, accum);
       }
       if (SCHEME == kHllcC)
         jac_hllc_coupled(J, 1, l.rho, l.u, l.p, hl, r.rho, r.u, r.p, hr, gam, area, n);
       else
         face_jac(SCHEME, J, 1, l, r, gam, area, n);  // Ami
       if (s0) {
-        store_block(off, J, false, accum);
+        store_block<T, NB>(off, J, false, accum);
       } else {
         for (int i = 0; i < 5; i++)
           for (int j = 0; j < 5; j++) D[i][j] -= J[i][j];
@@ -1358,19 +1365,24 @@ __device__ __forceinline__ void d_jac(const Blk &B, DMesh m, DParams P, const do
         double sV[3], smag, mul, w0, w1;
         visc_internal_geom(P, xa, xb, xfa, a.T, b.T, sV, smag, mul, w0, w1);
         const double alpha = dot3(n, sV);
+        double mut = 0.0;
+        if (P.sa) {
+          const double nuo = P.sa_phi[c], nun = P.sa_phi[en.y];
+          mut = sa_mut_face(w0, w1, a.rho, b.rho, s0 ? nuo : nun, s0 ? nun : nuo, mul);
+        }
         const double ufa[3] = {0.5 * (a.u[0] + b.u[0]), 0.5 * (a.u[1] + b.u[1]), 0.5 * (a.u[2] + b.u[2])};
         double VF[5], tau[3], keff;
-        visc_coupled(P, a, b, mul, 0.0, ufa, area, n, smag, sV, alpha, VF, tau, keff);
+        visc_coupled(P, a, b, mul, mut, ufa, area, n, smag, sV, alpha, VF, tau, keff, P.sa != 0);
         // icv0: A[noc00] += A0, A[noc01] += A1; icv1: A[noc11] -= A1, A[noc10] -= A0
-        visc_coupled_jac(P, J, s0 ? 0 : 1, vsel(s0, a, b), mul, keff, tau, ufa, area, alpha, smag);
+        visc_coupled_jac(P, J, s0 ? 0 : 1, vsel(s0, a, b), mul + mut, keff, tau, ufa, area, alpha, smag);
         for (int i = 0; i < 5; i++)
           for (int j = 0; j < 5; j++)
             if (s0) D[i][j] += J[i][j];
             else D[i][j] -= J[i][j];
-        visc_coupled_jac(P, J, s0 ? 1 : 0, vsel(s0, b, a), mul, keff, tau, ufa, area, alpha, smag);
+        visc_coupled_jac(P, J, s0 ? 1 : 0, vsel(s0, b, a), mul + mut, keff, tau, ufa, area, alpha, smag);
         for (int i = 0; i < 5; i++)
           for (int j = 0; j < 5; j++)
-            off[5 * i + j] = (T)(s0 ? (double)off[5 * i + j] + J[i][j] : (double)off[5 * i + j] - J[i][j]);
+            off[NB * i + j] = (T)(s0 ? (double)off[NB * i + j] + J[i][j] : (double)off[NB * i + j] - J[i][j]);
       }
     } else {
       const ZoneDev &z = m.zones[-en.y - 1];
@@ -1409,10 +1421,13 @@ __device__ __forceinline__ void d_jac(const Blk &B, DMesh m, DParams P, const do
           double sArg[3], smag, alpha;
           visc_coupled_bgeom(z.bc, xfa, pc + 5, n, sArg, smag, alpha);
           const double mul = mu_lam(P, bb[5]);
+          const double mut = (P.sa && z.bc != kBcWall)
+                                 ? sa_mut(pc[0] * P.sa_phi[c], P.sa_aux[kAuxN * (size_t)c + kAuxMue])
+                                 : 0.0;
           const VSide l = vsel(z.bc == kBcNeumann, vb, vo);
           double VF[5], tau[3], keff;
-          visc_coupled(P, l, vb, mul, 0.0, vb.u, area, n, smag, sArg, alpha, VF, tau, keff);
-          visc_coupled_jac(P, J, 0, l, mul, keff, tau, vb.u, area, alpha, smag);
+          visc_coupled(P, l, vb, mul, mut, vb.u, area, n, smag, sArg, alpha, VF, tau, keff, P.sa != 0);
+          visc_coupled_jac(P, J, 0, l, mul + mut, keff, tau, vb.u, area, alpha, smag);
           for (int i = 0; i < 5; i++)
             for (int j = 0; j < 5; j++) D[i][j] += J[i][j];
         }
@@ -1529,22 +1544,24 @@ __device__ __forceinline__ void d_jac(const Blk &B, DMesh m, DParams P, const do
     double tmp = vol / (dt[c]);
     for (int i = 0; i < 5; i++) D[i][i] += tmp;
     for (int k = 0; k < 5; k++) {
-      rhs_ls[5 * (size_t)c + k] = (T)(relax * rhs[(size_t)k * m.ncv + c]);
-      phi[5 * (size_t)c + k] = (T)0.0;
+      rhs_ls[NB * (size_t)c + k] = (T)(relax * rhs[(size_t)k * m.ncv + c]);
+      phi[NB * (size_t)c + k] = (T)0.0;
     }
   }
-  T *d = A + 25 * (size_t)row;
-  for (int i = 0; i < 5; i++)
-    for (int j = 0; j < 5; j++) d[5 * i + j] = (T)D[i][j];
+  T *d = A + NB * NB * (size_t)row;
+  for (int i = 0; i < 5; i++) {
+    for (int j = 0; j < 5; j++) d[NB * i + j] = (T)D[i][j];
+    if (NB == 6) d[NB * i + 5] = (T)0.0;
+  }
 }
 
-template <typename T, int SCHEME, bool VISC>
+template <typename T, int SCHEME, bool VISC, int NB = 5>
 __global__ void __launch_bounds__(kBlock) k_jac(DMesh m, DParams P, const double *__restrict__ prim,
                                                 const double *__restrict__ bfa, T *__restrict__ A, int step_mode,
                                                 const double *__restrict__ dt, const double *__restrict__ rhs,
                                                 double relax, T *__restrict__ rhs_ls, T *__restrict__ phi,
                                                 const double *__restrict__ gradv) {
-  d_jac<T, SCHEME, VISC>(hw_blk(), m, P, prim, bfa, A, step_mode, dt, rhs, relax, rhs_ls, phi, gradv);
+  d_jac<T, SCHEME, VISC, NB>(hw_blk(), m, P, prim, bfa, A, step_mode, dt, rhs, relax, rhs_ls, phi, gradv);
 }
 
 // ---------------------------------------------------------------------------
@@ -1876,6 +1893,244 @@ __global__ void __launch_bounds__(kBlock) k_scal(DMesh m, DParams P, const doubl
 }
 
 // ---------------------------------------------------------------------------
+// The transported scalar's row of the coupled system (runBackwardEulerCoupled
+// with Spalart-Allmaras, 6x6 blocks), one thread per cell, gathered in the
+// reference's order: per face of calcFluxCoupled (JoeWithModels.cpp
+// :4797-5796) the HLLC scalar flux (calcEulerFluxCoupled_HLLC, scalars
+// upwinded by sM, side enthalpies after reconstruction) and its Jacobian row
+// at cell-centre states (:5035-5048, 5150-5740), then the same face's scalar
+// diffusion with the SA diffusivity and its row entries [5][0], [5][5]
+// (calcViscousFluxCoupled, :5076-5105, 5373-5404, 5548-5575, 5744-5775;
+// diffusivityHookScalarRansTurb, TurbModel_SA.h:130-172); then the source
+// (sourceHookRansTurbCoupled, TurbModel_SA.h:359-434). Writes rhs_s and, with
+// A, row 5 of every block of the cell's row (A [nnz][6][6] in precision T);
+// in step mode also V/dt on [5][5], the relaxed right-hand side and a zero
+// initial guess of entry 5 (:4455-4473).
+// ---------------------------------------------------------------------------
+template <typename T, bool SND>
+__global__ void __launch_bounds__(kBlock) k_scal_c(DMesh m, DParams P, const double *__restrict__ prim,
+                                                   const double *__restrict__ grad, const double *__restrict__ bfa,
+                                                   const double *__restrict__ gradv, double *__restrict__ rhs_s,
+                                                   T *__restrict__ A, int step_mode, const double *__restrict__ dt,
+                                                   double relax, T *__restrict__ rhs_ls, T *__restrict__ x0) {
+  const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (t >= m.ncv) return;
+  const int c = t;
+  const double gam = P.gamma;
+  const double *pc = prim + 8 * (size_t)c;
+  const double *gc = grad + 16 * (size_t)c;
+  const double *phi = P.sa_phi;
+  const double *auxc = P.sa_aux + kAuxN * (size_t)c;
+  const double *gpc = auxc + kAuxGrad;
+  const double ph = phi[c];
+  const double invSigma = 1.0 / kSaSigma;
+  Side own;
+  own.rho = pc[0]; own.u[0] = pc[1]; own.u[1] = pc[2]; own.u[2] = pc[3]; own.p = pc[4];
+  const double hown = enthalpy_of(P, own.rho, own.p);
+  const double Tc = pc[4] / (pc[0] * P.R);
+  double acc = 0.0;
+  double D[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double S[6], R6[6], F[5], Fs, a0[2], a1[2];
+  const int e0 = m.cfl_i[c], e1 = m.cfl_i[c + 1];
+  for (int e = e0; e < e1; e++) {
+    const int2 en = m.cfl[e];
+    const int f = en.x & 0x7fffffff;
+    const double *geo = m.fgeo + 8 * (size_t)f;
+    const double n[3] = {geo[0], geo[1], geo[2]};
+    const double area = geo[3];
+    const double xfa[3] = {geo[4], geo[5], geo[6]};
+    if (en.y >= 0) {
+      const int nb = en.y;
+      const double *pn = prim + 8 * (size_t)nb;
+      const double phn = phi[nb];
+      const double *gpn = P.sa_aux + kAuxN * (size_t)nb + kAuxGrad;
+      Side nbs;
+      nbs.rho = pn[0]; nbs.u[0] = pn[1]; nbs.u[1] = pn[2]; nbs.u[2] = pn[3]; nbs.p = pn[4];
+      Side so = own, sn = nbs;
+      double sco = ph, scn = phn;
+      if (SND) {
+        if (reconstruct(so, pc, gc, xfa)) {
+          const double r[3] = {xfa[0] - pc[5], xfa[1] - pc[6], xfa[2] - pc[7]};
+          sco = sco + dot3(r, gpc);
+        }
+        if (reconstruct(sn, pn, grad + 16 * (size_t)nb, xfa)) {
+          const double r[3] = {xfa[0] - pn[5], xfa[1] - pn[6], xfa[2] - pn[7]};
+          scn = scn + dot3(r, gpn);
+        }
+      }
+      const bool s0 = en.x >= 0;
+      const int noc = m.cfl_noc[e];
+      const bool accum = (noc >> 30) & 1;
+      T *off = A ? A + 36 * (size_t)(noc & 0x3fffffff) + 30 : nullptr;
+      // the face's (icv0, icv1) sides; the flux and both rows evaluated once,
+      // routed by this cell's side (acc - x == acc + (-x))
+      const Side fl = s0 ? so : sn, fr = s0 ? sn : so;
+      const double phl = s0 ? sco : scn, phr = s0 ? scn : sco;
+      flux_hllc_coupled(F, fl.rho, fl.u, fl.p, enthalpy_of(P, fl.rho, fl.p), fr.rho, fr.u, fr.p,
+                        enthalpy_of(P, fr.rho, fr.p), gam, area, n, phl, phr, &Fs);
+      acc += s0 ? -Fs : Fs;
+      if (A) {
+        const Side jl = s0 ? own : nbs, jr = s0 ? nbs : own;
+        const double hjl = enthalpy_of(P, jl.rho, jl.p), hjr = enthalpy_of(P, jr.rho, jr.p);
+        const double pjl = s0 ? ph : phn, pjr = s0 ? phn : ph;
+        jac_hllc_coupled_srow(S, 0, jl.rho, jl.u, jl.p, hjl, pjl, jr.rho, jr.u, jr.p, hjr, pjr, gam, area, n);
+        jac_hllc_coupled_srow(R6, 1, jl.rho, jl.u, jl.p, hjl, pjl, jr.rho, jr.u, jr.p, hjr, pjr, gam, area, n);
+        // icv0: A[noc00] += SL, A[noc01] += SR; icv1: A[noc11] -= SR, A[noc10] -= SL
+        for (int k = 0; k < 6; k++) {
+          if (s0) {
+            D[k] += S[k];
+            off[k] = (T)(accum ? (double)off[k] + R6[k] : 0.0 + R6[k]);
+          } else {
+            D[k] -= R6[k];
+            off[k] = (T)(accum ? (double)off[k] - S[k] : 0.0 - S[k]);
+          }
+        }
+      }
+      {  // the face's scalar diffusion (:5076-5105)
+        const double Tn = pn[4] / (pn[0] * P.R);
+        double sV[3], smag, mul, w0, w1;
+        visc_internal_geom(P, s0 ? pc + 5 : pn + 5, s0 ? pn + 5 : pc + 5, xfa, s0 ? Tc : Tn, s0 ? Tn : Tc, sV, smag,
+                           mul, w0, w1);
+        const double rho0 = s0 ? pc[0] : pn[0], rho1 = s0 ? pn[0] : pc[0];
+        const double nu0 = s0 ? ph : phn, nu1 = s0 ? phn : ph;
+        const double *g0 = s0 ? gpc : gpn, *g1 = s0 ? gpn : gpc;
+        const double rho_fa = (w1 * rho0 + w0 * rho1) / (w0 + w1);
+        const double nuSA_fa = (w1 * nu0 + w0 * nu1) / (w0 + w1);
+        const double diff = invSigma * (mul + rho_fa * nuSA_fa);
+        const double VFs =
+            visc_coupled_srow(a0, a1, rho0, nu0, g0, rho1, nu1, g1, diff, 1.0, area, n, smag, sV, dot3(n, sV));
+        acc += s0 ? -VFs : VFs;
+        if (A) {  // icv0: [noc00] += a0, [noc01] += a1; icv1: [noc11] -= a1, [noc10] -= a0
+          if (s0) {
+            D[0] += a0[0];
+            D[5] += a0[1];
+            off[0] = (T)((double)off[0] + a1[0]);
+            off[5] = (T)((double)off[5] + a1[1]);
+          } else {
+            D[0] -= a1[0];
+            D[5] -= a1[1];
+            off[0] = (T)((double)off[0] - a0[0]);
+            off[5] = (T)((double)off[5] - a0[1]);
+          }
+        }
+      }
+    } else {
+      const ZoneDev &z = m.zones[-en.y - 1];
+      const double *bb = bfa + 8 * (size_t)f;
+      const double rb = bb[0], pb = bb[4], hb = bb[6], phb = bb[7];
+      const double ub[3] = {bb[1], bb[2], bb[3]};
+      // a zone without a nuSA value carries no diffusive scalar flux
+      // (DiffTerm = 0, :5453-5461, 5599-5607); NEUMANN zones keep it (:5293-5296)
+      const double DiffTerm = (z.bc == kBcNeumann || z.skind) ? 1.0 : 0.0;
+      const double mul = mu_lam(P, bb[5]);
+      const double diff = z.bc == kBcWall ? invSigma * mul : invSigma * (mul + rb * phb);
+      if (z.bc == kBcSymmetry || z.bc == kBcWall || z.bc == kBcNeumann) {
+        const bool op = z.bc != kBcNeumann && P.symm_bc == kSymmAnalytical;  // ONLY_PRESSURE (:1910-1917)
+        if (op) {
+          Fs = 0.0 * area;
+        } else {
+          flux_hllc_coupled(F, rb, ub, pb, hb, rb, ub, pb, hb, gam, area, n, phb, phb, &Fs);
+        }
+        acc += -Fs;
+        if (A) {
+          if (op) {  // a zero row through dU/dQ
+            for (int k = 0; k < 6; k++) S[k] = 0.0 * 1.0;
+            S[0] -= phb / rb * S[5];
+            S[5] /= rb;
+            for (int k = 0; k < 6; k++) D[k] += S[k];
+          } else {
+            jac_hllc_coupled_srow(S, 0, rb, ub, pb, hb, phb, rb, ub, pb, hb, phb, gam, area, n);
+            if (P.symm_bc == kSymmNew) {
+              jac_hllc_coupled_srow(R6, 1, rb, ub, pb, hb, phb, rb, ub, pb, hb, phb, gam, area, n);
+              for (int k = 0; k < 6; k++) D[k] += S[k] + R6[k];
+            } else {
+              for (int k = 0; k < 6; k++) D[k] += S[k];
+            }
+          }
+        }
+        if (z.bc != kBcSymmetry) {  // wall (:5548-5575) and NEUMANN (:5373-5404) diffusion along the normal
+          const double sV[3] = {xfa[0] - pc[5], xfa[1] - pc[6], xfa[2] - pc[7]};
+          const double smag = fabs(dot3(sV, n));
+          const double zero3[3] = {0.0, 0.0, 0.0};
+          const bool neu = z.bc == kBcNeumann;
+          const double VFs = visc_coupled_srow(a0, a1, neu ? rb : pc[0], neu ? phb : ph, neu ? zero3 : gpc, rb, phb,
+                                               neu ? zero3 : gpc, diff, DiffTerm, area, n, smag, n, 1.0);
+          acc += -VFs;
+          if (A) {
+            D[0] += a0[0];
+            D[5] += a0[1];
+          }
+        }
+      } else {
+        Side so = own;
+        double sco = ph;
+        if (SND && reconstruct(so, pc, gc, xfa)) {
+          const double r[3] = {xfa[0] - pc[5], xfa[1] - pc[6], xfa[2] - pc[7]};
+          sco = sco + dot3(r, gpc);
+        }
+        flux_hllc_coupled(F, so.rho, so.u, so.p, enthalpy_of(P, so.rho, so.p), rb, ub, pb, hb, gam, area, n, sco, phb,
+                          &Fs);
+        acc += -Fs;
+        if (A) {  // cell-centre state against the boundary state (:5731-5740)
+          jac_hllc_coupled_srow(S, 0, own.rho, own.u, own.p, hown, ph, rb, ub, pb, hb, phb, gam, area, n);
+          for (int k = 0; k < 6; k++) D[k] += S[k];
+        }
+        {  // the cell against the boundary state; its side-1 scalar is the
+           // reconstructed face value (:5744-5775)
+          double sArg[3], smag, alpha;
+          visc_coupled_bgeom(z.bc, xfa, pc + 5, n, sArg, smag, alpha);
+          const double VFs =
+              visc_coupled_srow(a0, a1, pc[0], ph, gpc, rb, sco, gpc, diff, DiffTerm, area, n, smag, sArg, alpha);
+          acc += -VFs;
+          if (A) {
+            D[0] += a0[0];
+            D[5] += a0[1];
+          }
+        }
+      }
+    }
+  }
+  // source and its diagonal entry (TurbModel_SA.h:359-434)
+  {
+    const double rho = pc[0], vol = m.vol[c];
+    const double mue = auxc[kAuxMue];
+    const double Om = Dev::mx(auxc[kAuxVort], 1.e-8);
+    const double Strainrate = Dev::mx(auxc[kAuxStr], 1.e-8);
+    const double d = auxc[kAuxWall];
+    const double chi = ph * rho / mue;
+    const double chi_3 = pow(chi, 3.0);
+    const double inv_KarmanConst2_d2 = 1.0 / (kSaKarman2 * d * d);
+    const double fv1 = chi_3 / (chi_3 + kSaCv1_3);
+    const double fv2 = 1.0 - chi / (1.0 + chi * fv1);
+    const double BlendTurbulentProduction = 0.0;
+    const double Sv = Dev::mx(Om + Dev::mn(0.0, Strainrate - Om) * BlendTurbulentProduction, 0.0);
+    const double Shat = Sv + ph * fv2 * inv_KarmanConst2_d2;
+    const double inv_Shat = 1.0 / Dev::mx(Shat, 1.0e-8);
+    const double r = Dev::mn(ph * inv_Shat * inv_KarmanConst2_d2, 10.0);
+    const double gg = r + kSaCw2 * (pow(r, 6.0) - r);
+    const double g_6 = pow(gg, 6.0);
+    const double fw = gg * pow((1.0 + kSaCw3_6) / (g_6 + kSaCw3_6), 1.0 / 6.0);
+    const double H1 = kSaCb1 * Shat * ph;
+    const double H2 = -kSaCw1 * fw * ph * ph / pow(d, 2.0);
+    const double H3 = kSaCb2 / kSaSigma * dot3(gpc, gpc);
+    acc += rho * vol * (H1 + H2 + H3);
+    const double d_H1 = 0.0;
+    const double d_H2 = -2.0 * kSaCw1 * fw * ph / pow(d, 2.0);
+    D[5] -= (d_H1 + d_H2) * vol;
+    if (step_mode) {
+      D[5] += vol / (dt[c]);
+      rhs_ls[6 * (size_t)c + 5] = (T)(relax * acc);
+      x0[6 * (size_t)c + 5] = (T)0.0;
+    }
+  }
+  rhs_s[c] = acc;
+  if (A) {
+    T *dg = A + 36 * (size_t)m.nbocv_i[c] + 30;
+    for (int k = 0; k < 6; k++) dg[k] = (T)D[k];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Deterministic two-stage reductions: every block writes its partial and a
 // one-block finalize kernel combines them in a fixed order. No device-scope
 // fences or tickets: on a multi-XCD part a device-scope release writes back
@@ -2174,7 +2429,7 @@ __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const
         const double ufa[3] = {0.5 * (a.u[0] + b.u[0]), 0.5 * (a.u[1] + b.u[1]), 0.5 * (a.u[2] + b.u[2])};
         if (CPL) {  // calcFluxCoupled internal faces (JoeWithModels.cpp:5076-5094)
           double VF[5], tau[3], keff;
-          visc_coupled(P, a, b, mul, mut, ufa, area, n, smag, sV, dot3(n, sV), VF, tau, keff);
+          visc_coupled(P, a, b, mul, mut, ufa, area, n, smag, sV, dot3(n, sV), VF, tau, keff, P.sa != 0);
           for (int k = 0; k < 5; k++) vcc[k] = s0 ? -VF[k] : VF[k];
         } else {
           double Fu[3], FE;
@@ -2207,6 +2462,8 @@ __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const
       }
       if (VISC && CPL) {  // calcFluxCoupled boundary faces; mul from T_bfa
         mul = mu_lam(P, bb[5]);
+        if (P.sa && z.bc != kBcWall)  // mut_fa of calcRansTurbViscMuet (TurbModel_SA.h:112-120)
+          mut = sa_mut(pc[0] * P.sa_phi[L.c], P.sa_aux[kAuxN * (size_t)L.c + kAuxMue]);
         const double *gvo = gradv + 12 * (size_t)L.c;
         if (z.bc == kBcSymmetry) {  // 2/3 (mu + mut) tr(grad u) on the momentum only (:5272-5284)
           double tmp = 2.0 / 3.0 * ((mul + mut) * (gvo[0] + gvo[4] + gvo[8]) + bb[0] * 0.0);
@@ -2218,7 +2475,7 @@ __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const
           visc_coupled_bgeom(z.bc, xfa, pc + 5, n, sArg, smag, alpha);
           double VF[5], tau[3], keff;
           visc_coupled(P, vsel(z.bc == kBcNeumann, vb, vo), vb, mul, z.bc == kBcWall ? 0.0 : mut, vb.u, area, n,
-                       smag, sArg, alpha, VF, tau, keff);
+                       smag, sArg, alpha, VF, tau, keff, P.sa != 0);
           for (int k = 0; k < 5; k++) vcc[k] = -VF[k];
         }
       } else if (VISC) {  // boundary faces (:3714-3851); boundary mul from T_bfa (ComputeBCProperties_T)
@@ -3019,37 +3276,58 @@ __global__ void __launch_bounds__(kBlock) k_update(int ncv, double *__restrict__
   d_update<T>(hw_blk(), ncv, q, dq);
 }
 
-// sourceHook terms computed by the host, in device order: src[c] holds five
-// residual terms then the 25 terms of the diagonal block (JoeWithModels.h:111).
+// the coupled loop's update with the scalar (JoeWithModels.cpp:4497-4516):
+// Q += dQ[0..4], then nuSA = (nuSA (rho - dQ0) + dQ5) / rho with the updated
+// rho, clipped to the model's bounds
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_add_source(int ncv, const int *__restrict__ nbocv_i,
-                                                       const double *__restrict__ src, int with_a,
-                                                       double *__restrict__ rhs, T *__restrict__ A) {
+__global__ void __launch_bounds__(kBlock) k_update6(int ncv, double *__restrict__ q, const T *__restrict__ dq,
+                                                    double *__restrict__ phi) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncv) return;
-  const double *s = src + 30 * (size_t)c;
+  const T *d = dq + 6 * (size_t)c;
+  for (int k = 0; k < 5; k++) q[(size_t)k * ncv + c] += (double)d[k];
+  const double rho = q[c];
+  const double v = (phi[c] * (rho - (double)d[0]) + (double)d[5]) / rho;
+  phi[c] = Dev::mn(Dev::mx(v, kSaLower), kSaUpper);
+}
+
+// sourceHook / sourceHookCoupled terms computed by the host, in device
+// order: src[c] holds NB residual terms then the NB x NB terms of the
+// diagonal block (JoeWithModels.h:111, 134); with NB = 6 the sixth residual
+// term is the scalar's (rhs_s).
+template <typename T, int NB>
+__global__ void __launch_bounds__(kBlock) k_add_source(int ncv, const int *__restrict__ nbocv_i,
+                                                       const double *__restrict__ src, int with_a,
+                                                       double *__restrict__ rhs, double *__restrict__ rhs_s,
+                                                       T *__restrict__ A) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncv) return;
+  const double *s = src + (NB + NB * NB) * (size_t)c;
   for (int k = 0; k < 5; k++) rhs[(size_t)k * ncv + c] += s[k];
+  if (NB == 6) rhs_s[c] += s[5];
   if (with_a) {
-    T *d = A + 25 * (size_t)nbocv_i[c];
-    for (int k = 0; k < 25; k++) d[k] = (T)((double)d[k] + s[5 + k]);
+    T *d = A + NB * NB * (size_t)nbocv_i[c];
+    for (int k = 0; k < NB * NB; k++) d[k] = (T)((double)d[k] + s[NB + k]);
   }
 }
 
-// backward-Euler time term and relaxed right-hand side (JoeWithModels.cpp:1011-1024)
-// for a Jacobian assembled without them
-template <typename T>
+// backward-Euler time term and relaxed right-hand side (JoeWithModels.cpp
+// :1011-1024, 4455-4473) for a Jacobian assembled without them; with NB = 6
+// the scalar's row too
+template <typename T, int NB>
 __global__ void __launch_bounds__(kBlock) k_time_term(int ncv, const int *__restrict__ nbocv_i,
                                                       const double *__restrict__ vol, const double *__restrict__ dt,
-                                                      double relax, const double *__restrict__ rhs, T *__restrict__ A,
+                                                      double relax, const double *__restrict__ rhs,
+                                                      const double *__restrict__ rhs_s, T *__restrict__ A,
                                                       T *__restrict__ rhs_ls, T *__restrict__ phi) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncv) return;
   const double tmp = vol[c] / (dt[c]);
-  T *d = A + 25 * (size_t)nbocv_i[c];
-  for (int i = 0; i < 5; i++) d[6 * i] = (T)((double)d[6 * i] + tmp);
-  for (int k = 0; k < 5; k++) {
-    rhs_ls[5 * (size_t)c + k] = (T)(relax * rhs[(size_t)k * ncv + c]);
-    phi[5 * (size_t)c + k] = (T)0.0;
+  T *d = A + NB * NB * (size_t)nbocv_i[c];
+  for (int i = 0; i < NB; i++) d[(NB + 1) * i] = (T)((double)d[(NB + 1) * i] + tmp);
+  for (int k = 0; k < NB; k++) {
+    rhs_ls[NB * (size_t)c + k] = (T)(relax * (k < 5 ? rhs[(size_t)k * ncv + c] : rhs_s[c]));
+    phi[NB * (size_t)c + k] = (T)0.0;
   }
 }
 
@@ -3545,7 +3823,6 @@ void parse_params(const std::string &joe_in, const Mesh &m, SolverParams &p, std
     if (ls != "PETSC_GMRES" && ls != "BCGSTAB")
       throw Unsupported("LINEAR_SOLVER_NS = " + ls + " is not supported (PETSC_GMRES, BCGSTAB)");
     if (p.scheme != kHllc) throw Unsupported("BACKWARD_EULER_COUPLED is supported with HLLC only");
-    if (p.sa) throw Unsupported("BACKWARD_EULER_COUPLED with scalars (TURB_MODEL) is not supported");
     // the coupled face loop limits grad u and grad h with the limiter of
     // GRAD_RECONSTRUCTION alone (JoeWithModels.cpp:4691-4716)
     p.visc_limiter = true;
@@ -3687,6 +3964,21 @@ struct Ctx::Impl {
   SolverScalars *h_scal_s = nullptr;  // pinned
   DBuf<int> foc;
   int scal_iters = 0;
+  // block size of the linear system: 5, or 6 in the coupled loop with the
+  // Spalart-Allmaras scalar (its row and column in every block)
+  int nbs = 5;
+  bool coupled_sa() const { return nbs == 6; }
+  // the Jacobian, preconditioner and Krylov vectors of the block system in
+  // precision T (nvec work vectors)
+  template <typename T>
+  void ensure_solver(int nvec, bool with_lu);
+  template <typename T>
+  void launch_scal_c(T *A, bool step_mode);
+  template <typename T>
+  void update();
+  template <typename T>
+  void time_term();
+  void push_source(bool with_a);
   template <typename T>
   void scalar_step(const T *dq);
   void finish_step(StepInfo &info, int it, int st);
@@ -3734,7 +4026,9 @@ struct Ctx::Impl {
   void launch_flux(hipStream_t s, const double *cfl_dev = nullptr);
   int flux_blocks() const;
   bool flux_entry() const;
-  void launch_grad_flux();
+  // jac_follows: the Jacobian pass comes next (it forms the coupled
+  // scalar row together with its blocks)
+  void launch_grad_flux(bool jac_follows = false);
   template <typename T>
   void launch_jac(bool step_mode);
 };
@@ -4086,9 +4380,11 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
     d.sa_aux.alloc(kAuxN * n);
     HIPCHECK(hipMemset(d.sa_aux.p, 0, sizeof(double) * kAuxN * n));
     d.rhs_s.alloc(n);
-    d.AS.alloc(6 * (size_t)d.nnz);
-    HIPCHECK(hipMemset(d.AS.p, 0, sizeof(double) * 6 * (size_t)d.nnz));
-    for (int k = 0; k < 10; k++) d.sw[k].alloc(n);
+    if (!d.par.coupled) {  // the segregated scalar's AScal rows and Jacobi BiCGStab
+      d.AS.alloc(6 * (size_t)d.nnz);
+      HIPCHECK(hipMemset(d.AS.p, 0, sizeof(double) * 6 * (size_t)d.nnz));
+      for (int k = 0; k < 10; k++) d.sw[k].alloc(n);
+    }
     d.scal_s.alloc(1);
     HIPCHECK(hipMemset(d.scal_s.p, 0, sizeof(SolverScalars)));
     HIPCHECK(hipHostMalloc((void **)&d.h_scal_s, sizeof(SolverScalars)));
@@ -4157,6 +4453,7 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
   d.dp.pr = d.par.pr;
   d.dp.pr_turb = d.par.pr_turb;
   d.dp.sa = d.par.sa ? 1 : 0;
+  d.nbs = (d.par.coupled && d.par.sa) ? 6 : 5;
   d.dp.sa_phi = d.sa_phi.p;
   d.dp.sa_aux = d.sa_aux.p;
   d.cfl = d.par.cfl;
@@ -4888,15 +5185,46 @@ void Ctx::Impl::launch_flux(hipStream_t s, const double *cfl_dev) {
   }
 }
 
-void Ctx::Impl::launch_grad_flux() {
+void Ctx::Impl::launch_grad_flux(bool jac_follows) {
   launch_grad(stream);
   launch_flux(stream);
   // dt_min of calcDt; CFL_MIN sets every local dt to it
   hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, flux_blocks(), (int)kFinDt, partials.p, scal.p);
   if (par.dt_mode == 1) hipLaunchKernelGGL(k_dt_fill, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, dt.p, scal.p);
-  // the nuSA equation: residual and AScal rows (calcRhs scalar terms)
-  if (par.sa)
+  if (coupled_sa()) {
+    // the scalar row of the coupled system: its residual here, or with its
+    // Jacobian row in launch_jac
+    if (!jac_follows) launch_scal_c<double>(nullptr, false);
+  } else if (par.sa) {
+    // the nuSA equation: residual and AScal rows (calcRhs scalar terms)
     hipLaunchKernelGGL(k_scal, dim3(nblk(ncv)), dim3(kBlock), 0, stream, dm, dp, prim.p, grad.p, bfa.p, rhs_s.p, AS.p);
+  }
+}
+
+template <typename T>
+void Ctx::Impl::launch_scal_c(T *A, bool step_mode) {
+  DBuf<T> *v = krylov<T>();
+  T *b = step_mode ? v[0].p : nullptr, *x0 = step_mode ? v[1].p : nullptr;
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(nblk(ncv)), dim3(kBlock), 0, stream, dm, dp, prim.p, grad.p, bfa.p, gradv.p,
+                       rhs_s.p, A, step_mode ? 1 : 0, dt.p, par.under_relax, b, x0);
+  };
+  if (par.second_order) go(k_scal_c<T, true>);
+  else go(k_scal_c<T, false>);
+}
+
+template <typename T>
+void Ctx::Impl::ensure_solver(int nvec, bool with_lu) {
+  const size_t b = nbs;
+  if constexpr (std::is_same<T, double>::value) {
+    ensure(A64, b * b * (size_t)nnz);
+    if (with_lu) ensure(LU64, b * b * (size_t)ncv);
+    for (int k = 0; k < nvec; k++) ensure(v64[k], b * (size_t)ncv);
+  } else {
+    ensure(A32, b * b * (size_t)nnz);
+    if (with_lu) ensure(LU32, b * b * (size_t)ncv);
+    for (int k = 0; k < nvec; k++) ensure(v32[k], b * (size_t)ncv);
+  }
 }
 
 // The segregated nuSA half of a step (JoeWithModels.cpp:1040-1091) after the
@@ -4950,13 +5278,18 @@ void Ctx::Impl::launch_jac(bool step_mode) {
   if constexpr (std::is_same<T, double>::value) v = v64; else v = v32;
   T *A = A_<T>();
   if (dup_blocks) {
-    const size_t nA = 25 * (size_t)nnz;
+    const size_t nA = (size_t)nbs * nbs * (size_t)nnz;
     hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)((nA + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream, nA, A, (T)0);
   }
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3(nblk(ncv)), dim3(kBlock), 0, stream, dm, dp, prim.p, bfa.p, A, step_mode ? 1 : 0,
                        dt.p, rhs.p, par.under_relax, v[0].p, v[1].p, gradv.p);
   };
+  if (coupled_sa()) {  // 6x6 blocks: the flow rows, then the scalar row (SA needs MU_REF > 0)
+    go(k_jac<T, kHllcC, true, 6>);
+    launch_scal_c<T>(A, step_mode);
+    return;
+  }
   if (dp.visc) {
     if (par.scheme == kRoe) go(k_jac<T, kRoe, true>);
     else if (par.scheme == kHllcC) go(k_jac<T, kHllcC, true>);
@@ -4992,17 +5325,20 @@ BcgSys Ctx::Impl::bcg_sys() {
 
 template <typename T>
 void Ctx::Impl::solve_prologue() {
-  bcg_prologue<T, 5>(bcg_sys<T>());
+  if (coupled_sa()) bcg_prologue<T, 6>(bcg_sys<T>());
+  else bcg_prologue<T, 5>(bcg_sys<T>());
 }
 
 template <typename T>
 void Ctx::Impl::solve_iterations(int count) {
-  bcg_iterations<T, 5>(bcg_sys<T>(), count);
+  if (coupled_sa()) bcg_iterations<T, 6>(bcg_sys<T>(), count);
+  else bcg_iterations<T, 5>(bcg_sys<T>(), count);
 }
 
 template <typename T>
 void Ctx::Impl::solve(int &iters, int &status) {
-  bcg_solve<T, 5>(bcg_sys<T>(), h_scal, iters, status);
+  if (coupled_sa()) bcg_solve<T, 6>(bcg_sys<T>(), h_scal, iters, status);
+  else bcg_solve<T, 5>(bcg_sys<T>(), h_scal, iters, status);
 }
 
 template <typename T>
@@ -5102,15 +5438,13 @@ void Ctx::calc_rhs(bool implicit) {
   Impl &d = *d_;
   if (!d.have_state) throw std::logic_error("no state set");
   d.launch_state_bc();
-  d.launch_grad_flux();
+  d.launch_grad_flux(implicit);
   if (implicit) {
     if (d.precision == 0) {
-      ensure(d.A64, 25 * (size_t)d.nnz);
-      for (int k = 0; k < 2; k++) ensure(d.v64[k], 5 * (size_t)d.ncv);
+      d.ensure_solver<double>(2, false);
       d.launch_jac<double>(false);
     } else {
-      ensure(d.A32, 25 * (size_t)d.nnz);
-      for (int k = 0; k < 2; k++) ensure(d.v32[k], 5 * (size_t)d.ncv);
+      d.ensure_solver<float>(2, false);
       d.launch_jac<float>(false);
     }
   }
@@ -5129,28 +5463,34 @@ StepInfo Ctx::step() {
   // CFL ramp (JoeWithModels.cpp:978)
   if ((d.step_no >= d.par.cfl_after) && (d.step_no % d.par.cfl_interval == 0) && (d.cfl < d.par.cfl_max))
     d.cfl *= d.par.cfl_factor;
-  d.launch_grad_flux();
+  d.launch_grad_flux(true);
   int it = 0, st = 0;
   if (d.precision == 0) {
-    ensure(d.A64, 25 * (size_t)d.nnz);
-    ensure(d.LU64, 25 * (size_t)d.ncv);
-    for (int k = 0; k < 10; k++) ensure(d.v64[k], 5 * (size_t)d.ncv);
+    d.ensure_solver<double>(10, true);
     d.launch_jac<double>(true);
     d.solve<double>(it, st);
-    hipLaunchKernelGGL(k_update<double>, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.q.p, d.v64[1].p);
-    if (d.par.sa) d.scalar_step<double>(d.v64[1].p);
+    d.update<double>();
   } else {
-    ensure(d.A32, 25 * (size_t)d.nnz);
-    ensure(d.LU32, 25 * (size_t)d.ncv);
-    for (int k = 0; k < 10; k++) ensure(d.v32[k], 5 * (size_t)d.ncv);
+    d.ensure_solver<float>(10, true);
     d.launch_jac<float>(true);
     d.solve<float>(it, st);
-    hipLaunchKernelGGL(k_update<float>, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.q.p, d.v32[1].p);
-    if (d.par.sa) d.scalar_step<float>(d.v32[1].p);
+    d.update<float>();
   }
   d.launch_state_bc();
   d.finish_step(info, it, st);
   return info;
+}
+
+// Q += dQ and the scalar's update: segregated (its own solve) or coupled
+template <typename T>
+void Ctx::Impl::update() {
+  const T *dq = krylov<T>()[1].p;
+  if (coupled_sa()) {
+    hipLaunchKernelGGL(k_update6<T>, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, q.p, dq, sa_phi.p);
+    return;
+  }
+  hipLaunchKernelGGL(k_update<T>, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, q.p, dq);
+  if (par.sa) scalar_step<T>(dq);
 }
 
 // residual L1 sums of the step just taken (JoeWithModels.cpp:1119-1136) and its report
@@ -5158,7 +5498,10 @@ void Ctx::Impl::finish_step(StepInfo &info, int it, int st) {
   hipLaunchKernelGGL(k_l1, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, rhs.p, partials.p);
   hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, nblk(ncv), (int)kFinL1, partials.p, scal.p);
   if (par.sa) {
-    hipLaunchKernelGGL(k_l1s, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, rhs_s.p, par.under_relax, partials.p);
+    // the segregated loop reports the scalar residual over the mean-flow
+    // relaxation (JoeWithModels.cpp:1132-1134), the coupled loop as it is (:4443-4453)
+    hipLaunchKernelGGL(k_l1s, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, rhs_s.p,
+                       coupled_sa() ? 1.0 : par.under_relax, partials.p);
     hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, nblk(ncv), (int)kFinL1S, partials.p, scal.p);
   }
   HIPCHECK(hipMemcpyAsync(h_scal, scal.p, sizeof(SolverScalars), hipMemcpyDeviceToHost, stream));
@@ -5171,7 +5514,7 @@ void Ctx::Impl::finish_step(StepInfo &info, int it, int st) {
   for (int k = 0; k < 5; k++) info.resid[k] = h_scal->resid[k];
   for (int k = 0; k < 5; k++) info.resid_l2[k] = h_scal->resid_l2[k];
   info.resid_s = par.sa ? h_scal->resid_s : 0.0;
-  info.scal_iters = par.sa ? scal_iters : 0;
+  info.scal_iters = (par.sa && !coupled_sa()) ? scal_iters : 0;
 }
 
 void Ctx::step_begin(bool host_source) {
@@ -5187,16 +5530,12 @@ void Ctx::step_begin(bool host_source) {
   // a host sourceHook replaces the default one (gravity)
   const DParams keep = d.dp;
   if (host_source) d.dp.gravity_on = 0;
-  d.launch_grad_flux();
+  d.launch_grad_flux(true);
   if (d.precision == 0) {
-    ensure(d.A64, 25 * (size_t)d.nnz);
-    ensure(d.LU64, 25 * (size_t)d.ncv);
-    for (int k = 0; k < 10; k++) ensure(d.v64[k], 5 * (size_t)d.ncv);
+    d.ensure_solver<double>(10, true);
     d.launch_jac<double>(false);
   } else {
-    ensure(d.A32, 25 * (size_t)d.nnz);
-    ensure(d.LU32, 25 * (size_t)d.ncv);
-    for (int k = 0; k < 10; k++) ensure(d.v32[k], 5 * (size_t)d.ncv);
+    d.ensure_solver<float>(10, true);
     d.launch_jac<float>(false);
   }
   d.dp = keep;
@@ -5207,27 +5546,67 @@ void Ctx::step_begin(bool host_source) {
 void Ctx::add_source(const double *rhs_rho, const double *rhs_rhou, const double *rhs_rhoE, const double *a_diag) {
   Impl &d = *d_;
   if (!d.begun) throw std::logic_error("add_source outside step_begin/step_end");
-  const size_t n = d.ncv;
-  d.h_src.assign(30 * n, 0.0);
+  const size_t n = d.ncv, b = d.nbs, rec = b + b * b;
+  d.h_src.assign(rec * n, 0.0);
   for (size_t cd = 0; cd < n; cd++) {
     const size_t c = d.perm[cd];
-    double *o = &d.h_src[30 * cd];
+    double *o = &d.h_src[rec * cd];
     o[0] = rhs_rho ? rhs_rho[c] : 0.0;
     for (int i = 0; i < 3; i++) o[1 + i] = rhs_rhou ? rhs_rhou[3 * c + i] : 0.0;
     o[4] = rhs_rhoE ? rhs_rhoE[c] : 0.0;
-    if (a_diag)
-      for (int k = 0; k < 25; k++) o[5 + k] = a_diag[25 * c + k];
+    if (a_diag)  // the flow part of the system's diagonal block
+      for (int i = 0; i < 5; i++)
+        for (int j = 0; j < 5; j++) o[b + b * i + j] = a_diag[25 * c + 5 * i + j];
   }
-  ensure(d.src, 30 * n);
-  HIPCHECK(hipMemcpyAsync(d.src.p, d.h_src.data(), 30 * n * sizeof(double), hipMemcpyHostToDevice, d.stream));
-  if (d.precision == 0)
-    hipLaunchKernelGGL(k_add_source<double>, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.nbocv_i.p,
-                       d.src.p, a_diag ? 1 : 0, d.rhs.p, d.A64.p);
-  else
-    hipLaunchKernelGGL(k_add_source<float>, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.nbocv_i.p,
-                       d.src.p, a_diag ? 1 : 0, d.rhs.p, d.A32.p);
+  d.push_source(a_diag != nullptr);
+}
+
+void Ctx::add_source_coupled(const double *rhs, const double *a_diag, int nb) {
+  Impl &d = *d_;
+  if (!d.begun) throw std::logic_error("add_source outside step_begin/step_end");
+  if (nb != d.nbs) throw std::invalid_argument("block size " + std::to_string(nb) + " != the system's " + std::to_string(d.nbs));
+  const size_t n = d.ncv, b = d.nbs, rec = b + b * b;
+  d.h_src.assign(rec * n, 0.0);
+  for (size_t cd = 0; cd < n; cd++) {
+    const size_t c = d.perm[cd];
+    double *o = &d.h_src[rec * cd];
+    if (rhs)
+      for (size_t k = 0; k < b; k++) o[k] = rhs[b * c + k];
+    if (a_diag)
+      for (size_t k = 0; k < b * b; k++) o[b + k] = a_diag[b * b * c + k];
+  }
+  d.push_source(a_diag != nullptr);
+}
+
+// the host's source terms (h_src, device order) into the residual and the
+// diagonal blocks
+void Ctx::Impl::push_source(bool with_a) {
+  const size_t n = ncv, rec = nbs + nbs * nbs;
+  ensure(src, rec * n);
+  HIPCHECK(hipMemcpyAsync(src.p, h_src.data(), rec * n * sizeof(double), hipMemcpyHostToDevice, stream));
+  const dim3 g(nblk(ncv)), bl(kBlock);
+  const int wa = with_a ? 1 : 0;
+  if (precision == 0) {
+    if (coupled_sa()) hipLaunchKernelGGL((k_add_source<double, 6>), g, bl, 0, stream, ncv, nbocv_i.p, src.p, wa, rhs.p, rhs_s.p, A64.p);
+    else hipLaunchKernelGGL((k_add_source<double, 5>), g, bl, 0, stream, ncv, nbocv_i.p, src.p, wa, rhs.p, rhs_s.p, A64.p);
+  } else {
+    if (coupled_sa()) hipLaunchKernelGGL((k_add_source<float, 6>), g, bl, 0, stream, ncv, nbocv_i.p, src.p, wa, rhs.p, rhs_s.p, A32.p);
+    else hipLaunchKernelGGL((k_add_source<float, 5>), g, bl, 0, stream, ncv, nbocv_i.p, src.p, wa, rhs.p, rhs_s.p, A32.p);
+  }
   // the host buffer must outlive the copy
-  HIPCHECK(hipStreamSynchronize(d.stream));
+  HIPCHECK(hipStreamSynchronize(stream));
+}
+
+template <typename T>
+void Ctx::Impl::time_term() {
+  DBuf<T> *v = krylov<T>();
+  const dim3 g(nblk(ncv)), bl(kBlock);
+  if (coupled_sa())
+    hipLaunchKernelGGL((k_time_term<T, 6>), g, bl, 0, stream, ncv, nbocv_i.p, vol.p, dt.p, par.under_relax, rhs.p,
+                       rhs_s.p, A_<T>(), v[0].p, v[1].p);
+  else
+    hipLaunchKernelGGL((k_time_term<T, 5>), g, bl, 0, stream, ncv, nbocv_i.p, vol.p, dt.p, par.under_relax, rhs.p,
+                       rhs_s.p, A_<T>(), v[0].p, v[1].p);
 }
 
 StepInfo Ctx::step_end() {
@@ -5236,17 +5615,13 @@ StepInfo Ctx::step_end() {
   d.begun = false;
   int it = 0, st = 0;
   if (d.precision == 0) {
-    hipLaunchKernelGGL(k_time_term<double>, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.nbocv_i.p,
-                       d.vol.p, d.dt.p, d.par.under_relax, d.rhs.p, d.A64.p, d.v64[0].p, d.v64[1].p);
+    d.time_term<double>();
     d.solve<double>(it, st);
-    hipLaunchKernelGGL(k_update<double>, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.q.p, d.v64[1].p);
-    if (d.par.sa) d.scalar_step<double>(d.v64[1].p);
+    d.update<double>();
   } else {
-    hipLaunchKernelGGL(k_time_term<float>, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.nbocv_i.p,
-                       d.vol.p, d.dt.p, d.par.under_relax, d.rhs.p, d.A32.p, d.v32[0].p, d.v32[1].p);
+    d.time_term<float>();
     d.solve<float>(it, st);
-    hipLaunchKernelGGL(k_update<float>, dim3(nblk(d.ncv)), dim3(kBlock), 0, d.stream, d.ncv, d.q.p, d.v32[1].p);
-    if (d.par.sa) d.scalar_step<float>(d.v32[1].p);
+    d.update<float>();
   }
   d.launch_state_bc();
   StepInfo info;
@@ -5406,6 +5781,8 @@ int64_t Ctx::get(const std::string &name, double *out) {
     if (name == "STR_MAG") return cellrec(d.sa_aux.p, kAuxN, kAuxStr, 1);
     if (name == "VORT_MAG") return cellrec(d.sa_aux.p, kAuxN, kAuxVort, 1);
     if (name == "RHS_NUSA") return cellrec(d.rhs_s.p, 1, 0, 1);
+    if ((name == "DSCAL" || name == "ASCAL") && d.coupled_sa())
+      throw std::invalid_argument(name + ": the coupled loop keeps the scalar in the 6x6 system (A, DQ)");
     if (name == "DSCAL") return cellrec(d.sw[1].p, 1, 0, 1);
     if (name == "ASCAL") {  // [6][nnz] in the reference's nbocv order, like runBackwardEuler's AScal[iScal]
       const size_t nnz = d.nnz;
@@ -5430,9 +5807,10 @@ int64_t Ctx::get(const std::string &name, double *out) {
     }
     return 4;
   }
-  if (name == "A" || name == "DQ") {
+  if (name == "A" || name == "DQ") {  // [nnz][b][b] and [ncv][b], b = 5, or 6 in the coupled loop with SA
     const bool isA = name == "A";
-    const size_t cnt = isA ? 25 * (size_t)d.nnz : 5 * n;
+    const size_t b = d.nbs, bb = b * b;
+    const size_t cnt = isA ? bb * (size_t)d.nnz : b * n;
     if (!out) return (int64_t)cnt;
     std::vector<double> dev(cnt);
     if (d.precision == 0) {
@@ -5450,12 +5828,12 @@ int64_t Ctx::get(const std::string &name, double *out) {
       for (size_t cd = 0; cd < n; cd++) {
         const size_t c = perm[cd];
         const size_t len = d.h_dev_nbocv_i[cd + 1] - d.h_dev_nbocv_i[cd];
-        std::memcpy(out + 25 * (size_t)d.h_ref_nbocv_i[c], dev.data() + 25 * (size_t)d.h_dev_nbocv_i[cd],
-                    25 * len * sizeof(double));
+        std::memcpy(out + bb * (size_t)d.h_ref_nbocv_i[c], dev.data() + bb * (size_t)d.h_dev_nbocv_i[cd],
+                    bb * len * sizeof(double));
       }
     } else {
       for (size_t cd = 0; cd < n; cd++)
-        for (int k = 0; k < 5; k++) out[5 * (size_t)perm[cd] + k] = dev[5 * cd + k];
+        for (size_t k = 0; k < b; k++) out[b * (size_t)perm[cd] + k] = dev[b * cd + k];
     }
     return (int64_t)cnt;
   }

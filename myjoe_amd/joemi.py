@@ -75,6 +75,7 @@ def lib():
         L.jm_set_boundary_state.argtypes = [vp, C.c_int, vp, vp, vp]
         L.jm_get_boundary_state.argtypes = [vp, C.c_int, vp, vp, vp]
         L.jm_block_bcgstab.argtypes = [C.c_int, vp, vp, vp, vp, vp, C.c_double, C.c_double, C.c_int, vp, vp, vp]
+        L.jm_add_source_coupled.argtypes = [vp, C.c_int, vp, vp]
         L.jm_device_state.argtypes = [vp, vp]
         L.jm_device_rhs.argtypes = [vp, vp]
         L.jm_sync.argtypes = [vp]
@@ -110,11 +111,60 @@ class UgpView(C.Structure):
         [("nzone", C.c_int)] + [(n, C.c_void_p) for n in ("zone_names", "zone_kind", "zone_ifa_f", "zone_ifa_l")]
 
 
+_HIP = None
+
+
+def hip():
+    """The HIP runtime the library links (ROCm's libamdhip64). Device arrays
+    that the library reads must come from this runtime: torch ships its own
+    copy, and the two do not share a device in one process."""
+    global _HIP
+    if _HIP is None:
+        lib()
+        rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+        _HIP = C.CDLL(os.path.join(rocm, "lib", "libamdhip64.so.7"))
+        _HIP.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        _HIP.hipFree.argtypes = [C.c_void_p]
+        _HIP.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        _HIP.hipDeviceSynchronize.argtypes = []
+    return _HIP
+
+
+class DeviceArray:
+    """A device copy of a numpy array (hipMalloc of the library's runtime)."""
+
+    def __init__(self, a):
+        self.host = np.ascontiguousarray(a)
+        self.ptr = C.c_void_p()
+        st = hip().hipMalloc(C.byref(self.ptr), max(self.host.nbytes, 1))
+        if st != 0:
+            raise JoemiError(4, f"hipMalloc failed ({st})")
+        self._copy(self.ptr, self.host.ctypes.data, 1)
+
+    def _copy(self, dst, src, kind):
+        st = hip().hipMemcpy(dst, src, self.host.nbytes, kind)  # 1 host->device, 2 device->host
+        if st != 0:
+            raise JoemiError(4, f"hipMemcpy failed ({st})")
+
+    def numpy(self):
+        out = np.empty_like(self.host)
+        self._copy(out.ctypes.data, self.ptr, 2)
+        return out
+
+    def data_ptr(self):
+        return self.ptr.value
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and self.ptr.value and _HIP is not None:
+            _HIP.hipFree(self.ptr)
+            self.ptr = None
+
+
 def block_bcgstab(nbocv_i, nbocv_v, A, rhs, phi, zero_abs=1e-8, zero_rel=1e-2, max_iter=30, stream=None):
-    """jm_block_bcgstab on device arrays (torch tensors on the GPU: int32
-    nbocv_i[ncv+1], nbocv_v[nnz]; float64 A[nnz][5][5], rhs[ncv][5], phi[ncv][5],
-    phi overwritten with the solution). Returns (iterations, status)."""
-    ncv = rhs.numel() // 5
+    """jm_block_bcgstab on device arrays (DeviceArray: int32 nbocv_i[ncv+1],
+    nbocv_v[nnz]; float64 A[nnz][5][5], rhs[ncv][5], phi[ncv][5], phi
+    overwritten with the solution). Returns (iterations, status)."""
+    ncv = rhs.host.size // 5
     it, st = C.c_int(), C.c_int()
     _check(lib().jm_block_bcgstab(ncv, nbocv_i.data_ptr(), nbocv_v.data_ptr(), A.data_ptr(), rhs.data_ptr(),
                                   phi.data_ptr(), zero_abs, zero_rel, max_iter, stream, C.byref(it), C.byref(st)))

@@ -1,7 +1,7 @@
-"""runBackwardEulerCoupled (TIME_INTEGRATION = BACKWARD_EULER_COUPLED) without
-scalars, HLLC, inviscid or laminar: calcRhsCoupled's residual and Jacobian
-blocks (JoeWithModels.cpp:4596-5825, UgpWithCvCompFlow.cpp:1808-2287,
-3040-3274).
+"""runBackwardEulerCoupled (TIME_INTEGRATION = BACKWARD_EULER_COUPLED), HLLC,
+inviscid, laminar, and with the Spalart-Allmaras scalar in 6x6 blocks:
+calcRhsCoupled's residual and Jacobian blocks (JoeWithModels.cpp:4596-5825,
+UgpWithCvCompFlow.cpp:1808-2287, 3040-3274; TurbModel_SA.h:359-434).
 
 Pinned against the reference's own calcRhsCoupled (tests/golden/coupled_*.npz,
 written by make_golden.py through the harness's rhs-coupled mode). The
@@ -131,7 +131,6 @@ def test_coupled_loop_reaches_the_segregated_steady_state():
 
 @pytest.mark.parametrize("edit,needle", [
     (lambda s: s.replace("SPACE_INTEGRATION = HLLC", "SPACE_INTEGRATION = ROE"), "HLLC only"),
-    (lambda s: s.replace("MU_REF = 0.0", "MU_REF = 0.002") + "TURB_MODEL = SA\n", "scalars"),
     (lambda s: s + "GPU_FLUX_KERNEL = CELL\n", "GPU_FLUX_KERNEL"),
     (lambda s: s.replace("PETSC_GMRES", "LU"), "LINEAR_SOLVER_NS"),
 ])
@@ -144,11 +143,16 @@ def test_coupled_input_checked_before_device(edit, needle):
     assert needle in str(e.value)
 
 
+# the SA fields the device keeps (MUT_FA and DIFF_NUSA are formed per face
+# where they are used, and pinned through the residual and the blocks)
+SA_DEVICE_FIELDS = ["RHS_NUSA", "NUSA_BFA", "GRAD_NUSA", "WALL_DIST", "STR_MAG", "VORT_MAG"]
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("fixture", COUPLED)
 def test_gpu_coupled_matches_reference(fixture):
     # the HIP path against calcRhsCoupled's outputs, at the residual bar of
-    # the segregated fixtures (1e-12 relative)
+    # the segregated fixtures (1e-12 relative); with SA, A is [nnz][6][6]
     from tests.helpers import compare
     from tests.test_mesh_golden import MESHES
 
@@ -157,9 +161,12 @@ def test_gpu_coupled_matches_reference(fixture):
     mesh = joemi.Mesh.from_faces(MESHES[_mesh_key(fixture).split("_")[0]]())
     s = joemi.Solver(mesh, joe_text(g))
     s.set_state(g["RHO"], g["RHOU"], g["RHOE"])
+    sa = "_sa" in fixture
+    if sa:
+        s.set_scalar(g["NUSA"])
     s.calc_rhs(True)
     exact = total = 0
-    for k in FIELDS + (["GRAD_ENTHALPY"] if "visc" in fixture else []):
+    for k in FIELDS + (["GRAD_ENTHALPY"] if "visc" in fixture else []) + (SA_DEVICE_FIELDS if sa else []):
         if k not in g:
             continue
         e, n = compare(s.get(k), g[k], 1e-12, f"{fixture}:{k}")
@@ -207,3 +214,85 @@ def test_gpu_coupled_graph_step_equals_eager_step():
         assert list(ia.resid) == list(ib.resid), k
     for x, y in zip(a.state(), b.state()):
         assert np.array_equal(x, y)
+
+
+def _sa_bump(nsteps):
+    mm, joe = cases.bump_case(24, 8, nsteps=nsteps)
+    visc = "MU_REF = 0.002\nMU_MODE = POWERLAW\nMU_POWER_LAW = 0.76\nPR = 0.72"
+    return mm, cases.coupled(cases.with_sa(joe.replace("MU_REF = 0.0", visc), 0.006, cases.SA_BUMP))
+
+
+def test_oracle_coupled_sa_converges():
+    # the coupled loop with the scalar on the bump: the energy residual falls
+    # by 1e-8 and nuSA stays inside the model's clip bounds
+    from oracle.oracle import Oracle
+
+    mm, joe = _sa_bump(1200)
+    mesh = joemi.Mesh.from_faces(mm)
+    o = Oracle(mesh.as_dump(), joe)
+    o.set_state(*_uniform(mesh.ncv))
+    n, hist = o.run(1200)
+    assert n == 1200 and hist.shape[1] == 6
+    assert hist[-1, 4] < 1e-7 * hist[0, 4]
+    assert hist[-1, 5] < 1e-7 * hist[0, 5]
+    nu = o.get("NUSA")
+    assert np.all(nu >= 1e-10) and np.all(nu <= 1.0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [joemi.JM_FP64, joemi.JM_MIXED])
+def test_gpu_coupled_sa_steps_track_oracle(precision):
+    # 40 coupled steps with the scalar in the 6x6 system. FP64: the
+    # trajectory bars of the segregated loop (L1 history 1e-7, state 1e-9);
+    # mixed (FP32 blocks, preconditioner and Krylov) follows a different
+    # inexact-Newton path, so only its L1 history is held, loosely
+    from oracle.oracle import Oracle
+    from tests.helpers import compare
+
+    print(f"coupled SA steps precision={precision}", flush=True)
+    mm, joe = _sa_bump(40)
+    mesh = joemi.Mesh.from_faces(mm)
+    o = Oracle(mesh.as_dump(), joe)
+    s = joemi.Solver(mesh, joe, precision)
+    o.set_state(*_uniform(mesh.ncv))
+    s.init_uniform()
+    n_o, hist_o = o.run(40)
+    n_g, hist_g = s.run(40)
+    assert n_o == n_g == 40
+    if precision == joemi.JM_FP64:
+        compare(hist_g[:, :5], hist_o[:, :5], 1e-7, "coupled SA L1 history")
+        for a, b, k in zip(s.state(), o.state(), ("rho", "rhou", "rhoE")):
+            compare(a, b, 1e-9, k)
+        compare(s.get("NUSA"), o.get("NUSA"), 1e-9, "nuSA")
+    else:
+        compare(hist_g[:, 4], hist_o[:, 4], 1e-2, "coupled SA mixed energy L1")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [joemi.JM_FP64, joemi.JM_MIXED])
+def test_gpu_coupled_sa_steady_state_matches_oracle(precision):
+    # both precisions reach the oracle's coupled steady state to 1e-8 of each
+    # variable's largest magnitude; in the coupled system nuSA converges with
+    # the mean flow (no scalar-solve stop band), so it is held to the same bar
+    from oracle.oracle import Oracle
+
+    print(f"coupled SA steady precision={precision}", flush=True)
+    nsteps = 1500
+    mm, joe = _sa_bump(nsteps)
+    mesh = joemi.Mesh.from_faces(mm)
+    o = Oracle(mesh.as_dump(), joe)
+    o.set_state(*_uniform(mesh.ncv))
+    n, rhist = o.run(nsteps)
+    assert rhist[-1, 4] < 1e-7 * rhist[0, 4]
+    s = joemi.Solver(mesh, joe, precision)
+    s.init_uniform()
+    n_g, hist = s.run(nsteps)
+    assert n_g == nsteps
+    ref = dict(zip(("rho", "rhou", "rhoE"), o.state()))
+    ref["nuSA"] = o.get("NUSA")
+    got = dict(zip(("rho", "rhou", "rhoE"), s.state()))
+    got["nuSA"] = s.get("NUSA")
+    worst = {k: float(np.max(np.abs(got[k] - b)) / np.max(np.abs(b))) for k, b in ref.items()}
+    print("coupled SA steady", precision, {k: f"{v:.1e}" for k, v in worst.items()}, flush=True)
+    for k, v in worst.items():
+        assert v <= 1e-8, (precision, k, v)
