@@ -244,6 +244,20 @@ jm_status jm_block_bcgstab(int ncv, const int *nbocv_i, const int *nbocv_v, cons
                            double *phi, double zero_abs, double zero_rel, int max_iter, void *stream, int *iters,
                            int *status);
 
+/* No counterpart in the reference: jm_block_bcgstab for callers that cannot
+ * allocate device memory (tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments, kernels and results; the work space is a
+ * static array of 64 MiB that the library holds on each device it runs on
+ * until the process ends, which bounds ncv to
+ * jm_block_bcgstab_noalloc_max_cells() (about 129 000 cells); a larger ncv
+ * fails with JM_ERR_ARG before any device call. No hipMalloc or hipFree is
+ * made. Every call on a device shares that device's array, so calls that
+ * overlap on one device (two host threads, or two streams) race. */
+jm_status jm_block_bcgstab_noalloc(int ncv, const int *nbocv_i, const int *nbocv_v, const double *A,
+                                   const double *rhs, double *phi, double zero_abs, double zero_rel, int max_iter,
+                                   void *stream, int *iters, int *status);
+int64_t jm_block_bcgstab_noalloc_max_cells(void);
+
 /* Measurement hook, no counterpart in the reference: launches each kernel
  * of one residual evaluation (state, bc, grad, flux, dt) reps times on the
  * context's stream and returns the mean duration of each in ms, measured

@@ -35,7 +35,9 @@ jm_status guard(F &&f) {
   } catch (const jm::Unsupported &e) {
     g_err = e.what();
     return JM_ERR_UNSUPPORTED;
-
+  } catch (const jm::ArgError &e) {
+    g_err = e.what();
+    return JM_ERR_ARG;
   } catch (const std::invalid_argument &e) {
     g_err = e.what();
     return JM_ERR_PARAM;
@@ -426,6 +428,31 @@ jm_status jm_block_bcgstab(int ncv, const int *nbocv_i, const int *nbocv_v, cons
     return JM_OK;
   });
 }
+
+jm_status jm_block_bcgstab_noalloc(int ncv, const int *nbocv_i, const int *nbocv_v, const double *A,
+                                   const double *rhs, double *phi, double zero_abs, double zero_rel, int max_iter,
+                                   void *stream, int *iters, int *status) {
+  return guard([&] {
+    jm::BlockSolve bs;
+    bs.ncv = ncv;
+    bs.nbocv_i = nbocv_i;
+    bs.nbocv_v = nbocv_v;
+    bs.A = A;
+    bs.rhs = rhs;
+    bs.phi = phi;
+    bs.zero_abs = zero_abs;
+    bs.zero_rel = zero_rel;
+    bs.max_iter = max_iter;
+    bs.stream = stream;
+    int it = 0, st = 0;
+    jm::block_bcgstab_noalloc(bs, it, st);
+    if (iters) *iters = it;
+    if (status) *status = st;
+    return JM_OK;
+  });
+}
+
+int64_t jm_block_bcgstab_noalloc_max_cells(void) { return (int64_t)jm::block_bcgstab_noalloc_max_cells(); }
 
 jm_status jm_get_boundary_state(jm_ctx *c, int zone, double *T, double *vel, double *p) {
   return guard([&] {

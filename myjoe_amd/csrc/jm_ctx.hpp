@@ -137,6 +137,13 @@ struct BlockSolve {
 };
 // solveCvVectorR5Bcgstab on it with a work space allocated for the call
 void block_bcgstab(const BlockSolve &bs, int &iters, int &status);
+// the same with the library's per-device static work space (no allocation):
+// kNoAllocScratchDoubles doubles (64 MiB), enough for
+// block_bcgstab_noalloc_max_cells() cells; larger systems are rejected
+// before any device call. Calls that overlap on one device race.
+constexpr size_t kNoAllocScratchDoubles = size_t(8) << 20;
+void block_bcgstab_noalloc(const BlockSolve &bs, int &iters, int &status);
+size_t block_bcgstab_noalloc_max_cells();
 
 // Solves of one kernel configuration (equal Ctx::batch_key) stepping in
 // lockstep for batched serving: each kernel of the step before the linear

@@ -18,6 +18,11 @@ struct Unsupported : std::invalid_argument {
   using std::invalid_argument::invalid_argument;
 };
 
+// a bad argument of a call (a size or pointer), reported as JM_ERR_ARG
+struct ArgError : std::invalid_argument {
+  using std::invalid_argument::invalid_argument;
+};
+
 
 class ParamMap {
  public:

@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -5901,6 +5902,64 @@ void block_bcgstab(const BlockSolve &bs, int &iters, int &status) {
   ws.alloc(bcg_workspace_doubles(bs.ncv));
   SolverScalars hs;
   bcg_solve<double, 5>(bcg_on_workspace(bs, ws.p, st), &hs, iters, status);
+}
+
+// ---------------------------------------------------------------------------
+// The same solve for callers that cannot allocate device memory: the work
+// space is this file-scope array, one copy per device, held until the process
+// ends. Host code never passes the array's name to a kernel (on the host it
+// is a stand-in, not a device address): each device's address is looked up
+// once with hipGetSymbolAddress, with that device current, and kept. Every
+// call on a device shares its array and leaves the last call's contents in
+// it; bcg_prologue clears what must start at zero (p, v) with fill kernels
+// and writes the solver scalars, LU and residuals before they are read, so
+// nothing carries over. Calls that overlap on one device race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_dev[kMaxDevices] = {};
+
+// the current device's address of scratch
+double *scratch_on_current_device() {
+  int dev = -1;
+  HIPCHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device ordinal out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      throw std::runtime_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    g_scratch_dev[dev] = (double *)ptr;
+  }
+  return g_scratch_dev[dev];
+}
+}  // namespace
+
+size_t block_bcgstab_noalloc_max_cells() {
+  size_t lo = 0, hi = kNoAllocScratchDoubles;  // largest ncv whose work space fits
+  while (lo < hi) {
+    const size_t mid = (lo + hi + 1) / 2;
+    if (bcg_workspace_doubles((int)mid) <= kNoAllocScratchDoubles) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+void block_bcgstab_noalloc(const BlockSolve &bs, int &iters, int &status) {
+  // the size is checked first, before any device call
+  if (bs.ncv > 0 && bcg_workspace_doubles(bs.ncv) > kNoAllocScratchDoubles)
+    throw ArgError("block BiCGStab without allocation: ncv = " + std::to_string(bs.ncv) +
+                                " needs more than the " + std::to_string(kNoAllocScratchDoubles * 8 >> 20) +
+                                " MiB work space (at most " + std::to_string(block_bcgstab_noalloc_max_cells()) +
+                                " cells)");
+  bcg_check(bs);
+  double *ws = scratch_on_current_device();
+  SolverScalars hs;
+  bcg_solve<double, 5>(bcg_on_workspace(bs, ws, (hipStream_t)bs.stream), &hs, iters, status);
 }
 
 }  // namespace jm

@@ -75,6 +75,9 @@ def lib():
         L.jm_set_boundary_state.argtypes = [vp, C.c_int, vp, vp, vp]
         L.jm_get_boundary_state.argtypes = [vp, C.c_int, vp, vp, vp]
         L.jm_block_bcgstab.argtypes = [C.c_int, vp, vp, vp, vp, vp, C.c_double, C.c_double, C.c_int, vp, vp, vp]
+        L.jm_block_bcgstab_noalloc.argtypes = L.jm_block_bcgstab.argtypes
+        L.jm_block_bcgstab_noalloc_max_cells.argtypes = []
+        L.jm_block_bcgstab_noalloc_max_cells.restype = C.c_int64
         L.jm_add_source_coupled.argtypes = [vp, C.c_int, vp, vp]
         L.jm_device_state.argtypes = [vp, vp]
         L.jm_device_rhs.argtypes = [vp, vp]
@@ -169,6 +172,22 @@ def block_bcgstab(nbocv_i, nbocv_v, A, rhs, phi, zero_abs=1e-8, zero_rel=1e-2, m
     _check(lib().jm_block_bcgstab(ncv, nbocv_i.data_ptr(), nbocv_v.data_ptr(), A.data_ptr(), rhs.data_ptr(),
                                   phi.data_ptr(), zero_abs, zero_rel, max_iter, stream, C.byref(it), C.byref(st)))
     return it.value, st.value
+
+
+def block_bcgstab_noalloc(nbocv_i, nbocv_v, A, rhs, phi, zero_abs=1e-8, zero_rel=1e-2, max_iter=30, stream=None):
+    """jm_block_bcgstab_noalloc: block_bcgstab on the library's static work
+    space (no device allocation; at most block_bcgstab_noalloc_max_cells()
+    cells). Returns (iterations, status)."""
+    ncv = rhs.host.size // 5
+    it, st = C.c_int(), C.c_int()
+    _check(lib().jm_block_bcgstab_noalloc(ncv, nbocv_i.data_ptr(), nbocv_v.data_ptr(), A.data_ptr(),
+                                          rhs.data_ptr(), phi.data_ptr(), zero_abs, zero_rel, max_iter, stream,
+                                          C.byref(it), C.byref(st)))
+    return it.value, st.value
+
+
+def block_bcgstab_noalloc_max_cells() -> int:
+    return int(lib().jm_block_bcgstab_noalloc_max_cells())
 
 
 class Mesh:

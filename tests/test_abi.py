@@ -74,6 +74,21 @@ def test_joe_in_errors_before_device(edit, needle):
     assert needle in str(e.value)
 
 
+def test_noalloc_solve_rejects_oversized_systems_first():
+    # jm_block_bcgstab_noalloc checks ncv against its static work space before
+    # any device call (no GPU needed): one cell past the limit is JM_ERR_ARG
+    # and the message names the limit; the pointers are never read
+    L = joemi.lib()
+    nmax = int(L.jm_block_bcgstab_noalloc_max_cells())
+    assert 100_000 <= nmax < 200_000
+    dummy = np.zeros(8)
+    p = dummy.ctypes.data
+    st = L.jm_block_bcgstab_noalloc(nmax + 1, p, p, p, p, p, 1e-8, 1e-2, 30, None, None, None)
+    assert st == 1
+    msg = L.jm_last_error().decode()
+    assert "64 MiB" in msg and str(nmax) in msg
+
+
 def test_no_device_is_an_error_not_a_fallback():
     import torch
 
