@@ -221,6 +221,16 @@ jm_status jm_set_boundary_state(jm_ctx *c, int zone, const double *T, const doub
  * the hook overwrites them. Any pointer may be NULL. Needs a state. */
 jm_status jm_get_boundary_state(jm_ctx *c, int zone, double *T, double *vel, double *p);
 
+/* writeWallValues of the reference's case class (joe.cpp:43-130) on boundary
+ * zone `zone` at the resident state: per face 10 columns as the reference
+ * writes them (x_fa[3], rho, press, temp of the face's cell, tau_wall =
+ * mulam |u_t| / walld sign(u_x), qdot with the reference's 300 K wall, yplus,
+ * mulam of the face), then 6 more: the face's pressure force on the wall
+ * p_bfa n A[3] and its shear force mulam u_t / walld A[3] (their sums over the
+ * zone give the wall's force, hence lift and drag). out [n][16] may be NULL
+ * to query *count = n. Needs MU_REF > 0. */
+jm_status jm_wall_values(jm_ctx *c, int zone, double *out, int64_t *count);
+
 /* device views for callers that keep data on the GPU: the conservative state
  * as SoA double[5][ncv] (rho, rhou_x, rhou_y, rhou_z, rhoE) and the residual
  * in the same layout. */
@@ -264,6 +274,15 @@ int64_t jm_block_bcgstab_noalloc_max_cells(void);
  * with HIP events on that stream. ms_out has room for 8 values; names gets
  * a comma-separated list of the kernels timed. */
 jm_status jm_time_residual_kernels(jm_ctx *c, int reps, double *ms_out, char *names, int names_len);
+
+/* Measurement hook, no counterpart in the reference: the parts of one
+ * implicit step on the context's stream, reps times each, mean ms into
+ * ms_out[0..3]: the residual (calcStateVariables, setBC, calcDt, calcRhs or
+ * calcRhsCoupled), the Jacobian assembly with V/dt and the relaxed
+ * right-hand side, the preconditioner set-up (block LU, first residual) and
+ * one BiCGStab iteration, timed with its stop tests off. Leaves the
+ * context's Jacobian and Krylov vectors overwritten (the state is kept). */
+jm_status jm_time_step_kernels(jm_ctx *c, int reps, double *ms_out, char *names, int names_len);
 
 /* Mesh quality of the reference's geometry checks (UgpWithCv.cpp:1000-1091):
  * minimum over internal faces of s.n/(|s||n|) and the number of sub-tets

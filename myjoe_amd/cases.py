@@ -11,6 +11,9 @@
   (the serving workload's requests), free stream p = rho = 1.
 * ``random_state``: rho ~ U(0.5, 1.5), p ~ lognormal(0, 0.2),
   u_i ~ N(0, 0.3) * c, as conservative variables.
+* ``plate_case``: zero-pressure-gradient flat plate, Mach 0.2, Reynolds
+  number per unit length Re, Spalart-Allmaras in the coupled loop (6x6
+  blocks), free-stream nuSA = 3 nu.
 * ``with_sa``: adds the Spalart-Allmaras scalar nuSA to a viscous case.
 * ``coupled``: the same case under runBackwardEulerCoupled.
 """
@@ -187,3 +190,54 @@ def with_sa(joe_in: str, nu_init: float, zone_values: dict) -> str:
 # nuSA boundary values of the SA variants of the two test geometries
 SA_BUMP = {"inlet": 0.006, "bump_wall": 0.0}
 SA_BOX = {"inlet": 0.006, "wall": 0.0, "farfield": 0.004}
+
+
+MACH_PLATE = 0.2
+
+
+def plate_case(nx_up: int = 40, nx: int = 324, ny: int = 110, nz: int = 50, re: float = 5.0e6,
+               ratio: float = 1.1, nsteps: int = 20000, cfl_max: float = 1.0e3, resid_th: float = 0.0,
+               precision_note: str = "", ls: str = "MAX_ITER=30 ABS_RESID=1.0e-8 REL_RESID=1.0e-2",
+               ratio_x: float = 1.05, cfl_ramp: str = "AFTER_ITER=10 INTERVAL_ITER=5 FACTOR_CFL=1.2",
+               restart: str = "plate.msh"):
+    """Implicit RANS on a zero-pressure-gradient flat plate: Mach 0.2,
+    p = rho = 1 (R = 1, so T = 1), Re per unit length from a constant
+    viscosity MU_REF = U / Re (power law with T = T_REF at the free stream),
+    Spalart-Allmaras in the coupled loop (runBackwardEulerCoupled, 6x6
+    blocks), free-stream and inlet nuSA = 3 nu, nuSA = 0 on the plate.
+    Characteristic inlet and top, static-pressure outlet, symmetry ahead of
+    the plate and in z. The defaults give the 2M-cell plate (40 + 324 cells
+    along x, 110 across the boundary layer with the first cell at y+ < 1, 50
+    in z)."""
+    mesh = meshgen.flat_plate(nx_up, nx, ny, nz, ratio=ratio, ratio_x=ratio_x)
+    u = MACH_PLATE * math.sqrt(GAMMA)
+    mu = u / re
+    nu_sa = 3.0 * mu
+    lines = [
+        "GAMMA = 1.4", "P_REF = 1.0", "RHO_REF = 1.0", "T_REF = 1.0",
+        f"MU_REF = {mu!r}", "MU_MODE = POWERLAW", "MU_POWER_LAW = 0.76", "PR = 0.72", "PR_TURB = 0.9",
+        "SPACE_INTEGRATION = HLLC", "GRAD_RECONSTRUCTION SCHEME=LEASTSQUARES LIMITER=YES EPS=0.01",
+        "SPATIAL_SECOND_ORDER",
+        "TIME_INTEGRATION = BACKWARD_EULER_COUPLED", "LINEAR_SOLVER_NS = PETSC_GMRES",
+        f"LINEAR_SOLVER_NS_TRESHOLDS {ls}",
+        "TURB_MODEL = SA", f"INITIAL_CONDITION_TURB = {nu_sa!r}", "CHECK_INTERVAL = 1",
+        f"RESTART = {restart}", "CFL = 1.0", f"CFL_RAMP {cfl_ramp} MAX_CFL={cfl_max!r}",
+        f"NSTEPS = {nsteps}", f"RESID_ENERG_TH = {resid_th!r}",
+        f"U_INITIAL = {u!r} 0.0 0.0",
+        f"inlet = CBC {u!r} 0.0 0.0 1.0 1.0",
+        "outlet = CBC_SUBSONIC_OUTLET 1.0",
+        "symm_up = SYMMETRY",
+        "plate = WALL",
+        f"top = CBC {u!r} 0.0 0.0 1.0 1.0",
+        "zsym = SYMMETRY",
+        f"inlet.nuSA = {nu_sa!r}", f"top.nuSA = {nu_sa!r}", "plate.nuSA = 0.0",
+    ]
+    return mesh, "\n".join(lines) + "\n"
+
+
+def plate_uniform(ncv: int):
+    """The plate's free stream as the initial state (rho, rhou, rhoE)."""
+    u = MACH_PLATE * math.sqrt(GAMMA)
+    rhou = np.zeros((ncv, 3))
+    rhou[:, 0] = u
+    return np.ones(ncv), rhou, np.full(ncv, 1.0 / (GAMMA - 1.0) + 0.5 * u * u)

@@ -454,6 +454,17 @@ jm_status jm_block_bcgstab_noalloc(int ncv, const int *nbocv_i, const int *nbocv
 
 int64_t jm_block_bcgstab_noalloc_max_cells(void) { return (int64_t)jm::block_bcgstab_noalloc_max_cells(); }
 
+jm_status jm_wall_values(jm_ctx *c, int zone, double *out, int64_t *count) {
+  return guard([&] {
+    if (!c || zone < 0 || zone >= (int)c->mesh->zones.size()) throw std::exception();
+    const jm::Zone &z = c->mesh->zones[zone];
+    if (z.kind != jm::kZoneBoundary) throw std::invalid_argument("zone is not a boundary zone");
+    const int n = c->c->wall_values(z.ifa_f, z.ifa_l - z.ifa_f + 1, out);
+    if (count) *count = n;
+    return JM_OK;
+  });
+}
+
 jm_status jm_get_boundary_state(jm_ctx *c, int zone, double *T, double *vel, double *p) {
   return guard([&] {
     if (!c || zone < 0 || zone >= (int)c->mesh->zones.size()) throw std::exception();
@@ -492,6 +503,18 @@ jm_status jm_time_residual_kernels(jm_ctx *c, int reps, double *ms_out, char *na
   return guard([&] {
     if (!c || !ms_out || reps < 1) throw std::exception();
     std::string n = c->c->time_kernels(reps, ms_out);
+    if (names && names_len > 0) {
+      std::strncpy(names, n.c_str(), names_len - 1);
+      names[names_len - 1] = 0;
+    }
+    return JM_OK;
+  });
+}
+
+jm_status jm_time_step_kernels(jm_ctx *c, int reps, double *ms_out, char *names, int names_len) {
+  return guard([&] {
+    if (!c || !ms_out || reps < 1) throw std::exception();
+    std::string n = c->c->time_step_kernels(reps, ms_out);
     if (names && names_len > 0) {
       std::strncpy(names, n.c_str(), names_len - 1);
       names[names_len - 1] = 0;

@@ -238,6 +238,10 @@ class Ctx {
   // doneSolver (JoeWithModels.h:42-95) after a step, plus RESID_ENERG_REL
   bool done_solver(const StepInfo &s);
   void set_boundary_state(int ifa_f, int nface, const double *T, const double *vel, const double *p);
+  // writeWallValues (joe.cpp:43-130) on a boundary face range at the current
+  // state: out [nface][16] (x_fa, rho, press, temp, tau_wall, qdot, yplus,
+  // mulam, pressure and shear force of the face); returns nface
+  int wall_values(int ifa_f, int nface, double *out);
   // the boundary values setBC last computed on a face range (T_bfa, vel_bfa, p_bfa)
   void get_boundary_state(int ifa_f, int nface, double *T, double *vel, double *p);
   double *device_state();
@@ -248,6 +252,9 @@ class Ctx {
   void launch_residual();
   // mean ms per kernel of one residual evaluation; returns the kernel names
   std::string time_kernels(int reps, double *ms_out);
+  // mean ms of the residual, the Jacobian assembly, the Krylov set-up and one
+  // BiCGStab iteration of an implicit step; returns their names
+  std::string time_step_kernels(int reps, double *ms_out);
   int ncv() const;
 
   struct Impl;

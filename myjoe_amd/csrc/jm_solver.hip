@@ -2132,6 +2132,57 @@ __global__ void __launch_bounds__(kBlock) k_scal_c(DMesh m, DParams P, const dou
 }
 
 // ---------------------------------------------------------------------------
+// writeWallValues of the reference's case class (joe.cpp:43-130), one thread
+// per face of a boundary face range [f0, f0 + n): x_fa, the cell's rho, p, T,
+// tau_wall = mulam |u_t| / walld sign(u_x), qdot with the reference's wall
+// temperature 300, y+, mulam (the face's laminar viscosity from T_bfa), then
+// the face's force on the wall: pressure p_bfa n A and shear mulam u_t / walld A
+// (n the unit normal, A the area). out [n][16].
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_wall_values(DMesh m, DParams P, const double *__restrict__ q,
+                                                        const double *__restrict__ prim,
+                                                        const double *__restrict__ bfa, int f0, int n,
+                                                        double *__restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int f = f0 + k;
+  const int c = m.bfa_cell[f];
+  const double *geo = m.fgeo + 8 * (size_t)f;
+  const double *pc = prim + 8 * (size_t)c;
+  const double *bb = bfa + 8 * (size_t)f;
+  const double nv[3] = {geo[0], geo[1], geo[2]};
+  const double area = geo[3];
+  const double rho = q[c];
+  double s_half[3], vel[3], velTang[3];
+  for (int i = 0; i < 3; i++) s_half[i] = geo[4 + i] - pc[5 + i];
+  for (int i = 0; i < 3; i++) vel[i] = q[(size_t)(1 + i) * m.ncv + c] / rho;
+  const double un = dot3(nv, vel);
+  for (int i = 0; i < 3; i++) velTang[i] = vel[i] - un * nv[i];
+  const double velMag = sqrt(dot3(velTang, velTang));
+  const double walld = fabs(dot3(s_half, nv));
+  const double wallTemp = 300.0;
+  const double mulam = mu_lam(P, bb[5]);
+  const double sgn = vel[0] > 0 ? 1.0 : (vel[0] < 0.0 ? -1.0 : 0.0);
+  const double tau = mulam * velMag / walld * sgn;
+  const double utau = sqrt(fabs(tau) / rho);
+  const double yplus = walld * utau / (mulam / rho);
+  const double temp = pc[4] / (pc[0] * P.R);
+  const double kTotal = P.R * P.gamma / (P.gamma - 1.0) * mulam / P.pr;
+  const double qDot = kTotal * (temp - wallTemp) / walld;
+  double *o = out + 16 * (size_t)k;
+  for (int i = 0; i < 3; i++) o[i] = geo[4 + i];
+  o[3] = rho;
+  o[4] = pc[4];
+  o[5] = temp;
+  o[6] = tau;
+  o[7] = qDot;
+  o[8] = yplus;
+  o[9] = mulam;
+  for (int i = 0; i < 3; i++) o[10 + i] = bb[4] * nv[i] * area;
+  for (int i = 0; i < 3; i++) o[13 + i] = mulam * velTang[i] / walld * area;
+}
+
+// ---------------------------------------------------------------------------
 // Deterministic two-stage reductions: every block writes its partial and a
 // one-block finalize kernel combines them in a fixed order. No device-scope
 // fences or tickets: on a multi-XCD part a device-scope release writes back
@@ -5435,6 +5486,56 @@ std::string Ctx::time_kernels(int reps, double *ms) {
   return "k_state,k_bc,k_grad,k_flux,residual";
 }
 
+// Mean ms of the parts of an implicit step: the residual (state, BC,
+// gradients, fluxes and dt; with the coupled scalar its row), the Jacobian
+// assembly of a step (blocks, V/dt, relaxed right-hand side), the block LU
+// of the preconditioner, and one BiCGStab iteration (its nine kernels) with
+// the stop tests switched off so that every timed iteration runs.
+std::string Ctx::time_step_kernels(int reps, double *ms) {
+  Impl &d = *d_;
+  if (!d.have_state) throw std::logic_error("no state set");
+  hipEvent_t e0, e1;
+  HIPCHECK(hipEventCreate(&e0));
+  HIPCHECK(hipEventCreate(&e1));
+  auto timed = [&](auto &&launch, int n) {
+    launch();  // warm
+    HIPCHECK(hipEventRecord(e0, d.stream));
+    for (int r = 0; r < n; r++) launch();
+    HIPCHECK(hipEventRecord(e1, d.stream));
+    HIPCHECK(hipEventSynchronize(e1));
+    float t = 0.f;
+    HIPCHECK(hipEventElapsedTime(&t, e0, e1));
+    return (double)t / n;
+  };
+  auto parts = [&](auto tag) {
+    using T = decltype(tag);
+    d.ensure_solver<T>(10, true);
+    ms[0] = timed([&] {
+      d.launch_state_bc();
+      d.launch_grad_flux(false);
+    }, reps);
+    ms[1] = timed([&] { d.launch_jac<T>(true); }, reps);
+    BcgSys y = d.bcg_sys<T>();
+    y.maxiter = 1 << 30;
+    y.zero_abs = 0.0;
+    y.zero_rel = 0.0;
+    const int nb = d.nbs;
+    ms[2] = timed([&] {
+      if (nb == 6) bcg_prologue<T, 6>(y);
+      else bcg_prologue<T, 5>(y);
+    }, reps);
+    ms[3] = timed([&] {
+      if (nb == 6) bcg_iterations<T, 6>(y, 1);
+      else bcg_iterations<T, 5>(y, 1);
+    }, reps);
+  };
+  if (d.precision == 0) parts(double());
+  else parts(float());
+  HIPCHECK(hipEventDestroy(e0));
+  HIPCHECK(hipEventDestroy(e1));
+  return "residual,jacobian,krylov_setup,krylov_iteration";
+}
+
 void Ctx::calc_rhs(bool implicit) {
   Impl &d = *d_;
   if (!d.have_state) throw std::logic_error("no state set");
@@ -5683,6 +5784,22 @@ void Ctx::set_boundary_state(int ifa_f, int nface, const double *T, const double
   HIPCHECK(hipMemcpyAsync(d.hook.p + 5 * (size_t)ifa_f, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
                           d.stream));
   HIPCHECK(hipStreamSynchronize(d.stream));
+}
+
+int Ctx::wall_values(int ifa_f, int nface, double *out) {
+  Impl &d = *d_;
+  if (ifa_f < 0 || nface < 0 || ifa_f + nface > d.nfa_b) throw std::out_of_range("boundary face range");
+  if (!d.have_state) throw std::logic_error("no state set");
+  if (!out || nface == 0) return nface;
+  if (!d.dp.visc) throw std::invalid_argument("wall values need MU_REF > 0");
+  DBuf<double> o;
+  o.alloc(16 * (size_t)nface);
+  hipLaunchKernelGGL(k_wall_values, dim3(nblk(nface)), dim3(kBlock), 0, d.stream, d.dm, d.dp, d.q.p, d.prim.p, d.bfa.p,
+                     ifa_f, nface, o.p);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipMemcpyAsync(out, o.p, 16 * (size_t)nface * sizeof(double), hipMemcpyDeviceToHost, d.stream));
+  HIPCHECK(hipStreamSynchronize(d.stream));
+  return nface;
 }
 
 void Ctx::get_boundary_state(int ifa_f, int nface, double *T, double *vel, double *p) {

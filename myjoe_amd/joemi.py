@@ -79,6 +79,8 @@ def lib():
         L.jm_block_bcgstab_noalloc_max_cells.argtypes = []
         L.jm_block_bcgstab_noalloc_max_cells.restype = C.c_int64
         L.jm_add_source_coupled.argtypes = [vp, C.c_int, vp, vp]
+        L.jm_wall_values.argtypes = [vp, C.c_int, vp, vp]
+        L.jm_time_step_kernels.argtypes = [vp, C.c_int, vp, C.c_char_p, C.c_int]
         L.jm_device_state.argtypes = [vp, vp]
         L.jm_device_rhs.argtypes = [vp, vp]
         L.jm_sync.argtypes = [vp]
@@ -390,6 +392,16 @@ class Solver:
         _check(lib().jm_run(self.h, max_steps, _ptr(hist), C.byref(n)))
         return n.value, hist[:n.value]
 
+    def wall_values(self, zone):
+        """jm_wall_values of a boundary zone (its index or name): [n][16]."""
+        if isinstance(zone, str):
+            zone = [z[0] for z in self.mesh.zones()].index(zone)
+        n = C.c_int64()
+        _check(lib().jm_wall_values(self.h, zone, None, C.byref(n)))
+        out = np.empty((n.value, 16))
+        _check(lib().jm_wall_values(self.h, zone, _ptr(out), C.byref(n)))
+        return out
+
     def set_boundary_state(self, zone, T, vel, p):
         T = np.ascontiguousarray(T, np.float64)
         vel = np.ascontiguousarray(vel, np.float64)
@@ -414,6 +426,15 @@ class Solver:
 
     def sync(self):
         _check(lib().jm_sync(self.h))
+
+    def time_step_kernels(self, reps=10):
+        """jm_time_step_kernels: mean ms of residual, Jacobian, Krylov set-up
+        and one BiCGStab iteration."""
+        ms = np.zeros(8)
+        buf = C.create_string_buffer(256)
+        _check(lib().jm_time_step_kernels(self.h, reps, _ptr(ms), buf, 256))
+        names = buf.value.decode().split(",")
+        return dict(zip(names, ms[:len(names)]))
 
     def time_kernels(self, reps=10):
         ms = np.zeros(8)

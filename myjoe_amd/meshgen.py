@@ -17,6 +17,10 @@ Workloads
 * ``naca_omesh``: O-mesh of ni x nj hexes around a NACA 4-digit airfoil (closed
   trailing edge, chord 1) out to a circle of radius 20, one hex layer in z,
   geometric radial stretching (the serving workload's airfoils).
+* ``flat_plate``: zero-pressure-gradient flat plate, x in [-0.33, 2] with the
+  plate from x = 0, height 1, cells stretched geometrically from the wall
+  (ratio 1.1 in the implicit RANS flat-plate solve) and towards the leading
+  edge, nz hex layers in z.
 * ``split_box``: [0,1]^3 box of n^3 hexes, a seeded fraction split into six
   Kuhn tetrahedra each (globally consistent diagonals), interior nodes
   jittered uniformly by +-jitter*h (160^3, 50 %, 0.2 in the single-residual
@@ -27,7 +31,7 @@ from __future__ import annotations
 import dataclasses
 import numpy as np
 
-__all__ = ["MshMesh", "bump_channel", "split_box", "naca_omesh", "write_msh"]
+__all__ = ["MshMesh", "bump_channel", "flat_plate", "plate_spacing", "split_box", "naca_omesh", "write_msh"]
 
 
 @dataclasses.dataclass
@@ -130,6 +134,81 @@ def bump_channel(nx: int = 96, ny: int = 32, thickness: float = 0.1, dz: float |
     faces = [(int_nodes, int_cells, 3), (zn_i, zc_i, 4), (zn_o, zc_o, 5), (zn_w, zc_w, 6),
              (zn_t, zc_t, 7), (np.concatenate([zn_z0, zn_z1]), np.concatenate([zc_z0, zc_z1]), 8)]
     return _assemble(x_no, faces, nx * ny, zones)
+
+
+def _geometric(n: int, length: float, ratio: float) -> np.ndarray:
+    """n + 1 coordinates from 0 to length whose spacing grows by ratio."""
+    if abs(ratio - 1.0) < 1e-14:
+        return np.linspace(0.0, length, n + 1)
+    d0 = length * (ratio - 1.0) / (ratio ** n - 1.0)
+    x = np.concatenate([[0.0], np.cumsum(d0 * ratio ** np.arange(n))])
+    x[-1] = length
+    return x
+
+
+def plate_spacing(ny: int, ratio: float = 1.1, height: float = 1.0) -> float:
+    """Height of the first cell above the plate."""
+    return float(_geometric(ny, height, ratio)[1])
+
+
+def flat_plate(nx_up: int, nx: int, ny: int, nz: int = 1, ratio: float = 1.1, x_up: float = 0.33,
+               length: float = 2.0, height: float = 1.0, ratio_x: float = 1.05, dz: float | None = None) -> MshMesh:
+    """Zero-pressure-gradient flat plate: x in [-x_up, length] with the plate
+    from x = 0, y in [0, height], nz hex layers in z. Cells grow from the
+    wall by ratio in y, from the leading edge by ratio_x along the plate and
+    towards the inlet upstream of it.
+
+    Zones: inlet (x = -x_up), outlet (x = length), symm_up (y = 0, x < 0),
+    plate (y = 0, x > 0), top (y = height), zsym (both z faces)."""
+    xp = _geometric(nx, length, ratio_x)
+    xu = -_geometric(nx_up, x_up, ratio_x)[::-1]
+    xs = np.concatenate([xu[:-1], xp])
+    nxt = nx_up + nx
+    ys = _geometric(ny, height, ratio)
+    if dz is None:
+        dz = length / nx
+    zs = dz * np.arange(nz + 1)
+    X = np.empty((nz + 1, ny + 1, nxt + 1, 3))
+    X[..., 0] = xs[None, None, :]
+    X[..., 1] = ys[None, :, None]
+    X[..., 2] = zs[:, None, None]
+    x_no = X.reshape(-1, 3)
+
+    def nid(i, j, k):
+        return (k * (ny + 1) + j) * (nxt + 1) + i
+
+    K, J, I = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nxt), indexing="ij")
+    I, J, K = I.reshape(-1), J.reshape(-1), K.reshape(-1)
+    cid = (K * ny + J) * nxt + I
+    corners = np.stack([nid(I, J, K), nid(I + 1, J, K), nid(I + 1, J + 1, K), nid(I, J + 1, K),
+                        nid(I, J, K + 1), nid(I + 1, J, K + 1), nid(I + 1, J + 1, K + 1), nid(I, J + 1, K + 1)], 1)
+    # interior faces, cell-major: +x, +y, +z face of each cell
+    blocks_nodes, blocks_cells, order_key = [], [], []
+    for axis, m, step in ((0, I < nxt - 1, 1), (1, J < ny - 1, nxt), (2, K < nz - 1, nxt * ny)):
+        blocks_nodes.append(corners[m][:, _HEX_FACES[(axis, 1)]])
+        blocks_cells.append(np.stack([cid[m], cid[m] + step], 1))
+        order_key.append(cid[m] * 3 + axis)
+    key = np.concatenate(order_key)
+    o = np.argsort(key, kind="stable")
+    int_nodes = np.concatenate(blocks_nodes)[o]
+    int_cells = np.concatenate(blocks_cells)[o]
+
+    def bnd(mask, axis, side):
+        return corners[mask][:, _HEX_FACES[(axis, side)]], np.stack([cid[mask], -np.ones(mask.sum(), np.int64)], 1)
+
+    zones = [(3, "interior", "int_fluid"), (4, "velocity-inlet", "inlet"), (5, "pressure-outlet", "outlet"),
+             (6, "symmetry", "symm_up"), (7, "wall", "plate"), (8, "pressure-far-field", "top"),
+             (9, "symmetry", "zsym")]
+    zi = bnd(I == 0, 0, 0)
+    zo = bnd(I == nxt - 1, 0, 1)
+    zu = bnd((J == 0) & (I < nx_up), 1, 0)
+    zp = bnd((J == 0) & (I >= nx_up), 1, 0)
+    zt = bnd(J == ny - 1, 1, 1)
+    z0 = bnd(K == 0, 2, 0)
+    z1 = bnd(K == nz - 1, 2, 1)
+    faces = [(int_nodes, int_cells, 3), (zi[0], zi[1], 4), (zo[0], zo[1], 5), (zu[0], zu[1], 6),
+             (zp[0], zp[1], 7), (zt[0], zt[1], 8), (np.concatenate([z0[0], z1[0]]), np.concatenate([z0[1], z1[1]]), 9)]
+    return _assemble(x_no, faces, nxt * ny * nz, zones)
 
 
 def naca_surface(psi: np.ndarray, thickness: float, camber: float, camber_pos: float = 0.4,

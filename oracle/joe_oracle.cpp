@@ -3212,4 +3212,53 @@ int or_linear_solve(or_ctx *c) {
   return it;
 }
 
+/* writeWallValues of the reference's case class (joe.cpp:43-130) on boundary
+ * zone `zone` (index among the boundary zones) at the current state and
+ * boundary values: per face 10 columns x_fa (3), rho, press, temp of its
+ * cell, tau_wall = mulam |u_t| / walld sign(u_x), qdot with the reference's
+ * wall temperature 300, yplus and mulam, the face's laminar viscosity
+ * (mul_fa of the boundary face, from T_bfa); then the face's force on the
+ * wall in 6 more columns: pressure p_bfa n A (3) and shear mulam u_t / walld
+ * A (3), n the unit normal and A the area. out [n][16] (may be NULL);
+ * returns n or -1. */
+int or_wall_values(or_ctx *c, int zone, double *out) {
+  const or_mesh &m = c->m;
+  if (zone < 0 || zone >= m.nzone) return -1;
+  const int f0 = m.zone_ifa[2 * zone], f1 = m.zone_ifa[2 * zone + 1];
+  const int n = f1 - f0 + 1;
+  if (!out) return n;
+  const double R = c->p.R_gas, gam = c->p.gamma;
+  for (int ifa = f0; ifa <= f1; ifa++) {
+    const int icv = m.cvofa[2 * ifa];
+    double nv[3], s_half[3], vel[3], velTang[3];
+    const double area = norm3(nv, &m.fa_normal[3 * ifa]);
+    for (int i = 0; i < 3; i++) s_half[i] = m.x_fa[3 * ifa + i] - m.x_cv[3 * icv + i];
+    for (int i = 0; i < 3; i++) vel[i] = c->rhou[3 * icv + i] / c->rho[icv];
+    const double un = dot3(nv, vel);
+    for (int i = 0; i < 3; i++) velTang[i] = vel[i] - un * nv[i];
+    const double velMag = sqrt(dot3(velTang, velTang));
+    const double walld = fabs(dot3(s_half, nv));
+    const double wallTemp = 300.0;
+    const double mulam = c->mul_fa[ifa];
+    const double sgn = vel[0] > 0 ? 1.0 : (vel[0] < 0.0 ? -1.0 : 0.0);
+    const double tau = mulam * velMag / walld * sgn;
+    const double utau = sqrt(fabs(tau) / c->rho[icv]);
+    const double yplus = walld * utau / (mulam / c->rho[icv]);
+    const double kTotal = R * gam / (gam - 1.0) * mulam / c->p.pr;
+    const double qDot = kTotal * (c->temp[icv] - wallTemp) / walld;
+    double *o = out + 16 * (size_t)(ifa - f0);
+    for (int i = 0; i < 3; i++) o[i] = m.x_fa[3 * ifa + i];
+    o[3] = c->rho[icv];
+    o[4] = c->press[icv];
+    o[5] = c->temp[icv];
+    o[6] = tau;
+    o[7] = qDot;
+    o[8] = yplus;
+    o[9] = mulam;
+    for (int i = 0; i < 3; i++) o[10 + i] = c->p_bfa[ifa] * nv[i] * area;
+    for (int i = 0; i < 3; i++) o[13 + i] = mulam * velTang[i] / walld * area;
+  }
+  return n;
+}
+
 }  // extern "C"

@@ -108,6 +108,11 @@ long long or_get(const or_ctx *c, const char *name, double *out);
 double or_cfl(const or_ctx *c);
 int or_nsteps_done(const or_ctx *c);
 
+/* writeWallValues (joe.cpp:43-130) on a boundary zone: out [n][16] = x_fa (3),
+ * rho, press, temp, tau_wall, qdot, yplus, mulam, pressure force (3), shear
+ * force (3); returns n (out may be NULL) or -1 */
+int or_wall_values(or_ctx *c, int zone, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -82,6 +82,7 @@ def lib():
         L.or_cfl.argtypes = [C.c_void_p]
         L.or_cfl.restype = C.c_double
         L.or_linear_solve.argtypes = [C.c_void_p]
+        L.or_wall_values.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.or_linear_solve.restype = C.c_int
         _LIB = L
     return _LIB
@@ -215,6 +216,7 @@ class Oracle:
             if key in P:
                 zscal[i] = (1.0, float(P[key][0][1]))
         self._keep.update(zbc=zbc, zifa=zifa, zpar=zpar, zscal=zscal)
+        self.zone_names = [z[0] for z in zl]
         self.nscal = 1 if self.params.sa else 0
         m = _Mesh()
         m.ncv, m.nfa, m.nfa_b, m.nnz = self.ncv, self.nfa, self.nfa_b, self.nnz
@@ -282,6 +284,14 @@ class Oracle:
         hist = np.zeros((max_steps, 5 + self.nscal))
         n = lib().or_run(self.ctx, max_steps, hist.ctypes.data)
         return n, hist[:n]
+
+    def wall_values(self, zone: str) -> np.ndarray:
+        """writeWallValues of a boundary zone: [n][16] (see or_wall_values)."""
+        k = self.zone_names.index(zone)
+        n = lib().or_wall_values(self.ctx, k, None)
+        out = np.empty((n, 16))
+        lib().or_wall_values(self.ctx, k, out.ctypes.data)
+        return out
 
     def get(self, name: str) -> np.ndarray:
         n = lib().or_get(self.ctx, name.encode(), None)
