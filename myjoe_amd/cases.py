@@ -197,7 +197,7 @@ MACH_PLATE = 0.2
 
 def plate_case(nx_up: int = 40, nx: int = 324, ny: int = 110, nz: int = 50, re: float = 5.0e6,
                ratio: float = 1.1, nsteps: int = 20000, cfl_max: float = 1.0e3, resid_th: float = 0.0,
-               precision_note: str = "", ls: str = "MAX_ITER=30 ABS_RESID=1.0e-8 REL_RESID=1.0e-2",
+               ls: str = "MAX_ITER=30 ABS_RESID=1.0e-8 REL_RESID=1.0e-2",
                ratio_x: float = 1.05, cfl_ramp: str = "AFTER_ITER=10 INTERVAL_ITER=5 FACTOR_CFL=1.2",
                restart: str = "plate.msh"):
     """Implicit RANS on a zero-pressure-gradient flat plate: Mach 0.2,

@@ -76,6 +76,9 @@ struct DParams {
   // grad nuSA (3), face-averaged mul, strain and vorticity magnitudes, wall distance
   int sa, pad2;
   const double *sa_phi, *sa_aux;
+  // parity readout: when set, the per-entry flux kernel writes each face's
+  // inviscid flux [nfa][5] (device face order) once, from the face's first cell
+  double *fdump;
 };
 
 // per-cell Spalart-Allmaras fields in DParams::sa_aux
@@ -2566,6 +2569,8 @@ __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const
       for (int i = 0; i < 3; i++) F[1 + i] = fr.p * n[i] * area;
       F[4] = 0.0;
     }
+    if (P.fdump && en.x >= 0)
+      for (int k = 0; k < 5; k++) P.fdump[5 * (size_t)f + k] = F[k];
     if (VISC) {
       if (CPL)
         for (int k = 0; k < 5; k++) slot[6 + k] = vcc[k];
@@ -3993,6 +3998,7 @@ struct Ctx::Impl {
   int nwave = 0;
   DBuf<ZoneDev> zones;
   std::vector<int> perm;              // device cell -> reference cell
+  std::vector<int> fnew;              // reference face -> device face
   std::vector<int> h_dev_nbocv_i, h_ref_nbocv_i;
   bool dup_blocks = false;
   // state
@@ -4248,6 +4254,7 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
     }
   }
   if (nf != nfa) throw std::runtime_error("internal face not reached by any cell");
+  d.fnew = fnew;
   if (d.par.flux_kernel == kResidFace && !(d.par.mu_ref > 0.0) && nfa > nfa_b) {
     std::vector<int2> fc(nfa - nfa_b);
     for (int f = nfa_b; f < nfa; f++)
@@ -4507,6 +4514,7 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
   d.dp.sa = d.par.sa ? 1 : 0;
   d.nbs = (d.par.coupled && d.par.sa) ? 6 : 5;
   d.dp.sa_phi = d.sa_phi.p;
+  d.dp.fdump = nullptr;
   d.dp.sa_aux = d.sa_aux.p;
   d.cfl = d.par.cfl;
   HIPCHECK(hipDeviceSynchronize());
@@ -5916,6 +5924,27 @@ int64_t Ctx::get(const std::string &name, double *out) {
       }
       return (int64_t)(6 * nnz);
     }
+  }
+  if (name == "FLUX_FA") {  // each face's inviscid flux [nfa][5], reference numbering
+    const size_t nf = d.nfa;
+    if (!out) return (int64_t)(5 * nf);
+    if (!d.have_state) throw std::logic_error("no state set");
+    if (!d.flux_entry() || d.face_form()) throw Unsupported("FLUX_FA is written by the per-entry flux kernel");
+    DBuf<double> fd;
+    fd.alloc(5 * nf);
+    d.dp.fdump = fd.p;
+    try {
+      d.launch_state_bc();
+      d.launch_grad_flux(false);
+    } catch (...) {
+      d.dp.fdump = nullptr;
+      throw;
+    }
+    d.dp.fdump = nullptr;
+    copy(fd.p, 5 * nf * sizeof(double));
+    for (size_t f = 0; f < nf; f++)
+      for (int k = 0; k < 5; k++) out[5 * f + k] = h[5 * (size_t)d.fnew[f] + k];
+    return (int64_t)(5 * nf);
   }
   if (name == "COUNTERS") {  // negative pressures, first-order fallbacks
     if (out) {

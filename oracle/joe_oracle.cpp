@@ -1006,6 +1006,9 @@ struct or_ctx {
   std::vector<double> rhs_scal, AScal, dscal;
   // the coupled loop with the scalar: its 6x6 blocks and correction [ncv][6]
   std::vector<double> A6, dq6;
+  // each face's inviscid flux [nfa][5] as the face loops last formed it
+  // (area included, icv0 -> icv1 orientation; boundary faces out of the domain)
+  std::vector<double> flux_fa;
   double cfl;
   int step;
   bool have_press;
@@ -1525,6 +1528,11 @@ static void euler_gradients(or_ctx &c) {
   }
 }
 
+// the face's inviscid flux as the face loop adds it (FLUX_FA)
+inline void keep_flux(or_ctx &c, int ifa, const double *F) {
+  for (int k = 0; k < 5; k++) c.flux_fa[5 * (size_t)ifa + k] = F[k];
+}
+
 void calc_euler_flux(or_ctx &c, double *A) {
   const or_mesh &m = c.m;
   const bool snd = c.p.second_order != 0;
@@ -1575,6 +1583,7 @@ void calc_euler_flux(or_ctx &c, double *A) {
       }
     }
     face_flux(c, F, rho0, u0, p0, rho1, u1, p1, area, nVec, sc0, sc1, sa ? &Fs : nullptr);
+    keep_flux(c, ifa, F);
     c.rhs_rho[icv0] -= F[0];
     for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= F[1 + i];
     c.rhs_rhoE[icv0] -= F[4];
@@ -1618,10 +1627,14 @@ void calc_euler_flux(or_ctx &c, double *A) {
         face_flux(c, F, c.rho_bfa[ifa], ub, c.p_bfa[ifa], c.rho_bfa[ifa], ub, c.p_bfa[ifa], area, nVec, sb, sb,
                   sa ? &Fs : nullptr);
         if (bc == OR_BC_SYMMETRY && c.p.symm_bc == OR_SYMM_ANALYTICAL) {
+          const double Fp[5] = {0.0, c.p_bfa[ifa] * nVec[0] * area, c.p_bfa[ifa] * nVec[1] * area,
+                                c.p_bfa[ifa] * nVec[2] * area, 0.0};
+          keep_flux(c, ifa, Fp);
           for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= c.p_bfa[ifa] * nVec[i] * area;
           c.rhs_rho[icv0] -= 0.0;
           c.rhs_rhoE[icv0] -= 0.0;
         } else {
+          keep_flux(c, ifa, F);
           c.rhs_rho[icv0] -= F[0];
           for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= F[1 + i];
           c.rhs_rhoE[icv0] -= F[4];
@@ -1675,6 +1688,7 @@ void calc_euler_flux(or_ctx &c, double *A) {
           }
         }
         face_flux(c, F, rho0, u0, p0, c.rho_bfa[ifa], ub, c.p_bfa[ifa], area, nVec, sc0, sb, sa ? &Fs : nullptr);
+        keep_flux(c, ifa, F);
         c.rhs_rho[icv0] -= F[0];
         for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= F[1 + i];
         c.rhs_rhoE[icv0] -= F[4];
@@ -1885,6 +1899,7 @@ void calc_flux_coupled(or_ctx &c, double *A) {
       h1 = thermo_h(rho1, p1);
     }
     flux_hllc_coupled(F, rho0, u0, p0, h0, g, rho1, u1, p1, h1, g, area, nVec, false, sc, sa ? &Fs : NULL);
+    keep_flux(c, ifa, F);
     c.rhs_rho[icv0] -= F[0];
     for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= F[1 + i];
     c.rhs_rhoE[icv0] -= F[4];
@@ -1975,6 +1990,7 @@ void calc_flux_coupled(or_ctx &c, double *A) {
       if (bc == OR_BC_SYMMETRY || bc == OR_BC_WALL) {
         const bool op = c.p.symm_bc == OR_SYMM_ANALYTICAL;
         flux_hllc_coupled(F, rb, ub, pb, hb, g, rb, ub, pb, hb, g, area, nVec, op, scb, sa ? &Fs : NULL);
+        keep_flux(c, ifa, F);
         c.rhs_rho[icv0] -= F[0];
         for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= F[1 + i];
         c.rhs_rhoE[icv0] -= F[4];
@@ -2020,6 +2036,7 @@ void calc_flux_coupled(or_ctx &c, double *A) {
         }
       } else if (bc == OR_BC_NEUMANN) {
         flux_hllc_coupled(F, rb, ub, pb, hb, g, rb, ub, pb, hb, g, area, nVec, false, scb, sa ? &Fs : NULL);
+        keep_flux(c, ifa, F);
         c.rhs_rho[icv0] -= F[0];
         for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= F[1 + i];
         c.rhs_rhoE[icv0] -= F[4];
@@ -2076,6 +2093,7 @@ void calc_flux_coupled(or_ctx &c, double *A) {
           h0 = thermo_h(rho0, p0);
         }
         flux_hllc_coupled(F, rho0, u0, p0, h0, g, rb, ub, pb, hb, g, area, nVec, false, sc, sa ? &Fs : NULL);
+        keep_flux(c, ifa, F);
         c.rhs_rho[icv0] -= F[0];
         for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= F[1 + i];
         c.rhs_rhoE[icv0] -= F[4];
@@ -2995,6 +3013,7 @@ or_ctx *or_ctx_create(const or_mesh *m, const or_params *p) {
   c->grad_h.assign(3 * n, 0.0);
   c->resid.assign(p->sa ? 6 : 5, 0.0);
   c->mut_fa.assign(m->nfa, 0.0);
+  c->flux_fa.assign(5 * (size_t)m->nfa, 0.0);
   c->sa = p->sa != 0;
   if (c->sa) {
     c->phi.assign(n, p->sa_init);  // initialHook, JoeWithModels.cpp:219-240
@@ -3160,6 +3179,7 @@ long long or_get(const or_ctx *c, const char *name, double *out) {
   else if (n == "TEMP") v = &c->temp;
   else if (n == "ENTHALPY") v = &c->enthalpy;
   else if (n == "MUL_FA") v = &c->mul_fa;
+  else if (n == "FLUX_FA") v = &c->flux_fa;
   else if (n == "GRAD_ENTHALPY") v = &c->grad_h;
   else if (n == "SOS") v = &c->sos;
   else if (n == "RHO_BFA") v = &c->rho_bfa;

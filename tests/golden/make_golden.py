@@ -13,6 +13,9 @@ Fixtures (numpy .npz, inputs and the reference's outputs only):
                        the final state after NSTEPS
   coupled_<name>_hllc  random state and calcRhsCoupled's residual, Jacobian
                        blocks, primitive and boundary states, local dt
+  steady_bump96.npz    the 96x32 bump's steady state from the reference's
+                       run() (python tests/golden/make_golden.py steady_bump96;
+                       not part of the default set: ~25 minutes of CPU)
 """
 from __future__ import annotations
 
@@ -139,6 +142,11 @@ def write_profiles(path, tab):
         f.write(f"n={tab.shape[0]}\td={tab.shape[1]}\n")
         for r in tab:
             f.write(" ".join(repr(float(v)) for v in r) + "\n")
+
+
+# the 96x32 bump's steady state (steady_bump96): steps and linear solve
+STEADY_STEPS = 6000
+STEADY_LS = "MAX_ITER=60 ABS_RESID=1.0e-14 REL_RESID=1.0e-3"
 
 
 def run_ref(args, cwd):
@@ -330,6 +338,28 @@ def main(only=None):
             np.savez_compressed(os.path.join(HERE, "solve_restart_bump24.npz"), **keep)
             print("wrote solve_restart_bump24, final step", int(d["STEP"][0]))
 
+        # the 96x32 bump run to its steady state by the reference itself, with
+        # a linear solve tight enough to converge (ABS_RESID = 1e-14): the
+        # final state of two runs, STEADY_STEPS and 1.5 times as many, whose
+        # difference says how far the state still moved
+        if only and only in "steady_bump96":
+            nsteps = STEADY_STEPS
+            finals = []
+            for n in (nsteps, nsteps * 3 // 2):
+                mm, joe = cases.bump_case(96, 32, nsteps=n, ls=STEADY_LS)
+                meshgen.write_msh(mm, os.path.join(wd, "bump.msh"))
+                with open(os.path.join(wd, "Joe.in"), "w") as f:
+                    f.write(joe)
+                run_ref(["solve", "Joe.in", "solve.bin", "0"], wd)
+                d = jmdump.read(os.path.join(wd, "solve.bin"))
+                finals.append({k: d[k] for k in ("RHO", "RHOU", "RHOE", "STEP")})
+                print("steady_bump96: reference ran", int(d["STEP"][0]), "steps", flush=True)
+            keep = dict(finals[1])
+            keep["MOVED"] = np.array([np.max(np.abs(finals[1][k] - finals[0][k])) / np.max(np.abs(finals[1][k]))
+                                      for k in ("RHO", "RHOU", "RHOE")])
+            keep["JOE_IN"] = np.frombuffer(joe.encode(), np.uint8)
+            np.savez_compressed(os.path.join(HERE, "steady_bump96.npz"), **keep)
+            print("wrote steady_bump96, moved", keep["MOVED"])
         if only:
             return
         # steady-loop trajectories of JoeWithModels::run (uniform initial state)
