@@ -147,7 +147,7 @@ def cpu_baseline(n_box=48, reps=12):
     return out
 
 
-def plate_bench(dims=(40, 324, 110, 50), steps=3, reps=10):
+def plate_bench(dims=(40, 324, 110, 50), steps=3, reps=10, jac_kernel=None):
     """One implicit step of the 2M-cell flat plate (Mach 0.2, Re 5e6 per unit
     length, Spalart-Allmaras in the coupled loop, 6x6 blocks), FP64 and the
     mixed form: wall-clock ms per step from the free stream (host-polled
@@ -162,6 +162,8 @@ def plate_bench(dims=(40, 324, 110, 50), steps=3, reps=10):
 
     t0 = time.perf_counter()
     mm, joe = cases.plate_case(*dims)
+    if jac_kernel:
+        joe += f"GPU_JAC_KERNEL = {jac_kernel}\n"
     mesh = joemi.Mesh.from_faces(mm)
     del mm
     gc.collect()
@@ -343,6 +345,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--grad-kernel", default=None, help="GPU_GRAD_KERNEL: ENTRY or CELL")
     ap.add_argument("--flux-kernel", default=None, help="GPU_FLUX_KERNEL: ENTRY or CELL")
+    ap.add_argument("--jac-kernel", default=None, help="GPU_JAC_KERNEL: ENTRY or CELL")
     ap.add_argument("--cell-order", default=None,
                     help="device cell numbering (GPU_CELL_ORDER): NATIVE, GROUPED, MORTON, MORTON_GROUPED")
     ap.add_argument("--serve", type=int, default=16, help="serving requests (0: skip the serving leg)")
@@ -404,6 +407,8 @@ def main():
             joe += f"GPU_GRAD_KERNEL = {args.grad_kernel}\n"
         if args.flux_kernel:
             joe += f"GPU_FLUX_KERNEL = {args.flux_kernel}\n"
+        if args.jac_kernel:
+            joe += f"GPU_JAC_KERNEL = {args.jac_kernel}\n"
         s = joemi.Solver(mesh, joe, joemi.JM_FP64)
         s.set_state(*state)
         for _ in range(args.warmup):
@@ -475,7 +480,8 @@ def main():
     del mesh
     gc.collect()
     if args.plate:
-        out["config"]["implicit_rans_flat_plate"] = plate_bench(tuple(int(x) for x in args.plate_dims.split(",")))
+        out["config"]["implicit_rans_flat_plate"] = plate_bench(tuple(int(x) for x in args.plate_dims.split(",")),
+                                                                jac_kernel=args.jac_kernel)
     if args.serve > 0:
         lo, hi = (int(x) for x in args.serve_cells.split(","))
         out["serving"] = serving_bench(args.serve, 7, lo, hi, args.serve_active, args.serve_steps, args.serve_rel, 64.0,

@@ -63,6 +63,7 @@ struct SolverParams {
   int visc_mode = 0;
   bool visc_limiter = true;
   int grad_kernel = kResidCell, flux_kernel = kResidEntry;
+  int jac_kernel = kResidEntry;  // GPU_JAC_KERNEL: ENTRY (one lane per cell-face entry) or CELL (one thread per row)
   // Spalart-Allmaras as one segregated scalar nuSA (TURB_MODEL = SA): initial
   // value (INITIAL_CONDITION_TURB) and UNDER_RELAXATION_SCALARS
   bool sa = false;

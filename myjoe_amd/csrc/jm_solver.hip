@@ -1569,6 +1569,346 @@ __global__ void __launch_bounds__(kBlock) k_jac(DMesh m, DParams P, const double
 }
 
 // ---------------------------------------------------------------------------
+// The same Jacobian assembly with one lane per cell-face entry (the flux
+// kernel's wave table: whole cells per wavefront). Each lane evaluates its
+// face's two blocks once: the off-diagonal one goes straight to A, the
+// diagonal contribution (signed: D -= J is D + (-J) exactly) to the lane's LDS
+// slot. The lanes of a cell then sum the diagonal block component-wise in the
+// cell's entry order (lane pos owns components pos, pos + nent, ...), which is
+// the reference's accumulation order, so the blocks are the same bits as
+// k_jac's. The segregated viscous blocks are a second pass over the same
+// slots (the reference adds them after every Euler block); the coupled loop
+// adds each face's viscous block right after its Euler block, so there the
+// slot also keeps the viscous block's eleven possibly nonzero entries.
+// Duplicate neighbour pairs (two faces between one pair of cells) need their
+// off-diagonal block accumulated in entry order and use k_jac.
+// ---------------------------------------------------------------------------
+// the entries of a coupled viscous block (visc_coupled_jac) that may be
+// nonzero: [1+i][0], [1+i][1+i] for i < 3 and row 4; -1 for the others
+__device__ __forceinline__ int vc_index(int k) {
+  switch (k) {
+    case 5: return 0;
+    case 6: return 1;
+    case 10: return 2;
+    case 12: return 3;
+    case 15: return 4;
+    case 18: return 5;
+    case 20: return 6;
+    case 21: return 7;
+    case 22: return 8;
+    case 23: return 9;
+    case 24: return 10;
+    default: return -1;
+  }
+}
+constexpr int kVcN = 11;
+constexpr int kMaxOwned = 7;  // components of the diagonal block per lane (cells have at least 4 faces)
+
+template <typename T, int SCHEME, bool VISC, int NB = 5>
+__device__ __forceinline__ void d_jac_e(const Blk &B, DMesh m, DParams P, const double *__restrict__ prim,
+                                        const double *__restrict__ bfa, T *__restrict__ A, int step_mode,
+                                        const double *__restrict__ dt, const double *__restrict__ rhs, double relax,
+                                        T *__restrict__ rhs_ls, T *__restrict__ phi,
+                                        const double *__restrict__ gradv) {
+  constexpr bool CPL = SCHEME == kHllcC;
+  constexpr bool CV = VISC && CPL;
+  constexpr int NS = CV ? 25 + kVcN : 25;
+  __shared__ double lds[kBlock * NS];
+  __shared__ signed char vsg[kBlock];  // the entry's viscous block: 0 none, +1 added, -1 subtracted
+  double *slot = lds + NS * threadIdx.x;
+  const EntryLane L = entry_lane(m, B);
+  const double gam = P.gamma;
+  const int c = L.c, pos = L.pos;
+  const int nent = L.e >= 0 ? m.cfl_i[c + 1] - m.cfl_i[c] : 0;
+  const double *pc = prim + 8 * (size_t)(L.e >= 0 ? c : 0);
+  Side own;
+  own.rho = pc[0]; own.u[0] = pc[1]; own.u[1] = pc[2]; own.u[2] = pc[3]; own.p = pc[4];
+  double J[5][5];
+  if (L.e >= 0) {
+    const int2 en = m.cfl[L.e];
+    const int f = en.x & 0x7fffffff;
+    const double *geo = m.fgeo + 8 * (size_t)f;
+    const double n[3] = {geo[0], geo[1], geo[2]};
+    const double area = geo[3];
+    signed char vs = 0;
+    if (en.y >= 0) {
+      const double *pn = prim + 8 * (size_t)en.y;
+      Side nb;
+      nb.rho = pn[0]; nb.u[0] = pn[1]; nb.u[1] = pn[2]; nb.u[2] = pn[3]; nb.p = pn[4];
+      const bool s0 = en.x >= 0;
+      const Side l = s0 ? own : nb, r = s0 ? nb : own;
+      double hl = 0.0, hr = 0.0;
+      if (CPL) {
+        hl = enthalpy_of(P, l.rho, l.p);
+        hr = enthalpy_of(P, r.rho, r.p);
+      }
+      // t = 0: the diagonal's block (icv0 adds Apl, which 0; icv1 subtracts
+      // Ami, which 1); t = 1: the off-diagonal block (icv0's A[noc01] = +Ami,
+      // icv1's A[noc10] = -Apl). One loop, so one copy of the Jacobian code.
+      T *off = A + NB * NB * (size_t)(m.cfl_noc[L.e] & 0x3fffffff);
+#pragma unroll 1
+      for (int t = 0; t < 2; t++) {
+        const int which = (t == 0) == s0 ? 0 : 1;
+        if (CPL) jac_hllc_coupled(J, which, l.rho, l.u, l.p, hl, r.rho, r.u, r.p, hr, gam, area, n);
+        else face_jac(SCHEME, J, which, l, r, gam, area, n);
+        if (t == 0) {
+          for (int i = 0; i < 5; i++)
+            for (int j = 0; j < 5; j++) slot[5 * i + j] = s0 ? J[i][j] : -J[i][j];
+        } else {
+          store_block<T, NB>(off, J, !s0, false);
+        }
+      }
+      if (CV) {  // the same face's viscous blocks next (JoeWithModels.cpp:5096-5104)
+        const VSide vo = vside_cell(P, pc, gradv + 12 * (size_t)c);
+        const VSide vn = vside_cell(P, pn, gradv + 12 * (size_t)en.y);
+        const VSide a = vsel(s0, vo, vn), b = vsel(s0, vn, vo);
+        const double xa[3] = {s0 ? pc[5] : pn[5], s0 ? pc[6] : pn[6], s0 ? pc[7] : pn[7]};
+        const double xb[3] = {s0 ? pn[5] : pc[5], s0 ? pn[6] : pc[6], s0 ? pn[7] : pc[7]};
+        const double xfa[3] = {geo[4], geo[5], geo[6]};
+        double sV[3], smag, mul, w0, w1;
+        visc_internal_geom(P, xa, xb, xfa, a.T, b.T, sV, smag, mul, w0, w1);
+        const double alpha = dot3(n, sV);
+        double mut = 0.0;
+        if (P.sa) {
+          const double nuo = P.sa_phi[c], nun = P.sa_phi[en.y];
+          mut = sa_mut_face(w0, w1, a.rho, b.rho, s0 ? nuo : nun, s0 ? nun : nuo, mul);
+        }
+        const double ufa[3] = {0.5 * (a.u[0] + b.u[0]), 0.5 * (a.u[1] + b.u[1]), 0.5 * (a.u[2] + b.u[2])};
+        double VF[5], tau[3], keff;
+        visc_coupled(P, a, b, mul, mut, ufa, area, n, smag, sV, alpha, VF, tau, keff, P.sa != 0);
+#pragma unroll 1
+        for (int t = 0; t < 2; t++) {
+          // t = 0: the diagonal's block from this cell's side; t = 1: the other side's, off the diagonal
+          visc_coupled_jac(P, J, (t == 0) == s0 ? 0 : 1, vsel((t == 0) == s0, a, b), mul + mut, keff, tau, ufa, area,
+                           alpha, smag);
+          if (t == 0) {
+            for (int k = 0; k < 25; k++) {
+              const int vi = vc_index(k);
+              if (vi >= 0) slot[25 + vi] = s0 ? J[k / 5][k % 5] : -J[k / 5][k % 5];
+            }
+          } else {
+            for (int i = 0; i < 5; i++)
+              for (int j = 0; j < 5; j++)
+                off[NB * i + j] =
+                    (T)(s0 ? (double)off[NB * i + j] + J[i][j] : (double)off[NB * i + j] - J[i][j]);
+          }
+        }
+        vs = s0 ? 1 : -1;
+      }
+    } else {
+      const ZoneDev &z = m.zones[-en.y - 1];
+      const double *bb = bfa + 8 * (size_t)f;
+      Side sb;
+      sb.rho = bb[0]; sb.u[0] = bb[1]; sb.u[1] = bb[2]; sb.u[2] = bb[3]; sb.p = bb[4];
+      if (CPL) {  // calcFluxCoupled boundary Jacobians (JoeWithModels.cpp:5150-5740)
+        const double hb = bb[6];
+        if (z.bc == kBcSymmetry || z.bc == kBcWall || z.bc == kBcNeumann) {
+          if (z.bc != kBcNeumann && P.symm_bc == kSymmAnalytical) {
+            jac_only_pressure(J, sb.rho, sb.u, sb.p, hb, gam, area, n);
+            for (int k = 0; k < 25; k++) slot[k] = J[k / 5][k % 5];
+          } else if (P.symm_bc == kSymmNew) {
+            double K[5][5];
+            jac_hllc_coupled(J, 0, sb.rho, sb.u, sb.p, hb, sb.rho, sb.u, sb.p, hb, gam, area, n);
+            jac_hllc_coupled(K, 1, sb.rho, sb.u, sb.p, hb, sb.rho, sb.u, sb.p, hb, gam, area, n);
+            for (int k = 0; k < 25; k++) slot[k] = J[k / 5][k % 5] + K[k / 5][k % 5];
+          } else {
+            jac_hllc_coupled(J, 0, sb.rho, sb.u, sb.p, hb, sb.rho, sb.u, sb.p, hb, gam, area, n);
+            for (int k = 0; k < 25; k++) slot[k] = J[k / 5][k % 5];
+          }
+        } else {  // cell-centre state against the boundary state
+          jac_hllc_coupled(J, 0, own.rho, own.u, own.p, enthalpy_of(P, own.rho, own.p), sb.rho, sb.u, sb.p, hb, gam,
+                           area, n);
+          for (int k = 0; k < 25; k++) slot[k] = J[k / 5][k % 5];
+        }
+        if (CV && z.bc != kBcSymmetry) {  // the face's viscous block (none on symmetry faces)
+          const double *gvo = gradv + 12 * (size_t)c;
+          const VSide vo = vside_cell(P, pc, gvo);
+          const VSide vb = vside_bfa(bb, gvo);
+          const double xfa[3] = {geo[4], geo[5], geo[6]};
+          double sArg[3], smag, alpha;
+          visc_coupled_bgeom(z.bc, xfa, pc + 5, n, sArg, smag, alpha);
+          const double mul = mu_lam(P, bb[5]);
+          const double mut = (P.sa && z.bc != kBcWall)
+                                 ? sa_mut(pc[0] * P.sa_phi[c], P.sa_aux[kAuxN * (size_t)c + kAuxMue])
+                                 : 0.0;
+          const VSide lv = vsel(z.bc == kBcNeumann, vb, vo);
+          double VF[5], tau[3], keff;
+          visc_coupled(P, lv, vb, mul, mut, vb.u, area, n, smag, sArg, alpha, VF, tau, keff, P.sa != 0);
+          visc_coupled_jac(P, J, 0, lv, mul + mut, keff, tau, vb.u, area, alpha, smag);
+          for (int k = 0; k < 25; k++) {
+            const int vi = vc_index(k);
+            if (vi >= 0) slot[25 + vi] = J[k / 5][k % 5];
+          }
+          vs = 1;
+        }
+      } else if (z.bc == kBcSymmetry || z.bc == kBcWall) {
+        if (z.bc == kBcWall || P.symm_bc == kSymmOriginal) {
+          face_jac(SCHEME, J, 0, sb, sb, gam, area, n);
+          for (int k = 0; k < 25; k++) slot[k] = J[k / 5][k % 5];
+        } else if (P.symm_bc == kSymmNew) {
+          double K[5][5];
+          face_jac(SCHEME, J, 0, sb, sb, gam, area, n);
+          face_jac(SCHEME, K, 1, sb, sb, gam, area, n);
+          for (int k = 0; k < 25; k++) slot[k] = J[k / 5][k % 5] + K[k / 5][k % 5];
+        } else {  // ANALYTICAL, JoeWithModels.cpp:3243-3269
+          const double a2 = gam - 1.0;
+          const double phi2 = 0.5 * a2 * (sb.u[0] * sb.u[0] + sb.u[1] * sb.u[1] + sb.u[2] * sb.u[2]);
+          for (int j = 0; j < 5; j++) {
+            slot[j] = 0.0;
+            slot[20 + j] = 0.0;
+          }
+          for (int i = 0; i < 3; i++) {
+            slot[5 * (i + 1)] = phi2 * n[i] * area;
+            for (int j = 0; j < 3; j++) slot[5 * (i + 1) + j + 1] = -a2 * sb.u[j] * n[i] * area;
+            slot[5 * (i + 1) + 4] = a2 * n[i] * area;
+          }
+        }
+      } else {
+        face_jac(SCHEME, J, 0, own, sb, gam, area, n);
+        for (int k = 0; k < 25; k++) slot[k] = J[k / 5][k % 5];
+      }
+    }
+    vsg[threadIdx.x] = vs;
+  }
+  __syncthreads();
+  // the cell's diagonal block, summed in entry order by the cell's lanes
+  double Dk[kMaxOwned];
+  const int first = threadIdx.x - pos;
+  if (L.e >= 0) {
+    int q = 0;
+    for (int k = pos; k < 25; k += nent, q++) {
+      double d = 0.0;
+      for (int j = 0; j < nent; j++) {
+        const double *sj = lds + NS * (first + j);
+        d += sj[k];
+        if (CV) {
+          const int sg = vsg[first + j];
+          if (sg != 0) {
+            const int vi = vc_index(k);
+            d += vi >= 0 ? sj[25 + vi] : (sg > 0 ? 0.0 : -0.0);
+          }
+        }
+      }
+      Dk[q] = d;
+    }
+  }
+  if (VISC && !CPL) {  // viscous blocks after every Euler block (JoeWithModels.cpp:3680-3697, 3789-3796, 3835-3842)
+    __syncthreads();
+    if (L.e >= 0) {
+      const int2 en = m.cfl[L.e];
+      const int f = en.x & 0x7fffffff;
+      const double *geo = m.fgeo + 8 * (size_t)f;
+      const double n[3] = {geo[0], geo[1], geo[2]};
+      const double area = geo[3];
+      const double xfa[3] = {geo[4], geo[5], geo[6]};
+      const VSide vo = vside_cell(P, pc, gradv + 12 * (size_t)c);
+      double Fu[3], FE;
+      VFace vf;
+      signed char vs = 0;
+      if (en.y >= 0) {
+        const double *pn = prim + 8 * (size_t)en.y;
+        const VSide vn = vside_cell(P, pn, gradv + 12 * (size_t)en.y);
+        const bool s0 = en.x >= 0;
+        const VSide &a = s0 ? vo : vn, &b = s0 ? vn : vo;
+        double sV[3], smag, mul, w0, w1;
+        visc_internal_geom(P, s0 ? pc + 5 : pn + 5, s0 ? pn + 5 : pc + 5, xfa, a.T, b.T, sV, smag, mul, w0, w1);
+        double mut = 0.0;
+        if (P.sa) {
+          const double nuo = P.sa_phi[c], nun = P.sa_phi[en.y];
+          mut = sa_mut_face(w0, w1, a.rho, b.rho, s0 ? nuo : nun, s0 ? nun : nuo, mul);
+        }
+        const double ufa[3] = {0.5 * (a.u[0] + b.u[0]), 0.5 * (a.u[1] + b.u[1]), 0.5 * (a.u[2] + b.u[2])};
+        visc_face(P, a, b, mul, mut, ufa, area, n, smag, sV, Fu, FE, vf);
+        T *off = A + NB * NB * (size_t)(m.cfl_noc[L.e] & 0x3fffffff);
+        // icv0: A[noc00] -= A0, A[noc01] -= A1; icv1: A[noc11] += A1, A[noc10] += A0
+        if (s0) visc_jac(P, J, 1.0, b, mul, mut, vf, ufa, area, smag);  // A1 -> off
+        else visc_jac(P, J, -1.0, a, mul, mut, vf, ufa, area, smag);    // A0 -> off
+        for (int i = 1; i < 5; i++)
+          for (int j = 0; j < 5; j++)
+            off[NB * i + j] = (T)(s0 ? (double)off[NB * i + j] - J[i][j] : (double)off[NB * i + j] + J[i][j]);
+        if (s0) visc_jac(P, J, -1.0, a, mul, mut, vf, ufa, area, smag);  // A0 -> diagonal, subtracted
+        else visc_jac(P, J, 1.0, b, mul, mut, vf, ufa, area, smag);     // A1 -> diagonal, added
+        for (int i = 1; i < 5; i++)
+          for (int j = 0; j < 5; j++) slot[5 * i + j] = s0 ? -J[i][j] : J[i][j];
+        vs = 1;
+      } else {
+        const ZoneDev &z = m.zones[-en.y - 1];
+        if (z.bc != kBcSymmetry) {
+          const double *bb = bfa + 8 * (size_t)f;
+          const double mul = mu_lam(P, bb[5]);
+          const VSide vb = vside_bfa(bb, gradv + 12 * (size_t)c);
+          double sV[3] = {xfa[0] - pc[5], xfa[1] - pc[6], xfa[2] - pc[7]};
+          double smag;
+          double sArg[3];
+          if (z.bc == kBcWall) {
+            smag = fabs(dot3(sV, n));
+            for (int i = 0; i < 3; i++) sArg[i] = n[i];
+          } else {
+            smag = sqrt(dot3(sV, sV));
+            for (int i = 0; i < 3; i++) sArg[i] = sV[i] / smag;
+          }
+          const double mut = (P.sa && z.bc != kBcWall)
+                                 ? sa_mut(pc[0] * P.sa_phi[c], P.sa_aux[kAuxN * (size_t)c + kAuxMue])
+                                 : 0.0;
+          visc_face(P, vo, vb, mul, mut, vb.u, area, n, smag, sArg, Fu, FE, vf);
+          visc_jac(P, J, -1.0, vo, mul, mut, vf, vb.u, area, smag);
+          for (int i = 1; i < 5; i++)
+            for (int j = 0; j < 5; j++) slot[5 * i + j] = -J[i][j];
+          vs = 1;
+        }
+      }
+      vsg[threadIdx.x] = vs;
+    }
+    __syncthreads();
+    if (L.e >= 0) {
+      int q = 0;
+      for (int k = pos; k < 25; k += nent, q++) {
+        if (k < 5) continue;  // rows 1-4 only
+        double d = Dk[q];
+        for (int j = 0; j < nent; j++)
+          if (vsg[first + j]) d += lds[NS * (first + j) + k];
+        Dk[q] = d;
+      }
+    }
+  }
+  if (L.e < 0) return;
+  // sourceHook (gravity), the time term, and the row's diagonal block
+  const double vol = m.vol[c];
+  const double tmp = step_mode ? vol / (dt[c]) : 0.0;
+  T *dgl = A + NB * NB * (size_t)m.nbocv_i[c];
+  int q = 0;
+  for (int k = pos; k < 25; k += nent, q++) {
+    const int i = k / 5, j = k % 5;
+    double d = Dk[q];
+    if (P.gravity_on) {
+      if (k == 5) d -= vol * P.g[0];
+      else if (k == 10) d -= vol * P.g[1];
+      else if (k == 15) d -= vol * P.g[2];
+      else if (k == 21) d -= vol * P.g[0];
+      else if (k == 22) d -= vol * P.g[1];
+      else if (k == 23) d -= vol * P.g[2];
+    }
+    if (step_mode && i == j) d += tmp;
+    dgl[NB * i + j] = (T)d;
+    if (NB == 6 && j == 4) dgl[NB * i + 5] = (T)0.0;
+  }
+  if (step_mode)
+    for (int k = pos; k < 5; k += nent) {
+      rhs_ls[NB * (size_t)c + k] = (T)(relax * rhs[(size_t)k * m.ncv + c]);
+      phi[NB * (size_t)c + k] = (T)0.0;
+    }
+}
+
+template <typename T, int SCHEME, bool VISC, int NB = 5>
+__global__ void __launch_bounds__(kBlock) k_jac_e(DMesh m, DParams P, const double *__restrict__ prim,
+                                                  const double *__restrict__ bfa, T *__restrict__ A, int step_mode,
+                                                  const double *__restrict__ dt, const double *__restrict__ rhs,
+                                                  double relax, T *__restrict__ rhs_ls, T *__restrict__ phi,
+                                                  const double *__restrict__ gradv) {
+  d_jac_e<T, SCHEME, VISC, NB>(hw_blk(), m, P, prim, bfa, A, step_mode, dt, rhs, relax, rhs_ls, phi, gradv);
+}
+
+// ---------------------------------------------------------------------------
 // Spalart-Allmaras per-cell fields after k_grad: the limited nuSA gradient
 // (calcCvScalarGrad with limiterNavierS and reference nuSA; JoeWithModels.cpp
 // :2736-2741, Scalars.cpp:134), the face-averaged laminar viscosity in faocv
@@ -3806,6 +4146,7 @@ void parse_params(const std::string &joe_in, const Mesh &m, SolverParams &p, std
   };
   p.grad_kernel = kform("GPU_GRAD_KERNEL", "CELL");
   p.flux_kernel = kform("GPU_FLUX_KERNEL", "ENTRY");
+  p.jac_kernel = kform("GPU_JAC_KERNEL", "ENTRY");
   p.gravity[0] = pm.dbl("GX", 0.0);
   p.gravity[1] = pm.dbl("GY", 0.0);
   p.gravity[2] = pm.dbl("GZ", 0.0);
@@ -4001,6 +4342,11 @@ struct Ctx::Impl {
   std::vector<int> fnew;              // reference face -> device face
   std::vector<int> h_dev_nbocv_i, h_ref_nbocv_i;
   bool dup_blocks = false;
+  int min_faces = 1 << 30;  // fewest faces of a cell
+  // the per-entry Jacobian kernel: needs no duplicate neighbour pairs (their
+  // shared block is accumulated in entry order by k_jac) and at least four
+  // faces per cell (its lanes own at most seven components of a block)
+  bool jac_entry() const { return par.jac_kernel == kResidEntry && !dup_blocks && min_faces >= 4; }
   // state
   DBuf<double> q, prim, grad, bfa, rhs, dt, hook;
   DBuf<double> gradv;  // [ncv][12] velocity and enthalpy gradients of the viscous flux
@@ -4206,6 +4552,7 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
       cf.push_back(make_int2(f, -(bz[f] + 1)));
       slot_of.push_back(0);
     }
+    d.min_faces = std::min(d.min_faces, (int)cf.size() - ci[c]);
   }
   ci[ncv] = (int)cf.size();
 
@@ -5345,9 +5692,27 @@ void Ctx::Impl::launch_jac(bool step_mode) {
     hipLaunchKernelGGL(kern, dim3(nblk(ncv)), dim3(kBlock), 0, stream, dm, dp, prim.p, bfa.p, A, step_mode ? 1 : 0,
                        dt.p, rhs.p, par.under_relax, v[0].p, v[1].p, gradv.p);
   };
+  const bool ent = jac_entry();
+  auto goe = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3((nwave + kWaves - 1) / kWaves), dim3(kBlock), 0, stream, dm, dp, prim.p, bfa.p, A,
+                       step_mode ? 1 : 0, dt.p, rhs.p, par.under_relax, v[0].p, v[1].p, gradv.p);
+  };
   if (coupled_sa()) {  // 6x6 blocks: the flow rows, then the scalar row (SA needs MU_REF > 0)
-    go(k_jac<T, kHllcC, true, 6>);
+    if (ent) goe(k_jac_e<T, kHllcC, true, 6>);
+    else go(k_jac<T, kHllcC, true, 6>);
     launch_scal_c<T>(A, step_mode);
+    return;
+  }
+  if (ent) {
+    if (dp.visc) {
+      if (par.scheme == kRoe) goe(k_jac_e<T, kRoe, true>);
+      else if (par.scheme == kHllcC) goe(k_jac_e<T, kHllcC, true>);
+      else goe(k_jac_e<T, kHllc, true>);
+    } else {
+      if (par.scheme == kRoe) goe(k_jac_e<T, kRoe, false>);
+      else if (par.scheme == kHllcC) goe(k_jac_e<T, kHllcC, false>);
+      else goe(k_jac_e<T, kHllc, false>);
+    }
     return;
   }
   if (dp.visc) {
