@@ -225,16 +225,19 @@ def streams_capacity(built, budget, max_active):
 
 
 def serving_bench(n, seed, min_cells, max_cells, max_active, nsteps, resid_rel, budget_gib,
-                  legs=("plain", "streams", "batched")):
+                  legs=("plain", "streams", "batched"), plain_every=4, cfl_max=20.0):
     """Steady solves/min and latency of the served mixed-precision solver
     against a plain FP64 one-at-a-time loop, on steady bump-channel requests
     of random size, bump thickness and Mach (cases.serving_requests), each
     solved until its energy residual has dropped by resid_rel. Capacity: all
     requests present at t = 0. Latency: Poisson arrivals at 0.8 x the served
     capacity, replayed in real time for the server; for the one-at-a-time
-    loop the same arrivals are queued in front of its measured solve times.
-    The served solver is the batched server (lockstep steps, each kernel of
-    the step and of the BiCGStab launched once for all active solves); the
+    loop the same arrivals are queued in front of its solve times. The plain
+    loop solves every plain_every-th request (its rate is one over the mean
+    solve time, which a sample of the same random sizes estimates); in its
+    queue each measured solve time stands for the plain_every requests from
+    it. The served solver is the batched server (lockstep steps, each kernel
+    of the step and of the BiCGStab launched once for all active solves); the
     per-stream graph server's capacity is reported beside it. legs names the
     parts to run (plain loop, per-stream graph server, batched server)."""
     import numpy as np
@@ -245,13 +248,14 @@ def serving_bench(n, seed, min_cells, max_cells, max_active, nsteps, resid_rel, 
     built = []
     for r in reqs:
         mm, joe = cases.bump_case(r["nx"], r["ny"], nsteps=nsteps, thickness=r["thickness"], mach=r["mach"],
-                                  resid_rel=resid_rel)
+                                  resid_rel=resid_rel, cfl_max=cfl_max)
         built.append((joemi.Mesh.from_faces(mm), joe))
         del mm
     cells = sum(m.ncv for m, _ in built)
     # plain loop: FP64, one solve at a time, host-polled steps
     service, steps_plain, plain_states = [], [], []
-    for mesh, joe in (built if "plain" in legs else []):
+    sampled = list(range(0, n, plain_every))
+    for mesh, joe in ([built[k] for k in sampled] if "plain" in legs else []):
         t0 = time.perf_counter()
         s = joemi.Solver(mesh, joe, joemi.JM_FP64)
         s.init_uniform()
@@ -261,14 +265,14 @@ def serving_bench(n, seed, min_cells, max_cells, max_active, nsteps, resid_rel, 
         steps_plain.append(k)
         plain_states.append(st)
         del s
-    cap_plain = n / sum(service) * 60.0 if service else None
+    cap_plain = len(service) / sum(service) * 60.0 if service else None
     if service:
-        print(f"# serving: plain FP64 loop {cap_plain:.1f} solves/min ({cells} cells in {n} solves)",
-              file=sys.stderr, flush=True)
+        print(f"# serving: plain FP64 loop {cap_plain:.1f} solves/min ({len(service)} of the {n} requests, "
+              f"{cells} cells in all)", file=sys.stderr, flush=True)
     budget = int(budget_gib * (1 << 30))
     res = {"workload": f"{n} steady inviscid bump-channel solves, {min_cells}-{max_cells} cells (nx = 3 ny, one "
-                       f"cell thick), bump 6-12 %, inlet Mach 0.3-0.5, HLLC 2nd order; stop at energy L1 drop "
-                       f"{resid_rel:g} or {nsteps} steps", "cells_total": int(cells)}
+                       f"cell thick), bump 6-12 %, inlet Mach 0.3-0.5, HLLC 2nd order, CFL ramp to {cfl_max:g}; "
+                       f"stop at energy L1 drop {resid_rel:g} or {nsteps} steps", "cells_total": int(cells)}
     if "streams" in legs:
         res["served_per_stream_graphs"] = {"solves_per_min": round(streams_capacity(built, budget, max_active), 2),
                                            "max_active": max_active}
@@ -288,7 +292,8 @@ def serving_bench(n, seed, min_cells, max_cells, max_active, nsteps, resid_rel, 
         st, rep = srv.result(t)
         steps_served.append(rep.steps)
         drops.append(rep.resid[4] / rep.resid0[4] if rep.resid0[4] > 0 else float("nan"))
-        for name, a, b in zip(("rho", "rhou", "rhoE"), st, plain_states[k] if plain_states else ()):
+        ref = plain_states[sampled.index(k)] if (plain_states and k in sampled) else ()
+        for name, a, b in zip(("rho", "rhou", "rhoE"), st, ref):
             d = float(np.max(np.abs(a - b))) / max(float(np.max(np.abs(b))), 1e-300)
             err[name] = d if not (d <= err[name]) else err[name]  # a NaN is kept, not hidden
         srv.release(t)
@@ -319,17 +324,21 @@ def serving_bench(n, seed, min_cells, max_cells, max_active, nsteps, resid_rel, 
                      "max_active": max_active, "p50_s": round(q(lat, 50), 3), "p99_s": round(q(lat, 99), 3),
                      "mean_steps": round(float(np.mean(steps_served)), 1)}
     res["arrival_rate_per_min"] = round(rate * 60.0, 2)
-    res["served_energy_resid_drop_max"] = float(f"{max(drops):.3g}")
+    # np.max keeps a NaN (a solve that diverged) instead of hiding it
+    res["served_energy_resid_drop_max"] = float(f"{float(np.max(np.array(drops))):.3g}")
+    res["served_nonfinite"] = int(np.sum(~np.isfinite(np.array(drops))))
     if service:
         # the one-at-a-time loop meets the same arrivals, queued in front of its measured solve times
         done_prev, lat_plain = 0.0, []
-        for a, sv in zip(arr, service):
+        service_all = [service[k // plain_every] for k in range(n)]
+        for a, sv in zip(arr, service_all):
             start = max(a, done_prev)
             done_prev = start + sv
             lat_plain.append(done_prev - a)
         res["plain_fp64_loop"] = {"solves_per_min": round(cap_plain, 2), "p50_s": round(q(lat_plain, 50), 3),
                                   "p99_s": round(q(lat_plain, 99), 3),
-                                  "mean_steps": round(float(np.mean(steps_plain)), 1)}
+                                  "mean_steps": round(float(np.mean(steps_plain)), 1),
+                                  "solves_timed": len(service)}
         res["speedup_solves_per_min"] = round(cap_served / cap_plain, 3)
         res["served_vs_fp64_max_rel_diff"] = {k: float(f"{v:.3g}") for k, v in err.items()}
     return res
@@ -348,11 +357,14 @@ def main():
     ap.add_argument("--jac-kernel", default=None, help="GPU_JAC_KERNEL: ENTRY or CELL")
     ap.add_argument("--cell-order", default=None,
                     help="device cell numbering (GPU_CELL_ORDER): NATIVE, GROUPED, MORTON, MORTON_GROUPED")
-    ap.add_argument("--serve", type=int, default=16, help="serving requests (0: skip the serving leg)")
+    ap.add_argument("--serve", type=int, default=64, help="serving requests (0: skip the serving leg)")
+    ap.add_argument("--serve-plain-every", type=int, default=4, help="the plain FP64 loop solves every k-th request")
     ap.add_argument("--serve-cells", default="3000,12000", help="min,max cells of a served request")
     ap.add_argument("--serve-active", type=int, default=16)
-    ap.add_argument("--serve-steps", type=int, default=3000)
-    ap.add_argument("--serve-rel", type=float, default=1e-4)
+    ap.add_argument("--serve-steps", type=int, default=4000)
+    ap.add_argument("--serve-cfl", type=float, default=20.0,
+                    help="MAX_CFL of the served requests (at 50, 6 of 16 sampled requests diverged)")
+    ap.add_argument("--serve-rel", type=float, default=1e-5)
     ap.add_argument("--serve-legs", default="plain,batched",
                     help="serving parts to run: plain (FP64 loop), streams (per-stream graph server), batched")
     ap.add_argument("--plate", type=int, default=1, help="time the 2M-cell flat-plate step (0: skip)")
@@ -485,7 +497,8 @@ def main():
     if args.serve > 0:
         lo, hi = (int(x) for x in args.serve_cells.split(","))
         out["serving"] = serving_bench(args.serve, 7, lo, hi, args.serve_active, args.serve_steps, args.serve_rel, 64.0,
-                                       legs=tuple(args.serve_legs.split(",")))
+                                       legs=tuple(args.serve_legs.split(",")), plain_every=args.serve_plain_every,
+                                       cfl_max=args.serve_cfl)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

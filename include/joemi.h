@@ -329,6 +329,8 @@ void jm_server_destroy(jm_server *s);
 jm_status jm_server_submit(jm_server *s, const jm_mesh *m, const char *joe_in, int precision, const double *rho,
                            const double *rhou, const double *rhoE, int *ticket);
 jm_status jm_server_status(jm_server *s, int ticket, int *status);
+/* pseudo-time steps a submitted solve has taken so far (0 while queued) */
+jm_status jm_server_progress(jm_server *s, int ticket, int *steps);
 jm_status jm_server_wait(jm_server *s, int ticket);
 /* final state (any pointer may be NULL) and report of a finished solve */
 jm_status jm_server_result(jm_server *s, int ticket, double *rho, double *rhou, double *rhoE, jm_solve_report *rep);

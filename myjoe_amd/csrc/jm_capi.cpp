@@ -561,6 +561,14 @@ jm_status jm_server_status(jm_server *s, int ticket, int *status) {
   });
 }
 
+jm_status jm_server_progress(jm_server *s, int ticket, int *steps) {
+  return guard([&] {
+    if (!s || !steps) throw std::exception();
+    *steps = s->s->progress(ticket);
+    return JM_OK;
+  });
+}
+
 jm_status jm_server_wait(jm_server *s, int ticket) {
   return guard([&] {
     if (!s) throw std::exception();
@@ -575,7 +583,7 @@ jm_status jm_server_result(jm_server *s, int ticket, double *rho, double *rhou, 
     auto r = s->s->finished(ticket);
     if (rep) {
       rep->status = r->status;
-      rep->steps = r->steps;
+      rep->steps = r->steps.load();
       for (int k = 0; k < 5; k++) {
         rep->resid[k] = r->last.resid[k];
         rep->resid0[k] = r->resid0[k];

@@ -130,6 +130,17 @@ int Server::status(int id) {
   return it->second->status;
 }
 
+int Server::progress(int id) {
+  std::shared_ptr<Request> r;
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    auto it = reqs_.find(id);
+    if (it == reqs_.end()) throw std::invalid_argument("unknown ticket");
+    r = it->second;
+  }
+  return r->steps.load();
+}
+
 void Server::wait(int id) {
   std::unique_lock<std::mutex> g(mu_);
   auto it = reqs_.find(id);
@@ -275,7 +286,7 @@ void Server::step_loop() {
         a.inflight = false;
         progress = true;
         auto &r = *a.req;
-        r.steps = s.step;
+        r.steps.store(s.step);
         r.last = s;
         if (s.step == 1)
           for (int k = 0; k < 5; k++) r.resid0[k] = s.resid[k];
@@ -378,7 +389,7 @@ void Server::batch_loop(int group) {
       try {
         StepInfo s = a.ctx->batch_result();
         auto &r = *a.req;
-        r.steps = s.step;
+        r.steps.store(s.step);
         r.last = s;
         if (s.step == 1)
           for (int k = 0; k < 5; k++) r.resid0[k] = s.resid[k];

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <condition_variable>
 #include <deque>
 #include <map>
@@ -31,7 +32,7 @@ class Server {
     std::vector<double> init;  // optional initial state (rho, rhou[3], rhoE), reference numbering
     int status = kQueued;
     double t_submit = 0, t_start = 0, t_done = 0;
-    int steps = 0;
+    std::atomic<int> steps{0};  // written by a stepping thread, read by progress()
     StepInfo last;
     double resid0[5] = {0, 0, 0, 0, 0};
     std::vector<double> rho, rhou, rhoE;
@@ -49,6 +50,8 @@ class Server {
   int submit(const Mesh *mesh, const std::string &joe_in, int precision, const double *rho, const double *rhou,
              const double *rhoE);
   int status(int id);
+  // pseudo-time steps the solve has taken so far
+  int progress(int id);
   void wait(int id);
   std::shared_ptr<Request> finished(int id);
   void release(int id);

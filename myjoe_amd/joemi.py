@@ -92,6 +92,7 @@ def lib():
         L.jm_server_destroy.restype = None
         L.jm_server_submit.argtypes = [vp, vp, C.c_char_p, C.c_int, vp, vp, vp, vp]
         L.jm_server_status.argtypes = [vp, C.c_int, vp]
+        L.jm_server_progress.argtypes = [vp, C.c_int, vp]
         L.jm_server_wait.argtypes = [vp, C.c_int]
         L.jm_server_result.argtypes = [vp, C.c_int, vp, vp, vp, vp]
         L.jm_server_release.argtypes = [vp, C.c_int]
@@ -480,6 +481,12 @@ class Server:
         st = C.c_int()
         _check(lib().jm_server_status(self.h, ticket, C.byref(st)))
         return st.value
+
+    def progress(self, ticket: int) -> int:
+        """Pseudo-time steps the solve has taken so far."""
+        n = C.c_int()
+        _check(lib().jm_server_progress(self.h, ticket, C.byref(n)))
+        return n.value
 
     def wait(self, ticket: int):
         _check(lib().jm_server_wait(self.h, ticket))

@@ -171,3 +171,71 @@ def test_batched_rejects_what_it_does_not_run():
     srv.wait(t)
     assert srv.result(t)[1].status == joemi.Server.DONE
     srv.close()
+
+
+def test_batched_request_joining_a_running_group_matches_its_alone_result():
+    # one lockstep group: a long solve is stepping before the second request
+    # is submitted (its progress is polled until it has taken 10 steps), so
+    # the second joins a group already stepping; both results are still the
+    # same bits as each solve alone
+    import os
+
+    print("serving: batched join", flush=True)
+    keep = os.environ.get("JOEMI_SERVER_GROUPS")
+    os.environ["JOEMI_SERVER_GROUPS"] = "1"
+    try:
+        mm_a, joe_a = _case(REQS[1], nsteps=120)
+        mm_b, joe_b = _case(REQS[0], nsteps=30)
+        ma, mb = joemi.Mesh.from_faces(mm_a), joemi.Mesh.from_faces(mm_b)
+        srv = joemi.Server(mem_budget_bytes=8 << 30, max_active=2, batched=True)
+        ta = srv.submit(ma, joe_a, joemi.JM_MIXED)
+        import time
+
+        t0 = time.time()
+        while srv.progress(ta) < 10:
+            assert srv.status(ta) <= joemi.Server.RUNNING and time.time() - t0 < 60
+            time.sleep(0.001)
+        joined_at = srv.progress(ta)
+        tb = srv.submit(mb, joe_b, joemi.JM_FP64)
+        srv.wait(tb)
+        srv.wait(ta)
+        (sa, ra), (sb, rb) = srv.result(ta), srv.result(tb)
+        srv.release(ta)
+        srv.release(tb)
+        srv.close()
+    finally:
+        if keep is None:
+            del os.environ["JOEMI_SERVER_GROUPS"]
+        else:
+            os.environ["JOEMI_SERVER_GROUPS"] = keep
+    assert 10 <= joined_at < 120 and ra.steps == 120 and rb.steps == 30
+    for (st, rep_), (m, joe, prec) in (((sa, ra), (ma, joe_a, joemi.JM_MIXED)), ((sb, rb), (mb, joe_b, joemi.JM_FP64))):
+        ((alone, arep),) = _served([(m, joe, prec)], True, 1)
+        for a, c in zip(st, alone):
+            assert np.array_equal(a, c)
+        assert list(rep_.resid) == list(arep.resid)
+
+
+def test_served_mixed_solve_reaches_the_oracle_steady_state():
+    # a solve served by the batched server in the mixed form (FP32 Jacobian,
+    # preconditioner and Krylov vectors; FP64 residual) converges to the FP64
+    # oracle's steady state of the 24x8 bump to 1e-8 of each variable's
+    # largest magnitude: batching and precision change the path, not the
+    # fixed point
+    from oracle.oracle import Oracle
+
+    print("serving: mixed steady state", flush=True)
+    mm, joe = cases.bump_case(24, 8, nsteps=1200)
+    mesh = joemi.Mesh.from_faces(mm)
+    o = Oracle(mesh.as_dump(), joe)
+    u = cases.U_BUMP
+    o.set_state(np.ones(mesh.ncv), np.tile([u, 0.0, 0.0], (mesh.ncv, 1)),
+                np.full(mesh.ncv, 1.0 / 0.4 + 0.5 * u * u))
+    n, hist = o.run(1200)
+    assert hist[-1, 4] < 1e-7 * hist[0, 4]
+    ((served, rep),) = _served([(mesh, joe, joemi.JM_MIXED)], True, 1)
+    assert rep.steps == 1200
+    for a, b, k in zip(served, o.state(), ("rho", "rhou", "rhoE")):
+        err = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+        print("served mixed vs oracle", k, f"{err:.1e}", flush=True)
+        assert err <= 1e-8, (k, err)
