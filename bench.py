@@ -17,6 +17,7 @@ from __future__ import annotations
 import argparse
 import gc
 import json
+import math
 import os
 import sys
 import time
@@ -224,6 +225,17 @@ def streams_capacity(built, budget, max_active):
     return cap
 
 
+def _strict_json(v):
+    """The record with every non-finite float as null (JSON has no NaN)."""
+    if isinstance(v, dict):
+        return {k: _strict_json(x) for k, x in v.items()}
+    if isinstance(v, (list, tuple)):
+        return [_strict_json(x) for x in v]
+    if isinstance(v, float) and not math.isfinite(v):
+        return None
+    return v
+
+
 def serving_bench(n, seed, min_cells, max_cells, max_active, nsteps, resid_rel, budget_gib,
                   legs=("plain", "streams", "batched"), plain_every=4, cfl_max=20.0):
     """Steady solves/min and latency of the served mixed-precision solver
@@ -288,9 +300,12 @@ def serving_bench(n, seed, min_cells, max_cells, max_active, nsteps, resid_rel, 
     makespan = time.perf_counter() - t0
     cap_served = n / makespan * 60.0
     steps_served, err, drops = [], {"rho": 0.0, "rhou": 0.0, "rhoE": 0.0}, []
+    retries, abandoned = 0, 0
     for k, t in enumerate(tickets):
         st, rep = srv.result(t)
         steps_served.append(rep.steps)
+        retries += rep.fp64_retries
+        abandoned += rep.steps_abandoned
         drops.append(rep.resid[4] / rep.resid0[4] if rep.resid0[4] > 0 else float("nan"))
         ref = plain_states[sampled.index(k)] if (plain_states and k in sampled) else ()
         for name, a, b in zip(("rho", "rhou", "rhoE"), st, ref):
@@ -324,9 +339,14 @@ def serving_bench(n, seed, min_cells, max_cells, max_active, nsteps, resid_rel, 
                      "max_active": max_active, "p50_s": round(q(lat, 50), 3), "p99_s": round(q(lat, 99), 3),
                      "mean_steps": round(float(np.mean(steps_served)), 1)}
     res["arrival_rate_per_min"] = round(rate * 60.0, 2)
-    # np.max keeps a NaN (a solve that diverged) instead of hiding it
-    res["served_energy_resid_drop_max"] = float(f"{float(np.max(np.array(drops))):.3g}")
+    # np.max keeps a NaN (a solve that diverged), reported as null
+    dmax = float(np.max(np.array(drops)))
+    res["served_energy_resid_drop_max"] = float(f"{dmax:.3g}") if np.isfinite(dmax) else None
     res["served_nonfinite"] = int(np.sum(~np.isfinite(np.array(drops))))
+    # mixed solves that went non-finite and were run again in FP64 (the
+    # server's retry), and the steps their abandoned attempts took
+    res["served_fp64_retries"] = int(retries)
+    res["served_steps_abandoned"] = int(abandoned)
     if service:
         # the one-at-a-time loop meets the same arrivals, queued in front of its measured solve times
         done_prev, lat_plain = 0.0, []
@@ -340,7 +360,8 @@ def serving_bench(n, seed, min_cells, max_cells, max_active, nsteps, resid_rel, 
                                   "mean_steps": round(float(np.mean(steps_plain)), 1),
                                   "solves_timed": len(service)}
         res["speedup_solves_per_min"] = round(cap_served / cap_plain, 3)
-        res["served_vs_fp64_max_rel_diff"] = {k: float(f"{v:.3g}") for k, v in err.items()}
+        res["served_vs_fp64_max_rel_diff"] = {k: (float(f"{v:.3g}") if np.isfinite(v) else None)
+                                              for k, v in err.items()}
     return res
 
 
@@ -501,7 +522,7 @@ def main():
                                        cfl_max=args.serve_cfl)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
-    print(json.dumps(out), flush=True)
+    print(json.dumps(_strict_json(out), allow_nan=False), flush=True)
 
 
 if __name__ == "__main__":

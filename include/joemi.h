@@ -312,7 +312,11 @@ typedef struct {
   double resid[5];    /* L1 residuals of the last step */
   double resid0[5];   /* L1 residuals of the first step */
   double queued_s;    /* submission to admission, seconds */
-  double run_s;       /* admission (context build included) to retirement */
+  double run_s;       /* admission (context build included) to retirement; a retried
+                         solve's covers both attempts */
+  int fp64_retries;   /* 1 if the solve was mixed precision, went non-finite and
+                         was run again in FP64 (steps and resid are then the FP64 run's) */
+  int steps_abandoned; /* steps of that abandoned mixed-precision attempt */
 } jm_solve_report;
 /* flags: JM_SERVER_BATCHED steps the active solves in lockstep (2 groups, each
  * on its own stream; JOEMI_SERVER_GROUPS = 1..4 in the environment changes it),
@@ -321,8 +325,14 @@ typedef struct {
  * jm_step_graph's captured step. Either way a solve's result is the same bits
  * as its own jm_step loop's. Batched serving runs the per-entry flux kernel
  * (GPU_FLUX_KERNEL = ENTRY) and no SA scalar; other requests fail with
- * JM_ERR_UNSUPPORTED. */
+ * JM_ERR_UNSUPPORTED.
+ * A mixed-precision solve whose residual turns non-finite (its FP32 Krylov
+ * solve stopped at the iteration limit on a far-off iterate) is queued again
+ * once, at the front, in FP64 from its initial state, when the FP64 footprint
+ * fits the budget; the report's fp64_retries says so. JM_SERVER_NO_FP64_RETRY
+ * retires it with its non-finite state instead. */
 #define JM_SERVER_BATCHED 1
+#define JM_SERVER_NO_FP64_RETRY 2
 jm_status jm_server_create(int64_t mem_budget_bytes, int max_active, int flags, jm_server **out);
 void jm_server_destroy(jm_server *s);
 /* initial state may be NULL (uniform, as jm_init_uniform) */

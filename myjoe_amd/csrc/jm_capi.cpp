@@ -533,9 +533,11 @@ jm_status jm_ctx_footprint(const jm_mesh *m, int precision, int64_t *bytes) {
 
 jm_status jm_server_create(int64_t mem_budget_bytes, int max_active, int flags, jm_server **out) {
   return guard([&] {
-    if (!out || mem_budget_bytes <= 0 || max_active < 1 || (flags & ~JM_SERVER_BATCHED)) throw std::exception();
+    if (!out || mem_budget_bytes <= 0 || max_active < 1 || (flags & ~(JM_SERVER_BATCHED | JM_SERVER_NO_FP64_RETRY)))
+      throw std::exception();
     std::unique_ptr<jm_server> s(new jm_server);
-    s->s.reset(new jm::Server((size_t)mem_budget_bytes, max_active, (flags & JM_SERVER_BATCHED) != 0));
+    s->s.reset(new jm::Server((size_t)mem_budget_bytes, max_active, (flags & JM_SERVER_BATCHED) != 0,
+                              (flags & JM_SERVER_NO_FP64_RETRY) == 0));
     *out = s.release();
     return JM_OK;
   });
@@ -590,6 +592,8 @@ jm_status jm_server_result(jm_server *s, int ticket, double *rho, double *rhou, 
       }
       rep->queued_s = r->t_start > 0 ? r->t_start - r->t_submit : r->t_done - r->t_submit;
       rep->run_s = r->t_start > 0 ? r->t_done - r->t_start : 0.0;
+      rep->fp64_retries = r->fp64_retries;
+      rep->steps_abandoned = r->steps_abandoned;
     }
     if (r->status != jm::Server::kDone) throw std::runtime_error("solve failed: " + r->error);
     const size_t n = r->rho.size();

@@ -19,12 +19,20 @@
 // refresh, again one launch per kernel (StepBatch::post), and one
 // synchronisation. The tables are rebuilt only when a solve joins or
 // leaves. A solve's result is the same bits as its ordinary step's.
+//
+// A mixed-precision solve (FP32 Jacobian and Krylov vectors) can go
+// non-finite where the FP64 one does not: a linear solve that ends at its
+// iteration limit keeps its last iterate (as the reference's BiCGStab does),
+// and in FP32 that iterate has been far off often enough to push a transonic
+// solve to negative pressure. Such a solve is queued again once, at the
+// front, in FP64 from its initial state; its report says so.
 #include "jm_server.hpp"
 #include "jm_param.hpp"
 
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <stdexcept>
@@ -38,8 +46,8 @@ double now_s() {
 }
 }  // namespace
 
-Server::Server(size_t mem_budget, int max_active, bool batched)
-    : budget_(mem_budget), max_active_(max_active), batched_(batched) {
+Server::Server(size_t mem_budget, int max_active, bool batched, bool fp64_retry)
+    : budget_(mem_budget), max_active_(max_active), batched_(batched), fp64_retry_(fp64_retry) {
   if (max_active < 1) throw std::invalid_argument("max_active must be at least 1");
   if (hipGetDevice(&device_) != hipSuccess) throw std::runtime_error("no HIP device available");
   if (batched_) {
@@ -197,10 +205,12 @@ void Server::admit_loop() {
       } else {
         const size_t n = r->mesh->ncv;
         a.ctx->set_state(r->init.data(), r->init.data() + n, r->init.data() + 4 * n);
-        r->init.clear();
-        r->init.shrink_to_fit();
+        if (r->precision == 0 || !fp64_retry_) {  // a mixed solve may start again from it
+          r->init.clear();
+          r->init.shrink_to_fit();
+        }
       }
-      r->t_start = now_s();
+      if (r->t_start == 0) r->t_start = now_s();  // a retry's latency counts from the first admission
       std::lock_guard<std::mutex> g(mu_);
       ready_.push_back(std::move(a));
     } catch (std::exception &e) {
@@ -233,18 +243,46 @@ void Server::retire(Active &a, bool failed, const std::string &err) {
   } else {
     r->error = err;
   }
-  {
-    std::lock_guard<std::mutex> hg(hip_mu_);
-    a.ctx.reset();
-    if (a.stream) (void)hipStreamDestroy(a.stream);
-    a.stream = nullptr;
-  }
+  release_ctx(a);
   std::lock_guard<std::mutex> g(mu_);
   r->t_done = now_s();
   r->status = failed ? kFailed : kDone;
+  r->init.clear();
+  r->init.shrink_to_fit();
   slots_used_--;
   used_ -= r->footprint;
   cv_.notify_all();
+}
+
+void Server::release_ctx(Active &a) {
+  std::lock_guard<std::mutex> hg(hip_mu_);
+  a.ctx.reset();
+  if (a.stream) (void)hipStreamDestroy(a.stream);
+  a.stream = nullptr;
+}
+
+bool Server::retry_in_fp64(Active &a, const StepInfo &s) {
+  auto r = a.req;
+  if (!fp64_retry_ || r->precision == 0 || r->fp64_retries > 0) return false;
+  bool finite = true;
+  for (int k = 0; k < 5; k++) finite = finite && std::isfinite(s.resid[k]);
+  if (finite) return false;
+  const size_t fp = Ctx::footprint(*r->mesh, 0);
+  if (fp > budget_) return false;  // the FP64 solve could never be admitted
+  release_ctx(a);
+  std::lock_guard<std::mutex> g(mu_);
+  slots_used_--;
+  used_ -= r->footprint;
+  r->precision = 0;
+  r->footprint = fp;
+  r->fp64_retries = 1;
+  r->steps_abandoned = s.step;
+  r->steps.store(0);
+  for (int k = 0; k < 5; k++) r->resid0[k] = 0.0;
+  r->status = kQueued;
+  queue_.push_front(r->id);
+  cv_.notify_all();
+  return true;
 }
 
 void Server::step_loop() {
@@ -292,7 +330,7 @@ void Server::step_loop() {
           for (int k = 0; k < 5; k++) r.resid0[k] = s.resid[k];
         if (a.ctx->done_solver(s)) {
           a.dead = true;
-          retire(a, false, "");
+          if (!retry_in_fp64(a, s)) retire(a, false, "");
         }
       } catch (std::exception &e) {
         a.dead = true;
@@ -395,14 +433,18 @@ void Server::batch_loop(int group) {
           for (int k = 0; k < 5; k++) r.resid0[k] = s.resid[k];
         if (a.ctx->done_solver(s)) {
           a.dead = true;
-          retire(a, false, "");
+          if (!retry_in_fp64(a, s)) retire(a, false, "");
         }
       } catch (std::exception &e) {
         fail(a, e.what());
       }
     }
+    const size_t before = active_.size();
     active_.erase(std::remove_if(active_.begin(), active_.end(), [](const Active &a) { return a.dead; }),
                   active_.end());
+    // a solve queued again in FP64 comes back under the same id with a new
+    // context: the tables must be rebuilt even if the id list matches again
+    if (active_.size() != before) members.clear();
   }
   (void)hipStreamSynchronize(bs);
 }

@@ -31,6 +31,8 @@ class Server {
     size_t footprint = 0;
     std::vector<double> init;  // optional initial state (rho, rhou[3], rhoE), reference numbering
     int status = kQueued;
+    int fp64_retries = 0;     // 1 once a mixed-precision solve that went non-finite was queued again in FP64
+    int steps_abandoned = 0;  // steps of that abandoned mixed-precision attempt
     double t_submit = 0, t_start = 0, t_done = 0;
     std::atomic<int> steps{0};  // written by a stepping thread, read by progress()
     StepInfo last;
@@ -42,7 +44,9 @@ class Server {
   // batched: every active solve steps in lockstep on one stream, their
   // linear solves in one launch (one workgroup per solve); otherwise each
   // solve steps on its own stream with the graph-captured device step
-  Server(size_t mem_budget, int max_active, bool batched = false);
+  // fp64_retry: a mixed-precision solve whose residual turns non-finite is
+  // queued again once, at the front, in FP64 from its initial state
+  Server(size_t mem_budget, int max_active, bool batched = false, bool fp64_retry = true);
   ~Server();
   Server(const Server &) = delete;
   Server &operator=(const Server &) = delete;
@@ -69,10 +73,15 @@ class Server {
   void step_loop();
   void batch_loop(int group);
   void retire(Active &a, bool failed, const std::string &err);
+  // the fp64_retry path for a solve that has stopped on step s: true when the
+  // solve was queued again (a's context and slot are then released)
+  bool retry_in_fp64(Active &a, const StepInfo &s);
+  void release_ctx(Active &a);
 
   size_t budget_;
   int max_active_;
   bool batched_ = false;
+  bool fp64_retry_ = true;
   // batched mode: ngroups_ lockstep groups (JOEMI_SERVER_GROUPS, 1 to
   // kMaxGroups), each with its own stepping thread, stream, solver jobs
   // (pinned) and batched-BiCGStab buffers, so one group's kernels fill the

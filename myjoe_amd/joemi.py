@@ -29,7 +29,8 @@ class JoemiError(RuntimeError):
 
 class SolveReport(C.Structure):
     _fields_ = [("status", C.c_int), ("steps", C.c_int), ("resid", C.c_double * 5), ("resid0", C.c_double * 5),
-                ("queued_s", C.c_double), ("run_s", C.c_double)]
+                ("queued_s", C.c_double), ("run_s", C.c_double), ("fp64_retries", C.c_int),
+                ("steps_abandoned", C.c_int)]
 
 
 class StepInfo(C.Structure):
@@ -449,13 +450,16 @@ class Server:
     """Many steady solves on one GPU (jm_server_*): admission by device
     footprint; batched=False: one stream and graph-captured steps per active
     solve; batched=True: the active solves step in lockstep, each kernel of
-    the step and of the BiCGStab launched once for all of them."""
+    the step and of the BiCGStab launched once for all of them. A mixed
+    solve that goes non-finite is run again once in FP64 unless
+    fp64_retry=False (the report's fp64_retries says so)."""
 
     QUEUED, RUNNING, DONE, FAILED = 0, 1, 2, 3
 
-    def __init__(self, mem_budget_bytes: int, max_active: int = 8, batched: bool = False):
+    def __init__(self, mem_budget_bytes: int, max_active: int = 8, batched: bool = False, fp64_retry: bool = True):
         self.h = C.c_void_p()
-        _check(lib().jm_server_create(int(mem_budget_bytes), int(max_active), 1 if batched else 0, C.byref(self.h)))
+        flags = (1 if batched else 0) | (0 if fp64_retry else 2)
+        _check(lib().jm_server_create(int(mem_budget_bytes), int(max_active), flags, C.byref(self.h)))
         self._meshes = {}
 
     def close(self):

@@ -239,3 +239,70 @@ def test_served_mixed_solve_reaches_the_oracle_steady_state():
         err = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
         print("served mixed vs oracle", k, f"{err:.1e}", flush=True)
         assert err <= 1e-8, (k, err)
+
+
+def _uniform_with_a_nan(mesh, joe):
+    s = joemi.Solver(mesh, joe, joemi.JM_FP64)
+    s.init_uniform()
+    rho, rhou, rhoE = (np.array(a) for a in s.state())
+    del s
+    rho[len(rho) // 2] = np.nan
+    return rho, rhou, rhoE
+
+
+@pytest.mark.parametrize("batched", [False, True])
+def test_non_finite_mixed_solve_is_retried_once_in_fp64(batched):
+    # a mixed solve whose first residual is NaN (one NaN density in its
+    # initial state) is queued again in FP64 from that same state; that run
+    # is NaN too and retires, not queued a third time. With fp64_retry=False,
+    # or for an FP64 request, there is no retry
+    print(f"serving: NaN retry batched={batched}", flush=True)
+    mm, joe = _case(REQS[0], nsteps=20)
+    mesh = joemi.Mesh.from_faces(mm)
+    init = _uniform_with_a_nan(mesh, joe)
+    for retry, prec, want in ((True, joemi.JM_MIXED, 1), (False, joemi.JM_MIXED, 0), (True, joemi.JM_FP64, 0)):
+        srv = joemi.Server(mem_budget_bytes=4 << 30, max_active=2, batched=batched, fp64_retry=retry)
+        t = srv.submit(mesh, joe, prec, state=init)
+        srv.wait(t)
+        st, rep = srv.result(t)
+        assert rep.status == joemi.Server.DONE, (retry, prec)
+        assert rep.fp64_retries == want and rep.steps_abandoned == want, (retry, prec, rep.fp64_retries)
+        assert rep.steps == 1 and not np.isfinite(rep.resid[4])
+        srv.release(t)
+        srv.close()
+
+
+def test_diverging_mixed_solve_finishes_as_its_fp64_solve():
+    # request 51 of the serving bench's 64 (seed 7): the 168 x 56 bump at
+    # Mach 0.494, bump 9.8 %, CFL ramp to 20. Its mixed solve's FP32 BiCGStab
+    # ended step 184 at the iteration limit on a far-off iterate (minimum
+    # pressure 0.87 -> 0.05) and the solve went non-finite at step 185, while
+    # the FP64 solve stays finite. Served with the retry, the result is the
+    # FP64 solve's bits and the abandoned attempt's steps are reported
+    print("serving: diverging mixed solve", flush=True)
+    r = cases.serving_requests(64, 7, 3000, 12000, rate_per_s=0.0)[51]
+    mm, joe = cases.bump_case(r["nx"], r["ny"], nsteps=300, thickness=r["thickness"], mach=r["mach"],
+                              resid_rel=1e-12, cfl_max=20.0)
+    mesh = joemi.Mesh.from_faces(mm)
+    ref, n64, _ = _alone(mesh, joe, joemi.JM_FP64)
+    assert n64 == 300 and all(np.all(np.isfinite(a)) for a in ref)
+    outcome = {}
+    for retry in (False, True):
+        srv = joemi.Server(mem_budget_bytes=4 << 30, max_active=2, batched=True, fp64_retry=retry)
+        t = srv.submit(mesh, joe, joemi.JM_MIXED)
+        srv.wait(t)
+        outcome[retry] = srv.result(t)
+        srv.release(t)
+        srv.close()
+    st0, rep0 = outcome[False]
+    diverged = not np.isfinite(rep0.resid[4])
+    print("mixed alone diverged:", diverged, "at step", rep0.steps, flush=True)
+    st1, rep1 = outcome[True]
+    assert rep1.fp64_retries == int(diverged)
+    if diverged:
+        assert rep1.steps_abandoned == rep0.steps and rep1.steps == 300
+        for a, b in zip(st1, ref):
+            assert np.array_equal(a, b)
+    else:
+        for a, b in zip(st1, st0):
+            assert np.array_equal(a, b)
