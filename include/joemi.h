@@ -146,7 +146,13 @@ jm_status jm_calc_rhs(jm_ctx *c, int flags);
  * NUSA NUSA_BFA GRAD_NUSA WALL_DIST STR_MAG VORT_MAG RHS_NUSA ASCAL
  * ([6][nbocv_s], runBackwardEuler's AScal[0]) DSCAL, and RESID has 6 entries
  * (in the coupled loop A is [nbocv_s][6][6] and DQ [ncv][6], and ASCAL and
- * DSCAL do not exist). out may be NULL to query *count. */
+ * DSCAL do not exist). Evaluated from the current state when asked (each
+ * runs the residual once more): FLUX_FA, each face's inviscid flux [nfa][5]
+ * (the terms calcRhs accumulates, JoeWithModels.cpp:2786-3105); MUL_FA and
+ * MUT_FA, each face's laminar and eddy viscosity [nfa] (mul_fa of
+ * calcMaterialProperties, UgpWithCvCompFlow.h:1701-1753; mut_fa of
+ * calcRansTurbViscMuet, TurbModel_SA.h:74-128), zero when the case is
+ * inviscid or has no turbulence model. out may be NULL to query *count. */
 jm_status jm_get(jm_ctx *c, const char *name, double *out, int64_t *count);
 
 /* One pass of the runBackwardEuler loop (JoeWithModels.cpp:975-1158): CFL

@@ -59,12 +59,15 @@ def build_host(verbose: bool = False) -> None:
     """The C++ JoeWithModels mirror (libjoemi_host.a, host code over the C
     ABI only) and the example case program bin/joe_cases linked against it."""
     flags = ["-O2", "-std=c++17", "-Wall", "-Wno-pmf-conversions", "-fPIC"]
-    o = os.path.join(BUILD, "JoeWithModels.o")
-    _run(["g++", *flags, "-c", os.path.join(HERE, "host", "JoeWithModels.cpp"), "-o", o], verbose)
+    objs = []
+    for src in ("UgpWithCvCompFlow", "JoeWithModels"):
+        o = os.path.join(BUILD, src + ".o")
+        _run(["g++", *flags, "-c", os.path.join(HERE, "host", src + ".cpp"), "-o", o], verbose)
+        objs.append(o)
     ar = os.path.join(HERE, "libjoemi_host.a")
     if os.path.exists(ar):
         os.remove(ar)
-    _run(["ar", "rcs", ar, o], verbose)
+    _run(["ar", "rcs", ar, *objs], verbose)
     os.makedirs(os.path.join(HERE, "bin"), exist_ok=True)
     exe = os.path.join(HERE, "bin", "joe_cases")
     _run(["g++", *flags, os.path.join(HERE, "host", "cases", "joe_cases.cpp"), "-o", exe + ".tmp", ar,

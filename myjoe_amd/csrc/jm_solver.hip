@@ -79,6 +79,9 @@ struct DParams {
   // parity readout: when set, the per-entry flux kernel writes each face's
   // inviscid flux [nfa][5] (device face order) once, from the face's first cell
   double *fdump;
+  // when set, the viscous per-entry flux kernel writes each face's laminar
+  // and eddy viscosity [nfa][2] (device face order), as it used them
+  double *vdump;
 };
 
 // per-cell Spalart-Allmaras fields in DParams::sa_aux
@@ -2911,6 +2914,10 @@ __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const
     }
     if (P.fdump && en.x >= 0)
       for (int k = 0; k < 5; k++) P.fdump[5 * (size_t)f + k] = F[k];
+    if (VISC && P.vdump && en.x >= 0) {
+      P.vdump[2 * (size_t)f] = mul;
+      P.vdump[2 * (size_t)f + 1] = mut;
+    }
     if (VISC) {
       if (CPL)
         for (int k = 0; k < 5; k++) slot[6 + k] = vcc[k];
@@ -4862,6 +4869,7 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
   d.nbs = (d.par.coupled && d.par.sa) ? 6 : 5;
   d.dp.sa_phi = d.sa_phi.p;
   d.dp.fdump = nullptr;
+  d.dp.vdump = nullptr;
   d.dp.sa_aux = d.sa_aux.p;
   d.cfl = d.par.cfl;
   HIPCHECK(hipDeviceSynchronize());
@@ -6310,6 +6318,36 @@ int64_t Ctx::get(const std::string &name, double *out) {
     for (size_t f = 0; f < nf; f++)
       for (int k = 0; k < 5; k++) out[5 * f + k] = h[5 * (size_t)d.fnew[f] + k];
     return (int64_t)(5 * nf);
+  }
+  if (name == "MUL_FA" || name == "MUT_FA") {
+    // each face's laminar (mul_fa, calcMaterialProperties,
+    // UgpWithCvCompFlow.h:1701-1753) or eddy viscosity (mut_fa,
+    // calcRansTurbViscMuet, TurbModel_SA.h:74-128) [nfa], reference
+    // numbering, as the viscous flux kernel forms them from the current state.
+    // Zero when the case is inviscid (the reference leaves mul_fa unset then)
+    // or has no turbulence model
+    const size_t nf = d.nfa;
+    if (!out) return (int64_t)nf;
+    if (!d.have_state) throw std::logic_error("no state set");
+    if (!d.dp.visc || (name == "MUT_FA" && !d.dp.sa)) {
+      for (size_t f = 0; f < nf; f++) out[f] = 0.0;
+      return (int64_t)nf;
+    }
+    DBuf<double> vd;
+    vd.alloc(2 * nf);
+    d.dp.vdump = vd.p;
+    try {
+      d.launch_state_bc();
+      d.launch_grad_flux(false);
+    } catch (...) {
+      d.dp.vdump = nullptr;
+      throw;
+    }
+    d.dp.vdump = nullptr;
+    copy(vd.p, 2 * nf * sizeof(double));
+    const int k = name == "MUL_FA" ? 0 : 1;
+    for (size_t f = 0; f < nf; f++) out[f] = h[2 * (size_t)d.fnew[f] + k];
+    return (int64_t)nf;
   }
   if (name == "COUNTERS") {  // negative pressures, first-order fallbacks
     if (out) {

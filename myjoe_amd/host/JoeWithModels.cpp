@@ -9,123 +9,16 @@
 #include <sstream>
 #include <stdexcept>
 
-namespace {
-
-// Param.cpp:247-298 tokenizer: separators ' ', '=', tab, CR; '#' starts a comment
-std::vector<std::string> tokens(const std::string &line) {
-  std::vector<std::string> out;
-  std::string cur;
-  for (char ch : line) {
-    if (ch == '#') break;
-    if (ch == ' ' || ch == '=' || ch == '\t' || ch == '\r') {
-      if (!cur.empty()) out.push_back(cur);
-      cur.clear();
-    } else {
-      cur.push_back(ch);
-    }
-  }
-  if (!cur.empty()) out.push_back(cur);
-  return out;
-}
-
-bool ends_with(const std::string &s, const std::string &suf) {
-  return s.size() >= suf.size() && s.compare(s.size() - suf.size(), suf.size(), suf) == 0;
-}
-
-}  // namespace
-
-void JoeWithModels::require(jm_status st, const char *what) const {
-  if (st != JM_OK) {
-    std::ostringstream os;
-    os << "joemi: " << what << " failed (status " << (int)st << "): " << jm_last_error();
-    throw std::runtime_error(os.str());
-  }
-}
-
-JoeWithModels::JoeWithModels(const char *inputFile) {
-  std::ifstream f(inputFile);
-  if (!f) throw std::runtime_error(std::string("cannot open input file ") + inputFile);
-  std::stringstream ss;
-  ss << f.rdbuf();
-  joe_in_ = ss.str();
-  std::istringstream is(joe_in_);
-  std::string line;
-  while (std::getline(is, line)) {
-    std::vector<std::string> t = tokens(line);
-    if (!t.empty()) params_[t[0]].push_back(t);
-  }
-  GAMMA = getDoubleParam("GAMMA", "1.4");
-  p_ref = getDoubleParam("P_REF");
-  rho_ref = getDoubleParam("RHO_REF");
-  T_ref = getDoubleParam("T_REF");
-  R_gas = p_ref / rho_ref / T_ref;
-  gravity[0] = getDoubleParam("GX", "0.0");
-  gravity[1] = getDoubleParam("GY", "0.0");
-  gravity[2] = getDoubleParam("GZ", "0.0");
-  nsteps = getIntParam("NSTEPS", "100");
-  resid_energ_th = getDoubleParam("RESID_ENERG_TH", "0.0");
-
-  // mesh: RESTART names a Fluent .msh (MshFilter path of UgpWithCv::init) or
-  // a restart file (Ugp::readRestart), UgpWithCv.h:316-347
-  const std::string restart = getStringParam("RESTART", "restart.out");
-  const bool from_restart = !ends_with(restart, ".msh");
-  if (from_restart)
-    require(jm_mesh_read_restart(restart.c_str(), &mesh_), "jm_mesh_read_restart");
-  else
-    require(jm_mesh_read_msh(restart.c_str(), &mesh_), "jm_mesh_read_msh");
-  int64_t counts[8];
-  require(jm_mesh_counts(mesh_, counts), "jm_mesh_counts");
-  nno = (int)counts[0];
-  nfa = (int)counts[1];
-  nfa_b = (int)counts[2];
-  ncv = ncv_g = (int)counts[3];
-  nbocv_s = (int)counts[4];
-  const int nzone = (int)counts[5];
-
-  auto load_i = [&](const char *name, std::vector<int> &v) {
-    int64_t n = 0;
-    require(jm_mesh_array(mesh_, name, nullptr, &n), name);
-    v.resize(n);
-    require(jm_mesh_array(mesh_, name, v.data(), &n), name);
-  };
-  auto load_d = [&](const char *name, std::vector<double> &v) {
-    int64_t n = 0;
-    require(jm_mesh_array(mesh_, name, nullptr, &n), name);
-    v.resize(n);
-    require(jm_mesh_array(mesh_, name, v.data(), &n), name);
-  };
-  load_i("CVOFA", cvofa_v_);
-  load_i("NBOCV_I", nbocv_i_v_);
-  load_i("NBOCV_V", nbocv_v_v_);
-  load_d("X_CV", x_cv_v_);
-  load_d("X_FA", x_fa_v_);
-  load_d("FA_NORMAL", fa_normal_v_);
-  load_d("CV_VOLUME", cv_volume_v_);
-  cvofa = reinterpret_cast<int(*)[2]>(cvofa_v_.data());
-  nbocv_i = nbocv_i_v_.data();
-  nbocv_v = nbocv_v_v_.data();
-  x_cv = reinterpret_cast<double(*)[3]>(x_cv_v_.data());
-  x_fa = reinterpret_cast<double(*)[3]>(x_fa_v_.data());
-  fa_normal = reinterpret_cast<double(*)[3]>(fa_normal_v_.data());
-  cv_volume = cv_volume_v_.data();
-
-  char name[256];
-  for (int i = 0; i < nzone; i++) {
-    FaZone z;
-    require(jm_mesh_zone(mesh_, i, name, sizeof(name), &z.kind_, &z.ifa_f, &z.ifa_l), "jm_mesh_zone");
-    z.name_ = name;
-    z.index_ = i;
-    faZoneList.push_back(z);
+JoeWithModels::JoeWithModels(const char *inputFile) : UgpWithCvCompFlow(inputFile) {
+  init(inputFile);  // when the most-derived class constructed the base without it
+  for (const FaZone &z : faZoneList)
     if (z.kind_ == FA_ZONE_BOUNDARY) {
       auto it = params_.find(z.name_);
-      if (it != params_.end() && it->second[0].size() > 1 && it->second[0][1] == "HOOK") hook_zones_.push_back(i);
+      if (it != params_.end() && it->second[0].size() > 1 && it->second[0][1] == "HOOK") hook_zones_.push_back(z.index_);
     }
-  }
-
   const std::string prec = getStringParam("IMPLICIT_PRECISION", "FP64");
   if (prec != "FP64" && prec != "MIXED") throw std::runtime_error("IMPLICIT_PRECISION must be FP64 or MIXED");
-  allocHost();
-  if (from_restart) {  // the registered fields the file held (Ugp.cpp:12570-12700)
+  if (from_restart_) {  // the registered fields the file held (Ugp.cpp:12570-12700)
     auto field = [&](const char *name, double *out, size_t n) {
       int64_t cnt = 0;
       if (jm_mesh_restart_field(mesh_, name, nullptr, &cnt) != JM_OK) return false;
@@ -178,46 +71,6 @@ void JoeWithModels::ensureContext() {
 
 JoeWithModels::~JoeWithModels() {
   if (ctx_) jm_ctx_destroy(ctx_);
-  if (mesh_) jm_mesh_destroy(mesh_);
-}
-
-void JoeWithModels::allocHost() {
-  const size_t n = ncv;
-  rho_v_.assign(n, 0.0);
-  rhou_v_.assign(3 * n, 0.0);
-  rhoE_v_.assign(n, 0.0);
-  vel_v_.assign(3 * n, 0.0);
-  press_v_.assign(n, 0.0);
-  temp_v_.assign(n, 0.0);
-  gamma_v_.assign(n, GAMMA);
-  rom_v_.assign(n, R_gas);
-  rho = rho_v_.data();
-  rhou = reinterpret_cast<double(*)[3]>(rhou_v_.data());
-  rhoE = rhoE_v_.data();
-  vel = reinterpret_cast<double(*)[3]>(vel_v_.data());
-  press = press_v_.data();
-  temp = temp_v_.data();
-  gamma = gamma_v_.data();
-  RoM = rom_v_.data();
-}
-
-bool JoeWithModels::checkParam(const std::string &name) const { return params_.count(name) != 0; }
-
-std::string JoeWithModels::getStringParam(const std::string &name, const std::string &def) const {
-  auto it = params_.find(name);
-  if (it == params_.end() || it->second[0].size() < 2) {
-    if (def.empty()) throw std::runtime_error("missing parameter " + name);
-    return def;
-  }
-  return it->second[0][1];
-}
-
-double JoeWithModels::getDoubleParam(const std::string &name, const std::string &def) const {
-  return std::stod(getStringParam(name, def));
-}
-
-int JoeWithModels::getIntParam(const std::string &name, const std::string &def) const {
-  return std::stoi(getStringParam(name, def));
 }
 
 // uniform branch of JoeWithModels::initialHook (JoeWithModels.cpp:180-211)
@@ -270,15 +123,43 @@ bool JoeWithModels::sourceHookOverridden() {
   return mine != base;
 }
 
+bool JoeWithModels::temporalHookOverridden() {
+  typedef void (*Fn)(JoeWithModels *);
+  JoeWithModels probe{ProbeTag{}};
+  Fn mine = (Fn)(this->*(&JoeWithModels::temporalHook));
+  Fn base = (Fn)(probe.*(&JoeWithModels::temporalHook));
+  return mine != base;
+}
+
 void JoeWithModels::syncStateFromDevice() {
   require(jm_get_state(ctx_, rho, &rhou[0][0], rhoE), "jm_get_state");
-  // calcStateVariables (UgpWithCvCompFlow.h:1662-1694), ideal gas
+  // calcStateVariables (UgpWithCvCompFlow.h:1662-1694), ideal gas, no
+  // kinetic energy; like the reference, a non-positive pressure keeps the
+  // previous value
   for (int icv = 0; icv < ncv; icv++) {
     for (int i = 0; i < 3; i++) vel[icv][i] = rhou[icv][i] / rho[icv];
-    double d = 0.0;
-    for (int i = 0; i < 3; i++) d += rhou[icv][i] * rhou[icv][i];
-    press[icv] = (gamma[icv] - 1.0) * (rhoE[icv] - 0.5 * d / rho[icv]);
+    const double pr = (gamma[icv] - 1.0) * (rhoE[icv] - 0.5 * vecDotVec3d(rhou[icv], rhou[icv]) / rho[icv]);
+    if (pr > 0.0) press[icv] = pr;
     temp[icv] = press[icv] / (rho[icv] * RoM[icv]);
+    enthalpy[icv] = gamma[icv] * RoM[icv] / (gamma[icv] - 1.0) * temp[icv];
+    sos[icv] = std::sqrt(gamma[icv] * press[icv] / rho[icv]);
+  }
+}
+
+void JoeWithModels::syncFieldsFromDevice() {
+  syncStateFromDevice();
+  int64_t n = nfa;
+  require(jm_get(ctx_, "MUL_FA", mul_fa, &n), "jm_get MUL_FA");
+  n = 9 * (int64_t)ncv;
+  require(jm_get(ctx_, "GRAD_U", &grad_u[0][0][0], &n), "jm_get GRAD_U");
+  if (RansTurbSA *sa = dynamic_cast<RansTurbSA *>(this)) {
+    n = ncv;
+    require(jm_get(ctx_, "NUSA", sa->nuSA, &n), "jm_get NUSA");
+    require(jm_get(ctx_, "WALL_DIST", sa->wallDist, &n), "jm_get WALL_DIST");
+    n = nfa;
+    require(jm_get(ctx_, "MUT_FA", sa->mut_fa_v_.data(), &n), "jm_get MUT_FA");
+    for (int icv = 0; icv < ncv; icv++)  // calcRansTurbViscMuet's muT (TurbModel_SA.h:126)
+      sa->muT[icv] = InterpolateAtCellCenterFromFaceValues(sa->mut_fa_v_.data(), icv);
   }
 }
 
@@ -326,6 +207,10 @@ int JoeWithModels::doneSolver(double residEnergy) {
 
 // JoeWithModels::run -> runBackwardEuler (JoeWithModels.cpp:881-1198)
 void JoeWithModels::run() {
+  // the device mesh is the file's as read: transformMeshHook, which the
+  // reference calls while reading it (Ugp.h:3045), is not called here
+  if (checkParam("TRANSFORM_MESH"))
+    throw std::runtime_error("TRANSFORM_MESH: mesh transforms are not supported (transformMeshHook is not called)");
   ensureContext();
   RansTurbSA *sa = dynamic_cast<RansTurbSA *>(this);
   if (sa) {  // initialHook's uniform nuSA unless the restart file held it (JoeWithModels.cpp:219-240)
@@ -338,6 +223,19 @@ void JoeWithModels::run() {
       read_nuSA_ = true;
     }
     sa->nuSA = sa->nuSA_v_.data();
+    sa->muT_v_.assign(ncv, 0.0);
+    sa->wallDist_v_.assign(ncv, 0.0);
+    sa->mut_fa_v_.assign(nfa, 0.0);
+    sa->muT = sa->muT_v_.data();
+    sa->wallDist = sa->wallDist_v_.data();
+    // the model's registered fields and its transported scalar (TurbModel_SA.h:30-50)
+    registerScalar(sa->nuSA, "nuSA");
+    registerScalar(sa->muT, "muT");
+    registerScalar(sa->wallDist, "wallDist");
+    if (!getScalarTransportData("nuSA")) {
+      scalarTranspEqVector.emplace_back("nuSA");
+      scalarTranspEqVector.back().phi = sa->nuSA;
+    }
   }
   initialHook();
   require(jm_set_state(ctx_, rho, &rhou[0][0], rhoE), "jm_set_state");
@@ -347,6 +245,9 @@ void JoeWithModels::run() {
   step = checkParam("RESET_STEP") ? 0 : restart_step_;
   if (step > 0) require(jm_set_step(ctx_, step), "jm_set_step");
   const int write_restart = getIntParam("WRITE_RESTART", "-1");
+  // host copies for an overridden temporalHook, every HOST_SYNC_INTERVAL
+  // steps (this build's key; 0 = never)
+  const int sync_every = temporalHookOverridden() ? getIntParam("HOST_SYNC_INTERVAL", "1") : 0;
   applyBoundaryHooks();
   // runBackwardEulerCoupled calls sourceHookCoupled (default: gravity, which
   // the device applies) instead of sourceHook (JoeWithModels.cpp:4596-4611),
@@ -389,6 +290,7 @@ void JoeWithModels::run() {
     lastLinearIterations = info.lin_iters;
     for (int k = 0; k < 5; k++) Residual[k] = info.resid[k];
     applyBoundaryHooks();
+    if (sync_every > 0 && step % sync_every == 0) syncFieldsFromDevice();
     temporalHook();
     done = doneSolver(Residual[4]);
     if ((write_restart > 0) && (step % write_restart == 0)) {  // JoeWithModels.cpp:1155-1156
@@ -402,14 +304,7 @@ void JoeWithModels::run() {
       writeRestart(name);
     }
   }
-  syncStateFromDevice();
-  if (sa) {
-    int64_t n = ncv;
-    require(jm_get(ctx_, "NUSA", sa->nuSA_v_.data(), &n), "jm_get NUSA");
-    sa->wallDist_v_.assign(ncv, 0.0);
-    require(jm_get(ctx_, "WALL_DIST", sa->wallDist_v_.data(), &n), "jm_get WALL_DIST");
-    sa->wallDist = sa->wallDist_v_.data();
-  }
+  syncFieldsFromDevice();
   finalHook();
   writeRestart();  // JoeWithModels.cpp:1175-1176
 }

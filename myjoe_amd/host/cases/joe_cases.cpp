@@ -10,6 +10,11 @@
 //     perturb   initialHook adds a smooth density perturbation
 //     sa        the Spalart-Allmaras mix-in (MyJoeSA, joe.cpp:286-296); nuSA
 //               is appended to out.bin
+//     walls     constructed as joe.cpp's MyJoe is (naming the virtual base
+//               UgpWithCvCompFlow), it reads the host copies in finalHook to
+//               form each WALL face's shear, y+, heat flux and viscosity
+//               (writeWallValues, joe.cpp:43-130) and appends their count
+//               and values to out.bin
 //
 // out.bin: step count, the five L1 residuals of every step, then rho[ncv],
 // rhou[ncv][3], rhoE[ncv] (reference numbering), all as doubles.
@@ -87,6 +92,43 @@ class Perturb : public Recorder {
   }
 };
 
+// the host copies a reference case class reads: mul_fa, the state, the gas
+// constants, faZoneList and getParam, as MyJoe::writeWallValues does
+class WallCase : public Recorder {
+ public:
+  explicit WallCase(const char *f) : Recorder(f), UgpWithCvCompFlow(f) {}
+  std::vector<double> walls;  // per wall face: tau_wall, y+, heat flux, mul
+  int fresh = 0;              // temporalHook calls that saw a new host state
+  double seen = -1.0;
+  void temporalHook() override {
+    Recorder::temporalHook();
+    if (rhoE[ncv / 2] != seen) fresh++;
+    seen = rhoE[ncv / 2];
+  }
+  void finalHook() override {
+    Recorder::finalHook();
+    for (list<FaZone>::iterator zone = faZoneList.begin(); zone != faZoneList.end(); zone++) {
+      Param *param;
+      if (zone->getKind() != FA_ZONE_BOUNDARY || !getParam(param, zone->getName()) || param->getString() != "WALL")
+        continue;
+      for (int ifa = zone->ifa_f; ifa <= zone->ifa_l; ifa++) {
+        const int icv = cvofa[ifa][0];
+        double n[3], d[3], v[3], vt[3];
+        normVec3d(n, fa_normal[ifa]);
+        vecMinVec3d(d, x_fa[ifa], x_cv[icv]);
+        for (int i = 0; i < 3; i++) v[i] = rhou[icv][i] / rho[icv];
+        const double un = vecDotVec3d(n, v);
+        for (int i = 0; i < 3; i++) vt[i] = v[i] - un * n[i];
+        const double dw = fabs(vecDotVec3d(d, n));
+        const double tau = mul_fa[ifa] * sqrt(vecDotVec3d(vt, vt)) / dw * sign(v[0]);
+        const double yplus = dw * sqrt(fabs(tau) / rho[icv]) / (mul_fa[ifa] / rho[icv]);
+        const double kappa = R_gas * gamma[icv] / (gamma[icv] - 1.0) * mul_fa[ifa] / Pr;
+        walls.insert(walls.end(), {tau, yplus, kappa * (temp[icv] - 300.0) / dw, mul_fa[ifa]});
+      }
+    }
+  }
+};
+
 class SaCase : public Recorder, public RansTurbSA {
  public:
   explicit SaCase(const char *f) : Recorder(f) {}
@@ -94,7 +136,7 @@ class SaCase : public Recorder, public RansTurbSA {
 
 int main(int argc, char **argv) {
   if (argc != 4) {
-    std::cerr << "usage: joe_cases <plain|inlethook|gravity|perturb|sa> <Joe.in> <out.bin>\n";
+    std::cerr << "usage: joe_cases <plain|inlethook|gravity|perturb|sa|walls> <Joe.in> <out.bin>\n";
     return 2;
   }
   try {
@@ -121,6 +163,17 @@ int main(int argc, char **argv) {
       j.run();
       j.write(argv[3]);
       std::printf("perturb steps=%d\n", j.step);
+    } else if (c == "walls") {
+      WallCase j(argv[2]);
+      j.run();
+      j.write(argv[3]);
+      std::FILE *fp = std::fopen(argv[3], "ab");
+      if (!fp) throw std::runtime_error("cannot write output");
+      const double nw = (double)(j.walls.size() / 4);
+      std::fwrite(&nw, sizeof(double), 1, fp);
+      std::fwrite(j.walls.data(), sizeof(double), j.walls.size(), fp);
+      std::fclose(fp);
+      std::printf("walls steps=%d fresh=%d faces=%d\n", j.step, j.fresh, (int)nw);
     } else if (c == "sa") {
       SaCase j(argv[2]);
       j.run();

@@ -166,7 +166,8 @@ def test_gpu_coupled_matches_reference(fixture):
         s.set_scalar(g["NUSA"])
     s.calc_rhs(True)
     exact = total = 0
-    for k in FIELDS + (["GRAD_ENTHALPY"] if "visc" in fixture else []) + (SA_DEVICE_FIELDS if sa else []):
+    for k in FIELDS + (["GRAD_ENTHALPY", "MUL_FA"] if "visc" in fixture else []) + \
+            (SA_DEVICE_FIELDS + ["MUT_FA"] if sa else []):
         if k not in g:
             continue
         e, n = compare(s.get(k), g[k], 1e-12, f"{fixture}:{k}")

@@ -1,0 +1,97 @@
+"""The reference's own case classes against the C++ mirror.
+
+joe.cpp's MyJoe and MyJoeSA (src/joe/joe.cpp:24-296: the vanilla case with
+its writeWallValues / writeAllValues / transformMeshHook / temporalHook /
+finalHook, and the Spalart-Allmaras mix-in) are compiled and linked as they
+stand against myjoe_amd/host/JoeWithModels.hpp, libjoemi_host.a and
+libjoemi.so: their constructors name the virtual base UgpWithCvCompFlow,
+and their hooks read mul_fa, Pr, faZoneList, getParam, the MPI calls,
+scalarTranspEqVector, getR1, grad_u and the Reynolds-stress arrays. The class
+text is read from the reference tree when the test runs and compiled in a
+temporary directory; nothing of it is kept. Skipped where the reference tree
+is absent.
+
+Without a GPU the program still builds each case on a bump mesh (the
+virtual base reads Joe.in and the mesh on the host) and then fails in
+run() with the no-device status, not a crash.
+"""
+import os
+import subprocess
+
+import pytest
+
+from myjoe_amd import cases, meshgen
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+JOE_CPP = "/root/reference/src/joe/joe.cpp"
+
+MAIN = r"""
+#include <cstdlib>
+int main(int argc, char **argv) {
+  JoeWithModels *joe = std::atoi(argv[1]) == 0 ? static_cast<JoeWithModels *>(new MyJoe(argv[2]))
+                                                : static_cast<JoeWithModels *>(new MyJoeSA(argv[2]));
+  std::printf("ncv %d nfa %d zones %d Pr %g\n", joe->ncv, joe->nfa, (int)joe->faZoneList.size(), joe->Pr);
+  Param *p;
+  for (FaZone &z : joe->faZoneList)
+    if (joe->getParam(p, z.getName())) std::printf("zone %s %s\n", z.getName(), p->getString().c_str());
+  try {
+    joe->run();
+  } catch (std::exception &e) {
+    std::printf("run: %s\n", e.what());
+    delete joe;
+    return 3;
+  }
+  delete joe;
+  return 0;
+}
+"""
+
+
+def case_classes():
+    """MyJoe and MyJoeSA as joe.cpp writes them: from the MyJoe class head to
+    the end of MyJoeSA's body."""
+    text = open(JOE_CPP).read()
+    start = text.index("class MyJoe : public JoeWithModels")
+    sa = text.index("class MyJoeSA : public MyJoe, public RansTurbSA")
+    end = text.index("};", sa) + 2
+    return text[start:end]
+
+
+@pytest.fixture(scope="module")
+def program(tmp_path_factory):
+    if not os.path.exists(JOE_CPP):
+        pytest.skip("the reference tree is not present")
+    d = tmp_path_factory.mktemp("dropin")
+    src = d / "myjoe_case.cpp"
+    src.write_text('#include "myjoe_amd/host/JoeWithModels.hpp"\n\n' + case_classes() + "\n" + MAIN)
+    exe = d / "myjoe_case"
+    lib = os.path.join(ROOT, "myjoe_amd")
+    r = subprocess.run(["g++", "-std=c++17", "-O1", "-w", f"-I{ROOT}", str(src), os.path.join(lib, "libjoemi_host.a"),
+                        f"-L{lib}", "-ljoemi", f"-Wl,-rpath,{lib}", "-o", str(exe)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-4000:]
+    return exe
+
+
+def test_reference_case_classes_compile_unchanged(program):
+    assert os.path.exists(program)
+
+
+@pytest.mark.parametrize("run", [0, 1])
+def test_reference_case_classes_construct_on_the_host(program, tmp_path, run):
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present: run() would solve")
+    mm, joe = cases.bump_case(24, 8, nsteps=5)
+    joe = joe.replace("MU_REF = 0.0", "MU_REF = 0.002\nMU_MODE = POWERLAW\nMU_POWER_LAW = 0.76\nPR = 0.7")
+    if run == 1:
+        joe = cases.with_sa(joe, 0.006, cases.SA_BUMP)
+    meshgen.write_msh(mm, os.path.join(tmp_path, "bump.msh"))
+    (tmp_path / "Joe.in").write_text(joe)
+    r = subprocess.run([str(program), str(run), "Joe.in"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    out = r.stdout
+    assert f"ncv {mm.ncv} " in out and "Pr 0.7" in out, out + r.stderr
+    assert "zone bump_wall WALL" in out and "zone top SYMMETRY" in out, out
+    # the case classes reach run(), which needs the device
+    assert r.returncode == 3 and "status 4" in out, out + r.stderr

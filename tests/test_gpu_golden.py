@@ -13,10 +13,9 @@ from myjoe_amd import joemi, meshgen
 from tests.helpers import compare
 from tests.test_oracle_golden import FIELDS, RHS_FIXTURES, joe_text, load
 
-# the Spalart-Allmaras fields the device exposes (the eddy viscosity itself
-# is formed per face inside the kernels and enters A, LOCAL_DT and the
-# residuals above)
-SA_GPU_FIELDS = ["NUSA_BFA", "GRAD_NUSA", "WALL_DIST", "STR_MAG", "VORT_MAG", "RHS_NUSA", "ASCAL"]
+# the Spalart-Allmaras fields the device exposes; the eddy viscosity is
+# formed per face inside the flux kernel, and MUT_FA reads back what it used
+SA_GPU_FIELDS = ["NUSA_BFA", "GRAD_NUSA", "WALL_DIST", "STR_MAG", "VORT_MAG", "RHS_NUSA", "ASCAL", "MUT_FA"]
 from tests.test_mesh_golden import MESHES
 
 pytestmark = pytest.mark.gpu
@@ -34,7 +33,7 @@ def test_residual_matches_reference(fixture):
         s.set_scalar(g["NUSA"])
     s.calc_rhs(True)
     exact = total = 0
-    for k in FIELDS + (["GRAD_ENTHALPY"] if "visc" in fixture else []) + (SA_GPU_FIELDS if sa else []):
+    for k in FIELDS + (["GRAD_ENTHALPY", "MUL_FA"] if "visc" in fixture else []) + (SA_GPU_FIELDS if sa else []):
         if k not in g:
             continue
         e, n = compare(s.get(k), g[k], 1e-12, f"{fixture}:{k}")
@@ -206,6 +205,6 @@ def test_residual_on_reference_arrays(fixture):
     if sa:
         s.set_scalar(g["NUSA"])
     s.calc_rhs(True)
-    for k in FIELDS + (["GRAD_ENTHALPY"] if "visc" in fixture else []) + (SA_GPU_FIELDS if sa else []):
+    for k in FIELDS + (["GRAD_ENTHALPY", "MUL_FA"] if "visc" in fixture else []) + (SA_GPU_FIELDS if sa else []):
         if k in g:
             compare(s.get(k), g[k], 1e-12, f"ugp {fixture}:{k}")

@@ -213,3 +213,35 @@ def test_case_resumes_from_its_restart_file(tmp_path):
     r2, ru2, rE2 = s.state()
     assert np.array_equal(rho, r2) and np.array_equal(rhou, ru2) and np.array_equal(rhoE, rE2)
     assert joemi.Mesh.read_restart(str(tmp_path / "restart.out")).restart_step()[0] == 25
+
+
+@pytest.mark.gpu
+def test_case_class_reads_wall_values_from_host_copies(tmp_path):
+    # a case class constructed as joe.cpp's MyJoe is (naming the virtual base
+    # UgpWithCvCompFlow) forms each wall face's shear, y+, heat flux and
+    # viscosity in finalHook from the mirror's host copies (mul_fa, the state,
+    # R_gas, gamma, Pr, faZoneList, getParam); they equal the device's
+    # jm_wall_values of the same solve, and its temporalHook saw a fresh host
+    # state after every step
+    print("host mirror: wall values", flush=True)
+    visc = "MU_REF = 0.002\nMU_MODE = POWERLAW\nMU_POWER_LAW = 0.76\nPR = 0.72"
+    joe = joe_text(**{"MU_REF = 0.0": visc})
+    mm = write_case(tmp_path, joe)
+    r = subprocess.run([EXE, "walls", "Joe.in", "w.bin"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert f"fresh={NSTEPS}" in r.stdout, r.stdout
+    raw = np.fromfile(tmp_path / "w.bin", np.float64)
+    n = int(raw[0])
+    assert n == NSTEPS
+    rest = raw[1 + 5 * n + 5 * mm.ncv:]
+    nw = int(rest[0])
+    walls = rest[1:].reshape(nw, 4)
+    s = joemi.Solver(joemi.Mesh.from_faces(mm), joe)
+    s.init_uniform()
+    s.run(NSTEPS)
+    wv = s.wall_values("bump_wall")
+    assert wv.shape[0] == nw
+    from tests.helpers import compare
+    for k, col in ((0, 6), (1, 8), (2, 7), (3, 9)):  # tau_wall, y+, heat flux, mul
+        compare(walls[:, k], wv[:, col], 1e-12, f"wall column {col}")
+    assert np.all(walls[:, 3] > 0.0) and np.any(walls[:, 0] != 0.0)
