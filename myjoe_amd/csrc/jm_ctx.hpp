@@ -64,6 +64,10 @@ struct SolverParams {
   bool visc_limiter = true;
   int grad_kernel = kResidCell, flux_kernel = kResidEntry;
   int jac_kernel = kResidEntry;  // GPU_JAC_KERNEL: ENTRY (one lane per cell-face entry) or CELL (one thread per row)
+  // preconditioner of the step's BiCGStab (GPU_PRECONDITIONER): 0 = block
+  // Jacobi (the reference's), else ILU0: block ILU(0) on subdomains of this
+  // many consecutive cells of the device order (GPU_ILU_SUBDOMAIN)
+  int ilu_sub = 0;
   // Spalart-Allmaras as one segregated scalar nuSA (TURB_MODEL = SA): initial
   // value (INITIAL_CONDITION_TURB) and UNDER_RELAXATION_SCALARS
   bool sa = false;

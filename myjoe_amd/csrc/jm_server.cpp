@@ -107,6 +107,7 @@ int Server::submit(const Mesh *mesh, const std::string &joe_in, int precision, c
   std::vector<ZoneBc> zbc;
   parse_params(joe_in, *mesh, p, zbc);  // reject bad input at submission
   if (batched_ && p.sa) throw Unsupported("batched serving does not carry the Spalart-Allmaras scalar");
+  if (batched_ && p.ilu_sub > 0) throw Unsupported("batched serving runs the block-Jacobi preconditioner");
   if (batched_ && p.mu_ref <= 0.0 && p.flux_kernel != kResidEntry)
     throw Unsupported("batched serving runs the per-entry flux kernel (GPU_FLUX_KERNEL = ENTRY)");
   auto r = std::make_shared<Request>();

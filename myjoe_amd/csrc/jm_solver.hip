@@ -3330,8 +3330,9 @@ __global__ void __launch_bounds__(kBlock) k_resid(int ncv, const int *__restrict
   }
 }
 
-// p = res - beta (p - omega v); phat = LU^-1 p   (one thread per cell)
-template <typename T, int NB>
+// p = res - beta (p - omega v); phat = LU^-1 p   (one thread per cell). With
+// BJ = false only p: the ILU solve forms phat after it.
+template <typename T, int NB, bool BJ = true>
 __global__ void __launch_bounds__(kBlock) k_p(int ncv, const T *__restrict__ LU, const T *__restrict__ res,
                                               const T *__restrict__ v, T *__restrict__ p, T *__restrict__ phat,
                                               const SolverScalars *S) {
@@ -3346,6 +3347,7 @@ __global__ void __launch_bounds__(kBlock) k_p(int ncv, const T *__restrict__ LU,
     p[o] = pv;
     t[k] = pv;
   }
+  if (!BJ) return;
   lusolv_n<T, NB>(LU + NB * NB * (size_t)c, t);
   for (int k = 0; k < NB; k++) phat[NB * (size_t)c + k] = t[k];
 }
@@ -3374,8 +3376,8 @@ __global__ void __launch_bounds__(kBlock) k_spmv_dot(int ncv, const int *__restr
   block_partials<2>(v, partials, sh, kSum);
 }
 
-// s = res - alpha v; shat = LU^-1 s
-template <typename T, int NB>
+// s = res - alpha v; shat = LU^-1 s (with BJ = false only s)
+template <typename T, int NB, bool BJ = true>
 __global__ void __launch_bounds__(kBlock) k_s(int ncv, const T *__restrict__ LU, const T *__restrict__ res,
                                               const T *__restrict__ v, T *__restrict__ s, T *__restrict__ shat,
                                               const SolverScalars *S) {
@@ -3390,6 +3392,7 @@ __global__ void __launch_bounds__(kBlock) k_s(int ncv, const T *__restrict__ LU,
     s[o] = sv;
     t[k] = sv;
   }
+  if (!BJ) return;
   lusolv_n<T, NB>(LU + NB * NB * (size_t)c, t);
   for (int k = 0; k < NB; k++) shat[NB * (size_t)c + k] = t[k];
 }
@@ -3402,6 +3405,150 @@ __global__ void __launch_bounds__(kBlock) k_x(int n5, T *__restrict__ phi, const
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n5) return;
   phi[k] += (T)S->alpha * phat[k] + (T)S->omega * shat[k];
+}
+
+// ---------------------------------------------------------------------------
+// Block ILU(0) on subdomains: this build's alternative to the block-Jacobi
+// preconditioner (GPU_PRECONDITIONER = ILU0). The reference's BCGSTAB has
+// block Jacobi only; its coupled loop hands the system to PETSc GMRES with an
+// additive-Schwarz preconditioner whose subdomain solve is ILU
+// (PetscSolver.h:373-387). Here a subdomain is `sub` consecutive cells of the
+// device order (Morton tiles, so spatially compact); couplings that leave it
+// are dropped (no overlap), and one thread factors and solves each subdomain,
+// sequentially over its rows, so subdomains run in parallel.
+//
+// Factors, in place of a copy F of A ([nnz][NB][NB], the blocks of A's
+// pattern): for row c in order and each in-subdomain lower neighbour k in
+// ascending order, F_ck <- F_ck D_k^-1, then F_cj -= F_ck F_kj for every
+// upper entry (k, j) of row k whose column j is also in row c; D_c is the
+// final diagonal block, kept inverted in Dinv. So L (unit lower) and U
+// (D on the diagonal) satisfy (L U)_cj = A_cj on the subdomain's pattern.
+// ---------------------------------------------------------------------------
+template <typename T, int NB>
+__device__ __forceinline__ void ilu_invert(const T *__restrict__ D, T *__restrict__ Dinv) {
+  T a[NB][NB];
+  for (int i = 0; i < NB; i++)
+    for (int j = 0; j < NB; j++) a[i][j] = D[NB * i + j];
+  ludeco_n<T, NB>(a);
+  for (int j = 0; j < NB; j++) {
+    T e[NB];
+    for (int i = 0; i < NB; i++) e[i] = i == j ? (T)1 : (T)0;
+    lusolv_n<T, NB>(&a[0][0], e);
+    for (int i = 0; i < NB; i++) Dinv[NB * i + j] = e[i];
+  }
+}
+
+template <typename T, int NB>
+__global__ void __launch_bounds__(64) k_ilu_factor(int ncv, int sub, const int *__restrict__ nbocv_i,
+                                                   const int *__restrict__ nbocv_v, const T *__restrict__ A,
+                                                   T *__restrict__ F, T *__restrict__ Dinv) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c0 = d * sub;
+  if (c0 >= ncv) return;
+  const int c1 = min(ncv, c0 + sub);
+  constexpr int BB = NB * NB;
+  for (int c = c0; c < c1; c++) {
+    const int e0 = nbocv_i[c], e1 = nbocv_i[c + 1];
+    for (int e = e0; e < e1; e++) {
+      const int k = nbocv_v[e];
+      if (k < c0 || k >= c1) continue;  // the coupling leaves the subdomain: dropped
+      for (int q = 0; q < BB; q++) F[BB * (size_t)e + q] = A[BB * (size_t)e + q];
+    }
+    int last = -1;
+    for (;;) {  // lower neighbours in ascending column order
+      int ek = -1, k = c;
+      for (int e = e0 + 1; e < e1; e++) {
+        const int col = nbocv_v[e];
+        if (col >= c0 && col > last && col < k) {
+          k = col;
+          ek = e;
+        }
+      }
+      if (ek < 0) break;
+      last = k;
+      T *L = F + BB * (size_t)ek;
+      const T *Di = Dinv + BB * (size_t)k;
+      for (int i = 0; i < NB; i++) {  // L_ck = F_ck D_k^-1, one row at a time
+        T row[NB];
+        for (int j = 0; j < NB; j++) {
+          T acc = (T)0;
+          for (int m = 0; m < NB; m++) acc += L[NB * i + m] * Di[NB * m + j];
+          row[j] = acc;
+        }
+        for (int j = 0; j < NB; j++) L[NB * i + j] = row[j];
+      }
+      for (int f = nbocv_i[k] + 1; f < nbocv_i[k + 1]; f++) {  // upper entries (k, j) of row k
+        const int j = nbocv_v[f];
+        if (j <= k || j >= c1) continue;
+        int ej = -1;
+        for (int e = e0; e < e1; e++)
+          if (nbocv_v[e] == j) {
+            ej = e;
+            break;
+          }
+        if (ej < 0) continue;  // not in row c's pattern: ILU(0) drops the fill
+        const T *U = F + BB * (size_t)f;
+        T *X = F + BB * (size_t)ej;
+        for (int i = 0; i < NB; i++)
+          for (int jj = 0; jj < NB; jj++) {
+            T acc = X[NB * i + jj];
+            for (int m = 0; m < NB; m++) acc -= L[NB * i + m] * U[NB * m + jj];
+            X[NB * i + jj] = acc;
+          }
+      }
+    }
+    ilu_invert<T, NB>(F + BB * (size_t)e0, Dinv + BB * (size_t)c);
+  }
+}
+
+// z = (L U)^-1 r on each subdomain: forward y_c = r_c - sum_{k<c} L_ck y_k,
+// backward z_c = D_c^-1 (y_c - sum_{j>c} U_cj z_j); y is kept in z
+template <typename T, int NB>
+__global__ void __launch_bounds__(64) k_ilu_solve(int ncv, int sub, const int *__restrict__ nbocv_i,
+                                                  const int *__restrict__ nbocv_v, const T *__restrict__ F,
+                                                  const T *__restrict__ Dinv, const T *__restrict__ r,
+                                                  T *__restrict__ z, const SolverScalars *S) {
+  if (S->done || S->stop) return;
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c0 = d * sub;
+  if (c0 >= ncv) return;
+  const int c1 = min(ncv, c0 + sub);
+  constexpr int BB = NB * NB;
+  for (int c = c0; c < c1; c++) {
+    T y[NB];
+    for (int i = 0; i < NB; i++) y[i] = r[NB * (size_t)c + i];
+    for (int e = nbocv_i[c] + 1; e < nbocv_i[c + 1]; e++) {
+      const int k = nbocv_v[e];
+      if (k < c0 || k >= c) continue;
+      const T *L = F + BB * (size_t)e;
+      const T *yk = z + NB * (size_t)k;
+      T v[NB];
+      for (int m = 0; m < NB; m++) v[m] = yk[m];
+      for (int i = 0; i < NB; i++)
+        for (int m = 0; m < NB; m++) y[i] -= L[NB * i + m] * v[m];
+    }
+    for (int i = 0; i < NB; i++) z[NB * (size_t)c + i] = y[i];
+  }
+  for (int c = c1 - 1; c >= c0; c--) {
+    T t[NB];
+    for (int i = 0; i < NB; i++) t[i] = z[NB * (size_t)c + i];
+    for (int e = nbocv_i[c] + 1; e < nbocv_i[c + 1]; e++) {
+      const int j = nbocv_v[e];
+      if (j <= c || j >= c1) continue;
+      const T *U = F + BB * (size_t)e;
+      const T *zj = z + NB * (size_t)j;
+      T v[NB];
+      for (int m = 0; m < NB; m++) v[m] = zj[m];
+      for (int i = 0; i < NB; i++)
+        for (int m = 0; m < NB; m++) t[i] -= U[NB * i + m] * v[m];
+    }
+    const T *Di = Dinv + BB * (size_t)c;
+    for (int i = 0; i < NB; i++) {
+      T acc = (T)0;
+      for (int m = 0; m < NB; m++) acc += Di[NB * i + m] * t[m];
+      z[NB * (size_t)c + i] = acc;
+    }
+  }
 }
 
 // One block system and its work space for the kernels above: A[nnz][NB][NB]
@@ -3418,6 +3565,11 @@ struct BcgSys {
   double *partials = nullptr;
   SolverScalars *S = nullptr;
   hipStream_t stream = nullptr;
+  // block ILU(0) on subdomains of ilu_sub cells instead of block Jacobi for
+  // phat and shat (the stop test keeps the block-Jacobi residual): F
+  // [nnz][NB][NB] and Dinv [ncv][NB][NB]; 0 = block Jacobi
+  int ilu_sub = 0;
+  void *F = nullptr, *Dinv = nullptr;
 };
 
 // start of a solve (UgpWithCv.cpp:2660-2735): p = v = 0, block LU of the
@@ -3433,6 +3585,11 @@ void bcg_prologue(const BcgSys &y) {
   hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n)), dim3(kBlock), 0, y.stream, (size_t)n, vv, (T)0);
   hipLaunchKernelGGL(k_solver_init, dim3(1), dim3(64), 0, y.stream, y.S, y.maxiter, y.zero_abs, y.zero_rel);
   hipLaunchKernelGGL((k_lu<T, NB>), dim3(nblk(ncv)), dim3(kBlock), 0, y.stream, ncv, y.nbocv_i, A, LU);
+  if (y.ilu_sub > 0) {
+    const int nsub = (ncv + y.ilu_sub - 1) / y.ilu_sub;
+    hipLaunchKernelGGL((k_ilu_factor<T, NB>), dim3((nsub + 63) / 64), dim3(64), 0, y.stream, ncv, y.ilu_sub,
+                       y.nbocv_i, y.nbocv_v, A, (T *)y.F, (T *)y.Dinv);
+  }
   const int gb = (ncv + cells_per_block<NB>() - 1) / cells_per_block<NB>();
   hipLaunchKernelGGL((k_resid<T, NB>), dim3(gb), dim3(kBlock), 0, y.stream, ncv, y.nbocv_i, y.nbocv_v, A, LU, phi,
                      rhs_, res, res0, 1, y.partials, y.S);
@@ -3449,12 +3606,28 @@ void bcg_iterations(const BcgSys &y, int count) {
     *vv = (T *)y.v[5], *phat = (T *)y.v[6], *s = (T *)y.v[7], *shat = (T *)y.v[8], *t = (T *)y.v[9];
   const int gb = (ncv + cells_per_block<NB>() - 1) / cells_per_block<NB>();
   hipStream_t st = y.stream;
+  const bool ilu = y.ilu_sub > 0;
+  const int nsub = ilu ? (ncv + y.ilu_sub - 1) / y.ilu_sub : 0;
+  auto ilu_solve = [&](const T *r, T *z) {
+    hipLaunchKernelGGL((k_ilu_solve<T, NB>), dim3((nsub + 63) / 64), dim3(64), 0, st, ncv, y.ilu_sub, y.nbocv_i,
+                       y.nbocv_v, (const T *)y.F, (const T *)y.Dinv, r, z, y.S);
+  };
   for (int k = 0; k < count; k++) {
-    hipLaunchKernelGGL((k_p<T, NB>), dim3(nblk(ncv)), dim3(kBlock), 0, st, ncv, LU, res, vv, p, phat, y.S);
+    if (ilu) {
+      hipLaunchKernelGGL((k_p<T, NB, false>), dim3(nblk(ncv)), dim3(kBlock), 0, st, ncv, LU, res, vv, p, phat, y.S);
+      ilu_solve(p, phat);
+    } else {
+      hipLaunchKernelGGL((k_p<T, NB>), dim3(nblk(ncv)), dim3(kBlock), 0, st, ncv, LU, res, vv, p, phat, y.S);
+    }
     hipLaunchKernelGGL((k_spmv_dot<T, NB>), dim3(gb), dim3(kBlock), 0, st, ncv, y.nbocv_i, y.nbocv_v, A, phat, vv,
                        res0, y.partials, y.S);
     hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, st, gb, (int)kFinAlpha, y.partials, y.S);
-    hipLaunchKernelGGL((k_s<T, NB>), dim3(nblk(ncv)), dim3(kBlock), 0, st, ncv, LU, res, vv, s, shat, y.S);
+    if (ilu) {
+      hipLaunchKernelGGL((k_s<T, NB, false>), dim3(nblk(ncv)), dim3(kBlock), 0, st, ncv, LU, res, vv, s, shat, y.S);
+      ilu_solve(s, shat);
+    } else {
+      hipLaunchKernelGGL((k_s<T, NB>), dim3(nblk(ncv)), dim3(kBlock), 0, st, ncv, LU, res, vv, s, shat, y.S);
+    }
     hipLaunchKernelGGL((k_spmv_dot<T, NB>), dim3(gb), dim3(kBlock), 0, st, ncv, y.nbocv_i, y.nbocv_v, A, shat, t, s,
                        y.partials, y.S);
     hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, st, gb, (int)kFinOmega, y.partials, y.S);
@@ -4154,6 +4327,15 @@ void parse_params(const std::string &joe_in, const Mesh &m, SolverParams &p, std
   p.grad_kernel = kform("GPU_GRAD_KERNEL", "CELL");
   p.flux_kernel = kform("GPU_FLUX_KERNEL", "ENTRY");
   p.jac_kernel = kform("GPU_JAC_KERNEL", "ENTRY");
+  {  // this build's preconditioner option (the reference's BCGSTAB is block Jacobi)
+    const std::string pc = pm.str("GPU_PRECONDITIONER", "BLOCK_JACOBI");
+    if (pc == "ILU0") {
+      p.ilu_sub = pm.integer("GPU_ILU_SUBDOMAIN", 64);
+      if (p.ilu_sub < 1 || p.ilu_sub > 4096) throw std::invalid_argument("GPU_ILU_SUBDOMAIN must be 1..4096");
+    } else if (pc != "BLOCK_JACOBI") {
+      throw std::invalid_argument("GPU_PRECONDITIONER = " + pc + " (BLOCK_JACOBI, ILU0)");
+    }
+  }
   p.gravity[0] = pm.dbl("GX", 0.0);
   p.gravity[1] = pm.dbl("GY", 0.0);
   p.gravity[2] = pm.dbl("GZ", 0.0);
@@ -4364,6 +4546,8 @@ struct Ctx::Impl {
   // solver (T = double or float, both allocated lazily)
   DBuf<double> A64, LU64, v64[10];
   DBuf<float> A32, LU32, v32[10];
+  DBuf<double> F64, Dinv64;  // block ILU(0) factors (GPU_PRECONDITIONER = ILU0)
+  DBuf<float> F32, Dinv32;
   DBuf<double> partials;
   DBuf<SolverScalars> scal;
   SolverScalars *h_scal = nullptr;  // pinned
@@ -5631,13 +5815,22 @@ void Ctx::Impl::launch_scal_c(T *A, bool step_mode) {
 template <typename T>
 void Ctx::Impl::ensure_solver(int nvec, bool with_lu) {
   const size_t b = nbs;
+  const bool ilu = with_lu && par.ilu_sub > 0;
   if constexpr (std::is_same<T, double>::value) {
     ensure(A64, b * b * (size_t)nnz);
     if (with_lu) ensure(LU64, b * b * (size_t)ncv);
+    if (ilu) {
+      ensure(F64, b * b * (size_t)nnz);
+      ensure(Dinv64, b * b * (size_t)ncv);
+    }
     for (int k = 0; k < nvec; k++) ensure(v64[k], b * (size_t)ncv);
   } else {
     ensure(A32, b * b * (size_t)nnz);
     if (with_lu) ensure(LU32, b * b * (size_t)ncv);
+    if (ilu) {
+      ensure(F32, b * b * (size_t)nnz);
+      ensure(Dinv32, b * b * (size_t)ncv);
+    }
     for (int k = 0; k < nvec; k++) ensure(v32[k], b * (size_t)ncv);
   }
 }
@@ -5753,6 +5946,17 @@ BcgSys Ctx::Impl::bcg_sys() {
   y.partials = partials.p;
   y.S = scal.p;
   y.stream = stream;
+  if (par.ilu_sub > 0) {
+    y.ilu_sub = par.ilu_sub;
+    if constexpr (std::is_same<T, double>::value) {
+      y.F = F64.p;
+      y.Dinv = Dinv64.p;
+    } else {
+      y.F = F32.p;
+      y.Dinv = Dinv32.p;
+    }
+    if (!y.F || !y.Dinv) throw std::logic_error("ILU factors not allocated");
+  }
   return y;
 }
 
@@ -6384,6 +6588,40 @@ int64_t Ctx::get(const std::string &name, double *out) {
     } else {
       for (size_t cd = 0; cd < n; cd++)
         for (size_t k = 0; k < b; k++) out[b * (size_t)perm[cd] + k] = dev[b * cd + k];
+    }
+    return (int64_t)cnt;
+  }
+  if (name == "ILU_F" || name == "ILU_DINV") {
+    // the block ILU(0) factors of the last solve (GPU_PRECONDITIONER = ILU0):
+    // F [nnz][b][b] in the reference's nbocv order like A (unit-lower L
+    // below the diagonal, U above it, within each subdomain; other blocks
+    // unused), and the inverted pivot blocks [ncv][b][b]
+    if (d.par.ilu_sub <= 0) throw Unsupported(name + " needs GPU_PRECONDITIONER = ILU0");
+    const bool isF = name == "ILU_F";
+    const size_t bb = (size_t)d.nbs * d.nbs;
+    const size_t cnt = bb * (isF ? (size_t)d.nnz : n);
+    if (!out) return (int64_t)cnt;
+    std::vector<double> dev(cnt);
+    if (d.precision == 0) {
+      const double *src = isF ? d.F64.p : d.Dinv64.p;
+      if (!src) throw std::logic_error(name + " not computed yet");
+      HIPCHECK(hipMemcpy(dev.data(), src, cnt * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+      const float *src = isF ? d.F32.p : d.Dinv32.p;
+      if (!src) throw std::logic_error(name + " not computed yet");
+      std::vector<float> f(cnt);
+      HIPCHECK(hipMemcpy(f.data(), src, cnt * sizeof(float), hipMemcpyDeviceToHost));
+      for (size_t k = 0; k < cnt; k++) dev[k] = f[k];
+    }
+    for (size_t cd = 0; cd < n; cd++) {
+      const size_t c = perm[cd];
+      if (isF) {
+        const size_t len = d.h_dev_nbocv_i[cd + 1] - d.h_dev_nbocv_i[cd];
+        std::memcpy(out + bb * (size_t)d.h_ref_nbocv_i[c], dev.data() + bb * (size_t)d.h_dev_nbocv_i[cd],
+                    bb * len * sizeof(double));
+      } else {
+        std::memcpy(out + bb * c, dev.data() + bb * cd, bb * sizeof(double));
+      }
     }
     return (int64_t)cnt;
   }
