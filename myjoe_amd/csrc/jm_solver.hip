@@ -598,9 +598,10 @@ struct EntryLane {
   int e, c, pos;  // entry (-1: idle lane), its cell, position in the cell's list
 };
 
+template <int W = kWaves>
 __device__ __forceinline__ EntryLane entry_lane(const DMesh &m, const Blk &B) {
   EntryLane L{-1, -1, 0};
-  const int w = xcd_block(B.id, B.n) * kWaves + (threadIdx.x >> 6);
+  const int w = xcd_block(B.id, B.n) * W + (threadIdx.x >> 6);
   if (w < m.nwave) {
     const int e = m.wave_e0[w] + (threadIdx.x & 63);
     if (e < m.wave_e0[w + 1]) {
@@ -613,7 +614,7 @@ __device__ __forceinline__ EntryLane entry_lane(const DMesh &m, const Blk &B) {
   return L;
 }
 
-__device__ __forceinline__ EntryLane entry_lane(const DMesh &m) { return entry_lane(m, hw_blk()); }
+__device__ __forceinline__ EntryLane entry_lane(const DMesh &m) { return entry_lane<>(m, hw_blk()); }
 
 template <bool LS>
 __device__ __forceinline__ void d_grad_e(const Blk &B, DMesh m, DParams P, const double *__restrict__ prim,
@@ -1607,7 +1608,7 @@ __device__ __forceinline__ int vc_index(int k) {
 constexpr int kVcN = 11;
 constexpr int kMaxOwned = 7;  // components of the diagonal block per lane (cells have at least 4 faces)
 
-template <typename T, int SCHEME, bool VISC, int NB = 5>
+template <typename T, int SCHEME, bool VISC, int NB = 5, int W = kWaves>
 __device__ __forceinline__ void d_jac_e(const Blk &B, DMesh m, DParams P, const double *__restrict__ prim,
                                         const double *__restrict__ bfa, T *__restrict__ A, int step_mode,
                                         const double *__restrict__ dt, const double *__restrict__ rhs, double relax,
@@ -1616,10 +1617,10 @@ __device__ __forceinline__ void d_jac_e(const Blk &B, DMesh m, DParams P, const 
   constexpr bool CPL = SCHEME == kHllcC;
   constexpr bool CV = VISC && CPL;
   constexpr int NS = CV ? 25 + kVcN : 25;
-  __shared__ double lds[kBlock * NS];
-  __shared__ signed char vsg[kBlock];  // the entry's viscous block: 0 none, +1 added, -1 subtracted
+  __shared__ double lds[64 * W * NS];
+  __shared__ signed char vsg[64 * W];  // the entry's viscous block: 0 none, +1 added, -1 subtracted
   double *slot = lds + NS * threadIdx.x;
-  const EntryLane L = entry_lane(m, B);
+  const EntryLane L = entry_lane<W>(m, B);
   const double gam = P.gamma;
   const int c = L.c, pos = L.pos;
   const int nent = L.e >= 0 ? m.cfl_i[c + 1] - m.cfl_i[c] : 0;
@@ -1902,13 +1903,18 @@ __device__ __forceinline__ void d_jac_e(const Blk &B, DMesh m, DParams P, const 
     }
 }
 
+// one wavefront per block, as the flux kernel (the diagonal sums wait only
+// for their own wavefront)
+constexpr int kJacWaves = 1;
+
 template <typename T, int SCHEME, bool VISC, int NB = 5>
-__global__ void __launch_bounds__(kBlock) k_jac_e(DMesh m, DParams P, const double *__restrict__ prim,
+__global__ void __launch_bounds__(64 * kJacWaves) k_jac_e(DMesh m, DParams P, const double *__restrict__ prim,
                                                   const double *__restrict__ bfa, T *__restrict__ A, int step_mode,
                                                   const double *__restrict__ dt, const double *__restrict__ rhs,
                                                   double relax, T *__restrict__ rhs_ls, T *__restrict__ phi,
                                                   const double *__restrict__ gradv) {
-  d_jac_e<T, SCHEME, VISC, NB>(hw_blk(), m, P, prim, bfa, A, step_mode, dt, rhs, relax, rhs_ls, phi, gradv);
+  d_jac_e<T, SCHEME, VISC, NB, kJacWaves>(hw_blk(), m, P, prim, bfa, A, step_mode, dt, rhs, relax, rhs_ls, phi,
+                                         gradv);
 }
 
 // ---------------------------------------------------------------------------
@@ -2669,6 +2675,20 @@ __global__ void __launch_bounds__(kFinBlock) k_fin(int nb, int op, const double 
   fin_body(nb, op, partials, (size_t)nb, S, sh);
 }
 
+// first stage of a minimum over many partials: block b reduces
+// in[b * kMinPre, (b + 1) * kMinPre) to out[b]. A minimum is exact in any
+// order, so two stages give the one-stage result's bits.
+constexpr int kMinPre = 2048;
+__global__ void __launch_bounds__(256) k_min_pre(const double *__restrict__ in, int n, double *__restrict__ out) {
+  __shared__ double sh[16];
+  const int b0 = blockIdx.x * kMinPre;
+  double m = __builtin_huge_val();
+  for (int k = b0 + (int)threadIdx.x; k < min(n, b0 + kMinPre); k += blockDim.x) m = Dev::mn(m, in[k]);
+  double v[1] = {m};
+  block_reduce<1>(v, sh, kMin);
+  if (threadIdx.x == 0) out[blockIdx.x] = v[0];
+}
+
 // fin_slot of a kFinBlock-thread block computed by a smaller block (a divisor
 // of kFinBlock, whole wavefronts): thread t plays the full block's threads
 // t, t + blockDim.x, ..., each wave's lanes in the same shuffle tree, and the
@@ -2718,7 +2738,7 @@ __global__ void __launch_bounds__(kBlock) k_flux(DMesh m, DParams P, const doubl
 // face's flux (signed for this cell) and the face's wave-speed bound; the
 // cell's owner lane adds the terms from LDS in entry order (the reference's
 // order), applies the gravity source, and writes the residual and local dt.
-template <int SCHEME, bool SND, bool VISC>
+template <int SCHEME, bool SND, bool VISC, int W = kWaves>
 __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const double *__restrict__ prim,
                                                    const double *__restrict__ grad, const double *__restrict__ bfa,
                                                    const double *__restrict__ q, double *__restrict__ rhs,
@@ -2728,10 +2748,10 @@ __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const
   constexpr bool CPL = SCHEME == kHllcC;
   constexpr int NS = VISC ? (CPL ? 12 : 11) : 6;
   constexpr int kSumSlot = CPL ? 11 : 10;
-  __shared__ double lds[kBlock * NS];
+  __shared__ double lds[64 * W * NS];
   __shared__ double sh[16];
   double *slot = lds + NS * threadIdx.x;
-  const EntryLane L = entry_lane(m, B);
+  const EntryLane L = entry_lane<W>(m, B);
   const double gam = P.gamma;
   double pc[8], gc[15];
   if (L.e >= 0) {
@@ -2992,13 +3012,19 @@ __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const
   block_partials<1>(v, partials, sh, kMin, B);
 }
 
+// one wavefront per block: the owner lanes' sum waits only for its own
+// wavefront's lanes, not for the slowest of a larger block
+constexpr int kFluxWaves = 1;
+
 template <int SCHEME, bool SND, bool VISC>
-__global__ void __launch_bounds__(kBlock) k_flux_e(DMesh m, DParams P, const double *__restrict__ prim,
-                                                   const double *__restrict__ grad, const double *__restrict__ bfa,
-                                                   const double *__restrict__ q, double *__restrict__ rhs,
-                                                   int *__restrict__ nfirst, DtArgs dta, double *__restrict__ dt,
-                                                   double *__restrict__ partials, const double *__restrict__ gradv) {
-  d_flux_e<SCHEME, SND, VISC>(hw_blk(), m, P, prim, grad, bfa, q, rhs, nfirst, dta, dt, partials, gradv);
+__global__ void __launch_bounds__(64 * kFluxWaves) k_flux_e(DMesh m, DParams P, const double *__restrict__ prim,
+                                                            const double *__restrict__ grad,
+                                                            const double *__restrict__ bfa,
+                                                            const double *__restrict__ q, double *__restrict__ rhs,
+                                                            int *__restrict__ nfirst, DtArgs dta,
+                                                            double *__restrict__ dt, double *__restrict__ partials,
+                                                            const double *__restrict__ gradv) {
+  d_flux_e<SCHEME, SND, VISC, kFluxWaves>(hw_blk(), m, P, prim, grad, bfa, q, rhs, nfirst, dta, dt, partials, gradv);
 }
 
 // Face form of the inviscid flux pass: k_face_flux evaluates each internal
@@ -4620,6 +4646,7 @@ struct Ctx::Impl {
   void launch_grad(hipStream_t s);
   void launch_flux(hipStream_t s, const double *cfl_dev = nullptr);
   int flux_blocks() const;
+  void launch_dt_fin();
   bool flux_entry() const;
   // jac_follows: the Jacobian pass comes next (it forms the coupled
   // scalar row together with its blocks)
@@ -4967,7 +4994,9 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
   }
   d.counters.alloc(4);
   HIPCHECK(hipMemset(d.counters.p, 0, 4 * sizeof(int)));
-  d.partials.alloc(8 * (size_t)(nblk(5 * n) + 64));
+  // reduction partials: the largest grid that writes them (five per block
+  // of the NB-lanes-per-cell kernels, or one per flux wavefront)
+  d.partials.alloc(std::max<size_t>(8 * (size_t)(nblk(5 * n) + 64), (size_t)d.nwave + d.nwave / kMinPre + 64));
   d.scal.alloc(1);
   HIPCHECK(hipMemset(d.scal.p, 0, sizeof(SolverScalars)));
   HIPCHECK(hipHostMalloc((void **)&d.h_scal, sizeof(SolverScalars)));
@@ -5391,7 +5420,8 @@ void StepBatch::set(Ctx *const *ctxs, int n, void *stream) {
     const bool ent = d.par.grad_kernel == kResidEntry && !d.dp.visc;
     J.n[kShCell] = nblk(d.ncv);
     J.n[kShGrad] = !grad ? 0 : ent ? (d.nwave + kWaves - 1) / kWaves : nblk(d.ncv);
-    J.n[kShFlux] = d.flux_blocks();
+    // the batched flux kernel runs blocks of kWaves wavefronts
+    J.n[kShFlux] = d.flux_entry() ? (d.nwave + kWaves - 1) / kWaves : nblk(d.ncv);
     J.n[kShBc] = d.nfa_b > 0 ? nblk(d.nfa_b) : 0;
     J.n[kShFill] = d.dup_blocks ? (int)((J.nA + kBlock - 1) / kBlock) : 0;
     J.n[kShDtFill] = d.par.dt_mode == 1 ? nblk(d.ncv) : 0;
@@ -5709,7 +5739,24 @@ void Ctx::Impl::launch_grad(hipStream_t s) {
 // the per-cell flux kernel has no viscous variant
 bool Ctx::Impl::flux_entry() const { return par.flux_kernel != kResidCell || dp.visc; }
 
-int Ctx::Impl::flux_blocks() const { return flux_entry() ? (nwave + kWaves - 1) / kWaves : nblk(ncv); }
+// the local time step's minimum over the flux blocks' partials into
+// SolverScalars::dt_min; many partials (one per flux wavefront) are first
+// reduced in parallel
+void Ctx::Impl::launch_dt_fin() {
+  const int nb = flux_blocks();
+  if (nb > 4 * kMinPre) {
+    const int nb2 = (nb + kMinPre - 1) / kMinPre;
+    hipLaunchKernelGGL(k_min_pre, dim3(nb2), dim3(256), 0, stream, partials.p, nb, partials.p + nb);
+    hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, nb2, (int)kFinDt, partials.p + nb, scal.p);
+  } else {
+    hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, nb, (int)kFinDt, partials.p, scal.p);
+  }
+}
+
+int Ctx::Impl::flux_blocks() const {
+  if (face_form()) return (nwave + kWaves - 1) / kWaves;  // k_gather_e
+  return flux_entry() ? (nwave + kFluxWaves - 1) / kFluxWaves : nblk(ncv);
+}
 
 void Ctx::Impl::launch_flux(hipStream_t s, const double *cfl_dev) {
   DtArgs dta;
@@ -5724,8 +5771,8 @@ void Ctx::Impl::launch_flux(hipStream_t s, const double *cfl_dev) {
                        dt.p, partials.p);
   };
   auto goe = [&](auto kern) {
-    hipLaunchKernelGGL(kern, g, dim3(kBlock), 0, s, dm, dp, prim.p, grad.p, bfa.p, q.p, rhs.p, counters.p + 1, dta,
-                       dt.p, partials.p, gradv.p);
+    hipLaunchKernelGGL(kern, g, dim3(64 * kFluxWaves), 0, s, dm, dp, prim.p, grad.p, bfa.p, q.p, rhs.p,
+                       counters.p + 1, dta, dt.p, partials.p, gradv.p);
   };
   if (face_form()) {
     const dim3 gf(nblk(nfa - nfa_b));
@@ -5788,7 +5835,7 @@ void Ctx::Impl::launch_grad_flux(bool jac_follows) {
   launch_grad(stream);
   launch_flux(stream);
   // dt_min of calcDt; CFL_MIN sets every local dt to it
-  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, flux_blocks(), (int)kFinDt, partials.p, scal.p);
+  launch_dt_fin();
   if (par.dt_mode == 1) hipLaunchKernelGGL(k_dt_fill, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, dt.p, scal.p);
   if (coupled_sa()) {
     // the scalar row of the coupled system: its residual here, or with its
@@ -5895,8 +5942,8 @@ void Ctx::Impl::launch_jac(bool step_mode) {
   };
   const bool ent = jac_entry();
   auto goe = [&](auto kern) {
-    hipLaunchKernelGGL(kern, dim3((nwave + kWaves - 1) / kWaves), dim3(kBlock), 0, stream, dm, dp, prim.p, bfa.p, A,
-                       step_mode ? 1 : 0, dt.p, rhs.p, par.under_relax, v[0].p, v[1].p, gradv.p);
+    hipLaunchKernelGGL(kern, dim3((nwave + kJacWaves - 1) / kJacWaves), dim3(64 * kJacWaves), 0, stream, dm, dp,
+                       prim.p, bfa.p, A, step_mode ? 1 : 0, dt.p, rhs.p, par.under_relax, v[0].p, v[1].p, gradv.p);
   };
   if (coupled_sa()) {  // 6x6 blocks: the flow rows, then the scalar row (SA needs MU_REF > 0)
     if (ent) goe(k_jac_e<T, kHllcC, true, 6>);
@@ -5984,7 +6031,7 @@ void Ctx::Impl::enqueue_step_head() {
                      par.cfl_max);
   launch_grad(stream);
   launch_flux(stream, &scal.p->cfl_cur);
-  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, flux_blocks(), (int)kFinDt, partials.p, scal.p);
+  launch_dt_fin();
   if (par.dt_mode == 1) hipLaunchKernelGGL(k_dt_fill, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, dt.p, scal.p);
   launch_jac<T>(true);
   solve_prologue<T>();
