@@ -376,6 +376,7 @@ def main():
     ap.add_argument("--grad-kernel", default=None, help="GPU_GRAD_KERNEL: ENTRY or CELL")
     ap.add_argument("--flux-kernel", default=None, help="GPU_FLUX_KERNEL: ENTRY or CELL")
     ap.add_argument("--jac-kernel", default=None, help="GPU_JAC_KERNEL: ENTRY or CELL")
+    ap.add_argument("--grad-thread-order", default=None, help="GPU_GRAD_THREAD_ORDER: SORTED or NATIVE")
     ap.add_argument("--cell-order", default=None,
                     help="device cell numbering (GPU_CELL_ORDER): NATIVE, GROUPED, MORTON, MORTON_GROUPED")
     ap.add_argument("--serve", type=int, default=64, help="serving requests (0: skip the serving leg)")
@@ -442,6 +443,8 @@ def main():
             joe += f"GPU_FLUX_KERNEL = {args.flux_kernel}\n"
         if args.jac_kernel:
             joe += f"GPU_JAC_KERNEL = {args.jac_kernel}\n"
+        if args.grad_thread_order:
+            joe += f"GPU_GRAD_THREAD_ORDER = {args.grad_thread_order}\n"
         s = joemi.Solver(mesh, joe, joemi.JM_FP64)
         s.set_state(*state)
         for _ in range(args.warmup):

@@ -68,6 +68,8 @@ struct SolverParams {
   // Jacobi (the reference's), else ILU0: block ILU(0) on subdomains of this
   // many consecutive cells of the device order (GPU_ILU_SUBDOMAIN)
   int ilu_sub = 0;
+  // the per-cell gradient's lanes take a tile's cells by face count (GPU_GRAD_THREAD_ORDER)
+  bool grad_thread_sorted = false;
   // Spalart-Allmaras as one segregated scalar nuSA (TURB_MODEL = SA): initial
   // value (INITIAL_CONDITION_TURB) and UNDER_RELAXATION_SCALARS
   bool sa = false;
