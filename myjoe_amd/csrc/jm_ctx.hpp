@@ -70,6 +70,8 @@ struct SolverParams {
   int ilu_sub = 0;
   // the per-cell gradient's lanes take a tile's cells by face count (GPU_GRAD_THREAD_ORDER)
   bool grad_thread_sorted = false;
+  // the FP32 BiCGStab's residual by the recurrence s - omega t (GPU_KRYLOV_RESIDUAL = RECURRENCE)
+  bool krylov_recurrence = true;
   // Spalart-Allmaras as one segregated scalar nuSA (TURB_MODEL = SA): initial
   // value (INITIAL_CONDITION_TURB) and UNDER_RELAXATION_SCALARS
   bool sa = false;

@@ -108,6 +108,9 @@ int Server::submit(const Mesh *mesh, const std::string &joe_in, int precision, c
   parse_params(joe_in, *mesh, p, zbc);  // reject bad input at submission
   if (batched_ && p.sa) throw Unsupported("batched serving does not carry the Spalart-Allmaras scalar");
   if (batched_ && p.ilu_sub > 0) throw Unsupported("batched serving runs the block-Jacobi preconditioner");
+  if (batched_ && precision != 0 && !p.krylov_recurrence)
+    throw Unsupported("batched serving takes the FP32 BiCGStab residual by the recurrence "
+                      "(GPU_KRYLOV_RESIDUAL = RECURRENCE)");
   if (batched_ && p.mu_ref <= 0.0 && p.flux_kernel != kResidEntry)
     throw Unsupported("batched serving runs the per-entry flux kernel (GPU_FLUX_KERNEL = ENTRY)");
   auto r = std::make_shared<Request>();

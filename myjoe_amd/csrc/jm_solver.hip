@@ -3361,6 +3361,46 @@ __global__ void __launch_bounds__(kBlock) k_resid(int ncv, const int *__restrict
   }
 }
 
+// res = s - omega t: the residual by the BiCGStab recurrence instead of
+// rhs - A phi (no product with A), for the FP32 solve when asked
+// (GPU_KRYLOV_RESIDUAL = RECURRENCE); partials as k_resid's
+template <typename T, int NB>
+__global__ void __launch_bounds__(kBlock) k_resid_rec(int ncv, const T *__restrict__ LU, const T *__restrict__ s,
+                                                      const T *__restrict__ t, T *__restrict__ res,
+                                                      const T *__restrict__ res0, double *partials,
+                                                      const SolverScalars *S) {
+  constexpr int CPB = cells_per_block<NB>();
+  __shared__ double sh[32];
+  __shared__ T cellv[CPB][NB];
+  if (S->done || S->stop) return;
+  const int lc = threadIdx.x / NB, i = threadIdx.x % NB;
+  const int c = blockIdx.x * CPB + lc;
+  const bool active = (lc < CPB) && (c < ncv);
+  T r = (T)0.0;
+  if (active) {
+    const size_t o = NB * (size_t)c + i;
+    r = s[o] - (T)S->omega * t[o];
+    res[o] = r;
+    cellv[lc][i] = r;
+  }
+  __syncthreads();
+  double vs[1] = {0.0}, vm[1] = {0.0};
+  if (active) {
+    vs[0] = (double)r * (double)res0[NB * (size_t)c + i];
+    T tmp[NB];
+    for (int k = 0; k < NB; k++) tmp[k] = cellv[lc][k];
+    lusolv_n<T, NB>(LU + NB * NB * (size_t)c, tmp);
+    vm[0] = fabs((double)tmp[i]);
+  }
+  block_reduce<1>(vs, sh, kSum);
+  __syncthreads();
+  block_reduce<1>(vm, sh + 16, kMax);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = vs[0];
+    partials[gridDim.x + blockIdx.x] = vm[0];
+  }
+}
+
 // p = res - beta (p - omega v); phat = LU^-1 p   (one thread per cell). With
 // BJ = false only p: the ILU solve forms phat after it.
 template <typename T, int NB, bool BJ = true>
@@ -3601,6 +3641,9 @@ struct BcgSys {
   // [nnz][NB][NB] and Dinv [ncv][NB][NB]; 0 = block Jacobi
   int ilu_sub = 0;
   void *F = nullptr, *Dinv = nullptr;
+  // the residual of each iteration by the recurrence s - omega t instead of
+  // rhs - A phi (FP32 solves only)
+  bool rec_resid = false;
 };
 
 // start of a solve (UgpWithCv.cpp:2660-2735): p = v = 0, block LU of the
@@ -3663,8 +3706,12 @@ void bcg_iterations(const BcgSys &y, int count) {
                        y.partials, y.S);
     hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, st, gb, (int)kFinOmega, y.partials, y.S);
     hipLaunchKernelGGL(k_x<T>, dim3(nblk(n)), dim3(kBlock), 0, st, n, phi, phat, shat, y.S);
-    hipLaunchKernelGGL((k_resid<T, NB>), dim3(gb), dim3(kBlock), 0, st, ncv, y.nbocv_i, y.nbocv_v, A, LU, phi, rhs_,
-                       res, res0, 0, y.partials, y.S);
+    if (y.rec_resid)
+      hipLaunchKernelGGL((k_resid_rec<T, NB>), dim3(gb), dim3(kBlock), 0, st, ncv, LU, s, t, res, res0, y.partials,
+                         y.S);
+    else
+      hipLaunchKernelGGL((k_resid<T, NB>), dim3(gb), dim3(kBlock), 0, st, ncv, y.nbocv_i, y.nbocv_v, A, LU, phi,
+                         rhs_, res, res0, 0, y.partials, y.S);
     hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, st, gb, (int)kFinResid, y.partials, y.S);
   }
 }
@@ -3718,7 +3765,9 @@ __global__ void __launch_bounds__(kBlock) kb_lu(const KrylovJob *__restrict__ jo
   }
 }
 
-template <typename T>
+// REC: the residual by the recurrence s - omega t, as k_resid_rec (the FP32
+// solves; omega is stored by kb_x)
+template <typename T, bool REC = false>
 __global__ void __launch_bounds__(kBlock) kb_resid(const KrylovJob *__restrict__ jobs, const int *__restrict__ map5,
                                                    int init, double *partials) {
   __shared__ double sh[32];
@@ -3732,7 +3781,12 @@ __global__ void __launch_bounds__(kBlock) kb_resid(const KrylovJob *__restrict__
   const int c = (blockIdx.x - J.b5) * kCellsPerBlock + lc;
   const bool active = (lc < kCellsPerBlock) && (c < J.ncv);
   T r = (T)0.0;
-  if (active) {
+  if (active && REC && !init) {
+    const size_t o = 5 * (size_t)c + i;
+    r = ((const T *)J.v[7])[o] - (T)S->omega * ((const T *)J.v[9])[o];
+    res[o] = r;
+    cellv[lc][i] = r;
+  } else if (active) {
     T ax = spmv_row<T, 5>(J.nbocv_i, J.nbocv_v, A, phi, c, i);
     r = rhs[5 * (size_t)c + i] - ax;
     res[5 * (size_t)c + i] = r;
@@ -4363,6 +4417,12 @@ void parse_params(const std::string &joe_in, const Mesh &m, SolverParams &p, std
     if (o != "SORTED" && o != "NATIVE")
       throw std::invalid_argument("GPU_GRAD_THREAD_ORDER = " + o + " (SORTED, NATIVE)");
     p.grad_thread_sorted = o == "SORTED";
+  }
+  {  // the FP32 BiCGStab's residual: RECURRENCE (s - omega t) or TRUE (rhs - A phi, as the reference);
+     // FP64 solves always take the reference's
+    const std::string o = pm.str("GPU_KRYLOV_RESIDUAL", "RECURRENCE");
+    if (o != "TRUE" && o != "RECURRENCE") throw std::invalid_argument("GPU_KRYLOV_RESIDUAL = " + o + " (TRUE, RECURRENCE)");
+    p.krylov_recurrence = o == "RECURRENCE";
   }
   {  // this build's preconditioner option (the reference's BCGSTAB is block Jacobi)
     const std::string pc = pm.str("GPU_PRECONDITIONER", "BLOCK_JACOBI");
@@ -5333,7 +5393,11 @@ static void krylov_batch_iterations(hipStream_t st, const KrylovBatchBuf &buf, i
     hipLaunchKernelGGL(kb_ps<T>, dim3(g1), dim3(kBlock), 0, st, jd, m1, 1, pt, g5);
     hipLaunchKernelGGL(kb_spmv_dot<T>, dim3(g5), dim3(kBlock), 0, st, jd, m5, 1, pt);
     hipLaunchKernelGGL(kb_x<T>, dim3(gx), dim3(kBlock), 0, st, jd, mx, pt, g5);
-    hipLaunchKernelGGL(kb_resid<T>, dim3(g5), dim3(kBlock), 0, st, jd, m5, 0, pt);
+    // FP32 solves take the residual by the recurrence (GPU_KRYLOV_RESIDUAL = RECURRENCE, the server's rule)
+    if (std::is_same<T, float>::value)
+      hipLaunchKernelGGL((kb_resid<T, true>), dim3(g5), dim3(kBlock), 0, st, jd, m5, 0, pt);
+    else
+      hipLaunchKernelGGL(kb_resid<T>, dim3(g5), dim3(kBlock), 0, st, jd, m5, 0, pt);
     hipLaunchKernelGGL(kb_fin, dim3(nj), dim3(kFinBlock), 0, st, jd, (int)kFinResid, pt, g5);
   }
 }
@@ -6023,6 +6087,7 @@ BcgSys Ctx::Impl::bcg_sys() {
   y.partials = partials.p;
   y.S = scal.p;
   y.stream = stream;
+  y.rec_resid = par.krylov_recurrence && !std::is_same<T, double>::value;
   if (par.ilu_sub > 0) {
     y.ilu_sub = par.ilu_sub;
     if constexpr (std::is_same<T, double>::value) {
