@@ -2252,202 +2252,227 @@ __global__ void __launch_bounds__(kBlock) k_scal(DMesh m, DParams P, const doubl
 
 // ---------------------------------------------------------------------------
 // The transported scalar's row of the coupled system (runBackwardEulerCoupled
-// with Spalart-Allmaras, 6x6 blocks), one thread per cell, gathered in the
-// reference's order: per face of calcFluxCoupled (JoeWithModels.cpp
-// :4797-5796) the HLLC scalar flux (calcEulerFluxCoupled_HLLC, scalars
-// upwinded by sM, side enthalpies after reconstruction) and its Jacobian row
-// at cell-centre states (:5035-5048, 5150-5740), then the same face's scalar
-// diffusion with the SA diffusivity and its row entries [5][0], [5][5]
-// (calcViscousFluxCoupled, :5076-5105, 5373-5404, 5548-5575, 5744-5775;
-// diffusivityHookScalarRansTurb, TurbModel_SA.h:130-172); then the source
-// (sourceHookRansTurbCoupled, TurbModel_SA.h:359-434). Writes rhs_s and, with
-// A, row 5 of every block of the cell's row (A [nnz][6][6] in precision T);
-// in step mode also V/dt on [5][5], the relaxed right-hand side and a zero
-// initial guess of entry 5 (:4455-4473).
+// with Spalart-Allmaras, 6x6 blocks), in the reference's order: per face of
+// calcFluxCoupled (JoeWithModels.cpp :4797-5796) the HLLC scalar flux
+// (calcEulerFluxCoupled_HLLC, scalars upwinded by sM, side enthalpies after
+// reconstruction) and its Jacobian row at cell-centre states (:5035-5048,
+// 5150-5740), then the same face's scalar diffusion with the SA diffusivity
+// and its row entries [5][0], [5][5] (calcViscousFluxCoupled, :5076-5105,
+// 5373-5404, 5548-5575, 5744-5775; diffusivityHookScalarRansTurb,
+// TurbModel_SA.h:130-172); then the source (sourceHookRansTurbCoupled,
+// TurbModel_SA.h:359-434).
 // ---------------------------------------------------------------------------
-template <typename T, bool SND>
-__global__ void __launch_bounds__(kBlock) k_scal_c(DMesh m, DParams P, const double *__restrict__ prim,
-                                                   const double *__restrict__ grad, const double *__restrict__ bfa,
-                                                   const double *__restrict__ gradv, double *__restrict__ rhs_s,
-                                                   T *__restrict__ A, int step_mode, const double *__restrict__ dt,
-                                                   double relax, T *__restrict__ rhs_ls, T *__restrict__ x0) {
-  const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (t >= m.ncv) return;
-  const int c = t;
-  const double gam = P.gamma;
-  const double *pc = prim + 8 * (size_t)c;
-  const double *gc = grad + 16 * (size_t)c;
-  const double *phi = P.sa_phi;
-  const double *auxc = P.sa_aux + kAuxN * (size_t)c;
-  const double *gpc = auxc + kAuxGrad;
-  const double ph = phi[c];
-  const double invSigma = 1.0 / kSaSigma;
+// The cell's own values every entry of the scalar row reads.
+struct ScalOwn {
+  const double *pc, *gc, *gpc;
+  double ph, hown, Tc;
   Side own;
-  own.rho = pc[0]; own.u[0] = pc[1]; own.u[1] = pc[2]; own.u[2] = pc[3]; own.p = pc[4];
-  const double hown = enthalpy_of(P, own.rho, own.p);
-  const double Tc = pc[4] / (pc[0] * P.R);
-  double acc = 0.0;
-  double D[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+};
+
+__device__ __forceinline__ ScalOwn scal_own(const DParams &P, const double *__restrict__ prim,
+                                            const double *__restrict__ grad, int c) {
+  ScalOwn o;
+  o.pc = prim + 8 * (size_t)c;
+  o.gc = grad + 16 * (size_t)c;
+  o.gpc = P.sa_aux + kAuxN * (size_t)c + kAuxGrad;
+  o.ph = P.sa_phi[c];
+  o.own.rho = o.pc[0]; o.own.u[0] = o.pc[1]; o.own.u[1] = o.pc[2]; o.own.u[2] = o.pc[3]; o.own.p = o.pc[4];
+  o.hown = enthalpy_of(P, o.own.rho, o.own.p);
+  o.Tc = o.pc[4] / (o.pc[0] * P.R);
+  return o;
+}
+
+// One entry (face) of the scalar row: what it adds to the cell's residual,
+// the Euler term then the diffusive one (out[0], out[1]), and to row 5 of
+// the cell's diagonal block, from the Euler terms (out[2..7]) then from the
+// diffusion (out[8] to [5][0], out[9] to [5][5]); -0.0 where the entry adds
+// nothing (x + -0.0 is x, signed zeros included), and subtractions stored
+// negated (a - b is a + (-b)). With A, row 5 of the entry's off-diagonal
+// block is written (internal faces).
+template <typename T, bool SND>
+__device__ __forceinline__ void scal_c_entry(const DMesh &m, const DParams &P, const double *__restrict__ prim,
+                                             const double *__restrict__ grad, const double *__restrict__ bfa,
+                                             T *__restrict__ A, const ScalOwn &o, int e, double out[10]) {
+  const double gam = P.gamma;
+  const double invSigma = 1.0 / kSaSigma;
+  const double *pc = o.pc, *gc = o.gc, *gpc = o.gpc;
+  const double ph = o.ph;
+  const Side &own = o.own;
+  const double *phi = P.sa_phi;
   double S[6], R6[6], F[5], Fs, a0[2], a1[2];
-  const int e0 = m.cfl_i[c], e1 = m.cfl_i[c + 1];
-  for (int e = e0; e < e1; e++) {
-    const int2 en = m.cfl[e];
-    const int f = en.x & 0x7fffffff;
-    const double *geo = m.fgeo + 8 * (size_t)f;
-    const double n[3] = {geo[0], geo[1], geo[2]};
-    const double area = geo[3];
-    const double xfa[3] = {geo[4], geo[5], geo[6]};
-    if (en.y >= 0) {
-      const int nb = en.y;
-      const double *pn = prim + 8 * (size_t)nb;
-      const double phn = phi[nb];
-      const double *gpn = P.sa_aux + kAuxN * (size_t)nb + kAuxGrad;
-      Side nbs;
-      nbs.rho = pn[0]; nbs.u[0] = pn[1]; nbs.u[1] = pn[2]; nbs.u[2] = pn[3]; nbs.p = pn[4];
-      Side so = own, sn = nbs;
-      double sco = ph, scn = phn;
-      if (SND) {
-        if (reconstruct(so, pc, gc, xfa)) {
-          const double r[3] = {xfa[0] - pc[5], xfa[1] - pc[6], xfa[2] - pc[7]};
-          sco = sco + dot3(r, gpc);
-        }
-        if (reconstruct(sn, pn, grad + 16 * (size_t)nb, xfa)) {
-          const double r[3] = {xfa[0] - pn[5], xfa[1] - pn[6], xfa[2] - pn[7]};
-          scn = scn + dot3(r, gpn);
+  for (int k = 0; k < 10; k++) out[k] = -0.0;
+  const int2 en = m.cfl[e];
+  const int f = en.x & 0x7fffffff;
+  const double *geo = m.fgeo + 8 * (size_t)f;
+  const double n[3] = {geo[0], geo[1], geo[2]};
+  const double area = geo[3];
+  const double xfa[3] = {geo[4], geo[5], geo[6]};
+  if (en.y >= 0) {
+    const int nb = en.y;
+    const double *pn = prim + 8 * (size_t)nb;
+    const double phn = phi[nb];
+    const double *gpn = P.sa_aux + kAuxN * (size_t)nb + kAuxGrad;
+    Side nbs;
+    nbs.rho = pn[0]; nbs.u[0] = pn[1]; nbs.u[1] = pn[2]; nbs.u[2] = pn[3]; nbs.p = pn[4];
+    Side so = own, sn = nbs;
+    double sco = ph, scn = phn;
+    if (SND) {
+      if (reconstruct(so, pc, gc, xfa)) {
+        const double r[3] = {xfa[0] - pc[5], xfa[1] - pc[6], xfa[2] - pc[7]};
+        sco = sco + dot3(r, gpc);
+      }
+      if (reconstruct(sn, pn, grad + 16 * (size_t)nb, xfa)) {
+        const double r[3] = {xfa[0] - pn[5], xfa[1] - pn[6], xfa[2] - pn[7]};
+        scn = scn + dot3(r, gpn);
+      }
+    }
+    const bool s0 = en.x >= 0;
+    const int noc = m.cfl_noc[e];
+    const bool accum = (noc >> 30) & 1;
+    T *off = A ? A + 36 * (size_t)(noc & 0x3fffffff) + 30 : nullptr;
+    T ov[6];  // row 5 of the off-diagonal block, kept until its one store
+    // the face's (icv0, icv1) sides; the flux and both rows evaluated once,
+    // routed by this cell's side
+    const Side fl = s0 ? so : sn, fr = s0 ? sn : so;
+    const double phl = s0 ? sco : scn, phr = s0 ? scn : sco;
+    flux_hllc_coupled(F, fl.rho, fl.u, fl.p, enthalpy_of(P, fl.rho, fl.p), fr.rho, fr.u, fr.p,
+                      enthalpy_of(P, fr.rho, fr.p), gam, area, n, phl, phr, &Fs);
+    out[0] = s0 ? -Fs : Fs;
+    if (A) {
+      const Side jl = s0 ? own : nbs, jr = s0 ? nbs : own;
+      const double hjl = enthalpy_of(P, jl.rho, jl.p), hjr = enthalpy_of(P, jr.rho, jr.p);
+      const double pjl = s0 ? ph : phn, pjr = s0 ? phn : ph;
+      jac_hllc_coupled_srow(S, 0, jl.rho, jl.u, jl.p, hjl, pjl, jr.rho, jr.u, jr.p, hjr, pjr, gam, area, n);
+      jac_hllc_coupled_srow(R6, 1, jl.rho, jl.u, jl.p, hjl, pjl, jr.rho, jr.u, jr.p, hjr, pjr, gam, area, n);
+      // icv0: A[noc00] += SL, A[noc01] += SR; icv1: A[noc11] -= SR, A[noc10] -= SL
+      for (int k = 0; k < 6; k++) {
+        if (s0) {
+          out[2 + k] = S[k];
+          ov[k] = (T)(accum ? (double)off[k] + R6[k] : 0.0 + R6[k]);
+        } else {
+          out[2 + k] = -R6[k];
+          ov[k] = (T)(accum ? (double)off[k] - S[k] : 0.0 - S[k]);
         }
       }
-      const bool s0 = en.x >= 0;
-      const int noc = m.cfl_noc[e];
-      const bool accum = (noc >> 30) & 1;
-      T *off = A ? A + 36 * (size_t)(noc & 0x3fffffff) + 30 : nullptr;
-      // the face's (icv0, icv1) sides; the flux and both rows evaluated once,
-      // routed by this cell's side (acc - x == acc + (-x))
-      const Side fl = s0 ? so : sn, fr = s0 ? sn : so;
-      const double phl = s0 ? sco : scn, phr = s0 ? scn : sco;
-      flux_hllc_coupled(F, fl.rho, fl.u, fl.p, enthalpy_of(P, fl.rho, fl.p), fr.rho, fr.u, fr.p,
-                        enthalpy_of(P, fr.rho, fr.p), gam, area, n, phl, phr, &Fs);
-      acc += s0 ? -Fs : Fs;
+    }
+    {  // the face's scalar diffusion (:5076-5105)
+      const double Tn = pn[4] / (pn[0] * P.R);
+      double sV[3], smag, mul, w0, w1;
+      visc_internal_geom(P, s0 ? pc + 5 : pn + 5, s0 ? pn + 5 : pc + 5, xfa, s0 ? o.Tc : Tn, s0 ? Tn : o.Tc, sV,
+                         smag, mul, w0, w1);
+      const double rho0 = s0 ? pc[0] : pn[0], rho1 = s0 ? pn[0] : pc[0];
+      const double nu0 = s0 ? ph : phn, nu1 = s0 ? phn : ph;
+      const double *g0 = s0 ? gpc : gpn, *g1 = s0 ? gpn : gpc;
+      const double rho_fa = (w1 * rho0 + w0 * rho1) / (w0 + w1);
+      const double nuSA_fa = (w1 * nu0 + w0 * nu1) / (w0 + w1);
+      const double diff = invSigma * (mul + rho_fa * nuSA_fa);
+      const double VFs =
+          visc_coupled_srow(a0, a1, rho0, nu0, g0, rho1, nu1, g1, diff, 1.0, area, n, smag, sV, dot3(n, sV));
+      out[1] = s0 ? -VFs : VFs;
+      if (A) {  // icv0: [noc00] += a0, [noc01] += a1; icv1: [noc11] -= a1, [noc10] -= a0
+        if (s0) {
+          out[8] = a0[0];
+          out[9] = a0[1];
+          ov[0] = (T)((double)ov[0] + a1[0]);
+          ov[5] = (T)((double)ov[5] + a1[1]);
+        } else {
+          out[8] = -a1[0];
+          out[9] = -a1[1];
+          ov[0] = (T)((double)ov[0] - a0[0]);
+          ov[5] = (T)((double)ov[5] - a0[1]);
+        }
+        for (int k = 0; k < 6; k++) off[k] = ov[k];
+      }
+    }
+  } else {
+    const ZoneDev &z = m.zones[-en.y - 1];
+    const double *bb = bfa + 8 * (size_t)f;
+    const double rb = bb[0], pb = bb[4], hb = bb[6], phb = bb[7];
+    const double ub[3] = {bb[1], bb[2], bb[3]};
+    // a zone without a nuSA value carries no diffusive scalar flux
+    // (DiffTerm = 0, :5453-5461, 5599-5607); NEUMANN zones keep it (:5293-5296)
+    const double DiffTerm = (z.bc == kBcNeumann || z.skind) ? 1.0 : 0.0;
+    const double mul = mu_lam(P, bb[5]);
+    const double diff = z.bc == kBcWall ? invSigma * mul : invSigma * (mul + rb * phb);
+    if (z.bc == kBcSymmetry || z.bc == kBcWall || z.bc == kBcNeumann) {
+      const bool op = z.bc != kBcNeumann && P.symm_bc == kSymmAnalytical;  // ONLY_PRESSURE (:1910-1917)
+      if (op) {
+        Fs = 0.0 * area;
+      } else {
+        flux_hllc_coupled(F, rb, ub, pb, hb, rb, ub, pb, hb, gam, area, n, phb, phb, &Fs);
+      }
+      out[0] = -Fs;
       if (A) {
-        const Side jl = s0 ? own : nbs, jr = s0 ? nbs : own;
-        const double hjl = enthalpy_of(P, jl.rho, jl.p), hjr = enthalpy_of(P, jr.rho, jr.p);
-        const double pjl = s0 ? ph : phn, pjr = s0 ? phn : ph;
-        jac_hllc_coupled_srow(S, 0, jl.rho, jl.u, jl.p, hjl, pjl, jr.rho, jr.u, jr.p, hjr, pjr, gam, area, n);
-        jac_hllc_coupled_srow(R6, 1, jl.rho, jl.u, jl.p, hjl, pjl, jr.rho, jr.u, jr.p, hjr, pjr, gam, area, n);
-        // icv0: A[noc00] += SL, A[noc01] += SR; icv1: A[noc11] -= SR, A[noc10] -= SL
-        for (int k = 0; k < 6; k++) {
-          if (s0) {
-            D[k] += S[k];
-            off[k] = (T)(accum ? (double)off[k] + R6[k] : 0.0 + R6[k]);
+        if (op) {  // a zero row through dU/dQ
+          for (int k = 0; k < 6; k++) S[k] = 0.0 * 1.0;
+          S[0] -= phb / rb * S[5];
+          S[5] /= rb;
+          for (int k = 0; k < 6; k++) out[2 + k] = S[k];
+        } else {
+          jac_hllc_coupled_srow(S, 0, rb, ub, pb, hb, phb, rb, ub, pb, hb, phb, gam, area, n);
+          if (P.symm_bc == kSymmNew) {
+            jac_hllc_coupled_srow(R6, 1, rb, ub, pb, hb, phb, rb, ub, pb, hb, phb, gam, area, n);
+            for (int k = 0; k < 6; k++) out[2 + k] = S[k] + R6[k];
           } else {
-            D[k] -= R6[k];
-            off[k] = (T)(accum ? (double)off[k] - S[k] : 0.0 - S[k]);
+            for (int k = 0; k < 6; k++) out[2 + k] = S[k];
           }
         }
       }
-      {  // the face's scalar diffusion (:5076-5105)
-        const double Tn = pn[4] / (pn[0] * P.R);
-        double sV[3], smag, mul, w0, w1;
-        visc_internal_geom(P, s0 ? pc + 5 : pn + 5, s0 ? pn + 5 : pc + 5, xfa, s0 ? Tc : Tn, s0 ? Tn : Tc, sV, smag,
-                           mul, w0, w1);
-        const double rho0 = s0 ? pc[0] : pn[0], rho1 = s0 ? pn[0] : pc[0];
-        const double nu0 = s0 ? ph : phn, nu1 = s0 ? phn : ph;
-        const double *g0 = s0 ? gpc : gpn, *g1 = s0 ? gpn : gpc;
-        const double rho_fa = (w1 * rho0 + w0 * rho1) / (w0 + w1);
-        const double nuSA_fa = (w1 * nu0 + w0 * nu1) / (w0 + w1);
-        const double diff = invSigma * (mul + rho_fa * nuSA_fa);
-        const double VFs =
-            visc_coupled_srow(a0, a1, rho0, nu0, g0, rho1, nu1, g1, diff, 1.0, area, n, smag, sV, dot3(n, sV));
-        acc += s0 ? -VFs : VFs;
-        if (A) {  // icv0: [noc00] += a0, [noc01] += a1; icv1: [noc11] -= a1, [noc10] -= a0
-          if (s0) {
-            D[0] += a0[0];
-            D[5] += a0[1];
-            off[0] = (T)((double)off[0] + a1[0]);
-            off[5] = (T)((double)off[5] + a1[1]);
-          } else {
-            D[0] -= a1[0];
-            D[5] -= a1[1];
-            off[0] = (T)((double)off[0] - a0[0]);
-            off[5] = (T)((double)off[5] - a0[1]);
-          }
+      if (z.bc != kBcSymmetry) {  // wall (:5548-5575) and NEUMANN (:5373-5404) diffusion along the normal
+        const double sV[3] = {xfa[0] - pc[5], xfa[1] - pc[6], xfa[2] - pc[7]};
+        const double smag = fabs(dot3(sV, n));
+        const bool neu = z.bc == kBcNeumann;
+        // the gradient by value (zero on NEUMANN faces): a pointer chosen
+        // between a local array and global memory puts the array in scratch
+        const double gv[3] = {neu ? 0.0 : gpc[0], neu ? 0.0 : gpc[1], neu ? 0.0 : gpc[2]};
+        const double VFs = visc_coupled_srow(a0, a1, neu ? rb : pc[0], neu ? phb : ph, gv, rb, phb, gv, diff,
+                                             DiffTerm, area, n, smag, n, 1.0);
+        out[1] = -VFs;
+        if (A) {
+          out[8] = a0[0];
+          out[9] = a0[1];
         }
       }
     } else {
-      const ZoneDev &z = m.zones[-en.y - 1];
-      const double *bb = bfa + 8 * (size_t)f;
-      const double rb = bb[0], pb = bb[4], hb = bb[6], phb = bb[7];
-      const double ub[3] = {bb[1], bb[2], bb[3]};
-      // a zone without a nuSA value carries no diffusive scalar flux
-      // (DiffTerm = 0, :5453-5461, 5599-5607); NEUMANN zones keep it (:5293-5296)
-      const double DiffTerm = (z.bc == kBcNeumann || z.skind) ? 1.0 : 0.0;
-      const double mul = mu_lam(P, bb[5]);
-      const double diff = z.bc == kBcWall ? invSigma * mul : invSigma * (mul + rb * phb);
-      if (z.bc == kBcSymmetry || z.bc == kBcWall || z.bc == kBcNeumann) {
-        const bool op = z.bc != kBcNeumann && P.symm_bc == kSymmAnalytical;  // ONLY_PRESSURE (:1910-1917)
-        if (op) {
-          Fs = 0.0 * area;
-        } else {
-          flux_hllc_coupled(F, rb, ub, pb, hb, rb, ub, pb, hb, gam, area, n, phb, phb, &Fs);
-        }
-        acc += -Fs;
+      Side so = own;
+      double sco = ph;
+      if (SND && reconstruct(so, pc, gc, xfa)) {
+        const double r[3] = {xfa[0] - pc[5], xfa[1] - pc[6], xfa[2] - pc[7]};
+        sco = sco + dot3(r, gpc);
+      }
+      flux_hllc_coupled(F, so.rho, so.u, so.p, enthalpy_of(P, so.rho, so.p), rb, ub, pb, hb, gam, area, n, sco, phb,
+                        &Fs);
+      out[0] = -Fs;
+      if (A) {  // cell-centre state against the boundary state (:5731-5740)
+        jac_hllc_coupled_srow(S, 0, own.rho, own.u, own.p, o.hown, ph, rb, ub, pb, hb, phb, gam, area, n);
+        for (int k = 0; k < 6; k++) out[2 + k] = S[k];
+      }
+      {  // the cell against the boundary state; its side-1 scalar is the
+         // reconstructed face value (:5744-5775)
+        double sArg[3], smag, alpha;
+        visc_coupled_bgeom(z.bc, xfa, pc + 5, n, sArg, smag, alpha);
+        const double VFs =
+            visc_coupled_srow(a0, a1, pc[0], ph, gpc, rb, sco, gpc, diff, DiffTerm, area, n, smag, sArg, alpha);
+        out[1] = -VFs;
         if (A) {
-          if (op) {  // a zero row through dU/dQ
-            for (int k = 0; k < 6; k++) S[k] = 0.0 * 1.0;
-            S[0] -= phb / rb * S[5];
-            S[5] /= rb;
-            for (int k = 0; k < 6; k++) D[k] += S[k];
-          } else {
-            jac_hllc_coupled_srow(S, 0, rb, ub, pb, hb, phb, rb, ub, pb, hb, phb, gam, area, n);
-            if (P.symm_bc == kSymmNew) {
-              jac_hllc_coupled_srow(R6, 1, rb, ub, pb, hb, phb, rb, ub, pb, hb, phb, gam, area, n);
-              for (int k = 0; k < 6; k++) D[k] += S[k] + R6[k];
-            } else {
-              for (int k = 0; k < 6; k++) D[k] += S[k];
-            }
-          }
-        }
-        if (z.bc != kBcSymmetry) {  // wall (:5548-5575) and NEUMANN (:5373-5404) diffusion along the normal
-          const double sV[3] = {xfa[0] - pc[5], xfa[1] - pc[6], xfa[2] - pc[7]};
-          const double smag = fabs(dot3(sV, n));
-          const double zero3[3] = {0.0, 0.0, 0.0};
-          const bool neu = z.bc == kBcNeumann;
-          const double VFs = visc_coupled_srow(a0, a1, neu ? rb : pc[0], neu ? phb : ph, neu ? zero3 : gpc, rb, phb,
-                                               neu ? zero3 : gpc, diff, DiffTerm, area, n, smag, n, 1.0);
-          acc += -VFs;
-          if (A) {
-            D[0] += a0[0];
-            D[5] += a0[1];
-          }
-        }
-      } else {
-        Side so = own;
-        double sco = ph;
-        if (SND && reconstruct(so, pc, gc, xfa)) {
-          const double r[3] = {xfa[0] - pc[5], xfa[1] - pc[6], xfa[2] - pc[7]};
-          sco = sco + dot3(r, gpc);
-        }
-        flux_hllc_coupled(F, so.rho, so.u, so.p, enthalpy_of(P, so.rho, so.p), rb, ub, pb, hb, gam, area, n, sco, phb,
-                          &Fs);
-        acc += -Fs;
-        if (A) {  // cell-centre state against the boundary state (:5731-5740)
-          jac_hllc_coupled_srow(S, 0, own.rho, own.u, own.p, hown, ph, rb, ub, pb, hb, phb, gam, area, n);
-          for (int k = 0; k < 6; k++) D[k] += S[k];
-        }
-        {  // the cell against the boundary state; its side-1 scalar is the
-           // reconstructed face value (:5744-5775)
-          double sArg[3], smag, alpha;
-          visc_coupled_bgeom(z.bc, xfa, pc + 5, n, sArg, smag, alpha);
-          const double VFs =
-              visc_coupled_srow(a0, a1, pc[0], ph, gpc, rb, sco, gpc, diff, DiffTerm, area, n, smag, sArg, alpha);
-          acc += -VFs;
-          if (A) {
-            D[0] += a0[0];
-            D[5] += a0[1];
-          }
+          out[8] = a0[0];
+          out[9] = a0[1];
         }
       }
     }
   }
+}
+
+// The cell's source (sourceHookRansTurbCoupled, TurbModel_SA.h:359-434) after
+// its faces' terms acc and D, then the step-mode terms and the stores.
+template <typename T>
+__device__ __forceinline__ void scal_c_finish(const DMesh &m, const DParams &P, const ScalOwn &o, int c, double acc,
+                                              double (&D)[6], double *__restrict__ rhs_s, T *__restrict__ A,
+                                              int step_mode, const double *__restrict__ dt, double relax,
+                                              T *__restrict__ rhs_ls, T *__restrict__ x0) {
+  const double *pc = o.pc, *gpc = o.gpc;
+  const double ph = o.ph;
+  const double *auxc = P.sa_aux + kAuxN * (size_t)c;
   // source and its diagonal entry (TurbModel_SA.h:359-434)
   {
     const double rho = pc[0], vol = m.vol[c];
@@ -2486,6 +2511,79 @@ __global__ void __launch_bounds__(kBlock) k_scal_c(DMesh m, DParams P, const dou
     T *dg = A + 36 * (size_t)m.nbocv_i[c] + 30;
     for (int k = 0; k < 6; k++) dg[k] = (T)D[k];
   }
+}
+
+// ---------------------------------------------------------------------------
+// The nuSA row of the coupled system (runBackwardEulerCoupled with
+// Spalart-Allmaras, 6x6 blocks), gathered in the reference's order (see
+// scal_c_entry for the face terms, scal_c_finish for the source). Writes
+// rhs_s and, with A, row 5 of every block of the cell's row (A [nnz][6][6]
+// in precision T); in step mode also V/dt on [5][5], the relaxed right-hand
+// side and a zero initial guess of entry 5 (:4455-4473). k_scal_c: one
+// thread per cell; k_scal_e: one lane per cell-face entry, the cell's first
+// lane summing its entries' terms from LDS in entry order (the same sums).
+// ---------------------------------------------------------------------------
+template <typename T, bool SND>
+__global__ void __launch_bounds__(kBlock) k_scal_c(DMesh m, DParams P, const double *__restrict__ prim,
+                                                   const double *__restrict__ grad, const double *__restrict__ bfa,
+                                                   const double *__restrict__ gradv, double *__restrict__ rhs_s,
+                                                   T *__restrict__ A, int step_mode, const double *__restrict__ dt,
+                                                   double relax, T *__restrict__ rhs_ls, T *__restrict__ x0) {
+  const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (t >= m.ncv) return;
+  const int c = t;
+  const ScalOwn o = scal_own(P, prim, grad, c);
+  double acc = 0.0;
+  double D[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int e = m.cfl_i[c]; e < m.cfl_i[c + 1]; e++) {
+    double t10[10];
+    scal_c_entry<T, SND>(m, P, prim, grad, bfa, A, o, e, t10);
+    acc += t10[0];
+    acc += t10[1];
+    for (int k = 0; k < 6; k++) D[k] += t10[2 + k];
+    D[0] += t10[8];
+    D[5] += t10[9];
+  }
+  scal_c_finish<T>(m, P, o, c, acc, D, rhs_s, A, step_mode, dt, relax, rhs_ls, x0);
+}
+
+constexpr int kScalWaves = 1;
+
+template <typename T, bool SND>
+__global__ void __launch_bounds__(64 * kScalWaves) k_scal_e(DMesh m, DParams P, const double *__restrict__ prim,
+                                                            const double *__restrict__ grad,
+                                                            const double *__restrict__ bfa,
+                                                            const double *__restrict__ gradv,
+                                                            double *__restrict__ rhs_s, T *__restrict__ A,
+                                                            int step_mode, const double *__restrict__ dt,
+                                                            double relax, T *__restrict__ rhs_ls,
+                                                            T *__restrict__ x0) {
+  constexpr int NS = 10;
+  __shared__ double lds[64 * kScalWaves * NS];
+  double *slot = lds + NS * threadIdx.x;
+  const EntryLane L = entry_lane<kScalWaves>(m, hw_blk());
+  ScalOwn o;
+  if (L.e >= 0) {
+    o = scal_own(P, prim, grad, L.c);
+    double t10[NS];
+    scal_c_entry<T, SND>(m, P, prim, grad, bfa, A, o, L.e, t10);
+    for (int k = 0; k < NS; k++) slot[k] = t10[k];
+  }
+  __syncthreads();
+  if (L.e < 0 || L.pos != 0) return;
+  const int c = L.c;
+  const int nent = m.cfl_i[c + 1] - m.cfl_i[c];
+  double acc = 0.0;
+  double D[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int j = 0; j < nent; j++) {
+    const double *s = slot + NS * j;
+    acc += s[0];
+    acc += s[1];
+    for (int k = 0; k < 6; k++) D[k] += s[2 + k];
+    D[0] += s[8];
+    D[5] += s[9];
+  }
+  scal_c_finish<T>(m, P, o, c, acc, D, rhs_s, A, step_mode, dt, relax, rhs_ls, x0);
 }
 
 // ---------------------------------------------------------------------------
@@ -5949,8 +6047,21 @@ void Ctx::Impl::launch_scal_c(T *A, bool step_mode) {
     hipLaunchKernelGGL(kern, dim3(nblk(ncv)), dim3(kBlock), 0, stream, dm, dp, prim.p, grad.p, bfa.p, gradv.p,
                        rhs_s.p, A, step_mode ? 1 : 0, dt.p, par.under_relax, b, x0);
   };
-  if (par.second_order) go(k_scal_c<T, true>);
-  else go(k_scal_c<T, false>);
+  // one lane per entry, as the entry Jacobian (each off-diagonal block written
+  // by one lane, so not with duplicate neighbour pairs); GPU_JAC_KERNEL = CELL
+  // takes the per-cell form
+  auto goe = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3((nwave + kScalWaves - 1) / kScalWaves), dim3(64 * kScalWaves), 0, stream, dm, dp,
+                       prim.p, grad.p, bfa.p, gradv.p, rhs_s.p, A, step_mode ? 1 : 0, dt.p, par.under_relax, b, x0);
+  };
+  const bool ent = par.jac_kernel == kResidEntry && !dup_blocks && nwave > 0;
+  if (ent) {
+    if (par.second_order) goe(k_scal_e<T, true>);
+    else goe(k_scal_e<T, false>);
+  } else {
+    if (par.second_order) go(k_scal_c<T, true>);
+    else go(k_scal_c<T, false>);
+  }
 }
 
 template <typename T>
