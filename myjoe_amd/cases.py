@@ -15,6 +15,7 @@
   number per unit length Re, Spalart-Allmaras in the coupled loop (6x6
   blocks), free-stream nuSA = 3 nu.
 * ``with_sa``: adds the Spalart-Allmaras scalar nuSA to a viscous case.
+* ``with_sst``: adds the SST k-omega scalars kine, omega to a viscous case.
 * ``coupled``: the same case under runBackwardEulerCoupled.
 """
 from __future__ import annotations
@@ -190,6 +191,26 @@ def with_sa(joe_in: str, nu_init: float, zone_values: dict) -> str:
 # nuSA boundary values of the SA variants of the two test geometries
 SA_BUMP = {"inlet": 0.006, "bump_wall": 0.0}
 SA_BOX = {"inlet": 0.006, "wall": 0.0, "farfield": 0.004}
+
+
+def with_sst(joe_in: str, kine_init: float, omega_init: float, zone_values: dict) -> str:
+    """The case with the Menter SST k-omega model: ``TURB_MODEL = SST`` (the
+    reference picks its RansTurbKOmSST class by run number instead), the
+    initial kine and omega (INITIAL_CONDITION_TURB) and per zone a pair
+    (kine, omega) of Dirichlet values, omega ``"HOOK"`` on walls (the model's
+    wall value 60 mu / (rho beta_1 d^2)); zones without values get
+    zero-gradient scalars. The case must be viscous."""
+    joe_in = joe_in.replace("INITIAL_CONDITION_TURB = 0.0", f"INITIAL_CONDITION_TURB = {kine_init!r} {omega_init!r}")
+    lines = ["TURB_MODEL = SST"]
+    for z, (k, w) in zone_values.items():
+        lines.append(f"{z}.kine = {k!r}")
+        lines.append(f"{z}.omega = {w if isinstance(w, str) else repr(w)}")
+    return joe_in + "\n".join(lines) + "\n"
+
+
+# kine, omega boundary values of the SST variants of the two test geometries
+SST_BUMP = {"inlet": (0.001, 5.0), "bump_wall": (0.0, "HOOK")}
+SST_BOX = {"inlet": (0.001, 5.0), "wall": (0.0, "HOOK"), "farfield": (0.0008, 4.0)}
 
 
 MACH_PLATE = 0.2

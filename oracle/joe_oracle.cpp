@@ -31,11 +31,11 @@ inline double norm3(double *n, const double *v) {  // tc_vec3d.h:36-46
 // pointwise flux functions
 // ---------------------------------------------------------------------------
 
-// HLLC, UgpWithCvCompFlow.cpp:120-235 (surfVeloc = 0). With Fs, one
-// transported scalar: FrhoScal = Frho * (upwind side value) * area.
+// HLLC, UgpWithCvCompFlow.cpp:120-235 (surfVeloc = 0). With Fs, nsc
+// transported scalars: FrhoScal[s] = Frho * (upwind side value) * area.
 void flux_hllc(double F[5], double rhoL, const double *uL, double pL, double gammaL, double kL, double rhoR,
                const double *uR, double pR, double gammaR, double kR, double area, const double *nVec,
-               double phiL = 0.0, double phiR = 0.0, double *Fs = nullptr) {
+               const double *phiL = nullptr, const double *phiR = nullptr, double *Fs = nullptr, int nsc = 0) {
   double surfv = 0.0;
   double unL = dot3(uL, nVec) - surfv;
   double unLr = unL - surfv;
@@ -61,13 +61,14 @@ void flux_hllc(double F[5], double rhoL, const double *uL, double pL, double gam
   double sR = max(uRoe + cRoe, unRr + cR);
   double sM = (pL - pR - rhoL * unL * (sL - unL) + rhoR * unR * (sR - unR)) / (rhoR * (sR - unR) - rhoL * (sL - unL));
   double pStar = rhoR * (unR - sR) * (unR - sM) + pR;
-  double Frho, Frhou[3], FrhoE, FrhoScal;
+  double Frho, Frhou[3], FrhoE;
+  bool left;
   if (sM >= 0.0) {
+    left = true;
     if (sL > 0.0) {
       Frho = rhoL * unLr;
       for (int i = 0; i < 3; i++) Frhou[i] = rhoL * uL[i] * unLr + pL * nVec[i];
       FrhoE = eL * unLr + pL * unL;
-      FrhoScal = Frho * phiL;
     } else {
       double invSLmSs = 1.0 / (sL - sM);
       double sLmuL = sL - unL;
@@ -78,9 +79,9 @@ void flux_hllc(double F[5], double rhoL, const double *uL, double pL, double gam
       Frho = rhoSL * sM;
       for (int i = 0; i < 3; i++) Frhou[i] = rhouSL[i] * sM + pStar * nVec[i];
       FrhoE = (eSL + pStar) * sM;
-      FrhoScal = Frho * phiL;
     }
   } else {
+    left = false;
     if (sR >= 0.0) {
       double invSRmSs = 1.0 / (sR - sM);
       double sRmuR = sR - unR;
@@ -91,18 +92,20 @@ void flux_hllc(double F[5], double rhoL, const double *uL, double pL, double gam
       Frho = rhoSR * sM;
       for (int i = 0; i < 3; i++) Frhou[i] = rhouSR[i] * sM + pStar * nVec[i];
       FrhoE = (eSR + pStar) * sM;
-      FrhoScal = Frho * phiR;
     } else {
       Frho = rhoR * unRr;
       for (int i = 0; i < 3; i++) Frhou[i] = rhoR * uR[i] * unRr + pR * nVec[i];
       FrhoE = eR * unRr + pR * unR;
-      FrhoScal = Frho * phiR;
     }
   }
   F[0] = Frho * area;
   for (int i = 0; i < 3; i++) F[1 + i] = Frhou[i] * area;
   F[4] = FrhoE * area;
-  if (Fs) *Fs = FrhoScal * area;
+  if (Fs)
+    for (int s = 0; s < nsc; s++) {
+      double FrhoScal = Frho * (left ? phiL[s] : phiR[s]);
+      Fs[s] = FrhoScal * area;
+    }
 }
 
 // Roe, UgpWithCvCompFlow.cpp:3281-3380. When the Roe-averaged a^2 < 1e-12 the
@@ -1004,6 +1007,13 @@ struct or_ctx {
   bool sa = false;
   std::vector<double> phi, phi_bfa, grad_phi, diff, mut_fa, wall_dist, str_mag, vort_mag;
   std::vector<double> rhs_scal, AScal, dscal;
+  // Menter SST (RansTurbKOmSST): kine and omega, registered in that order
+  // (TurbModel_KOMSST.h:36-52), in the same arrays scalar by scalar (phi
+  // [nsc][ncv], phi_bfa [nsc][nfa_b], grad_phi [nsc][ncv][3], diff [nsc][nfa],
+  // rhs_scal and dscal [nsc][ncv], AScal [nsc][6][nnz]); its model fields
+  bool sst = false;
+  int nsc = 0;
+  std::vector<double> diverg, cross_diff, f1, f2;
   // the coupled loop with the scalar: its 6x6 blocks and correction [ncv][6]
   std::vector<double> A6, dq6;
   // each face's inviscid flux [nfa][5] as the face loops last formed it
@@ -1039,13 +1049,14 @@ static void gg_weights(const or_mesh &m, int ifa, int icv0, int icv1, double &w0
   w1 = sqrt(s1);
 }
 
-// calcStateVariables, UgpWithCvCompFlow.h:1662-1694 (kine = NULL; gamma, RoM
-// constant: initialHookScalarRansCombModel sets them to GAMMA, R_gas)
+// calcStateVariables, UgpWithCvCompFlow.h:1662-1694 (kine = NULL but with SST;
+// gamma, RoM constant: initialHookScalarRansCombModel sets them to GAMMA, R_gas)
 void calc_state(or_ctx &c) {
   const double gam = c.p.gamma, R = c.p.R_gas;
   for (int icv = 0; icv < c.ncv; icv++) {
     for (int i = 0; i < 3; i++) c.vel[3 * icv + i] = c.rhou[3 * icv + i] / c.rho[icv];
     double kinecv = 0.0;
+    if (c.sst) kinecv = c.phi[icv];
     double pr = (gam - 1.0) * (c.rhoE[icv] - 0.5 * dot3(&c.rhou[3 * icv], &c.rhou[3 * icv]) / c.rho[icv] - c.rho[icv] * kinecv);
     if (pr > 0.0) c.press[icv] = pr;
     c.temp[icv] = c.press[icv] / (c.rho[icv] * R);
@@ -1460,37 +1471,43 @@ void limit_vector(or_ctx &c, double *g, const double *phi, const double *ref) {
 
 // Scalars are transported with HLLC only (the build rejects other schemes
 // together with a turbulence model), so Fs / SL / SR are HLLC's.
-void face_flux(const or_ctx &c, double F[5], double rho0, const double *u0, double p0, double rho1, const double *u1,
-               double p1, double area, const double *nVec, double phi0 = 0.0, double phi1 = 0.0,
-               double *Fs = nullptr) {
+// k0, k1: the turbulent kinetic energy of each side (kineFA / kineCV of the
+// face loops, 0 without SST); phi0/phi1/Fs: the c.nsc transported scalars.
+void face_flux(const or_ctx &c, double F[5], double rho0, const double *u0, double p0, double k0, double rho1,
+               const double *u1, double p1, double k1, double area, const double *nVec,
+               const double *phi0 = nullptr, const double *phi1 = nullptr, double *Fs = nullptr) {
   const double g = c.p.gamma;
   if (c.p.scheme == OR_ROE)
     flux_roe(F, rho0, u0, p0, g, rho1, u1, p1, g, area, nVec);
   else if (c.p.scheme == OR_AUSM)
-    flux_ausm(F, rho0, u0, p0, g, 0.0, rho1, u1, p1, g, 0.0, area, nVec);
+    flux_ausm(F, rho0, u0, p0, g, k0, rho1, u1, p1, g, k1, area, nVec);
   else
-    flux_hllc(F, rho0, u0, p0, g, 0.0, rho1, u1, p1, g, 0.0, area, nVec, phi0, phi1, Fs);
+    flux_hllc(F, rho0, u0, p0, g, k0, rho1, u1, p1, g, k1, area, nVec, phi0, phi1, Fs, Fs ? c.nsc : 0);
 }
 
-void face_jac(const or_ctx &c, double (*Apl)[5], double (*Ami)[5], double rho0, const double *u0, double p0,
-              double rho1, const double *u1, double p1, double area, const double *nVec, double phi0 = 0.0,
-              double phi1 = 0.0, double *SL = nullptr, double *SR = nullptr) {
+// SL/SR: [nsc][6], the scalar rows of each scalar (AplScal / AmiScal)
+void face_jac(const or_ctx &c, double (*Apl)[5], double (*Ami)[5], double rho0, const double *u0, double p0, double k0,
+              double rho1, const double *u1, double p1, double k1, double area, const double *nVec,
+              const double *phi0 = nullptr, const double *phi1 = nullptr, double *SL = nullptr, double *SR = nullptr) {
   const double g = c.p.gamma;
   if (c.p.scheme == OR_ROE) {
     jac_roe(Apl, Ami, rho0, u0, p0, g, rho1, u1, p1, g, area, nVec);
   } else {  // HLLC, and AUSM which uses the HLLC Jacobian (JoeWithModels.cpp:3057-3061)
     HllcBranch br;
-    jac_hllc(Apl, Ami, rho0, u0, p0, g, 0.0, rho1, u1, p1, g, 0.0, area, nVec, &br);
-    if (SL || SR) jac_hllc_scal(SL, SR, Apl, Ami, br, phi0, phi1);
+    jac_hllc(Apl, Ami, rho0, u0, p0, g, k0, rho1, u1, p1, g, k1, area, nVec, &br);
+    if (SL || SR)
+      for (int s = 0; s < c.nsc; s++)
+        jac_hllc_scal(SL ? SL + 6 * s : nullptr, SR ? SR + 6 * s : nullptr, Apl, Ami, br, phi0[s], phi1[s]);
   }
 }
 
-// AScal[k][noc] (k = 0..5) += sgn * S[k]
-inline void add_ascal(or_ctx &c, int noc, double sgn, const double *S) {
+// AScal[s][k][noc] (k = 0..5) += sgn * S[k]
+inline void add_ascal(or_ctx &c, int noc, double sgn, const double *S, int s = 0) {
   const size_t nnz = c.nnz;
+  double *AS = &c.AScal[6 * nnz * (size_t)s];
   for (int k = 0; k <= 5; k++) {
-    if (sgn > 0.0) c.AScal[k * nnz + noc] += S[k];
-    else c.AScal[k * nnz + noc] -= S[k];
+    if (sgn > 0.0) AS[k * nnz + noc] += S[k];
+    else AS[k * nnz + noc] -= S[k];
   }
 }
 
@@ -1517,13 +1534,15 @@ static void euler_gradients(or_ctx &c) {
     else
       grad_scalar_gg(c, c.grad_p.data(), c.press.data(), c.p_bfa.data());
     if (c.p.limiter) limit_scalar(c, c.grad_p.data(), c.press.data(), c.press.data(), NULL);
-    // STANDARD reconstruction of the transported scalar (:2736-2741)
-    if (c.sa) {
+    // STANDARD reconstruction of the transported scalars (:2736-2741)
+    for (int s = 0; s < c.nsc; s++) {
+      double *g = &c.grad_phi[3 * (size_t)c.ncv * s], *ph = &c.phi[(size_t)c.ncv * s];
+      const double *pb = &c.phi_bfa[(size_t)c.nfa_b * s];
       if (c.p.grad_scheme == OR_GRAD_LS)
-        grad_scalar_ls(c, c.grad_phi.data(), c.phi.data(), c.phi_bfa.data());
+        grad_scalar_ls(c, g, ph, pb);
       else
-        grad_scalar_gg(c, c.grad_phi.data(), c.phi.data(), c.phi_bfa.data());
-      if (c.p.limiter) limit_scalar(c, c.grad_phi.data(), c.phi.data(), c.phi.data(), NULL);
+        grad_scalar_gg(c, g, ph, pb);
+      if (c.p.limiter) limit_scalar(c, g, ph, ph, NULL);
     }
   }
 }
@@ -1537,8 +1556,14 @@ void calc_euler_flux(or_ctx &c, double *A) {
   const or_mesh &m = c.m;
   const bool snd = c.p.second_order != 0;
   if (snd) euler_gradients(c);
-  double F[5], Apl[5][5], Ami[5][5], Fs = 0.0, SL[6], SR[6];
-  const bool sa = c.sa;
+  double F[5], Apl[5][5], Ami[5][5], Fs[2] = {0.0, 0.0}, SL[12], SR[12];
+  const int nsc = c.nsc;
+  const size_t n = c.ncv, nb = c.nfa_b;
+  // scalar s of cell icv, boundary face ifa, its gradient; its residual
+  auto ph = [&](int s, int icv) { return c.phi[n * s + icv]; };
+  auto phb = [&](int s, int ifa) { return c.phi_bfa[nb * s + ifa]; };
+  auto gph = [&](int s, int icv) { return &c.grad_phi[3 * (n * s + icv)]; };
+  auto rs = [&](int s, int icv) -> double & { return c.rhs_scal[n * s + icv]; };
   for (int ifa = c.nfa_b; ifa < c.nfa; ifa++) {
     int icv0 = m.cvofa[2 * ifa], icv1 = m.cvofa[2 * ifa + 1];
     int noc00 = 0, noc01 = 0, noc11 = 0, noc10 = 0;
@@ -1556,7 +1581,11 @@ void calc_euler_flux(or_ctx &c, double *A) {
     double u0[3] = {c.vel[3 * icv0], c.vel[3 * icv0 + 1], c.vel[3 * icv0 + 2]};
     double rho1 = c.rho[icv1], p1 = c.press[icv1];
     double u1[3] = {c.vel[3 * icv1], c.vel[3 * icv1 + 1], c.vel[3 * icv1 + 2]};
-    double sc0 = sa ? c.phi[icv0] : 0.0, sc1 = sa ? c.phi[icv1] : 0.0;
+    double sc0[2], sc1[2], cv0[2], cv1[2];  // Scalar0/1 (faces), ScalCV0/1 (cells)
+    for (int s = 0; s < nsc; s++) {
+      sc0[s] = cv0[s] = ph(s, icv0);
+      sc1[s] = cv1[s] = ph(s, icv1);
+    }
     if (snd) {
       double r0[3], r1[3];
       for (int i = 0; i < 3; i++) {
@@ -1570,7 +1599,7 @@ void calc_euler_flux(or_ctx &c, double *A) {
         rho0 = c.rho[icv0];
       } else {
         for (int i = 0; i < 3; i++) u0[i] += dot3(r0, &c.grad_u[9 * icv0 + 3 * i]);
-        if (sa) sc0 = sc0 + dot3(r0, &c.grad_phi[3 * icv0]);  // :2862-2869
+        for (int s = 0; s < nsc; s++) sc0[s] = sc0[s] + dot3(r0, gph(s, icv0));  // :2862-2869
       }
       rho1 += dot3(r1, &c.grad_rho[3 * icv1]);
       p1 += dot3(r1, &c.grad_p[3 * icv1]);
@@ -1579,23 +1608,25 @@ void calc_euler_flux(or_ctx &c, double *A) {
         rho1 = c.rho[icv1];
       } else {
         for (int i = 0; i < 3; i++) u1[i] += dot3(r1, &c.grad_u[9 * icv1 + 3 * i]);
-        if (sa) sc1 = sc1 + dot3(r1, &c.grad_phi[3 * icv1]);
+        for (int s = 0; s < nsc; s++) sc1[s] = sc1[s] + dot3(r1, gph(s, icv1));
       }
     }
-    face_flux(c, F, rho0, u0, p0, rho1, u1, p1, area, nVec, sc0, sc1, sa ? &Fs : nullptr);
+    // kineFA0/1 and kineCV0/1 (:2924-2930)
+    const double kfa0 = c.sst ? sc0[0] : 0.0, kfa1 = c.sst ? sc1[0] : 0.0;
+    const double kcv0 = c.sst ? cv0[0] : 0.0, kcv1 = c.sst ? cv1[0] : 0.0;
+    face_flux(c, F, rho0, u0, p0, kfa0, rho1, u1, p1, kfa1, area, nVec, sc0, sc1, nsc ? Fs : nullptr);
     keep_flux(c, ifa, F);
     c.rhs_rho[icv0] -= F[0];
     for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= F[1 + i];
     c.rhs_rhoE[icv0] -= F[4];
-    if (sa) c.rhs_scal[icv0] -= 1.0 * Fs;  // ScalConvTerm = 1 (:3014-3015)
+    for (int s = 0; s < nsc; s++) rs(s, icv0) -= 1.0 * Fs[s];  // ScalConvTerm = 1 (:3014-3015)
     c.rhs_rho[icv1] += F[0];
     for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv1 + i] += F[1 + i];
     c.rhs_rhoE[icv1] += F[4];
-    if (sa) c.rhs_scal[icv1] += 1.0 * Fs;
+    for (int s = 0; s < nsc; s++) rs(s, icv1) += 1.0 * Fs[s];
     if (A) {
-      face_jac(c, Apl, Ami, c.rho[icv0], &c.vel[3 * icv0], c.press[icv0], c.rho[icv1], &c.vel[3 * icv1],
-               c.press[icv1], area, nVec, sa ? c.phi[icv0] : 0.0, sa ? c.phi[icv1] : 0.0, sa ? SL : nullptr,
-               sa ? SR : nullptr);
+      face_jac(c, Apl, Ami, c.rho[icv0], &c.vel[3 * icv0], c.press[icv0], kcv0, c.rho[icv1], &c.vel[3 * icv1],
+               c.press[icv1], kcv1, area, nVec, cv0, cv1, nsc ? SL : nullptr, nsc ? SR : nullptr);
       for (int i = 0; i < 5; i++)
         for (int j = 0; j < 5; j++) {
           A[25 * noc00 + 5 * i + j] += Apl[i][j];
@@ -1606,11 +1637,11 @@ void calc_euler_flux(or_ctx &c, double *A) {
           A[25 * noc11 + 5 * i + j] -= Ami[i][j];
           A[25 * noc10 + 5 * i + j] -= Apl[i][j];
         }
-      if (sa) {  // :3088-3101
-        add_ascal(c, noc00, 1.0, SL);
-        add_ascal(c, noc01, 1.0, SR);
-        add_ascal(c, noc11, -1.0, SR);
-        add_ascal(c, noc10, -1.0, SL);
+      for (int s = 0; s < nsc; s++) {  // :3088-3101
+        add_ascal(c, noc00, 1.0, SL + 6 * s, s);
+        add_ascal(c, noc01, 1.0, SR + 6 * s, s);
+        add_ascal(c, noc11, -1.0, SR + 6 * s, s);
+        add_ascal(c, noc10, -1.0, SL + 6 * s, s);
       }
     }
   }
@@ -1622,10 +1653,13 @@ void calc_euler_flux(or_ctx &c, double *A) {
       double nVec[3];
       double area = norm3(nVec, &m.fa_normal[3 * ifa]);
       const double *ub = &c.vel_bfa[3 * ifa];
+      double sb[2];  // phi_bfa on both sides
+      for (int s = 0; s < nsc; s++) sb[s] = phb(s, ifa);
       if (bc == OR_BC_SYMMETRY || bc == OR_BC_WALL) {
-        const double sb = sa ? c.phi_bfa[ifa] : 0.0;  // both sides phi_bfa (:3134-3137, 3291-3294)
-        face_flux(c, F, c.rho_bfa[ifa], ub, c.p_bfa[ifa], c.rho_bfa[ifa], ub, c.p_bfa[ifa], area, nVec, sb, sb,
-                  sa ? &Fs : nullptr);
+        // both sides phi_bfa, kineFA = kine's phi_bfa (:3134-3147, 3291-3294)
+        const double kfa = c.sst ? sb[0] : 0.0;
+        face_flux(c, F, c.rho_bfa[ifa], ub, c.p_bfa[ifa], kfa, c.rho_bfa[ifa], ub, c.p_bfa[ifa], kfa, area, nVec, sb,
+                  sb, nsc ? Fs : nullptr);
         if (bc == OR_BC_SYMMETRY && c.p.symm_bc == OR_SYMM_ANALYTICAL) {
           const double Fp[5] = {0.0, c.p_bfa[ifa] * nVec[0] * area, c.p_bfa[ifa] * nVec[1] * area,
                                 c.p_bfa[ifa] * nVec[2] * area, 0.0};
@@ -1639,20 +1673,22 @@ void calc_euler_flux(or_ctx &c, double *A) {
           for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= F[1 + i];
           c.rhs_rhoE[icv0] -= F[4];
         }
-        if (sa) c.rhs_scal[icv0] -= 1.0 * Fs;  // :3197-3198, 3341-3342
+        for (int s = 0; s < nsc; s++) rs(s, icv0) -= 1.0 * Fs[s];  // :3197-3198, 3341-3342
         if (A) {
           int noc00 = m.nbocv_i[icv0];
-          if (sa) {  // scalar rows from the bfa/bfa HLLC Jacobian (:3202-3206, 3346-3350, 3270-3274)
+          if (nsc) {  // scalar rows from the bfa/bfa HLLC Jacobian (:3202-3206, 3346-3350, 3270-3274)
             double Al[5][5];
-            face_jac(c, Al, NULL, c.rho_bfa[ifa], ub, c.p_bfa[ifa], c.rho_bfa[ifa], ub, c.p_bfa[ifa], area, nVec, sb,
-                     sb, SL, nullptr);
-            add_ascal(c, noc00, 1.0, SL);
+            face_jac(c, Al, NULL, c.rho_bfa[ifa], ub, c.p_bfa[ifa], kfa, c.rho_bfa[ifa], ub, c.p_bfa[ifa], kfa, area,
+                     nVec, sb, sb, SL, nullptr);
+            for (int s = 0; s < nsc; s++) add_ascal(c, noc00, 1.0, SL + 6 * s, s);
           }
           if (bc == OR_BC_WALL || c.p.symm_bc == OR_SYMM_ORIGINAL_NUMERIC) {
-            face_jac(c, Apl, NULL, c.rho_bfa[ifa], ub, c.p_bfa[ifa], c.rho_bfa[ifa], ub, c.p_bfa[ifa], area, nVec);
+            face_jac(c, Apl, NULL, c.rho_bfa[ifa], ub, c.p_bfa[ifa], kfa, c.rho_bfa[ifa], ub, c.p_bfa[ifa], kfa, area,
+                     nVec);
             for (int k = 0; k < 25; k++) A[25 * noc00 + k] += Apl[k / 5][k % 5];
           } else if (c.p.symm_bc == OR_SYMM_NEW_NUMERIC) {
-            face_jac(c, Apl, Ami, c.rho_bfa[ifa], ub, c.p_bfa[ifa], c.rho_bfa[ifa], ub, c.p_bfa[ifa], area, nVec);
+            face_jac(c, Apl, Ami, c.rho_bfa[ifa], ub, c.p_bfa[ifa], kfa, c.rho_bfa[ifa], ub, c.p_bfa[ifa], kfa, area,
+                     nVec);
             for (int k = 0; k < 25; k++) A[25 * noc00 + k] += Apl[k / 5][k % 5] + Ami[k / 5][k % 5];
           } else {  // ANALYTICAL (JoeWithModels.cpp:3243-3269)
             double S[5][5];
@@ -1673,7 +1709,8 @@ void calc_euler_flux(or_ctx &c, double *A) {
       } else {
         double rho0 = c.rho[icv0], p0 = c.press[icv0];
         double u0[3] = {c.vel[3 * icv0], c.vel[3 * icv0 + 1], c.vel[3 * icv0 + 2]};
-        double sc0 = sa ? c.phi[icv0] : 0.0, sb = sa ? c.phi_bfa[ifa] : 0.0;  // :3413-3419
+        double sc0[2], cv0[2];  // :3413-3419
+        for (int s = 0; s < nsc; s++) sc0[s] = cv0[s] = ph(s, icv0);
         if (snd) {
           double r0[3];
           for (int i = 0; i < 3; i++) r0[i] = m.x_fa[3 * ifa + i] - m.x_cv[3 * icv0 + i];
@@ -1684,21 +1721,24 @@ void calc_euler_flux(or_ctx &c, double *A) {
             rho0 = c.rho[icv0];
           } else {
             for (int i = 0; i < 3; i++) u0[i] += dot3(r0, &c.grad_u[9 * icv0 + 3 * i]);
-            if (sa) sc0 = sc0 + dot3(r0, &c.grad_phi[3 * icv0]);  // :3445-3454
+            for (int s = 0; s < nsc; s++) sc0[s] = sc0[s] + dot3(r0, gph(s, icv0));  // :3445-3454
           }
         }
-        face_flux(c, F, rho0, u0, p0, c.rho_bfa[ifa], ub, c.p_bfa[ifa], area, nVec, sc0, sb, sa ? &Fs : nullptr);
+        // kineCV0, kineFA0 (reconstructed), kineFA1 = phi_bfa (:3466-3471)
+        const double kcv0 = c.sst ? cv0[0] : 0.0, kfa0 = c.sst ? sc0[0] : 0.0, kfa1 = c.sst ? sb[0] : 0.0;
+        face_flux(c, F, rho0, u0, p0, kfa0, c.rho_bfa[ifa], ub, c.p_bfa[ifa], kfa1, area, nVec, sc0, sb,
+                  nsc ? Fs : nullptr);
         keep_flux(c, ifa, F);
         c.rhs_rho[icv0] -= F[0];
         for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= F[1 + i];
         c.rhs_rhoE[icv0] -= F[4];
-        if (sa) c.rhs_scal[icv0] -= 1.0 * Fs;
+        for (int s = 0; s < nsc; s++) rs(s, icv0) -= 1.0 * Fs[s];
         if (A) {
-          face_jac(c, Apl, NULL, c.rho[icv0], &c.vel[3 * icv0], c.press[icv0], c.rho_bfa[ifa], ub, c.p_bfa[ifa], area,
-                   nVec, sa ? c.phi[icv0] : 0.0, sb, sa ? SL : nullptr, nullptr);
+          face_jac(c, Apl, NULL, c.rho[icv0], &c.vel[3 * icv0], c.press[icv0], kcv0, c.rho_bfa[ifa], ub, c.p_bfa[ifa],
+                   kfa1, area, nVec, cv0, sb, nsc ? SL : nullptr, nullptr);
           int noc00 = m.nbocv_i[icv0];
           for (int k = 0; k < 25; k++) A[25 * noc00 + k] += Apl[k / 5][k % 5];
-          if (sa) add_ascal(c, noc00, 1.0, SL);  // :3559-3561
+          for (int s = 0; s < nsc; s++) add_ascal(c, noc00, 1.0, SL + 6 * s, s);  // :3559-3561
         }
       }
     }
@@ -2148,7 +2188,7 @@ void add_visc_flux(const or_ctx &c, double *Frhou, double &FrhoE, double (*A0)[5
                    double gam0, double kine0, double rho1, const double *u1, const double *grad_u1, double h1,
                    const double *grad_h1, double T1, double R1, double gam1, double kine1, double mul, double mut,
                    double lambdaOverCp, const double *u_fa, double area, const double *nVec, double smag,
-                   const double *sVec) {
+                   const double *sVec, double kine_fa = 0.0) {
   const double nonL = 0.0;
   const double rij_d[3] = {0.0, 0.0, 0.0}, rij_offd[3] = {0.0, 0.0, 0.0};
   double alpha = dot3(nVec, sVec);
@@ -2171,6 +2211,8 @@ void add_visc_flux(const or_ctx &c, double *Frhou, double &FrhoE, double (*A0)[5
                        grad_u_f[i][1] * fCorr[1] + grad_u_f[1][i] * nVec[1] + grad_u_f[i][2] * fCorr[2] +
                        grad_u_f[2][i] * nVec[2]);
   tmp = (1.0 - nonL) * 2.0 / 3.0 * mut * (grad_u_f[0][0] + grad_u_f[1][1] + grad_u_f[2][2]);
+  // -2/3 rho k (:1615; with SA kine_fa = 0 and the term is left out)
+  if (c.sst) tmp += (1.0 - nonL) * 1.0 / 3.0 * (rho0 + rho1) * kine_fa;
   tauTurbij_nj[0] -= tmp * nVec[0];
   tauTurbij_nj[1] -= tmp * nVec[1];
   tauTurbij_nj[2] -= tmp * nVec[2];
@@ -2184,6 +2226,7 @@ void add_visc_flux(const or_ctx &c, double *Frhou, double &FrhoE, double (*A0)[5
                                                    (grad_h0[1] + grad_h1[1]) * fCorr[1] +
                                                    (grad_h0[2] + grad_h1[2]) * fCorr[2]));
   double psi = 0.0;
+  if (c.sst) psi = (mul + mut) * (alpha * (kine1 - kine0) / smag);  // triple correlation (:1657-1660)
   FrhoE = -area * (enth + tauij_nj[0] * u_fa[0] + tauij_nj[1] * u_fa[1] + tauij_nj[2] * u_fa[2] + psi);
 
   double dfdU[5][5] = {{0.0}}, dUdQ[5][5] = {{0.0}};
@@ -2243,7 +2286,7 @@ void add_visc_flux(const or_ctx &c, double *Frhou, double &FrhoE, double (*A0)[5
   }
 }
 
-// calcViscousFluxNS, JoeWithModels.cpp:3591-3855 (no kine: laminar or SA)
+// calcViscousFluxNS, JoeWithModels.cpp:3591-3855 (kine from SST's first scalar)
 void calc_viscous_flux(or_ctx &c, double *A) {
   const or_mesh &m = c.m;
   const double gam = c.p.gamma, R = c.p.R_gas;
@@ -2282,10 +2325,18 @@ void calc_viscous_flux(or_ctx &c, double *A) {
     for (int i = 0; i < 3; i++) sVec[i] /= smag;
     const double *v0 = &c.vel[3 * icv0], *v1 = &c.vel[3 * icv1];
     double uAux_fa[3] = {0.5 * (v0[0] + v1[0]), 0.5 * (v0[1] + v1[1]), 0.5 * (v0[2] + v1[2])};
+    double kine0 = 0.0, kine1 = 0.0, kine_fa = 0.0;
+    if (c.sst) {  // :3654-3669
+      kine0 = c.phi[icv0];
+      kine1 = c.phi[icv1];
+      double w0, w1;
+      gg_weights(m, ifa, icv0, icv1, w0, w1);
+      kine_fa = (w1 * kine0 + w0 * kine1) / (w0 + w1);
+    }
     add_visc_flux(c, Frhou, FrhoE, A ? A0 : NULL, A ? A1 : NULL, c.rho[icv0], v0, &c.grad_u[9 * icv0],
-                  c.enthalpy[icv0], &c.grad_h[3 * icv0], c.temp[icv0], R, gam, 0.0, c.rho[icv1], v1,
-                  &c.grad_u[9 * icv1], c.enthalpy[icv1], &c.grad_h[3 * icv1], c.temp[icv1], R, gam, 0.0,
-                  c.mul_fa[ifa], c.mut_fa[ifa], c.lamOcp_fa[ifa], uAux_fa, area, nVec, smag, sVec);
+                  c.enthalpy[icv0], &c.grad_h[3 * icv0], c.temp[icv0], R, gam, kine0, c.rho[icv1], v1,
+                  &c.grad_u[9 * icv1], c.enthalpy[icv1], &c.grad_h[3 * icv1], c.temp[icv1], R, gam, kine1,
+                  c.mul_fa[ifa], c.mut_fa[ifa], c.lamOcp_fa[ifa], uAux_fa, area, nVec, smag, sVec, kine_fa);
     if (A) {
       for (int i = 1; i < 5; i++)
         for (int j = 0; j < 5; j++) {
@@ -2306,8 +2357,17 @@ void calc_viscous_flux(or_ctx &c, double *A) {
   // boundary faces, zone by zone (:3714-3851)
   for (int iz = 0; iz < m.nzone; iz++) {
     const int bc = m.zone_bc[iz];
-    if (bc == OR_BC_SYMMETRY) continue;  // only the k term, absent without a turbulence model
     const int f0 = m.zone_ifa[2 * iz], f1 = m.zone_ifa[2 * iz + 1];
+    if (bc == OR_BC_SYMMETRY) {  // only the k term (:3725-3745)
+      if (c.sst)
+        for (int ifa = f0; ifa <= f1; ifa++) {
+          int icv0 = m.cvofa[2 * ifa];
+          double kine_fa = c.phi_bfa[ifa];
+          double tmp = 1.0 / 3.0 * (c.rho[icv0] + c.rho_bfa[ifa]) * kine_fa;
+          for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv0 + i] -= tmp * m.fa_normal[3 * ifa + i];
+        }
+      continue;
+    }
     for (int ifa = f0; ifa <= f1; ifa++) {
       int icv0 = m.cvofa[2 * ifa];
       double nVec[3];
@@ -2325,10 +2385,17 @@ void calc_viscous_flux(or_ctx &c, double *A) {
         sArg = sVec;
       }
       const double *vb = &c.vel_bfa[3 * ifa];
+      // walls: kine0 of the cell, kine1 = kine_fa = 0 (:3767-3776); other zones
+      // kine1 = kine_fa = phi_bfa (:3817-3826)
+      double kine0 = 0.0, kine1 = 0.0, kine_fa = 0.0;
+      if (c.sst) {
+        kine0 = c.phi[icv0];
+        if (bc != OR_BC_WALL) kine1 = kine_fa = c.phi_bfa[ifa];
+      }
       add_visc_flux(c, Frhou, FrhoE, A ? A0 : NULL, NULL, c.rho[icv0], &c.vel[3 * icv0], &c.grad_u[9 * icv0],
-                    c.enthalpy[icv0], &c.grad_h[3 * icv0], c.temp[icv0], R, gam, 0.0, c.rho_bfa[ifa], vb,
-                    &c.grad_u[9 * icv0], c.h_bfa[ifa], &c.grad_h[3 * icv0], c.T_bfa[ifa], R, gam, 0.0, c.mul_fa[ifa],
-                    c.mut_fa[ifa], c.lamOcp_fa[ifa], vb, area, nVec, smag, sArg);
+                    c.enthalpy[icv0], &c.grad_h[3 * icv0], c.temp[icv0], R, gam, kine0, c.rho_bfa[ifa], vb,
+                    &c.grad_u[9 * icv0], c.h_bfa[ifa], &c.grad_h[3 * icv0], c.T_bfa[ifa], R, gam, kine1, c.mul_fa[ifa],
+                    c.mut_fa[ifa], c.lamOcp_fa[ifa], vb, area, nVec, smag, sArg, kine_fa);
       if (A) {
         int noc00 = m.nbocv_i[icv0];
         for (int i = 1; i < 5; i++)
@@ -2444,19 +2511,190 @@ void calc_turb_visc(or_ctx &c) {
   }
 }
 
-// setScalarBC, Scalars.cpp:289-360: a zone with "<zone>.nuSA = v" takes v on
-// inflow faces and the cell value on outflow faces; any other zone the cell value
+namespace sst {
+// model constants, getDoubleParam defaults (TurbModel_KOMSST.h:16-23)
+const double sigma_k1 = 0.85, sigma_k2 = 1.0, sigma_om1 = 0.5, sigma_om2 = 0.856, beta_1 = 0.075, beta_2 = 0.0828,
+             betaStar = 0.09, a1 = 0.31;
+// ScalarTranspEq settings of kine and omega (:36-52)
+const double phiZero = 1.0e-8, phiZeroRel = 1.0e-2;
+const int phiMaxiter = 1000;
+const double lowerBound[2] = {1.0e-10, 1.0e-4}, upperBound[2] = {1.0e10, 1.0e15};
+}  // namespace sst
+
+// the limited gradient of scalar s with its boundary values (calcCvScalarGrad
+// as calcEulerFlux, calcViscousFluxScalar_new and calcMenterBlendingFunctions
+// call it: gradreconstruction, limiterNavierS, the scalar as its own reference)
+static void scalar_grad(or_ctx &c, int s) {
+  double *g = &c.grad_phi[3 * (size_t)c.ncv * s], *ph = &c.phi[(size_t)c.ncv * s];
+  const double *pb = &c.phi_bfa[(size_t)c.nfa_b * s];
+  if (c.p.grad_scheme == OR_GRAD_LS)
+    grad_scalar_ls(c, g, ph, pb);
+  else
+    grad_scalar_gg(c, g, ph, pb);
+  if (c.p.limiter) limit_scalar(c, g, ph, ph, NULL);
+}
+
+// setScalarBC, Scalars.cpp:289-360, per scalar: a zone with "<zone>.<name> = v"
+// takes v on inflow faces and the cell value on outflow faces; "= HOOK" calls
+// boundaryHookScalarRansTurb, which for SST's omega on WALL zones sets
+// 60 mu / (rho beta_1 d^2) (TurbModel_KOMSST.h:342-358) and otherwise leaves
+// the face values alone; any other zone takes the cell value
 void set_scalar_bc(or_ctx &c) {
   const or_mesh &m = c.m;
+  for (int s = 0; s < c.nsc; s++) {
+    const double *zs = s == 0 ? m.zone_scal : m.zone_scal2;
+    double *pb = &c.phi_bfa[(size_t)c.nfa_b * s];
+    const double *ph = &c.phi[(size_t)c.ncv * s];
+    for (int iz = 0; iz < m.nzone; iz++) {
+      const double kind = zs[2 * iz];
+      const double val = zs[2 * iz + 1];
+      for (int ifa = m.zone_ifa[2 * iz]; ifa <= m.zone_ifa[2 * iz + 1]; ifa++) {
+        int icv0 = m.cvofa[2 * ifa];
+        if (kind == 2.0) {
+          if (c.sst && s == 1 && m.zone_bc[iz] == OR_BC_WALL) {
+            double muLamCV = mu_lam(c, c.temp[icv0]);
+            pb[ifa] = 60.0 * muLamCV / (c.rho[icv0] * sst::beta_1 * c.wall_dist[icv0] * c.wall_dist[icv0]);
+          }
+        } else if (kind != 0.0 && !(dot3(&c.vel_bfa[3 * ifa], &m.fa_normal[3 * ifa]) > 1.0e-8)) {
+          pb[ifa] = val;
+        } else {
+          pb[ifa] = ph[icv0];
+        }
+      }
+    }
+  }
+}
+
+// calcRansTurbViscMuet of RansTurbKOmSST (TurbModel_KOMSST.h:100-158): calcGradVel,
+// calcStrainRateAndDivergence, calcVorticity (UgpWithCvCompFlow.h:1300-1392),
+// calcMenterBlendingFunctions (TurbModel_KOMSST.h:360-392), then mut_fa
+void calc_turb_visc_sst(or_ctx &c) {
+  const or_mesh &m = c.m;
+  const size_t n = c.ncv;
+  if (c.p.grad_scheme == OR_GRAD_LS)
+    grad_vector_ls(c, c.grad_u.data(), c.vel.data(), c.vel_bfa.data());
+  else
+    grad_vector_gg(c, c.grad_u.data(), c.vel.data(), c.vel_bfa.data());
+  if (c.p.limiter) limit_vector(c, c.grad_u.data(), c.vel.data(), c.sos.data());
+  for (int icv = 0; icv < c.ncv; icv++) {
+    const double *g = &c.grad_u[9 * icv];
+    c.diverg[icv] = g[0] + g[4] + g[8];
+    double str = 0.0;
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++)
+        if (i == j) str += pow(0.5 * (g[3 * i + j] + g[3 * j + i]) - 1. / 3. * c.diverg[icv], 2.0);
+        else str += pow(0.5 * (g[3 * i + j] + g[3 * j + i]), 2.0);
+    c.str_mag[icv] = sqrt(2.0 * str);
+    double vort = 0.0;
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) vort += pow(0.5 * (g[3 * i + j] - g[3 * j + i]), 2.0);
+    c.vort_mag[icv] = sqrt(2.0 * vort);
+  }
+  scalar_grad(c, 0);
+  scalar_grad(c, 1);
+  const double *kine = &c.phi[0], *omega = &c.phi[n];
+  for (int icv = 0; icv < c.ncv; icv++) {
+    const double d = c.wall_dist[icv], rho = c.rho[icv];
+    const double mue = cell_avg(c, c.mul_fa, icv);
+    c.cross_diff[icv] = max(2.0 * rho * sst::sigma_om2 / omega[icv] * dot3(&c.grad_phi[3 * icv], &c.grad_phi[3 * (n + icv)]),
+                            1.0e-20);
+    double gamma1 = 500.0 * mue / (pow(d, 2.0) * rho * omega[icv]);
+    double gamma2 = 4.0 * sst::sigma_om2 * rho * kine[icv] / (d * d * c.cross_diff[icv]);
+    double gamma3 = sqrt(kine[icv]) / (sst::betaStar * omega[icv] * d);
+    double gamma = min(max(gamma1, gamma3), gamma2);
+    c.f1[icv] = tanh(pow(gamma, 4.0));
+    gamma = max(2.0 * gamma3, gamma1);
+    c.f2[icv] = tanh(pow(gamma, 2.0));
+  }
+  const double *lim = c.p.sst_limiter_shear == 1 ? c.str_mag.data() : c.vort_mag.data();
+  for (int ifa = c.nfa_b; ifa < c.nfa; ifa++) {
+    int icv0 = m.cvofa[2 * ifa], icv1 = m.cvofa[2 * ifa + 1];
+    double w0, w1;
+    gg_weights(m, ifa, icv0, icv1, w0, w1);
+    double rho_fa = (w1 * c.rho[icv0] + w0 * c.rho[icv1]) / (w0 + w1);
+    double kine_fa = (w1 * kine[icv0] + w0 * kine[icv1]) / (w0 + w1);
+    double om_fa = (w1 * omega[icv0] + w0 * omega[icv1]) / (w0 + w1);
+    double limitFunc_fa = (w1 * lim[icv0] + w0 * lim[icv1]) / (w0 + w1);
+    double blendFuncF2_fa = (w1 * c.f2[icv0] + w0 * c.f2[icv1]) / (w0 + w1);
+    double zeta = min(1.0 / om_fa, sst::a1 / (limitFunc_fa * blendFuncF2_fa));
+    c.mut_fa[ifa] = min(max(rho_fa * kine_fa * zeta, 0.0), 1.0);
+  }
   for (int iz = 0; iz < m.nzone; iz++) {
-    const bool dir = m.zone_scal[2 * iz] != 0.0;
-    const double val = m.zone_scal[2 * iz + 1];
-    for (int ifa = m.zone_ifa[2 * iz]; ifa <= m.zone_ifa[2 * iz + 1]; ifa++) {
+    const int f0 = m.zone_ifa[2 * iz], f1 = m.zone_ifa[2 * iz + 1];
+    for (int ifa = f0; ifa <= f1; ifa++) {
+      if (m.zone_bc[iz] == OR_BC_WALL) {
+        c.mut_fa[ifa] = 0.0;
+        continue;
+      }
       int icv0 = m.cvofa[2 * ifa];
-      if (dir && !(dot3(&c.vel_bfa[3 * ifa], &m.fa_normal[3 * ifa]) > 1.0e-8))
-        c.phi_bfa[ifa] = val;
-      else
-        c.phi_bfa[ifa] = c.phi[icv0];
+      double zeta = min(1.0 / omega[icv0], sst::a1 / (lim[icv0] * c.f2[icv0]));
+      c.mut_fa[ifa] = min(max(c.rho[icv0] * kine[icv0] * zeta, 0.0), 1.0);
+    }
+  }
+}
+
+// diffusivityHookScalarRansTurb of RansTurbKOmSST (TurbModel_KOMSST.h:160-254)
+void sst_diffusivity(or_ctx &c, int s) {
+  const or_mesh &m = c.m;
+  const double sg1 = s == 0 ? sst::sigma_k1 : sst::sigma_om1, sg2 = s == 0 ? sst::sigma_k2 : sst::sigma_om2;
+  double *diff = &c.diff[(size_t)c.nfa * s];
+  for (int ifa = c.nfa_b; ifa < c.nfa; ifa++) {
+    int icv0 = m.cvofa[2 * ifa], icv1 = m.cvofa[2 * ifa + 1];
+    double w0, w1;
+    gg_weights(m, ifa, icv0, icv1, w0, w1);
+    double f1_fa = (w1 * c.f1[icv0] + w0 * c.f1[icv1]) / (w0 + w1);
+    double coeff = sg1 * f1_fa + (1. - f1_fa) * sg2;
+    diff[ifa] = c.mul_fa[ifa] + coeff * c.mut_fa[ifa];
+  }
+  for (int iz = 0; iz < m.nzone; iz++)
+    for (int ifa = m.zone_ifa[2 * iz]; ifa <= m.zone_ifa[2 * iz + 1]; ifa++) {
+      if (m.zone_bc[iz] == OR_BC_WALL) {
+        diff[ifa] = c.mul_fa[ifa];
+      } else {
+        int icv0 = m.cvofa[2 * ifa];
+        double coeff = sg1 * c.f1[icv0] + (1. - c.f1[icv0]) * sg2;
+        diff[ifa] = c.mul_fa[ifa] + coeff * c.mut_fa[ifa];
+      }
+    }
+}
+
+// sourceHookScalarRansTurb_new of RansTurbKOmSST (TurbModel_KOMSST.h:256-304)
+void sst_source(or_ctx &c, int s, bool implicit) {
+  const or_mesh &m = c.m;
+  const size_t n = c.ncv, nnz = c.nnz;
+  const double *kine = &c.phi[0], *omega = &c.phi[n];
+  const double *lim = c.p.sst_limiter_shear == 1 ? c.str_mag.data() : c.vort_mag.data();
+  double *rhs = &c.rhs_scal[n * s];
+  double *A = &c.AScal[(6 * (size_t)s + 5) * nnz];
+  const double alfa_1 = sst::beta_1 / sst::betaStar - sst::sigma_om1 * pow(0.41, 2.0) / sqrt(sst::betaStar);
+  const double alfa_2 = sst::beta_2 / sst::betaStar - sst::sigma_om2 * pow(0.41, 2.0) / sqrt(sst::betaStar);
+  for (int icv = 0; icv < c.ncv; icv++) {
+    const double rho = c.rho[icv], vol = m.cv_volume[icv];
+    if (s == 0) {
+      double zeta = min(1.0 / omega[icv], sst::a1 / (lim[icv] * c.f2[icv]));
+      double mut = min(max(rho * kine[icv] * zeta, 0.0), 1.0e5);
+      double Pk = mut * c.str_mag[icv] * c.str_mag[icv] - 2. / 3. * rho * kine[icv] * c.diverg[icv];
+      if (c.p.sst_limit_pk == 1) Pk = min(Pk, 20.0 * sst::betaStar * rho * kine[icv] * omega[icv]);
+      Pk = max(Pk, 0.0);
+      double src = Pk - sst::betaStar * rho * omega[icv] * kine[icv];
+      rhs[icv] += src * vol;
+      if (implicit) {
+        double dsrcdphi = -sst::betaStar * omega[icv];
+        A[m.nbocv_i[icv]] -= dsrcdphi * vol;
+      }
+    } else {
+      double F1 = c.f1[icv];
+      double alfa = F1 * alfa_1 + (1.0 - F1) * alfa_2;
+      double beta = F1 * sst::beta_1 + (1.0 - F1) * sst::beta_2;
+      double zeta = max(omega[icv], lim[icv] * c.f2[icv] / sst::a1);
+      double Pk = c.str_mag[icv] * c.str_mag[icv] - 2. / 3. * zeta * c.diverg[icv];
+      Pk = max(Pk, 0.0);
+      double src = alfa * rho * Pk + (1.0 - F1) * c.cross_diff[icv] - beta * rho * omega[icv] * omega[icv];
+      rhs[icv] += src * vol;
+      if (implicit) {
+        double dsrcdphi = -2.0 * beta * omega[icv];
+        A[m.nbocv_i[icv]] -= dsrcdphi * vol;
+      }
     }
   }
 }
@@ -2479,18 +2717,20 @@ void sa_diffusivity(or_ctx &c) {
                                                 : invSigma * (c.mul_fa[ifa] + c.rho_bfa[ifa] * c.phi_bfa[ifa]);
 }
 
-// calcViscousFluxScalar_new, Scalars.cpp:117-247, with the SA diffusivity
-// (diffusivityHookScalarRansTurb, TurbModel_SA.h:130-172)
-void calc_viscous_flux_scalar(or_ctx &c, bool implicit) {
+// calcViscousFluxScalar_new, Scalars.cpp:117-247, for scalar s with the model's
+// diffusivity (diffusivityHookScalarRansTurb, TurbModel_SA.h:130-172,
+// TurbModel_KOMSST.h:160-254)
+void calc_viscous_flux_scalar(or_ctx &c, bool implicit, int s = 0) {
   const or_mesh &m = c.m;
-  const size_t nnz = c.nnz;
-  double *A5 = &c.AScal[5 * nnz];
-  if (c.p.grad_scheme == OR_GRAD_LS)
-    grad_scalar_ls(c, c.grad_phi.data(), c.phi.data(), c.phi_bfa.data());
-  else
-    grad_scalar_gg(c, c.grad_phi.data(), c.phi.data(), c.phi_bfa.data());
-  if (c.p.limiter) limit_scalar(c, c.grad_phi.data(), c.phi.data(), c.phi.data(), NULL);
-  sa_diffusivity(c);
+  const size_t nnz = c.nnz, n = c.ncv;
+  double *A5 = &c.AScal[(6 * (size_t)s + 5) * nnz];
+  scalar_grad(c, s);
+  if (c.sst) sst_diffusivity(c, s);
+  else sa_diffusivity(c);
+  const double *phi = &c.phi[n * s], *phi_bfa = &c.phi_bfa[(size_t)c.nfa_b * s], *grad_phi = &c.grad_phi[3 * n * s];
+  const double *diff = &c.diff[(size_t)c.nfa * s];
+  double *rhs_scal = &c.rhs_scal[n * s];
+  const double *zs = s == 0 ? m.zone_scal : m.zone_scal2;
   for (int ifa = c.nfa_b; ifa < c.nfa; ifa++) {
     int icv0 = m.cvofa[2 * ifa], icv1 = m.cvofa[2 * ifa + 1];
     double n[3], sv[3];
@@ -2499,35 +2739,35 @@ void calc_viscous_flux_scalar(or_ctx &c, bool implicit) {
     double smag = sqrt(dot3(sv, sv));
     for (int i = 0; i < 3; i++) sv[i] /= smag;
     double alpha = dot3(n, sv);
-    const double *g0 = &c.grad_phi[3 * icv0], *g1 = &c.grad_phi[3 * icv1];
-    double viscFlux = c.diff[ifa] * nmag *
-                      (alpha * (c.phi[icv1] - c.phi[icv0]) / smag + 0.5 * (g0[0] + g1[0]) * (n[0] - alpha * sv[0]) +
+    const double *g0 = &grad_phi[3 * icv0], *g1 = &grad_phi[3 * icv1];
+    double viscFlux = diff[ifa] * nmag *
+                      (alpha * (phi[icv1] - phi[icv0]) / smag + 0.5 * (g0[0] + g1[0]) * (n[0] - alpha * sv[0]) +
                        0.5 * (g0[1] + g1[1]) * (n[1] - alpha * sv[1]) + 0.5 * (g0[2] + g1[2]) * (n[2] - alpha * sv[2]));
-    c.rhs_scal[icv0] += viscFlux;
-    c.rhs_scal[icv1] -= viscFlux;
+    rhs_scal[icv0] += viscFlux;
+    rhs_scal[icv1] -= viscFlux;
     if (implicit) {
       int noc00 = m.nbocv_i[icv0], noc01 = noc00;
       while (m.nbocv_v[++noc01] != icv1) {}
       int noc11 = m.nbocv_i[icv1], noc10 = noc11;
       while (m.nbocv_v[++noc10] != icv0) {}
-      A5[noc00] += c.diff[ifa] * nmag * alpha / smag / c.rho[icv0];
-      A5[noc01] += -c.diff[ifa] * nmag * alpha / smag / c.rho[icv1];
-      A5[noc11] += c.diff[ifa] * nmag * alpha / smag / c.rho[icv1];
-      A5[noc10] += -c.diff[ifa] * nmag * alpha / smag / c.rho[icv0];
+      A5[noc00] += diff[ifa] * nmag * alpha / smag / c.rho[icv0];
+      A5[noc01] += -diff[ifa] * nmag * alpha / smag / c.rho[icv1];
+      A5[noc11] += diff[ifa] * nmag * alpha / smag / c.rho[icv1];
+      A5[noc10] += -diff[ifa] * nmag * alpha / smag / c.rho[icv0];
     }
   }
-  // boundary zones with a nuSA value (Dirichlet, :201-219); others carry no diffusive flux
+  // boundary zones with a value or HOOK (:168-219); others carry no diffusive flux
   for (int iz = 0; iz < m.nzone; iz++) {
-    if (m.zone_scal[2 * iz] == 0.0) continue;
+    if (zs[2 * iz] == 0.0) continue;
     for (int ifa = m.zone_ifa[2 * iz]; ifa <= m.zone_ifa[2 * iz + 1]; ifa++) {
       int icv0 = m.cvofa[2 * ifa];
       double n[3], s_half[3];
       double nmag = norm3(n, &m.fa_normal[3 * ifa]);
       for (int i = 0; i < 3; i++) s_half[i] = m.x_fa[3 * ifa + i] - m.x_cv[3 * icv0 + i];
       double smag_half = fabs(dot3(s_half, n));
-      double viscFlux = c.diff[ifa] * nmag * (c.phi_bfa[ifa] - c.phi[icv0]) / smag_half;
-      c.rhs_scal[icv0] += viscFlux;
-      if (implicit) A5[m.nbocv_i[icv0]] += c.diff[ifa] * nmag / smag_half / c.rho[icv0];
+      double viscFlux = diff[ifa] * nmag * (phi_bfa[ifa] - phi[icv0]) / smag_half;
+      rhs_scal[icv0] += viscFlux;
+      if (implicit) A5[m.nbocv_i[icv0]] += diff[ifa] * nmag / smag_half / c.rho[icv0];
     }
   }
 }
@@ -2630,26 +2870,28 @@ int bcgstab_scalar(const or_ctx &c, double *phi, const double *Ap, const double 
 // AScal[0..4] moved to the right with dq, Jacobi BiCGStab, update and clip.
 void scalar_solve(or_ctx &c, bool update) {
   const or_mesh &m = c.m;
-  const size_t nnz = c.nnz;
-  for (int icv = 0; icv < c.ncv; icv++) {
-    c.rhs_scal[icv] *= c.p.relax_scal;
-    int noc_f = m.nbocv_i[icv], noc_l = m.nbocv_i[icv + 1] - 1;
-    double tmp = m.cv_volume[icv] / (c.local_dt[icv]);
-    c.AScal[5 * nnz + noc_f] += tmp;
-    for (int noc = noc_f; noc <= noc_l; noc++) {
-      const double *dqn = &c.dq[5 * m.nbocv_v[noc]];
-      c.rhs_scal[icv] = c.rhs_scal[icv] - c.AScal[0 * nnz + noc] * dqn[0] - c.AScal[1 * nnz + noc] * dqn[1] -
-                        c.AScal[2 * nnz + noc] * dqn[2] - c.AScal[3 * nnz + noc] * dqn[3] -
-                        c.AScal[4 * nnz + noc] * dqn[4];
-    }
-  }
+  const size_t nnz = c.nnz, n = c.ncv;
   std::fill(c.dscal.begin(), c.dscal.end(), 0.0);
-  bcgstab_scalar(c, c.dscal.data(), &c.AScal[5 * nnz], c.rhs_scal.data(), sa::phiZero, sa::phiZeroRel,
-                 sa::phiMaxiter);
-  if (update)
-    for (int icv = 0; icv < c.ncv; icv++)
-      c.phi[icv] = min(max((c.phi[icv] * (c.rho[icv] - c.dq[5 * icv]) + c.dscal[icv]) / c.rho[icv], sa::lowerBound),
-                       sa::upperBound);
+  for (int s = 0; s < c.nsc; s++) {  // in scalarTranspEqVector order
+    double *rhs_scal = &c.rhs_scal[n * s], *AS = &c.AScal[6 * nnz * s], *dscal = &c.dscal[n * s], *phi = &c.phi[n * s];
+    for (int icv = 0; icv < c.ncv; icv++) {
+      rhs_scal[icv] *= c.p.relax_scal;
+      int noc_f = m.nbocv_i[icv], noc_l = m.nbocv_i[icv + 1] - 1;
+      double tmp = m.cv_volume[icv] / (c.local_dt[icv]);
+      AS[5 * nnz + noc_f] += tmp;
+      for (int noc = noc_f; noc <= noc_l; noc++) {
+        const double *dqn = &c.dq[5 * m.nbocv_v[noc]];
+        rhs_scal[icv] = rhs_scal[icv] - AS[0 * nnz + noc] * dqn[0] - AS[1 * nnz + noc] * dqn[1] -
+                        AS[2 * nnz + noc] * dqn[2] - AS[3 * nnz + noc] * dqn[3] - AS[4 * nnz + noc] * dqn[4];
+      }
+    }
+    const double zero = c.sst ? sst::phiZero : sa::phiZero, zeroRel = c.sst ? sst::phiZeroRel : sa::phiZeroRel;
+    const double lo = c.sst ? sst::lowerBound[s] : sa::lowerBound, hi = c.sst ? sst::upperBound[s] : sa::upperBound;
+    bcgstab_scalar(c, dscal, &AS[5 * nnz], rhs_scal, zero, zeroRel, c.sst ? sst::phiMaxiter : sa::phiMaxiter);
+    if (update)
+      for (int icv = 0; icv < c.ncv; icv++)
+        phi[icv] = min(max((phi[icv] * (c.rho[icv] - c.dq[5 * icv]) + dscal[icv]) / c.rho[icv], lo), hi);
+  }
 }
 
 void calc_rhs(or_ctx &c, double *A) {
@@ -2658,7 +2900,7 @@ void calc_rhs(or_ctx &c, double *A) {
     for (int i = 0; i < 3; i++) c.rhs_rhou[3 * icv + i] = 0.0;
     c.rhs_rhoE[icv] = 0.0;
   }
-  if (c.sa) std::fill(c.rhs_scal.begin(), c.rhs_scal.end(), 0.0);
+  if (c.nsc) std::fill(c.rhs_scal.begin(), c.rhs_scal.end(), 0.0);
   if (c.p.coupled) {  // calcRhsCoupled, JoeWithModels.cpp:4596-4611
     calc_flux_coupled(c, A);
   } else {
@@ -2691,6 +2933,11 @@ void calc_rhs(or_ctx &c, double *A) {
     if (c.p.mu_ref > 0.0 && !c.p.coupled) calc_viscous_flux_scalar(c, A != NULL);
     sa_source(c, A != NULL);
   }
+  if (c.sst)
+    for (int s = 0; s < 2; s++) {
+      if (c.p.mu_ref > 0.0) calc_viscous_flux_scalar(c, A != NULL, s);
+      sst_source(c, s, A != NULL);
+    }
 }
 
 void mat_vec(const or_ctx &c, double *res, const double *A, const double *phi) {  // UgpWithCv.cpp:2580-2596
@@ -2945,6 +3192,10 @@ void refresh(or_ctx &c) {
     set_scalar_bc(c);
     calc_turb_visc(c);
   }
+  if (c.sst) {
+    set_scalar_bc(c);
+    calc_turb_visc_sst(c);
+  }
 }
 
 }  // namespace
@@ -3011,22 +3262,34 @@ or_ctx *or_ctx_create(const or_mesh *m, const or_params *p) {
   c->mul_fa.assign(m->nfa, 0.0);
   c->lamOcp_fa.assign(m->nfa, 0.0);
   c->grad_h.assign(3 * n, 0.0);
-  c->resid.assign(p->sa ? 6 : 5, 0.0);
+  c->sst = p->sst != 0;
+  c->nsc = p->sa ? 1 : (p->sst ? 2 : 0);
+  c->resid.assign(5 + c->nsc, 0.0);
   c->mut_fa.assign(m->nfa, 0.0);
   c->flux_fa.assign(5 * (size_t)m->nfa, 0.0);
   c->sa = p->sa != 0;
-  if (c->sa) {
-    c->phi.assign(n, p->sa_init);  // initialHook, JoeWithModels.cpp:219-240
-    c->phi_bfa.assign(nb, 0.0);
-    c->grad_phi.assign(3 * n, 0.0);
-    c->diff.assign(m->nfa, 0.0);
+  if (c->nsc) {
+    const size_t ns = c->nsc;
+    c->phi.assign(ns * n, 0.0);  // initialHook, JoeWithModels.cpp:219-240
+    for (size_t s = 0; s < ns; s++)
+      std::fill(c->phi.begin() + s * n, c->phi.begin() + (s + 1) * n, c->sst ? p->sst_init[s] : p->sa_init);
+    c->phi_bfa.assign(ns * nb, 0.0);
+    c->grad_phi.assign(3 * ns * n, 0.0);
+    c->diff.assign(ns * m->nfa, 0.0);
     c->wall_dist.assign(n, 0.0);
     c->str_mag.assign(n, 0.0);
     c->vort_mag.assign(n, 0.0);
-    c->rhs_scal.assign(n, 0.0);
-    c->AScal.assign(6 * (size_t)m->nnz, 0.0);
-    c->dscal.assign(n, 0.0);
-    calc_wall_distance(*c);  // initialHookScalarRansTurbModel, TurbModel_SA.h:62-72
+    c->rhs_scal.assign(ns * n, 0.0);
+    c->AScal.assign(6 * ns * (size_t)m->nnz, 0.0);
+    c->dscal.assign(ns * n, 0.0);
+    if (c->sst) {
+      c->diverg.assign(n, 0.0);
+      c->cross_diff.assign(n, 0.0);
+      c->f1.assign(n, 0.0);
+      c->f2.assign(n, 0.0);
+    }
+    // initialHookScalarRansTurbModel, TurbModel_SA.h:62-72, TurbModel_KOMSST.h:82-98
+    calc_wall_distance(*c);
   }
   c->cfl = p->cfl;
   c->step = 0;
@@ -3034,7 +3297,7 @@ or_ctx *or_ctx_create(const or_mesh *m, const or_params *p) {
 }
 
 void or_set_scalar(or_ctx *c, const double *phi) {
-  if (c->sa) std::memcpy(c->phi.data(), phi, sizeof(double) * c->ncv);
+  if (c->nsc) std::memcpy(c->phi.data(), phi, sizeof(double) * c->ncv * c->nsc);
 }
 
 void or_ctx_destroy(or_ctx *c) { delete c; }
@@ -3055,7 +3318,7 @@ void or_residual(or_ctx *c, int implicit) {
   refresh(*c);
   calc_dt(*c);
   if (implicit) std::fill(c->A.begin(), c->A.end(), 0.0);
-  if (c->sa) std::fill(c->AScal.begin(), c->AScal.end(), 0.0);
+  if (c->nsc) std::fill(c->AScal.begin(), c->AScal.end(), 0.0);
   calc_rhs(*c, implicit ? c->A.data() : NULL);
   if (implicit && c->sa && c->p.coupled) assemble_blocks6(*c, c->A6);
 }
@@ -3068,7 +3331,7 @@ static int step_once(or_ctx &c) {
   calc_dt(c);
   std::fill(c.A.begin(), c.A.end(), 0.0);
   std::fill(c.dq.begin(), c.dq.end(), 0.0);
-  if (c.sa) std::fill(c.AScal.begin(), c.AScal.end(), 0.0);
+  if (c.nsc) std::fill(c.AScal.begin(), c.AScal.end(), 0.0);
   calc_rhs(c, c.A.data());
   if (c.sa && c.p.coupled) {  // runBackwardEulerCoupled with the scalar (JoeWithModels.cpp:4437-4516)
     for (size_t i = 0; i < c.resid.size(); i++) c.resid[i] = 0.0;
@@ -3113,7 +3376,7 @@ static int step_once(or_ctx &c) {
     c.rhou[3 * icv + 2] += c.dq[5 * icv + 3];
     c.rhoE[icv] += c.dq[5 * icv + 4];
   }
-  if (c.sa) scalar_solve(c, true);
+  if (c.nsc) scalar_solve(c, true);
   refresh(c);
   for (size_t i = 0; i < c.resid.size(); i++) c.resid[i] = 0.0;
   for (int icv = 0; icv < c.ncv; icv++) {
@@ -3121,8 +3384,8 @@ static int step_once(or_ctx &c) {
     for (int i = 0; i < 3; i++) c.resid[i + 1] += fabs(c.rhs_rhou[3 * icv + i]);
     c.resid[4] += fabs(c.rhs_rhoE[icv]);
   }
-  if (c.sa)  // JoeWithModels.cpp:1132-1134
-    for (int icv = 0; icv < c.ncv; icv++) c.resid[5] += fabs(c.rhs_scal[icv] / c.p.under_relax);
+  for (int s = 0; s < c.nsc; s++)  // JoeWithModels.cpp:1132-1134
+    for (int icv = 0; icv < c.ncv; icv++) c.resid[5 + s] += fabs(c.rhs_scal[(size_t)c.ncv * s + icv] / c.p.under_relax);
   return it;
 }
 
@@ -3204,6 +3467,32 @@ long long or_get(const or_ctx *c, const char *name, double *out) {
     else if (n == "RHS_NUSA") v = &c->rhs_scal;
     else if (n == "ASCAL") v = &c->AScal;
     else if (n == "DSCAL") v = &c->dscal;
+  } else if (c->sst) {
+    static thread_local std::vector<double> part;
+    const size_t nc = c->ncv, nb = c->nfa_b, nf = c->nfa;
+    auto half = [&](const std::vector<double> &src, size_t len, int s) {
+      part.assign(src.begin() + len * s, src.begin() + len * (s + 1));
+      return &part;
+    };
+    if (n == "KINE") v = half(c->phi, nc, 0);
+    else if (n == "OMEGA") v = half(c->phi, nc, 1);
+    else if (n == "KINE_BFA") v = half(c->phi_bfa, nb, 0);
+    else if (n == "OMEGA_BFA") v = half(c->phi_bfa, nb, 1);
+    else if (n == "GRAD_KINE") v = half(c->grad_phi, 3 * nc, 0);
+    else if (n == "GRAD_OMEGA") v = half(c->grad_phi, 3 * nc, 1);
+    else if (n == "DIFF_KINE") v = half(c->diff, nf, 0);
+    else if (n == "DIFF_OMEGA") v = half(c->diff, nf, 1);
+    else if (n == "RHS_KINE") v = half(c->rhs_scal, nc, 0);
+    else if (n == "RHS_OMEGA") v = half(c->rhs_scal, nc, 1);
+    else if (n == "WALL_DIST") v = &c->wall_dist;
+    else if (n == "STR_MAG") v = &c->str_mag;
+    else if (n == "VORT_MAG") v = &c->vort_mag;
+    else if (n == "DIVERG") v = &c->diverg;
+    else if (n == "CROSS_DIFF") v = &c->cross_diff;
+    else if (n == "F1") v = &c->f1;
+    else if (n == "F2") v = &c->f2;
+    else if (n == "ASCAL") v = &c->AScal;
+    else if (n == "DSCAL") v = &c->dscal;
   }
   if (!v) return -1;
   if (out) std::memcpy(out, v->data(), sizeof(double) * v->size());
@@ -3228,7 +3517,7 @@ int or_linear_solve(or_ctx *c) {
   }
   std::fill(c->dq.begin(), c->dq.end(), 0.0);
   int it = bcgstab(*c, c->dq.data(), c->A.data(), rhs.data(), c->p.ls_abs, c->p.ls_rel, c->p.ls_maxiter);
-  if (c->sa) scalar_solve(*c, false);  // DSCAL: the scalar correction of that first step
+  if (c->nsc) scalar_solve(*c, false);  // DSCAL: the scalar corrections of that first step
   return it;
 }
 

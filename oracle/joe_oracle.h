@@ -42,7 +42,9 @@ typedef struct {
   const int *zone_bc;        /* OR_BC_* */
   const int *zone_ifa;       /* [nzone][2] first, last face */
   const double *zone_par;    /* [nzone][8] keyword values (see or_ctx_create) */
-  const double *zone_scal;   /* [nzone][2] nuSA boundary: kind (0 none, 1 value), value; NULL without SA */
+  const double *zone_scal;   /* [nzone][2] first scalar's boundary (nuSA, or kine with SST): kind (0 none,
+                              1 value, 2 HOOK), value; NULL without a turbulence model */
+  const double *zone_scal2;  /* [nzone][2] the same for omega (SST); NULL otherwise */
   /* the boundary-profile file of PROFILES zones (JoeWithModels.cpp:262-290):
    * nprof rows of nval_prof values (y, rho, u, v, p, scalars); NULL without */
   const double *profiles;
@@ -79,6 +81,13 @@ typedef struct {
    * Jacobian (nScal = 0, MU_REF = 0, HLLC); the loop is runBackwardEuler's
    * with the block-Jacobi BiCGStab in place of PETSc GMRES */
   int coupled;
+  /* Menter SST k-omega (RansTurbKOmSST, turbModels/TurbModel_KOMSST.h) as two
+   * segregated scalars kine, omega (registration order): initial values
+   * (INITIAL_CONDITION_TURB), SST_LIMIT_PK, SST_LIMITER_SHEARSTRESS; the model
+   * constants are the reference's defaults (:16-23) */
+  int sst;
+  double sst_init[2];
+  int sst_limit_pk, sst_limiter_shear;
 } or_params;
 
 typedef struct or_ctx or_ctx;
@@ -90,7 +99,7 @@ void or_set_state(or_ctx *c, const double *rho, const double *rhou, const double
 void or_get_state(const or_ctx *c, double *rho, double *rhou, double *rhoE);
 /* calcStateVariables + setBC + calcDt + calcRhs (implicit: also A). */
 void or_residual(or_ctx *c, int implicit);
-/* the nuSA field (SA only), reference numbering */
+/* the transported scalars, reference numbering: nuSA [ncv] (SA) or kine, omega [2][ncv] (SST) */
 void or_set_scalar(or_ctx *c, const double *phi);
 /* one backward-Euler step (JoeWithModels.cpp:975-1158); returns BiCGStab iterations */
 int or_step(or_ctx *c);
@@ -102,7 +111,9 @@ int or_run(or_ctx *c, int max_steps, double *resid_hist /* [max_steps][5 + nScal
  * ALPHA_RHO VEL PRESS TEMP ENTHALPY SOS RHO_BFA VEL_BFA P_BFA T_BFA H_BFA DQ
  * RESID (5 + nScal) MUL_FA GRAD_ENTHALPY; with SA also NUSA NUSA_BFA
  * GRAD_NUSA MUT_FA WALL_DIST STR_MAG VORT_MAG DIFF_NUSA RHS_NUSA ASCAL
- * ([6][nnz]) DSCAL; returns element count or -1 */
+ * ([6][nnz]) DSCAL; with SST KINE OMEGA KINE_BFA OMEGA_BFA GRAD_KINE GRAD_OMEGA
+ * DIFF_KINE DIFF_OMEGA RHS_KINE RHS_OMEGA MUT_FA WALL_DIST STR_MAG VORT_MAG DIVERG
+ * CROSS_DIFF F1 F2 ASCAL ([2][6][nnz]) DSCAL ([2][ncv]); returns element count or -1 */
 long long or_get(const or_ctx *c, const char *name, double *out);
 /* current CFL and step counter */
 double or_cfl(const or_ctx *c);

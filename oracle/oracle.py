@@ -42,7 +42,8 @@ class _Mesh(C.Structure):
                 ("x_fa", C.c_void_p), ("fa_normal", C.c_void_p), ("x_cv", C.c_void_p), ("cv_volume", C.c_void_p),
                 ("nbocv_lsg", C.c_void_p), ("bfa_lsg", C.c_void_p),
                 ("nzone", C.c_int), ("zone_bc", C.c_void_p), ("zone_ifa", C.c_void_p), ("zone_par", C.c_void_p),
-                ("zone_scal", C.c_void_p), ("profiles", C.c_void_p), ("nprof", C.c_int), ("nval_prof", C.c_int)]
+                ("zone_scal", C.c_void_p), ("zone_scal2", C.c_void_p), ("profiles", C.c_void_p),
+                ("nprof", C.c_int), ("nval_prof", C.c_int)]
 
 
 class _Params(C.Structure):
@@ -55,7 +56,8 @@ class _Params(C.Structure):
                 ("visc_mode", C.c_int), ("mu_power_law", C.c_double), ("sl_tref", C.c_double),
                 ("sl_sref", C.c_double), ("T_ref", C.c_double), ("pr", C.c_double), ("pr_turb", C.c_double),
                 ("visc_limiter", C.c_int), ("sa", C.c_int), ("sa_init", C.c_double), ("relax_scal", C.c_double),
-                ("time_step_sum", C.c_int), ("coupled", C.c_int)]
+                ("time_step_sum", C.c_int), ("coupled", C.c_int), ("sst", C.c_int), ("sst_init", C.c_double * 2),
+                ("sst_limit_pk", C.c_int), ("sst_limiter_shear", C.c_int)]
 
 
 def lib():
@@ -174,6 +176,14 @@ def make_params(joe_in: str) -> tuple[_Params, dict]:
     if p.sa:
         p.sa_init = float(P["INITIAL_CONDITION_TURB"][0][1])
         p.relax_scal = float(get("UNDER_RELAXATION_SCALARS", "0.2"))
+    # Menter SST k-omega (RansTurbKOmSST): kine, omega initial values and options
+    p.sst = 1 if get("TURB_MODEL", "NONE") == "SST" else 0
+    if p.sst:
+        p.sst_init[0] = float(P["INITIAL_CONDITION_TURB"][0][1])
+        p.sst_init[1] = float(P["INITIAL_CONDITION_TURB"][0][2])
+        p.relax_scal = float(get("UNDER_RELAXATION_SCALARS", "0.2"))
+        p.sst_limit_pk = int(get("SST_LIMIT_PK", "1"))
+        p.sst_limiter_shear = int(get("SST_LIMITER_SHEARSTRESS", "0"))
     return p, P
 
 
@@ -209,15 +219,19 @@ class Oracle:
             else:
                 vals = [float(t) for t in toks[2:]]
                 zpar[i, :len(vals)] = vals
-        # "<zone>.nuSA = v": Dirichlet value of the SA scalar (scalarZoneIsDirichlet)
-        zscal = np.zeros((len(zl), 2), np.float64)
-        for i, (name, _k, f, l) in enumerate(zl):
-            key = name + ".nuSA"
-            if key in P:
-                zscal[i] = (1.0, float(P[key][0][1]))
-        self._keep.update(zbc=zbc, zifa=zifa, zpar=zpar, zscal=zscal)
+        # "<zone>.<scalar> = v" (scalarZoneIsDirichlet) or "= HOOK" (scalarZoneIsHook)
+        names = ["kine", "omega"] if self.params.sst else ["nuSA"]
+        zs = [np.zeros((len(zl), 2), np.float64) for _ in range(2)]
+        for s, sname in enumerate(names):
+            for i, (name, _k, f, l) in enumerate(zl):
+                key = name + "." + sname
+                if key in P:
+                    v = P[key][0][1]
+                    zs[s][i] = (2.0, 0.0) if v in ("HOOK", "Hook", "hook") else (1.0, float(v))
+        zscal, zscal2 = zs
+        self._keep.update(zbc=zbc, zifa=zifa, zpar=zpar, zscal=zscal, zscal2=zscal2)
         self.zone_names = [z[0] for z in zl]
-        self.nscal = 1 if self.params.sa else 0
+        self.nscal = 1 if self.params.sa else (2 if self.params.sst else 0)
         m = _Mesh()
         m.ncv, m.nfa, m.nfa_b, m.nnz = self.ncv, self.nfa, self.nfa_b, self.nnz
         m.cvofa = arr("CVOFA", np.int32)
@@ -236,6 +250,7 @@ class Oracle:
         m.zone_ifa = zifa.ctypes.data
         m.zone_par = zpar.ctypes.data
         m.zone_scal = zscal.ctypes.data
+        m.zone_scal2 = zscal2.ctypes.data
         if prof_file is not None:
             prof = np.ascontiguousarray(read_profiles(prof_file))
             if prof.shape[1] - 5 != self.nscal:

@@ -18,12 +18,16 @@
 //                                         the nuSA row and the model fields
 // rhs-sa / solve-sa run the same with the Spalart-Allmaras mix-in (RansTurbSA,
 // as joe.cpp's MyJoeSA); the state file may then carry NUSA.
+// rhs-sst / solve-sst run it with the Menter SST mix-in (RansTurbKOmSST, as
+// joe.cpp's MyJoeSST); the state file then carries KINE and OMEGA, and the
+// scalar records are named after each scalar (KINE_BFA, RHS_OMEGA, ...).
 //
 // The dump format is a sequence of records:
 //   char name[32]; int32 kind (0 = f64, 1 = i32, 2 = bytes); int32 pad;
 //   int64 count; payload.
 #include "JoeWithModels.h"
 #include "TurbModel_SA.h"
+#include "TurbModel_KOMSST.h"
 #include "myMem.h"
 
 #include <chrono>
@@ -114,14 +118,30 @@ class RefJoe : public JoeWithModels {
   // fields of a turbulence model mix-in (none here)
   virtual void dumpModel(Dump &d) {}
 
+  // the record name of transported scalar s: its registered name in capitals
+  // (NUSA; KINE, OMEGA)
+  std::string key(int s) {
+    std::string k = scalarTranspEqVector[s].getName();
+    for (char &ch : k) ch = (char)toupper((unsigned char)ch);
+    return k;
+  }
+
+  // every transported scalar from the state file
+  void loadScalars(const Load &st) {
+    for (int s = 0; s < (int)scalarTranspEqVector.size(); s++) {
+      const double *v = st.f64(key(s).c_str(), ncv);
+      for (int icv = 0; icv < ncv; icv++) scalarTranspEqVector[s].phi[icv] = v[icv];
+    }
+  }
+
   void dumpState(const char *suffix) {
     char n[32];
     snprintf(n, 32, "RHO%s", suffix);  snap->f64(n, rho, ncv);
     snprintf(n, 32, "RHOU%s", suffix); snap->f64(n, &rhou[0][0], 3LL * ncv);
     snprintf(n, 32, "RHOE%s", suffix); snap->f64(n, rhoE, ncv);
-    if (scalarTranspEqVector.size() > 0) {
-      snprintf(n, 32, "NUSA%s", suffix);
-      snap->f64(n, scalarTranspEqVector[0].phi, ncv);
+    for (int s = 0; s < (int)scalarTranspEqVector.size(); s++) {
+      snprintf(n, 32, "%s%s", key(s).c_str(), suffix);
+      snap->f64(n, scalarTranspEqVector[s].phi, ncv);
     }
   }
 
@@ -245,10 +265,7 @@ class RefJoe : public JoeWithModels {
       for (int i = 0; i < 3; i++) rhou[icv][i] = ru[3 * icv + i];
       rhoE[icv] = re[icv];
     }
-    if (scalarTranspEqVector.size() > 0) {
-      const double *nu = st.f64("NUSA", ncv);
-      for (int icv = 0; icv < ncv; icv++) scalarTranspEqVector[0].phi[icv] = nu[icv];
-    }
+    loadScalars(st);
     calcStateVariables();
     writeRestart(path);
   }
@@ -260,7 +277,7 @@ class RefJoe : public JoeWithModels {
     d.f64("RHO", rho, ncv);
     d.f64("RHOU", &rhou[0][0], 3LL * ncv);
     d.f64("RHOE", rhoE, ncv);
-    if (scalarTranspEqVector.size() > 0) d.f64("NUSA", scalarTranspEqVector[0].phi, ncv);
+    for (int s = 0; s < (int)scalarTranspEqVector.size(); s++) d.f64(key(s).c_str(), scalarTranspEqVector[s].phi, ncv);
   }
 
   // One coupled residual evaluation in the order runBackwardEulerCoupled uses
@@ -281,10 +298,7 @@ class RefJoe : public JoeWithModels {
     }
     const int nScal = scalarTranspEqVector.size();
     const int b = 5 + nScal;
-    if (nScal > 0) {
-      const double *nu = st.f64("NUSA", ncv);
-      for (int icv = 0; icv < ncv; icv++) scalarTranspEqVector[0].phi[icv] = nu[icv];
-    }
+    loadScalars(st);
     calcStateVariables();
     calcMaterialProperties();
     setBC();
@@ -306,14 +320,17 @@ class RefJoe : public JoeWithModels {
     d.f64("RHS_RHOU", vu.data(), 3LL * ncv);
     for (int icv = 0; icv < ncv; icv++) v[icv] = rhs[icv][4];
     d.f64("RHS_RHOE", v.data(), ncv);
-    if (nScal > 0) {  // the transported scalar's row of the coupled residual and its fields
-      for (int icv = 0; icv < ncv; icv++) v[icv] = rhs[icv][5];
-      d.f64("RHS_NUSA", v.data(), ncv);
-      ScalarTranspEq &eq = scalarTranspEqVector[0];
-      d.f64("NUSA", eq.phi, ncv);
-      d.f64("NUSA_BFA", eq.phi_bfa, nfa_b);
-      d.f64("GRAD_NUSA", &eq.grad_phi[0][0], 3LL * ncv);
-      d.f64("DIFF_NUSA", eq.diff, nfa);
+    if (nScal > 0) {  // the transported scalars' rows of the coupled residual and their fields
+      for (int s = 0; s < nScal; s++) {
+        for (int icv = 0; icv < ncv; icv++) v[icv] = rhs[icv][5 + s];
+        const std::string k = key(s);
+        d.f64(("RHS_" + k).c_str(), v.data(), ncv);
+        ScalarTranspEq &eq = scalarTranspEqVector[s];
+        d.f64(k.c_str(), eq.phi, ncv);
+        d.f64((k + "_BFA").c_str(), eq.phi_bfa, nfa_b);
+        d.f64(("GRAD_" + k).c_str(), &eq.grad_phi[0][0], 3LL * ncv);
+        d.f64(("DIFF_" + k).c_str(), eq.diff, nfa);
+      }
       d.f64("MUT_FA", mut_fa, nfa);
       dumpModel(d);
     }
@@ -353,10 +370,7 @@ class RefJoe : public JoeWithModels {
       rhoE[icv] = re[icv];
     }
     const int nScal = scalarTranspEqVector.size();
-    if (nScal > 0) {
-      const double *nu = st.f64("NUSA", ncv);
-      for (int icv = 0; icv < ncv; icv++) scalarTranspEqVector[0].phi[icv] = nu[icv];
-    }
+    loadScalars(st);
     calcStateVariables();
     calcMaterialProperties();
     setBC();
@@ -391,17 +405,20 @@ class RefJoe : public JoeWithModels {
     auto t1 = std::chrono::steady_clock::now();
     double secs = std::chrono::duration<double>(t1 - t0).count();
     if (nScal > 0) {
-      ScalarTranspEq &eq = scalarTranspEqVector[0];
-      d.f64("NUSA", eq.phi, ncv);
-      d.f64("NUSA_BFA", eq.phi_bfa, nfa_b);
-      d.f64("GRAD_NUSA", &eq.grad_phi[0][0], 3LL * ncv);
-      d.f64("DIFF_NUSA", eq.diff, nfa);
+      std::vector<double> as(6LL * nbocv_s * nScal);
+      for (int s = 0; s < nScal; s++) {
+        ScalarTranspEq &eq = scalarTranspEqVector[s];
+        const std::string k = key(s);
+        d.f64(k.c_str(), eq.phi, ncv);
+        d.f64((k + "_BFA").c_str(), eq.phi_bfa, nfa_b);
+        d.f64(("GRAD_" + k).c_str(), &eq.grad_phi[0][0], 3LL * ncv);
+        d.f64(("DIFF_" + k).c_str(), eq.diff, nfa);
+        d.f64(("RHS_" + k).c_str(), rhsScal[s], ncv);
+        for (int kk = 0; kk < 6; kk++)
+          for (int noc = 0; noc < nbocv_s; noc++) as[((size_t)s * 6 + kk) * nbocv_s + noc] = AScal[s][kk][noc];
+      }
       d.f64("MUT_FA", mut_fa, nfa);
-      d.f64("RHS_NUSA", rhsScal[0], ncv);
-      std::vector<double> as(6LL * nbocv_s);
-      for (int k = 0; k < 6; k++)
-        for (int noc = 0; noc < nbocv_s; noc++) as[(size_t)k * nbocv_s + noc] = AScal[0][k][noc];
-      d.f64("ASCAL", as.data(), 6LL * nbocv_s);
+      d.f64("ASCAL", as.data(), (long long)as.size());
       dumpModel(d);
     }
 
@@ -446,22 +463,26 @@ class RefJoe : public JoeWithModels {
     if (nScal > 0) {
       // the scalar half of that step (JoeWithModels.cpp:1054-1082), without the update
       double underRelaxScalars = getDoubleParam("UNDER_RELAXATION_SCALARS", "0.2");
-      ScalarTranspEq &eq = scalarTranspEqVector[0];
+      std::vector<double> ds((size_t)nScal * ncv);
       double *dScal = new double[ncv_g];
-      for (int icv = 0; icv < ncv_g; icv++) dScal[icv] = 0.0;
-      for (int icv = 0; icv < ncv; ++icv) {
-        rhsScal[0][icv] *= underRelaxScalars;
-        int noc_f = nbocv_i[icv];
-        int noc_l = nbocv_i[icv + 1] - 1;
-        double tmp = cv_volume[icv] / (local_dt[icv]);
-        AScal[0][5][noc_f] += tmp;
-        for (int noc = noc_f; noc <= noc_l; noc++)
-          rhsScal[0][icv] = rhsScal[0][icv] - AScal[0][0][noc] * dq[nbocv_v[noc]][0] -
-                            AScal[0][1][noc] * dq[nbocv_v[noc]][1] - AScal[0][2][noc] * dq[nbocv_v[noc]][2] -
-                            AScal[0][3][noc] * dq[nbocv_v[noc]][3] - AScal[0][4][noc] * dq[nbocv_v[noc]][4];
+      for (int s = 0; s < nScal; s++) {
+        ScalarTranspEq &eq = scalarTranspEqVector[s];
+        for (int icv = 0; icv < ncv_g; icv++) dScal[icv] = 0.0;
+        for (int icv = 0; icv < ncv; ++icv) {
+          rhsScal[s][icv] *= underRelaxScalars;
+          int noc_f = nbocv_i[icv];
+          int noc_l = nbocv_i[icv + 1] - 1;
+          double tmp = cv_volume[icv] / (local_dt[icv]);
+          AScal[s][5][noc_f] += tmp;
+          for (int noc = noc_f; noc <= noc_l; noc++)
+            rhsScal[s][icv] = rhsScal[s][icv] - AScal[s][0][noc] * dq[nbocv_v[noc]][0] -
+                              AScal[s][1][noc] * dq[nbocv_v[noc]][1] - AScal[s][2][noc] * dq[nbocv_v[noc]][2] -
+                              AScal[s][3][noc] * dq[nbocv_v[noc]][3] - AScal[s][4][noc] * dq[nbocv_v[noc]][4];
+        }
+        solveLinSysScalar(dScal, AScal[s][5], rhsScal[s], eq.phiZero, eq.phiZeroRel, eq.phiMaxiter, eq.getName());
+        for (int icv = 0; icv < ncv; icv++) ds[(size_t)s * ncv + icv] = dScal[icv];
       }
-      solveLinSysScalar(dScal, AScal[0][5], rhsScal[0], eq.phiZero, eq.phiZeroRel, eq.phiMaxiter, eq.getName());
-      d.f64("DSCAL", dScal, ncv);
+      d.f64("DSCAL", ds.data(), (long long)ds.size());
       delete[] dScal;
     }
 
@@ -481,6 +502,21 @@ class RefJoeSA : public RefJoe, public RansTurbSA {
   }
 };
 
+// the reference's own SST case class (joe.cpp:301-310)
+class RefJoeSST : public RefJoe, public RansTurbKOmSST {
+ public:
+  explicit RefJoeSST(char *name) : RefJoe(name), UgpWithCvCompFlow(name) {}
+  void dumpModel(Dump &d) {
+    d.f64("WALL_DIST", wallDist, ncv);
+    d.f64("STR_MAG", strMag, ncv);
+    d.f64("VORT_MAG", vortMag, ncv);
+    d.f64("DIVERG", diverg, ncv);
+    d.f64("CROSS_DIFF", crossDiff, ncv);
+    d.f64("F1", blendFuncF1, ncv);
+    d.f64("F2", blendFuncF2, ncv);
+  }
+};
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -493,8 +529,11 @@ int main(int argc, char **argv) {
   std::string mode = argv[1];
   const bool with_sa = mode.size() > 3 && mode.compare(mode.size() - 3, 3, "-sa") == 0;
   if (with_sa) mode.resize(mode.size() - 3);
+  const bool with_sst = mode.size() > 4 && mode.compare(mode.size() - 4, 4, "-sst") == 0;
+  if (with_sst) mode.resize(mode.size() - 4);
   try {
-    RefJoe *jp = with_sa ? new RefJoeSA(argv[2]) : new RefJoe(argv[2]);
+    RefJoe *jp = with_sa ? static_cast<RefJoe *>(new RefJoeSA(argv[2]))
+                         : (with_sst ? static_cast<RefJoe *>(new RefJoeSST(argv[2])) : new RefJoe(argv[2]));
     RefJoe &joe = *jp;
     if (mode == "init") {
       joe.initializeFromRestartFile(joe.getStringParam("RESTART"));

@@ -9,6 +9,8 @@ Fixtures (numpy .npz, inputs and the reference's outputs only):
                        coefficients and zones after Ugp/UgpWithCv init
   rhs_<name>_<sch>.npz random state, primitive and boundary states, local dt,
                        limited gradients, residual, Jacobian and one BiCGStab dq
+                       (with _sa / _sst: the turbulence scalars' rows, matrix
+                       rows, model fields and scalar corrections too)
   solve_<name>.npz     JoeWithModels::run() state snapshots (steps 1..5) and
                        the final state after NSTEPS
   coupled_<name>_hllc  random state and calcRhsCoupled's residual, Jacobian
@@ -39,7 +41,8 @@ MESH_KEYS = ["COUNTS", "X_NO", "NOOFA_I", "NOOFA_V", "CVOFA", "FAOCV_I", "FAOCV_
 RHS_KEYS = ["PRESS", "VEL", "RHO_BFA", "VEL_BFA", "P_BFA", "T_BFA", "H_BFA", "LOCAL_DT", "GRAD_RHO", "GRAD_P",
             "GRAD_U", "ALPHA_RHO", "RHS_RHO", "RHS_RHOU", "RHS_RHOE", "A", "DQ", "MUL_FA", "GRAD_ENTHALPY",
             "NUSA_BFA", "GRAD_NUSA", "DIFF_NUSA", "MUT_FA", "RHS_NUSA", "ASCAL", "DSCAL", "WALL_DIST", "STR_MAG",
-            "VORT_MAG"]
+            "VORT_MAG", "KINE_BFA", "OMEGA_BFA", "GRAD_KINE", "GRAD_OMEGA", "DIFF_KINE", "DIFF_OMEGA", "RHS_KINE",
+            "RHS_OMEGA", "DIVERG", "CROSS_DIFF", "F1", "F2"]
 
 # viscous variants: Joe.in lines that replace "MU_REF = 0.0"
 VISC = {
@@ -79,7 +82,14 @@ RHS_CASES = [("bump24", "HLLC", {}), ("bump24", "ROE", {}), ("bump24", "AUSM", {
              # PROFILES boundary condition (inlet state interpolated from a profile file)
              ("bump24", "HLLC", {"prof": True}),
              ("box4", "ROE", {"prof": True}),
-             ("bump24", "HLLC", {"visc": "pl", "sa": True, "prof": True})]
+             ("bump24", "HLLC", {"visc": "pl", "sa": True, "prof": True}),
+             # Menter SST k-omega (segregated kine, omega)
+             ("bump24", "HLLC", {"visc": "pl", "sst": True}),
+             ("box4", "HLLC", {"visc": "su", "sst": True}),
+             ("box4", "HLLC", {"visc": "pl", "sst": True, "bcs": cases.BOX_BCS_ALT, "symm_bc": "ANALYTICAL",
+                               "grad": "GREENGAUSS"}),
+             ("box4", "HLLC", {"visc": "nl", "sst": True, "second_order": False}),
+             ("box4", "HLLC", {"visc": "hi", "sst": True, "dtsum": True})]
 
 
 # runBackwardEulerCoupled's residual: SYMM_BC absent (the key the reference's
@@ -159,6 +169,8 @@ def tag(kw):
     t = "_visc_" + kw["visc"] if "visc" in kw else ""
     if kw.get("sa"):
         t += "_sa"
+    if kw.get("sst"):
+        t += "_sst"
     if "bcs" in kw:
         t += "_alt_" + {"NEW_NUMERIC": "new", "ANALYTICAL": "ana"}[kw["symm_bc"]]
     if kw.get("grad") == "GREENGAUSS":
@@ -173,8 +185,8 @@ def tag(kw):
 
 
 def main(only=None):
-    """All fixtures, or (only = a substring) just the residual fixtures whose
-    file name holds it, leaving the others as they are."""
+    """All fixtures, or (only = a substring) just the residual, restart and
+    trajectory fixtures whose file name holds it, leaving the others as they are."""
     if not os.path.exists(REF):
         raise SystemExit("oracle/_ref/joe_ref_dump is not built (make -C oracle ref, needs the reference tree)")
     with tempfile.TemporaryDirectory() as wd:
@@ -183,6 +195,7 @@ def main(only=None):
             kw = dict(kw)
             visc = kw.pop("visc", None)
             with_sa = kw.pop("sa", False)
+            with_sst = kw.pop("sst", False)
             dtsum = kw.pop("dtsum", False)
             prof = kw.pop("prof", False)
             mm, joe = MESHES[name](scheme, **kw)
@@ -190,7 +203,7 @@ def main(only=None):
                 kw["dtsum"] = True
                 joe += "TIME_STEP = SUM_BASED\n"
             tk = dict(kw, **({"visc": visc} if visc else {}), **({"sa": True} if with_sa else {}),
-                      **({"prof": True} if prof else {}))
+                      **({"sst": True} if with_sst else {}), **({"prof": True} if prof else {}))
             fixture = f"rhs_{name}_{scheme.lower()}{tag(tk)}"
             if only and only not in fixture:
                 continue
@@ -207,6 +220,9 @@ def main(only=None):
             if with_sa:
                 kw["sa"] = True
                 joe = cases.with_sa(joe, 0.005, cases.SA_BUMP if name.startswith("bump") else cases.SA_BOX)
+            if with_sst:
+                kw["sst"] = True
+                joe = cases.with_sst(joe, 0.001, 5.0, cases.SST_BUMP if name.startswith("bump") else cases.SST_BOX)
             fn = "box.msh" if name.startswith("box") else "bump.msh"
             meshgen.write_msh(mm, os.path.join(wd, fn))
             with open(os.path.join(wd, "Joe.in"), "w") as f:
@@ -215,8 +231,13 @@ def main(only=None):
             state = {"RHO": rho, "RHOU": rhou.reshape(-1), "RHOE": rhoE}
             if with_sa:
                 state["NUSA"] = np.random.default_rng(11).uniform(0.001, 0.01, mm.ncv)
+            if with_sst:
+                rng = np.random.default_rng(17)
+                state["KINE"] = rng.uniform(1.0e-4, 5.0e-3, mm.ncv)
+                state["OMEGA"] = rng.uniform(0.5, 20.0, mm.ncv)
             jmdump.write(os.path.join(wd, "state.bin"), state)
-            run_ref(["rhs-sa" if with_sa else "rhs", "Joe.in", "state.bin", "out.bin"], wd)
+            mode = "rhs-sa" if with_sa else ("rhs-sst" if with_sst else "rhs")
+            run_ref([mode, "Joe.in", "state.bin", "out.bin"], wd)
             d = jmdump.read(os.path.join(wd, "out.bin"))
             if name not in done_mesh and not only:
                 np.savez_compressed(os.path.join(HERE, f"mesh_{name}.npz"), **{k: d[k] for k in MESH_KEYS})
@@ -225,6 +246,8 @@ def main(only=None):
             out.update(RHO=rho, RHOU=rhou.reshape(-1), RHOE=rhoE, JOE_IN=np.frombuffer(joe.encode(), np.uint8))
             if with_sa:
                 out["NUSA"] = state["NUSA"]
+            if with_sst:
+                out["KINE"], out["OMEGA"] = state["KINE"], state["OMEGA"]
             if tab is not None:
                 out["PROFILE"] = tab
             np.savez_compressed(os.path.join(HERE, fixture + ".npz"), **out)
@@ -360,22 +383,26 @@ def main(only=None):
             keep["JOE_IN"] = np.frombuffer(joe.encode(), np.uint8)
             np.savez_compressed(os.path.join(HERE, "steady_bump96.npz"), **keep)
             print("wrote steady_bump96, moved", keep["MOVED"])
-        if only:
-            return
         # steady-loop trajectories of JoeWithModels::run (uniform initial state)
         for name, nx, ny, nsteps in [("bump24", 24, 8, 60), ("bump96", 96, 32, 300), ("bump24visc", 24, 8, 60),
-                                     ("bump24sa", 24, 8, 60)]:
+                                     ("bump24sa", 24, 8, 60), ("bump24sst", 24, 8, 60)]:
+            if only and only not in f"solve_{name}":
+                continue
             mm, joe = cases.bump_case(nx, ny, nsteps=nsteps)
-            if name.endswith(("visc", "sa")):
+            if name.endswith(("visc", "sa", "sst")):
                 joe = joe.replace("MU_REF = 0.0", VISC["pl"])
             if name.endswith("sa"):
                 joe = cases.with_sa(joe, 0.006, cases.SA_BUMP)
+            if name.endswith("sst"):
+                joe = cases.with_sst(joe, 0.001, 5.0, cases.SST_BUMP)
             meshgen.write_msh(mm, os.path.join(wd, "bump.msh"))
             with open(os.path.join(wd, "Joe.in"), "w") as f:
                 f.write(joe)
-            run_ref(["solve-sa" if name.endswith("sa") else "solve", "Joe.in", "solve.bin", "5"], wd)
+            mode = "solve-sa" if name.endswith("sa") else ("solve-sst" if name.endswith("sst") else "solve")
+            run_ref([mode, "Joe.in", "solve.bin", "5"], wd)
             d = jmdump.read(os.path.join(wd, "solve.bin"))
-            keep = {k: v for k, v in d.items() if k.split("@")[0] in ("RHO", "RHOU", "RHOE", "NUSA", "STEP")
+            keep = {k: v for k, v in d.items() if k.split("@")[0] in ("RHO", "RHOU", "RHOE", "NUSA", "KINE", "OMEGA",
+                                                                          "STEP")
                     and (k.endswith(("@1", "@2", "@3", "@4", "@5")) or "@" not in k)}
             keep["JOE_IN"] = np.frombuffer(joe.encode(), np.uint8)
             np.savez_compressed(os.path.join(HERE, f"solve_{name}.npz"), **keep)

@@ -25,6 +25,11 @@ VISC_FIELDS = ["MUL_FA", "GRAD_ENTHALPY"]
 # Spalart-Allmaras: model fields, the nuSA boundary values, gradient,
 # diffusivity, eddy viscosity, residual and matrix rows AScal[6][nnz]
 SA_FIELDS = ["WALL_DIST", "STR_MAG", "VORT_MAG", "MUT_FA", "NUSA_BFA", "GRAD_NUSA", "DIFF_NUSA", "RHS_NUSA", "ASCAL"]
+# Menter SST: the model fields (divergence, cross diffusion, blending functions
+# F1 and F2), kine and omega boundary values (omega's wall hook included),
+# gradients, diffusivities, eddy viscosity, residuals and AScal[2][6][nnz]
+SST_FIELDS = ["WALL_DIST", "STR_MAG", "VORT_MAG", "DIVERG", "CROSS_DIFF", "F1", "F2", "MUT_FA", "KINE_BFA",
+              "OMEGA_BFA", "GRAD_KINE", "GRAD_OMEGA", "DIFF_KINE", "DIFF_OMEGA", "RHS_KINE", "RHS_OMEGA", "ASCAL"]
 
 
 def load(name):
@@ -79,6 +84,9 @@ def test_fixture_set_complete():
     sa = [f for f in RHS_FIXTURES if "_sa" in f]
     assert len(sa) >= 4 and any("gg" in f for f in sa) and any("first" in f for f in sa)
     assert any(f.startswith("bump") for f in sa) and any("alt" in f for f in sa)
+    sst = [f for f in RHS_FIXTURES if "_sst" in f]
+    assert len(sst) >= 4 and any("gg" in f for f in sst) and any("first" in f for f in sst)
+    assert any(f.startswith("bump") for f in sst) and any("alt" in f for f in sst) and any("dtsum" in f for f in sst)
 
 
 @pytest.mark.parametrize("fixture", RHS_FIXTURES)
@@ -86,12 +94,15 @@ def test_residual_bit_exact(fixture):
     g = load("rhs_" + fixture)
     o = Oracle(mesh_for(fixture), joe_text(g))
     o.set_state(g["RHO"], g["RHOU"], g["RHOE"])
-    sa = "_sa" in fixture
+    sa, sst = "_sa" in fixture, "_sst" in fixture
     if sa:
         o.set_scalar(g["NUSA"])
+    if sst:
+        o.set_scalar(np.concatenate([g["KINE"], g["OMEGA"]]))
     o.residual(True)
     first = "first" in fixture
-    for k in FIELDS + (VISC_FIELDS if "visc" in fixture else []) + (SA_FIELDS if sa else []):
+    for k in (FIELDS + (VISC_FIELDS if "visc" in fixture else []) + (SA_FIELDS if sa else [])
+              + (SST_FIELDS if sst else [])):
         if k not in g:
             assert first and k.startswith(("GRAD", "ALPHA")), k
             continue
@@ -101,7 +112,7 @@ def test_residual_bit_exact(fixture):
         assert nbad == 0, f"{fixture}:{k}: {nbad} of {got.size} differ; max |d| {np.max(np.abs(got - g[k])):.3e}"
     o.linear_solve()
     assert np.array_equal(o.get("DQ"), g["DQ"]), fixture
-    if sa:  # the scalar half of the step: relaxation, coupling columns, Jacobi BiCGStab
+    if sa or sst:  # the scalar half of the step: relaxation, coupling columns, Jacobi BiCGStab
         assert np.array_equal(o.get("DSCAL"), g["DSCAL"]), fixture
 
 
@@ -129,12 +140,12 @@ def test_sa_fields_physical():
     assert np.all(o.get("STR_MAG") >= 0.0) and np.all(o.get("VORT_MAG") >= 0.0)
 
 
-@pytest.mark.parametrize("name", ["bump24", "bump96", "bump24visc", "bump24sa"])
+@pytest.mark.parametrize("name", ["bump24", "bump96", "bump24visc", "bump24sa", "bump24sst"])
 def test_run_trajectory_bit_exact(name):
     g = load("solve_" + name)
     joe = joe_text(g)
     P = parse_joe_in(joe)
-    m = load("mesh_" + name.replace("visc", "").replace("sa", ""))
+    m = load("mesh_" + name.replace("visc", "").replace("sst", "").replace("sa", ""))
     o = Oracle(m, joe)
     ncv = int(m["COUNTS"][3])
     # JoeWithModels::initialHook uniform state (JoeWithModels.cpp:180-205)
@@ -153,6 +164,9 @@ def test_run_trajectory_bit_exact(name):
         assert np.array_equal(rhoE, g[f"RHOE@{k}"]), (name, k)
         if name.endswith("sa"):
             assert np.array_equal(o.get("NUSA"), g[f"NUSA@{k}"]), (name, k)
+        if name.endswith("sst"):
+            for s in ("KINE", "OMEGA"):
+                assert np.array_equal(o.get(s), g[f"{s}@{k}"]), (name, k, s)
     nsteps = int(g["STEP"][0])
     n, _ = o.run(nsteps - 5)
     assert n == nsteps - 5
@@ -161,6 +175,8 @@ def test_run_trajectory_bit_exact(name):
     assert np.array_equal(rhou.reshape(-1), g["RHOU"].reshape(-1)), name
     if name.endswith("sa"):
         assert np.array_equal(o.get("NUSA"), g["NUSA"]), name
+    if name.endswith("sst"):
+        assert np.array_equal(o.get("KINE"), g["KINE"]) and np.array_equal(o.get("OMEGA"), g["OMEGA"]), name
 
 
 def test_uniform_state_matches_case_definition():
