@@ -128,8 +128,10 @@ jm_status jm_set_step(jm_ctx *c, int step);
 jm_status jm_init_uniform(jm_ctx *c);
 /* a transported scalar's field in reference numbering, [ncv]: stands in for
  * writing ScalarTranspEq::phi (UgpWithCv.h:50-113) of "nuSA", the one scalar
- * of the Spalart-Allmaras mix-in (RansTurbSA, TurbModel_SA.h:31). The
- * context must have been created with TURB_MODEL = SA. */
+ * of the Spalart-Allmaras mix-in (RansTurbSA, TurbModel_SA.h:31), or of
+ * "kine" / "omega", the two of the SST mix-in (RansTurbKOmSST,
+ * TurbModel_KOMSST.h:36-52). The context must have been created with
+ * TURB_MODEL = SA or SST. */
 jm_status jm_set_scalar(jm_ctx *c, const char *name, const double *phi);
 
 /* calcStateVariables + setBC + calcDt + calcRhs (JoeWithModels.cpp:943-1005,
@@ -173,6 +175,8 @@ typedef struct {
                           loop, where nuSA is part of the block system) */
   double resid_l2[5];  /* L2 norms (square root of the sum of squares) of the same residuals;
                           the reference reports L1 only */
+  double resid_scal2;  /* with TURB_MODEL = SST omega's L1 residual (resid_scal is kine's,
+                          scal_iters the two solves' iterations); else 0 */
 } jm_step_info;
 jm_status jm_step(jm_ctx *c, jm_step_info *info);
 /* The same step driven by the device: CFL ramp and step counter on the

@@ -279,8 +279,10 @@ jm_status jm_set_step(jm_ctx *c, int step) {
 jm_status jm_set_scalar(jm_ctx *c, const char *name, const double *phi) {
   return guard([&] {
     if (!c || !name || !phi) throw std::exception();
-    if (std::string(name) != "nuSA") throw std::invalid_argument(std::string("no transported scalar ") + name);
-    c->c->set_scalar(phi);
+    const std::string n(name);
+    if (n == "nuSA" || n == "kine") c->c->set_scalar(phi, 0);
+    else if (n == "omega") c->c->set_scalar(phi, 1);
+    else throw std::invalid_argument("no transported scalar " + n);
     return JM_OK;
   });
 }
@@ -327,6 +329,7 @@ static void fill_info(const jm::StepInfo &s, jm_step_info *info) {
   for (int k = 0; k < 5; k++) info->resid[k] = s.resid[k];
   for (int k = 0; k < 5; k++) info->resid_l2[k] = s.resid_l2[k];
   info->resid_scal = s.resid_s;
+  info->resid_scal2 = s.resid_s2;
   info->scal_iters = s.scal_iters;
 }
 

@@ -76,6 +76,11 @@ struct SolverParams {
   // value (INITIAL_CONDITION_TURB) and UNDER_RELAXATION_SCALARS
   bool sa = false;
   double sa_init = 0.0, relax_scal = 0.2;
+  // Menter SST k-omega (TURB_MODEL = SST): kine, omega initial values,
+  // SST_LIMIT_PK and SST_LIMITER_SHEARSTRESS
+  bool sst = false;
+  double sst_init[2] = {0.0, 0.0};
+  int sst_limit_pk = 1, sst_limiter_shear = 0;
   // the boundary-profile file of PROFILES zones (the last such zone's)
   std::string profiles_file;
 };
@@ -83,8 +88,10 @@ struct SolverParams {
 struct ZoneBc {
   int bc = 0;
   double par[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  int skind = 0;  // 1: "<zone>.nuSA = sval" (Dirichlet on inflow faces)
+  int skind = 0;  // 1: "<zone>.nuSA = sval" (Dirichlet on inflow faces); with SST kine's
   double sval = 0.0;
+  int skind2 = 0;  // SST omega: 1 "<zone>.omega = sval2", 2 "= HOOK" (the model's wall value)
+  double sval2 = 0.0;
 };
 
 void parse_params(const std::string &joe_in, const Mesh &m, SolverParams &p, std::vector<ZoneBc> &zbc);
@@ -183,8 +190,9 @@ struct StepInfo {
   double cfl = 0, dt_min = 0;
   double resid[5] = {0, 0, 0, 0, 0};
   double resid_l2[5] = {0, 0, 0, 0, 0};  // L2 norms of the same residual
-  double resid_s = 0;  // L1 of the nuSA residual (Spalart-Allmaras)
-  int scal_iters = 0;  // its Jacobi BiCGStab iterations
+  double resid_s = 0;   // L1 of the nuSA residual (Spalart-Allmaras), or SST's kine
+  double resid_s2 = 0;  // SST: omega's
+  int scal_iters = 0;   // the scalars' Jacobi BiCGStab iterations
 };
 
 class Ctx {
@@ -198,7 +206,8 @@ class Ctx {
   // the step count a solve resumed from a restart file continues from
   void set_step(int step);
   // the transported scalar nuSA (Spalart-Allmaras), reference numbering
-  void set_scalar(const double *phi);
+  // the transported scalar s (0: nuSA or kine, 1: omega) in reference numbering
+  void set_scalar(const double *phi, int s = 0);
   void get_state(double *rho, double *rhou, double *rhoE);
   void init_uniform();
   void calc_rhs(bool implicit);

@@ -24,13 +24,14 @@ __device__ __forceinline__ double dot3(const double *a, const double *b) {
   return d;
 }
 
-// HLLC flux, UgpWithCvCompFlow.cpp:120-235 (k = 0, surfVeloc = 0). With Fs,
-// one transported scalar: FrhoScal = Frho * (upwind side's value) * area.
+// HLLC flux, UgpWithCvCompFlow.cpp:120-235 (surfVeloc = 0; kL, kR the sides'
+// turbulent kinetic energy, 0 without SST). With Fs, one transported scalar:
+// FrhoScal = Frho * (upwind side's value) * area.
 __device__ __forceinline__ void flux_hllc(double F[5], double rhoL, const double *uL, double pL, double rhoR,
                                           const double *uR, double pR, double gam, double area,
                                           const double *nVec, double phiL = 0.0, double phiR = 0.0,
-                                          double *Fs = nullptr) {
-  const double gammaL = gam, gammaR = gam, kL = 0.0, kR = 0.0, surfv = 0.0;
+                                          double *Fs = nullptr, double kL = 0.0, double kR = 0.0) {
+  const double gammaL = gam, gammaR = gam, surfv = 0.0;
   double unL = dot3(uL, nVec) - surfv;
   double unLr = unL - surfv;
   double uLuL = dot3(uL, uL);
@@ -232,8 +233,9 @@ __device__ __forceinline__ void jac_a(double A[5][5], const double *vel, double 
 // computes the two independently (calcSubSonicJacobeanHLLC uses only the
 // matching dSMdU/dpsdU), so evaluating one at a time gives the same bits.
 __device__ __forceinline__ void jac_hllc(double J[5][5], int which, double rhoL, const double *uL, double pL, double rhoR,
-                                      const double *uR, double pR, double gam, double area, const double *nVec) {
-  const double gammaL = gam, gammaR = gam, kL = 0.0, kR = 0.0;
+                                      const double *uR, double pR, double gam, double area, const double *nVec,
+                                      double kL = 0.0, double kR = 0.0) {
+  const double gammaL = gam, gammaR = gam;
   double unL = dot3(uL, nVec);
   double uLuL = dot3(uL, uL);
   double cL = sqrt(gammaL * pL / rhoL);
@@ -395,10 +397,12 @@ __device__ __forceinline__ void jac_hllc(double J[5][5], int which, double rhoL,
 // downwind side has no d/d(rho phi), and a side outside the fan is zero.
 __device__ __forceinline__ void jac_hllc_scal(double S[6], int which, double rhoL, const double *uL, double pL,
                                            double rhoR, const double *uR, double pR, double gam, double area,
-                                           const double *nVec, double phiL, double phiR) {
+                                           const double *nVec, double phiL, double phiR, double kL = 0.0,
+                                           double kR = 0.0) {
   double J[5][5];
-  jac_hllc(J, which, rhoL, uL, pL, rhoR, uR, pR, gam, area, nVec);
-  // the wave speeds again, with the same operations as jac_hllc
+  jac_hllc(J, which, rhoL, uL, pL, rhoR, uR, pR, gam, area, nVec, kL, kR);
+  // the wave speeds again, with the same operations as jac_hllc (they do not
+  // depend on the kinetic energy)
   const double gammaL = gam, gammaR = gam;
   double unL = dot3(uL, nVec);
   double uLuL = dot3(uL, uL);

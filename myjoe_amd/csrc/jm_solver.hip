@@ -38,10 +38,11 @@ namespace jm {
 
 struct ZoneDev {
   int bc;
-  int skind;  // Spalart-Allmaras nuSA boundary: 0 zero gradient, 1 "<zone>.nuSA = sval"
-  int pad[2];
+  int skind;   // first scalar's boundary (nuSA, or SST's kine): 0 zero gradient, 1 "<zone>.<name> = sval"
+  int skind2;  // SST omega: 0 zero gradient, 1 "<zone>.omega = sval2", 2 HOOK (wall value)
+  int pad;
   double par[8];
-  double sval, pad2;
+  double sval, sval2;
 };
 
 struct DMesh {
@@ -77,6 +78,11 @@ struct DParams {
   // grad nuSA (3), face-averaged mul, strain and vorticity magnitudes, wall distance
   int sa, pad2;
   const double *sa_phi, *sa_aux;
+  // Menter SST k-omega: kine, omega [2][ncv], per-cell model fields
+  // [ncv][kSstN], boundary values [nfa_b][2]; SST_LIMIT_PK, SST_LIMITER_SHEARSTRESS
+  int sst, sst_limit_pk, sst_lim_shear, pad5;
+  const double *sst_phi, *sst_aux, *sst_bfa;
+  double sst_alfa[2];  // alfa_1, alfa_2 (TurbModel_KOMSST.h:24-25)
   // parity readout: when set, the per-entry flux kernel writes each face's
   // inviscid flux [nfa][5] (device face order) once, from the face's first cell
   double *fdump;
@@ -87,6 +93,13 @@ struct DParams {
 
 // per-cell Spalart-Allmaras fields in DParams::sa_aux
 enum SaAux { kAuxGrad = 0, kAuxMue = 3, kAuxStr = 4, kAuxVort = 5, kAuxWall = 6, kAuxN = 8 };
+// per-cell SST fields in DParams::sst_aux: limited kine and omega gradients,
+// face-averaged mul, strain and vorticity magnitudes, divergence, cross
+// diffusion, blending functions F1 and F2, wall distance
+enum SstAux {
+  kSGradK = 0, kSGradW = 3, kSMue = 6, kSStr = 7, kSVort = 8, kSDiv = 9, kSCd = 10, kSF1 = 11, kSF2 = 12,
+  kSWall = 13, kSstN = 16
+};
 
 // per-solve scalars of the BiCGStab and of reductions (device resident)
 struct SolverScalars {
@@ -95,7 +108,8 @@ struct SolverScalars {
   double dt_min;
   double resid[5];
   double resid_l2[5];  // L2 norms of the same residual
-  double resid_s;  // L1 of the transported scalar's residual (Spalart-Allmaras)
+  double resid_s;  // L1 of the transported scalar's residual (Spalart-Allmaras; SST's kine)
+  double resid_s2;  // SST: omega's
   int iter, done, maxiter, pad;
   double zero_abs, zero_rel;
   // pseudo-time step counter and CFL of the device-driven (graph) step
@@ -163,6 +177,32 @@ __device__ __forceinline__ void d_state(const Blk &B, int ncv, const double *__r
 __global__ void __launch_bounds__(kBlock) k_state(int ncv, const double *__restrict__ q, double *__restrict__ prim,
                                                   double gam, int *__restrict__ negp) {
   d_state(hw_blk(), ncv, q, prim, gam, negp);
+}
+
+// calcStateVariables with SST's kinetic energy in the pressure
+// (UgpWithCvCompFlow.h:1672-1676)
+__global__ void __launch_bounds__(kBlock) k_state_sst(int ncv, const double *__restrict__ q, double *__restrict__ prim,
+                                                      double gam, int *__restrict__ negp,
+                                                      const double *__restrict__ kine) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncv) return;
+  double rho = q[c], r0 = q[ncv + c], r1 = q[2 * ncv + c], r2 = q[3 * ncv + c], rE = q[4 * ncv + c];
+  double *P = prim + 8 * (size_t)c;
+  double d = 0.0;
+  d += r0 * r0;
+  d += r1 * r1;
+  d += r2 * r2;
+  double pr = (gam - 1.0) * (rE - 0.5 * d / rho - rho * kine[c]);
+  double p = P[4];
+  if (pr > 0.0)
+    p = pr;
+  else
+    atomicAdd(negp, 1);
+  P[0] = rho;
+  P[1] = r0 / rho;
+  P[2] = r1 / rho;
+  P[3] = r2 / rho;
+  P[4] = p;
 }
 
 // ---------------------------------------------------------------------------
@@ -851,14 +891,15 @@ __device__ __forceinline__ bool reconstruct(Side &s, const double *pc, const dou
   return true;
 }
 
+// kL, kR: the sides' turbulent kinetic energy (SST, HLLC only; else 0)
 __device__ __forceinline__ void face_flux(int scheme, double F[5], const Side &L, const Side &R, double gam,
-                                          double area, const double *n) {
+                                          double area, const double *n, double kL = 0.0, double kR = 0.0) {
   if (scheme == kRoe)
     flux_roe(F, L.rho, L.u, L.p, R.rho, R.u, R.p, gam, area, n);
   else if (scheme == kAusm)
     flux_ausm(F, L.rho, L.u, L.p, R.rho, R.u, R.p, gam, area, n);
   else
-    flux_hllc(F, L.rho, L.u, L.p, R.rho, R.u, R.p, gam, area, n);
+    flux_hllc(F, L.rho, L.u, L.p, R.rho, R.u, R.p, gam, area, n, 0.0, 0.0, nullptr, kL, kR);
 }
 
 // ---------------------------------------------------------------------------
@@ -1024,6 +1065,7 @@ __device__ __forceinline__ double flux_cell(const DMesh &m, const DParams &P, co
 struct VSide {
   double rho, u[3], h, T;
   const double *gu, *gh;  // grad u [3][3], grad h [3]
+  double k;               // turbulent kinetic energy (SST; else 0)
 };
 
 __device__ __forceinline__ double mu_lam(const DParams &P, double T) {
@@ -1036,9 +1078,13 @@ struct VFace {
   double keff, alpha;
 };
 
+// With SST the kinetic-energy terms: -2/3 rho k on the normal stress and the
+// triple correlation (mul + mut) dk/ds in the energy flux (:1614-1615, 1657-1660)
+template <bool SST = false>
 __device__ __forceinline__ void visc_face(const DParams &P, const VSide &s0, const VSide &s1, double mul, double mut,
                                        const double *u_fa, double area, const double *nVec, double smag,
-                                       const double *sVec, double Frhou[3], double &FrhoE, VFace &vf) {
+                                       const double *sVec, double Frhou[3], double &FrhoE, VFace &vf,
+                                       double kine_fa = 0.0) {
   const double nonL = 0.0;
   const double alpha = dot3(nVec, sVec);
   double guf[3][3];
@@ -1056,6 +1102,7 @@ __device__ __forceinline__ void visc_face(const DParams &P, const VSide &s0, con
                  (alpha * (s1.u[i] - s0.u[i]) / smag + guf[i][0] * fCorr[0] + guf[0][i] * nVec[0] +
                   guf[i][1] * fCorr[1] + guf[1][i] * nVec[1] + guf[i][2] * fCorr[2] + guf[2][i] * nVec[2]);
   tmp = (1.0 - nonL) * 2.0 / 3.0 * mut * (guf[0][0] + guf[1][1] + guf[2][2]);
+  if (SST) tmp += (1.0 - nonL) * 1.0 / 3.0 * (s0.rho + s1.rho) * kine_fa;
   for (int i = 0; i < 3; i++) tauTurb[i] -= tmp * nVec[i];
   tauTurb[0] += nonL * (0.0 * nVec[0] + 0.0 * nVec[1] + 0.0 * nVec[2]);
   tauTurb[1] += nonL * (0.0 * nVec[0] + 0.0 * nVec[1] + 0.0 * nVec[2]);
@@ -1066,7 +1113,7 @@ __device__ __forceinline__ void visc_face(const DParams &P, const VSide &s0, con
   const double enth = keff * (alpha * (s1.h - s0.h) / smag + 0.5 * ((s0.gh[0] + s1.gh[0]) * fCorr[0] +
                                                                     (s0.gh[1] + s1.gh[1]) * fCorr[1] +
                                                                     (s0.gh[2] + s1.gh[2]) * fCorr[2]));
-  const double psi = 0.0;
+  const double psi = SST ? (mul + mut) * (alpha * (s1.k - s0.k) / smag) : 0.0;
   FrhoE = -area * (enth + vf.tau[0] * u_fa[0] + vf.tau[1] * u_fa[1] + vf.tau[2] * u_fa[2] + psi);
   vf.keff = keff;
   vf.alpha = alpha;
@@ -1074,9 +1121,10 @@ __device__ __forceinline__ void visc_face(const DParams &P, const VSide &s0, con
 
 // one half of the viscous Jacobian (A0: sgn -1 with side 0's state, A1: +1
 // with side 1's), dFdU x dUdQ (:1662-1762, :712-745); rows 1-4 only
+template <bool SST = false>
 __device__ __forceinline__ void visc_jac(const DParams &P, double J[5][5], double sgn, const VSide &s, double mul,
                                       double mut, const VFace &vf, const double *u_fa, double area, double smag) {
-  const double kine = 0.0;
+  const double kine = SST ? s.k : 0.0;
   const double gam = P.gamma, R = P.R;
   double f11 = sgn * (mul + mut) * vf.alpha / smag * area;
   double f22 = sgn * (mul + mut) * vf.alpha / smag * area;
@@ -1131,6 +1179,7 @@ __device__ __forceinline__ VSide vside_cell(const DParams &P, const double *pc, 
   s.h = P.gamma * P.R / (P.gamma - 1.0) * s.T;
   s.gu = gv;
   s.gh = gv + 9;
+  s.k = 0.0;
   return s;
 }
 
@@ -1142,6 +1191,7 @@ __device__ __forceinline__ VSide vside_bfa(const double *bb, const double *gv) {
   s.h = bb[6];
   s.gu = gv;
   s.gh = gv + 9;
+  s.k = 0.0;
   return s;
 }
 
@@ -1174,6 +1224,7 @@ __device__ __forceinline__ VSide vsel(bool c, const VSide &x, const VSide &y) {
   r.T = c ? x.T : y.T;
   r.gu = c ? x.gu : y.gu;
   r.gh = c ? x.gh : y.gh;
+  r.k = c ? x.k : y.k;
   return r;
 }
 
@@ -1252,6 +1303,46 @@ __device__ __forceinline__ void visc_coupled_bgeom(int bc, const double *xfa, co
 }
 
 // ---------------------------------------------------------------------------
+// Menter SST k-omega (RansTurbKOmSST, turbModels/TurbModel_KOMSST.h) as two
+// segregated scalars, kine and omega, in the reference's registration order
+// ---------------------------------------------------------------------------
+// model constants, getDoubleParam defaults (TurbModel_KOMSST.h:16-23); alfa_1
+// and alfa_2 come from the host (DParams::sst_alfa), formed there as the
+// reference forms them
+constexpr double kSstSigmaK1 = 0.85, kSstSigmaK2 = 1.0, kSstSigmaW1 = 0.5, kSstSigmaW2 = 0.856, kSstBeta1 = 0.075,
+                 kSstBeta2 = 0.0828, kSstBetaStar = 0.09, kSstA1 = 0.31;
+// kine and omega transport settings (TurbModel_KOMSST.h:36-52)
+constexpr double kSstPhiZero = 1.0e-8, kSstPhiZeroRel = 1.0e-2;
+constexpr int kSstMaxIter = 1000;
+constexpr double kSstLower[2] = {1.0e-10, 1.0e-4}, kSstUpper[2] = {1.0e10, 1.0e15};
+
+// the shear-stress limiter's field: vorticity, or with SST_LIMITER_SHEARSTRESS = 1 strain (:90-93)
+__device__ __forceinline__ double sst_lim(const DParams &P, const double *a) {
+  return P.sst_lim_shear == 1 ? a[kSStr] : a[kSVort];
+}
+
+// mut_fa of calcRansTurbViscMuet (TurbModel_KOMSST.h:108-131) at an internal
+// face: distance-weighted rho, kine, omega, limiter and F2 (icv0, icv1 order)
+__device__ __forceinline__ double sst_mut_face(const DParams &P, int ncv, double w0, double w1, double rho0,
+                                               double rho1, int c0, int c1) {
+  const double *a0 = P.sst_aux + kSstN * (size_t)c0, *a1 = P.sst_aux + kSstN * (size_t)c1;
+  const double rho_fa = (w1 * rho0 + w0 * rho1) / (w0 + w1);
+  const double kine_fa = (w1 * P.sst_phi[c0] + w0 * P.sst_phi[c1]) / (w0 + w1);
+  const double om_fa = (w1 * P.sst_phi[ncv + c0] + w0 * P.sst_phi[ncv + c1]) / (w0 + w1);
+  const double limitFunc_fa = (w1 * sst_lim(P, a0) + w0 * sst_lim(P, a1)) / (w0 + w1);
+  const double blendFuncF2_fa = (w1 * a0[kSF2] + w0 * a1[kSF2]) / (w0 + w1);
+  const double zeta = Dev::mn(1.0 / om_fa, kSstA1 / (limitFunc_fa * blendFuncF2_fa));
+  return Dev::mn(Dev::mx(rho_fa * kine_fa * zeta, 0.0), 1.0);
+}
+
+// the same on a non-wall boundary face, from its cell (:143-149)
+__device__ __forceinline__ double sst_mut_cell(const DParams &P, int ncv, double rho, int c) {
+  const double *a = P.sst_aux + kSstN * (size_t)c;
+  const double zeta = Dev::mn(1.0 / P.sst_phi[ncv + c], kSstA1 / (sst_lim(P, a) * a[kSF2]));
+  return Dev::mn(Dev::mx(rho * P.sst_phi[c] * zeta, 0.0), 1.0);
+}
+
+// ---------------------------------------------------------------------------
 // Spalart-Allmaras (RansTurbSA, TurbModel_SA.h) as one segregated scalar nuSA
 // ---------------------------------------------------------------------------
 // model constants, getDoubleParam defaults (TurbModel_SA.h:20-27)
@@ -1298,11 +1389,11 @@ __device__ __forceinline__ void store_block(T *dst, const double J[5][5], bool n
 }
 
 __device__ __forceinline__ void face_jac(int scheme, double J[5][5], int which, const Side &L, const Side &R,
-                                         double gam, double area, const double *n) {
+                                         double gam, double area, const double *n, double kL = 0.0, double kR = 0.0) {
   if (scheme == kRoe)
     jac_roe(J, which, L.rho, L.u, L.p, R.rho, R.u, R.p, gam, area, n);
   else
-    jac_hllc(J, which, L.rho, L.u, L.p, R.rho, R.u, R.p, gam, area, n);
+    jac_hllc(J, which, L.rho, L.u, L.p, R.rho, R.u, R.p, gam, area, n, kL, kR);
 }
 
 template <typename T, int SCHEME, bool VISC, int NB = 5>
@@ -1613,7 +1704,10 @@ __device__ __forceinline__ int vc_index(int k) {
 constexpr int kVcN = 11;
 constexpr int kMaxOwned = 7;  // components of the diagonal block per lane (cells have at least 4 faces)
 
-template <typename T, int SCHEME, bool VISC, int NB = 5, int W = kWaves>
+// SST: the cells' (or the boundary's) turbulent kinetic energy in the HLLC
+// Jacobian (kineCV / kineFA, JoeWithModels.cpp:3034-3037, 3202-3206, 3490-3495)
+// and in the viscous blocks' dU/dQ; the SST eddy viscosity
+template <typename T, int SCHEME, bool VISC, int NB = 5, int W = kWaves, bool SST = false>
 __device__ __forceinline__ void d_jac_e(const Blk &B, DMesh m, DParams P, const double *__restrict__ prim,
                                         const double *__restrict__ bfa, T *__restrict__ A, int step_mode,
                                         const double *__restrict__ dt, const double *__restrict__ rhs, double relax,
@@ -1646,6 +1740,7 @@ __device__ __forceinline__ void d_jac_e(const Blk &B, DMesh m, DParams P, const 
       nb.rho = pn[0]; nb.u[0] = pn[1]; nb.u[1] = pn[2]; nb.u[2] = pn[3]; nb.p = pn[4];
       const bool s0 = en.x >= 0;
       const Side l = s0 ? own : nb, r = s0 ? nb : own;
+      const double kcl = SST ? P.sst_phi[s0 ? c : en.y] : 0.0, kcr = SST ? P.sst_phi[s0 ? en.y : c] : 0.0;
       double hl = 0.0, hr = 0.0;
       if (CPL) {
         hl = enthalpy_of(P, l.rho, l.p);
@@ -1659,7 +1754,7 @@ __device__ __forceinline__ void d_jac_e(const Blk &B, DMesh m, DParams P, const 
       for (int t = 0; t < 2; t++) {
         const int which = (t == 0) == s0 ? 0 : 1;
         if (CPL) jac_hllc_coupled(J, which, l.rho, l.u, l.p, hl, r.rho, r.u, r.p, hr, gam, area, n);
-        else face_jac(SCHEME, J, which, l, r, gam, area, n);
+        else face_jac(SCHEME, J, which, l, r, gam, area, n, kcl, kcr);
         if (t == 0) {
           for (int i = 0; i < 5; i++)
             for (int j = 0; j < 5; j++) slot[5 * i + j] = s0 ? J[i][j] : -J[i][j];
@@ -1751,13 +1846,14 @@ __device__ __forceinline__ void d_jac_e(const Blk &B, DMesh m, DParams P, const 
           vs = 1;
         }
       } else if (z.bc == kBcSymmetry || z.bc == kBcWall) {
+        const double kb = SST ? P.sst_bfa[2 * (size_t)f] : 0.0;
         if (z.bc == kBcWall || P.symm_bc == kSymmOriginal) {
-          face_jac(SCHEME, J, 0, sb, sb, gam, area, n);
+          face_jac(SCHEME, J, 0, sb, sb, gam, area, n, kb, kb);
           for (int k = 0; k < 25; k++) slot[k] = J[k / 5][k % 5];
         } else if (P.symm_bc == kSymmNew) {
           double K[5][5];
-          face_jac(SCHEME, J, 0, sb, sb, gam, area, n);
-          face_jac(SCHEME, K, 1, sb, sb, gam, area, n);
+          face_jac(SCHEME, J, 0, sb, sb, gam, area, n, kb, kb);
+          face_jac(SCHEME, K, 1, sb, sb, gam, area, n, kb, kb);
           for (int k = 0; k < 25; k++) slot[k] = J[k / 5][k % 5] + K[k / 5][k % 5];
         } else {  // ANALYTICAL, JoeWithModels.cpp:3243-3269
           const double a2 = gam - 1.0;
@@ -1773,7 +1869,8 @@ __device__ __forceinline__ void d_jac_e(const Blk &B, DMesh m, DParams P, const 
           }
         }
       } else {
-        face_jac(SCHEME, J, 0, own, sb, gam, area, n);
+        if (SST) face_jac(SCHEME, J, 0, own, sb, gam, area, n, P.sst_phi[c], P.sst_bfa[2 * (size_t)f]);
+        else face_jac(SCHEME, J, 0, own, sb, gam, area, n);
         for (int k = 0; k < 25; k++) slot[k] = J[k / 5][k % 5];
       }
     }
@@ -1810,33 +1907,38 @@ __device__ __forceinline__ void d_jac_e(const Blk &B, DMesh m, DParams P, const 
       const double n[3] = {geo[0], geo[1], geo[2]};
       const double area = geo[3];
       const double xfa[3] = {geo[4], geo[5], geo[6]};
-      const VSide vo = vside_cell(P, pc, gradv + 12 * (size_t)c);
+      VSide vo = vside_cell(P, pc, gradv + 12 * (size_t)c);
+      if (SST) vo.k = P.sst_phi[c];
       double Fu[3], FE;
       VFace vf;
       signed char vs = 0;
       if (en.y >= 0) {
         const double *pn = prim + 8 * (size_t)en.y;
-        const VSide vn = vside_cell(P, pn, gradv + 12 * (size_t)en.y);
+        VSide vn = vside_cell(P, pn, gradv + 12 * (size_t)en.y);
+        if (SST) vn.k = P.sst_phi[en.y];
         const bool s0 = en.x >= 0;
         const VSide &a = s0 ? vo : vn, &b = s0 ? vn : vo;
         double sV[3], smag, mul, w0, w1;
         visc_internal_geom(P, s0 ? pc + 5 : pn + 5, s0 ? pn + 5 : pc + 5, xfa, a.T, b.T, sV, smag, mul, w0, w1);
-        double mut = 0.0;
-        if (P.sa) {
+        double mut = 0.0, kine_fa = 0.0;
+        if (SST) {
+          mut = sst_mut_face(P, m.ncv, w0, w1, a.rho, b.rho, s0 ? c : en.y, s0 ? en.y : c);
+          kine_fa = (w1 * a.k + w0 * b.k) / (w0 + w1);
+        } else if (P.sa) {
           const double nuo = P.sa_phi[c], nun = P.sa_phi[en.y];
           mut = sa_mut_face(w0, w1, a.rho, b.rho, s0 ? nuo : nun, s0 ? nun : nuo, mul);
         }
         const double ufa[3] = {0.5 * (a.u[0] + b.u[0]), 0.5 * (a.u[1] + b.u[1]), 0.5 * (a.u[2] + b.u[2])};
-        visc_face(P, a, b, mul, mut, ufa, area, n, smag, sV, Fu, FE, vf);
+        visc_face<SST>(P, a, b, mul, mut, ufa, area, n, smag, sV, Fu, FE, vf, kine_fa);
         T *off = A + NB * NB * (size_t)(m.cfl_noc[L.e] & 0x3fffffff);
         // icv0: A[noc00] -= A0, A[noc01] -= A1; icv1: A[noc11] += A1, A[noc10] += A0
-        if (s0) visc_jac(P, J, 1.0, b, mul, mut, vf, ufa, area, smag);  // A1 -> off
-        else visc_jac(P, J, -1.0, a, mul, mut, vf, ufa, area, smag);    // A0 -> off
+        if (s0) visc_jac<SST>(P, J, 1.0, b, mul, mut, vf, ufa, area, smag);  // A1 -> off
+        else visc_jac<SST>(P, J, -1.0, a, mul, mut, vf, ufa, area, smag);    // A0 -> off
         for (int i = 1; i < 5; i++)
           for (int j = 0; j < 5; j++)
             off[NB * i + j] = (T)(s0 ? (double)off[NB * i + j] - J[i][j] : (double)off[NB * i + j] + J[i][j]);
-        if (s0) visc_jac(P, J, -1.0, a, mul, mut, vf, ufa, area, smag);  // A0 -> diagonal, subtracted
-        else visc_jac(P, J, 1.0, b, mul, mut, vf, ufa, area, smag);     // A1 -> diagonal, added
+        if (s0) visc_jac<SST>(P, J, -1.0, a, mul, mut, vf, ufa, area, smag);  // A0 -> diagonal, subtracted
+        else visc_jac<SST>(P, J, 1.0, b, mul, mut, vf, ufa, area, smag);     // A1 -> diagonal, added
         for (int i = 1; i < 5; i++)
           for (int j = 0; j < 5; j++) slot[5 * i + j] = s0 ? -J[i][j] : J[i][j];
         vs = 1;
@@ -1845,7 +1947,12 @@ __device__ __forceinline__ void d_jac_e(const Blk &B, DMesh m, DParams P, const 
         if (z.bc != kBcSymmetry) {
           const double *bb = bfa + 8 * (size_t)f;
           const double mul = mu_lam(P, bb[5]);
-          const VSide vb = vside_bfa(bb, gradv + 12 * (size_t)c);
+          VSide vb = vside_bfa(bb, gradv + 12 * (size_t)c);
+          double kine_fa = 0.0;
+          if (SST) {  // walls: kine1 = kine_fa = 0; other zones phi_bfa
+            vb.k = z.bc == kBcWall ? 0.0 : P.sst_bfa[2 * (size_t)f];
+            kine_fa = vb.k;
+          }
           double sV[3] = {xfa[0] - pc[5], xfa[1] - pc[6], xfa[2] - pc[7]};
           double smag;
           double sArg[3];
@@ -1856,11 +1963,12 @@ __device__ __forceinline__ void d_jac_e(const Blk &B, DMesh m, DParams P, const 
             smag = sqrt(dot3(sV, sV));
             for (int i = 0; i < 3; i++) sArg[i] = sV[i] / smag;
           }
-          const double mut = (P.sa && z.bc != kBcWall)
-                                 ? sa_mut(pc[0] * P.sa_phi[c], P.sa_aux[kAuxN * (size_t)c + kAuxMue])
-                                 : 0.0;
-          visc_face(P, vo, vb, mul, mut, vb.u, area, n, smag, sArg, Fu, FE, vf);
-          visc_jac(P, J, -1.0, vo, mul, mut, vf, vb.u, area, smag);
+          const double mut = z.bc == kBcWall ? 0.0
+                             : SST       ? sst_mut_cell(P, m.ncv, pc[0], c)
+                             : P.sa      ? sa_mut(pc[0] * P.sa_phi[c], P.sa_aux[kAuxN * (size_t)c + kAuxMue])
+                                         : 0.0;
+          visc_face<SST>(P, vo, vb, mul, mut, vb.u, area, n, smag, sArg, Fu, FE, vf, kine_fa);
+          visc_jac<SST>(P, J, -1.0, vo, mul, mut, vf, vb.u, area, smag);
           for (int i = 1; i < 5; i++)
             for (int j = 0; j < 5; j++) slot[5 * i + j] = -J[i][j];
           vs = 1;
@@ -1912,14 +2020,14 @@ __device__ __forceinline__ void d_jac_e(const Blk &B, DMesh m, DParams P, const 
 // for their own wavefront)
 constexpr int kJacWaves = 1;
 
-template <typename T, int SCHEME, bool VISC, int NB = 5>
+template <typename T, int SCHEME, bool VISC, int NB = 5, bool SST = false>
 __global__ void __launch_bounds__(64 * kJacWaves) k_jac_e(DMesh m, DParams P, const double *__restrict__ prim,
                                                   const double *__restrict__ bfa, T *__restrict__ A, int step_mode,
                                                   const double *__restrict__ dt, const double *__restrict__ rhs,
                                                   double relax, T *__restrict__ rhs_ls, T *__restrict__ phi,
                                                   const double *__restrict__ gradv) {
-  d_jac_e<T, SCHEME, VISC, NB, kJacWaves>(hw_blk(), m, P, prim, bfa, A, step_mode, dt, rhs, relax, rhs_ls, phi,
-                                         gradv);
+  d_jac_e<T, SCHEME, VISC, NB, kJacWaves, SST>(hw_blk(), m, P, prim, bfa, A, step_mode, dt, rhs, relax, rhs_ls,
+                                              phi, gradv);
 }
 
 // ---------------------------------------------------------------------------
@@ -2062,6 +2170,201 @@ __global__ void __launch_bounds__(kBlock) k_sa_cell(DMesh m, DParams P, const do
   o[kAuxMue] = mue;
   o[kAuxStr] = sqrt(2.0 * str);
   o[kAuxVort] = sqrt(2.0 * vort);
+}
+
+// ---------------------------------------------------------------------------
+// SST boundary values of kine and omega: setScalarBC (Scalars.cpp:289-360) with
+// the omega wall hook 60 mu / (rho beta_1 d^2) (boundaryHookScalarRansTurb,
+// TurbModel_KOMSST.h:342-358). After k_bc, whose boundary velocity decides
+// inflow (vel_bfa . fa_normal <= 1e-8).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_sst_bc(DMesh m, DParams P, const double *__restrict__ prim,
+                                                   const double *__restrict__ bfa, double *__restrict__ sbfa) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= m.nfa_b) return;
+  const int c = m.bfa_cell[f];
+  const ZoneDev &z = m.zones[m.bfa_zone[f]];
+  const double *bb = bfa + 8 * (size_t)f;
+  const double vb[3] = {bb[1], bb[2], bb[3]};
+  const bool inflow = !(dot3(vb, m.fnorm + 3 * (size_t)f) > 1.0e-8);
+  sbfa[2 * (size_t)f] = (z.skind && inflow) ? z.sval : P.sst_phi[c];
+  double w;
+  if (z.skind2 == 2) {
+    const double *pc = prim + 8 * (size_t)c;
+    const double muLamCV = mu_lam(P, pc[4] / (pc[0] * P.R));
+    const double d = P.sst_aux[kSstN * (size_t)c + kSWall];
+    w = 60.0 * muLamCV / (pc[0] * kSstBeta1 * d * d);
+  } else {
+    w = (z.skind2 == 1 && inflow) ? z.sval2 : P.sst_phi[m.ncv + c];
+  }
+  sbfa[2 * (size_t)f + 1] = w;
+}
+
+// the limited gradient of one SST scalar with its boundary values
+// (calcCvScalarGrad with limiterNavierS and the scalar as its own reference:
+// UgpWithCv.h:881-937, 1009-1188), as k_sa_cell forms nuSA's
+template <bool LS>
+__device__ __forceinline__ void sst_grad(const DMesh &m, const DParams &P, const double *__restrict__ prim,
+                                         const double *__restrict__ phi, int s, int c, double gout[3]) {
+  const double *pc = prim + 8 * (size_t)c;
+  const double ph = phi[c];
+  const double xcv[3] = {pc[5], pc[6], pc[7]};
+  double g[3];
+  double pmax = ph, pmin = ph;
+  const int e0 = m.cfl_i[c], e1 = m.cfl_i[c + 1];
+  int e_int = e0;
+  if (LS) {
+    const int noc_f = m.nbocv_i[c];
+    const double *l0 = m.lsg + 3 * (size_t)noc_f;
+    for (int j = 0; j < 3; j++) g[j] = l0[j] * ph;
+    int k = 1;
+    for (int e = e0; e < e1; e++) {
+      int2 en = m.cfl[e];
+      if (en.y < 0) break;
+      const double pn = phi[en.y];
+      const double *l = m.lsg + 3 * (size_t)(noc_f + k);
+      for (int j = 0; j < 3; j++) g[j] += l[j] * pn;
+      pmax = Dev::mx(pmax, pn);
+      pmin = Dev::mn(pmin, pn);
+      k++;
+      e_int = e + 1;
+    }
+    for (int e = e0; e < e1; e++) {
+      int2 en = m.cfl[e];
+      if (en.y >= 0) continue;
+      int f = en.x & 0x7fffffff;
+      const double *bl = m.bfa_lsg + 3 * (size_t)f;
+      const double pb = P.sst_bfa[2 * (size_t)f + s];
+      for (int j = 0; j < 3; j++) g[j] += bl[j] * pb;
+    }
+  } else {
+    for (int j = 0; j < 3; j++) g[j] = 0.0;
+    for (int e = e0; e < e1; e++) {
+      int2 en = m.cfl[e];
+      int f = en.x & 0x7fffffff;
+      const double *fn = m.fnorm + 3 * (size_t)f;
+      if (en.y >= 0) {
+        const double *pn = prim + 8 * (size_t)en.y;
+        const double phn = phi[en.y];
+        const double *xf = m.fgeo + 8 * (size_t)f + 4;
+        double ds[3], dn[3];
+        for (int j = 0; j < 3; j++) {
+          ds[j] = xf[j] - xcv[j];
+          dn[j] = xf[j] - pn[5 + j];
+        }
+        const bool s0 = (en.x >= 0);
+        const double wself = sqrt(dot3(ds, ds)), wnb = sqrt(dot3(dn, dn));
+        const double w0 = s0 ? wself : wnb, w1 = s0 ? wnb : wself;
+        const double fv = s0 ? (w1 * ph + w0 * phn) / (w0 + w1) : (w1 * phn + w0 * ph) / (w0 + w1);
+        for (int j = 0; j < 3; j++) {
+          if (s0)
+            g[j] += fv * fn[j];
+          else
+            g[j] -= fv * fn[j];
+        }
+        pmax = Dev::mx(pmax, phn);
+        pmin = Dev::mn(pmin, phn);
+        e_int = e + 1;
+      } else {
+        const double pb = P.sst_bfa[2 * (size_t)f + s];
+        for (int j = 0; j < 3; j++) g[j] += pb * fn[j];
+      }
+    }
+    const double vol = m.vol[c];
+    for (int j = 0; j < 3; j++) g[j] /= vol;
+  }
+  double al = 1.0;
+  if (P.limiter) {
+    double e2 = Dev::mx(P.eps * fabs(ph), 1.0e-8);
+    e2 = e2 * e2;
+    if ((pmax - pmin) > 1.0e-12) {
+      for (int e = e0; e < e_int; e++) {
+        const double *rr = m.cfl_r0 + 3 * (size_t)e;
+        double r0[3] = {rr[0], rr[1], rr[2]};
+        al = bj_face(al, ph, ph + dot3(r0, g), pmax, pmin, e2);
+      }
+    } else {
+      al = 0.0;
+    }
+  }
+  for (int j = 0; j < 3; j++) gout[j] = g[j] * al;
+}
+
+// ---------------------------------------------------------------------------
+// SST per-cell fields after k_grad and k_sst_bc: the limited kine and omega
+// gradients, the face-averaged laminar viscosity, strain and vorticity
+// magnitudes and the divergence of the limited velocity gradient (calcGradVel,
+// calcStrainRateAndDivergence, calcVorticity, UgpWithCvCompFlow.h:1300-1392),
+// the cross diffusion and the blending functions F1, F2
+// (calcMenterBlendingFunctions, TurbModel_KOMSST.h:360-392)
+// ---------------------------------------------------------------------------
+template <bool LS>
+__global__ void __launch_bounds__(kBlock) k_sst_cell(DMesh m, DParams P, const double *__restrict__ prim,
+                                                     const double *__restrict__ bfa, const double *__restrict__ grad,
+                                                     double *__restrict__ aux) {
+  const int c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (c >= m.ncv) return;
+  const double *pc = prim + 8 * (size_t)c;
+  double gk[3], gw[3];
+  sst_grad<LS>(m, P, prim, P.sst_phi, 0, c, gk);
+  sst_grad<LS>(m, P, prim, P.sst_phi + m.ncv, 1, c, gw);
+  // face-averaged laminar viscosity over the faocv list
+  const int e0 = m.cfl_i[c], e1 = m.cfl_i[c + 1];
+  const double Tc = pc[4] / (pc[0] * P.R);
+  const int nf = e1 - e0;
+  double mue = 0.0;
+  for (int k = 0; k < nf; k++) {
+    const int e = e0 + m.foc[e0 + k];
+    const int2 en = m.cfl[e];
+    const int f = en.x & 0x7fffffff;
+    double mul;
+    if (en.y >= 0) {
+      const double *pn = prim + 8 * (size_t)en.y;
+      const double Tn = pn[4] / (pn[0] * P.R);
+      const bool s0 = (en.x >= 0);
+      double sV[3], smag, w0, w1;
+      visc_internal_geom(P, s0 ? pc + 5 : pn + 5, s0 ? pn + 5 : pc + 5, m.fgeo + 8 * (size_t)f + 4, s0 ? Tc : Tn,
+                         s0 ? Tn : Tc, sV, smag, mul, w0, w1);
+    } else {
+      mul = mu_lam(P, bfa[8 * (size_t)f + 5]);
+    }
+    mue += mul;
+  }
+  mue /= (double)nf;
+  const double *gu = grad + 16 * (size_t)c + 6;
+  double str = 0.0, vort = 0.0;
+  const double diverg = gu[0] + gu[4] + gu[8];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double v = i == j ? 0.5 * (gu[3 * i + j] + gu[3 * j + i]) - 1. / 3. * diverg : 0.5 * (gu[3 * i + j] + gu[3 * j + i]);
+      str += v * v;
+    }
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double v = 0.5 * (gu[3 * i + j] - gu[3 * j + i]);
+      vort += v * v;
+    }
+  double *o = aux + kSstN * (size_t)c;
+  const double rho = pc[0], kine = P.sst_phi[c], omega = P.sst_phi[m.ncv + c], d = o[kSWall];
+  const double crossDiff = Dev::mx(2.0 * rho * kSstSigmaW2 / omega * dot3(gk, gw), 1.0e-20);
+  const double gamma1 = 500.0 * mue / (d * d * rho * omega);
+  const double gamma2 = 4.0 * kSstSigmaW2 * rho * kine / (d * d * crossDiff);
+  const double gamma3 = sqrt(kine) / (kSstBetaStar * omega * d);
+  double gamma = Dev::mn(Dev::mx(gamma1, gamma3), gamma2);
+  const double f1 = tanh(pow(gamma, 4.0));
+  gamma = Dev::mx(2.0 * gamma3, gamma1);
+  const double f2 = tanh(gamma * gamma);
+  for (int j = 0; j < 3; j++) {
+    o[kSGradK + j] = gk[j];
+    o[kSGradW + j] = gw[j];
+  }
+  o[kSMue] = mue;
+  o[kSStr] = sqrt(2.0 * str);
+  o[kSVort] = sqrt(2.0 * vort);
+  o[kSDiv] = diverg;
+  o[kSCd] = crossDiff;
+  o[kSF1] = f1;
+  o[kSF2] = f2;
 }
 
 // ---------------------------------------------------------------------------
@@ -2248,6 +2551,218 @@ __global__ void __launch_bounds__(kBlock) k_scal(DMesh m, DParams P, const doubl
   rhs_s[c] = acc;
   double *dg = AS + 6 * (size_t)m.nbocv_i[c];
   for (int k = 0; k < 6; k++) dg[k] = D[k];
+}
+
+// ---------------------------------------------------------------------------
+// One SST scalar's equation of one cell (S = 0 kine, 1 omega), gathered in the
+// reference's order as k_scal gathers nuSA's: the HLLC convective flux with
+// the sides' kinetic energy in the enthalpy (calcEulerFlux scalar terms,
+// JoeWithModels.cpp:2824-2930, 3014-3026, 3134-3198, 3408-3495) and its
+// AScal rows at cell-centre states (:3088-3101, 3270-3274, 3559-3561);
+// diffusion with the blended SST diffusivity (calcViscousFluxScalar_new,
+// Scalars.cpp:117-247; diffusivityHookScalarRansTurb, TurbModel_KOMSST.h
+// :160-254); the source and its diagonal (sourceHookScalarRansTurb_new,
+// :256-304). rhs_s and AS are this scalar's [ncv] and [nnz][6].
+// ---------------------------------------------------------------------------
+template <int S>
+__global__ void __launch_bounds__(kBlock) k_sst_scal(DMesh m, DParams P, const double *__restrict__ prim,
+                                                     const double *__restrict__ grad, const double *__restrict__ bfa,
+                                                     double *__restrict__ rhs_s, double *__restrict__ AS) {
+  const int c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (c >= m.ncv) return;
+  const int ncv = m.ncv;
+  const double gam = P.gamma;
+  const bool snd = P.second_order != 0;
+  const double *pc = prim + 8 * (size_t)c;
+  const double *gc = grad + 16 * (size_t)c;
+  const double *phi = P.sst_phi + (size_t)S * ncv, *kin = P.sst_phi;
+  const double *auxc = P.sst_aux + kSstN * (size_t)c;
+  constexpr int gS = S == 0 ? kSGradK : kSGradW;
+  const double ph = phi[c], kc = kin[c];
+  Side own;
+  own.rho = pc[0]; own.u[0] = pc[1]; own.u[1] = pc[2]; own.u[2] = pc[3]; own.p = pc[4];
+  double acc = 0.0;
+  double D[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double Sr[6], F[5], Fs;
+  const int e0 = m.cfl_i[c], e1 = m.cfl_i[c + 1];
+  for (int e = e0; e < e1; e++) {
+    const int2 en = m.cfl[e];
+    const int f = en.x & 0x7fffffff;
+    const double *geo = m.fgeo + 8 * (size_t)f;
+    const double n[3] = {geo[0], geo[1], geo[2]};
+    const double area = geo[3];
+    const double xfa[3] = {geo[4], geo[5], geo[6]};
+    if (en.y >= 0) {
+      const int nb = en.y;
+      const double *pn = prim + 8 * (size_t)nb;
+      const double *auxn = P.sst_aux + kSstN * (size_t)nb;
+      const double phn = phi[nb], kn = kin[nb];
+      Side nbs;
+      nbs.rho = pn[0]; nbs.u[0] = pn[1]; nbs.u[1] = pn[2]; nbs.u[2] = pn[3]; nbs.p = pn[4];
+      Side so = own, sn = nbs;
+      double sco = ph, scn = phn, kfo = kc, kfn = kn;
+      if (snd) {
+        if (reconstruct(so, pc, gc, xfa)) {
+          const double r[3] = {xfa[0] - pc[5], xfa[1] - pc[6], xfa[2] - pc[7]};
+          sco = sco + dot3(r, auxc + gS);
+          kfo = kfo + dot3(r, auxc + kSGradK);
+        }
+        if (reconstruct(sn, pn, grad + 16 * (size_t)nb, xfa)) {
+          const double r[3] = {xfa[0] - pn[5], xfa[1] - pn[6], xfa[2] - pn[7]};
+          scn = scn + dot3(r, auxn + gS);
+          kfn = kfn + dot3(r, auxn + kSGradK);
+        }
+      }
+      const bool s0 = en.x >= 0;
+      const int noc = m.cfl_noc[e];
+      const bool accum = (noc >> 30) & 1;
+      double *off = AS + 6 * (size_t)(noc & 0x3fffffff);
+      const Side fl = s0 ? so : sn, fr = s0 ? sn : so;
+      const Side jl = s0 ? own : nbs, jr = s0 ? nbs : own;
+      const double phl = s0 ? sco : scn, phr = s0 ? scn : sco, pjl = s0 ? ph : phn, pjr = s0 ? phn : ph;
+      const double kl = s0 ? kfo : kfn, kr = s0 ? kfn : kfo, kjl = s0 ? kc : kn, kjr = s0 ? kn : kc;
+      flux_hllc(F, fl.rho, fl.u, fl.p, fr.rho, fr.u, fr.p, gam, area, n, phl, phr, &Fs, kl, kr);
+      acc += s0 ? -Fs : Fs;
+      jac_hllc_scal(Sr, 0, jl.rho, jl.u, jl.p, jr.rho, jr.u, jr.p, gam, area, n, pjl, pjr, kjl, kjr);
+      if (s0) {
+        for (int k = 0; k < 6; k++) D[k] += Sr[k];
+      } else {
+        for (int k = 0; k < 6; k++) off[k] = accum ? off[k] - Sr[k] : 0.0 - Sr[k];
+      }
+      jac_hllc_scal(Sr, 1, jl.rho, jl.u, jl.p, jr.rho, jr.u, jr.p, gam, area, n, pjl, pjr, kjl, kjr);
+      if (s0) {
+        for (int k = 0; k < 6; k++) off[k] = accum ? off[k] + Sr[k] : 0.0 + Sr[k];
+      } else {
+        for (int k = 0; k < 6; k++) D[k] -= Sr[k];
+      }
+    } else {
+      const ZoneDev &z = m.zones[-en.y - 1];
+      const double *bb = bfa + 8 * (size_t)f;
+      const double phb = P.sst_bfa[2 * (size_t)f + S], kb = P.sst_bfa[2 * (size_t)f];
+      const double ub[3] = {bb[1], bb[2], bb[3]};
+      if (z.bc == kBcSymmetry || z.bc == kBcWall) {
+        flux_hllc(F, bb[0], ub, bb[4], bb[0], ub, bb[4], gam, area, n, phb, phb, &Fs, kb, kb);
+        acc += -Fs;
+        jac_hllc_scal(Sr, 0, bb[0], ub, bb[4], bb[0], ub, bb[4], gam, area, n, phb, phb, kb, kb);
+      } else {
+        Side so = own;
+        double sco = ph, kfo = kc;
+        if (snd && reconstruct(so, pc, gc, xfa)) {
+          const double r[3] = {xfa[0] - pc[5], xfa[1] - pc[6], xfa[2] - pc[7]};
+          sco = sco + dot3(r, auxc + gS);
+          kfo = kfo + dot3(r, auxc + kSGradK);
+        }
+        flux_hllc(F, so.rho, so.u, so.p, bb[0], ub, bb[4], gam, area, n, sco, phb, &Fs, kfo, kb);
+        acc += -Fs;
+        jac_hllc_scal(Sr, 0, own.rho, own.u, own.p, bb[0], ub, bb[4], gam, area, n, ph, phb, kc, kb);
+      }
+      for (int k = 0; k < 6; k++) D[k] += Sr[k];
+    }
+  }
+  // diffusion, after the convective pass (JoeWithModels.cpp:2595-2610)
+  constexpr double sg1 = S == 0 ? kSstSigmaK1 : kSstSigmaW1, sg2 = S == 0 ? kSstSigmaK2 : kSstSigmaW2;
+  const double Tc = pc[4] / (pc[0] * P.R);
+  for (int e = e0; e < e1; e++) {
+    const int2 en = m.cfl[e];
+    const int f = en.x & 0x7fffffff;
+    const double *geo = m.fgeo + 8 * (size_t)f;
+    const double n[3] = {geo[0], geo[1], geo[2]};
+    const double nmag = geo[3];
+    const double xfa[3] = {geo[4], geo[5], geo[6]};
+    if (en.y >= 0) {
+      const int nb = en.y;
+      const double *pn = prim + 8 * (size_t)nb;
+      const double Tn = pn[4] / (pn[0] * P.R);
+      const bool s0 = en.x >= 0;
+      double sV[3], smag, mul, w0, w1;
+      visc_internal_geom(P, s0 ? pc + 5 : pn + 5, s0 ? pn + 5 : pc + 5, xfa, s0 ? Tc : Tn, s0 ? Tn : Tc, sV, smag,
+                         mul, w0, w1);
+      const int c0 = s0 ? c : nb, c1 = s0 ? nb : c;
+      const double rho0 = s0 ? pc[0] : pn[0], rho1 = s0 ? pn[0] : pc[0];
+      const double *a0 = P.sst_aux + kSstN * (size_t)c0, *a1 = P.sst_aux + kSstN * (size_t)c1;
+      const double mut = sst_mut_face(P, ncv, w0, w1, rho0, rho1, c0, c1);
+      const double f1_fa = (w1 * a0[kSF1] + w0 * a1[kSF1]) / (w0 + w1);
+      const double coeff = sg1 * f1_fa + (1. - f1_fa) * sg2;
+      const double diff = mul + coeff * mut;
+      const double phi0 = phi[c0], phi1 = phi[c1];
+      const double *g0 = a0 + gS, *g1 = a1 + gS;
+      const double alpha = dot3(n, sV);
+      const double viscFlux = diff * nmag *
+                              (alpha * (phi1 - phi0) / smag + 0.5 * (g0[0] + g1[0]) * (n[0] - alpha * sV[0]) +
+                               0.5 * (g0[1] + g1[1]) * (n[1] - alpha * sV[1]) +
+                               0.5 * (g0[2] + g1[2]) * (n[2] - alpha * sV[2]));
+      const double a = diff * nmag * alpha / smag;
+      double *off = AS + 6 * (size_t)(m.cfl_noc[e] & 0x3fffffff);
+      if (s0) {
+        acc += viscFlux;
+        D[5] += a / rho0;
+        off[5] = off[5] + -(a / rho1);
+      } else {
+        acc += -viscFlux;
+        D[5] += a / rho1;
+        off[5] = off[5] + -(a / rho0);
+      }
+    } else {
+      const ZoneDev &z = m.zones[-en.y - 1];
+      if (!(S == 0 ? z.skind : z.skind2)) continue;  // no diffusive flux without a value or hook (Scalars.cpp:239-245)
+      const double *bb = bfa + 8 * (size_t)f;
+      const double mul = mu_lam(P, bb[5]);
+      double diff = mul;
+      if (z.bc != kBcWall) {
+        const double coeff = sg1 * auxc[kSF1] + (1. - auxc[kSF1]) * sg2;
+        diff = mul + coeff * sst_mut_cell(P, ncv, pc[0], c);
+      }
+      const double phb = P.sst_bfa[2 * (size_t)f + S];
+      const double s_half[3] = {xfa[0] - pc[5], xfa[1] - pc[6], xfa[2] - pc[7]};
+      const double smag_half = fabs(dot3(s_half, n));
+      acc += diff * nmag * (phb - ph) / smag_half;
+      D[5] += diff * nmag / smag_half / pc[0];
+    }
+  }
+  // source (TurbModel_KOMSST.h:256-304)
+  {
+    const double rho = pc[0], vol = m.vol[c];
+    const double kine = kc, omega = P.sst_phi[ncv + c];
+    const double lim = sst_lim(P, auxc), F2 = auxc[kSF2], str = auxc[kSStr], div = auxc[kSDiv];
+    if (S == 0) {
+      const double zeta = Dev::mn(1.0 / omega, kSstA1 / (lim * F2));
+      const double mut = Dev::mn(Dev::mx(rho * kine * zeta, 0.0), 1.0e5);
+      double Pk = mut * str * str - 2. / 3. * rho * kine * div;
+      if (P.sst_limit_pk == 1) Pk = Dev::mn(Pk, 20.0 * kSstBetaStar * rho * kine * omega);
+      Pk = Dev::mx(Pk, 0.0);
+      const double src = Pk - kSstBetaStar * rho * omega * kine;
+      acc += src * vol;
+      const double dsrcdphi = -kSstBetaStar * omega;
+      D[5] -= dsrcdphi * vol;
+    } else {
+      const double F1 = auxc[kSF1];
+      const double alfa = F1 * P.sst_alfa[0] + (1.0 - F1) * P.sst_alfa[1];
+      const double beta = F1 * kSstBeta1 + (1.0 - F1) * kSstBeta2;
+      const double zeta = Dev::mx(omega, lim * F2 / kSstA1);
+      double Pk = str * str - 2. / 3. * zeta * div;
+      Pk = Dev::mx(Pk, 0.0);
+      const double src = alfa * rho * Pk + (1.0 - F1) * auxc[kSCd] - beta * rho * omega * omega;
+      acc += src * vol;
+      const double dsrcdphi = -2.0 * beta * omega;
+      D[5] -= dsrcdphi * vol;
+    }
+  }
+  rhs_s[c] = acc;
+  double *dg = AS + 6 * (size_t)m.nbocv_i[c];
+  for (int k = 0; k < 6; k++) dg[k] = D[k];
+}
+
+// the segregated scalar's update and clip with its own bounds
+// (JoeWithModels.cpp:1084-1088)
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_sc_update_b(int ncv, const double *__restrict__ q,
+                                                        const T *__restrict__ dq, const double *__restrict__ x,
+                                                        double *__restrict__ phi, double lo, double hi) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncv) return;
+  const double rho = q[c];
+  const double v = (phi[c] * (rho - (double)dq[5 * (size_t)c]) + x[c]) / rho;
+  phi[c] = Dev::mn(Dev::mx(v, lo), hi);
 }
 
 // ---------------------------------------------------------------------------
@@ -2703,7 +3218,8 @@ __device__ double fin_slot(const double *__restrict__ p, int nb, int op, double 
 enum FinOp {
   kFinDt = 0, kFinL1 = 1, kFinAlpha = 2, kFinOmega = 3, kFinResid0 = 4, kFinResid = 5,
   kFinResid0J = 6,  // kFinResid0 of the Jacobi (scalar) solver, which stops at once if already converged
-  kFinL1S = 7       // L1 of the scalar residual
+  kFinL1S = 7,      // L1 of the scalar residual
+  kFinL1S2 = 8      // L1 of the second scalar's residual (SST omega)
 };
 
 // one block: finishes a reduction and applies the scalar update of its owner
@@ -2728,6 +3244,9 @@ __device__ void fin_body(int nb, int op, const double *__restrict__ partials, si
   } else if (op == kFinL1S) {
     double v = fin_slot(partials, nb, kSum, sh);
     if (t0) S->resid_s = v;
+  } else if (op == kFinL1S2) {
+    double v = fin_slot(partials, nb, kSum, sh);
+    if (t0) S->resid_s2 = v;
   } else if (op == kFinAlpha) {  // UgpWithCv.cpp:2775-2782
     double gamma = fin_slot(partials, nb, kSum, sh);
     if (t0) {
@@ -2841,7 +3360,9 @@ __global__ void __launch_bounds__(kBlock) k_flux(DMesh m, DParams P, const doubl
 // face's flux (signed for this cell) and the face's wave-speed bound; the
 // cell's owner lane adds the terms from LDS in entry order (the reference's
 // order), applies the gravity source, and writes the residual and local dt.
-template <int SCHEME, bool SND, bool VISC, int W = kWaves>
+// SST: the sides' turbulent kinetic energy in the HLLC flux (kineFA, JoeWithModels.cpp:2924-2930,
+// 3140-3147, 3466-3471), the kinetic-energy viscous terms and the SST eddy viscosity
+template <int SCHEME, bool SND, bool VISC, int W = kWaves, bool SST = false>
 __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const double *__restrict__ prim,
                                                    const double *__restrict__ grad, const double *__restrict__ bfa,
                                                    const double *__restrict__ q, double *__restrict__ rhs,
@@ -2904,6 +3425,7 @@ __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const
     // after the branches, so a wavefront whose lanes face different ways, or
     // mixes internal and boundary faces, runs the flux arithmetic once
     Side fl, fr;
+    double kl = 0.0, kr = 0.0;  // SST: kineFA of the flux's two sides
     bool symm_analytic = false;
     double hl = 0.0, hr = 0.0;  // side enthalpies of the coupled flux
     if (en.y >= 0) {
@@ -2916,6 +3438,7 @@ __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const
         pn[k + 1] = v.y;
       }
       Side so, sn;
+      bool okn = true;
       if (SND) {
         const double *g = grad + 16 * (size_t)en.y;
         for (int k = 0; k < 14; k += 2) {
@@ -2925,7 +3448,7 @@ __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const
         }
         gn[14] = g[14];
         low = !reconstruct(so, pc, gc, xfa);
-        reconstruct(sn, pn, gn, xfa);
+        okn = reconstruct(sn, pn, gn, xfa);
       } else {
         so = own_cc;
         sn.rho = pn[0]; sn.u[0] = pn[1]; sn.u[1] = pn[2]; sn.u[2] = pn[3]; sn.p = pn[4];
@@ -2933,17 +3456,41 @@ __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const
       fl = s0 ? so : sn;
       fr = s0 ? sn : so;
       if (!s0) sgn = 1.0;
+      double ko = 0.0, kn = 0.0;  // SST: kine of the cell and the neighbour
+      if (SST) {
+        ko = P.sst_phi[L.c];
+        kn = P.sst_phi[en.y];
+        double kfo = ko, kfn = kn;
+        if (SND && !low) {
+          const double r[3] = {xfa[0] - pc[5], xfa[1] - pc[6], xfa[2] - pc[7]};
+          kfo = kfo + dot3(r, P.sst_aux + kSstN * (size_t)L.c + kSGradK);
+        }
+        if (SND && okn) {
+          const double r[3] = {xfa[0] - pn[5], xfa[1] - pn[6], xfa[2] - pn[7]};
+          kfn = kfn + dot3(r, P.sst_aux + kSstN * (size_t)en.y + kSGradK);
+        }
+        kl = s0 ? kfo : kfn;
+        kr = s0 ? kfn : kfo;
+      }
       if (SCHEME == kHllcC) {  // calcThermoProp_p after reconstruction, or the cell's (JoeWithModels.cpp:4920-4928)
         hl = enthalpy_of(P, fl.rho, fl.p);
         hr = enthalpy_of(P, fr.rho, fr.p);
       }
       if (VISC) {  // calcViscousFluxNS internal faces (JoeWithModels.cpp:3635-3710)
-        const VSide vo = vside_cell(P, pc, gradv + 12 * (size_t)L.c);
-        const VSide vn = vside_cell(P, pn, gradv + 12 * (size_t)en.y);
+        VSide vo = vside_cell(P, pc, gradv + 12 * (size_t)L.c);
+        VSide vn = vside_cell(P, pn, gradv + 12 * (size_t)en.y);
+        if (SST) {
+          vo.k = ko;
+          vn.k = kn;
+        }
         const VSide &a = s0 ? vo : vn, &b = s0 ? vn : vo;
         double sV[3], smag, w0, w1;
         visc_internal_geom(P, s0 ? pc + 5 : pn + 5, s0 ? pn + 5 : pc + 5, xfa, a.T, b.T, sV, smag, mul, w0, w1);
-        if (P.sa) {
+        double kine_fa = 0.0;
+        if (SST) {
+          mut = sst_mut_face(P, m.ncv, w0, w1, a.rho, b.rho, s0 ? L.c : en.y, s0 ? en.y : L.c);
+          kine_fa = (w1 * a.k + w0 * b.k) / (w0 + w1);
+        } else if (P.sa) {
           const double nuo = P.sa_phi[L.c], nun = P.sa_phi[en.y];
           mut = sa_mut_face(w0, w1, a.rho, b.rho, s0 ? nuo : nun, s0 ? nun : nuo, mul);
         }
@@ -2955,7 +3502,7 @@ __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const
         } else {
           double Fu[3], FE;
           VFace vf;
-          visc_face(P, a, b, mul, mut, ufa, area, n, smag, sV, Fu, FE, vf);
+          visc_face<SST>(P, a, b, mul, mut, ufa, area, n, smag, sV, Fu, FE, vf, kine_fa);
           for (int k = 0; k < 3; k++) vc[k] = s0 ? -Fu[k] : Fu[k];
           vc[3] = s0 ? -FE : FE;
         }
@@ -2967,9 +3514,12 @@ __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const
       sb.rho = bb[0]; sb.u[0] = bb[1]; sb.u[1] = bb[2]; sb.u[2] = bb[3]; sb.p = bb[4];
       hl = bb[6];
       hr = bb[6];
+      const double kb = SST ? P.sst_bfa[2 * (size_t)f] : 0.0;
       if (z.bc == kBcSymmetry || z.bc == kBcWall || (SCHEME == kHllcC && z.bc == kBcNeumann)) {
         fl = sb;
         fr = sb;
+        kl = kb;
+        kr = kb;
         // the coupled loop treats walls as symmetry faces and NEUMANN faces
         // with the boundary state on both sides (JoeWithModels.cpp:5150-5470)
         symm_analytic = SCHEME == kHllcC ? (z.bc != kBcNeumann && P.symm_bc == kSymmAnalytical)
@@ -2979,6 +3529,14 @@ __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const
         if (SND) low = !reconstruct(so, pc, gc, xfa);
         fl = so;
         fr = sb;
+        if (SST) {
+          kl = P.sst_phi[L.c];
+          if (SND && !low) {
+            const double r[3] = {xfa[0] - pc[5], xfa[1] - pc[6], xfa[2] - pc[7]};
+            kl = kl + dot3(r, P.sst_aux + kSstN * (size_t)L.c + kSGradK);
+          }
+          kr = kb;
+        }
         if (SCHEME == kHllcC) hl = enthalpy_of(P, so.rho, so.p);
       }
       if (VISC && CPL) {  // calcFluxCoupled boundary faces; mul from T_bfa
@@ -3001,12 +3559,25 @@ __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const
         }
       } else if (VISC) {  // boundary faces (:3714-3851); boundary mul from T_bfa (ComputeBCProperties_T)
         mul = mu_lam(P, bb[5]);
-        if (P.sa && z.bc != kBcWall)  // mut of every non-wall boundary face, symmetry included (calcDt reads it)
+        if (SST && z.bc != kBcWall)  // mut of every non-wall boundary face, symmetry included (calcDt reads it)
+          mut = sst_mut_cell(P, m.ncv, pc[0], L.c);
+        else if (P.sa && z.bc != kBcWall)
           mut = sa_mut(pc[0] * P.sa_phi[L.c], P.sa_aux[kAuxN * (size_t)L.c + kAuxMue]);
+        if (SST && z.bc == kBcSymmetry) {  // -1/3 (rho + rho_bfa) kine_fa on the momentum (:3727-3745)
+          const double tmp = 1.0 / 3.0 * (pc[0] + bb[0]) * kb;
+          const double *fan = m.fnorm + 3 * (size_t)f;
+          for (int k = 0; k < 3; k++) vc[k] = -(tmp * fan[k]);
+        }
         if (z.bc != kBcSymmetry) {
           const double *gvo = gradv + 12 * (size_t)L.c;
-          const VSide vo = vside_cell(P, pc, gvo);
-          const VSide vb = vside_bfa(bb, gvo);
+          VSide vo = vside_cell(P, pc, gvo);
+          VSide vb = vside_bfa(bb, gvo);
+          double kine_fa = 0.0;
+          if (SST) {  // walls: kine1 = kine_fa = 0 (:3767-3776); other zones phi_bfa (:3817-3826)
+            vo.k = P.sst_phi[L.c];
+            vb.k = z.bc == kBcWall ? 0.0 : kb;
+            kine_fa = vb.k;
+          }
           double sV[3] = {xfa[0] - pc[5], xfa[1] - pc[6], xfa[2] - pc[7]};
           double smag;
           const double *sArg;
@@ -3020,7 +3591,7 @@ __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const
           }
           double Fu[3], FE;
           VFace vf;
-          visc_face(P, vo, vb, mul, mut, vb.u, area, n, smag, sArg, Fu, FE, vf);
+          visc_face<SST>(P, vo, vb, mul, mut, vb.u, area, n, smag, sArg, Fu, FE, vf, kine_fa);
           for (int k = 0; k < 3; k++) vc[k] = -Fu[k];
           vc[3] = -FE;
         }
@@ -3029,7 +3600,7 @@ __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const
     if (SCHEME == kHllcC)
       flux_hllc_coupled(F, fl.rho, fl.u, fl.p, hl, fr.rho, fr.u, fr.p, hr, gam, area, n);
     else
-      face_flux(SCHEME, F, fl, fr, gam, area, n);
+      face_flux(SCHEME, F, fl, fr, gam, area, n, kl, kr);
     if (symm_analytic) {  // the ONLY_PRESSURE flux
       F[0] = 0.0;
       for (int i = 0; i < 3; i++) F[1 + i] = fr.p * n[i] * area;
@@ -3119,7 +3690,7 @@ __device__ __forceinline__ void d_flux_e(const Blk &B, DMesh m, DParams P, const
 // wavefront's lanes, not for the slowest of a larger block
 constexpr int kFluxWaves = 1;
 
-template <int SCHEME, bool SND, bool VISC>
+template <int SCHEME, bool SND, bool VISC, bool SST = false>
 __global__ void __launch_bounds__(64 * kFluxWaves) k_flux_e(DMesh m, DParams P, const double *__restrict__ prim,
                                                             const double *__restrict__ grad,
                                                             const double *__restrict__ bfa,
@@ -3127,7 +3698,8 @@ __global__ void __launch_bounds__(64 * kFluxWaves) k_flux_e(DMesh m, DParams P, 
                                                             int *__restrict__ nfirst, DtArgs dta,
                                                             double *__restrict__ dt, double *__restrict__ partials,
                                                             const double *__restrict__ gradv) {
-  d_flux_e<SCHEME, SND, VISC, kFluxWaves>(hw_blk(), m, P, prim, grad, bfa, q, rhs, nfirst, dta, dt, partials, gradv);
+  d_flux_e<SCHEME, SND, VISC, kFluxWaves, SST>(hw_blk(), m, P, prim, grad, bfa, q, rhs, nfirst, dta, dt, partials,
+                                               gradv);
 }
 
 // Face form of the inviscid flux pass: k_face_flux evaluates each internal
@@ -4367,7 +4939,7 @@ __global__ void __launch_bounds__(kBlock) k_l1s(int ncv, const double *__restric
 // centroids and area-weighted normals of the wall zones, in zone order.
 __global__ void __launch_bounds__(kBlock) k_wall_dist(int ncv, const double *__restrict__ prim, int nw,
                                                       const double *__restrict__ wx, const double *__restrict__ wn,
-                                                      double *__restrict__ aux) {
+                                                      double *__restrict__ aux, int stride, int slot) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncv) return;
   const double x[3] = {prim[8 * (size_t)c + 5], prim[8 * (size_t)c + 6], prim[8 * (size_t)c + 7]};
@@ -4389,7 +4961,7 @@ __global__ void __launch_bounds__(kBlock) k_wall_dist(int ncv, const double *__r
   const double dNorm2 = dot3(s_half, n) * dot3(s_half, n);
   const double dTang2 = sMag2 - dNorm2;
   const double specRadius2 = 0.5 * nmag;
-  aux[kAuxN * (size_t)c + kAuxWall] = dTang2 > specRadius2 ? minDist : fabs(dot3(s_half, n));
+  aux[stride * (size_t)c + slot] = dTang2 > specRadius2 ? minDist : fabs(dot3(s_half, n));
 }
 
 
@@ -4547,8 +5119,26 @@ void parse_params(const std::string &joe_in, const Mesh &m, SolverParams &p, std
       p.sa_init = ParamMap::to_d((*t)[1], "INITIAL_CONDITION_TURB");
       p.relax_scal = pm.dbl("UNDER_RELAXATION_SCALARS", 0.2);
       if (pm.has("SCALAR_FIRST_ORDER")) throw Unsupported("SCALAR_FIRST_ORDER is not supported");
+    } else if (tm == "SST") {  // RansTurbKOmSST (TurbModel_KOMSST.h), kine and omega segregated
+      p.sst = true;
+      if (!(p.mu_ref > 0.0)) throw std::invalid_argument("TURB_MODEL = SST needs MU_REF > 0");
+      const auto *t = pm.get("INITIAL_CONDITION_TURB");
+      if (!t || t->size() < 3)
+        throw std::invalid_argument("INITIAL_CONDITION_TURB needs kine and omega (JoeWithModels.cpp:223-232)");
+      p.sst_init[0] = ParamMap::to_d((*t)[1], "INITIAL_CONDITION_TURB");
+      p.sst_init[1] = ParamMap::to_d((*t)[2], "INITIAL_CONDITION_TURB");
+      p.relax_scal = pm.dbl("UNDER_RELAXATION_SCALARS", 0.2);
+      p.sst_limit_pk = pm.integer("SST_LIMIT_PK", 1);
+      p.sst_limiter_shear = pm.integer("SST_LIMITER_SHEARSTRESS", 0);
+      if (pm.has("SCALAR_FIRST_ORDER")) throw Unsupported("SCALAR_FIRST_ORDER is not supported");
+      if (pm.str("SCALAR_TURB_RECONSTRUCTION", "STANDARD") != "STANDARD")
+        throw Unsupported("SCALAR_TURB_RECONSTRUCTION other than STANDARD is not supported");
+      // the model constants and solver settings are the reference's defaults (TurbModel_KOMSST.h:16-52)
+      for (const char *k : {"sigma_k1", "sigma_k2", "sigma_om1", "sigma_om2", "beta_1", "beta_2", "betaStar", "a1",
+                            "ZERO_kine", "ZERO_REL_kine", "ZERO_omega", "ZERO_REL_omega"})
+        if (pm.has(k)) throw Unsupported(std::string(k) + ": only the SST defaults are supported");
     } else if (tm != "NONE") {
-      throw Unsupported("TURB_MODEL = " + tm + " is not supported (NONE, SA)");
+      throw Unsupported("TURB_MODEL = " + tm + " is not supported (NONE, SA, SST)");
     }
   }
   std::string si = pm.str("SPACE_INTEGRATION", "HLLC");
@@ -4557,6 +5147,9 @@ void parse_params(const std::string &joe_in, const Mesh &m, SolverParams &p, std
   else if (si == "AUSM") p.scheme = kAusm;
   else throw Unsupported("SPACE_INTEGRATION = " + si + " is not supported (HLLC, ROE, AUSM)");
   if (p.sa && p.scheme != kHllc) throw Unsupported("TURB_MODEL = SA is supported with HLLC only");
+  if (p.sst && p.scheme != kHllc) throw Unsupported("TURB_MODEL = SST is supported with HLLC only");
+  if (p.sst && p.jac_kernel != kResidEntry)
+    throw Unsupported("TURB_MODEL = SST needs the per-entry Jacobian (GPU_JAC_KERNEL = ENTRY)");
   if (pm.has("GRAD_RECONSTRUCTION")) {
     std::string s = pm.sub("GRAD_RECONSTRUCTION", "SCHEME");
     if (s == "LEASTSQUARES") p.grad_ls = true;
@@ -4597,6 +5190,7 @@ void parse_params(const std::string &joe_in, const Mesh &m, SolverParams &p, std
   if (ti == "BACKWARD_EULER_COUPLED") p.coupled = true;  // JoeWithModels.cpp:39, 4326-4594
   else if (ti != "BACKWARD_EULER")
     throw Unsupported("TIME_INTEGRATION = " + ti + " is not supported (BACKWARD_EULER, BACKWARD_EULER_COUPLED)");
+  if (p.coupled && p.sst) throw Unsupported("TURB_MODEL = SST in the coupled loop is not supported");
   std::string ls = pm.str("LINEAR_SOLVER_NS", "BCGSTAB");
   if (p.coupled) {
     // the reference's coupled loop solves with PETSc GMRES only
@@ -4688,12 +5282,43 @@ void parse_params(const std::string &joe_in, const Mesh &m, SolverParams &p, std
         b.sval = ParamMap::to_d(v, z.name + ".nuSA");
       }
     }
+    // kine and omega boundary values (the same rules; omega = HOOK on a WALL
+    // zone takes boundaryHookScalarRansTurb's wall value, TurbModel_KOMSST.h:342-358)
+    if (p.sst) {
+      for (int s = 0; s < 2; s++) {
+        const std::string nm = s == 0 ? "kine" : "omega";
+        if (pm.has(z.name + "." + nm + ".flux")) throw Unsupported("zone " + z.name + ": scalar flux boundaries");
+        const auto *st = pm.get(z.name + "." + nm);
+        if (!st) continue;
+        if (st->size() < 2) throw std::invalid_argument("zone " + z.name + "." + nm + ": missing value");
+        const std::string &v = (*st)[1];
+        int kind = 1;
+        double val = 0.0;
+        if (v == "HOOK" || v == "Hook" || v == "hook") {
+          if (s == 0 || b.bc != kBcWall)
+            throw Unsupported("zone " + z.name + "." + nm + " = HOOK: the SST hook sets omega on WALL zones only");
+          kind = 2;
+        } else if (v == "PROFILES") {
+          throw Unsupported("zone " + z.name + "." + nm + " = PROFILES is not supported");
+        } else {
+          val = ParamMap::to_d(v, z.name + "." + nm);
+        }
+        if (s == 0) {
+          b.skind = kind;
+          b.sval = val;
+        } else {
+          b.skind2 = kind;
+          b.sval2 = val;
+        }
+      }
+    }
   }
-  if (p.sa) {
+  if (p.sa || p.sst) {
     bool wall = false;
     for (const ZoneBc &b : zbc) wall = wall || b.bc == kBcWall;
-    if (!wall) throw std::invalid_argument("TURB_MODEL = SA needs a WALL zone for the wall distance");
+    if (!wall) throw std::invalid_argument("the turbulence model needs a WALL zone for the wall distance");
   }
+  if (p.sst && !p.profiles_file.empty()) throw Unsupported("PROFILES boundaries with SST are not supported");
   if (!p.profiles_file.empty()) (void)profile_faces(p.profiles_file, m, zbc, p.sa ? 1 : 0);  // checked here, read again at set-up
 }
 
@@ -4750,6 +5375,11 @@ struct Ctx::Impl {
   // rows [nnz][6], Jacobi BiCGStab vectors (b, x, res, res0, p, v, phat, s,
   // shat, t), its scalars, the faocv entry map
   DBuf<double> sa_phi, sa_aux, rhs_s, AS, sw[10];
+  // Menter SST: kine, omega [2][ncv], per-cell fields [ncv][kSstN], boundary
+  // values [nfa_b][2]; residuals rhs_s [2][ncv] and AScal rows AS [2][nnz][6]
+  DBuf<double> sst_phi, sst_aux, sst_bfa;
+  DBuf<double> sst_dscal;  // the last step's kine and omega corrections [2][ncv] (DSCAL)
+  int scal_iters2 = 0;
   DBuf<SolverScalars> scal_s;
   SolverScalars *h_scal_s = nullptr;  // pinned
   DBuf<int> foc;
@@ -4772,6 +5402,8 @@ struct Ctx::Impl {
   void push_source(bool with_a);
   template <typename T>
   void scalar_step(const T *dq);
+  template <typename T>
+  void sst_scalar_step(const T *dq);
   void finish_step(StepInfo &info, int it, int st);
   // split step (host hooks): host source terms in device order, phase flag
   DBuf<double> src;
@@ -4892,6 +5524,8 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
     for (int k = 0; k < 8; k++) z.par[k] = d.zbc[iz].par[k];
     z.skind = d.zbc[iz].skind;
     z.sval = d.zbc[iz].sval;
+    z.skind2 = d.zbc[iz].skind2;
+    z.sval2 = d.zbc[iz].sval2;
     d.zone_slot[iz] = (int)zd.size();
     zd.push_back(z);
   }
@@ -4943,6 +5577,10 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
     d.min_faces = std::min(d.min_faces, (int)cf.size() - ci[c]);
   }
   ci[ncv] = (int)cf.size();
+  // SST runs the per-entry Jacobian only
+  if (d.par.sst && !d.jac_entry())
+    throw Unsupported("TURB_MODEL = SST needs the per-entry Jacobian (GPU_JAC_KERNEL = ENTRY, cells of four or more "
+                      "faces, no duplicate neighbour pairs)");
 
   // Device numbering (internal only; the ABI speaks reference numbering),
   // chosen by GPU_CELL_ORDER: the reference's own order (NATIVE), or Morton
@@ -5001,7 +5639,7 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
   // face records in device order: unit normal and area as normVec3d, x_fa
   std::vector<double> fg(8 * (size_t)nfa);
   std::vector<double> fn;
-  if (!d.par.grad_ls || d.par.sa) fn.resize(3 * (size_t)nfa);
+  if (!d.par.grad_ls || d.par.sa || d.par.sst) fn.resize(3 * (size_t)nfa);
   for (int f = 0; f < nfa; f++) {
     const double *n = &m.fa_normal[3 * (size_t)f];
     double mag = 0.0;
@@ -5102,7 +5740,7 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
     // Spalart-Allmaras and SUM_BASED time steps: for each cell, the entry
     // offsets of its faces in the reference's faocv order (the order of
     // InterpolateAtCellCenterFromFaceValues and of calcDt's face loop)
-    if (d.par.sa || d.par.dt_sum) {
+    if (d.par.sa || d.par.sst || d.par.dt_sum) {
       std::vector<int> fo(std::max<size_t>(1, dcf.size()), 0);
       size_t k = 0;
       for (int cd = 0; cd < ncv; cd++) {
@@ -5216,7 +5854,47 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
     dwx.upload(wx.data(), wx.size());
     dwn.upload(wn.data(), wn.size());
     hipLaunchKernelGGL(k_wall_dist, dim3(nblk(ncv)), dim3(kBlock), 0, d.stream, ncv, d.prim.p, (int)(wx.size() / 3),
-                       dwx.p, dwn.p, d.sa_aux.p);
+                       dwx.p, dwn.p, d.sa_aux.p, (int)kAuxN, (int)kAuxWall);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(d.stream));
+  }
+  if (d.par.sst) {
+    std::vector<double> ph(2 * (size_t)n);  // initialHook (JoeWithModels.cpp:219-240)
+    for (size_t c = 0; c < n; c++) {
+      ph[c] = d.par.sst_init[0];
+      ph[n + c] = d.par.sst_init[1];
+    }
+    d.sst_phi.upload(ph.data(), ph.size());
+    d.sst_aux.alloc(kSstN * n);
+    HIPCHECK(hipMemset(d.sst_aux.p, 0, sizeof(double) * kSstN * n));
+    d.sst_bfa.alloc(2 * (size_t)std::max(nfa_b, 1));
+    HIPCHECK(hipMemset(d.sst_bfa.p, 0, sizeof(double) * 2 * (size_t)std::max(nfa_b, 1)));
+    d.rhs_s.alloc(2 * n);
+    d.AS.alloc(12 * (size_t)d.nnz);
+    HIPCHECK(hipMemset(d.AS.p, 0, sizeof(double) * 12 * (size_t)d.nnz));
+    for (int k = 0; k < 10; k++) d.sw[k].alloc(n);
+    d.sst_dscal.alloc(2 * n);
+    HIPCHECK(hipMemset(d.sst_dscal.p, 0, sizeof(double) * 2 * n));
+    d.scal_s.alloc(1);
+    HIPCHECK(hipMemset(d.scal_s.p, 0, sizeof(SolverScalars)));
+    HIPCHECK(hipHostMalloc((void **)&d.h_scal_s, sizeof(SolverScalars)));
+    // wall distance (calcWallDistance, initialHookScalarRansTurbModel, TurbModel_KOMSST.h:82-98)
+    std::vector<double> wx, wn;
+    for (size_t iz = 0; iz < m.zones.size(); iz++) {
+      const Zone &z = m.zones[iz];
+      if (z.kind != kZoneBoundary || d.zbc[iz].bc != kBcWall) continue;
+      for (int f = z.ifa_f; f <= z.ifa_l; f++)
+        for (int i = 0; i < 3; i++) {
+          wx.push_back(m.x_fa[3 * (size_t)f + i]);
+          wn.push_back(m.fa_normal[3 * (size_t)f + i]);
+        }
+    }
+    if (wx.empty()) throw std::invalid_argument("TURB_MODEL = SST: the WALL zones have no faces");
+    DBuf<double> dwx, dwn;
+    dwx.upload(wx.data(), wx.size());
+    dwn.upload(wn.data(), wn.size());
+    hipLaunchKernelGGL(k_wall_dist, dim3(nblk(ncv)), dim3(kBlock), 0, d.stream, ncv, d.prim.p, (int)(wx.size() / 3),
+                       dwx.p, dwn.p, d.sst_aux.p, (int)kSstN, (int)kSWall);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(d.stream));
   }
@@ -5271,6 +5949,14 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
   d.dp.fdump = nullptr;
   d.dp.vdump = nullptr;
   d.dp.sa_aux = d.sa_aux.p;
+  d.dp.sst = d.par.sst ? 1 : 0;
+  d.dp.sst_limit_pk = d.par.sst_limit_pk;
+  d.dp.sst_lim_shear = d.par.sst_limiter_shear;
+  d.dp.sst_phi = d.sst_phi.p;
+  d.dp.sst_aux = d.sst_aux.p;
+  d.dp.sst_bfa = d.sst_bfa.p;
+  d.dp.sst_alfa[0] = 0.075 / 0.09 - 0.5 * std::pow(0.41, 2.0) / std::sqrt(0.09);
+  d.dp.sst_alfa[1] = 0.0828 / 0.09 - 0.856 * std::pow(0.41, 2.0) / std::sqrt(0.09);
   d.cfl = d.par.cfl;
   HIPCHECK(hipDeviceSynchronize());
 }
@@ -5320,6 +6006,7 @@ size_t Ctx::footprint(const Mesh &m, int precision, bool sa) {
 // hipGraph and replayed; results are identical to step()'s.
 void Ctx::device_step_async() {
   if (d_->par.sa) throw Unsupported("the graph-captured step does not carry the Spalart-Allmaras scalar");
+  if (d_->par.sst) throw Unsupported("the graph-captured step does not carry the SST scalars");
   Impl &d = *d_;
   if (!d.have_state) throw std::logic_error("no state set");
   if (d.begun) throw std::logic_error("device step between step_begin and step_end");
@@ -5757,6 +6444,7 @@ void StepBatch::post(void *stream) const {
 void Ctx::batch_prepare() {
   Impl &d = *d_;
   if (d.par.sa) throw Unsupported("batched serving does not carry the Spalart-Allmaras scalar");
+  if (d.par.sst) throw Unsupported("batched serving does not carry the SST scalars");
   if (!d.flux_entry() || d.face_form())
     throw Unsupported("batched serving runs the per-entry flux kernel (GPU_FLUX_KERNEL = ENTRY)");
   if (!d.have_state) throw std::logic_error("no state set");
@@ -5883,23 +6571,37 @@ void Ctx::init_uniform() {
     std::vector<double> ph(n, d.par.sa_init);
     set_scalar(ph.data());
   }
+  if (d.par.sst)  // uniform kine and omega
+    for (int s = 0; s < 2; s++) {
+      std::vector<double> ph(n, d.par.sst_init[s]);
+      set_scalar(ph.data(), s);
+    }
 }
 
-void Ctx::set_scalar(const double *phi) {
+void Ctx::set_scalar(const double *phi, int s) {
   Impl &d = *d_;
-  if (!d.par.sa) throw std::logic_error("no transported scalar (TURB_MODEL = SA)");
+  if (!d.par.sa && !d.par.sst) throw std::logic_error("no transported scalar (TURB_MODEL = SA or SST)");
+  if (s < 0 || s > (d.par.sst ? 1 : 0)) throw std::invalid_argument("no such transported scalar");
   const size_t n = d.ncv;
   std::vector<double> h(n);
   for (size_t cd = 0; cd < n; cd++) h[cd] = phi[d.perm[cd]];
-  HIPCHECK(hipMemcpyAsync(d.sa_phi.p, h.data(), sizeof(double) * n, hipMemcpyHostToDevice, d.stream));
+  double *dst = d.par.sst ? d.sst_phi.p + (size_t)s * n : d.sa_phi.p;
+  HIPCHECK(hipMemcpyAsync(dst, h.data(), sizeof(double) * n, hipMemcpyHostToDevice, d.stream));
   HIPCHECK(hipStreamSynchronize(d.stream));
   d.dev_step_dirty = true;
 }
 
 void Ctx::Impl::launch_state_bc() {
-  hipLaunchKernelGGL(k_state, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, q.p, prim.p, par.gamma, counters.p);
-  if (nfa_b > 0)
+  if (par.sst)
+    hipLaunchKernelGGL(k_state_sst, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, q.p, prim.p, par.gamma, counters.p,
+                       sst_phi.p);
+  else
+    hipLaunchKernelGGL(k_state, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, q.p, prim.p, par.gamma, counters.p);
+  if (nfa_b > 0) {
     hipLaunchKernelGGL(k_bc, dim3(nblk(nfa_b)), dim3(kBlock), 0, stream, dm, dp, prim.p, hook.p, bfa.p);
+    if (par.sst)  // kine and omega on the boundary faces (setScalarBC)
+      hipLaunchKernelGGL(k_sst_bc, dim3(nblk(nfa_b)), dim3(kBlock), 0, stream, dm, dp, prim.p, bfa.p, sst_bfa.p);
+  }
 }
 
 void Ctx::Impl::launch_grad(hipStream_t s) {
@@ -5925,6 +6627,14 @@ void Ctx::Impl::launch_grad(hipStream_t s) {
       hipLaunchKernelGGL(k_sa_cell<true>, dim3(nblk(ncv)), dim3(kBlock), 0, s, dm, dp, prim.p, bfa.p, grad.p, sa_aux.p);
     else
       hipLaunchKernelGGL(k_sa_cell<false>, dim3(nblk(ncv)), dim3(kBlock), 0, s, dm, dp, prim.p, bfa.p, grad.p, sa_aux.p);
+  }
+  if (par.sst) {  // kine and omega gradients, strain, vorticity, divergence, cross diffusion, F1, F2
+    if (par.grad_ls)
+      hipLaunchKernelGGL(k_sst_cell<true>, dim3(nblk(ncv)), dim3(kBlock), 0, s, dm, dp, prim.p, bfa.p, grad.p,
+                         sst_aux.p);
+    else
+      hipLaunchKernelGGL(k_sst_cell<false>, dim3(nblk(ncv)), dim3(kBlock), 0, s, dm, dp, prim.p, bfa.p, grad.p,
+                         sst_aux.p);
   }
 }
 
@@ -5985,6 +6695,11 @@ void Ctx::Impl::launch_flux(hipStream_t s, const double *cfl_dev) {
     return;
   }
   if (flux_entry()) {
+    if (par.sst) {  // HLLC, viscous (the parser requires both)
+      if (par.second_order) goe(k_flux_e<kHllc, true, true, true>);
+      else goe(k_flux_e<kHllc, false, true, true>);
+      return;
+    }
     if (dp.visc) {
       if (par.second_order) {
         if (par.scheme == kRoe) goe(k_flux_e<kRoe, true, true>);
@@ -6036,6 +6751,12 @@ void Ctx::Impl::launch_grad_flux(bool jac_follows) {
   } else if (par.sa) {
     // the nuSA equation: residual and AScal rows (calcRhs scalar terms)
     hipLaunchKernelGGL(k_scal, dim3(nblk(ncv)), dim3(kBlock), 0, stream, dm, dp, prim.p, grad.p, bfa.p, rhs_s.p, AS.p);
+  } else if (par.sst) {
+    // kine's and omega's equations: residuals and AScal rows
+    hipLaunchKernelGGL(k_sst_scal<0>, dim3(nblk(ncv)), dim3(kBlock), 0, stream, dm, dp, prim.p, grad.p, bfa.p, rhs_s.p,
+                       AS.p);
+    hipLaunchKernelGGL(k_sst_scal<1>, dim3(nblk(ncv)), dim3(kBlock), 0, stream, dm, dp, prim.p, grad.p, bfa.p,
+                       rhs_s.p + ncv, AS.p + 6 * (size_t)nnz);
   }
 }
 
@@ -6132,6 +6853,61 @@ void Ctx::Impl::scalar_step(const T *dq) {
   hipLaunchKernelGGL(k_sc_update<T>, g, bl, 0, stream, ncv, q.p, dq, x, sa_phi.p);
 }
 
+// SST's half of a segregated step: kine, then omega, each as scalar_step
+// solves nuSA (JoeWithModels.cpp:1050-1090: relaxation, V/dt, the coupling
+// columns moved right with dq, Jacobi BiCGStab, update and clip with the
+// scalar's bounds); omega's system was formed before kine's update, so the
+// order only sets the order of the two updates
+template <typename T>
+void Ctx::Impl::sst_scalar_step(const T *dq) {
+  double *b = sw[0].p, *x = sw[1].p, *res = sw[2].p, *res0 = sw[3].p, *p = sw[4].p, *vv = sw[5].p, *phat = sw[6].p,
+         *sv = sw[7].p, *shat = sw[8].p, *tv = sw[9].p;
+  const dim3 g(nblk(ncv)), bl(kBlock);
+  const int nb = nblk(ncv);
+  int iters[2] = {0, 0};
+  for (int s = 0; s < 2; s++) {
+    double *rs = rhs_s.p + (size_t)s * ncv, *as = AS.p + 6 * (size_t)nnz * s;
+    hipLaunchKernelGGL(k_sc_prep<T>, g, bl, 0, stream, ncv, nbocv_i.p, nbocv_v.p, vol.p, dt.p, par.relax_scal, rs, as,
+                       dq, b, x);
+    hipLaunchKernelGGL(k_fill<double>, g, bl, 0, stream, (size_t)ncv, p, 0.0);
+    hipLaunchKernelGGL(k_fill<double>, g, bl, 0, stream, (size_t)ncv, vv, 0.0);
+    hipLaunchKernelGGL(k_solver_init, dim3(1), dim3(64), 0, stream, scal_s.p, kSstMaxIter, kSstPhiZero,
+                       kSstPhiZeroRel);
+    hipLaunchKernelGGL(k_sc_resid, g, bl, 0, stream, ncv, nbocv_i.p, nbocv_v.p, as, x, b, res, res0, 1, partials.p,
+                       scal_s.p);
+    hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, nb, (int)kFinResid0J, partials.p, scal_s.p);
+    const int max_it = kSstMaxIter + 1;
+    int launched = 0;
+    while (launched < max_it) {
+      const int chunk = std::min(launched == 0 ? 2 : 4, max_it - launched);
+      for (int k = 0; k < chunk; k++) {
+        hipLaunchKernelGGL(k_sc_p, g, bl, 0, stream, ncv, nbocv_i.p, as, res, vv, p, phat, scal_s.p);
+        hipLaunchKernelGGL(k_sc_spmv_dot, g, bl, 0, stream, ncv, nbocv_i.p, nbocv_v.p, as, phat, vv, res0, partials.p,
+                           scal_s.p);
+        hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, nb, (int)kFinAlpha, partials.p, scal_s.p);
+        hipLaunchKernelGGL(k_sc_s, g, bl, 0, stream, ncv, nbocv_i.p, as, res, vv, sv, shat, scal_s.p);
+        hipLaunchKernelGGL(k_sc_spmv_dot, g, bl, 0, stream, ncv, nbocv_i.p, nbocv_v.p, as, shat, tv, sv, partials.p,
+                           scal_s.p);
+        hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, nb, (int)kFinOmega, partials.p, scal_s.p);
+        hipLaunchKernelGGL(k_sc_x, g, bl, 0, stream, ncv, x, phat, shat, scal_s.p);
+        hipLaunchKernelGGL(k_sc_resid, g, bl, 0, stream, ncv, nbocv_i.p, nbocv_v.p, as, x, b, res, res0, 0,
+                           partials.p, scal_s.p);
+        hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, nb, (int)kFinResid, partials.p, scal_s.p);
+      }
+      launched += chunk;
+      HIPCHECK(hipMemcpyAsync(h_scal_s, scal_s.p, sizeof(SolverScalars), hipMemcpyDeviceToHost, stream));
+      HIPCHECK(hipStreamSynchronize(stream));
+      if (h_scal_s->done) break;
+    }
+    iters[s] = h_scal_s->iter;
+    HIPCHECK(hipMemcpyAsync(sst_dscal.p + (size_t)s * ncv, x, sizeof(double) * ncv, hipMemcpyDeviceToDevice, stream));
+    hipLaunchKernelGGL(k_sc_update_b<T>, g, bl, 0, stream, ncv, q.p, dq, x, sst_phi.p + (size_t)s * ncv,
+                       kSstLower[s], kSstUpper[s]);
+  }
+  scal_iters = iters[0];
+  scal_iters2 = iters[1];
+}
+
 template <typename T>
 void Ctx::Impl::launch_jac(bool step_mode) {
   DBuf<T> *v = nullptr;
@@ -6154,6 +6930,10 @@ void Ctx::Impl::launch_jac(bool step_mode) {
     if (ent) goe(k_jac_e<T, kHllcC, true, 6>);
     else go(k_jac<T, kHllcC, true, 6>);
     launch_scal_c<T>(A, step_mode);
+    return;
+  }
+  if (par.sst) {  // HLLC, viscous, per entry (the constructor checks the mesh allows it)
+    goe(k_jac_e<T, kHllc, true, 5, true>);
     return;
   }
   if (ent) {
@@ -6431,6 +7211,7 @@ void Ctx::Impl::update() {
   }
   hipLaunchKernelGGL(k_update<T>, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, q.p, dq);
   if (par.sa) scalar_step<T>(dq);
+  if (par.sst) sst_scalar_step<T>(dq);
 }
 
 // residual L1 sums of the step just taken (JoeWithModels.cpp:1119-1136) and its report
@@ -6444,6 +7225,13 @@ void Ctx::Impl::finish_step(StepInfo &info, int it, int st) {
                        coupled_sa() ? 1.0 : par.under_relax, partials.p);
     hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, nblk(ncv), (int)kFinL1S, partials.p, scal.p);
   }
+  if (par.sst) {  // kine's and omega's residuals over the mean-flow relaxation (:1132-1134)
+    hipLaunchKernelGGL(k_l1s, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, rhs_s.p, par.under_relax, partials.p);
+    hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, nblk(ncv), (int)kFinL1S, partials.p, scal.p);
+    hipLaunchKernelGGL(k_l1s, dim3(nblk(ncv)), dim3(kBlock), 0, stream, ncv, rhs_s.p + ncv, par.under_relax,
+                       partials.p);
+    hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, stream, nblk(ncv), (int)kFinL1S2, partials.p, scal.p);
+  }
   HIPCHECK(hipMemcpyAsync(h_scal, scal.p, sizeof(SolverScalars), hipMemcpyDeviceToHost, stream));
   HIPCHECK(hipStreamSynchronize(stream));
   info.step = step_no;
@@ -6453,8 +7241,9 @@ void Ctx::Impl::finish_step(StepInfo &info, int it, int st) {
   info.dt_min = h_scal->dt_min;
   for (int k = 0; k < 5; k++) info.resid[k] = h_scal->resid[k];
   for (int k = 0; k < 5; k++) info.resid_l2[k] = h_scal->resid_l2[k];
-  info.resid_s = par.sa ? h_scal->resid_s : 0.0;
-  info.scal_iters = (par.sa && !coupled_sa()) ? scal_iters : 0;
+  info.resid_s = (par.sa || par.sst) ? h_scal->resid_s : 0.0;
+  info.resid_s2 = par.sst ? h_scal->resid_s2 : 0.0;
+  info.scal_iters = (par.sa && !coupled_sa()) ? scal_iters : (par.sst ? scal_iters + scal_iters2 : 0);
 }
 
 void Ctx::step_begin(bool host_source) {
@@ -6720,14 +7509,63 @@ int64_t Ctx::get(const std::string &name, double *out) {
   if (name == "P_BFA") return bfarec(4, 1);
   if (name == "T_BFA") return bfarec(5, 1);
   if (name == "H_BFA") return bfarec(6, 1);
-  if (name == "RESID") {  // L1 residuals of the last step; with SA the nuSA one sixth
-    const int nr = d.par.sa ? 6 : 5;
+  if (name == "RESID") {  // L1 residuals of the last step; with SA nuSA's sixth, with SST kine's and omega's
+    const int nr = d.par.sa ? 6 : (d.par.sst ? 7 : 5);
     if (out) {
       HIPCHECK(hipMemcpy(d.h_scal, d.scal.p, sizeof(SolverScalars), hipMemcpyDeviceToHost));
       for (int k = 0; k < 5; k++) out[k] = d.h_scal->resid[k];
-      if (d.par.sa) out[5] = d.h_scal->resid_s;
+      if (d.par.sa || d.par.sst) out[5] = d.h_scal->resid_s;
+      if (d.par.sst) out[6] = d.h_scal->resid_s2;
     }
     return nr;
+  }
+  if (d.par.sst) {
+    auto sbfa = [&](int s) -> int64_t {  // boundary values keep the reference face index
+      if (out) {
+        copy(d.sst_bfa.p, 2 * nb * sizeof(double));
+        for (size_t f = 0; f < nb; f++) out[f] = h[2 * f + s];
+      }
+      return (int64_t)nb;
+    };
+    if (name == "KINE") return cellrec(d.sst_phi.p, 1, 0, 1);
+    if (name == "OMEGA") return cellrec(d.sst_phi.p + n, 1, 0, 1);
+    if (name == "KINE_BFA") return sbfa(0);
+    if (name == "OMEGA_BFA") return sbfa(1);
+    if (name == "GRAD_KINE") return cellrec(d.sst_aux.p, kSstN, kSGradK, 3);
+    if (name == "GRAD_OMEGA") return cellrec(d.sst_aux.p, kSstN, kSGradW, 3);
+    if (name == "WALL_DIST") return cellrec(d.sst_aux.p, kSstN, kSWall, 1);
+    if (name == "STR_MAG") return cellrec(d.sst_aux.p, kSstN, kSStr, 1);
+    if (name == "VORT_MAG") return cellrec(d.sst_aux.p, kSstN, kSVort, 1);
+    if (name == "DIVERG") return cellrec(d.sst_aux.p, kSstN, kSDiv, 1);
+    if (name == "CROSS_DIFF") return cellrec(d.sst_aux.p, kSstN, kSCd, 1);
+    if (name == "F1") return cellrec(d.sst_aux.p, kSstN, kSF1, 1);
+    if (name == "F2") return cellrec(d.sst_aux.p, kSstN, kSF2, 1);
+    if (name == "RHS_KINE") return cellrec(d.rhs_s.p, 1, 0, 1);
+    if (name == "RHS_OMEGA") return cellrec(d.rhs_s.p + n, 1, 0, 1);
+    if (name == "DSCAL") {  // [2][ncv]: the last step's kine and omega corrections
+      if (out) {
+        copy(d.sst_dscal.p, 2 * n * sizeof(double));
+        for (int s = 0; s < 2; s++)
+          for (size_t cd = 0; cd < n; cd++) out[(size_t)s * n + perm[cd]] = h[(size_t)s * n + cd];
+      }
+      return (int64_t)(2 * n);
+    }
+    if (name == "ASCAL") {  // [2][6][nnz] in the reference's nbocv order, runBackwardEuler's AScal
+      const size_t nnz = d.nnz;
+      if (out) {
+        copy(d.AS.p, 12 * nnz * sizeof(double));
+        for (int s = 0; s < 2; s++)
+          for (size_t cd = 0; cd < n; cd++) {
+            const size_t c = perm[cd];
+            const size_t len = d.h_dev_nbocv_i[cd + 1] - d.h_dev_nbocv_i[cd];
+            for (size_t j = 0; j < len; j++)
+              for (int k = 0; k < 6; k++)
+                out[(6 * (size_t)s + k) * nnz + d.h_ref_nbocv_i[c] + j] =
+                    h[6 * nnz * s + 6 * (d.h_dev_nbocv_i[cd] + j) + k];
+          }
+      }
+      return (int64_t)(12 * nnz);
+    }
   }
   if (d.par.sa) {
     if (name == "NUSA") return cellrec(d.sa_phi.p, 1, 0, 1);
@@ -6786,7 +7624,7 @@ int64_t Ctx::get(const std::string &name, double *out) {
     const size_t nf = d.nfa;
     if (!out) return (int64_t)nf;
     if (!d.have_state) throw std::logic_error("no state set");
-    if (!d.dp.visc || (name == "MUT_FA" && !d.dp.sa)) {
+    if (!d.dp.visc || (name == "MUT_FA" && !d.dp.sa && !d.dp.sst)) {
       for (size_t f = 0; f < nf; f++) out[f] = 0.0;
       return (int64_t)nf;
     }

@@ -36,7 +36,7 @@ class SolveReport(C.Structure):
 class StepInfo(C.Structure):
     _fields_ = [("step", C.c_int), ("lin_iters", C.c_int), ("lin_status", C.c_int), ("cfl", C.c_double),
                 ("dt_min", C.c_double), ("resid", C.c_double * 5), ("resid_scal", C.c_double),
-                ("scal_iters", C.c_int), ("resid_l2", C.c_double * 5)]
+                ("scal_iters", C.c_int), ("resid_l2", C.c_double * 5), ("resid_scal2", C.c_double)]
 
 
 def lib():

@@ -132,13 +132,43 @@ def _sa_box_joe():
     (lambda s: s.replace("wall = WALL", "wall = SYMMETRY"), 3, "WALL"),              # no wall: no wall distance
     (lambda s: s.replace("SPACE_INTEGRATION = HLLC", "SPACE_INTEGRATION = ROE"), 5, "HLLC"),
     (lambda s: s.replace("inlet.nuSA = 0.006", "inlet.nuSA = HOOK"), 5, "inlet.nuSA"),
-    (lambda s: s.replace("TURB_MODEL = SA", "TURB_MODEL = SST"), 5, "TURB_MODEL"),
+    (lambda s: s.replace("TURB_MODEL = SA", "TURB_MODEL = KOMEGA"), 5, "TURB_MODEL"),
 ])
 def test_sa_input_checked_before_device(edit, status, needle):
     # TURB_MODEL = SA (the reference's RansTurbSA mix-in) is parsed before any
     # device work: what it needs and what this build does not provide fail
     # with the parameter and unsupported statuses
     mm, joe = _sa_box_joe()
+    mesh = joemi.Mesh.from_faces(mm)
+    with pytest.raises(joemi.JoemiError) as e:
+        joemi.Solver(mesh, edit(joe))
+    assert e.value.status == status, str(e.value)
+    assert needle in str(e.value)
+
+
+def _sst_box_joe():
+    mm, joe = cases.box_case(2, seed=0)
+    joe = joe.replace("MU_REF = 0.0", "MU_REF = 0.002")
+    return mm, cases.with_sst(joe, 0.001, 5.0, cases.SST_BOX)
+
+
+@pytest.mark.parametrize("edit,status,needle", [
+    (lambda s: s.replace("MU_REF = 0.002", "MU_REF = 0.0"), 3, "MU_REF"),             # SST needs viscosity
+    (lambda s: s.replace("INITIAL_CONDITION_TURB = 0.001 5.0", "INITIAL_CONDITION_TURB = 0.001"), 3,
+     "INITIAL_CONDITION_TURB"),                                                      # kine and omega
+    (lambda s: s.replace("wall = WALL", "wall = SYMMETRY").replace("wall.omega = HOOK", "wall.omega = 1.0"), 3,
+     "WALL"),                                                                        # no wall: no wall distance
+    (lambda s: s.replace("SPACE_INTEGRATION = HLLC", "SPACE_INTEGRATION = ROE"), 5, "HLLC"),
+    (lambda s: s.replace("inlet.kine = 0.001", "inlet.kine = HOOK"), 5, "inlet.kine"),  # the hook sets omega only
+    (lambda s: s.replace("inlet.omega = 5.0", "inlet.omega = HOOK"), 5, "inlet.omega"),  # on walls only
+    (lambda s: s + "sigma_k1 = 0.9\n", 5, "sigma_k1"),                                # the model's defaults only
+    (lambda s: cases.coupled(s), 5, "coupled"),
+    (lambda s: s + "GPU_JAC_KERNEL = CELL\n", 5, "per-entry Jacobian"),
+])
+def test_sst_input_checked_before_device(edit, status, needle):
+    # TURB_MODEL = SST (the reference's RansTurbKOmSST mix-in) is parsed, and
+    # the mesh checked for the kernel forms it runs, before any device work
+    mm, joe = _sst_box_joe()
     mesh = joemi.Mesh.from_faces(mm)
     with pytest.raises(joemi.JoemiError) as e:
         joemi.Solver(mesh, edit(joe))

@@ -16,6 +16,10 @@ from tests.test_oracle_golden import FIELDS, RHS_FIXTURES, joe_text, load
 # the Spalart-Allmaras fields the device exposes; the eddy viscosity is
 # formed per face inside the flux kernel, and MUT_FA reads back what it used
 SA_GPU_FIELDS = ["NUSA_BFA", "GRAD_NUSA", "WALL_DIST", "STR_MAG", "VORT_MAG", "RHS_NUSA", "ASCAL", "MUT_FA"]
+# the SST fields the device exposes (the diffusivities are formed per face
+# inside the scalar kernels, as the eddy viscosity inside the flux kernel)
+SST_GPU_FIELDS = ["KINE_BFA", "OMEGA_BFA", "GRAD_KINE", "GRAD_OMEGA", "WALL_DIST", "STR_MAG", "VORT_MAG", "DIVERG",
+                  "CROSS_DIFF", "F1", "F2", "RHS_KINE", "RHS_OMEGA", "ASCAL", "MUT_FA"]
 from tests.test_mesh_golden import MESHES
 
 pytestmark = pytest.mark.gpu
@@ -28,12 +32,16 @@ def test_residual_matches_reference(fixture):
     mesh = joemi.Mesh.from_faces(MESHES[fixture.split("_")[0]]())
     s = joemi.Solver(mesh, joe_text(g))
     s.set_state(g["RHO"], g["RHOU"], g["RHOE"])
-    sa = "_sa" in fixture
+    sa, sst = "_sa" in fixture, "_sst" in fixture
     if sa:
         s.set_scalar(g["NUSA"])
+    if sst:
+        s.set_scalar(g["KINE"], "kine")
+        s.set_scalar(g["OMEGA"], "omega")
     s.calc_rhs(True)
     exact = total = 0
-    for k in FIELDS + (["GRAD_ENTHALPY", "MUL_FA"] if "visc" in fixture else []) + (SA_GPU_FIELDS if sa else []):
+    for k in (FIELDS + (["GRAD_ENTHALPY", "MUL_FA"] if "visc" in fixture else []) + (SA_GPU_FIELDS if sa else [])
+              + (SST_GPU_FIELDS if sst else [])):
         if k not in g:
             continue
         e, n = compare(s.get(k), g[k], 1e-12, f"{fixture}:{k}")
@@ -46,9 +54,12 @@ def test_residual_matches_reference(fixture):
     if sa:  # the scalar half of the step (Jacobi BiCGStab, block-reduced dots)
         assert info.scal_iters >= 1
         compare(s.get("DSCAL"), g["DSCAL"], 1e-9, f"{fixture}:DSCAL")
+    if sst:  # kine's and omega's solves
+        assert info.scal_iters >= 2
+        compare(s.get("DSCAL"), g["DSCAL"], 1e-9, f"{fixture}:DSCAL")
 
 
-@pytest.mark.parametrize("name", ["bump24", "bump96", "bump24visc", "bump24sa"])
+@pytest.mark.parametrize("name", ["bump24", "bump96", "bump24visc", "bump24sa", "bump24sst"])
 def test_run_matches_reference(name):
     print(f"golden solve {name}", flush=True)
     g = load("solve_" + name)
@@ -64,6 +75,9 @@ def test_run_matches_reference(name):
         compare(rhoE, g[f"RHOE@{k}"], 1e-9, f"{name} rhoE@{k}")
         if name.endswith("sa"):
             compare(s.get("NUSA"), g[f"NUSA@{k}"], 1e-9, f"{name} nuSA@{k}")
+        if name.endswith("sst"):
+            compare(s.get("KINE"), g[f"KINE@{k}"], 1e-9, f"{name} kine@{k}")
+            compare(s.get("OMEGA"), g[f"OMEGA@{k}"], 1e-9, f"{name} omega@{k}")
 
 
 def test_device_numbering_is_invisible():
