@@ -220,7 +220,7 @@ def plate_case(nx_up: int = 40, nx: int = 324, ny: int = 110, nz: int = 50, re: 
                ratio: float = 1.1, nsteps: int = 20000, cfl_max: float = 1.0e3, resid_th: float = 0.0,
                ls: str = "MAX_ITER=30 ABS_RESID=1.0e-8 REL_RESID=1.0e-2",
                ratio_x: float = 1.05, cfl_ramp: str = "AFTER_ITER=10 INTERVAL_ITER=5 FACTOR_CFL=1.2",
-               restart: str = "plate.msh"):
+               restart: str = "plate.msh", turb: str = "SA"):
     """Implicit RANS on a zero-pressure-gradient flat plate: Mach 0.2,
     p = rho = 1 (R = 1, so T = 1), Re per unit length from a constant
     viscosity MU_REF = U / Re (power law with T = T_REF at the free stream),
@@ -229,7 +229,10 @@ def plate_case(nx_up: int = 40, nx: int = 324, ny: int = 110, nz: int = 50, re: 
     Characteristic inlet and top, static-pressure outlet, symmetry ahead of
     the plate and in z. The defaults give the 2M-cell plate (40 + 324 cells
     along x, 110 across the boundary layer with the first cell at y+ < 1, 50
-    in z)."""
+    in z). With turb = "SST" the Menter SST model in the segregated loop
+    (runBackwardEuler): free-stream kine from 1 % turbulence intensity,
+    omega from an eddy viscosity equal to the laminar one, kine = 0 and the
+    model's omega hook on the plate."""
     mesh = meshgen.flat_plate(nx_up, nx, ny, nz, ratio=ratio, ratio_x=ratio_x)
     u = MACH_PLATE * math.sqrt(GAMMA)
     mu = u / re
@@ -253,6 +256,17 @@ def plate_case(nx_up: int = 40, nx: int = 324, ny: int = 110, nz: int = 50, re: 
         "zsym = SYMMETRY",
         f"inlet.nuSA = {nu_sa!r}", f"top.nuSA = {nu_sa!r}", "plate.nuSA = 0.0",
     ]
+    if turb == "SST":
+        k = 1.5 * (0.01 * u) ** 2
+        w = k / mu
+        lines = [ln for ln in lines if ".nuSA" not in ln and not ln.startswith(("TURB_MODEL", "INITIAL_CONDITION"))]
+        lines = [ln.replace("BACKWARD_EULER_COUPLED", "BACKWARD_EULER").replace("PETSC_GMRES", "BCGSTAB")
+                 for ln in lines]
+        lines += ["TURB_MODEL = SST", f"INITIAL_CONDITION_TURB = {k!r} {w!r}",
+                  f"inlet.kine = {k!r}", f"inlet.omega = {w!r}", f"top.kine = {k!r}", f"top.omega = {w!r}",
+                  "plate.kine = 0.0", "plate.omega = HOOK"]
+    elif turb != "SA":
+        raise ValueError(turb)
     return mesh, "\n".join(lines) + "\n"
 
 
