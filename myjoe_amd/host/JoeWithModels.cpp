@@ -40,7 +40,10 @@ bool JoeWithModels::checkDataFlag(const void *field) const {
   if (field == rhou) return read_rhou_;
   if (field == rhoE) return read_rhoE_;
   const RansTurbSA *sa = dynamic_cast<const RansTurbSA *>(this);
-  return sa && field == sa->nuSA && read_nuSA_;
+  if (sa && field == sa->nuSA) return read_nuSA_;
+  const RansTurbKOmSST *sst = dynamic_cast<const RansTurbKOmSST *>(this);
+  if (sst && field == kine) return read_kine_;
+  return sst && field == sst->omega && read_omega_;
 }
 
 void JoeWithModels::writeRestart(const std::string &filename) {
@@ -51,6 +54,15 @@ void JoeWithModels::writeRestart(const std::string &filename) {
   if (sa && sa->nuSA) {
     names.push_back("nuSA");
     data.push_back(sa->nuSA);
+    dims.push_back(1);
+  }
+  const RansTurbKOmSST *sst = dynamic_cast<const RansTurbKOmSST *>(this);
+  if (sst && kine && sst->omega) {
+    names.push_back("kine");
+    data.push_back(kine);
+    dims.push_back(1);
+    names.push_back("omega");
+    data.push_back(sst->omega);
     dims.push_back(1);
   }
   require(jm_write_restart(mesh_, filename.c_str(), (int)names.size(), names.data(), data.data(), dims.data(), step,
@@ -65,6 +77,7 @@ void JoeWithModels::ensureContext() {
   if (ctx_) return;
   std::string text = joe_in_;
   if (dynamic_cast<RansTurbSA *>(this) && !checkParam("TURB_MODEL")) text += "\nTURB_MODEL = SA\n";
+  if (dynamic_cast<RansTurbKOmSST *>(this) && !checkParam("TURB_MODEL")) text += "\nTURB_MODEL = SST\n";
   const std::string prec = getStringParam("IMPLICIT_PRECISION", "FP64");
   require(jm_ctx_create(mesh_, text.c_str(), prec == "MIXED" ? JM_MIXED : JM_FP64, nullptr, &ctx_), "jm_ctx_create");
 }
@@ -161,6 +174,19 @@ void JoeWithModels::syncFieldsFromDevice() {
     for (int icv = 0; icv < ncv; icv++)  // calcRansTurbViscMuet's muT (TurbModel_SA.h:126)
       sa->muT[icv] = InterpolateAtCellCenterFromFaceValues(sa->mut_fa_v_.data(), icv);
   }
+  if (RansTurbKOmSST *sst = dynamic_cast<RansTurbKOmSST *>(this)) {
+    n = ncv;
+    require(jm_get(ctx_, "KINE", kine, &n), "jm_get KINE");
+    require(jm_get(ctx_, "OMEGA", sst->omega, &n), "jm_get OMEGA");
+    require(jm_get(ctx_, "WALL_DIST", sst->wallDist, &n), "jm_get WALL_DIST");
+    require(jm_get(ctx_, "CROSS_DIFF", sst->crossDiff, &n), "jm_get CROSS_DIFF");
+    require(jm_get(ctx_, "F1", sst->blendFuncF1, &n), "jm_get F1");
+    require(jm_get(ctx_, "F2", sst->blendFuncF2, &n), "jm_get F2");
+    n = nfa;
+    require(jm_get(ctx_, "MUT_FA", sst->mut_fa_v_.data(), &n), "jm_get MUT_FA");
+    for (int icv = 0; icv < ncv; icv++)  // calcRansTurbViscMuet's muT (TurbModel_KOMSST.h:152)
+      sst->muT[icv] = InterpolateAtCellCenterFromFaceValues(sst->mut_fa_v_.data(), icv);
+  }
 }
 
 // setBC's HOOK branch (JoeWithModels.cpp:3870-3881) for every HOOK zone, then
@@ -237,9 +263,56 @@ void JoeWithModels::run() {
       scalarTranspEqVector.back().phi = sa->nuSA;
     }
   }
+  RansTurbKOmSST *sst = dynamic_cast<RansTurbKOmSST *>(this);
+  if (sst) {  // initialHook's uniform kine and omega unless the restart file held them (JoeWithModels.cpp:219-240)
+    Param *ic = nullptr;
+    if (!getParam(ic, "INITIAL_CONDITION_TURB") || ic->getSize() < 3)
+      throw std::runtime_error("INITIAL_CONDITION_TURB needs kine and omega");
+    sst->kine_v_.assign(ncv, ic->getDouble(1));
+    sst->omega_v_.assign(ncv, ic->getDouble(2));
+    int64_t cnt = 0;
+    if (jm_mesh_restart_field(mesh_, "kine", nullptr, &cnt) == JM_OK && cnt == ncv) {
+      require(jm_mesh_restart_field(mesh_, "kine", sst->kine_v_.data(), &cnt), "kine");
+      read_kine_ = true;
+    }
+    cnt = 0;
+    if (jm_mesh_restart_field(mesh_, "omega", nullptr, &cnt) == JM_OK && cnt == ncv) {
+      require(jm_mesh_restart_field(mesh_, "omega", sst->omega_v_.data(), &cnt), "omega");
+      read_omega_ = true;
+    }
+    kine = sst->kine_v_.data();
+    sst->omega = sst->omega_v_.data();
+    for (auto *v : {&sst->muT_v_, &sst->wallDist_v_, &sst->crossDiff_v_, &sst->f1_v_, &sst->f2_v_}) v->assign(ncv, 0.0);
+    sst->mut_fa_v_.assign(nfa, 0.0);
+    sst->muT = sst->muT_v_.data();
+    sst->wallDist = sst->wallDist_v_.data();
+    sst->crossDiff = sst->crossDiff_v_.data();
+    sst->blendFuncF1 = sst->f1_v_.data();
+    sst->blendFuncF2 = sst->f2_v_.data();
+    // the model's registered fields and its transported scalars (TurbModel_KOMSST.h:36-68)
+    registerScalar(kine, "kine");
+    registerScalar(sst->omega, "omega");
+    registerScalar(sst->muT, "muT");
+    registerScalar(sst->wallDist, "wallDist");
+    registerScalar(sst->crossDiff, "crossDiff");
+    registerScalar(sst->blendFuncF1, "blendFuncF1");
+    registerScalar(sst->blendFuncF2, "blendFuncF2");
+    if (!getScalarTransportData("kine")) {
+      scalarTranspEqVector.emplace_back("kine");
+      scalarTranspEqVector.back().phi = kine;
+    }
+    if (!getScalarTransportData("omega")) {
+      scalarTranspEqVector.emplace_back("omega");
+      scalarTranspEqVector.back().phi = sst->omega;
+    }
+  }
   initialHook();
   require(jm_set_state(ctx_, rho, &rhou[0][0], rhoE), "jm_set_state");
   if (sa) require(jm_set_scalar(ctx_, "nuSA", sa->nuSA), "jm_set_scalar");
+  if (sst) {
+    require(jm_set_scalar(ctx_, "kine", kine), "jm_set_scalar kine");
+    require(jm_set_scalar(ctx_, "omega", sst->omega), "jm_set_scalar omega");
+  }
   // the STEP of a restart file carries on (runBackwardEuler does not reset
   // it) unless RESET_STEP (JoeWithModels.cpp:12-17)
   step = checkParam("RESET_STEP") ? 0 : restart_step_;
@@ -298,6 +371,11 @@ void JoeWithModels::run() {
       if (sa) {
         int64_t n = ncv;
         require(jm_get(ctx_, "NUSA", sa->nuSA_v_.data(), &n), "jm_get NUSA");
+      }
+      if (sst) {
+        int64_t n = ncv;
+        require(jm_get(ctx_, "KINE", kine, &n), "jm_get KINE");
+        require(jm_get(ctx_, "OMEGA", sst->omega, &n), "jm_get OMEGA");
       }
       char name[32];
       std::snprintf(name, sizeof name, "restart.%06d.out", step);

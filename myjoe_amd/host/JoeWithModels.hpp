@@ -67,6 +67,23 @@ class RansTurbSA : virtual public UgpWithCvCompFlow {
   std::vector<double> nuSA_v_, muT_v_, wallDist_v_, mut_fa_v_;
 };
 
+// the Menter SST k-omega mix-in (TurbModel_KOMSST.h:8-75): a case class
+// deriving from it (joe.cpp's MyJoeSST) solves kine (the base's kine) and
+// omega; the model's fields are host copies like RansTurbSA's
+class RansTurbKOmSST : virtual public UgpWithCvCompFlow {
+ public:
+  RansTurbKOmSST() {}
+  virtual ~RansTurbKOmSST() {}
+  double *omega = nullptr;
+  double *muT = nullptr;
+  double *wallDist = nullptr;
+  double *crossDiff = nullptr, *blendFuncF1 = nullptr, *blendFuncF2 = nullptr;
+
+ private:
+  friend class JoeWithModels;
+  std::vector<double> kine_v_, omega_v_, muT_v_, wallDist_v_, crossDiff_v_, f1_v_, f2_v_, mut_fa_v_;
+};
+
 class JoeWithModels : virtual public UgpWithCvCompFlow {
  public:
   // inputFile: a Joe.in (JoeWithModels.h:18-21); read here unless the
@@ -119,5 +136,6 @@ class JoeWithModels : virtual public UgpWithCvCompFlow {
   std::vector<double> t_bfa_, vel_bfa_, p_bfa_;
   // read from a restart file: the fields it held, its STEP
   bool read_rho_ = false, read_rhou_ = false, read_rhoE_ = false, read_nuSA_ = false;
+  bool read_kine_ = false, read_omega_ = false;
   int restart_step_ = 0;
 };

@@ -224,6 +224,7 @@ class UgpWithCvCompFlow {
   double *mul_fa = nullptr;                       // laminar viscosity per face (zero when inviscid)
   double (*grad_u)[3][3] = nullptr;               // velocity gradient of the last residual
   double (*rij_diag)[3] = nullptr, (*rij_offdiag)[3] = nullptr;  // Reynolds stresses (no model here: zero)
+  double *kine = nullptr;  // turbulent kinetic energy of a model with one (SST; UgpWithCvCompFlow.h:656)
   std::vector<ScalarTranspEq> scalarTranspEqVector;
 
   double GAMMA = 1.4, R_gas = 1.0, rho_ref = 1.0, p_ref = 1.0, T_ref = 1.0, Pr = 0.72, PrTurb = 0.9;

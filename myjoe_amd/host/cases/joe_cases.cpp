@@ -10,6 +10,8 @@
 //     perturb   initialHook adds a smooth density perturbation
 //     sa        the Spalart-Allmaras mix-in (MyJoeSA, joe.cpp:286-296); nuSA
 //               is appended to out.bin
+//     sst       the Menter SST mix-in (MyJoeSST, joe.cpp:301-310); kine and
+//               omega are appended to out.bin
 //     walls     constructed as joe.cpp's MyJoe is (naming the virtual base
 //               UgpWithCvCompFlow), it reads the host copies in finalHook to
 //               form each WALL face's shear, y+, heat flux and viscosity
@@ -134,9 +136,14 @@ class SaCase : public Recorder, public RansTurbSA {
   explicit SaCase(const char *f) : Recorder(f) {}
 };
 
+class SstCase : public Recorder, public RansTurbKOmSST {
+ public:
+  explicit SstCase(const char *f) : Recorder(f) {}
+};
+
 int main(int argc, char **argv) {
   if (argc != 4) {
-    std::cerr << "usage: joe_cases <plain|inlethook|gravity|perturb|sa|walls> <Joe.in> <out.bin>\n";
+    std::cerr << "usage: joe_cases <plain|inlethook|gravity|perturb|sa|sst|walls> <Joe.in> <out.bin>\n";
     return 2;
   }
   try {
@@ -183,6 +190,16 @@ int main(int argc, char **argv) {
       std::fwrite(j.nuSA, sizeof(double), j.ncv, fp);
       std::fclose(fp);
       std::printf("sa steps=%d\n", j.step);
+    } else if (c == "sst") {
+      SstCase j(argv[2]);
+      j.run();
+      j.write(argv[3]);
+      std::FILE *fp = std::fopen(argv[3], "ab");
+      if (!fp) throw std::runtime_error("cannot write output");
+      std::fwrite(j.kine, sizeof(double), j.ncv, fp);
+      std::fwrite(j.omega, sizeof(double), j.ncv, fp);
+      std::fclose(fp);
+      std::printf("sst steps=%d\n", j.step);
     } else {
       std::cerr << "unknown case " << c << "\n";
       return 2;

@@ -1,8 +1,9 @@
 """The reference's own case classes against the C++ mirror.
 
-joe.cpp's MyJoe and MyJoeSA (src/joe/joe.cpp:24-296: the vanilla case with
-its writeWallValues / writeAllValues / transformMeshHook / temporalHook /
-finalHook, and the Spalart-Allmaras mix-in) are compiled and linked as they
+joe.cpp's MyJoe, MyJoeSA and MyJoeSST (src/joe/joe.cpp:24-310: the vanilla
+case with its writeWallValues / writeAllValues / transformMeshHook /
+temporalHook / finalHook, the Spalart-Allmaras and the Menter SST mix-ins)
+are compiled and linked as they
 stand against myjoe_amd/host/JoeWithModels.hpp, libjoemi_host.a and
 libjoemi.so: their constructors name the virtual base UgpWithCvCompFlow,
 and their hooks read mul_fa, Pr, faZoneList, getParam, the MPI calls,
@@ -28,8 +29,10 @@ JOE_CPP = "/root/reference/src/joe/joe.cpp"
 MAIN = r"""
 #include <cstdlib>
 int main(int argc, char **argv) {
-  JoeWithModels *joe = std::atoi(argv[1]) == 0 ? static_cast<JoeWithModels *>(new MyJoe(argv[2]))
-                                                : static_cast<JoeWithModels *>(new MyJoeSA(argv[2]));
+  const int run = std::atoi(argv[1]);
+  JoeWithModels *joe = run == 0   ? static_cast<JoeWithModels *>(new MyJoe(argv[2]))
+                       : run == 1 ? static_cast<JoeWithModels *>(new MyJoeSA(argv[2]))
+                                  : static_cast<JoeWithModels *>(new MyJoeSST(argv[2]));
   std::printf("ncv %d nfa %d zones %d Pr %g\n", joe->ncv, joe->nfa, (int)joe->faZoneList.size(), joe->Pr);
   Param *p;
   for (FaZone &z : joe->faZoneList)
@@ -48,12 +51,12 @@ int main(int argc, char **argv) {
 
 
 def case_classes():
-    """MyJoe and MyJoeSA as joe.cpp writes them: from the MyJoe class head to
-    the end of MyJoeSA's body."""
+    """MyJoe, MyJoeSA and MyJoeSST as joe.cpp writes them: from the MyJoe class
+    head to the end of MyJoeSST's body."""
     text = open(JOE_CPP).read()
     start = text.index("class MyJoe : public JoeWithModels")
-    sa = text.index("class MyJoeSA : public MyJoe, public RansTurbSA")
-    end = text.index("};", sa) + 2
+    sst = text.index("class MyJoeSST : public MyJoe, public RansTurbKOmSST")
+    end = text.index("};", sst) + 2
     return text[start:end]
 
 
@@ -77,7 +80,7 @@ def test_reference_case_classes_compile_unchanged(program):
     assert os.path.exists(program)
 
 
-@pytest.mark.parametrize("run", [0, 1])
+@pytest.mark.parametrize("run", [0, 1, 2])
 def test_reference_case_classes_construct_on_the_host(program, tmp_path, run):
     import torch
 
@@ -87,6 +90,8 @@ def test_reference_case_classes_construct_on_the_host(program, tmp_path, run):
     joe = joe.replace("MU_REF = 0.0", "MU_REF = 0.002\nMU_MODE = POWERLAW\nMU_POWER_LAW = 0.76\nPR = 0.7")
     if run == 1:
         joe = cases.with_sa(joe, 0.006, cases.SA_BUMP)
+    if run == 2:
+        joe = cases.with_sst(joe, 0.001, 5.0, cases.SST_BUMP)
     meshgen.write_msh(mm, os.path.join(tmp_path, "bump.msh"))
     (tmp_path / "Joe.in").write_text(joe)
     r = subprocess.run([str(program), str(run), "Joe.in"], cwd=tmp_path, capture_output=True, text=True, timeout=120)

@@ -189,6 +189,35 @@ def test_sa_mixin_case_matches_abi_run(tmp_path):
 
 
 @pytest.mark.gpu
+def test_sst_mixin_case_matches_abi_run(tmp_path):
+    # a case class deriving from RansTurbKOmSST (joe.cpp's MyJoeSST) solves
+    # kine and omega: the Joe.in carries no model keyword, the class does
+    print("host mirror: sst", flush=True)
+    joe = joe_text(**{"MU_REF = 0.0": "MU_REF = 0.002\nMU_MODE = POWERLAW\nMU_POWER_LAW = 0.76\nPR = 0.72"})
+    joe_sst = cases.with_sst(joe, 0.001, 5.0, cases.SST_BUMP)
+    joe_case = joe_sst.replace("TURB_MODEL = SST\n", "")
+    mm = write_case(tmp_path, joe_case)
+    out = os.path.join(tmp_path, "sst.bin")
+    r = subprocess.run([EXE, "sst", "Joe.in", out], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = np.fromfile(out, np.float64)
+    n = int(raw[0])
+    ncv = mm.ncv
+    hist = raw[1:1 + 5 * n].reshape(n, 5)
+    rest = raw[1 + 5 * n:]
+    rho, rhou, rhoE = rest[:ncv], rest[ncv:4 * ncv], rest[4 * ncv:5 * ncv]
+    kine, omega = rest[5 * ncv:6 * ncv], rest[6 * ncv:7 * ncv]
+    s = joemi.Solver(joemi.Mesh.from_faces(mm), joe_sst)
+    s.init_uniform()
+    n2, hist2 = s.run(NSTEPS)
+    assert n == n2 == NSTEPS
+    assert np.array_equal(hist, hist2)
+    r2, ru2, rE2 = s.state()
+    assert np.array_equal(rho, r2) and np.array_equal(rhou, ru2.reshape(-1)) and np.array_equal(rhoE, rE2)
+    assert np.array_equal(kine, s.get("KINE")) and np.array_equal(omega, s.get("OMEGA"))
+
+
+@pytest.mark.gpu
 def test_case_resumes_from_its_restart_file(tmp_path):
     # run() writes restart.out at its end (runBackwardEuler); a second run with
     # RESTART = restart.out reads mesh, state and STEP from it and carries on.
