@@ -148,12 +148,15 @@ jm_status jm_calc_rhs(jm_ctx *c, int flags);
  * NUSA NUSA_BFA GRAD_NUSA WALL_DIST STR_MAG VORT_MAG RHS_NUSA ASCAL
  * ([6][nbocv_s], runBackwardEuler's AScal[0]) DSCAL, and RESID has 6 entries
  * (in the coupled loop A is [nbocv_s][6][6] and DQ [ncv][6], and ASCAL and
- * DSCAL do not exist). Evaluated from the current state when asked (each
+ * DSCAL do not exist); with TURB_MODEL = SST (TurbModel_KOMSST.h) KINE OMEGA
+ * KINE_BFA OMEGA_BFA GRAD_KINE GRAD_OMEGA WALL_DIST STR_MAG VORT_MAG DIVERG
+ * CROSS_DIFF F1 F2 RHS_KINE RHS_OMEGA ASCAL ([2][6][nbocv_s]) DSCAL
+ * ([2][ncv]), and RESID has 7 entries. Evaluated from the current state when asked (each
  * runs the residual once more): FLUX_FA, each face's inviscid flux [nfa][5]
  * (the terms calcRhs accumulates, JoeWithModels.cpp:2786-3105); MUL_FA and
  * MUT_FA, each face's laminar and eddy viscosity [nfa] (mul_fa of
  * calcMaterialProperties, UgpWithCvCompFlow.h:1701-1753; mut_fa of
- * calcRansTurbViscMuet, TurbModel_SA.h:74-128), zero when the case is
+ * calcRansTurbViscMuet, TurbModel_SA.h:74-128, TurbModel_KOMSST.h:100-158), zero when the case is
  * inviscid or has no turbulence model. out may be NULL to query *count. */
 jm_status jm_get(jm_ctx *c, const char *name, double *out, int64_t *count);
 
