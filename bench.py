@@ -148,13 +148,15 @@ def cpu_baseline(n_box=48, reps=12):
     return out
 
 
-def plate_bench(dims=(40, 324, 110, 50), steps=3, reps=10, jac_kernel=None):
+def plate_bench(dims=(40, 324, 110, 50), steps=3, reps=10, jac_kernel=None, turb="SA"):
     """One implicit step of the 2M-cell flat plate (Mach 0.2, Re 5e6 per unit
-    length, Spalart-Allmaras in the coupled loop, 6x6 blocks), FP64 and the
-    mixed form: wall-clock ms per step from the free stream (host-polled
-    steps, BiCGStab iterations counted), and the residual, the Jacobian
-    assembly and one BiCGStab iteration timed alone with HIP events, each
-    against its compulsory bytes (b = 6, s = 8, s_J = s_v = 8 or 4):
+    length, Spalart-Allmaras in the coupled loop, 6x6 blocks; with turb =
+    "SST" the Menter SST model in the segregated loop, 5x5 blocks and the two
+    scalar solves), FP64 and the mixed form: wall-clock ms per step from the
+    free stream (host-polled steps, BiCGStab iterations counted), and the
+    residual (with SST its kine and omega rows), the Jacobian assembly and
+    one BiCGStab iteration timed alone with HIP events, each against its
+    compulsory bytes (b = 6 or 5, s = 8, s_J = s_v = 8 or 4):
       Jacobian  nnz b^2 s_J + ncv (b s + 8) + nfa_i (3 s + 24) + nfa_b (3 s + 8)
       iteration nnz (2 b^2 s_J + 8) + ncv (2 b^2 s_J + 24 b s_v + 8)
       residual  780 ncv + 32 nnz + 4 nfoc + 80 nfa_i + 160 nfa_b (the
@@ -162,7 +164,7 @@ def plate_bench(dims=(40, 324, 110, 50), steps=3, reps=10, jac_kernel=None):
     from myjoe_amd import cases, joemi
 
     t0 = time.perf_counter()
-    mm, joe = cases.plate_case(*dims)
+    mm, joe = cases.plate_case(*dims, turb=turb)
     if jac_kernel:
         joe += f"GPU_JAC_KERNEL = {jac_kernel}\n"
     mesh = joemi.Mesh.from_faces(mm)
@@ -172,11 +174,13 @@ def plate_bench(dims=(40, 324, 110, 50), steps=3, reps=10, jac_kernel=None):
     nfa_i = nfa - nfa_b
     print(f"# plate: ncv={ncv} nfa={nfa} nfa_b={nfa_b} nnz={nnz} setup {time.perf_counter() - t0:.1f}s",
           file=sys.stderr, flush=True)
-    b = 6
+    b = 6 if turb == "SA" else 5
     res_bytes = 780 * ncv + 32 * nnz + 4 * (2 * nfa_i + nfa_b) + 80 * nfa_i + 160 * nfa_b
     out = {"workload": f"flat plate {dims[0]}+{dims[1]} x {dims[2]} x {dims[3]} hexes (x ahead of / along the plate, "
-                       f"wall-normal at ratio 1.1 with y+ < 1, spanwise), Mach 0.2, Re 5e6 per unit length, SA in "
-                       f"the coupled loop (6x6 blocks), HLLC 2nd order",
+                       f"wall-normal at ratio 1.1 with y+ < 1, spanwise), Mach 0.2, Re 5e6 per unit length, "
+                       + ("SA in the coupled loop (6x6 blocks)" if turb == "SA" else
+                          "Menter SST in the segregated loop (5x5 blocks, kine and omega solved after the flow)")
+                       + ", HLLC 2nd order",
            "ncv": ncv, "nfa": nfa, "nfa_b": nfa_b, "nnz": nnz}
     for prec, name in ((joemi.JM_FP64, "fp64"), (joemi.JM_MIXED, "mixed")):
         sj = 8 if prec == joemi.JM_FP64 else 4
@@ -185,9 +189,11 @@ def plate_bench(dims=(40, 324, 110, 50), steps=3, reps=10, jac_kernel=None):
         s.step()
         s.sync()
         t1 = time.perf_counter()
-        iters = 0
+        iters = sits = 0
         for _ in range(steps):
-            iters += s.step().lin_iters
+            info = s.step()
+            iters += info.lin_iters
+            sits += info.scal_iters
         s.sync()
         step_ms = (time.perf_counter() - t1) / steps * 1e3
         km = s.time_step_kernels(reps)
@@ -195,13 +201,14 @@ def plate_bench(dims=(40, 324, 110, 50), steps=3, reps=10, jac_kernel=None):
         it_bytes = nnz * (2 * b * b * sj + 8) + ncv * (2 * b * b * sj + 24 * b * sj + 8)
         frac = lambda by, ms: round(by / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
         out[name] = {"step_ms": round(step_ms, 3), "mean_bicgstab_iterations": round(iters / steps, 1),
+                     **({"mean_scalar_bicgstab_iterations": round(sits / steps, 1)} if turb == "SST" else {}),
                      "kernel_ms": {k: round(v, 4) for k, v in km.items()},
                      "residual_frac_of_hbm_peak": frac(res_bytes, km["residual"]),
                      "jacobian_frac_of_hbm_peak": frac(jac_bytes, km["jacobian"]),
                      "bicgstab_iteration_frac_of_hbm_peak": frac(it_bytes, km["krylov_iteration"]),
                      "bytes": {"residual": res_bytes, "jacobian": jac_bytes, "bicgstab_iteration": it_bytes},
                      "mcell_it_s_residual": round(ncv / (km["residual"] * 1e-3) / 1e6, 1)}
-        print(f"# plate {name}: step {step_ms:.2f} ms ({iters / steps:.1f} BiCGStab its); "
+        print(f"# plate {turb} {name}: step {step_ms:.2f} ms ({iters / steps:.1f} BiCGStab its); "
               + " ".join(f"{k}={v:.3f}ms" for k, v in km.items()), file=sys.stderr, flush=True)
         del s
         gc.collect()
@@ -391,6 +398,8 @@ def main():
                     help="serving parts to run: plain (FP64 loop), streams (per-stream graph server), batched")
     ap.add_argument("--plate", type=int, default=1, help="time the 2M-cell flat-plate step (0: skip)")
     ap.add_argument("--plate-dims", default="40,324,110,50", help="cells ahead of / along the plate, wall-normal, z")
+    ap.add_argument("--plate-turb", default="SA,SST", help="turbulence models of the plate leg (SA coupled, SST "
+                                                          "segregated)")
     ap.add_argument("--copy-calib-gib", type=float, default=0.0,
                     help="first run a plain device copy of this many GiB (calibrates profiler byte counters)")
     args = ap.parse_args()
@@ -516,8 +525,10 @@ def main():
     del mesh
     gc.collect()
     if args.plate:
-        out["config"]["implicit_rans_flat_plate"] = plate_bench(tuple(int(x) for x in args.plate_dims.split(",")),
-                                                                jac_kernel=args.jac_kernel)
+        dims = tuple(int(x) for x in args.plate_dims.split(","))
+        for turb in args.plate_turb.split(","):
+            key = "implicit_rans_flat_plate" + ("" if turb == "SA" else "_" + turb.lower())
+            out["config"][key] = plate_bench(dims, jac_kernel=args.jac_kernel, turb=turb)
     if args.serve > 0:
         lo, hi = (int(x) for x in args.serve_cells.split(","))
         out["serving"] = serving_bench(args.serve, 7, lo, hi, args.serve_active, args.serve_steps, args.serve_rel, 64.0,

@@ -26,11 +26,12 @@ __device__ __forceinline__ double dot3(const double *a, const double *b) {
 
 // HLLC flux, UgpWithCvCompFlow.cpp:120-235 (surfVeloc = 0; kL, kR the sides'
 // turbulent kinetic energy, 0 without SST). With Fs, one transported scalar:
-// FrhoScal = Frho * (upwind side's value) * area.
+// FrhoScal = Frho * (upwind side's value) * area; with Fs2 a second one.
 __device__ __forceinline__ void flux_hllc(double F[5], double rhoL, const double *uL, double pL, double rhoR,
                                           const double *uR, double pR, double gam, double area,
                                           const double *nVec, double phiL = 0.0, double phiR = 0.0,
-                                          double *Fs = nullptr, double kL = 0.0, double kR = 0.0) {
+                                          double *Fs = nullptr, double kL = 0.0, double kR = 0.0,
+                                          double *Fs2 = nullptr, double phiL2 = 0.0, double phiR2 = 0.0) {
   const double gammaL = gam, gammaR = gam, surfv = 0.0;
   double unL = dot3(uL, nVec) - surfv;
   double unLr = unL - surfv;
@@ -107,6 +108,7 @@ __device__ __forceinline__ void flux_hllc(double F[5], double rhoL, const double
   F[3] = f3 * area;
   F[4] = f4 * area;
   if (Fs) *Fs = f0 * (sM >= 0.0 ? phiL : phiR) * area;
+  if (Fs2) *Fs2 = f0 * (sM >= 0.0 ? phiL2 : phiR2) * area;
 }
 
 // Roe flux without entropy fix, UgpWithCvCompFlow.cpp:3281-3380; returns the
@@ -454,6 +456,71 @@ __device__ __forceinline__ void jac_hllc_scal(double S[6], int which, double rho
     for (int i = 0; i < 5; i++) S[i] = J[0][i] * phi;
     S[5] = 0.0;
   }
+}
+
+// the same for two transported scalars at once (one Jacobian, two rows)
+__device__ __forceinline__ void jac_hllc_scal2(double S[6], double S2[6], int which, double rhoL,
+                                            const double *uL, double pL, double rhoR, const double *uR,
+                                            double pR, double gam, double area, const double *nVec,
+                                            double phiL, double phiR, double phiL2, double phiR2,
+                                            double kL, double kR) {
+  double J[5][5];
+  jac_hllc(J, which, rhoL, uL, pL, rhoR, uR, pR, gam, area, nVec, kL, kR);
+  // the wave speeds again, with the same operations as jac_hllc (they do not
+  // depend on the kinetic energy)
+  const double gammaL = gam, gammaR = gam;
+  double unL = dot3(uL, nVec);
+  double uLuL = dot3(uL, uL);
+  double cL = sqrt(gammaL * pL / rhoL);
+  double unR = dot3(uR, nVec);
+  double uRuR = dot3(uR, uR);
+  double cR = sqrt(gammaR * pR / rhoR);
+  double Rrho = sqrt(rhoR / rhoL);
+  double tmp = 1.0 / (1.0 + Rrho);
+  double velRoe[3];
+  for (int i = 0; i < 3; i++) velRoe[i] = tmp * (uL[i] + uR[i] * Rrho);
+  double uRoe = dot3(velRoe, nVec);
+  double gamPdivRho = tmp * ((gammaL * pL / rhoL + 0.5 * (gammaL - 1.0) * uLuL) +
+                             (gammaR * pR / rhoR + 0.5 * (gammaR - 1.0) * uRuR) * Rrho);
+  double cRoe = sqrt(gamPdivRho - ((gammaL + gammaR) * 0.5 - 1.0) * 0.5 * dot3(velRoe, velRoe));
+  double sL = Dev::mn(uRoe - cRoe, unL - cL);
+  double sR = Dev::mx(uRoe + cRoe, unR + cR);
+  double sM = (pL - pR - rhoL * unL * (sL - unL) + rhoR * unR * (sR - unR)) / (rhoR * (sR - unR) - rhoL * (sL - unL));
+  int k;
+  double fso;
+  if (sM >= 0.0) {
+    if (sL > 0.0) {
+      k = 0;
+      fso = unL * area;
+    } else {
+      k = 1;
+      fso = 1.0 / (sL - sM) * (sL - unL) * sM * area;
+    }
+  } else {
+    if (sR >= 0.0) {
+      k = 2;
+      fso = 1.0 / (sR - sM) * (sR - unR) * sM * area;
+    } else {
+      k = 3;
+      fso = unR * area;
+    }
+  }
+  const bool upL = k <= 1;
+  const double phi = upL ? phiL : phiR, phi2 = upL ? phiL2 : phiR2;
+  const bool zero = which == 0 ? (k == 3) : (k == 0);
+  const bool upwind = which == 0 ? upL : !upL;
+  // the three cases as jac_hllc_scal's, each row written once from selected
+  // values (both rows by branch would keep one of them in scratch memory)
+  const double j00 = upwind ? J[0][0] - fso : J[0][0];
+#pragma unroll
+  for (int i = 0; i < 5; i++) {
+    const double ji = i == 0 ? j00 : J[0][i];
+    S[i] = zero ? 0.0 : ji * phi;
+    S2[i] = zero ? 0.0 : ji * phi2;
+  }
+  const double d = (zero || !upwind) ? 0.0 : fso;
+  S[5] = d;
+  S2[5] = d;
 }
 
 // One side of the Roe Jacobian pair, UgpWithCvCompFlow.cpp:3382-3559

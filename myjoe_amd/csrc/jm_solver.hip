@@ -2554,36 +2554,37 @@ __global__ void __launch_bounds__(kBlock) k_scal(DMesh m, DParams P, const doubl
 }
 
 // ---------------------------------------------------------------------------
-// One SST scalar's equation of one cell (S = 0 kine, 1 omega), gathered in the
+// Both SST scalars' equations of one cell (kine, omega), gathered in the
 // reference's order as k_scal gathers nuSA's: the HLLC convective flux with
 // the sides' kinetic energy in the enthalpy (calcEulerFlux scalar terms,
-// JoeWithModels.cpp:2824-2930, 3014-3026, 3134-3198, 3408-3495) and its
-// AScal rows at cell-centre states (:3088-3101, 3270-3274, 3559-3561);
-// diffusion with the blended SST diffusivity (calcViscousFluxScalar_new,
-// Scalars.cpp:117-247; diffusivityHookScalarRansTurb, TurbModel_KOMSST.h
-// :160-254); the source and its diagonal (sourceHookScalarRansTurb_new,
-// :256-304). rhs_s and AS are this scalar's [ncv] and [nnz][6].
+// JoeWithModels.cpp:2824-2930, 3014-3026, 3134-3198, 3408-3495) and the AScal
+// rows at cell-centre states (:3088-3101, 3270-3274, 3559-3561), one face
+// evaluation serving both scalars; diffusion with the blended SST
+// diffusivities (calcViscousFluxScalar_new, Scalars.cpp:117-247;
+// diffusivityHookScalarRansTurb, TurbModel_KOMSST.h:160-254); the sources and
+// their diagonals (sourceHookScalarRansTurb_new, :256-304). rhs_s is
+// [2][ncv], AS [2][nnz][6].
 // ---------------------------------------------------------------------------
-template <int S>
 __global__ void __launch_bounds__(kBlock) k_sst_scal(DMesh m, DParams P, const double *__restrict__ prim,
                                                      const double *__restrict__ grad, const double *__restrict__ bfa,
                                                      double *__restrict__ rhs_s, double *__restrict__ AS) {
   const int c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (c >= m.ncv) return;
   const int ncv = m.ncv;
+  const size_t nnz = m.nnz;
   const double gam = P.gamma;
   const bool snd = P.second_order != 0;
   const double *pc = prim + 8 * (size_t)c;
   const double *gc = grad + 16 * (size_t)c;
-  const double *phi = P.sst_phi + (size_t)S * ncv, *kin = P.sst_phi;
+  const double *kin = P.sst_phi, *omg = P.sst_phi + ncv;
   const double *auxc = P.sst_aux + kSstN * (size_t)c;
-  constexpr int gS = S == 0 ? kSGradK : kSGradW;
-  const double ph = phi[c], kc = kin[c];
+  const double kc = kin[c], wc = omg[c];
+  double *AS1 = AS + 6 * nnz;
   Side own;
   own.rho = pc[0]; own.u[0] = pc[1]; own.u[1] = pc[2]; own.u[2] = pc[3]; own.p = pc[4];
-  double acc = 0.0;
-  double D[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  double Sr[6], F[5], Fs;
+  double acc0 = 0.0, acc1 = 0.0;
+  double D0[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, D1[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double S0[6], S1[6], F[5], Fs0, Fs1;
   const int e0 = m.cfl_i[c], e1 = m.cfl_i[c + 1];
   for (int e = e0; e < e1; e++) {
     const int2 en = m.cfl[e];
@@ -2596,71 +2597,88 @@ __global__ void __launch_bounds__(kBlock) k_sst_scal(DMesh m, DParams P, const d
       const int nb = en.y;
       const double *pn = prim + 8 * (size_t)nb;
       const double *auxn = P.sst_aux + kSstN * (size_t)nb;
-      const double phn = phi[nb], kn = kin[nb];
+      const double kn = kin[nb], wn = omg[nb];
       Side nbs;
       nbs.rho = pn[0]; nbs.u[0] = pn[1]; nbs.u[1] = pn[2]; nbs.u[2] = pn[3]; nbs.p = pn[4];
       Side so = own, sn = nbs;
-      double sco = ph, scn = phn, kfo = kc, kfn = kn;
+      double kfo = kc, kfn = kn, wfo = wc, wfn = wn;
       if (snd) {
         if (reconstruct(so, pc, gc, xfa)) {
           const double r[3] = {xfa[0] - pc[5], xfa[1] - pc[6], xfa[2] - pc[7]};
-          sco = sco + dot3(r, auxc + gS);
           kfo = kfo + dot3(r, auxc + kSGradK);
+          wfo = wfo + dot3(r, auxc + kSGradW);
         }
         if (reconstruct(sn, pn, grad + 16 * (size_t)nb, xfa)) {
           const double r[3] = {xfa[0] - pn[5], xfa[1] - pn[6], xfa[2] - pn[7]};
-          scn = scn + dot3(r, auxn + gS);
           kfn = kfn + dot3(r, auxn + kSGradK);
+          wfn = wfn + dot3(r, auxn + kSGradW);
         }
       }
       const bool s0 = en.x >= 0;
       const int noc = m.cfl_noc[e];
       const bool accum = (noc >> 30) & 1;
-      double *off = AS + 6 * (size_t)(noc & 0x3fffffff);
+      double *off0 = AS + 6 * (size_t)(noc & 0x3fffffff), *off1 = AS1 + 6 * (size_t)(noc & 0x3fffffff);
       const Side fl = s0 ? so : sn, fr = s0 ? sn : so;
       const Side jl = s0 ? own : nbs, jr = s0 ? nbs : own;
-      const double phl = s0 ? sco : scn, phr = s0 ? scn : sco, pjl = s0 ? ph : phn, pjr = s0 ? phn : ph;
       const double kl = s0 ? kfo : kfn, kr = s0 ? kfn : kfo, kjl = s0 ? kc : kn, kjr = s0 ? kn : kc;
-      flux_hllc(F, fl.rho, fl.u, fl.p, fr.rho, fr.u, fr.p, gam, area, n, phl, phr, &Fs, kl, kr);
-      acc += s0 ? -Fs : Fs;
-      jac_hllc_scal(Sr, 0, jl.rho, jl.u, jl.p, jr.rho, jr.u, jr.p, gam, area, n, pjl, pjr, kjl, kjr);
+      const double wl = s0 ? wfo : wfn, wr = s0 ? wfn : wfo, wjl = s0 ? wc : wn, wjr = s0 ? wn : wc;
+      flux_hllc(F, fl.rho, fl.u, fl.p, fr.rho, fr.u, fr.p, gam, area, n, kl, kr, &Fs0, kl, kr, &Fs1, wl, wr);
+      acc0 += s0 ? -Fs0 : Fs0;
+      acc1 += s0 ? -Fs1 : Fs1;
+      jac_hllc_scal2(S0, S1, 0, jl.rho, jl.u, jl.p, jr.rho, jr.u, jr.p, gam, area, n, kjl, kjr, wjl, wjr, kjl, kjr);
       if (s0) {
-        for (int k = 0; k < 6; k++) D[k] += Sr[k];
+        for (int k = 0; k < 6; k++) {
+          D0[k] += S0[k];
+          D1[k] += S1[k];
+        }
       } else {
-        for (int k = 0; k < 6; k++) off[k] = accum ? off[k] - Sr[k] : 0.0 - Sr[k];
+        for (int k = 0; k < 6; k++) {
+          off0[k] = accum ? off0[k] - S0[k] : 0.0 - S0[k];
+          off1[k] = accum ? off1[k] - S1[k] : 0.0 - S1[k];
+        }
       }
-      jac_hllc_scal(Sr, 1, jl.rho, jl.u, jl.p, jr.rho, jr.u, jr.p, gam, area, n, pjl, pjr, kjl, kjr);
+      jac_hllc_scal2(S0, S1, 1, jl.rho, jl.u, jl.p, jr.rho, jr.u, jr.p, gam, area, n, kjl, kjr, wjl, wjr, kjl, kjr);
       if (s0) {
-        for (int k = 0; k < 6; k++) off[k] = accum ? off[k] + Sr[k] : 0.0 + Sr[k];
+        for (int k = 0; k < 6; k++) {
+          off0[k] = accum ? off0[k] + S0[k] : 0.0 + S0[k];
+          off1[k] = accum ? off1[k] + S1[k] : 0.0 + S1[k];
+        }
       } else {
-        for (int k = 0; k < 6; k++) D[k] -= Sr[k];
+        for (int k = 0; k < 6; k++) {
+          D0[k] -= S0[k];
+          D1[k] -= S1[k];
+        }
       }
     } else {
       const ZoneDev &z = m.zones[-en.y - 1];
       const double *bb = bfa + 8 * (size_t)f;
-      const double phb = P.sst_bfa[2 * (size_t)f + S], kb = P.sst_bfa[2 * (size_t)f];
+      const double kb = P.sst_bfa[2 * (size_t)f], wb = P.sst_bfa[2 * (size_t)f + 1];
       const double ub[3] = {bb[1], bb[2], bb[3]};
       if (z.bc == kBcSymmetry || z.bc == kBcWall) {
-        flux_hllc(F, bb[0], ub, bb[4], bb[0], ub, bb[4], gam, area, n, phb, phb, &Fs, kb, kb);
-        acc += -Fs;
-        jac_hllc_scal(Sr, 0, bb[0], ub, bb[4], bb[0], ub, bb[4], gam, area, n, phb, phb, kb, kb);
+        flux_hllc(F, bb[0], ub, bb[4], bb[0], ub, bb[4], gam, area, n, kb, kb, &Fs0, kb, kb, &Fs1, wb, wb);
+        acc0 += -Fs0;
+        acc1 += -Fs1;
+        jac_hllc_scal2(S0, S1, 0, bb[0], ub, bb[4], bb[0], ub, bb[4], gam, area, n, kb, kb, wb, wb, kb, kb);
       } else {
         Side so = own;
-        double sco = ph, kfo = kc;
+        double kfo = kc, wfo = wc;
         if (snd && reconstruct(so, pc, gc, xfa)) {
           const double r[3] = {xfa[0] - pc[5], xfa[1] - pc[6], xfa[2] - pc[7]};
-          sco = sco + dot3(r, auxc + gS);
           kfo = kfo + dot3(r, auxc + kSGradK);
+          wfo = wfo + dot3(r, auxc + kSGradW);
         }
-        flux_hllc(F, so.rho, so.u, so.p, bb[0], ub, bb[4], gam, area, n, sco, phb, &Fs, kfo, kb);
-        acc += -Fs;
-        jac_hllc_scal(Sr, 0, own.rho, own.u, own.p, bb[0], ub, bb[4], gam, area, n, ph, phb, kc, kb);
+        flux_hllc(F, so.rho, so.u, so.p, bb[0], ub, bb[4], gam, area, n, kfo, kb, &Fs0, kfo, kb, &Fs1, wfo, wb);
+        acc0 += -Fs0;
+        acc1 += -Fs1;
+        jac_hllc_scal2(S0, S1, 0, own.rho, own.u, own.p, bb[0], ub, bb[4], gam, area, n, kc, kb, wc, wb, kc, kb);
       }
-      for (int k = 0; k < 6; k++) D[k] += Sr[k];
+      for (int k = 0; k < 6; k++) {
+        D0[k] += S0[k];
+        D1[k] += S1[k];
+      }
     }
   }
   // diffusion, after the convective pass (JoeWithModels.cpp:2595-2610)
-  constexpr double sg1 = S == 0 ? kSstSigmaK1 : kSstSigmaW1, sg2 = S == 0 ? kSstSigmaK2 : kSstSigmaW2;
   const double Tc = pc[4] / (pc[0] * P.R);
   for (int e = e0; e < e1; e++) {
     const int2 en = m.cfl[e];
@@ -2682,59 +2700,69 @@ __global__ void __launch_bounds__(kBlock) k_sst_scal(DMesh m, DParams P, const d
       const double *a0 = P.sst_aux + kSstN * (size_t)c0, *a1 = P.sst_aux + kSstN * (size_t)c1;
       const double mut = sst_mut_face(P, ncv, w0, w1, rho0, rho1, c0, c1);
       const double f1_fa = (w1 * a0[kSF1] + w0 * a1[kSF1]) / (w0 + w1);
-      const double coeff = sg1 * f1_fa + (1. - f1_fa) * sg2;
-      const double diff = mul + coeff * mut;
-      const double phi0 = phi[c0], phi1 = phi[c1];
-      const double *g0 = a0 + gS, *g1 = a1 + gS;
       const double alpha = dot3(n, sV);
-      const double viscFlux = diff * nmag *
-                              (alpha * (phi1 - phi0) / smag + 0.5 * (g0[0] + g1[0]) * (n[0] - alpha * sV[0]) +
-                               0.5 * (g0[1] + g1[1]) * (n[1] - alpha * sV[1]) +
-                               0.5 * (g0[2] + g1[2]) * (n[2] - alpha * sV[2]));
-      const double a = diff * nmag * alpha / smag;
-      double *off = AS + 6 * (size_t)(m.cfl_noc[e] & 0x3fffffff);
-      if (s0) {
-        acc += viscFlux;
-        D[5] += a / rho0;
-        off[5] = off[5] + -(a / rho1);
-      } else {
-        acc += -viscFlux;
-        D[5] += a / rho1;
-        off[5] = off[5] + -(a / rho0);
-      }
+      const int noc = m.cfl_noc[e] & 0x3fffffff;
+      // one scalar's face term, called with that scalar's own accumulators
+      auto face = [&](double sg1, double sg2, const double *phi, int gS, double &acc, double &d5, double *off) {
+        const double coeff = sg1 * f1_fa + (1. - f1_fa) * sg2;
+        const double diff = mul + coeff * mut;
+        const double phi0 = phi[c0], phi1 = phi[c1];
+        const double *g0 = a0 + gS, *g1 = a1 + gS;
+        const double viscFlux = diff * nmag *
+                                (alpha * (phi1 - phi0) / smag + 0.5 * (g0[0] + g1[0]) * (n[0] - alpha * sV[0]) +
+                                 0.5 * (g0[1] + g1[1]) * (n[1] - alpha * sV[1]) +
+                                 0.5 * (g0[2] + g1[2]) * (n[2] - alpha * sV[2]));
+        const double a = diff * nmag * alpha / smag;
+        if (s0) {
+          acc += viscFlux;
+          d5 += a / rho0;
+          off[5] = off[5] + -(a / rho1);
+        } else {
+          acc += -viscFlux;
+          d5 += a / rho1;
+          off[5] = off[5] + -(a / rho0);
+        }
+      };
+      face(kSstSigmaK1, kSstSigmaK2, kin, kSGradK, acc0, D0[5], AS + 6 * (size_t)noc);
+      face(kSstSigmaW1, kSstSigmaW2, omg, kSGradW, acc1, D1[5], AS1 + 6 * (size_t)noc);
     } else {
       const ZoneDev &z = m.zones[-en.y - 1];
-      if (!(S == 0 ? z.skind : z.skind2)) continue;  // no diffusive flux without a value or hook (Scalars.cpp:239-245)
+      if (!z.skind && !z.skind2) continue;  // no diffusive flux without a value or hook (Scalars.cpp:239-245)
       const double *bb = bfa + 8 * (size_t)f;
       const double mul = mu_lam(P, bb[5]);
-      double diff = mul;
-      if (z.bc != kBcWall) {
-        const double coeff = sg1 * auxc[kSF1] + (1. - auxc[kSF1]) * sg2;
-        diff = mul + coeff * sst_mut_cell(P, ncv, pc[0], c);
-      }
-      const double phb = P.sst_bfa[2 * (size_t)f + S];
+      const double mutb = z.bc != kBcWall ? sst_mut_cell(P, ncv, pc[0], c) : 0.0;
       const double s_half[3] = {xfa[0] - pc[5], xfa[1] - pc[6], xfa[2] - pc[7]};
       const double smag_half = fabs(dot3(s_half, n));
-      acc += diff * nmag * (phb - ph) / smag_half;
-      D[5] += diff * nmag / smag_half / pc[0];
+      auto bface = [&](double sg1, double sg2, double phb, double ph, double &acc, double &d5) {
+        double diff = mul;
+        if (z.bc != kBcWall) {
+          const double coeff = sg1 * auxc[kSF1] + (1. - auxc[kSF1]) * sg2;
+          diff = mul + coeff * mutb;
+        }
+        acc += diff * nmag * (phb - ph) / smag_half;
+        d5 += diff * nmag / smag_half / pc[0];
+      };
+      if (z.skind) bface(kSstSigmaK1, kSstSigmaK2, P.sst_bfa[2 * (size_t)f], kc, acc0, D0[5]);
+      if (z.skind2) bface(kSstSigmaW1, kSstSigmaW2, P.sst_bfa[2 * (size_t)f + 1], wc, acc1, D1[5]);
     }
   }
-  // source (TurbModel_KOMSST.h:256-304)
+  // sources (TurbModel_KOMSST.h:256-304)
   {
     const double rho = pc[0], vol = m.vol[c];
-    const double kine = kc, omega = P.sst_phi[ncv + c];
+    const double kine = kc, omega = wc;
     const double lim = sst_lim(P, auxc), F2 = auxc[kSF2], str = auxc[kSStr], div = auxc[kSDiv];
-    if (S == 0) {
+    {
       const double zeta = Dev::mn(1.0 / omega, kSstA1 / (lim * F2));
       const double mut = Dev::mn(Dev::mx(rho * kine * zeta, 0.0), 1.0e5);
       double Pk = mut * str * str - 2. / 3. * rho * kine * div;
       if (P.sst_limit_pk == 1) Pk = Dev::mn(Pk, 20.0 * kSstBetaStar * rho * kine * omega);
       Pk = Dev::mx(Pk, 0.0);
       const double src = Pk - kSstBetaStar * rho * omega * kine;
-      acc += src * vol;
+      acc0 += src * vol;
       const double dsrcdphi = -kSstBetaStar * omega;
-      D[5] -= dsrcdphi * vol;
-    } else {
+      D0[5] -= dsrcdphi * vol;
+    }
+    {
       const double F1 = auxc[kSF1];
       const double alfa = F1 * P.sst_alfa[0] + (1.0 - F1) * P.sst_alfa[1];
       const double beta = F1 * kSstBeta1 + (1.0 - F1) * kSstBeta2;
@@ -2742,14 +2770,18 @@ __global__ void __launch_bounds__(kBlock) k_sst_scal(DMesh m, DParams P, const d
       double Pk = str * str - 2. / 3. * zeta * div;
       Pk = Dev::mx(Pk, 0.0);
       const double src = alfa * rho * Pk + (1.0 - F1) * auxc[kSCd] - beta * rho * omega * omega;
-      acc += src * vol;
+      acc1 += src * vol;
       const double dsrcdphi = -2.0 * beta * omega;
-      D[5] -= dsrcdphi * vol;
+      D1[5] -= dsrcdphi * vol;
     }
   }
-  rhs_s[c] = acc;
-  double *dg = AS + 6 * (size_t)m.nbocv_i[c];
-  for (int k = 0; k < 6; k++) dg[k] = D[k];
+  rhs_s[c] = acc0;
+  rhs_s[ncv + c] = acc1;
+  const size_t dgi = 6 * (size_t)m.nbocv_i[c];
+  for (int k = 0; k < 6; k++) {
+    AS[dgi + k] = D0[k];
+    AS1[dgi + k] = D1[k];
+  }
 }
 
 // the segregated scalar's update and clip with its own bounds
@@ -6753,10 +6785,8 @@ void Ctx::Impl::launch_grad_flux(bool jac_follows) {
     hipLaunchKernelGGL(k_scal, dim3(nblk(ncv)), dim3(kBlock), 0, stream, dm, dp, prim.p, grad.p, bfa.p, rhs_s.p, AS.p);
   } else if (par.sst) {
     // kine's and omega's equations: residuals and AScal rows
-    hipLaunchKernelGGL(k_sst_scal<0>, dim3(nblk(ncv)), dim3(kBlock), 0, stream, dm, dp, prim.p, grad.p, bfa.p, rhs_s.p,
+    hipLaunchKernelGGL(k_sst_scal, dim3(nblk(ncv)), dim3(kBlock), 0, stream, dm, dp, prim.p, grad.p, bfa.p, rhs_s.p,
                        AS.p);
-    hipLaunchKernelGGL(k_sst_scal<1>, dim3(nblk(ncv)), dim3(kBlock), 0, stream, dm, dp, prim.p, grad.p, bfa.p,
-                       rhs_s.p + ncv, AS.p + 6 * (size_t)nnz);
   }
 }
 
