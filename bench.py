@@ -148,7 +148,7 @@ def cpu_baseline(n_box=48, reps=12):
     return out
 
 
-def plate_bench(dims=(40, 324, 110, 50), steps=3, reps=10, jac_kernel=None, turb="SA"):
+def plate_bench(dims=(40, 324, 110, 50), steps=3, reps=10, jac_kernel=None, turb="SA", scal_kernel=None):
     """One implicit step of the 2M-cell flat plate (Mach 0.2, Re 5e6 per unit
     length, Spalart-Allmaras in the coupled loop, 6x6 blocks; with turb =
     "SST" the Menter SST model in the segregated loop, 5x5 blocks and the two
@@ -167,6 +167,8 @@ def plate_bench(dims=(40, 324, 110, 50), steps=3, reps=10, jac_kernel=None, turb
     mm, joe = cases.plate_case(*dims, turb=turb)
     if jac_kernel:
         joe += f"GPU_JAC_KERNEL = {jac_kernel}\n"
+    if scal_kernel and turb == "SST":
+        joe += f"GPU_SCAL_KERNEL = {scal_kernel}\n"
     mesh = joemi.Mesh.from_faces(mm)
     del mm
     gc.collect()
@@ -383,6 +385,7 @@ def main():
     ap.add_argument("--grad-kernel", default=None, help="GPU_GRAD_KERNEL: ENTRY or CELL")
     ap.add_argument("--flux-kernel", default=None, help="GPU_FLUX_KERNEL: ENTRY or CELL")
     ap.add_argument("--jac-kernel", default=None, help="GPU_JAC_KERNEL: ENTRY or CELL")
+    ap.add_argument("--scal-kernel", default=None, help="GPU_SCAL_KERNEL (the SST plate's scalar rows): ENTRY or CELL")
     ap.add_argument("--grad-thread-order", default=None, help="GPU_GRAD_THREAD_ORDER: SORTED or NATIVE")
     ap.add_argument("--cell-order", default=None,
                     help="device cell numbering (GPU_CELL_ORDER): NATIVE, GROUPED, MORTON, MORTON_GROUPED")
@@ -528,7 +531,7 @@ def main():
         dims = tuple(int(x) for x in args.plate_dims.split(","))
         for turb in args.plate_turb.split(","):
             key = "implicit_rans_flat_plate" + ("" if turb == "SA" else "_" + turb.lower())
-            out["config"][key] = plate_bench(dims, jac_kernel=args.jac_kernel, turb=turb)
+            out["config"][key] = plate_bench(dims, jac_kernel=args.jac_kernel, turb=turb, scal_kernel=args.scal_kernel)
     if args.serve > 0:
         lo, hi = (int(x) for x in args.serve_cells.split(","))
         out["serving"] = serving_bench(args.serve, 7, lo, hi, args.serve_active, args.serve_steps, args.serve_rel, 64.0,

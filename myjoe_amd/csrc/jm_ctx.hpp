@@ -64,6 +64,7 @@ struct SolverParams {
   bool visc_limiter = true;
   int grad_kernel = kResidCell, flux_kernel = kResidEntry;
   int jac_kernel = kResidEntry;  // GPU_JAC_KERNEL: ENTRY (one lane per cell-face entry) or CELL (one thread per row)
+  int scal_kernel = kResidEntry;  // GPU_SCAL_KERNEL: the SST scalar rows per entry or per cell
   // preconditioner of the step's BiCGStab (GPU_PRECONDITIONER): 0 = block
   // Jacobi (the reference's), else ILU0: block ILU(0) on subdomains of this
   // many consecutive cells of the device order (GPU_ILU_SUBDOMAIN)

@@ -2784,6 +2784,228 @@ __global__ void __launch_bounds__(kBlock) k_sst_scal(DMesh m, DParams P, const d
   }
 }
 
+// The same rows per cell-face entry: a lane evaluates its entry's face for
+// both scalars, writes the off-diagonal AScal entries it owns, and leaves its
+// contributions to the cell's sums in LDS: the convective acc (2) and
+// diagonal entries (2 x 6, signed for the cell: d - s == d + (-s)), then the
+// diffusive acc (2) and diagonal [5] (2), -0.0 where it adds nothing. The
+// owner lane sums the convective terms of the cell's entries in entry order,
+// then the diffusive ones, then adds the sources, as k_sst_scal does; so the
+// bits are k_sst_scal's. Each off-diagonal block has one writer (the SST path
+// requires a mesh without duplicate neighbour pairs).
+constexpr int kSstScalWaves = 1;
+
+__global__ void __launch_bounds__(64 * kSstScalWaves) k_sst_scal_e(DMesh m, DParams P, const double *__restrict__ prim,
+                                                                  const double *__restrict__ grad,
+                                                                  const double *__restrict__ bfa,
+                                                                  double *__restrict__ rhs_s, double *__restrict__ AS) {
+  constexpr int NS = 18;
+  __shared__ double lds[64 * kSstScalWaves * NS];
+  double *slot = lds + NS * threadIdx.x;
+  const EntryLane L = entry_lane<kSstScalWaves>(m, hw_blk());
+  const int ncv = m.ncv;
+  const size_t nnz = m.nnz;
+  double *AS1 = AS + 6 * nnz;
+  const double *kin = P.sst_phi, *omg = P.sst_phi + ncv;
+  const int c = L.e >= 0 ? L.c : 0;
+  const double *pc = prim + 8 * (size_t)c;
+  const double *auxc = P.sst_aux + kSstN * (size_t)c;
+  const double kc = kin[c], wc = omg[c];
+  if (L.e >= 0) {
+    const double gam = P.gamma;
+    const bool snd = P.second_order != 0;
+    const double *gc = grad + 16 * (size_t)c;
+    Side own;
+    own.rho = pc[0]; own.u[0] = pc[1]; own.u[1] = pc[2]; own.u[2] = pc[3]; own.p = pc[4];
+    double S0[6], S1[6], F[5], Fs0, Fs1;
+    const int2 en = m.cfl[L.e];
+    const int f = en.x & 0x7fffffff;
+    const double *geo = m.fgeo + 8 * (size_t)f;
+    const double n[3] = {geo[0], geo[1], geo[2]};
+    const double area = geo[3];
+    const double xfa[3] = {geo[4], geo[5], geo[6]};
+    double dacc0 = -0.0, dacc1 = -0.0, dd0 = -0.0, dd1 = -0.0;  // diffusive terms
+    if (en.y >= 0) {
+      const int nb = en.y;
+      const double *pn = prim + 8 * (size_t)nb;
+      const double *auxn = P.sst_aux + kSstN * (size_t)nb;
+      const double kn = kin[nb], wn = omg[nb];
+      Side nbs;
+      nbs.rho = pn[0]; nbs.u[0] = pn[1]; nbs.u[1] = pn[2]; nbs.u[2] = pn[3]; nbs.p = pn[4];
+      Side so = own, sn = nbs;
+      double kfo = kc, kfn = kn, wfo = wc, wfn = wn;
+      if (snd) {
+        if (reconstruct(so, pc, gc, xfa)) {
+          const double r[3] = {xfa[0] - pc[5], xfa[1] - pc[6], xfa[2] - pc[7]};
+          kfo = kfo + dot3(r, auxc + kSGradK);
+          wfo = wfo + dot3(r, auxc + kSGradW);
+        }
+        if (reconstruct(sn, pn, grad + 16 * (size_t)nb, xfa)) {
+          const double r[3] = {xfa[0] - pn[5], xfa[1] - pn[6], xfa[2] - pn[7]};
+          kfn = kfn + dot3(r, auxn + kSGradK);
+          wfn = wfn + dot3(r, auxn + kSGradW);
+        }
+      }
+      const bool s0 = en.x >= 0;
+      const int noc = m.cfl_noc[L.e] & 0x3fffffff;
+      double *off0 = AS + 6 * (size_t)noc, *off1 = AS1 + 6 * (size_t)noc;
+      const Side fl = s0 ? so : sn, fr = s0 ? sn : so;
+      const Side jl = s0 ? own : nbs, jr = s0 ? nbs : own;
+      const double kl = s0 ? kfo : kfn, kr = s0 ? kfn : kfo, kjl = s0 ? kc : kn, kjr = s0 ? kn : kc;
+      const double wl = s0 ? wfo : wfn, wr = s0 ? wfn : wfo, wjl = s0 ? wc : wn, wjr = s0 ? wn : wc;
+      flux_hllc(F, fl.rho, fl.u, fl.p, fr.rho, fr.u, fr.p, gam, area, n, kl, kr, &Fs0, kl, kr, &Fs1, wl, wr);
+      slot[0] = s0 ? -Fs0 : Fs0;
+      slot[1] = s0 ? -Fs1 : Fs1;
+      // the diagonal's side (icv0 adds A_L's rows, icv1 subtracts A_R's)
+      jac_hllc_scal2(S0, S1, s0 ? 0 : 1, jl.rho, jl.u, jl.p, jr.rho, jr.u, jr.p, gam, area, n, kjl, kjr, wjl, wjr, kjl,
+                     kjr);
+      for (int k = 0; k < 6; k++) {
+        slot[2 + k] = s0 ? S0[k] : -S0[k];
+        slot[8 + k] = s0 ? S1[k] : -S1[k];
+      }
+      // the off-diagonal block (icv0's = +A_R's rows, icv1's = -A_L's)
+      jac_hllc_scal2(S0, S1, s0 ? 1 : 0, jl.rho, jl.u, jl.p, jr.rho, jr.u, jr.p, gam, area, n, kjl, kjr, wjl, wjr, kjl,
+                     kjr);
+      for (int k = 0; k < 6; k++) {
+        off0[k] = s0 ? 0.0 + S0[k] : 0.0 - S0[k];
+        off1[k] = s0 ? 0.0 + S1[k] : 0.0 - S1[k];
+      }
+      // diffusion (the cell's second pass)
+      const double Tc = pc[4] / (pc[0] * P.R), Tn = pn[4] / (pn[0] * P.R);
+      double sV[3], smag, mul, w0, w1;
+      visc_internal_geom(P, s0 ? pc + 5 : pn + 5, s0 ? pn + 5 : pc + 5, xfa, s0 ? Tc : Tn, s0 ? Tn : Tc, sV, smag,
+                         mul, w0, w1);
+      const int c0 = s0 ? c : nb, c1 = s0 ? nb : c;
+      const double rho0 = s0 ? pc[0] : pn[0], rho1 = s0 ? pn[0] : pc[0];
+      const double *a0 = P.sst_aux + kSstN * (size_t)c0, *a1 = P.sst_aux + kSstN * (size_t)c1;
+      const double mut = sst_mut_face(P, ncv, w0, w1, rho0, rho1, c0, c1);
+      const double f1_fa = (w1 * a0[kSF1] + w0 * a1[kSF1]) / (w0 + w1);
+      const double alpha = dot3(n, sV);
+      const double nmag = area;
+      auto face = [&](double sg1, double sg2, const double *phi, int gS, double &dacc, double &dd, double *off) {
+        const double coeff = sg1 * f1_fa + (1. - f1_fa) * sg2;
+        const double diff = mul + coeff * mut;
+        const double phi0 = phi[c0], phi1 = phi[c1];
+        const double *g0 = a0 + gS, *g1 = a1 + gS;
+        const double viscFlux = diff * nmag *
+                                (alpha * (phi1 - phi0) / smag + 0.5 * (g0[0] + g1[0]) * (n[0] - alpha * sV[0]) +
+                                 0.5 * (g0[1] + g1[1]) * (n[1] - alpha * sV[1]) +
+                                 0.5 * (g0[2] + g1[2]) * (n[2] - alpha * sV[2]));
+        const double a = diff * nmag * alpha / smag;
+        dacc = s0 ? viscFlux : -viscFlux;
+        dd = s0 ? a / rho0 : a / rho1;
+        off[5] = off[5] + -(s0 ? a / rho1 : a / rho0);
+      };
+      face(kSstSigmaK1, kSstSigmaK2, kin, kSGradK, dacc0, dd0, off0);
+      face(kSstSigmaW1, kSstSigmaW2, omg, kSGradW, dacc1, dd1, off1);
+    } else {
+      const ZoneDev &z = m.zones[-en.y - 1];
+      const double *bb = bfa + 8 * (size_t)f;
+      const double kb = P.sst_bfa[2 * (size_t)f], wb = P.sst_bfa[2 * (size_t)f + 1];
+      const double ub[3] = {bb[1], bb[2], bb[3]};
+      if (z.bc == kBcSymmetry || z.bc == kBcWall) {
+        flux_hllc(F, bb[0], ub, bb[4], bb[0], ub, bb[4], gam, area, n, kb, kb, &Fs0, kb, kb, &Fs1, wb, wb);
+        jac_hllc_scal2(S0, S1, 0, bb[0], ub, bb[4], bb[0], ub, bb[4], gam, area, n, kb, kb, wb, wb, kb, kb);
+      } else {
+        Side so = own;
+        double kfo = kc, wfo = wc;
+        if (snd && reconstruct(so, pc, gc, xfa)) {
+          const double r[3] = {xfa[0] - pc[5], xfa[1] - pc[6], xfa[2] - pc[7]};
+          kfo = kfo + dot3(r, auxc + kSGradK);
+          wfo = wfo + dot3(r, auxc + kSGradW);
+        }
+        flux_hllc(F, so.rho, so.u, so.p, bb[0], ub, bb[4], gam, area, n, kfo, kb, &Fs0, kfo, kb, &Fs1, wfo, wb);
+        jac_hllc_scal2(S0, S1, 0, own.rho, own.u, own.p, bb[0], ub, bb[4], gam, area, n, kc, kb, wc, wb, kc, kb);
+      }
+      slot[0] = -Fs0;
+      slot[1] = -Fs1;
+      for (int k = 0; k < 6; k++) {
+        slot[2 + k] = S0[k];
+        slot[8 + k] = S1[k];
+      }
+      if (z.skind || z.skind2) {
+        const double mul = mu_lam(P, bb[5]);
+        const double mutb = z.bc != kBcWall ? sst_mut_cell(P, ncv, pc[0], c) : 0.0;
+        const double s_half[3] = {xfa[0] - pc[5], xfa[1] - pc[6], xfa[2] - pc[7]};
+        const double smag_half = fabs(dot3(s_half, n));
+        const double nmag = area;
+        auto bface = [&](double sg1, double sg2, double phb, double ph, double &dacc, double &dd) {
+          double diff = mul;
+          if (z.bc != kBcWall) {
+            const double coeff = sg1 * auxc[kSF1] + (1. - auxc[kSF1]) * sg2;
+            diff = mul + coeff * mutb;
+          }
+          dacc = diff * nmag * (phb - ph) / smag_half;
+          dd = diff * nmag / smag_half / pc[0];
+        };
+        if (z.skind) bface(kSstSigmaK1, kSstSigmaK2, kb, kc, dacc0, dd0);
+        if (z.skind2) bface(kSstSigmaW1, kSstSigmaW2, wb, wc, dacc1, dd1);
+      }
+    }
+    slot[14] = dacc0;
+    slot[15] = dacc1;
+    slot[16] = dd0;
+    slot[17] = dd1;
+  }
+  __syncthreads();
+  if (L.e < 0 || L.pos != 0) return;
+  const int nent = m.cfl_i[c + 1] - m.cfl_i[c];
+  double acc0 = 0.0, acc1 = 0.0;
+  double D0[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, D1[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int j = 0; j < nent; j++) {
+    const double *s = slot + NS * j;
+    acc0 += s[0];
+    acc1 += s[1];
+    for (int k = 0; k < 6; k++) {
+      D0[k] += s[2 + k];
+      D1[k] += s[8 + k];
+    }
+  }
+  for (int j = 0; j < nent; j++) {
+    const double *s = slot + NS * j;
+    acc0 += s[14];
+    acc1 += s[15];
+    D0[5] += s[16];
+    D1[5] += s[17];
+  }
+  // sources (TurbModel_KOMSST.h:256-304)
+  {
+    const double rho = pc[0], vol = m.vol[c];
+    const double kine = kc, omega = wc;
+    const double lim = sst_lim(P, auxc), F2 = auxc[kSF2], str = auxc[kSStr], div = auxc[kSDiv];
+    {
+      const double zeta = Dev::mn(1.0 / omega, kSstA1 / (lim * F2));
+      const double mut = Dev::mn(Dev::mx(rho * kine * zeta, 0.0), 1.0e5);
+      double Pk = mut * str * str - 2. / 3. * rho * kine * div;
+      if (P.sst_limit_pk == 1) Pk = Dev::mn(Pk, 20.0 * kSstBetaStar * rho * kine * omega);
+      Pk = Dev::mx(Pk, 0.0);
+      const double src = Pk - kSstBetaStar * rho * omega * kine;
+      acc0 += src * vol;
+      const double dsrcdphi = -kSstBetaStar * omega;
+      D0[5] -= dsrcdphi * vol;
+    }
+    {
+      const double F1 = auxc[kSF1];
+      const double alfa = F1 * P.sst_alfa[0] + (1.0 - F1) * P.sst_alfa[1];
+      const double beta = F1 * kSstBeta1 + (1.0 - F1) * kSstBeta2;
+      const double zeta = Dev::mx(omega, lim * F2 / kSstA1);
+      double Pk = str * str - 2. / 3. * zeta * div;
+      Pk = Dev::mx(Pk, 0.0);
+      const double src = alfa * rho * Pk + (1.0 - F1) * auxc[kSCd] - beta * rho * omega * omega;
+      acc1 += src * vol;
+      const double dsrcdphi = -2.0 * beta * omega;
+      D1[5] -= dsrcdphi * vol;
+    }
+  }
+  rhs_s[c] = acc0;
+  rhs_s[ncv + c] = acc1;
+  const size_t dgi = 6 * (size_t)m.nbocv_i[c];
+  for (int k = 0; k < 6; k++) {
+    AS[dgi + k] = D0[k];
+    AS1[dgi + k] = D1[k];
+  }
+}
+
 // the segregated scalar's update and clip with its own bounds
 // (JoeWithModels.cpp:1084-1088)
 template <typename T>
@@ -5114,6 +5336,7 @@ void parse_params(const std::string &joe_in, const Mesh &m, SolverParams &p, std
   p.grad_kernel = kform("GPU_GRAD_KERNEL", "CELL");
   p.flux_kernel = kform("GPU_FLUX_KERNEL", "ENTRY");
   p.jac_kernel = kform("GPU_JAC_KERNEL", "ENTRY");
+  p.scal_kernel = kform("GPU_SCAL_KERNEL", "ENTRY");
   {  // the per-cell gradient's thread order (this build's option; results do not depend on it)
     const std::string o = pm.str("GPU_GRAD_THREAD_ORDER", "NATIVE");
     if (o != "SORTED" && o != "NATIVE")
@@ -6784,9 +7007,13 @@ void Ctx::Impl::launch_grad_flux(bool jac_follows) {
     // the nuSA equation: residual and AScal rows (calcRhs scalar terms)
     hipLaunchKernelGGL(k_scal, dim3(nblk(ncv)), dim3(kBlock), 0, stream, dm, dp, prim.p, grad.p, bfa.p, rhs_s.p, AS.p);
   } else if (par.sst) {
-    // kine's and omega's equations: residuals and AScal rows
-    hipLaunchKernelGGL(k_sst_scal, dim3(nblk(ncv)), dim3(kBlock), 0, stream, dm, dp, prim.p, grad.p, bfa.p, rhs_s.p,
-                       AS.p);
+    // kine's and omega's equations: residuals and AScal rows, per entry or per cell
+    if (par.scal_kernel == kResidEntry && nwave > 0)
+      hipLaunchKernelGGL(k_sst_scal_e, dim3((nwave + kSstScalWaves - 1) / kSstScalWaves), dim3(64 * kSstScalWaves), 0,
+                         stream, dm, dp, prim.p, grad.p, bfa.p, rhs_s.p, AS.p);
+    else
+      hipLaunchKernelGGL(k_sst_scal, dim3(nblk(ncv)), dim3(kBlock), 0, stream, dm, dp, prim.p, grad.p, bfa.p, rhs_s.p,
+                         AS.p);
   }
 }
 

@@ -164,6 +164,7 @@ def _sst_box_joe():
     (lambda s: s + "sigma_k1 = 0.9\n", 5, "sigma_k1"),                                # the model's defaults only
     (lambda s: cases.coupled(s), 5, "coupled"),
     (lambda s: s + "GPU_JAC_KERNEL = CELL\n", 5, "per-entry Jacobian"),
+    (lambda s: s + "GPU_SCAL_KERNEL = FACE\n", 3, "GPU_SCAL_KERNEL"),                 # ENTRY or CELL
 ])
 def test_sst_input_checked_before_device(edit, status, needle):
     # TURB_MODEL = SST (the reference's RansTurbKOmSST mix-in) is parsed, and

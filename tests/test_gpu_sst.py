@@ -22,13 +22,19 @@ test measures that band with the oracle and holds the device to it:
   largest magnitude, and kine and omega within twice the oracle's band
   (about 9e-7 and 4e-8: the scalar solves stop moving them at slightly
   different points).
+
+The scalar rows have two kernel forms (GPU_SCAL_KERNEL = ENTRY, one lane per
+cell-face entry, the default; CELL, one thread per cell); they give the same
+bits, on the reference's residual fixtures and over 50 steps of the bump.
 """
 import numpy as np
 import pytest
 
 from myjoe_amd import cases, joemi, meshgen
 from oracle.oracle import Oracle
-from tests.test_oracle_golden import load
+from tests.helpers import compare
+from tests.test_mesh_golden import MESHES
+from tests.test_oracle_golden import joe_text, load
 from tests.test_plate import SMALL
 
 pytestmark = pytest.mark.gpu
@@ -134,3 +140,48 @@ def test_sst_bump_tracks_oracle():
         assert v <= 1e-12, (key, v)
     for key, v in late.items():
         assert v <= 2.0 * band[key], (key, v, band[key])
+
+
+SST_RHS = ["bump24_hllc_visc_pl_sst", "box4_hllc_visc_su_sst", "box4_hllc_visc_pl_sst_alt_ana_gg",
+           "box4_hllc_visc_nl_sst_first", "box4_hllc_visc_hi_sst_dtsum"]
+
+
+@pytest.mark.parametrize("fixture", SST_RHS)
+def test_sst_scalar_row_forms_agree(fixture):
+    # the scalar rows per cell-face entry (the default) and per cell give the
+    # same bits, and both the reference's
+    print(f"sst scalar rows {fixture}", flush=True)
+    g = load("rhs_" + fixture)
+    mesh = joemi.Mesh.from_faces(MESHES[fixture.split("_")[0]]())
+    got = []
+    for form in ("ENTRY", "CELL"):
+        s = joemi.Solver(mesh, joe_text(g) + f"GPU_SCAL_KERNEL = {form}\n")
+        s.set_state(g["RHO"], g["RHOU"], g["RHOE"])
+        s.set_scalar(g["KINE"], "kine")
+        s.set_scalar(g["OMEGA"], "omega")
+        s.calc_rhs(True)
+        f = {k: s.get(k).copy() for k in ("RHS_KINE", "RHS_OMEGA", "ASCAL")}
+        for k, v in f.items():
+            compare(v, g[k], 1e-12, f"{form}:{fixture}:{k}")
+        s.step()
+        f["DSCAL"] = s.get("DSCAL").copy()
+        f["KINE"], f["OMEGA"] = s.get("KINE").copy(), s.get("OMEGA").copy()
+        got.append(f)
+    for k in got[0]:
+        assert np.array_equal(got[0][k], got[1][k]), (fixture, k)
+
+
+def test_sst_scalar_row_forms_agree_over_a_run():
+    print("sst scalar rows, bump run", flush=True)
+    g = load("solve_bump24sst")
+    joe = bytes(g["JOE_IN"]).decode()
+    mesh = joemi.Mesh.from_faces(meshgen.bump_channel(24, 8))
+    out = []
+    for form in ("ENTRY", "CELL"):
+        s = joemi.Solver(mesh, joe + f"GPU_SCAL_KERNEL = {form}\n")
+        s.init_uniform()
+        for _ in range(50):
+            s.step()
+        out.append(scal_fields(s))
+    for k in out[0]:
+        assert np.array_equal(out[0][k], out[1][k]), k
