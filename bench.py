@@ -148,6 +148,66 @@ def cpu_baseline(n_box=48, reps=12):
     return out
 
 
+def chem_bench(npts=8_000_000, reps=20, cpu_sample=500_000):
+    """The FPVA chemistry-table lookup: the reference's LookupCoeff (ten FP32
+    coefficients, trilinear with FP64 weights) on a 128 x 32 x 64 table, over
+    a field of npts points in cell order (a round jet's mean mixture fraction,
+    its variance and a progress variable, smooth along the cells), timed with
+    HIP events on the lookup's stream. Compulsory bytes per launch: the points
+    (24 B each), the results (80 B each) and the table once (128 x 32 x 64 x
+    10 floats = 10.5 MB). The oracle (one core) is timed on cpu_sample points."""
+    import numpy as np
+
+    from myjoe_amd import chemtable, joemi
+
+    x1, x2, x3, data, names = chemtable.synthetic_table(128, 32, 64, seed=0)
+    m = round(npts ** (1.0 / 3.0))
+    n = m * m * m
+    g = (np.arange(m) + 0.5) / m
+    X, Y, Zc = np.meshgrid(g, g, g, indexing="ij")  # cell centres, x along the jet
+    r2 = (Y - 0.5) ** 2 + (Zc - 0.5) ** 2
+    width = 0.02 + 0.1 * X
+    zm = np.exp(-r2 / (2 * width ** 2)) * (0.05 / (0.05 + X))
+    zv = (0.25 * zm * (1 - zm) * np.exp(-4 * X)).reshape(-1)
+    pc = (np.clip(X * 1.5, 0, 1) * (1 - np.exp(-r2 / 0.01))).reshape(-1)
+    zm = zm.reshape(-1)
+    del X, Y, Zc, r2, width
+    t = joemi.Chemtable.create(x1, x2, x3, data, names)
+    d = [joemi.DeviceArray(v) for v in (zm, zv, pc)]
+    out = joemi.DeviceArray(np.zeros((10, n)))
+    ms = t.time_lookup(*d, t.COEFF, out, reps=reps)
+    sl = np.arange(0, n, max(1, n // 20000))
+    k = min(cpu_sample, n)
+    sample = {"table": (x1, x2, x3, data), "points": (zm[:k], zv[:k], pc[:k]),
+              "check": ((zm[sl], zv[sl], pc[sl]), out.numpy()[:, sl])}
+    bytes_ = n * (24 + 80) + 128 * 32 * 64 * 10 * 4
+    res = {"workload": f"LookupCoeff (10 FP32 coefficients) on a 128 x 32 x 64 table, {n} points ({m}^3 cells "
+                       "of a round jet's Zmean, Zvar, C in cell order)",
+           "points": n, "ms": round(ms, 4), "gpoints_per_s": round(n / (ms * 1e-3) / 1e9, 3),
+           "bytes_per_launch": int(bytes_), "achieved_GBs": round(bytes_ / (ms * 1e-3) / 1e9, 1),
+           "frac_of_hbm_peak": round(bytes_ / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    return res, sample
+
+
+def chem_cpu_baseline(sample):
+    """The oracle's lookup (one core) timed on the sample's points, and the
+    device's results at every n/20000-th point checked against it."""
+    import numpy as np
+
+    from oracle.oracle import chemtable_lookup
+
+    tab = sample["table"]
+    pts = sample["points"]
+    t0 = time.perf_counter()
+    chemtable_lookup(*tab, *pts, range(10))
+    cpu_s = time.perf_counter() - t0
+    cpts, dev = sample["check"]
+    same = bool(np.array_equal(dev, chemtable_lookup(*tab, *cpts, range(10))))
+    return {"cpu_oracle_1core": {"mpoints_per_s": round(pts[0].size / cpu_s / 1e6, 2),
+                                 "sample_points": int(pts[0].size)},
+            "sampled_points_equal_oracle": same}
+
+
 def plate_bench(dims=(40, 324, 110, 50), steps=3, reps=10, jac_kernel=None, turb="SA", scal_kernel=None):
     """One implicit step of the 2M-cell flat plate (Mach 0.2, Re 5e6 per unit
     length, Spalart-Allmaras in the coupled loop, 6x6 blocks; with turb =
@@ -400,6 +460,7 @@ def main():
     ap.add_argument("--serve-legs", default="plain,batched",
                     help="serving parts to run: plain (FP64 loop), streams (per-stream graph server), batched")
     ap.add_argument("--plate", type=int, default=1, help="time the 2M-cell flat-plate step (0: skip)")
+    ap.add_argument("--chem", type=int, default=8_000_000, help="points of the chemistry-table lookup leg (0: skip)")
     ap.add_argument("--plate-dims", default="40,324,110,50", help="cells ahead of / along the plate, wall-normal, z")
     ap.add_argument("--plate-turb", default="SA,SST", help="turbulence models of the plate leg (SA coupled, SST "
                                                           "segregated)")
@@ -532,6 +593,9 @@ def main():
         for turb in args.plate_turb.split(","):
             key = "implicit_rans_flat_plate" + ("" if turb == "SA" else "_" + turb.lower())
             out["config"][key] = plate_bench(dims, jac_kernel=args.jac_kernel, turb=turb, scal_kernel=args.scal_kernel)
+    chem_sample = None
+    if args.chem > 0:
+        out["config"]["fpva_chemtable_lookup"], chem_sample = chem_bench(args.chem)
     if args.serve > 0:
         lo, hi = (int(x) for x in args.serve_cells.split(","))
         out["serving"] = serving_bench(args.serve, 7, lo, hi, args.serve_active, args.serve_steps, args.serve_rel, 64.0,
@@ -539,6 +603,8 @@ def main():
                                        cfl_max=args.serve_cfl)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
+        if chem_sample is not None:
+            out["config"]["fpva_chemtable_lookup"].update(chem_cpu_baseline(chem_sample))
     print(json.dumps(_strict_json(out), allow_nan=False), flush=True)
 
 

@@ -359,6 +359,49 @@ jm_status jm_server_wait(jm_server *s, int ticket);
 jm_status jm_server_result(jm_server *s, int ticket, double *rho, double *rhou, double *rhoE, jm_solve_report *rep);
 jm_status jm_server_release(jm_server *s, int ticket);
 
+/* ---------------------------------------------------------------------------
+ * FPVA chemistry table. Replaces ChemtableCartesianLinear_single
+ * (mixing_combustion/ChemtableCartesianLinear_single.h:44-199): a structured
+ * Cartesian table of FP32 values over (Zmean, Zvar, progress variable),
+ * looked up by linear interpolation along each axis with FP64 weights. The
+ * reference looks up one point per call (Lookup, LookupCoeff,
+ * LookupSelectedCoeff, :374-550); here one call looks up a whole field of
+ * points on the device, each value equal to the reference's bit for bit.
+ * ------------------------------------------------------------------------- */
+typedef struct jm_chemtable jm_chemtable;
+
+/* ChemtableCartesianLinear_single::Load (:19-246): the reference's binary
+ * table file (version 2.0), checked, then uploaded to the current device.
+ * A file that is not such a table fails with JM_ERR_PARAM, a table with an
+ * axis of fewer than two strictly increasing values with JM_ERR_ARG, both
+ * before any device call. */
+jm_status jm_chemtable_load(const char *path, jm_chemtable **out);
+/* The same table from arrays: x1[n1], x2[n2], x3[n3], data[n1][n2][n3][nvar]
+ * (the variable fastest, the file's order); names[nvar] may be NULL (then
+ * "0", "1", ...). */
+jm_status jm_chemtable_create(int n1, int n2, int n3, int nvar, const double *x1, const double *x2, const double *x3,
+                              const float *data, const char *const *names, jm_chemtable **out);
+void jm_chemtable_destroy(jm_chemtable *t);
+/* GetChemtableDimension1..4 into dims[4]; GetReferencePressure, GetToxi,
+ * GetTfuel into ref[3]; either may be NULL */
+jm_status jm_chemtable_info(const jm_chemtable *t, int *dims, double *ref);
+/* the index of a variable by name, as Lookup's tag (:394-400); an unknown
+ * name fails with JM_ERR_ARG */
+jm_status jm_chemtable_var(const jm_chemtable *t, const char *name, int *index);
+/* Lookup / LookupCoeff / LookupSelectedCoeff over n points: device arrays
+ * zmean_dev, zvar_dev, prog_dev [n] (clipped to the table as the reference
+ * clips them); out_dev[k][n] (device) receives variable vars[k] (host array,
+ * 1 <= nout <= 32; LookupCoeff is vars 0..9, LookupSelectedCoeff 0..4).
+ * Launched on stream (NULL: the null stream) and not waited for. */
+jm_status jm_chemtable_lookup(const jm_chemtable *t, int64_t n, const double *zmean_dev, const double *zvar_dev,
+                              const double *prog_dev, int nout, const int *vars, double *out_dev, void *stream);
+/* Measurement hook, no counterpart in the reference: jm_chemtable_lookup
+ * launched reps times on stream after one untimed launch; *ms = the mean
+ * duration of one, from HIP events recorded on that stream. */
+jm_status jm_chemtable_time_lookup(const jm_chemtable *t, int64_t n, const double *zmean_dev, const double *zvar_dev,
+                                   const double *prog_dev, int nout, const int *vars, double *out_dev, void *stream,
+                                   int reps, double *ms);
+
 #ifdef __cplusplus
 }
 #endif

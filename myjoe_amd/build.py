@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libjoemi.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 HOST_SRC = ["jm_mesh.cpp", "jm_capi.cpp", "jm_server.cpp"]
-DEV_SRC = ["jm_solver.hip"]
+DEV_SRC = ["jm_solver.hip", "jm_chemtable.hip"]
 
 HOST_FLAGS = ["-O2", "-ffp-contract=off", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
               f"-I{ROCM}/include", "-D__HIP_PLATFORM_AMD__"]

@@ -7,6 +7,7 @@
 #include <string>
 
 #include "../../include/joemi.h"
+#include "jm_chemtable.hpp"
 #include "jm_ctx.hpp"
 #include "jm_param.hpp"
 #include "jm_mesh.hpp"
@@ -17,6 +18,9 @@ struct jm_mesh {
 };
 struct jm_server {
   std::unique_ptr<jm::Server> s;
+};
+struct jm_chemtable {
+  std::unique_ptr<jm::Chemtable> t;
 };
 struct jm_ctx {
   std::unique_ptr<jm::Ctx> c;
@@ -456,6 +460,78 @@ jm_status jm_block_bcgstab_noalloc(int ncv, const int *nbocv_i, const int *nbocv
 }
 
 int64_t jm_block_bcgstab_noalloc_max_cells(void) { return (int64_t)jm::block_bcgstab_noalloc_max_cells(); }
+
+jm_status jm_chemtable_load(const char *path, jm_chemtable **out) {
+  return guard([&] {
+    if (!path || !out) throw jm::ArgError("jm_chemtable_load: null argument");
+    *out = nullptr;
+    auto t = std::make_unique<jm_chemtable>();
+    t->t = std::make_unique<jm::Chemtable>(jm::read_chemtable(path));
+    *out = t.release();
+    return JM_OK;
+  });
+}
+
+jm_status jm_chemtable_create(int n1, int n2, int n3, int nvar, const double *x1, const double *x2, const double *x3,
+                              const float *data, const char *const *names, jm_chemtable **out) {
+  return guard([&] {
+    if (!out || !x1 || !x2 || !x3 || !data) throw jm::ArgError("jm_chemtable_create: null argument");
+    *out = nullptr;
+    if (n1 < 1 || n2 < 1 || n3 < 1 || nvar < 1) throw jm::ArgError("jm_chemtable_create: dimensions must be >= 1");
+    jm::ChemtableHost h;
+    h.n1 = n1, h.n2 = n2, h.n3 = n3, h.nvar = nvar;
+    h.x1.assign(x1, x1 + n1);
+    h.x2.assign(x2, x2 + n2);
+    h.x3.assign(x3, x3 + n3);
+    for (int l = 0; l < nvar; l++) h.names.push_back(names ? std::string(names[l]) : std::to_string(l));
+    h.data.assign(data, data + (size_t)n1 * n2 * n3 * nvar);
+    auto t = std::make_unique<jm_chemtable>();
+    t->t = std::make_unique<jm::Chemtable>(std::move(h));
+    *out = t.release();
+    return JM_OK;
+  });
+}
+
+void jm_chemtable_destroy(jm_chemtable *t) { delete t; }
+
+jm_status jm_chemtable_info(const jm_chemtable *t, int *dims, double *ref) {
+  return guard([&] {
+    if (!t) throw jm::ArgError("jm_chemtable_info: null table");
+    const auto &h = t->t->meta();
+    if (dims) dims[0] = h.n1, dims[1] = h.n2, dims[2] = h.n3, dims[3] = h.nvar;
+    if (ref) ref[0] = h.pref, ref[1] = h.toxi, ref[2] = h.tfuel;
+    return JM_OK;
+  });
+}
+
+jm_status jm_chemtable_var(const jm_chemtable *t, const char *name, int *index) {
+  return guard([&] {
+    if (!t || !name || !index) throw jm::ArgError("jm_chemtable_var: null argument");
+    const int l = t->t->var_index(name);
+    if (l < 0) throw jm::ArgError(std::string("Unknown variable in chemistry table: ") + name);
+    *index = l;
+    return JM_OK;
+  });
+}
+
+jm_status jm_chemtable_lookup(const jm_chemtable *t, int64_t n, const double *zmean_dev, const double *zvar_dev,
+                              const double *prog_dev, int nout, const int *vars, double *out_dev, void *stream) {
+  return guard([&] {
+    if (!t) throw jm::ArgError("jm_chemtable_lookup: null table");
+    t->t->lookup(n, zmean_dev, zvar_dev, prog_dev, nout, vars, out_dev, stream);
+    return JM_OK;
+  });
+}
+
+jm_status jm_chemtable_time_lookup(const jm_chemtable *t, int64_t n, const double *zmean_dev, const double *zvar_dev,
+                                   const double *prog_dev, int nout, const int *vars, double *out_dev, void *stream,
+                                   int reps, double *ms) {
+  return guard([&] {
+    if (!t || !ms) throw jm::ArgError("jm_chemtable_time_lookup: null argument");
+    *ms = t->t->time_lookup(n, zmean_dev, zvar_dev, prog_dev, nout, vars, out_dev, stream, reps);
+    return JM_OK;
+  });
+}
 
 jm_status jm_wall_values(jm_ctx *c, int zone, double *out, int64_t *count) {
   return guard([&] {

@@ -97,6 +97,14 @@ def lib():
         L.jm_server_wait.argtypes = [vp, C.c_int]
         L.jm_server_result.argtypes = [vp, C.c_int, vp, vp, vp, vp]
         L.jm_server_release.argtypes = [vp, C.c_int]
+        L.jm_chemtable_load.argtypes = [C.c_char_p, vp]
+        L.jm_chemtable_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp, vp]
+        L.jm_chemtable_destroy.argtypes = [vp]
+        L.jm_chemtable_destroy.restype = None
+        L.jm_chemtable_info.argtypes = [vp, vp, vp]
+        L.jm_chemtable_var.argtypes = [vp, C.c_char_p, ip]
+        L.jm_chemtable_lookup.argtypes = [vp, C.c_int64, vp, vp, vp, C.c_int, vp, vp, vp]
+        L.jm_chemtable_time_lookup.argtypes = [vp, C.c_int64, vp, vp, vp, C.c_int, vp, vp, vp, C.c_int, dp]
         _lib = L
     return _lib
 
@@ -505,3 +513,83 @@ class Server:
     def release(self, ticket: int):
         _check(lib().jm_server_release(self.h, ticket))
         self._meshes.pop(ticket, None)
+
+
+class Chemtable:
+    """An FPVA chemistry table on the device (jm_chemtable; the reference's
+    ChemtableCartesianLinear_single). lookup() takes host arrays and returns
+    host arrays; lookup_device() works on DeviceArrays."""
+
+    COEFF = tuple(range(10))     # LookupCoeff: RoM, T0, E0, Gam0, a_Gam, mu0, a_mu, lamOcp0, a_lamOcp, src_prog
+    SELECTED = tuple(range(5))   # LookupSelectedCoeff: RoM, T0, E0, Gam0, a_Gam
+
+    def __init__(self, handle):
+        self.h = handle
+        dims = np.zeros(4, np.int32)
+        ref = np.zeros(3)
+        _check(lib().jm_chemtable_info(self.h, _ptr(dims), _ptr(ref)))
+        self.n1, self.n2, self.n3, self.nvar = (int(x) for x in dims)
+        self.pref, self.toxi, self.tfuel = (float(x) for x in ref)
+
+    @classmethod
+    def load(cls, path):
+        h = C.c_void_p()
+        _check(lib().jm_chemtable_load(str(path).encode(), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def create(cls, x1, x2, x3, data, names=None):
+        x1, x2, x3 = (np.ascontiguousarray(x, np.float64) for x in (x1, x2, x3))
+        data = np.ascontiguousarray(data, np.float32)
+        n1, n2, n3, nvar = data.shape
+        nm = None
+        if names is not None:
+            keep = [s.encode() for s in names]
+            nm = (C.c_char_p * len(keep))(*keep)
+        h = C.c_void_p()
+        _check(lib().jm_chemtable_create(n1, n2, n3, nvar, _ptr(x1), _ptr(x2), _ptr(x3), _ptr(data), nm,
+                                         C.byref(h)))
+        return cls(h)
+
+    def var(self, name) -> int:
+        i = C.c_int()
+        _check(lib().jm_chemtable_var(self.h, name.encode(), C.byref(i)))
+        return i.value
+
+    def _vars(self, vars):
+        v = np.ascontiguousarray([self.var(x) if isinstance(x, str) else int(x) for x in vars], np.int32)
+        return v
+
+    def lookup_device(self, zm, zv, pc, vars, out, stream=None):
+        """jm_chemtable_lookup on DeviceArrays zm, zv, pc [n] into out [len(vars)][n]."""
+        v = self._vars(vars)
+        n = zm.host.size
+        _check(lib().jm_chemtable_lookup(self.h, n, zm.data_ptr(), zv.data_ptr(), pc.data_ptr(), len(v), _ptr(v),
+                                         out.data_ptr(), stream))
+
+    def lookup(self, zm, zv, pc, vars):
+        """Variables vars (indices or names) at the points: [len(vars)][n] float64."""
+        zm, zv, pc = (np.ascontiguousarray(x, np.float64) for x in (zm, zv, pc))
+        n = zm.size
+        if zv.size != n or pc.size != n:
+            raise ValueError("zm, zv, pc must have one value per point")
+        if n == 0:
+            return np.zeros((len(vars), 0))
+        d = [DeviceArray(x) for x in (zm, zv, pc)]
+        out = DeviceArray(np.zeros((len(vars), n)))
+        self.lookup_device(*d, vars, out)
+        return out.numpy()
+
+    def time_lookup(self, zm, zv, pc, vars, out, reps=20, stream=None) -> float:
+        """Mean ms of one lookup over DeviceArrays (HIP events on the stream)."""
+        v = self._vars(vars)
+        ms = C.c_double()
+        _check(lib().jm_chemtable_time_lookup(self.h, zm.host.size, zm.data_ptr(), zv.data_ptr(), pc.data_ptr(),
+                                              len(v), _ptr(v), out.data_ptr(), stream, reps, C.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.jm_chemtable_destroy(self.h)
+            self.h = None
+

@@ -319,3 +319,23 @@ class Oracle:
     @property
     def cfl(self) -> float:
         return lib().or_cfl(self.ctx)
+
+
+def chemtable_lookup(x1, x2, x3, data, zm, zv, pc, vars):
+    """The oracle's Lookup (ChemtableCartesianLinear_single.cpp:374-429) of
+    variables vars at the points: [len(vars)][n] float64."""
+    L = lib()
+    f = L.or_chemtable_lookup
+    f.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int] + [C.c_void_p] * 4 + [C.c_int64] + [C.c_void_p] * 3 + \
+        [C.c_int, C.c_void_p, C.c_void_p]
+    f.restype = None
+    x1, x2, x3 = (np.ascontiguousarray(x, np.float64) for x in (x1, x2, x3))
+    data = np.ascontiguousarray(data, np.float32)
+    zm, zv, pc = (np.ascontiguousarray(x, np.float64) for x in (zm, zv, pc))
+    v = np.ascontiguousarray(vars, np.int32)
+    n1, n2, n3, nvar = data.shape
+    out = np.empty((len(v), zm.size))
+    f(n1, n2, n3, nvar, x1.ctypes.data, x2.ctypes.data, x3.ctypes.data, data.ctypes.data, zm.size, zm.ctypes.data,
+      zv.ctypes.data, pc.ctypes.data, len(v), v.ctypes.data, out.ctypes.data)
+    return out
+
