@@ -38,7 +38,8 @@ namespace jm {
 
 struct ZoneDev {
   int bc;
-  int skind;   // first scalar's boundary (nuSA, or SST's kine): 0 zero gradient, 1 "<zone>.<name> = sval"
+  int skind;   // first scalar's boundary (nuSA, or SST's kine): 0 zero gradient, 1 "<zone>.<name> = sval",
+               // 3 nuSA = PROFILES (the face's value from the profile file)
   int skind2;  // SST omega: 0 zero gradient, 1 "<zone>.omega = sval2", 2 HOOK (wall value)
   int pad;
   double par[8];
@@ -78,6 +79,7 @@ struct DParams {
   // grad nuSA (3), face-averaged mul, strain and vorticity magnitudes, wall distance
   int sa, pad2;
   const double *sa_phi, *sa_aux;
+  const double *sa_prof;  // [nfa_b] nuSA of "<zone>.nuSA = PROFILES" faces, from the profile file
   // Menter SST k-omega: kine, omega [2][ncv], per-cell model fields
   // [ncv][kSstN], boundary values [nfa_b][2]; SST_LIMIT_PK, SST_LIMITER_SHEARSTRESS
   int sst, sst_limit_pk, sst_lim_shear, pad5;
@@ -327,12 +329,13 @@ __device__ __forceinline__ void d_bc(const Blk &B, DMesh m, DParams P, const dou
       break;
   }
   if (!h_set) h_b = GAMMA * R_gas / (GAMMA - 1.0) * T_b;
-  // setScalarBC (Scalars.cpp:289-360): a "<zone>.nuSA" value on inflow faces
-  // (vel_bfa . fa_normal <= 1e-8), the cell value otherwise
+  // setScalarBC (Scalars.cpp:289-360): a "<zone>.nuSA" value, or with
+  // PROFILES the face's profile value, on inflow faces (vel_bfa . fa_normal
+  // <= 1e-8), the cell value otherwise
   double phi_b = 0.0;
   if (P.sa) {
     const double *fa_n = m.fnorm + 3 * (size_t)f;
-    phi_b = (z.skind && !(dot3(vb, fa_n) > 1.0e-8)) ? z.sval : P.sa_phi[c];
+    phi_b = (z.skind && !(dot3(vb, fa_n) > 1.0e-8)) ? (z.skind == 3 ? P.sa_prof[f] : z.sval) : P.sa_phi[c];
   }
   double *o = bfa + 8 * (size_t)f;
   o[0] = p_b / (R_gas * T_b);
@@ -5255,8 +5258,10 @@ void ensure(DBuf<T> &b, size_t n) {
 // ("n=<rows>\td=<values>", then rows of y, rho, u, v, p and one value per
 // transported scalar; JoeWithModels.cpp:262-290) and its values at each
 // PROFILES face: linear in the face's y, clamped at both ends, v dropped
-// (:296-326). Returns [nfa_b][5] = rho, u, v, w, p (zero on other faces).
-std::vector<double> profile_faces(const std::string &file, const Mesh &m, const std::vector<ZoneBc> &zbc, int nscal) {
+// (:296-326). Returns [nfa_b][5] = rho, u, v, w, p (zero on other faces);
+// with scal, the first scalar's column there too, interpolated alike (:331-357).
+std::vector<double> profile_faces(const std::string &file, const Mesh &m, const std::vector<ZoneBc> &zbc, int nscal,
+                                  std::vector<double> *scal = nullptr) {
   FILE *fp = std::fopen(file.c_str(), "rt");
   if (!fp) throw std::invalid_argument("could not open the boundary-profile file " + file);
   int nn = 0, nval = 0;
@@ -5271,6 +5276,7 @@ std::vector<double> profile_faces(const std::string &file, const Mesh &m, const 
   if (nval - 5 != nscal)
     throw std::invalid_argument("boundary-profile file " + file + " does not have the required number of values");
   std::vector<double> h(5 * (size_t)std::max(1, m.nfa_b), 0.0);
+  if (scal) scal->assign((size_t)std::max(1, m.nfa_b), 0.0);
   for (size_t iz = 0; iz < m.zones.size(); iz++) {
     if (m.zones[iz].kind != kZoneBoundary || zbc[iz].bc != kBcProfiles) continue;
     for (int ifa = m.zones[iz].ifa_f; ifa <= m.zones[iz].ifa_l; ifa++) {
@@ -5288,6 +5294,7 @@ std::vector<double> profile_faces(const std::string &file, const Mesh &m, const 
       o[2] = 0.0;
       o[3] = 0.0;
       o[4] = r1[4] + f * (r2[4] - r1[4]);
+      if (scal && nscal > 0) (*scal)[ifa] = r1[5] + f * (r2[5] - r1[5]);
     }
   }
   return h;
@@ -5531,10 +5538,18 @@ void parse_params(const std::string &joe_in, const Mesh &m, SolverParams &p, std
       if (const auto *st = pm.get(z.name + ".nuSA")) {
         if (st->size() < 2) throw std::invalid_argument("zone " + z.name + ".nuSA: missing value");
         const std::string &v = (*st)[1];
-        if (v == "HOOK" || v == "Hook" || v == "hook" || v == "PROFILES")
+        if (v == "HOOK" || v == "Hook" || v == "hook")
           throw Unsupported("zone " + z.name + ".nuSA = " + v + " is not supported");
-        b.skind = 1;
-        b.sval = ParamMap::to_d(v, z.name + ".nuSA");
+        if (v == "PROFILES") {
+          // the reference fills the scalar's profile values on the zones whose
+          // flow boundary is PROFILES only (JoeWithModels.cpp:296-357)
+          if (b.bc != kBcProfiles)
+            throw Unsupported("zone " + z.name + ".nuSA = PROFILES needs the zone's own PROFILES boundary");
+          b.skind = 3;
+        } else {
+          b.skind = 1;
+          b.sval = ParamMap::to_d(v, z.name + ".nuSA");
+        }
       }
     }
     // kine and omega boundary values (the same rules; omega = HOOK on a WALL
@@ -5629,7 +5644,7 @@ struct Ctx::Impl {
   // Spalart-Allmaras: nuSA, per-cell fields [ncv][kAuxN], residual, AScal
   // rows [nnz][6], Jacobi BiCGStab vectors (b, x, res, res0, p, v, phat, s,
   // shat, t), its scalars, the faocv entry map
-  DBuf<double> sa_phi, sa_aux, rhs_s, AS, sw[10];
+  DBuf<double> sa_phi, sa_aux, sa_prof, rhs_s, AS, sw[10];
   // Menter SST: kine, omega [2][ncv], per-cell fields [ncv][kSstN], boundary
   // values [nfa_b][2]; residuals rhs_s [2][ncv] and AScal rows AS [2][nnz][6]
   DBuf<double> sst_phi, sst_aux, sst_bfa;
@@ -6068,8 +6083,14 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
     // PROFILES zones: the face values rho, u, 0, 0, p interpolated in y from
     // the profile file (JoeWithModels.cpp:262-326), kept in the per-face
     // boundary buffer that HOOK zones fill from the host
-    std::vector<double> h = profile_faces(d.par.profiles_file, m, d.zbc, d.par.sa ? 1 : 0);
+    std::vector<double> sp;
+    std::vector<double> h = profile_faces(d.par.profiles_file, m, d.zbc, d.par.sa ? 1 : 0, &sp);
     HIPCHECK(hipMemcpy(d.hook.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (d.par.sa) d.sa_prof.upload(sp.data(), sp.size());
+  }
+  if (!d.sa_prof.p) {
+    const double zero = 0.0;
+    d.sa_prof.upload(&zero, 1);
   }
   d.counters.alloc(4);
   HIPCHECK(hipMemset(d.counters.p, 0, 4 * sizeof(int)));
@@ -6201,6 +6222,7 @@ Ctx::Ctx(const Mesh &m, const std::string &joe_in, int precision, void *stream) 
   d.dp.sa = d.par.sa ? 1 : 0;
   d.nbs = (d.par.coupled && d.par.sa) ? 6 : 5;
   d.dp.sa_phi = d.sa_phi.p;
+  d.dp.sa_prof = d.sa_prof.p;
   d.dp.fdump = nullptr;
   d.dp.vdump = nullptr;
   d.dp.sa_aux = d.sa_aux.p;

@@ -1005,7 +1005,7 @@ struct or_ctx {
   // Spalart-Allmaras: the transported scalar nuSA (ScalarTranspEq, UgpWithCv.h:50-113),
   // its matrix rows AScal[6][nnz] (JoeWithModels.cpp:892), mut_fa and the model fields
   bool sa = false;
-  std::vector<double> phi, phi_bfa, grad_phi, diff, mut_fa, wall_dist, str_mag, vort_mag;
+  std::vector<double> phi, phi_bfa, phi_bpr, grad_phi, diff, mut_fa, wall_dist, str_mag, vort_mag;
   std::vector<double> rhs_scal, AScal, dscal;
   // Menter SST (RansTurbKOmSST): kine and omega, registered in that order
   // (TurbModel_KOMSST.h:36-52), in the same arrays scalar by scalar (phi
@@ -2535,7 +2535,8 @@ static void scalar_grad(or_ctx &c, int s) {
 }
 
 // setScalarBC, Scalars.cpp:289-360, per scalar: a zone with "<zone>.<name> = v"
-// takes v on inflow faces and the cell value on outflow faces; "= HOOK" calls
+// takes v on inflow faces and the cell value on outflow faces ("= PROFILES":
+// the face's value interpolated from the profile file); "= HOOK" calls
 // boundaryHookScalarRansTurb, which for SST's omega on WALL zones sets
 // 60 mu / (rho beta_1 d^2) (TurbModel_KOMSST.h:342-358) and otherwise leaves
 // the face values alone; any other zone takes the cell value
@@ -2556,7 +2557,7 @@ void set_scalar_bc(or_ctx &c) {
             pb[ifa] = 60.0 * muLamCV / (c.rho[icv0] * sst::beta_1 * c.wall_dist[icv0] * c.wall_dist[icv0]);
           }
         } else if (kind != 0.0 && !(dot3(&c.vel_bfa[3 * ifa], &m.fa_normal[3 * ifa]) > 1.0e-8)) {
-          pb[ifa] = val;
+          pb[ifa] = kind == 3.0 ? c.phi_bpr[(size_t)c.nfa_b * s + ifa] : val;  // PROFILES: the face's profile value
         } else {
           pb[ifa] = ph[icv0];
         }
@@ -3274,6 +3275,26 @@ or_ctx *or_ctx_create(const or_mesh *m, const or_params *p) {
     for (size_t s = 0; s < ns; s++)
       std::fill(c->phi.begin() + s * n, c->phi.begin() + (s + 1) * n, c->sst ? p->sst_init[s] : p->sa_init);
     c->phi_bfa.assign(ns * nb, 0.0);
+    // each scalar's boundary profile on PROFILES zones (JoeWithModels.cpp:331-357)
+    c->phi_bpr.assign(ns * nb, 0.0);
+    if (m->profiles)
+      for (int z = 0; z < m->nzone; z++) {
+        if (m->zone_bc[z] != OR_BC_PROFILES) continue;
+        const int nn = m->nprof, nv = m->nval_prof;
+        const double *pr = m->profiles;
+        for (size_t s = 0; s < ns && (int)(5 + s) < nv; s++)
+          for (int ifa = m->zone_ifa[2 * z]; ifa <= m->zone_ifa[2 * z + 1]; ifa++) {
+            const double y = m->x_fa[3 * ifa + 1];
+            int pos = 1;
+            while (pr[pos * nv] < y && (pos < nn - 1)) pos++;
+            double fi;
+            if (y > pr[pos * nv]) fi = 1.0;
+            else if (y < pr[(pos - 1) * nv]) fi = 0.0;
+            else fi = (y - pr[(pos - 1) * nv]) / (pr[pos * nv] - pr[(pos - 1) * nv]);
+            const double phi1 = pr[(pos - 1) * nv + 5 + s], phi2 = pr[pos * nv + 5 + s];
+            c->phi_bpr[s * nb + ifa] = phi1 + fi * (phi2 - phi1);
+          }
+      }
     c->grad_phi.assign(3 * ns * n, 0.0);
     c->diff.assign(ns * m->nfa, 0.0);
     c->wall_dist.assign(n, 0.0);

@@ -43,7 +43,7 @@ typedef struct {
   const int *zone_ifa;       /* [nzone][2] first, last face */
   const double *zone_par;    /* [nzone][8] keyword values (see or_ctx_create) */
   const double *zone_scal;   /* [nzone][2] first scalar's boundary (nuSA, or kine with SST): kind (0 none,
-                              1 value, 2 HOOK), value; NULL without a turbulence model */
+                              1 value, 2 HOOK, 3 PROFILES), value; NULL without a turbulence model */
   const double *zone_scal2;  /* [nzone][2] the same for omega (SST); NULL otherwise */
   /* the boundary-profile file of PROFILES zones (JoeWithModels.cpp:262-290):
    * nprof rows of nval_prof values (y, rho, u, v, p, scalars); NULL without */

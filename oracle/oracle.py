@@ -227,7 +227,8 @@ class Oracle:
                 key = name + "." + sname
                 if key in P:
                     v = P[key][0][1]
-                    zs[s][i] = (2.0, 0.0) if v in ("HOOK", "Hook", "hook") else (1.0, float(v))
+                    zs[s][i] = ((2.0, 0.0) if v in ("HOOK", "Hook", "hook") else
+                                (3.0, 0.0) if v == "PROFILES" else (1.0, float(v)))
         zscal, zscal2 = zs
         self._keep.update(zbc=zbc, zifa=zifa, zpar=zpar, zscal=zscal, zscal2=zscal2)
         self.zone_names = [z[0] for z in zl]

@@ -83,6 +83,8 @@ RHS_CASES = [("bump24", "HLLC", {}), ("bump24", "ROE", {}), ("bump24", "AUSM", {
              ("bump24", "HLLC", {"prof": True}),
              ("box4", "ROE", {"prof": True}),
              ("bump24", "HLLC", {"visc": "pl", "sa": True, "prof": True}),
+             # and nuSA from the profile file too ("inlet.nuSA = PROFILES")
+             ("bump24", "HLLC", {"visc": "pl", "sa": True, "prof": True, "sprof": True}),
              # Menter SST k-omega (segregated kine, omega)
              ("bump24", "HLLC", {"visc": "pl", "sst": True}),
              ("box4", "HLLC", {"visc": "su", "sst": True}),
@@ -181,6 +183,8 @@ def tag(kw):
         t += "_dtsum"
     if kw.get("prof"):
         t += "_prof"
+    if kw.get("sprof"):
+        t += "_sprof"
     return t
 
 
@@ -198,12 +202,13 @@ def main(only=None):
             with_sst = kw.pop("sst", False)
             dtsum = kw.pop("dtsum", False)
             prof = kw.pop("prof", False)
+            sprof = kw.pop("sprof", False)
             mm, joe = MESHES[name](scheme, **kw)
             if dtsum:
                 kw["dtsum"] = True
                 joe += "TIME_STEP = SUM_BASED\n"
             tk = dict(kw, **({"visc": visc} if visc else {}), **({"sa": True} if with_sa else {}),
-                      **({"sst": True} if with_sst else {}), **({"prof": True} if prof else {}))
+                      **({"sst": True} if with_sst else {}), **({"prof": True} if prof else {}), **({"sprof": True} if sprof else {}))
             fixture = f"rhs_{name}_{scheme.lower()}{tag(tk)}"
             if only and only not in fixture:
                 continue
@@ -220,6 +225,10 @@ def main(only=None):
             if with_sa:
                 kw["sa"] = True
                 joe = cases.with_sa(joe, 0.005, cases.SA_BUMP if name.startswith("bump") else cases.SA_BOX)
+                if sprof:
+                    kw["sprof"] = True
+                    joe = joe.replace("inlet.nuSA = 0.006", "inlet.nuSA = PROFILES")
+                    assert "inlet.nuSA = PROFILES" in joe
             if with_sst:
                 kw["sst"] = True
                 joe = cases.with_sst(joe, 0.001, 5.0, cases.SST_BUMP if name.startswith("bump") else cases.SST_BOX)

@@ -132,6 +132,7 @@ def _sa_box_joe():
     (lambda s: s.replace("wall = WALL", "wall = SYMMETRY"), 3, "WALL"),              # no wall: no wall distance
     (lambda s: s.replace("SPACE_INTEGRATION = HLLC", "SPACE_INTEGRATION = ROE"), 5, "HLLC"),
     (lambda s: s.replace("inlet.nuSA = 0.006", "inlet.nuSA = HOOK"), 5, "inlet.nuSA"),
+    (lambda s: s.replace("inlet.nuSA = 0.006", "inlet.nuSA = PROFILES"), 5, "own PROFILES"),  # flow BC not PROFILES
     (lambda s: s.replace("TURB_MODEL = SA", "TURB_MODEL = KOMEGA"), 5, "TURB_MODEL"),
 ])
 def test_sa_input_checked_before_device(edit, status, needle):
