@@ -395,6 +395,17 @@ jm_status jm_chemtable_var(const jm_chemtable *t, const char *name, int *index);
  * Launched on stream (NULL: the null stream) and not waited for. */
 jm_status jm_chemtable_lookup(const jm_chemtable *t, int64_t n, const double *zmean_dev, const double *zvar_dev,
                               const double *prog_dev, int nout, const int *vars, double *out_dev, void *stream);
+/* jm_chemtable_lookup on host arrays (zmean, zvar, prog [n]; out [nout][n]):
+ * copied to the device and back, the call returns when out holds the values.
+ * Serves the reference's one-point Lookup / LookupCoeff calls (n = 1) and
+ * batches of cells kept on the host. */
+jm_status jm_chemtable_lookup_host(const jm_chemtable *t, int64_t n, const double *zmean, const double *zvar,
+                                   const double *prog, int nout, const int *vars, double *out);
+/* GetChemtableCoordinate1..3 (:89-94): axis d (1, 2, 3) into x[dims[d-1]] */
+jm_status jm_chemtable_axis(const jm_chemtable *t, int d, double *x);
+/* GetCombustionModel (which = -2), GetChemtableType (-1), GetChemtableVariables(i)
+ * (which = i >= 0): the text into buf (len bytes, NUL-terminated) */
+jm_status jm_chemtable_text(const jm_chemtable *t, int which, char *buf, int len);
 /* Measurement hook, no counterpart in the reference: jm_chemtable_lookup
  * launched reps times on stream after one untimed launch; *ms = the mean
  * duration of one, from HIP events recorded on that stream. */

@@ -523,6 +523,38 @@ jm_status jm_chemtable_lookup(const jm_chemtable *t, int64_t n, const double *zm
   });
 }
 
+jm_status jm_chemtable_lookup_host(const jm_chemtable *t, int64_t n, const double *zmean, const double *zvar,
+                                   const double *prog, int nout, const int *vars, double *out) {
+  return guard([&] {
+    if (!t) throw jm::ArgError("jm_chemtable_lookup_host: null table");
+    if (n > 0 && (!zmean || !zvar || !prog || !out)) throw jm::ArgError("jm_chemtable_lookup_host: null array");
+    t->t->lookup_host(n, zmean, zvar, prog, nout, vars, out);
+    return JM_OK;
+  });
+}
+
+jm_status jm_chemtable_axis(const jm_chemtable *t, int d, double *x) {
+  return guard([&] {
+    if (!t || !x || d < 1 || d > 3) throw jm::ArgError("jm_chemtable_axis: axis 1, 2 or 3");
+    const auto &h = t->t->meta();
+    const std::vector<double> &a = d == 1 ? h.x1 : (d == 2 ? h.x2 : h.x3);
+    std::memcpy(x, a.data(), sizeof(double) * a.size());
+    return JM_OK;
+  });
+}
+
+jm_status jm_chemtable_text(const jm_chemtable *t, int which, char *buf, int len) {
+  return guard([&] {
+    if (!t || !buf || len < 1) throw jm::ArgError("jm_chemtable_text: null argument");
+    const auto &h = t->t->meta();
+    if (which < -2 || which >= h.nvar) throw jm::ArgError("jm_chemtable_text: no such entry");
+    const std::string &s = which == -2 ? h.model : (which == -1 ? h.type : h.names[which]);
+    if ((int)s.size() >= len) throw jm::ArgError("jm_chemtable_text: buffer too small");
+    std::memcpy(buf, s.c_str(), s.size() + 1);
+    return JM_OK;
+  });
+}
+
 jm_status jm_chemtable_time_lookup(const jm_chemtable *t, int64_t n, const double *zmean_dev, const double *zvar_dev,
                                    const double *prog_dev, int nout, const int *vars, double *out_dev, void *stream,
                                    int reps, double *ms) {

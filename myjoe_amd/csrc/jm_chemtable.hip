@@ -17,6 +17,7 @@
 // out; the table (a few MB) is read from the caches.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -324,6 +325,29 @@ void Chemtable::lookup(int64_t n, const double *zm, const double *zv, const doub
   else
     hipLaunchKernelGGL(k_chem_lookup<false>, dim3((unsigned)nblk), dim3(256), 0, st, T, n, zm, zv, pc, S, out);
   CTCHECK(hipGetLastError());
+}
+
+void Chemtable::lookup_host(int64_t n, const double *zm, const double *zv, const double *pc, int nout,
+                            const int *vars, double *out) const {
+  if (n < 0) throw ArgError("chemistry lookup: n < 0");
+  if (n == 0) {
+    lookup(0, nullptr, nullptr, nullptr, nout, vars, nullptr, nullptr);  // checks the selection
+    return;
+  }
+  double *d = nullptr;
+  const size_t bytes = sizeof(double) * (size_t)n;
+  CTCHECK(hipMalloc((void **)&d, bytes * (3 + (size_t)std::max(nout, 0))));
+  try {
+    CTCHECK(hipMemcpy(d, zm, bytes, hipMemcpyHostToDevice));
+    CTCHECK(hipMemcpy(d + n, zv, bytes, hipMemcpyHostToDevice));
+    CTCHECK(hipMemcpy(d + 2 * n, pc, bytes, hipMemcpyHostToDevice));
+    lookup(n, d, d + n, d + 2 * n, nout, vars, d + 3 * n, nullptr);
+    CTCHECK(hipMemcpy(out, d + 3 * n, bytes * nout, hipMemcpyDeviceToHost));
+  } catch (...) {
+    (void)hipFree(d);
+    throw;
+  }
+  CTCHECK(hipFree(d));
 }
 
 double Chemtable::time_lookup(int64_t n, const double *zm, const double *zv, const double *pc, int nout,

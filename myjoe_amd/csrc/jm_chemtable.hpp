@@ -41,6 +41,9 @@ class Chemtable {
   // pointers; launched on stream (0: the null stream), not waited for
   void lookup(int64_t n, const double *zm, const double *zv, const double *pc, int nout, const int *vars,
               double *out, void *stream) const;
+  // the same on host arrays: copied in, looked up, copied back, waited for
+  void lookup_host(int64_t n, const double *zm, const double *zv, const double *pc, int nout, const int *vars,
+                   double *out) const;
   // the lookup kernel launched reps times on stream, mean ms from HIP events
   double time_lookup(int64_t n, const double *zm, const double *zv, const double *pc, int nout, const int *vars,
                      double *out, void *stream, int reps) const;

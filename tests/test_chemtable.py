@@ -12,6 +12,8 @@ included): the lookup is FP32 nodes with FP64 weights and sums in the
 reference's order.
 """
 import hashlib
+import os
+import subprocess
 
 import numpy as np
 import pytest
@@ -192,3 +194,103 @@ def test_device_lookup_long_axes_and_wide_rows():
     for vars in (range(19), range(3, 19), [18, 2, 7], [5]):
         vars = list(vars)
         assert same_bits(t.lookup(zm, zv, pc, vars), chemtable_lookup(x1, x2, x3, data, zm, zv, pc, vars)), vars
+
+
+# ---------------------------------------------------------------------------
+# the C++ mirror of the reference's class (myjoe_amd/host/ChemtableCartesianLinear_single.hpp)
+# ---------------------------------------------------------------------------
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+MIRROR_PROGRAM = r"""
+#include <cstdio>
+#include <vector>
+#include "myjoe_amd/host/ChemtableCartesianLinear_single.hpp"
+
+// a caller written against the reference's class: Load, the accessors, one
+// point at a time by name and LookupCoeff, then the batched form
+int main(int argc, char **argv) {
+  ChemtableCartesianLinear_single tab;
+  try {
+    tab.Load(argv[1]);
+  } catch (const std::exception &e) {
+    std::fprintf(stderr, "%s\n", e.what());
+    return 3;
+  }
+  FILE *fp = std::fopen(argv[2], "rb");
+  long n = 0;
+  if (std::fread(&n, sizeof n, 1, fp) != 1) return 4;
+  std::vector<double> a(3 * n);
+  if (std::fread(a.data(), sizeof(double), 3 * n, fp) != (size_t)(3 * n)) return 4;
+  std::fclose(fp);
+  const int nvar = tab.GetChemtableDimension4(), np = n < 64 ? n : 64;
+  std::vector<double> one, coeff(10 * (size_t)n);
+  for (int p = 0; p < np; p++)
+    for (int l = 0; l < nvar; l++) one.push_back(tab.Lookup(a[p], a[n + p], a[2 * n + p], tab.GetChemtableVariables(l)));
+  for (int p = 0; p < np; p++) {
+    double c[10];
+    tab.LookupCoeff(c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7], c[8], c[9], a[p], a[n + p], a[2 * n + p]);
+    one.insert(one.end(), c, c + 10);
+  }
+  tab.LookupCoeff((int)n, a.data(), a.data() + n, a.data() + 2 * n, coeff.data());
+  fp = std::fopen(argv[3], "wb");
+  double meta[5] = {(double)tab.GetChemtableDimension1(), tab.GetChemtableCoordinate1(1),
+                    tab.GetChemtableCoordinate3(tab.GetChemtableDimension3()), tab.GetReferencePressure(),
+                    (double)(tab.GetCombustionModel() == "FPVA")};
+  std::fwrite(meta, sizeof(double), 5, fp);
+  std::fwrite(one.data(), sizeof(double), one.size(), fp);
+  std::fwrite(coeff.data(), sizeof(double), coeff.size(), fp);
+  std::fclose(fp);
+  return 0;
+}
+"""
+
+
+def _mirror_program(tmp_path):
+    src = tmp_path / "mirror.cpp"
+    src.write_text(MIRROR_PROGRAM)
+    exe = tmp_path / "mirror"
+    lib = os.path.join(ROOT, "myjoe_amd")
+    r = subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", f"-I{ROOT}", str(src), f"-L{lib}", "-ljoemi",
+                        f"-Wl,-rpath,{lib}", "-o", str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    return exe
+
+
+def _mirror_inputs(tmp_path, name):
+    g, names = fixture(name)
+    table = tmp_path / "table.bin"
+    chemtable.write_table(table, g["X1"], g["X2"], g["X3"], g["DATA"], names)
+    pts = tmp_path / "points.bin"
+    pts.write_bytes(np.int64(g["ZM"].size).tobytes() + np.concatenate([g["ZM"], g["ZV"], g["PC"]]).tobytes())
+    return g, names, table, pts
+
+
+def test_mirror_compiles_and_fails_loudly_without_device(tmp_path):
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    exe = _mirror_program(tmp_path)
+    _, _, table, pts = _mirror_inputs(tmp_path, "fpva9x5x7")
+    r = subprocess.run([str(exe), str(table), str(pts), str(tmp_path / "out.bin")], capture_output=True, text=True)
+    assert r.returncode == 3 and "HIP" in r.stderr, (r.returncode, r.stderr)
+
+
+@pytest.mark.gpu
+def test_mirror_class_matches_reference(tmp_path):
+    print("chemtable C++ mirror", flush=True)
+    exe = _mirror_program(tmp_path)
+    g, names, table, pts = _mirror_inputs(tmp_path, "fpva9x5x7")
+    out = tmp_path / "out.bin"
+    r = subprocess.run([str(exe), str(table), str(pts), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = np.fromfile(out, np.float64)
+    n, nvar, np_ = g["ZM"].size, len(names), min(64, g["ZM"].size)
+    meta, got = got[:5], got[5:]
+    assert meta.tolist() == [9.0, g["X1"][0], g["X3"][-1], 101325.0, 1.0]
+    one = got[:np_ * nvar].reshape(np_, nvar)
+    coeff1 = got[np_ * nvar:np_ * (nvar + 10)].reshape(np_, 10)
+    batch = got[np_ * (nvar + 10):].reshape(10, n)
+    assert same_bits(one, g["LOOKUP"][:np_])
+    assert same_bits(coeff1, g["COEFF"][:np_])
+    assert same_bits(batch.T, g["COEFF"])
