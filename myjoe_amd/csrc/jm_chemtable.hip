@@ -124,7 +124,10 @@ __global__ void __launch_bounds__(256) k_chem_lookup(TabDev T, int64_t n, const 
   if (p >= n) return;
   int idx[3];
   double w[3][2];
-  locate(ax, T.n1, T.n2, T.n3, zm[p], zv[p], pc[p], idx, w);
+  // the points and the results stream through once: non-temporal, so they do
+  // not push the table's lines out of the caches
+  locate(ax, T.n1, T.n2, T.n3, __builtin_nontemporal_load(zm + p), __builtin_nontemporal_load(zv + p),
+         __builtin_nontemporal_load(pc + p), idx, w);
   const float *node = T.data + (size_t)idx[0] * T.s1 + (size_t)idx[1] * T.s2 + (size_t)idx[2] * T.nvp;
   if (S.first >= 0) {
     const int c0 = S.first >> 2, c1 = (S.first + S.nout - 1) >> 2;
@@ -153,7 +156,7 @@ __global__ void __launch_bounds__(256) k_chem_lookup(TabDev T, int64_t n, const 
 #pragma unroll
       for (int e = 0; e < 4; e++) {
         const int q = 4 * c + e - S.first;
-        if (q >= 0 && q < S.nout) out[(size_t)q * n + p] = val[e];
+        if (q >= 0 && q < S.nout) __builtin_nontemporal_store(val[e], out + (size_t)q * n + p);
       }
     }
   } else {
@@ -171,7 +174,7 @@ __global__ void __launch_bounds__(256) k_chem_lookup(TabDev T, int64_t n, const 
         }
         val += w[2][kk] * g;
       }
-      out[(size_t)q * n + p] = val;
+      __builtin_nontemporal_store(val, out + (size_t)q * n + p);
     }
   }
 }
