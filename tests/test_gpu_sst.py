@@ -25,7 +25,8 @@ test measures that band with the oracle and holds the device to it:
 
 The scalar rows have two kernel forms (GPU_SCAL_KERNEL = ENTRY, one lane per
 cell-face entry, the default; CELL, one thread per cell); they give the same
-bits, on the reference's residual fixtures and over 50 steps of the bump.
+bits, on the reference's residual fixtures, over 50 steps of the bump, and on
+a 216 000-cell plate.
 """
 import numpy as np
 import pytest
@@ -185,3 +186,25 @@ def test_sst_scalar_row_forms_agree_over_a_run():
         out.append(scal_fields(s))
     for k in out[0]:
         assert np.array_equal(out[0][k], out[1][k]), k
+
+
+def test_sst_scalar_row_forms_agree_on_a_larger_plate():
+    # the two forms at 10 + 80 x 60 x 40 cells (216 000), from a developing
+    # state (a few steps from the free stream): residuals and AScal rows
+    # bit for bit, and the next step's fields
+    print("sst scalar rows, 216k-cell plate", flush=True)
+    mm, joe = cases.plate_case(10, 80, 60, 40, turb="SST", nsteps=10)
+    mesh = joemi.Mesh.from_faces(mm)
+    got = []
+    for form in ("ENTRY", "CELL"):
+        s = joemi.Solver(mesh, joe + f"GPU_SCAL_KERNEL = {form}\n")
+        s.set_state(*cases.plate_uniform(mesh.ncv))
+        for _ in range(3):
+            s.step()
+        s.calc_rhs(True)
+        f = {k: s.get(k).copy() for k in ("RHS_KINE", "RHS_OMEGA", "ASCAL")}
+        s.step()
+        f.update(scal_fields(s))
+        got.append(f)
+    for k in got[0]:
+        assert np.array_equal(got[0][k], got[1][k]), k
